@@ -1,0 +1,166 @@
+/*
+ * uft.h -- C-ABI of the MI355X SIMP topology-optimization core.
+ *
+ * One context (uft_ctx) holds one design problem on one GPU: the mesh, the
+ * boundary conditions, the parameters, every design vector of the reference's
+ * OptPara (device resident), the solver hierarchy and the MMA state.  Entry
+ * points mirror the calls of the reference's design loop (main.cpp:64-119);
+ * each cites the reference interface it replaces.  Plain pointers and sizes
+ * only; no exceptions cross this boundary.  Every function returns an int
+ * status (UFT_OK == 0); uft_last_error() describes the last failure.
+ *
+ * Threading: a context is used from one host thread at a time; calls are
+ * ordered on the context's own HIP stream.  Contexts are independent.
+ *
+ * Vectors are addressed by id.  Element vectors (length ne) replace the PETSc
+ * Vecs of OptPara (OptPara.h:28-35); UFT_U is the displacement (length 3*nn,
+ * node-interleaved DOF 3n+c, SolveFem.h:42); UFT_RX the filter's node field.
+ */
+#ifndef UFT_H
+#define UFT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct uft_ctx uft_ctx;
+
+enum uft_status {
+  UFT_OK = 0,
+  UFT_ERR_ARG = 1,      /* bad argument / size / id */
+  UFT_ERR_IO = 2,       /* missing or short mesh/BC file */
+  UFT_ERR_NOCONV = 3,   /* Krylov solve did not reach its tolerance */
+  UFT_ERR_DETJ = 4,     /* non-positive Jacobian determinant in an element */
+  UFT_ERR_OOM = 5,      /* device or host allocation failed */
+  UFT_ERR_HIP = 6,      /* HIP runtime error */
+  UFT_ERR_STATE = 7     /* call out of order (e.g. before uft_setup) */
+};
+
+enum uft_vec_id {
+  UFT_X = 0, UFT_XFILED = 1, UFT_XPHYS = 2, UFT_XOLD = 3, UFT_DFDX = 4,
+  UFT_XMIN = 5, UFT_XMAX = 6, UFT_VOL = 7, UFT_U = 8, UFT_RX = 9,
+  UFT_DGDX0 = 16 /* UFT_DGDX0 + j for constraint j < m */
+};
+
+enum uft_size_id { UFT_SIZE_NE = 0, UFT_SIZE_NN = 1, UFT_SIZE_NDOF = 2, UFT_SIZE_NPE = 3 };
+
+enum uft_precision { UFT_FP64 = 0, UFT_MIXED = 1 };
+
+/* Parameters.  Defaults (uft_default_params) are the reference's
+ * (OptPara.cpp:66-78, SolveFem.cpp:8-11,134, PDEFilter.cpp:117-120). */
+typedef struct uft_params {
+  double Emin, Emax, penal, volfrac, rmin, nu, movlim, Xmin, Xmax, x0, beta0, betamax;
+  int proj, m, maxItr;
+  double rtol_el;      /* elasticity solve: ||F-KU|| <= rtol_el ||F||   (-ksp_rtol) */
+  double rtol_f;       /* filter CG: ||D^-1 r|| <= rtol_f ||D^-1 b||    (-ksp_rtol) */
+  int maxit_el, maxit_f;
+  int precision;       /* enum uft_precision */
+  int amg_levels;      /* total levels incl. the fine one (reference: PCGAMGSetNlevels 4) */
+  int cheb_degree;     /* Chebyshev-Jacobi smoother degree per level */
+  int coarse_max;      /* stop coarsening at or below this many coarse DOFs */
+  int inner_rtol_exp;  /* mixed form: inner FP32 solve reduces the residual by 10^-exp */
+} uft_params;
+
+/* --- context ------------------------------------------------------------ */
+/* Replaces PetscInitialize + OptPara() (main.cpp:41,50). */
+int uft_create(uft_ctx **ctx, int device);
+void uft_destroy(uft_ctx *ctx);
+const char *uft_last_error(const uft_ctx *ctx);
+void uft_default_params(uft_params *p);
+/* OptPara::SetUp / SetUpOpt option overrides (OptPara.cpp:123-218). */
+int uft_params_set(uft_ctx *ctx, const uft_params *p);
+int uft_params_get(const uft_ctx *ctx, uft_params *p);
+
+/* --- mesh and boundary conditions -------------------------------------- */
+/* InputMeshData::GetNodeAndElementData (InputMeshData.cpp:33-226): elem is
+ * ne x npe 0-based node ids (npe 8 = hex8 diagonal numbering, 4 = tet4),
+ * xyz is nn x 3. */
+int uft_mesh_set(uft_ctx *ctx, int npe, int64_t nn, int64_t ne, const int32_t *elem, const double *xyz);
+/* Same, reading the reference text formats vertex.txt / element.txt
+ * (InputMeshData.cpp:100-138): whitespace-separated, no terminator needed. */
+int uft_mesh_load_text(uft_ctx *ctx, int npe, int64_t nn, int64_t ne, const char *vertex_path,
+                       const char *element_path);
+/* InputMeshData::GetBcs (InputMeshData.cpp:228-285): fixed DOFs, and nlc load
+ * cases of nload (dof, value) pairs each, INSERTed into one RHS (SolveFem.cpp:277-283). */
+int uft_bc_set(uft_ctx *ctx, int nfixed, const int32_t *fixed, int nlc, int nload,
+               const int32_t *load_dofs, const double *load_vals);
+/* Same, reading dir/fixeddofs.txt, dir/loaddof_%d.txt, dir/load_%d.txt
+ * (InputMeshData.cpp:238,252,271). */
+int uft_bc_load_text(uft_ctx *ctx, const char *dir, int nfixed, int nlc, int nload);
+/* Builds the per-element operators, volumes, filter, solver hierarchy and MMA
+ * state: SolveFem ctor (SolveFem.cpp:6-15), PDEFilter ctor
+ * (PDEFilter.cpp:36-140), OptPara::SetMMA (OptPara.cpp:158-177); sets
+ * X = XFiled = XPhys = Xold = volfrac (OptPara.cpp:150-153). */
+int uft_setup(uft_ctx *ctx);
+/* Host copy of the mesh (elem ne x npe, xyz nn x 3), for output such as the
+ * VTU writer (Write2Vtu.cpp:5-173 gathers the same data on rank 0). */
+int uft_mesh_get(const uft_ctx *ctx, int32_t *elem, double *xyz);
+int64_t uft_size(const uft_ctx *ctx, int which);
+double uft_total_volume(const uft_ctx *ctx);
+
+/* --- vectors (PETSc Vec replacement) ------------------------------------ */
+int uft_vec_get(uft_ctx *ctx, int id, double *host, int64_t n);
+int uft_vec_set(uft_ctx *ctx, int id, const double *host, int64_t n);
+int uft_vec_fill(uft_ctx *ctx, int id, double value);
+int uft_vec_copy(uft_ctx *ctx, int src, int dst);
+/* Device address of a vector (stream-ordered on the context stream). */
+void *uft_vec_device(uft_ctx *ctx, int id);
+
+/* --- the design iteration (main.cpp:72-107) ------------------------------ */
+/* SolveFem::ComputeObjectiveAndSensitivities (SolveFem.h:39-40,
+ * SolveFem.cpp:159-265): solves K(xphys) U = F, then fx = sum E(x) u^T KE u,
+ * gx[0] = sum x V / TV - volfrac, dfdx, dgdx[0] = V/TV. */
+int uft_fem_obj_sens(uft_ctx *ctx, double *fx, double *gx, int dfdx_id, int m, const int *dgdx_ids,
+                     int xphys_id);
+/* PDEFilter::Filter (PDEFilter.h:22, PDEFilter.cpp:142-165). */
+int uft_filter(uft_ctx *ctx, int x_id, int xf_id);
+/* PDEFilter::ComputeFilteredSens (PDEFilter.h:24, PDEFilter.cpp:167-183). */
+int uft_filter_sens(uft_ctx *ctx, int dfdx_id, int m, const int *dgdx_ids);
+/* PDEFilter::HeavisideFilter (PDEFilter.h:26, PDEFilter.cpp:242-293). */
+int uft_heaviside(uft_ctx *ctx, int xf_id, int xp_id, int itr);
+/* PDEFilter::ComputeProjectedSens (PDEFilter.h:28, PDEFilter.cpp:295-341). */
+int uft_projected_sens(uft_ctx *ctx, int xf_id, int dfdx_id, int m, const int *dgdx_ids);
+/* MMA::SetOuterMovelimit (MMA.h:43-44, MMA.cpp:370-389). */
+int uft_mma_movelimit(uft_ctx *ctx, int x_id, int xmin_id, int xmax_id);
+/* MMA::Update (MMA.h:23, MMA.cpp:483-502): x is overwritten in place. */
+int uft_mma_update(uft_ctx *ctx, int x_id, int dfdx_id, const double *gx, const int *dgdx_ids,
+                   int xmin_id, int xmax_id);
+/* MMA::DesignChange (MMA.h:52, MMA.cpp:391-410): ch = max|x - xold|; xold := x. */
+int uft_mma_design_change(uft_ctx *ctx, int x_id, int xold_id, double *ch);
+/* Whole loop body of main.cpp:72-107 on the context's own vectors. */
+int uft_design_iteration(uft_ctx *ctx, int itr, double *fx, double *gx, double *ch);
+
+/* --- diagnostics and checks ---------------------------------------------- */
+/* Iterations and true relative residual of the last elasticity solve
+ * (the "Norm of error %e, Iterations %d" line, SolveFem.cpp:142-152), and the
+ * iteration count of the last filter solve. */
+int uft_solve_stats(const uft_ctx *ctx, int *it_el, double *relres_el, int *it_f, int *outer_el);
+/* y = K(xphys) u with Dirichlet rows/columns eliminated and unit diagonal
+ * (SolveFem.cpp:297-333, matrix-free), host buffers of length 3*nn. */
+int uft_apply_K(uft_ctx *ctx, int xphys_id, const double *u, double *y);
+/* Unscaled element stiffness KE of element e as a full nd x nd row-major
+ * matrix (SolveFem.cpp:339-408 / tet4/Tet4Isoparametric.cpp). */
+int uft_element_ke(uft_ctx *ctx, int64_t e, double *ke);
+/* Helmholtz element matrix KF (npe x npe) and T (npe) (PDEFilter.cpp:185-240). */
+int uft_element_kf(uft_ctx *ctx, int64_t e, double *kf, double *t);
+/* Synchronise the context stream. */
+int uft_sync(uft_ctx *ctx);
+
+/* --- workload generator (synthetic cantilever) --------------------------- */
+int64_t uft_box_nodes(int nx, int ny, int nz);
+int64_t uft_box_elems(int npe, int nx, int ny, int nz);
+/* Box [0,lx]x[0,ly]x[0,lz], node id i+(nx+1)(j+(ny+1)k), interior nodes
+ * jittered by (2u-1)*jitter*h per axis from a SplitMix64 stream. */
+int uft_gen_box(int npe, int nx, int ny, int nz, double lx, double ly, double lz, double jitter,
+                uint64_t seed, double *xyz, int32_t *elem);
+/* Cantilever BCs: x=0 face clamped; load_kind 0 = -1/(nz+1) on the y-DOFs of
+ * the bottom edge of x=lx, 1 = -1/9 on a 3x3 node patch at mid-height. */
+int uft_box_bcs(int nx, int ny, int nz, int load_kind, int32_t *fixed, int *nfixed, int32_t *load_dofs,
+                double *load_vals, int *nload);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

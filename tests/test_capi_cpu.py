@@ -1,0 +1,70 @@
+"""CPU checks of the C-ABI library: it loads, exports every function that
+include/uft.h declares, and its host-side workload generator is bit-exact with
+the oracle's.  No GPU compute is called here."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import conftest
+
+
+@pytest.fixture(scope="module")
+def U():
+    import uft
+    if not os.path.exists(uft.lib_path()):
+        subprocess.run(["make", "-s", "-j8", "-C", conftest.PKG, "lib/libuft.so"], check=True)
+    uft.lib()
+    return uft
+
+
+def test_exports_every_declared_symbol(U):
+    declared = U.header_symbols()
+    assert len(declared) >= 30
+    missing = [s for s in declared if s not in U.exported_symbols()]
+    assert not missing, missing
+
+
+def test_params_struct_layout_matches_header(U, tmp_path):
+    """The ctypes mirror of uft_params has the C compiler's layout."""
+    fields = [f for f, _ in U.Params._fields_]
+    src = tmp_path / "layout.c"
+    src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "uft.h"\nint main(void){\n'
+                   + 'printf("%zu\\n", sizeof(uft_params));\n'
+                   + "".join(f'printf("%zu\\n", offsetof(uft_params, {f}));\n' for f in fields) + "return 0;}\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(conftest.ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    import ctypes
+    assert out[0] == ctypes.sizeof(U.Params)
+    assert out[1:] == [getattr(U.Params, f).offset for f in fields]
+
+
+@pytest.mark.parametrize("npe,dims,seed", [(8, (8, 4, 4), 7), (8, (140, 70, 40), 392000), (4, (60, 30, 30), 324000),
+                                           (4, (5, 3, 2), 1)])
+def test_generator_bit_exact(U, O, npe, dims, seed):
+    a = U.gen_box(npe, *dims, jitter=0.15, seed=seed)
+    b = O.gen_box(npe, *dims, jitter=0.15, seed=seed)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+@pytest.mark.parametrize("kind,dims", [(0, (140, 70, 40)), (0, (40, 20, 20)), (1, (60, 30, 30))])
+def test_bcs_bit_exact(U, O, kind, dims):
+    for x, y in zip(U.box_bcs(*dims, load_kind=kind), O.box_bcs(*dims, load_kind=kind)):
+        assert np.array_equal(x, y)
+
+
+def test_bad_generator_arguments(U):
+    L = U.lib()
+    xyz = np.zeros(3)
+    el = np.zeros(8, np.int32)
+    assert L.uft_gen_box(8, 0, 1, 1, 2.0, 1.0, 1.0, 0.0, 0, xyz, el) == 1
+    assert L.uft_gen_box(5, 1, 1, 1, 2.0, 1.0, 1.0, 0.0, 0, xyz, el) == 1
+
+
+def test_dropin_driver_links(U):
+    exe = os.path.join(conftest.PKG, "bin", "uft_opt")
+    if not os.path.exists(exe):
+        subprocess.run(["make", "-s", "-j8", "-C", conftest.PKG], check=True)
+    assert os.access(exe, os.X_OK)

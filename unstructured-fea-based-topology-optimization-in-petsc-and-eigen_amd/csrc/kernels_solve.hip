@@ -1,0 +1,115 @@
+// gfx950 kernels of the Krylov loops: fused vector updates of the Helmholtz
+// filter's Jacobi-CG (PETSc KSPCG semantics, preconditioned residual norm)
+// and the device-side scalar recurrences of CG, so a Krylov step needs no
+// host round trip except its convergence test.
+#include "uft_internal.h"
+
+namespace {
+constexpr int TPB = 256;
+inline unsigned nblk(int64_t n) { return (unsigned)((n + TPB - 1) / TPB); }
+
+template <int K>
+__device__ __forceinline__ void block_sum_store(double (&v)[K], double *part) {
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_down(v[k], off, 64);
+  }
+  __shared__ double sm[K][TPB / 64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) sm[k][w] = v[k];
+  }
+  __syncthreads();
+  if (threadIdx.x < K) {
+    const int k = threadIdx.x;
+    part[(size_t)blockIdx.x * K + k] = (sm[k][0] + sm[k][1]) + (sm[k][2] + sm[k][3]);
+  }
+}
+
+// r = b - q ; z = D^-1 r ; partials ||D^-1 b||^2, ||z||^2, z.r
+__global__ __launch_bounds__(TPB) void k_fcg_init(int64_t n, const double *__restrict__ b, const double *__restrict__ q,
+                                                  const double *__restrict__ dinv, double *__restrict__ r,
+                                                  double *__restrict__ z, double *__restrict__ part) {
+  const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  double v[3] = {0.0, 0.0, 0.0};
+  if (i < n) {
+    const double di = dinv[i], bi = b[i];
+    const double ri = bi - q[i], zi = ri * di, zb = bi * di;
+    r[i] = ri;
+    z[i] = zi;
+    v[0] = zb * zb;
+    v[1] = zi * zi;
+    v[2] = zi * ri;
+  }
+  block_sum_store<3>(v, part);
+}
+
+// x += alpha p ; r -= alpha q ; z = D^-1 r ; partials ||z||^2, z.r
+__global__ __launch_bounds__(TPB) void k_fcg_update(int64_t n, const double *__restrict__ sc, int ia,
+                                                    double *__restrict__ x, const double *__restrict__ p,
+                                                    double *__restrict__ r, const double *__restrict__ q,
+                                                    const double *__restrict__ dinv, double *__restrict__ z,
+                                                    double *__restrict__ part) {
+  const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  const double al = sc[ia];
+  double v[2] = {0.0, 0.0};
+  if (i < n) {
+    x[i] += al * p[i];
+    const double ri = r[i] - al * q[i];
+    r[i] = ri;
+    const double zi = ri * dinv[i];
+    z[i] = zi;
+    v[0] = zi * zi;
+    v[1] = zi * ri;
+  }
+  block_sum_store<2>(v, part);
+}
+
+// p = z + (sc[ib] / sc[ibo]) p, or p = z when first
+__global__ __launch_bounds__(TPB) void k_fcg_p(int64_t n, const double *__restrict__ z, const double *__restrict__ sc,
+                                               int ib, int ibo, int first, double *__restrict__ p) {
+  const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  if (i >= n) return;
+  if (first) {
+    p[i] = z[i];
+  } else {
+    const double bb = sc[ib] / sc[ibo];
+    p[i] = z[i] + bb * p[i];
+  }
+}
+
+// CG scalars: sc[dst] = sc[a] / sc[b]
+__global__ void k_div(double *sc, int a, int b, int dst) { sc[dst] = sc[a] / sc[b]; }
+// beta step: sc[ibeta] = sc[inew] / sc[iold]; sc[iold] = sc[inew]; optional log
+__global__ void k_beta(double *sc, int inew, int iold, int ibeta, double *log2) {
+  const double b = sc[inew] / sc[iold];
+  if (log2) {
+    log2[0] = sc[iold];
+    log2[1] = sc[inew];
+  }
+  sc[ibeta] = b;
+  sc[iold] = sc[inew];
+}
+__global__ void k_copy_scalar(const double *src, double *dst) { *dst = *src; }
+}  // namespace
+
+namespace uftk {
+void fcg_init(hipStream_t s, int64_t n, const double *b, const double *q, const double *dinv, double *r, double *z,
+              double *part) {
+  k_fcg_init<<<nblk(n), TPB, 0, s>>>(n, b, q, dinv, r, z, part);
+}
+void fcg_update(hipStream_t s, int64_t n, const double *sc, int ia, double *x, const double *p, double *r,
+                const double *q, const double *dinv, double *z, double *part) {
+  k_fcg_update<<<nblk(n), TPB, 0, s>>>(n, sc, ia, x, p, r, q, dinv, z, part);
+}
+void fcg_p(hipStream_t s, int64_t n, const double *z, const double *sc, int ib, int ibo, int first, double *p) {
+  k_fcg_p<<<nblk(n), TPB, 0, s>>>(n, z, sc, ib, ibo, first, p);
+}
+void sc_div(hipStream_t s, double *sc, int a, int b, int dst) { k_div<<<1, 1, 0, s>>>(sc, a, b, dst); }
+void sc_beta(hipStream_t s, double *sc, int inew, int iold, int ibeta, double *log2) {
+  k_beta<<<1, 1, 0, s>>>(sc, inew, iold, ibeta, log2);
+}
+void sc_copy(hipStream_t s, const double *src, double *dst) { k_copy_scalar<<<1, 1, 0, s>>>(src, dst); }
+}  // namespace uftk

@@ -1,0 +1,1028 @@
+// C-ABI implementation: context lifetime, mesh/BC ingest, setup, and the
+// host orchestration of the device design iteration (main.cpp:64-119).
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+
+#include "uft_internal.h"
+
+// ---------------------------------------------------------------- errors
+int uft_set_error(uft_ctx *C, int code, const char *fmt, ...) {
+  if (C) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    C->err = buf;
+  }
+  return code;
+}
+
+namespace {
+
+// device scalar slots
+enum {
+  S_RZ = 0, S_PQ = 1, S_ALPHA = 2, S_RR = 3, S_RZNEW = 4, S_BETA = 5,
+  S_F0 = 8,  // filter: 8 ||Db||^2, 9 ||z||^2, 10 z.r (beta), 11 beta_old, 12 pq, 13 alpha
+  S_MISC = 16,
+  S_RED = 32  // general reduction outputs (up to 48 values)
+};
+
+int dalloc(uft_ctx *C, void **p, size_t bytes) {
+  if (bytes == 0) bytes = 8;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return uft_set_error(C, UFT_ERR_OOM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+  }
+  C->allocs.push_back(*p);
+  return UFT_OK;
+}
+template <typename T>
+int dalloc_n(uft_ctx *C, T **p, int64_t n) {
+  return dalloc(C, (void **)p, (size_t)std::max<int64_t>(n, 1) * sizeof(T));
+}
+template <typename T>
+int upload(uft_ctx *C, T **p, const std::vector<T> &h) {
+  int rc = dalloc_n(C, p, (int64_t)h.size());
+  if (rc) return rc;
+  if (!h.empty()) UFT_HIP(C, hipMemcpyAsync(*p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, C->s));
+  return UFT_OK;
+}
+
+void free_all(uft_ctx *C) {
+  if (C->s) (void)hipStreamSynchronize(C->s);
+  for (void *p : C->allocs) (void)hipFree(p);
+  C->allocs.clear();
+}
+
+int sync(uft_ctx *C) {
+  UFT_HIP(C, hipStreamSynchronize(C->s));
+  UFT_HIP(C, hipGetLastError());
+  return UFT_OK;
+}
+
+// copy n device scalars to the pinned mirror and wait
+int fetch(uft_ctx *C, const double *dsrc, int n, double *hdst) {
+  UFT_HIP(C, hipMemcpyAsync(hdst, dsrc, n * sizeof(double), hipMemcpyDeviceToHost, C->s));
+  return sync(C);
+}
+
+bool valid_vec(const uft_ctx *C, int id) {
+  return id >= 0 && id < (int)C->vec.size() && C->vec[id] != nullptr;
+}
+
+// ---------------------------------------------------------------- operators
+// y = A_l x on multigrid level l
+void level_apply(uft_ctx *C, int l, const double *x, double *y) {
+  AmgLevel &Lv = C->amg.L[l];
+  if (l == 0)
+    uftk::apply_K(C->s, C, C->cur_xphys, x, y, nullptr, nullptr);
+  else
+    uftk::bsr6_mv(C->s, Lv.nunit, Lv.d_rp, Lv.d_ci, Lv.d_val, x, y);
+}
+
+// Chebyshev interval [0.1, 1.1] * lambda_max(D^-1 A), the PCGAMG default
+// (-mg_levels_ksp_chebyshev_esteig 0,0.1,0,1.1).
+struct ChebCoef {
+  double inv_theta, c1[16], c2[16];
+};
+ChebCoef cheb_coef(double lmax, int deg) {
+  ChebCoef cc{};
+  const double emax = 1.1 * lmax, emin = 0.1 * lmax;
+  const double theta = 0.5 * (emax + emin), delta = 0.5 * (emax - emin);
+  const double sigma = theta / delta;
+  cc.inv_theta = 1.0 / theta;
+  double rho = 1.0 / sigma;
+  for (int k = 1; k < deg && k < 16; ++k) {
+    const double rn = 1.0 / (2.0 * sigma - rho);
+    cc.c1[k] = rn * rho;
+    cc.c2[k] = 2.0 * rn / delta;
+    rho = rn;
+  }
+  return cc;
+}
+
+// V-cycle: x = M_l b (x overwritten)
+void vcycle(uft_ctx *C, int l, const double *b, double *x) {
+  AmgHier &H = C->amg;
+  hipStream_t s = C->s;
+  if (l == H.nlev - 1) {
+    uftk::dense_mv(s, H.ncoarse, H.d_cinv, b, x);
+    return;
+  }
+  AmgLevel &Lv = H.L[l];
+  AmgLevel &Ln = H.L[l + 1];
+  const int64_t n = Lv.n;
+  const int deg = std::max(1, C->P.cheb_degree);
+  const ChebCoef cc = cheb_coef(Lv.lmax, deg);
+  // pre-smoothing from x = 0
+  uftk::cheb_first(s, n, b, Lv.d_dinv, cc.inv_theta, Lv.r, Lv.d, x, 1);
+  for (int k = 1; k < deg; ++k) {
+    level_apply(C, l, Lv.d, Lv.t);
+    uftk::cheb_step(s, n, Lv.t, Lv.d_dinv, cc.c1[k], cc.c2[k], Lv.r, Lv.d, x);
+  }
+  level_apply(C, l, Lv.d, Lv.t);
+  uftk::sub_inplace(s, n, Lv.t, Lv.r);  // r = b - A x
+  // coarse-grid correction
+  uftk::restrict_(s, Lv.bs, Ln.nunit, Lv.d_cptr, Lv.d_child, Lv.d_P, Lv.r, Ln.b);
+  vcycle(C, l + 1, Ln.b, Ln.x);
+  uftk::prolong_add(s, Lv.bs, Lv.nunit, Lv.d_parent, Lv.d_P, Ln.x, x);
+  // post-smoothing
+  level_apply(C, l, x, Lv.t);
+  uftk::sub(s, n, b, Lv.t, Lv.r);
+  uftk::cheb_first(s, n, b, Lv.d_dinv, cc.inv_theta, Lv.r, Lv.d, x, 0);
+  for (int k = 1; k < deg; ++k) {
+    level_apply(C, l, Lv.d, Lv.t);
+    uftk::cheb_step(s, n, Lv.t, Lv.d_dinv, cc.c1[k], cc.c2[k], Lv.r, Lv.d, x);
+  }
+}
+
+// largest eigenvalue of a symmetric tridiagonal matrix (bisection, Sturm counts)
+double tridiag_lmax(const std::vector<double> &d, const std::vector<double> &e) {
+  const int k = (int)d.size();
+  double lo = 1e300, hi = -1e300;
+  for (int i = 0; i < k; ++i) {
+    const double r = (i > 0 ? std::fabs(e[i - 1]) : 0.0) + (i + 1 < k ? std::fabs(e[i]) : 0.0);
+    lo = std::min(lo, d[i] - r);
+    hi = std::max(hi, d[i] + r);
+  }
+  auto count_below = [&](double x) {  // number of eigenvalues < x
+    int c = 0;
+    double q = d[0] - x;
+    if (q < 0) ++c;
+    for (int i = 1; i < k; ++i) {
+      const double qq = (q == 0.0) ? 1e-300 : q;
+      q = d[i] - x - e[i - 1] * e[i - 1] / qq;
+      if (q < 0) ++c;
+    }
+    return c;
+  };
+  for (int it = 0; it < 200 && hi - lo > 1e-12 * std::max(1.0, std::fabs(hi)); ++it) {
+    const double mid = 0.5 * (lo + hi);
+    if (count_below(mid) >= k) hi = mid; else lo = mid;
+  }
+  return hi;
+}
+
+// lambda_max(D^-1 A_l) by k steps of Jacobi-PCG (Lanczos from the CG
+// coefficients).  The scalars stay on the device until the end.
+int estimate_lmax(uft_ctx *C, int l, int ksteps) {
+  AmgLevel &Lv = C->amg.L[l];
+  hipStream_t s = C->s;
+  const int64_t n = Lv.n;
+  double *x = Lv.x, *r = Lv.r, *z = Lv.d, *p = Lv.b, *q = Lv.t;
+  // deterministic start vector: r = 1 + small alternating perturbation
+  uftk::fill(s, n, 1.0, r);
+  uftk::scale(s, n, r, Lv.d_dinv, z);
+  uftk::copy(s, n, z, p);
+  uftk::dot(s, n, r, z, C->d_part, C->d_scal + S_MISC);  // rz in S_MISC
+  double *log = C->d_lanczos;
+  for (int j = 0; j < ksteps; ++j) {
+    level_apply(C, l, p, q);
+    uftk::dot(s, n, p, q, C->d_part, C->d_scal + S_MISC + 1);
+    uftk::sc_div(s, C->d_scal, S_MISC, S_MISC + 1, S_MISC + 2);  // alpha
+    uftk::sc_copy(s, C->d_scal + S_MISC + 2, log + 3 * j);       // log alpha
+    uftk::cg_update(s, n, C->d_scal, S_MISC + 2, x, p, r, q, C->d_part);
+    uftk::scale(s, n, r, Lv.d_dinv, z);
+    uftk::dot(s, n, r, z, C->d_part, C->d_scal + S_MISC + 3);
+    uftk::sc_beta(s, C->d_scal, S_MISC + 3, S_MISC, S_MISC + 4, nullptr);
+    uftk::sc_copy(s, C->d_scal + S_MISC + 4, log + 3 * j + 1);  // log beta
+    uftk::xpby(s, n, z, C->d_scal, S_MISC + 4, p);
+  }
+  std::vector<double> hl(3 * ksteps);
+  UFT_HIP(C, hipMemcpyAsync(hl.data(), log, hl.size() * sizeof(double), hipMemcpyDeviceToHost, C->s));
+  int rc = sync(C);
+  if (rc) return rc;
+  std::vector<double> d, e;
+  for (int j = 0; j < ksteps; ++j) {
+    const double al = hl[3 * j], be = hl[3 * j + 1];
+    if (!(al > 0.0) || !std::isfinite(al)) break;
+    double dj = 1.0 / al;
+    if (j > 0) dj += hl[3 * (j - 1) + 1] / hl[3 * (j - 1)];
+    d.push_back(dj);
+    if (!(be > 0.0) || !std::isfinite(be)) break;
+    e.push_back(std::sqrt(be) / al);
+  }
+  if (d.empty()) {
+    Lv.lmax = 2.0;
+    return UFT_OK;
+  }
+  e.resize(d.size());
+  Lv.lmax = tridiag_lmax(d, e);
+  return UFT_OK;
+}
+
+// Rebuild the density-dependent parts of the hierarchy for xphys.
+int amg_refresh(uft_ctx *C, const double *xphys) {
+  AmgHier &H = C->amg;
+  hipStream_t s = C->s;
+  C->cur_xphys = xphys;
+  uftk::diag_K(s, C, xphys, H.L[0].d_dinv, H.L[0].d_dinvf);
+  for (int l = 0; l + 1 < H.nlev; ++l) {
+    AmgLevel &Lv = H.L[l], &Ln = H.L[l + 1];
+    if (l == 0)
+      uftk::galerkin_fine(s, C, xphys, Ln.nblk, Lv.d_glptr, Lv.d_gl, Lv.d_P, Ln.d_val);
+    else
+      uftk::galerkin_coarse(s, Ln.nblk, Lv.d_glptr, Lv.d_gl, Lv.d_brow, Lv.d_ci, Lv.d_P, Lv.d_val, Ln.d_val);
+    uftk::fixed_unit_diag(s, Ln.nunit, Ln.d_rp, Ln.d_ci, Ln.d_val);
+    uftk::bsr6_dinv(s, Ln.nunit, Ln.d_rp, Ln.d_ci, Ln.d_val, Ln.d_dinv, Ln.d_dinvf);
+  }
+  AmgLevel &Lc = H.L[H.nlev - 1];
+  uftk::bsr_to_dense(s, Lc.nunit, Lc.d_rp, Lc.d_ci, Lc.d_val, H.ncoarse, H.d_cmat);
+  uftk::dense_spd_inverse(s, H.ncoarse, H.d_cmat, H.d_cinv, H.d_cinvf);
+  for (int l = 0; l + 1 < H.nlev; ++l) {
+    int rc = estimate_lmax(C, l, 12);
+    if (rc) return rc;
+  }
+  return UFT_OK;
+}
+
+// AMG-preconditioned CG on K(x) U = F from U = 0; converged when
+// ||F - K U|| <= rtol ||F|| (the reference's FGMRES uses right
+// preconditioning, i.e. the unpreconditioned residual, SolveFem.cpp:126-134).
+int solve_el(uft_ctx *C, const double *xphys, double *U) {
+  hipStream_t s = C->s;
+  const int64_t n = C->ndof;
+  double *r = C->d_r, *z = C->d_z, *p = C->d_p, *q = C->d_q;
+  int rc = amg_refresh(C, xphys);
+  if (rc) return rc;
+  uftk::fill(s, n, 0.0, U);
+  uftk::copy(s, n, C->d_F, r);
+  uftk::dot(s, n, r, r, C->d_part, C->d_scal + S_RR);
+  rc = fetch(C, C->d_scal + S_RR, 1, C->h_scal);
+  if (rc) return rc;
+  const double bn = std::sqrt(C->h_scal[0]);
+  C->fnorm = bn;
+  C->it_el = 0;
+  C->outer_el = 1;
+  if (bn == 0.0) {
+    C->relres_el = 0.0;
+    return UFT_OK;
+  }
+  vcycle(C, 0, r, z);
+  uftk::dot(s, n, r, z, C->d_part, C->d_scal + S_RZ);
+  uftk::copy(s, n, z, p);
+  const double tol = C->P.rtol_el * bn;
+  int it = 0;
+  bool conv = false;
+  while (it < C->P.maxit_el) {
+    uftk::apply_K(s, C, xphys, p, q, p, C->d_part);
+    uftk::final_sum(s, C->d_part, uftk::red_blocks(C->nn), 1, C->d_scal + S_PQ);
+    uftk::sc_div(s, C->d_scal, S_RZ, S_PQ, S_ALPHA);
+    uftk::cg_update(s, n, C->d_scal, S_ALPHA, U, p, r, q, C->d_part);
+    uftk::final_sum(s, C->d_part, uftk::red_blocks(n), 1, C->d_scal + S_RR);
+    ++it;
+    rc = fetch(C, C->d_scal + S_RR, 1, C->h_scal);
+    if (rc) return rc;
+    const double rn = std::sqrt(C->h_scal[0]);
+    if (!std::isfinite(rn)) return uft_set_error(C, UFT_ERR_NOCONV, "elasticity CG diverged at iteration %d", it);
+    if (rn <= tol) {
+      conv = true;
+      break;
+    }
+    vcycle(C, 0, r, z);
+    uftk::dot(s, n, r, z, C->d_part, C->d_scal + S_RZNEW);
+    uftk::sc_beta(s, C->d_scal, S_RZNEW, S_RZ, S_BETA, nullptr);
+    uftk::xpby(s, n, z, C->d_scal, S_BETA, p);
+  }
+  C->it_el = it;
+  // true residual (SolveFem.cpp:142-152)
+  uftk::apply_K(s, C, xphys, U, q, nullptr, nullptr);
+  uftk::sub(s, n, C->d_F, q, q);
+  uftk::dot(s, n, q, q, C->d_part, C->d_scal + S_MISC);
+  rc = fetch(C, C->d_scal + S_MISC, 1, C->h_scal);
+  if (rc) return rc;
+  C->relres_el = std::sqrt(C->h_scal[0]) / bn;
+  if (!conv)
+    return uft_set_error(C, UFT_ERR_NOCONV, "elasticity solve: %d iterations, relative residual %.3e > %.3e", it,
+                         C->relres_el, C->P.rtol_el);
+  return UFT_OK;
+}
+
+// Jacobi-CG on KF RX = TX with the reference's KSPCG settings
+// (PDEFilter.cpp:117-137): warm start from the previous RX, preconditioned
+// residual norm relative to ||D^-1 b||, atol 1e-50, dtol 1e5.
+int solve_filter(uft_ctx *C, const double *b, double *x) {
+  hipStream_t s = C->s;
+  const int64_t n = C->nn;
+  double *r = C->d_fr, *z = C->d_fz, *p = C->d_fp, *q = C->d_fq;
+  double *sc = C->d_scal;
+  uftk::apply_KF(s, C, x, q);
+  uftk::fcg_init(s, n, b, q, C->d_fdinv, r, z, C->d_part);
+  uftk::final_sum(s, C->d_part, uftk::red_blocks(n), 3, sc + S_F0);  // 8: |Db|^2 9: |z|^2 10: z.r
+  int rc = fetch(C, sc + S_F0, 2, C->h_scal);
+  if (rc) return rc;
+  double snorm = std::sqrt(C->h_scal[0]);
+  double dp = std::sqrt(C->h_scal[1]);
+  if (snorm == 0.0) snorm = dp;
+  const double ttol = std::max(C->P.rtol_f * snorm, 1e-50);
+  int it = 0;
+  C->it_f = 0;
+  if (dp <= ttol) return UFT_OK;
+  while (it < C->P.maxit_f) {
+    uftk::fcg_p(s, n, z, sc, S_F0 + 2, S_F0 + 3, it == 0, p);
+    uftk::apply_KF(s, C, p, q);
+    uftk::dot(s, n, p, q, C->d_part, sc + S_F0 + 4);
+    uftk::sc_div(s, sc, S_F0 + 2, S_F0 + 4, S_F0 + 5);
+    uftk::fcg_update(s, n, sc, S_F0 + 5, x, p, r, q, C->d_fdinv, z, C->d_part);
+    uftk::sc_copy(s, sc + S_F0 + 2, sc + S_F0 + 3);                     // beta_old = beta
+    uftk::final_sum(s, C->d_part, uftk::red_blocks(n), 2, sc + S_F0 + 6);  // 14: |z|^2 15: z.r
+    uftk::sc_copy(s, sc + S_F0 + 7, sc + S_F0 + 2);                     // beta = z.r
+    ++it;
+    rc = fetch(C, sc + S_F0 + 6, 1, C->h_scal);
+    if (rc) return rc;
+    dp = std::sqrt(C->h_scal[0]);
+    if (dp <= ttol) break;
+    if (!(dp < 1e5 * snorm)) {
+      C->it_f = it;
+      return uft_set_error(C, UFT_ERR_NOCONV, "filter CG diverged at iteration %d", it);
+    }
+  }
+  C->it_f = it;
+  if (it >= C->P.maxit_f && dp > ttol)
+    return uft_set_error(C, UFT_ERR_NOCONV, "filter CG: %d iterations without convergence", it);
+  return UFT_OK;
+}
+
+int check_setup(uft_ctx *C) {
+  if (!C) return UFT_ERR_ARG;
+  if (!C->is_setup) return uft_set_error(C, UFT_ERR_STATE, "uft_setup has not been called");
+  return UFT_OK;
+}
+
+}  // namespace
+
+// ==========================================================================
+extern "C" {
+
+const char *uft_last_error(const uft_ctx *C) { return C ? C->err.c_str() : "null context"; }
+
+void uft_default_params(uft_params *p) {
+  std::memset(p, 0, sizeof(*p));
+  // OptPara::SetUp (OptPara.cpp:123-146)
+  p->maxItr = 200; p->rmin = 0.01; p->penal = 3.0; p->Emin = 1.0e-9; p->Emax = 1.0e6; p->movlim = 0.2;
+  p->Xmin = 0.01; p->Xmax = 1.0; p->volfrac = 0.12; p->x0 = 0.5; p->beta0 = 1.0; p->betamax = 8.0; p->proj = 0;
+  p->m = 1;
+  p->nu = 0.3;                          // SolveFem.cpp:8
+  p->rtol_el = 1.0e-5;                  // SolveFem.cpp:134
+  p->rtol_f = 1.0e-5;                   // PDEFilter.cpp:117
+  p->maxit_el = 10000;
+  p->maxit_f = 10000;                   // PDEFilter.cpp:120
+  p->precision = UFT_FP64;
+  p->amg_levels = 4;                    // SolveFem.cpp:131
+  p->cheb_degree = 2;
+  p->coarse_max = 600;
+  p->inner_rtol_exp = 4;
+}
+
+int uft_create(uft_ctx **out, int device) {
+  if (!out) return UFT_ERR_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return UFT_ERR_HIP;
+  if (device < 0 || device >= ndev) return UFT_ERR_ARG;
+  uft_ctx *C = new uft_ctx();
+  C->device = device;
+  uft_default_params(&C->P);
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&C->s, hipStreamNonBlocking) != hipSuccess) {
+    delete C;
+    return UFT_ERR_HIP;
+  }
+  if (hipHostMalloc((void **)&C->h_scal, 256 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+    (void)hipStreamDestroy(C->s);
+    delete C;
+    return UFT_ERR_HIP;
+  }
+  *out = C;
+  return UFT_OK;
+}
+
+void uft_destroy(uft_ctx *C) {
+  if (!C) return;
+  (void)hipSetDevice(C->device);
+  free_all(C);
+  if (C->h_scal) (void)hipHostFree(C->h_scal);
+  if (C->s) (void)hipStreamDestroy(C->s);
+  delete C;
+}
+
+int uft_params_set(uft_ctx *C, const uft_params *p) {
+  if (!C || !p) return UFT_ERR_ARG;
+  if (p->m < 1 || p->m > UFT_MAXM) return uft_set_error(C, UFT_ERR_ARG, "m must be in [1, %d]", UFT_MAXM);
+  if (!(p->rtol_el > 0) || !(p->rtol_f > 0) || p->maxit_el < 1 || p->maxit_f < 1)
+    return uft_set_error(C, UFT_ERR_ARG, "bad solver tolerances");
+  if (C->is_setup && (p->m != C->P.m || p->rmin != C->P.rmin || p->nu != C->P.nu || p->precision != C->P.precision ||
+                      p->amg_levels != C->P.amg_levels || p->coarse_max != C->P.coarse_max))
+    return uft_set_error(C, UFT_ERR_STATE, "m, rmin, nu, precision and the hierarchy shape are fixed at setup");
+  C->P = *p;
+  return UFT_OK;
+}
+int uft_params_get(const uft_ctx *C, uft_params *p) {
+  if (!C || !p) return UFT_ERR_ARG;
+  *p = C->P;
+  return UFT_OK;
+}
+
+int uft_mesh_set(uft_ctx *C, int npe, int64_t nn, int64_t ne, const int32_t *elem, const double *xyz) {
+  if (!C) return UFT_ERR_ARG;
+  if (C->is_setup) return uft_set_error(C, UFT_ERR_STATE, "mesh already set up");
+  if ((npe != 8 && npe != 4) || nn <= 0 || ne <= 0 || !elem || !xyz)
+    return uft_set_error(C, UFT_ERR_ARG, "uft_mesh_set: bad arguments");
+  if (nn >= (int64_t)INT32_MAX / 3 || ne * npe * 36 >= (int64_t)INT32_MAX * 8)
+    return uft_set_error(C, UFT_ERR_ARG, "mesh too large");
+  for (int64_t i = 0; i < ne * npe; ++i)
+    if (elem[i] < 0 || elem[i] >= nn)
+      return uft_set_error(C, UFT_ERR_ARG, "element %lld references node %d outside [0,%lld)", (long long)(i / npe),
+                           elem[i], (long long)nn);
+  C->npe = npe;
+  C->nd = 3 * npe;
+  C->nn = nn;
+  C->ne = ne;
+  C->ndof = 3 * nn;
+  C->elem.assign(elem, elem + ne * npe);
+  C->xyz.assign(xyz, xyz + nn * 3);
+  C->have_mesh = true;
+  return UFT_OK;
+}
+
+int uft_bc_set(uft_ctx *C, int nfixed, const int32_t *fixed, int nlc, int nload, const int32_t *ld,
+               const double *lv) {
+  if (!C) return UFT_ERR_ARG;
+  if (!C->have_mesh) return uft_set_error(C, UFT_ERR_STATE, "uft_bc_set before uft_mesh_set");
+  if (C->is_setup) return uft_set_error(C, UFT_ERR_STATE, "boundary conditions are fixed after setup");
+  if (nfixed < 0 || nlc < 1 || nload < 0 || (nfixed > 0 && !fixed) || (nload > 0 && (!ld || !lv)))
+    return uft_set_error(C, UFT_ERR_ARG, "uft_bc_set: bad arguments");
+  for (int i = 0; i < nfixed; ++i)
+    if (fixed[i] < 0 || fixed[i] >= C->ndof) return uft_set_error(C, UFT_ERR_ARG, "fixed DOF %d out of range", fixed[i]);
+  for (int64_t i = 0; i < (int64_t)nlc * nload; ++i)
+    if (ld[i] < 0 || ld[i] >= C->ndof) return uft_set_error(C, UFT_ERR_ARG, "load DOF %d out of range", ld[i]);
+  C->fixed.assign(fixed, fixed + nfixed);
+  C->nlc = nlc;
+  C->nload = nload;
+  C->ldofs.assign(ld, ld + (int64_t)nlc * nload);
+  C->lvals.assign(lv, lv + (int64_t)nlc * nload);
+  C->have_bc = true;
+  return UFT_OK;
+}
+
+int uft_setup(uft_ctx *C) {
+  if (!C) return UFT_ERR_ARG;
+  if (!C->have_mesh || !C->have_bc) return uft_set_error(C, UFT_ERR_STATE, "mesh and BCs must be set before setup");
+  if (C->is_setup) return uft_set_error(C, UFT_ERR_STATE, "already set up");
+  UFT_HIP(C, hipSetDevice(C->device));
+  const int npe = C->npe, nd = C->nd, m = C->P.m;
+  const int64_t ne = C->ne, nn = C->nn, ndof = C->ndof;
+  const int np = nd * (nd + 1) / 2, npf = npe * (npe + 1) / 2;
+  hipStream_t s = C->s;
+  int rc;
+  // ---- mesh on device (element ids structure-of-arrays)
+  {
+    std::vector<int32_t> soa((size_t)ne * npe);
+    for (int64_t e = 0; e < ne; ++e)
+      for (int a = 0; a < npe; ++a) soa[(size_t)a * ne + e] = C->elem[e * npe + a];
+    if ((rc = upload(C, &C->d_elem, soa))) return rc;
+    if ((rc = upload(C, &C->d_xyz, C->xyz))) return rc;
+    std::vector<uint8_t> fm(nn, 0);
+    for (int32_t d : C->fixed) fm[d / 3] |= (uint8_t)(1u << (d % 3));
+    if ((rc = upload(C, &C->d_fixm, fm))) return rc;
+    std::vector<int32_t> iptr, inc;
+    uft_node_incidence(nn, ne, npe, C->elem.data(), iptr, inc);
+    if ((rc = upload(C, &C->d_iptr, iptr))) return rc;
+    if ((rc = upload(C, &C->d_inc, inc))) return rc;
+    UFT_HIP(C, hipStreamSynchronize(s));
+  }
+  // ---- vectors
+  C->vec.assign(UFT_DGDX0 + m, nullptr);
+  C->vlen.assign(UFT_DGDX0 + m, 0);
+  const int elem_ids[] = {UFT_X, UFT_XFILED, UFT_XPHYS, UFT_XOLD, UFT_DFDX, UFT_XMIN, UFT_XMAX, UFT_VOL};
+  for (int id : elem_ids) {
+    if ((rc = dalloc_n(C, &C->vec[id], ne))) return rc;
+    C->vlen[id] = ne;
+  }
+  for (int j = 0; j < m; ++j) {
+    if ((rc = dalloc_n(C, &C->vec[UFT_DGDX0 + j], ne))) return rc;
+    C->vlen[UFT_DGDX0 + j] = ne;
+    uftk::fill(s, ne, 0.0, C->vec[UFT_DGDX0 + j]);
+  }
+  if ((rc = dalloc_n(C, &C->vec[UFT_U], ndof))) return rc;
+  C->vlen[UFT_U] = ndof;
+  if ((rc = dalloc_n(C, &C->vec[UFT_RX], nn))) return rc;
+  C->vlen[UFT_RX] = nn;
+  // ---- element operators
+  if ((rc = dalloc_n(C, &C->d_KE, (int64_t)np * ne))) return rc;
+  if (C->P.precision == UFT_MIXED) {
+    if ((rc = dalloc_n(C, &C->d_KEf, (int64_t)np * ne))) return rc;
+    if ((rc = dalloc_n(C, &C->d_Yef, (int64_t)nd * ne))) return rc;
+  }
+  if ((rc = dalloc_n(C, &C->d_KF, (int64_t)npf * ne))) return rc;
+  if ((rc = dalloc_n(C, &C->d_T, (int64_t)npe * ne))) return rc;
+  if ((rc = dalloc_n(C, &C->d_detmin, ne))) return rc;
+  if ((rc = dalloc_n(C, &C->d_Ye, (int64_t)nd * ne))) return rc;
+  const int64_t maxn = std::max(ndof, ne);
+  if ((rc = dalloc_n(C, &C->d_part, (maxn / 256 + 2) * 48))) return rc;
+  if ((rc = dalloc_n(C, &C->d_scal, 256))) return rc;
+  if ((rc = dalloc_n(C, &C->d_lanczos, 3 * 64))) return rc;
+  uftk::fill(s, 256, 0.0, C->d_scal);
+  uftk::gen_ke(s, npe, ne, C->d_elem, C->d_xyz, C->P.nu, C->d_KE, C->d_KEf, C->d_detmin);
+  uftk::gen_kf(s, npe, ne, C->d_elem, C->d_xyz, C->P.rmin, C->d_KF, C->d_T, C->vec[UFT_VOL], C->d_part);
+  uftk::final_sum(s, C->d_part, uftk::red_blocks(ne), 1, C->d_scal + S_MISC);
+  {
+    std::vector<double> dm(ne);
+    UFT_HIP(C, hipMemcpyAsync(dm.data(), C->d_detmin, ne * sizeof(double), hipMemcpyDeviceToHost, s));
+    if ((rc = fetch(C, C->d_scal + S_MISC, 1, C->h_scal))) return rc;
+    C->TV = C->h_scal[0];
+    for (int64_t e = 0; e < ne; ++e)
+      if (!(dm[e] > 0.0))
+        return uft_set_error(C, UFT_ERR_DETJ, "element %lld has non-positive Jacobian determinant %.6e", (long long)e,
+                             dm[e]);
+  }
+  // ---- work vectors
+  double **dv[] = {&C->d_r, &C->d_z, &C->d_p, &C->d_q, &C->d_F, &C->d_w};
+  for (double **p : dv)
+    if ((rc = dalloc_n(C, p, ndof))) return rc;
+  double **fv[] = {&C->d_fr, &C->d_fz, &C->d_fp, &C->d_fq, &C->d_fdinv, &C->d_TX};
+  for (double **p : fv)
+    if ((rc = dalloc_n(C, p, nn))) return rc;
+  if (C->P.precision == UFT_MIXED) {
+    float **ff[] = {&C->d_rf, &C->d_zf, &C->d_pf, &C->d_qf, &C->d_uf};
+    for (float **p : ff)
+      if ((rc = dalloc_n(C, p, ndof))) return rc;
+  }
+  // ---- load vector: INSERT per case, then zero the fixed DOFs (SolveFem.cpp:277-333)
+  {
+    uftk::fill(s, ndof, 0.0, C->d_F);
+    int32_t *dld;
+    double *dlv;
+    if ((rc = upload(C, &dld, C->ldofs))) return rc;
+    if ((rc = upload(C, &dlv, C->lvals))) return rc;
+    for (int lc = 0; lc < C->nlc; ++lc)
+      uftk::set_dofs(s, C->nload, dld + (size_t)lc * C->nload, dlv + (size_t)lc * C->nload, C->d_F);
+    int32_t *dfx;
+    if ((rc = upload(C, &dfx, C->fixed))) return rc;
+    uftk::set_dofs(s, (int)C->fixed.size(), dfx, nullptr, C->d_F);
+  }
+  // ---- filter diagonal (KF is density independent)
+  uftk::diag_KF(s, C, C->d_fdinv);
+  // ---- multigrid hierarchy
+  {
+    std::vector<AmgHostLevel> H;
+    std::vector<int64_t> nun;
+    if ((rc = uft_amg_build_host(C, H, nun))) return rc;
+    AmgHier &A = C->amg;
+    A.nlev = (int)nun.size();
+    for (int l = 0; l < A.nlev; ++l) {
+      AmgLevel &Lv = A.L[l];
+      Lv.nunit = nun[l];
+      Lv.bs = l == 0 ? 3 : 6;
+      Lv.n = Lv.nunit * Lv.bs;
+      double **wv[] = {&Lv.x, &Lv.b, &Lv.r, &Lv.d, &Lv.t};
+      for (double **p : wv)
+        if ((rc = dalloc_n(C, p, Lv.n))) return rc;
+      if ((rc = dalloc_n(C, &Lv.d_dinv, Lv.n))) return rc;
+      if (C->P.precision == UFT_MIXED) {
+        float **wf[] = {&Lv.xf, &Lv.bf, &Lv.rf, &Lv.df, &Lv.tf};
+        for (float **p : wf)
+          if ((rc = dalloc_n(C, p, Lv.n))) return rc;
+        if ((rc = dalloc_n(C, &Lv.d_dinvf, Lv.n))) return rc;
+      }
+    }
+    for (int l = 0; l + 1 < A.nlev; ++l) {
+      AmgHostLevel &h = H[l];
+      AmgLevel &Lv = A.L[l], &Ln = A.L[l + 1];
+      if ((rc = upload(C, &Lv.d_P, h.P))) return rc;
+      if (C->P.precision == UFT_MIXED) {
+        std::vector<float> pf(h.P.begin(), h.P.end());
+        if ((rc = upload(C, &Lv.d_Pf, pf))) return rc;
+      }
+      if ((rc = upload(C, &Lv.d_parent, h.parent))) return rc;
+      if ((rc = upload(C, &Lv.d_cptr, h.cptr))) return rc;
+      if ((rc = upload(C, &Lv.d_child, h.child))) return rc;
+      if ((rc = upload(C, &Lv.d_glptr, h.glptr))) return rc;
+      if ((rc = upload(C, &Lv.d_gl, h.gl))) return rc;
+      Lv.ngl = (int64_t)h.gl.size();
+      Ln.nblk = (int64_t)h.ci.size();
+      if ((rc = upload(C, &Ln.d_rp, h.rp))) return rc;
+      if ((rc = upload(C, &Ln.d_ci, h.ci))) return rc;
+      std::vector<int32_t> brow(Ln.nblk);
+      for (int64_t A2 = 0; A2 < Ln.nunit; ++A2)
+        for (int32_t k = h.rp[A2]; k < h.rp[A2 + 1]; ++k) brow[k] = (int32_t)A2;
+      if ((rc = upload(C, &Ln.d_brow, brow))) return rc;
+      if ((rc = dalloc_n(C, &Ln.d_val, Ln.nblk * 36))) return rc;
+      if (C->P.precision == UFT_MIXED)
+        if ((rc = dalloc_n(C, &Ln.d_valf, Ln.nblk * 36))) return rc;
+      UFT_HIP(C, hipStreamSynchronize(s));  // host vectors of H go out of scope level by level
+    }
+    A.ncoarse = A.L[A.nlev - 1].n;
+    if ((rc = dalloc_n(C, &A.d_cmat, 2 * A.ncoarse * A.ncoarse))) return rc;
+    if ((rc = dalloc_n(C, &A.d_cinv, A.ncoarse * A.ncoarse))) return rc;
+    if (C->P.precision == UFT_MIXED)
+      if ((rc = dalloc_n(C, &A.d_cinvf, A.ncoarse * A.ncoarse))) return rc;
+    UFT_HIP(C, hipStreamSynchronize(s));
+  }
+  // ---- design state (OptPara.cpp:150-153) and MMA (OptPara.cpp:158-177, MMA.cpp:178-224)
+  for (int id : {UFT_X, UFT_XFILED, UFT_XPHYS, UFT_XOLD}) uftk::fill(s, ne, C->P.volfrac, C->vec[id]);
+  for (int id : {UFT_DFDX, UFT_XMIN, UFT_XMAX}) uftk::fill(s, ne, 0.0, C->vec[id]);
+  uftk::fill(s, ndof, 0.0, C->vec[UFT_U]);
+  uftk::fill(s, nn, 0.0, C->vec[UFT_RX]);
+  double **mv[] = {&C->d_L, &C->d_U, &C->d_alpha, &C->d_beta, &C->d_p0, &C->d_q0, &C->d_xo1, &C->d_xo2};
+  for (double **p : mv) {
+    if ((rc = dalloc_n(C, p, ne))) return rc;
+    uftk::fill(s, ne, 0.0, *p);
+  }
+  if ((rc = dalloc_n(C, &C->d_pij, (int64_t)m * ne))) return rc;
+  if ((rc = dalloc_n(C, &C->d_qij, (int64_t)m * ne))) return rc;
+  C->mk = 0;
+  for (int j = 0; j < m; ++j) {
+    C->a[j] = 0.0;
+    C->c[j] = 1000.0;
+    C->dd[j] = 0.0;
+  }
+  C->beta = C->P.beta0;
+  C->btemp = C->P.beta0;
+  if ((rc = sync(C))) return rc;
+  C->is_setup = true;
+  return UFT_OK;
+}
+
+int uft_mesh_get(const uft_ctx *C, int32_t *elem, double *xyz) {
+  if (!C || !elem || !xyz || !C->have_mesh) return UFT_ERR_ARG;
+  std::memcpy(elem, C->elem.data(), C->elem.size() * sizeof(int32_t));
+  std::memcpy(xyz, C->xyz.data(), C->xyz.size() * sizeof(double));
+  return UFT_OK;
+}
+
+int64_t uft_size(const uft_ctx *C, int which) {
+  if (!C) return -1;
+  switch (which) {
+    case UFT_SIZE_NE: return C->ne;
+    case UFT_SIZE_NN: return C->nn;
+    case UFT_SIZE_NDOF: return C->ndof;
+    case UFT_SIZE_NPE: return C->npe;
+  }
+  return -1;
+}
+double uft_total_volume(const uft_ctx *C) { return C ? C->TV : 0.0; }
+
+int uft_vec_get(uft_ctx *C, int id, double *host, int64_t n) {
+  int rc = check_setup(C);
+  if (rc) return rc;
+  if (!valid_vec(C, id) || !host || n != C->vlen[id]) return uft_set_error(C, UFT_ERR_ARG, "uft_vec_get: bad id/size");
+  UFT_HIP(C, hipMemcpyAsync(host, C->vec[id], n * sizeof(double), hipMemcpyDeviceToHost, C->s));
+  return sync(C);
+}
+int uft_vec_set(uft_ctx *C, int id, const double *host, int64_t n) {
+  int rc = check_setup(C);
+  if (rc) return rc;
+  if (!valid_vec(C, id) || !host || n != C->vlen[id]) return uft_set_error(C, UFT_ERR_ARG, "uft_vec_set: bad id/size");
+  UFT_HIP(C, hipMemcpyAsync(C->vec[id], host, n * sizeof(double), hipMemcpyHostToDevice, C->s));
+  return sync(C);
+}
+int uft_vec_fill(uft_ctx *C, int id, double v) {
+  int rc = check_setup(C);
+  if (rc) return rc;
+  if (!valid_vec(C, id)) return uft_set_error(C, UFT_ERR_ARG, "uft_vec_fill: bad id");
+  uftk::fill(C->s, C->vlen[id], v, C->vec[id]);
+  return UFT_OK;
+}
+int uft_vec_copy(uft_ctx *C, int src, int dst) {
+  int rc = check_setup(C);
+  if (rc) return rc;
+  if (!valid_vec(C, src) || !valid_vec(C, dst) || C->vlen[src] != C->vlen[dst])
+    return uft_set_error(C, UFT_ERR_ARG, "uft_vec_copy: bad ids");
+  uftk::copy(C->s, C->vlen[src], C->vec[src], C->vec[dst]);
+  return UFT_OK;
+}
+void *uft_vec_device(uft_ctx *C, int id) {
+  if (!C || !C->is_setup || !valid_vec(C, id)) return nullptr;
+  return C->vec[id];
+}
+
+int uft_fem_obj_sens(uft_ctx *C, double *fx, double *gx, int dfdx_id, int m, const int *dgdx_ids, int xphys_id) {
+  int rc = check_setup(C);
+  if (rc) return rc;
+  if (!fx || !gx || !valid_vec(C, dfdx_id) || !valid_vec(C, xphys_id) || m < 1 || m > C->P.m || !dgdx_ids)
+    return uft_set_error(C, UFT_ERR_ARG, "uft_fem_obj_sens: bad arguments");
+  for (int j = 0; j < m; ++j)
+    if (!valid_vec(C, dgdx_ids[j]) || C->vlen[dgdx_ids[j]] != C->ne)
+      return uft_set_error(C, UFT_ERR_ARG, "uft_fem_obj_sens: bad dgdx id");
+  const double *xp = C->vec[xphys_id];
+  double *U = C->vec[UFT_U];
+  const int src = solve_el(C, xp, U);
+  if (src != UFT_OK && src != UFT_ERR_NOCONV) return src;
+  // SolveFem.cpp:208-249: only constraint 0 is filled by the reference
+  uftk::sens(C->s, C, xp, U, C->vec[dfdx_id], C->vec[dgdx_ids[0]], C->d_part);
+  uftk::final_sum(C->s, C->d_part, uftk::red_blocks(C->ne), 2, C->d_scal + S_RED);
+  rc = fetch(C, C->d_scal + S_RED, 2, C->h_scal);
+  if (rc) return rc;
+  fx[0] = C->h_scal[0];
+  for (int j = 0; j < m; ++j) gx[j] = 0.0;
+  gx[0] = C->h_scal[1] / C->TV - C->P.volfrac;
+  C->fx = fx[0];
+  C->gx[0] = gx[0];
+  return src;
+}
+
+int uft_filter(uft_ctx *C, int x_id, int xf_id) {
+  int rc = check_setup(C);
+  if (rc) return rc;
+  if (!valid_vec(C, x_id) || !valid_vec(C, xf_id) || C->vlen[x_id] != C->ne || C->vlen[xf_id] != C->ne)
+    return uft_set_error(C, UFT_ERR_ARG, "uft_filter: bad ids");
+  uftk::tf_mult(C->s, C, C->vec[x_id], C->d_TX);
+  rc = solve_filter(C, C->d_TX, C->vec[UFT_RX]);
+  uftk::tfx_t(C->s, C, C->vec[UFT_RX], C->vec[xf_id], 1, C->P.Xmin);
+  return rc;
+}
+
+int uft_filter_sens(uft_ctx *C, int dfdx_id, int m, const int *dgdx_ids) {
+  int rc = check_setup(C);
+  if (rc) return rc;
+  if (!valid_vec(C, dfdx_id) || m < 0 || (m > 0 && !dgdx_ids)) return uft_set_error(C, UFT_ERR_ARG, "uft_filter_sens: bad ids");
+  int out = UFT_OK;
+  std::vector<int> ids{dfdx_id};
+  for (int j = 0; j < m; ++j) ids.push_back(dgdx_ids[j]);
+  for (int id : ids) {
+    if (!valid_vec(C, id) || C->vlen[id] != C->ne) return uft_set_error(C, UFT_ERR_ARG, "uft_filter_sens: bad id");
+    uftk::tf_mult(C->s, C, C->vec[id], C->d_TX);
+    rc = solve_filter(C, C->d_TX, C->vec[UFT_RX]);
+    if (rc && rc != UFT_ERR_NOCONV) return rc;
+    if (rc) out = rc;
+    uftk::tfx_t(C->s, C, C->vec[UFT_RX], C->vec[id], 0, 0.0);
+  }
+  return out;
+}
+
+int uft_heaviside(uft_ctx *C, int xf_id, int xp_id, int itr) {
+  int rc = check_setup(C);
+  if (rc) return rc;
+  if (!valid_vec(C, xf_id) || !valid_vec(C, xp_id)) return uft_set_error(C, UFT_ERR_ARG, "uft_heaviside: bad ids");
+  if (!C->P.proj) {
+    uftk::copy(C->s, C->ne, C->vec[xf_id], C->vec[xp_id]);
+    return UFT_OK;
+  }
+  if (itr < 30)
+    C->beta = C->P.beta0;
+  else if (itr % 10 == 0)
+    C->beta = std::min(C->P.betamax, C->btemp + 1);
+  uftk::heaviside(C->s, C->ne, C->vec[xf_id], C->vec[xp_id], C->beta, C->P.x0, C->P.Xmin);
+  C->btemp = C->beta;
+  return UFT_OK;
+}
+
+int uft_projected_sens(uft_ctx *C, int xf_id, int dfdx_id, int m, const int *dgdx_ids) {
+  int rc = check_setup(C);
+  if (rc) return rc;
+  if (!valid_vec(C, xf_id) || !valid_vec(C, dfdx_id)) return uft_set_error(C, UFT_ERR_ARG, "uft_projected_sens: bad ids");
+  uftk::projected_sens(C->s, C->ne, C->vec[xf_id], C->beta, C->P.x0, C->vec[dfdx_id]);
+  for (int j = 0; j < m; ++j) {
+    if (!valid_vec(C, dgdx_ids[j])) return uft_set_error(C, UFT_ERR_ARG, "uft_projected_sens: bad id");
+    uftk::projected_sens(C->s, C->ne, C->vec[xf_id], C->beta, C->P.x0, C->vec[dgdx_ids[j]]);
+  }
+  return UFT_OK;
+}
+
+int uft_mma_movelimit(uft_ctx *C, int x_id, int xmin_id, int xmax_id) {
+  int rc = check_setup(C);
+  if (rc) return rc;
+  if (!valid_vec(C, x_id) || !valid_vec(C, xmin_id) || !valid_vec(C, xmax_id))
+    return uft_set_error(C, UFT_ERR_ARG, "uft_mma_movelimit: bad ids");
+  uftk::mma_movelimit(C->s, C->ne, C->vec[x_id], C->P.Xmin, C->P.Xmax, C->P.movlim, C->vec[xmin_id], C->vec[xmax_id]);
+  return UFT_OK;
+}
+
+int uft_mma_update(uft_ctx *C, int x_id, int dfdx_id, const double *gx, const int *dgdx_ids, int xmin_id,
+                   int xmax_id) {
+  int rc = check_setup(C);
+  if (rc) return rc;
+  const int m = C->P.m;
+  const int64_t n = C->ne;
+  if (!gx || !dgdx_ids || !valid_vec(C, x_id) || !valid_vec(C, dfdx_id) || !valid_vec(C, xmin_id) ||
+      !valid_vec(C, xmax_id))
+    return uft_set_error(C, UFT_ERR_ARG, "uft_mma_update: bad arguments");
+  const double *dg[UFT_MAXM];
+  for (int j = 0; j < m; ++j) {
+    if (!valid_vec(C, dgdx_ids[j])) return uft_set_error(C, UFT_ERR_ARG, "uft_mma_update: bad dgdx id");
+    dg[j] = C->vec[dgdx_ids[j]];
+  }
+  hipStream_t s = C->s;
+  double *x = C->vec[x_id];
+  // GenSub (MMA.cpp:506-637)
+  C->mk++;
+  uftk::mma_gensub(s, C, n, m, C->mk, x, C->vec[xmin_id], C->vec[xmax_id], C->vec[dfdx_id], dg, C->d_part);
+  uftk::final_sum(s, C->d_part, uftk::red_blocks(n), m, C->d_scal + S_RED);
+  if ((rc = fetch(C, C->d_scal + S_RED, m, C->h_scal))) return rc;
+  for (int j = 0; j < m; ++j) C->b[j] = C->h_scal[j] - gx[j];
+  // xo2 := xo1; xo1 := x (MMA.cpp:496-497)
+  std::swap(C->d_xo1, C->d_xo2);
+  uftk::copy(s, n, x, C->d_xo1);
+
+  // SolveDIP (MMA.cpp:639-676).  One fused device pass per lambda gives x(lambda),
+  // the constraint sums (shared by DualGrad and DualResidual) and the Hessian sum.
+  const int K = m + m * (m + 1) / 2;
+  double S[UFT_MAXM], Hs[UFT_MAXM * UFT_MAXM];
+  auto pass = [&]() -> int {
+    uftk::mma_dual_pass(s, C, n, m, C->lam, x, C->d_part);
+    uftk::final_sum(s, C->d_part, uftk::red_blocks(n), K, C->d_scal + S_RED);
+    int r2 = fetch(C, C->d_scal + S_RED, K, C->h_scal);
+    if (r2) return r2;
+    for (int j = 0; j < m; ++j) S[j] = C->h_scal[j];
+    int h = m;
+    for (int j = 0; j < m; ++j)
+      for (int k = j; k < m; ++k, ++h) Hs[j * m + k] = Hs[k * m + j] = C->h_scal[h];
+    return UFT_OK;
+  };
+  auto yz = [&]() {  // XYZofLAMBDA's scalar part (MMA.cpp:693-701)
+    double lamai = 0.0;
+    for (int j = 0; j < m; ++j) {
+      if (C->lam[j] < 0.0) C->lam[j] = 0.0;
+      C->y[j] = std::max(0.0, C->lam[j] - C->c[j]);
+      lamai += C->lam[j] * C->a[j];
+    }
+    C->z = std::max(0.0, 10.0 * (lamai - 1.0));
+  };
+  for (int j = 0; j < m; ++j) {
+    C->lam[j] = C->c[j] / 2.0;
+    C->mu[j] = 1.0;
+  }
+  const double tol = 1.0e-9 * std::sqrt((double)(m + n));
+  double epsi = 1.0, err = 1.0;
+  int steps = 0;
+  yz();
+  if ((rc = pass())) return rc;
+  while (epsi > tol) {
+    int loop = 0;
+    while (err > 0.9 * epsi && loop < 100) {
+      loop++;
+      steps++;
+      bool neg = false;
+      for (int j = 0; j < m; ++j) neg |= C->lam[j] < 0.0;
+      yz();
+      if (neg && (rc = pass())) return rc;
+      // DualGrad (MMA.cpp:730-765) and the Newton right-hand side
+      for (int j = 0; j < m; ++j) C->grad[j] = S[j] - C->b[j] - C->a[j] * C->z - C->y[j];
+      for (int j = 0; j < m; ++j) C->grad[j] = -1.0 * C->grad[j] - epsi / C->lam[j];
+      // DualHess (MMA.cpp:767-868)
+      for (int j = 0; j < m * m; ++j) C->Hess[j] = Hs[j];
+      double lamai = 0.0;
+      for (int j = 0; j < m; ++j) {
+        lamai += C->lam[j] * C->a[j];
+        if (C->lam[j] > C->c[j]) C->Hess[j * m + j] += -1.0;
+        C->Hess[j * m + j] += -C->mu[j] / C->lam[j];
+      }
+      if (lamai > 0.0)
+        for (int j = 0; j < m; ++j)
+          for (int k = 0; k < m; ++k) C->Hess[j * m + k] += -10.0 * C->a[j] * C->a[k];
+      double tr = 0.0;
+      for (int j = 0; j < m; ++j) tr += C->Hess[j * m + j];
+      double corr = 1e-4 * tr / m;
+      if (-1.0 * corr < 1.0e-7) corr = -1.0e-7;
+      for (int j = 0; j < m; ++j) C->Hess[j * m + j] += corr;
+      // LU without pivoting + solve (MMA.cpp:936-969)
+      double *Kh = C->Hess, *xv = C->grad;
+      for (int ss = 0; ss < m - 1; ++ss)
+        for (int i = ss + 1; i < m; ++i) {
+          Kh[i * m + ss] = Kh[i * m + ss] / Kh[ss * m + ss];
+          for (int j = ss + 1; j < m; ++j) Kh[i * m + j] -= Kh[i * m + ss] * Kh[ss * m + j];
+        }
+      for (int i = 1; i < m; ++i) {
+        double acc = 0.0;
+        for (int j = 0; j < i; ++j) acc -= Kh[i * m + j] * xv[j];
+        xv[i] += acc;
+      }
+      xv[m - 1] = xv[m - 1] / Kh[(m - 1) * m + (m - 1)];
+      for (int i = m - 2; i >= 0; --i) {
+        double acc = xv[i];
+        for (int j = i + 1; j < m; ++j) acc -= Kh[i * m + j] * xv[j];
+        xv[i] = acc / Kh[i * m + i];
+      }
+      for (int j = 0; j < m; ++j) C->sv[j] = C->grad[j];
+      for (int j = 0; j < m; ++j) C->sv[m + j] = -C->mu[j] + epsi / C->lam[j] - C->sv[j] * C->mu[j] / C->lam[j];
+      // DualLineSearch (MMA.cpp:870-888)
+      double theta = 1.005;
+      for (int j = 0; j < m; ++j) {
+        if (theta < -1.01 * C->sv[j] / C->lam[j]) theta = -1.01 * C->sv[j] / C->lam[j];
+        if (theta < -1.01 * C->sv[j + m] / C->mu[j]) theta = -1.01 * C->sv[j + m] / C->mu[j];
+      }
+      theta = 1.0 / theta;
+      for (int j = 0; j < m; ++j) {
+        C->lam[j] += theta * C->sv[j];
+        C->mu[j] += theta * C->sv[j + m];
+      }
+      // XYZofLAMBDA + DualResidual (MMA.cpp:890-934)
+      yz();
+      if ((rc = pass())) return rc;
+      double nrI = 0.0;
+      for (int j = 0; j < m; ++j) {
+        const double r1 = S[j] - C->b[j] - C->a[j] * C->z - C->y[j] + C->mu[j];
+        const double r2 = C->mu[j] * C->lam[j] - epsi;
+        nrI = std::max(nrI, std::max(std::fabs(r1), std::fabs(r2)));
+      }
+      err = nrI;
+    }
+    epsi = epsi * 0.1;
+  }
+  C->dip_steps = steps;
+  return UFT_OK;
+}
+
+int uft_mma_design_change(uft_ctx *C, int x_id, int xold_id, double *ch) {
+  int rc = check_setup(C);
+  if (rc) return rc;
+  if (!ch || !valid_vec(C, x_id) || !valid_vec(C, xold_id))
+    return uft_set_error(C, UFT_ERR_ARG, "uft_mma_design_change: bad arguments");
+  uftk::design_change(C->s, C->ne, C->vec[x_id], C->vec[xold_id], C->d_part);
+  uftk::final_max(C->s, C->d_part, uftk::red_blocks(C->ne), C->d_scal + S_RED);
+  rc = fetch(C, C->d_scal + S_RED, 1, C->h_scal);
+  if (rc) return rc;
+  *ch = C->h_scal[0];
+  return UFT_OK;
+}
+
+int uft_design_iteration(uft_ctx *C, int itr, double *fx, double *gx, double *ch) {
+  int rc = check_setup(C);
+  if (rc) return rc;
+  if (!fx || !gx || !ch) return uft_set_error(C, UFT_ERR_ARG, "uft_design_iteration: null output");
+  const int m = C->P.m;
+  int dg[UFT_MAXM];
+  for (int j = 0; j < m; ++j) dg[j] = UFT_DGDX0 + j;
+  int status = UFT_OK;
+  rc = uft_fem_obj_sens(C, fx, gx, UFT_DFDX, m, dg, UFT_XPHYS);
+  if (rc == UFT_ERR_NOCONV) status = rc; else if (rc) return rc;
+  if (C->P.proj && (rc = uft_projected_sens(C, UFT_XFILED, UFT_DFDX, m, dg))) return rc;
+  rc = uft_filter_sens(C, UFT_DFDX, m, dg);
+  if (rc == UFT_ERR_NOCONV) status = rc; else if (rc) return rc;
+  if ((rc = uft_mma_movelimit(C, UFT_X, UFT_XMIN, UFT_XMAX))) return rc;
+  if ((rc = uft_mma_update(C, UFT_X, UFT_DFDX, gx, dg, UFT_XMIN, UFT_XMAX))) return rc;
+  if ((rc = uft_mma_design_change(C, UFT_X, UFT_XOLD, ch))) return rc;
+  rc = uft_filter(C, UFT_X, UFT_XFILED);
+  if (rc == UFT_ERR_NOCONV) status = rc; else if (rc) return rc;
+  if ((rc = uft_heaviside(C, UFT_XFILED, UFT_XPHYS, itr))) return rc;
+  if ((rc = sync(C))) return rc;
+  return status;
+}
+
+int uft_solve_stats(const uft_ctx *C, int *it_el, double *relres_el, int *it_f, int *outer_el) {
+  if (!C) return UFT_ERR_ARG;
+  if (it_el) *it_el = C->it_el;
+  if (relres_el) *relres_el = C->relres_el;
+  if (it_f) *it_f = C->it_f;
+  if (outer_el) *outer_el = C->outer_el;
+  return UFT_OK;
+}
+
+int uft_apply_K(uft_ctx *C, int xphys_id, const double *u, double *y) {
+  int rc = check_setup(C);
+  if (rc) return rc;
+  if (!valid_vec(C, xphys_id) || !u || !y) return uft_set_error(C, UFT_ERR_ARG, "uft_apply_K: bad arguments");
+  UFT_HIP(C, hipMemcpyAsync(C->d_p, u, C->ndof * sizeof(double), hipMemcpyHostToDevice, C->s));
+  uftk::apply_K(C->s, C, C->vec[xphys_id], C->d_p, C->d_q, nullptr, nullptr);
+  UFT_HIP(C, hipMemcpyAsync(y, C->d_q, C->ndof * sizeof(double), hipMemcpyDeviceToHost, C->s));
+  return sync(C);
+}
+
+static int pk_index_h(int r, int c, int nd) {
+  if (r > c) std::swap(r, c);
+  return r * nd - (r * (r - 1)) / 2 + (c - r);
+}
+
+int uft_element_ke(uft_ctx *C, int64_t e, double *ke) {
+  int rc = check_setup(C);
+  if (rc) return rc;
+  if (e < 0 || e >= C->ne || !ke) return uft_set_error(C, UFT_ERR_ARG, "uft_element_ke: bad element");
+  const int nd = C->nd, np = nd * (nd + 1) / 2;
+  std::vector<double> pk(np);
+  for (int p = 0; p < np; ++p)
+    UFT_HIP(C, hipMemcpyAsync(&pk[p], C->d_KE + ((size_t)(p >> 1) * C->ne + e) * 2 + (p & 1), sizeof(double),
+                              hipMemcpyDeviceToHost, C->s));
+  if ((rc = sync(C))) return rc;
+  for (int r = 0; r < nd; ++r)
+    for (int c = 0; c < nd; ++c) ke[r * nd + c] = pk[pk_index_h(r, c, nd)];
+  return UFT_OK;
+}
+
+int uft_element_kf(uft_ctx *C, int64_t e, double *kf, double *t) {
+  int rc = check_setup(C);
+  if (rc) return rc;
+  if (e < 0 || e >= C->ne || !kf || !t) return uft_set_error(C, UFT_ERR_ARG, "uft_element_kf: bad element");
+  const int npe = C->npe, np = npe * (npe + 1) / 2;
+  std::vector<double> pk(np);
+  for (int p = 0; p < np; ++p)
+    UFT_HIP(C, hipMemcpyAsync(&pk[p], C->d_KF + ((size_t)(p >> 1) * C->ne + e) * 2 + (p & 1), sizeof(double),
+                              hipMemcpyDeviceToHost, C->s));
+  for (int a = 0; a < npe; ++a)
+    UFT_HIP(C, hipMemcpyAsync(&t[a], C->d_T + (size_t)a * C->ne + e, sizeof(double), hipMemcpyDeviceToHost, C->s));
+  if ((rc = sync(C))) return rc;
+  for (int r = 0; r < npe; ++r)
+    for (int c = 0; c < npe; ++c) kf[r * npe + c] = pk[pk_index_h(r, c, npe)];
+  return UFT_OK;
+}
+
+int uft_sync(uft_ctx *C) {
+  if (!C) return UFT_ERR_ARG;
+  return sync(C);
+}
+
+}  // extern "C"

@@ -1,0 +1,268 @@
+// Internal declarations shared by the host orchestration (uft_api.cpp,
+// uft_amg_setup.cpp, uft_mesh.cpp) and the gfx950 kernels (*.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "uft.h"
+
+#define UFT_MAXM 8
+#define UFT_MAXLEV 8
+#define UFT_RED_BLOCK 256
+
+// E(x) = Emin + x^penal * (Emax - Emin)   (SolveFem.cpp:304)
+struct SimpCoef {
+  double Emin, dE, penal;
+};
+
+// --------------------------------------------------------------------------
+// Multigrid hierarchy (host-built aggregates, device-applied V-cycle)
+// --------------------------------------------------------------------------
+struct AmgLevel {
+  // operator of this level (level 0 is the matrix-free fine operator)
+  int64_t n = 0;        // scalar unknowns
+  int64_t nunit = 0;    // nodes (level 0) or aggregates (level >= 1)
+  int bs = 3;           // unknowns per unit: 3 on level 0, 6 above
+  int64_t nblk = 0;     // BSR blocks (level >= 1)
+  int32_t *d_rp = nullptr, *d_ci = nullptr;  // BSR pattern (level >= 1)
+  int32_t *d_brow = nullptr;                 // block -> block row (level >= 1)
+  double *d_val = nullptr;                   // nblk * 36
+  float *d_valf = nullptr;
+  double *d_dinv = nullptr;
+  float *d_dinvf = nullptr;
+  double lmax = 2.0;
+  // transfer to the next coarser level: each unit u of this level belongs to
+  // parent[u] on the next level with a bs x 6 block P_u.
+  double *d_P = nullptr;
+  float *d_Pf = nullptr;
+  int32_t *d_parent = nullptr;
+  int32_t *d_cptr = nullptr, *d_child = nullptr;  // next-level unit -> children
+  // deterministic Galerkin contribution lists for the next level's blocks
+  int64_t ngl = 0;
+  int32_t *d_glptr = nullptr;  // per next-level block
+  int32_t *d_gl = nullptr;     // level 0: e*npe*npe + la*npe + lb ; else block index
+  // work vectors (length n)
+  double *x = nullptr, *b = nullptr, *r = nullptr, *d = nullptr, *t = nullptr;
+  float *xf = nullptr, *bf = nullptr, *rf = nullptr, *df = nullptr, *tf = nullptr;
+};
+
+struct AmgHier {
+  int nlev = 0;  // levels incl. fine; last level solved densely
+  AmgLevel L[UFT_MAXLEV];
+  int64_t ncoarse = 0;
+  double *d_cmat = nullptr;  // dense coarsest matrix / its inverse (ncoarse^2)
+  double *d_cinv = nullptr;
+  float *d_cinvf = nullptr;
+  double *d_cwork = nullptr;
+};
+
+// host-side description produced by uft_amg_setup.cpp
+struct AmgHostLevel {
+  int64_t nunit_next = 0;
+  std::vector<int32_t> parent;       // per unit of this level
+  std::vector<double> P;             // per unit: bs*6 (row-major bs x 6)
+  std::vector<int32_t> cptr, child;  // next-level unit -> children of this level
+  // pattern of the next level operator
+  std::vector<int32_t> rp, ci;
+  // Galerkin lists for the next level blocks
+  std::vector<int32_t> glptr, gl;
+};
+
+// --------------------------------------------------------------------------
+// Context
+// --------------------------------------------------------------------------
+struct uft_ctx {
+  std::vector<void *> allocs;  // every device allocation, freed by uft_destroy
+  int device = 0;
+  hipStream_t s = nullptr;
+  std::string err;
+  uft_params P{};
+  bool have_mesh = false, have_bc = false, is_setup = false;
+
+  // mesh (host copies)
+  int npe = 0, nd = 0;
+  int64_t nn = 0, ne = 0, ndof = 0;
+  std::vector<int32_t> elem;  // ne x npe
+  std::vector<double> xyz;    // nn x 3
+  std::vector<int32_t> fixed;
+  int nlc = 0, nload = 0;
+  std::vector<int32_t> ldofs;
+  std::vector<double> lvals;
+
+  // device mesh / element operators
+  int32_t *d_elem = nullptr;   // [npe][ne]
+  double *d_xyz = nullptr;     // nn x 3
+  double *d_KE = nullptr;      // packed upper triangle, pair-interleaved [np/2][ne][2]
+  float *d_KEf = nullptr;      // FP32 copy for the mixed form [np/2][ne][2]
+  double *d_KF = nullptr;      // Helmholtz packed [npf/2][ne][2]
+  double *d_T = nullptr;       // [npe][ne]
+  double *d_detmin = nullptr;  // per element minimum det J
+  uint8_t *d_fixm = nullptr;   // per node: bit c set if DOF 3n+c is fixed
+  int32_t *d_iptr = nullptr, *d_inc = nullptr;  // node -> (e*npe + local)
+  double *d_F = nullptr;
+  double *d_Ye = nullptr;      // element-local scratch nd x ne
+  float *d_Yef = nullptr;
+  double TV = 0.0;
+
+  // vectors
+  std::vector<double *> vec;   // by id
+  std::vector<int64_t> vlen;
+
+  // elasticity solver work (ndof)
+  double *d_r = nullptr, *d_z = nullptr, *d_p = nullptr, *d_q = nullptr;
+  double *d_w = nullptr;       // mixed form: FP64 correction accumulator
+  float *d_rf = nullptr, *d_zf = nullptr, *d_pf = nullptr, *d_qf = nullptr, *d_uf = nullptr;
+  AmgHier amg;
+  bool amg_built = false;
+
+  // filter work (nn)
+  double *d_fr = nullptr, *d_fz = nullptr, *d_fp = nullptr, *d_fq = nullptr, *d_fdinv = nullptr;
+  double *d_TX = nullptr;
+
+  // reductions
+  double *d_part = nullptr;    // partials
+  double *d_scal = nullptr;    // device scalars
+  double *h_scal = nullptr;    // pinned host mirror
+
+  // heaviside state (PDEFilter.h:50-51)
+  double beta = 1.0, btemp = 1.0;
+
+  // MMA state (MMA.h:76-109)
+  int mk = 0;
+  double asyminit = 0.5, asymdec = 0.7, asyminc = 1.2;
+  double a[UFT_MAXM], c[UFT_MAXM], dd[UFT_MAXM], y[UFT_MAXM], z = 0.0;
+  double lam[UFT_MAXM], mu[UFT_MAXM], sv[2 * UFT_MAXM], b[UFT_MAXM], grad[UFT_MAXM],
+      Hess[UFT_MAXM * UFT_MAXM];
+  double *d_L = nullptr, *d_U = nullptr, *d_alpha = nullptr, *d_beta = nullptr, *d_p0 = nullptr,
+         *d_q0 = nullptr, *d_pij = nullptr, *d_qij = nullptr, *d_xo1 = nullptr, *d_xo2 = nullptr;
+
+  const double *cur_xphys = nullptr;  // density of the operator being solved
+  double fnorm = 0.0;
+  double *d_lanczos = nullptr;       // CG scalar log for eigenvalue estimates
+
+  // stats
+  int it_el = 0, it_f = 0, outer_el = 0, dip_steps = 0;
+  double relres_el = 0.0;
+  double fx = 0.0, gx[UFT_MAXM];
+};
+
+// --------------------------------------------------------------------------
+// Kernel launch wrappers (kernels_*.hip)
+// --------------------------------------------------------------------------
+namespace uftk {
+int red_blocks(int64_t n);
+// element operators
+void gen_ke(hipStream_t s, int npe, int64_t ne, const int32_t *elem, const double *xyz, double nu,
+            double *KE, float *KEf, double *detmin);
+void gen_kf(hipStream_t s, int npe, int64_t ne, const int32_t *elem, const double *xyz, double rmin,
+            double *KF, double *T, double *vol, double *part);
+// fine operator apply: y = K_D(x) u   (two-phase element/node)
+void apply_K(hipStream_t s, const uft_ctx *C, const double *xphys, const double *u, double *y,
+             const double *dotw, double *part);
+void apply_Kf(hipStream_t s, const uft_ctx *C, const double *xphys, const float *u, float *y);
+void diag_K(hipStream_t s, const uft_ctx *C, const double *xphys, double *dinv, float *dinvf);
+void sens(hipStream_t s, const uft_ctx *C, const double *xphys, const double *u, double *dfdx,
+          double *dgdx0, double *part);
+// Helmholtz filter
+void apply_KF(hipStream_t s, const uft_ctx *C, const double *x, double *y);
+void diag_KF(hipStream_t s, const uft_ctx *C, double *dinv);
+void tf_mult(hipStream_t s, const uft_ctx *C, const double *xe, double *TX);
+void tfx_t(hipStream_t s, const uft_ctx *C, const double *RX, double *xe, int clamp, double Xmin);
+// reductions / vector ops
+void final_sum(hipStream_t s, const double *part, int nb, int K, double *out);
+void final_max(hipStream_t s, const double *part, int nb, double *out);
+void dot(hipStream_t s, int64_t n, const double *a, const double *b, double *part, double *out);
+void dot2(hipStream_t s, int64_t n, const double *a, const double *b, const double *c, double *part,
+          double *out);  // out[0]=a.b, out[1]=c.c
+void scalar_div(hipStream_t s, double *sc, int num, int den, int dst);
+void fill(hipStream_t s, int64_t n, double v, double *x);
+void fillf(hipStream_t s, int64_t n, float v, float *x);
+void copy(hipStream_t s, int64_t n, const double *a, double *b);
+void d2f(hipStream_t s, int64_t n, const double *a, float *b);
+void f2d_add(hipStream_t s, int64_t n, const float *a, double *b);
+void scale(hipStream_t s, int64_t n, const double *a, const double *dinv, double *z);
+void cg_update(hipStream_t s, int64_t n, const double *sc, int ialpha, double *u, const double *p,
+               double *r, const double *q, double *part);
+void xpby(hipStream_t s, int64_t n, const double *z, const double *sc, int ibeta, double *p);
+void axpy_inplace(hipStream_t s, int64_t n, double a, const double *x, double *y);
+void sub(hipStream_t s, int64_t n, const double *a, const double *b, double *c);
+void set_dofs(hipStream_t s, int n, const int32_t *idx, const double *val, double *x);
+// templated float variants for the mixed form
+void cg_updatef(hipStream_t s, int64_t n, const double *sc, int ialpha, float *u, const float *p,
+                float *r, const float *q, double *part);
+void xpbyf(hipStream_t s, int64_t n, const float *z, const double *sc, int ibeta, float *p);
+void dotf(hipStream_t s, int64_t n, const float *a, const float *b, double *part, double *out);
+// multigrid
+void cheb_first(hipStream_t s, int64_t n, const double *b, const double *dinv, double inv_theta,
+                double *r, double *d, double *x, int x_is_zero);
+void cheb_step(hipStream_t s, int64_t n, const double *t, const double *dinv, double c1, double c2,
+               double *r, double *d, double *x);
+void cheb_firstf(hipStream_t s, int64_t n, const float *b, const float *dinv, double inv_theta, float *r,
+                 float *d, float *x, int x_is_zero);
+void cheb_stepf(hipStream_t s, int64_t n, const float *t, const float *dinv, double c1, double c2,
+                float *r, float *d, float *x);
+void sub_inplace(hipStream_t s, int64_t n, const double *t, double *r);
+void sub_inplacef(hipStream_t s, int64_t n, const float *t, float *r);
+void bsr6_mv(hipStream_t s, int64_t nb_rows, const int32_t *rp, const int32_t *ci, const double *val,
+             const double *x, double *y);
+void bsr6_mvf(hipStream_t s, int64_t nb_rows, const int32_t *rp, const int32_t *ci, const float *val,
+              const float *x, float *y);
+void bsr6_dinv(hipStream_t s, int64_t nb_rows, const int32_t *rp, const int32_t *ci, double *val,
+               double *dinv, float *dinvf);
+void restrict_(hipStream_t s, int bs, int64_t ncoarse_units, const int32_t *cptr, const int32_t *child,
+               const double *P, const double *r, double *bc);
+void prolong_add(hipStream_t s, int bs, int64_t nunits, const int32_t *parent, const double *P,
+                 const double *xc, double *x);
+void restrictf(hipStream_t s, int bs, int64_t ncoarse_units, const int32_t *cptr, const int32_t *child,
+               const float *P, const float *r, float *bc);
+void prolong_addf(hipStream_t s, int bs, int64_t nunits, const int32_t *parent, const float *P,
+                  const float *xc, float *x);
+void galerkin_fine(hipStream_t s, const uft_ctx *C, const double *xphys, int64_t nblk, const int32_t *glptr,
+                   const int32_t *gl, const double *Pq, double *val);
+void galerkin_coarse(hipStream_t s, int64_t nblk, const int32_t *glptr, const int32_t *gl,
+                     const int32_t *brow, const int32_t *ci, const double *P, const double *val,
+                     double *valc);
+void bsr_to_dense(hipStream_t s, int64_t nb_rows, const int32_t *rp, const int32_t *ci, const double *val,
+                  int64_t n, double *dense);
+void dense_spd_inverse(hipStream_t s, int64_t n, double *A, double *Ainv, float *Ainvf);
+void dense_mv(hipStream_t s, int64_t n, const double *A, const double *x, double *y);
+void dense_mvf(hipStream_t s, int64_t n, const float *A, const float *x, float *y);
+void fixed_unit_diag(hipStream_t s, int64_t nb_rows, const int32_t *rp, const int32_t *ci, double *val);
+// MMA
+void mma_movelimit(hipStream_t s, int64_t n, const double *x, double Xmin, double Xmax, double movlim,
+                   double *xmin, double *xmax);
+void mma_gensub(hipStream_t s, uft_ctx *C, int64_t n, int m, int k, const double *x, const double *xmin,
+                const double *xmax, const double *dfdx, const double *const *dgdx, double *part);
+void mma_dual_pass(hipStream_t s, uft_ctx *C, int64_t n, int m, const double *lam, double *x, double *part);
+void design_change(hipStream_t s, int64_t n, const double *x, double *xold, double *part);
+void heaviside(hipStream_t s, int64_t n, const double *xf, double *xp, double beta, double x0, double Xmin);
+void projected_sens(hipStream_t s, int64_t n, const double *xf, double beta, double x0, double *v);
+// Krylov helpers (kernels_solve.hip)
+void fcg_init(hipStream_t s, int64_t n, const double *b, const double *q, const double *dinv, double *r, double *z,
+              double *part);
+void fcg_update(hipStream_t s, int64_t n, const double *sc, int ia, double *x, const double *p, double *r,
+                const double *q, const double *dinv, double *z, double *part);
+void fcg_p(hipStream_t s, int64_t n, const double *z, const double *sc, int ib, int ibo, int first, double *p);
+void sc_div(hipStream_t s, double *sc, int a, int b, int dst);
+void sc_beta(hipStream_t s, double *sc, int inew, int iold, int ibeta, double *log2);
+void sc_copy(hipStream_t s, const double *src, double *dst);
+}  // namespace uftk
+
+// host helpers (uft_api.cpp)
+int uft_set_error(uft_ctx *C, int code, const char *fmt, ...);
+#define UFT_HIP(C, call)                                                                       \
+  do {                                                                                         \
+    hipError_t _e = (call);                                                                    \
+    if (_e != hipSuccess)                                                                      \
+      return uft_set_error((C), UFT_ERR_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(_e), \
+                           __FILE__, __LINE__);                                                \
+  } while (0)
+
+// amg setup (uft_amg_setup.cpp)
+int uft_amg_build_host(uft_ctx *C, std::vector<AmgHostLevel> &H, std::vector<int64_t> &nunits);
+// mesh helpers (uft_mesh.cpp)
+void uft_node_incidence(int64_t nn, int64_t ne, int npe, const int32_t *elem, std::vector<int32_t> &ptr,
+                        std::vector<int32_t> &inc);

@@ -1,0 +1,166 @@
+// Mesh-side host code: the synthetic cantilever generator, the reference's
+// text formats (InputMeshData.cpp:100-149,228-285; GetBcs/GetBCs.m) and the
+// node -> (element, local node) incidence used by the atomic-free node phase.
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+
+#include "uft_internal.h"
+
+namespace {
+// SplitMix64 (Steele, Lea, Flood 2014)
+uint64_t sm64_next(uint64_t &s) {
+  uint64_t z = (s += 0x9E3779B97F4A7C15ULL);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+double sm64_u01(uint64_t &s) { return (double)(sm64_next(s) >> 11) * 0x1.0p-53; }
+}  // namespace
+
+void uft_node_incidence(int64_t nn, int64_t ne, int npe, const int32_t *elem, std::vector<int32_t> &ptr,
+                        std::vector<int32_t> &inc) {
+  ptr.assign(nn + 1, 0);
+  for (int64_t e = 0; e < ne; ++e)
+    for (int a = 0; a < npe; ++a) ptr[elem[e * npe + a] + 1]++;
+  for (int64_t i = 0; i < nn; ++i) ptr[i + 1] += ptr[i];
+  inc.assign(ptr[nn], 0);
+  std::vector<int32_t> pos(ptr.begin(), ptr.end() - 1);
+  // element-major fill keeps each node's list in ascending element order
+  for (int64_t e = 0; e < ne; ++e)
+    for (int a = 0; a < npe; ++a) inc[pos[elem[e * npe + a]]++] = (int32_t)(e * npe + a);
+}
+
+extern "C" {
+
+int64_t uft_box_nodes(int nx, int ny, int nz) { return (int64_t)(nx + 1) * (ny + 1) * (nz + 1); }
+int64_t uft_box_elems(int npe, int nx, int ny, int nz) {
+  const int64_t h = (int64_t)nx * ny * nz;
+  return npe == 4 ? 6 * h : h;
+}
+
+int uft_gen_box(int npe, int nx, int ny, int nz, double lx, double ly, double lz, double jitter, uint64_t seed,
+                double *xyz, int32_t *elem) {
+  if (nx < 1 || ny < 1 || nz < 1 || (npe != 8 && npe != 4) || !xyz || !elem) return UFT_ERR_ARG;
+  if (uft_box_elems(npe, nx, ny, nz) * npe >= (int64_t)INT32_MAX) return UFT_ERR_ARG;
+  uint64_t s = seed;
+  const double hx = lx / nx, hy = ly / ny, hz = lz / nz;
+  int64_t n = 0;
+  for (int k = 0; k <= nz; ++k)
+    for (int j = 0; j <= ny; ++j)
+      for (int i = 0; i <= nx; ++i, ++n) {
+        double x = lx * (double)i / (double)nx;
+        double y = ly * (double)j / (double)ny;
+        double z = lz * (double)k / (double)nz;
+        if (jitter != 0.0 && i > 0 && i < nx && j > 0 && j < ny && k > 0 && k < nz) {
+          x += (2.0 * sm64_u01(s) - 1.0) * jitter * hx;
+          y += (2.0 * sm64_u01(s) - 1.0) * jitter * hy;
+          z += (2.0 * sm64_u01(s) - 1.0) * jitter * hz;
+        }
+        xyz[3 * n] = x;
+        xyz[3 * n + 1] = y;
+        xyz[3 * n + 2] = z;
+      }
+  // six positively oriented tets around the local 0-7 diagonal
+  static const int tets[6][4] = {{0, 1, 3, 7}, {0, 3, 2, 7}, {0, 2, 6, 7}, {0, 6, 4, 7}, {0, 4, 5, 7}, {0, 5, 1, 7}};
+  const int64_t sx = nx + 1, sxy = (int64_t)(nx + 1) * (ny + 1);
+  int64_t e = 0;
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) {
+        const int64_t b = i + sx * j + sxy * k;
+        const int32_t hv[8] = {(int32_t)b,         (int32_t)(b + 1),         (int32_t)(b + sx),
+                               (int32_t)(b + sx + 1), (int32_t)(b + sxy),       (int32_t)(b + sxy + 1),
+                               (int32_t)(b + sxy + sx), (int32_t)(b + sxy + sx + 1)};
+        if (npe == 8) {
+          for (int a = 0; a < 8; ++a) elem[8 * e + a] = hv[a];
+          ++e;
+        } else {
+          for (int t = 0; t < 6; ++t, ++e)
+            for (int a = 0; a < 4; ++a) elem[4 * e + a] = hv[tets[t][a]];
+        }
+      }
+  return UFT_OK;
+}
+
+int uft_box_bcs(int nx, int ny, int nz, int load_kind, int32_t *fixed, int *nfixed, int32_t *load_dofs,
+                double *load_vals, int *nload) {
+  if (nx < 1 || ny < 1 || nz < 1 || !fixed || !nfixed || !load_dofs || !load_vals || !nload) return UFT_ERR_ARG;
+  const int64_t sx = nx + 1, sxy = (int64_t)(nx + 1) * (ny + 1);
+  int nf = 0;
+  for (int k = 0; k <= nz; ++k)
+    for (int j = 0; j <= ny; ++j) {
+      const int64_t n = sx * j + sxy * k;
+      fixed[nf++] = (int32_t)(3 * n);
+      fixed[nf++] = (int32_t)(3 * n + 1);
+      fixed[nf++] = (int32_t)(3 * n + 2);
+    }
+  *nfixed = nf;
+  int nl = 0;
+  if (load_kind == 0) {
+    for (int k = 0; k <= nz; ++k) {
+      const int64_t n = nx + sxy * k;
+      load_dofs[nl] = (int32_t)(3 * n + 1);
+      load_vals[nl++] = -1.0 / (double)(nz + 1);
+    }
+  } else {
+    if (ny < 2 || nz < 2) return UFT_ERR_ARG;
+    const int jc = ny / 2, kc = nz / 2;
+    for (int k = kc - 1; k <= kc + 1; ++k)
+      for (int j = jc - 1; j <= jc + 1; ++j) {
+        const int64_t n = nx + sx * j + sxy * k;
+        load_dofs[nl] = (int32_t)(3 * n + 1);
+        load_vals[nl++] = -1.0 / 9.0;
+      }
+  }
+  *nload = nl;
+  return UFT_OK;
+}
+
+}  // extern "C"
+
+// ---- text readers: whitespace-separated tokens, no terminator required
+template <typename T>
+static int read_tokens(uft_ctx *C, const char *path, int64_t count, T *out) {
+  std::ifstream in(path);
+  if (!in.is_open()) return uft_set_error(C, UFT_ERR_IO, "cannot open %s", path);
+  for (int64_t i = 0; i < count; ++i) {
+    if (!(in >> out[i]))
+      return uft_set_error(C, UFT_ERR_IO, "%s: expected %lld values, read %lld", path, (long long)count, (long long)i);
+  }
+  return UFT_OK;
+}
+
+extern "C" int uft_mesh_load_text(uft_ctx *C, int npe, int64_t nn, int64_t ne, const char *vertex_path,
+                                  const char *element_path) {
+  if (!C) return UFT_ERR_ARG;
+  if (nn <= 0 || ne <= 0 || (npe != 8 && npe != 4) || !vertex_path || !element_path)
+    return uft_set_error(C, UFT_ERR_ARG, "uft_mesh_load_text: bad arguments");
+  std::vector<double> xyz(nn * 3);
+  std::vector<int32_t> elem(ne * npe);
+  int rc = read_tokens(C, vertex_path, nn * 3, xyz.data());
+  if (rc) return rc;
+  rc = read_tokens(C, element_path, ne * npe, elem.data());
+  if (rc) return rc;
+  return uft_mesh_set(C, npe, nn, ne, elem.data(), xyz.data());
+}
+
+extern "C" int uft_bc_load_text(uft_ctx *C, const char *dir, int nfixed, int nlc, int nload) {
+  if (!C) return UFT_ERR_ARG;
+  if (!dir || nfixed < 0 || nlc < 1 || nload < 0) return uft_set_error(C, UFT_ERR_ARG, "uft_bc_load_text: bad arguments");
+  std::vector<int32_t> fixed(nfixed), ld((size_t)nlc * nload);
+  std::vector<double> lv((size_t)nlc * nload);
+  char path[4096];
+  for (int i = 0; i < nlc; ++i) {
+    std::snprintf(path, sizeof(path), "%s/loaddof_%d.txt", dir, i + 1);
+    int rc = read_tokens(C, path, nload, ld.data() + (size_t)i * nload);
+    if (rc) return rc;
+    std::snprintf(path, sizeof(path), "%s/load_%d.txt", dir, i + 1);
+    rc = read_tokens(C, path, nload, lv.data() + (size_t)i * nload);
+    if (rc) return rc;
+  }
+  std::snprintf(path, sizeof(path), "%s/fixeddofs.txt", dir);
+  int rc = read_tokens(C, path, nfixed, fixed.data());
+  if (rc) return rc;
+  return uft_bc_set(C, nfixed, fixed.data(), nlc, nload, ld.data(), lv.data());
+}

@@ -1,0 +1,354 @@
+// Drop-in C++ host classes with the reference's names and call signatures
+// (OptPara.h, InputMeshData.h, SolveFem.h, PDEFilter.h, MMA.h, Write2Vtu.h),
+// implemented over the C-ABI of include/uft.h.  A driver written against the
+// reference's main.cpp call sequence compiles against these unchanged except
+// for the initialisation call (UftInitialize replaces PetscInitialize).
+//
+// PETSc's Vec becomes a handle to a device-resident vector of the context;
+// the mesh matrices that the reference passes around (elementEigen,
+// eleXCoord, ...) become MeshRef handles, because the mesh lives on the
+// device once InputMeshData has loaded it.
+#pragma once
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "uft.h"
+
+typedef int PetscErrorCode;
+typedef double PetscScalar;
+typedef int PetscInt;
+typedef bool PetscBool;
+#define PETSC_TRUE true
+#define PETSC_FALSE false
+#define CHKERRQ(e)                                                                                   \
+  do {                                                                                               \
+    if (e) {                                                                                         \
+      std::fprintf(stderr, "uft error %d at %s:%d: %s\n", (int)(e), __FILE__, __LINE__, UftLastError()); \
+      return (e);                                                                                    \
+    }                                                                                                \
+  } while (0)
+
+// ------------------------------------------------------------------ runtime
+// Options database of "-key value" pairs (the PETSc options the reference reads:
+// OptPara.cpp:92-104, SolveFem.cpp:11, -ksp_rtol for both solvers) plus the
+// mesh source: -mesh_dir DIR (reference text files, with -nn -ne -nfixed -nload)
+// or -box NX,NY,NZ [-jitter J -seed S -load_kind K] for the synthetic cantilever;
+// -npe 8|4 selects hex8 or tet4.
+struct UftRuntime {
+  std::map<std::string, std::string> opt;
+  uft_ctx *ctx = nullptr;
+  int device = 0;
+  static UftRuntime &get() {
+    static UftRuntime r;
+    return r;
+  }
+  bool has(const char *k) const { return opt.count(k) > 0; }
+  double real(const char *k, double d) const { return has(k) ? std::atof(opt.at(k).c_str()) : d; }
+  int integer(const char *k, int d) const { return has(k) ? std::atoi(opt.at(k).c_str()) : d; }
+  std::string str(const char *k, const char *d) const { return has(k) ? opt.at(k) : std::string(d); }
+};
+
+inline const char *UftLastError() { return uft_last_error(UftRuntime::get().ctx); }
+
+inline PetscErrorCode UftInitialize(int *argc, char ***argv) {
+  UftRuntime &R = UftRuntime::get();
+  for (int i = 1; i < *argc; ++i) {
+    if ((*argv)[i][0] != '-') continue;
+    std::string k = (*argv)[i];
+    std::string v = (i + 1 < *argc && ((*argv)[i + 1][0] != '-' || std::isdigit((unsigned char)(*argv)[i + 1][1]) ||
+                                       (*argv)[i + 1][1] == '.'))
+                        ? (*argv)[++i]
+                        : "1";
+    R.opt[k] = v;
+  }
+  R.device = R.integer("-device", 0);
+  return uft_create(&R.ctx, R.device);
+}
+inline PetscErrorCode UftFinalize() {
+  UftRuntime &R = UftRuntime::get();
+  uft_destroy(R.ctx);
+  R.ctx = nullptr;
+  return 0;
+}
+
+// handle to a device vector of the context (replaces PETSc Vec)
+struct Vec {
+  int id = -1;
+};
+struct MeshRef {};  // stands in for the reference's Eigen mesh matrices and raw BC arrays
+
+// ------------------------------------------------------------------ OptPara
+class OptPara {
+ public:
+  OptPara() : OptPara(1) {}
+  explicit OptPara(PetscInt nconstraints) {
+    m = nconstraints;
+    SetUp();
+  }
+  Vec X{UFT_X}, Xold{UFT_XOLD}, XFiled{UFT_XFILED}, XPhys{UFT_XPHYS}, dfdx{UFT_DFDX};
+  Vec *dgdx = nullptr;
+  Vec xmin{UFT_XMIN}, xmax{UFT_XMAX}, Vol{UFT_VOL};
+  PetscScalar fx = 0, Xmin, Xmax, Emin, Emax, volfrac, penal, movlim, rmin;
+  PetscScalar *gx = nullptr;
+  PetscInt maxItr, m;
+  PetscScalar x0, beta0, betamax, fscale = 1.0;
+  PetscBool proj;
+  uft_params params;
+  ~OptPara() {
+    delete[] gx;
+    delete[] dgdx;
+  }
+  PetscErrorCode SetMMA(PetscInt *itr, class MMA **mma);
+
+ private:
+  // defaults and options (OptPara.cpp:123-218)
+  void SetUp() {
+    UftRuntime &R = UftRuntime::get();
+    uft_default_params(&params);
+    maxItr = R.integer("-maxItr", params.maxItr);
+    rmin = R.real("-rmin", params.rmin);
+    penal = R.real("-penal", params.penal);
+    Emin = R.real("-Emin", params.Emin);
+    Emax = R.real("-Emax", params.Emax);
+    movlim = R.real("-movlim", params.movlim);
+    Xmin = R.real("-Xmin", params.Xmin);
+    Xmax = R.real("-Xmax", params.Xmax);
+    volfrac = R.real("-volfrac", params.volfrac);
+    x0 = R.real("-x0", params.x0);
+    beta0 = R.real("-beta0", params.beta0);
+    betamax = R.real("-betamax", params.betamax);
+    proj = R.integer("-proj", 0) != 0;
+    params.Emin = Emin; params.Emax = Emax; params.penal = penal; params.volfrac = volfrac; params.rmin = rmin;
+    params.movlim = movlim; params.Xmin = Xmin; params.Xmax = Xmax; params.x0 = x0; params.beta0 = beta0;
+    params.betamax = betamax; params.proj = proj; params.m = m; params.maxItr = maxItr;
+    params.nu = R.real("-nu", params.nu);
+    params.rtol_el = R.real("-ksp_rtol", params.rtol_el);  // unprefixed: both KSPs (SolveFem.cpp:135, PDEFilter.cpp:135)
+    params.rtol_f = R.real("-ksp_rtol", params.rtol_f);
+    params.precision = R.integer("-mixed", 0) ? UFT_MIXED : UFT_FP64;
+    params.cheb_degree = R.integer("-cheb_degree", params.cheb_degree);
+    std::printf("################### Optimization settings ####################\n");
+    std::printf("# -rmin: %f\n# -volfrac: %f\n# -penal: %f\n# -Emin/-Emax: %e - %e \n", rmin, volfrac, penal, Emin, Emax);
+    std::printf("# -Xmin/-Xmax: %e - %e \n# -maxItr: %i\n# -movlim: %f\n# -x0: %f\n", Xmin, Xmax, maxItr, movlim, x0);
+    std::printf("# -beta0: %f\n# -betamax: %f\n# -proj: %i  (0/1)\n", beta0, betamax, (int)proj);
+    std::printf("##############################################################\n");
+    gx = new PetscScalar[m]();
+    dgdx = new Vec[m];
+    for (int j = 0; j < m; ++j) dgdx[j].id = UFT_DGDX0 + j;
+    uft_params_set(R.ctx, &params);
+  }
+};
+
+// ------------------------------------------------------------------ InputMeshData
+class InputMeshData {
+ public:
+  // InputMeshData::GetNodeAndElementData + GetBcs (InputMeshData.cpp:33-285)
+  explicit InputMeshData(Vec /*X*/) {
+    UftRuntime &R = UftRuntime::get();
+    const int npe = R.integer("-npe", 8);
+    int rc = 0;
+    if (R.has("-mesh_dir")) {
+      const std::string d = R.str("-mesh_dir", ".");
+      const long long nn = R.integer("-nn", 410451), ne = R.integer("-ne", 392000);
+      rc = uft_mesh_load_text(R.ctx, npe, nn, ne, (d + "/vertex.txt").c_str(), (d + "/element.txt").c_str());
+      if (!rc) rc = uft_bc_load_text(R.ctx, d.c_str(), R.integer("-nfixed", 8733), R.integer("-nlc", 1),
+                                     R.integer("-nload", 41));
+    } else {
+      int nx = 40, ny = 20, nz = 20;
+      std::sscanf(R.str("-box", "40,20,20").c_str(), "%d,%d,%d", &nx, &ny, &nz);
+      const long long nn = uft_box_nodes(nx, ny, nz), ne = uft_box_elems(npe, nx, ny, nz);
+      std::vector<double> xyz(nn * 3);
+      std::vector<int32_t> el(ne * npe);
+      rc = uft_gen_box(npe, nx, ny, nz, 2.0, 1.0, 1.0, R.real("-jitter", 0.0), (uint64_t)R.real("-seed", 0),
+                       xyz.data(), el.data());
+      std::vector<int32_t> fixed(3 * nn), ld(nn + 9);
+      std::vector<double> lv(nn + 9);
+      int nf = 0, nl = 0;
+      if (!rc) rc = uft_box_bcs(nx, ny, nz, R.integer("-load_kind", 0), fixed.data(), &nf, ld.data(), lv.data(), &nl);
+      if (!rc) rc = uft_mesh_set(R.ctx, npe, nn, ne, el.data(), xyz.data());
+      if (!rc) rc = uft_bc_set(R.ctx, nf, fixed.data(), 1, nl, ld.data(), lv.data());
+    }
+    if (rc) {
+      std::fprintf(stderr, "InputMeshData: %s\n", UftLastError());
+      std::exit(1);
+    }
+  }
+  MeshRef elementEigen, eleXCoord, eleYCoord, eleZCoord, vertex, element;
+  int **loadDofs = nullptr;
+  double **load = nullptr;
+  int *fixedDofs = nullptr;
+  double *vertex_loc = nullptr;
+};
+
+// ------------------------------------------------------------------ SolveFem
+class SolveFem {
+ public:
+  // SolveFem ctor (SolveFem.cpp:6-15): DOF map, volumes; here the whole
+  // device setup (element operators, filter, hierarchy) happens once.
+  SolveFem(MeshRef, MeshRef, MeshRef, MeshRef, Vec /*Vol*/, Vec /*X*/) {
+    UftRuntime &R = UftRuntime::get();
+    if (uft_setup(R.ctx)) {
+      std::fprintf(stderr, "SolveFem: %s\n", UftLastError());
+      std::exit(1);
+    }
+  }
+  Vec U{UFT_U};
+  PetscScalar nu = 0.3;
+  // SolveFem::ComputeObjectiveAndSensitivities (SolveFem.h:39-40)
+  PetscErrorCode ComputeObjectiveAndSensitivities(PetscScalar *fx, PetscScalar *gx, Vec dfdx, PetscInt m, Vec *dgdx,
+                                                  Vec XPhys, PetscScalar Emin, PetscScalar Emax, PetscScalar penal,
+                                                  PetscScalar volfrac, MeshRef, MeshRef, MeshRef, Vec /*Vol*/, int **,
+                                                  double **, int *, double *) {
+    uft_ctx *c = UftRuntime::get().ctx;
+    uft_params p;
+    uft_params_get(c, &p);
+    if (p.Emin != Emin || p.Emax != Emax || p.penal != penal || p.volfrac != volfrac) {
+      p.Emin = Emin; p.Emax = Emax; p.penal = penal; p.volfrac = volfrac;
+      PetscErrorCode e = uft_params_set(c, &p);
+      if (e) return e;
+    }
+    std::vector<int> ids(m);
+    for (int j = 0; j < m; ++j) ids[j] = dgdx[j].id;
+    PetscErrorCode e = uft_fem_obj_sens(c, fx, gx, dfdx.id, m, ids.data(), XPhys.id);
+    int it = 0, itf = 0, outer = 0;
+    double rr = 0;
+    uft_solve_stats(c, &it, &rr, &itf, &outer);
+    std::printf("Norm of error %e, Iterations %d\n", rr, it);
+    return e == UFT_ERR_NOCONV ? 0 : e;  // the reference accepts an unconverged solve
+  }
+};
+
+// ------------------------------------------------------------------ PDEFilter
+class PDEFilter {
+ public:
+  PDEFilter(MeshRef, MeshRef, MeshRef, MeshRef, PetscScalar /*rmin*/, Vec /*X*/) {}
+  PetscErrorCode Filter(Vec X, Vec XFiled, PetscScalar Xmin) {
+    uft_ctx *c = UftRuntime::get().ctx;
+    set_xmin(c, Xmin);
+    return uft_filter(c, X.id, XFiled.id);
+  }
+  PetscErrorCode ComputeFilteredSens(Vec dfdx, Vec *dgdx, PetscInt m) {
+    std::vector<int> ids(m);
+    for (int j = 0; j < m; ++j) ids[j] = dgdx[j].id;
+    return uft_filter_sens(UftRuntime::get().ctx, dfdx.id, m, ids.data());
+  }
+  PetscErrorCode HeavisideFilter(Vec XFiled, Vec XPhys, PetscScalar x0, PetscScalar beta0, PetscScalar betamax,
+                                 PetscInt itr, PetscScalar Xmin, PetscBool proj) {
+    uft_ctx *c = UftRuntime::get().ctx;
+    uft_params p;
+    uft_params_get(c, &p);
+    p.x0 = x0; p.beta0 = beta0; p.betamax = betamax; p.Xmin = Xmin; p.proj = proj;
+    PetscErrorCode e = uft_params_set(c, &p);
+    return e ? e : uft_heaviside(c, XFiled.id, XPhys.id, itr);
+  }
+  PetscErrorCode ComputeProjectedSens(Vec XFiled, Vec dfdx, Vec *dgdx, PetscInt m, PetscScalar /*x0*/) {
+    std::vector<int> ids(m);
+    for (int j = 0; j < m; ++j) ids[j] = dgdx[j].id;
+    return uft_projected_sens(UftRuntime::get().ctx, XFiled.id, dfdx.id, m, ids.data());
+  }
+
+ private:
+  static void set_xmin(uft_ctx *c, double Xmin) {
+    uft_params p;
+    uft_params_get(c, &p);
+    if (p.Xmin != Xmin) {
+      p.Xmin = Xmin;
+      uft_params_set(c, &p);
+    }
+  }
+};
+
+// ------------------------------------------------------------------ MMA
+class MMA {
+ public:
+  MMA(PetscInt n, PetscInt m, Vec /*x*/, PetscScalar * /*a*/, PetscScalar * /*c*/, PetscScalar * /*d*/)
+      : n_(n), m_(m) {}
+  PetscErrorCode SetOuterMovelimit(PetscScalar Xmin, PetscScalar Xmax, PetscScalar movlim, Vec x, Vec xmin, Vec xmax) {
+    uft_ctx *c = UftRuntime::get().ctx;
+    uft_params p;
+    uft_params_get(c, &p);
+    if (p.Xmin != Xmin || p.Xmax != Xmax || p.movlim != movlim) {
+      p.Xmin = Xmin; p.Xmax = Xmax; p.movlim = movlim;
+      PetscErrorCode e = uft_params_set(c, &p);
+      if (e) return e;
+    }
+    return uft_mma_movelimit(c, x.id, xmin.id, xmax.id);
+  }
+  PetscErrorCode Update(Vec xval, Vec dfdx, PetscScalar *gx, Vec *dgdx, Vec xmin, Vec xmax) {
+    std::vector<int> ids(m_);
+    for (int j = 0; j < m_; ++j) ids[j] = dgdx[j].id;
+    return uft_mma_update(UftRuntime::get().ctx, xval.id, dfdx.id, gx, ids.data(), xmin.id, xmax.id);
+  }
+  PetscScalar DesignChange(Vec x, Vec xold) {
+    double ch = 0.0;
+    uft_mma_design_change(UftRuntime::get().ctx, x.id, xold.id, &ch);
+    return ch;
+  }
+
+ private:
+  PetscInt n_, m_;
+};
+
+inline PetscErrorCode OptPara::SetMMA(PetscInt * /*itr*/, MMA **mma) {
+  // a = 0, c = 1000, d = 0 (OptPara.cpp:166-174) are the context's MMA constants
+  *mma = new MMA((PetscInt)uft_size(UftRuntime::get().ctx, UFT_SIZE_NE), m, X, nullptr, nullptr, nullptr);
+  return 0;
+}
+
+// ------------------------------------------------------------------ Write2Vtu
+// ASCII VTU of the mesh with point data ux/uy/uz and cell data xphys
+// (Write2Vtu.cpp:5-173: hex connectivity reordered 0,1,3,2,4,5,7,6, cell type 12;
+// tet4 cells are type 10).
+class Write2Vtu {
+ public:
+  void WriteResults(MeshRef, MeshRef, Vec XPhys, Vec U, PetscInt itr) {
+    uft_ctx *c = UftRuntime::get().ctx;
+    const std::string dir = UftRuntime::get().str("-vtu_dir", "");
+    if (dir.empty()) return;
+    WriteFile(c, dir, XPhys, U, itr);
+  }
+  static int WriteFile(uft_ctx *c, const std::string &dir, Vec XPhys, Vec U, PetscInt itr) {
+    const long long ne = uft_size(c, UFT_SIZE_NE), nn = uft_size(c, UFT_SIZE_NN);
+    const int npe = (int)uft_size(c, UFT_SIZE_NPE);
+    std::vector<int32_t> el(ne * npe);
+    std::vector<double> xyz(nn * 3), u(nn * 3), x(ne);
+    int rc = uft_mesh_get(c, el.data(), xyz.data());
+    if (!rc) rc = uft_vec_get(c, U.id, u.data(), nn * 3);
+    if (!rc) rc = uft_vec_get(c, XPhys.id, x.data(), ne);
+    if (rc) return rc;
+    char name[4096];
+    std::snprintf(name, sizeof(name), "%s/result_%d.vtu", dir.c_str(), (int)itr);
+    FILE *f = std::fopen(name, "w");
+    if (!f) return UFT_ERR_IO;
+    std::fprintf(f, "<?xml version=\"1.0\"?>\n<VTKFile type=\"UnstructuredGrid\" version=\"0.1\" byte_order=\"LittleEndian\">\n");
+    std::fprintf(f, "<UnstructuredGrid>\n<Piece NumberOfPoints=\"%lld\" NumberOfCells=\"%lld\">\n", nn, ne);
+    std::fprintf(f, "<Points>\n<DataArray type=\"Float64\" NumberOfComponents=\"3\" format=\"ascii\">\n");
+    for (long long i = 0; i < nn; ++i) std::fprintf(f, "%.9g %.9g %.9g\n", xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
+    std::fprintf(f, "</DataArray>\n</Points>\n<Cells>\n<DataArray type=\"Int32\" Name=\"connectivity\" format=\"ascii\">\n");
+    static const int hexorder[8] = {0, 1, 3, 2, 4, 5, 7, 6};
+    for (long long e = 0; e < ne; ++e) {
+      for (int a = 0; a < npe; ++a) std::fprintf(f, "%d ", el[e * npe + (npe == 8 ? hexorder[a] : a)]);
+      std::fprintf(f, "\n");
+    }
+    std::fprintf(f, "</DataArray>\n<DataArray type=\"Int32\" Name=\"offsets\" format=\"ascii\">\n");
+    for (long long e = 0; e < ne; ++e) std::fprintf(f, "%lld\n", (e + 1) * npe);
+    std::fprintf(f, "</DataArray>\n<DataArray type=\"UInt8\" Name=\"types\" format=\"ascii\">\n");
+    for (long long e = 0; e < ne; ++e) std::fprintf(f, "%d\n", npe == 8 ? 12 : 10);
+    std::fprintf(f, "</DataArray>\n</Cells>\n<PointData>\n");
+    const char *nm[3] = {"ux", "uy", "uz"};
+    for (int cmp = 0; cmp < 3; ++cmp) {
+      std::fprintf(f, "<DataArray type=\"Float64\" Name=\"%s\" format=\"ascii\">\n", nm[cmp]);
+      for (long long i = 0; i < nn; ++i) std::fprintf(f, "%.9g\n", u[3 * i + cmp]);
+      std::fprintf(f, "</DataArray>\n");
+    }
+    std::fprintf(f, "</PointData>\n<CellData>\n<DataArray type=\"Float64\" Name=\"xphys\" format=\"ascii\">\n");
+    for (long long e = 0; e < ne; ++e) std::fprintf(f, "%.9g\n", x[e]);
+    std::fprintf(f, "</DataArray>\n</CellData>\n</Piece>\n</UnstructuredGrid>\n</VTKFile>\n");
+    std::fclose(f);
+    return 0;
+  }
+};

@@ -1,0 +1,334 @@
+"""ctypes binding of include/uft.h.
+
+The binding is plain plumbing: every computation happens in libuft.so on the
+GPU.  Host arrays are numpy float64/int32.
+"""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+_ROOT = os.path.dirname(_PKG)
+HEADER = os.path.join(_ROOT, "include", "uft.h")
+
+FP64, MIXED = 0, 1
+VEC = {"X": 0, "XFiled": 1, "XPhys": 2, "Xold": 3, "dfdx": 4, "xmin": 5, "xmax": 6, "Vol": 7, "U": 8,
+       "RX": 9, "dgdx0": 16}
+SIZE = {"ne": 0, "nn": 1, "ndof": 2, "npe": 3}
+STATUS = {0: "UFT_OK", 1: "UFT_ERR_ARG", 2: "UFT_ERR_IO", 3: "UFT_ERR_NOCONV", 4: "UFT_ERR_DETJ",
+          5: "UFT_ERR_OOM", 6: "UFT_ERR_HIP", 7: "UFT_ERR_STATE"}
+
+
+class UFTError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Params(C.Structure):
+    _fields_ = [(n, C.c_double) for n in
+                ("Emin", "Emax", "penal", "volfrac", "rmin", "nu", "movlim", "Xmin", "Xmax", "x0", "beta0",
+                 "betamax")] + \
+               [("proj", C.c_int), ("m", C.c_int), ("maxItr", C.c_int),
+                ("rtol_el", C.c_double), ("rtol_f", C.c_double), ("maxit_el", C.c_int), ("maxit_f", C.c_int),
+                ("precision", C.c_int), ("amg_levels", C.c_int), ("cheb_degree", C.c_int),
+                ("coarse_max", C.c_int), ("inner_rtol_exp", C.c_int)]
+
+
+_LIB = None
+
+
+def lib_path():
+    return os.path.join(_PKG, "lib", "libuft.so")
+
+
+def lib():
+    """Load libuft.so; raises if it has not been built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    path = lib_path()
+    if not os.path.exists(path):
+        raise FileNotFoundError(f"{path} is missing: run __graft_entry__.build() (make -C the package) first")
+    L = C.CDLL(path)
+    dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+    ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+    vp = C.c_void_p
+    pi = C.POINTER(C.c_int)
+    pd = C.POINTER(C.c_double)
+    sig = {
+        "uft_create": (C.c_int, [C.POINTER(vp), C.c_int]),
+        "uft_destroy": (None, [vp]),
+        "uft_last_error": (C.c_char_p, [vp]),
+        "uft_default_params": (None, [C.POINTER(Params)]),
+        "uft_params_set": (C.c_int, [vp, C.POINTER(Params)]),
+        "uft_params_get": (C.c_int, [vp, C.POINTER(Params)]),
+        "uft_mesh_set": (C.c_int, [vp, C.c_int, C.c_int64, C.c_int64, ip, dp]),
+        "uft_mesh_load_text": (C.c_int, [vp, C.c_int, C.c_int64, C.c_int64, C.c_char_p, C.c_char_p]),
+        "uft_bc_set": (C.c_int, [vp, C.c_int, ip, C.c_int, C.c_int, ip, dp]),
+        "uft_bc_load_text": (C.c_int, [vp, C.c_char_p, C.c_int, C.c_int, C.c_int]),
+        "uft_setup": (C.c_int, [vp]),
+        "uft_size": (C.c_int64, [vp, C.c_int]),
+        "uft_total_volume": (C.c_double, [vp]),
+        "uft_vec_get": (C.c_int, [vp, C.c_int, dp, C.c_int64]),
+        "uft_vec_set": (C.c_int, [vp, C.c_int, dp, C.c_int64]),
+        "uft_vec_fill": (C.c_int, [vp, C.c_int, C.c_double]),
+        "uft_vec_copy": (C.c_int, [vp, C.c_int, C.c_int]),
+        "uft_vec_device": (vp, [vp, C.c_int]),
+        "uft_fem_obj_sens": (C.c_int, [vp, pd, pd, C.c_int, C.c_int, pi, C.c_int]),
+        "uft_filter": (C.c_int, [vp, C.c_int, C.c_int]),
+        "uft_filter_sens": (C.c_int, [vp, C.c_int, C.c_int, pi]),
+        "uft_heaviside": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
+        "uft_projected_sens": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, pi]),
+        "uft_mma_movelimit": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
+        "uft_mma_update": (C.c_int, [vp, C.c_int, C.c_int, pd, pi, C.c_int, C.c_int]),
+        "uft_mma_design_change": (C.c_int, [vp, C.c_int, C.c_int, pd]),
+        "uft_design_iteration": (C.c_int, [vp, C.c_int, pd, pd, pd]),
+        "uft_solve_stats": (C.c_int, [vp, pi, pd, pi, pi]),
+        "uft_apply_K": (C.c_int, [vp, C.c_int, dp, dp]),
+        "uft_element_ke": (C.c_int, [vp, C.c_int64, dp]),
+        "uft_element_kf": (C.c_int, [vp, C.c_int64, dp, dp]),
+        "uft_sync": (C.c_int, [vp]),
+        "uft_box_nodes": (C.c_int64, [C.c_int] * 3),
+        "uft_box_elems": (C.c_int64, [C.c_int] * 4),
+        "uft_gen_box": (C.c_int, [C.c_int] * 4 + [C.c_double] * 4 + [C.c_uint64, dp, ip]),
+        "uft_box_bcs": (C.c_int, [C.c_int] * 4 + [ip, pi, ip, dp, pi]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _LIB = L
+    return L
+
+
+def header_symbols():
+    """Function names declared in include/uft.h."""
+    txt = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(uft_[a-z0-9_]+)\s*\(", txt)))
+
+
+def exported_symbols():
+    L = lib()
+    return sorted(n for n in header_symbols() if hasattr(L, n))
+
+
+def gen_box(npe, nx, ny, nz, lx=2.0, ly=1.0, lz=1.0, jitter=0.0, seed=0):
+    L = lib()
+    nn = L.uft_box_nodes(nx, ny, nz)
+    ne = L.uft_box_elems(npe, nx, ny, nz)
+    xyz = np.zeros(nn * 3)
+    elem = np.zeros(ne * npe, dtype=np.int32)
+    rc = L.uft_gen_box(npe, nx, ny, nz, lx, ly, lz, jitter, seed, xyz, elem)
+    if rc:
+        raise UFTError(rc, "uft_gen_box")
+    return xyz.reshape(nn, 3), elem.reshape(ne, npe)
+
+
+def box_bcs(nx, ny, nz, load_kind=0):
+    L = lib()
+    nn = (nx + 1) * (ny + 1) * (nz + 1)
+    fixed = np.zeros(3 * nn, dtype=np.int32)
+    ld = np.zeros(max(nn, 9), dtype=np.int32)
+    lv = np.zeros(max(nn, 9))
+    nf, nl = C.c_int(), C.c_int()
+    rc = L.uft_box_bcs(nx, ny, nz, load_kind, fixed, C.byref(nf), ld, lv, C.byref(nl))
+    if rc:
+        raise UFTError(rc, "uft_box_bcs")
+    return fixed[:nf.value].copy(), ld[:nl.value].copy(), lv[:nl.value].copy()
+
+
+class Context:
+    """One design problem on one GPU (uft_ctx)."""
+
+    def __init__(self, device=0):
+        L = lib()
+        h = C.c_void_p()
+        rc = L.uft_create(C.byref(h), device)
+        if rc:
+            raise UFTError(rc, "uft_create failed (no HIP device?)")
+        self.h = h
+        self.L = L
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.uft_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _chk(self, rc, what, allow=()):
+        if rc and rc not in allow:
+            raise UFTError(rc, f"{what}: {self.L.uft_last_error(self.h).decode()}")
+        return rc
+
+    # ---- configuration
+    @staticmethod
+    def default_params(**kw):
+        p = Params()
+        lib().uft_default_params(C.byref(p))
+        for k, v in kw.items():
+            if not hasattr(p, k):
+                raise AttributeError(k)
+            setattr(p, k, v)
+        return p
+
+    def set_params(self, p):
+        self._chk(self.L.uft_params_set(self.h, C.byref(p)), "uft_params_set")
+        self.params = p
+
+    def get_params(self):
+        p = Params()
+        self._chk(self.L.uft_params_get(self.h, C.byref(p)), "uft_params_get")
+        return p
+
+    def set_mesh(self, elem, xyz):
+        elem = np.ascontiguousarray(elem, dtype=np.int32)
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+        ne, npe = elem.shape
+        self._chk(self.L.uft_mesh_set(self.h, npe, xyz.shape[0], ne, elem.ravel(), xyz.ravel()), "uft_mesh_set")
+
+    def set_bcs(self, fixed, load_dofs, load_vals, nlc=1):
+        ld = np.ascontiguousarray(load_dofs, dtype=np.int32).ravel()
+        lv = np.ascontiguousarray(load_vals, dtype=np.float64).ravel()
+        fixed = np.ascontiguousarray(fixed, dtype=np.int32)
+        self._chk(self.L.uft_bc_set(self.h, len(fixed), fixed, nlc, len(ld) // nlc, ld, lv), "uft_bc_set")
+
+    def setup(self):
+        self._chk(self.L.uft_setup(self.h), "uft_setup")
+
+    def size(self, which):
+        return self.L.uft_size(self.h, SIZE[which])
+
+    @property
+    def total_volume(self):
+        return self.L.uft_total_volume(self.h)
+
+    # ---- vectors
+    def vid(self, name):
+        if isinstance(name, int):
+            return name
+        if name.startswith("dgdx") and name != "dgdx0":
+            return 16 + int(name[4:])
+        return VEC[name]
+
+    def vlen(self, name):
+        i = self.vid(name)
+        if i == VEC["U"]:
+            return self.size("ndof")
+        if i == VEC["RX"]:
+            return self.size("nn")
+        return self.size("ne")
+
+    def get(self, name):
+        out = np.zeros(self.vlen(name))
+        self._chk(self.L.uft_vec_get(self.h, self.vid(name), out, len(out)), "uft_vec_get")
+        return out
+
+    def set(self, name, arr):
+        arr = np.ascontiguousarray(arr, dtype=np.float64)
+        self._chk(self.L.uft_vec_set(self.h, self.vid(name), arr, len(arr)), "uft_vec_set")
+
+    # ---- the design iteration
+    def fem_obj_sens(self, m=1, allow_noconv=False):
+        fx = C.c_double()
+        gx = (C.c_double * 8)()
+        ids = (C.c_int * m)(*[16 + j for j in range(m)])
+        rc = self.L.uft_fem_obj_sens(self.h, C.byref(fx), gx, VEC["dfdx"], m, ids, VEC["XPhys"])
+        self._chk(rc, "uft_fem_obj_sens", allow=(3,) if allow_noconv else ())
+        return fx.value, [gx[j] for j in range(m)]
+
+    def filter(self, src="X", dst="XFiled"):
+        self._chk(self.L.uft_filter(self.h, self.vid(src), self.vid(dst)), "uft_filter")
+
+    def filter_sens(self, m=1):
+        ids = (C.c_int * m)(*[16 + j for j in range(m)])
+        self._chk(self.L.uft_filter_sens(self.h, VEC["dfdx"], m, ids), "uft_filter_sens")
+
+    def heaviside(self, itr):
+        self._chk(self.L.uft_heaviside(self.h, VEC["XFiled"], VEC["XPhys"], itr), "uft_heaviside")
+
+    def projected_sens(self, m=1):
+        ids = (C.c_int * m)(*[16 + j for j in range(m)])
+        self._chk(self.L.uft_projected_sens(self.h, VEC["XFiled"], VEC["dfdx"], m, ids), "uft_projected_sens")
+
+    def mma_movelimit(self):
+        self._chk(self.L.uft_mma_movelimit(self.h, VEC["X"], VEC["xmin"], VEC["xmax"]), "uft_mma_movelimit")
+
+    def mma_update(self, gx):
+        m = len(gx)
+        g = (C.c_double * m)(*gx)
+        ids = (C.c_int * m)(*[16 + j for j in range(m)])
+        self._chk(self.L.uft_mma_update(self.h, VEC["X"], VEC["dfdx"], g, ids, VEC["xmin"], VEC["xmax"]),
+                  "uft_mma_update")
+
+    def design_change(self):
+        ch = C.c_double()
+        self._chk(self.L.uft_mma_design_change(self.h, VEC["X"], VEC["Xold"], C.byref(ch)), "uft_mma_design_change")
+        return ch.value
+
+    def design_iteration(self, itr, allow_noconv=False):
+        fx, gx, ch = C.c_double(), (C.c_double * 8)(), C.c_double()
+        rc = self.L.uft_design_iteration(self.h, itr, C.byref(fx), gx, C.byref(ch))
+        self._chk(rc, "uft_design_iteration", allow=(3,) if allow_noconv else ())
+        m = self.get_params().m
+        return fx.value, [gx[j] for j in range(m)], ch.value
+
+    def init_design(self):
+        """Initial filter and projection (main.cpp:64-66)."""
+        self.filter("X", "XFiled")
+        self.heaviside(0)
+
+    def run(self, maxitr):
+        """while (itr < maxItr && ch > 0.01) (main.cpp:70)."""
+        self.init_design()
+        fxs, gxs, chs = [], [], []
+        ch, itr = 1.0, 0
+        while itr < maxitr and ch > 0.01:
+            itr += 1
+            fx, gx, ch = self.design_iteration(itr)
+            fxs.append(fx)
+            gxs.append(gx[0])
+            chs.append(ch)
+        return itr, np.array(fxs), np.array(gxs), np.array(chs)
+
+    def stats(self):
+        a, b, c, d = C.c_int(), C.c_double(), C.c_int(), C.c_int()
+        self._chk(self.L.uft_solve_stats(self.h, C.byref(a), C.byref(b), C.byref(c), C.byref(d)), "uft_solve_stats")
+        return {"it_el": a.value, "relres_el": b.value, "it_f": c.value, "outer_el": d.value}
+
+    def apply_K(self, u, xphys_name="XPhys"):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        y = np.zeros_like(u)
+        self._chk(self.L.uft_apply_K(self.h, self.vid(xphys_name), u, y), "uft_apply_K")
+        return y
+
+    def element_ke(self, e):
+        nd = 3 * self.size("npe")
+        out = np.zeros(nd * nd)
+        self._chk(self.L.uft_element_ke(self.h, e, out), "uft_element_ke")
+        return out.reshape(nd, nd)
+
+    def element_kf(self, e):
+        npe = self.size("npe")
+        kf, t = np.zeros(npe * npe), np.zeros(npe)
+        self._chk(self.L.uft_element_kf(self.h, e, kf, t), "uft_element_kf")
+        return kf.reshape(npe, npe), t
+
+    def sync(self):
+        self._chk(self.L.uft_sync(self.h), "uft_sync")
+
+
+def cantilever(npe=8, nx=8, ny=4, nz=4, jitter=0.0, seed=0, load_kind=0, device=0, **params):
+    """Box [0,2]x[0,1]x[0,1] cantilever, clamped at x=0 (uft_box_bcs)."""
+    xyz, elem = gen_box(npe, nx, ny, nz, 2.0, 1.0, 1.0, jitter, seed)
+    fixed, ld, lv = box_bcs(nx, ny, nz, load_kind)
+    ctx = Context(device)
+    ctx.set_params(Context.default_params(**params))
+    ctx.set_mesh(elem, xyz)
+    ctx.set_bcs(fixed, ld, lv)
+    ctx.setup()
+    return ctx
