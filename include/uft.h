@@ -147,6 +147,15 @@ int uft_element_ke(uft_ctx *ctx, int64_t e, double *ke);
 int uft_element_kf(uft_ctx *ctx, int64_t e, double *kf, double *t);
 /* Synchronise the context stream. */
 int uft_sync(uft_ctx *ctx);
+/* Device timers: HIP events recorded on the context stream around every
+ * launch of a timed kernel while enabled (no counterpart in the reference,
+ * whose only instrumentation is MPI_Wtime around the loop body,
+ * main.cpp:74,109).  uft_profile_read returns the number of launches timed and
+ * their summed duration in ms for kind 0 = element phase of the FP64 fine
+ * stiffness apply, 1 = its node phase, 2/3 = the same for the FP32 apply of
+ * the mixed form. */
+int uft_profile_enable(uft_ctx *ctx, int on);
+int uft_profile_read(uft_ctx *ctx, int kind, int64_t *count, double *total_ms);
 
 /* --- workload generator (synthetic cantilever) --------------------------- */
 int64_t uft_box_nodes(int nx, int ny, int nz);

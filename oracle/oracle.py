@@ -80,6 +80,7 @@ def lib():
                                                C.POINTER(C.c_double), C.POINTER(C.c_double)]),
             "orc_init_design": (C.c_int, [vp]),
             "orc_run": (C.c_int, [vp, C.c_int, dp, dp, dp]),
+            "orc_sample_costs": (C.c_int, [vp, C.c_int64, C.c_int, dp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -240,6 +241,12 @@ class Problem:
         fx, gx, ch = C.c_double(), (C.c_double * 8)(), C.c_double()
         rc = lib().orc_design_iteration(self.h, itr, C.byref(fx), gx, C.byref(ch))
         return fx.value, [gx[j] for j in range(self.m)], ch.value, rc
+
+    def sample_costs(self, ne_sample=4000, n_it=10):
+        out = np.zeros(8)
+        lib().orc_sample_costs(self.h, ne_sample, n_it, out)
+        return {"ke_per_elem": out[0], "asm_per_elem": out[1], "uku_per_elem": out[2], "pcg_iter": out[3],
+                "filter_iter": out[4], "mma_update": out[5]}
 
     def run(self, maxitr):
         fx = np.zeros(maxitr)

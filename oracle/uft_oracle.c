@@ -704,7 +704,9 @@ static int pcg_el(orc_ctx *C) {
   double rz = dot(n, r, z), rn = bn;
   int it = 0;
   if (bn == 0.0) { C->it_el = 0; C->relres_el = 0.0; free(r); free(z); free(p); free(q); free(dinv); return 0; }
+  const int progress = getenv("ORC_PROGRESS") != NULL;
   while (rn > C->P.rtol_el * bn && it < C->P.maxit_el) {
+    if (progress && it % 500 == 0) { fprintf(stderr, "[oracle pcg] it %d relres %.3e\n", it, rn / bn); fflush(stderr); }
     csr_mv(&C->K, p, q);
     double al = rz / dot(n, p, q);
     for (int64_t i = 0; i < n; i++) { C->U[i] += al * p[i]; r[i] -= al * q[i]; }
@@ -1123,4 +1125,94 @@ int orc_run(orc_ctx *C, int maxitr, double *fx_hist, double *gx_hist, double *ch
     if (ch_hist) ch_hist[itr - 1] = ch;
   }
   return itr;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Bounded CPU-baseline sample (used only by bench.py's cpu_baseline leg)     */
+/* ------------------------------------------------------------------------- */
+#include <time.h>
+static double wclock(void) {
+  struct timespec t;
+  clock_gettime(CLOCK_MONOTONIC, &t);
+  return t.tv_sec + 1e-9 * t.tv_nsec;
+}
+
+/* Times the pieces of one reference-structure design iteration on the full
+ * mesh held by C without running a full solve:
+ *   out[0] seconds per element of Hex8Isoparametric/Tet4Isoparametric (KE)
+ *   out[1] seconds per element of the 24x24 (12x12) CSR scatter-add
+ *   out[2] seconds per element of the u^T KE u sensitivity loop body
+ *   out[3] seconds per Jacobi-PCG iteration on the full assembled K pattern
+ *   out[4] seconds per filter CG iteration on the full KF
+ *   out[5] seconds of one MMA update (GenSub + dual interior point) on ne vars
+ * ne_sample elements are used for the per-element timings; the CSR values
+ * come from the cached KE of the first ne_sample elements replicated
+ * (timings do not depend on the values). */
+int orc_sample_costs(orc_ctx *C, int64_t ne_sample, int n_it, double *out) {
+  if (ne_sample > C->ne) ne_sample = C->ne;
+  int nd = C->nd, npe = C->npe;
+  double KE[576];
+  double t0 = wclock(), sink = 0.0;
+  for (int64_t e = 0; e < ne_sample; e++) {
+    double X[24];
+    elem_coords(C, e, X);
+    if (npe == 8) orc_hex8_ke(X, C->P.nu, KE); else orc_tet4_ke(X, C->P.nu, KE);
+    sink += KE[e % (nd * nd)];
+  }
+  out[0] = (wclock() - t0) / ne_sample;
+  memset(C->K.v, 0, C->K.rp[C->K.n] * sizeof(double));
+  t0 = wclock();
+  for (int64_t e = 0; e < ne_sample; e++) {
+    int32_t dof[24];
+    for (int k = 0; k < nd; k++) dof[k] = 3 * C->elem[e * npe + (k % npe)] + k / npe;
+    for (int p = 0; p < nd; p++)
+      for (int q = 0; q < nd; q++) C->K.v[csr_find(&C->K, dof[p], dof[q])] += KE[p * nd + q] * 0.5;
+  }
+  out[1] = (wclock() - t0) / ne_sample;
+  t0 = wclock();
+  for (int64_t e = 0; e < ne_sample; e++) {
+    double ue[24];
+    for (int k = 0; k < nd; k++) ue[k] = C->U[3 * C->elem[e * npe + (k % npe)] + k / npe] + 1e-3 * k;
+    double uKu = 0.0;
+    for (int p = 0; p < nd; p++)
+      for (int q = 0; q < nd; q++) uKu += ue[p] * KE[p * nd + q] * ue[q];
+    sink += uKu;
+  }
+  out[2] = (wclock() - t0) / ne_sample;
+  /* full-size K values from the last element matrix replicated, SPD-shifted */
+  memset(C->K.v, 0, C->K.rp[C->K.n] * sizeof(double));
+  for (int64_t e = 0; e < C->ne; e++) {
+    int32_t dof[24];
+    for (int k = 0; k < nd; k++) dof[k] = 3 * C->elem[e * npe + (k % npe)] + k / npe;
+    for (int p = 0; p < nd; p++)
+      for (int q = 0; q < nd; q++) C->K.v[csr_find(&C->K, dof[p], dof[q])] += KE[p * nd + q] + (p == q ? 1e-3 : 0.0);
+  }
+  for (int64_t i = 0; i < C->ndof; i++) C->F[i] = 1.0;
+  double rtol = C->P.rtol_el;
+  int maxit = C->P.maxit_el;
+  C->P.rtol_el = 1e-300;
+  C->P.maxit_el = n_it;
+  t0 = wclock();
+  pcg_el(C);
+  out[3] = (wclock() - t0) / n_it;
+  C->P.rtol_el = rtol;
+  C->P.maxit_el = maxit;
+  double rtf = C->P.rtol_f;
+  int mf = C->P.maxit_f;
+  C->P.rtol_f = 1e-300;
+  C->P.maxit_f = n_it;
+  for (int64_t i = 0; i < C->nn; i++) { C->TX[i] = 1.0 + 1e-3 * (i % 7); C->RX[i] = 0.0; }
+  t0 = wclock();
+  cg_filter(C, C->TX, C->RX);
+  out[4] = (wclock() - t0) / (C->it_f > 0 ? C->it_f : 1);
+  C->P.rtol_f = rtf;
+  C->P.maxit_f = mf;
+  double gx[ORC_MAXM] = {0.0};
+  for (int64_t e = 0; e < C->ne; e++) { C->dfdx[e] = -1.0 - (e % 13); C->dgdx[e] = C->vol[e] / C->TV; }
+  orc_mma_movelimit(C, C->X, C->xmin, C->xmax);
+  t0 = wclock();
+  orc_mma_update(C, C->X, C->dfdx, gx, C->dgdx, C->xmin, C->xmax);
+  out[5] = wclock() - t0;
+  out[6] = sink;
+  return 0;
 }

@@ -58,6 +58,12 @@ __global__ __launch_bounds__(TPB) void k_sub_inplace(int64_t n, const T *__restr
   const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
   if (i < n) r[i] -= t[i];
 }
+template <typename T>
+__global__ __launch_bounds__(TPB) void k_subv(int64_t n, const T *__restrict__ a, const T *__restrict__ b,
+                                              T *__restrict__ c) {
+  const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  if (i < n) c[i] = a[i] - b[i];
+}
 
 // ---------------------------------------------------------------- BSR 6x6
 // one thread per scalar row: y[6A+i] = sum_blocks sum_j V[blk][i][j] x[6B+j]
@@ -353,6 +359,9 @@ void cheb_stepf(hipStream_t s, int64_t n, const float *t, const float *dinv, dou
 }
 void sub_inplace(hipStream_t s, int64_t n, const double *t, double *r) { k_sub_inplace<double><<<nblk(n), TPB, 0, s>>>(n, t, r); }
 void sub_inplacef(hipStream_t s, int64_t n, const float *t, float *r) { k_sub_inplace<float><<<nblk(n), TPB, 0, s>>>(n, t, r); }
+void subf(hipStream_t s, int64_t n, const float *a, const float *b, float *c) {
+  k_subv<float><<<nblk(n), TPB, 0, s>>>(n, a, b, c);
+}
 void bsr6_mv(hipStream_t s, int64_t nb_rows, const int32_t *rp, const int32_t *ci, const double *val, const double *x,
              double *y) {
   k_bsr6_mv<double><<<nblk(nb_rows * 6), TPB, 0, s>>>(nb_rows, rp, ci, val, x, y);

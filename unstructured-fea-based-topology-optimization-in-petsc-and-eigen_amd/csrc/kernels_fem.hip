@@ -910,34 +910,42 @@ void gen_kf(hipStream_t s, int npe, int64_t ne, const int32_t *elem, const doubl
 static SimpCoef simp(const uft_ctx *C) { return SimpCoef{C->P.Emin, C->P.Emax - C->P.Emin, C->P.penal}; }
 
 void apply_K(hipStream_t s, const uft_ctx *C, const double *xphys, const double *u, double *y, const double *dotw,
-             double *part) {
+             double *part, hipEvent_t *ev) {
   const int ne = (int)C->ne, nn = (int)C->nn;
-  if (C->npe == 8) {
+  if (ev) (void)hipEventRecord(ev[0], s);
+  if (C->npe == 8)
     k_elem_apply<8, double, double2><<<nblk(ne), TPB, 0, s>>>(ne, C->d_elem, (const double2 *)C->d_KE, xphys,
                                                                simp(C), C->d_fixm, u, C->d_Ye);
-    k_node_gather<8, 3, 1, double><<<nblk(nn), TPB, 0, s>>>(nn, ne, C->d_iptr, C->d_inc, C->d_Ye, C->d_fixm, u, y,
-                                                            dotw, part);
-  } else {
+  else
     k_elem_apply<4, double, double2><<<nblk(ne), TPB, 0, s>>>(ne, C->d_elem, (const double2 *)C->d_KE, xphys,
                                                                simp(C), C->d_fixm, u, C->d_Ye);
+  if (ev) (void)hipEventRecord(ev[1], s);
+  if (C->npe == 8)
+    k_node_gather<8, 3, 1, double><<<nblk(nn), TPB, 0, s>>>(nn, ne, C->d_iptr, C->d_inc, C->d_Ye, C->d_fixm, u, y,
+                                                            dotw, part);
+  else
     k_node_gather<4, 3, 1, double><<<nblk(nn), TPB, 0, s>>>(nn, ne, C->d_iptr, C->d_inc, C->d_Ye, C->d_fixm, u, y,
                                                             dotw, part);
-  }
+  if (ev) (void)hipEventRecord(ev[2], s);
 }
 
-void apply_Kf(hipStream_t s, const uft_ctx *C, const double *xphys, const float *u, float *y) {
+void apply_Kf(hipStream_t s, const uft_ctx *C, const double *xphys, const float *u, float *y, hipEvent_t *ev) {
   const int ne = (int)C->ne, nn = (int)C->nn;
-  if (C->npe == 8) {
+  if (ev) (void)hipEventRecord(ev[0], s);
+  if (C->npe == 8)
     k_elem_apply<8, float, float2><<<nblk(ne), TPB, 0, s>>>(ne, C->d_elem, (const float2 *)C->d_KEf, xphys,
                                                              simp(C), C->d_fixm, u, C->d_Yef);
-    k_node_gather<8, 3, 1, float><<<nblk(nn), TPB, 0, s>>>(nn, ne, C->d_iptr, C->d_inc, C->d_Yef, C->d_fixm, u, y,
-                                                           nullptr, nullptr);
-  } else {
+  else
     k_elem_apply<4, float, float2><<<nblk(ne), TPB, 0, s>>>(ne, C->d_elem, (const float2 *)C->d_KEf, xphys,
                                                              simp(C), C->d_fixm, u, C->d_Yef);
+  if (ev) (void)hipEventRecord(ev[1], s);
+  if (C->npe == 8)
+    k_node_gather<8, 3, 1, float><<<nblk(nn), TPB, 0, s>>>(nn, ne, C->d_iptr, C->d_inc, C->d_Yef, C->d_fixm, u, y,
+                                                           nullptr, nullptr);
+  else
     k_node_gather<4, 3, 1, float><<<nblk(nn), TPB, 0, s>>>(nn, ne, C->d_iptr, C->d_inc, C->d_Yef, C->d_fixm, u, y,
                                                            nullptr, nullptr);
-  }
+  if (ev) (void)hipEventRecord(ev[2], s);
 }
 
 __global__ void k_inv(int64_t n, const double *__restrict__ d, double *__restrict__ dinv, float *__restrict__ dinvf) {

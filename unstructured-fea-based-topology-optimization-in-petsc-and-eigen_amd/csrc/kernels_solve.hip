@@ -113,3 +113,36 @@ void sc_beta(hipStream_t s, double *sc, int inew, int iold, int ibeta, double *l
 }
 void sc_copy(hipStream_t s, const double *src, double *dst) { k_copy_scalar<<<1, 1, 0, s>>>(src, dst); }
 }  // namespace uftk
+
+namespace {
+__global__ __launch_bounds__(TPB) void k_f2d(int64_t n, const float *__restrict__ a, double *__restrict__ b) {
+  const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  if (i < n) b[i] = (double)a[i];
+}
+// partials of r.z and r.zo (flexible CG, Polak-Ribiere form)
+__global__ __launch_bounds__(TPB) void k_dot_pr(int64_t n, const double *__restrict__ r, const double *__restrict__ z,
+                                                const double *__restrict__ zo, double *__restrict__ part) {
+  const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  double v[2] = {0.0, 0.0};
+  if (i < n) {
+    v[0] = r[i] * z[i];
+    v[1] = r[i] * zo[i];
+  }
+  block_sum_store<2>(v, part);
+}
+// beta = (rz - rzo) / rz_prev ; rz_prev = rz
+__global__ void k_beta_flex(double *sc, int irz, int irzo, int iprev, int ibeta) {
+  sc[ibeta] = (sc[irz] - sc[irzo]) / sc[iprev];
+  sc[iprev] = sc[irz];
+}
+}  // namespace
+
+namespace uftk {
+void f2d(hipStream_t s, int64_t n, const float *a, double *b) { k_f2d<<<nblk(n), TPB, 0, s>>>(n, a, b); }
+void dot_pr(hipStream_t s, int64_t n, const double *r, const double *z, const double *zo, double *part) {
+  k_dot_pr<<<nblk(n), TPB, 0, s>>>(n, r, z, zo, part);
+}
+void sc_beta_flex(hipStream_t s, double *sc, int irz, int irzo, int iprev, int ibeta) {
+  k_beta_flex<<<1, 1, 0, s>>>(sc, irz, irzo, iprev, ibeta);
+}
+}  // namespace uftk

@@ -76,13 +76,43 @@ bool valid_vec(const uft_ctx *C, int id) {
 }
 
 // ---------------------------------------------------------------- operators
+// fine stiffness apply, with an event triple recorded around its two phases
+// while the device timers are enabled
+hipEvent_t *prof_triple(uft_ctx *C, int kind) {
+  if (!C->prof) return nullptr;
+  while (C->evpool.size() < C->evused + 3) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    C->evpool.push_back(e);
+  }
+  hipEvent_t *ev = &C->evpool[C->evused];
+  C->evkind.resize(C->evpool.size() / 3 + 1);
+  C->evkind[C->evused / 3] = kind;
+  C->evused += 3;
+  return ev;
+}
+
+void fine_apply(uft_ctx *C, const double *xphys, const double *x, double *y, const double *dotw, double *part) {
+  uftk::apply_K(C->s, C, xphys, x, y, dotw, part, prof_triple(C, 0));
+}
+void fine_applyf(uft_ctx *C, const double *xphys, const float *x, float *y) {
+  uftk::apply_Kf(C->s, C, xphys, x, y, prof_triple(C, 1));
+}
+
 // y = A_l x on multigrid level l
 void level_apply(uft_ctx *C, int l, const double *x, double *y) {
   AmgLevel &Lv = C->amg.L[l];
   if (l == 0)
-    uftk::apply_K(C->s, C, C->cur_xphys, x, y, nullptr, nullptr);
+    fine_apply(C, C->cur_xphys, x, y, nullptr, nullptr);
   else
     uftk::bsr6_mv(C->s, Lv.nunit, Lv.d_rp, Lv.d_ci, Lv.d_val, x, y);
+}
+void level_applyf(uft_ctx *C, int l, const float *x, float *y) {
+  AmgLevel &Lv = C->amg.L[l];
+  if (l == 0)
+    fine_applyf(C, C->cur_xphys, x, y);
+  else
+    uftk::bsr6_mvf(C->s, Lv.nunit, Lv.d_rp, Lv.d_ci, Lv.d_valf, x, y);
 }
 
 // Chebyshev interval [0.1, 1.1] * lambda_max(D^-1 A), the PCGAMG default
@@ -138,6 +168,42 @@ void vcycle(uft_ctx *C, int l, const double *b, double *x) {
   for (int k = 1; k < deg; ++k) {
     level_apply(C, l, Lv.d, Lv.t);
     uftk::cheb_step(s, n, Lv.t, Lv.d_dinv, cc.c1[k], cc.c2[k], Lv.r, Lv.d, x);
+  }
+}
+
+// The same V-cycle on FP32 vectors with FP32 operators (mixed form); the
+// coarsest solve stays FP64.
+void vcycle_f(uft_ctx *C, int l, const float *b, float *x) {
+  AmgHier &H = C->amg;
+  hipStream_t s = C->s;
+  if (l == H.nlev - 1) {
+    AmgLevel &Lc = H.L[l];
+    uftk::f2d(s, Lc.n, b, Lc.b);
+    uftk::dense_mv(s, H.ncoarse, H.d_cinv, Lc.b, Lc.x);
+    uftk::d2f(s, Lc.n, Lc.x, x);
+    return;
+  }
+  AmgLevel &Lv = H.L[l];
+  AmgLevel &Ln = H.L[l + 1];
+  const int64_t n = Lv.n;
+  const int deg = std::max(1, C->P.cheb_degree);
+  const ChebCoef cc = cheb_coef(Lv.lmax, deg);
+  uftk::cheb_firstf(s, n, b, Lv.d_dinvf, cc.inv_theta, Lv.rf, Lv.df, x, 1);
+  for (int k = 1; k < deg; ++k) {
+    level_applyf(C, l, Lv.df, Lv.tf);
+    uftk::cheb_stepf(s, n, Lv.tf, Lv.d_dinvf, cc.c1[k], cc.c2[k], Lv.rf, Lv.df, x);
+  }
+  level_applyf(C, l, Lv.df, Lv.tf);
+  uftk::sub_inplacef(s, n, Lv.tf, Lv.rf);
+  uftk::restrictf(s, Lv.bs, Ln.nunit, Lv.d_cptr, Lv.d_child, Lv.d_Pf, Lv.rf, Ln.bf);
+  vcycle_f(C, l + 1, Ln.bf, Ln.xf);
+  uftk::prolong_addf(s, Lv.bs, Lv.nunit, Lv.d_parent, Lv.d_Pf, Ln.xf, x);
+  level_applyf(C, l, x, Lv.tf);
+  uftk::subf(s, n, b, Lv.tf, Lv.rf);  // r = b - A x
+  uftk::cheb_firstf(s, n, b, Lv.d_dinvf, cc.inv_theta, Lv.rf, Lv.df, x, 0);
+  for (int k = 1; k < deg; ++k) {
+    level_applyf(C, l, Lv.df, Lv.tf);
+    uftk::cheb_stepf(s, n, Lv.tf, Lv.d_dinvf, cc.c1[k], cc.c2[k], Lv.rf, Lv.df, x);
   }
 }
 
@@ -230,6 +296,7 @@ int amg_refresh(uft_ctx *C, const double *xphys) {
       uftk::galerkin_coarse(s, Ln.nblk, Lv.d_glptr, Lv.d_gl, Lv.d_brow, Lv.d_ci, Lv.d_P, Lv.d_val, Ln.d_val);
     uftk::fixed_unit_diag(s, Ln.nunit, Ln.d_rp, Ln.d_ci, Ln.d_val);
     uftk::bsr6_dinv(s, Ln.nunit, Ln.d_rp, Ln.d_ci, Ln.d_val, Ln.d_dinv, Ln.d_dinvf);
+    if (Ln.d_valf) uftk::d2f(s, Ln.nblk * 36, Ln.d_val, Ln.d_valf);
   }
   AmgLevel &Lc = H.L[H.nlev - 1];
   uftk::bsr_to_dense(s, Lc.nunit, Lc.d_rp, Lc.d_ci, Lc.d_val, H.ncoarse, H.d_cmat);
@@ -270,7 +337,7 @@ int solve_el(uft_ctx *C, const double *xphys, double *U) {
   int it = 0;
   bool conv = false;
   while (it < C->P.maxit_el) {
-    uftk::apply_K(s, C, xphys, p, q, p, C->d_part);
+    fine_apply(C, xphys, p, q, p, C->d_part);
     uftk::final_sum(s, C->d_part, uftk::red_blocks(C->nn), 1, C->d_scal + S_PQ);
     uftk::sc_div(s, C->d_scal, S_RZ, S_PQ, S_ALPHA);
     uftk::cg_update(s, n, C->d_scal, S_ALPHA, U, p, r, q, C->d_part);
@@ -291,7 +358,7 @@ int solve_el(uft_ctx *C, const double *xphys, double *U) {
   }
   C->it_el = it;
   // true residual (SolveFem.cpp:142-152)
-  uftk::apply_K(s, C, xphys, U, q, nullptr, nullptr);
+  fine_apply(C, xphys, U, q, nullptr, nullptr);
   uftk::sub(s, n, C->d_F, q, q);
   uftk::dot(s, n, q, q, C->d_part, C->d_scal + S_MISC);
   rc = fetch(C, C->d_scal + S_MISC, 1, C->h_scal);
@@ -300,6 +367,73 @@ int solve_el(uft_ctx *C, const double *xphys, double *U) {
   if (!conv)
     return uft_set_error(C, UFT_ERR_NOCONV, "elasticity solve: %d iterations, relative residual %.3e > %.3e", it,
                          C->relres_el, C->P.rtol_el);
+  return UFT_OK;
+}
+
+// Mixed form: FP64 flexible CG (Polak-Ribiere beta, which tolerates a
+// preconditioner that is not exactly linear) whose residual, inner products
+// and iterate stay FP64; the preconditioner is one V-cycle on FP32 vectors
+// with FP32 element blocks and FP32 coarse operators, coarsest solve FP64.
+int solve_el_mixed(uft_ctx *C, const double *xphys, double *U) {
+  hipStream_t s = C->s;
+  const int64_t n = C->ndof;
+  double *r = C->d_r, *z = C->d_z, *p = C->d_p, *q = C->d_q, *zo = C->d_w;
+  float *rf = C->d_rf, *zf = C->d_zf;
+  int rc = amg_refresh(C, xphys);
+  if (rc) return rc;
+  uftk::fill(s, n, 0.0, U);
+  uftk::copy(s, n, C->d_F, r);
+  uftk::dot(s, n, r, r, C->d_part, C->d_scal + S_RR);
+  if ((rc = fetch(C, C->d_scal + S_RR, 1, C->h_scal))) return rc;
+  const double bn = std::sqrt(C->h_scal[0]);
+  C->fnorm = bn;
+  C->it_el = 0;
+  C->outer_el = 1;
+  if (bn == 0.0) {
+    C->relres_el = 0.0;
+    return UFT_OK;
+  }
+  auto precond = [&]() {
+    uftk::d2f(s, n, r, rf);
+    vcycle_f(C, 0, rf, zf);
+    uftk::f2d(s, n, zf, z);
+  };
+  precond();
+  uftk::dot(s, n, r, z, C->d_part, C->d_scal + S_RZ);
+  uftk::copy(s, n, z, p);
+  const double tol = C->P.rtol_el * bn;
+  int it = 0;
+  bool conv = false;
+  while (it < C->P.maxit_el) {
+    fine_apply(C, xphys, p, q, p, C->d_part);
+    uftk::final_sum(s, C->d_part, uftk::red_blocks(C->nn), 1, C->d_scal + S_PQ);
+    uftk::sc_div(s, C->d_scal, S_RZ, S_PQ, S_ALPHA);
+    uftk::cg_update(s, n, C->d_scal, S_ALPHA, U, p, r, q, C->d_part);
+    uftk::final_sum(s, C->d_part, uftk::red_blocks(n), 1, C->d_scal + S_RR);
+    ++it;
+    if ((rc = fetch(C, C->d_scal + S_RR, 1, C->h_scal))) return rc;
+    const double rn = std::sqrt(C->h_scal[0]);
+    if (!std::isfinite(rn)) return uft_set_error(C, UFT_ERR_NOCONV, "mixed CG diverged at iteration %d", it);
+    if (rn <= tol) {
+      conv = true;
+      break;
+    }
+    std::swap(z, zo);  // zo keeps the previous preconditioned residual
+    precond();
+    uftk::dot_pr(s, n, r, z, zo, C->d_part);
+    uftk::final_sum(s, C->d_part, uftk::red_blocks(n), 2, C->d_scal + S_RZNEW);  // RZNEW, RZNEW+1 = r.zo
+    uftk::sc_beta_flex(s, C->d_scal, S_RZNEW, S_RZNEW + 1, S_RZ, S_BETA + 2);
+    uftk::xpby(s, n, z, C->d_scal, S_BETA + 2, p);
+  }
+  C->it_el = it;
+  fine_apply(C, xphys, U, q, nullptr, nullptr);
+  uftk::sub(s, n, C->d_F, q, q);
+  uftk::dot(s, n, q, q, C->d_part, C->d_scal + S_MISC);
+  if ((rc = fetch(C, C->d_scal + S_MISC, 1, C->h_scal))) return rc;
+  C->relres_el = std::sqrt(C->h_scal[0]) / bn;
+  if (!conv)
+    return uft_set_error(C, UFT_ERR_NOCONV, "mixed elasticity solve: %d iterations, relative residual %.3e", it,
+                         C->relres_el);
   return UFT_OK;
 }
 
@@ -711,7 +845,7 @@ int uft_fem_obj_sens(uft_ctx *C, double *fx, double *gx, int dfdx_id, int m, con
       return uft_set_error(C, UFT_ERR_ARG, "uft_fem_obj_sens: bad dgdx id");
   const double *xp = C->vec[xphys_id];
   double *U = C->vec[UFT_U];
-  const int src = solve_el(C, xp, U);
+  const int src = C->P.precision == UFT_MIXED ? solve_el_mixed(C, xp, U) : solve_el(C, xp, U);
   if (src != UFT_OK && src != UFT_ERR_NOCONV) return src;
   // SolveFem.cpp:208-249: only constraint 0 is filled by the reference
   uftk::sens(C->s, C, xp, U, C->vec[dfdx_id], C->vec[dgdx_ids[0]], C->d_part);
@@ -1026,3 +1160,34 @@ int uft_sync(uft_ctx *C) {
 }
 
 }  // extern "C"
+
+extern "C" int uft_profile_enable(uft_ctx *C, int on) {
+  if (!C) return UFT_ERR_ARG;
+  C->prof = on != 0;
+  C->evused = 0;
+  for (int k = 0; k < 4; ++k) {
+    C->prof_ms[k] = 0.0;
+    C->prof_n[k] = 0;
+  }
+  return UFT_OK;
+}
+
+extern "C" int uft_profile_read(uft_ctx *C, int kind, int64_t *count, double *total_ms) {
+  if (!C || kind < 0 || kind > 3 || !count || !total_ms) return UFT_ERR_ARG;
+  UFT_HIP(C, hipStreamSynchronize(C->s));
+  // fold the recorded triples into the running totals
+  for (size_t i = 0; i + 3 <= C->evused; i += 3) {
+    float a = 0.f, b = 0.f;
+    UFT_HIP(C, hipEventElapsedTime(&a, C->evpool[i], C->evpool[i + 1]));
+    UFT_HIP(C, hipEventElapsedTime(&b, C->evpool[i + 1], C->evpool[i + 2]));
+    const int k = 2 * C->evkind[i / 3];
+    C->prof_ms[k] += a;
+    C->prof_ms[k + 1] += b;
+    C->prof_n[k]++;
+    C->prof_n[k + 1]++;
+  }
+  C->evused = 0;
+  *count = C->prof_n[kind];
+  *total_ms = C->prof_ms[kind];
+  return UFT_OK;
+}

@@ -143,6 +143,14 @@ struct uft_ctx {
   double fnorm = 0.0;
   double *d_lanczos = nullptr;       // CG scalar log for eigenvalue estimates
 
+  // device timers (uft_profile_*): event triples around fine applies
+  bool prof = false;
+  std::vector<hipEvent_t> evpool;
+  std::vector<int> evkind;  // per triple: 0 FP64 apply, 1 FP32 apply
+  size_t evused = 0;
+  double prof_ms[4] = {0.0, 0.0, 0.0, 0.0};
+  int64_t prof_n[4] = {0, 0, 0, 0};
+
   // stats
   int it_el = 0, it_f = 0, outer_el = 0, dip_steps = 0;
   double relres_el = 0.0;
@@ -161,8 +169,9 @@ void gen_kf(hipStream_t s, int npe, int64_t ne, const int32_t *elem, const doubl
             double *KF, double *T, double *vol, double *part);
 // fine operator apply: y = K_D(x) u   (two-phase element/node)
 void apply_K(hipStream_t s, const uft_ctx *C, const double *xphys, const double *u, double *y,
-             const double *dotw, double *part);
-void apply_Kf(hipStream_t s, const uft_ctx *C, const double *xphys, const float *u, float *y);
+             const double *dotw, double *part, hipEvent_t *ev = nullptr);
+void apply_Kf(hipStream_t s, const uft_ctx *C, const double *xphys, const float *u, float *y,
+              hipEvent_t *ev = nullptr);
 void diag_K(hipStream_t s, const uft_ctx *C, const double *xphys, double *dinv, float *dinvf);
 void sens(hipStream_t s, const uft_ctx *C, const double *xphys, const double *u, double *dfdx,
           double *dgdx0, double *part);
@@ -206,6 +215,7 @@ void cheb_stepf(hipStream_t s, int64_t n, const float *t, const float *dinv, dou
                 float *r, float *d, float *x);
 void sub_inplace(hipStream_t s, int64_t n, const double *t, double *r);
 void sub_inplacef(hipStream_t s, int64_t n, const float *t, float *r);
+void subf(hipStream_t s, int64_t n, const float *a, const float *b, float *c);
 void bsr6_mv(hipStream_t s, int64_t nb_rows, const int32_t *rp, const int32_t *ci, const double *val,
              const double *x, double *y);
 void bsr6_mvf(hipStream_t s, int64_t nb_rows, const int32_t *rp, const int32_t *ci, const float *val,
@@ -249,6 +259,9 @@ void fcg_p(hipStream_t s, int64_t n, const double *z, const double *sc, int ib, 
 void sc_div(hipStream_t s, double *sc, int a, int b, int dst);
 void sc_beta(hipStream_t s, double *sc, int inew, int iold, int ibeta, double *log2);
 void sc_copy(hipStream_t s, const double *src, double *dst);
+void f2d(hipStream_t s, int64_t n, const float *a, double *b);
+void dot_pr(hipStream_t s, int64_t n, const double *r, const double *z, const double *zo, double *part);
+void sc_beta_flex(hipStream_t s, double *sc, int irz, int irzo, int iprev, int ibeta);
 }  // namespace uftk
 
 // host helpers (uft_api.cpp)

@@ -91,6 +91,9 @@ def lib():
         "uft_element_ke": (C.c_int, [vp, C.c_int64, dp]),
         "uft_element_kf": (C.c_int, [vp, C.c_int64, dp, dp]),
         "uft_sync": (C.c_int, [vp]),
+        "uft_profile_enable": (C.c_int, [vp, C.c_int]),
+        "uft_profile_read": (C.c_int, [vp, C.c_int, C.POINTER(C.c_int64), pd]),
+        "uft_mesh_get": (C.c_int, [vp, ip, dp]),
         "uft_box_nodes": (C.c_int64, [C.c_int] * 3),
         "uft_box_elems": (C.c_int64, [C.c_int] * 4),
         "uft_gen_box": (C.c_int, [C.c_int] * 4 + [C.c_double] * 4 + [C.c_uint64, dp, ip]),
