@@ -41,7 +41,7 @@ WORKLOADS = {
 # the build host by running the oracle's design iterations 1-2 in full
 # ("CPU baseline").  Used to scale the per-iteration costs the
 # bounded sample measures.
-ORACLE_ITERS = {"c1": {"pcg": None, "filter": None}}
+ORACLE_ITERS = {"c1": {"pcg": 1624, "filter": 42}}
 
 
 def workload(name, precision):
@@ -95,7 +95,18 @@ def main():
     ap.add_argument("--workload", default="c1", choices=sorted(WORKLOADS))
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device copy three times (reference for rocprof PMC scaling)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        a = torch.ones(2 ** 27, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": a.numel() * 8, "copies": 3}))
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one MI355X: this build has no multi-GPU path (one GPU per design; "
               "see DESIGN.md for how the path would shard). Run with --gpus 1.", file=sys.stderr)
