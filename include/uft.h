@@ -60,7 +60,7 @@ typedef struct uft_params {
   int amg_levels;      /* total levels incl. the fine one (reference: PCGAMGSetNlevels 4) */
   int cheb_degree;     /* Chebyshev-Jacobi smoother degree per level */
   int coarse_max;      /* stop coarsening at or below this many coarse DOFs */
-  int inner_rtol_exp;  /* mixed form: inner FP32 solve reduces the residual by 10^-exp */
+  int cycle_type;      /* coarse-grid cycle: 0 V-cycle, 1 W-cycle, 2 K-cycle (two FCG steps per coarse level) */
 } uft_params;
 
 /* --- context ------------------------------------------------------------ */
@@ -145,6 +145,16 @@ int uft_apply_K(uft_ctx *ctx, int xphys_id, const double *u, double *y);
 int uft_element_ke(uft_ctx *ctx, int64_t e, double *ke);
 /* Helmholtz element matrix KF (npe x npe) and T (npe) (PDEFilter.cpp:185-240). */
 int uft_element_kf(uft_ctx *ctx, int64_t e, double *kf, double *t);
+/* Shape of the solver hierarchy (no counterpart in the reference, whose GAMG
+ * hierarchy is internal to PETSc): level count (at most 8), unknowns per level
+ * and, per level, the element count (level 0) or 6x6 block count. */
+int uft_solver_info(const uft_ctx *ctx, int *nlev, int64_t *n_per_level, int64_t *blocks_per_level);
+/* Wall time in ms of the phases of the last uft_design_iteration (the
+ * reference prints only the whole iteration, main.cpp:112): 0 solver
+ * hierarchy refresh, 1 elasticity Krylov solve, 2 sensitivities, 3 sensitivity
+ * filter, 4 MMA update + design change, 5 density filter + projection,
+ * 6 whole iteration.  ms must hold 7 doubles. */
+int uft_phase_times(const uft_ctx *ctx, double *ms);
 /* Synchronise the context stream. */
 int uft_sync(uft_ctx *ctx);
 /* Device timers: HIP events recorded on the context stream around every

@@ -117,23 +117,34 @@ __global__ __launch_bounds__(TPB) void k_bsr6_dinv(int64_t nrows, const int32_t 
 }
 
 // ---------------------------------------------------------------- transfers
+// one wavefront per coarse unit: lanes stride over the (child, row) pairs of
+// the unit, then a fixed-order cross-lane sum of the 6 components
 template <int BS, typename T>
 __global__ __launch_bounds__(TPB) void k_restrict(int64_t nc, const int32_t *__restrict__ cptr,
                                                   const int32_t *__restrict__ child, const T *__restrict__ P,
                                                   const T *__restrict__ r, T *__restrict__ bc) {
-  const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
-  if (t >= nc * 6) return;
-  const int64_t A = t / 6;
-  const int j = (int)(t - A * 6);
-  T acc = T(0);
-  for (int k = cptr[A]; k < cptr[A + 1]; ++k) {
-    const int64_t u = child[k];
-    const T *Pu = P + (size_t)u * BS * 6;
-    const T *ru = r + u * BS;
+  const int64_t A = ((int64_t)blockIdx.x * TPB + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (A >= nc) return;  // whole wavefront
+  const int k0 = cptr[A], npair = (cptr[A + 1] - k0) * BS;
+  T acc[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
+  for (int q = lane; q < npair; q += 64) {
+    const int kk = q / BS, i = q - kk * BS;
+    const int64_t u = child[k0 + kk];
+    const T *Pu = P + (size_t)u * BS * 6 + i * 6;
+    const T ri = r[u * BS + i];
 #pragma unroll
-    for (int i = 0; i < BS; ++i) acc += Pu[i * 6 + j] * ru[i];
+    for (int j = 0; j < 6; ++j) acc[j] += Pu[j] * ri;
   }
-  bc[t] = acc;
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) acc[j] += __shfl_down(acc[j], off, 64);
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int j = 0; j < 6; ++j) bc[A * 6 + j] = acc[j];
+  }
 }
 template <int BS, typename T>
 __global__ __launch_bounds__(TPB) void k_prolong_add(int64_t nu, const int32_t *__restrict__ parent,
@@ -282,6 +293,42 @@ __global__ __launch_bounds__(TPB) void k_bsr_to_dense(int64_t nrows, const int32
   }
 }
 
+// In-place Gauss-Jordan inversion of an SPD matrix (no pivoting needed): for
+// each pivot k the pivot row is scaled and every other row is eliminated in
+// parallel; the pivot column is staged in LDS first.  One workgroup.
+__global__ __launch_bounds__(1024) void k_dense_gj_inv(int n, double *__restrict__ A, double *__restrict__ Ainv,
+                                                       float *__restrict__ Ainvf) {
+  extern __shared__ double lds[];  // pivot column (n) and scaled pivot row (n)
+  double *colk = lds, *rowk = lds + n;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int lane = tid & 63, wave = tid >> 6, nwave = nt >> 6;
+  for (int i = wave; i < n; i += nwave)
+    for (int j = lane; j < n; j += 64) Ainv[(size_t)i * n + j] = A[(size_t)i * n + j];
+  __syncthreads();
+  for (int k = 0; k < n; ++k) {
+    for (int i = tid; i < n; i += nt) colk[i] = Ainv[(size_t)i * n + k];
+    __syncthreads();
+    double piv = colk[k];
+    if (!(piv > 0.0)) piv = 1.0;  // decoupled zero row/column of a rank-deficient aggregate
+    const double ip = 1.0 / piv;
+    for (int j = tid; j < n; j += nt) {
+      const double v = (j == k ? 1.0 : Ainv[(size_t)k * n + j]) * ip;
+      rowk[j] = v;
+      Ainv[(size_t)k * n + j] = v;
+    }
+    __syncthreads();
+    for (int i = wave; i < n; i += nwave) {
+      if (i == k) continue;
+      const double f = colk[i];
+      double *Ai = Ainv + (size_t)i * n;
+      for (int j = lane; j < n; j += 64) Ai[j] = (j == k ? 0.0 : Ai[j]) - f * rowk[j];
+    }
+    __syncthreads();
+  }
+  if (Ainvf)
+    for (int64_t t = tid; t < (int64_t)n * n; t += nt) Ainvf[t] = (float)Ainv[t];
+}
+
 // In-place Cholesky A = L L^T (lower triangle), then Ainv = L^-T L^-1.
 // One workgroup of 1024 threads; n is at most a few thousand.
 __global__ __launch_bounds__(1024) void k_dense_chol_inv(int n, double *__restrict__ A, double *__restrict__ W,
@@ -380,9 +427,9 @@ void fixed_unit_diag(hipStream_t s, int64_t nb_rows, const int32_t *rp, const in
 void restrict_(hipStream_t s, int bs, int64_t nc, const int32_t *cptr, const int32_t *child, const double *P,
                const double *r, double *bc) {
   if (bs == 3)
-    k_restrict<3, double><<<nblk(nc * 6), TPB, 0, s>>>(nc, cptr, child, P, r, bc);
+    k_restrict<3, double><<<nblk(nc * 64), TPB, 0, s>>>(nc, cptr, child, P, r, bc);
   else
-    k_restrict<6, double><<<nblk(nc * 6), TPB, 0, s>>>(nc, cptr, child, P, r, bc);
+    k_restrict<6, double><<<nblk(nc * 64), TPB, 0, s>>>(nc, cptr, child, P, r, bc);
 }
 void prolong_add(hipStream_t s, int bs, int64_t nu, const int32_t *parent, const double *P, const double *xc,
                  double *x) {
@@ -394,9 +441,9 @@ void prolong_add(hipStream_t s, int bs, int64_t nu, const int32_t *parent, const
 void restrictf(hipStream_t s, int bs, int64_t nc, const int32_t *cptr, const int32_t *child, const float *P,
                const float *r, float *bc) {
   if (bs == 3)
-    k_restrict<3, float><<<nblk(nc * 6), TPB, 0, s>>>(nc, cptr, child, P, r, bc);
+    k_restrict<3, float><<<nblk(nc * 64), TPB, 0, s>>>(nc, cptr, child, P, r, bc);
   else
-    k_restrict<6, float><<<nblk(nc * 6), TPB, 0, s>>>(nc, cptr, child, P, r, bc);
+    k_restrict<6, float><<<nblk(nc * 64), TPB, 0, s>>>(nc, cptr, child, P, r, bc);
 }
 void prolong_addf(hipStream_t s, int bs, int64_t nu, const int32_t *parent, const float *P, const float *xc,
                   float *x) {
@@ -424,6 +471,9 @@ void bsr_to_dense(hipStream_t s, int64_t nb_rows, const int32_t *rp, const int32
   k_bsr_to_dense<<<nblk(nb_rows * 6), TPB, 0, s>>>(nb_rows, rp, ci, val, n, dense);
 }
 void dense_spd_inverse(hipStream_t s, int64_t n, double *A, double *Ainv, float *Ainvf) {
+  k_dense_gj_inv<<<1, 1024, 2 * (size_t)n * sizeof(double), s>>>((int)n, A, Ainv, Ainvf);
+}
+void dense_spd_inverse_chol(hipStream_t s, int64_t n, double *A, double *Ainv, float *Ainvf) {
   // W (n x n) lives right after A in the same allocation
   k_dense_chol_inv<<<1, 1024, 0, s>>>((int)n, A, A + n * n, Ainv, Ainvf);
 }

@@ -19,6 +19,7 @@
 #include <type_traits>
 
 #include "uft_internal.h"
+#include "uft_reduce.h"
 
 namespace {
 
@@ -508,7 +509,7 @@ __global__ __launch_bounds__(TPB) void k_node_gather(int nn, int ne, const int32
                                                      const int32_t *__restrict__ inc, const T *__restrict__ Ye,
                                                      const uint8_t *__restrict__ fixm, const T *__restrict__ ufix,
                                                      T *__restrict__ y, const T *__restrict__ dotw,
-                                                     double *__restrict__ part) {
+                                                     double *__restrict__ part, double *__restrict__ out) {
   const int n = blockIdx.x * TPB + threadIdx.x;
   double ds[1] = {0.0};
   if (n < nn) {
@@ -539,7 +540,7 @@ __global__ __launch_bounds__(TPB) void k_node_gather(int nn, int ne, const int32
       if (dotw) ds[0] += (double)acc[0] * (double)dotw[n];
     }
   }
-  if (part) block_sum_store<1>(ds, part);
+  if (part) uftr::block_partials<1>(ds, part);
 }
 
 // Sensitivities (SolveFem.cpp:208-237): uKu = u_e^T KE u_e from the packed
@@ -669,11 +670,11 @@ __global__ __launch_bounds__(TPB) void k_scale(int64_t n, const double *__restri
 }
 template <typename T>
 __global__ __launch_bounds__(TPB) void k_dot(int64_t n, const T *__restrict__ a, const T *__restrict__ b,
-                                             double *__restrict__ part) {
+                                             double *__restrict__ part, double *__restrict__ out) {
   const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
   double v[1] = {0.0};
   if (i < n) v[0] = (double)a[i] * (double)b[i];
-  block_sum_store<1>(v, part);
+  uftr::block_partials<1>(v, part);
 }
 __global__ __launch_bounds__(TPB) void k_dot2(int64_t n, const double *__restrict__ a, const double *__restrict__ b,
                                               const double *__restrict__ c, double *__restrict__ part) {
@@ -691,7 +692,8 @@ __global__ void k_scalar_div(double *sc, int num, int den, int dst) { sc[dst] = 
 template <typename T>
 __global__ __launch_bounds__(TPB) void k_cg_update(int64_t n, const double *__restrict__ sc, int ia,
                                                    T *__restrict__ u, const T *__restrict__ p, T *__restrict__ r,
-                                                   const T *__restrict__ q, double *__restrict__ part) {
+                                                   const T *__restrict__ q, double *__restrict__ part,
+                                                   double *__restrict__ out) {
   const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
   const double al = sc[ia];
   double v[1] = {0.0};
@@ -701,7 +703,7 @@ __global__ __launch_bounds__(TPB) void k_cg_update(int64_t n, const double *__re
     r[i] = rr;
     v[0] = (double)rr * (double)rr;
   }
-  block_sum_store<1>(v, part);
+  uftr::block_partials<1>(v, part);
 }
 template <typename T>
 __global__ __launch_bounds__(TPB) void k_xpby(int64_t n, const T *__restrict__ z, const double *__restrict__ sc, int ib,
@@ -910,7 +912,8 @@ void gen_kf(hipStream_t s, int npe, int64_t ne, const int32_t *elem, const doubl
 static SimpCoef simp(const uft_ctx *C) { return SimpCoef{C->P.Emin, C->P.Emax - C->P.Emin, C->P.penal}; }
 
 void apply_K(hipStream_t s, const uft_ctx *C, const double *xphys, const double *u, double *y, const double *dotw,
-             double *part, hipEvent_t *ev) {
+             double *dot_out, hipEvent_t *ev) {
+  double *part = dotw ? C->d_part : nullptr;
   const int ne = (int)C->ne, nn = (int)C->nn;
   if (ev) (void)hipEventRecord(ev[0], s);
   if (C->npe == 8)
@@ -922,11 +925,12 @@ void apply_K(hipStream_t s, const uft_ctx *C, const double *xphys, const double 
   if (ev) (void)hipEventRecord(ev[1], s);
   if (C->npe == 8)
     k_node_gather<8, 3, 1, double><<<nblk(nn), TPB, 0, s>>>(nn, ne, C->d_iptr, C->d_inc, C->d_Ye, C->d_fixm, u, y,
-                                                            dotw, part);
+                                                            dotw, part, dot_out);
   else
     k_node_gather<4, 3, 1, double><<<nblk(nn), TPB, 0, s>>>(nn, ne, C->d_iptr, C->d_inc, C->d_Ye, C->d_fixm, u, y,
-                                                            dotw, part);
+                                                            dotw, part, dot_out);
   if (ev) (void)hipEventRecord(ev[2], s);
+  if (dotw) final_sum(s, part, (int)nblk(nn), 1, dot_out);
 }
 
 void apply_Kf(hipStream_t s, const uft_ctx *C, const double *xphys, const float *u, float *y, hipEvent_t *ev) {
@@ -941,10 +945,10 @@ void apply_Kf(hipStream_t s, const uft_ctx *C, const double *xphys, const float 
   if (ev) (void)hipEventRecord(ev[1], s);
   if (C->npe == 8)
     k_node_gather<8, 3, 1, float><<<nblk(nn), TPB, 0, s>>>(nn, ne, C->d_iptr, C->d_inc, C->d_Yef, C->d_fixm, u, y,
-                                                           nullptr, nullptr);
+                                                           nullptr, nullptr, nullptr);
   else
     k_node_gather<4, 3, 1, float><<<nblk(nn), TPB, 0, s>>>(nn, ne, C->d_iptr, C->d_inc, C->d_Yef, C->d_fixm, u, y,
-                                                           nullptr, nullptr);
+                                                           nullptr, nullptr, nullptr);
   if (ev) (void)hipEventRecord(ev[2], s);
 }
 
@@ -962,11 +966,11 @@ void diag_K(hipStream_t s, const uft_ctx *C, const double *xphys, double *dinv, 
   if (C->npe == 8) {
     k_elem_diag<8><<<nblk(ne), TPB, 0, s>>>(ne, C->d_KE, xphys, simp(C), C->d_Ye);
     k_node_gather<8, 3, 2, double><<<nblk(nn), TPB, 0, s>>>(nn, ne, C->d_iptr, C->d_inc, C->d_Ye, C->d_fixm,
-                                                            nullptr, tmp, nullptr, nullptr);
+                                                            nullptr, tmp, nullptr, nullptr, nullptr);
   } else {
     k_elem_diag<4><<<nblk(ne), TPB, 0, s>>>(ne, C->d_KE, xphys, simp(C), C->d_Ye);
     k_node_gather<4, 3, 2, double><<<nblk(nn), TPB, 0, s>>>(nn, ne, C->d_iptr, C->d_inc, C->d_Ye, C->d_fixm,
-                                                            nullptr, tmp, nullptr, nullptr);
+                                                            nullptr, tmp, nullptr, nullptr, nullptr);
   }
   k_inv<<<nblk(C->ndof), TPB, 0, s>>>(C->ndof, tmp, dinv, dinvf);
 }
@@ -987,11 +991,11 @@ void apply_KF(hipStream_t s, const uft_ctx *C, const double *x, double *y) {
   if (C->npe == 8) {
     k_helm_elem<8><<<nblk(ne), TPB, 0, s>>>(ne, C->d_elem, (const double2 *)C->d_KF, x, C->d_Ye);
     k_node_gather<8, 1, 0, double><<<nblk(nn), TPB, 0, s>>>(nn, ne, C->d_iptr, C->d_inc, C->d_Ye, nullptr, nullptr, y,
-                                                            nullptr, nullptr);
+                                                            nullptr, nullptr, nullptr);
   } else {
     k_helm_elem<4><<<nblk(ne), TPB, 0, s>>>(ne, C->d_elem, (const double2 *)C->d_KF, x, C->d_Ye);
     k_node_gather<4, 1, 0, double><<<nblk(nn), TPB, 0, s>>>(nn, ne, C->d_iptr, C->d_inc, C->d_Ye, nullptr, nullptr, y,
-                                                            nullptr, nullptr);
+                                                            nullptr, nullptr, nullptr);
   }
 }
 
@@ -1000,11 +1004,11 @@ void diag_KF(hipStream_t s, const uft_ctx *C, double *dinv) {
   if (C->npe == 8) {
     k_helm_diag_elem<8><<<nblk(ne), TPB, 0, s>>>(ne, C->d_KF, C->d_Ye);
     k_node_gather<8, 1, 0, double><<<nblk(nn), TPB, 0, s>>>(nn, ne, C->d_iptr, C->d_inc, C->d_Ye, nullptr, nullptr,
-                                                            C->d_fq, nullptr, nullptr);
+                                                            C->d_fq, nullptr, nullptr, nullptr);
   } else {
     k_helm_diag_elem<4><<<nblk(ne), TPB, 0, s>>>(ne, C->d_KF, C->d_Ye);
     k_node_gather<4, 1, 0, double><<<nblk(nn), TPB, 0, s>>>(nn, ne, C->d_iptr, C->d_inc, C->d_Ye, nullptr, nullptr,
-                                                            C->d_fq, nullptr, nullptr);
+                                                            C->d_fq, nullptr, nullptr, nullptr);
   }
   k_inv<<<nblk(nn), TPB, 0, s>>>(nn, C->d_fq, dinv, nullptr);
 }
@@ -1014,11 +1018,11 @@ void tf_mult(hipStream_t s, const uft_ctx *C, const double *xe, double *TX) {
   if (C->npe == 8) {
     k_tf_elem<8><<<nblk(ne), TPB, 0, s>>>(ne, C->d_T, xe, C->d_Ye);
     k_node_gather<8, 1, 0, double><<<nblk(nn), TPB, 0, s>>>(nn, ne, C->d_iptr, C->d_inc, C->d_Ye, nullptr, nullptr, TX,
-                                                            nullptr, nullptr);
+                                                            nullptr, nullptr, nullptr);
   } else {
     k_tf_elem<4><<<nblk(ne), TPB, 0, s>>>(ne, C->d_T, xe, C->d_Ye);
     k_node_gather<4, 1, 0, double><<<nblk(nn), TPB, 0, s>>>(nn, ne, C->d_iptr, C->d_inc, C->d_Ye, nullptr, nullptr, TX,
-                                                            nullptr, nullptr);
+                                                            nullptr, nullptr, nullptr);
   }
 }
 
@@ -1035,11 +1039,11 @@ void final_sum(hipStream_t s, const double *part, int nb, int K, double *out) {
 }
 void final_max(hipStream_t s, const double *part, int nb, double *out) { k_final_max<<<1, TPB, 0, s>>>(part, nb, out); }
 void dot(hipStream_t s, int64_t n, const double *a, const double *b, double *part, double *out) {
-  k_dot<double><<<nblk(n), TPB, 0, s>>>(n, a, b, part);
+  k_dot<double><<<nblk(n), TPB, 0, s>>>(n, a, b, part, out);
   final_sum(s, part, (int)nblk(n), 1, out);
 }
 void dotf(hipStream_t s, int64_t n, const float *a, const float *b, double *part, double *out) {
-  k_dot<float><<<nblk(n), TPB, 0, s>>>(n, a, b, part);
+  k_dot<float><<<nblk(n), TPB, 0, s>>>(n, a, b, part, out);
   final_sum(s, part, (int)nblk(n), 1, out);
 }
 void dot2(hipStream_t s, int64_t n, const double *a, const double *b, const double *c, double *part, double *out) {
@@ -1062,12 +1066,14 @@ void scale(hipStream_t s, int64_t n, const double *a, const double *dinv, double
   k_scale<<<nblk(n), TPB, 0, s>>>(n, a, dinv, z);
 }
 void cg_update(hipStream_t s, int64_t n, const double *sc, int ia, double *u, const double *p, double *r,
-               const double *q, double *part) {
-  k_cg_update<double><<<nblk(n), TPB, 0, s>>>(n, sc, ia, u, p, r, q, part);
+               const double *q, double *part, double *out) {
+  k_cg_update<double><<<nblk(n), TPB, 0, s>>>(n, sc, ia, u, p, r, q, part, out);
+  final_sum(s, part, (int)nblk(n), 1, out);
 }
 void cg_updatef(hipStream_t s, int64_t n, const double *sc, int ia, float *u, const float *p, float *r, const float *q,
-                double *part) {
-  k_cg_update<float><<<nblk(n), TPB, 0, s>>>(n, sc, ia, u, p, r, q, part);
+                double *part, double *out) {
+  k_cg_update<float><<<nblk(n), TPB, 0, s>>>(n, sc, ia, u, p, r, q, part, out);
+  final_sum(s, part, (int)nblk(n), 1, out);
 }
 void xpby(hipStream_t s, int64_t n, const double *z, const double *sc, int ib, double *p) {
   k_xpby<double><<<nblk(n), TPB, 0, s>>>(n, z, sc, ib, p);

@@ -3,6 +3,7 @@
 // and the device-side scalar recurrences of CG, so a Krylov step needs no
 // host round trip except its convergence test.
 #include "uft_internal.h"
+#include "uft_reduce.h"
 
 namespace {
 constexpr int TPB = 256;
@@ -31,7 +32,8 @@ __device__ __forceinline__ void block_sum_store(double (&v)[K], double *part) {
 // r = b - q ; z = D^-1 r ; partials ||D^-1 b||^2, ||z||^2, z.r
 __global__ __launch_bounds__(TPB) void k_fcg_init(int64_t n, const double *__restrict__ b, const double *__restrict__ q,
                                                   const double *__restrict__ dinv, double *__restrict__ r,
-                                                  double *__restrict__ z, double *__restrict__ part) {
+                                                  double *__restrict__ z, double *__restrict__ part,
+                                                  double *__restrict__ out) {
   const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
   double v[3] = {0.0, 0.0, 0.0};
   if (i < n) {
@@ -43,7 +45,7 @@ __global__ __launch_bounds__(TPB) void k_fcg_init(int64_t n, const double *__res
     v[1] = zi * zi;
     v[2] = zi * ri;
   }
-  block_sum_store<3>(v, part);
+  uftr::block_partials<3>(v, part);
 }
 
 // x += alpha p ; r -= alpha q ; z = D^-1 r ; partials ||z||^2, z.r
@@ -51,7 +53,7 @@ __global__ __launch_bounds__(TPB) void k_fcg_update(int64_t n, const double *__r
                                                     double *__restrict__ x, const double *__restrict__ p,
                                                     double *__restrict__ r, const double *__restrict__ q,
                                                     const double *__restrict__ dinv, double *__restrict__ z,
-                                                    double *__restrict__ part) {
+                                                    double *__restrict__ part, double *__restrict__ out) {
   const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
   const double al = sc[ia];
   double v[2] = {0.0, 0.0};
@@ -64,7 +66,7 @@ __global__ __launch_bounds__(TPB) void k_fcg_update(int64_t n, const double *__r
     v[0] = zi * zi;
     v[1] = zi * ri;
   }
-  block_sum_store<2>(v, part);
+  uftr::block_partials<2>(v, part);
 }
 
 // p = z + (sc[ib] / sc[ibo]) p, or p = z when first
@@ -97,12 +99,14 @@ __global__ void k_copy_scalar(const double *src, double *dst) { *dst = *src; }
 
 namespace uftk {
 void fcg_init(hipStream_t s, int64_t n, const double *b, const double *q, const double *dinv, double *r, double *z,
-              double *part) {
-  k_fcg_init<<<nblk(n), TPB, 0, s>>>(n, b, q, dinv, r, z, part);
+              double *part, double *out) {
+  k_fcg_init<<<nblk(n), TPB, 0, s>>>(n, b, q, dinv, r, z, part, out);
+  final_sum(s, part, (int)nblk(n), 3, out);
 }
 void fcg_update(hipStream_t s, int64_t n, const double *sc, int ia, double *x, const double *p, double *r,
-                const double *q, const double *dinv, double *z, double *part) {
-  k_fcg_update<<<nblk(n), TPB, 0, s>>>(n, sc, ia, x, p, r, q, dinv, z, part);
+                const double *q, const double *dinv, double *z, double *part, double *out) {
+  k_fcg_update<<<nblk(n), TPB, 0, s>>>(n, sc, ia, x, p, r, q, dinv, z, part, out);
+  final_sum(s, part, (int)nblk(n), 2, out);
 }
 void fcg_p(hipStream_t s, int64_t n, const double *z, const double *sc, int ib, int ibo, int first, double *p) {
   k_fcg_p<<<nblk(n), TPB, 0, s>>>(n, z, sc, ib, ibo, first, p);
@@ -121,14 +125,15 @@ __global__ __launch_bounds__(TPB) void k_f2d(int64_t n, const float *__restrict_
 }
 // partials of r.z and r.zo (flexible CG, Polak-Ribiere form)
 __global__ __launch_bounds__(TPB) void k_dot_pr(int64_t n, const double *__restrict__ r, const double *__restrict__ z,
-                                                const double *__restrict__ zo, double *__restrict__ part) {
+                                                const double *__restrict__ zo, double *__restrict__ part,
+                                                double *__restrict__ out) {
   const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
   double v[2] = {0.0, 0.0};
   if (i < n) {
     v[0] = r[i] * z[i];
     v[1] = r[i] * zo[i];
   }
-  block_sum_store<2>(v, part);
+  uftr::block_partials<2>(v, part);
 }
 // beta = (rz - rzo) / rz_prev ; rz_prev = rz
 __global__ void k_beta_flex(double *sc, int irz, int irzo, int iprev, int ibeta) {
@@ -139,10 +144,62 @@ __global__ void k_beta_flex(double *sc, int irz, int irzo, int iprev, int ibeta)
 
 namespace uftk {
 void f2d(hipStream_t s, int64_t n, const float *a, double *b) { k_f2d<<<nblk(n), TPB, 0, s>>>(n, a, b); }
-void dot_pr(hipStream_t s, int64_t n, const double *r, const double *z, const double *zo, double *part) {
-  k_dot_pr<<<nblk(n), TPB, 0, s>>>(n, r, z, zo, part);
+void dot_pr(hipStream_t s, int64_t n, const double *r, const double *z, const double *zo, double *part, double *out) {
+  k_dot_pr<<<nblk(n), TPB, 0, s>>>(n, r, z, zo, part, out);
+  final_sum(s, part, (int)nblk(n), 2, out);
 }
 void sc_beta_flex(hipStream_t s, double *sc, int irz, int irzo, int iprev, int ibeta) {
   k_beta_flex<<<1, 1, 0, s>>>(sc, irz, irzo, iprev, ibeta);
 }
+}  // namespace uftk
+
+// ---- K-cycle helpers: linear combinations with device-resident coefficients
+namespace {
+template <typename T>
+__global__ __launch_bounds__(TPB) void k_lincomb(int64_t n, const double *__restrict__ sc, int ia,
+                                                 const T *__restrict__ a, int ib, const T *__restrict__ b,
+                                                 T *__restrict__ y) {
+  const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  if (i >= n) return;
+  const T ca = ia < 0 ? T(1) : (T)sc[ia];
+  const T cb = ib < 0 ? T(1) : (T)sc[ib];
+  y[i] = ca * a[i] + cb * b[i];
+}
+template <typename T>
+__global__ __launch_bounds__(TPB) void k_addto(int64_t n, const T *__restrict__ d, T *__restrict__ x) {
+  const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  if (i < n) x[i] += d[i];
+}
+// sc: 0 rho1 = c.v, 1 alpha1 = c.b  ->  7 = -alpha1/rho1
+__global__ void k_kc1(double *sc) { sc[7] = sc[0] != 0.0 ? -sc[1] / sc[0] : 0.0; }
+// 2 gamma = d.v, 3 beta = d.w, 4 alpha2 = d.r  ->  5, 6 coefficients of c and d
+__global__ void k_kc2(double *sc) {
+  const double rho1 = sc[0], a1 = sc[1], g = sc[2], be = sc[3], a2 = sc[4];
+  const double t1 = rho1 != 0.0 ? a1 / rho1 : 0.0;
+  const double rho2 = rho1 != 0.0 ? be - g * g / rho1 : 0.0;
+  if (rho2 > 0.0) {
+    sc[5] = t1 - g * a2 / (rho1 * rho2);
+    sc[6] = a2 / rho2;
+  } else {
+    sc[5] = t1;
+    sc[6] = 0.0;
+  }
+}
+}  // namespace
+
+namespace uftk {
+void lincomb(hipStream_t s, int64_t n, const double *sc, int ia, const double *a, int ib, const double *b, double *y) {
+  k_lincomb<double><<<nblk(n), TPB, 0, s>>>(n, sc, ia, a, ib, b, y);
+}
+void lincombf(hipStream_t s, int64_t n, const double *sc, int ia, const float *a, int ib, const float *b, float *y) {
+  k_lincomb<float><<<nblk(n), TPB, 0, s>>>(n, sc, ia, a, ib, b, y);
+}
+void axpy_inplace_t(hipStream_t s, int64_t n, const double *d, double *x) {
+  k_addto<double><<<nblk(n), TPB, 0, s>>>(n, d, x);
+}
+void axpy_inplace_t(hipStream_t s, int64_t n, const float *d, float *x) {
+  k_addto<float><<<nblk(n), TPB, 0, s>>>(n, d, x);
+}
+void kcycle_coef1(hipStream_t s, double *sc) { k_kc1<<<1, 1, 0, s>>>(sc); }
+void kcycle_coef2(hipStream_t s, double *sc) { k_kc2<<<1, 1, 0, s>>>(sc); }
 }  // namespace uftk

@@ -1,10 +1,12 @@
 // C-ABI implementation: context lifetime, mesh/BC ingest, setup, and the
 // host orchestration of the device design iteration (main.cpp:64-119).
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <type_traits>
 
 #include "uft_internal.h"
 
@@ -28,7 +30,8 @@ enum {
   S_RZ = 0, S_PQ = 1, S_ALPHA = 2, S_RR = 3, S_RZNEW = 4, S_BETA = 5,
   S_F0 = 8,  // filter: 8 ||Db||^2, 9 ||z||^2, 10 z.r (beta), 11 beta_old, 12 pq, 13 alpha
   S_MISC = 16,
-  S_RED = 32  // general reduction outputs (up to 48 values)
+  S_RED = 32,  // general reduction outputs (up to 48 values)
+  S_K = 96     // K-cycle scalars, 8 per level
 };
 
 int dalloc(uft_ctx *C, void **p, size_t bytes) {
@@ -57,6 +60,10 @@ void free_all(uft_ctx *C) {
   if (C->s) (void)hipStreamSynchronize(C->s);
   for (void *p : C->allocs) (void)hipFree(p);
   C->allocs.clear();
+}
+
+double now_ms() {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
 int sync(uft_ctx *C) {
@@ -136,76 +143,140 @@ ChebCoef cheb_coef(double lmax, int deg) {
   return cc;
 }
 
-// V-cycle: x = M_l b (x overwritten)
-void vcycle(uft_ctx *C, int l, const double *b, double *x) {
+// ---------------------------------------------------------------- cycles
+// Overloads so one cycle template serves FP64 and FP32 vectors.
+void k_cheb_first(hipStream_t s, int64_t n, const double *b, const double *di, double it, double *r, double *d,
+                  double *x, int z) { uftk::cheb_first(s, n, b, di, it, r, d, x, z); }
+void k_cheb_first(hipStream_t s, int64_t n, const float *b, const float *di, double it, float *r, float *d, float *x,
+                  int z) { uftk::cheb_firstf(s, n, b, di, it, r, d, x, z); }
+void k_cheb_step(hipStream_t s, int64_t n, const double *t, const double *di, double c1, double c2, double *r,
+                 double *d, double *x) { uftk::cheb_step(s, n, t, di, c1, c2, r, d, x); }
+void k_cheb_step(hipStream_t s, int64_t n, const float *t, const float *di, double c1, double c2, float *r, float *d,
+                 float *x) { uftk::cheb_stepf(s, n, t, di, c1, c2, r, d, x); }
+void k_sub_inplace(hipStream_t s, int64_t n, const double *t, double *r) { uftk::sub_inplace(s, n, t, r); }
+void k_sub_inplace(hipStream_t s, int64_t n, const float *t, float *r) { uftk::sub_inplacef(s, n, t, r); }
+void k_sub(hipStream_t s, int64_t n, const double *a, const double *b, double *c) { uftk::sub(s, n, a, b, c); }
+void k_sub(hipStream_t s, int64_t n, const float *a, const float *b, float *c) { uftk::subf(s, n, a, b, c); }
+void k_restrict(hipStream_t s, const AmgLevel &Lv, int64_t nc, const double *r, double *bc) {
+  uftk::restrict_(s, Lv.bs, nc, Lv.d_cptr, Lv.d_child, Lv.d_P, r, bc);
+}
+void k_restrict(hipStream_t s, const AmgLevel &Lv, int64_t nc, const float *r, float *bc) {
+  uftk::restrictf(s, Lv.bs, nc, Lv.d_cptr, Lv.d_child, Lv.d_Pf, r, bc);
+}
+void k_prolong(hipStream_t s, const AmgLevel &Lv, const double *xc, double *x) {
+  uftk::prolong_add(s, Lv.bs, Lv.nunit, Lv.d_parent, Lv.d_P, xc, x);
+}
+void k_prolong(hipStream_t s, const AmgLevel &Lv, const float *xc, float *x) {
+  uftk::prolong_addf(s, Lv.bs, Lv.nunit, Lv.d_parent, Lv.d_Pf, xc, x);
+}
+void k_apply(uft_ctx *C, int l, const double *x, double *y) { level_apply(C, l, x, y); }
+void k_apply(uft_ctx *C, int l, const float *x, float *y) { level_applyf(C, l, x, y); }
+void k_dot(uft_ctx *C, int64_t n, const double *a, const double *b, double *out) {
+  uftk::dot(C->s, n, a, b, C->d_part, out);
+}
+void k_dot(uft_ctx *C, int64_t n, const float *a, const float *b, double *out) {
+  uftk::dotf(C->s, n, a, b, C->d_part, out);
+}
+void k_lincomb(hipStream_t s, int64_t n, const double *sc, int ia, const double *a, int ib, const double *b,
+               double *y) { uftk::lincomb(s, n, sc, ia, a, ib, b, y); }
+void k_lincomb(hipStream_t s, int64_t n, const double *sc, int ia, const float *a, int ib, const float *b, float *y) {
+  uftk::lincombf(s, n, sc, ia, a, ib, b, y);
+}
+
+template <typename T>
+struct LV {
+  T *x, *b, *r, *d, *t, *kc, *kv, *kd, *kr;
+  const T *dinv;
+};
+LV<double> lv(AmgLevel &L, double *) { return {L.x, L.b, L.r, L.d, L.t, L.kc, L.kv, L.kd, L.kr, L.d_dinv}; }
+LV<float> lv(AmgLevel &L, float *) { return {L.xf, L.bf, L.rf, L.df, L.tf, L.kcf, L.kvf, L.kdf, L.krf, L.d_dinvf}; }
+
+template <typename T>
+void coarse_solve(uft_ctx *C, int l, const T *b, T *x);
+
+// One multigrid cycle at level l: x = B_l b.  Chebyshev-Jacobi pre- and
+// post-smoothing with the same polynomial (so the cycle is symmetric), and a
+// coarse-grid correction by coarse_solve.
+template <typename T>
+void cycle(uft_ctx *C, int l, const T *b, T *x) {
   AmgHier &H = C->amg;
   hipStream_t s = C->s;
-  if (l == H.nlev - 1) {
-    uftk::dense_mv(s, H.ncoarse, H.d_cinv, b, x);
-    return;
-  }
   AmgLevel &Lv = H.L[l];
   AmgLevel &Ln = H.L[l + 1];
+  LV<T> v = lv(Lv, (T *)nullptr);
+  LV<T> vn = lv(Ln, (T *)nullptr);
   const int64_t n = Lv.n;
   const int deg = std::max(1, C->P.cheb_degree);
   const ChebCoef cc = cheb_coef(Lv.lmax, deg);
-  // pre-smoothing from x = 0
-  uftk::cheb_first(s, n, b, Lv.d_dinv, cc.inv_theta, Lv.r, Lv.d, x, 1);
+  k_cheb_first(s, n, b, v.dinv, cc.inv_theta, v.r, v.d, x, 1);
   for (int k = 1; k < deg; ++k) {
-    level_apply(C, l, Lv.d, Lv.t);
-    uftk::cheb_step(s, n, Lv.t, Lv.d_dinv, cc.c1[k], cc.c2[k], Lv.r, Lv.d, x);
+    k_apply(C, l, v.d, v.t);
+    k_cheb_step(s, n, v.t, v.dinv, cc.c1[k], cc.c2[k], v.r, v.d, x);
   }
-  level_apply(C, l, Lv.d, Lv.t);
-  uftk::sub_inplace(s, n, Lv.t, Lv.r);  // r = b - A x
-  // coarse-grid correction
-  uftk::restrict_(s, Lv.bs, Ln.nunit, Lv.d_cptr, Lv.d_child, Lv.d_P, Lv.r, Ln.b);
-  vcycle(C, l + 1, Ln.b, Ln.x);
-  uftk::prolong_add(s, Lv.bs, Lv.nunit, Lv.d_parent, Lv.d_P, Ln.x, x);
-  // post-smoothing
-  level_apply(C, l, x, Lv.t);
-  uftk::sub(s, n, b, Lv.t, Lv.r);
-  uftk::cheb_first(s, n, b, Lv.d_dinv, cc.inv_theta, Lv.r, Lv.d, x, 0);
+  k_apply(C, l, v.d, v.t);
+  k_sub_inplace(s, n, v.t, v.r);  // r = b - A x
+  k_restrict(s, Lv, Ln.nunit, v.r, vn.b);
+  coarse_solve(C, l + 1, vn.b, vn.x);
+  k_prolong(s, Lv, vn.x, x);
+  k_apply(C, l, x, v.t);
+  k_sub(s, n, b, v.t, v.r);
+  k_cheb_first(s, n, b, v.dinv, cc.inv_theta, v.r, v.d, x, 0);
   for (int k = 1; k < deg; ++k) {
-    level_apply(C, l, Lv.d, Lv.t);
-    uftk::cheb_step(s, n, Lv.t, Lv.d_dinv, cc.c1[k], cc.c2[k], Lv.r, Lv.d, x);
+    k_apply(C, l, v.d, v.t);
+    k_cheb_step(s, n, v.t, v.dinv, cc.c1[k], cc.c2[k], v.r, v.d, x);
   }
 }
 
-// The same V-cycle on FP32 vectors with FP32 operators (mixed form); the
-// coarsest solve stays FP64.
-void vcycle_f(uft_ctx *C, int l, const float *b, float *x) {
+// Approximate solve of A_l x = b on a coarse level (l >= 1).  The coarsest
+// level is solved exactly in FP64.  Otherwise: one cycle (V), two stationary
+// cycles (W), or two flexible-CG steps preconditioned by the cycle (K-cycle,
+// Notay & Vassilevski 2008), with all scalars kept on the device.
+template <typename T>
+void coarse_solve(uft_ctx *C, int l, const T *b, T *x) {
   AmgHier &H = C->amg;
   hipStream_t s = C->s;
+  AmgLevel &Lv = H.L[l];
   if (l == H.nlev - 1) {
-    AmgLevel &Lc = H.L[l];
-    uftk::f2d(s, Lc.n, b, Lc.b);
-    uftk::dense_mv(s, H.ncoarse, H.d_cinv, Lc.b, Lc.x);
-    uftk::d2f(s, Lc.n, Lc.x, x);
+    if constexpr (std::is_same<T, double>::value) {
+      uftk::dense_mv(s, H.ncoarse, H.d_cinv, b, x);
+    } else {
+      uftk::f2d(s, Lv.n, b, Lv.kc);
+      uftk::dense_mv(s, H.ncoarse, H.d_cinv, Lv.kc, Lv.kv);
+      uftk::d2f(s, Lv.n, Lv.kv, x);
+    }
     return;
   }
-  AmgLevel &Lv = H.L[l];
-  AmgLevel &Ln = H.L[l + 1];
+  LV<T> v = lv(Lv, (T *)nullptr);
   const int64_t n = Lv.n;
-  const int deg = std::max(1, C->P.cheb_degree);
-  const ChebCoef cc = cheb_coef(Lv.lmax, deg);
-  uftk::cheb_firstf(s, n, b, Lv.d_dinvf, cc.inv_theta, Lv.rf, Lv.df, x, 1);
-  for (int k = 1; k < deg; ++k) {
-    level_applyf(C, l, Lv.df, Lv.tf);
-    uftk::cheb_stepf(s, n, Lv.tf, Lv.d_dinvf, cc.c1[k], cc.c2[k], Lv.rf, Lv.df, x);
-  }
-  level_applyf(C, l, Lv.df, Lv.tf);
-  uftk::sub_inplacef(s, n, Lv.tf, Lv.rf);
-  uftk::restrictf(s, Lv.bs, Ln.nunit, Lv.d_cptr, Lv.d_child, Lv.d_Pf, Lv.rf, Ln.bf);
-  vcycle_f(C, l + 1, Ln.bf, Ln.xf);
-  uftk::prolong_addf(s, Lv.bs, Lv.nunit, Lv.d_parent, Lv.d_Pf, Ln.xf, x);
-  level_applyf(C, l, x, Lv.tf);
-  uftk::subf(s, n, b, Lv.tf, Lv.rf);  // r = b - A x
-  uftk::cheb_firstf(s, n, b, Lv.d_dinvf, cc.inv_theta, Lv.rf, Lv.df, x, 0);
-  for (int k = 1; k < deg; ++k) {
-    level_applyf(C, l, Lv.df, Lv.tf);
-    uftk::cheb_stepf(s, n, Lv.tf, Lv.d_dinvf, cc.c1[k], cc.c2[k], Lv.rf, Lv.df, x);
+  const int ct = C->P.cycle_type;
+  if (ct == 0) {
+    cycle(C, l, b, x);
+  } else if (ct == 1) {
+    cycle(C, l, b, x);
+    k_apply(C, l, x, v.kv);
+    k_sub(s, n, b, v.kv, v.kr);
+    cycle(C, l, v.kr, v.kd);
+    uftk::axpy_inplace_t(s, n, v.kd, x);
+  } else {
+    double *sc = C->d_scal + S_K + 8 * l;  // 0 rho1 1 alpha1 2 gamma 3 beta 4 alpha2 5 c1 6 c2 7 -a1/rho1
+    cycle(C, l, b, v.kc);
+    k_apply(C, l, v.kc, v.kv);
+    k_dot(C, n, v.kc, v.kv, sc + 0);
+    k_dot(C, n, v.kc, b, sc + 1);
+    uftk::kcycle_coef1(s, sc);  // sc[7] = -alpha1/rho1
+    k_lincomb(s, n, sc, -1, b, 7, v.kv, v.kr);  // r = b - (alpha1/rho1) v
+    cycle(C, l, v.kr, v.kd);
+    k_apply(C, l, v.kd, v.t);
+    k_dot(C, n, v.kd, v.kv, sc + 2);
+    k_dot(C, n, v.kd, v.t, sc + 3);
+    k_dot(C, n, v.kd, v.kr, sc + 4);
+    uftk::kcycle_coef2(s, sc);  // sc[5], sc[6]
+    k_lincomb(s, n, sc, 5, v.kc, 6, v.kd, x);
   }
 }
+
+void vcycle(uft_ctx *C, int l, const double *b, double *x) { cycle<double>(C, l, b, x); }
+void vcycle_f(uft_ctx *C, int l, const float *b, float *x) { cycle<float>(C, l, b, x); }
 
 // largest eigenvalue of a symmetric tridiagonal matrix (bisection, Sturm counts)
 double tridiag_lmax(const std::vector<double> &d, const std::vector<double> &e) {
@@ -252,7 +323,7 @@ int estimate_lmax(uft_ctx *C, int l, int ksteps) {
     uftk::dot(s, n, p, q, C->d_part, C->d_scal + S_MISC + 1);
     uftk::sc_div(s, C->d_scal, S_MISC, S_MISC + 1, S_MISC + 2);  // alpha
     uftk::sc_copy(s, C->d_scal + S_MISC + 2, log + 3 * j);       // log alpha
-    uftk::cg_update(s, n, C->d_scal, S_MISC + 2, x, p, r, q, C->d_part);
+    uftk::cg_update(s, n, C->d_scal, S_MISC + 2, x, p, r, q, C->d_part, C->d_scal + S_MISC + 5);
     uftk::scale(s, n, r, Lv.d_dinv, z);
     uftk::dot(s, n, r, z, C->d_part, C->d_scal + S_MISC + 3);
     uftk::sc_beta(s, C->d_scal, S_MISC + 3, S_MISC, S_MISC + 4, nullptr);
@@ -314,9 +385,11 @@ int amg_refresh(uft_ctx *C, const double *xphys) {
 int solve_el(uft_ctx *C, const double *xphys, double *U) {
   hipStream_t s = C->s;
   const int64_t n = C->ndof;
-  double *r = C->d_r, *z = C->d_z, *p = C->d_p, *q = C->d_q;
+  double *r = C->d_r, *z = C->d_z, *p = C->d_p, *q = C->d_q, *zo = C->d_w;
+  const double t0 = now_ms();
   int rc = amg_refresh(C, xphys);
   if (rc) return rc;
+  C->t_phase[0] = now_ms() - t0;
   uftk::fill(s, n, 0.0, U);
   uftk::copy(s, n, C->d_F, r);
   uftk::dot(s, n, r, r, C->d_part, C->d_scal + S_RR);
@@ -337,11 +410,9 @@ int solve_el(uft_ctx *C, const double *xphys, double *U) {
   int it = 0;
   bool conv = false;
   while (it < C->P.maxit_el) {
-    fine_apply(C, xphys, p, q, p, C->d_part);
-    uftk::final_sum(s, C->d_part, uftk::red_blocks(C->nn), 1, C->d_scal + S_PQ);
+    fine_apply(C, xphys, p, q, p, C->d_scal + S_PQ);
     uftk::sc_div(s, C->d_scal, S_RZ, S_PQ, S_ALPHA);
-    uftk::cg_update(s, n, C->d_scal, S_ALPHA, U, p, r, q, C->d_part);
-    uftk::final_sum(s, C->d_part, uftk::red_blocks(n), 1, C->d_scal + S_RR);
+    uftk::cg_update(s, n, C->d_scal, S_ALPHA, U, p, r, q, C->d_part, C->d_scal + S_RR);
     ++it;
     rc = fetch(C, C->d_scal + S_RR, 1, C->h_scal);
     if (rc) return rc;
@@ -351,10 +422,13 @@ int solve_el(uft_ctx *C, const double *xphys, double *U) {
       conv = true;
       break;
     }
+    // flexible CG (Polak-Ribiere beta): the K-cycle preconditioner contains
+    // inner products, so it is not a fixed linear operator
+    std::swap(z, zo);
     vcycle(C, 0, r, z);
-    uftk::dot(s, n, r, z, C->d_part, C->d_scal + S_RZNEW);
-    uftk::sc_beta(s, C->d_scal, S_RZNEW, S_RZ, S_BETA, nullptr);
-    uftk::xpby(s, n, z, C->d_scal, S_BETA, p);
+    uftk::dot_pr(s, n, r, z, zo, C->d_part, C->d_scal + S_RZNEW);
+    uftk::sc_beta_flex(s, C->d_scal, S_RZNEW, S_RZNEW + 1, S_RZ, S_BETA + 2);
+    uftk::xpby(s, n, z, C->d_scal, S_BETA + 2, p);
   }
   C->it_el = it;
   // true residual (SolveFem.cpp:142-152)
@@ -379,8 +453,10 @@ int solve_el_mixed(uft_ctx *C, const double *xphys, double *U) {
   const int64_t n = C->ndof;
   double *r = C->d_r, *z = C->d_z, *p = C->d_p, *q = C->d_q, *zo = C->d_w;
   float *rf = C->d_rf, *zf = C->d_zf;
+  const double t0 = now_ms();
   int rc = amg_refresh(C, xphys);
   if (rc) return rc;
+  C->t_phase[0] = now_ms() - t0;
   uftk::fill(s, n, 0.0, U);
   uftk::copy(s, n, C->d_F, r);
   uftk::dot(s, n, r, r, C->d_part, C->d_scal + S_RR);
@@ -405,11 +481,9 @@ int solve_el_mixed(uft_ctx *C, const double *xphys, double *U) {
   int it = 0;
   bool conv = false;
   while (it < C->P.maxit_el) {
-    fine_apply(C, xphys, p, q, p, C->d_part);
-    uftk::final_sum(s, C->d_part, uftk::red_blocks(C->nn), 1, C->d_scal + S_PQ);
+    fine_apply(C, xphys, p, q, p, C->d_scal + S_PQ);
     uftk::sc_div(s, C->d_scal, S_RZ, S_PQ, S_ALPHA);
-    uftk::cg_update(s, n, C->d_scal, S_ALPHA, U, p, r, q, C->d_part);
-    uftk::final_sum(s, C->d_part, uftk::red_blocks(n), 1, C->d_scal + S_RR);
+    uftk::cg_update(s, n, C->d_scal, S_ALPHA, U, p, r, q, C->d_part, C->d_scal + S_RR);
     ++it;
     if ((rc = fetch(C, C->d_scal + S_RR, 1, C->h_scal))) return rc;
     const double rn = std::sqrt(C->h_scal[0]);
@@ -420,8 +494,7 @@ int solve_el_mixed(uft_ctx *C, const double *xphys, double *U) {
     }
     std::swap(z, zo);  // zo keeps the previous preconditioned residual
     precond();
-    uftk::dot_pr(s, n, r, z, zo, C->d_part);
-    uftk::final_sum(s, C->d_part, uftk::red_blocks(n), 2, C->d_scal + S_RZNEW);  // RZNEW, RZNEW+1 = r.zo
+    uftk::dot_pr(s, n, r, z, zo, C->d_part, C->d_scal + S_RZNEW);
     uftk::sc_beta_flex(s, C->d_scal, S_RZNEW, S_RZNEW + 1, S_RZ, S_BETA + 2);
     uftk::xpby(s, n, z, C->d_scal, S_BETA + 2, p);
   }
@@ -446,8 +519,7 @@ int solve_filter(uft_ctx *C, const double *b, double *x) {
   double *r = C->d_fr, *z = C->d_fz, *p = C->d_fp, *q = C->d_fq;
   double *sc = C->d_scal;
   uftk::apply_KF(s, C, x, q);
-  uftk::fcg_init(s, n, b, q, C->d_fdinv, r, z, C->d_part);
-  uftk::final_sum(s, C->d_part, uftk::red_blocks(n), 3, sc + S_F0);  // 8: |Db|^2 9: |z|^2 10: z.r
+  uftk::fcg_init(s, n, b, q, C->d_fdinv, r, z, C->d_part, sc + S_F0);  // 8: |Db|^2 9: |z|^2 10: z.r
   int rc = fetch(C, sc + S_F0, 2, C->h_scal);
   if (rc) return rc;
   double snorm = std::sqrt(C->h_scal[0]);
@@ -462,9 +534,8 @@ int solve_filter(uft_ctx *C, const double *b, double *x) {
     uftk::apply_KF(s, C, p, q);
     uftk::dot(s, n, p, q, C->d_part, sc + S_F0 + 4);
     uftk::sc_div(s, sc, S_F0 + 2, S_F0 + 4, S_F0 + 5);
-    uftk::fcg_update(s, n, sc, S_F0 + 5, x, p, r, q, C->d_fdinv, z, C->d_part);
+    uftk::fcg_update(s, n, sc, S_F0 + 5, x, p, r, q, C->d_fdinv, z, C->d_part, sc + S_F0 + 6);
     uftk::sc_copy(s, sc + S_F0 + 2, sc + S_F0 + 3);                     // beta_old = beta
-    uftk::final_sum(s, C->d_part, uftk::red_blocks(n), 2, sc + S_F0 + 6);  // 14: |z|^2 15: z.r
     uftk::sc_copy(s, sc + S_F0 + 7, sc + S_F0 + 2);                     // beta = z.r
     ++it;
     rc = fetch(C, sc + S_F0 + 6, 1, C->h_scal);
@@ -510,7 +581,7 @@ void uft_default_params(uft_params *p) {
   p->amg_levels = 4;                    // SolveFem.cpp:131
   p->cheb_degree = 2;
   p->coarse_max = 600;
-  p->inner_rtol_exp = 4;
+  p->cycle_type = 2;
 }
 
 int uft_create(uft_ctx **out, int device) {
@@ -657,6 +728,7 @@ int uft_setup(uft_ctx *C) {
   if ((rc = dalloc_n(C, &C->d_detmin, ne))) return rc;
   if ((rc = dalloc_n(C, &C->d_Ye, (int64_t)nd * ne))) return rc;
   const int64_t maxn = std::max(ndof, ne);
+  // block partials of the deterministic two-launch reductions
   if ((rc = dalloc_n(C, &C->d_part, (maxn / 256 + 2) * 48))) return rc;
   if ((rc = dalloc_n(C, &C->d_scal, 256))) return rc;
   if ((rc = dalloc_n(C, &C->d_lanczos, 3 * 64))) return rc;
@@ -717,11 +789,21 @@ int uft_setup(uft_ctx *C) {
       for (double **p : wv)
         if ((rc = dalloc_n(C, p, Lv.n))) return rc;
       if ((rc = dalloc_n(C, &Lv.d_dinv, Lv.n))) return rc;
+      if (l > 0) {
+        double **kv[] = {&Lv.kc, &Lv.kv, &Lv.kd, &Lv.kr};
+        for (double **p : kv)
+          if ((rc = dalloc_n(C, p, Lv.n))) return rc;
+      }
       if (C->P.precision == UFT_MIXED) {
         float **wf[] = {&Lv.xf, &Lv.bf, &Lv.rf, &Lv.df, &Lv.tf};
         for (float **p : wf)
           if ((rc = dalloc_n(C, p, Lv.n))) return rc;
         if ((rc = dalloc_n(C, &Lv.d_dinvf, Lv.n))) return rc;
+        if (l > 0) {
+          float **kf[] = {&Lv.kcf, &Lv.kvf, &Lv.kdf, &Lv.krf};
+          for (float **p : kf)
+            if ((rc = dalloc_n(C, p, Lv.n))) return rc;
+        }
       }
     }
     for (int l = 0; l + 1 < A.nlev; ++l) {
@@ -845,7 +927,10 @@ int uft_fem_obj_sens(uft_ctx *C, double *fx, double *gx, int dfdx_id, int m, con
       return uft_set_error(C, UFT_ERR_ARG, "uft_fem_obj_sens: bad dgdx id");
   const double *xp = C->vec[xphys_id];
   double *U = C->vec[UFT_U];
+  const double t0 = now_ms();
   const int src = C->P.precision == UFT_MIXED ? solve_el_mixed(C, xp, U) : solve_el(C, xp, U);
+  const double t1 = now_ms();
+  C->t_phase[1] = t1 - t0 - C->t_phase[0];
   if (src != UFT_OK && src != UFT_ERR_NOCONV) return src;
   // SolveFem.cpp:208-249: only constraint 0 is filled by the reference
   uftk::sens(C->s, C, xp, U, C->vec[dfdx_id], C->vec[dgdx_ids[0]], C->d_part);
@@ -857,6 +942,7 @@ int uft_fem_obj_sens(uft_ctx *C, double *fx, double *gx, int dfdx_id, int m, con
   gx[0] = C->h_scal[1] / C->TV - C->P.volfrac;
   C->fx = fx[0];
   C->gx[0] = gx[0];
+  C->t_phase[2] = now_ms() - t1;
   return src;
 }
 
@@ -1083,18 +1169,28 @@ int uft_design_iteration(uft_ctx *C, int itr, double *fx, double *gx, double *ch
   int dg[UFT_MAXM];
   for (int j = 0; j < m; ++j) dg[j] = UFT_DGDX0 + j;
   int status = UFT_OK;
+  const double t0 = now_ms();
   rc = uft_fem_obj_sens(C, fx, gx, UFT_DFDX, m, dg, UFT_XPHYS);
   if (rc == UFT_ERR_NOCONV) status = rc; else if (rc) return rc;
+  const double t1 = now_ms();
   if (C->P.proj && (rc = uft_projected_sens(C, UFT_XFILED, UFT_DFDX, m, dg))) return rc;
   rc = uft_filter_sens(C, UFT_DFDX, m, dg);
   if (rc == UFT_ERR_NOCONV) status = rc; else if (rc) return rc;
+  if ((rc = sync(C))) return rc;
+  const double t2 = now_ms();
   if ((rc = uft_mma_movelimit(C, UFT_X, UFT_XMIN, UFT_XMAX))) return rc;
   if ((rc = uft_mma_update(C, UFT_X, UFT_DFDX, gx, dg, UFT_XMIN, UFT_XMAX))) return rc;
   if ((rc = uft_mma_design_change(C, UFT_X, UFT_XOLD, ch))) return rc;
+  const double t3 = now_ms();
   rc = uft_filter(C, UFT_X, UFT_XFILED);
   if (rc == UFT_ERR_NOCONV) status = rc; else if (rc) return rc;
   if ((rc = uft_heaviside(C, UFT_XFILED, UFT_XPHYS, itr))) return rc;
   if ((rc = sync(C))) return rc;
+  const double t4 = now_ms();
+  C->t_phase[3] = t2 - t1;
+  C->t_phase[4] = t3 - t2;
+  C->t_phase[5] = t4 - t3;
+  C->t_phase[6] = t4 - t0;
   return status;
 }
 
@@ -1189,5 +1285,22 @@ extern "C" int uft_profile_read(uft_ctx *C, int kind, int64_t *count, double *to
   C->evused = 0;
   *count = C->prof_n[kind];
   *total_ms = C->prof_ms[kind];
+  return UFT_OK;
+}
+
+extern "C" int uft_solver_info(const uft_ctx *C, int *nlev, int64_t *n_per_level, int64_t *blocks_per_level) {
+  if (!C || !nlev) return UFT_ERR_ARG;
+  if (!C->is_setup) return UFT_ERR_STATE;
+  *nlev = C->amg.nlev;
+  for (int l = 0; l < C->amg.nlev; ++l) {
+    if (n_per_level) n_per_level[l] = C->amg.L[l].n;
+    if (blocks_per_level) blocks_per_level[l] = l == 0 ? C->ne : C->amg.L[l].nblk;
+  }
+  return UFT_OK;
+}
+
+extern "C" int uft_phase_times(const uft_ctx *C, double *ms) {
+  if (!C || !ms) return UFT_ERR_ARG;
+  for (int k = 0; k < 7; ++k) ms[k] = C->t_phase[k];
   return UFT_OK;
 }

@@ -47,6 +47,9 @@ struct AmgLevel {
   // work vectors (length n)
   double *x = nullptr, *b = nullptr, *r = nullptr, *d = nullptr, *t = nullptr;
   float *xf = nullptr, *bf = nullptr, *rf = nullptr, *df = nullptr, *tf = nullptr;
+  // coarse levels: W/K-cycle work vectors
+  double *kc = nullptr, *kv = nullptr, *kd = nullptr, *kr = nullptr;
+  float *kcf = nullptr, *kvf = nullptr, *kdf = nullptr, *krf = nullptr;
 };
 
 struct AmgHier {
@@ -151,6 +154,11 @@ struct uft_ctx {
   double prof_ms[4] = {0.0, 0.0, 0.0, 0.0};
   int64_t prof_n[4] = {0, 0, 0, 0};
 
+  // wall time of the phases of the last design iteration (ms): 0 hierarchy
+  // refresh, 1 elasticity Krylov solve, 2 sensitivities, 3 sensitivity
+  // filter, 4 MMA, 5 density filter + projection, 6 whole iteration
+  double t_phase[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+
   // stats
   int it_el = 0, it_f = 0, outer_el = 0, dip_steps = 0;
   double relres_el = 0.0;
@@ -168,8 +176,9 @@ void gen_ke(hipStream_t s, int npe, int64_t ne, const int32_t *elem, const doubl
 void gen_kf(hipStream_t s, int npe, int64_t ne, const int32_t *elem, const double *xyz, double rmin,
             double *KF, double *T, double *vol, double *part);
 // fine operator apply: y = K_D(x) u   (two-phase element/node)
+// dotw != nullptr: also y.dotw, reduced in the same launch into *dot_out
 void apply_K(hipStream_t s, const uft_ctx *C, const double *xphys, const double *u, double *y,
-             const double *dotw, double *part, hipEvent_t *ev = nullptr);
+             const double *dotw, double *dot_out, hipEvent_t *ev = nullptr);
 void apply_Kf(hipStream_t s, const uft_ctx *C, const double *xphys, const float *u, float *y,
               hipEvent_t *ev = nullptr);
 void diag_K(hipStream_t s, const uft_ctx *C, const double *xphys, double *dinv, float *dinvf);
@@ -194,14 +203,14 @@ void d2f(hipStream_t s, int64_t n, const double *a, float *b);
 void f2d_add(hipStream_t s, int64_t n, const float *a, double *b);
 void scale(hipStream_t s, int64_t n, const double *a, const double *dinv, double *z);
 void cg_update(hipStream_t s, int64_t n, const double *sc, int ialpha, double *u, const double *p,
-               double *r, const double *q, double *part);
+               double *r, const double *q, double *part, double *rr_out);
 void xpby(hipStream_t s, int64_t n, const double *z, const double *sc, int ibeta, double *p);
 void axpy_inplace(hipStream_t s, int64_t n, double a, const double *x, double *y);
 void sub(hipStream_t s, int64_t n, const double *a, const double *b, double *c);
 void set_dofs(hipStream_t s, int n, const int32_t *idx, const double *val, double *x);
 // templated float variants for the mixed form
 void cg_updatef(hipStream_t s, int64_t n, const double *sc, int ialpha, float *u, const float *p,
-                float *r, const float *q, double *part);
+                float *r, const float *q, double *part, double *rr_out);
 void xpbyf(hipStream_t s, int64_t n, const float *z, const double *sc, int ibeta, float *p);
 void dotf(hipStream_t s, int64_t n, const float *a, const float *b, double *part, double *out);
 // multigrid
@@ -252,16 +261,22 @@ void heaviside(hipStream_t s, int64_t n, const double *xf, double *xp, double be
 void projected_sens(hipStream_t s, int64_t n, const double *xf, double beta, double x0, double *v);
 // Krylov helpers (kernels_solve.hip)
 void fcg_init(hipStream_t s, int64_t n, const double *b, const double *q, const double *dinv, double *r, double *z,
-              double *part);
+              double *part, double *out);
 void fcg_update(hipStream_t s, int64_t n, const double *sc, int ia, double *x, const double *p, double *r,
-                const double *q, const double *dinv, double *z, double *part);
+                const double *q, const double *dinv, double *z, double *part, double *out);
 void fcg_p(hipStream_t s, int64_t n, const double *z, const double *sc, int ib, int ibo, int first, double *p);
 void sc_div(hipStream_t s, double *sc, int a, int b, int dst);
 void sc_beta(hipStream_t s, double *sc, int inew, int iold, int ibeta, double *log2);
 void sc_copy(hipStream_t s, const double *src, double *dst);
 void f2d(hipStream_t s, int64_t n, const float *a, double *b);
-void dot_pr(hipStream_t s, int64_t n, const double *r, const double *z, const double *zo, double *part);
+void dot_pr(hipStream_t s, int64_t n, const double *r, const double *z, const double *zo, double *part, double *out);
 void sc_beta_flex(hipStream_t s, double *sc, int irz, int irzo, int iprev, int ibeta);
+void lincomb(hipStream_t s, int64_t n, const double *sc, int ia, const double *a, int ib, const double *b, double *y);
+void lincombf(hipStream_t s, int64_t n, const double *sc, int ia, const float *a, int ib, const float *b, float *y);
+void axpy_inplace_t(hipStream_t s, int64_t n, const double *d, double *x);
+void axpy_inplace_t(hipStream_t s, int64_t n, const float *d, float *x);
+void kcycle_coef1(hipStream_t s, double *sc);
+void kcycle_coef2(hipStream_t s, double *sc);
 }  // namespace uftk
 
 // host helpers (uft_api.cpp)

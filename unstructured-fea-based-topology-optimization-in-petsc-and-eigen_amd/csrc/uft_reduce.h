@@ -1,0 +1,39 @@
+// Deterministic block reductions for the gfx950 kernels.
+//
+// Every block writes its K partial sums in a fixed intra-block order (64-lane
+// shuffle tree, then the four waves in order); a one-block final kernel
+// (uftk::final_sum) then sums the partials in a fixed order.  Results do not
+// depend on scheduling, so reductions are bit-reproducible run to run.
+//
+// A single-launch variant (last block finalises after a device-scope fence and
+// a ticket atomic) was measured slower on MI355X: the device-scope release per
+// block has to make every block's partials visible across the eight XCD L2
+// caches, which costs more than the second launch it saves.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace uftr {
+
+constexpr int RTPB = 256;
+
+template <int K>
+__device__ __forceinline__ void block_partials(double (&v)[K], double *part) {
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_down(v[k], off, 64);
+  }
+  __shared__ double sm[K][RTPB / 64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) sm[k][w] = v[k];
+  }
+  __syncthreads();
+  if (threadIdx.x < K) {
+    const int k = threadIdx.x;
+    part[(size_t)blockIdx.x * K + k] = (sm[k][0] + sm[k][1]) + (sm[k][2] + sm[k][3]);
+  }
+}
+
+}  // namespace uftr
