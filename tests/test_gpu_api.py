@@ -1,0 +1,144 @@
+"""GPU: the C-ABI boundary — reference text formats, error handling, the
+drop-in driver binary, determinism and the VTU writer."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import conftest
+
+pytestmark = pytest.mark.gpu
+
+
+def _write_ws(path, values, fmt):
+    # whitespace separated, no trailing newline (like the reference's BC fixtures)
+    with open(path, "w") as f:
+        f.write(" ".join(fmt % v for v in values))
+
+
+def _ctx(**prm):
+    from uft.capi import Context
+    c = Context(0)
+    c.set_params(Context.default_params(**prm))
+    return c
+
+
+def test_text_readers_match_arrays(O, tmp_path):
+    import uft
+    print("[case] text readers", flush=True)
+    nx, ny, nz = 6, 3, 3
+    xyz, elem = uft.gen_box(8, nx, ny, nz, jitter=0.1, seed=5)
+    fixed, ld, lv = uft.box_bcs(nx, ny, nz)
+    _write_ws(tmp_path / "vertex.txt", xyz.ravel(), "%.17g")
+    _write_ws(tmp_path / "element.txt", elem.ravel(), "%d")
+    _write_ws(tmp_path / "fixeddofs.txt", fixed, "%d")
+    _write_ws(tmp_path / "loaddof_1.txt", ld, "%d")
+    _write_ws(tmp_path / "load_1.txt", lv, "%.17g")
+    prm = dict(Emax=1.0, volfrac=0.3, rmin=0.5, rtol_el=1e-12, rtol_f=1e-12)
+    a = _ctx(**prm)
+    L = a.L
+    assert L.uft_mesh_load_text(a.h, 8, len(xyz), len(elem), str(tmp_path / "vertex.txt").encode(),
+                                str(tmp_path / "element.txt").encode()) == 0
+    assert L.uft_bc_load_text(a.h, str(tmp_path).encode(), len(fixed), 1, len(ld)) == 0
+    a.setup()
+    b = _ctx(**prm)
+    b.set_mesh(elem, xyz)
+    b.set_bcs(fixed, ld, lv)
+    b.setup()
+    e2 = np.zeros(elem.size, np.int32)
+    x2 = np.zeros(xyz.size)
+    assert L.uft_mesh_get(a.h, e2, x2) == 0
+    assert np.array_equal(e2, elem.ravel()) and np.array_equal(x2, xyz.ravel())
+    fa, ga = a.fem_obj_sens()
+    fb, gb = b.fem_obj_sens()
+    assert fa == fb and ga == gb
+
+
+def test_io_errors(tmp_path):
+    print("[case] io errors", flush=True)
+    a = _ctx()
+    rc = a.L.uft_mesh_load_text(a.h, 8, 10, 1, str(tmp_path / "nope.txt").encode(), b"x")
+    assert rc == 2  # UFT_ERR_IO
+    assert b"cannot open" in a.L.uft_last_error(a.h)
+    _write_ws(tmp_path / "v.txt", [0.0] * 5, "%g")
+    _write_ws(tmp_path / "e.txt", [0] * 8, "%d")
+    rc = a.L.uft_mesh_load_text(a.h, 8, 8, 1, str(tmp_path / "v.txt").encode(), str(tmp_path / "e.txt").encode())
+    assert rc == 2 and b"expected" in a.L.uft_last_error(a.h)
+
+
+def test_state_and_argument_errors():
+    from uft.capi import UFTError
+    print("[case] state errors", flush=True)
+    a = _ctx()
+    with pytest.raises(UFTError) as e:
+        a.fem_obj_sens()
+    assert e.value.code == 7  # UFT_ERR_STATE
+    elem = np.array([[0, 1, 2, 3, 4, 5, 6, 99]], np.int32)
+    with pytest.raises(UFTError) as e:
+        a.set_mesh(elem, np.zeros((8, 3)))
+    assert e.value.code == 1
+
+
+def test_inverted_element_detected():
+    import uft
+    from uft.capi import UFTError
+    print("[case] inverted element", flush=True)
+    xyz, elem = uft.gen_box(8, 2, 1, 1)
+    elem = elem.copy()
+    elem[0] = elem[0][[1, 0, 3, 2, 5, 4, 7, 6]]  # mirrored node order: det J < 0
+    fixed, ld, lv = uft.box_bcs(2, 1, 1)
+    a = _ctx()
+    a.set_mesh(elem, xyz)
+    a.set_bcs(fixed, ld, lv)
+    with pytest.raises(UFTError) as e:
+        a.setup()
+    assert e.value.code == 4  # UFT_ERR_DETJ
+
+
+def test_nonconvergence_is_reported():
+    from uft.capi import cantilever, UFTError
+    print("[case] maxit 1", flush=True)
+    ctx = cantilever(8, 8, 4, 4, jitter=0.1, seed=2, Emax=1.0, rtol_el=1e-12, maxit_el=1)
+    with pytest.raises(UFTError) as e:
+        ctx.fem_obj_sens()
+    assert e.value.code == 3  # UFT_ERR_NOCONV
+    fx, gx = ctx.fem_obj_sens(allow_noconv=True)
+    assert np.isfinite(fx) and ctx.stats()["it_el"] == 1
+
+
+def test_bitwise_reproducible():
+    from uft.capi import cantilever
+    print("[case] determinism", flush=True)
+    out = []
+    for _ in range(2):
+        ctx = cantilever(8, 10, 5, 5, jitter=0.15, seed=4, Emax=1.0, volfrac=0.3, rmin=0.3, rtol_el=1e-8,
+                         rtol_f=1e-8)
+        ctx.init_design()
+        r = [ctx.design_iteration(it) for it in (1, 2, 3)]
+        out.append((r, ctx.get("X"), ctx.get("U")))
+        ctx.close()
+    assert out[0][0] == out[1][0]
+    assert np.array_equal(out[0][1], out[1][1]) and np.array_equal(out[0][2], out[1][2])
+
+
+def test_dropin_driver_runs(O, tmp_path):
+    """bin/uft_opt (the reference's main.cpp sequence over the drop-in classes)."""
+    exe = os.path.join(conftest.PKG, "bin", "uft_opt")
+    print("[case] drop-in driver", flush=True)
+    args = [exe, "-box", "8,4,4", "-jitter", "0.15", "-seed", "7", "-Emax", "1", "-volfrac", "0.3", "-rmin", "0.375",
+            "-maxItr", "5", "-ksp_rtol", "1e-12", "-vtu_dir", str(tmp_path)]
+    res = subprocess.run(args, capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stderr
+    lines = [l for l in res.stdout.splitlines() if l.startswith("It.:")]
+    assert len(lines) == 5
+    ref = O.cantilever(8, 8, 4, 4, jitter=0.15, seed=7, Emax=1.0, volfrac=0.3, rmin=0.375, rtol_el=1e-12,
+                       rtol_f=1e-12)
+    n, fx, gx, ch = ref.run(5)
+    for line, f in zip(lines, fx):
+        got = float(line.split("fx:")[1].split(",")[0])
+        assert abs(got - f) <= 1e-6 * abs(f) + 1e-6  # the log line prints %f
+    vtus = sorted(os.listdir(tmp_path))
+    assert "result_0.vtu" in vtus and "result_-1.vtu" in vtus
+    txt = open(tmp_path / "result_0.vtu").read()
+    assert 'NumberOfCells="128"' in txt and 'Name="xphys"' in txt
