@@ -60,7 +60,7 @@ typedef struct uft_params {
   int amg_levels;      /* total levels incl. the fine one (reference: PCGAMGSetNlevels 4) */
   int cheb_degree;     /* Chebyshev-Jacobi smoother degree per level */
   int coarse_max;      /* stop coarsening at or below this many coarse DOFs */
-  int cycle_type;      /* coarse-grid cycle: 0 V-cycle, 1 W-cycle, 2 K-cycle (two FCG steps per coarse level) */
+  int cycle_type;      /* coarse-grid cycle: 0 V, 1 W, 2 K-cycle on every coarse level, 3 K-cycle on level 1 only */
 } uft_params;
 
 /* --- context ------------------------------------------------------------ */
@@ -155,6 +155,13 @@ int uft_solver_info(const uft_ctx *ctx, int *nlev, int64_t *n_per_level, int64_t
  * filter, 4 MMA update + design change, 5 density filter + projection,
  * 6 whole iteration.  ms must hold 7 doubles. */
 int uft_phase_times(const uft_ctx *ctx, double *ms);
+/* Autotuner (no counterpart in the reference): times every element-phase
+ * kernel variant of the fine stiffness apply (workgroup size, streaming
+ * loads, occupancy floor) on the device and keeps the fastest for this mesh;
+ * uft_setup runs it once.  All variants give bit-identical results.
+ * variant[0]/[1] receive the FP64/FP32 choice; us_per_variant (nmax entries)
+ * the FP64 timings in microseconds. */
+int uft_autotune(uft_ctx *ctx, int reps, int *variant, double *us_per_variant, int nmax);
 /* Synchronise the context stream. */
 int uft_sync(uft_ctx *ctx);
 /* Device timers: HIP events recorded on the context stream around every

@@ -66,24 +66,32 @@ __global__ __launch_bounds__(TPB) void k_subv(int64_t n, const T *__restrict__ a
 }
 
 // ---------------------------------------------------------------- BSR 6x6
-// one thread per scalar row: y[6A+i] = sum_blocks sum_j V[blk][i][j] x[6B+j]
+// LPR lanes per scalar row: y[6A+i] = sum_blocks sum_j V[blk][i][j] x[6B+j];
+// lane l of a row takes blocks l, l+LPR, ...; the LPR partial sums are then
+// combined with a fixed shuffle tree (deterministic).
+constexpr int LPR = 8;
 template <typename T>
 __global__ __launch_bounds__(TPB) void k_bsr6_mv(int64_t nrows, const int32_t *__restrict__ rp,
                                                  const int32_t *__restrict__ ci, const T *__restrict__ val,
                                                  const T *__restrict__ x, T *__restrict__ y) {
-  const int64_t row = (int64_t)blockIdx.x * TPB + threadIdx.x;
-  if (row >= nrows * 6) return;
-  const int64_t A = row / 6;
-  const int i = (int)(row - A * 6);
+  const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  const int64_t row = t / LPR;
+  const int l = (int)(t - row * LPR);
   T acc = T(0);
-  for (int k = rp[A]; k < rp[A + 1]; ++k) {
-    const int64_t B = ci[k];
-    const T *v = val + (size_t)k * 36 + i * 6;
-    const T *xb = x + B * 6;
+  if (row < nrows * 6) {
+    const int64_t A = row / 6;
+    const int i = (int)(row - A * 6);
+    for (int k = rp[A] + l; k < rp[A + 1]; k += LPR) {
+      const int64_t B = ci[k];
+      const T *v = val + (size_t)k * 36 + i * 6;
+      const T *xb = x + B * 6;
 #pragma unroll
-    for (int j = 0; j < 6; ++j) acc += v[j] * xb[j];
+      for (int j = 0; j < 6; ++j) acc += v[j] * xb[j];
+    }
   }
-  y[row] = acc;
+#pragma unroll
+  for (int off = LPR / 2; off > 0; off >>= 1) acc += __shfl_down(acc, off, LPR);
+  if (l == 0 && row < nrows * 6) y[row] = acc;
 }
 
 // zero diagonal entries of diagonal blocks (rank-deficient coarse columns,
@@ -411,11 +419,11 @@ void subf(hipStream_t s, int64_t n, const float *a, const float *b, float *c) {
 }
 void bsr6_mv(hipStream_t s, int64_t nb_rows, const int32_t *rp, const int32_t *ci, const double *val, const double *x,
              double *y) {
-  k_bsr6_mv<double><<<nblk(nb_rows * 6), TPB, 0, s>>>(nb_rows, rp, ci, val, x, y);
+  k_bsr6_mv<double><<<nblk(nb_rows * 6 * LPR), TPB, 0, s>>>(nb_rows, rp, ci, val, x, y);
 }
 void bsr6_mvf(hipStream_t s, int64_t nb_rows, const int32_t *rp, const int32_t *ci, const float *val, const float *x,
               float *y) {
-  k_bsr6_mv<float><<<nblk(nb_rows * 6), TPB, 0, s>>>(nb_rows, rp, ci, val, x, y);
+  k_bsr6_mv<float><<<nblk(nb_rows * 6 * LPR), TPB, 0, s>>>(nb_rows, rp, ci, val, x, y);
 }
 void bsr6_dinv(hipStream_t s, int64_t nb_rows, const int32_t *rp, const int32_t *ci, double *val, double *dinv,
                float *dinvf) {

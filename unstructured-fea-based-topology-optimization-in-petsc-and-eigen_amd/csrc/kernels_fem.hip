@@ -453,6 +453,65 @@ __device__ __forceinline__ void kacc(T k, const T *ue, T *ye) {
 
 // Element phase: Ye[k][e] = E(x_e) (KE_e u_e)_k with the fixed DOFs of u
 // masked to zero (MatZeroRowsColumns zeroes the columns, SolveFem.cpp:324).
+// Streaming load of one packed KE pair: optionally non-temporal (the element
+// blocks are read once per apply and should not evict the gathered vectors).
+typedef double v2d __attribute__((ext_vector_type(2)));
+typedef float v2f __attribute__((ext_vector_type(2)));
+template <bool NT>
+__device__ __forceinline__ double2 ld_pair(const double2 *p) {
+  if constexpr (NT) {
+    const v2d v = __builtin_nontemporal_load(reinterpret_cast<const v2d *>(p));
+    return make_double2(v.x, v.y);
+  } else {
+    return *p;
+  }
+}
+template <bool NT>
+__device__ __forceinline__ float2 ld_pair(const float2 *p) {
+  if constexpr (NT) {
+    const v2f v = __builtin_nontemporal_load(reinterpret_cast<const v2f *>(p));
+    return make_float2(v.x, v.y);
+  } else {
+    return *p;
+  }
+}
+
+// Element phase variants (picked at setup by uft_autotune): workgroup size BT,
+// non-temporal KE streaming NT, and an occupancy floor of WPE waves per SIMD
+// (0 = compiler's choice).  All variants perform the same arithmetic in the
+// same order, so their results are bit-identical.
+template <int NPE, typename T, typename T2, int BT, bool NT, int WPE>
+__global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1)))
+void k_elem_apply_v(int ne, const int32_t *__restrict__ elem, const T2 *__restrict__ KE,
+                    const double *__restrict__ xphys, SimpCoef sc, const uint8_t *__restrict__ fixm,
+                    const T *__restrict__ u, T *__restrict__ Ye) {
+  constexpr int ND = 3 * NPE, NP = ND * (ND + 1) / 2, NQ = NP / 2;
+  const int e = blockIdx.x * BT + threadIdx.x;
+  if (e >= ne) return;
+  T ue[ND], ye[ND];
+#pragma unroll
+  for (int a = 0; a < NPE; ++a) {
+    const int n = elem[(size_t)a * ne + e];
+    const unsigned m = fixm[n];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const T v = u[3 * (size_t)n + c];
+      ue[c * NPE + a] = ((m >> c) & 1u) ? T(0) : v;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < ND; ++k) ye[k] = T(0);
+  static_for<0, NQ>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    const T2 kk = ld_pair<NT>(&KE[(size_t)q * ne + e]);
+    kacc<pk_row(2 * q, ND), pk_col(2 * q, ND)>(kk.x, ue, ye);
+    kacc<pk_row(2 * q + 1, ND), pk_col(2 * q + 1, ND)>(kk.y, ue, ye);
+  });
+  const T E = (T)simpE(sc, xphys[e]);
+#pragma unroll
+  for (int k = 0; k < ND; ++k) Ye[(size_t)k * ne + e] = E * ye[k];
+}
+
 template <int NPE, typename T, typename T2>
 __global__ __launch_bounds__(TPB) void k_elem_apply(int ne, const int32_t *__restrict__ elem,
                                                     const T2 *__restrict__ KE, const double *__restrict__ xphys,
@@ -883,10 +942,48 @@ __global__ __launch_bounds__(TPB) void k_projsens(int64_t n, const double *__res
 
 }  // namespace
 
+
+// ---- element-phase variant dispatch
+template <int NPE, typename T, typename T2>
+static void elem_phase_t(hipStream_t s, int variant, int ne, const int32_t *elem, const T2 *KE, const double *xphys,
+                         SimpCoef sc, const uint8_t *fixm, const T *u, T *Ye) {
+  auto g = [&](int bt) { return (unsigned)((ne + bt - 1) / bt); };
+  switch (variant) {
+    case 1: k_elem_apply_v<NPE, T, T2, 256, true, 0><<<g(256), 256, 0, s>>>(ne, elem, KE, xphys, sc, fixm, u, Ye); break;
+    case 2: k_elem_apply_v<NPE, T, T2, 256, true, 4><<<g(256), 256, 0, s>>>(ne, elem, KE, xphys, sc, fixm, u, Ye); break;
+    case 3: k_elem_apply_v<NPE, T, T2, 128, true, 0><<<g(128), 128, 0, s>>>(ne, elem, KE, xphys, sc, fixm, u, Ye); break;
+    case 4: k_elem_apply_v<NPE, T, T2, 512, true, 0><<<g(512), 512, 0, s>>>(ne, elem, KE, xphys, sc, fixm, u, Ye); break;
+    case 5: k_elem_apply_v<NPE, T, T2, 256, false, 4><<<g(256), 256, 0, s>>>(ne, elem, KE, xphys, sc, fixm, u, Ye); break;
+    case 6: k_elem_apply_v<NPE, T, T2, 64, true, 0><<<g(64), 64, 0, s>>>(ne, elem, KE, xphys, sc, fixm, u, Ye); break;
+    default: k_elem_apply<NPE, T, T2><<<g(TPB), TPB, 0, s>>>(ne, elem, KE, xphys, sc, fixm, u, Ye); break;
+  }
+}
+
 // ==========================================================================
 // launch wrappers
 // ==========================================================================
 namespace uftk {
+
+int elem_variant_count() { return 7; }
+
+void elem_phase(hipStream_t s, const uft_ctx *C, int variant, const double *xphys, const double *u, double *Ye) {
+  SimpCoef sc{C->P.Emin, C->P.Emax - C->P.Emin, C->P.penal};
+  if (C->npe == 8)
+    elem_phase_t<8, double, double2>(s, variant, (int)C->ne, C->d_elem, (const double2 *)C->d_KE, xphys, sc, C->d_fixm,
+                                     u, Ye);
+  else
+    elem_phase_t<4, double, double2>(s, variant, (int)C->ne, C->d_elem, (const double2 *)C->d_KE, xphys, sc, C->d_fixm,
+                                     u, Ye);
+}
+void elem_phase_f(hipStream_t s, const uft_ctx *C, int variant, const double *xphys, const float *u, float *Ye) {
+  SimpCoef sc{C->P.Emin, C->P.Emax - C->P.Emin, C->P.penal};
+  if (C->npe == 8)
+    elem_phase_t<8, float, float2>(s, variant, (int)C->ne, C->d_elem, (const float2 *)C->d_KEf, xphys, sc, C->d_fixm, u,
+                                   Ye);
+  else
+    elem_phase_t<4, float, float2>(s, variant, (int)C->ne, C->d_elem, (const float2 *)C->d_KEf, xphys, sc, C->d_fixm, u,
+                                   Ye);
+}
 
 int red_blocks(int64_t n) { return (int)nblk(n); }
 
@@ -916,12 +1013,7 @@ void apply_K(hipStream_t s, const uft_ctx *C, const double *xphys, const double 
   double *part = dotw ? C->d_part : nullptr;
   const int ne = (int)C->ne, nn = (int)C->nn;
   if (ev) (void)hipEventRecord(ev[0], s);
-  if (C->npe == 8)
-    k_elem_apply<8, double, double2><<<nblk(ne), TPB, 0, s>>>(ne, C->d_elem, (const double2 *)C->d_KE, xphys,
-                                                               simp(C), C->d_fixm, u, C->d_Ye);
-  else
-    k_elem_apply<4, double, double2><<<nblk(ne), TPB, 0, s>>>(ne, C->d_elem, (const double2 *)C->d_KE, xphys,
-                                                               simp(C), C->d_fixm, u, C->d_Ye);
+  elem_phase(s, C, C->elem_variant, xphys, u, C->d_Ye);
   if (ev) (void)hipEventRecord(ev[1], s);
   if (C->npe == 8)
     k_node_gather<8, 3, 1, double><<<nblk(nn), TPB, 0, s>>>(nn, ne, C->d_iptr, C->d_inc, C->d_Ye, C->d_fixm, u, y,
@@ -936,12 +1028,7 @@ void apply_K(hipStream_t s, const uft_ctx *C, const double *xphys, const double 
 void apply_Kf(hipStream_t s, const uft_ctx *C, const double *xphys, const float *u, float *y, hipEvent_t *ev) {
   const int ne = (int)C->ne, nn = (int)C->nn;
   if (ev) (void)hipEventRecord(ev[0], s);
-  if (C->npe == 8)
-    k_elem_apply<8, float, float2><<<nblk(ne), TPB, 0, s>>>(ne, C->d_elem, (const float2 *)C->d_KEf, xphys,
-                                                             simp(C), C->d_fixm, u, C->d_Yef);
-  else
-    k_elem_apply<4, float, float2><<<nblk(ne), TPB, 0, s>>>(ne, C->d_elem, (const float2 *)C->d_KEf, xphys,
-                                                             simp(C), C->d_fixm, u, C->d_Yef);
+  elem_phase_f(s, C, C->elem_variant_f, xphys, u, C->d_Yef);
   if (ev) (void)hipEventRecord(ev[1], s);
   if (C->npe == 8)
     k_node_gather<8, 3, 1, float><<<nblk(nn), TPB, 0, s>>>(nn, ne, C->d_iptr, C->d_inc, C->d_Yef, C->d_fixm, u, y,

@@ -249,7 +249,7 @@ void coarse_solve(uft_ctx *C, int l, const T *b, T *x) {
   LV<T> v = lv(Lv, (T *)nullptr);
   const int64_t n = Lv.n;
   const int ct = C->P.cycle_type;
-  if (ct == 0) {
+  if (ct == 0 || (ct == 3 && l > 1)) {
     cycle(C, l, b, x);
   } else if (ct == 1) {
     cycle(C, l, b, x);
@@ -561,6 +561,8 @@ int check_setup(uft_ctx *C) {
 
 }  // namespace
 
+extern "C" int uft_autotune(uft_ctx *C, int reps, int *variant, double *us_per_variant, int nmax);
+
 // ==========================================================================
 extern "C" {
 
@@ -861,7 +863,7 @@ int uft_setup(uft_ctx *C) {
   C->btemp = C->P.beta0;
   if ((rc = sync(C))) return rc;
   C->is_setup = true;
-  return UFT_OK;
+  return uft_autotune(C, 3, nullptr, nullptr, 0);
 }
 
 int uft_mesh_get(const uft_ctx *C, int32_t *elem, double *xyz) {
@@ -1303,4 +1305,55 @@ extern "C" int uft_phase_times(const uft_ctx *C, double *ms) {
   if (!C || !ms) return UFT_ERR_ARG;
   for (int k = 0; k < 7; ++k) ms[k] = C->t_phase[k];
   return UFT_OK;
+}
+
+// Element-phase autotuner: every variant computes bit-identical results, so
+// the choice only affects speed.  Each variant is timed over `reps` launches
+// with HIP events on the context stream (one untimed launch first).
+extern "C" int uft_autotune(uft_ctx *C, int reps, int *variant, double *us_per_variant, int nmax) {
+  int rc = check_setup(C);
+  if (rc) return rc;
+  if (reps < 1) reps = 1;
+  const int nv = uftk::elem_variant_count();
+  hipEvent_t e0, e1;
+  UFT_HIP(C, hipEventCreate(&e0));
+  UFT_HIP(C, hipEventCreate(&e1));
+  const double *xp = C->vec[UFT_XPHYS];
+  auto tune = [&](bool f32, int &best, double *us) -> int {
+    double bt = 1e300;
+    for (int v = 0; v < nv; ++v) {
+      if (f32) uftk::elem_phase_f(C->s, C, v, xp, C->d_rf, C->d_Yef);
+      else uftk::elem_phase(C->s, C, v, xp, C->d_r, C->d_Ye);
+      UFT_HIP(C, hipEventRecord(e0, C->s));
+      for (int k = 0; k < reps; ++k) {
+        if (f32) uftk::elem_phase_f(C->s, C, v, xp, C->d_rf, C->d_Yef);
+        else uftk::elem_phase(C->s, C, v, xp, C->d_r, C->d_Ye);
+      }
+      UFT_HIP(C, hipEventRecord(e1, C->s));
+      UFT_HIP(C, hipEventSynchronize(e1));
+      float ms = 0.f;
+      UFT_HIP(C, hipEventElapsedTime(&ms, e0, e1));
+      us[v] = 1e3 * ms / reps;
+      if (us[v] < bt) {
+        bt = us[v];
+        best = v;
+      }
+    }
+    return UFT_OK;
+  };
+  uftk::fill(C->s, C->ndof, 1.0, C->d_r);
+  if ((rc = tune(false, C->elem_variant, C->tune_us))) return rc;
+  if (C->d_KEf) {
+    uftk::fillf(C->s, C->ndof, 1.0f, C->d_rf);
+    if ((rc = tune(true, C->elem_variant_f, C->tune_us_f))) return rc;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (variant) {
+    variant[0] = C->elem_variant;
+    if (nmax > 1) variant[1] = C->elem_variant_f;
+  }
+  if (us_per_variant)
+    for (int v = 0; v < nv && v < nmax; ++v) us_per_variant[v] = C->tune_us[v];
+  return sync(C);
 }

@@ -159,6 +159,10 @@ struct uft_ctx {
   // filter, 4 MMA, 5 density filter + projection, 6 whole iteration
   double t_phase[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 
+  // element-phase kernel variants chosen by uft_autotune
+  int elem_variant = 0, elem_variant_f = 0;
+  double tune_us[16] = {0}, tune_us_f[16] = {0};
+
   // stats
   int it_el = 0, it_f = 0, outer_el = 0, dip_steps = 0;
   double relres_el = 0.0;
@@ -181,6 +185,9 @@ void apply_K(hipStream_t s, const uft_ctx *C, const double *xphys, const double 
              const double *dotw, double *dot_out, hipEvent_t *ev = nullptr);
 void apply_Kf(hipStream_t s, const uft_ctx *C, const double *xphys, const float *u, float *y,
               hipEvent_t *ev = nullptr);
+int elem_variant_count();
+void elem_phase(hipStream_t s, const uft_ctx *C, int variant, const double *xphys, const double *u, double *Ye);
+void elem_phase_f(hipStream_t s, const uft_ctx *C, int variant, const double *xphys, const float *u, float *Ye);
 void diag_K(hipStream_t s, const uft_ctx *C, const double *xphys, double *dinv, float *dinvf);
 void sens(hipStream_t s, const uft_ctx *C, const double *xphys, const double *u, double *dfdx,
           double *dgdx0, double *part);
