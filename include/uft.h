@@ -149,6 +149,16 @@ int uft_element_kf(uft_ctx *ctx, int64_t e, double *kf, double *t);
  * hierarchy is internal to PETSc): level count (at most 8), unknowns per level
  * and, per level, the element count (level 0) or 6x6 block count. */
 int uft_solver_info(const uft_ctx *ctx, int *nlev, int64_t *n_per_level, int64_t *blocks_per_level);
+/* Multigrid inspection (no counterpart in the reference): the 6x6-block
+ * operator of level >= 1 as refreshed for the last solve (rp: units+1, ci and
+ * val: blocks and 36*blocks, row-major blocks), and the transfer of level l to
+ * l+1 (parent: units of level l; P: units * bs * 6 with bs = 3 on level 0, 6
+ * above; the operator of level l+1 is P^T A_l P). */
+int uft_amg_operator_get(uft_ctx *ctx, int level, int32_t *rp, int32_t *ci, double *val);
+int uft_amg_transfer_get(uft_ctx *ctx, int level, int32_t *parent, double *P);
+/* The dense inverse of the coarsest operator used by the coarse solve
+ * (row-major n x n, n = unknowns of the last level). */
+int uft_amg_coarse_inverse_get(uft_ctx *ctx, double *inv);
 /* Wall time in ms of the phases of the last uft_design_iteration (the
  * reference prints only the whole iteration, main.cpp:112): 0 solver
  * hierarchy refresh, 1 elasticity Krylov solve, 2 sensitivities, 3 sensitivity

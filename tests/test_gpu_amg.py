@@ -1,0 +1,113 @@
+"""GPU: the multigrid hierarchy behind the elasticity solve.
+
+The coarse operators are checked against explicit Galerkin products
+P^T A P built on the CPU from the oracle's assembled stiffness matrix (with
+the reference's Dirichlet elimination, SolveFem.cpp:324-333), level by level.
+The preconditioner only steers the Krylov iteration, so these checks pin the
+hierarchy itself; the solution is pinned by the parity tests."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+pytestmark = pytest.mark.gpu
+
+
+def _transfer_matrix(parent, P):
+    nu, bs, _ = P.shape
+    rows, cols, vals = [], [], []
+    for u in range(nu):
+        if parent[u] < 0:
+            continue
+        for i in range(bs):
+            for j in range(6):
+                rows.append(bs * u + i)
+                cols.append(6 * parent[u] + j)
+                vals.append(P[u, i, j])
+    nc = 6 * (parent.max() + 1)
+    return sp.csr_matrix((vals, (rows, cols)), shape=(nu * bs, nc))
+
+
+def _bsr_dense(rp, ci, val):
+    nu = len(rp) - 1
+    D = np.zeros((6 * nu, 6 * nu))
+    for A in range(nu):
+        for k in range(rp[A], rp[A + 1]):
+            B = ci[k]
+            D[6 * A:6 * A + 6, 6 * B:6 * B + 6] = val[k]
+    return D
+
+
+def _unit_diag_rule(D):
+    # zero diagonal entries (decoupled rank-deficient coarse columns) become 1
+    d = np.diag(D).copy()
+    d[d == 0.0] = 1.0
+    np.fill_diagonal(D, d)
+    return D
+
+
+@pytest.mark.parametrize("npe,dims", [(8, (8, 4, 4)), (4, (5, 3, 3))])
+def test_galerkin_operators(O, npe, dims):
+    from uft.capi import cantilever
+    nx, ny, nz = dims
+    print(f"[case] galerkin npe={npe} dims={dims}", flush=True)
+    prm = dict(Emax=1.0, volfrac=0.3, rmin=0.3, rtol_el=1e-10, amg_levels=4, coarse_max=6)
+    ctx = cantilever(npe, nx, ny, nz, jitter=0.15, seed=21, **prm)
+    ref = O.cantilever(npe, nx, ny, nz, jitter=0.15, seed=21, **prm)
+    rng = np.random.default_rng(5)
+    x = rng.uniform(0.01, 1.0, ref.ne)
+    ctx.set("XPhys", x)
+    ctx.fem_obj_sens()
+    nlev = len(ctx.solver_info()[0])
+    assert nlev >= 3
+    ndof = 3 * ref.nn
+    K = np.column_stack([ref.apply_K(x, e) for e in np.eye(ndof)])
+    A = K
+    for lvl in range(1, nlev):
+        parent, P = ctx.amg_transfer(lvl - 1)
+        Pm = _transfer_matrix(parent, P).toarray()
+        want = _unit_diag_rule(Pm.T @ A @ Pm)
+        got = _bsr_dense(*ctx.amg_operator(lvl))
+        scale = np.abs(want).max()
+        assert np.abs(got - got.T).max() <= 1e-13 * scale, "coarse operator not symmetric"
+        assert np.abs(got - want).max() <= 1e-12 * scale, lvl
+        A = got
+
+
+def test_transfer_columns_orthonormal():
+    """Tentative prolongator: per aggregate the 6 columns are orthonormal
+    (local QR of the rigid-body modes), zero rows at fixed DOFs."""
+    from uft.capi import cantilever
+    import uft
+    print("[case] transfer orthonormal", flush=True)
+    ctx = cantilever(8, 8, 4, 4, jitter=0.1, seed=3, Emax=1.0, coarse_max=6)
+    parent, P = ctx.amg_transfer(0)
+    fixed, _, _ = uft.box_bcs(8, 4, 4)
+    Pm = _transfer_matrix(parent, P).toarray()
+    assert np.all(Pm[fixed] == 0.0)
+    G = Pm.T @ Pm
+    d = np.diag(G)
+    assert np.all((np.abs(d - 1.0) < 1e-12) | (d == 0.0))
+    off = G - np.diag(d)
+    assert np.abs(off).max() < 1e-12
+
+
+@pytest.mark.parametrize("dims,coarse_max", [((8, 4, 4), 6), ((24, 12, 12), 600)])
+def test_coarse_inverse(dims, coarse_max):
+    """The blocked Gauss-Jordan inverse of the coarsest operator."""
+    from uft.capi import cantilever
+    print(f"[case] coarse inverse {dims}", flush=True)
+    nx, ny, nz = dims
+    ctx = cantilever(8, nx, ny, nz, jitter=0.15, seed=9, Emax=1.0, volfrac=0.3, coarse_max=coarse_max)
+    rng = np.random.default_rng(2)
+    ctx.set("XPhys", rng.uniform(0.01, 1.0, nx * ny * nz))
+    ctx.fem_obj_sens()
+    lvl = len(ctx.solver_info()[0]) - 1
+    A = _bsr_dense(*ctx.amg_operator(lvl))
+    inv = ctx.amg_coarse_inverse()
+    n = A.shape[0]
+    cond = np.linalg.cond(A)
+    print("coarsest n", n, "cond", cond)
+    tol = max(1e-12, 1e-14 * cond)  # backward-stable elimination: error ~ eps * cond
+    want = np.linalg.inv(A)
+    assert np.abs(inv - want).max() <= tol * np.abs(want).max()
+    assert np.abs(inv @ A - np.eye(n)).max() <= tol

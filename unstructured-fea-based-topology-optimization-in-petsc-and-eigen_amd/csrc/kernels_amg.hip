@@ -191,53 +191,81 @@ __device__ __forceinline__ void wave_sum(double (&v)[K]) {
   }
 }
 
-// level 0 -> 1: contribution c = e*npe*npe + la*npe + lb adds
-//   E(x_e) Q_{n(la)}^T KE_e[la-block, lb-block] Q_{n(lb)}   (Q: 3x6 per node)
+// level 0 -> 1, once per mesh: entry k (element e, upper block (A,B)) gets
+//   G_k = sum_{la: agg(n_la) = A} sum_{lb: agg(n_lb) = B} Q_{n_la}^T KE_e[la-block, lb-block] Q_{n_lb}
+// (Q: 3x6 per node, KE_e the density-free element matrix).  One thread per entry.
 template <int NPE>
-__global__ __launch_bounds__(TPB) void k_galerkin_fine(int64_t nblk_c, int ne, const int32_t *__restrict__ glptr,
-                                                       const int32_t *__restrict__ gl, const int32_t *__restrict__ elem,
-                                                       const double *__restrict__ KE, const double *__restrict__ xphys,
-                                                       SimpCoef sc, const double *__restrict__ Q,
-                                                       double *__restrict__ out) {
+__global__ __launch_bounds__(TPB) void k_galerkin_pre(int64_t nent, int ne, const int32_t *__restrict__ gblk,
+                                                      const int32_t *__restrict__ gl, const int32_t *__restrict__ brow,
+                                                      const int32_t *__restrict__ ci, const int32_t *__restrict__ parent,
+                                                      const int32_t *__restrict__ elem, const double *__restrict__ KE,
+                                                      const double *__restrict__ Q, double *__restrict__ G) {
   constexpr int ND = 3 * NPE;
-  const int64_t blk = ((int64_t)blockIdx.x * TPB + threadIdx.x) >> 6;
-  const int lane = threadIdx.x & 63;
-  if (blk >= nblk_c) return;  // whole wavefront exits together
+  const int64_t k = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  if (k >= nent) return;
+  const int e = gl[k], kb = gblk[k];
+  const int A = brow[kb], B = ci[kb];
   double acc[36];
 #pragma unroll
-  for (int k = 0; k < 36; ++k) acc[k] = 0.0;
-  const int k0 = glptr[blk], k1 = glptr[blk + 1];
-  for (int k = k0 + lane; k < k1; k += 64) {
-    const int c = gl[k];
-    const int e = c / (NPE * NPE);
-    const int rem = c - e * NPE * NPE;
-    const int la = rem / NPE, lb = rem - la * NPE;
-    const int na = elem[(size_t)la * ne + e], nb = elem[(size_t)lb * ne + e];
-    const double E = sc.Emin + pow(xphys[e], sc.penal) * sc.dE;
-    double Kb[3][3];
+  for (int q = 0; q < 36; ++q) acc[q] = 0.0;
+  for (int la = 0; la < NPE; ++la) {
+    const int na = elem[(size_t)la * ne + e];
+    if (parent[na] != A) continue;
+    const double *Qa = Q + (size_t)na * 18;
+    for (int lb = 0; lb < NPE; ++lb) {
+      const int nb = elem[(size_t)lb * ne + e];
+      if (parent[nb] != B) continue;
+      double Kb[3][3];
 #pragma unroll
-    for (int r = 0; r < 3; ++r)
+      for (int r = 0; r < 3; ++r)
 #pragma unroll
-      for (int s = 0; s < 3; ++s) {
-        const int p = pk_index(r * NPE + la, s * NPE + lb, ND);
-        Kb[r][s] = E * KE[((size_t)(p >> 1) * ne + e) * 2 + (p & 1)];
-      }
-    const double *Qa = Q + (size_t)na * 18, *Qb = Q + (size_t)nb * 18;
-    double W[3][6];  // Kb Qb
+        for (int c = 0; c < 3; ++c) {
+          const int p = pk_index(r * NPE + la, c * NPE + lb, ND);
+          Kb[r][c] = KE[((size_t)(p >> 1) * ne + e) * 2 + (p & 1)];
+        }
+      const double *Qb = Q + (size_t)nb * 18;
+      double W[3][6];  // Kb Qb
 #pragma unroll
-    for (int r = 0; r < 3; ++r)
+      for (int r = 0; r < 3; ++r)
 #pragma unroll
-      for (int j = 0; j < 6; ++j) W[r][j] = Kb[r][0] * Qb[j] + Kb[r][1] * Qb[6 + j] + Kb[r][2] * Qb[12 + j];
+        for (int j = 0; j < 6; ++j) W[r][j] = Kb[r][0] * Qb[j] + Kb[r][1] * Qb[6 + j] + Kb[r][2] * Qb[12 + j];
 #pragma unroll
-    for (int i = 0; i < 6; ++i)
+      for (int i = 0; i < 6; ++i)
 #pragma unroll
-      for (int j = 0; j < 6; ++j) acc[i * 6 + j] += Qa[i] * W[0][j] + Qa[6 + i] * W[1][j] + Qa[12 + i] * W[2][j];
+        for (int j = 0; j < 6; ++j) acc[i * 6 + j] += Qa[i] * W[0][j] + Qa[6 + i] * W[1][j] + Qa[12 + i] * W[2][j];
+    }
   }
-  wave_sum<36>(acc);
-  if (lane == 0) {
 #pragma unroll
-    for (int k = 0; k < 36; ++k) out[(size_t)blk * 36 + k] = acc[k];
+  for (int q = 0; q < 36; ++q) G[(size_t)k * 36 + q] = acc[q];
+}
+
+__global__ __launch_bounds__(TPB) void k_simp_E(int64_t ne, const double *__restrict__ x, SimpCoef sc,
+                                                double *__restrict__ Ee) {
+  const int64_t e = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  if (e < ne) Ee[e] = sc.Emin + pow(x[e], sc.penal) * sc.dE;
+}
+
+// level 0 -> 1 per density field: one thread per coarse block entry (kb, q);
+// a block below the diagonal reads its transposed block's entries transposed,
+// so off-diagonal blocks are exact transposes of each other.  Fixed entry order.
+__global__ __launch_bounds__(TPB) void k_galerkin_fine(int64_t nblk_c, const int32_t *__restrict__ glptr,
+                                                       const int32_t *__restrict__ gl, const int32_t *__restrict__ gtr,
+                                                       const int32_t *__restrict__ brow, const int32_t *__restrict__ ci,
+                                                       const double *__restrict__ Ee, const double *__restrict__ G,
+                                                       double *__restrict__ out) {
+  const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  if (t >= nblk_c * 36) return;
+  const int64_t kb = t / 36;
+  const int q = (int)(t - kb * 36);
+  int64_t src = kb;
+  int qs = q;
+  if (brow[kb] > ci[kb]) {
+    src = gtr[kb];
+    qs = (q % 6) * 6 + q / 6;
   }
+  double acc = 0.0;
+  for (int k = glptr[src]; k < glptr[src + 1]; ++k) acc += Ee[gl[k]] * G[(size_t)k * 36 + qs];
+  out[t] = acc;
 }
 
 // level l -> l+1 (l >= 1): contribution = block index kb of level l with row
@@ -301,81 +329,98 @@ __global__ __launch_bounds__(TPB) void k_bsr_to_dense(int64_t nrows, const int32
   }
 }
 
-// In-place Gauss-Jordan inversion of an SPD matrix (no pivoting needed): for
-// each pivot k the pivot row is scaled and every other row is eliminated in
-// parallel; the pivot column is staged in LDS first.  One workgroup.
-__global__ __launch_bounds__(1024) void k_dense_gj_inv(int n, double *__restrict__ A, double *__restrict__ Ainv,
-                                                       float *__restrict__ Ainvf) {
-  extern __shared__ double lds[];  // pivot column (n) and scaled pivot row (n)
-  double *colk = lds, *rowk = lds + n;
-  const int tid = threadIdx.x, nt = blockDim.x;
-  const int lane = tid & 63, wave = tid >> 6, nwave = nt >> 6;
-  for (int i = wave; i < n; i += nwave)
-    for (int j = lane; j < n; j += 64) Ainv[(size_t)i * n + j] = A[(size_t)i * n + j];
+// Blocked Gauss-Jordan inversion of an SPD matrix (no pivoting needed: every
+// pivot block is a Schur complement of an SPD matrix).  For each block K of
+// nb <= GJ_NB consecutive indices:
+//   Dinv = M[K,K]^-1;  R = Dinv M[K,:];  Cp = M[:,K]      (k_gj_diag, k_gj_panels)
+//   M[i,j] -= Cp[i,:] R[:,j]   (i, j not in K)              (k_gj_update)
+//   M[i,K]  = -Cp[i,:] Dinv,  M[K,j] = R[:,j],  M[K,K] = Dinv
+// after the last block M holds the inverse.  Three launches per block, all
+// entries updated in parallel, panels in a separate scratch area so that no
+// kernel reads what another workgroup of the same launch writes.
+constexpr int GJ_NB = 32;
+
+__global__ __launch_bounds__(TPB) void k_gj_diag(int n, int k0, int nb, const double *__restrict__ M,
+                                                 double *__restrict__ Dinv) {
+  __shared__ double D[GJ_NB * GJ_NB];
+  __shared__ double colk[GJ_NB], rowk[GJ_NB];
+  const int tid = threadIdx.x;
+  for (int t = tid; t < nb * nb; t += TPB) D[t] = M[(size_t)(k0 + t / nb) * n + k0 + t % nb];
   __syncthreads();
-  for (int k = 0; k < n; ++k) {
-    for (int i = tid; i < n; i += nt) colk[i] = Ainv[(size_t)i * n + k];
+  for (int k = 0; k < nb; ++k) {
+    if (tid < nb) colk[tid] = D[tid * nb + k];
     __syncthreads();
     double piv = colk[k];
     if (!(piv > 0.0)) piv = 1.0;  // decoupled zero row/column of a rank-deficient aggregate
     const double ip = 1.0 / piv;
-    for (int j = tid; j < n; j += nt) {
-      const double v = (j == k ? 1.0 : Ainv[(size_t)k * n + j]) * ip;
-      rowk[j] = v;
-      Ainv[(size_t)k * n + j] = v;
+    if (tid < nb) {
+      const double v = (tid == k ? 1.0 : D[k * nb + tid]) * ip;
+      rowk[tid] = v;
+      D[k * nb + tid] = v;
     }
     __syncthreads();
-    for (int i = wave; i < n; i += nwave) {
-      if (i == k) continue;
-      const double f = colk[i];
-      double *Ai = Ainv + (size_t)i * n;
-      for (int j = lane; j < n; j += 64) Ai[j] = (j == k ? 0.0 : Ai[j]) - f * rowk[j];
+    for (int t = tid; t < nb * nb; t += TPB) {
+      const int i = t / nb, j = t - (t / nb) * nb;
+      if (i != k) D[t] = (j == k ? 0.0 : D[t]) - colk[i] * rowk[j];
     }
     __syncthreads();
   }
-  if (Ainvf)
-    for (int64_t t = tid; t < (int64_t)n * n; t += nt) Ainvf[t] = (float)Ainv[t];
+  for (int t = tid; t < nb * nb; t += TPB) Dinv[t] = D[t];
 }
 
-// In-place Cholesky A = L L^T (lower triangle), then Ainv = L^-T L^-1.
-// One workgroup of 1024 threads; n is at most a few thousand.
-__global__ __launch_bounds__(1024) void k_dense_chol_inv(int n, double *__restrict__ A, double *__restrict__ W,
-                                                         double *__restrict__ Ainv, float *__restrict__ Ainvf) {
-  const int tid = threadIdx.x, nt = blockDim.x;
-  for (int k = 0; k < n; ++k) {
-    if (tid == 0) {
-      double akk = A[(size_t)k * n + k];
-      A[(size_t)k * n + k] = akk > 0.0 ? sqrt(akk) : 1.0;
-    }
-    __syncthreads();
-    const double lkk = A[(size_t)k * n + k];
-    for (int i = k + 1 + tid; i < n; i += nt) A[(size_t)i * n + k] /= lkk;
-    __syncthreads();
-    const int m = n - k - 1;
-    for (int64_t t = tid; t < (int64_t)m * m; t += nt) {
-      const int i = k + 1 + (int)(t / m), j = k + 1 + (int)(t % m);
-      if (j <= i) A[(size_t)i * n + j] -= A[(size_t)i * n + k] * A[(size_t)j * n + k];
-    }
-    __syncthreads();
-  }
-  // W = L^-1, column by column (each thread owns columns)
-  for (int c = tid; c < n; c += nt) {
-    for (int i = 0; i < n; ++i) {
-      double s = (i == c) ? 1.0 : 0.0;
-      for (int k = c; k < i; ++k) s -= A[(size_t)i * n + k] * W[(size_t)k * n + c];
-      W[(size_t)i * n + c] = (i < c) ? 0.0 : s / A[(size_t)i * n + i];
-    }
-  }
+// thread j: R[:, j] = Dinv M[K, j] (j not in K) and Cp[j, :] = M[j, K]
+__global__ __launch_bounds__(TPB) void k_gj_panels(int n, int k0, int nb, const double *__restrict__ M,
+                                                   const double *__restrict__ Dinv, double *__restrict__ R,
+                                                   double *__restrict__ Cp) {
+  __shared__ double Ds[GJ_NB * GJ_NB];
+  for (int t = threadIdx.x; t < nb * nb; t += TPB) Ds[t] = Dinv[t];
   __syncthreads();
-  // Ainv = W^T W
-  for (int64_t t = tid; t < (int64_t)n * n; t += nt) {
-    const int i = (int)(t / n), j = (int)(t % n);
-    const int k0 = i > j ? i : j;
-    double s = 0.0;
-    for (int k = k0; k < n; ++k) s += W[(size_t)k * n + i] * W[(size_t)k * n + j];
-    Ainv[t] = s;
-    if (Ainvf) Ainvf[t] = (float)s;
+  const int j = blockIdx.x * TPB + threadIdx.x;
+  if (j >= n) return;
+  for (int m = 0; m < nb; ++m) Cp[(size_t)j * nb + m] = M[(size_t)j * n + k0 + m];
+  if (j >= k0 && j < k0 + nb) return;
+  double v[GJ_NB];
+#pragma unroll
+  for (int m = 0; m < GJ_NB; ++m) v[m] = m < nb ? M[(size_t)(k0 + m) * n + j] : 0.0;
+  for (int l = 0; l < nb; ++l) {
+    double acc = 0.0;
+#pragma unroll
+    for (int m = 0; m < GJ_NB; ++m) acc += Ds[l * nb + (m < nb ? m : 0)] * v[m];
+    R[(size_t)l * n + j] = acc;
   }
+}
+
+// one row i per blockIdx.y, threads over columns j
+__global__ __launch_bounds__(TPB) void k_gj_update(int n, int k0, int nb, double *__restrict__ M,
+                                                   const double *__restrict__ Dinv, const double *__restrict__ R,
+                                                   const double *__restrict__ Cp) {
+  const int i = blockIdx.y;
+  const int j = blockIdx.x * TPB + threadIdx.x;
+  if (j >= n) return;
+  const bool iK = i >= k0 && i < k0 + nb, jK = j >= k0 && j < k0 + nb;
+  double *Mij = M + (size_t)i * n + j;
+  if (iK) {
+    *Mij = jK ? Dinv[(i - k0) * nb + (j - k0)] : R[(size_t)(i - k0) * n + j];
+    return;
+  }
+  const double *ci = Cp + (size_t)i * nb;
+  double acc = 0.0;
+  if (jK) {
+    for (int l = 0; l < nb; ++l) acc += ci[l] * Dinv[l * nb + (j - k0)];
+    *Mij = -acc;
+  } else {
+    for (int l = 0; l < nb; ++l) acc += ci[l] * R[(size_t)l * n + j];
+    *Mij -= acc;
+  }
+}
+
+__global__ __launch_bounds__(TPB) void k_copy_n(int64_t n, const double *__restrict__ a, double *__restrict__ b) {
+  const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  if (i < n) b[i] = a[i];
+}
+__global__ __launch_bounds__(TPB) void k_d2f_n(int64_t n, const double *__restrict__ a, float *__restrict__ b) {
+  const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  if (i < n) b[i] = (float)a[i];
 }
 
 template <typename T>
@@ -460,14 +505,19 @@ void prolong_addf(hipStream_t s, int bs, int64_t nu, const int32_t *parent, cons
   else
     k_prolong_add<6, float><<<nblk(nu * 6), TPB, 0, s>>>(nu, parent, P, xc, x);
 }
-void galerkin_fine(hipStream_t s, const uft_ctx *C, const double *xphys, int64_t nb, const int32_t *glptr,
-                   const int32_t *gl, const double *Q, double *val) {
-  SimpCoef sc{C->P.Emin, C->P.Emax - C->P.Emin, C->P.penal};
-  const unsigned g = (unsigned)((nb * 64 + TPB - 1) / TPB);
+void galerkin_pre(hipStream_t s, const uft_ctx *C, int64_t ngl, const int32_t *gblk, const int32_t *gl,
+                  const int32_t *brow, const int32_t *ci, const int32_t *parent, const double *Q, double *G) {
   if (C->npe == 8)
-    k_galerkin_fine<8><<<g, TPB, 0, s>>>(nb, (int)C->ne, glptr, gl, C->d_elem, C->d_KE, xphys, sc, Q, val);
+    k_galerkin_pre<8><<<nblk(ngl), TPB, 0, s>>>(ngl, (int)C->ne, gblk, gl, brow, ci, parent, C->d_elem, C->d_KE, Q, G);
   else
-    k_galerkin_fine<4><<<g, TPB, 0, s>>>(nb, (int)C->ne, glptr, gl, C->d_elem, C->d_KE, xphys, sc, Q, val);
+    k_galerkin_pre<4><<<nblk(ngl), TPB, 0, s>>>(ngl, (int)C->ne, gblk, gl, brow, ci, parent, C->d_elem, C->d_KE, Q, G);
+}
+void galerkin_fine(hipStream_t s, const uft_ctx *C, const double *xphys, int64_t nb, const int32_t *glptr,
+                   const int32_t *gl, const int32_t *gtr, const int32_t *brow, const int32_t *ci, const double *G,
+                   double *Ee, double *val) {
+  SimpCoef sc{C->P.Emin, C->P.Emax - C->P.Emin, C->P.penal};
+  k_simp_E<<<nblk(C->ne), TPB, 0, s>>>(C->ne, xphys, sc, Ee);
+  k_galerkin_fine<<<nblk(nb * 36), TPB, 0, s>>>(nb, glptr, gl, gtr, brow, ci, Ee, G, val);
 }
 void galerkin_coarse(hipStream_t s, int64_t nb, const int32_t *glptr, const int32_t *gl, const int32_t *brow,
                      const int32_t *ci, const double *P, const double *val, double *valc) {
@@ -478,12 +528,20 @@ void bsr_to_dense(hipStream_t s, int64_t nb_rows, const int32_t *rp, const int32
                   double *dense) {
   k_bsr_to_dense<<<nblk(nb_rows * 6), TPB, 0, s>>>(nb_rows, rp, ci, val, n, dense);
 }
+// A (2 n^2 doubles): A[0, n^2) holds the matrix on entry, the rest is scratch
+// for the panels; the inverse is written to Ainv (and to Ainvf when given).
 void dense_spd_inverse(hipStream_t s, int64_t n, double *A, double *Ainv, float *Ainvf) {
-  k_dense_gj_inv<<<1, 1024, 2 * (size_t)n * sizeof(double), s>>>((int)n, A, Ainv, Ainvf);
-}
-void dense_spd_inverse_chol(hipStream_t s, int64_t n, double *A, double *Ainv, float *Ainvf) {
-  // W (n x n) lives right after A in the same allocation
-  k_dense_chol_inv<<<1, 1024, 0, s>>>((int)n, A, A + n * n, Ainv, Ainvf);
+  const int nn = (int)n;
+  const int nb = std::max(1, std::min(GJ_NB, nn / 4));
+  k_copy_n<<<nblk(n * n), TPB, 0, s>>>(n * n, A, Ainv);
+  double *Dinv = A + n * n, *R = Dinv + nb * nb, *Cp = R + (size_t)nb * n;
+  for (int k0 = 0; k0 < nn; k0 += nb) {
+    const int b = std::min(nb, nn - k0);
+    k_gj_diag<<<1, TPB, 0, s>>>(nn, k0, b, Ainv, Dinv);
+    k_gj_panels<<<nblk(n), TPB, 0, s>>>(nn, k0, b, Ainv, Dinv, R, Cp);
+    k_gj_update<<<dim3(nblk(n), (unsigned)n), TPB, 0, s>>>(nn, k0, b, Ainv, Dinv, R, Cp);
+  }
+  if (Ainvf) k_d2f_n<<<nblk(n * n), TPB, 0, s>>>(n * n, Ainv, Ainvf);
 }
 void dense_mv(hipStream_t s, int64_t n, const double *A, const double *x, double *y) {
   k_dense_mv<double><<<(unsigned)((n * 64 + TPB - 1) / TPB), TPB, 0, s>>>(n, A, x, y);

@@ -161,7 +161,7 @@ int uft_amg_build_host(uft_ctx *C, std::vector<AmgHostLevel> &H, std::vector<int
   nunits.clear();
   const int npe = C->npe;
   const int64_t nn = C->nn, ne = C->ne;
-  if (ne * npe * npe >= (int64_t)INT32_MAX)
+  if (ne * npe * npe / 2 >= (int64_t)INT32_MAX)
     return uft_set_error(C, UFT_ERR_ARG, "mesh too large for 32-bit Galerkin lists (ne=%lld)", (long long)ne);
   std::vector<uint8_t> fixm(nn, 0);
   for (int32_t d : C->fixed) fixm[d / 3] |= (uint8_t)(1u << (d % 3));
@@ -231,24 +231,45 @@ int uft_amg_build_host(uft_ctx *C, std::vector<AmgHostLevel> &H, std::vector<int
     for (int64_t A = 0; A < na; ++A) h.ci.insert(h.ci.end(), rows[A].begin(), rows[A].end());
     rows.clear();
     rows.shrink_to_fit();
-    // contribution lists, element order
+    // Galerkin entries for level 1: one per (upper block (A,B), A <= B,
+    // element touching both), grouped by block, in element order within a
+    // block.  Entry k carries G_k = sum over the element's node pairs (la, lb)
+    // with agg(la) = A, agg(lb) = B of Q_la^T KE_e[la,lb] Q_lb (density free,
+    // built once on the device); a refresh is then sum_k E(x_e) G_k.
     const int64_t nbk = h.rp[na];
     h.glptr.assign(nbk + 1, 0);
-    std::vector<int32_t> bidx((size_t)ne * npe * npe);
-    for (int64_t e = 0; e < ne; ++e)
+    h.tr.assign(nbk, 0);
+    for (int64_t A = 0; A < na; ++A)
+      for (int32_t k = h.rp[A]; k < h.rp[A + 1]; ++k) h.tr[k] = find_col(h.rp, h.ci, h.ci[k], (int32_t)A);
+    std::vector<int32_t> eblk;  // (element, upper block) pairs in element order
+    std::vector<int32_t> eent;
+    std::vector<int32_t> loc;
+    for (int64_t e = 0; e < ne; ++e) {
+      loc.clear();
       for (int la = 0; la < npe; ++la) {
         const int32_t A = agg[C->elem[e * npe + la]];
         for (int lb = 0; lb < npe; ++lb) {
           const int32_t Bc = agg[C->elem[e * npe + lb]];
-          const int32_t kb = find_col(h.rp, h.ci, A, Bc);
-          bidx[(e * npe + la) * npe + lb] = kb;
-          h.glptr[kb + 1]++;
+          if (A <= Bc) loc.push_back(find_col(h.rp, h.ci, A, Bc));
         }
       }
+      std::sort(loc.begin(), loc.end());
+      loc.erase(std::unique(loc.begin(), loc.end()), loc.end());
+      for (int32_t kb : loc) {
+        eblk.push_back(kb);
+        eent.push_back((int32_t)e);
+        h.glptr[kb + 1]++;
+      }
+    }
     for (int64_t k = 0; k < nbk; ++k) h.glptr[k + 1] += h.glptr[k];
-    h.gl.assign(h.glptr[nbk], 0);
+    h.gl.assign(eblk.size(), 0);
+    h.gblk.assign(eblk.size(), 0);
     std::vector<int32_t> pos(h.glptr.begin(), h.glptr.end() - 1);
-    for (int64_t c = 0; c < ne * npe * npe; ++c) h.gl[pos[bidx[c]]++] = (int32_t)c;
+    for (size_t q = 0; q < eblk.size(); ++q) {
+      const int32_t k = pos[eblk[q]]++;
+      h.gl[k] = eent[q];
+      h.gblk[k] = eblk[q];
+    }
     H.push_back(std::move(h));
     nunits.push_back(na);
     // ---- levels >= 1 (Rn is the coarse near-null space of level 1)

@@ -362,7 +362,8 @@ int amg_refresh(uft_ctx *C, const double *xphys) {
   for (int l = 0; l + 1 < H.nlev; ++l) {
     AmgLevel &Lv = H.L[l], &Ln = H.L[l + 1];
     if (l == 0)
-      uftk::galerkin_fine(s, C, xphys, Ln.nblk, Lv.d_glptr, Lv.d_gl, Lv.d_P, Ln.d_val);
+      uftk::galerkin_fine(s, C, xphys, Ln.nblk, Lv.d_glptr, Lv.d_gl, Lv.d_gtr, Ln.d_brow, Ln.d_ci, Lv.d_G, C->d_Ee,
+                          Ln.d_val);
     else
       uftk::galerkin_coarse(s, Ln.nblk, Lv.d_glptr, Lv.d_gl, Lv.d_brow, Lv.d_ci, Lv.d_P, Lv.d_val, Ln.d_val);
     uftk::fixed_unit_diag(s, Ln.nunit, Ln.d_rp, Ln.d_ci, Ln.d_val);
@@ -830,12 +831,21 @@ int uft_setup(uft_ctx *C) {
         for (int32_t k = h.rp[A2]; k < h.rp[A2 + 1]; ++k) brow[k] = (int32_t)A2;
       if ((rc = upload(C, &Ln.d_brow, brow))) return rc;
       if ((rc = dalloc_n(C, &Ln.d_val, Ln.nblk * 36))) return rc;
+      if (l == 0) {  // density-free Galerkin entries of the fine level
+        int32_t *d_gblk;
+        if ((rc = upload(C, &Lv.d_gtr, h.tr))) return rc;
+        if ((rc = upload(C, &d_gblk, h.gblk))) return rc;
+        if ((rc = dalloc_n(C, &Lv.d_G, Lv.ngl * 36))) return rc;
+        if ((rc = dalloc_n(C, &C->d_Ee, C->ne))) return rc;
+        uftk::galerkin_pre(s, C, Lv.ngl, d_gblk, Lv.d_gl, Ln.d_brow, Ln.d_ci, Lv.d_parent, Lv.d_P, Lv.d_G);
+      }
       if (C->P.precision == UFT_MIXED)
         if ((rc = dalloc_n(C, &Ln.d_valf, Ln.nblk * 36))) return rc;
       UFT_HIP(C, hipStreamSynchronize(s));  // host vectors of H go out of scope level by level
     }
     A.ncoarse = A.L[A.nlev - 1].n;
-    if ((rc = dalloc_n(C, &A.d_cmat, 2 * A.ncoarse * A.ncoarse))) return rc;
+    // matrix plus the inverse's panel scratch (at most n^2/16 + n^2/2 + 2n + 1 doubles)
+    if ((rc = dalloc_n(C, &A.d_cmat, 2 * A.ncoarse * A.ncoarse + 2 * A.ncoarse + 1))) return rc;
     if ((rc = dalloc_n(C, &A.d_cinv, A.ncoarse * A.ncoarse))) return rc;
     if (C->P.precision == UFT_MIXED)
       if ((rc = dalloc_n(C, &A.d_cinvf, A.ncoarse * A.ncoarse))) return rc;
@@ -1299,6 +1309,34 @@ extern "C" int uft_solver_info(const uft_ctx *C, int *nlev, int64_t *n_per_level
     if (blocks_per_level) blocks_per_level[l] = l == 0 ? C->ne : C->amg.L[l].nblk;
   }
   return UFT_OK;
+}
+
+extern "C" int uft_amg_operator_get(uft_ctx *C, int level, int32_t *rp, int32_t *ci, double *val) {
+  if (int rc = check_setup(C)) return rc;
+  if (level < 1 || level >= C->amg.nlev) return uft_set_error(C, UFT_ERR_ARG, "level %d out of range", level);
+  const AmgLevel &Lv = C->amg.L[level];
+  if (rp) UFT_HIP(C, hipMemcpyAsync(rp, Lv.d_rp, (Lv.nunit + 1) * sizeof(int32_t), hipMemcpyDeviceToHost, C->s));
+  if (ci) UFT_HIP(C, hipMemcpyAsync(ci, Lv.d_ci, Lv.nblk * sizeof(int32_t), hipMemcpyDeviceToHost, C->s));
+  if (val) UFT_HIP(C, hipMemcpyAsync(val, Lv.d_val, Lv.nblk * 36 * sizeof(double), hipMemcpyDeviceToHost, C->s));
+  return sync(C);
+}
+
+extern "C" int uft_amg_transfer_get(uft_ctx *C, int level, int32_t *parent, double *P) {
+  if (int rc = check_setup(C)) return rc;
+  if (level < 0 || level + 1 >= C->amg.nlev) return uft_set_error(C, UFT_ERR_ARG, "level %d out of range", level);
+  const AmgLevel &Lv = C->amg.L[level];
+  if (parent)
+    UFT_HIP(C, hipMemcpyAsync(parent, Lv.d_parent, Lv.nunit * sizeof(int32_t), hipMemcpyDeviceToHost, C->s));
+  if (P) UFT_HIP(C, hipMemcpyAsync(P, Lv.d_P, Lv.nunit * Lv.bs * 6 * sizeof(double), hipMemcpyDeviceToHost, C->s));
+  return sync(C);
+}
+
+extern "C" int uft_amg_coarse_inverse_get(uft_ctx *C, double *inv) {
+  if (int rc = check_setup(C)) return rc;
+  if (!inv) return UFT_ERR_ARG;
+  const int64_t n = C->amg.ncoarse;
+  UFT_HIP(C, hipMemcpyAsync(inv, C->amg.d_cinv, n * n * sizeof(double), hipMemcpyDeviceToHost, C->s));
+  return sync(C);
 }
 
 extern "C" int uft_phase_times(const uft_ctx *C, double *ms) {

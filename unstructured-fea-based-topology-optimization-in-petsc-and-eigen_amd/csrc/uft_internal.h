@@ -43,7 +43,9 @@ struct AmgLevel {
   // deterministic Galerkin contribution lists for the next level's blocks
   int64_t ngl = 0;
   int32_t *d_glptr = nullptr;  // per next-level block
-  int32_t *d_gl = nullptr;     // level 0: e*npe*npe + la*npe + lb ; else block index
+  int32_t *d_gl = nullptr;     // level 0: element of each entry ; else block index
+  int32_t *d_gtr = nullptr;    // level 0: block -> its transposed block
+  double *d_G = nullptr;       // level 0: 6x6 density-free entry matrices (ngl * 36)
   // work vectors (length n)
   double *x = nullptr, *b = nullptr, *r = nullptr, *d = nullptr, *t = nullptr;
   float *xf = nullptr, *bf = nullptr, *rf = nullptr, *df = nullptr, *tf = nullptr;
@@ -70,8 +72,9 @@ struct AmgHostLevel {
   std::vector<int32_t> cptr, child;  // next-level unit -> children of this level
   // pattern of the next level operator
   std::vector<int32_t> rp, ci;
-  // Galerkin lists for the next level blocks
-  std::vector<int32_t> glptr, gl;
+  // Galerkin lists for the next level blocks (level 0: per upper block, the
+  // elements of its entries, the entries' block and the transposed block)
+  std::vector<int32_t> glptr, gl, gblk, tr;
 };
 
 // --------------------------------------------------------------------------
@@ -107,6 +110,7 @@ struct uft_ctx {
   int32_t *d_iptr = nullptr, *d_inc = nullptr;  // node -> (e*npe + local)
   double *d_F = nullptr;
   double *d_Ye = nullptr;      // element-local scratch nd x ne
+  double *d_Ee = nullptr;      // per element E(x_e) of the last hierarchy refresh
   float *d_Yef = nullptr;
   double TV = 0.0;
 
@@ -246,8 +250,12 @@ void restrictf(hipStream_t s, int bs, int64_t ncoarse_units, const int32_t *cptr
                const float *P, const float *r, float *bc);
 void prolong_addf(hipStream_t s, int bs, int64_t nunits, const int32_t *parent, const float *P,
                   const float *xc, float *x);
+// level 0 -> 1: G (ngl x 36) once per mesh; then val = sum_k E(x_e(k)) G_k per block
+void galerkin_pre(hipStream_t s, const uft_ctx *C, int64_t ngl, const int32_t *gblk, const int32_t *gl,
+                  const int32_t *brow, const int32_t *ci, const int32_t *parent, const double *Q, double *G);
 void galerkin_fine(hipStream_t s, const uft_ctx *C, const double *xphys, int64_t nblk, const int32_t *glptr,
-                   const int32_t *gl, const double *Pq, double *val);
+                   const int32_t *gl, const int32_t *gtr, const int32_t *brow, const int32_t *ci,
+                   const double *G, double *Ee, double *val);
 void galerkin_coarse(hipStream_t s, int64_t nblk, const int32_t *glptr, const int32_t *gl,
                      const int32_t *brow, const int32_t *ci, const double *P, const double *val,
                      double *valc);

@@ -98,6 +98,12 @@ def lib():
         "uft_box_elems": (C.c_int64, [C.c_int] * 4),
         "uft_gen_box": (C.c_int, [C.c_int] * 4 + [C.c_double] * 4 + [C.c_uint64, dp, ip]),
         "uft_box_bcs": (C.c_int, [C.c_int] * 4 + [ip, pi, ip, dp, pi]),
+        "uft_solver_info": (C.c_int, [vp, pi, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+        "uft_amg_operator_get": (C.c_int, [vp, C.c_int, ip, ip, dp]),
+        "uft_amg_transfer_get": (C.c_int, [vp, C.c_int, ip, dp]),
+        "uft_amg_coarse_inverse_get": (C.c_int, [vp, dp]),
+        "uft_phase_times": (C.c_int, [vp, pd]),
+        "uft_autotune": (C.c_int, [vp, C.c_int, pi, pd, C.c_int]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -323,6 +329,46 @@ class Context:
 
     def sync(self):
         self._chk(self.L.uft_sync(self.h), "uft_sync")
+
+    # ---- solver inspection
+    def solver_info(self):
+        """(unknowns per level, element count / 6x6 block count per level)."""
+        nl = C.c_int()
+        n = (C.c_int64 * 8)()
+        b = (C.c_int64 * 8)()
+        self._chk(self.L.uft_solver_info(self.h, C.byref(nl), n, b), "uft_solver_info")
+        return [n[i] for i in range(nl.value)], [b[i] for i in range(nl.value)]
+
+    def amg_operator(self, level):
+        """BSR (rp, ci, val[nblk, 6, 6]) of multigrid level >= 1."""
+        n, b = self.solver_info()
+        nu = n[level] // 6
+        rp = np.zeros(nu + 1, np.int32)
+        ci = np.zeros(b[level], np.int32)
+        val = np.zeros(b[level] * 36)
+        self._chk(self.L.uft_amg_operator_get(self.h, level, rp, ci, val), "uft_amg_operator_get")
+        return rp, ci, val.reshape(-1, 6, 6)
+
+    def amg_transfer(self, level):
+        """(parent, P[units, bs, 6]) of the transfer from level to level + 1."""
+        n, _ = self.solver_info()
+        bs = 3 if level == 0 else 6
+        nu = n[level] // bs
+        parent = np.zeros(nu, np.int32)
+        P = np.zeros(nu * bs * 6)
+        self._chk(self.L.uft_amg_transfer_get(self.h, level, parent, P), "uft_amg_transfer_get")
+        return parent, P.reshape(nu, bs, 6)
+
+    def amg_coarse_inverse(self):
+        n = self.solver_info()[0][-1]
+        inv = np.zeros(n * n)
+        self._chk(self.L.uft_amg_coarse_inverse_get(self.h, inv), "uft_amg_coarse_inverse_get")
+        return inv.reshape(n, n)
+
+    def phase_times(self):
+        ms = (C.c_double * 8)()
+        self._chk(self.L.uft_phase_times(self.h, ms), "uft_phase_times")
+        return [ms[i] for i in range(7)]
 
 
 def cantilever(npe=8, nx=8, ny=4, nz=4, jitter=0.0, seed=0, load_kind=0, device=0, **params):
