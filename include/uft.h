@@ -46,7 +46,12 @@ enum uft_vec_id {
 
 enum uft_size_id { UFT_SIZE_NE = 0, UFT_SIZE_NN = 1, UFT_SIZE_NDOF = 2, UFT_SIZE_NPE = 3 };
 
-enum uft_precision { UFT_FP64 = 0, UFT_MIXED = 1 };
+/* UFT_FP64: every operator product in FP64.  UFT_MIXED: FP64 flexible outer
+ * Krylov loop (residual, inner products, iterate), multigrid preconditioner
+ * with FP32 element blocks and coarse operators, FP64 coarsest solve.
+ * UFT_MIXED_H: as UFT_MIXED with the smoother's fine element blocks stored
+ * in FP16 (scaled per element, FP32 arithmetic). */
+enum uft_precision { UFT_FP64 = 0, UFT_MIXED = 1, UFT_MIXED_H = 2 };
 
 /* Parameters.  Defaults (uft_default_params) are the reference's
  * (OptPara.cpp:66-78, SolveFem.cpp:8-11,134, PDEFilter.cpp:117-120). */

@@ -1,5 +1,6 @@
-"""GPU: the mixed-precision form (FP32 element blocks and V-cycle inside an
-FP64 flexible outer loop) against the FP64 oracle and the FP64 GPU path.
+"""GPU: the mixed-precision forms (FP32 or FP16 element blocks and an FP32
+multigrid cycle inside an FP64 flexible outer loop) against the FP64 oracle
+and the FP64 GPU path.
 relative compliance error <= 1e-6 against the FP64 oracle."""
 import numpy as np
 import pytest
@@ -7,12 +8,14 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-def test_mixed_loop_vs_oracle(O):
+@pytest.mark.parametrize("precision", [1, 2])
+def test_mixed_loop_vs_oracle(O, precision):
+    """precision 1: FP32 preconditioner blocks; 2: FP16 smoother blocks."""
     from uft.capi import cantilever
     args = dict(nx=10, ny=5, nz=5, jitter=0.15, seed=13)
     prm = dict(Emax=1.0, volfrac=0.3, rmin=0.3, rtol_el=1e-12, rtol_f=1e-12)
-    print("[case] mixed hex8 10x5x5 vs oracle", flush=True)
-    ctx = cantilever(8, **args, precision=1, **prm)
+    print(f"[case] mixed precision={precision} hex8 10x5x5 vs oracle", flush=True)
+    ctx = cantilever(8, **args, precision=precision, **prm)
     ref = O.cantilever(8, **args, **prm)
     ctx.init_design()
     ref.init_design()

@@ -418,6 +418,17 @@ __global__ __launch_bounds__(TPB) void k_copy_n(int64_t n, const double *__restr
   const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
   if (i < n) b[i] = a[i];
 }
+// S = (S + S^T) / 2 in place: the elimination's rounding leaves the inverse
+// of a symmetric matrix only nearly symmetric; exact symmetry lets the solve
+// read it by columns.  One thread per (i < j) pair.
+__global__ __launch_bounds__(TPB) void k_symmetrize(int64_t n, double *__restrict__ S) {
+  const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  const int64_t i = t / n, j = t - i * n;
+  if (i >= n || j <= i) return;
+  const double v = 0.5 * (S[i * n + j] + S[j * n + i]);
+  S[i * n + j] = v;
+  S[j * n + i] = v;
+}
 __global__ __launch_bounds__(TPB) void k_d2f_n(int64_t n, const double *__restrict__ a, float *__restrict__ b) {
   const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
   if (i < n) b[i] = (float)a[i];
@@ -541,6 +552,7 @@ void dense_spd_inverse(hipStream_t s, int64_t n, double *A, double *Ainv, float 
     k_gj_panels<<<nblk(n), TPB, 0, s>>>(nn, k0, b, Ainv, Dinv, R, Cp);
     k_gj_update<<<dim3(nblk(n), (unsigned)n), TPB, 0, s>>>(nn, k0, b, Ainv, Dinv, R, Cp);
   }
+  k_symmetrize<<<nblk(n * n), TPB, 0, s>>>(n, Ainv);
   if (Ainvf) k_d2f_n<<<nblk(n * n), TPB, 0, s>>>(n * n, Ainv, Ainvf);
 }
 void dense_mv(hipStream_t s, int64_t n, const double *A, const double *x, double *y) {

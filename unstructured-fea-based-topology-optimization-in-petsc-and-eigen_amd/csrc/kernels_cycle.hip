@@ -1,0 +1,395 @@
+// gfx950 kernels of the fused multigrid cycle.  Every operator apply of the
+// cycle (the matrix-free fine operator's node phase, and the 6x6 block-sparse
+// coarse operators) finishes with a per-row epilogue that performs the vector
+// update the cycle needs next (Chebyshev step, residual, first smoothing step,
+// K-cycle inner products), so a smoothing step is one launch instead of two.
+// Restriction also performs the first Chebyshev step of the next level,
+// prolongation forms the K-cycle's linear combination on the fly, and the
+// last restriction and the dense coarsest solve share one launch.  Row-local epilogues never write the vector that the
+// same launch multiplies, and every reduction keeps a fixed order.
+#include "uft_internal.h"
+#include "uft_reduce.h"
+
+namespace {
+
+constexpr int TPB = 256;
+inline unsigned nblk(int64_t n) { return (unsigned)((n + TPB - 1) / TPB); }
+
+// t = (A u)_i ; ui = u_i
+template <typename T>
+__device__ __forceinline__ void row_epi(const RowOp<T> &op, int64_t i, T t, T ui, double (&dv)[3]) {
+  switch (op.mode) {
+    case ROW_STORE:
+      op.y[i] = t;
+      break;
+    case ROW_STEP: {  // r -= t ; d' = c1 u + c2 D^-1 r ; x += d' (+ u)
+      const T ri = op.r[i] - t;
+      op.r[i] = ri;
+      const T dn = (T)op.c1 * ui + (T)op.c2 * op.dinv[i] * ri;
+      op.dout[i] = dn;
+      op.x[i] += op.addprev ? dn + ui : dn;
+      break;
+    }
+    case ROW_RES:  // r = base - t
+      op.r[i] = op.base[i] - t;
+      break;
+    case ROW_RESFIRST: {  // r = base - t ; d = it D^-1 r
+      const T ri = op.base[i] - t;
+      op.r[i] = ri;
+      op.dout[i] = (T)op.it * op.dinv[i] * ri;
+      break;
+    }
+    case ROW_RESFIRSTX: {  // kr = r = base - t ; d = kd = it D^-1 r
+      const T ri = op.base[i] - t;
+      op.r[i] = ri;
+      op.kr[i] = ri;
+      const T dn = (T)op.it * op.dinv[i] * ri;
+      op.dout[i] = dn;
+      op.kd[i] = dn;
+      break;
+    }
+    case ROW_DOTS2:  // y = t ; u.t, u.w1
+      op.y[i] = t;
+      dv[0] += (double)ui * (double)t;
+      dv[1] += (double)ui * (double)op.w1[i];
+      break;
+    case ROW_DOTS3:  // u.w1, u.t, u.w2
+      dv[0] += (double)ui * (double)op.w1[i];
+      dv[1] += (double)ui * (double)t;
+      dv[2] += (double)ui * (double)op.w2[i];
+      break;
+    default:
+      break;
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void epi_partials(const RowOp<T> &op, double (&dv)[3]) {
+  if (op.mode == ROW_DOTS2) {
+    double v2[2] = {dv[0], dv[1]};
+    uftr::block_partials<2>(v2, op.part);
+  } else if (op.mode == ROW_DOTS3) {
+    uftr::block_partials<3>(dv, op.part);
+  }
+}
+
+// ---------------------------------------------------------------- 6x6 BSR
+// LPR lanes per scalar row, fixed shuffle-tree combine: 8 on large levels,
+// 32 on small ones so that the launch still spreads over the chip.
+inline int lpr_for(int64_t nb_rows) { return nb_rows * 6 < 65536 ? 32 : 8; }
+template <typename T, int LPR>
+__global__ __launch_bounds__(TPB) void k_bsr6_op(int64_t nrows, const int32_t *__restrict__ rp,
+                                                 const int32_t *__restrict__ ci, const T *__restrict__ val,
+                                                 const T *__restrict__ u, RowOp<T> op) {
+  const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  const int64_t row = t / LPR;
+  const int l = (int)(t - row * LPR);
+  T acc = T(0);
+  if (row < nrows * 6) {
+    const int64_t A = row / 6;
+    const int i = (int)(row - A * 6);
+    for (int k = rp[A] + l; k < rp[A + 1]; k += LPR) {
+      const int64_t B = ci[k];
+      const T *v = val + (size_t)k * 36 + i * 6;
+      const T *xb = u + B * 6;
+#pragma unroll
+      for (int j = 0; j < 6; ++j) acc += v[j] * xb[j];
+    }
+  }
+#pragma unroll
+  for (int off = LPR / 2; off > 0; off >>= 1) acc += __shfl_down(acc, off, LPR);
+  double dv[3] = {0.0, 0.0, 0.0};
+  if (l == 0 && row < nrows * 6) row_epi(op, row, acc, u[row], dv);
+  epi_partials(op, dv);
+}
+
+// ---------------------------------------------------------------- fine node phase
+// y_n = sum of the element-local results of node n (fixed incidence order);
+// fixed DOFs have the unit diagonal row (y = u).
+template <int NPE, typename T>
+__global__ __launch_bounds__(TPB) void k_node_gather_op(int nn, int ne, const int32_t *__restrict__ iptr,
+                                                        const int32_t *__restrict__ inc, const T *__restrict__ Ye,
+                                                        const uint8_t *__restrict__ fixm, const T *__restrict__ u,
+                                                        RowOp<T> op) {
+  const int n = blockIdx.x * TPB + threadIdx.x;
+  double dv[3] = {0.0, 0.0, 0.0};
+  if (n < nn) {
+    T acc[3] = {T(0), T(0), T(0)};
+    const int k0 = iptr[n], k1 = iptr[n + 1];
+    for (int k = k0; k < k1; ++k) {
+      const int v = inc[k];
+      const int e = v / NPE, l = v - e * NPE;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) acc[c] += Ye[(size_t)(c * NPE + l) * ne + e];
+    }
+    const unsigned m = fixm[n];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const int64_t i = 3 * (int64_t)n + c;
+      const T ui = u[i];
+      row_epi(op, i, ((m >> c) & 1u) ? ui : acc[c], ui, dv);
+    }
+  }
+  epi_partials(op, dv);
+}
+
+// ---------------------------------------------------------------- transfers
+// one wavefront per coarse unit A: bc_A = sum_children P_u^T r_u (fixed
+// order); with dinv given, also the next level's first Chebyshev step from a
+// zero guess: rc = bc, dc = it D^-1 bc, xc = dc.
+template <int BS, typename T>
+__global__ __launch_bounds__(TPB) void k_restrict_first(int64_t nc, const int32_t *__restrict__ cptr,
+                                                        const int32_t *__restrict__ child, const T *__restrict__ P,
+                                                        const T *__restrict__ r, T *__restrict__ bc,
+                                                        const T *__restrict__ dinv, double it, T *__restrict__ rc,
+                                                        T *__restrict__ dc, T *__restrict__ xc) {
+  const int64_t A = ((int64_t)blockIdx.x * TPB + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (A >= nc) return;  // whole wavefront
+  const int k0 = cptr[A], npair = (cptr[A + 1] - k0) * BS;
+  T acc[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
+  for (int q = lane; q < npair; q += 64) {
+    const int kk = q / BS, i = q - kk * BS;
+    const int64_t u = child[k0 + kk];
+    const T *Pu = P + (size_t)u * BS * 6 + i * 6;
+    const T ri = r[u * BS + i];
+#pragma unroll
+    for (int j = 0; j < 6; ++j) acc[j] += Pu[j] * ri;
+  }
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) acc[j] += __shfl_down(acc[j], off, 64);
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+      const int64_t i = A * 6 + j;
+      bc[i] = acc[j];
+      if (dinv) {
+        const T d = (T)it * dinv[i] * acc[j];
+        rc[i] = acc[j];
+        dc[i] = d;
+        xc[i] = d;
+      }
+    }
+  }
+}
+
+// x_u += P_u (ca xa + cb xb)_{parent(u)}: ca, cb from the device scalars
+// sc[ia], sc[ib] (index < 0: 1.0); xb may be null.
+template <int BS, typename T>
+__global__ __launch_bounds__(TPB) void k_prolong_comb(int64_t nu, const int32_t *__restrict__ parent,
+                                                      const T *__restrict__ P, const double *__restrict__ sc, int ia,
+                                                      int ib, const T *__restrict__ xa, const T *__restrict__ xb,
+                                                      T *__restrict__ x) {
+  const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  if (t >= nu * BS) return;
+  const int64_t u = t / BS;
+  const int i = (int)(t - u * BS);
+  const int64_t A = parent[u];
+  if (A < 0) return;
+  const T ca = ia < 0 ? T(1) : (T)sc[ia];
+  const T cb = ib < 0 ? T(1) : (T)sc[ib];
+  const T *Pu = P + (size_t)u * BS * 6 + i * 6;
+  T acc = T(0);
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+    const T c = xb ? ca * xa[A * 6 + j] + cb * xb[A * 6 + j] : xa[A * 6 + j];
+    acc += Pu[j] * c;
+  }
+  x[t] += acc;
+}
+
+// K-cycle second start: kr = b + sc[7] v ; r = kr ; d = it D^-1 kr ; kd = d
+template <typename T>
+__global__ __launch_bounds__(TPB) void k_lincomb_first(int64_t n, const double *__restrict__ sc,
+                                                       const T *__restrict__ b, const T *__restrict__ v,
+                                                       T *__restrict__ kr, T *__restrict__ r, const T *__restrict__ dinv,
+                                                       double it, T *__restrict__ d, T *__restrict__ kd) {
+  const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  if (i >= n) return;
+  const T c = (T)sc[7];
+  const T ri = b[i] + c * v[i];
+  kr[i] = ri;
+  r[i] = ri;
+  const T di = (T)it * dinv[i] * ri;
+  d[i] = di;
+  kd[i] = di;
+}
+
+// fixed-order sum of K partial columns by one block, then the K-cycle
+// coefficients (Notay & Vassilevski 2008, Alg. 2.1 with two inner steps):
+//  which 1: sc[0] = rho1 = c.v, sc[1] = alpha1 = c.b, sc[7] = -alpha1/rho1
+//  which 2: sc[2] = gamma = d.v, sc[3] = beta = d.Ad, sc[4] = alpha2 = d.r,
+//           sc[5], sc[6] = coefficients of c and d in the result
+template <int K>
+__global__ __launch_bounds__(TPB) void k_final_kc(const double *__restrict__ part, int nb, double *__restrict__ sc) {
+  double res[K];
+  uftr::block_final<K>(part, nb, res);
+  if (threadIdx.x != 0) return;
+  if constexpr (K == 2) {
+    sc[0] = res[0];
+    sc[1] = res[1];
+    sc[7] = res[0] != 0.0 ? -res[1] / res[0] : 0.0;
+  } else {
+    const double rho1 = sc[0], a1 = sc[1], g = res[0], be = res[1], a2 = res[2];
+    sc[2] = g;
+    sc[3] = be;
+    sc[4] = a2;
+    const double t1 = rho1 != 0.0 ? a1 / rho1 : 0.0;
+    const double rho2 = rho1 != 0.0 ? be - g * g / rho1 : 0.0;
+    if (rho2 > 0.0) {
+      sc[5] = t1 - g * a2 / (rho1 * rho2);
+      sc[6] = a2 / rho2;
+    } else {
+      sc[5] = t1;
+      sc[6] = 0.0;
+    }
+  }
+}
+
+// The last restriction and the coarsest solve in one workgroup (upper level
+// with 6 unknowns per unit): bc = P^T r with one wavefront per coarsest unit
+// (fixed order), then xc = Ainv bc in FP64 with the exactly symmetric inverse
+// read by columns (loads coalesced over rows; four column groups added in a
+// fixed order).  The prolongation x += P xc follows as an ordinary launch.
+constexpr int TAIL_TPB = 1024;
+template <typename T>
+__global__ __launch_bounds__(TAIL_TPB) void k_tail_solve(int nc_units, const int32_t *__restrict__ cptr,
+                                                         const int32_t *__restrict__ child, const T *__restrict__ P,
+                                                         const double *__restrict__ Ainv, const T *__restrict__ r,
+                                                         T *__restrict__ xc) {
+  __shared__ double bc[UFT_TAIL_MAX];
+  __shared__ double part4[4][TAIL_TPB / 4];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = TAIL_TPB / 64;
+  const int n = 6 * nc_units;
+  for (int A = w; A < nc_units; A += nw) {
+    const int k0 = cptr[A], npair = (cptr[A + 1] - k0) * 6;
+    double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int q = lane; q < npair; q += 64) {
+      const int kk = q / 6, i = q - kk * 6;
+      const int64_t u = child[k0 + kk];
+      const T *Pu = P + (size_t)u * 36 + i * 6;
+      const double ri = (double)r[u * 6 + i];
+#pragma unroll
+      for (int j = 0; j < 6; ++j) acc[j] += (double)Pu[j] * ri;
+    }
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) acc[j] += __shfl_down(acc[j], off, 64);
+    }
+    if (lane == 0) {
+#pragma unroll
+      for (int j = 0; j < 6; ++j) bc[A * 6 + j] = acc[j];
+    }
+  }
+  __syncthreads();
+  const int g = threadIdx.x / (TAIL_TPB / 4), row0 = threadIdx.x % (TAIL_TPB / 4);
+  for (int rb = 0; rb < n; rb += TAIL_TPB / 4) {
+    const int row = rb + row0;
+    double acc = 0.0;
+    if (row < n)
+      for (int c = g; c < n; c += 4) acc += Ainv[(size_t)c * n + row] * bc[c];
+    part4[g][row0] = acc;
+    __syncthreads();
+    if (g == 0 && row < n) xc[row] = (T)((part4[0][row0] + part4[1][row0]) + (part4[2][row0] + part4[3][row0]));
+    __syncthreads();
+  }
+}
+
+}  // namespace
+
+namespace uftk {
+
+template <typename T>
+static void bsr6_op_t(hipStream_t s, int64_t nb_rows, const int32_t *rp, const int32_t *ci, const T *val, const T *u,
+                      const RowOp<T> &op) {
+  if (lpr_for(nb_rows) == 32)
+    k_bsr6_op<T, 32><<<nblk(nb_rows * 6 * 32), TPB, 0, s>>>(nb_rows, rp, ci, val, u, op);
+  else
+    k_bsr6_op<T, 8><<<nblk(nb_rows * 6 * 8), TPB, 0, s>>>(nb_rows, rp, ci, val, u, op);
+}
+void bsr6_op(hipStream_t s, int64_t nb_rows, const int32_t *rp, const int32_t *ci, const double *val, const double *u,
+             const RowOp<double> &op) {
+  bsr6_op_t<double>(s, nb_rows, rp, ci, val, u, op);
+}
+void bsr6_op(hipStream_t s, int64_t nb_rows, const int32_t *rp, const int32_t *ci, const float *val, const float *u,
+             const RowOp<float> &op) {
+  bsr6_op_t<float>(s, nb_rows, rp, ci, val, u, op);
+}
+int bsr6_op_blocks(int64_t nb_rows) { return (int)nblk(nb_rows * 6 * lpr_for(nb_rows)); }
+
+void gather_op(hipStream_t s, const uft_ctx *C, const double *u, const double *Ye, const RowOp<double> &op) {
+  const int nn = (int)C->nn, ne = (int)C->ne;
+  if (C->npe == 8)
+    k_node_gather_op<8, double><<<nblk(nn), TPB, 0, s>>>(nn, ne, C->d_iptr, C->d_inc, Ye, C->d_fixm, u, op);
+  else
+    k_node_gather_op<4, double><<<nblk(nn), TPB, 0, s>>>(nn, ne, C->d_iptr, C->d_inc, Ye, C->d_fixm, u, op);
+}
+void gather_op(hipStream_t s, const uft_ctx *C, const float *u, const float *Ye, const RowOp<float> &op) {
+  const int nn = (int)C->nn, ne = (int)C->ne;
+  if (C->npe == 8)
+    k_node_gather_op<8, float><<<nblk(nn), TPB, 0, s>>>(nn, ne, C->d_iptr, C->d_inc, Ye, C->d_fixm, u, op);
+  else
+    k_node_gather_op<4, float><<<nblk(nn), TPB, 0, s>>>(nn, ne, C->d_iptr, C->d_inc, Ye, C->d_fixm, u, op);
+}
+int gather_op_blocks(int64_t nn) { return (int)nblk(nn); }
+
+template <typename T>
+static void restrict_first_t(hipStream_t s, int bs, int64_t nc, const int32_t *cptr, const int32_t *child,
+                             const T *P, const T *r, T *bc, const T *dinv, double it, T *rc, T *dc, T *xc) {
+  if (bs == 3)
+    k_restrict_first<3, T><<<nblk(nc * 64), TPB, 0, s>>>(nc, cptr, child, P, r, bc, dinv, it, rc, dc, xc);
+  else
+    k_restrict_first<6, T><<<nblk(nc * 64), TPB, 0, s>>>(nc, cptr, child, P, r, bc, dinv, it, rc, dc, xc);
+}
+void restrict_first(hipStream_t s, int bs, int64_t nc, const int32_t *cptr, const int32_t *child, const double *P,
+                    const double *r, double *bc, const double *dinv, double it, double *rc, double *dc, double *xc) {
+  restrict_first_t<double>(s, bs, nc, cptr, child, P, r, bc, dinv, it, rc, dc, xc);
+}
+void restrict_first(hipStream_t s, int bs, int64_t nc, const int32_t *cptr, const int32_t *child, const float *P,
+                    const float *r, float *bc, const float *dinv, double it, float *rc, float *dc, float *xc) {
+  restrict_first_t<float>(s, bs, nc, cptr, child, P, r, bc, dinv, it, rc, dc, xc);
+}
+
+template <typename T>
+static void prolong_comb_t(hipStream_t s, int bs, int64_t nu, const int32_t *parent, const T *P, const double *sc,
+                           int ia, int ib, const T *xa, const T *xb, T *x) {
+  if (bs == 3)
+    k_prolong_comb<3, T><<<nblk(nu * 3), TPB, 0, s>>>(nu, parent, P, sc, ia, ib, xa, xb, x);
+  else
+    k_prolong_comb<6, T><<<nblk(nu * 6), TPB, 0, s>>>(nu, parent, P, sc, ia, ib, xa, xb, x);
+}
+void prolong_comb(hipStream_t s, int bs, int64_t nu, const int32_t *parent, const double *P, const double *sc, int ia,
+                  int ib, const double *xa, const double *xb, double *x) {
+  prolong_comb_t<double>(s, bs, nu, parent, P, sc, ia, ib, xa, xb, x);
+}
+void prolong_comb(hipStream_t s, int bs, int64_t nu, const int32_t *parent, const float *P, const double *sc, int ia,
+                  int ib, const float *xa, const float *xb, float *x) {
+  prolong_comb_t<float>(s, bs, nu, parent, P, sc, ia, ib, xa, xb, x);
+}
+
+void lincomb_first(hipStream_t s, int64_t n, const double *sc, const double *b, const double *v, double *kr, double *r,
+                   const double *dinv, double it, double *d, double *kd) {
+  k_lincomb_first<double><<<nblk(n), TPB, 0, s>>>(n, sc, b, v, kr, r, dinv, it, d, kd);
+}
+void lincomb_first(hipStream_t s, int64_t n, const double *sc, const float *b, const float *v, float *kr, float *r,
+                   const float *dinv, double it, float *d, float *kd) {
+  k_lincomb_first<float><<<nblk(n), TPB, 0, s>>>(n, sc, b, v, kr, r, dinv, it, d, kd);
+}
+
+void final_kc1(hipStream_t s, const double *part, int nb, double *sc) { k_final_kc<2><<<1, TPB, 0, s>>>(part, nb, sc); }
+void final_kc2(hipStream_t s, const double *part, int nb, double *sc) { k_final_kc<3><<<1, TPB, 0, s>>>(part, nb, sc); }
+
+void tail_solve(hipStream_t s, int nc_units, const int32_t *cptr, const int32_t *child, const double *P,
+                const double *Ainv, const double *r, double *xc) {
+  k_tail_solve<double><<<1, TAIL_TPB, 0, s>>>(nc_units, cptr, child, P, Ainv, r, xc);
+}
+void tail_solve(hipStream_t s, int nc_units, const int32_t *cptr, const int32_t *child, const float *P,
+                const double *Ainv, const float *r, float *xc) {
+  k_tail_solve<float><<<1, TAIL_TPB, 0, s>>>(nc_units, cptr, child, P, Ainv, r, xc);
+}
+
+}  // namespace uftk

@@ -18,6 +18,8 @@
 // a fixed order.  Results are therefore bit-reproducible run to run.
 #include <type_traits>
 
+#include <hip/hip_fp16.h>
+
 #include "uft_internal.h"
 #include "uft_reduce.h"
 
@@ -106,6 +108,16 @@ __global__ __launch_bounds__(TPB) void k_final_sum(const double *__restrict__ pa
     }
     if (threadIdx.x == 0) out[k] = sm[0];
     __syncthreads();
+  }
+}
+// K <= 4: shuffle-tree variant of the same fixed-order sum
+template <int K>
+__global__ __launch_bounds__(TPB) void k_final_sum_k(const double *__restrict__ part, int nb, double *__restrict__ out) {
+  double res[K];
+  uftr::block_final<K>(part, nb, res);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) out[k] = res[k];
   }
 }
 
@@ -508,6 +520,86 @@ void k_elem_apply_v(int ne, const int32_t *__restrict__ elem, const T2 *__restri
     kacc<pk_row(2 * q + 1, ND), pk_col(2 * q + 1, ND)>(kk.y, ue, ye);
   });
   const T E = (T)simpE(sc, xphys[e]);
+#pragma unroll
+  for (int k = 0; k < ND; ++k) Ye[(size_t)k * ne + e] = E * ye[k];
+}
+
+// FP16 element blocks for the multigrid smoother of the mixed form: the packed
+// triangle of KE_e / s_e (s_e = max |KE_e|, so every entry lies in [-1, 1]
+// and keeps FP16's full relative precision down to 6e-5 of the largest
+// entry) in groups of four halves, [NG][ne] x 8 bytes; FP32 arithmetic with
+// E(x_e) s_e applied at the end.  Entries past the triangle are zero.
+template <int NPE>
+__global__ __launch_bounds__(TPB) void k_ke_to_half(int ne, const double2 *__restrict__ KE, uint2 *__restrict__ KEh,
+                                                    float *__restrict__ KEs) {
+  constexpr int ND = 3 * NPE, NP = ND * (ND + 1) / 2, NQ = NP / 2, NG = (NP + 3) / 4;
+  const int e = blockIdx.x * TPB + threadIdx.x;
+  if (e >= ne) return;
+  double mx = 0.0;
+  for (int q = 0; q < NQ; ++q) {
+    const double2 v = KE[(size_t)q * ne + e];
+    mx = fmax(mx, fmax(fabs(v.x), fabs(v.y)));
+  }
+  const double sc = mx > 0.0 ? mx : 1.0;
+  KEs[e] = (float)sc;
+  for (int g = 0; g < NG; ++g) {
+    __half h[4];
+    for (int j = 0; j < 4; ++j) {
+      const int p = 4 * g + j;
+      double v = 0.0;
+      if (p < NP) {
+        const double2 pr = KE[(size_t)(p >> 1) * ne + e];
+        v = (p & 1) ? pr.y : pr.x;
+      }
+      h[j] = __float2half_rn((float)(v / sc));
+    }
+    uint2 w;
+    w.x = (unsigned)__half_as_ushort(h[0]) | ((unsigned)__half_as_ushort(h[1]) << 16);
+    w.y = (unsigned)__half_as_ushort(h[2]) | ((unsigned)__half_as_ushort(h[3]) << 16);
+    KEh[(size_t)g * ne + e] = w;
+  }
+}
+
+__device__ __forceinline__ float h_lo(unsigned w) { return __half2float(__ushort_as_half((unsigned short)(w & 0xffffu))); }
+__device__ __forceinline__ float h_hi(unsigned w) { return __half2float(__ushort_as_half((unsigned short)(w >> 16))); }
+
+template <int NPE, int BT, bool NT, int WPE>
+__global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1)))
+void k_elem_apply_h(int ne, const int32_t *__restrict__ elem, const uint2 *__restrict__ KEh,
+                    const float *__restrict__ KEs, const double *__restrict__ xphys, SimpCoef sc,
+                    const uint8_t *__restrict__ fixm, const float *__restrict__ u, float *__restrict__ Ye) {
+  constexpr int ND = 3 * NPE, NP = ND * (ND + 1) / 2, NG = (NP + 3) / 4;
+  const int e = blockIdx.x * BT + threadIdx.x;
+  if (e >= ne) return;
+  float ue[ND], ye[ND];
+#pragma unroll
+  for (int a = 0; a < NPE; ++a) {
+    const int n = elem[(size_t)a * ne + e];
+    const unsigned m = fixm[n];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const float v = u[3 * (size_t)n + c];
+      ue[c * NPE + a] = ((m >> c) & 1u) ? 0.f : v;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < ND; ++k) ye[k] = 0.f;
+  static_for<0, NG>([&](auto gc) {
+    constexpr int g = decltype(gc)::value;
+    uint2 w;
+    if constexpr (NT) {
+      typedef unsigned v2u __attribute__((ext_vector_type(2)));
+      const v2u v = __builtin_nontemporal_load(reinterpret_cast<const v2u *>(&KEh[(size_t)g * ne + e]));
+      w = make_uint2(v.x, v.y);
+    } else {
+      w = KEh[(size_t)g * ne + e];
+    }
+    kacc<pk_row(4 * g, ND), pk_col(4 * g, ND)>(h_lo(w.x), ue, ye);
+    if constexpr (4 * g + 1 < NP) kacc<pk_row(4 * g + 1, ND), pk_col(4 * g + 1, ND)>(h_hi(w.x), ue, ye);
+    if constexpr (4 * g + 2 < NP) kacc<pk_row(4 * g + 2, ND), pk_col(4 * g + 2, ND)>(h_lo(w.y), ue, ye);
+    if constexpr (4 * g + 3 < NP) kacc<pk_row(4 * g + 3, ND), pk_col(4 * g + 3, ND)>(h_hi(w.y), ue, ye);
+  });
+  const float E = (float)simpE(sc, xphys[e]) * KEs[e];
 #pragma unroll
   for (int k = 0; k < ND; ++k) Ye[(size_t)k * ne + e] = E * ye[k];
 }
@@ -985,6 +1077,38 @@ void elem_phase_f(hipStream_t s, const uft_ctx *C, int variant, const double *xp
                                    Ye);
 }
 
+template <int NPE>
+static void elem_phase_h_t(hipStream_t s, int variant, int ne, const int32_t *elem, const uint2 *KEh, const float *KEs,
+                           const double *xphys, SimpCoef sc, const uint8_t *fixm, const float *u, float *Ye) {
+  auto g = [ne](int bt) { return (unsigned)((ne + bt - 1) / bt); };
+  switch (variant) {
+    case 1: k_elem_apply_h<NPE, 256, true, 0><<<g(256), 256, 0, s>>>(ne, elem, KEh, KEs, xphys, sc, fixm, u, Ye); break;
+    case 2: k_elem_apply_h<NPE, 256, true, 4><<<g(256), 256, 0, s>>>(ne, elem, KEh, KEs, xphys, sc, fixm, u, Ye); break;
+    case 3: k_elem_apply_h<NPE, 128, true, 0><<<g(128), 128, 0, s>>>(ne, elem, KEh, KEs, xphys, sc, fixm, u, Ye); break;
+    case 4: k_elem_apply_h<NPE, 512, true, 0><<<g(512), 512, 0, s>>>(ne, elem, KEh, KEs, xphys, sc, fixm, u, Ye); break;
+    case 5: k_elem_apply_h<NPE, 256, false, 4><<<g(256), 256, 0, s>>>(ne, elem, KEh, KEs, xphys, sc, fixm, u, Ye); break;
+    case 6: k_elem_apply_h<NPE, 64, true, 0><<<g(64), 64, 0, s>>>(ne, elem, KEh, KEs, xphys, sc, fixm, u, Ye); break;
+    default: k_elem_apply_h<NPE, 256, false, 0><<<g(256), 256, 0, s>>>(ne, elem, KEh, KEs, xphys, sc, fixm, u, Ye); break;
+  }
+}
+void elem_phase_h(hipStream_t s, const uft_ctx *C, int variant, const double *xphys, const float *u, float *Ye) {
+  const SimpCoef sc{C->P.Emin, C->P.Emax - C->P.Emin, C->P.penal};
+  if (C->npe == 8)
+    elem_phase_h_t<8>(s, variant, (int)C->ne, C->d_elem, C->d_KEh, C->d_KEs, xphys, sc, C->d_fixm, u, Ye);
+  else
+    elem_phase_h_t<4>(s, variant, (int)C->ne, C->d_elem, C->d_KEh, C->d_KEs, xphys, sc, C->d_fixm, u, Ye);
+}
+void ke_to_half(hipStream_t s, const uft_ctx *C) {
+  if (C->npe == 8)
+    k_ke_to_half<8><<<nblk(C->ne), TPB, 0, s>>>((int)C->ne, (const double2 *)C->d_KE, C->d_KEh, C->d_KEs);
+  else
+    k_ke_to_half<4><<<nblk(C->ne), TPB, 0, s>>>((int)C->ne, (const double2 *)C->d_KE, C->d_KEh, C->d_KEs);
+}
+int ke_half_groups(int npe) {
+  const int nd = 3 * npe, np = nd * (nd + 1) / 2;
+  return (np + 3) / 4;
+}
+
 int red_blocks(int64_t n) { return (int)nblk(n); }
 
 void gen_ke(hipStream_t s, int npe, int64_t ne, const int32_t *elem, const double *xyz, double nu, double *KE,
@@ -1122,7 +1246,13 @@ void tfx_t(hipStream_t s, const uft_ctx *C, const double *RX, double *xe, int cl
 }
 
 void final_sum(hipStream_t s, const double *part, int nb, int K, double *out) {
-  k_final_sum<<<1, TPB, 0, s>>>(part, nb, K, out);
+  switch (K) {
+    case 1: k_final_sum_k<1><<<1, TPB, 0, s>>>(part, nb, out); break;
+    case 2: k_final_sum_k<2><<<1, TPB, 0, s>>>(part, nb, out); break;
+    case 3: k_final_sum_k<3><<<1, TPB, 0, s>>>(part, nb, out); break;
+    case 4: k_final_sum_k<4><<<1, TPB, 0, s>>>(part, nb, out); break;
+    default: k_final_sum<<<1, TPB, 0, s>>>(part, nb, K, out); break;
+  }
 }
 void final_max(hipStream_t s, const double *part, int nb, double *out) { k_final_max<<<1, TPB, 0, s>>>(part, nb, out); }
 void dot(hipStream_t s, int64_t n, const double *a, const double *b, double *part, double *out) {

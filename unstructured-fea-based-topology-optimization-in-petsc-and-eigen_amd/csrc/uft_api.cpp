@@ -99,11 +99,10 @@ hipEvent_t *prof_triple(uft_ctx *C, int kind) {
   return ev;
 }
 
+bool is_mixed(const uft_ctx *C) { return C->P.precision != UFT_FP64; }
+
 void fine_apply(uft_ctx *C, const double *xphys, const double *x, double *y, const double *dotw, double *part) {
   uftk::apply_K(C->s, C, xphys, x, y, dotw, part, prof_triple(C, 0));
-}
-void fine_applyf(uft_ctx *C, const double *xphys, const float *x, float *y) {
-  uftk::apply_Kf(C->s, C, xphys, x, y, prof_triple(C, 1));
 }
 
 // y = A_l x on multigrid level l
@@ -113,13 +112,6 @@ void level_apply(uft_ctx *C, int l, const double *x, double *y) {
     fine_apply(C, C->cur_xphys, x, y, nullptr, nullptr);
   else
     uftk::bsr6_mv(C->s, Lv.nunit, Lv.d_rp, Lv.d_ci, Lv.d_val, x, y);
-}
-void level_applyf(uft_ctx *C, int l, const float *x, float *y) {
-  AmgLevel &Lv = C->amg.L[l];
-  if (l == 0)
-    fine_applyf(C, C->cur_xphys, x, y);
-  else
-    uftk::bsr6_mvf(C->s, Lv.nunit, Lv.d_rp, Lv.d_ci, Lv.d_valf, x, y);
 }
 
 // Chebyshev interval [0.1, 1.1] * lambda_max(D^-1 A), the PCGAMG default
@@ -149,40 +141,12 @@ void k_cheb_first(hipStream_t s, int64_t n, const double *b, const double *di, d
                   double *x, int z) { uftk::cheb_first(s, n, b, di, it, r, d, x, z); }
 void k_cheb_first(hipStream_t s, int64_t n, const float *b, const float *di, double it, float *r, float *d, float *x,
                   int z) { uftk::cheb_firstf(s, n, b, di, it, r, d, x, z); }
-void k_cheb_step(hipStream_t s, int64_t n, const double *t, const double *di, double c1, double c2, double *r,
-                 double *d, double *x) { uftk::cheb_step(s, n, t, di, c1, c2, r, d, x); }
-void k_cheb_step(hipStream_t s, int64_t n, const float *t, const float *di, double c1, double c2, float *r, float *d,
-                 float *x) { uftk::cheb_stepf(s, n, t, di, c1, c2, r, d, x); }
-void k_sub_inplace(hipStream_t s, int64_t n, const double *t, double *r) { uftk::sub_inplace(s, n, t, r); }
-void k_sub_inplace(hipStream_t s, int64_t n, const float *t, float *r) { uftk::sub_inplacef(s, n, t, r); }
-void k_sub(hipStream_t s, int64_t n, const double *a, const double *b, double *c) { uftk::sub(s, n, a, b, c); }
-void k_sub(hipStream_t s, int64_t n, const float *a, const float *b, float *c) { uftk::subf(s, n, a, b, c); }
 void k_restrict(hipStream_t s, const AmgLevel &Lv, int64_t nc, const double *r, double *bc) {
   uftk::restrict_(s, Lv.bs, nc, Lv.d_cptr, Lv.d_child, Lv.d_P, r, bc);
 }
 void k_restrict(hipStream_t s, const AmgLevel &Lv, int64_t nc, const float *r, float *bc) {
   uftk::restrictf(s, Lv.bs, nc, Lv.d_cptr, Lv.d_child, Lv.d_Pf, r, bc);
 }
-void k_prolong(hipStream_t s, const AmgLevel &Lv, const double *xc, double *x) {
-  uftk::prolong_add(s, Lv.bs, Lv.nunit, Lv.d_parent, Lv.d_P, xc, x);
-}
-void k_prolong(hipStream_t s, const AmgLevel &Lv, const float *xc, float *x) {
-  uftk::prolong_addf(s, Lv.bs, Lv.nunit, Lv.d_parent, Lv.d_Pf, xc, x);
-}
-void k_apply(uft_ctx *C, int l, const double *x, double *y) { level_apply(C, l, x, y); }
-void k_apply(uft_ctx *C, int l, const float *x, float *y) { level_applyf(C, l, x, y); }
-void k_dot(uft_ctx *C, int64_t n, const double *a, const double *b, double *out) {
-  uftk::dot(C->s, n, a, b, C->d_part, out);
-}
-void k_dot(uft_ctx *C, int64_t n, const float *a, const float *b, double *out) {
-  uftk::dotf(C->s, n, a, b, C->d_part, out);
-}
-void k_lincomb(hipStream_t s, int64_t n, const double *sc, int ia, const double *a, int ib, const double *b,
-               double *y) { uftk::lincomb(s, n, sc, ia, a, ib, b, y); }
-void k_lincomb(hipStream_t s, int64_t n, const double *sc, int ia, const float *a, int ib, const float *b, float *y) {
-  uftk::lincombf(s, n, sc, ia, a, ib, b, y);
-}
-
 template <typename T>
 struct LV {
   T *x, *b, *r, *d, *t, *kc, *kv, *kd, *kr;
@@ -190,93 +154,223 @@ struct LV {
 };
 LV<double> lv(AmgLevel &L, double *) { return {L.x, L.b, L.r, L.d, L.t, L.kc, L.kv, L.kd, L.kr, L.d_dinv}; }
 LV<float> lv(AmgLevel &L, float *) { return {L.xf, L.bf, L.rf, L.df, L.tf, L.kcf, L.kvf, L.kdf, L.krf, L.d_dinvf}; }
+const double *tr_P(const AmgLevel &L, double *) { return L.d_P; }
+const float *tr_P(const AmgLevel &L, float *) { return L.d_Pf; }
 
-template <typename T>
-void coarse_solve(uft_ctx *C, int l, const T *b, T *x);
-
-// One multigrid cycle at level l: x = B_l b.  Chebyshev-Jacobi pre- and
-// post-smoothing with the same polynomial (so the cycle is symmetric), and a
-// coarse-grid correction by coarse_solve.
-template <typename T>
-void cycle(uft_ctx *C, int l, const T *b, T *x) {
-  AmgHier &H = C->amg;
-  hipStream_t s = C->s;
-  AmgLevel &Lv = H.L[l];
-  AmgLevel &Ln = H.L[l + 1];
-  LV<T> v = lv(Lv, (T *)nullptr);
-  LV<T> vn = lv(Ln, (T *)nullptr);
-  const int64_t n = Lv.n;
-  const int deg = std::max(1, C->P.cheb_degree);
-  const ChebCoef cc = cheb_coef(Lv.lmax, deg);
-  k_cheb_first(s, n, b, v.dinv, cc.inv_theta, v.r, v.d, x, 1);
-  for (int k = 1; k < deg; ++k) {
-    k_apply(C, l, v.d, v.t);
-    k_cheb_step(s, n, v.t, v.dinv, cc.c1[k], cc.c2[k], v.r, v.d, x);
+// ---- fused multigrid cycle
+// Operator apply with a per-row epilogue on level l (l = 0: matrix-free fine
+// operator of the current density; above: 6x6 block-sparse).
+void lev_op(uft_ctx *C, int l, const double *u, const RowOp<double> &op) {
+  if (l == 0) {
+    hipEvent_t *ev = prof_triple(C, 0);
+    if (ev) (void)hipEventRecord(ev[0], C->s);
+    uftk::elem_phase(C->s, C, C->elem_variant, C->cur_xphys, u, C->d_Ye);
+    if (ev) (void)hipEventRecord(ev[1], C->s);
+    uftk::gather_op(C->s, C, u, C->d_Ye, op);
+    if (ev) (void)hipEventRecord(ev[2], C->s);
+  } else {
+    const AmgLevel &Lv = C->amg.L[l];
+    uftk::bsr6_op(C->s, Lv.nunit, Lv.d_rp, Lv.d_ci, Lv.d_val, u, op);
   }
-  k_apply(C, l, v.d, v.t);
-  k_sub_inplace(s, n, v.t, v.r);  // r = b - A x
-  k_restrict(s, Lv, Ln.nunit, v.r, vn.b);
-  coarse_solve(C, l + 1, vn.b, vn.x);
-  k_prolong(s, Lv, vn.x, x);
-  k_apply(C, l, x, v.t);
-  k_sub(s, n, b, v.t, v.r);
-  k_cheb_first(s, n, b, v.dinv, cc.inv_theta, v.r, v.d, x, 0);
-  for (int k = 1; k < deg; ++k) {
-    k_apply(C, l, v.d, v.t);
-    k_cheb_step(s, n, v.t, v.dinv, cc.c1[k], cc.c2[k], v.r, v.d, x);
+}
+void lev_op(uft_ctx *C, int l, const float *u, const RowOp<float> &op) {
+  if (l == 0) {
+    hipEvent_t *ev = prof_triple(C, 1);
+    if (ev) (void)hipEventRecord(ev[0], C->s);
+    if (C->d_KEh)
+      uftk::elem_phase_h(C->s, C, C->elem_variant_h, C->cur_xphys, u, C->d_Yef);
+    else
+      uftk::elem_phase_f(C->s, C, C->elem_variant_f, C->cur_xphys, u, C->d_Yef);
+    if (ev) (void)hipEventRecord(ev[1], C->s);
+    uftk::gather_op(C->s, C, u, C->d_Yef, op);
+    if (ev) (void)hipEventRecord(ev[2], C->s);
+  } else {
+    const AmgLevel &Lv = C->amg.L[l];
+    uftk::bsr6_op(C->s, Lv.nunit, Lv.d_rp, Lv.d_ci, Lv.d_valf, u, op);
+  }
+}
+int lev_op_blocks(const uft_ctx *C, int l) {
+  return l == 0 ? uftk::gather_op_blocks(C->nn) : uftk::bsr6_op_blocks(C->amg.L[l].nunit);
+}
+
+// How a coarse level l >= 1 is solved inside the cycle of level l-1:
+// 0 one cycle (V), 1 two stationary cycles (W), 2 a two-step K-cycle.
+int level_kind(const uft_ctx *C, int l) {
+  const int ct = C->P.cycle_type;
+  if (ct == 0 || (ct == 3 && l > 1)) return 0;
+  return ct == 1 ? 1 : 2;
+}
+
+template <typename T>
+void cycle_body(uft_ctx *C, int l, const T *b, T *x);
+
+// Solve on coarse level l (1 <= l < coarsest) whose first smoothing step from
+// a zero guess has been done by the restriction; the result is left where the
+// prolongation reads it (prolong_from).
+template <typename T>
+void coarse_body(uft_ctx *C, int l) {
+  AmgLevel &Lv = C->amg.L[l];
+  LV<T> v = lv(Lv, (T *)nullptr);
+  const int kind = level_kind(C, l);
+  const double it = cheb_coef(Lv.lmax, std::max(1, C->P.cheb_degree)).inv_theta;
+  if (kind == 0) {
+    cycle_body<T>(C, l, v.b, v.x);
+  } else if (kind == 1) {
+    cycle_body<T>(C, l, v.b, v.x);
+    RowOp<T> op;
+    op.mode = ROW_RESFIRSTX;
+    op.base = v.b;
+    op.r = v.r;
+    op.kr = v.kr;
+    op.dinv = v.dinv;
+    op.it = it;
+    op.dout = v.d;
+    op.kd = v.kd;
+    lev_op(C, l, v.x, op);
+    cycle_body<T>(C, l, v.kr, v.kd);
+  } else {
+    double *sc = C->d_scal + S_K + 8 * l;
+    cycle_body<T>(C, l, v.b, v.kc);
+    RowOp<T> op;
+    op.mode = ROW_DOTS2;
+    op.y = v.kv;
+    op.w1 = v.b;
+    op.part = C->d_part;
+    lev_op(C, l, v.kc, op);
+    uftk::final_kc1(C->s, C->d_part, lev_op_blocks(C, l), sc);
+    uftk::lincomb_first(C->s, Lv.n, sc, v.b, v.kv, v.kr, v.r, v.dinv, it, v.d, v.kd);
+    cycle_body<T>(C, l, v.kr, v.kd);
+    RowOp<T> o3;
+    o3.mode = ROW_DOTS3;
+    o3.w1 = v.kv;
+    o3.w2 = v.kr;
+    o3.part = C->d_part;
+    lev_op(C, l, v.kd, o3);
+    uftk::final_kc2(C->s, C->d_part, lev_op_blocks(C, l), sc);
   }
 }
 
-// Approximate solve of A_l x = b on a coarse level (l >= 1).  The coarsest
-// level is solved exactly in FP64.  Otherwise: one cycle (V), two stationary
-// cycles (W), or two flexible-CG steps preconditioned by the cycle (K-cycle,
-// Notay & Vassilevski 2008), with all scalars kept on the device.
+// x_l += P (result of coarse level l+1, 1 <= l+1 < coarsest)
 template <typename T>
-void coarse_solve(uft_ctx *C, int l, const T *b, T *x) {
+void prolong_from(uft_ctx *C, int l, T *x) {
+  AmgLevel &Lv = C->amg.L[l], &Ln = C->amg.L[l + 1];
+  LV<T> vn = lv(Ln, (T *)nullptr);
+  const T *P = tr_P(Lv, (T *)nullptr);
+  switch (level_kind(C, l + 1)) {
+    case 0:
+      uftk::prolong_comb(C->s, Lv.bs, Lv.nunit, Lv.d_parent, P, nullptr, -1, -1, vn.x, (const T *)nullptr, x);
+      break;
+    case 1:  // x + kd of the two stationary cycles
+      uftk::prolong_comb(C->s, Lv.bs, Lv.nunit, Lv.d_parent, P, nullptr, -1, -1, vn.x, vn.kd, x);
+      break;
+    default:  // K-cycle: c5 kc + c6 kd
+      uftk::prolong_comb(C->s, Lv.bs, Lv.nunit, Lv.d_parent, P, C->d_scal + S_K + 8 * (l + 1), 5, 6, vn.kc, vn.kd,
+                         x);
+      break;
+  }
+}
+
+// Coarse-grid correction of level l when level l+1 is the coarsest (dense
+// FP64 inverse): x += P Ainv P^T r.
+template <typename T>
+void coarse_direct(uft_ctx *C, int l, const T *r, T *x) {
   AmgHier &H = C->amg;
-  hipStream_t s = C->s;
-  AmgLevel &Lv = H.L[l];
-  if (l == H.nlev - 1) {
-    if constexpr (std::is_same<T, double>::value) {
-      uftk::dense_mv(s, H.ncoarse, H.d_cinv, b, x);
-    } else {
-      uftk::f2d(s, Lv.n, b, Lv.kc);
-      uftk::dense_mv(s, H.ncoarse, H.d_cinv, Lv.kc, Lv.kv);
-      uftk::d2f(s, Lv.n, Lv.kv, x);
-    }
+  AmgLevel &Lv = H.L[l], &Ln = H.L[l + 1];
+  LV<T> vn = lv(Ln, (T *)nullptr);
+  const T *P = tr_P(Lv, (T *)nullptr);
+  if (Lv.bs == 6 && H.ncoarse <= UFT_TAIL_MAX) {
+    uftk::tail_solve(C->s, (int)Ln.nunit, Lv.d_cptr, Lv.d_child, P, H.d_cinv, r, vn.x);
+    uftk::prolong_comb(C->s, Lv.bs, Lv.nunit, Lv.d_parent, P, nullptr, -1, -1, vn.x, (const T *)nullptr, x);
     return;
   }
-  LV<T> v = lv(Lv, (T *)nullptr);
-  const int64_t n = Lv.n;
-  const int ct = C->P.cycle_type;
-  if (ct == 0 || (ct == 3 && l > 1)) {
-    cycle(C, l, b, x);
-  } else if (ct == 1) {
-    cycle(C, l, b, x);
-    k_apply(C, l, x, v.kv);
-    k_sub(s, n, b, v.kv, v.kr);
-    cycle(C, l, v.kr, v.kd);
-    uftk::axpy_inplace_t(s, n, v.kd, x);
+  k_restrict(C->s, Lv, Ln.nunit, r, vn.b);
+  if constexpr (std::is_same<T, double>::value) {
+    uftk::dense_mv(C->s, H.ncoarse, H.d_cinv, vn.b, vn.x);
   } else {
-    double *sc = C->d_scal + S_K + 8 * l;  // 0 rho1 1 alpha1 2 gamma 3 beta 4 alpha2 5 c1 6 c2 7 -a1/rho1
-    cycle(C, l, b, v.kc);
-    k_apply(C, l, v.kc, v.kv);
-    k_dot(C, n, v.kc, v.kv, sc + 0);
-    k_dot(C, n, v.kc, b, sc + 1);
-    uftk::kcycle_coef1(s, sc);  // sc[7] = -alpha1/rho1
-    k_lincomb(s, n, sc, -1, b, 7, v.kv, v.kr);  // r = b - (alpha1/rho1) v
-    cycle(C, l, v.kr, v.kd);
-    k_apply(C, l, v.kd, v.t);
-    k_dot(C, n, v.kd, v.kv, sc + 2);
-    k_dot(C, n, v.kd, v.t, sc + 3);
-    k_dot(C, n, v.kd, v.kr, sc + 4);
-    uftk::kcycle_coef2(s, sc);  // sc[5], sc[6]
-    k_lincomb(s, n, sc, 5, v.kc, 6, v.kd, x);
+    uftk::f2d(C->s, Ln.n, vn.b, Ln.kc);
+    uftk::dense_mv(C->s, H.ncoarse, H.d_cinv, Ln.kc, Ln.kv);
+    uftk::d2f(C->s, Ln.n, Ln.kv, vn.x);
+  }
+  uftk::prolong_comb(C->s, Lv.bs, Lv.nunit, Lv.d_parent, P, nullptr, -1, -1, vn.x, (const T *)nullptr, x);
+}
+
+// One cycle on level l for right-hand side b, after its first Chebyshev
+// step from a zero guess (r = b, d = it D^-1 b, x = d) is in place: the
+// remaining pre-smoothing steps, the coarse-grid correction, and the same
+// polynomial as post-smoothing (a symmetric cycle).  Result in x.
+template <typename T>
+void cycle_body(uft_ctx *C, int l, const T *b, T *x) {
+  AmgHier &H = C->amg;
+  AmgLevel &Lv = H.L[l];
+  LV<T> v = lv(Lv, (T *)nullptr);
+  const int deg = std::max(1, C->P.cheb_degree);
+  const ChebCoef cc = cheb_coef(Lv.lmax, deg);
+  T *dc = v.d, *dn = v.t;
+  RowOp<T> st;
+  st.mode = ROW_STEP;
+  st.r = v.r;
+  st.dinv = v.dinv;
+  st.x = x;
+  for (int k = 1; k < deg; ++k) {
+    st.c1 = cc.c1[k];
+    st.c2 = cc.c2[k];
+    st.dout = dn;
+    st.addprev = 0;
+    lev_op(C, l, dc, st);
+    std::swap(dc, dn);
+  }
+  RowOp<T> rs;
+  rs.mode = ROW_RES;  // r = b - A x
+  rs.base = v.r;
+  rs.r = v.r;
+  lev_op(C, l, dc, rs);
+  if (l + 1 == H.nlev - 1) {
+    coarse_direct<T>(C, l, v.r, x);
+  } else {
+    AmgLevel &Ln = H.L[l + 1];
+    LV<T> vn = lv(Ln, (T *)nullptr);
+    const double itn = cheb_coef(Ln.lmax, deg).inv_theta;
+    T *xn = level_kind(C, l + 1) == 2 ? vn.kc : vn.x;
+    uftk::restrict_first(C->s, Lv.bs, Ln.nunit, Lv.d_cptr, Lv.d_child, tr_P(Lv, (T *)nullptr), v.r, vn.b, vn.dinv,
+                         itn, vn.r, vn.d, xn);
+    coarse_body<T>(C, l + 1);
+    prolong_from<T>(C, l, x);
+  }
+  RowOp<T> rf;
+  rf.mode = ROW_RESFIRST;  // r = b - A x ; d = it D^-1 r
+  rf.base = b;
+  rf.r = v.r;
+  rf.dinv = v.dinv;
+  rf.it = cc.inv_theta;
+  rf.dout = v.d;
+  lev_op(C, l, x, rf);
+  dc = v.d;
+  dn = v.t;
+  if (deg == 1) {
+    uftk::axpy_inplace_t(C->s, Lv.n, v.d, x);
+    return;
+  }
+  for (int k = 1; k < deg; ++k) {
+    st.c1 = cc.c1[k];
+    st.c2 = cc.c2[k];
+    st.dout = dn;
+    st.addprev = k == 1;
+    lev_op(C, l, dc, st);
+    std::swap(dc, dn);
   }
 }
 
-void vcycle(uft_ctx *C, int l, const double *b, double *x) { cycle<double>(C, l, b, x); }
-void vcycle_f(uft_ctx *C, int l, const float *b, float *x) { cycle<float>(C, l, b, x); }
+// x = B_0 b: one fine-level cycle (the preconditioner of the elasticity solve)
+template <typename T>
+void fine_cycle(uft_ctx *C, const T *b, T *x) {
+  AmgLevel &L0 = C->amg.L[0];
+  LV<T> v = lv(L0, (T *)nullptr);
+  const double it = cheb_coef(L0.lmax, std::max(1, C->P.cheb_degree)).inv_theta;
+  k_cheb_first(C->s, L0.n, b, v.dinv, it, v.r, v.d, x, 1);
+  cycle_body<T>(C, 0, b, x);
+}
+void vcycle(uft_ctx *C, int, const double *b, double *x) { fine_cycle<double>(C, b, x); }
+void vcycle_f(uft_ctx *C, int, const float *b, float *x) { fine_cycle<float>(C, b, x); }
 
 // largest eigenvalue of a symmetric tridiagonal matrix (bisection, Sturm counts)
 double tridiag_lmax(const std::vector<double> &d, const std::vector<double> &e) {
@@ -623,6 +717,8 @@ int uft_params_set(uft_ctx *C, const uft_params *p) {
   if (p->m < 1 || p->m > UFT_MAXM) return uft_set_error(C, UFT_ERR_ARG, "m must be in [1, %d]", UFT_MAXM);
   if (!(p->rtol_el > 0) || !(p->rtol_f > 0) || p->maxit_el < 1 || p->maxit_f < 1)
     return uft_set_error(C, UFT_ERR_ARG, "bad solver tolerances");
+  if (p->precision < UFT_FP64 || p->precision > UFT_MIXED_H)
+    return uft_set_error(C, UFT_ERR_ARG, "precision must be UFT_FP64, UFT_MIXED or UFT_MIXED_H");
   if (C->is_setup && (p->m != C->P.m || p->rmin != C->P.rmin || p->nu != C->P.nu || p->precision != C->P.precision ||
                       p->amg_levels != C->P.amg_levels || p->coarse_max != C->P.coarse_max))
     return uft_set_error(C, UFT_ERR_STATE, "m, rmin, nu, precision and the hierarchy shape are fixed at setup");
@@ -722,7 +818,7 @@ int uft_setup(uft_ctx *C) {
   C->vlen[UFT_RX] = nn;
   // ---- element operators
   if ((rc = dalloc_n(C, &C->d_KE, (int64_t)np * ne))) return rc;
-  if (C->P.precision == UFT_MIXED) {
+  if (is_mixed(C)) {
     if ((rc = dalloc_n(C, &C->d_KEf, (int64_t)np * ne))) return rc;
     if ((rc = dalloc_n(C, &C->d_Yef, (int64_t)nd * ne))) return rc;
   }
@@ -731,12 +827,19 @@ int uft_setup(uft_ctx *C) {
   if ((rc = dalloc_n(C, &C->d_detmin, ne))) return rc;
   if ((rc = dalloc_n(C, &C->d_Ye, (int64_t)nd * ne))) return rc;
   const int64_t maxn = std::max(ndof, ne);
-  // block partials of the deterministic two-launch reductions
-  if ((rc = dalloc_n(C, &C->d_part, (maxn / 256 + 2) * 48))) return rc;
+  // block partials of the deterministic two-launch reductions; the coarse
+  // operators' fused K-cycle products write up to 3 per block of 256/32 rows
+  // (a level has at most 6 rows per fine node)
+  if ((rc = dalloc_n(C, &C->d_part, (maxn / 256 + 2) * 48 + 3 * (2 * ndof * 32 / 256 + 2)))) return rc;
   if ((rc = dalloc_n(C, &C->d_scal, 256))) return rc;
   if ((rc = dalloc_n(C, &C->d_lanczos, 3 * 64))) return rc;
   uftk::fill(s, 256, 0.0, C->d_scal);
   uftk::gen_ke(s, npe, ne, C->d_elem, C->d_xyz, C->P.nu, C->d_KE, C->d_KEf, C->d_detmin);
+  if (C->P.precision == UFT_MIXED_H) {
+    if ((rc = dalloc_n(C, &C->d_KEh, (int64_t)uftk::ke_half_groups(npe) * ne))) return rc;
+    if ((rc = dalloc_n(C, &C->d_KEs, ne))) return rc;
+    uftk::ke_to_half(s, C);
+  }
   uftk::gen_kf(s, npe, ne, C->d_elem, C->d_xyz, C->P.rmin, C->d_KF, C->d_T, C->vec[UFT_VOL], C->d_part);
   uftk::final_sum(s, C->d_part, uftk::red_blocks(ne), 1, C->d_scal + S_MISC);
   {
@@ -756,7 +859,7 @@ int uft_setup(uft_ctx *C) {
   double **fv[] = {&C->d_fr, &C->d_fz, &C->d_fp, &C->d_fq, &C->d_fdinv, &C->d_TX};
   for (double **p : fv)
     if ((rc = dalloc_n(C, p, nn))) return rc;
-  if (C->P.precision == UFT_MIXED) {
+  if (is_mixed(C)) {
     float **ff[] = {&C->d_rf, &C->d_zf, &C->d_pf, &C->d_qf, &C->d_uf};
     for (float **p : ff)
       if ((rc = dalloc_n(C, p, ndof))) return rc;
@@ -797,7 +900,7 @@ int uft_setup(uft_ctx *C) {
         for (double **p : kv)
           if ((rc = dalloc_n(C, p, Lv.n))) return rc;
       }
-      if (C->P.precision == UFT_MIXED) {
+      if (is_mixed(C)) {
         float **wf[] = {&Lv.xf, &Lv.bf, &Lv.rf, &Lv.df, &Lv.tf};
         for (float **p : wf)
           if ((rc = dalloc_n(C, p, Lv.n))) return rc;
@@ -813,7 +916,7 @@ int uft_setup(uft_ctx *C) {
       AmgHostLevel &h = H[l];
       AmgLevel &Lv = A.L[l], &Ln = A.L[l + 1];
       if ((rc = upload(C, &Lv.d_P, h.P))) return rc;
-      if (C->P.precision == UFT_MIXED) {
+      if (is_mixed(C)) {
         std::vector<float> pf(h.P.begin(), h.P.end());
         if ((rc = upload(C, &Lv.d_Pf, pf))) return rc;
       }
@@ -839,7 +942,7 @@ int uft_setup(uft_ctx *C) {
         if ((rc = dalloc_n(C, &C->d_Ee, C->ne))) return rc;
         uftk::galerkin_pre(s, C, Lv.ngl, d_gblk, Lv.d_gl, Ln.d_brow, Ln.d_ci, Lv.d_parent, Lv.d_P, Lv.d_G);
       }
-      if (C->P.precision == UFT_MIXED)
+      if (is_mixed(C))
         if ((rc = dalloc_n(C, &Ln.d_valf, Ln.nblk * 36))) return rc;
       UFT_HIP(C, hipStreamSynchronize(s));  // host vectors of H go out of scope level by level
     }
@@ -847,7 +950,7 @@ int uft_setup(uft_ctx *C) {
     // matrix plus the inverse's panel scratch (at most n^2/16 + n^2/2 + 2n + 1 doubles)
     if ((rc = dalloc_n(C, &A.d_cmat, 2 * A.ncoarse * A.ncoarse + 2 * A.ncoarse + 1))) return rc;
     if ((rc = dalloc_n(C, &A.d_cinv, A.ncoarse * A.ncoarse))) return rc;
-    if (C->P.precision == UFT_MIXED)
+    if (is_mixed(C))
       if ((rc = dalloc_n(C, &A.d_cinvf, A.ncoarse * A.ncoarse))) return rc;
     UFT_HIP(C, hipStreamSynchronize(s));
   }
@@ -940,7 +1043,7 @@ int uft_fem_obj_sens(uft_ctx *C, double *fx, double *gx, int dfdx_id, int m, con
   const double *xp = C->vec[xphys_id];
   double *U = C->vec[UFT_U];
   const double t0 = now_ms();
-  const int src = C->P.precision == UFT_MIXED ? solve_el_mixed(C, xp, U) : solve_el(C, xp, U);
+  const int src = is_mixed(C) ? solve_el_mixed(C, xp, U) : solve_el(C, xp, U);
   const double t1 = now_ms();
   C->t_phase[1] = t1 - t0 - C->t_phase[0];
   if (src != UFT_OK && src != UFT_ERR_NOCONV) return src;
@@ -1357,16 +1460,18 @@ extern "C" int uft_autotune(uft_ctx *C, int reps, int *variant, double *us_per_v
   UFT_HIP(C, hipEventCreate(&e0));
   UFT_HIP(C, hipEventCreate(&e1));
   const double *xp = C->vec[UFT_XPHYS];
-  auto tune = [&](bool f32, int &best, double *us) -> int {
+  // kind 0: FP64 blocks, 1: FP32 blocks, 2: FP16 smoother blocks
+  auto run = [&](int kind, int v) {
+    if (kind == 1) uftk::elem_phase_f(C->s, C, v, xp, C->d_rf, C->d_Yef);
+    else if (kind == 2) uftk::elem_phase_h(C->s, C, v, xp, C->d_rf, C->d_Yef);
+    else uftk::elem_phase(C->s, C, v, xp, C->d_r, C->d_Ye);
+  };
+  auto tune = [&](int kind, int &best, double *us) -> int {
     double bt = 1e300;
     for (int v = 0; v < nv; ++v) {
-      if (f32) uftk::elem_phase_f(C->s, C, v, xp, C->d_rf, C->d_Yef);
-      else uftk::elem_phase(C->s, C, v, xp, C->d_r, C->d_Ye);
+      run(kind, v);
       UFT_HIP(C, hipEventRecord(e0, C->s));
-      for (int k = 0; k < reps; ++k) {
-        if (f32) uftk::elem_phase_f(C->s, C, v, xp, C->d_rf, C->d_Yef);
-        else uftk::elem_phase(C->s, C, v, xp, C->d_r, C->d_Ye);
-      }
+      for (int k = 0; k < reps; ++k) run(kind, v);
       UFT_HIP(C, hipEventRecord(e1, C->s));
       UFT_HIP(C, hipEventSynchronize(e1));
       float ms = 0.f;
@@ -1380,16 +1485,19 @@ extern "C" int uft_autotune(uft_ctx *C, int reps, int *variant, double *us_per_v
     return UFT_OK;
   };
   uftk::fill(C->s, C->ndof, 1.0, C->d_r);
-  if ((rc = tune(false, C->elem_variant, C->tune_us))) return rc;
+  if ((rc = tune(0, C->elem_variant, C->tune_us))) return rc;
   if (C->d_KEf) {
     uftk::fillf(C->s, C->ndof, 1.0f, C->d_rf);
-    if ((rc = tune(true, C->elem_variant_f, C->tune_us_f))) return rc;
+    if ((rc = tune(1, C->elem_variant_f, C->tune_us_f))) return rc;
   }
+  if (C->d_KEh)
+    if ((rc = tune(2, C->elem_variant_h, C->tune_us_h))) return rc;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (variant) {
     variant[0] = C->elem_variant;
     if (nmax > 1) variant[1] = C->elem_variant_f;
+    if (nmax > 2) variant[2] = C->elem_variant_h;
   }
   if (us_per_variant)
     for (int v = 0; v < nv && v < nmax; ++v) us_per_variant[v] = C->tune_us[v];

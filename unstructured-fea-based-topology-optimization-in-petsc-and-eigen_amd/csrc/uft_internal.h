@@ -12,6 +12,26 @@
 #define UFT_MAXM 8
 #define UFT_MAXLEV 8
 #define UFT_RED_BLOCK 256
+#define UFT_TAIL_MAX 3072  // coarsest unknowns handled by the fused coarse tail (LDS)
+
+// Per-row epilogue of a fused operator apply t = A u (see kernels_cycle.hip):
+//  STORE     y = t
+//  STEP      r -= t; dout = c1 u + c2 D^-1 r; x += dout (+ u when addprev)
+//  RES       r = base - t
+//  RESFIRST  r = base - t; dout = it D^-1 r
+//  RESFIRSTX r = kr = base - t; dout = kd = it D^-1 r
+//  DOTS2     y = t; partials of u.t, u.w1
+//  DOTS3     partials of u.w1, u.t, u.w2
+enum { ROW_STORE = 0, ROW_STEP, ROW_RES, ROW_RESFIRST, ROW_RESFIRSTX, ROW_DOTS2, ROW_DOTS3 };
+template <typename T>
+struct RowOp {
+  int mode = ROW_STORE;
+  int addprev = 0;
+  T *y = nullptr, *r = nullptr, *dout = nullptr, *x = nullptr, *kr = nullptr, *kd = nullptr;
+  const T *base = nullptr, *dinv = nullptr, *w1 = nullptr, *w2 = nullptr;
+  double c1 = 0.0, c2 = 0.0, it = 0.0;
+  double *part = nullptr;
+};
 
 // E(x) = Emin + x^penal * (Emax - Emin)   (SolveFem.cpp:304)
 struct SimpCoef {
@@ -103,6 +123,8 @@ struct uft_ctx {
   double *d_xyz = nullptr;     // nn x 3
   double *d_KE = nullptr;      // packed upper triangle, pair-interleaved [np/2][ne][2]
   float *d_KEf = nullptr;      // FP32 copy for the mixed form [np/2][ne][2]
+  uint2 *d_KEh = nullptr;      // FP16 smoother copy, KE/s_e in groups of four [ng][ne]
+  float *d_KEs = nullptr;      // per element scale s_e = max |KE_e|
   double *d_KF = nullptr;      // Helmholtz packed [npf/2][ne][2]
   double *d_T = nullptr;       // [npe][ne]
   double *d_detmin = nullptr;  // per element minimum det J
@@ -164,8 +186,8 @@ struct uft_ctx {
   double t_phase[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 
   // element-phase kernel variants chosen by uft_autotune
-  int elem_variant = 0, elem_variant_f = 0;
-  double tune_us[16] = {0}, tune_us_f[16] = {0};
+  int elem_variant = 0, elem_variant_f = 0, elem_variant_h = 0;
+  double tune_us[16] = {0}, tune_us_f[16] = {0}, tune_us_h[16] = {0};
 
   // stats
   int it_el = 0, it_f = 0, outer_el = 0, dip_steps = 0;
@@ -192,6 +214,9 @@ void apply_Kf(hipStream_t s, const uft_ctx *C, const double *xphys, const float 
 int elem_variant_count();
 void elem_phase(hipStream_t s, const uft_ctx *C, int variant, const double *xphys, const double *u, double *Ye);
 void elem_phase_f(hipStream_t s, const uft_ctx *C, int variant, const double *xphys, const float *u, float *Ye);
+void elem_phase_h(hipStream_t s, const uft_ctx *C, int variant, const double *xphys, const float *u, float *Ye);
+void ke_to_half(hipStream_t s, const uft_ctx *C);
+int ke_half_groups(int npe);
 void diag_K(hipStream_t s, const uft_ctx *C, const double *xphys, double *dinv, float *dinvf);
 void sens(hipStream_t s, const uft_ctx *C, const double *xphys, const double *u, double *dfdx,
           double *dgdx0, double *part);
@@ -264,6 +289,34 @@ void bsr_to_dense(hipStream_t s, int64_t nb_rows, const int32_t *rp, const int32
 void dense_spd_inverse(hipStream_t s, int64_t n, double *A, double *Ainv, float *Ainvf);
 void dense_mv(hipStream_t s, int64_t n, const double *A, const double *x, double *y);
 void dense_mvf(hipStream_t s, int64_t n, const float *A, const float *x, float *y);
+// fused cycle kernels (kernels_cycle.hip)
+void bsr6_op(hipStream_t s, int64_t nb_rows, const int32_t *rp, const int32_t *ci, const double *val, const double *u,
+             const RowOp<double> &op);
+void bsr6_op(hipStream_t s, int64_t nb_rows, const int32_t *rp, const int32_t *ci, const float *val, const float *u,
+             const RowOp<float> &op);
+int bsr6_op_blocks(int64_t nb_rows);
+void gather_op(hipStream_t s, const uft_ctx *C, const double *u, const double *Ye, const RowOp<double> &op);
+void gather_op(hipStream_t s, const uft_ctx *C, const float *u, const float *Ye, const RowOp<float> &op);
+int gather_op_blocks(int64_t nn);
+void restrict_first(hipStream_t s, int bs, int64_t nc, const int32_t *cptr, const int32_t *child, const double *P,
+                    const double *r, double *bc, const double *dinv, double it, double *rc, double *dc, double *xc);
+void restrict_first(hipStream_t s, int bs, int64_t nc, const int32_t *cptr, const int32_t *child, const float *P,
+                    const float *r, float *bc, const float *dinv, double it, float *rc, float *dc, float *xc);
+void prolong_comb(hipStream_t s, int bs, int64_t nu, const int32_t *parent, const double *P, const double *sc, int ia,
+                  int ib, const double *xa, const double *xb, double *x);
+void prolong_comb(hipStream_t s, int bs, int64_t nu, const int32_t *parent, const float *P, const double *sc, int ia,
+                  int ib, const float *xa, const float *xb, float *x);
+void lincomb_first(hipStream_t s, int64_t n, const double *sc, const double *b, const double *v, double *kr, double *r,
+                   const double *dinv, double it, double *d, double *kd);
+void lincomb_first(hipStream_t s, int64_t n, const double *sc, const float *b, const float *v, float *kr, float *r,
+                   const float *dinv, double it, float *d, float *kd);
+void final_kc1(hipStream_t s, const double *part, int nb, double *sc);
+void final_kc2(hipStream_t s, const double *part, int nb, double *sc);
+// xc = Ainv P^T r on the coarsest level (one workgroup; ncoarse <= UFT_TAIL_MAX)
+void tail_solve(hipStream_t s, int nc_units, const int32_t *cptr, const int32_t *child, const double *P,
+                const double *Ainv, const double *r, double *xc);
+void tail_solve(hipStream_t s, int nc_units, const int32_t *cptr, const int32_t *child, const float *P,
+                const double *Ainv, const float *r, float *xc);
 void fixed_unit_diag(hipStream_t s, int64_t nb_rows, const int32_t *rp, const int32_t *ci, double *val);
 // MMA
 void mma_movelimit(hipStream_t s, int64_t n, const double *x, double Xmin, double Xmax, double movlim,

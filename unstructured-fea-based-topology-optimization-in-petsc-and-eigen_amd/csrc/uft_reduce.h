@@ -36,4 +36,32 @@ __device__ __forceinline__ void block_partials(double (&v)[K], double *part) {
   }
 }
 
+// One block (RTPB threads) sums the K columns of nb partial rows in a fixed
+// order: strided per-thread sums, a 64-lane shuffle tree, then the four waves
+// in order.  Every thread receives the K results.
+template <int K>
+__device__ __forceinline__ void block_final(const double *__restrict__ part, int nb, double (&res)[K]) {
+  double v[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) v[k] = 0.0;
+  for (int i = threadIdx.x; i < nb; i += RTPB) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] += part[(size_t)i * K + k];
+  }
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_down(v[k], off, 64);
+  }
+  __shared__ double sm[K][RTPB / 64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) sm[k][w] = v[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; ++k) res[k] = (sm[k][0] + sm[k][1]) + (sm[k][2] + sm[k][3]);
+}
+
 }  // namespace uftr

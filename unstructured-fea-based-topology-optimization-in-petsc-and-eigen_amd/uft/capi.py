@@ -13,7 +13,7 @@ _PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 _ROOT = os.path.dirname(_PKG)
 HEADER = os.path.join(_ROOT, "include", "uft.h")
 
-FP64, MIXED = 0, 1
+FP64, MIXED, MIXED_H = 0, 1, 2
 VEC = {"X": 0, "XFiled": 1, "XPhys": 2, "Xold": 3, "dfdx": 4, "xmin": 5, "xmax": 6, "Vol": 7, "U": 8,
        "RX": 9, "dgdx0": 16}
 SIZE = {"ne": 0, "nn": 1, "ndof": 2, "npe": 3}
