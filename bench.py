@@ -7,7 +7,7 @@ design change, density filter, projection) on the 392,000-element jittered
 hex8 cantilever (140x70x40 on [0,2]x[0,1]x[0,1], interior nodes jittered
 +-0.15h from seed 392000, E0=1, Emin=1e-9, nu=0.3, penal=3, volfrac=0.3,
 rmin=1.5h, x=0 clamped, -1/41 on the y-DOFs of the bottom edge of x=2),
-FP64, Krylov rtol 1e-8 for both solvers.  All inputs are resident in HBM
+Krylov rtol 1e-8 for both solvers.  All inputs are resident in HBM
 before the timed region; setup (element operators, hierarchy) is reported
 separately.
 
@@ -15,7 +15,13 @@ Prints one JSON line.  Extra fields: roofline (element phase of the fine
 stiffness apply, timed with HIP events on its launch stream) and
 cpu_baseline (the CPU oracle on a bounded sample of the same workload).
 
-  python bench.py [--steps K] [--warmup W] [--precision fp64|mixed] [--no-cpu-baseline]
+  python bench.py [--steps K] [--warmup W] [--precision both|fp64|mixed|mixed_h] [--no-cpu-baseline]
+
+By default both the FP64 form and the mixed form (FP32 operator and
+multigrid cycle with FP16 smoother blocks inside the FP64 flexible outer
+loop) are timed on the same workload; value is the mixed form's rate and
+config carries the FP64 rate, their ratio and the largest relative
+compliance difference between the two forms over the timed steps.
 """
 import argparse
 import json
@@ -53,10 +59,13 @@ def workload(name, precision):
 
 
 def algorithmic_bytes_elem_phase(npe, ne, ndof, w=8):
-    """Element phase of one fine stiffness apply: the packed unique KE entries,
-    the element's node ids and density per element, plus one read of u."""
+    """Element phase of one fine stiffness apply: the packed unique KE entries
+    (w bytes each), the element's node ids and FP64 density per element, plus
+    one read of u (FP64 for w = 8, else FP32); FP16 blocks (w = 2) add the
+    FP32 per-element scale."""
     S = 300 if npe == 8 else 78
-    return ne * (S * w + 4 * npe + 8) + ndof * w
+    wv = 8 if w == 8 else 4
+    return ne * (S * w + 4 * npe + 8 + (4 if w == 2 else 0)) + ndof * wv
 
 
 def algorithmic_bytes_apply(npe, ne, ndof, w=8):
@@ -87,13 +96,81 @@ def cpu_baseline(name, gpu_iters):
             "seconds_per_iter": t_iter, "components": c}
 
 
+PREC = {"fp64": 0, "mixed": 1, "mixed_h": 2}
+
+
+def run_form(args, precision):
+    """Set up the workload in one precision form and time args.steps design
+    iterations after args.warmup untimed ones."""
+    import ctypes
+    import uft
+    from uft.capi import Context
+
+    npe, dims, seed, lk, prm = workload(args.workload, PREC[precision])
+    xyz, elem = uft.gen_box(npe, *dims, jitter=0.15, seed=seed)
+    fixed, ld, lv = uft.box_bcs(*dims, load_kind=lk)
+    ctx = Context(0)
+    ctx.set_params(Context.default_params(**prm))
+    ctx.set_mesh(elem, xyz)
+    ctx.set_bcs(fixed, ld, lv)
+    t0 = time.time()
+    ctx.setup()
+    ctx.sync()
+    setup_s = time.time() - t0
+    ne, ndof = ctx.size("ne"), ctx.size("ndof")
+    ctx.init_design()
+    itr = 0
+    for _ in range(args.warmup):
+        itr += 1
+        ctx.design_iteration(itr)
+    L = ctx.L
+    L.uft_profile_enable(ctx.h, 1)
+    its, fxs = [], []
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        itr += 1
+        fx, gx, ch = ctx.design_iteration(itr)
+        its.append(ctx.stats()["it_el"])
+        fxs.append(fx)
+    ctx.sync()
+    t1 = time.perf_counter()
+    prof = {}
+    for kind in range(4):
+        cnt, ms = ctypes.c_int64(), ctypes.c_double()
+        L.uft_profile_read(ctx.h, kind, ctypes.byref(cnt), ctypes.byref(ms))
+        prof[kind] = (cnt.value, ms.value)
+    L.uft_profile_enable(ctx.h, 0)
+    ctx.close()
+    return {"npe": npe, "dims": dims, "seed": seed, "ne": ne, "ndof": ndof, "setup_s": setup_s, "dt": t1 - t0,
+            "its": its, "fxs": fxs, "prof": prof}
+
+
+def elem_roofline(name, npe, ne, ndof, prof_elem, prof_node, word, kernel, traffic_key):
+    """HBM roofline of an element phase from its HIP-event timings."""
+    n, ms = prof_elem
+    algo = algorithmic_bytes_elem_phase(npe, ne, ndof, word)
+    avg_s = ms / max(n, 1) / 1e3
+    achieved = algo / avg_s / 1e9 if n else None
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tfile):
+        traffic = json.load(open(tfile)).get(traffic_key)
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": traffic, "kernel": kernel,
+            "algorithmic_bytes_per_launch": algo, "launches": n, "avg_launch_us": avg_s * 1e6,
+            "total_ms_timed": ms, "node_phase_avg_us": prof_node[1] / max(prof_node[0], 1) * 1e3}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="c1", choices=sorted(WORKLOADS))
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--precision", default="both", choices=["both", "fp64", "mixed", "mixed_h"],
+                    help="both (default): time the FP64 form and the mixed form (FP16 smoother blocks); "
+                         "the value is the mixed form's rate")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy three times (reference for rocprof PMC scaling)")
@@ -112,92 +189,62 @@ def main():
               "see DESIGN.md for how the path would shard). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
-    import uft
-    from uft.capi import Context
-    import ctypes
-
-    prec = uft.FP64 if args.precision == "fp64" else uft.MIXED
-    npe, dims, seed, lk, prm = workload(args.workload, prec)
-    xyz, elem = uft.gen_box(npe, *dims, jitter=0.15, seed=seed)
-    fixed, ld, lv = uft.box_bcs(*dims, load_kind=lk)
-    ctx = Context(0)
-    ctx.set_params(Context.default_params(**prm))
-    ctx.set_mesh(elem, xyz)
-    ctx.set_bcs(fixed, ld, lv)
-    t0 = time.time()
-    ctx.setup()
-    ctx.sync()
-    setup_s = time.time() - t0
-    ne, ndof = ctx.size("ne"), ctx.size("ndof")
-
-    ctx.init_design()
-    itr = 0
-    for _ in range(args.warmup):
-        itr += 1
-        ctx.design_iteration(itr)
-    L = ctx.L
-    L.uft_profile_enable(ctx.h, 1)
-    its, itf = [], []
-    ctx.sync()
-    t0 = time.perf_counter()
-    fx = None
-    for _ in range(args.steps):
-        itr += 1
-        fx, gx, ch = ctx.design_iteration(itr)
-        st = ctx.stats()
-        its.append(st["it_el"])
-    ctx.sync()
-    t1 = time.perf_counter()
-    cnt, ms = ctypes.c_int64(), ctypes.c_double()
-    L.uft_profile_read.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int64),
-                                   ctypes.POINTER(ctypes.c_double)]
-    L.uft_profile_enable.argtypes = [ctypes.c_void_p, ctypes.c_int]
-    L.uft_profile_read(ctx.h, 0, ctypes.byref(cnt), ctypes.byref(ms))
-    n_elem, ms_elem = cnt.value, ms.value
-    L.uft_profile_read(ctx.h, 1, ctypes.byref(cnt), ctypes.byref(ms))
-    n_node, ms_node = cnt.value, ms.value
-    L.uft_profile_enable(ctx.h, 0)
-    dt = t1 - t0
-    value = args.steps / dt
-
-    w = 8 if prec == uft.FP64 else 4
-    algo_elem = algorithmic_bytes_elem_phase(npe, ne, ndof, 8)
-    avg_elem_s = ms_elem / max(n_elem, 1) / 1e3
-    achieved = algo_elem / avg_elem_s / 1e9 if n_elem else None
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", "traffic.json")
-    if os.path.exists(tfile):
-        traffic = json.load(open(tfile)).get(f"{args.workload}_{args.precision}_elem_apply")
-    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": traffic,
-                "kernel": f"k_elem_apply<{npe},double> (element phase of the fine stiffness apply)",
-                "algorithmic_bytes_per_launch": algo_elem, "launches": n_elem,
-                "avg_launch_us": avg_elem_s * 1e6,
-                "full_apply": {"algorithmic_bytes": algorithmic_bytes_apply(npe, ne, ndof, w),
-                               "avg_us": (ms_elem + ms_node) / max(n_elem, 1) * 1e3,
-                               "GBps": algorithmic_bytes_apply(npe, ne, ndof, w) /
-                               ((ms_elem + ms_node) / max(n_elem, 1) / 1e3) / 1e9 if n_elem else None}}
+    forms = ["fp64", "mixed_h"] if args.precision == "both" else [args.precision]
+    res = {f: run_form(args, f) for f in forms}
+    main_form = forms[-1]
+    r = res[main_form]
+    npe, dims, seed, ne, ndof = r["npe"], r["dims"], r["seed"], r["ne"], r["ndof"]
+    value = args.steps / r["dt"]
+    ename = "hex8" if npe == 8 else "tet4"
+    rooflines = []
+    if "fp64" in res:
+        q = res["fp64"]
+        rooflines.append(elem_roofline("fp64", npe, ne, ndof, q["prof"][0], q["prof"][1], 8,
+                                       f"k_elem_apply_v<{npe},double> (FP64 element phase of the fine stiffness "
+                                       f"apply, FP64 form)", f"{args.workload}_fp64_elem_apply"))
+    if main_form != "fp64":
+        w = 2 if main_form == "mixed_h" else 4
+        kname = (f"k_elem_apply_h<{npe}> (FP16-block element phase of the smoother, mixed form)"
+                 if main_form == "mixed_h" else f"k_elem_apply_v<{npe},float> (FP32 element phase, mixed form)")
+        rooflines.append(elem_roofline(main_form, npe, ne, ndof, r["prof"][2], r["prof"][3], w, kname,
+                                       f"{args.workload}_{main_form}_elem_apply"))
+        rooflines.append(elem_roofline("outer", npe, ne, ndof, r["prof"][0], r["prof"][1], 8,
+                                       f"k_elem_apply_v<{npe},double> (FP64 element phase of the outer loop, "
+                                       f"mixed form)", f"{args.workload}_fp64_elem_apply"))
+    # the dominant element phase of the timed form: the one with the most time
+    cand = [x for x in rooflines if x["launches"]] if main_form == "fp64" else \
+        [x for x in rooflines[-2:] if x["launches"]]
+    dom = max(cand, key=lambda x: x["total_ms_timed"])
+    config = {"workload": f"{args.workload}: {ne} {ename} jittered cantilever {dims[0]}x{dims[1]}x{dims[2]}, "
+                          f"seed {seed}, rtol 1e-8, {main_form} form",
+              "global_batch": 1, "seq_len": None, "parallelism": "single GPU", "precision": main_form,
+              "setup_s": r["setup_s"], "krylov_iters_per_solve": sum(r["its"]) / len(r["its"]),
+              "final_fx": r["fxs"][-1]}
+    if "fp64" in res and main_form != "fp64":
+        f64 = res["fp64"]
+        config["fp64_design_iters_per_s"] = args.steps / f64["dt"]
+        config["mixed_over_fp64"] = value / config["fp64_design_iters_per_s"]
+        config["fp64_krylov_iters_per_solve"] = sum(f64["its"]) / len(f64["its"])
+        config["max_rel_compliance_diff_vs_fp64"] = max(abs(a - b) / abs(b) for a, b in zip(r["fxs"], f64["fxs"]))
     out = {
         "metric": "design iters/s @392k hex8 cantilever; mixed vs FP64 GPU at 1e-6 rel compliance",
         "value": value, "unit": "design iters/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": dt / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64" if prec == uft.FP64 else "f32+f64",
+        "ms_per_step": r["dt"] / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if main_form == "fp64" else "f32+f64",
         "data": "synthetic",
-        "config": {"workload": f"{args.workload}: {ne} {'hex8' if npe == 8 else 'tet4'} jittered cantilever "
-                               f"{dims[0]}x{dims[1]}x{dims[2]}, seed {seed}, FP64 drop-in, rtol 1e-8",
-                   "global_batch": 1, "seq_len": None, "parallelism": "single GPU",
-                   "precision": args.precision, "setup_s": setup_s,
-                   "krylov_iters_per_solve": sum(its) / len(its), "final_fx": fx},
-        "roofline": roofline,
+        "config": config,
+        "roofline": {k: v for k, v in dom.items()},
+        "rooflines": rooflines,
     }
     if not args.no_cpu_baseline and args.workload in ("c1", "c0", "c2"):
         try:
             cb = cpu_baseline(args.workload, {"pcg_fallback": 2000, "filter_fallback": 60})
             out["cpu_baseline"] = cb
             out["vs_cpu_baseline"] = value / cb["value"]
+            if "fp64" in res:
+                out["fp64_vs_cpu_baseline"] = config.get("fp64_design_iters_per_s", value) / cb["value"]
         except Exception as e:  # the baseline is reported, never required
             out["cpu_baseline"] = {"value": None, "error": repr(e)}
-    ctx.close()
     print(json.dumps(out))
 
 

@@ -66,6 +66,9 @@ typedef struct uft_params {
   int cheb_degree;     /* Chebyshev-Jacobi smoother degree per level */
   int coarse_max;      /* stop coarsening at or below this many coarse DOFs */
   int cycle_type;      /* coarse-grid cycle: 0 V, 1 W, 2 K-cycle on every coarse level, 3 K-cycle on level 1 only */
+  double cheb_lo, cheb_hi; /* Chebyshev interval [cheb_lo, cheb_hi] * lambda_max estimate
+                              (-mg_levels_ksp_chebyshev_esteig 0,lo,0,hi; PCGAMG: 0.1, 1.1) */
+  int esteig_steps;    /* Lanczos steps of each level's lambda_max(D^-1 A) estimate */
 } uft_params;
 
 /* --- context ------------------------------------------------------------ */

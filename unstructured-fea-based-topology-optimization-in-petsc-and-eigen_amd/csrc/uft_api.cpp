@@ -114,14 +114,15 @@ void level_apply(uft_ctx *C, int l, const double *x, double *y) {
     uftk::bsr6_mv(C->s, Lv.nunit, Lv.d_rp, Lv.d_ci, Lv.d_val, x, y);
 }
 
-// Chebyshev interval [0.1, 1.1] * lambda_max(D^-1 A), the PCGAMG default
-// (-mg_levels_ksp_chebyshev_esteig 0,0.1,0,1.1).
+// Chebyshev interval [lo, hi] * lambda_max(D^-1 A); PCGAMG's default is
+// [0.1, 1.1] (-mg_levels_ksp_chebyshev_esteig 0,0.1,0,1.1).
 struct ChebCoef {
   double inv_theta, c1[16], c2[16];
 };
-ChebCoef cheb_coef(double lmax, int deg) {
+// lmax: the level's estimate already scaled by cheb_hi; emin = lo/hi of it.
+ChebCoef cheb_coef(double lmax, int deg, double lo_over_hi) {
   ChebCoef cc{};
-  const double emax = 1.1 * lmax, emin = 0.1 * lmax;
+  const double emax = lmax, emin = lo_over_hi * lmax;
   const double theta = 0.5 * (emax + emin), delta = 0.5 * (emax - emin);
   const double sigma = theta / delta;
   cc.inv_theta = 1.0 / theta;
@@ -212,7 +213,7 @@ void coarse_body(uft_ctx *C, int l) {
   AmgLevel &Lv = C->amg.L[l];
   LV<T> v = lv(Lv, (T *)nullptr);
   const int kind = level_kind(C, l);
-  const double it = cheb_coef(Lv.lmax, std::max(1, C->P.cheb_degree)).inv_theta;
+  const double it = cheb_coef(Lv.lmax, std::max(1, C->P.cheb_degree), C->P.cheb_lo / C->P.cheb_hi).inv_theta;
   if (kind == 0) {
     cycle_body<T>(C, l, v.b, v.x);
   } else if (kind == 1) {
@@ -304,7 +305,7 @@ void cycle_body(uft_ctx *C, int l, const T *b, T *x) {
   AmgLevel &Lv = H.L[l];
   LV<T> v = lv(Lv, (T *)nullptr);
   const int deg = std::max(1, C->P.cheb_degree);
-  const ChebCoef cc = cheb_coef(Lv.lmax, deg);
+  const ChebCoef cc = cheb_coef(Lv.lmax, deg, C->P.cheb_lo / C->P.cheb_hi);
   T *dc = v.d, *dn = v.t;
   RowOp<T> st;
   st.mode = ROW_STEP;
@@ -329,7 +330,7 @@ void cycle_body(uft_ctx *C, int l, const T *b, T *x) {
   } else {
     AmgLevel &Ln = H.L[l + 1];
     LV<T> vn = lv(Ln, (T *)nullptr);
-    const double itn = cheb_coef(Ln.lmax, deg).inv_theta;
+    const double itn = cheb_coef(Ln.lmax, deg, C->P.cheb_lo / C->P.cheb_hi).inv_theta;
     T *xn = level_kind(C, l + 1) == 2 ? vn.kc : vn.x;
     uftk::restrict_first(C->s, Lv.bs, Ln.nunit, Lv.d_cptr, Lv.d_child, tr_P(Lv, (T *)nullptr), v.r, vn.b, vn.dinv,
                          itn, vn.r, vn.d, xn);
@@ -365,7 +366,7 @@ template <typename T>
 void fine_cycle(uft_ctx *C, const T *b, T *x) {
   AmgLevel &L0 = C->amg.L[0];
   LV<T> v = lv(L0, (T *)nullptr);
-  const double it = cheb_coef(L0.lmax, std::max(1, C->P.cheb_degree)).inv_theta;
+  const double it = cheb_coef(L0.lmax, std::max(1, C->P.cheb_degree), C->P.cheb_lo / C->P.cheb_hi).inv_theta;
   k_cheb_first(C->s, L0.n, b, v.dinv, it, v.r, v.d, x, 1);
   cycle_body<T>(C, 0, b, x);
 }
@@ -439,11 +440,11 @@ int estimate_lmax(uft_ctx *C, int l, int ksteps) {
     e.push_back(std::sqrt(be) / al);
   }
   if (d.empty()) {
-    Lv.lmax = 2.0;
+    Lv.lmax = 2.0 * C->P.cheb_hi;
     return UFT_OK;
   }
   e.resize(d.size());
-  Lv.lmax = tridiag_lmax(d, e);
+  Lv.lmax = C->P.cheb_hi * tridiag_lmax(d, e);
   return UFT_OK;
 }
 
@@ -468,7 +469,7 @@ int amg_refresh(uft_ctx *C, const double *xphys) {
   uftk::bsr_to_dense(s, Lc.nunit, Lc.d_rp, Lc.d_ci, Lc.d_val, H.ncoarse, H.d_cmat);
   uftk::dense_spd_inverse(s, H.ncoarse, H.d_cmat, H.d_cinv, H.d_cinvf);
   for (int l = 0; l + 1 < H.nlev; ++l) {
-    int rc = estimate_lmax(C, l, 12);
+    int rc = estimate_lmax(C, l, std::min(64, std::max(2, C->P.esteig_steps)));
     if (rc) return rc;
   }
   return UFT_OK;
@@ -679,6 +680,9 @@ void uft_default_params(uft_params *p) {
   p->cheb_degree = 2;
   p->coarse_max = 600;
   p->cycle_type = 2;
+  p->cheb_lo = 0.1;
+  p->cheb_hi = 1.1;
+  p->esteig_steps = 12;
 }
 
 int uft_create(uft_ctx **out, int device) {
@@ -717,6 +721,8 @@ int uft_params_set(uft_ctx *C, const uft_params *p) {
   if (p->m < 1 || p->m > UFT_MAXM) return uft_set_error(C, UFT_ERR_ARG, "m must be in [1, %d]", UFT_MAXM);
   if (!(p->rtol_el > 0) || !(p->rtol_f > 0) || p->maxit_el < 1 || p->maxit_f < 1)
     return uft_set_error(C, UFT_ERR_ARG, "bad solver tolerances");
+  if (!(p->cheb_lo >= 0.0) || !(p->cheb_hi > p->cheb_lo) || p->esteig_steps < 2 || p->esteig_steps > 64)
+    return uft_set_error(C, UFT_ERR_ARG, "bad Chebyshev interval or estimate steps");
   if (p->precision < UFT_FP64 || p->precision > UFT_MIXED_H)
     return uft_set_error(C, UFT_ERR_ARG, "precision must be UFT_FP64, UFT_MIXED or UFT_MIXED_H");
   if (C->is_setup && (p->m != C->P.m || p->rmin != C->P.rmin || p->nu != C->P.nu || p->precision != C->P.precision ||

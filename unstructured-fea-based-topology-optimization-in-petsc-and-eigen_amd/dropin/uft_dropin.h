@@ -129,8 +129,14 @@ class OptPara {
     params.nu = R.real("-nu", params.nu);
     params.rtol_el = R.real("-ksp_rtol", params.rtol_el);  // unprefixed: both KSPs (SolveFem.cpp:135, PDEFilter.cpp:135)
     params.rtol_f = R.real("-ksp_rtol", params.rtol_f);
-    params.precision = R.integer("-mixed", 0) ? UFT_MIXED : UFT_FP64;
+    // -mixed 1: FP32 preconditioner blocks; -mixed 2: FP16 smoother blocks
+    {
+      const int mx = R.integer("-mixed", 0);
+      params.precision = mx == 2 ? UFT_MIXED_H : (mx ? UFT_MIXED : UFT_FP64);
+    }
     params.cheb_degree = R.integer("-cheb_degree", params.cheb_degree);
+    params.cycle_type = R.integer("-cycle_type", params.cycle_type);
+    params.esteig_steps = R.integer("-esteig_steps", params.esteig_steps);
     std::printf("################### Optimization settings ####################\n");
     std::printf("# -rmin: %f\n# -volfrac: %f\n# -penal: %f\n# -Emin/-Emax: %e - %e \n", rmin, volfrac, penal, Emin, Emax);
     std::printf("# -Xmin/-Xmax: %e - %e \n# -maxItr: %i\n# -movlim: %f\n# -x0: %f\n", Xmin, Xmax, maxItr, movlim, x0);

@@ -34,7 +34,8 @@ class Params(C.Structure):
                [("proj", C.c_int), ("m", C.c_int), ("maxItr", C.c_int),
                 ("rtol_el", C.c_double), ("rtol_f", C.c_double), ("maxit_el", C.c_int), ("maxit_f", C.c_int),
                 ("precision", C.c_int), ("amg_levels", C.c_int), ("cheb_degree", C.c_int),
-                ("coarse_max", C.c_int), ("cycle_type", C.c_int)]
+                ("coarse_max", C.c_int), ("cycle_type", C.c_int),
+                ("cheb_lo", C.c_double), ("cheb_hi", C.c_double), ("esteig_steps", C.c_int)]
 
 
 _LIB = None
