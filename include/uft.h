@@ -41,6 +41,7 @@ enum uft_status {
 enum uft_vec_id {
   UFT_X = 0, UFT_XFILED = 1, UFT_XPHYS = 2, UFT_XOLD = 3, UFT_DFDX = 4,
   UFT_XMIN = 5, UFT_XMAX = 6, UFT_VOL = 7, UFT_U = 8, UFT_RX = 9,
+  UFT_VMS = 10, /* von Mises stress per element (SolveFem::VMS, stress/ComputeStress.cpp:6) */
   UFT_DGDX0 = 16 /* UFT_DGDX0 + j for constraint j < m */
 };
 
@@ -157,6 +158,36 @@ int uft_element_kf(uft_ctx *ctx, int64_t e, double *kf, double *t);
  * hierarchy is internal to PETSc): level count (at most 8), unknowns per level
  * and, per level, the element count (level 0) or 6x6 block count. */
 int uft_solver_info(const uft_ctx *ctx, int *nlev, int64_t *n_per_level, int64_t *blocks_per_level);
+/* --- MMA settings, restart and KKT residual (MMA.h:8-52) ------------------ */
+/* MMA::SetAsymptotes (MMA.cpp:347-355): asymptote initialisation,
+ * decrease and increase factors (defaults 0.5, 0.7, 1.2). */
+int uft_mma_set_asymptotes(uft_ctx *ctx, double init, double decrease, double increase);
+/* MMA::SetRobustAsymptotesType (MMA.cpp:357-368): 0 (default) or 1; any
+ * other value selects 0 and returns UFT_ERR_ARG. */
+int uft_mma_set_robust_asymptotes(uft_ctx *ctx, int type);
+/* MMA::ConstraintModification (MMA.h:33-36): 1 adds the objective's
+ * regularisation to the constraint approximations (default 0). */
+int uft_mma_constraint_modification(uft_ctx *ctx, int on);
+/* User-defined subproblem penalisation a, c, d (m each) of
+ * MMA(n, m, x, a, c, d) (MMA.h:13; defaults a = 0, c = 1000, d = 0). */
+int uft_mma_set_penalization(uft_ctx *ctx, const double *a, const double *c, const double *d);
+/* MMA::Restart (MMA.cpp:302-342): the state a restart needs -- iteration
+ * counter k and xo1, xo2, U, L (host buffers of ne doubles). */
+int uft_mma_state_get(uft_ctx *ctx, int *k, double *xo1, double *xo2, double *U, double *L);
+/* The restart constructors MMA(n, m, k, xo1, xo2, U, L[, a, c, d])
+ * (MMA.cpp:5-176): load that state.  k < 3 is accepted with the reference's
+ * warning in uft_last_error ("not a legal restart point"). */
+int uft_mma_state_set(uft_ctx *ctx, int k, const double *xo1, const double *xo2, const double *U, const double *L);
+/* MMA::KKTresidual (MMA.cpp:412-480) of the last uft_mma_update's
+ * multipliers: 2-norm and inf-norm. */
+int uft_mma_kkt_residual(uft_ctx *ctx, int x_id, int dfdx_id, const double *gx, const int *dgdx_ids, int xmin_id,
+                         int xmax_id, double *norm2, double *norminf);
+/* SolveFem::ComputeStress (stress/ComputeStress.cpp:1-113): von Mises
+ * stress of the current displacement U at each element's centroid,
+ * sqrt(xphys) * Emax * D * B(0) * u_e, into vector vms_id (UFT_VMS).  For
+ * tet4 the constant strain is used (the reference's routine is hex8 only). */
+int uft_stress(uft_ctx *ctx, int xphys_id, int vms_id);
+
 /* Multigrid inspection (no counterpart in the reference): the 6x6-block
  * operator of level >= 1 as refreshed for the last solve (rp: units+1, ci and
  * val: blocks and 36*blocks, row-major blocks), and the transfer of level l to

@@ -15,7 +15,7 @@ _LIB = None
 
 VEC = {"X": 0, "XFiled": 1, "XPhys": 2, "Xold": 3, "dfdx": 4, "dgdx": 5, "xmin": 6,
        "xmax": 7, "Vol": 8, "U": 9, "RX": 10, "F": 11, "L": 12, "Uas": 13, "xo1": 14,
-       "xo2": 15, "Tel": 16, "KFdiag": 17}
+       "xo2": 15, "Tel": 16, "KFdiag": 17, "alpha": 18, "beta": 19, "p0": 20, "q0": 21, "pij": 22, "qij": 23}
 
 
 class Params(C.Structure):
@@ -81,6 +81,12 @@ def lib():
             "orc_init_design": (C.c_int, [vp]),
             "orc_run": (C.c_int, [vp, C.c_int, dp, dp, dp]),
             "orc_sample_costs": (C.c_int, [vp, C.c_int64, C.c_int, dp]),
+            "orc_mma_settings": (None, [vp, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int]),
+            "orc_mma_set_acd": (None, [vp, dp, dp, dp]),
+            "orc_mma_state_get": (None, [vp, C.POINTER(C.c_int), dp, dp, dp, dp]),
+            "orc_mma_state_set": (None, [vp, C.c_int, dp, dp, dp, dp]),
+            "orc_mma_kkt": (None, [vp, dp, dp, dp, dp, dp, dp, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+            "orc_stress": (None, [vp, dp, dp, dp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -190,7 +196,8 @@ class Problem:
 
     def vec(self, name):
         n = {"U": 3 * self.nn, "F": 3 * self.nn, "RX": self.nn, "KFdiag": self.nn,
-             "dgdx": self.m * self.ne, "Tel": self.npe * self.ne}.get(name, self.ne)
+             "dgdx": self.m * self.ne, "pij": self.m * self.ne, "qij": self.m * self.ne,
+             "Tel": self.npe * self.ne}.get(name, self.ne)
         p = lib().orc_vec(self.h, VEC[name])
         return np.ctypeslib.as_array(p, shape=(n,))
 
@@ -247,6 +254,36 @@ class Problem:
         lib().orc_sample_costs(self.h, ne_sample, n_it, out)
         return {"ke_per_elem": out[0], "asm_per_elem": out[1], "uku_per_elem": out[2], "pcg_iter": out[3],
                 "filter_iter": out[4], "mma_update": out[5]}
+
+    # ---- MMA settings, restart state and KKT residual
+    def mma_settings(self, init=0.5, dec=0.7, inc=1.2, robust=0, conmod=0):
+        lib().orc_mma_settings(self.h, init, dec, inc, robust, conmod)
+
+    def mma_set_acd(self, a, c, d):
+        f = lambda v: np.ascontiguousarray(v, dtype=np.float64)
+        lib().orc_mma_set_acd(self.h, f(a), f(c), f(d))
+
+    def mma_state(self):
+        k = C.c_int()
+        v = [np.zeros(self.ne) for _ in range(4)]
+        lib().orc_mma_state_get(self.h, C.byref(k), *v)
+        return k.value, v[0], v[1], v[2], v[3]
+
+    def mma_set_state(self, k, xo1, xo2, U, L):
+        f = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+        lib().orc_mma_state_set(self.h, k, f(xo1), f(xo2), f(U), f(L))
+
+    def mma_kkt(self, x, dfdx, gx, dgdx, xmin, xmax):
+        f = lambda a: np.ascontiguousarray(a, dtype=np.float64).ravel()
+        n2, ni = C.c_double(), C.c_double()
+        lib().orc_mma_kkt(self.h, f(x), f(dfdx), f(gx), f(dgdx), f(xmin), f(xmax), C.byref(n2), C.byref(ni))
+        return n2.value, ni.value
+
+    def stress(self, xphys, u):
+        out = np.zeros(self.ne)
+        lib().orc_stress(self.h, np.ascontiguousarray(xphys, dtype=np.float64),
+                         np.ascontiguousarray(u, dtype=np.float64), out)
+        return out
 
     def run(self, maxitr):
         fx = np.zeros(maxitr)

@@ -496,6 +496,7 @@ typedef struct {
   /* MMA state (MMA.h:76-109) */
   int mk;
   double asyminit, asymdec, asyminc;
+  int robust, conmod;   /* RobustAsymptotesType, constraintModification (MMA.h) */
   double a[ORC_MAXM], c[ORC_MAXM], d[ORC_MAXM], y[ORC_MAXM], z;
   double lam[ORC_MAXM], mu[ORC_MAXM], s[2 * ORC_MAXM], b[ORC_MAXM], grad[ORC_MAXM], Hess[ORC_MAXM * ORC_MAXM];
   double *L, *Uas, *alpha, *betav, *p0, *q0, *pij, *qij, *xo1, *xo2;
@@ -613,7 +614,7 @@ orc_ctx *orc_create(int npe, int64_t nn, int64_t ne, const int32_t *elem, const 
   /* OptPara.cpp:150-153 */
   for (int64_t e = 0; e < ne; e++) C->X[e] = C->XFiled[e] = C->XPhys[e] = C->Xold[e] = P->volfrac;
   /* MMA(n, m, x, a, c, d) with a=0, c=1000, d=0 (OptPara.cpp:166-174; MMA.cpp:178-224) */
-  C->mk = 0; C->asyminit = 0.5; C->asymdec = 0.7; C->asyminc = 1.2;
+  C->mk = 0; C->asyminit = 0.5; C->asymdec = 0.7; C->asyminc = 1.2; C->robust = 0; C->conmod = 0;
   for (int j = 0; j < m; j++) { C->a[j] = 0.0; C->c[j] = 1000.0; C->d[j] = 0.0; }
   C->L = calloc(ne, sizeof(double)); C->Uas = calloc(ne, sizeof(double));
   C->alpha = calloc(ne, sizeof(double)); C->betav = calloc(ne, sizeof(double));
@@ -643,6 +644,8 @@ double *orc_vec(orc_ctx *C, int which) {
     case 8: return C->vol; case 9: return C->U; case 10: return C->RX; case 11: return C->F;
     case 12: return C->L; case 13: return C->Uas; case 14: return C->xo1; case 15: return C->xo2;
     case 16: return C->Tel; case 17: return C->KFdiag;
+    case 18: return C->alpha; case 19: return C->betav; case 20: return C->p0; case 21: return C->q0;
+    case 22: return C->pij; case 23: return C->qij;
   }
   return NULL;
 }
@@ -653,6 +656,7 @@ void orc_diag(orc_ctx *C, int *it_el, int *it_f, double *relres_el, int *dip_ste
 void orc_set_params(orc_ctx *C, const orc_params *P) { C->P = *P; }
 int orc_mma_k(orc_ctx *C) { return C->mk; }
 double orc_mma_lam(orc_ctx *C, int j) { return C->lam[j]; }
+
 
 /* ------------------------------------------------------------------------- */
 /* SolveFem                                                                   */
@@ -1010,10 +1014,20 @@ void orc_mma_update(orc_ctx *C, double *x, const double *dfdx, const double *gx,
       double Li = x[i] - gm * (C->xo1[i] - C->L[i]);
       double Ui = x[i] + gm * (C->Uas[i] - C->xo1[i]);
       double xmi = dmax(1.0e-5, xmax[i] - xmin[i]);
-      Li = dmax(Li, x[i] - 10.0 * xmi);
-      Li = dmin(Li, x[i] - 0.01 * xmi);
-      Ui = dmax(Ui, x[i] + 0.01 * xmi);
-      Ui = dmin(Ui, x[i] + 10.0 * xmi);
+      if (C->robust == 0) {
+        Li = dmax(Li, x[i] - 10.0 * xmi);
+        Li = dmin(Li, x[i] - 0.01 * xmi);
+        Ui = dmax(Ui, x[i] + 0.01 * xmi);
+        Ui = dmin(Ui, x[i] + 10.0 * xmi);
+      } else { /* RobustAsymptotesType 1 (MMA.cpp:557-575) */
+        Li = dmax(Li, x[i] - 100.0 * xmi);
+        Li = dmin(Li, x[i] - 1.0e-4 * xmi);
+        Ui = dmax(Ui, x[i] + 1.0e-4 * xmi);
+        Ui = dmin(Ui, x[i] + 100.0 * xmi);
+        double lo = xmin[i] - 1.0e-5, hi = xmax[i] + 1.0e-5;
+        if (x[i] < lo) { Li = x[i] - (hi - x[i]) / 0.9; Ui = x[i] + (hi - x[i]) / 0.9; }
+        if (x[i] > hi) { Li = x[i] - (x[i] - lo) / 0.9; Ui = x[i] + (x[i] - lo) / 0.9; }
+      }
       C->L[i] = Li; C->Uas[i] = Ui;
     }
   }
@@ -1028,8 +1042,9 @@ void orc_mma_update(orc_ctx *C, double *x, const double *dfdx, const double *gx,
     C->q0[i] = pow(xi - Li, 2.0) * (dfm + 0.001 * fabs(dfdx[i]) + 0.5 * feps / (Ui - Li));
     for (int j = 0; j < m; j++) {
       double g = dgdx[j * n + i];
-      C->pij[j * n + i] = pow(Ui - xi, 2.0) * dmax(0.0, g);
-      C->qij[j * n + i] = pow(xi - Li, 2.0) * dmax(0.0, -1.0 * g);
+      double reg = C->conmod ? 0.001 * fabs(g) + 0.5 * feps / (Ui - Li) : 0.0; /* MMA.cpp:588-597 */
+      C->pij[j * n + i] = pow(Ui - xi, 2.0) * (dmax(0.0, g) + reg);
+      C->qij[j * n + i] = pow(xi - Li, 2.0) * (dmax(0.0, -1.0 * g) + reg);
     }
   }
   /* b = sum(p/(U-x) + q/(x-L)) - gx (MMA.cpp:603-620) */
@@ -1215,4 +1230,103 @@ int orc_sample_costs(orc_ctx *C, int64_t ne_sample, int n_it, double *out) {
   out[5] = wclock() - t0;
   out[6] = sink;
   return 0;
+}
+
+/* MMA settings: SetAsymptotes (MMA.cpp:347-355), SetRobustAsymptotesType
+ * (MMA.cpp:357-368), ConstraintModification (MMA.h:33-36), user a, c, d
+ * (MMA.h:13, OptPara.cpp:166-174). */
+void orc_mma_settings(orc_ctx *C, double init, double dec, double inc, int robust, int conmod) {
+  C->asyminit = init; C->asymdec = dec; C->asyminc = inc; C->robust = robust; C->conmod = conmod;
+}
+void orc_mma_set_acd(orc_ctx *C, const double *a, const double *c, const double *d) {
+  for (int j = 0; j < C->P.m; j++) { C->a[j] = a[j]; C->c[j] = c[j]; C->d[j] = d[j]; }
+}
+/* MMA::Restart (MMA.cpp:302-342): copy the state out; the restart
+ * constructors (MMA.cpp:5-176) copy it in with iteration counter k. */
+void orc_mma_state_get(orc_ctx *C, int *k, double *xo1, double *xo2, double *U, double *L) {
+  size_t b = C->ne * sizeof(double);
+  *k = C->mk; memcpy(xo1, C->xo1, b); memcpy(xo2, C->xo2, b); memcpy(U, C->Uas, b); memcpy(L, C->L, b);
+}
+void orc_mma_state_set(orc_ctx *C, int k, const double *xo1, const double *xo2, const double *U, const double *L) {
+  size_t b = C->ne * sizeof(double);
+  C->mk = k; memcpy(C->xo1, xo1, b); memcpy(C->xo2, xo2, b); memcpy(C->Uas, U, b); memcpy(C->L, L, b);
+}
+/* MMA::KKTresidual (MMA.cpp:412-480) with the multipliers lam, y, z of the
+ * last update. */
+void orc_mma_kkt(orc_ctx *C, const double *x, const double *dfdx, const double *gx, const double *dgdx,
+                 const double *xmin, const double *xmax, double *norm2, double *norminf) {
+  int m = C->P.m;
+  int64_t n = C->ne;
+  double n2 = 0.0, ni = 0.0;
+  for (int64_t i = 0; i < n; i++) {
+    double ri = dfdx[i];
+    for (int j = 0; j < m; j++) ri += C->lam[j] * dgdx[j * n + i];
+    double mumin = (x[i] < xmin[i] + 1.0e-5 && ri > 0.0) ? ri : 0.0;
+    double mumax = (x[i] > xmax[i] - 1.0e-5 && ri < 0.0) ? -ri : 0.0;
+    ri += -mumin + mumax;
+    double r1 = mumin * (x[i] - xmin[i]), r2 = mumax * (xmax[i] - x[i]);
+    n2 += ri * ri + r1 * r1 + r2 * r2;
+    ni = dmax(ni, dmax(fabs(ri), dmax(fabs(r1), fabs(r2))));
+  }
+  double rl = 0.0;
+  for (int j = 0; j < m; j++) rl += C->lam[j] * (C->a[j] * C->z + C->y[j] - gx[j]);
+  n2 += rl * rl;
+  *norm2 = sqrt(n2);
+  *norminf = dmax(ni, fabs(rl));
+}
+
+/* SolveFem::ComputeStress (stress/ComputeStress.cpp:53-104): centroid strain
+ * B(0,0,0) u_e of each hex8, sigma = Emax D B u_e scaled by sqrt(x_e), von
+ * Mises of the six components (shear order xy, yz, zx).  tet4: the element's
+ * constant strain (an extension: the reference has the hex8 routine only). */
+static int inv3_cof(const double *J, double *Ji) {
+  double det = J[0] * (J[4] * J[8] - J[5] * J[7]) - J[1] * (J[3] * J[8] - J[5] * J[6]) + J[2] * (J[3] * J[7] - J[4] * J[6]);
+  if (det == 0.0) return -1;
+  Ji[0] = (J[4] * J[8] - J[5] * J[7]) / det; Ji[1] = (J[2] * J[7] - J[1] * J[8]) / det; Ji[2] = (J[1] * J[5] - J[2] * J[4]) / det;
+  Ji[3] = (J[5] * J[6] - J[3] * J[8]) / det; Ji[4] = (J[0] * J[8] - J[2] * J[6]) / det; Ji[5] = (J[2] * J[3] - J[0] * J[5]) / det;
+  Ji[6] = (J[3] * J[7] - J[4] * J[6]) / det; Ji[7] = (J[1] * J[6] - J[0] * J[7]) / det; Ji[8] = (J[0] * J[4] - J[1] * J[3]) / det;
+  return 0;
+}
+void orc_stress(orc_ctx *C, const double *xphys, const double *u, double *vms) {
+  const int npe = C->npe;
+  const double nu = C->P.nu, Emax = C->P.Emax;
+  const double lam = nu / ((1.0 + nu) * (1.0 - 2.0 * nu)), mu = 1.0 / (2.0 * (1.0 + nu));
+  /* reference derivatives at the centroid: hex8 diagonal order, tet4 linear */
+  double dNr[3][8];
+  if (npe == 8) {
+    for (int a = 0; a < 8; a++) {
+      double sx = (a & 1) ? 1.0 : -1.0, sy = (a & 2) ? 1.0 : -1.0, sz = (a & 4) ? 1.0 : -1.0;
+      dNr[0][a] = 0.125 * sx; dNr[1][a] = 0.125 * sy; dNr[2][a] = 0.125 * sz;
+    }
+  } else {
+    double t[3][4] = {{-1, 1, 0, 0}, {-1, 0, 1, 0}, {-1, 0, 0, 1}};
+    for (int r = 0; r < 3; r++) for (int a = 0; a < 4; a++) dNr[r][a] = t[r][a];
+  }
+  for (int64_t e = 0; e < C->ne; e++) {
+    double J[9] = {0}, Ji[9];
+    for (int r = 0; r < 3; r++)
+      for (int a = 0; a < npe; a++) {
+        const double *X = &C->xyz[3 * (int64_t)C->elem[e * npe + a]];
+        for (int c = 0; c < 3; c++) J[r * 3 + c] += dNr[r][a] * X[c];
+      }
+    inv3_cof(J, Ji);
+    double g[3][8];
+    for (int r = 0; r < 3; r++)
+      for (int a = 0; a < npe; a++) g[r][a] = Ji[r * 3 + 0] * dNr[0][a] + Ji[r * 3 + 1] * dNr[1][a] + Ji[r * 3 + 2] * dNr[2][a];
+    double eps[6] = {0};
+    for (int a = 0; a < npe; a++) {
+      const double *ua = &u[3 * (int64_t)C->elem[e * npe + a]];
+      eps[0] += g[0][a] * ua[0];
+      eps[1] += g[1][a] * ua[1];
+      eps[2] += g[2][a] * ua[2];
+      eps[3] += g[1][a] * ua[0] + g[0][a] * ua[1];
+      eps[4] += g[2][a] * ua[1] + g[1][a] * ua[2];
+      eps[5] += g[2][a] * ua[0] + g[0][a] * ua[2];
+    }
+    double tr = eps[0] + eps[1] + eps[2], sc = Emax * sqrt(xphys[e]), sg[6];
+    for (int k = 0; k < 3; k++) sg[k] = sc * (lam * tr + 2.0 * mu * eps[k]);
+    for (int k = 3; k < 6; k++) sg[k] = sc * mu * eps[k];
+    vms[e] = (1.0 / sqrt(2.0)) * sqrt(pow(sg[0] - sg[1], 2) + pow(sg[0] - sg[2], 2) + pow(sg[1] - sg[2], 2) +
+                                      6.0 * (sg[3] * sg[3] + sg[4] * sg[4] + sg[5] * sg[5]));
+  }
 }

@@ -68,3 +68,29 @@ def test_dropin_driver_links(U):
     if not os.path.exists(exe):
         subprocess.run(["make", "-s", "-j8", "-C", conftest.PKG], check=True)
     assert os.access(exe, os.X_OK)
+
+
+def test_dropin_mma_restart_kkt_stress_compile(tmp_path):
+    """The drop-in classes expose the reference's MMA restart constructors,
+    Restart, SetAsymptotes, ConstraintModification, SetRobustAsymptotesType,
+    KKTresidual (MMA.h:8-52) and SolveFem::ComputeStress."""
+    src = tmp_path / "use.cpp"
+    src.write_text(r'''
+#include "uft_dropin.h"
+int use(SolveFem &fem, Vec X, Vec dfdx, Vec *dgdx, Vec xmin, Vec xmax, MeshRef e) {
+  const long long n = 16;
+  Vec xo1 = VecCreateHost(n), xo2 = VecCreateHost(n), U = VecCreateHost(n), L = VecCreateHost(n);
+  MMA a(n, 1, X);
+  a.Restart(xo1, xo2, U, L);
+  MMA b(n, 1, a.RestartIteration(), xo1, xo2, U, L);
+  double ac[1] = {0}, cc[1] = {1000}, dc[1] = {0}, gx[1] = {0}, n2 = 0, ni = 0;
+  MMA c(n, 1, 5, xo1, xo2, U, L, ac, cc, dc);
+  b.SetAsymptotes(0.5, 0.7, 1.2);
+  b.ConstraintModification(true);
+  b.SetRobustAsymptotesType(1);
+  b.KKTresidual(X, dfdx, gx, dgdx, xmin, xmax, &n2, &ni);
+  return fem.ComputeStress(X, 1.0, e, e, e, e) + (int)n2 + (int)ni;
+}
+''')
+    inc = [os.path.join(conftest.ROOT, "include"), os.path.join(conftest.PKG, "dropin")]
+    subprocess.run(["g++", "-std=c++17", "-fsyntax-only"] + [f"-I{d}" for d in inc] + [str(src)], check=True)

@@ -898,6 +898,7 @@ struct PtrArr {
 // coefficients; partial sums of sum_i pij/(U-x) + qij/(x-L) per constraint.
 template <int M>
 __global__ __launch_bounds__(TPB) void k_gensub(int64_t n, int k, double asyminit, double asymdec, double asyminc,
+                                                int robust, int conmod,
                                                 const double *__restrict__ x, const double *__restrict__ xo1,
                                                 const double *__restrict__ xo2, const double *__restrict__ xmin,
                                                 const double *__restrict__ xmax, const double *__restrict__ dfdx,
@@ -923,10 +924,26 @@ __global__ __launch_bounds__(TPB) void k_gensub(int64_t n, int k, double asymini
       Li = xi - gm * (x1 - L[i]);
       Ui = xi + gm * (U[i] - x1);
       const double w = fmax(1.0e-5, xma - xmi);
-      Li = fmax(Li, xi - 10.0 * w);
-      Li = fmin(Li, xi - 0.01 * w);
-      Ui = fmax(Ui, xi + 0.01 * w);
-      Ui = fmin(Ui, xi + 10.0 * w);
+      if (robust == 0) {
+        Li = fmax(Li, xi - 10.0 * w);
+        Li = fmin(Li, xi - 0.01 * w);
+        Ui = fmax(Ui, xi + 0.01 * w);
+        Ui = fmin(Ui, xi + 10.0 * w);
+      } else {  // RobustAsymptotesType 1 (MMA.cpp:557-575)
+        Li = fmax(Li, xi - 100.0 * w);
+        Li = fmin(Li, xi - 1.0e-4 * w);
+        Ui = fmax(Ui, xi + 1.0e-4 * w);
+        Ui = fmin(Ui, xi + 100.0 * w);
+        const double lo = xmi - 1.0e-5, hi = xma + 1.0e-5;
+        if (xi < lo) {
+          Li = xi - (hi - xi) / 0.9;
+          Ui = xi + (hi - xi) / 0.9;
+        }
+        if (xi > hi) {
+          Li = xi - (xi - lo) / 0.9;
+          Ui = xi + (xi - lo) / 0.9;
+        }
+      }
     }
     L[i] = Li;
     U[i] = Ui;
@@ -942,7 +959,9 @@ __global__ __launch_bounds__(TPB) void k_gensub(int64_t n, int k, double asymini
 #pragma unroll
     for (int j = 0; j < M; ++j) {
       const double g = dg.p[j][i];
-      const double pj = ux * ux * fmax(0.0, g), qj = xl * xl * fmax(0.0, -1.0 * g);
+      // constraintModification adds the objective's regularisation (MMA.cpp:588-597)
+      const double reg = conmod ? 0.001 * fabs(g) + 0.5 * feps / ul : 0.0;
+      const double pj = ux * ux * (fmax(0.0, g) + reg), qj = xl * xl * (fmax(0.0, -1.0 * g) + reg);
       pij[(size_t)j * n + i] = pj;
       qij[(size_t)j * n + i] = qj;
       v[j] = pj / ux + qj / xl;
@@ -1316,7 +1335,8 @@ static void gensub_m(hipStream_t s, uft_ctx *C, int64_t n, int k, const double *
                      const double *xmax, const double *dfdx, const double *const *dgdx, double *part) {
   PtrArr pa{};
   for (int j = 0; j < M; ++j) pa.p[j] = dgdx[j];
-  k_gensub<M><<<nblk(n), TPB, 0, s>>>(n, k, C->asyminit, C->asymdec, C->asyminc, x, C->d_xo1, C->d_xo2, xmin, xmax,
+  k_gensub<M><<<nblk(n), TPB, 0, s>>>(n, k, C->asyminit, C->asymdec, C->asyminc, C->mma_robust, C->mma_conmod, x,
+                                      C->d_xo1, C->d_xo2, xmin, xmax,
                                       dfdx, pa, C->d_L, C->d_U, C->d_alpha, C->d_beta, C->d_p0, C->d_q0, C->d_pij,
                                       C->d_qij, part);
 }
@@ -1364,4 +1384,182 @@ void projected_sens(hipStream_t s, int64_t n, const double *xf, double beta, dou
   k_projsens<<<nblk(n), TPB, 0, s>>>(n, xf, beta, x0, v);
 }
 
+}  // namespace uftk
+
+// ---------------------------------------------------------------- MMA KKT residual
+// MMA::KKTresidual (MMA.cpp:412-480): per variable the stationarity residual
+// with the bound multipliers it implies, and the two complementarity terms;
+// partials (sum of squares, max abs) per block.
+namespace {
+template <int M>
+__global__ __launch_bounds__(TPB) void k_kkt(int64_t n, LamArr lam, const double *__restrict__ x,
+                                             const double *__restrict__ dfdx, PtrArr dg,
+                                             const double *__restrict__ xmin, const double *__restrict__ xmax,
+                                             double *__restrict__ part) {
+  const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  double v[2] = {0.0, 0.0};
+  if (i < n) {
+    double ri = dfdx[i];
+#pragma unroll
+    for (int j = 0; j < M; ++j) ri += lam.v[j] * dg.p[j][i];
+    const double xi = x[i], lo = xmin[i], hi = xmax[i];
+    const double mumin = (xi < lo + 1.0e-5 && ri > 0.0) ? ri : 0.0;
+    const double mumax = (xi > hi - 1.0e-5 && ri < 0.0) ? -ri : 0.0;
+    ri += -mumin + mumax;
+    const double r1 = mumin * (xi - lo), r2 = mumax * (hi - xi);
+    v[0] = ri * ri + r1 * r1 + r2 * r2;
+    v[1] = fmax(fabs(ri), fmax(fabs(r1), fabs(r2)));
+  }
+  // fixed-order block combine: sum for the squares, max for the abs values
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    v[0] += __shfl_down(v[0], off, 64);
+    v[1] = fmax(v[1], __shfl_down(v[1], off, 64));
+  }
+  __shared__ double sm[2][TPB / 64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) {
+    sm[0][w] = v[0];
+    sm[1][w] = v[1];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    part[2 * (size_t)blockIdx.x] = (sm[0][0] + sm[0][1]) + (sm[0][2] + sm[0][3]);
+    part[2 * (size_t)blockIdx.x + 1] = fmax(fmax(sm[1][0], sm[1][1]), fmax(sm[1][2], sm[1][3]));
+  }
+}
+__global__ __launch_bounds__(TPB) void k_final_sum_max(const double *__restrict__ part, int nb,
+                                                       double *__restrict__ out) {
+  double v[2] = {0.0, 0.0};
+  for (int i = threadIdx.x; i < nb; i += TPB) {
+    v[0] += part[2 * (size_t)i];
+    v[1] = fmax(v[1], part[2 * (size_t)i + 1]);
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    v[0] += __shfl_down(v[0], off, 64);
+    v[1] = fmax(v[1], __shfl_down(v[1], off, 64));
+  }
+  __shared__ double sm[2][TPB / 64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) {
+    sm[0][w] = v[0];
+    sm[1][w] = v[1];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    out[0] = (sm[0][0] + sm[0][1]) + (sm[0][2] + sm[0][3]);
+    out[1] = fmax(fmax(sm[1][0], sm[1][1]), fmax(sm[1][2], sm[1][3]));
+  }
+}
+
+// ---------------------------------------------------------------- von Mises stress
+// SolveFem::ComputeStress (stress/ComputeStress.cpp:53-104): strain at the
+// element centroid (hex8: B(0,0,0); tet4: the constant strain), stress
+// Emax D eps scaled by sqrt(x_e), von Mises of the six components.
+template <int NPE>
+__global__ __launch_bounds__(TPB) void k_stress(int ne, const int32_t *__restrict__ elem,
+                                                const double *__restrict__ xyz, const double *__restrict__ xphys,
+                                                const double *__restrict__ u, double Emax, double lam, double mu,
+                                                double *__restrict__ vms) {
+  const int e = blockIdx.x * TPB + threadIdx.x;
+  if (e >= ne) return;
+  double dNr[3][NPE];
+#pragma unroll
+  for (int a = 0; a < NPE; ++a) {
+    if constexpr (NPE == 8) {
+      dNr[0][a] = (a & 1) ? 0.125 : -0.125;
+      dNr[1][a] = (a & 2) ? 0.125 : -0.125;
+      dNr[2][a] = (a & 4) ? 0.125 : -0.125;
+    } else {
+#pragma unroll
+      for (int r = 0; r < 3; ++r) dNr[r][a] = a == 0 ? -1.0 : (a == r + 1 ? 1.0 : 0.0);
+    }
+  }
+  int nid[NPE];
+  double J[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+  for (int a = 0; a < NPE; ++a) {
+    nid[a] = elem[(size_t)a * ne + e];
+    const double X0 = xyz[3 * (size_t)nid[a]], X1 = xyz[3 * (size_t)nid[a] + 1], X2 = xyz[3 * (size_t)nid[a] + 2];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      J[r * 3 + 0] += dNr[r][a] * X0;
+      J[r * 3 + 1] += dNr[r][a] * X1;
+      J[r * 3 + 2] += dNr[r][a] * X2;
+    }
+  }
+  const double det = J[0] * (J[4] * J[8] - J[5] * J[7]) - J[1] * (J[3] * J[8] - J[5] * J[6]) +
+                     J[2] * (J[3] * J[7] - J[4] * J[6]);
+  double Ji[9];
+  Ji[0] = (J[4] * J[8] - J[5] * J[7]) / det;
+  Ji[1] = (J[2] * J[7] - J[1] * J[8]) / det;
+  Ji[2] = (J[1] * J[5] - J[2] * J[4]) / det;
+  Ji[3] = (J[5] * J[6] - J[3] * J[8]) / det;
+  Ji[4] = (J[0] * J[8] - J[2] * J[6]) / det;
+  Ji[5] = (J[2] * J[3] - J[0] * J[5]) / det;
+  Ji[6] = (J[3] * J[7] - J[4] * J[6]) / det;
+  Ji[7] = (J[1] * J[6] - J[0] * J[7]) / det;
+  Ji[8] = (J[0] * J[4] - J[1] * J[3]) / det;
+  double eps[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+  for (int a = 0; a < NPE; ++a) {
+    const double g0 = Ji[0] * dNr[0][a] + Ji[1] * dNr[1][a] + Ji[2] * dNr[2][a];
+    const double g1 = Ji[3] * dNr[0][a] + Ji[4] * dNr[1][a] + Ji[5] * dNr[2][a];
+    const double g2 = Ji[6] * dNr[0][a] + Ji[7] * dNr[1][a] + Ji[8] * dNr[2][a];
+    const double ux = u[3 * (size_t)nid[a]], uy = u[3 * (size_t)nid[a] + 1], uz = u[3 * (size_t)nid[a] + 2];
+    eps[0] += g0 * ux;
+    eps[1] += g1 * uy;
+    eps[2] += g2 * uz;
+    eps[3] += g1 * ux + g0 * uy;
+    eps[4] += g2 * uy + g1 * uz;
+    eps[5] += g2 * ux + g0 * uz;
+  }
+  const double tr = eps[0] + eps[1] + eps[2], sc = Emax * sqrt(xphys[e]);
+  double sg[6];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) sg[k] = sc * (lam * tr + 2.0 * mu * eps[k]);
+#pragma unroll
+  for (int k = 3; k < 6; ++k) sg[k] = sc * mu * eps[k];
+  const double d01 = sg[0] - sg[1], d02 = sg[0] - sg[2], d12 = sg[1] - sg[2];
+  vms[e] = (1.0 / sqrt(2.0)) *
+           sqrt(d01 * d01 + d02 * d02 + d12 * d12 + 6.0 * (sg[3] * sg[3] + sg[4] * sg[4] + sg[5] * sg[5]));
+}
+}  // namespace
+
+namespace uftk {
+template <int M>
+static void kkt_m(hipStream_t s, int64_t n, const double *lam, const double *x, const double *dfdx,
+                  const double *const *dgdx, const double *xmin, const double *xmax, double *part) {
+  PtrArr pa{};
+  LamArr la{};
+  for (int j = 0; j < M; ++j) {
+    pa.p[j] = dgdx[j];
+    la.v[j] = lam[j];
+  }
+  k_kkt<M><<<nblk(n), TPB, 0, s>>>(n, la, x, dfdx, pa, xmin, xmax, part);
+}
+void mma_kkt(hipStream_t s, int64_t n, int m, const double *lam, const double *x, const double *dfdx,
+             const double *const *dgdx, const double *xmin, const double *xmax, double *part, double *out) {
+  switch (m) {
+    case 1: kkt_m<1>(s, n, lam, x, dfdx, dgdx, xmin, xmax, part); break;
+    case 2: kkt_m<2>(s, n, lam, x, dfdx, dgdx, xmin, xmax, part); break;
+    case 3: kkt_m<3>(s, n, lam, x, dfdx, dgdx, xmin, xmax, part); break;
+    case 4: kkt_m<4>(s, n, lam, x, dfdx, dgdx, xmin, xmax, part); break;
+    case 5: kkt_m<5>(s, n, lam, x, dfdx, dgdx, xmin, xmax, part); break;
+    case 6: kkt_m<6>(s, n, lam, x, dfdx, dgdx, xmin, xmax, part); break;
+    case 7: kkt_m<7>(s, n, lam, x, dfdx, dgdx, xmin, xmax, part); break;
+    default: kkt_m<8>(s, n, lam, x, dfdx, dgdx, xmin, xmax, part); break;
+  }
+  k_final_sum_max<<<1, TPB, 0, s>>>(part, (int)nblk(n), out);
+}
+void stress(hipStream_t s, const uft_ctx *C, const double *xphys, const double *u, double *vms) {
+  const double nu = C->P.nu;
+  const double lam = nu / ((1.0 + nu) * (1.0 - 2.0 * nu)), mu = 1.0 / (2.0 * (1.0 + nu));
+  const int ne = (int)C->ne;
+  if (C->npe == 8)
+    k_stress<8><<<nblk(ne), TPB, 0, s>>>(ne, C->d_elem, C->d_xyz, xphys, u, C->P.Emax, lam, mu, vms);
+  else
+    k_stress<4><<<nblk(ne), TPB, 0, s>>>(ne, C->d_elem, C->d_xyz, xphys, u, C->P.Emax, lam, mu, vms);
+}
 }  // namespace uftk

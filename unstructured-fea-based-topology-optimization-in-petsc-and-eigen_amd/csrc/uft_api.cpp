@@ -808,7 +808,7 @@ int uft_setup(uft_ctx *C) {
   // ---- vectors
   C->vec.assign(UFT_DGDX0 + m, nullptr);
   C->vlen.assign(UFT_DGDX0 + m, 0);
-  const int elem_ids[] = {UFT_X, UFT_XFILED, UFT_XPHYS, UFT_XOLD, UFT_DFDX, UFT_XMIN, UFT_XMAX, UFT_VOL};
+  const int elem_ids[] = {UFT_X, UFT_XFILED, UFT_XPHYS, UFT_XOLD, UFT_DFDX, UFT_XMIN, UFT_XMAX, UFT_VOL, UFT_VMS};
   for (int id : elem_ids) {
     if ((rc = dalloc_n(C, &C->vec[id], ne))) return rc;
     C->vlen[id] = ne;
@@ -1417,6 +1417,100 @@ extern "C" int uft_solver_info(const uft_ctx *C, int *nlev, int64_t *n_per_level
     if (n_per_level) n_per_level[l] = C->amg.L[l].n;
     if (blocks_per_level) blocks_per_level[l] = l == 0 ? C->ne : C->amg.L[l].nblk;
   }
+  return UFT_OK;
+}
+
+// ---- MMA settings, restart state, KKT residual (MMA.h:8-52)
+extern "C" int uft_mma_set_asymptotes(uft_ctx *C, double init, double decrease, double increase) {
+  if (int rc = check_setup(C)) return rc;
+  C->asyminit = init;
+  C->asymdec = decrease;
+  C->asyminc = increase;
+  return UFT_OK;
+}
+
+extern "C" int uft_mma_set_robust_asymptotes(uft_ctx *C, int type) {
+  if (int rc = check_setup(C)) return rc;
+  if (type != 0 && type != 1) {
+    C->mma_robust = 0;
+    return uft_set_error(C, UFT_ERR_ARG, "RobustAsymptotesType cannot be set to: %d (using 0)", type);
+  }
+  C->mma_robust = type;
+  return UFT_OK;
+}
+
+extern "C" int uft_mma_constraint_modification(uft_ctx *C, int on) {
+  if (int rc = check_setup(C)) return rc;
+  C->mma_conmod = on ? 1 : 0;
+  return UFT_OK;
+}
+
+extern "C" int uft_mma_set_penalization(uft_ctx *C, const double *a, const double *c, const double *d) {
+  if (int rc = check_setup(C)) return rc;
+  if (!a || !c || !d) return uft_set_error(C, UFT_ERR_ARG, "uft_mma_set_penalization: null array");
+  for (int j = 0; j < C->P.m; ++j) {
+    C->a[j] = a[j];
+    C->c[j] = c[j];
+    C->dd[j] = d[j];
+  }
+  return UFT_OK;
+}
+
+extern "C" int uft_mma_state_get(uft_ctx *C, int *k, double *xo1, double *xo2, double *U, double *L) {
+  if (int rc = check_setup(C)) return rc;
+  if (!k || !xo1 || !xo2 || !U || !L) return uft_set_error(C, UFT_ERR_ARG, "uft_mma_state_get: null output");
+  const size_t b = C->ne * sizeof(double);
+  *k = C->mk;
+  UFT_HIP(C, hipMemcpyAsync(xo1, C->d_xo1, b, hipMemcpyDeviceToHost, C->s));
+  UFT_HIP(C, hipMemcpyAsync(xo2, C->d_xo2, b, hipMemcpyDeviceToHost, C->s));
+  UFT_HIP(C, hipMemcpyAsync(U, C->d_U, b, hipMemcpyDeviceToHost, C->s));
+  UFT_HIP(C, hipMemcpyAsync(L, C->d_L, b, hipMemcpyDeviceToHost, C->s));
+  return sync(C);
+}
+
+extern "C" int uft_mma_state_set(uft_ctx *C, int k, const double *xo1, const double *xo2, const double *U,
+                                 const double *L) {
+  if (int rc = check_setup(C)) return rc;
+  if (!xo1 || !xo2 || !U || !L || k < 0) return uft_set_error(C, UFT_ERR_ARG, "uft_mma_state_set: bad arguments");
+  const size_t b = C->ne * sizeof(double);
+  C->mk = k;
+  UFT_HIP(C, hipMemcpyAsync(C->d_xo1, xo1, b, hipMemcpyHostToDevice, C->s));
+  UFT_HIP(C, hipMemcpyAsync(C->d_xo2, xo2, b, hipMemcpyHostToDevice, C->s));
+  UFT_HIP(C, hipMemcpyAsync(C->d_U, U, b, hipMemcpyHostToDevice, C->s));
+  UFT_HIP(C, hipMemcpyAsync(C->d_L, L, b, hipMemcpyHostToDevice, C->s));
+  if (int rc = sync(C)) return rc;
+  if (k < 3) return uft_set_error(C, UFT_OK, "not a legal restart point (k < 3): expect breakdown");
+  return UFT_OK;
+}
+
+extern "C" int uft_mma_kkt_residual(uft_ctx *C, int x_id, int dfdx_id, const double *gx, const int *dgdx_ids,
+                                    int xmin_id, int xmax_id, double *norm2, double *norminf) {
+  if (int rc = check_setup(C)) return rc;
+  const int m = C->P.m;
+  if (!gx || !dgdx_ids || !norm2 || !norminf || !valid_vec(C, x_id) || !valid_vec(C, dfdx_id) ||
+      !valid_vec(C, xmin_id) || !valid_vec(C, xmax_id))
+    return uft_set_error(C, UFT_ERR_ARG, "uft_mma_kkt_residual: bad arguments");
+  const double *dg[UFT_MAXM];
+  for (int j = 0; j < m; ++j) {
+    if (!valid_vec(C, dgdx_ids[j])) return uft_set_error(C, UFT_ERR_ARG, "uft_mma_kkt_residual: bad dgdx id");
+    dg[j] = C->vec[dgdx_ids[j]];
+  }
+  uftk::mma_kkt(C->s, C->ne, m, C->lam, C->vec[x_id], C->vec[dfdx_id], dg, C->vec[xmin_id], C->vec[xmax_id],
+                C->d_part, C->d_scal + S_RED);
+  if (int rc = fetch(C, C->d_scal + S_RED, 2, C->h_scal)) return rc;
+  double rl = 0.0;
+  for (int j = 0; j < m; ++j) rl += C->lam[j] * (C->a[j] * C->z + C->y[j] - gx[j]);
+  *norm2 = std::sqrt(C->h_scal[0] + rl * rl);
+  *norminf = std::max(C->h_scal[1], std::fabs(rl));
+  return UFT_OK;
+}
+
+// ---- von Mises stress of the current displacement (stress/ComputeStress.cpp)
+extern "C" int uft_stress(uft_ctx *C, int xphys_id, int vms_id) {
+  if (int rc = check_setup(C)) return rc;
+  if (!valid_vec(C, xphys_id) || !valid_vec(C, vms_id) || C->vlen[xphys_id] != C->ne || C->vlen[vms_id] != C->ne)
+    return uft_set_error(C, UFT_ERR_ARG, "uft_stress: bad ids");
+  uftk::stress(C->s, C, C->vec[xphys_id], C->vec[UFT_U], C->vec[vms_id]);
   return UFT_OK;
 }
 

@@ -162,6 +162,7 @@ struct uft_ctx {
   // MMA state (MMA.h:76-109)
   int mk = 0;
   double asyminit = 0.5, asymdec = 0.7, asyminc = 1.2;
+  int mma_robust = 0, mma_conmod = 0;  // RobustAsymptotesType, constraintModification
   double a[UFT_MAXM], c[UFT_MAXM], dd[UFT_MAXM], y[UFT_MAXM], z = 0.0;
   double lam[UFT_MAXM], mu[UFT_MAXM], sv[2 * UFT_MAXM], b[UFT_MAXM], grad[UFT_MAXM],
       Hess[UFT_MAXM * UFT_MAXM];
@@ -324,6 +325,10 @@ void mma_movelimit(hipStream_t s, int64_t n, const double *x, double Xmin, doubl
 void mma_gensub(hipStream_t s, uft_ctx *C, int64_t n, int m, int k, const double *x, const double *xmin,
                 const double *xmax, const double *dfdx, const double *const *dgdx, double *part);
 void mma_dual_pass(hipStream_t s, uft_ctx *C, int64_t n, int m, const double *lam, double *x, double *part);
+// out[0] = sum of squares, out[1] = max abs of the KKT residual terms
+void mma_kkt(hipStream_t s, int64_t n, int m, const double *lam, const double *x, const double *dfdx,
+             const double *const *dgdx, const double *xmin, const double *xmax, double *part, double *out);
+void stress(hipStream_t s, const uft_ctx *C, const double *xphys, const double *u, double *vms);
 void design_change(hipStream_t s, int64_t n, const double *x, double *xold, double *part);
 void heaviside(hipStream_t s, int64_t n, const double *xf, double *xp, double beta, double x0, double Xmin);
 void projected_sens(hipStream_t s, int64_t n, const double *xf, double beta, double x0, double *v);

@@ -15,6 +15,7 @@
 #include <cstring>
 #include <map>
 #include <string>
+#include <memory>
 #include <vector>
 
 #include "uft.h"
@@ -77,9 +78,18 @@ inline PetscErrorCode UftFinalize() {
 }
 
 // handle to a device vector of the context (replaces PETSc Vec)
+// A PETSc Vec stand-in: a context vector id, or (for the MMA restart
+// vectors, which the reference keeps as user-owned Vecs) a host buffer made
+// by VecCreateHost.
 struct Vec {
   int id = -1;
+  std::shared_ptr<std::vector<double>> host;
 };
+inline Vec VecCreateHost(long long n) {
+  Vec v;
+  v.host = std::make_shared<std::vector<double>>(n, 0.0);
+  return v;
+}
 struct MeshRef {};  // stands in for the reference's Eigen mesh matrices and raw BC arrays
 
 // ------------------------------------------------------------------ OptPara
@@ -203,7 +213,21 @@ class SolveFem {
     }
   }
   Vec U{UFT_U};
+  Vec VMS{UFT_VMS};
   PetscScalar nu = 0.3;
+  // SolveFem::ComputeStress (stress/ComputeStress.cpp:1-113): von Mises
+  // stress of U into VMS
+  PetscErrorCode ComputeStress(Vec XPhys, PetscScalar Emax, MeshRef, MeshRef, MeshRef, MeshRef) {
+    uft_ctx *c = UftRuntime::get().ctx;
+    uft_params p;
+    uft_params_get(c, &p);
+    if (p.Emax != Emax) {
+      p.Emax = Emax;
+      PetscErrorCode e = uft_params_set(c, &p);
+      if (e) return e;
+    }
+    return uft_stress(c, XPhys.id, VMS.id);
+  }
   // SolveFem::ComputeObjectiveAndSensitivities (SolveFem.h:39-40)
   PetscErrorCode ComputeObjectiveAndSensitivities(PetscScalar *fx, PetscScalar *gx, Vec dfdx, PetscInt m, Vec *dgdx,
                                                   Vec XPhys, PetscScalar Emin, PetscScalar Emax, PetscScalar penal,
@@ -271,8 +295,26 @@ class PDEFilter {
 // ------------------------------------------------------------------ MMA
 class MMA {
  public:
-  MMA(PetscInt n, PetscInt m, Vec /*x*/, PetscScalar * /*a*/, PetscScalar * /*c*/, PetscScalar * /*d*/)
-      : n_(n), m_(m) {}
+  // MMA(n, m, x[, a, c, d]) (MMA.h:10-13): a, c, d default to 0, 1000, 0
+  MMA(PetscInt n, PetscInt m, Vec /*x*/) : n_(n), m_(m) {}
+  MMA(PetscInt n, PetscInt m, Vec /*x*/, PetscScalar *a, PetscScalar *c, PetscScalar *d) : n_(n), m_(m) {
+    if (a && c && d) uft_mma_set_penalization(UftRuntime::get().ctx, a, c, d);
+  }
+  // restart constructors (MMA.h:14-18, MMA.cpp:5-176): state from host Vecs
+  MMA(PetscInt n, PetscInt m, PetscInt itr, Vec xo1, Vec xo2, Vec U, Vec L) : n_(n), m_(m) {
+    uft_ctx *c = UftRuntime::get().ctx;
+    if (!(xo1.host && xo2.host && U.host && L.host)) {
+      std::fprintf(stderr, "MMA restart: xo1, xo2, U, L must be host Vecs (VecCreateHost)\n");
+      return;
+    }
+    uft_mma_state_set(c, (int)itr, xo1.host->data(), xo2.host->data(), U.host->data(), L.host->data());
+    if (itr < 3) std::printf("NOT A LEGAL RESTART POINT (k<3): EXPECT BREAKDOWN\n");
+  }
+  MMA(PetscInt n, PetscInt m, PetscInt itr, Vec xo1, Vec xo2, Vec U, Vec L, PetscScalar *a, PetscScalar *c,
+      PetscScalar *d)
+      : MMA(n, m, itr, xo1, xo2, U, L) {
+    if (a && c && d) uft_mma_set_penalization(UftRuntime::get().ctx, a, c, d);
+  }
   PetscErrorCode SetOuterMovelimit(PetscScalar Xmin, PetscScalar Xmax, PetscScalar movlim, Vec x, Vec xmin, Vec xmax) {
     uft_ctx *c = UftRuntime::get().ctx;
     uft_params p;
@@ -289,6 +331,32 @@ class MMA {
     for (int j = 0; j < m_; ++j) ids[j] = dgdx[j].id;
     return uft_mma_update(UftRuntime::get().ctx, xval.id, dfdx.id, gx, ids.data(), xmin.id, xmax.id);
   }
+  // MMA::Restart (MMA.cpp:302-342): copies the restart state out; the
+  // iteration counter is returned by RestartIteration
+  PetscErrorCode Restart(Vec xo1, Vec xo2, Vec U, Vec L) {
+    if (!(xo1.host && xo2.host && U.host && L.host)) return UFT_ERR_ARG;
+    return uft_mma_state_get(UftRuntime::get().ctx, &k_, xo1.host->data(), xo2.host->data(), U.host->data(),
+                             L.host->data());
+  }
+  PetscInt RestartIteration() const { return k_; }
+  PetscErrorCode SetAsymptotes(PetscScalar init, PetscScalar decrease, PetscScalar increase) {
+    return uft_mma_set_asymptotes(UftRuntime::get().ctx, init, decrease, increase);
+  }
+  PetscErrorCode ConstraintModification(PetscBool conMod) {
+    return uft_mma_constraint_modification(UftRuntime::get().ctx, conMod ? 1 : 0);
+  }
+  PetscErrorCode SetRobustAsymptotesType(PetscInt val) {
+    if (uft_mma_set_robust_asymptotes(UftRuntime::get().ctx, (int)val))
+      std::printf("ERROR in MMA.cc/h: RobustAsymptotesType cannot be set to: %d \n", (int)val);
+    return 0;
+  }
+  PetscErrorCode KKTresidual(Vec x, Vec dfdx, PetscScalar *gx, Vec *dgdx, Vec xmin, Vec xmax, PetscScalar *norm2,
+                             PetscScalar *normInf) {
+    std::vector<int> ids(m_);
+    for (int j = 0; j < m_; ++j) ids[j] = dgdx[j].id;
+    return uft_mma_kkt_residual(UftRuntime::get().ctx, x.id, dfdx.id, gx, ids.data(), xmin.id, xmax.id, norm2,
+                                normInf);
+  }
   PetscScalar DesignChange(Vec x, Vec xold) {
     double ch = 0.0;
     uft_mma_design_change(UftRuntime::get().ctx, x.id, xold.id, &ch);
@@ -297,6 +365,7 @@ class MMA {
 
  private:
   PetscInt n_, m_;
+  int k_ = 0;
 };
 
 inline PetscErrorCode OptPara::SetMMA(PetscInt * /*itr*/, MMA **mma) {

@@ -15,7 +15,7 @@ HEADER = os.path.join(_ROOT, "include", "uft.h")
 
 FP64, MIXED, MIXED_H = 0, 1, 2
 VEC = {"X": 0, "XFiled": 1, "XPhys": 2, "Xold": 3, "dfdx": 4, "xmin": 5, "xmax": 6, "Vol": 7, "U": 8,
-       "RX": 9, "dgdx0": 16}
+       "RX": 9, "VMS": 10, "dgdx0": 16}
 SIZE = {"ne": 0, "nn": 1, "ndof": 2, "npe": 3}
 STATUS = {0: "UFT_OK", 1: "UFT_ERR_ARG", 2: "UFT_ERR_IO", 3: "UFT_ERR_NOCONV", 4: "UFT_ERR_DETJ",
           5: "UFT_ERR_OOM", 6: "UFT_ERR_HIP", 7: "UFT_ERR_STATE"}
@@ -104,6 +104,14 @@ def lib():
         "uft_amg_transfer_get": (C.c_int, [vp, C.c_int, ip, dp]),
         "uft_amg_coarse_inverse_get": (C.c_int, [vp, dp]),
         "uft_phase_times": (C.c_int, [vp, pd]),
+        "uft_mma_set_asymptotes": (C.c_int, [vp, C.c_double, C.c_double, C.c_double]),
+        "uft_mma_set_robust_asymptotes": (C.c_int, [vp, C.c_int]),
+        "uft_mma_constraint_modification": (C.c_int, [vp, C.c_int]),
+        "uft_mma_set_penalization": (C.c_int, [vp, dp, dp, dp]),
+        "uft_mma_state_get": (C.c_int, [vp, pi, dp, dp, dp, dp]),
+        "uft_mma_state_set": (C.c_int, [vp, C.c_int, dp, dp, dp, dp]),
+        "uft_mma_kkt_residual": (C.c_int, [vp, C.c_int, C.c_int, pd, pi, C.c_int, C.c_int, pd, pd]),
+        "uft_stress": (C.c_int, [vp, C.c_int, C.c_int]),
         "uft_autotune": (C.c_int, [vp, C.c_int, pi, pd, C.c_int]),
     }
     for name, (res, args) in sig.items():
@@ -279,6 +287,40 @@ class Context:
         ch = C.c_double()
         self._chk(self.L.uft_mma_design_change(self.h, VEC["X"], VEC["Xold"], C.byref(ch)), "uft_mma_design_change")
         return ch.value
+
+    # ---- MMA settings, restart state, KKT residual
+    def mma_settings(self, init=0.5, dec=0.7, inc=1.2, robust=0, conmod=0):
+        self._chk(self.L.uft_mma_set_asymptotes(self.h, init, dec, inc), "uft_mma_set_asymptotes")
+        self._chk(self.L.uft_mma_set_robust_asymptotes(self.h, robust), "uft_mma_set_robust_asymptotes")
+        self._chk(self.L.uft_mma_constraint_modification(self.h, conmod), "uft_mma_constraint_modification")
+
+    def mma_set_penalization(self, a, c, d):
+        f = lambda v: np.ascontiguousarray(v, dtype=np.float64)
+        self._chk(self.L.uft_mma_set_penalization(self.h, f(a), f(c), f(d)), "uft_mma_set_penalization")
+
+    def mma_state(self):
+        ne = self.size("ne")
+        k = C.c_int()
+        v = [np.zeros(ne) for _ in range(4)]
+        self._chk(self.L.uft_mma_state_get(self.h, C.byref(k), *v), "uft_mma_state_get")
+        return k.value, v[0], v[1], v[2], v[3]
+
+    def mma_set_state(self, k, xo1, xo2, U, L):
+        f = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+        self._chk(self.L.uft_mma_state_set(self.h, k, f(xo1), f(xo2), f(U), f(L)), "uft_mma_state_set")
+
+    def mma_kkt(self, gx):
+        m = len(gx)
+        g = (C.c_double * m)(*gx)
+        ids = (C.c_int * m)(*[16 + j for j in range(m)])
+        n2, ni = C.c_double(), C.c_double()
+        self._chk(self.L.uft_mma_kkt_residual(self.h, VEC["X"], VEC["dfdx"], g, ids, VEC["xmin"], VEC["xmax"],
+                                              C.byref(n2), C.byref(ni)), "uft_mma_kkt_residual")
+        return n2.value, ni.value
+
+    def stress(self, xphys="XPhys"):
+        self._chk(self.L.uft_stress(self.h, self.vid(xphys), VEC["VMS"]), "uft_stress")
+        return self.get("VMS")
 
     def design_iteration(self, itr, allow_noconv=False):
         fx, gx, ch = C.c_double(), (C.c_double * 8)(), C.c_double()
