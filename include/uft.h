@@ -98,6 +98,13 @@ int uft_bc_set(uft_ctx *ctx, int nfixed, const int32_t *fixed, int nlc, int nloa
 /* Same, reading dir/fixeddofs.txt, dir/loaddof_%d.txt, dir/load_%d.txt
  * (InputMeshData.cpp:238,252,271). */
 int uft_bc_load_text(uft_ctx *ctx, const char *dir, int nfixed, int nlc, int nload);
+/* Binary cache of the mesh and BCs set on ctx (no counterpart in the
+ * reference, which re-parses the text files on every start,
+ * InputMeshData.cpp:100-149,228-285): uft_cache_save writes them with a
+ * checksum; uft_cache_load reads a cache back (UFT_ERR_IO on a bad header,
+ * a truncated file or a checksum mismatch) as uft_mesh_set + uft_bc_set. */
+int uft_cache_save(uft_ctx *ctx, const char *path);
+int uft_cache_load(uft_ctx *ctx, const char *path);
 /* Builds the per-element operators, volumes, filter, solver hierarchy and MMA
  * state: SolveFem ctor (SolveFem.cpp:6-15), PDEFilter ctor
  * (PDEFilter.cpp:36-140), OptPara::SetMMA (OptPara.cpp:158-177); sets

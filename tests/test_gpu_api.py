@@ -140,5 +140,48 @@ def test_dropin_driver_runs(O, tmp_path):
         assert abs(got - f) <= 1e-6 * abs(f) + 1e-6  # the log line prints %f
     vtus = sorted(os.listdir(tmp_path))
     assert "result_0.vtu" in vtus and "result_-1.vtu" in vtus
+    # the same run through the binary mesh cache: written on the first run,
+    # read on the second, identical log
+    cache = str(tmp_path / "box.uftc")
+    runs = [subprocess.run(args + ["-mesh_cache", cache], capture_output=True, text=True, timeout=120)
+            for _ in range(2)]
+    assert all(r.returncode == 0 for r in runs) and os.path.exists(cache)
+    logs = [[l.split("time:")[0] for l in r.stdout.splitlines() if l.startswith("It.:")] for r in runs]
+    assert logs[0] == logs[1] == [l.split("time:")[0] for l in lines]
     txt = open(tmp_path / "result_0.vtu").read()
     assert 'NumberOfCells="128"' in txt and 'Name="xphys"' in txt
+
+
+def test_binary_cache_roundtrip(O, tmp_path):
+    """Text files -> context -> binary cache -> new context: identical mesh,
+    BCs and results; corrupt and truncated caches are rejected."""
+    import uft
+    print("[case] binary cache", flush=True)
+    nx, ny, nz = 6, 3, 3
+    xyz, elem = uft.gen_box(8, nx, ny, nz, jitter=0.1, seed=9)
+    fixed, ld, lv = uft.box_bcs(nx, ny, nz)
+    prm = dict(Emax=1.0, volfrac=0.3, rmin=0.5, rtol_el=1e-12, rtol_f=1e-12)
+    a = _ctx(**prm)
+    a.set_mesh(elem, xyz)
+    a.set_bcs(fixed, ld, lv)
+    path = str(tmp_path / "mesh.uftc").encode()
+    assert a.L.uft_cache_save(a.h, path) == 0
+    a.setup()
+    b = _ctx(**prm)
+    assert b.L.uft_cache_load(b.h, path) == 0
+    b.setup()
+    e2 = np.zeros(elem.size, np.int32)
+    x2 = np.zeros(xyz.size)
+    assert b.L.uft_mesh_get(b.h, e2, x2) == 0
+    assert np.array_equal(e2, elem.ravel()) and np.array_equal(x2, xyz.ravel())
+    assert a.fem_obj_sens() == b.fem_obj_sens()
+    raw = bytearray(open(path, "rb").read())
+    raw[100] ^= 0x01
+    bad = tmp_path / "bad.uftc"
+    bad.write_bytes(bytes(raw))
+    c = _ctx(**prm)
+    assert c.L.uft_cache_load(c.h, str(bad).encode()) == 2 and b"checksum" in c.L.uft_last_error(c.h)
+    bad.write_bytes(bytes(raw[:500]))
+    assert c.L.uft_cache_load(c.h, str(bad).encode()) == 2 and b"truncated" in c.L.uft_last_error(c.h)
+    bad.write_bytes(b"NOTACACHE" + bytes(64))
+    assert c.L.uft_cache_load(c.h, str(bad).encode()) == 2 and b"bad header" in c.L.uft_last_error(c.h)

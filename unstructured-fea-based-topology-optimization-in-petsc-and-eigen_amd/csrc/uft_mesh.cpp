@@ -164,3 +164,94 @@ extern "C" int uft_bc_load_text(uft_ctx *C, const char *dir, int nfixed, int nlc
   if (rc) return rc;
   return uft_bc_set(C, nfixed, fixed.data(), nlc, nload, ld.data(), lv.data());
 }
+
+// ---- binary cache of a mesh and its boundary conditions
+// Layout (little endian): "UFTMESH1", int32 version 1, int32 npe, int64 nn,
+// int64 ne, int32 nfixed, int32 nlc, int32 nload, int32 0, then xyz (nn*3
+// f64), elem (ne*npe i32), fixed (i32), load dofs (nlc*nload i32), load
+// values (f64), and a trailing FNV-1a 64 checksum of everything before it.
+namespace {
+constexpr char kMagic[8] = {'U', 'F', 'T', 'M', 'E', 'S', 'H', '1'};
+struct CacheHead {
+  char magic[8];
+  int32_t version, npe;
+  int64_t nn, ne;
+  int32_t nfixed, nlc, nload, pad;
+};
+uint64_t fnv1a(uint64_t h, const void *p, size_t n) {
+  const unsigned char *b = static_cast<const unsigned char *>(p);
+  for (size_t i = 0; i < n; ++i) {
+    h ^= b[i];
+    h *= 1099511628211ull;
+  }
+  return h;
+}
+}  // namespace
+
+extern "C" int uft_cache_save(uft_ctx *C, const char *path) {
+  if (!C || !path) return UFT_ERR_ARG;
+  if (!C->have_mesh || !C->have_bc) return uft_set_error(C, UFT_ERR_STATE, "uft_cache_save: mesh and BCs not set");
+  CacheHead h{};
+  std::memcpy(h.magic, kMagic, 8);
+  h.version = 1;
+  h.npe = C->npe;
+  h.nn = C->nn;
+  h.ne = C->ne;
+  h.nfixed = (int32_t)C->fixed.size();
+  h.nlc = C->nlc;
+  h.nload = C->nload;
+  std::FILE *f = std::fopen(path, "wb");
+  if (!f) return uft_set_error(C, UFT_ERR_IO, "cannot create %s", path);
+  uint64_t ck = 1469598103934665603ull;
+  bool ok = true;
+  auto put = [&](const void *p, size_t n) {
+    if (!ok || n == 0) return;
+    ok = std::fwrite(p, 1, n, f) == n;
+    ck = fnv1a(ck, p, n);
+  };
+  put(&h, sizeof(h));
+  put(C->xyz.data(), C->xyz.size() * sizeof(double));
+  put(C->elem.data(), C->elem.size() * sizeof(int32_t));
+  put(C->fixed.data(), C->fixed.size() * sizeof(int32_t));
+  put(C->ldofs.data(), C->ldofs.size() * sizeof(int32_t));
+  put(C->lvals.data(), C->lvals.size() * sizeof(double));
+  if (ok) ok = std::fwrite(&ck, 1, sizeof(ck), f) == sizeof(ck);
+  if (std::fclose(f) != 0) ok = false;
+  return ok ? UFT_OK : uft_set_error(C, UFT_ERR_IO, "write error on %s", path);
+}
+
+extern "C" int uft_cache_load(uft_ctx *C, const char *path) {
+  if (!C || !path) return UFT_ERR_ARG;
+  std::FILE *f = std::fopen(path, "rb");
+  if (!f) return uft_set_error(C, UFT_ERR_IO, "cannot open %s", path);
+  uint64_t ck = 1469598103934665603ull;
+  bool ok = true;
+  auto get = [&](void *p, size_t n) {
+    if (!ok || n == 0) return;
+    ok = std::fread(p, 1, n, f) == n;
+    if (ok) ck = fnv1a(ck, p, n);
+  };
+  CacheHead h{};
+  get(&h, sizeof(h));
+  if (!ok || std::memcmp(h.magic, kMagic, 8) != 0 || h.version != 1 || (h.npe != 8 && h.npe != 4) || h.nn <= 0 ||
+      h.ne <= 0 || h.nfixed < 0 || h.nlc < 1 || h.nload < 0 || h.nn > (int64_t)1 << 31 || h.ne > (int64_t)1 << 31) {
+    std::fclose(f);
+    return uft_set_error(C, UFT_ERR_IO, "%s: not a mesh cache (bad header)", path);
+  }
+  std::vector<double> xyz(h.nn * 3), lv((size_t)h.nlc * h.nload);
+  std::vector<int32_t> elem(h.ne * h.npe), fixed(h.nfixed), ld((size_t)h.nlc * h.nload);
+  get(xyz.data(), xyz.size() * sizeof(double));
+  get(elem.data(), elem.size() * sizeof(int32_t));
+  get(fixed.data(), fixed.size() * sizeof(int32_t));
+  get(ld.data(), ld.size() * sizeof(int32_t));
+  get(lv.data(), lv.size() * sizeof(double));
+  uint64_t stored = 0;
+  const uint64_t want = ck;
+  if (ok) ok = std::fread(&stored, 1, sizeof(stored), f) == sizeof(stored);
+  std::fclose(f);
+  if (!ok) return uft_set_error(C, UFT_ERR_IO, "%s: truncated mesh cache", path);
+  if (stored != want) return uft_set_error(C, UFT_ERR_IO, "%s: checksum mismatch", path);
+  int rc = uft_mesh_set(C, h.npe, h.nn, h.ne, elem.data(), xyz.data());
+  if (rc) return rc;
+  return uft_bc_set(C, h.nfixed, fixed.data(), h.nlc, h.nload, ld.data(), lv.data());
+}

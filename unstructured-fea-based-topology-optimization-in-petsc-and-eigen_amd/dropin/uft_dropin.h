@@ -167,7 +167,19 @@ class InputMeshData {
     UftRuntime &R = UftRuntime::get();
     const int npe = R.integer("-npe", 8);
     int rc = 0;
-    if (R.has("-mesh_dir")) {
+    const std::string cache = R.str("-mesh_cache", "");
+    bool from_cache = false;
+    if (!cache.empty()) {
+      std::FILE *f = std::fopen(cache.c_str(), "rb");
+      if (f) {
+        std::fclose(f);
+        rc = uft_cache_load(R.ctx, cache.c_str());
+        from_cache = true;
+      }
+    }
+    if (from_cache) {
+      // mesh and BCs came from the binary cache
+    } else if (R.has("-mesh_dir")) {
       const std::string d = R.str("-mesh_dir", ".");
       const long long nn = R.integer("-nn", 410451), ne = R.integer("-ne", 392000);
       rc = uft_mesh_load_text(R.ctx, npe, nn, ne, (d + "/vertex.txt").c_str(), (d + "/element.txt").c_str());
@@ -188,6 +200,7 @@ class InputMeshData {
       if (!rc) rc = uft_mesh_set(R.ctx, npe, nn, ne, el.data(), xyz.data());
       if (!rc) rc = uft_bc_set(R.ctx, nf, fixed.data(), 1, nl, ld.data(), lv.data());
     }
+    if (!rc && !cache.empty() && !from_cache) rc = uft_cache_save(R.ctx, cache.c_str());
     if (rc) {
       std::fprintf(stderr, "InputMeshData: %s\n", UftLastError());
       std::exit(1);

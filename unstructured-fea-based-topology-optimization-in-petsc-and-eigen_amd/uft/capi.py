@@ -112,6 +112,8 @@ def lib():
         "uft_mma_state_set": (C.c_int, [vp, C.c_int, dp, dp, dp, dp]),
         "uft_mma_kkt_residual": (C.c_int, [vp, C.c_int, C.c_int, pd, pi, C.c_int, C.c_int, pd, pd]),
         "uft_stress": (C.c_int, [vp, C.c_int, C.c_int]),
+        "uft_cache_save": (C.c_int, [vp, C.c_char_p]),
+        "uft_cache_load": (C.c_int, [vp, C.c_char_p]),
         "uft_autotune": (C.c_int, [vp, C.c_int, pi, pd, C.c_int]),
     }
     for name, (res, args) in sig.items():
