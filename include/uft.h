@@ -70,6 +70,9 @@ typedef struct uft_params {
   double cheb_lo, cheb_hi; /* Chebyshev interval [cheb_lo, cheb_hi] * lambda_max estimate
                               (-mg_levels_ksp_chebyshev_esteig 0,lo,0,hi; PCGAMG: 0.1, 1.1) */
   int esteig_steps;    /* Lanczos steps of each level's lambda_max(D^-1 A) estimate */
+  int load_cases;      /* with several load cases: 0 superposed into one right-hand side (the reference's
+                          INSERT, SolveFem.cpp:277-283); 1 separate right-hand sides (at most 8) solved
+                          together, compliance and sensitivities summed over the cases */
 } uft_params;
 
 /* --- context ------------------------------------------------------------ */
@@ -165,6 +168,20 @@ int uft_element_kf(uft_ctx *ctx, int64_t e, double *kf, double *t);
  * hierarchy is internal to PETSc): level count (at most 8), unknowns per level
  * and, per level, the element count (level 0) or 6x6 block count. */
 int uft_solver_info(const uft_ctx *ctx, int *nlev, int64_t *n_per_level, int64_t *blocks_per_level);
+/* --- multi-load-case form (load_cases = 1; no counterpart in the reference,
+ * which superposes its cases) --------------------------------------------- */
+/* Number of load cases; per case the Krylov iterations and final relative
+ * residual of the last solve (arrays of up to 8).  0 in the other forms. */
+int uft_load_case_stats(const uft_ctx *ctx, int *iters, double *relres);
+/* Displacement of load case k after the last solve (host, 3*nn doubles);
+ * UFT_U holds case 0. */
+int uft_load_case_u(uft_ctx *ctx, int k, double *host);
+/* y_k = K(XPhys) u_k for every load case through the multi-load-case
+ * operator kernels (host arrays ncase x 3nn, case-major): kind 0 the FP64
+ * kernel of the outer loop, kind 1 the preconditioner's (FP32 vectors;
+ * BF16 matrix-core element products in the mixed forms). */
+int uft_apply_K_cases(uft_ctx *ctx, int kind, const double *u, double *y);
+
 /* --- MMA settings, restart and KKT residual (MMA.h:8-52) ------------------ */
 /* MMA::SetAsymptotes (MMA.cpp:347-355): asymptote initialisation,
  * decrease and increase factors (defaults 0.5, 0.7, 1.2). */
@@ -238,7 +255,10 @@ int64_t uft_box_elems(int npe, int nx, int ny, int nz);
 int uft_gen_box(int npe, int nx, int ny, int nz, double lx, double ly, double lz, double jitter,
                 uint64_t seed, double *xyz, int32_t *elem);
 /* Cantilever BCs: x=0 face clamped; load_kind 0 = -1/(nz+1) on the y-DOFs of
- * the bottom edge of x=lx, 1 = -1/9 on a 3x3 node patch at mid-height. */
+ * the bottom edge of x=lx, 1 = -1/9 on a 3x3 node patch at mid-height,
+ * 2 = eight load cases of nine entries each (load_dofs/load_vals need 72):
+ * 3x3 patches of -1/9 on x=lx centred at (j, k) = (1 + floor(u (ny-1)),
+ * 1 + floor(u (nz-1))), u from the SplitMix64 stream of seed 8. */
 int uft_box_bcs(int nx, int ny, int nz, int load_kind, int32_t *fixed, int *nfixed, int32_t *load_dofs,
                 double *load_vals, int *nload);
 

@@ -87,6 +87,8 @@ def lib():
             "orc_mma_state_set": (None, [vp, C.c_int, dp, dp, dp, dp]),
             "orc_mma_kkt": (None, [vp, dp, dp, dp, dp, dp, dp, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
             "orc_stress": (None, [vp, dp, dp, dp]),
+            "orc_set_load_separate": (None, [vp, C.c_int]),
+            "orc_case_u": (C.POINTER(C.c_double), [vp, C.c_int]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -121,8 +123,8 @@ def box_bcs(nx, ny, nz, load_kind=0):
     L = lib()
     nn = (nx + 1) * (ny + 1) * (nz + 1)
     fixed = np.zeros(3 * nn, dtype=np.int32)
-    ld = np.zeros(nn, dtype=np.int32)
-    lv = np.zeros(nn)
+    ld = np.zeros(max(nn, 72), dtype=np.int32)
+    lv = np.zeros(max(nn, 72))
     nf, nl = C.c_int(), C.c_int()
     L.orc_box_bcs(nx, ny, nz, load_kind, fixed, C.byref(nf), ld, lv, C.byref(nl))
     return fixed[:nf.value].copy(), ld[:nl.value].copy(), lv[:nl.value].copy()
@@ -279,6 +281,13 @@ class Problem:
         lib().orc_mma_kkt(self.h, f(x), f(dfdx), f(gx), f(dgdx), f(xmin), f(xmax), C.byref(n2), C.byref(ni))
         return n2.value, ni.value
 
+    def set_load_separate(self, on=True):
+        lib().orc_set_load_separate(self.h, int(on))
+
+    def case_u(self, k):
+        p = lib().orc_case_u(self.h, k)
+        return np.ctypeslib.as_array(p, shape=(3 * self.nn,)).copy() if p else None
+
     def stress(self, xphys, u):
         out = np.zeros(self.ne)
         lib().orc_stress(self.h, np.ascontiguousarray(xphys, dtype=np.float64),
@@ -298,4 +307,4 @@ def cantilever(etype=8, nx=8, ny=4, nz=4, jitter=0.0, seed=0, load_kind=0, **par
     xyz, elem = gen_box(etype, nx, ny, nz, 2.0, 1.0, 1.0, jitter, seed)
     fixed, ld, lv = box_bcs(nx, ny, nz, load_kind)
     p = default_params(**params)
-    return Problem(elem, xyz, fixed, ld, lv, p)
+    return Problem(elem, xyz, fixed, ld, lv, p, nlc=8 if load_kind == 2 else 1)

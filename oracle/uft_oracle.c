@@ -97,7 +97,11 @@ int orc_gen_box(int etype, int nx, int ny, int nz, double lx, double ly, double 
  * order as triples {3n,3n+1,3n+2} (GetBcs/GetBCs.m:46-51).  Load on the
  * y-DOF 3n+1 (GetBcs/GetBCs.m:67):
  *   kind 0: nodes i=nx, j=0, all k, value -1/(nz+1) each (bottom edge of x=Lx);
- *   kind 1: 3x3 node patch i=nx, j in {ny/2-1..ny/2+1}, k in {nz/2-1..nz/2+1}, -1/9. */
+ *   kind 1: 3x3 node patch i=nx, j in {ny/2-1..ny/2+1}, k in {nz/2-1..nz/2+1}, -1/9;
+ *   kind 2: eight load cases, each a 3x3 patch on x=Lx (-1/9 per node,
+ *           unit total) centred at (j, k) = (1 + floor(u (ny-1)),
+ *           1 + floor(u (nz-1))) with u from the SplitMix64 stream of seed 8
+ *           (j then k, case by case); 72 entries, 9 per case. */
 int orc_box_bcs(int nx, int ny, int nz, int load_kind, int32_t *fixed, int *nfixed,
                 int32_t *ldofs, double *lvals, int *nload) {
   int64_t sx = nx + 1, sxy = (int64_t)(nx + 1) * (ny + 1);
@@ -117,7 +121,7 @@ int orc_box_bcs(int nx, int ny, int nz, int load_kind, int32_t *fixed, int *nfix
       ldofs[nl] = (int32_t)(3 * n + 1);
       lvals[nl++] = -1.0 / (double)(nz + 1);
     }
-  } else {
+  } else if (load_kind == 1) {
     int jc = ny / 2, kc = nz / 2;
     for (int k = kc - 1; k <= kc + 1; k++)
       for (int j = jc - 1; j <= jc + 1; j++) {
@@ -125,6 +129,18 @@ int orc_box_bcs(int nx, int ny, int nz, int load_kind, int32_t *fixed, int *nfix
         ldofs[nl] = (int32_t)(3 * n + 1);
         lvals[nl++] = -1.0 / 9.0;
       }
+  } else {
+    uint64_t st = 8;
+    for (int lc = 0; lc < 8; lc++) {
+      int jc = 1 + (int)floor(sm64_u01(&st) * (ny - 1));
+      int kc = 1 + (int)floor(sm64_u01(&st) * (nz - 1));
+      for (int k = kc - 1; k <= kc + 1; k++)
+        for (int j = jc - 1; j <= jc + 1; j++) {
+          int64_t n = nx + sx * j + sxy * k;
+          ldofs[nl] = (int32_t)(3 * n + 1);
+          lvals[nl++] = -1.0 / 9.0;
+        }
+    }
   }
   *nload = nl;
   return 0;
@@ -497,6 +513,8 @@ typedef struct {
   int mk;
   double asyminit, asymdec, asyminc;
   int robust, conmod;   /* RobustAsymptotesType, constraintModification (MMA.h) */
+  int load_separate;    /* multi-load-case form: one solve per case, summed compliance */
+  double *Ucase;        /* displacements of every case (nlc * ndof) in that form */
   double a[ORC_MAXM], c[ORC_MAXM], d[ORC_MAXM], y[ORC_MAXM], z;
   double lam[ORC_MAXM], mu[ORC_MAXM], s[2 * ORC_MAXM], b[ORC_MAXM], grad[ORC_MAXM], Hess[ORC_MAXM * ORC_MAXM];
   double *L, *Uas, *alpha, *betav, *p0, *q0, *pij, *qij, *xo1, *xo2;
@@ -632,7 +650,7 @@ void orc_destroy(orc_ctx *C) {
   free(C->X); free(C->XFiled); free(C->XPhys); free(C->Xold); free(C->dfdx); free(C->dgdx);
   free(C->xmin); free(C->xmax);
   free(C->L); free(C->Uas); free(C->alpha); free(C->betav); free(C->p0); free(C->q0);
-  free(C->pij); free(C->qij); free(C->xo1); free(C->xo2);
+  free(C->pij); free(C->qij); free(C->xo1); free(C->xo2); free(C->Ucase);
   free(C);
 }
 
@@ -734,7 +752,50 @@ static int pcg_el(orc_ctx *C) {
 /* SolveFem::ComputeObjectiveAndSensitivities (SolveFem.cpp:159-265):
  * uKu_e = u_e^T KE_e u_e, fx = sum E(x_e) uKu_e, df_e = -p x^(p-1)(Emax-Emin) uKu_e,
  * gx = sum x_e V_e / TV - volfrac, dg_e = V_e / TV. */
+/* Multi-load-case form (an extension; the reference superposes its load
+ * cases into one right-hand side, SolveFem.cpp:277-283): every case is
+ * solved on its own and the compliance and its sensitivities are summed. */
+static int fem_obj_sens_cases(orc_ctx *C, const double *xphys, double *fx, double *gx, double *dfdx,
+                              double *dgdx) {
+  const orc_params *P = &C->P;
+  int nd = C->nd, npe = C->npe, rc = 0;
+  double KE[576];
+  if (!C->Ucase) C->Ucase = calloc((size_t)C->nlc * C->ndof, sizeof(double));
+  orc_assemble(C, xphys);
+  for (int64_t e = 0; e < C->ne; e++) dfdx[e] = 0.0;
+  double f = 0.0;
+  for (int lc = 0; lc < C->nlc; lc++) {
+    memset(C->F, 0, C->ndof * sizeof(double));
+    for (int i = 0; i < C->nload; i++) C->F[C->ldofs[lc * C->nload + i]] = C->lvals[lc * C->nload + i];
+    for (int i = 0; i < C->nfixed; i++) C->F[C->fixed[i]] = 0.0;
+    int r = pcg_el(C);
+    if (r) rc = r;
+    memcpy(C->Ucase + (size_t)lc * C->ndof, C->U, C->ndof * sizeof(double));
+    for (int64_t e = 0; e < C->ne; e++) {
+      elem_ke(C, e, KE);
+      double ue[24];
+      for (int k = 0; k < nd; k++) ue[k] = C->U[3 * C->elem[e * npe + (k % npe)] + k / npe];
+      double uKu = 0.0;
+      for (int p = 0; p < nd; p++)
+        for (int q = 0; q < nd; q++) uKu += ue[p] * KE[p * nd + q] * ue[q];
+      f += simp_E(P, xphys[e]) * uKu;
+      dfdx[e] += -1.0 * P->penal * pow(xphys[e], P->penal - 1) * (P->Emax - P->Emin) * uKu;
+    }
+  }
+  double g = 0.0;
+  for (int64_t e = 0; e < C->ne; e++) {
+    g += xphys[e] * C->vol[e];
+    dgdx[e] = C->vol[e] / C->TV;
+  }
+  fx[0] = f;
+  gx[0] = g / C->TV - P->volfrac;
+  return rc;
+}
+void orc_set_load_separate(orc_ctx *C, int on) { C->load_separate = on; }
+const double *orc_case_u(orc_ctx *C, int lc) { return C->Ucase ? C->Ucase + (size_t)lc * C->ndof : NULL; }
+
 int orc_fem_obj_sens(orc_ctx *C, const double *xphys, double *fx, double *gx, double *dfdx, double *dgdx) {
+  if (C->load_separate && C->nlc > 1) return fem_obj_sens_cases(C, xphys, fx, gx, dfdx, dgdx);
   orc_assemble(C, xphys);
   int rc = pcg_el(C);
   int nd = C->nd, npe = C->npe;

@@ -58,6 +58,10 @@ __device__ __forceinline__ void row_epi(const RowOp<T> &op, int64_t i, T t, T ui
       dv[1] += (double)ui * (double)t;
       dv[2] += (double)ui * (double)op.w2[i];
       break;
+    case ROW_STOREDOT:  // y = t ; u.t
+      op.y[i] = t;
+      dv[0] += (double)ui * (double)t;
+      break;
     default:
       break;
   }
@@ -70,6 +74,9 @@ __device__ __forceinline__ void epi_partials(const RowOp<T> &op, double (&dv)[3]
     uftr::block_partials<2>(v2, op.part);
   } else if (op.mode == ROW_DOTS3) {
     uftr::block_partials<3>(dv, op.part);
+  } else if (op.mode == ROW_STOREDOT) {
+    double v1[1] = {dv[0]};
+    uftr::block_partials<1>(v1, op.part);
   }
 }
 

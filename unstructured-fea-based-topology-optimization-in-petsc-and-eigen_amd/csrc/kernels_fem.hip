@@ -22,46 +22,13 @@
 
 #include "uft_internal.h"
 #include "uft_reduce.h"
+#include "uft_elem.h"
 
 namespace {
 
 constexpr int TPB = 256;
 
-template <int I, int N, typename F>
-__device__ __forceinline__ void static_for(F &&f) {
-  if constexpr (I < N) {
-    f(std::integral_constant<int, I>{});
-    static_for<I + 1, N>(f);
-  }
-}
-
-// row / column of entry p of the row-major packed upper triangle of an nd x nd matrix
-__host__ __device__ constexpr int pk_row(int p, int nd) {
-  int r = 0;
-  while (p >= nd - r) {
-    p -= nd - r;
-    ++r;
-  }
-  return r;
-}
-__host__ __device__ constexpr int pk_col(int p, int nd) {
-  int r = 0;
-  while (p >= nd - r) {
-    p -= nd - r;
-    ++r;
-  }
-  return r + p;
-}
-__host__ __device__ inline int pk_index(int r, int c, int nd) {
-  if (r > c) {
-    int t = r;
-    r = c;
-    c = t;
-  }
-  return r * nd - (r * (r - 1)) / 2 + (c - r);
-}
-
-__device__ __forceinline__ double simpE(const SimpCoef &sc, double x) { return sc.Emin + pow(x, sc.penal) * sc.dE; }
+using namespace uftd;
 
 // ---------------------------------------------------------------- reductions
 template <int K>
@@ -453,15 +420,6 @@ __global__ __launch_bounds__(TPB) void k_kf_tet4(int ne, const int32_t *__restri
 }
 
 // ---------------------------------------------------------------- stiffness apply
-template <int I, int J, typename T>
-__device__ __forceinline__ void kacc(T k, const T *ue, T *ye) {
-  if constexpr (I == J) {
-    ye[I] += k * ue[I];
-  } else {
-    ye[I] += k * ue[J];
-    ye[J] += k * ue[I];
-  }
-}
 
 // Element phase: Ye[k][e] = E(x_e) (KE_e u_e)_k with the fixed DOFs of u
 // masked to zero (MatZeroRowsColumns zeroes the columns, SolveFem.cpp:324).

@@ -1,0 +1,840 @@
+// gfx950 kernels of the multi-load-case form: NR right-hand sides (NR = 2,
+// 4 or 8; unused ones are zero) solved together.  Every vector is stored
+// interleaved, [unknown][NR], so one read of an operator entry (element
+// block, 6x6 coarse block, transfer block) serves all NR vectors, and the NR
+// values of one unknown sit in one 16-64 byte segment.  Per-vector scalars
+// (Krylov and K-cycle coefficients) live in device arrays indexed by the
+// right-hand side.
+//
+// The fine element products of the mixed form run on the matrix cores:
+// Y_e (nd x NR) = KE_e (nd x nd) U_e (nd x NR) as v_mfma_f32_16x16x32_bf16
+// tiles (BF16 operands, FP32 accumulation), one element per wavefront step.
+#include "uft_elem.h"
+#include "uft_internal.h"
+#include "uft_reduce.h"
+
+namespace {
+using namespace uftd;
+
+constexpr int TPB = 256;
+inline unsigned nblk(int64_t n) { return (unsigned)((n + TPB - 1) / TPB); }
+
+// Per-RHS block partials: v[k] of lane (.., r = lane % NR) is summed over the
+// lanes with the same r (fixed shuffle tree, then the four waves in order)
+// into part[block][k * NR + r].
+template <int K, int NR>
+__device__ __forceinline__ void mr_block_partials(double (&v)[K], double *part) {
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+#pragma unroll
+    for (int off = 32; off >= NR; off >>= 1) v[k] += __shfl_down(v[k], off, 64);
+  }
+  __shared__ double sm[K][NR][TPB / 64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane < NR) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) sm[k][lane][w] = v[k];
+  }
+  __syncthreads();
+  if (threadIdx.x < K * NR) {
+    const int k = threadIdx.x / NR, r = threadIdx.x % NR;
+    part[(size_t)blockIdx.x * K * NR + threadIdx.x] = (sm[k][r][0] + sm[k][r][1]) + (sm[k][r][2] + sm[k][r][3]);
+  }
+}
+
+// per-row epilogue of a fused operator apply (the single-RHS RowOp modes);
+// scalar coefficients that differ per right-hand side come from sc[r * 8 + .]
+template <typename T>
+struct MrOp {
+  int mode;
+  int addprev;
+  T *y, *r, *dout, *x, *kr, *kd;
+  const T *base, *dinv, *w1, *w2;
+  double c1, c2, it;
+  double *part;
+};
+enum { MR_STORE = 0, MR_STEP, MR_RES, MR_RESFIRST, MR_RESFIRSTX, MR_DOTS2, MR_DOTS3, MR_STOREDOT };
+
+// element row i (dinv index), interleaved index ii = i * NR + rr
+template <typename T>
+__device__ __forceinline__ void mr_epi(const MrOp<T> &op, int64_t i, int64_t ii, T t, T ui, double (&dv)[3]) {
+  switch (op.mode) {
+    case MR_STORE:
+      op.y[ii] = t;
+      break;
+    case MR_STEP: {
+      const T ri = op.r[ii] - t;
+      op.r[ii] = ri;
+      const T dn = (T)op.c1 * ui + (T)op.c2 * op.dinv[i] * ri;
+      op.dout[ii] = dn;
+      op.x[ii] += op.addprev ? dn + ui : dn;
+      break;
+    }
+    case MR_RES:
+      op.r[ii] = op.base[ii] - t;
+      break;
+    case MR_RESFIRST: {
+      const T ri = op.base[ii] - t;
+      op.r[ii] = ri;
+      op.dout[ii] = (T)op.it * op.dinv[i] * ri;
+      break;
+    }
+    case MR_RESFIRSTX: {
+      const T ri = op.base[ii] - t;
+      op.r[ii] = ri;
+      op.kr[ii] = ri;
+      const T dn = (T)op.it * op.dinv[i] * ri;
+      op.dout[ii] = dn;
+      op.kd[ii] = dn;
+      break;
+    }
+    case MR_DOTS2:
+      op.y[ii] = t;
+      dv[0] += (double)ui * (double)t;
+      dv[1] += (double)ui * (double)op.w1[ii];
+      break;
+    case MR_DOTS3:
+      dv[0] += (double)ui * (double)op.w1[ii];
+      dv[1] += (double)ui * (double)t;
+      dv[2] += (double)ui * (double)op.w2[ii];
+      break;
+    case MR_STOREDOT:
+      op.y[ii] = t;
+      dv[0] += (double)ui * (double)t;
+      break;
+    default:
+      break;
+  }
+}
+template <typename T, int NR>
+__device__ __forceinline__ void mr_epi_partials(const MrOp<T> &op, double (&dv)[3]) {
+  if (op.mode == MR_DOTS2) {
+    double v[2] = {dv[0], dv[1]};
+    mr_block_partials<2, NR>(v, op.part);
+  } else if (op.mode == MR_DOTS3) {
+    mr_block_partials<3, NR>(dv, op.part);
+  } else if (op.mode == MR_STOREDOT) {
+    double v[1] = {dv[0]};
+    mr_block_partials<1, NR>(v, op.part);
+  }
+}
+
+// ---------------------------------------------------------------- elementwise
+template <typename T, int NR>
+__global__ __launch_bounds__(TPB) void k_mr_cheb_first(int64_t n, const T *__restrict__ b, const T *__restrict__ dinv,
+                                                        double it, T *__restrict__ r, T *__restrict__ d,
+                                                        T *__restrict__ x) {
+  const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  if (t >= n * NR) return;
+  const T ri = b[t];
+  r[t] = ri;
+  const T di = (T)it * dinv[t / NR] * ri;
+  d[t] = di;
+  x[t] = di;
+}
+
+// ---------------------------------------------------------------- element phase, vector form
+// lane (e, rr), rr = t % NR: the NR lanes of an element read the same packed
+// KE pair (one coalesced request) and their own vector.
+template <int NPE, typename T, typename T2, int NR>
+__global__ __launch_bounds__(TPB) void k_mr_elem(int ne, const int32_t *__restrict__ elem, const T2 *__restrict__ KE,
+                                                 const double *__restrict__ xphys, SimpCoef sc,
+                                                 const uint8_t *__restrict__ fixm, const T *__restrict__ u,
+                                                 T *__restrict__ Ye) {
+  constexpr int ND = 3 * NPE, NP = ND * (ND + 1) / 2, NQ = NP / 2;
+  const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  const int64_t e = t / NR;
+  const int rr = (int)(t % NR);
+  if (e >= ne) return;
+  T ue[ND], ye[ND];
+#pragma unroll
+  for (int a = 0; a < NPE; ++a) {
+    const int n = elem[(size_t)a * ne + e];
+    const unsigned m = fixm[n];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const T v = u[(3 * (size_t)n + c) * NR + rr];
+      ue[c * NPE + a] = ((m >> c) & 1u) ? T(0) : v;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < ND; ++k) ye[k] = T(0);
+  static_for<0, NQ>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    const T2 kk = KE[(size_t)q * ne + e];
+    kacc<pk_row(2 * q, ND), pk_col(2 * q, ND)>(kk.x, ue, ye);
+    kacc<pk_row(2 * q + 1, ND), pk_col(2 * q + 1, ND)>(kk.y, ue, ye);
+  });
+  const T E = (T)simpE(sc, xphys[e]);
+#pragma unroll
+  for (int k = 0; k < ND; ++k) Ye[((size_t)k * ne + e) * NR + rr] = E * ye[k];
+}
+
+// ---------------------------------------------------------------- element phase, matrix cores
+// One wavefront handles MF_EL consecutive elements at a time: their packed
+// BF16 KE pairs ([NQ][ne] words, two BF16 per word) and node ids are staged
+// in LDS with coalesced loads (MF_EL x 4 bytes per KE row), then per element
+// two (hex8) or one (tet4) 16x16x32 BF16 MFMA tiles form Y_e = KE_e U_e for
+// the 8 right-hand sides.  Operand layout of v_mfma_f32_16x16x32_bf16
+// (lane l): A[l % 16][8 (l / 16) + j], B[8 (l / 16) + j][l % 16],
+// D[4 (l / 16) + i][l % 16].  A rows are element DOFs (padded to 16 per
+// tile), K is the element DOF (padded to 32), the columns are the
+// right-hand sides (8 of 16 used).
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+constexpr int MF_WAVES = 4;  // wavefronts per workgroup
+constexpr int MF_EL = 16;    // elements per wavefront batch
+
+template <int NPE>
+__global__ __launch_bounds__(64 * MF_WAVES) void k_mr_elem_mfma(int ne, const int32_t *__restrict__ elem,
+                                                               const uint32_t *__restrict__ KEb,
+                                                               const double *__restrict__ xphys, SimpCoef sc,
+                                                               const uint8_t *__restrict__ fixm,
+                                                               const float *__restrict__ u, float *__restrict__ Ye) {
+  constexpr int ND = 3 * NPE, NP = ND * (ND + 1) / 2, NQ = NP / 2, TILES = (ND + 15) / 16;
+  constexpr int NR = 8;
+  __shared__ uint32_t ks[MF_WAVES][NQ][MF_EL];  // packed pairs, element fastest
+  __shared__ int32_t nodes[MF_WAVES][NPE][MF_EL];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int64_t e0 = ((int64_t)blockIdx.x * MF_WAVES + w) * MF_EL;
+  // stage: lane = (row slot lane / MF_EL, element lane % MF_EL)
+  {
+    const int el = lane % MF_EL, rs = lane / MF_EL;
+    const int64_t e = e0 + el;
+    const bool ok = e < ne;
+    for (int q = rs; q < NQ; q += 64 / MF_EL) ks[w][q][el] = ok ? KEb[(size_t)q * ne + e] : 0u;
+    for (int a = rs; a < NPE; a += 64 / MF_EL) nodes[w][a][el] = ok ? elem[(size_t)a * ne + e] : 0;
+  }
+  __syncthreads();
+  // per-lane operand coordinates (element independent)
+  const int col = lane % 16, kb = 8 * (lane / 16);
+  int pidx[TILES][8];  // packed index of A[row][kb + j] per tile, -1 for padding
+#pragma unroll
+  for (int tl = 0; tl < TILES; ++tl)
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int row = 16 * tl + col, k = kb + j;
+      pidx[tl][j] = (row < ND && k < ND) ? pk_index(row, k, ND) : -1;
+    }
+  for (int el = 0; el < MF_EL; ++el) {
+    const int64_t e = e0 + el;
+    if (e >= ne) break;  // uniform across the wavefront
+    // B operand: U_e[k][col] for k = kb..kb+7 (col = right-hand side)
+    bf16x8 bv;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int k = kb + j;
+      float v = 0.f;
+      if (col < NR && k < ND) {
+        const int a = k % NPE, c = k / NPE;
+        const int n = nodes[w][a][el];
+        v = ((fixm[n] >> c) & 1u) ? 0.f : u[(3 * (size_t)n + c) * NR + col];
+      }
+      bv[j] = (__bf16)v;
+    }
+    const float E = (float)simpE(sc, xphys[e]);
+#pragma unroll
+    for (int tl = 0; tl < TILES; ++tl) {
+      bf16x8 av;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int p = pidx[tl][j];
+        unsigned short bits = 0;
+        if (p >= 0) {
+          const uint32_t wd = ks[w][p >> 1][el];
+          bits = (unsigned short)((p & 1) ? (wd >> 16) : (wd & 0xffffu));
+        }
+        av[j] = __builtin_bit_cast(__bf16, bits);
+      }
+      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc, 0, 0, 0);
+      if (col < NR) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int row = 16 * tl + 4 * (lane / 16) + i;
+          if (row < ND) Ye[((size_t)row * ne + e) * NR + col] = E * acc[i];
+        }
+      }
+    }
+  }
+}
+
+// FP64 KE (pairs) -> BF16 pairs (round to nearest even), same [NQ][ne] layout
+__global__ __launch_bounds__(TPB) void k_ke_to_bf16(int64_t npairs, const double2 *__restrict__ KE,
+                                                    uint32_t *__restrict__ KEb) {
+  const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  if (t >= npairs) return;
+  const double2 v = KE[t];
+  const __bf16 a = (__bf16)(float)v.x, b = (__bf16)(float)v.y;
+  KEb[t] = (uint32_t)__builtin_bit_cast(unsigned short, a) | ((uint32_t)__builtin_bit_cast(unsigned short, b) << 16);
+}
+
+// ---------------------------------------------------------------- node phase
+template <int NPE, typename T, int NR>
+__global__ __launch_bounds__(TPB) void k_mr_gather_op(int nn, int ne, const int32_t *__restrict__ iptr,
+                                                      const int32_t *__restrict__ inc, const T *__restrict__ Ye,
+                                                      const uint8_t *__restrict__ fixm, const T *__restrict__ u,
+                                                      MrOp<T> op) {
+  const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  const int64_t n = t / NR;
+  const int rr = (int)(t % NR);
+  double dv[3] = {0.0, 0.0, 0.0};
+  if (n < nn) {
+    T acc[3] = {T(0), T(0), T(0)};
+    const int k0 = iptr[n], k1 = iptr[n + 1];
+    for (int k = k0; k < k1; ++k) {
+      const int v = inc[k];
+      const int e = v / NPE, l = v - e * NPE;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) acc[c] += Ye[((size_t)(c * NPE + l) * ne + e) * NR + rr];
+    }
+    const unsigned m = fixm[n];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const int64_t i = 3 * n + c, ii = i * NR + rr;
+      const T ui = u[ii];
+      mr_epi(op, i, ii, ((m >> c) & 1u) ? ui : acc[c], ui, dv);
+    }
+  }
+  mr_epi_partials<T, NR>(op, dv);
+}
+
+// ---------------------------------------------------------------- 6x6 BSR
+// 32 lanes per scalar row: LPR = 32 / NR block slots x NR right-hand sides.
+template <typename T, int NR>
+__global__ __launch_bounds__(TPB) void k_mr_bsr_op(int64_t nrows, const int32_t *__restrict__ rp,
+                                                   const int32_t *__restrict__ ci, const T *__restrict__ val,
+                                                   const T *__restrict__ u, MrOp<T> op) {
+  constexpr int LPR = 32 / NR;
+  const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  const int64_t row = t / 32;
+  const int l = (int)((t % 32) / NR), rr = (int)(t % NR);
+  T acc = T(0);
+  if (row < nrows * 6) {
+    const int64_t A = row / 6;
+    const int i = (int)(row - A * 6);
+    for (int k = rp[A] + l; k < rp[A + 1]; k += LPR) {
+      const int64_t B = ci[k];
+      const T *v = val + (size_t)k * 36 + i * 6;
+      const T *xb = u + (B * 6) * NR + rr;
+#pragma unroll
+      for (int j = 0; j < 6; ++j) acc += v[j] * xb[j * NR];
+    }
+  }
+#pragma unroll
+  for (int off = 16; off >= NR; off >>= 1) acc += __shfl_down(acc, off, 32);
+  double dv[3] = {0.0, 0.0, 0.0};
+  if (l == 0 && row < nrows * 6) mr_epi(op, row, row * NR + rr, acc, u[row * NR + rr], dv);
+  mr_epi_partials<T, NR>(op, dv);
+}
+
+// ---------------------------------------------------------------- transfers
+// wavefront per coarse unit; lane = slot * NR + rr
+template <int BS, typename T, int NR>
+__global__ __launch_bounds__(TPB) void k_mr_restrict_first(int64_t nc, const int32_t *__restrict__ cptr,
+                                                           const int32_t *__restrict__ child, const T *__restrict__ P,
+                                                           const T *__restrict__ r, T *__restrict__ bc,
+                                                           const T *__restrict__ dinv, double it, T *__restrict__ rc,
+                                                           T *__restrict__ dc, T *__restrict__ xc) {
+  constexpr int SL = 64 / NR;
+  const int64_t A = ((int64_t)blockIdx.x * TPB + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63, slot = lane / NR, rr = lane % NR;
+  if (A >= nc) return;  // whole wavefront
+  const int k0 = cptr[A], npair = (cptr[A + 1] - k0) * BS;
+  T acc[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
+  for (int q = slot; q < npair; q += SL) {
+    const int kk = q / BS, i = q - kk * BS;
+    const int64_t uu = child[k0 + kk];
+    const T *Pu = P + (size_t)uu * BS * 6 + i * 6;
+    const T ri = r[(uu * BS + i) * NR + rr];
+#pragma unroll
+    for (int j = 0; j < 6; ++j) acc[j] += Pu[j] * ri;
+  }
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+#pragma unroll
+    for (int off = 32; off >= NR; off >>= 1) acc[j] += __shfl_down(acc[j], off, 64);
+  }
+  if (slot == 0) {
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+      const int64_t i = A * 6 + j, ii = i * NR + rr;
+      bc[ii] = acc[j];
+      if (dinv) {
+        const T d = (T)it * dinv[i] * acc[j];
+        rc[ii] = acc[j];
+        dc[ii] = d;
+        xc[ii] = d;
+      }
+    }
+  }
+}
+
+// x_u += P_u (ca_r xa + cb_r xb)_{parent(u)}; ca_r = sc[r * 8 + ia] (ia < 0: 1)
+template <int BS, typename T, int NR>
+__global__ __launch_bounds__(TPB) void k_mr_prolong_comb(int64_t nu, const int32_t *__restrict__ parent,
+                                                         const T *__restrict__ P, const double *__restrict__ sc, int ia,
+                                                         int ib, const T *__restrict__ xa, const T *__restrict__ xb,
+                                                         T *__restrict__ x) {
+  const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  if (t >= nu * BS * NR) return;
+  const int64_t row = t / NR;
+  const int rr = (int)(t % NR);
+  const int64_t uu = row / BS;
+  const int i = (int)(row - uu * BS);
+  const int64_t A = parent[uu];
+  if (A < 0) return;
+  const T ca = ia < 0 ? T(1) : (T)sc[rr * 8 + ia];
+  const T cb = ib < 0 ? T(1) : (T)sc[rr * 8 + ib];
+  const T *Pu = P + (size_t)uu * BS * 6 + i * 6;
+  T acc = T(0);
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+    const int64_t ii = (A * 6 + j) * NR + rr;
+    const T c = xb ? ca * xa[ii] + cb * xb[ii] : xa[ii];
+    acc += Pu[j] * c;
+  }
+  x[t] += acc;
+}
+
+template <typename T, int NR>
+__global__ __launch_bounds__(TPB) void k_mr_lincomb_first(int64_t n, const double *__restrict__ sc,
+                                                          const T *__restrict__ b, const T *__restrict__ v,
+                                                          T *__restrict__ kr, T *__restrict__ r,
+                                                          const T *__restrict__ dinv, double it, T *__restrict__ d,
+                                                          T *__restrict__ kd) {
+  const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  if (t >= n * NR) return;
+  const int rr = (int)(t % NR);
+  const T ri = b[t] + (T)sc[rr * 8 + 7] * v[t];
+  kr[t] = ri;
+  r[t] = ri;
+  const T di = (T)it * dinv[t / NR] * ri;
+  d[t] = di;
+  kd[t] = di;
+}
+
+// per-RHS K-cycle coefficients from the K * NR partial columns (same
+// recurrences as the single right-hand-side K-cycle)
+template <int K, int NR>
+__global__ __launch_bounds__(TPB) void k_mr_final_kc(const double *__restrict__ part, int nb, double *__restrict__ sc) {
+  double res[K * NR];
+  uftr::block_final<K * NR>(part, nb, res);
+  if (threadIdx.x >= NR) return;
+  const int rr = threadIdx.x;
+  double *s = sc + rr * 8;
+  if constexpr (K == 2) {
+    s[0] = res[rr];
+    s[1] = res[NR + rr];
+    s[7] = s[0] != 0.0 ? -s[1] / s[0] : 0.0;
+  } else {
+    const double rho1 = s[0], a1 = s[1], g = res[rr], be = res[NR + rr], a2 = res[2 * NR + rr];
+    s[2] = g;
+    s[3] = be;
+    s[4] = a2;
+    const double t1 = rho1 != 0.0 ? a1 / rho1 : 0.0;
+    const double rho2 = rho1 != 0.0 ? be - g * g / rho1 : 0.0;
+    if (rho2 > 0.0) {
+      s[5] = t1 - g * a2 / (rho1 * rho2);
+      s[6] = a2 / rho2;
+    } else {
+      s[5] = t1;
+      s[6] = 0.0;
+    }
+  }
+}
+
+// last restriction + dense coarsest solve for NR vectors, one workgroup
+constexpr int MT_TPB = 1024;
+template <int BS, typename T, int NR>
+__global__ __launch_bounds__(MT_TPB) void k_mr_tail_solve(int nc_units, const int32_t *__restrict__ cptr,
+                                                          const int32_t *__restrict__ child, const T *__restrict__ P,
+                                                          const double *__restrict__ Ainv, const T *__restrict__ r,
+                                                          T *__restrict__ xc) {
+  __shared__ double bc[UFT_MR_TAIL_MAX * NR];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = MT_TPB / 64;
+  const int slot = lane / NR, rr = lane % NR;
+  constexpr int SL = 64 / NR;
+  const int n = 6 * nc_units;
+  for (int A = w; A < nc_units; A += nw) {
+    const int k0 = cptr[A], npair = (cptr[A + 1] - k0) * BS;
+    double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int q = slot; q < npair; q += SL) {
+      const int kk = q / BS, i = q - kk * BS;
+      const int64_t uu = child[k0 + kk];
+      const T *Pu = P + (size_t)uu * BS * 6 + i * 6;
+      const double ri = (double)r[(uu * BS + i) * NR + rr];
+#pragma unroll
+      for (int j = 0; j < 6; ++j) acc[j] += (double)Pu[j] * ri;
+    }
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+#pragma unroll
+      for (int off = 32; off >= NR; off >>= 1) acc[j] += __shfl_down(acc[j], off, 64);
+    }
+    if (slot == 0) {
+#pragma unroll
+      for (int j = 0; j < 6; ++j) bc[(A * 6 + j) * NR + rr] = acc[j];
+    }
+  }
+  __syncthreads();
+  // xc[row][r] = sum_c Ainv[c][row] bc[c][r] (symmetric inverse, rows coalesced)
+  for (int t = threadIdx.x; t < n * NR; t += MT_TPB) {
+    const int row = t / NR, q = t % NR;
+    double acc = 0.0;
+    for (int c = 0; c < n; ++c) acc += Ainv[(size_t)c * n + row] * bc[c * NR + q];
+    xc[t] = (T)acc;
+  }
+}
+
+// ---------------------------------------------------------------- outer Krylov (FP64)
+template <int NR>
+__global__ __launch_bounds__(TPB) void k_mr_dot(int64_t n, const double *__restrict__ a, const double *__restrict__ b,
+                                                double *__restrict__ part) {
+  const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  double v[1] = {t < n * NR ? a[t] * b[t] : 0.0};
+  mr_block_partials<1, NR>(v, part);
+}
+template <int NR>
+__global__ __launch_bounds__(TPB) void k_mr_dot_pr(int64_t n, const double *__restrict__ r, const double *__restrict__ z,
+                                                   const double *__restrict__ zo, double *__restrict__ part) {
+  const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  double v[2] = {0.0, 0.0};
+  if (t < n * NR) {
+    v[0] = r[t] * z[t];
+    v[1] = r[t] * zo[t];
+  }
+  mr_block_partials<2, NR>(v, part);
+}
+// x += alpha_r p ; r -= alpha_r q ; partial ||r||^2 per RHS
+template <int NR>
+__global__ __launch_bounds__(TPB) void k_mr_cg_update(int64_t n, const double *__restrict__ alpha,
+                                                      double *__restrict__ x, const double *__restrict__ p,
+                                                      double *__restrict__ r, const double *__restrict__ q,
+                                                      double *__restrict__ part) {
+  const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  double v[1] = {0.0};
+  if (t < n * NR) {
+    const double al = alpha[t % NR];
+    x[t] += al * p[t];
+    const double ri = r[t] - al * q[t];
+    r[t] = ri;
+    v[0] = ri * ri;
+  }
+  mr_block_partials<1, NR>(v, part);
+}
+template <int NR>
+__global__ __launch_bounds__(TPB) void k_mr_xpby(int64_t n, const double *__restrict__ z, const double *__restrict__ beta,
+                                                 double *__restrict__ p) {
+  const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  if (t < n * NR) p[t] = z[t] + beta[t % NR] * p[t];
+}
+// per-RHS scalars: alpha = rz / pq on the active right-hand sides, else 0
+__global__ void k_mr_alpha(int nr, unsigned mask, const double *__restrict__ rz, const double *__restrict__ pq,
+                           double *__restrict__ alpha) {
+  const int r = threadIdx.x;
+  if (r < nr) alpha[r] = ((mask >> r) & 1u) && pq[r] != 0.0 ? rz[r] / pq[r] : 0.0;
+}
+// Polak-Ribiere: beta = (rz - rzo) / rzprev ; rzprev = rz (inactive: 0)
+__global__ void k_mr_beta(int nr, unsigned mask, const double *__restrict__ rzn, double *__restrict__ rzprev,
+                          double *__restrict__ beta) {
+  const int r = threadIdx.x;
+  if (r >= nr) return;
+  const double rz = rzn[r], rzo = rzn[nr + r];
+  beta[r] = ((mask >> r) & 1u) && rzprev[r] != 0.0 ? (rz - rzo) / rzprev[r] : 0.0;
+  rzprev[r] = rz;
+}
+
+// ---------------------------------------------------------------- sensitivities
+// lane (e, rr): uKu of right-hand side rr; the NR values of an element are
+// summed over its lanes in a fixed order; fx partial, dfdx.
+template <int NPE, int NR>
+__global__ __launch_bounds__(TPB) void k_mr_sens(int ne, const int32_t *__restrict__ elem,
+                                                 const double2 *__restrict__ KE, const double *__restrict__ xphys,
+                                                 SimpCoef sc, const double *__restrict__ u, double *__restrict__ dfdx,
+                                                 double *__restrict__ part) {
+  constexpr int ND = 3 * NPE, NP = ND * (ND + 1) / 2, NQ = NP / 2;
+  const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  const int64_t e = t / NR;
+  const int rr = (int)(t % NR);
+  double uku = 0.0;
+  if (e < ne) {
+    double ue[ND];
+#pragma unroll
+    for (int a = 0; a < NPE; ++a) {
+      const int n = elem[(size_t)a * ne + e];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) ue[c * NPE + a] = u[(3 * (size_t)n + c) * NR + rr];
+    }
+    double dg = 0.0, off = 0.0;
+    static_for<0, NQ>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      const double2 kk = KE[(size_t)q * ne + e];
+      constexpr int r0 = pk_row(2 * q, ND), c0 = pk_col(2 * q, ND);
+      constexpr int r1 = pk_row(2 * q + 1, ND), c1 = pk_col(2 * q + 1, ND);
+      if constexpr (r0 == c0) dg += kk.x * ue[r0] * ue[c0]; else off += kk.x * ue[r0] * ue[c0];
+      if constexpr (r1 == c1) dg += kk.y * ue[r1] * ue[c1]; else off += kk.y * ue[r1] * ue[c1];
+    });
+    uku = dg + 2.0 * off;
+  }
+  // sum the element's NR lanes (adjacent lanes, fixed tree)
+#pragma unroll
+  for (int off = NR / 2; off > 0; off >>= 1) uku += __shfl_down(uku, off, NR);
+  double v[1] = {0.0};
+  if (e < ne && rr == 0) {
+    const double x = xphys[e];
+    v[0] = simpE(sc, x) * uku;
+    dfdx[e] = -1.0 * sc.penal * pow(x, sc.penal - 1.0) * sc.dE * uku;
+  }
+  uftr::block_partials<1>(v, part);
+}
+
+// case k of an interleaved vector <-> a plain vector
+__global__ __launch_bounds__(TPB) void k_mr_extract(int64_t n, int nr, int k, const double *__restrict__ src,
+                                                    double *__restrict__ dst) {
+  const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  if (i < n) dst[i] = src[i * nr + k];
+}
+// F[idx * nr + k] = val (INSERT of load case k)
+__global__ __launch_bounds__(TPB) void k_mr_set_dofs(int n, int nr, int k, const int32_t *__restrict__ idx,
+                                                     const double *__restrict__ val, double *__restrict__ F) {
+  const int i = blockIdx.x * TPB + threadIdx.x;
+  if (i < n) F[(size_t)idx[i] * nr + k] = val ? val[i] : 0.0;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- host wrappers
+namespace uftk {
+
+template <typename T>
+static MrOp<T> to_mr(const RowOp<T> &o) {
+  MrOp<T> m;
+  m.mode = o.mode == ROW_STORE ? MR_STORE
+         : o.mode == ROW_STEP ? MR_STEP
+         : o.mode == ROW_RES ? MR_RES
+         : o.mode == ROW_RESFIRST ? MR_RESFIRST
+         : o.mode == ROW_RESFIRSTX ? MR_RESFIRSTX
+         : o.mode == ROW_DOTS2 ? MR_DOTS2
+         : o.mode == ROW_DOTS3 ? MR_DOTS3
+                               : MR_STOREDOT;
+  m.addprev = o.addprev;
+  m.y = o.y; m.r = o.r; m.dout = o.dout; m.x = o.x; m.kr = o.kr; m.kd = o.kd;
+  m.base = o.base; m.dinv = o.dinv; m.w1 = o.w1; m.w2 = o.w2;
+  m.c1 = o.c1; m.c2 = o.c2; m.it = o.it; m.part = o.part;
+  return m;
+}
+
+#define UFT_NR_SWITCH(nr, CALL) \
+  switch (nr) {                 \
+    case 2: { constexpr int NR = 2; CALL; } break; \
+    case 4: { constexpr int NR = 4; CALL; } break; \
+    default: { constexpr int NR = 8; CALL; } break; \
+  }
+
+template <typename T>
+static void mr_cheb_first_t(hipStream_t s, int nr, int64_t n, const T *b, const T *dinv, double it, T *r, T *d, T *x) {
+  UFT_NR_SWITCH(nr, (k_mr_cheb_first<T, NR><<<nblk(n * NR), TPB, 0, s>>>(n, b, dinv, it, r, d, x)));
+}
+void mr_cheb_first(hipStream_t s, int nr, int64_t n, const double *b, const double *dinv, double it, double *r,
+                   double *d, double *x) { mr_cheb_first_t<double>(s, nr, n, b, dinv, it, r, d, x); }
+void mr_cheb_first(hipStream_t s, int nr, int64_t n, const float *b, const float *dinv, double it, float *r, float *d,
+                   float *x) { mr_cheb_first_t<float>(s, nr, n, b, dinv, it, r, d, x); }
+
+void mr_elem_phase(hipStream_t s, const uft_ctx *C, int nr, const double *xphys, const double *u, double *Ye) {
+  const SimpCoef sc{C->P.Emin, C->P.Emax - C->P.Emin, C->P.penal};
+  const int ne = (int)C->ne;
+  const double2 *KE = (const double2 *)C->d_KE;
+  if (C->npe == 8) {
+    UFT_NR_SWITCH(nr, (k_mr_elem<8, double, double2, NR><<<nblk((int64_t)ne * NR), TPB, 0, s>>>(
+                          ne, C->d_elem, KE, xphys, sc, C->d_fixm, u, Ye)));
+  } else {
+    UFT_NR_SWITCH(nr, (k_mr_elem<4, double, double2, NR><<<nblk((int64_t)ne * NR), TPB, 0, s>>>(
+                          ne, C->d_elem, KE, xphys, sc, C->d_fixm, u, Ye)));
+  }
+}
+void mr_elem_phase(hipStream_t s, const uft_ctx *C, int nr, const double *xphys, const float *u, float *Ye) {
+  const SimpCoef sc{C->P.Emin, C->P.Emax - C->P.Emin, C->P.penal};
+  const int ne = (int)C->ne;
+  if (nr == 8 && C->d_KEb) {  // matrix-core form
+    const unsigned g = (unsigned)((ne + MF_EL * MF_WAVES - 1) / (MF_EL * MF_WAVES));
+    if (C->npe == 8)
+      k_mr_elem_mfma<8><<<g, 64 * MF_WAVES, 0, s>>>(ne, C->d_elem, C->d_KEb, xphys, sc, C->d_fixm, u, Ye);
+    else
+      k_mr_elem_mfma<4><<<g, 64 * MF_WAVES, 0, s>>>(ne, C->d_elem, C->d_KEb, xphys, sc, C->d_fixm, u, Ye);
+    return;
+  }
+  const float2 *KE = (const float2 *)C->d_KEf;
+  if (C->npe == 8) {
+    UFT_NR_SWITCH(nr, (k_mr_elem<8, float, float2, NR><<<nblk((int64_t)ne * NR), TPB, 0, s>>>(
+                          ne, C->d_elem, KE, xphys, sc, C->d_fixm, u, Ye)));
+  } else {
+    UFT_NR_SWITCH(nr, (k_mr_elem<4, float, float2, NR><<<nblk((int64_t)ne * NR), TPB, 0, s>>>(
+                          ne, C->d_elem, KE, xphys, sc, C->d_fixm, u, Ye)));
+  }
+}
+void ke_to_bf16(hipStream_t s, const uft_ctx *C) {
+  const int np = 3 * C->npe * (3 * C->npe + 1) / 2;
+  const int64_t npairs = (int64_t)(np / 2) * C->ne;
+  k_ke_to_bf16<<<nblk(npairs), TPB, 0, s>>>(npairs, (const double2 *)C->d_KE, C->d_KEb);
+}
+
+template <typename T>
+static void mr_gather_op_t(hipStream_t s, const uft_ctx *C, int nr, const T *u, const T *Ye, const RowOp<T> &o) {
+  const MrOp<T> op = to_mr(o);
+  const int nn = (int)C->nn, ne = (int)C->ne;
+  if (C->npe == 8) {
+    UFT_NR_SWITCH(nr, (k_mr_gather_op<8, T, NR><<<nblk((int64_t)nn * NR), TPB, 0, s>>>(nn, ne, C->d_iptr, C->d_inc, Ye,
+                                                                                     C->d_fixm, u, op)));
+  } else {
+    UFT_NR_SWITCH(nr, (k_mr_gather_op<4, T, NR><<<nblk((int64_t)nn * NR), TPB, 0, s>>>(nn, ne, C->d_iptr, C->d_inc, Ye,
+                                                                                     C->d_fixm, u, op)));
+  }
+}
+void mr_gather_op(hipStream_t s, const uft_ctx *C, int nr, const double *u, const double *Ye, const RowOp<double> &o) {
+  mr_gather_op_t<double>(s, C, nr, u, Ye, o);
+}
+void mr_gather_op(hipStream_t s, const uft_ctx *C, int nr, const float *u, const float *Ye, const RowOp<float> &o) {
+  mr_gather_op_t<float>(s, C, nr, u, Ye, o);
+}
+int mr_gather_blocks(int64_t nn, int nr) { return (int)nblk(nn * nr); }
+
+template <typename T>
+static void mr_bsr_op_t(hipStream_t s, int nr, int64_t nb_rows, const int32_t *rp, const int32_t *ci, const T *val,
+                        const T *u, const RowOp<T> &o) {
+  const MrOp<T> op = to_mr(o);
+  UFT_NR_SWITCH(nr, (k_mr_bsr_op<T, NR><<<nblk(nb_rows * 6 * 32), TPB, 0, s>>>(nb_rows, rp, ci, val, u, op)));
+}
+void mr_bsr_op(hipStream_t s, int nr, int64_t nb_rows, const int32_t *rp, const int32_t *ci, const double *val,
+               const double *u, const RowOp<double> &o) { mr_bsr_op_t<double>(s, nr, nb_rows, rp, ci, val, u, o); }
+void mr_bsr_op(hipStream_t s, int nr, int64_t nb_rows, const int32_t *rp, const int32_t *ci, const float *val,
+               const float *u, const RowOp<float> &o) { mr_bsr_op_t<float>(s, nr, nb_rows, rp, ci, val, u, o); }
+int mr_bsr_blocks(int64_t nb_rows) { return (int)nblk(nb_rows * 6 * 32); }
+
+template <typename T>
+static void mr_restrict_first_t(hipStream_t s, int nr, int bs, int64_t nc, const int32_t *cptr, const int32_t *child,
+                                const T *P, const T *r, T *bc, const T *dinv, double it, T *rc, T *dc, T *xc) {
+  if (bs == 3) {
+    UFT_NR_SWITCH(nr, (k_mr_restrict_first<3, T, NR><<<nblk(nc * 64), TPB, 0, s>>>(nc, cptr, child, P, r, bc, dinv, it,
+                                                                                   rc, dc, xc)));
+  } else {
+    UFT_NR_SWITCH(nr, (k_mr_restrict_first<6, T, NR><<<nblk(nc * 64), TPB, 0, s>>>(nc, cptr, child, P, r, bc, dinv, it,
+                                                                                   rc, dc, xc)));
+  }
+}
+void mr_restrict_first(hipStream_t s, int nr, int bs, int64_t nc, const int32_t *cptr, const int32_t *child,
+                       const double *P, const double *r, double *bc, const double *dinv, double it, double *rc,
+                       double *dc, double *xc) {
+  mr_restrict_first_t<double>(s, nr, bs, nc, cptr, child, P, r, bc, dinv, it, rc, dc, xc);
+}
+void mr_restrict_first(hipStream_t s, int nr, int bs, int64_t nc, const int32_t *cptr, const int32_t *child,
+                       const float *P, const float *r, float *bc, const float *dinv, double it, float *rc, float *dc,
+                       float *xc) {
+  mr_restrict_first_t<float>(s, nr, bs, nc, cptr, child, P, r, bc, dinv, it, rc, dc, xc);
+}
+
+template <typename T>
+static void mr_prolong_comb_t(hipStream_t s, int nr, int bs, int64_t nu, const int32_t *parent, const T *P,
+                              const double *sc, int ia, int ib, const T *xa, const T *xb, T *x) {
+  if (bs == 3) {
+    UFT_NR_SWITCH(nr, (k_mr_prolong_comb<3, T, NR><<<nblk(nu * 3 * NR), TPB, 0, s>>>(nu, parent, P, sc, ia, ib, xa, xb,
+                                                                                     x)));
+  } else {
+    UFT_NR_SWITCH(nr, (k_mr_prolong_comb<6, T, NR><<<nblk(nu * 6 * NR), TPB, 0, s>>>(nu, parent, P, sc, ia, ib, xa, xb,
+                                                                                     x)));
+  }
+}
+void mr_prolong_comb(hipStream_t s, int nr, int bs, int64_t nu, const int32_t *parent, const double *P,
+                     const double *sc, int ia, int ib, const double *xa, const double *xb, double *x) {
+  mr_prolong_comb_t<double>(s, nr, bs, nu, parent, P, sc, ia, ib, xa, xb, x);
+}
+void mr_prolong_comb(hipStream_t s, int nr, int bs, int64_t nu, const int32_t *parent, const float *P, const double *sc,
+                     int ia, int ib, const float *xa, const float *xb, float *x) {
+  mr_prolong_comb_t<float>(s, nr, bs, nu, parent, P, sc, ia, ib, xa, xb, x);
+}
+
+template <typename T>
+static void mr_lincomb_first_t(hipStream_t s, int nr, int64_t n, const double *sc, const T *b, const T *v, T *kr, T *r,
+                               const T *dinv, double it, T *d, T *kd) {
+  UFT_NR_SWITCH(nr, (k_mr_lincomb_first<T, NR><<<nblk(n * NR), TPB, 0, s>>>(n, sc, b, v, kr, r, dinv, it, d, kd)));
+}
+void mr_lincomb_first(hipStream_t s, int nr, int64_t n, const double *sc, const double *b, const double *v, double *kr,
+                      double *r, const double *dinv, double it, double *d, double *kd) {
+  mr_lincomb_first_t<double>(s, nr, n, sc, b, v, kr, r, dinv, it, d, kd);
+}
+void mr_lincomb_first(hipStream_t s, int nr, int64_t n, const double *sc, const float *b, const float *v, float *kr,
+                      float *r, const float *dinv, double it, float *d, float *kd) {
+  mr_lincomb_first_t<float>(s, nr, n, sc, b, v, kr, r, dinv, it, d, kd);
+}
+
+void mr_final_kc1(hipStream_t s, int nr, const double *part, int nb, double *sc) {
+  UFT_NR_SWITCH(nr, (k_mr_final_kc<2, NR><<<1, TPB, 0, s>>>(part, nb, sc)));
+}
+void mr_final_kc2(hipStream_t s, int nr, const double *part, int nb, double *sc) {
+  UFT_NR_SWITCH(nr, (k_mr_final_kc<3, NR><<<1, TPB, 0, s>>>(part, nb, sc)));
+}
+
+template <typename T>
+static void mr_tail_solve_t(hipStream_t s, int nr, int bs, int nc_units, const int32_t *cptr, const int32_t *child,
+                            const T *P, const double *Ainv, const T *r, T *xc) {
+  if (bs == 3) {
+    UFT_NR_SWITCH(nr, (k_mr_tail_solve<3, T, NR><<<1, MT_TPB, 0, s>>>(nc_units, cptr, child, P, Ainv, r, xc)));
+  } else {
+    UFT_NR_SWITCH(nr, (k_mr_tail_solve<6, T, NR><<<1, MT_TPB, 0, s>>>(nc_units, cptr, child, P, Ainv, r, xc)));
+  }
+}
+void mr_tail_solve(hipStream_t s, int nr, int bs, int nc_units, const int32_t *cptr, const int32_t *child,
+                   const double *P, const double *Ainv, const double *r, double *xc) {
+  mr_tail_solve_t<double>(s, nr, bs, nc_units, cptr, child, P, Ainv, r, xc);
+}
+void mr_tail_solve(hipStream_t s, int nr, int bs, int nc_units, const int32_t *cptr, const int32_t *child,
+                   const float *P, const double *Ainv, const float *r, float *xc) {
+  mr_tail_solve_t<float>(s, nr, bs, nc_units, cptr, child, P, Ainv, r, xc);
+}
+
+// ---- outer loop
+void mr_dot(hipStream_t s, int nr, int64_t n, const double *a, const double *b, double *part, double *out) {
+  UFT_NR_SWITCH(nr, (k_mr_dot<NR><<<nblk(n * NR), TPB, 0, s>>>(n, a, b, part)));
+  final_sum(s, part, (int)nblk(n * nr), nr, out);
+}
+void mr_dot_pr(hipStream_t s, int nr, int64_t n, const double *r, const double *z, const double *zo, double *part,
+               double *out) {
+  UFT_NR_SWITCH(nr, (k_mr_dot_pr<NR><<<nblk(n * NR), TPB, 0, s>>>(n, r, z, zo, part)));
+  final_sum(s, part, (int)nblk(n * nr), 2 * nr, out);
+}
+void mr_cg_update(hipStream_t s, int nr, int64_t n, const double *alpha, double *x, const double *p, double *r,
+                  const double *q, double *part, double *rr_out) {
+  UFT_NR_SWITCH(nr, (k_mr_cg_update<NR><<<nblk(n * NR), TPB, 0, s>>>(n, alpha, x, p, r, q, part)));
+  final_sum(s, part, (int)nblk(n * nr), nr, rr_out);
+}
+void mr_xpby(hipStream_t s, int nr, int64_t n, const double *z, const double *beta, double *p) {
+  UFT_NR_SWITCH(nr, (k_mr_xpby<NR><<<nblk(n * NR), TPB, 0, s>>>(n, z, beta, p)));
+}
+void mr_alpha(hipStream_t s, int nr, unsigned mask, const double *rz, const double *pq, double *alpha) {
+  k_mr_alpha<<<1, 64, 0, s>>>(nr, mask, rz, pq, alpha);
+}
+void mr_beta(hipStream_t s, int nr, unsigned mask, const double *rzn, double *rzprev, double *beta) {
+  k_mr_beta<<<1, 64, 0, s>>>(nr, mask, rzn, rzprev, beta);
+}
+void mr_sens(hipStream_t s, const uft_ctx *C, int nr, const double *xphys, const double *u, double *dfdx,
+             double *part) {
+  const SimpCoef sc{C->P.Emin, C->P.Emax - C->P.Emin, C->P.penal};
+  const int ne = (int)C->ne;
+  const double2 *KE = (const double2 *)C->d_KE;
+  if (C->npe == 8) {
+    UFT_NR_SWITCH(nr, (k_mr_sens<8, NR><<<nblk((int64_t)ne * NR), TPB, 0, s>>>(ne, C->d_elem, KE, xphys, sc, u, dfdx,
+                                                                              part)));
+  } else {
+    UFT_NR_SWITCH(nr, (k_mr_sens<4, NR><<<nblk((int64_t)ne * NR), TPB, 0, s>>>(ne, C->d_elem, KE, xphys, sc, u, dfdx,
+                                                                              part)));
+  }
+}
+int mr_sens_blocks(int64_t ne, int nr) { return (int)nblk(ne * nr); }
+void mr_extract(hipStream_t s, int64_t n, int nr, int k, const double *src, double *dst) {
+  k_mr_extract<<<nblk(n), TPB, 0, s>>>(n, nr, k, src, dst);
+}
+void mr_set_dofs(hipStream_t s, int n, int nr, int k, const int32_t *idx, const double *val, double *F) {
+  if (n > 0) k_mr_set_dofs<<<nblk(n), TPB, 0, s>>>(n, nr, k, idx, val, F);
+}
+
+}  // namespace uftk

@@ -373,6 +373,189 @@ void fine_cycle(uft_ctx *C, const T *b, T *x) {
 void vcycle(uft_ctx *C, int, const double *b, double *x) { fine_cycle<double>(C, b, x); }
 void vcycle_f(uft_ctx *C, int, const float *b, float *x) { fine_cycle<float>(C, b, x); }
 
+// ---- multi-load-case cycle: the same fused cycle on interleaved
+// [unknown][nr] vectors, per right-hand-side K-cycle scalars at
+// mr.sc + 128 + 64 l + 8 r.
+template <typename T>
+struct MLV {
+  T *x, *b, *r, *d, *t, *kc, *kv, *kd, *kr;
+  const T *dinv;
+};
+template <typename T>
+MLV<T> mlv(uft_ctx *C, int l) {
+  MrLevelBuf &B = C->mr.L[l];
+  AmgLevel &Lv = C->amg.L[l];
+  const T *dinv;
+  if constexpr (std::is_same<T, double>::value) dinv = Lv.d_dinv; else dinv = Lv.d_dinvf;
+  return {(T *)B.x, (T *)B.b, (T *)B.r, (T *)B.d, (T *)B.t, (T *)B.kc, (T *)B.kv, (T *)B.kd, (T *)B.kr, dinv};
+}
+double *mr_ksc(uft_ctx *C, int l) { return C->mr.sc + 128 + 64 * l; }
+
+template <typename T>
+void mr_lev_op(uft_ctx *C, int l, const T *u, const RowOp<T> &op) {
+  const int nr = C->mr.nr;
+  if (l == 0) {
+    hipEvent_t *ev = prof_triple(C, std::is_same<T, double>::value ? 0 : 1);
+    if (ev) (void)hipEventRecord(ev[0], C->s);
+    uftk::mr_elem_phase(C->s, C, nr, C->cur_xphys, u, (T *)C->mr.Ye);
+    if (ev) (void)hipEventRecord(ev[1], C->s);
+    uftk::mr_gather_op(C->s, C, nr, u, (const T *)C->mr.Ye, op);
+    if (ev) (void)hipEventRecord(ev[2], C->s);
+  } else {
+    const AmgLevel &Lv = C->amg.L[l];
+    const T *val;
+    if constexpr (std::is_same<T, double>::value) val = Lv.d_val; else val = Lv.d_valf;
+    uftk::mr_bsr_op(C->s, nr, Lv.nunit, Lv.d_rp, Lv.d_ci, val, u, op);
+  }
+}
+int mr_lev_blocks(const uft_ctx *C, int l) {
+  return l == 0 ? uftk::mr_gather_blocks(C->nn, C->mr.nr) : uftk::mr_bsr_blocks(C->amg.L[l].nunit);
+}
+
+template <typename T>
+void mr_cycle_body(uft_ctx *C, int l, const T *b, T *x);
+
+template <typename T>
+void mr_coarse_body(uft_ctx *C, int l) {
+  AmgLevel &Lv = C->amg.L[l];
+  MLV<T> v = mlv<T>(C, l);
+  const int nr = C->mr.nr;
+  const int kind = level_kind(C, l);
+  const double it = cheb_coef(Lv.lmax, std::max(1, C->P.cheb_degree), C->P.cheb_lo / C->P.cheb_hi).inv_theta;
+  if (kind == 0) {
+    mr_cycle_body<T>(C, l, v.b, v.x);
+  } else if (kind == 1) {
+    mr_cycle_body<T>(C, l, v.b, v.x);
+    RowOp<T> op;
+    op.mode = ROW_RESFIRSTX;
+    op.base = v.b;
+    op.r = v.r;
+    op.kr = v.kr;
+    op.dinv = v.dinv;
+    op.it = it;
+    op.dout = v.d;
+    op.kd = v.kd;
+    mr_lev_op(C, l, (const T *)v.x, op);
+    mr_cycle_body<T>(C, l, v.kr, v.kd);
+  } else {
+    double *sc = mr_ksc(C, l);
+    mr_cycle_body<T>(C, l, v.b, v.kc);
+    RowOp<T> op;
+    op.mode = ROW_DOTS2;
+    op.y = v.kv;
+    op.w1 = v.b;
+    op.part = C->d_part;
+    mr_lev_op(C, l, (const T *)v.kc, op);
+    uftk::mr_final_kc1(C->s, nr, C->d_part, mr_lev_blocks(C, l), sc);
+    uftk::mr_lincomb_first(C->s, nr, Lv.n, sc, v.b, v.kv, v.kr, v.r, v.dinv, it, v.d, v.kd);
+    mr_cycle_body<T>(C, l, v.kr, v.kd);
+    RowOp<T> o3;
+    o3.mode = ROW_DOTS3;
+    o3.w1 = v.kv;
+    o3.w2 = v.kr;
+    o3.part = C->d_part;
+    mr_lev_op(C, l, (const T *)v.kd, o3);
+    uftk::mr_final_kc2(C->s, nr, C->d_part, mr_lev_blocks(C, l), sc);
+  }
+}
+
+template <typename T>
+void mr_prolong_from(uft_ctx *C, int l, T *x) {
+  AmgLevel &Lv = C->amg.L[l];
+  MLV<T> vn = mlv<T>(C, l + 1);
+  const int nr = C->mr.nr;
+  const T *P = tr_P(Lv, (T *)nullptr);
+  switch (level_kind(C, l + 1)) {
+    case 0:
+      uftk::mr_prolong_comb(C->s, nr, Lv.bs, Lv.nunit, Lv.d_parent, P, nullptr, -1, -1, vn.x, (const T *)nullptr, x);
+      break;
+    case 1:
+      uftk::mr_prolong_comb(C->s, nr, Lv.bs, Lv.nunit, Lv.d_parent, P, nullptr, -1, -1, vn.x, vn.kd, x);
+      break;
+    default:
+      uftk::mr_prolong_comb(C->s, nr, Lv.bs, Lv.nunit, Lv.d_parent, P, mr_ksc(C, l + 1), 5, 6, vn.kc, vn.kd, x);
+      break;
+  }
+}
+
+template <typename T>
+void mr_cycle_body(uft_ctx *C, int l, const T *b, T *x) {
+  AmgHier &H = C->amg;
+  AmgLevel &Lv = H.L[l];
+  MLV<T> v = mlv<T>(C, l);
+  const int nr = C->mr.nr;
+  const int deg = std::max(1, C->P.cheb_degree);
+  const double lohi = C->P.cheb_lo / C->P.cheb_hi;
+  const ChebCoef cc = cheb_coef(Lv.lmax, deg, lohi);
+  T *dc = v.d, *dn = v.t;
+  RowOp<T> st;
+  st.mode = ROW_STEP;
+  st.r = v.r;
+  st.dinv = v.dinv;
+  st.x = x;
+  for (int k = 1; k < deg; ++k) {
+    st.c1 = cc.c1[k];
+    st.c2 = cc.c2[k];
+    st.dout = dn;
+    st.addprev = 0;
+    mr_lev_op(C, l, (const T *)dc, st);
+    std::swap(dc, dn);
+  }
+  RowOp<T> rs;
+  rs.mode = ROW_RES;
+  rs.base = v.r;
+  rs.r = v.r;
+  mr_lev_op(C, l, (const T *)dc, rs);
+  if (l + 1 == H.nlev - 1) {
+    AmgLevel &Ln = H.L[l + 1];
+    MLV<T> vn = mlv<T>(C, l + 1);
+    const T *P = tr_P(Lv, (T *)nullptr);
+    uftk::mr_tail_solve(C->s, nr, Lv.bs, (int)Ln.nunit, Lv.d_cptr, Lv.d_child, P, H.d_cinv, v.r, vn.x);
+    uftk::mr_prolong_comb(C->s, nr, Lv.bs, Lv.nunit, Lv.d_parent, P, nullptr, -1, -1, vn.x, (const T *)nullptr, x);
+  } else {
+    AmgLevel &Ln = H.L[l + 1];
+    MLV<T> vn = mlv<T>(C, l + 1);
+    const double itn = cheb_coef(Ln.lmax, deg, lohi).inv_theta;
+    T *xn = level_kind(C, l + 1) == 2 ? vn.kc : vn.x;
+    uftk::mr_restrict_first(C->s, nr, Lv.bs, Ln.nunit, Lv.d_cptr, Lv.d_child, tr_P(Lv, (T *)nullptr), v.r, vn.b,
+                            vn.dinv, itn, vn.r, vn.d, xn);
+    mr_coarse_body<T>(C, l + 1);
+    mr_prolong_from<T>(C, l, x);
+  }
+  RowOp<T> rf;
+  rf.mode = ROW_RESFIRST;
+  rf.base = b;
+  rf.r = v.r;
+  rf.dinv = v.dinv;
+  rf.it = cc.inv_theta;
+  rf.dout = v.d;
+  mr_lev_op(C, l, (const T *)x, rf);
+  dc = v.d;
+  dn = v.t;
+  if (deg == 1) {
+    uftk::axpy_inplace_t(C->s, Lv.n * nr, v.d, x);
+    return;
+  }
+  for (int k = 1; k < deg; ++k) {
+    st.c1 = cc.c1[k];
+    st.c2 = cc.c2[k];
+    st.dout = dn;
+    st.addprev = k == 1;
+    mr_lev_op(C, l, (const T *)dc, st);
+    std::swap(dc, dn);
+  }
+}
+
+template <typename T>
+void mr_fine_cycle(uft_ctx *C, const T *b, T *x) {
+  AmgLevel &L0 = C->amg.L[0];
+  MLV<T> v = mlv<T>(C, 0);
+  const double it =
+      cheb_coef(L0.lmax, std::max(1, C->P.cheb_degree), C->P.cheb_lo / C->P.cheb_hi).inv_theta;
+  uftk::mr_cheb_first(C->s, C->mr.nr, L0.n, b, v.dinv, it, v.r, v.d, x);
+  mr_cycle_body<T>(C, 0, b, x);
+}
+
 // largest eigenvalue of a symmetric tridiagonal matrix (bisection, Sturm counts)
 double tridiag_lmax(const std::vector<double> &d, const std::vector<double> &e) {
   const int k = (int)d.size();
@@ -606,6 +789,101 @@ int solve_el_mixed(uft_ctx *C, const double *xphys, double *U) {
   return UFT_OK;
 }
 
+// Multi-load-case solve: one flexible CG per right-hand side, all advanced
+// together (FP64 iterate, residual and inner products; per-RHS scalars); the
+// cycle runs in FP64 (UFT_FP64) or FP32 with matrix-core element products
+// (mixed forms).  A right-hand side stops updating once
+// ||F_k - K U_k|| <= rtol ||F_k||; the loop ends when all have.
+enum { MS_RZ = 0, MS_PQ = 8, MS_ALPHA = 16, MS_RR = 24, MS_RZN = 32, MS_BETA = 48, MS_FN = 56, MS_TR = 64 };
+int solve_el_block(uft_ctx *C, const double *xphys) {
+  hipStream_t s = C->s;
+  MrState &M = C->mr;
+  const int nr = M.nr;
+  const int64_t n = C->ndof;
+  double *sc = M.sc;
+  const double t0 = now_ms();
+  int rc = amg_refresh(C, xphys);
+  if (rc) return rc;
+  C->t_phase[0] = now_ms() - t0;
+  uftk::fill(s, n * nr, 0.0, M.U);
+  uftk::copy(s, n * nr, M.F, M.r);
+  uftk::mr_dot(s, nr, n, M.F, M.F, C->d_part, sc + MS_FN);
+  if ((rc = fetch(C, sc + MS_FN, nr, C->h_scal))) return rc;
+  double fn[8], tol[8];
+  unsigned mask = 0;
+  for (int k = 0; k < nr; ++k) {
+    fn[k] = std::sqrt(C->h_scal[k]);
+    tol[k] = C->P.rtol_el * fn[k];
+    M.it[k] = 0;
+    M.relres[k] = 0.0;
+    if (fn[k] > 0.0) mask |= 1u << k;
+  }
+  const bool f32 = is_mixed(C);
+  auto precond = [&](const double *r, double *z) {
+    if (f32) {
+      uftk::d2f(s, n * nr, r, M.rf);
+      mr_fine_cycle<float>(C, M.rf, M.zf);
+      uftk::f2d(s, n * nr, M.zf, z);
+    } else {
+      mr_fine_cycle<double>(C, r, z);
+    }
+  };
+  double *z = M.z, *zo = M.zo;
+  int it = 0;
+  if (mask) {
+    precond(M.r, z);
+    uftk::mr_dot(s, nr, n, M.r, z, C->d_part, sc + MS_RZ);
+    uftk::copy(s, n * nr, z, M.p);
+  }
+  RowOp<double> pq;
+  pq.mode = ROW_STOREDOT;
+  pq.y = M.q;
+  pq.part = C->d_part;
+  while (mask && it < C->P.maxit_el) {
+    mr_lev_op<double>(C, 0, M.p, pq);  // q = K p and p.q per right-hand side
+    uftk::final_sum(s, C->d_part, mr_lev_blocks(C, 0), nr, sc + MS_PQ);
+    uftk::mr_alpha(s, nr, mask, sc + MS_RZ, sc + MS_PQ, sc + MS_ALPHA);
+    uftk::mr_cg_update(s, nr, n, sc + MS_ALPHA, M.U, M.p, M.r, M.q, C->d_part, sc + MS_RR);
+    ++it;
+    if ((rc = fetch(C, sc + MS_RR, nr, C->h_scal))) return rc;
+    for (int k = 0; k < nr; ++k) {
+      if (!((mask >> k) & 1u)) continue;
+      const double rn = std::sqrt(C->h_scal[k]);
+      if (!std::isfinite(rn)) return uft_set_error(C, UFT_ERR_NOCONV, "load case %d diverged at iteration %d", k, it);
+      M.it[k] = it;
+      if (rn <= tol[k]) mask &= ~(1u << k);
+    }
+    if (!mask) break;
+    std::swap(z, zo);
+    precond(M.r, z);
+    uftk::mr_dot_pr(s, nr, n, M.r, z, zo, C->d_part, sc + MS_RZN);
+    uftk::mr_beta(s, nr, mask, sc + MS_RZN, sc + MS_RZ, sc + MS_BETA);
+    uftk::mr_xpby(s, nr, n, z, sc + MS_BETA, M.p);
+  }
+  C->it_el = it;
+  C->outer_el = 1;
+  // true residuals F - K U per right-hand side (SolveFem.cpp:142-152)
+  RowOp<double> tr;
+  tr.mode = ROW_RES;
+  tr.base = M.F;
+  tr.r = M.q;
+  mr_lev_op<double>(C, 0, M.U, tr);
+  uftk::mr_dot(s, nr, n, M.q, M.q, C->d_part, sc + MS_TR);
+  if ((rc = fetch(C, sc + MS_TR, nr, C->h_scal))) return rc;
+  double worst = 0.0;
+  for (int k = 0; k < nr; ++k) {
+    M.relres[k] = fn[k] > 0.0 ? std::sqrt(C->h_scal[k]) / fn[k] : 0.0;
+    worst = std::max(worst, M.relres[k]);
+  }
+  C->relres_el = worst;
+  C->fnorm = fn[0];
+  // case 0 is the context's U (for the VTU writer and uft_vec_get)
+  uftk::mr_extract(s, n, nr, 0, M.U, C->vec[UFT_U]);
+  if (mask)
+    return uft_set_error(C, UFT_ERR_NOCONV, "load cases: %d iterations, worst relative residual %.3e", it, worst);
+  return UFT_OK;
+}
+
 // Jacobi-CG on KF RX = TX with the reference's KSPCG settings
 // (PDEFilter.cpp:117-137): warm start from the previous RX, preconditioned
 // residual norm relative to ||D^-1 b||, atol 1e-50, dtol 1e5.
@@ -683,6 +961,7 @@ void uft_default_params(uft_params *p) {
   p->cheb_lo = 0.1;
   p->cheb_hi = 1.1;
   p->esteig_steps = 12;
+  p->load_cases = 0;
 }
 
 int uft_create(uft_ctx **out, int device) {
@@ -723,9 +1002,12 @@ int uft_params_set(uft_ctx *C, const uft_params *p) {
     return uft_set_error(C, UFT_ERR_ARG, "bad solver tolerances");
   if (!(p->cheb_lo >= 0.0) || !(p->cheb_hi > p->cheb_lo) || p->esteig_steps < 2 || p->esteig_steps > 64)
     return uft_set_error(C, UFT_ERR_ARG, "bad Chebyshev interval or estimate steps");
+  if (p->load_cases != 0 && p->load_cases != 1)
+    return uft_set_error(C, UFT_ERR_ARG, "load_cases must be 0 (superposed) or 1 (separate)");
   if (p->precision < UFT_FP64 || p->precision > UFT_MIXED_H)
     return uft_set_error(C, UFT_ERR_ARG, "precision must be UFT_FP64, UFT_MIXED or UFT_MIXED_H");
   if (C->is_setup && (p->m != C->P.m || p->rmin != C->P.rmin || p->nu != C->P.nu || p->precision != C->P.precision ||
+                      p->load_cases != C->P.load_cases ||
                       p->amg_levels != C->P.amg_levels || p->coarse_max != C->P.coarse_max))
     return uft_set_error(C, UFT_ERR_STATE, "m, rmin, nu, precision and the hierarchy shape are fixed at setup");
   C->P = *p;
@@ -836,7 +1118,8 @@ int uft_setup(uft_ctx *C) {
   // block partials of the deterministic two-launch reductions; the coarse
   // operators' fused K-cycle products write up to 3 per block of 256/32 rows
   // (a level has at most 6 rows per fine node)
-  if ((rc = dalloc_n(C, &C->d_part, (maxn / 256 + 2) * 48 + 3 * (2 * ndof * 32 / 256 + 2)))) return rc;
+  C->part_cap = (maxn / 256 + 2) * 48 + 3 * (2 * ndof * 32 / 256 + 2);
+  if ((rc = dalloc_n(C, &C->d_part, C->part_cap))) return rc;
   if ((rc = dalloc_n(C, &C->d_scal, 256))) return rc;
   if ((rc = dalloc_n(C, &C->d_lanczos, 3 * 64))) return rc;
   uftk::fill(s, 256, 0.0, C->d_scal);
@@ -960,6 +1243,61 @@ int uft_setup(uft_ctx *C) {
       if ((rc = dalloc_n(C, &A.d_cinvf, A.ncoarse * A.ncoarse))) return rc;
     UFT_HIP(C, hipStreamSynchronize(s));
   }
+  // ---- multi-load-case form: interleaved right-hand sides and work vectors
+  if (C->P.load_cases == 1 && C->nlc > 1) {
+    if (C->nlc > 8) return uft_set_error(C, UFT_ERR_ARG, "the multi-load-case form takes at most 8 load cases");
+    MrState &M = C->mr;
+    M.ncase = C->nlc;
+    M.nr = C->nlc <= 2 ? 2 : (C->nlc <= 4 ? 4 : 8);
+    const int nr = M.nr;
+    const size_t ew = is_mixed(C) ? sizeof(float) : sizeof(double);
+    double **mv[] = {&M.F, &M.U, &M.r, &M.z, &M.p, &M.q, &M.zo};
+    for (double **p : mv)
+      if ((rc = dalloc_n(C, p, ndof * nr))) return rc;
+    if (is_mixed(C)) {
+      if ((rc = dalloc_n(C, &M.rf, ndof * nr))) return rc;
+      if ((rc = dalloc_n(C, &M.zf, ndof * nr))) return rc;
+      if ((rc = dalloc_n(C, &C->d_KEb, (int64_t)(np / 2) * ne))) return rc;
+      uftk::ke_to_bf16(s, C);
+    }
+    if ((rc = dalloc(C, &M.Ye, (size_t)nd * ne * nr * sizeof(double)))) return rc;
+    if ((rc = dalloc_n(C, &M.sc, 1024))) return rc;
+    uftk::fill(s, 1024, 0.0, M.sc);
+    int64_t need = 0;
+    for (int l = 0; l < C->amg.nlev; ++l) {
+      const AmgLevel &Lv = C->amg.L[l];
+      MrLevelBuf &B = M.L[l];
+      void **bv[] = {&B.x, &B.b, &B.r, &B.d, &B.t};
+      for (void **p : bv)
+        if ((rc = dalloc(C, p, (size_t)Lv.n * nr * ew))) return rc;
+      if (l > 0) {
+        void **kv[] = {&B.kc, &B.kv, &B.kd, &B.kr};
+        for (void **p : kv)
+          if ((rc = dalloc(C, p, (size_t)Lv.n * nr * ew))) return rc;
+        need = std::max<int64_t>(need, (int64_t)uftk::mr_bsr_blocks(Lv.nunit) * 3 * nr);
+      }
+    }
+    if (6 * C->amg.L[C->amg.nlev - 1].nunit > UFT_MR_TAIL_MAX)
+      return uft_set_error(C, UFT_ERR_ARG, "multi-load-case form: coarsest level above %d unknowns", UFT_MR_TAIL_MAX);
+    need = std::max<int64_t>(need, (int64_t)uftk::mr_gather_blocks(nn, nr) * nr);
+    need = std::max<int64_t>(need, (int64_t)((ndof * nr + 255) / 256) * 2 * nr);
+    if (need > C->part_cap) {  // the block partials outgrow the single-vector buffer
+      if ((rc = dalloc_n(C, &C->d_part, need + 64))) return rc;
+      C->part_cap = need + 64;
+    }
+    // loads INSERTed case by case, then the fixed DOFs zeroed (SolveFem.cpp:277-333)
+    uftk::fill(s, ndof * nr, 0.0, M.F);
+    int32_t *dld, *dfx;
+    double *dlv;
+    if ((rc = upload(C, &dld, C->ldofs))) return rc;
+    if ((rc = upload(C, &dlv, C->lvals))) return rc;
+    if ((rc = upload(C, &dfx, C->fixed))) return rc;
+    for (int k = 0; k < C->nlc; ++k) {
+      uftk::mr_set_dofs(s, C->nload, nr, k, dld + (size_t)k * C->nload, dlv + (size_t)k * C->nload, M.F);
+      uftk::mr_set_dofs(s, (int)C->fixed.size(), nr, k, dfx, nullptr, M.F);
+    }
+    UFT_HIP(C, hipStreamSynchronize(s));
+  }
   // ---- design state (OptPara.cpp:150-153) and MMA (OptPara.cpp:158-177, MMA.cpp:178-224)
   for (int id : {UFT_X, UFT_XFILED, UFT_XPHYS, UFT_XOLD}) uftk::fill(s, ne, C->P.volfrac, C->vec[id]);
   for (int id : {UFT_DFDX, UFT_XMIN, UFT_XMAX}) uftk::fill(s, ne, 0.0, C->vec[id]);
@@ -1049,13 +1387,17 @@ int uft_fem_obj_sens(uft_ctx *C, double *fx, double *gx, int dfdx_id, int m, con
   const double *xp = C->vec[xphys_id];
   double *U = C->vec[UFT_U];
   const double t0 = now_ms();
-  const int src = is_mixed(C) ? solve_el_mixed(C, xp, U) : solve_el(C, xp, U);
+  const int src = C->mr.nr ? solve_el_block(C, xp) : is_mixed(C) ? solve_el_mixed(C, xp, U) : solve_el(C, xp, U);
   const double t1 = now_ms();
   C->t_phase[1] = t1 - t0 - C->t_phase[0];
   if (src != UFT_OK && src != UFT_ERR_NOCONV) return src;
   // SolveFem.cpp:208-249: only constraint 0 is filled by the reference
   uftk::sens(C->s, C, xp, U, C->vec[dfdx_id], C->vec[dgdx_ids[0]], C->d_part);
   uftk::final_sum(C->s, C->d_part, uftk::red_blocks(C->ne), 2, C->d_scal + S_RED);
+  if (C->mr.nr) {  // summed over the load cases
+    uftk::mr_sens(C->s, C, C->mr.nr, xp, C->mr.U, C->vec[dfdx_id], C->d_part);
+    uftk::final_sum(C->s, C->d_part, uftk::mr_sens_blocks(C->ne, C->mr.nr), 1, C->d_scal + S_RED);
+  }
   rc = fetch(C, C->d_scal + S_RED, 2, C->h_scal);
   if (rc) return rc;
   fx[0] = C->h_scal[0];
@@ -1512,6 +1854,65 @@ extern "C" int uft_stress(uft_ctx *C, int xphys_id, int vms_id) {
     return uft_set_error(C, UFT_ERR_ARG, "uft_stress: bad ids");
   uftk::stress(C->s, C, C->vec[xphys_id], C->vec[UFT_U], C->vec[vms_id]);
   return UFT_OK;
+}
+
+// ---- multi-load-case form: per-case results
+extern "C" int uft_load_case_stats(const uft_ctx *C, int *iters, double *relres) {
+  if (!C || !C->is_setup) return -1;
+  const int n = C->mr.nr ? C->mr.ncase : 0;
+  for (int k = 0; k < n; ++k) {
+    if (iters) iters[k] = C->mr.it[k];
+    if (relres) relres[k] = C->mr.relres[k];
+  }
+  return n;
+}
+
+// y_k = K(XPhys) u_k for every load case through the multi-load-case
+// operator kernels: kind 0 the FP64 kernel of the outer loop, kind 1 the
+// preconditioner's (FP32 vectors; matrix-core BF16 element products in the
+// mixed forms).  u, y: host arrays of ncase x 3nn, case-major.
+extern "C" int uft_apply_K_cases(uft_ctx *C, int kind, const double *u, double *y) {
+  if (int rc = check_setup(C)) return rc;
+  MrState &M = C->mr;
+  if (!u || !y || !M.nr || (kind != 0 && kind != 1))
+    return uft_set_error(C, UFT_ERR_ARG, "uft_apply_K_cases: bad arguments or not the multi-load-case form");
+  const int nr = M.nr;
+  const int64_t n = C->ndof;
+  std::vector<double> ui((size_t)n * nr, 0.0), yi((size_t)n * nr);
+  for (int k = 0; k < M.ncase; ++k)
+    for (int64_t i = 0; i < n; ++i) ui[(size_t)i * nr + k] = u[(size_t)k * n + i];
+  C->cur_xphys = C->vec[UFT_XPHYS];
+  if (kind == 0 || !is_mixed(C)) {
+    UFT_HIP(C, hipMemcpyAsync(M.p, ui.data(), ui.size() * sizeof(double), hipMemcpyHostToDevice, C->s));
+    RowOp<double> op;
+    op.mode = ROW_STORE;
+    op.y = M.q;
+    mr_lev_op<double>(C, 0, M.p, op);
+    UFT_HIP(C, hipMemcpyAsync(yi.data(), M.q, yi.size() * sizeof(double), hipMemcpyDeviceToHost, C->s));
+  } else {
+    std::vector<float> uf(ui.begin(), ui.end()), yf(yi.size());
+    UFT_HIP(C, hipMemcpyAsync(M.rf, uf.data(), uf.size() * sizeof(float), hipMemcpyHostToDevice, C->s));
+    RowOp<float> op;
+    op.mode = ROW_STORE;
+    op.y = M.zf;
+    mr_lev_op<float>(C, 0, M.rf, op);
+    UFT_HIP(C, hipMemcpyAsync(yf.data(), M.zf, yf.size() * sizeof(float), hipMemcpyDeviceToHost, C->s));
+    if (int rc = sync(C)) return rc;
+    for (size_t i = 0; i < yi.size(); ++i) yi[i] = yf[i];
+  }
+  if (int rc = sync(C)) return rc;
+  for (int k = 0; k < M.ncase; ++k)
+    for (int64_t i = 0; i < n; ++i) y[(size_t)k * n + i] = yi[(size_t)i * nr + k];
+  return UFT_OK;
+}
+
+extern "C" int uft_load_case_u(uft_ctx *C, int k, double *host) {
+  if (int rc = check_setup(C)) return rc;
+  if (!host || !C->mr.nr || k < 0 || k >= C->mr.ncase)
+    return uft_set_error(C, UFT_ERR_ARG, "uft_load_case_u: no load case %d in the multi-load-case form", k);
+  uftk::mr_extract(C->s, C->ndof, C->mr.nr, k, C->mr.U, C->d_q);
+  UFT_HIP(C, hipMemcpyAsync(host, C->d_q, C->ndof * sizeof(double), hipMemcpyDeviceToHost, C->s));
+  return sync(C);
 }
 
 extern "C" int uft_amg_operator_get(uft_ctx *C, int level, int32_t *rp, int32_t *ci, double *val) {

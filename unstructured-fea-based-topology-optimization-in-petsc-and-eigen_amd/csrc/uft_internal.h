@@ -13,6 +13,7 @@
 #define UFT_MAXLEV 8
 #define UFT_RED_BLOCK 256
 #define UFT_TAIL_MAX 3072  // coarsest unknowns handled by the fused coarse tail (LDS)
+#define UFT_MR_TAIL_MAX 512  // the same for the multi-load-case tail (x 8 right-hand sides)
 
 // Per-row epilogue of a fused operator apply t = A u (see kernels_cycle.hip):
 //  STORE     y = t
@@ -22,7 +23,8 @@
 //  RESFIRSTX r = kr = base - t; dout = kd = it D^-1 r
 //  DOTS2     y = t; partials of u.t, u.w1
 //  DOTS3     partials of u.w1, u.t, u.w2
-enum { ROW_STORE = 0, ROW_STEP, ROW_RES, ROW_RESFIRST, ROW_RESFIRSTX, ROW_DOTS2, ROW_DOTS3 };
+//  STOREDOT  y = t; partials of u.t
+enum { ROW_STORE = 0, ROW_STEP, ROW_RES, ROW_RESFIRST, ROW_RESFIRSTX, ROW_DOTS2, ROW_DOTS3, ROW_STOREDOT };
 template <typename T>
 struct RowOp {
   int mode = ROW_STORE;
@@ -84,6 +86,25 @@ struct AmgHier {
   double *d_cwork = nullptr;
 };
 
+// Multi-load-case form: nr right-hand sides solved together, vectors stored
+// interleaved [unknown][nr] (kernels_mrhs.hip).
+struct MrLevelBuf {
+  void *x = nullptr, *b = nullptr, *r = nullptr, *d = nullptr, *t = nullptr;
+  void *kc = nullptr, *kv = nullptr, *kd = nullptr, *kr = nullptr;
+};
+struct MrState {
+  int nr = 0;      // right-hand sides per block (2, 4 or 8); 0: single right-hand side
+  int ncase = 0;   // load cases in use (<= nr; the rest are zero)
+  double *F = nullptr, *U = nullptr;                                // ndof x nr
+  double *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr, *zo = nullptr;
+  float *rf = nullptr, *zf = nullptr;                               // mixed: FP32 copies for the cycle
+  MrLevelBuf L[UFT_MAXLEV];                                         // cycle work (FP64 or FP32)
+  void *Ye = nullptr;                                               // nd x ne x nr element-local scratch
+  double *sc = nullptr;                                             // per right-hand-side scalars
+  double relres[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  int it[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+};
+
 // host-side description produced by uft_amg_setup.cpp
 struct AmgHostLevel {
   int64_t nunit_next = 0;
@@ -124,6 +145,7 @@ struct uft_ctx {
   double *d_KE = nullptr;      // packed upper triangle, pair-interleaved [np/2][ne][2]
   float *d_KEf = nullptr;      // FP32 copy for the mixed form [np/2][ne][2]
   uint2 *d_KEh = nullptr;      // FP16 smoother copy, KE/s_e in groups of four [ng][ne]
+  uint32_t *d_KEb = nullptr;   // BF16 pairs for the multi-load-case matrix-core products [np/2][ne]
   float *d_KEs = nullptr;      // per element scale s_e = max |KE_e|
   double *d_KF = nullptr;      // Helmholtz packed [npf/2][ne][2]
   double *d_T = nullptr;       // [npe][ne]
@@ -146,6 +168,7 @@ struct uft_ctx {
   float *d_rf = nullptr, *d_zf = nullptr, *d_pf = nullptr, *d_qf = nullptr, *d_uf = nullptr;
   AmgHier amg;
   bool amg_built = false;
+  MrState mr;
 
   // filter work (nn)
   double *d_fr = nullptr, *d_fz = nullptr, *d_fp = nullptr, *d_fq = nullptr, *d_fdinv = nullptr;
@@ -153,6 +176,7 @@ struct uft_ctx {
 
   // reductions
   double *d_part = nullptr;    // partials
+  int64_t part_cap = 0;        // doubles in d_part
   double *d_scal = nullptr;    // device scalars
   double *h_scal = nullptr;    // pinned host mirror
 
@@ -290,6 +314,55 @@ void bsr_to_dense(hipStream_t s, int64_t nb_rows, const int32_t *rp, const int32
 void dense_spd_inverse(hipStream_t s, int64_t n, double *A, double *Ainv, float *Ainvf);
 void dense_mv(hipStream_t s, int64_t n, const double *A, const double *x, double *y);
 void dense_mvf(hipStream_t s, int64_t n, const float *A, const float *x, float *y);
+// multi-load-case kernels (kernels_mrhs.hip)
+void mr_cheb_first(hipStream_t s, int nr, int64_t n, const double *b, const double *dinv, double it, double *r,
+                   double *d, double *x);
+void mr_cheb_first(hipStream_t s, int nr, int64_t n, const float *b, const float *dinv, double it, float *r, float *d,
+                   float *x);
+void mr_elem_phase(hipStream_t s, const uft_ctx *C, int nr, const double *xphys, const double *u, double *Ye);
+void mr_elem_phase(hipStream_t s, const uft_ctx *C, int nr, const double *xphys, const float *u, float *Ye);
+void ke_to_bf16(hipStream_t s, const uft_ctx *C);
+void mr_gather_op(hipStream_t s, const uft_ctx *C, int nr, const double *u, const double *Ye, const RowOp<double> &o);
+void mr_gather_op(hipStream_t s, const uft_ctx *C, int nr, const float *u, const float *Ye, const RowOp<float> &o);
+int mr_gather_blocks(int64_t nn, int nr);
+void mr_bsr_op(hipStream_t s, int nr, int64_t nb_rows, const int32_t *rp, const int32_t *ci, const double *val,
+               const double *u, const RowOp<double> &o);
+void mr_bsr_op(hipStream_t s, int nr, int64_t nb_rows, const int32_t *rp, const int32_t *ci, const float *val,
+               const float *u, const RowOp<float> &o);
+int mr_bsr_blocks(int64_t nb_rows);
+void mr_restrict_first(hipStream_t s, int nr, int bs, int64_t nc, const int32_t *cptr, const int32_t *child,
+                       const double *P, const double *r, double *bc, const double *dinv, double it, double *rc,
+                       double *dc, double *xc);
+void mr_restrict_first(hipStream_t s, int nr, int bs, int64_t nc, const int32_t *cptr, const int32_t *child,
+                       const float *P, const float *r, float *bc, const float *dinv, double it, float *rc, float *dc,
+                       float *xc);
+void mr_prolong_comb(hipStream_t s, int nr, int bs, int64_t nu, const int32_t *parent, const double *P,
+                     const double *sc, int ia, int ib, const double *xa, const double *xb, double *x);
+void mr_prolong_comb(hipStream_t s, int nr, int bs, int64_t nu, const int32_t *parent, const float *P, const double *sc,
+                     int ia, int ib, const float *xa, const float *xb, float *x);
+void mr_lincomb_first(hipStream_t s, int nr, int64_t n, const double *sc, const double *b, const double *v, double *kr,
+                      double *r, const double *dinv, double it, double *d, double *kd);
+void mr_lincomb_first(hipStream_t s, int nr, int64_t n, const double *sc, const float *b, const float *v, float *kr,
+                      float *r, const float *dinv, double it, float *d, float *kd);
+void mr_final_kc1(hipStream_t s, int nr, const double *part, int nb, double *sc);
+void mr_final_kc2(hipStream_t s, int nr, const double *part, int nb, double *sc);
+void mr_tail_solve(hipStream_t s, int nr, int bs, int nc_units, const int32_t *cptr, const int32_t *child,
+                   const double *P, const double *Ainv, const double *r, double *xc);
+void mr_tail_solve(hipStream_t s, int nr, int bs, int nc_units, const int32_t *cptr, const int32_t *child,
+                   const float *P, const double *Ainv, const float *r, float *xc);
+void mr_dot(hipStream_t s, int nr, int64_t n, const double *a, const double *b, double *part, double *out);
+void mr_dot_pr(hipStream_t s, int nr, int64_t n, const double *r, const double *z, const double *zo, double *part,
+               double *out);
+void mr_cg_update(hipStream_t s, int nr, int64_t n, const double *alpha, double *x, const double *p, double *r,
+                  const double *q, double *part, double *rr_out);
+void mr_xpby(hipStream_t s, int nr, int64_t n, const double *z, const double *beta, double *p);
+void mr_alpha(hipStream_t s, int nr, unsigned mask, const double *rz, const double *pq, double *alpha);
+void mr_beta(hipStream_t s, int nr, unsigned mask, const double *rzn, double *rzprev, double *beta);
+void mr_sens(hipStream_t s, const uft_ctx *C, int nr, const double *xphys, const double *u, double *dfdx,
+             double *part);
+int mr_sens_blocks(int64_t ne, int nr);
+void mr_extract(hipStream_t s, int64_t n, int nr, int k, const double *src, double *dst);
+void mr_set_dofs(hipStream_t s, int n, int nr, int k, const int32_t *idx, const double *val, double *F);
 // fused cycle kernels (kernels_cycle.hip)
 void bsr6_op(hipStream_t s, int64_t nb_rows, const int32_t *rp, const int32_t *ci, const double *val, const double *u,
              const RowOp<double> &op);

@@ -1,6 +1,7 @@
 // Mesh-side host code: the synthetic cantilever generator, the reference's
 // text formats (InputMeshData.cpp:100-149,228-285; GetBcs/GetBCs.m) and the
 // node -> (element, local node) incidence used by the atomic-free node phase.
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
@@ -103,7 +104,7 @@ int uft_box_bcs(int nx, int ny, int nz, int load_kind, int32_t *fixed, int *nfix
       load_dofs[nl] = (int32_t)(3 * n + 1);
       load_vals[nl++] = -1.0 / (double)(nz + 1);
     }
-  } else {
+  } else if (load_kind == 1) {
     if (ny < 2 || nz < 2) return UFT_ERR_ARG;
     const int jc = ny / 2, kc = nz / 2;
     for (int k = kc - 1; k <= kc + 1; ++k)
@@ -112,6 +113,22 @@ int uft_box_bcs(int nx, int ny, int nz, int load_kind, int32_t *fixed, int *nfix
         load_dofs[nl] = (int32_t)(3 * n + 1);
         load_vals[nl++] = -1.0 / 9.0;
       }
+  } else if (load_kind == 2) {
+    // eight load cases: seeded 3x3 patches on x = Lx, unit total each
+    if (ny < 2 || nz < 2) return UFT_ERR_ARG;
+    uint64_t st = 8;
+    for (int lc = 0; lc < 8; ++lc) {
+      const int jc = 1 + (int)std::floor(sm64_u01(st) * (ny - 1));
+      const int kc = 1 + (int)std::floor(sm64_u01(st) * (nz - 1));
+      for (int k = kc - 1; k <= kc + 1; ++k)
+        for (int j = jc - 1; j <= jc + 1; ++j) {
+          const int64_t n = nx + sx * j + sxy * k;
+          load_dofs[nl] = (int32_t)(3 * n + 1);
+          load_vals[nl++] = -1.0 / 9.0;
+        }
+    }
+  } else {
+    return UFT_ERR_ARG;
   }
   *nload = nl;
   return UFT_OK;

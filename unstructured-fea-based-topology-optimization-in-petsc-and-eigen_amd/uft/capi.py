@@ -35,7 +35,8 @@ class Params(C.Structure):
                 ("rtol_el", C.c_double), ("rtol_f", C.c_double), ("maxit_el", C.c_int), ("maxit_f", C.c_int),
                 ("precision", C.c_int), ("amg_levels", C.c_int), ("cheb_degree", C.c_int),
                 ("coarse_max", C.c_int), ("cycle_type", C.c_int),
-                ("cheb_lo", C.c_double), ("cheb_hi", C.c_double), ("esteig_steps", C.c_int)]
+                ("cheb_lo", C.c_double), ("cheb_hi", C.c_double), ("esteig_steps", C.c_int),
+                ("load_cases", C.c_int)]
 
 
 _LIB = None
@@ -113,6 +114,9 @@ def lib():
         "uft_mma_kkt_residual": (C.c_int, [vp, C.c_int, C.c_int, pd, pi, C.c_int, C.c_int, pd, pd]),
         "uft_stress": (C.c_int, [vp, C.c_int, C.c_int]),
         "uft_cache_save": (C.c_int, [vp, C.c_char_p]),
+        "uft_load_case_stats": (C.c_int, [vp, pi, pd]),
+        "uft_load_case_u": (C.c_int, [vp, C.c_int, dp]),
+        "uft_apply_K_cases": (C.c_int, [vp, C.c_int, dp, dp]),
         "uft_cache_load": (C.c_int, [vp, C.c_char_p]),
         "uft_autotune": (C.c_int, [vp, C.c_int, pi, pd, C.c_int]),
     }
@@ -151,8 +155,8 @@ def box_bcs(nx, ny, nz, load_kind=0):
     L = lib()
     nn = (nx + 1) * (ny + 1) * (nz + 1)
     fixed = np.zeros(3 * nn, dtype=np.int32)
-    ld = np.zeros(max(nn, 9), dtype=np.int32)
-    lv = np.zeros(max(nn, 9))
+    ld = np.zeros(max(nn, 72), dtype=np.int32)
+    lv = np.zeros(max(nn, 72))
     nf, nl = C.c_int(), C.c_int()
     rc = L.uft_box_bcs(nx, ny, nz, load_kind, fixed, C.byref(nf), ld, lv, C.byref(nl))
     if rc:
@@ -349,6 +353,25 @@ class Context:
             chs.append(ch)
         return itr, np.array(fxs), np.array(gxs), np.array(chs)
 
+    def load_case_stats(self):
+        """Per load case (multi-load-case form): Krylov iterations and final
+        relative residual."""
+        it = (C.c_int * 8)()
+        rr = (C.c_double * 8)()
+        n = self.L.uft_load_case_stats(self.h, it, rr)
+        return [(it[k], rr[k]) for k in range(max(n, 0))]
+
+    def apply_K_cases(self, U, kind=0):
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        Y = np.zeros_like(U)
+        self._chk(self.L.uft_apply_K_cases(self.h, kind, U.ravel(), Y.ravel()), "uft_apply_K_cases")
+        return Y
+
+    def load_case_u(self, k):
+        out = np.zeros(self.size("ndof"))
+        self._chk(self.L.uft_load_case_u(self.h, k, out), "uft_load_case_u")
+        return out
+
     def stats(self):
         a, b, c, d = C.c_int(), C.c_double(), C.c_int(), C.c_int()
         self._chk(self.L.uft_solve_stats(self.h, C.byref(a), C.byref(b), C.byref(c), C.byref(d)), "uft_solve_stats")
@@ -423,6 +446,6 @@ def cantilever(npe=8, nx=8, ny=4, nz=4, jitter=0.0, seed=0, load_kind=0, device=
     ctx = Context(device)
     ctx.set_params(Context.default_params(**params))
     ctx.set_mesh(elem, xyz)
-    ctx.set_bcs(fixed, ld, lv)
+    ctx.set_bcs(fixed, ld, lv, nlc=8 if load_kind == 2 else 1)  # kind 2: eight cases of nine loads
     ctx.setup()
     return ctx
