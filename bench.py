@@ -40,6 +40,9 @@ WORKLOADS = {
     "c1": (8, 140, 70, 40, 392000, 0, 0.30, 1.5),
     "c0": (8, 40, 20, 20, 0, 0, 0.30, 1.5),
     "c2": (4, 60, 30, 30, 324000, 1, 0.25, 2.0),
+    # the c1 mesh with eight separate load cases (seeded 3x3 patches on x=2,
+    # unit total each), compliance summed over the cases
+    "c4a": (8, 140, 70, 40, 392000, 2, 0.30, 1.5),
 }
 
 # Krylov iteration counts of the oracle's reference-structure iteration on the
@@ -54,17 +57,19 @@ def workload(name, precision):
     npe, nx, ny, nz, seed, lk, vf, rf = WORKLOADS[name]
     h = 2.0 / nx
     params = dict(Emax=1.0, Emin=1e-9, nu=0.3, penal=3.0, volfrac=vf, rmin=rf * h, rtol_el=1e-8, rtol_f=1e-8,
-                  precision=precision)
+                  precision=precision, load_cases=1 if lk == 2 else 0)
     return npe, (nx, ny, nz), seed, lk, params
 
 
-def algorithmic_bytes_elem_phase(npe, ne, ndof, w=8):
+def algorithmic_bytes_elem_phase(npe, ne, ndof, w=8, nrhs=1):
     """Element phase of one fine stiffness apply: the packed unique KE entries
     (w bytes each), the element's node ids and FP64 density per element, plus
     one read of u (FP64 for w = 8, else FP32); FP16 blocks (w = 2) add the
     FP32 per-element scale."""
     S = 300 if npe == 8 else 78
     wv = 8 if w == 8 else 4
+    if nrhs > 1:  # BF16 blocks (w = 2, no scale) or FP64 blocks, nrhs interleaved vectors
+        return ne * (S * w + 4 * npe + 8) + nrhs * ndof * wv
     return ne * (S * w + 4 * npe + 8 + (4 if w == 2 else 0)) + ndof * wv
 
 
@@ -79,6 +84,7 @@ def cpu_baseline(name, gpu_iters):
     import oracle as O
     npe, (nx, ny, nz), seed, lk, prm = workload(name, 0)
     prm.pop("precision")
+    prm.pop("load_cases")
     t0 = time.time()
     P = O.cantilever(npe, nx, ny, nz, jitter=0.15, seed=seed, load_kind=lk, ke_cache=0, **prm)
     t_create = time.time() - t0
@@ -112,7 +118,7 @@ def run_form(args, precision):
     ctx = Context(0)
     ctx.set_params(Context.default_params(**prm))
     ctx.set_mesh(elem, xyz)
-    ctx.set_bcs(fixed, ld, lv)
+    ctx.set_bcs(fixed, ld, lv, nlc=8 if lk == 2 else 1)
     t0 = time.time()
     ctx.setup()
     ctx.sync()
@@ -146,10 +152,10 @@ def run_form(args, precision):
             "its": its, "fxs": fxs, "prof": prof}
 
 
-def elem_roofline(name, npe, ne, ndof, prof_elem, prof_node, word, kernel, traffic_key):
+def elem_roofline(name, npe, ne, ndof, prof_elem, prof_node, word, kernel, traffic_key, nrhs=1):
     """HBM roofline of an element phase from its HIP-event timings."""
     n, ms = prof_elem
-    algo = algorithmic_bytes_elem_phase(npe, ne, ndof, word)
+    algo = algorithmic_bytes_elem_phase(npe, ne, ndof, word, nrhs)
     avg_s = ms / max(n, 1) / 1e3
     achieved = algo / avg_s / 1e9 if n else None
     traffic = None
@@ -197,20 +203,27 @@ def main():
     value = args.steps / r["dt"]
     ename = "hex8" if npe == 8 else "tet4"
     rooflines = []
+    multi = WORKLOADS[args.workload][5] == 2
+    nrhs = 8 if multi else 1
+    k64 = f"k_mr_elem<{npe},double,8>" if multi else f"k_elem_apply_v<{npe},double>"
     if "fp64" in res:
         q = res["fp64"]
         rooflines.append(elem_roofline("fp64", npe, ne, ndof, q["prof"][0], q["prof"][1], 8,
-                                       f"k_elem_apply_v<{npe},double> (FP64 element phase of the fine stiffness "
-                                       f"apply, FP64 form)", f"{args.workload}_fp64_elem_apply"))
+                                       f"{k64} (FP64 element phase of the fine stiffness apply, FP64 form)",
+                                       f"{args.workload}_fp64_elem_apply", nrhs))
     if main_form != "fp64":
-        w = 2 if main_form == "mixed_h" else 4
-        kname = (f"k_elem_apply_h<{npe}> (FP16-block element phase of the smoother, mixed form)"
-                 if main_form == "mixed_h" else f"k_elem_apply_v<{npe},float> (FP32 element phase, mixed form)")
+        if multi:
+            w, kname = 2, (f"k_mr_elem_mfma<{npe}> (BF16 v_mfma_f32_16x16x32 element products of the "
+                           f"preconditioner, 8 load cases, mixed form)")
+        elif main_form == "mixed_h":
+            w, kname = 2, f"k_elem_apply_h<{npe}> (FP16-block element phase of the smoother, mixed form)"
+        else:
+            w, kname = 4, f"k_elem_apply_v<{npe},float> (FP32 element phase, mixed form)"
         rooflines.append(elem_roofline(main_form, npe, ne, ndof, r["prof"][2], r["prof"][3], w, kname,
-                                       f"{args.workload}_{main_form}_elem_apply"))
+                                       f"{args.workload}_{main_form}_elem_apply", nrhs))
         rooflines.append(elem_roofline("outer", npe, ne, ndof, r["prof"][0], r["prof"][1], 8,
-                                       f"k_elem_apply_v<{npe},double> (FP64 element phase of the outer loop, "
-                                       f"mixed form)", f"{args.workload}_fp64_elem_apply"))
+                                       f"{k64} (FP64 element phase of the outer loop, mixed form)",
+                                       f"{args.workload}_fp64_elem_apply", nrhs))
     # the dominant element phase of the timed form: the one with the most time
     cand = [x for x in rooflines if x["launches"]] if main_form == "fp64" else \
         [x for x in rooflines[-2:] if x["launches"]]

@@ -523,6 +523,10 @@ void galerkin_pre(hipStream_t s, const uft_ctx *C, int64_t ngl, const int32_t *g
   else
     k_galerkin_pre<4><<<nblk(ngl), TPB, 0, s>>>(ngl, (int)C->ne, gblk, gl, brow, ci, parent, C->d_elem, C->d_KE, Q, G);
 }
+void simp_moduli(hipStream_t s, const uft_ctx *C, const double *xphys, double *Ee) {
+  SimpCoef sc{C->P.Emin, C->P.Emax - C->P.Emin, C->P.penal};
+  k_simp_E<<<nblk(C->ne), TPB, 0, s>>>(C->ne, xphys, sc, Ee);
+}
 void galerkin_fine(hipStream_t s, const uft_ctx *C, const double *xphys, int64_t nb, const int32_t *glptr,
                    const int32_t *gl, const int32_t *gtr, const int32_t *brow, const int32_t *ci, const double *G,
                    double *Ee, double *val) {

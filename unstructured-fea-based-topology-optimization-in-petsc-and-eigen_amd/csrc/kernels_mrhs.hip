@@ -9,6 +9,7 @@
 // The fine element products of the mixed form run on the matrix cores:
 // Y_e (nd x NR) = KE_e (nd x nd) U_e (nd x NR) as v_mfma_f32_16x16x32_bf16
 // tiles (BF16 operands, FP32 accumulation), one element per wavefront step.
+// Element-local results are stored element-major, [ne][nd][NR].
 #include "uft_elem.h"
 #include "uft_internal.h"
 #include "uft_reduce.h"
@@ -167,19 +168,22 @@ __global__ __launch_bounds__(TPB) void k_mr_elem(int ne, const int32_t *__restri
   });
   const T E = (T)simpE(sc, xphys[e]);
 #pragma unroll
-  for (int k = 0; k < ND; ++k) Ye[((size_t)k * ne + e) * NR + rr] = E * ye[k];
+  for (int k = 0; k < ND; ++k) Ye[((size_t)e * ND + k) * NR + rr] = E * ye[k];
 }
 
 // ---------------------------------------------------------------- element phase, matrix cores
-// One wavefront handles MF_EL consecutive elements at a time: their packed
-// BF16 KE pairs ([NQ][ne] words, two BF16 per word) and node ids are staged
-// in LDS with coalesced loads (MF_EL x 4 bytes per KE row), then per element
-// two (hex8) or one (tet4) 16x16x32 BF16 MFMA tiles form Y_e = KE_e U_e for
-// the 8 right-hand sides.  Operand layout of v_mfma_f32_16x16x32_bf16
-// (lane l): A[l % 16][8 (l / 16) + j], B[8 (l / 16) + j][l % 16],
-// D[4 (l / 16) + i][l % 16].  A rows are element DOFs (padded to 16 per
-// tile), K is the element DOF (padded to 32), the columns are the
-// right-hand sides (8 of 16 used).
+// Y_e (nd x 8) = E(x_e) KE_e U_e for the 8 right-hand sides on the matrix
+// cores (E(x_e) from the per-element moduli of the last hierarchy refresh,
+// which used the same density)
+// cores; per element two (hex8) or one (tet4) v_mfma_f32_16x16x32_bf16 tiles
+// (BF16 operands, FP32 accumulation).  Operand layout (lane l):
+// A[l % 16][8 (l / 16) + j], B[8 (l / 16) + j][l % 16], D[4 (l / 16) + i][l % 16];
+// A rows are element DOFs (16 per tile), K is the element DOF (padded to 32),
+// the columns are the right-hand sides (8 of 16 used).  The element blocks
+// are stored for this product as full BF16 rows ([ne][nd][ndp], ndp = nd
+// rounded up to 8, zero padded) so each lane's A fragment is one 16-byte
+// load; U_e of MF_EL elements is staged in LDS as BF16 (fixed DOFs zero,
+// one 96-byte node row per load) and read as one 16-byte B fragment.
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 constexpr int MF_WAVES = 4;  // wavefronts per workgroup
@@ -187,86 +191,113 @@ constexpr int MF_EL = 16;    // elements per wavefront batch
 
 template <int NPE>
 __global__ __launch_bounds__(64 * MF_WAVES) void k_mr_elem_mfma(int ne, const int32_t *__restrict__ elem,
-                                                               const uint32_t *__restrict__ KEb,
-                                                               const double *__restrict__ xphys, SimpCoef sc,
+                                                               const uint4 *__restrict__ KEr,
+                                                               const double *__restrict__ Ee,
                                                                const uint8_t *__restrict__ fixm,
                                                                const float *__restrict__ u, float *__restrict__ Ye) {
-  constexpr int ND = 3 * NPE, NP = ND * (ND + 1) / 2, NQ = NP / 2, TILES = (ND + 15) / 16;
+  constexpr int ND = 3 * NPE, NDP = (ND + 7) / 8 * 8, CH = NDP / 8, TILES = (ND + 15) / 16;
   constexpr int NR = 8;
-  __shared__ uint32_t ks[MF_WAVES][NQ][MF_EL];  // packed pairs, element fastest
-  __shared__ int32_t nodes[MF_WAVES][NPE][MF_EL];
+  __shared__ __attribute__((aligned(16))) __bf16 ub[MF_WAVES][MF_EL][NR][32];  // U_e^T, K padded to 32
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int64_t e0 = ((int64_t)blockIdx.x * MF_WAVES + w) * MF_EL;
-  // stage: lane = (row slot lane / MF_EL, element lane % MF_EL)
   {
     const int el = lane % MF_EL, rs = lane / MF_EL;
     const int64_t e = e0 + el;
     const bool ok = e < ne;
-    for (int q = rs; q < NQ; q += 64 / MF_EL) ks[w][q][el] = ok ? KEb[(size_t)q * ne + e] : 0u;
-    for (int a = rs; a < NPE; a += 64 / MF_EL) nodes[w][a][el] = ok ? elem[(size_t)a * ne + e] : 0;
+    for (int a = rs; a < NPE; a += 64 / MF_EL) {
+      float4 v[6];
+      unsigned m = 7u;
+      if (ok) {
+        const int n = elem[(size_t)a * ne + e];
+        m = fixm[n];
+        const float4 *row = reinterpret_cast<const float4 *>(u + 3 * (size_t)n * NR);
+#pragma unroll
+        for (int h = 0; h < 6; ++h) v[h] = row[h];
+      }
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const bool z = !ok || ((m >> c) & 1u);
+        const float4 lo = v[2 * c], hi = v[2 * c + 1];
+        const float vals[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+#pragma unroll
+        for (int r = 0; r < NR; ++r) ub[w][el][r][c * NPE + a] = (__bf16)(z ? 0.f : vals[r]);
+      }
+    }
+    for (int t = lane; t < MF_EL * NR * (32 - ND); t += 64) {
+      const int el2 = t / (NR * (32 - ND)), rem = t % (NR * (32 - ND));
+      ub[w][el2][rem / (32 - ND)][ND + rem % (32 - ND)] = (__bf16)0.f;
+    }
   }
   __syncthreads();
-  // per-lane operand coordinates (element independent)
-  const int col = lane % 16, kb = 8 * (lane / 16);
-  int pidx[TILES][8];  // packed index of A[row][kb + j] per tile, -1 for padding
+  const int col = lane % 16, kq = lane / 16, kb = 8 * kq;
+  uint4 an[TILES];
 #pragma unroll
-  for (int tl = 0; tl < TILES; ++tl)
+  for (int tl = 0; tl < TILES; ++tl) an[tl] = make_uint4(0u, 0u, 0u, 0u);
+  if (e0 < ne) {
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const int row = 16 * tl + col, k = kb + j;
-      pidx[tl][j] = (row < ND && k < ND) ? pk_index(row, k, ND) : -1;
+    for (int tl = 0; tl < TILES; ++tl) {
+      const int row = 16 * tl + col;
+      if (row < ND && kb < ND) an[tl] = KEr[((size_t)e0 * ND + row) * CH + kq];
     }
+  }
   for (int el = 0; el < MF_EL; ++el) {
     const int64_t e = e0 + el;
     if (e >= ne) break;  // uniform across the wavefront
-    // B operand: U_e[k][col] for k = kb..kb+7 (col = right-hand side)
-    bf16x8 bv;
+    uint4 ac[TILES];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const int k = kb + j;
-      float v = 0.f;
-      if (col < NR && k < ND) {
-        const int a = k % NPE, c = k / NPE;
-        const int n = nodes[w][a][el];
-        v = ((fixm[n] >> c) & 1u) ? 0.f : u[(3 * (size_t)n + c) * NR + col];
+    for (int tl = 0; tl < TILES; ++tl) ac[tl] = an[tl];
+    if (el + 1 < MF_EL && e + 1 < ne) {  // the next element's blocks in flight
+#pragma unroll
+      for (int tl = 0; tl < TILES; ++tl) {
+        const int row = 16 * tl + col;
+        if (row < ND && kb < ND) an[tl] = KEr[((size_t)(e + 1) * ND + row) * CH + kq];
       }
-      bv[j] = (__bf16)v;
     }
-    const float E = (float)simpE(sc, xphys[e]);
+    bf16x8 bv;
+    if (col < NR) {
+      bv = *reinterpret_cast<const bf16x8 *>(&ub[w][el][col][kb]);
+    } else {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) bv[j] = (__bf16)0.f;
+    }
+    const float E = (float)Ee[e];
+    float *Yel = Ye + (size_t)e * ND * NR;
 #pragma unroll
     for (int tl = 0; tl < TILES; ++tl) {
-      bf16x8 av;
-#pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        const int p = pidx[tl][j];
-        unsigned short bits = 0;
-        if (p >= 0) {
-          const uint32_t wd = ks[w][p >> 1][el];
-          bits = (unsigned short)((p & 1) ? (wd >> 16) : (wd & 0xffffu));
-        }
-        av[j] = __builtin_bit_cast(__bf16, bits);
-      }
+      const bf16x8 av = __builtin_bit_cast(bf16x8, ac[tl]);
       f32x4 acc = {0.f, 0.f, 0.f, 0.f};
       acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc, 0, 0, 0);
       if (col < NR) {
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-          const int row = 16 * tl + 4 * (lane / 16) + i;
-          if (row < ND) Ye[((size_t)row * ne + e) * NR + col] = E * acc[i];
+          const int row = 16 * tl + 4 * kq + i;
+          if (row < ND) Yel[row * NR + col] = E * acc[i];
         }
       }
     }
   }
 }
 
-// FP64 KE (pairs) -> BF16 pairs (round to nearest even), same [NQ][ne] layout
-__global__ __launch_bounds__(TPB) void k_ke_to_bf16(int64_t npairs, const double2 *__restrict__ KE,
-                                                    uint32_t *__restrict__ KEb) {
+// FP64 packed KE -> full BF16 rows [ne][nd][ndp] (round to nearest even,
+// zero padded): one thread per (element, row)
+template <int NPE>
+__global__ __launch_bounds__(TPB) void k_ke_to_bf16_rows(int ne, const double2 *__restrict__ KE,
+                                                         __bf16 *__restrict__ out) {
+  constexpr int ND = 3 * NPE, NDP = (ND + 7) / 8 * 8;
   const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
-  if (t >= npairs) return;
-  const double2 v = KE[t];
-  const __bf16 a = (__bf16)(float)v.x, b = (__bf16)(float)v.y;
-  KEb[t] = (uint32_t)__builtin_bit_cast(unsigned short, a) | ((uint32_t)__builtin_bit_cast(unsigned short, b) << 16);
+  if (t >= (int64_t)ne * ND) return;
+  const int64_t e = t / ND;
+  const int row = (int)(t - e * ND);
+  __bf16 *o = out + t * NDP;
+  for (int c = 0; c < NDP; ++c) {
+    float v = 0.f;
+    if (c < ND) {
+      const int p = pk_index(row, c, ND);
+      const double2 pr = KE[(size_t)(p >> 1) * ne + e];
+      v = (float)((p & 1) ? pr.y : pr.x);
+    }
+    o[c] = (__bf16)v;
+  }
 }
 
 // ---------------------------------------------------------------- node phase
@@ -286,7 +317,7 @@ __global__ __launch_bounds__(TPB) void k_mr_gather_op(int nn, int ne, const int3
       const int v = inc[k];
       const int e = v / NPE, l = v - e * NPE;
 #pragma unroll
-      for (int c = 0; c < 3; ++c) acc[c] += Ye[((size_t)(c * NPE + l) * ne + e) * NR + rr];
+      for (int c = 0; c < 3; ++c) acc[c] += Ye[((size_t)e * (3 * NPE) + c * NPE + l) * NR + rr];
     }
     const unsigned m = fixm[n];
 #pragma unroll
@@ -659,9 +690,11 @@ void mr_elem_phase(hipStream_t s, const uft_ctx *C, int nr, const double *xphys,
   if (nr == 8 && C->d_KEb) {  // matrix-core form
     const unsigned g = (unsigned)((ne + MF_EL * MF_WAVES - 1) / (MF_EL * MF_WAVES));
     if (C->npe == 8)
-      k_mr_elem_mfma<8><<<g, 64 * MF_WAVES, 0, s>>>(ne, C->d_elem, C->d_KEb, xphys, sc, C->d_fixm, u, Ye);
+      k_mr_elem_mfma<8><<<g, 64 * MF_WAVES, 0, s>>>(ne, C->d_elem, (const uint4 *)C->d_KEb, C->d_Ee, C->d_fixm, u,
+                                                    Ye);
     else
-      k_mr_elem_mfma<4><<<g, 64 * MF_WAVES, 0, s>>>(ne, C->d_elem, C->d_KEb, xphys, sc, C->d_fixm, u, Ye);
+      k_mr_elem_mfma<4><<<g, 64 * MF_WAVES, 0, s>>>(ne, C->d_elem, (const uint4 *)C->d_KEb, C->d_Ee, C->d_fixm, u,
+                                                    Ye);
     return;
   }
   const float2 *KE = (const float2 *)C->d_KEf;
@@ -674,9 +707,15 @@ void mr_elem_phase(hipStream_t s, const uft_ctx *C, int nr, const double *xphys,
   }
 }
 void ke_to_bf16(hipStream_t s, const uft_ctx *C) {
-  const int np = 3 * C->npe * (3 * C->npe + 1) / 2;
-  const int64_t npairs = (int64_t)(np / 2) * C->ne;
-  k_ke_to_bf16<<<nblk(npairs), TPB, 0, s>>>(npairs, (const double2 *)C->d_KE, C->d_KEb);
+  const int ne = (int)C->ne;
+  if (C->npe == 8)
+    k_ke_to_bf16_rows<8><<<nblk((int64_t)ne * 24), TPB, 0, s>>>(ne, (const double2 *)C->d_KE, (__bf16 *)C->d_KEb);
+  else
+    k_ke_to_bf16_rows<4><<<nblk((int64_t)ne * 12), TPB, 0, s>>>(ne, (const double2 *)C->d_KE, (__bf16 *)C->d_KEb);
+}
+int ke_bf16_rows_words(int npe) {  // 32-bit words per element of the row layout
+  const int nd = 3 * npe, ndp = (nd + 7) / 8 * 8;
+  return nd * ndp / 2;
 }
 
 template <typename T>

@@ -1257,7 +1257,7 @@ int uft_setup(uft_ctx *C) {
     if (is_mixed(C)) {
       if ((rc = dalloc_n(C, &M.rf, ndof * nr))) return rc;
       if ((rc = dalloc_n(C, &M.zf, ndof * nr))) return rc;
-      if ((rc = dalloc_n(C, &C->d_KEb, (int64_t)(np / 2) * ne))) return rc;
+      if ((rc = dalloc_n(C, &C->d_KEb, (int64_t)uftk::ke_bf16_rows_words(npe) * ne))) return rc;
       uftk::ke_to_bf16(s, C);
     }
     if ((rc = dalloc(C, &M.Ye, (size_t)nd * ne * nr * sizeof(double)))) return rc;
@@ -1882,6 +1882,7 @@ extern "C" int uft_apply_K_cases(uft_ctx *C, int kind, const double *u, double *
   for (int k = 0; k < M.ncase; ++k)
     for (int64_t i = 0; i < n; ++i) ui[(size_t)i * nr + k] = u[(size_t)k * n + i];
   C->cur_xphys = C->vec[UFT_XPHYS];
+  uftk::simp_moduli(C->s, C, C->cur_xphys, C->d_Ee);  // the matrix-core kernel reads E(x_e) from d_Ee
   if (kind == 0 || !is_mixed(C)) {
     UFT_HIP(C, hipMemcpyAsync(M.p, ui.data(), ui.size() * sizeof(double), hipMemcpyHostToDevice, C->s));
     RowOp<double> op;

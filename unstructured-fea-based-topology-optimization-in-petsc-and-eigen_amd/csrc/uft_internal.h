@@ -145,7 +145,7 @@ struct uft_ctx {
   double *d_KE = nullptr;      // packed upper triangle, pair-interleaved [np/2][ne][2]
   float *d_KEf = nullptr;      // FP32 copy for the mixed form [np/2][ne][2]
   uint2 *d_KEh = nullptr;      // FP16 smoother copy, KE/s_e in groups of four [ng][ne]
-  uint32_t *d_KEb = nullptr;   // BF16 pairs for the multi-load-case matrix-core products [np/2][ne]
+  uint32_t *d_KEb = nullptr;   // BF16 full rows for the multi-load-case matrix-core products [ne][nd][ndp]
   float *d_KEs = nullptr;      // per element scale s_e = max |KE_e|
   double *d_KF = nullptr;      // Helmholtz packed [npf/2][ne][2]
   double *d_T = nullptr;       // [npe][ne]
@@ -300,6 +300,8 @@ void restrictf(hipStream_t s, int bs, int64_t ncoarse_units, const int32_t *cptr
                const float *P, const float *r, float *bc);
 void prolong_addf(hipStream_t s, int bs, int64_t nunits, const int32_t *parent, const float *P,
                   const float *xc, float *x);
+// Ee[e] = E(x_e) (SIMP modulus)
+void simp_moduli(hipStream_t s, const uft_ctx *C, const double *xphys, double *Ee);
 // level 0 -> 1: G (ngl x 36) once per mesh; then val = sum_k E(x_e(k)) G_k per block
 void galerkin_pre(hipStream_t s, const uft_ctx *C, int64_t ngl, const int32_t *gblk, const int32_t *gl,
                   const int32_t *brow, const int32_t *ci, const int32_t *parent, const double *Q, double *G);
@@ -322,6 +324,7 @@ void mr_cheb_first(hipStream_t s, int nr, int64_t n, const float *b, const float
 void mr_elem_phase(hipStream_t s, const uft_ctx *C, int nr, const double *xphys, const double *u, double *Ye);
 void mr_elem_phase(hipStream_t s, const uft_ctx *C, int nr, const double *xphys, const float *u, float *Ye);
 void ke_to_bf16(hipStream_t s, const uft_ctx *C);
+int ke_bf16_rows_words(int npe);
 void mr_gather_op(hipStream_t s, const uft_ctx *C, int nr, const double *u, const double *Ye, const RowOp<double> &o);
 void mr_gather_op(hipStream_t s, const uft_ctx *C, int nr, const float *u, const float *Ye, const RowOp<float> &o);
 int mr_gather_blocks(int64_t nn, int nr);
