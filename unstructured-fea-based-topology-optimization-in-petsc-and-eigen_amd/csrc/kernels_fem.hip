@@ -77,7 +77,7 @@ __global__ __launch_bounds__(TPB) void k_final_sum(const double *__restrict__ pa
     __syncthreads();
   }
 }
-// K <= 4: shuffle-tree variant of the same fixed-order sum
+// K in {1, 2, 3, 4, 8, 16}: shuffle-tree variant of the same fixed-order sum
 template <int K>
 __global__ __launch_bounds__(TPB) void k_final_sum_k(const double *__restrict__ part, int nb, double *__restrict__ out) {
   double res[K];
@@ -1228,6 +1228,8 @@ void final_sum(hipStream_t s, const double *part, int nb, int K, double *out) {
     case 2: k_final_sum_k<2><<<1, TPB, 0, s>>>(part, nb, out); break;
     case 3: k_final_sum_k<3><<<1, TPB, 0, s>>>(part, nb, out); break;
     case 4: k_final_sum_k<4><<<1, TPB, 0, s>>>(part, nb, out); break;
+    case 8: k_final_sum_k<8><<<1, TPB, 0, s>>>(part, nb, out); break;
+    case 16: k_final_sum_k<16><<<1, TPB, 0, s>>>(part, nb, out); break;
     default: k_final_sum<<<1, TPB, 0, s>>>(part, nb, K, out); break;
   }
 }

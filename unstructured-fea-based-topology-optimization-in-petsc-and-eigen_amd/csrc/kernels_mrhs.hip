@@ -475,65 +475,40 @@ __global__ __launch_bounds__(TPB) void k_mr_final_kc(const double *__restrict__ 
   }
 }
 
-// last restriction + dense coarsest solve for NR vectors, one workgroup
-constexpr int MT_TPB = 1024;
-template <int BS, typename T, int NR>
-__global__ __launch_bounds__(MT_TPB) void k_mr_tail_solve(int nc_units, const int32_t *__restrict__ cptr,
-                                                          const int32_t *__restrict__ child, const T *__restrict__ P,
-                                                          const double *__restrict__ Ainv, const T *__restrict__ r,
-                                                          T *__restrict__ xc) {
-  __shared__ double bc[UFT_MR_TAIL_MAX * NR];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = MT_TPB / 64;
-  const int slot = lane / NR, rr = lane % NR;
-  constexpr int SL = 64 / NR;
-  const int n = 6 * nc_units;
-  for (int A = w; A < nc_units; A += nw) {
-    const int k0 = cptr[A], npair = (cptr[A + 1] - k0) * BS;
-    double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    for (int q = slot; q < npair; q += SL) {
-      const int kk = q / BS, i = q - kk * BS;
-      const int64_t uu = child[k0 + kk];
-      const T *Pu = P + (size_t)uu * BS * 6 + i * 6;
-      const double ri = (double)r[(uu * BS + i) * NR + rr];
-#pragma unroll
-      for (int j = 0; j < 6; ++j) acc[j] += (double)Pu[j] * ri;
-    }
-#pragma unroll
-    for (int j = 0; j < 6; ++j) {
-#pragma unroll
-      for (int off = 32; off >= NR; off >>= 1) acc[j] += __shfl_down(acc[j], off, 64);
-    }
-    if (slot == 0) {
-#pragma unroll
-      for (int j = 0; j < 6; ++j) bc[(A * 6 + j) * NR + rr] = acc[j];
-    }
-  }
-  __syncthreads();
-  // xc[row][r] = sum_c Ainv[c][row] bc[c][r] (symmetric inverse, rows coalesced)
-  for (int t = threadIdx.x; t < n * NR; t += MT_TPB) {
-    const int row = t / NR, q = t % NR;
-    double acc = 0.0;
-    for (int c = 0; c < n; ++c) acc += Ainv[(size_t)c * n + row] * bc[c * NR + q];
-    xc[t] = (T)acc;
-  }
+// xc[row][r] = sum_c Ainv[c][row] bc[c][r] (FP64, the symmetric inverse read
+// by columns), one thread per (row, r)
+template <typename T, int NR>
+__global__ __launch_bounds__(TPB) void k_mr_dense_solve(int n, const double *__restrict__ Ainv, const T *__restrict__ bc,
+                                                        T *__restrict__ xc) {
+  const int t = blockIdx.x * TPB + threadIdx.x;
+  if (t >= n * NR) return;
+  const int row = t / NR, q = t % NR;
+  double acc = 0.0;
+  for (int c = 0; c < n; ++c) acc += Ainv[(size_t)c * n + row] * (double)bc[c * NR + q];
+  xc[t] = (T)acc;
 }
 
 // ---------------------------------------------------------------- outer Krylov (FP64)
+// The outer-loop reductions run on a fixed grid (MR_RED_BLOCKS) with a
+// grid-stride loop, so the final one-block sum reads few partials; the
+// stride is a multiple of NR, so every thread keeps its right-hand side.
+constexpr int MR_RED_BLOCKS = 1024;
+inline unsigned mr_red_grid(int64_t n) { return (unsigned)std::min<int64_t>(MR_RED_BLOCKS, nblk(n)); }
 template <int NR>
 __global__ __launch_bounds__(TPB) void k_mr_dot(int64_t n, const double *__restrict__ a, const double *__restrict__ b,
                                                 double *__restrict__ part) {
-  const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
-  double v[1] = {t < n * NR ? a[t] * b[t] : 0.0};
+  double v[1] = {0.0};
+  for (int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x; t < n * NR; t += (int64_t)gridDim.x * TPB)
+    v[0] += a[t] * b[t];
   mr_block_partials<1, NR>(v, part);
 }
 template <int NR>
 __global__ __launch_bounds__(TPB) void k_mr_dot_pr(int64_t n, const double *__restrict__ r, const double *__restrict__ z,
                                                    const double *__restrict__ zo, double *__restrict__ part) {
-  const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
   double v[2] = {0.0, 0.0};
-  if (t < n * NR) {
-    v[0] = r[t] * z[t];
-    v[1] = r[t] * zo[t];
+  for (int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x; t < n * NR; t += (int64_t)gridDim.x * TPB) {
+    v[0] += r[t] * z[t];
+    v[1] += r[t] * zo[t];
   }
   mr_block_partials<2, NR>(v, part);
 }
@@ -543,14 +518,13 @@ __global__ __launch_bounds__(TPB) void k_mr_cg_update(int64_t n, const double *_
                                                       double *__restrict__ x, const double *__restrict__ p,
                                                       double *__restrict__ r, const double *__restrict__ q,
                                                       double *__restrict__ part) {
-  const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
   double v[1] = {0.0};
-  if (t < n * NR) {
-    const double al = alpha[t % NR];
+  const double al = alpha[((int64_t)blockIdx.x * TPB + threadIdx.x) % NR];
+  for (int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x; t < n * NR; t += (int64_t)gridDim.x * TPB) {
     x[t] += al * p[t];
     const double ri = r[t] - al * q[t];
     r[t] = ri;
-    v[0] = ri * ri;
+    v[0] += ri * ri;
   }
   mr_block_partials<1, NR>(v, part);
 }
@@ -815,36 +789,35 @@ void mr_final_kc2(hipStream_t s, int nr, const double *part, int nb, double *sc)
 
 template <typename T>
 static void mr_tail_solve_t(hipStream_t s, int nr, int bs, int nc_units, const int32_t *cptr, const int32_t *child,
-                            const T *P, const double *Ainv, const T *r, T *xc) {
-  if (bs == 3) {
-    UFT_NR_SWITCH(nr, (k_mr_tail_solve<3, T, NR><<<1, MT_TPB, 0, s>>>(nc_units, cptr, child, P, Ainv, r, xc)));
-  } else {
-    UFT_NR_SWITCH(nr, (k_mr_tail_solve<6, T, NR><<<1, MT_TPB, 0, s>>>(nc_units, cptr, child, P, Ainv, r, xc)));
-  }
+                            const T *P, const double *Ainv, T *bc, const T *r, T *xc) {
+  // restriction over the whole chip, then the dense solve (FP64 accumulation)
+  mr_restrict_first_t<T>(s, nr, bs, nc_units, cptr, child, P, r, bc, nullptr, 0.0, nullptr, nullptr, nullptr);
+  const int n = 6 * nc_units;
+  UFT_NR_SWITCH(nr, (k_mr_dense_solve<T, NR><<<nblk((int64_t)n * NR), TPB, 0, s>>>(n, Ainv, bc, xc)));
 }
 void mr_tail_solve(hipStream_t s, int nr, int bs, int nc_units, const int32_t *cptr, const int32_t *child,
-                   const double *P, const double *Ainv, const double *r, double *xc) {
-  mr_tail_solve_t<double>(s, nr, bs, nc_units, cptr, child, P, Ainv, r, xc);
+                   const double *P, const double *Ainv, double *bc, const double *r, double *xc) {
+  mr_tail_solve_t<double>(s, nr, bs, nc_units, cptr, child, P, Ainv, bc, r, xc);
 }
 void mr_tail_solve(hipStream_t s, int nr, int bs, int nc_units, const int32_t *cptr, const int32_t *child,
-                   const float *P, const double *Ainv, const float *r, float *xc) {
-  mr_tail_solve_t<float>(s, nr, bs, nc_units, cptr, child, P, Ainv, r, xc);
+                   const float *P, const double *Ainv, float *bc, const float *r, float *xc) {
+  mr_tail_solve_t<float>(s, nr, bs, nc_units, cptr, child, P, Ainv, bc, r, xc);
 }
 
 // ---- outer loop
 void mr_dot(hipStream_t s, int nr, int64_t n, const double *a, const double *b, double *part, double *out) {
-  UFT_NR_SWITCH(nr, (k_mr_dot<NR><<<nblk(n * NR), TPB, 0, s>>>(n, a, b, part)));
-  final_sum(s, part, (int)nblk(n * nr), nr, out);
+  UFT_NR_SWITCH(nr, (k_mr_dot<NR><<<mr_red_grid(n * NR), TPB, 0, s>>>(n, a, b, part)));
+  final_sum(s, part, (int)mr_red_grid(n * nr), nr, out);
 }
 void mr_dot_pr(hipStream_t s, int nr, int64_t n, const double *r, const double *z, const double *zo, double *part,
                double *out) {
-  UFT_NR_SWITCH(nr, (k_mr_dot_pr<NR><<<nblk(n * NR), TPB, 0, s>>>(n, r, z, zo, part)));
-  final_sum(s, part, (int)nblk(n * nr), 2 * nr, out);
+  UFT_NR_SWITCH(nr, (k_mr_dot_pr<NR><<<mr_red_grid(n * NR), TPB, 0, s>>>(n, r, z, zo, part)));
+  final_sum(s, part, (int)mr_red_grid(n * nr), 2 * nr, out);
 }
 void mr_cg_update(hipStream_t s, int nr, int64_t n, const double *alpha, double *x, const double *p, double *r,
                   const double *q, double *part, double *rr_out) {
-  UFT_NR_SWITCH(nr, (k_mr_cg_update<NR><<<nblk(n * NR), TPB, 0, s>>>(n, alpha, x, p, r, q, part)));
-  final_sum(s, part, (int)nblk(n * nr), nr, rr_out);
+  UFT_NR_SWITCH(nr, (k_mr_cg_update<NR><<<mr_red_grid(n * NR), TPB, 0, s>>>(n, alpha, x, p, r, q, part)));
+  final_sum(s, part, (int)mr_red_grid(n * nr), nr, rr_out);
 }
 void mr_xpby(hipStream_t s, int nr, int64_t n, const double *z, const double *beta, double *p) {
   UFT_NR_SWITCH(nr, (k_mr_xpby<NR><<<nblk(n * NR), TPB, 0, s>>>(n, z, beta, p)));

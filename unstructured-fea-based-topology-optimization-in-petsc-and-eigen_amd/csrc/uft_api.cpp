@@ -510,7 +510,7 @@ void mr_cycle_body(uft_ctx *C, int l, const T *b, T *x) {
     AmgLevel &Ln = H.L[l + 1];
     MLV<T> vn = mlv<T>(C, l + 1);
     const T *P = tr_P(Lv, (T *)nullptr);
-    uftk::mr_tail_solve(C->s, nr, Lv.bs, (int)Ln.nunit, Lv.d_cptr, Lv.d_child, P, H.d_cinv, v.r, vn.x);
+    uftk::mr_tail_solve(C->s, nr, Lv.bs, (int)Ln.nunit, Lv.d_cptr, Lv.d_child, P, H.d_cinv, vn.b, v.r, vn.x);
     uftk::mr_prolong_comb(C->s, nr, Lv.bs, Lv.nunit, Lv.d_parent, P, nullptr, -1, -1, vn.x, (const T *)nullptr, x);
   } else {
     AmgLevel &Ln = H.L[l + 1];

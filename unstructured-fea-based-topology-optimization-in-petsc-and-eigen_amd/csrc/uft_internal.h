@@ -13,7 +13,7 @@
 #define UFT_MAXLEV 8
 #define UFT_RED_BLOCK 256
 #define UFT_TAIL_MAX 3072  // coarsest unknowns handled by the fused coarse tail (LDS)
-#define UFT_MR_TAIL_MAX 512  // the same for the multi-load-case tail (x 8 right-hand sides)
+#define UFT_MR_TAIL_MAX 3072  // coarsest unknowns of the multi-load-case dense solve
 
 // Per-row epilogue of a fused operator apply t = A u (see kernels_cycle.hip):
 //  STORE     y = t
@@ -349,10 +349,11 @@ void mr_lincomb_first(hipStream_t s, int nr, int64_t n, const double *sc, const 
                       float *r, const float *dinv, double it, float *d, float *kd);
 void mr_final_kc1(hipStream_t s, int nr, const double *part, int nb, double *sc);
 void mr_final_kc2(hipStream_t s, int nr, const double *part, int nb, double *sc);
+// xc = Ainv P^T r on the coarsest level (bc: restriction scratch of the coarsest size x nr)
 void mr_tail_solve(hipStream_t s, int nr, int bs, int nc_units, const int32_t *cptr, const int32_t *child,
-                   const double *P, const double *Ainv, const double *r, double *xc);
+                   const double *P, const double *Ainv, double *bc, const double *r, double *xc);
 void mr_tail_solve(hipStream_t s, int nr, int bs, int nc_units, const int32_t *cptr, const int32_t *child,
-                   const float *P, const double *Ainv, const float *r, float *xc);
+                   const float *P, const double *Ainv, float *bc, const float *r, float *xc);
 void mr_dot(hipStream_t s, int nr, int64_t n, const double *a, const double *b, double *part, double *out);
 void mr_dot_pr(hipStream_t s, int nr, int64_t n, const double *r, const double *z, const double *zo, double *part,
                double *out);
