@@ -9,7 +9,9 @@
 // The fine element products of the mixed form run on the matrix cores:
 // Y_e (nd x NR) = KE_e (nd x nd) U_e (nd x NR) as v_mfma_f32_16x16x32_bf16
 // tiles (BF16 operands, FP32 accumulation), one element per wavefront step.
-// Element-local results are stored element-major, [ne][nd][NR].
+// Element-local results are stored node-row major per element,
+// [ne][npe][3][NR], so the node phase reads each incidence's 3 x NR values
+// as one contiguous segment.
 #include "uft_elem.h"
 #include "uft_internal.h"
 #include "uft_reduce.h"
@@ -168,7 +170,7 @@ __global__ __launch_bounds__(TPB) void k_mr_elem(int ne, const int32_t *__restri
   });
   const T E = (T)simpE(sc, xphys[e]);
 #pragma unroll
-  for (int k = 0; k < ND; ++k) Ye[((size_t)e * ND + k) * NR + rr] = E * ye[k];
+  for (int k = 0; k < ND; ++k) Ye[(((size_t)e * NPE + k % NPE) * 3 + k / NPE) * NR + rr] = E * ye[k];
 }
 
 // ---------------------------------------------------------------- element phase, matrix cores
@@ -271,7 +273,7 @@ __global__ __launch_bounds__(64 * MF_WAVES) void k_mr_elem_mfma(int ne, const in
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
           const int row = 16 * tl + 4 * kq + i;
-          if (row < ND) Yel[row * NR + col] = E * acc[i];
+          if (row < ND) Yel[((row % NPE) * 3 + row / NPE) * NR + col] = E * acc[i];
         }
       }
     }
@@ -317,7 +319,7 @@ __global__ __launch_bounds__(TPB) void k_mr_gather_op(int nn, int ne, const int3
       const int v = inc[k];
       const int e = v / NPE, l = v - e * NPE;
 #pragma unroll
-      for (int c = 0; c < 3; ++c) acc[c] += Ye[((size_t)e * (3 * NPE) + c * NPE + l) * NR + rr];
+      for (int c = 0; c < 3; ++c) acc[c] += Ye[(((size_t)e * NPE + l) * 3 + c) * NR + rr];
     }
     const unsigned m = fixm[n];
 #pragma unroll
