@@ -262,6 +262,35 @@ int uft_gen_box(int npe, int nx, int ny, int nz, double lx, double ly, double lz
 int uft_box_bcs(int nx, int ny, int nz, int load_kind, int32_t *fixed, int *nfixed, int32_t *load_dofs,
                 double *load_vals, int *nload);
 
+/* --- independent designs in one context ---------------------------------
+ * No counterpart in the reference, which optimises one design per run; for
+ * callers that optimise many designs (parameter sweeps, ensembles).  The mesh
+ * given to uft_mesh_set is the concatenation of ndesign independent meshes:
+ * design d owns elements [elem_off[d], elem_off[d+1]) and nodes
+ * [node_off[d], node_off[d+1]), and its elements use only its own nodes;
+ * uft_bc_set takes the fixed and load DOFs in the concatenated numbering.
+ * volfrac[d] and rmin[d] replace those parameters for design d; all other
+ * parameters are shared.  Call after uft_mesh_set and before uft_setup.
+ * Every kernel launch then covers all designs, while each design keeps its
+ * own Krylov scalars and stopping tests, filter solve, volume constraint,
+ * sensitivities and MMA dual problem, so each design follows the iterates of
+ * a context that holds it alone (to rounding, and to the solver tolerance:
+ * its multigrid levels are formed per design).  uft_design_iteration then
+ * advances every design by one iteration and returns fx = the sum of the
+ * compliances, gx[0] = the largest volume-constraint value and ch = the
+ * largest design change; uft_batch_results returns them per design.
+ * Restrictions: m = 1, superposed load cases, at most 64 designs, each
+ * design's coarsest multigrid level at most 3072 unknowns; the K-cycle
+ * (cycle_type 2, 3) runs as the W-cycle. */
+int uft_batch_set(uft_ctx *ctx, int ndesign, const int64_t *elem_off, const int64_t *node_off,
+                  const double *volfrac, const double *rmin);
+/* Per design results of the last iteration (any pointer may be NULL; arrays
+ * of ndesign): compliance, volume constraint, design change, elasticity
+ * Krylov iterations and final relative residual, filter CG iterations.
+ * Returns ndesign (0 without a batch). */
+int uft_batch_results(const uft_ctx *ctx, double *fx, double *gx, double *ch, int *it_el,
+                      double *relres_el, int *it_f);
+
 #ifdef __cplusplus
 }
 #endif

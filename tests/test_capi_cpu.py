@@ -94,3 +94,34 @@ int use(SolveFem &fem, Vec X, Vec dfdx, Vec *dgdx, Vec xmin, Vec xmax, MeshRef e
 ''')
     inc = [os.path.join(conftest.ROOT, "include"), os.path.join(conftest.PKG, "dropin")]
     subprocess.run(["g++", "-std=c++17", "-fsyntax-only"] + [f"-I{d}" for d in inc] + [str(src)], check=True)
+
+
+def test_uneven_design_draw(U):
+    """The 32 uneven cantilevers (seed 32): every size inside [8000, 64000]
+    elements, nx ~ 2 ny ~ 2 nz, volfrac in [0.2, 0.4); the draw is fixed."""
+    from uft.capi import uneven_designs, SplitMix64
+    ds = uneven_designs()
+    assert len(ds) == 32
+    ne = [D["nx"] * D["ny"] * D["nz"] for D in ds]
+    assert all(8000 <= v <= 64000 for v in ne)
+    assert all(abs(D["nx"] - 2 * D["ny"]) <= 2 and abs(D["nz"] - D["ny"]) <= 1 for D in ds)
+    assert all(0.2 <= D["volfrac"] < 0.4 for D in ds)
+    assert sum(ne) == 868008 and (ds[0]["nx"], ds[0]["ny"], ds[0]["nz"]) == (46, 23, 24)
+    # SplitMix64 known answer (seed 0: first output of the published generator)
+    assert SplitMix64(0).next() == 0xE220A8397B1DCDAF
+
+
+def test_batch_mesh_layout(U):
+    """Concatenated batch meshes: each design's elements use only its own
+    nodes, its clamped DOFs are its x = 0 nodes, its loads its own tip edge."""
+    from uft.capi import batch_mesh
+    ds = [dict(nx=6, ny=3, nz=3, volfrac=0.3, seed=1), dict(nx=4, ny=2, nz=2, volfrac=0.3, seed=2)]
+    elem, xyz, fixed, ld, lv, eoff, noff = batch_mesh(ds)
+    assert list(eoff) == [0, 54, 70] and list(noff) == [0, 112, 157]
+    for d in range(2):
+        e = elem[eoff[d]:eoff[d + 1]]
+        assert e.min() >= noff[d] and e.max() < noff[d + 1]
+    fn = np.unique(fixed // 3)
+    assert np.all(np.abs(xyz[fn, 0]) < 1e-12)
+    assert len(fn) == 16 + 9
+    assert len(ld) == 4 + 3 and np.all(np.abs(xyz[ld // 3, 0] - 2.0) < 1e-12)

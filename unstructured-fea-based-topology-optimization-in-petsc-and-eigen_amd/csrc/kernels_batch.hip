@@ -1,0 +1,399 @@
+// gfx950 kernels of design batches: many independent designs held in one
+// context, every launch covering all of them.  Rows of every vector space are
+// grouped by design, and a segment grid (SegG) cuts each design's rows into
+// blocks of at most 256 that never span two designs, so a block's design is
+// uniform: block partial sums are per design, and a per-design final kernel
+// sums its own blocks in a fixed order (deterministic, like the single-design
+// reductions).  Each design keeps its own Krylov scalars at sc + d * UFT_BSC;
+// one thread per design runs the scalar recurrences, including the stopping
+// tests, so a design that has converged keeps zero step lengths while the
+// others continue.
+#include "uft_internal.h"
+#include "uft_reduce.h"
+
+namespace {
+
+constexpr int TPB = 256;
+using uftr::seg_row;
+
+// ---------------------------------------------------------------- per-design final reductions
+template <int K>
+__global__ __launch_bounds__(TPB) void k_seg_final(const double *__restrict__ part, const int32_t *__restrict__ dfirst,
+                                                   double *__restrict__ sc, int base) {
+  const int d = blockIdx.x;
+  const int b0 = dfirst[d], nb = dfirst[d + 1] - b0;
+  double res[K];
+  uftr::block_final<K>(part + (size_t)b0 * K, nb, res);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) sc[(size_t)d * UFT_BSC + base + k] = res[k];
+  }
+}
+
+__global__ __launch_bounds__(TPB) void k_seg_final_max(const double *__restrict__ part,
+                                                       const int32_t *__restrict__ dfirst, double *__restrict__ sc,
+                                                       int base) {
+  const int d = blockIdx.x;
+  double acc = 0.0;
+  for (int i = dfirst[d] + threadIdx.x; i < dfirst[d + 1]; i += TPB) acc = fmax(acc, part[i]);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) acc = fmax(acc, __shfl_down(acc, off, 64));
+  __shared__ double sm[TPB / 64];
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) sc[(size_t)d * UFT_BSC + base] = fmax(fmax(sm[0], sm[1]), fmax(sm[2], sm[3]));
+}
+
+// ---------------------------------------------------------------- vector kernels
+__global__ __launch_bounds__(TPB) void k_seg_dot(SegG g, int64_t n, const double *__restrict__ a,
+                                                 const double *__restrict__ b, double *__restrict__ part) {
+  int64_t i;
+  int d;
+  double v[1] = {0.0};
+  if (seg_row(g, n, i, d)) v[0] = a[i] * b[i];
+  uftr::block_partials<1>(v, part);
+}
+
+__global__ __launch_bounds__(TPB) void k_seg_dot_pr(SegG g, int64_t n, const double *__restrict__ r,
+                                                    const double *__restrict__ z, const double *__restrict__ zo,
+                                                    double *__restrict__ part) {
+  int64_t i;
+  int d;
+  double v[2] = {0.0, 0.0};
+  if (seg_row(g, n, i, d)) {
+    v[0] = r[i] * z[i];
+    v[1] = r[i] * zo[i];
+  }
+  uftr::block_partials<2>(v, part);
+}
+
+// x += a_d p ; r -= a_d q ; partials of r.r
+__global__ __launch_bounds__(TPB) void k_seg_cg_update(SegG g, int64_t n, const double *__restrict__ sc, int ia,
+                                                       double *__restrict__ x, const double *__restrict__ p,
+                                                       double *__restrict__ r, const double *__restrict__ q,
+                                                       double *__restrict__ part) {
+  int64_t i;
+  int d;
+  double v[1] = {0.0};
+  if (seg_row(g, n, i, d)) {
+    const double al = sc[(size_t)d * UFT_BSC + ia];
+    x[i] += al * p[i];
+    const double ri = r[i] - al * q[i];
+    r[i] = ri;
+    v[0] = ri * ri;
+  }
+  uftr::block_partials<1>(v, part);
+}
+
+// p = z + b_d p
+__global__ __launch_bounds__(TPB) void k_seg_xpby(SegG g, int64_t n, const double *__restrict__ z,
+                                                  const double *__restrict__ sc, int ib, double *__restrict__ p) {
+  int64_t i;
+  int d;
+  if (!seg_row(g, n, i, d)) return;
+  p[i] = z[i] + sc[(size_t)d * UFT_BSC + ib] * p[i];
+}
+
+__global__ __launch_bounds__(TPB) void k_seg_fill(SegG g, int64_t n, const double *__restrict__ vals,
+                                                  double *__restrict__ x) {
+  int64_t i;
+  int d;
+  if (seg_row(g, n, i, d)) x[i] = vals[d];
+}
+
+// Chebyshev scaling folded into the inverse diagonal: D^-1 / lambda_max of
+// the row's design (and its FP32 copy)
+__global__ __launch_bounds__(TPB) void k_seg_scale_dinv(SegG g, int64_t n, const double *__restrict__ fac,
+                                                        double *__restrict__ dinv, float *__restrict__ dinvf) {
+  int64_t i;
+  int d;
+  if (!seg_row(g, n, i, d)) return;
+  const double v = dinv[i] * fac[d];
+  dinv[i] = v;
+  if (dinvf) dinvf[i] = (float)v;
+}
+
+// Helmholtz filter CG (PETSc KSPCG semantics, preconditioned residual norm):
+// r = b - q ; z = D^-1 r ; partials ||D^-1 b||^2, ||z||^2, z.r
+__global__ __launch_bounds__(TPB) void k_seg_fcg_init(SegG g, int64_t n, const double *__restrict__ b,
+                                                      const double *__restrict__ q, const double *__restrict__ dinv,
+                                                      double *__restrict__ r, double *__restrict__ z,
+                                                      double *__restrict__ part) {
+  int64_t i;
+  int d;
+  double v[3] = {0.0, 0.0, 0.0};
+  if (seg_row(g, n, i, d)) {
+    const double di = dinv[i], bi = b[i];
+    const double ri = bi - q[i], zi = ri * di, zb = bi * di;
+    r[i] = ri;
+    z[i] = zi;
+    v[0] = zb * zb;
+    v[1] = zi * zi;
+    v[2] = zi * ri;
+  }
+  uftr::block_partials<3>(v, part);
+}
+
+// x += a_d p ; r -= a_d q ; z = D^-1 r ; partials ||z||^2, z.r
+__global__ __launch_bounds__(TPB) void k_seg_fcg_update(SegG g, int64_t n, const double *__restrict__ sc, int ia,
+                                                        double *__restrict__ x, const double *__restrict__ p,
+                                                        double *__restrict__ r, const double *__restrict__ q,
+                                                        const double *__restrict__ dinv, double *__restrict__ z,
+                                                        double *__restrict__ part) {
+  int64_t i;
+  int d;
+  double v[2] = {0.0, 0.0};
+  if (seg_row(g, n, i, d)) {
+    const double al = sc[(size_t)d * UFT_BSC + ia];
+    x[i] += al * p[i];
+    const double ri = r[i] - al * q[i];
+    r[i] = ri;
+    const double zi = ri * dinv[i];
+    z[i] = zi;
+    v[0] = zi * zi;
+    v[1] = zi * ri;
+  }
+  uftr::block_partials<2>(v, part);
+}
+
+// p = z (first step) or p = z + c_d p
+__global__ __launch_bounds__(TPB) void k_seg_fcg_p(SegG g, int64_t n, const double *__restrict__ z,
+                                                   const double *__restrict__ sc, int ic, int first,
+                                                   double *__restrict__ p) {
+  int64_t i;
+  int d;
+  if (!seg_row(g, n, i, d)) return;
+  p[i] = first ? z[i] : z[i] + sc[(size_t)d * UFT_BSC + ic] * p[i];
+}
+
+// ---------------------------------------------------------------- per-design scalar recurrences
+#define FOR_DESIGN(nseg)                 \
+  const int d = threadIdx.x;             \
+  if (d >= nseg) return;                 \
+  double *S = sc + (size_t)d * UFT_BSC;
+
+// after r = F and ||r||^2 in B_RR: ||F||, tolerance, active flag
+__global__ void k_bsc_el_init(int nseg, double *sc, double rtol) {
+  FOR_DESIGN(nseg)
+  const double fn = sqrt(S[B_RR]);
+  S[B_FN] = fn;
+  S[B_TOL] = rtol * fn;
+  S[B_ACT] = fn > 0.0 ? 1.0 : 0.0;
+  S[B_IT] = 0.0;
+}
+__global__ void k_bsc_el_alpha(int nseg, double *sc) {
+  FOR_DESIGN(nseg)
+  S[B_ALPHA] = S[B_ACT] != 0.0 ? S[B_RZ] / S[B_PQ] : 0.0;
+}
+// after the update: count the step and apply the stopping test
+// ||F - K U|| <= rtol ||F|| (a non-finite residual also stops the design; the
+// host reports it from the true residual)
+__global__ void k_bsc_el_check(int nseg, double *sc) {
+  FOR_DESIGN(nseg)
+  if (S[B_ACT] == 0.0) return;
+  S[B_IT] += 1.0;
+  if (!(sqrt(S[B_RR]) > S[B_TOL])) S[B_ACT] = 0.0;
+}
+// flexible CG (Polak-Ribiere): beta = (r.z - r.zo) / rz_prev
+__global__ void k_bsc_el_beta(int nseg, double *sc) {
+  FOR_DESIGN(nseg)
+  if (S[B_ACT] != 0.0) {
+    S[B_BETA] = (S[B_RZN] - S[B_RZO]) / S[B_RZ];
+    S[B_RZ] = S[B_RZN];
+  } else {
+    S[B_BETA] = 0.0;
+  }
+}
+
+// filter CG start (PDEFilter.cpp:117-137): ttol = max(rtol ||D^-1 b||, 1e-50)
+// (||z|| when b = 0); a design already below it takes no step
+__global__ void k_bsc_f_init(int nseg, double *sc, double rtol) {
+  FOR_DESIGN(nseg)
+  double snorm = sqrt(S[B_FB]);
+  const double dp = sqrt(S[B_FZZ]);
+  if (snorm == 0.0) snorm = dp;
+  S[B_FSN] = snorm;
+  S[B_FTOL] = fmax(rtol * snorm, 1e-50);
+  S[B_FACT] = dp > S[B_FTOL] ? 1.0 : 0.0;
+  S[B_FIT] = 0.0;
+  S[B_FDIV] = 0.0;
+  S[B_FPC] = 0.0;
+}
+__global__ void k_bsc_f_alpha(int nseg, double *sc) {
+  FOR_DESIGN(nseg)
+  S[B_FAL] = S[B_FACT] != 0.0 ? S[B_FBETA] / S[B_FPQ] : 0.0;
+}
+// after the update: beta_old = beta, beta = z.r, stopping and divergence
+// tests (dtol 1e5), and the next direction's coefficient beta / beta_old
+__global__ void k_bsc_f_after(int nseg, double *sc) {
+  FOR_DESIGN(nseg)
+  if (S[B_FACT] == 0.0) {
+    S[B_FPC] = 0.0;
+    return;
+  }
+  S[B_FIT] += 1.0;
+  S[B_FBO] = S[B_FBETA];
+  S[B_FBETA] = S[B_FZRN];
+  const double dp = sqrt(S[B_FZZN]);
+  if (dp <= S[B_FTOL]) {
+    S[B_FACT] = 0.0;
+  } else if (!(dp < 1e5 * S[B_FSN])) {
+    S[B_FACT] = 0.0;
+    S[B_FDIV] = 1.0;
+  }
+  S[B_FPC] = S[B_FACT] != 0.0 ? S[B_FBETA] / S[B_FBO] : 0.0;
+}
+
+// Lanczos from Jacobi-CG coefficients: alpha = rz / pq, logged per step
+__global__ void k_bsc_l_alpha(int nseg, double *sc, double *log, int step) {
+  FOR_DESIGN(nseg)
+  const double a = S[B_LRZ] / S[B_LPQ];
+  S[B_LAL] = a;
+  log[((size_t)d * 64 + step) * 2] = a;
+}
+__global__ void k_bsc_l_beta(int nseg, double *sc, double *log, int step) {
+  FOR_DESIGN(nseg)
+  const double b = S[B_LRZN] / S[B_LRZ];
+  S[B_LBE] = b;
+  S[B_LRZ] = S[B_LRZN];
+  log[((size_t)d * 64 + step) * 2 + 1] = b;
+}
+#undef FOR_DESIGN
+
+// ---------------------------------------------------------------- coarsest level of a batch
+// Each design's block of the block-diagonal coarsest operator, dense and
+// row-major at dense + coff[d] (n_d = 6 (cu[d+1] - cu[d]) rows).
+__global__ __launch_bounds__(TPB) void k_seg_bsr_to_dense(int64_t nrows, const int32_t *__restrict__ rp,
+                                                          const int32_t *__restrict__ ci,
+                                                          const double *__restrict__ val,
+                                                          const int32_t *__restrict__ udes,
+                                                          const int32_t *__restrict__ cu,
+                                                          const int64_t *__restrict__ coff, double *__restrict__ D) {
+  const int64_t row = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  if (row >= nrows * 6) return;
+  const int64_t A = row / 6;
+  const int i = (int)(row - A * 6);
+  const int d = udes[A];
+  const int64_t u0 = cu[d], n = 6 * (int64_t)(cu[d + 1] - cu[d]);
+  double *Dr = D + coff[d] + ((A - u0) * 6 + i) * n;
+  for (int64_t c = 0; c < n; ++c) Dr[c] = 0.0;
+  for (int k = rp[A]; k < rp[A + 1]; ++k) {
+    const int64_t B = ci[k] - u0;
+    if (B < 0 || 6 * B >= n) continue;  // designs do not couple; never write outside the block
+#pragma unroll
+    for (int j = 0; j < 6; ++j) Dr[B * 6 + j] = val[(size_t)k * 36 + i * 6 + j];
+  }
+}
+
+// In-place Gauss-Jordan inversion of each design's SPD block (no pivoting: an
+// SPD matrix's pivots are positive), one workgroup per design; step k with
+// p = a_kk:  a_kk <- 1/p, a_kj <- a_kj/p, a_ik <- -a_ik/p,
+// a_ij <- a_ij - a_ik a_kj / p.  The result is symmetrised into Ainv.
+constexpr int GJB_TPB = 1024;
+__global__ __launch_bounds__(GJB_TPB) void k_seg_gj(const int32_t *__restrict__ cu, const int64_t *__restrict__ coff,
+                                                    double *__restrict__ A, double *__restrict__ Ainv) {
+  __shared__ double rowk[UFT_TAIL_MAX], colk[UFT_TAIL_MAX];
+  const int d = blockIdx.x;
+  const int n = 6 * (cu[d + 1] - cu[d]);
+  double *M = A + coff[d];
+  double *Mi = Ainv + coff[d];
+  const int64_t nn = (int64_t)n * n;
+  for (int k = 0; k < n; ++k) {
+    for (int j = threadIdx.x; j < n; j += GJB_TPB) {
+      rowk[j] = M[(int64_t)k * n + j];
+      colk[j] = M[(int64_t)j * n + k];
+    }
+    __syncthreads();
+    const double piv = 1.0 / rowk[k];
+    for (int64_t t = threadIdx.x; t < nn; t += GJB_TPB) {
+      const int i = (int)(t / n), j = (int)(t - (int64_t)i * n);
+      double v;
+      if (i == k)
+        v = j == k ? piv : rowk[j] * piv;
+      else
+        v = j == k ? -colk[i] * piv : M[t] - colk[i] * rowk[j] * piv;
+      M[t] = v;
+    }
+    __syncthreads();
+  }
+  for (int64_t t = threadIdx.x; t < nn; t += GJB_TPB) {
+    const int i = (int)(t / n), j = (int)(t - (int64_t)i * n);
+    Mi[t] = 0.5 * (M[t] + M[(int64_t)j * n + i]);
+  }
+}
+
+inline unsigned gsz(SegG g) { return (unsigned)g.nb; }
+
+}  // namespace
+
+namespace uftk {
+void seg_final(hipStream_t s, const double *part, const int32_t *dfirst, int nseg, int K, double *sc, int base) {
+  switch (K) {
+    case 1: k_seg_final<1><<<nseg, TPB, 0, s>>>(part, dfirst, sc, base); break;
+    case 2: k_seg_final<2><<<nseg, TPB, 0, s>>>(part, dfirst, sc, base); break;
+    default: k_seg_final<3><<<nseg, TPB, 0, s>>>(part, dfirst, sc, base); break;
+  }
+}
+void seg_final_max(hipStream_t s, const double *part, const int32_t *dfirst, int nseg, double *sc, int base) {
+  k_seg_final_max<<<nseg, TPB, 0, s>>>(part, dfirst, sc, base);
+}
+void seg_dot(hipStream_t s, SegG g, const int32_t *dfirst, int nseg, int64_t n, const double *a, const double *b,
+             double *part, double *sc, int base) {
+  k_seg_dot<<<gsz(g), TPB, 0, s>>>(g, n, a, b, part);
+  seg_final(s, part, dfirst, nseg, 1, sc, base);
+}
+void seg_dot_pr(hipStream_t s, SegG g, const int32_t *dfirst, int nseg, int64_t n, const double *r, const double *z,
+                const double *zo, double *part, double *sc, int base) {
+  k_seg_dot_pr<<<gsz(g), TPB, 0, s>>>(g, n, r, z, zo, part);
+  seg_final(s, part, dfirst, nseg, 2, sc, base);
+}
+void seg_cg_update(hipStream_t s, SegG g, const int32_t *dfirst, int nseg, int64_t n, double *sc, int ia, double *x,
+                   const double *p, double *r, const double *q, double *part, int base) {
+  k_seg_cg_update<<<gsz(g), TPB, 0, s>>>(g, n, sc, ia, x, p, r, q, part);
+  seg_final(s, part, dfirst, nseg, 1, sc, base);
+}
+void seg_xpby(hipStream_t s, SegG g, int64_t n, const double *z, const double *sc, int ib, double *p) {
+  k_seg_xpby<<<gsz(g), TPB, 0, s>>>(g, n, z, sc, ib, p);
+}
+void seg_fill(hipStream_t s, SegG g, int64_t n, const double *vals, double *x) {
+  k_seg_fill<<<gsz(g), TPB, 0, s>>>(g, n, vals, x);
+}
+void seg_scale_dinv(hipStream_t s, SegG g, int64_t n, const double *fac, double *dinv, float *dinvf) {
+  k_seg_scale_dinv<<<gsz(g), TPB, 0, s>>>(g, n, fac, dinv, dinvf);
+}
+void seg_fcg_init(hipStream_t s, SegG g, const int32_t *dfirst, int nseg, int64_t n, const double *b, const double *q,
+                  const double *dinv, double *r, double *z, double *part, double *sc, int base) {
+  k_seg_fcg_init<<<gsz(g), TPB, 0, s>>>(g, n, b, q, dinv, r, z, part);
+  seg_final(s, part, dfirst, nseg, 3, sc, base);
+}
+void seg_fcg_update(hipStream_t s, SegG g, const int32_t *dfirst, int nseg, int64_t n, double *sc, int ia, double *x,
+                    const double *p, double *r, const double *q, const double *dinv, double *z, double *part,
+                    int base) {
+  k_seg_fcg_update<<<gsz(g), TPB, 0, s>>>(g, n, sc, ia, x, p, r, q, dinv, z, part);
+  seg_final(s, part, dfirst, nseg, 2, sc, base);
+}
+void seg_fcg_p(hipStream_t s, SegG g, int64_t n, const double *z, const double *sc, int ic, int first, double *p) {
+  k_seg_fcg_p<<<gsz(g), TPB, 0, s>>>(g, n, z, sc, ic, first, p);
+}
+void bsc_el_init(hipStream_t s, int nseg, double *sc, double rtol) { k_bsc_el_init<<<1, 64, 0, s>>>(nseg, sc, rtol); }
+void bsc_el_alpha(hipStream_t s, int nseg, double *sc) { k_bsc_el_alpha<<<1, 64, 0, s>>>(nseg, sc); }
+void bsc_el_check(hipStream_t s, int nseg, double *sc) { k_bsc_el_check<<<1, 64, 0, s>>>(nseg, sc); }
+void bsc_el_beta(hipStream_t s, int nseg, double *sc) { k_bsc_el_beta<<<1, 64, 0, s>>>(nseg, sc); }
+void bsc_f_init(hipStream_t s, int nseg, double *sc, double rtol) { k_bsc_f_init<<<1, 64, 0, s>>>(nseg, sc, rtol); }
+void bsc_f_alpha(hipStream_t s, int nseg, double *sc) { k_bsc_f_alpha<<<1, 64, 0, s>>>(nseg, sc); }
+void bsc_f_after(hipStream_t s, int nseg, double *sc) { k_bsc_f_after<<<1, 64, 0, s>>>(nseg, sc); }
+void bsc_lanczos_alpha(hipStream_t s, int nseg, double *sc, double *log, int step) {
+  k_bsc_l_alpha<<<1, 64, 0, s>>>(nseg, sc, log, step);
+}
+void bsc_lanczos_beta(hipStream_t s, int nseg, double *sc, double *log, int step) {
+  k_bsc_l_beta<<<1, 64, 0, s>>>(nseg, sc, log, step);
+}
+void seg_bsr_to_dense(hipStream_t s, int64_t nb_rows, const int32_t *rp, const int32_t *ci, const double *val,
+                      const int32_t *udes, const int32_t *cu, const int64_t *coff, double *dense) {
+  k_seg_bsr_to_dense<<<(unsigned)((nb_rows * 6 + TPB - 1) / TPB), TPB, 0, s>>>(nb_rows, rp, ci, val, udes, cu, coff,
+                                                                              dense);
+}
+void seg_dense_inverse(hipStream_t s, int nseg, const int32_t *cu, const int64_t *coff, double *A, double *Ainv) {
+  k_seg_gj<<<nseg, GJB_TPB, 0, s>>>(cu, coff, A, Ainv);
+}
+}  // namespace uftk

@@ -266,13 +266,23 @@ template <typename T>
 __global__ __launch_bounds__(TAIL_TPB) void k_tail_solve(int nc_units, const int32_t *__restrict__ cptr,
                                                          const int32_t *__restrict__ child, const T *__restrict__ P,
                                                          const double *__restrict__ Ainv, const T *__restrict__ r,
-                                                         T *__restrict__ xc) {
+                                                         T *__restrict__ xc, const int32_t *__restrict__ cu,
+                                                         const int64_t *__restrict__ coff) {
   __shared__ double bc[UFT_TAIL_MAX];
   __shared__ double part4[4][TAIL_TPB / 4];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = TAIL_TPB / 64;
+  // a design batch solves its block-diagonal coarsest level one design per
+  // workgroup: units [u0, u0 + nu) with the inverse block at Ainv + coff
+  int u0 = 0;
+  if (cu) {
+    u0 = cu[blockIdx.x];
+    nc_units = cu[blockIdx.x + 1] - u0;
+    Ainv += coff[blockIdx.x];
+    xc += (size_t)u0 * 6;
+  }
   const int n = 6 * nc_units;
   for (int A = w; A < nc_units; A += nw) {
-    const int k0 = cptr[A], npair = (cptr[A + 1] - k0) * 6;
+    const int k0 = cptr[u0 + A], npair = (cptr[u0 + A + 1] - k0) * 6;
     double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     for (int q = lane; q < npair; q += 64) {
       const int kk = q / 6, i = q - kk * 6;
@@ -391,12 +401,12 @@ void final_kc1(hipStream_t s, const double *part, int nb, double *sc) { k_final_
 void final_kc2(hipStream_t s, const double *part, int nb, double *sc) { k_final_kc<3><<<1, TPB, 0, s>>>(part, nb, sc); }
 
 void tail_solve(hipStream_t s, int nc_units, const int32_t *cptr, const int32_t *child, const double *P,
-                const double *Ainv, const double *r, double *xc) {
-  k_tail_solve<double><<<1, TAIL_TPB, 0, s>>>(nc_units, cptr, child, P, Ainv, r, xc);
+                const double *Ainv, const double *r, double *xc, int nseg, const int32_t *cu, const int64_t *coff) {
+  k_tail_solve<double><<<nseg, TAIL_TPB, 0, s>>>(nc_units, cptr, child, P, Ainv, r, xc, cu, coff);
 }
 void tail_solve(hipStream_t s, int nc_units, const int32_t *cptr, const int32_t *child, const float *P,
-                const double *Ainv, const float *r, float *xc) {
-  k_tail_solve<float><<<1, TAIL_TPB, 0, s>>>(nc_units, cptr, child, P, Ainv, r, xc);
+                const double *Ainv, const float *r, float *xc, int nseg, const int32_t *cu, const int64_t *coff) {
+  k_tail_solve<float><<<nseg, TAIL_TPB, 0, s>>>(nc_units, cptr, child, P, Ainv, r, xc, cu, coff);
 }
 
 }  // namespace uftk

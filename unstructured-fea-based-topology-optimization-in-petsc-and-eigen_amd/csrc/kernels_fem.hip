@@ -275,13 +275,19 @@ __device__ __forceinline__ double tet6(const double *a, const double *b, const d
   return (u0 * (v1 * w2 - v2 * w1) + u1 * (v2 * w0 - v0 * w2) + u2 * (v0 * w1 - v1 * w0)) / 6.0;
 }
 
+// r2d (design batches): the squared filter length of each design
 __global__ __launch_bounds__(TPB) void k_kf_hex8(int ne, const int32_t *__restrict__ elem,
                                                  const double *__restrict__ xyz, double r2,
                                                  double *__restrict__ KF, double *__restrict__ T,
-                                                 double *__restrict__ vol, double *__restrict__ part) {
-  const int e = blockIdx.x * TPB + threadIdx.x;
+                                                 double *__restrict__ vol, double *__restrict__ part, SegG sg,
+                                                 const double *__restrict__ r2d) {
+  int64_t e64;
+  int dsg;
+  const bool ok = uftr::seg_row(sg, ne, e64, dsg);
+  const int e = (int)e64;
+  if (r2d) r2 = r2d[dsg];
   double v[1] = {0.0};
-  if (e < ne) {
+  if (ok) {
     double X[8][3];
 #pragma unroll
     for (int n = 0; n < 8; ++n) {
@@ -359,10 +365,15 @@ __global__ __launch_bounds__(TPB) void k_kf_hex8(int ne, const int32_t *__restri
 __global__ __launch_bounds__(TPB) void k_kf_tet4(int ne, const int32_t *__restrict__ elem,
                                                  const double *__restrict__ xyz, double r2,
                                                  double *__restrict__ KF, double *__restrict__ T,
-                                                 double *__restrict__ vol, double *__restrict__ part) {
-  const int e = blockIdx.x * TPB + threadIdx.x;
+                                                 double *__restrict__ vol, double *__restrict__ part, SegG sg,
+                                                 const double *__restrict__ r2d) {
+  int64_t e64;
+  int dsg;
+  const bool ok = uftr::seg_row(sg, ne, e64, dsg);
+  const int e = (int)e64;
+  if (r2d) r2 = r2d[dsg];
   double v[1] = {0.0};
-  if (e < ne) {
+  if (ok) {
     double X[4][3];
 #pragma unroll
     for (int n = 0; n < 4; ++n) {
@@ -659,11 +670,16 @@ __global__ __launch_bounds__(TPB) void k_sens(int ne, const int32_t *__restrict_
                                               const double2 *__restrict__ KE, const double *__restrict__ xphys,
                                               SimpCoef sc, double Emax_minus_Emin, const double *__restrict__ vol,
                                               double invTV, const double *__restrict__ u, double *__restrict__ dfdx,
-                                              double *__restrict__ dgdx0, double *__restrict__ part) {
+                                              double *__restrict__ dgdx0, double *__restrict__ part, SegG sg,
+                                              const double *__restrict__ invTVd) {
   constexpr int ND = 3 * NPE, NP = ND * (ND + 1) / 2, NQ = NP / 2;
-  const int e = blockIdx.x * TPB + threadIdx.x;
+  int64_t e64;
+  int dsg;
+  const bool ok = uftr::seg_row(sg, ne, e64, dsg);
+  const int e = (int)e64;
+  if (invTVd) invTV = invTVd[dsg];
   double v[2] = {0.0, 0.0};
-  if (e < ne) {
+  if (ok) {
     double ue[ND];
 #pragma unroll
     for (int a = 0; a < NPE; ++a) {
@@ -864,12 +880,14 @@ __global__ __launch_bounds__(TPB) void k_gensub(int64_t n, int k, double asymini
                                                 double *__restrict__ alpha, double *__restrict__ beta,
                                                 double *__restrict__ p0, double *__restrict__ q0,
                                                 double *__restrict__ pij, double *__restrict__ qij,
-                                                double *__restrict__ part) {
-  const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
+                                                double *__restrict__ part, SegG sg) {
+  int64_t i;
+  int dsg;
+  const bool ok = uftr::seg_row(sg, n, i, dsg);
   double v[M];
 #pragma unroll
   for (int j = 0; j < M; ++j) v[j] = 0.0;
-  if (i < n) {
+  if (ok) {
     const double xi = x[i], xmi = xmin[i], xma = xmax[i];
     double Li, Ui;
     if (k < 3) {
@@ -940,13 +958,19 @@ __global__ __launch_bounds__(TPB) void k_dual(int64_t n, LamArr lam, const doubl
                                               const double *__restrict__ beta, const double *__restrict__ p0,
                                               const double *__restrict__ q0, const double *__restrict__ pij,
                                               const double *__restrict__ qij, double *__restrict__ x,
-                                              double *__restrict__ part) {
+                                              double *__restrict__ part, SegG sg, const double *__restrict__ lamd) {
   constexpr int K = M + M * (M + 1) / 2;
-  const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  int64_t i;
+  int dsg;
+  const bool ok = uftr::seg_row(sg, n, i, dsg);
+  if (lamd) {  // per design (design batches, M = 1)
+#pragma unroll
+    for (int j = 0; j < M; ++j) lam.v[j] = lamd[dsg * M + j];
+  }
   double v[K];
 #pragma unroll
   for (int j = 0; j < K; ++j) v[j] = 0.0;
-  if (i < n) {
+  if (ok) {
     double pl = p0[i], ql = q0[i];
     double pj[M], qj[M];
 #pragma unroll
@@ -983,10 +1007,12 @@ __global__ __launch_bounds__(TPB) void k_dual(int64_t n, LamArr lam, const doubl
 }
 
 __global__ __launch_bounds__(TPB) void k_design_change(int64_t n, const double *__restrict__ x, double *__restrict__ xold,
-                                                       double *__restrict__ part) {
-  const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
+                                                       double *__restrict__ part, SegG sg) {
+  int64_t i;
+  int dsg;
+  const bool ok = uftr::seg_row(sg, n, i, dsg);
   double v = 0.0;
-  if (i < n) {
+  if (ok) {
     v = fabs(x[i] - xold[i]);
     xold[i] = x[i];
   }
@@ -1099,12 +1125,13 @@ void gen_ke(hipStream_t s, int npe, int64_t ne, const int32_t *elem, const doubl
 }
 
 void gen_kf(hipStream_t s, int npe, int64_t ne, const int32_t *elem, const double *xyz, double rmin, double *KF,
-            double *T, double *vol, double *part) {
+            double *T, double *vol, double *part, SegG sg, const double *r2d) {
   const double r = 1.0 / (2.0 * sqrt(3.0)) * rmin;
+  const unsigned g = uftr::seg_grid(sg, ne);
   if (npe == 8)
-    k_kf_hex8<<<nblk(ne), TPB, 0, s>>>((int)ne, elem, xyz, r * r, KF, T, vol, part);
+    k_kf_hex8<<<g, TPB, 0, s>>>((int)ne, elem, xyz, r * r, KF, T, vol, part, sg, r2d);
   else
-    k_kf_tet4<<<nblk(ne), TPB, 0, s>>>((int)ne, elem, xyz, r * r, KF, T, vol, part);
+    k_kf_tet4<<<g, TPB, 0, s>>>((int)ne, elem, xyz, r * r, KF, T, vol, part, sg, r2d);
 }
 
 static SimpCoef simp(const uft_ctx *C) { return SimpCoef{C->P.Emin, C->P.Emax - C->P.Emin, C->P.penal}; }
@@ -1164,14 +1191,15 @@ void diag_K(hipStream_t s, const uft_ctx *C, const double *xphys, double *dinv, 
 }
 
 void sens(hipStream_t s, const uft_ctx *C, const double *xphys, const double *u, double *dfdx, double *dgdx0,
-          double *part) {
+          double *part, SegG sg, const double *invTVd) {
   const int ne = (int)C->ne;
+  const unsigned g = uftr::seg_grid(sg, ne);
   if (C->npe == 8)
-    k_sens<8><<<nblk(ne), TPB, 0, s>>>(ne, C->d_elem, (const double2 *)C->d_KE, xphys, simp(C), C->P.Emax - C->P.Emin,
-                                       C->vec[UFT_VOL], 1.0 / C->TV, u, dfdx, dgdx0, part);
+    k_sens<8><<<g, TPB, 0, s>>>(ne, C->d_elem, (const double2 *)C->d_KE, xphys, simp(C), C->P.Emax - C->P.Emin,
+                                C->vec[UFT_VOL], 1.0 / C->TV, u, dfdx, dgdx0, part, sg, invTVd);
   else
-    k_sens<4><<<nblk(ne), TPB, 0, s>>>(ne, C->d_elem, (const double2 *)C->d_KE, xphys, simp(C), C->P.Emax - C->P.Emin,
-                                       C->vec[UFT_VOL], 1.0 / C->TV, u, dfdx, dgdx0, part);
+    k_sens<4><<<g, TPB, 0, s>>>(ne, C->d_elem, (const double2 *)C->d_KE, xphys, simp(C), C->P.Emax - C->P.Emin,
+                                C->vec[UFT_VOL], 1.0 / C->TV, u, dfdx, dgdx0, part, sg, invTVd);
 }
 
 void apply_KF(hipStream_t s, const uft_ctx *C, const double *x, double *y) {
@@ -1292,50 +1320,53 @@ void mma_movelimit(hipStream_t s, int64_t n, const double *x, double Xmin, doubl
 
 template <int M>
 static void gensub_m(hipStream_t s, uft_ctx *C, int64_t n, int k, const double *x, const double *xmin,
-                     const double *xmax, const double *dfdx, const double *const *dgdx, double *part) {
+                     const double *xmax, const double *dfdx, const double *const *dgdx, double *part, SegG sg) {
   PtrArr pa{};
   for (int j = 0; j < M; ++j) pa.p[j] = dgdx[j];
-  k_gensub<M><<<nblk(n), TPB, 0, s>>>(n, k, C->asyminit, C->asymdec, C->asyminc, C->mma_robust, C->mma_conmod, x,
+  k_gensub<M><<<uftr::seg_grid(sg, n), TPB, 0, s>>>(n, k, C->asyminit, C->asymdec, C->asyminc, C->mma_robust, C->mma_conmod, x,
                                       C->d_xo1, C->d_xo2, xmin, xmax,
                                       dfdx, pa, C->d_L, C->d_U, C->d_alpha, C->d_beta, C->d_p0, C->d_q0, C->d_pij,
-                                      C->d_qij, part);
+                                      C->d_qij, part, sg);
 }
 void mma_gensub(hipStream_t s, uft_ctx *C, int64_t n, int m, int k, const double *x, const double *xmin,
-                const double *xmax, const double *dfdx, const double *const *dgdx, double *part) {
+                const double *xmax, const double *dfdx, const double *const *dgdx, double *part, SegG sg) {
   switch (m) {
-    case 1: gensub_m<1>(s, C, n, k, x, xmin, xmax, dfdx, dgdx, part); break;
-    case 2: gensub_m<2>(s, C, n, k, x, xmin, xmax, dfdx, dgdx, part); break;
-    case 3: gensub_m<3>(s, C, n, k, x, xmin, xmax, dfdx, dgdx, part); break;
-    case 4: gensub_m<4>(s, C, n, k, x, xmin, xmax, dfdx, dgdx, part); break;
-    case 5: gensub_m<5>(s, C, n, k, x, xmin, xmax, dfdx, dgdx, part); break;
-    case 6: gensub_m<6>(s, C, n, k, x, xmin, xmax, dfdx, dgdx, part); break;
-    case 7: gensub_m<7>(s, C, n, k, x, xmin, xmax, dfdx, dgdx, part); break;
-    default: gensub_m<8>(s, C, n, k, x, xmin, xmax, dfdx, dgdx, part); break;
+    case 1: gensub_m<1>(s, C, n, k, x, xmin, xmax, dfdx, dgdx, part, sg); break;
+    case 2: gensub_m<2>(s, C, n, k, x, xmin, xmax, dfdx, dgdx, part, sg); break;
+    case 3: gensub_m<3>(s, C, n, k, x, xmin, xmax, dfdx, dgdx, part, sg); break;
+    case 4: gensub_m<4>(s, C, n, k, x, xmin, xmax, dfdx, dgdx, part, sg); break;
+    case 5: gensub_m<5>(s, C, n, k, x, xmin, xmax, dfdx, dgdx, part, sg); break;
+    case 6: gensub_m<6>(s, C, n, k, x, xmin, xmax, dfdx, dgdx, part, sg); break;
+    case 7: gensub_m<7>(s, C, n, k, x, xmin, xmax, dfdx, dgdx, part, sg); break;
+    default: gensub_m<8>(s, C, n, k, x, xmin, xmax, dfdx, dgdx, part, sg); break;
   }
 }
 
 template <int M>
-static void dual_m(hipStream_t s, uft_ctx *C, int64_t n, const double *lam, double *x, double *part) {
+static void dual_m(hipStream_t s, uft_ctx *C, int64_t n, const double *lam, double *x, double *part, SegG sg,
+                   const double *lamd) {
   LamArr la{};
-  for (int j = 0; j < M; ++j) la.v[j] = lam[j];
-  k_dual<M><<<nblk(n), TPB, 0, s>>>(n, la, C->d_L, C->d_U, C->d_alpha, C->d_beta, C->d_p0, C->d_q0, C->d_pij,
-                                    C->d_qij, x, part);
+  if (lam)
+    for (int j = 0; j < M; ++j) la.v[j] = lam[j];
+  k_dual<M><<<uftr::seg_grid(sg, n), TPB, 0, s>>>(n, la, C->d_L, C->d_U, C->d_alpha, C->d_beta, C->d_p0, C->d_q0,
+                                                  C->d_pij, C->d_qij, x, part, sg, lamd);
 }
-void mma_dual_pass(hipStream_t s, uft_ctx *C, int64_t n, int m, const double *lam, double *x, double *part) {
+void mma_dual_pass(hipStream_t s, uft_ctx *C, int64_t n, int m, const double *lam, double *x, double *part, SegG sg,
+                   const double *lamd) {
   switch (m) {
-    case 1: dual_m<1>(s, C, n, lam, x, part); break;
-    case 2: dual_m<2>(s, C, n, lam, x, part); break;
-    case 3: dual_m<3>(s, C, n, lam, x, part); break;
-    case 4: dual_m<4>(s, C, n, lam, x, part); break;
-    case 5: dual_m<5>(s, C, n, lam, x, part); break;
-    case 6: dual_m<6>(s, C, n, lam, x, part); break;
-    case 7: dual_m<7>(s, C, n, lam, x, part); break;
-    default: dual_m<8>(s, C, n, lam, x, part); break;
+    case 1: dual_m<1>(s, C, n, lam, x, part, sg, lamd); break;
+    case 2: dual_m<2>(s, C, n, lam, x, part, sg, lamd); break;
+    case 3: dual_m<3>(s, C, n, lam, x, part, sg, lamd); break;
+    case 4: dual_m<4>(s, C, n, lam, x, part, sg, lamd); break;
+    case 5: dual_m<5>(s, C, n, lam, x, part, sg, lamd); break;
+    case 6: dual_m<6>(s, C, n, lam, x, part, sg, lamd); break;
+    case 7: dual_m<7>(s, C, n, lam, x, part, sg, lamd); break;
+    default: dual_m<8>(s, C, n, lam, x, part, sg, lamd); break;
   }
 }
 
-void design_change(hipStream_t s, int64_t n, const double *x, double *xold, double *part) {
-  k_design_change<<<nblk(n), TPB, 0, s>>>(n, x, xold, part);
+void design_change(hipStream_t s, int64_t n, const double *x, double *xold, double *part, SegG sg) {
+  k_design_change<<<uftr::seg_grid(sg, n), TPB, 0, s>>>(n, x, xold, part, sg);
 }
 void heaviside(hipStream_t s, int64_t n, const double *xf, double *xp, double beta, double x0, double Xmin) {
   k_heaviside<<<nblk(n), TPB, 0, s>>>(n, xf, xp, beta, x0, Xmin);

@@ -142,6 +142,34 @@ int32_t find_col(const std::vector<int32_t> &rp, const std::vector<int32_t> &ci,
   return (it != e && *it == col) ? (int32_t)(it - ci.begin()) : -1;
 }
 
+// Design batches: renumber aggregates in order of first appearance over the
+// units, so that (units being grouped by design, and no aggregate spanning
+// two designs) each design's aggregates are contiguous; returns the design of
+// every aggregate and each design's first aggregate.
+void batch_renumber(std::vector<int32_t> &agg, int64_t na, const std::vector<int32_t> &udes, int nseg,
+                    std::vector<int32_t> &ades, std::vector<int64_t> &aoff) {
+  std::vector<int32_t> map(na, -1);
+  int32_t next = 0;
+  ades.assign(na, 0);
+  for (size_t u = 0; u < agg.size(); ++u) {
+    int32_t &a = agg[u];
+    if (map[a] < 0) {
+      map[a] = next;
+      ades[next] = udes[u];
+      ++next;
+    }
+    a = map[a];
+  }
+  aoff.assign(nseg + 1, 0);
+  for (int64_t a = 0; a < na; ++a) aoff[ades[a] + 1]++;
+  for (int d = 0; d < nseg; ++d) aoff[d + 1] += aoff[d];
+}
+int64_t max_per_design(const std::vector<int64_t> &off) {
+  int64_t m = 0;
+  for (size_t d = 0; d + 1 < off.size(); ++d) m = std::max(m, off[d + 1] - off[d]);
+  return m;
+}
+
 // children lists of an aggregation
 void children(const std::vector<int32_t> &agg, int64_t na, std::vector<int32_t> &cptr, std::vector<int32_t> &child) {
   cptr.assign(na + 1, 0);
@@ -165,6 +193,15 @@ int uft_amg_build_host(uft_ctx *C, std::vector<AmgHostLevel> &H, std::vector<int
     return uft_set_error(C, UFT_ERR_ARG, "mesh too large for 32-bit Galerkin lists (ne=%lld)", (long long)ne);
   std::vector<uint8_t> fixm(nn, 0);
   for (int32_t d : C->fixed) fixm[d / 3] |= (uint8_t)(1u << (d % 3));
+  // design batches: per-unit designs and per-design unit offsets of each level
+  const int nseg = C->bt.nseg;
+  std::vector<int32_t> udes, ades;
+  if (nseg > 0) {
+    udes.assign(nn, 0);
+    for (int d = 0; d < nseg; ++d)
+      for (int64_t n = C->bt.noff[d]; n < C->bt.noff[d + 1]; ++n) udes[n] = d;
+    C->bt.uoff[0] = C->bt.noff;
+  }
 
   const int maxlev = std::max(2, std::min((int)UFT_MAXLEV, C->P.amg_levels));
   const int64_t cmax = std::max(6, C->P.coarse_max);
@@ -177,6 +214,10 @@ int uft_amg_build_host(uft_ctx *C, std::vector<AmgHostLevel> &H, std::vector<int
     AmgHostLevel h;
     std::vector<int32_t> agg;
     const int64_t na = aggregate(g, agg);
+    if (nseg > 0) {
+      batch_renumber(agg, na, udes, nseg, ades, C->bt.uoff[1]);
+      udes.swap(ades);
+    }
     h.nunit_next = na;
     h.parent = agg;
     children(agg, na, h.cptr, h.child);
@@ -278,7 +319,11 @@ int uft_amg_build_host(uft_ctx *C, std::vector<AmgHostLevel> &H, std::vector<int
     while (true) {
       const AmgHostLevel &hp = H.back();
       const int64_t nu = hp.nunit_next;
-      const bool want_more = (6 * nu > cmax && lev + 1 < maxlev) || 6 * nu > dense_limit;
+      // a batch coarsens until each design's block of the coarsest level fits
+      const int64_t nu_c = nseg > 0 ? max_per_design(C->bt.uoff[lev]) : nu;
+      // (and to at least three levels: its coarsest solve works on 6x6 blocks)
+      const bool want_more =
+          (6 * nu_c > cmax && lev + 1 < maxlev) || 6 * nu_c > dense_limit || (nseg > 0 && lev < 2);
       if (!want_more || lev + 1 >= UFT_MAXLEV) break;
       // unit graph from the operator pattern
       Graph gu;
@@ -291,6 +336,10 @@ int uft_amg_build_host(uft_ctx *C, std::vector<AmgHostLevel> &H, std::vector<int
       std::vector<int32_t> ag;
       const int64_t nb2 = aggregate(gu, ag);
       if (nb2 >= nu || nb2 * 3 > nu * 2) break;  // not coarsening any more
+      if (nseg > 0) {
+        batch_renumber(ag, nb2, udes, nseg, ades, C->bt.uoff[lev + 1]);
+        udes.swap(ades);
+      }
       AmgHostLevel h2;
       h2.nunit_next = nb2;
       h2.parent = ag;
@@ -340,6 +389,12 @@ int uft_amg_build_host(uft_ctx *C, std::vector<AmgHostLevel> &H, std::vector<int
     }
   }
   const int64_t nlast = nunits.back();
+  if (nseg > 0) {
+    if (6 * max_per_design(C->bt.uoff[nunits.size() - 1]) > UFT_TAIL_MAX)
+      return uft_set_error(C, UFT_ERR_ARG, "a design's coarsest level exceeds %d unknowns", UFT_TAIL_MAX);
+    if (nunits.size() < 3)
+      return uft_set_error(C, UFT_ERR_ARG, "design batches need at least three multigrid levels");
+  }
   if (6 * nlast > dense_limit * 2)
     return uft_set_error(C, UFT_ERR_ARG, "coarsest level too large for the dense solve (%lld units)", (long long)nlast);
   return UFT_OK;

@@ -56,6 +56,32 @@ int upload(uft_ctx *C, T **p, const std::vector<T> &h) {
   return UFT_OK;
 }
 
+// segment grid of a vector space whose rows are grouped by design (off:
+// nseg + 1 row offsets): blocks of at most 256 rows inside one design
+int build_space(uft_ctx *C, SegSpace &S, const std::vector<int64_t> &off) {
+  const int nseg = (int)off.size() - 1;
+  S.off = off;
+  S.nrows = off[nseg];
+  std::vector<int64_t> span;
+  std::vector<int32_t> des;
+  S.dfirst.assign(nseg + 1, 0);
+  for (int d = 0; d < nseg; ++d) {
+    S.dfirst[d] = (int32_t)des.size();
+    for (int64_t r = off[d]; r < off[d + 1]; r += 256) {
+      span.push_back(r);
+      span.push_back(std::min(off[d + 1], r + 256));
+      des.push_back(d);
+    }
+  }
+  S.dfirst[nseg] = (int32_t)des.size();
+  S.nb = (int)des.size();
+  if (S.nb == 0) return uft_set_error(C, UFT_ERR_ARG, "design batch: empty vector space");
+  int rc;
+  if ((rc = upload(C, &S.d_span, span))) return rc;
+  if ((rc = upload(C, &S.d_des, des))) return rc;
+  return upload(C, &S.d_dfirst, S.dfirst);
+}
+
 void free_all(uft_ctx *C) {
   if (C->s) (void)hipStreamSynchronize(C->s);
   for (void *p : C->allocs) (void)hipFree(p);
@@ -199,6 +225,7 @@ int lev_op_blocks(const uft_ctx *C, int l) {
 int level_kind(const uft_ctx *C, int l) {
   const int ct = C->P.cycle_type;
   if (ct == 0 || (ct == 3 && l > 1)) return 0;
+  if (C->bt.nseg) return 1;  // design batches: the W-cycle in place of the K-cycle (no inner products)
   return ct == 1 ? 1 : 2;
 }
 
@@ -279,6 +306,12 @@ void coarse_direct(uft_ctx *C, int l, const T *r, T *x) {
   AmgLevel &Lv = H.L[l], &Ln = H.L[l + 1];
   LV<T> vn = lv(Ln, (T *)nullptr);
   const T *P = tr_P(Lv, (T *)nullptr);
+  if (C->bt.nseg) {  // block-diagonal coarsest level: one workgroup per design
+    uftk::tail_solve(C->s, (int)Ln.nunit, Lv.d_cptr, Lv.d_child, P, H.d_cinv, r, vn.x, C->bt.nseg, C->bt.d_cu,
+                     C->bt.d_coff);
+    uftk::prolong_comb(C->s, Lv.bs, Lv.nunit, Lv.d_parent, P, nullptr, -1, -1, vn.x, (const T *)nullptr, x);
+    return;
+  }
   if (Lv.bs == 6 && H.ncoarse <= UFT_TAIL_MAX) {
     uftk::tail_solve(C->s, (int)Ln.nunit, Lv.d_cptr, Lv.d_child, P, H.d_cinv, r, vn.x);
     uftk::prolong_comb(C->s, Lv.bs, Lv.nunit, Lv.d_parent, P, nullptr, -1, -1, vn.x, (const T *)nullptr, x);
@@ -631,6 +664,68 @@ int estimate_lmax(uft_ctx *C, int l, int ksteps) {
   return UFT_OK;
 }
 
+// lambda_max from a Lanczos log (alpha_j, beta_j of k Jacobi-CG steps); 0 when
+// the log is unusable
+double lanczos_lmax(const double *hl, int ksteps) {
+  std::vector<double> d, e;
+  for (int j = 0; j < ksteps; ++j) {
+    const double al = hl[2 * j], be = hl[2 * j + 1];
+    if (!(al > 0.0) || !std::isfinite(al)) break;
+    double dj = 1.0 / al;
+    if (j > 0) dj += hl[2 * (j - 1) + 1] / hl[2 * (j - 1)];
+    d.push_back(dj);
+    if (!(be > 0.0) || !std::isfinite(be)) break;
+    e.push_back(std::sqrt(be) / al);
+  }
+  if (d.empty()) return 0.0;
+  e.resize(d.size());
+  return tridiag_lmax(d, e);
+}
+
+const SegSpace &level_space(const uft_ctx *C, int l) { return l == 0 ? C->bt.D : C->bt.R[l]; }
+
+// Design batches: lambda_max(D^-1 A_l) of every design by the same Jacobi-CG
+// Lanczos steps with per-design scalars; each design's Chebyshev scaling is
+// then folded into its rows of D^-1 (D^-1 / lambda_max), so the cycle runs
+// with lambda_max = 1 on every level and needs no per-design coefficients.
+int estimate_lmax_batch(uft_ctx *C, int l, int ksteps) {
+  AmgLevel &Lv = C->amg.L[l];
+  BatchState &B = C->bt;
+  const SegSpace &S = level_space(C, l);
+  const SegG g = S.g();
+  hipStream_t s = C->s;
+  const int ns = B.nseg;
+  const int64_t n = Lv.n;
+  double *x = Lv.x, *r = Lv.r, *z = Lv.d, *p = Lv.b, *q = Lv.t;
+  uftk::fill(s, n, 1.0, r);
+  uftk::scale(s, n, r, Lv.d_dinv, z);
+  uftk::copy(s, n, z, p);
+  uftk::fill(s, n, 0.0, x);
+  uftk::seg_dot(s, g, S.d_dfirst, ns, n, r, z, C->d_part, B.sc, B_LRZ);
+  for (int j = 0; j < ksteps; ++j) {
+    level_apply(C, l, p, q);
+    uftk::seg_dot(s, g, S.d_dfirst, ns, n, p, q, C->d_part, B.sc, B_LPQ);
+    uftk::bsc_lanczos_alpha(s, ns, B.sc, B.d_log, j);
+    uftk::seg_cg_update(s, g, S.d_dfirst, ns, n, B.sc, B_LAL, x, p, r, q, C->d_part, B_LRR);
+    uftk::scale(s, n, r, Lv.d_dinv, z);
+    uftk::seg_dot(s, g, S.d_dfirst, ns, n, r, z, C->d_part, B.sc, B_LRZN);
+    uftk::bsc_lanczos_beta(s, ns, B.sc, B.d_log, j);
+    uftk::seg_xpby(s, g, n, z, B.sc, B_LBE, p);
+  }
+  std::vector<double> hl((size_t)ns * 128);
+  int rc = fetch(C, B.d_log, ns * 128, hl.data());
+  if (rc) return rc;
+  double *fac = B.h + (size_t)ns * UFT_BSC;
+  for (int d = 0; d < ns; ++d) {
+    const double lm = lanczos_lmax(&hl[(size_t)d * 128], ksteps);
+    fac[d] = 1.0 / (lm > 0.0 ? C->P.cheb_hi * lm : 2.0 * C->P.cheb_hi);
+  }
+  UFT_HIP(C, hipMemcpyAsync(B.d_fac, fac, ns * sizeof(double), hipMemcpyHostToDevice, s));
+  uftk::seg_scale_dinv(s, g, n, B.d_fac, Lv.d_dinv, Lv.d_dinvf);
+  Lv.lmax = 1.0;
+  return sync(C);  // the pinned staging of fac is reused by the next level
+}
+
 // Rebuild the density-dependent parts of the hierarchy for xphys.
 int amg_refresh(uft_ctx *C, const double *xphys) {
   AmgHier &H = C->amg;
@@ -649,6 +744,16 @@ int amg_refresh(uft_ctx *C, const double *xphys) {
     if (Ln.d_valf) uftk::d2f(s, Ln.nblk * 36, Ln.d_val, Ln.d_valf);
   }
   AmgLevel &Lc = H.L[H.nlev - 1];
+  if (C->bt.nseg) {
+    BatchState &B = C->bt;
+    uftk::seg_bsr_to_dense(s, Lc.nunit, Lc.d_rp, Lc.d_ci, Lc.d_val, B.d_cdes, B.d_cu, B.d_coff, H.d_cmat);
+    uftk::seg_dense_inverse(s, B.nseg, B.d_cu, B.d_coff, H.d_cmat, H.d_cinv);
+    for (int l = 0; l + 1 < H.nlev; ++l) {
+      int rc = estimate_lmax_batch(C, l, std::min(64, std::max(2, C->P.esteig_steps)));
+      if (rc) return rc;
+    }
+    return UFT_OK;
+  }
   uftk::bsr_to_dense(s, Lc.nunit, Lc.d_rp, Lc.d_ci, Lc.d_val, H.ncoarse, H.d_cmat);
   uftk::dense_spd_inverse(s, H.ncoarse, H.d_cmat, H.d_cinv, H.d_cinvf);
   for (int l = 0; l + 1 < H.nlev; ++l) {
@@ -884,10 +989,140 @@ int solve_el_block(uft_ctx *C, const double *xphys) {
   return UFT_OK;
 }
 
+// fetch every design's scalars; true when some design has flag `slot` set
+int fetch_batch(uft_ctx *C, int slot, bool *any) {
+  BatchState &B = C->bt;
+  int rc = fetch(C, B.sc, B.nseg * UFT_BSC, B.h);
+  if (rc) return rc;
+  bool a = false;
+  for (int d = 0; d < B.nseg; ++d) a |= B.h[(size_t)d * UFT_BSC + slot] != 0.0;
+  if (any) *any = a;
+  return UFT_OK;
+}
+
+// Design batches: one flexible CG per design (the single-design loop of
+// solve_el / solve_el_mixed with per-design scalars, stopping tests on the
+// device), all advanced together; every operator apply and every cycle
+// covers all designs.  A design stops updating once
+// ||F_d - K U_d|| <= rtol ||F_d||; the loop ends when all have.
+int solve_el_batch(uft_ctx *C, const double *xphys, double *U) {
+  hipStream_t s = C->s;
+  BatchState &B = C->bt;
+  const int ns = B.nseg;
+  const SegG g = B.D.g();
+  const int32_t *df = B.D.d_dfirst;
+  const int64_t n = C->ndof;
+  double *r = C->d_r, *z = C->d_z, *p = C->d_p, *q = C->d_q, *zo = C->d_w;
+  const double t0 = now_ms();
+  int rc = amg_refresh(C, xphys);
+  if (rc) return rc;
+  C->t_phase[0] = now_ms() - t0;
+  uftk::fill(s, n, 0.0, U);
+  uftk::copy(s, n, C->d_F, r);
+  uftk::seg_dot(s, g, df, ns, n, r, r, C->d_part, B.sc, B_RR);
+  uftk::bsc_el_init(s, ns, B.sc, C->P.rtol_el);
+  const bool f32 = is_mixed(C);
+  auto precond = [&]() {
+    if (f32) {
+      uftk::d2f(s, n, r, C->d_rf);
+      vcycle_f(C, 0, C->d_rf, C->d_zf);
+      uftk::f2d(s, n, C->d_zf, z);
+    } else {
+      vcycle(C, 0, r, z);
+    }
+  };
+  bool any = false;
+  if ((rc = fetch_batch(C, B_ACT, &any))) return rc;
+  int it = 0;
+  if (any) {
+    precond();
+    uftk::seg_dot(s, g, df, ns, n, r, z, C->d_part, B.sc, B_RZ);
+    uftk::copy(s, n, z, p);
+  }
+  while (any && it < C->P.maxit_el) {
+    fine_apply(C, xphys, p, q, nullptr, nullptr);
+    uftk::seg_dot(s, g, df, ns, n, p, q, C->d_part, B.sc, B_PQ);
+    uftk::bsc_el_alpha(s, ns, B.sc);
+    uftk::seg_cg_update(s, g, df, ns, n, B.sc, B_ALPHA, U, p, r, q, C->d_part, B_RR);
+    uftk::bsc_el_check(s, ns, B.sc);
+    ++it;
+    if ((rc = fetch_batch(C, B_ACT, &any))) return rc;
+    if (!any) break;
+    std::swap(z, zo);
+    precond();
+    uftk::seg_dot_pr(s, g, df, ns, n, r, z, zo, C->d_part, B.sc, B_RZN);
+    uftk::bsc_el_beta(s, ns, B.sc);
+    uftk::seg_xpby(s, g, n, z, B.sc, B_BETA, p);
+  }
+  C->it_el = it;
+  C->outer_el = 1;
+  // true residuals per design (SolveFem.cpp:142-152)
+  fine_apply(C, xphys, U, q, nullptr, nullptr);
+  uftk::sub(s, n, C->d_F, q, q);
+  uftk::seg_dot(s, g, df, ns, n, q, q, C->d_part, B.sc, B_TR);
+  if ((rc = fetch_batch(C, B_ACT, nullptr))) return rc;
+  double worst = 0.0;
+  int bad = -1;
+  for (int d = 0; d < ns; ++d) {
+    const double *S = B.h + (size_t)d * UFT_BSC;
+    const double fn = S[B_FN];
+    B.relres[d] = fn > 0.0 ? std::sqrt(S[B_TR]) / fn : 0.0;
+    B.it[d] = (int)S[B_IT];
+    if (!(B.relres[d] <= C->P.rtol_el) && bad < 0) bad = d;
+    if (!(B.relres[d] <= worst)) worst = B.relres[d];
+  }
+  C->relres_el = worst;
+  C->fnorm = B.h[B_FN];
+  if (bad >= 0)
+    return uft_set_error(C, UFT_ERR_NOCONV, "design %d: %d iterations, relative residual %.3e > %.3e", bad, B.it[bad],
+                         B.relres[bad], C->P.rtol_el);
+  return UFT_OK;
+}
+
+// Design batches: the filter's Jacobi-CG (solve_filter) per design
+int solve_filter_batch(uft_ctx *C, const double *b, double *x) {
+  hipStream_t s = C->s;
+  BatchState &B = C->bt;
+  const int ns = B.nseg;
+  const SegG g = B.N.g();
+  const int32_t *df = B.N.d_dfirst;
+  const int64_t n = C->nn;
+  double *r = C->d_fr, *z = C->d_fz, *p = C->d_fp, *q = C->d_fq;
+  uftk::apply_KF(s, C, x, q);
+  uftk::seg_fcg_init(s, g, df, ns, n, b, q, C->d_fdinv, r, z, C->d_part, B.sc, B_FB);
+  uftk::bsc_f_init(s, ns, B.sc, C->P.rtol_f);
+  bool any = false;
+  int rc = fetch_batch(C, B_FACT, &any);
+  if (rc) return rc;
+  int it = 0;
+  while (any && it < C->P.maxit_f) {
+    uftk::seg_fcg_p(s, g, n, z, B.sc, B_FPC, it == 0, p);
+    uftk::apply_KF(s, C, p, q);
+    uftk::seg_dot(s, g, df, ns, n, p, q, C->d_part, B.sc, B_FPQ);
+    uftk::bsc_f_alpha(s, ns, B.sc);
+    uftk::seg_fcg_update(s, g, df, ns, n, B.sc, B_FAL, x, p, r, q, C->d_fdinv, z, C->d_part, B_FZZN);
+    uftk::bsc_f_after(s, ns, B.sc);
+    ++it;
+    if ((rc = fetch_batch(C, B_FACT, &any))) return rc;
+  }
+  int itmax = 0;
+  for (int d = 0; d < ns; ++d) {
+    const double *S = B.h + (size_t)d * UFT_BSC;
+    B.itf[d] = (int)S[B_FIT];
+    itmax = std::max(itmax, B.itf[d]);
+    if (S[B_FDIV] != 0.0) return uft_set_error(C, UFT_ERR_NOCONV, "design %d: filter CG diverged", d);
+    if (S[B_FACT] != 0.0)
+      return uft_set_error(C, UFT_ERR_NOCONV, "design %d: filter CG, %d iterations without convergence", d, it);
+  }
+  C->it_f = itmax;
+  return UFT_OK;
+}
+
 // Jacobi-CG on KF RX = TX with the reference's KSPCG settings
 // (PDEFilter.cpp:117-137): warm start from the previous RX, preconditioned
 // residual norm relative to ||D^-1 b||, atol 1e-50, dtol 1e5.
 int solve_filter(uft_ctx *C, const double *b, double *x) {
+  if (C->bt.nseg) return solve_filter_batch(C, b, x);
   hipStream_t s = C->s;
   const int64_t n = C->nn;
   double *r = C->d_fr, *z = C->d_fz, *p = C->d_fp, *q = C->d_fq;
@@ -924,6 +1159,121 @@ int solve_filter(uft_ctx *C, const double *b, double *x) {
   C->it_f = it;
   if (it >= C->P.maxit_f && dp > ttol)
     return uft_set_error(C, UFT_ERR_NOCONV, "filter CG: %d iterations without convergence", it);
+  return UFT_OK;
+}
+
+// Design batches: GenSub and the dual interior-point solve (SolveDIP,
+// MMA.cpp:639-934) of every design, m = 1.  Each design runs its own copy of
+// the single-design control flow (epsilon levels, at most 100 Newton steps
+// per level, residual carried across levels, tolerance 1e-9 sqrt(1 + n_d));
+// the designs advance in lockstep, one fused dual pass per step giving every
+// design's x(lambda_d), constraint sum and Hessian sum.  A design that has
+// finished keeps its lambda, so later passes rewrite its x unchanged.
+int mma_update_batch(uft_ctx *C, double *x, const double *dfdx, const double *dgdx, const double *xmin,
+                     const double *xmax) {
+  hipStream_t s = C->s;
+  BatchState &B = C->bt;
+  const int ns = B.nseg;
+  const int64_t n = C->ne;
+  const SegG g = B.E.g();
+  int rc;
+  C->mk++;
+  const double *dg[1] = {dgdx};
+  uftk::mma_gensub(s, C, n, 1, C->mk, x, xmin, xmax, dfdx, dg, C->d_part, g);
+  uftk::seg_final(s, C->d_part, B.E.d_dfirst, ns, 1, B.sc, B_RED);
+  if ((rc = fetch(C, B.sc, ns * UFT_BSC, B.h))) return rc;
+  for (int d = 0; d < ns; ++d) B.b[d] = B.h[(size_t)d * UFT_BSC + B_RED] - B.gx[d];
+  std::swap(C->d_xo1, C->d_xo2);
+  uftk::copy(s, n, x, C->d_xo1);
+
+  const double a = C->a[0], c = C->c[0];
+  std::vector<double> S(ns), H(ns), y(ns), z(ns), epsi(ns, 1.0), err(ns, 1.0), tol(ns);
+  std::vector<int> loop(ns, 0), steps(ns, 0);
+  std::vector<char> done(ns, 0), step(ns, 0);
+  double *hl = B.h + (size_t)ns * UFT_BSC;  // pinned staging of the lambdas
+  auto pass = [&]() -> int {
+    for (int d = 0; d < ns; ++d) hl[d] = B.lam[d];
+    UFT_HIP(C, hipMemcpyAsync(B.d_lam, hl, ns * sizeof(double), hipMemcpyHostToDevice, s));
+    uftk::mma_dual_pass(s, C, n, 1, nullptr, x, C->d_part, g, B.d_lam);
+    uftk::seg_final(s, C->d_part, B.E.d_dfirst, ns, 2, B.sc, B_RED);
+    int r2 = fetch(C, B.sc, ns * UFT_BSC, B.h);
+    if (r2) return r2;
+    for (int d = 0; d < ns; ++d) {
+      S[d] = B.h[(size_t)d * UFT_BSC + B_RED];
+      H[d] = B.h[(size_t)d * UFT_BSC + B_RED + 1];
+    }
+    return UFT_OK;
+  };
+  auto yz = [&](int d) {  // XYZofLAMBDA's scalar part (MMA.cpp:693-701)
+    if (B.lam[d] < 0.0) B.lam[d] = 0.0;
+    y[d] = std::max(0.0, B.lam[d] - c);
+    z[d] = std::max(0.0, 10.0 * (B.lam[d] * a - 1.0));
+  };
+  for (int d = 0; d < ns; ++d) {
+    B.lam[d] = c / 2.0;
+    B.mu[d] = 1.0;
+    tol[d] = 1.0e-9 * std::sqrt((double)(1 + B.eoff[d + 1] - B.eoff[d]));
+    yz(d);
+  }
+  if ((rc = pass())) return rc;
+  while (true) {
+    bool any = false, neg = false;
+    for (int d = 0; d < ns; ++d) {
+      step[d] = 0;
+      if (done[d]) continue;
+      // leave epsilon levels whose inner loop has ended
+      while (!(err[d] > 0.9 * epsi[d] && loop[d] < 100)) {
+        epsi[d] *= 0.1;
+        loop[d] = 0;
+        if (!(epsi[d] > tol[d])) {
+          done[d] = 1;
+          break;
+        }
+      }
+      if (done[d]) continue;
+      step[d] = 1;
+      any = true;
+      neg |= B.lam[d] < 0.0;
+    }
+    if (!any) break;
+    for (int d = 0; d < ns; ++d)
+      if (step[d]) yz(d);
+    if (neg && (rc = pass())) return rc;
+    for (int d = 0; d < ns; ++d) {
+      if (!step[d]) continue;
+      loop[d]++;
+      steps[d]++;
+      const double lam = B.lam[d], mu = B.mu[d], ep = epsi[d];
+      // DualGrad, DualHess (MMA.cpp:730-868), m = 1
+      double grad = S[d] - B.b[d] - a * z[d] - y[d];
+      grad = -1.0 * grad - ep / lam;
+      double hs = H[d];
+      if (lam > c) hs += -1.0;
+      hs += -mu / lam;
+      if (lam * a > 0.0) hs += -10.0 * a * a;
+      double corr = 1e-4 * hs;
+      if (-1.0 * corr < 1.0e-7) corr = -1.0e-7;
+      hs += corr;
+      const double s0 = grad / hs;
+      const double s1 = -mu + ep / lam - s0 * mu / lam;
+      // DualLineSearch (MMA.cpp:870-888)
+      double theta = 1.005;
+      if (theta < -1.01 * s0 / lam) theta = -1.01 * s0 / lam;
+      if (theta < -1.01 * s1 / mu) theta = -1.01 * s1 / mu;
+      theta = 1.0 / theta;
+      B.lam[d] += theta * s0;
+      B.mu[d] += theta * s1;
+      yz(d);
+    }
+    if ((rc = pass())) return rc;
+    for (int d = 0; d < ns; ++d) {  // DualResidual (MMA.cpp:890-934)
+      if (!step[d]) continue;
+      const double r1 = S[d] - B.b[d] - a * z[d] - y[d] + B.mu[d];
+      const double r2 = B.mu[d] * B.lam[d] - epsi[d];
+      err[d] = std::max(std::fabs(r1), std::fabs(r2));
+    }
+  }
+  C->dip_steps = *std::max_element(steps.begin(), steps.end());
   return UFT_OK;
 }
 
@@ -991,6 +1341,7 @@ void uft_destroy(uft_ctx *C) {
   (void)hipSetDevice(C->device);
   free_all(C);
   if (C->h_scal) (void)hipHostFree(C->h_scal);
+  if (C->bt.h) (void)hipHostFree(C->bt.h);
   if (C->s) (void)hipStreamDestroy(C->s);
   delete C;
 }
@@ -1087,6 +1438,25 @@ int uft_setup(uft_ctx *C) {
     if ((rc = upload(C, &C->d_inc, inc))) return rc;
     UFT_HIP(C, hipStreamSynchronize(s));
   }
+  // ---- design batch: segment grids of the element, node and DOF spaces
+  BatchState &B = C->bt;
+  if (B.nseg) {
+    if (m != 1) return uft_set_error(C, UFT_ERR_ARG, "design batches take one constraint (m = 1)");
+    if (C->P.load_cases == 1 && C->nlc > 1)
+      return uft_set_error(C, UFT_ERR_ARG, "design batches take superposed load cases (load_cases = 0)");
+    std::vector<int64_t> doff(B.noff.size());
+    for (size_t d = 0; d < doff.size(); ++d) doff[d] = 3 * B.noff[d];
+    if ((rc = build_space(C, B.E, B.eoff)) || (rc = build_space(C, B.N, B.noff)) || (rc = build_space(C, B.D, doff)))
+      return rc;
+    if ((rc = dalloc_n(C, &B.sc, (int64_t)B.nseg * UFT_BSC))) return rc;
+    uftk::fill(s, (int64_t)B.nseg * UFT_BSC, 0.0, B.sc);
+    if ((rc = upload(C, &B.d_vf, B.volfrac))) return rc;
+    double **bv[] = {&B.d_invTV, &B.d_lam, &B.d_fac};
+    for (double **p : bv)
+      if ((rc = dalloc_n(C, p, (int64_t)B.nseg))) return rc;
+    if ((rc = dalloc_n(C, &B.d_log, (int64_t)B.nseg * 128))) return rc;
+    uftk::fill(s, (int64_t)B.nseg * 128, 0.0, B.d_log);
+  }
   // ---- vectors
   C->vec.assign(UFT_DGDX0 + m, nullptr);
   C->vlen.assign(UFT_DGDX0 + m, 0);
@@ -1129,8 +1499,31 @@ int uft_setup(uft_ctx *C) {
     if ((rc = dalloc_n(C, &C->d_KEs, ne))) return rc;
     uftk::ke_to_half(s, C);
   }
-  uftk::gen_kf(s, npe, ne, C->d_elem, C->d_xyz, C->P.rmin, C->d_KF, C->d_T, C->vec[UFT_VOL], C->d_part);
-  uftk::final_sum(s, C->d_part, uftk::red_blocks(ne), 1, C->d_scal + S_MISC);
+  if (B.nseg) {  // per design filter length and total volume
+    std::vector<double> r2(B.nseg), inv(B.nseg);
+    for (int d = 0; d < B.nseg; ++d) {
+      const double r = 1.0 / (2.0 * std::sqrt(3.0)) * B.rmin[d];
+      r2[d] = r * r;
+    }
+    double *d_r2;
+    if ((rc = upload(C, &d_r2, r2))) return rc;
+    uftk::gen_kf(s, npe, ne, C->d_elem, C->d_xyz, C->P.rmin, C->d_KF, C->d_T, C->vec[UFT_VOL], C->d_part, B.E.g(),
+                 d_r2);
+    uftk::seg_final(s, C->d_part, B.E.d_dfirst, B.nseg, 1, B.sc, B_RED);
+    if ((rc = fetch(C, B.sc, B.nseg * UFT_BSC, B.h))) return rc;
+    B.TV.assign(B.nseg, 0.0);
+    double tv = 0.0;
+    for (int d = 0; d < B.nseg; ++d) {
+      B.TV[d] = B.h[(size_t)d * UFT_BSC + B_RED];
+      inv[d] = 1.0 / B.TV[d];
+      tv += B.TV[d];
+    }
+    UFT_HIP(C, hipMemcpyAsync(B.d_invTV, inv.data(), B.nseg * sizeof(double), hipMemcpyHostToDevice, s));
+    uftk::fill(s, 1, tv, C->d_scal + S_MISC);
+  } else {
+    uftk::gen_kf(s, npe, ne, C->d_elem, C->d_xyz, C->P.rmin, C->d_KF, C->d_T, C->vec[UFT_VOL], C->d_part);
+    uftk::final_sum(s, C->d_part, uftk::red_blocks(ne), 1, C->d_scal + S_MISC);
+  }
   {
     std::vector<double> dm(ne);
     UFT_HIP(C, hipMemcpyAsync(dm.data(), C->d_detmin, ne * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -1236,11 +1629,32 @@ int uft_setup(uft_ctx *C) {
       UFT_HIP(C, hipStreamSynchronize(s));  // host vectors of H go out of scope level by level
     }
     A.ncoarse = A.L[A.nlev - 1].n;
-    // matrix plus the inverse's panel scratch (at most n^2/16 + n^2/2 + 2n + 1 doubles)
-    if ((rc = dalloc_n(C, &A.d_cmat, 2 * A.ncoarse * A.ncoarse + 2 * A.ncoarse + 1))) return rc;
-    if ((rc = dalloc_n(C, &A.d_cinv, A.ncoarse * A.ncoarse))) return rc;
-    if (is_mixed(C))
-      if ((rc = dalloc_n(C, &A.d_cinvf, A.ncoarse * A.ncoarse))) return rc;
+    if (B.nseg) {
+      // rows of every coarse level, and the block-diagonal coarsest level
+      for (int l = 1; l < A.nlev; ++l) {
+        std::vector<int64_t> roff(B.uoff[l].size());
+        for (size_t d = 0; d < roff.size(); ++d) roff[d] = 6 * B.uoff[l][d];
+        if ((rc = build_space(C, B.R[l], roff))) return rc;
+      }
+      const std::vector<int64_t> &cu = B.uoff[A.nlev - 1];
+      std::vector<int32_t> cu32(cu.begin(), cu.end()), cdes(A.L[A.nlev - 1].nunit);
+      B.coff.assign(B.nseg + 1, 0);
+      for (int d = 0; d < B.nseg; ++d) {
+        const int64_t nd6 = 6 * (cu[d + 1] - cu[d]);
+        B.coff[d + 1] = B.coff[d] + nd6 * nd6;
+        for (int64_t u = cu[d]; u < cu[d + 1]; ++u) cdes[u] = d;
+      }
+      if ((rc = upload(C, &B.d_cu, cu32)) || (rc = upload(C, &B.d_coff, B.coff)) || (rc = upload(C, &B.d_cdes, cdes)))
+        return rc;
+      if ((rc = dalloc_n(C, &A.d_cmat, B.coff[B.nseg]))) return rc;
+      if ((rc = dalloc_n(C, &A.d_cinv, B.coff[B.nseg]))) return rc;
+    } else {
+      // matrix plus the inverse's panel scratch (at most n^2/16 + n^2/2 + 2n + 1 doubles)
+      if ((rc = dalloc_n(C, &A.d_cmat, 2 * A.ncoarse * A.ncoarse + 2 * A.ncoarse + 1))) return rc;
+      if ((rc = dalloc_n(C, &A.d_cinv, A.ncoarse * A.ncoarse))) return rc;
+      if (is_mixed(C))
+        if ((rc = dalloc_n(C, &A.d_cinvf, A.ncoarse * A.ncoarse))) return rc;
+    }
     UFT_HIP(C, hipStreamSynchronize(s));
   }
   // ---- multi-load-case form: interleaved right-hand sides and work vectors
@@ -1299,7 +1713,12 @@ int uft_setup(uft_ctx *C) {
     UFT_HIP(C, hipStreamSynchronize(s));
   }
   // ---- design state (OptPara.cpp:150-153) and MMA (OptPara.cpp:158-177, MMA.cpp:178-224)
-  for (int id : {UFT_X, UFT_XFILED, UFT_XPHYS, UFT_XOLD}) uftk::fill(s, ne, C->P.volfrac, C->vec[id]);
+  for (int id : {UFT_X, UFT_XFILED, UFT_XPHYS, UFT_XOLD}) {
+    if (B.nseg)
+      uftk::seg_fill(s, B.E.g(), ne, B.d_vf, C->vec[id]);
+    else
+      uftk::fill(s, ne, C->P.volfrac, C->vec[id]);
+  }
   for (int id : {UFT_DFDX, UFT_XMIN, UFT_XMAX}) uftk::fill(s, ne, 0.0, C->vec[id]);
   uftk::fill(s, ndof, 0.0, C->vec[UFT_U]);
   uftk::fill(s, nn, 0.0, C->vec[UFT_RX]);
@@ -1387,10 +1806,34 @@ int uft_fem_obj_sens(uft_ctx *C, double *fx, double *gx, int dfdx_id, int m, con
   const double *xp = C->vec[xphys_id];
   double *U = C->vec[UFT_U];
   const double t0 = now_ms();
-  const int src = C->mr.nr ? solve_el_block(C, xp) : is_mixed(C) ? solve_el_mixed(C, xp, U) : solve_el(C, xp, U);
+  const int src = C->bt.nseg  ? solve_el_batch(C, xp, U)
+                  : C->mr.nr  ? solve_el_block(C, xp)
+                  : is_mixed(C) ? solve_el_mixed(C, xp, U)
+                                : solve_el(C, xp, U);
   const double t1 = now_ms();
   C->t_phase[1] = t1 - t0 - C->t_phase[0];
   if (src != UFT_OK && src != UFT_ERR_NOCONV) return src;
+  if (C->bt.nseg) {  // per design: compliance, volume constraint (SolveFem.cpp:208-249)
+    BatchState &B = C->bt;
+    uftk::sens(C->s, C, xp, U, C->vec[dfdx_id], C->vec[dgdx_ids[0]], C->d_part, B.E.g(), B.d_invTV);
+    uftk::seg_final(C->s, C->d_part, B.E.d_dfirst, B.nseg, 2, B.sc, B_RED);
+    if ((rc = fetch(C, B.sc, B.nseg * UFT_BSC, B.h))) return rc;
+    fx[0] = 0.0;
+    for (int j = 0; j < m; ++j) gx[j] = 0.0;
+    double gmax = -1e300;
+    for (int d = 0; d < B.nseg; ++d) {
+      const double *S = B.h + (size_t)d * UFT_BSC;
+      B.fx[d] = S[B_RED];
+      B.gx[d] = S[B_RED + 1] / B.TV[d] - B.volfrac[d];
+      fx[0] += B.fx[d];
+      gmax = std::max(gmax, B.gx[d]);
+    }
+    gx[0] = gmax;
+    C->fx = fx[0];
+    C->gx[0] = gx[0];
+    C->t_phase[2] = now_ms() - t1;
+    return src;
+  }
   // SolveFem.cpp:208-249: only constraint 0 is filled by the reference
   uftk::sens(C->s, C, xp, U, C->vec[dfdx_id], C->vec[dgdx_ids[0]], C->d_part);
   uftk::final_sum(C->s, C->d_part, uftk::red_blocks(C->ne), 2, C->d_scal + S_RED);
@@ -1492,6 +1935,7 @@ int uft_mma_update(uft_ctx *C, int x_id, int dfdx_id, const double *gx, const in
   }
   hipStream_t s = C->s;
   double *x = C->vec[x_id];
+  if (C->bt.nseg) return mma_update_batch(C, x, C->vec[dfdx_id], dg[0], C->vec[xmin_id], C->vec[xmax_id]);
   // GenSub (MMA.cpp:506-637)
   C->mk++;
   uftk::mma_gensub(s, C, n, m, C->mk, x, C->vec[xmin_id], C->vec[xmax_id], C->vec[dfdx_id], dg, C->d_part);
@@ -1616,6 +2060,18 @@ int uft_mma_design_change(uft_ctx *C, int x_id, int xold_id, double *ch) {
   if (rc) return rc;
   if (!ch || !valid_vec(C, x_id) || !valid_vec(C, xold_id))
     return uft_set_error(C, UFT_ERR_ARG, "uft_mma_design_change: bad arguments");
+  if (C->bt.nseg) {
+    BatchState &B = C->bt;
+    uftk::design_change(C->s, C->ne, C->vec[x_id], C->vec[xold_id], C->d_part, B.E.g());
+    uftk::seg_final_max(C->s, C->d_part, B.E.d_dfirst, B.nseg, B.sc, B_RED);
+    if ((rc = fetch(C, B.sc, B.nseg * UFT_BSC, B.h))) return rc;
+    *ch = 0.0;
+    for (int d = 0; d < B.nseg; ++d) {
+      B.ch[d] = B.h[(size_t)d * UFT_BSC + B_RED];
+      *ch = std::max(*ch, B.ch[d]);
+    }
+    return UFT_OK;
+  }
   uftk::design_change(C->s, C->ne, C->vec[x_id], C->vec[xold_id], C->d_part);
   uftk::final_max(C->s, C->d_part, uftk::red_blocks(C->ne), C->d_scal + S_RED);
   rc = fetch(C, C->d_scal + S_RED, 1, C->h_scal);
@@ -1940,8 +2396,80 @@ extern "C" int uft_amg_coarse_inverse_get(uft_ctx *C, double *inv) {
   if (int rc = check_setup(C)) return rc;
   if (!inv) return UFT_ERR_ARG;
   const int64_t n = C->amg.ncoarse;
+  if (C->bt.nseg) {  // block-diagonal: scatter each design's block
+    const BatchState &B = C->bt;
+    std::vector<double> blk(B.coff[B.nseg]);
+    UFT_HIP(C, hipMemcpyAsync(blk.data(), C->amg.d_cinv, blk.size() * sizeof(double), hipMemcpyDeviceToHost, C->s));
+    if (int rc = sync(C)) return rc;
+    std::memset(inv, 0, (size_t)(n * n) * sizeof(double));
+    const std::vector<int64_t> &cu = B.uoff[C->amg.nlev - 1];
+    for (int d = 0; d < B.nseg; ++d) {
+      const int64_t r0 = 6 * cu[d], nd = 6 * (cu[d + 1] - cu[d]);
+      for (int64_t i = 0; i < nd; ++i)
+        for (int64_t j = 0; j < nd; ++j) inv[(r0 + i) * n + r0 + j] = blk[B.coff[d] + i * nd + j];
+    }
+    return UFT_OK;
+  }
   UFT_HIP(C, hipMemcpyAsync(inv, C->amg.d_cinv, n * n * sizeof(double), hipMemcpyDeviceToHost, C->s));
   return sync(C);
+}
+
+extern "C" int uft_batch_set(uft_ctx *C, int nd, const int64_t *eoff, const int64_t *noff, const double *volfrac,
+                             const double *rmin) {
+  if (!C) return UFT_ERR_ARG;
+  if (C->is_setup || !C->have_mesh) return uft_set_error(C, UFT_ERR_STATE, "uft_batch_set: after the mesh, before setup");
+  if (C->bt.nseg) return uft_set_error(C, UFT_ERR_STATE, "uft_batch_set: already set");
+  if (nd < 1 || nd > UFT_MAXSEG || !eoff || !noff || !volfrac || !rmin)
+    return uft_set_error(C, UFT_ERR_ARG, "uft_batch_set: 1 to %d designs with offsets, volfrac and rmin", UFT_MAXSEG);
+  if (eoff[0] != 0 || noff[0] != 0 || eoff[nd] != C->ne || noff[nd] != C->nn)
+    return uft_set_error(C, UFT_ERR_ARG, "uft_batch_set: offsets must run from 0 to ne and nn");
+  for (int d = 0; d < nd; ++d) {
+    if (eoff[d + 1] <= eoff[d] || noff[d + 1] <= noff[d])
+      return uft_set_error(C, UFT_ERR_ARG, "uft_batch_set: design %d is empty", d);
+    if (!(volfrac[d] > 0.0 && volfrac[d] <= 1.0) || !(rmin[d] > 0.0))
+      return uft_set_error(C, UFT_ERR_ARG, "uft_batch_set: design %d: volfrac in (0, 1] and rmin > 0", d);
+    for (int64_t e = eoff[d]; e < eoff[d + 1]; ++e)
+      for (int a = 0; a < C->npe; ++a) {
+        const int64_t v = C->elem[e * C->npe + a];
+        if (v < noff[d] || v >= noff[d + 1])
+          return uft_set_error(C, UFT_ERR_ARG, "uft_batch_set: element %lld uses node %lld outside design %d",
+                               (long long)e, (long long)v, d);
+      }
+  }
+  BatchState &B = C->bt;
+  if (hipHostMalloc((void **)&B.h, ((size_t)nd * UFT_BSC + 2 * UFT_MAXSEG) * sizeof(double), hipHostMallocDefault) !=
+      hipSuccess)
+    return uft_set_error(C, UFT_ERR_OOM, "uft_batch_set: pinned host buffer");
+  B.eoff.assign(eoff, eoff + nd + 1);
+  B.noff.assign(noff, noff + nd + 1);
+  B.volfrac.assign(volfrac, volfrac + nd);
+  B.rmin.assign(rmin, rmin + nd);
+  B.fx.assign(nd, 0.0);
+  B.gx.assign(nd, 0.0);
+  B.ch.assign(nd, 0.0);
+  B.relres.assign(nd, 0.0);
+  B.it.assign(nd, 0);
+  B.itf.assign(nd, 0);
+  B.lam.assign(nd, 0.0);
+  B.mu.assign(nd, 0.0);
+  B.b.assign(nd, 0.0);
+  B.nseg = nd;
+  return UFT_OK;
+}
+
+extern "C" int uft_batch_results(const uft_ctx *C, double *fx, double *gx, double *ch, int *it_el, double *relres_el,
+                                 int *it_f) {
+  if (!C) return UFT_ERR_ARG;
+  const BatchState &B = C->bt;
+  for (int d = 0; d < B.nseg; ++d) {
+    if (fx) fx[d] = B.fx[d];
+    if (gx) gx[d] = B.gx[d];
+    if (ch) ch[d] = B.ch[d];
+    if (it_el) it_el[d] = B.it[d];
+    if (relres_el) relres_el[d] = B.relres[d];
+    if (it_f) it_f[d] = B.itf[d];
+  }
+  return B.nseg;
 }
 
 extern "C" int uft_phase_times(const uft_ctx *C, double *ms) {

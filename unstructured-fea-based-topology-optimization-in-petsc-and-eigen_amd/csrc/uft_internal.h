@@ -35,6 +35,29 @@ struct RowOp {
   double *part = nullptr;
 };
 
+// Segment grid of a design batch: blocks of at most 256 consecutive rows,
+// none spanning two designs, so that block partial sums are per design and a
+// block's design is uniform.  span == nullptr: the plain grid of nb(n) blocks
+// over all rows.
+struct SegG {
+  const int64_t *span = nullptr;  // per block: first row, end row
+  const int32_t *des = nullptr;   // per block: design
+  int nb = 0;
+};
+#define UFT_MAXSEG 64   // designs per batch
+#define UFT_BSC 64      // per-design device scalars
+
+// Slots of a batch design's device scalars (sc + d * UFT_BSC): elasticity
+// flexible CG, filter CG, Lanczos, then general reductions from B_RED.
+enum {
+  B_RZ = 0, B_PQ = 1, B_ALPHA = 2, B_RR = 3, B_RZN = 4, B_RZO = 5, B_BETA = 6, B_FN = 7, B_TOL = 8, B_ACT = 9,
+  B_IT = 10, B_TR = 11,
+  B_FB = 16, B_FZZ = 17, B_FBETA = 18, B_FBO = 19, B_FPQ = 20, B_FAL = 21, B_FPC = 22, B_FTOL = 23, B_FACT = 24,
+  B_FIT = 25, B_FSN = 26, B_FDIV = 27, B_FZZN = 28, B_FZRN = 29,
+  B_LRZ = 32, B_LPQ = 33, B_LAL = 34, B_LRR = 35, B_LRZN = 36, B_LBE = 37,
+  B_RED = 40
+};
+
 // E(x) = Emin + x^penal * (Emax - Emin)   (SolveFem.cpp:304)
 struct SimpCoef {
   double Emin, dE, penal;
@@ -103,6 +126,50 @@ struct MrState {
   double *sc = nullptr;                                             // per right-hand-side scalars
   double relres[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   int it[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+};
+
+// Rows of one vector space of a design batch and its segment grid.
+struct SegSpace {
+  int64_t nrows = 0;
+  std::vector<int64_t> off;     // nseg + 1 row offsets
+  std::vector<int32_t> dfirst;  // nseg + 1: first block of each design
+  int64_t *d_span = nullptr;
+  int32_t *d_des = nullptr;
+  int32_t *d_dfirst = nullptr;
+  int nb = 0;
+  SegG g() const {
+    SegG r;
+    r.span = d_span;
+    r.des = d_des;
+    r.nb = nb;
+    return r;
+  }
+};
+
+// Independent designs in one context (uft_batch_set).
+struct BatchState {
+  int nseg = 0;                       // 0: one design
+  std::vector<int64_t> eoff, noff;    // element / node offsets
+  std::vector<double> volfrac, rmin, TV;
+  SegSpace E, N, D;                   // elements, nodes, DOFs
+  SegSpace R[UFT_MAXLEV];             // rows of multigrid level l >= 1
+  std::vector<int64_t> uoff[UFT_MAXLEV];  // unit offsets per level (host)
+  std::vector<int64_t> coff;          // coarsest dense blocks: offsets of the n_d x n_d blocks
+  int32_t *d_cu = nullptr;            // coarsest unit offsets (nseg + 1)
+  int64_t *d_coff = nullptr;
+  int32_t *d_cdes = nullptr;          // coarsest units: design
+  double *sc = nullptr;               // nseg x UFT_BSC device scalars
+  double *h = nullptr;                // pinned host mirror of sc
+  double *d_vf = nullptr;             // per design volfrac
+  double *d_invTV = nullptr;          // per design 1 / total volume
+  double *d_lam = nullptr;            // per design MMA dual variable
+  double *d_fac = nullptr;            // per design Chebyshev scaling 1 / lambda_max
+  double *d_log = nullptr;            // Lanczos log: nseg x 64 steps x 2
+  // per design results of the last iteration
+  std::vector<double> fx, gx, ch, relres;
+  std::vector<int> it, itf;
+  // per design MMA dual state (m = 1)
+  std::vector<double> lam, mu, b;
 };
 
 // host-side description produced by uft_amg_setup.cpp
@@ -214,6 +281,8 @@ struct uft_ctx {
   int elem_variant = 0, elem_variant_f = 0, elem_variant_h = 0;
   double tune_us[16] = {0}, tune_us_f[16] = {0}, tune_us_h[16] = {0};
 
+  BatchState bt;
+
   // stats
   int it_el = 0, it_f = 0, outer_el = 0, dip_steps = 0;
   double relres_el = 0.0;
@@ -229,7 +298,7 @@ int red_blocks(int64_t n);
 void gen_ke(hipStream_t s, int npe, int64_t ne, const int32_t *elem, const double *xyz, double nu,
             double *KE, float *KEf, double *detmin);
 void gen_kf(hipStream_t s, int npe, int64_t ne, const int32_t *elem, const double *xyz, double rmin,
-            double *KF, double *T, double *vol, double *part);
+            double *KF, double *T, double *vol, double *part, SegG sg = SegG(), const double *rmind = nullptr);
 // fine operator apply: y = K_D(x) u   (two-phase element/node)
 // dotw != nullptr: also y.dotw, reduced in the same launch into *dot_out
 void apply_K(hipStream_t s, const uft_ctx *C, const double *xphys, const double *u, double *y,
@@ -244,7 +313,7 @@ void ke_to_half(hipStream_t s, const uft_ctx *C);
 int ke_half_groups(int npe);
 void diag_K(hipStream_t s, const uft_ctx *C, const double *xphys, double *dinv, float *dinvf);
 void sens(hipStream_t s, const uft_ctx *C, const double *xphys, const double *u, double *dfdx,
-          double *dgdx0, double *part);
+          double *dgdx0, double *part, SegG sg = SegG(), const double *invTVd = nullptr);
 // Helmholtz filter
 void apply_KF(hipStream_t s, const uft_ctx *C, const double *x, double *y);
 void diag_KF(hipStream_t s, const uft_ctx *C, double *dinv);
@@ -391,22 +460,28 @@ void lincomb_first(hipStream_t s, int64_t n, const double *sc, const float *b, c
 void final_kc1(hipStream_t s, const double *part, int nb, double *sc);
 void final_kc2(hipStream_t s, const double *part, int nb, double *sc);
 // xc = Ainv P^T r on the coarsest level (one workgroup; ncoarse <= UFT_TAIL_MAX)
+// With a design batch (nseg > 1): one workgroup per design, units
+// [cu[d], cu[d+1]) and the inverse block at Ainv + coff[d].
 void tail_solve(hipStream_t s, int nc_units, const int32_t *cptr, const int32_t *child, const double *P,
-                const double *Ainv, const double *r, double *xc);
+                const double *Ainv, const double *r, double *xc, int nseg = 1, const int32_t *cu = nullptr,
+                const int64_t *coff = nullptr);
 void tail_solve(hipStream_t s, int nc_units, const int32_t *cptr, const int32_t *child, const float *P,
-                const double *Ainv, const float *r, float *xc);
+                const double *Ainv, const float *r, float *xc, int nseg = 1, const int32_t *cu = nullptr,
+                const int64_t *coff = nullptr);
 void fixed_unit_diag(hipStream_t s, int64_t nb_rows, const int32_t *rp, const int32_t *ci, double *val);
 // MMA
 void mma_movelimit(hipStream_t s, int64_t n, const double *x, double Xmin, double Xmax, double movlim,
                    double *xmin, double *xmax);
 void mma_gensub(hipStream_t s, uft_ctx *C, int64_t n, int m, int k, const double *x, const double *xmin,
-                const double *xmax, const double *dfdx, const double *const *dgdx, double *part);
-void mma_dual_pass(hipStream_t s, uft_ctx *C, int64_t n, int m, const double *lam, double *x, double *part);
+                const double *xmax, const double *dfdx, const double *const *dgdx, double *part, SegG sg = SegG());
+// lamd (device, per design of the segment grid) replaces lam when set (m = 1)
+void mma_dual_pass(hipStream_t s, uft_ctx *C, int64_t n, int m, const double *lam, double *x, double *part,
+                   SegG sg = SegG(), const double *lamd = nullptr);
 // out[0] = sum of squares, out[1] = max abs of the KKT residual terms
 void mma_kkt(hipStream_t s, int64_t n, int m, const double *lam, const double *x, const double *dfdx,
              const double *const *dgdx, const double *xmin, const double *xmax, double *part, double *out);
 void stress(hipStream_t s, const uft_ctx *C, const double *xphys, const double *u, double *vms);
-void design_change(hipStream_t s, int64_t n, const double *x, double *xold, double *part);
+void design_change(hipStream_t s, int64_t n, const double *x, double *xold, double *part, SegG sg = SegG());
 void heaviside(hipStream_t s, int64_t n, const double *xf, double *xp, double beta, double x0, double Xmin);
 void projected_sens(hipStream_t s, int64_t n, const double *xf, double beta, double x0, double *v);
 // Krylov helpers (kernels_solve.hip)
@@ -427,6 +502,37 @@ void axpy_inplace_t(hipStream_t s, int64_t n, const double *d, double *x);
 void axpy_inplace_t(hipStream_t s, int64_t n, const float *d, float *x);
 void kcycle_coef1(hipStream_t s, double *sc);
 void kcycle_coef2(hipStream_t s, double *sc);
+// design batches (kernels_batch.hip): per-design scalars at sc + d * UFT_BSC
+void seg_final(hipStream_t s, const double *part, const int32_t *dfirst, int nseg, int K, double *sc, int base);
+void seg_final_max(hipStream_t s, const double *part, const int32_t *dfirst, int nseg, double *sc, int base);
+void seg_dot(hipStream_t s, SegG g, const int32_t *dfirst, int nseg, int64_t n, const double *a, const double *b,
+             double *part, double *sc, int base);
+void seg_dot_pr(hipStream_t s, SegG g, const int32_t *dfirst, int nseg, int64_t n, const double *r, const double *z,
+                const double *zo, double *part, double *sc, int base);
+void seg_cg_update(hipStream_t s, SegG g, const int32_t *dfirst, int nseg, int64_t n, double *sc, int ia, double *x,
+                   const double *p, double *r, const double *q, double *part, int base);
+void seg_xpby(hipStream_t s, SegG g, int64_t n, const double *z, const double *sc, int ib, double *p);
+void seg_fill(hipStream_t s, SegG g, int64_t n, const double *vals, double *x);
+void seg_scale_dinv(hipStream_t s, SegG g, int64_t n, const double *fac, double *dinv, float *dinvf);
+void seg_fcg_init(hipStream_t s, SegG g, const int32_t *dfirst, int nseg, int64_t n, const double *b, const double *q,
+                  const double *dinv, double *r, double *z, double *part, double *sc, int base);
+void seg_fcg_update(hipStream_t s, SegG g, const int32_t *dfirst, int nseg, int64_t n, double *sc, int ia, double *x,
+                    const double *p, double *r, const double *q, const double *dinv, double *z, double *part, int base);
+void seg_fcg_p(hipStream_t s, SegG g, int64_t n, const double *z, const double *sc, int ic, int first, double *p);
+// per-design scalar recurrences (one thread per design)
+void bsc_el_init(hipStream_t s, int nseg, double *sc, double rtol);
+void bsc_el_alpha(hipStream_t s, int nseg, double *sc);
+void bsc_el_check(hipStream_t s, int nseg, double *sc);
+void bsc_el_beta(hipStream_t s, int nseg, double *sc);
+void bsc_f_init(hipStream_t s, int nseg, double *sc, double rtol);
+void bsc_f_alpha(hipStream_t s, int nseg, double *sc);
+void bsc_f_after(hipStream_t s, int nseg, double *sc);
+void bsc_lanczos_alpha(hipStream_t s, int nseg, double *sc, double *log, int step);
+void bsc_lanczos_beta(hipStream_t s, int nseg, double *sc, double *log, int step);
+// coarsest level of a batch: block-diagonal dense matrix and its inverse
+void seg_bsr_to_dense(hipStream_t s, int64_t nb_rows, const int32_t *rp, const int32_t *ci, const double *val,
+                      const int32_t *udes, const int32_t *cu, const int64_t *coff, double *dense);
+void seg_dense_inverse(hipStream_t s, int nseg, const int32_t *cu, const int64_t *coff, double *A, double *Ainv);
 }  // namespace uftk
 
 // host helpers (uft_api.cpp)

@@ -12,9 +12,29 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "uft_internal.h"
+
 namespace uftr {
 
 constexpr int RTPB = 256;
+
+// Row of this thread under a segment grid (blocks of RTPB threads): the
+// plain grid (row = block * RTPB + thread, design 0, valid below n) or the
+// block's own row range and design.
+__device__ __forceinline__ bool seg_row(const SegG &g, int64_t n, int64_t &i, int &d) {
+  if (g.span) {
+    const int64_t lo = g.span[2 * blockIdx.x], hi = g.span[2 * blockIdx.x + 1];
+    i = lo + threadIdx.x;
+    d = g.des[blockIdx.x];
+    return i < hi;
+  }
+  i = (int64_t)blockIdx.x * RTPB + threadIdx.x;
+  d = 0;
+  return i < n;
+}
+inline unsigned seg_grid(const SegG &g, int64_t n) {
+  return g.span ? (unsigned)g.nb : (unsigned)((n + RTPB - 1) / RTPB);
+}
 
 template <int K>
 __device__ __forceinline__ void block_partials(double (&v)[K], double *part) {

@@ -57,6 +57,7 @@ def lib():
     L = C.CDLL(path)
     dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
     ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+    lp = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
     vp = C.c_void_p
     pi = C.POINTER(C.c_int)
     pd = C.POINTER(C.c_double)
@@ -119,6 +120,8 @@ def lib():
         "uft_apply_K_cases": (C.c_int, [vp, C.c_int, dp, dp]),
         "uft_cache_load": (C.c_int, [vp, C.c_char_p]),
         "uft_autotune": (C.c_int, [vp, C.c_int, pi, pd, C.c_int]),
+        "uft_batch_set": (C.c_int, [vp, C.c_int, lp, lp, dp, dp]),
+        "uft_batch_results": (C.c_int, [vp, pd, pd, pd, pi, pd, pi]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -223,6 +226,24 @@ class Context:
 
     def setup(self):
         self._chk(self.L.uft_setup(self.h), "uft_setup")
+
+    # ---- independent designs in one context
+    def batch_set(self, elem_off, node_off, volfrac, rmin):
+        f = lambda a, t: np.ascontiguousarray(a, dtype=t)
+        nd = len(volfrac)
+        self._chk(self.L.uft_batch_set(self.h, nd, f(elem_off, np.int64), f(node_off, np.int64),
+                                       f(volfrac, np.float64), f(rmin, np.float64)), "uft_batch_set")
+        self.ndesign = nd
+
+    def batch_results(self):
+        """Per design: fx, gx, ch, elasticity iterations, relative residual,
+        filter iterations of the last iteration."""
+        nd = self.ndesign
+        fx, gx, ch, rr = ((C.c_double * nd)() for _ in range(4))
+        it, itf = (C.c_int * nd)(), (C.c_int * nd)()
+        self.L.uft_batch_results(self.h, fx, gx, ch, it, rr, itf)
+        return {"fx": np.array(fx[:]), "gx": np.array(gx[:]), "ch": np.array(ch[:]), "it_el": np.array(it[:]),
+                "relres_el": np.array(rr[:]), "it_f": np.array(itf[:])}
 
     def size(self, which):
         return self.L.uft_size(self.h, SIZE[which])
@@ -447,5 +468,73 @@ def cantilever(npe=8, nx=8, ny=4, nz=4, jitter=0.0, seed=0, load_kind=0, device=
     ctx.set_params(Context.default_params(**params))
     ctx.set_mesh(elem, xyz)
     ctx.set_bcs(fixed, ld, lv, nlc=8 if load_kind == 2 else 1)  # kind 2: eight cases of nine loads
+    ctx.setup()
+    return ctx
+
+
+class SplitMix64:
+    """SplitMix64 stream (Steele, Lea, Flood 2014): u = (r >> 11) 2^-53."""
+    M = (1 << 64) - 1
+
+    def __init__(self, seed):
+        self.s = seed & self.M
+
+    def next(self):
+        self.s = (self.s + 0x9E3779B97F4A7C15) & self.M
+        z = self.s
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & self.M
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & self.M
+        return z ^ (z >> 31)
+
+    def uniform(self):
+        return (self.next() >> 11) * 2.0 ** -53
+
+
+def uneven_designs(n=32, seed=32, ne_min=8000, ne_max=64000):
+    """Cantilevers of uneven size: per design ny = nz +- 1 in [10, 33] and
+    nx = 2 ny +- 2 drawn from a SplitMix64 stream, kept when
+    ne_min <= nx ny nz <= ne_max; volfrac ~ U(0.2, 0.4); jitter seed
+    1000 seed + index."""
+    rng = SplitMix64(seed)
+    out = []
+    while len(out) < n:
+        ny = 10 + int(rng.uniform() * 24)
+        nz = ny + int(rng.uniform() * 3) - 1
+        nx = 2 * ny + int(rng.uniform() * 5) - 2
+        vf = 0.2 + 0.2 * rng.uniform()
+        if ne_min <= nx * ny * nz <= ne_max:
+            out.append(dict(nx=nx, ny=ny, nz=nz, volfrac=vf, seed=1000 * seed + len(out)))
+    return out
+
+
+def batch_mesh(designs, npe=8, jitter=0.15, load_kind=0):
+    """Concatenated mesh, BCs and offsets of independent box cantilevers
+    (each [0,2]x[0,1]x[0,1], clamped at x=0)."""
+    xyzs, elems, fixed, ld, lv = [], [], [], [], []
+    eoff, noff = [0], [0]
+    for D in designs:
+        xyz, elem = gen_box(npe, D["nx"], D["ny"], D["nz"], 2.0, 1.0, 1.0, jitter, D["seed"])
+        f, l, v = box_bcs(D["nx"], D["ny"], D["nz"], load_kind)
+        base = noff[-1]
+        xyzs.append(xyz)
+        elems.append(elem + base)
+        fixed.append(f + 3 * base)
+        ld.append(l + 3 * base)
+        lv.append(v)
+        eoff.append(eoff[-1] + len(elem))
+        noff.append(base + len(xyz))
+    return (np.concatenate(elems), np.concatenate(xyzs), np.concatenate(fixed), np.concatenate(ld),
+            np.concatenate(lv), np.array(eoff, np.int64), np.array(noff, np.int64))
+
+
+def batch_cantilever(designs, npe=8, jitter=0.15, device=0, rmin_h=1.5, **params):
+    """One context holding independent cantilevers (uft_batch_set); design d
+    uses its own volfrac and rmin = rmin_h * 2 / nx."""
+    elem, xyz, fixed, ld, lv, eoff, noff = batch_mesh(designs, npe, jitter)
+    ctx = Context(device)
+    ctx.set_params(Context.default_params(**params))
+    ctx.set_mesh(elem, xyz)
+    ctx.set_bcs(fixed, ld, lv, nlc=1)
+    ctx.batch_set(eoff, noff, [D["volfrac"] for D in designs], [rmin_h * 2.0 / D["nx"] for D in designs])
     ctx.setup()
     return ctx
