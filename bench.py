@@ -43,6 +43,10 @@ WORKLOADS = {
     # the c1 mesh with eight separate load cases (seeded 3x3 patches on x=2,
     # unit total each), compliance summed over the cases
     "c4a": (8, 140, 70, 40, 392000, 2, 0.30, 1.5),
+    # 32 independent cantilevers of uneven size (8,000-64,000 hex8 each,
+    # 868,008 in total; sizes and volfrac ~ U(0.2, 0.4) drawn from seed 32),
+    # one context: a step advances every design by one iteration
+    "c4b": (8, None, None, None, 32, 0, None, 1.5),
 }
 
 # Krylov iteration counts of the oracle's reference-structure iteration on the
@@ -112,17 +116,30 @@ def run_form(args, precision):
     import uft
     from uft.capi import Context
 
-    npe, dims, seed, lk, prm = workload(args.workload, PREC[precision])
-    xyz, elem = uft.gen_box(npe, *dims, jitter=0.15, seed=seed)
-    fixed, ld, lv = uft.box_bcs(*dims, load_kind=lk)
-    ctx = Context(0)
-    ctx.set_params(Context.default_params(**prm))
-    ctx.set_mesh(elem, xyz)
-    ctx.set_bcs(fixed, ld, lv, nlc=8 if lk == 2 else 1)
-    t0 = time.time()
-    ctx.setup()
-    ctx.sync()
-    setup_s = time.time() - t0
+    ndesign = 1
+    if args.workload == "c4b":
+        from uft.capi import uneven_designs, batch_cantilever, batch_mesh
+        ds = uneven_designs()
+        ndesign = len(ds)
+        npe, dims, seed = 8, None, 32
+        t0 = time.time()
+        ctx = batch_cantilever(ds, jitter=0.15, rmin_h=1.5, precision=PREC[precision], Emax=1.0, Emin=1e-9, nu=0.3,
+                               penal=3.0, rtol_el=1e-8, rtol_f=1e-8, graph=args.graph)
+        ctx.sync()
+        setup_s = time.time() - t0
+    else:
+        npe, dims, seed, lk, prm = workload(args.workload, PREC[precision])
+        prm["graph"] = args.graph
+        xyz, elem = uft.gen_box(npe, *dims, jitter=0.15, seed=seed)
+        fixed, ld, lv = uft.box_bcs(*dims, load_kind=lk)
+        ctx = Context(0)
+        ctx.set_params(Context.default_params(**prm))
+        ctx.set_mesh(elem, xyz)
+        ctx.set_bcs(fixed, ld, lv, nlc=8 if lk == 2 else 1)
+        t0 = time.time()
+        ctx.setup()
+        ctx.sync()
+        setup_s = time.time() - t0
     ne, ndof = ctx.size("ne"), ctx.size("ndof")
     ctx.init_design()
     itr = 0
@@ -130,8 +147,8 @@ def run_form(args, precision):
         itr += 1
         ctx.design_iteration(itr)
     L = ctx.L
-    L.uft_profile_enable(ctx.h, 1)
-    its, fxs = [], []
+    L.uft_profile_enable(ctx.h, args.timers)
+    its, fxs, per_design = [], [], []
     ctx.sync()
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -139,6 +156,9 @@ def run_form(args, precision):
         fx, gx, ch = ctx.design_iteration(itr)
         its.append(ctx.stats()["it_el"])
         fxs.append(fx)
+        if ndesign > 1:
+            br = ctx.batch_results()
+            per_design.append({"fx": br["fx"].tolist(), "it_el": br["it_el"].tolist()})
     ctx.sync()
     t1 = time.perf_counter()
     prof = {}
@@ -149,7 +169,7 @@ def run_form(args, precision):
     L.uft_profile_enable(ctx.h, 0)
     ctx.close()
     return {"npe": npe, "dims": dims, "seed": seed, "ne": ne, "ndof": ndof, "setup_s": setup_s, "dt": t1 - t0,
-            "its": its, "fxs": fxs, "prof": prof}
+            "its": its, "fxs": fxs, "prof": prof, "ndesign": ndesign, "per_design": per_design}
 
 
 def elem_roofline(name, npe, ne, ndof, prof_elem, prof_node, word, kernel, traffic_key, nrhs=1):
@@ -178,6 +198,12 @@ def main():
                     help="both (default): time the FP64 form and the mixed form (FP16 smoother blocks); "
                          "the value is the mixed form's rate")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--timers", type=int, default=16,
+                    help="HIP-event timers around one in every TIMERS fine applies of each kind in the timed "
+                         "region (the roofline figures; default 16, 1 = every apply), 0 = none")
+    ap.add_argument("--graph", type=int, default=0, choices=[0, 1],
+                    help="replay each Krylov iteration from a captured hipGraph (1) or launch it kernel by "
+                         "kernel (0, default; with timers on the solve launches kernel by kernel anyway)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy three times (reference for rocprof PMC scaling)")
     args = ap.parse_args()
@@ -200,7 +226,7 @@ def main():
     main_form = forms[-1]
     r = res[main_form]
     npe, dims, seed, ne, ndof = r["npe"], r["dims"], r["seed"], r["ne"], r["ndof"]
-    value = args.steps / r["dt"]
+    value = args.steps * r["ndesign"] / r["dt"]
     ename = "hex8" if npe == 8 else "tet4"
     rooflines = []
     multi = WORKLOADS[args.workload][5] == 2
@@ -227,15 +253,21 @@ def main():
     # the dominant element phase of the timed form: the one with the most time
     cand = [x for x in rooflines if x["launches"]] if main_form == "fp64" else \
         [x for x in rooflines[-2:] if x["launches"]]
-    dom = max(cand, key=lambda x: x["total_ms_timed"])
-    config = {"workload": f"{args.workload}: {ne} {ename} jittered cantilever {dims[0]}x{dims[1]}x{dims[2]}, "
-                          f"seed {seed}, rtol 1e-8, {main_form} form",
-              "global_batch": 1, "seq_len": None, "parallelism": "single GPU", "precision": main_form,
+    dom = max(cand, key=lambda x: x["total_ms_timed"]) if cand else rooflines[-1]
+    if r["ndesign"] > 1:
+        wl = (f"{args.workload}: {r['ndesign']} independent {ename} jittered cantilevers of uneven size "
+              f"({ne} elements in total, 8,000-64,000 each, sizes and volfrac drawn from seed {seed}), one context, "
+              f"rtol 1e-8, {main_form} form; value counts design iterations summed over the designs")
+    else:
+        wl = (f"{args.workload}: {ne} {ename} jittered cantilever {dims[0]}x{dims[1]}x{dims[2]}, "
+              f"seed {seed}, rtol 1e-8, {main_form} form")
+    config = {"workload": wl,
+              "global_batch": r["ndesign"], "graph": args.graph, "seq_len": None, "parallelism": "single GPU", "precision": main_form,
               "setup_s": r["setup_s"], "krylov_iters_per_solve": sum(r["its"]) / len(r["its"]),
               "final_fx": r["fxs"][-1]}
     if "fp64" in res and main_form != "fp64":
         f64 = res["fp64"]
-        config["fp64_design_iters_per_s"] = args.steps / f64["dt"]
+        config["fp64_design_iters_per_s"] = args.steps * f64["ndesign"] / f64["dt"]
         config["mixed_over_fp64"] = value / config["fp64_design_iters_per_s"]
         config["fp64_krylov_iters_per_solve"] = sum(f64["its"]) / len(f64["its"])
         config["max_rel_compliance_diff_vs_fp64"] = max(abs(a - b) / abs(b) for a, b in zip(r["fxs"], f64["fxs"]))

@@ -73,6 +73,11 @@ typedef struct uft_params {
   int load_cases;      /* with several load cases: 0 superposed into one right-hand side (the reference's
                           INSERT, SolveFem.cpp:277-283); 1 separate right-hand sides (at most 8) solved
                           together, compliance and sensitivities summed over the cases */
+  int graph;           /* 1: each Krylov iteration of the elasticity solve after the first is captured once
+                          in a hipGraph and replayed (re-captured when a single design's hierarchy is
+                          refreshed); 0 (default): launched kernel by kernel.  Same results either way.
+                          While the device timers are on (uft_profile_enable) the solve launches kernel by
+                          kernel. */
 } uft_params;
 
 /* --- context ------------------------------------------------------------ */
@@ -235,15 +240,25 @@ int uft_phase_times(const uft_ctx *ctx, double *ms);
  * variant[0]/[1] receive the FP64/FP32 choice; us_per_variant (nmax entries)
  * the FP64 timings in microseconds. */
 int uft_autotune(uft_ctx *ctx, int reps, int *variant, double *us_per_variant, int nmax);
+/* Smoother-schedule autotuner (no counterpart in the reference, whose
+ * multigrid schedule is fixed by options): times one elasticity solve at the
+ * current XPhys for each Chebyshev degree 1..3 and each cycle type (V, W,
+ * K-cycle; V and W for design batches), keeps the fastest schedule in the
+ * context's parameters and returns it; ms (12 doubles, [degree - 1][cycle])
+ * receives the times, an unconverged schedule timing as 1e300.  The solve
+ * leaves UFT_U as the last candidate's solution. */
+int uft_autotune_smoother(uft_ctx *ctx, int *degree, int *cycle_type, double *ms);
 /* Synchronise the context stream. */
 int uft_sync(uft_ctx *ctx);
-/* Device timers: HIP events recorded on the context stream around every
- * launch of a timed kernel while enabled (no counterpart in the reference,
- * whose only instrumentation is MPI_Wtime around the loop body,
- * main.cpp:74,109).  uft_profile_read returns the number of launches timed and
- * their summed duration in ms for kind 0 = element phase of the FP64 fine
- * stiffness apply, 1 = its node phase, 2/3 = the same for the FP32 apply of
- * the mixed form. */
+/* Device timers: HIP events recorded on the context stream around the
+ * launches of the timed kernels while enabled (no counterpart in the
+ * reference, whose only instrumentation is MPI_Wtime around the loop body,
+ * main.cpp:74,109).  on = k > 0 times one in every k fine applies (1: all;
+ * a sample keeps the events' own cost out of the throughput); 0 disables.
+ * uft_profile_read returns the number of launches timed and their summed
+ * duration in ms for kind 0 = element phase of the FP64 fine stiffness
+ * apply, 1 = its node phase, 2/3 = the same for the FP32 apply of the mixed
+ * form. */
 int uft_profile_enable(uft_ctx *ctx, int on);
 int uft_profile_read(uft_ctx *ctx, int kind, int64_t *count, double *total_ms);
 

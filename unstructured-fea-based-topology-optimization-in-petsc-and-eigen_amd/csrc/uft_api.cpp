@@ -113,6 +113,7 @@ bool valid_vec(const uft_ctx *C, int id) {
 // while the device timers are enabled
 hipEvent_t *prof_triple(uft_ctx *C, int kind) {
   if (!C->prof) return nullptr;
+  if (C->prof_tick[kind & 1]++ % C->prof_every != 0) return nullptr;
   while (C->evpool.size() < C->evused + 3) {
     hipEvent_t e;
     if (hipEventCreate(&e) != hipSuccess) return nullptr;
@@ -126,6 +127,50 @@ hipEvent_t *prof_triple(uft_ctx *C, int kind) {
 }
 
 bool is_mixed(const uft_ctx *C) { return C->P.precision != UFT_FP64; }
+
+void graph_release(uft_ctx *C) {
+  if (C->ig_exec) (void)hipGraphExecDestroy(C->ig_exec);
+  if (C->ig_graph) (void)hipGraphDestroy(C->ig_graph);
+  C->ig_exec = nullptr;
+  C->ig_graph = nullptr;
+  C->ig_refresh = -1;
+}
+
+// Graph replays are used unless the device timers are on: event records
+// captured into a graph cannot be read back as elapsed times, so a timed
+// solve launches kernel by kernel (the same kernels and arguments).
+bool use_graph(const uft_ctx *C) { return C->P.graph && !C->prof; }
+
+// Launch `body` (a fixed sequence of kernel launches on C->s whose arguments
+// depend only on the key pointers and the hierarchy refresh) through a
+// hipGraph: captured on first use, replayed afterwards.  The body contains no
+// memset or copy nodes (vector fills and copies are kernels).
+template <typename F>
+int graph_launch(uft_ctx *C, const void *k0, const void *k1, const void *k2, F &&body) {
+  const void *key[4] = {k0, k1, k2, (const void *)(intptr_t)(16 * C->P.cheb_degree + C->P.cycle_type + 1)};
+  // a batch folds its Chebyshev scaling into D^-1, so its cycle's launch
+  // arguments do not change with a refresh; a single design's do
+  bool same = C->ig_exec && (C->bt.nseg || C->ig_refresh == C->refreshes);
+  for (int k = 0; k < 4; ++k) same = same && key[k] == C->ig_key[k];
+  if (!same) {
+    graph_release(C);
+    hipError_t e = hipStreamBeginCapture(C->s, hipStreamCaptureModeRelaxed);
+    if (e == hipSuccess) {
+      body();
+      e = hipStreamEndCapture(C->s, &C->ig_graph);
+    }
+    if (e != hipSuccess) {
+      graph_release(C);
+      return uft_set_error(C, UFT_ERR_HIP, "Krylov iteration capture failed: %s", hipGetErrorString(e));
+    }
+    UFT_HIP(C, hipGraphInstantiate(&C->ig_exec, C->ig_graph, nullptr, nullptr, 0));
+    for (int k = 0; k < 4; ++k) C->ig_key[k] = key[k];
+    C->ig_refresh = C->refreshes;
+  }
+  UFT_HIP(C, hipGraphLaunch(C->ig_exec, C->s));
+  return UFT_OK;
+}
+
 
 void fine_apply(uft_ctx *C, const double *xphys, const double *x, double *y, const double *dotw, double *part) {
   uftk::apply_K(C->s, C, xphys, x, y, dotw, part, prof_triple(C, 0));
@@ -731,6 +776,7 @@ int amg_refresh(uft_ctx *C, const double *xphys) {
   AmgHier &H = C->amg;
   hipStream_t s = C->s;
   C->cur_xphys = xphys;
+  C->refreshes++;
   uftk::diag_K(s, C, xphys, H.L[0].d_dinv, H.L[0].d_dinvf);
   for (int l = 0; l + 1 < H.nlev; ++l) {
     AmgLevel &Lv = H.L[l], &Ln = H.L[l + 1];
@@ -793,10 +839,29 @@ int solve_el(uft_ctx *C, const double *xphys, double *U) {
   const double tol = C->P.rtol_el * bn;
   int it = 0;
   bool conv = false;
-  while (it < C->P.maxit_el) {
+  auto head = [&]() {
     fine_apply(C, xphys, p, q, p, C->d_scal + S_PQ);
     uftk::sc_div(s, C->d_scal, S_RZ, S_PQ, S_ALPHA);
     uftk::cg_update(s, n, C->d_scal, S_ALPHA, U, p, r, q, C->d_part, C->d_scal + S_RR);
+  };
+  // flexible CG (Polak-Ribiere beta): the K-cycle preconditioner contains
+  // inner products, so it is not a fixed linear operator
+  auto next = [&]() {
+    uftk::copy(s, n, z, zo);  // the previous preconditioned residual
+    vcycle(C, 0, r, z);
+    uftk::dot_pr(s, n, r, z, zo, C->d_part, C->d_scal + S_RZNEW);
+    uftk::sc_beta_flex(s, C->d_scal, S_RZNEW, S_RZNEW + 1, S_RZ, S_BETA + 2);
+    uftk::xpby(s, n, z, C->d_scal, S_BETA + 2, p);
+    head();
+  };
+  static const char tag = 0;
+  while (it < C->P.maxit_el) {
+    if (it == 0)
+      head();
+    else if (!use_graph(C))
+      next();
+    else if ((rc = graph_launch(C, xphys, U, &tag, next)))
+      return rc;
     ++it;
     rc = fetch(C, C->d_scal + S_RR, 1, C->h_scal);
     if (rc) return rc;
@@ -806,13 +871,6 @@ int solve_el(uft_ctx *C, const double *xphys, double *U) {
       conv = true;
       break;
     }
-    // flexible CG (Polak-Ribiere beta): the K-cycle preconditioner contains
-    // inner products, so it is not a fixed linear operator
-    std::swap(z, zo);
-    vcycle(C, 0, r, z);
-    uftk::dot_pr(s, n, r, z, zo, C->d_part, C->d_scal + S_RZNEW);
-    uftk::sc_beta_flex(s, C->d_scal, S_RZNEW, S_RZNEW + 1, S_RZ, S_BETA + 2);
-    uftk::xpby(s, n, z, C->d_scal, S_BETA + 2, p);
   }
   C->it_el = it;
   // true residual (SolveFem.cpp:142-152)
@@ -864,10 +922,27 @@ int solve_el_mixed(uft_ctx *C, const double *xphys, double *U) {
   const double tol = C->P.rtol_el * bn;
   int it = 0;
   bool conv = false;
-  while (it < C->P.maxit_el) {
+  auto head = [&]() {
     fine_apply(C, xphys, p, q, p, C->d_scal + S_PQ);
     uftk::sc_div(s, C->d_scal, S_RZ, S_PQ, S_ALPHA);
     uftk::cg_update(s, n, C->d_scal, S_ALPHA, U, p, r, q, C->d_part, C->d_scal + S_RR);
+  };
+  auto next = [&]() {
+    uftk::copy(s, n, z, zo);  // zo keeps the previous preconditioned residual
+    precond();
+    uftk::dot_pr(s, n, r, z, zo, C->d_part, C->d_scal + S_RZNEW);
+    uftk::sc_beta_flex(s, C->d_scal, S_RZNEW, S_RZNEW + 1, S_RZ, S_BETA + 2);
+    uftk::xpby(s, n, z, C->d_scal, S_BETA + 2, p);
+    head();
+  };
+  static const char tag = 0;
+  while (it < C->P.maxit_el) {
+    if (it == 0)
+      head();
+    else if (!use_graph(C))
+      next();
+    else if ((rc = graph_launch(C, xphys, U, &tag, next)))
+      return rc;
     ++it;
     if ((rc = fetch(C, C->d_scal + S_RR, 1, C->h_scal))) return rc;
     const double rn = std::sqrt(C->h_scal[0]);
@@ -876,11 +951,6 @@ int solve_el_mixed(uft_ctx *C, const double *xphys, double *U) {
       conv = true;
       break;
     }
-    std::swap(z, zo);  // zo keeps the previous preconditioned residual
-    precond();
-    uftk::dot_pr(s, n, r, z, zo, C->d_part, C->d_scal + S_RZNEW);
-    uftk::sc_beta_flex(s, C->d_scal, S_RZNEW, S_RZNEW + 1, S_RZ, S_BETA + 2);
-    uftk::xpby(s, n, z, C->d_scal, S_BETA + 2, p);
   }
   C->it_el = it;
   fine_apply(C, xphys, U, q, nullptr, nullptr);
@@ -1039,20 +1109,31 @@ int solve_el_batch(uft_ctx *C, const double *xphys, double *U) {
     uftk::seg_dot(s, g, df, ns, n, r, z, C->d_part, B.sc, B_RZ);
     uftk::copy(s, n, z, p);
   }
-  while (any && it < C->P.maxit_el) {
+  auto head = [&]() {
     fine_apply(C, xphys, p, q, nullptr, nullptr);
     uftk::seg_dot(s, g, df, ns, n, p, q, C->d_part, B.sc, B_PQ);
     uftk::bsc_el_alpha(s, ns, B.sc);
     uftk::seg_cg_update(s, g, df, ns, n, B.sc, B_ALPHA, U, p, r, q, C->d_part, B_RR);
     uftk::bsc_el_check(s, ns, B.sc);
-    ++it;
-    if ((rc = fetch_batch(C, B_ACT, &any))) return rc;
-    if (!any) break;
-    std::swap(z, zo);
+  };
+  auto next = [&]() {
+    uftk::copy(s, n, z, zo);  // the previous preconditioned residual
     precond();
     uftk::seg_dot_pr(s, g, df, ns, n, r, z, zo, C->d_part, B.sc, B_RZN);
     uftk::bsc_el_beta(s, ns, B.sc);
     uftk::seg_xpby(s, g, n, z, B.sc, B_BETA, p);
+    head();
+  };
+  static const char tag = 0;
+  while (any && it < C->P.maxit_el) {
+    if (it == 0)
+      head();
+    else if (!use_graph(C))
+      next();
+    else if ((rc = graph_launch(C, xphys, U, &tag, next)))
+      return rc;
+    ++it;
+    if ((rc = fetch_batch(C, B_ACT, &any))) return rc;
   }
   C->it_el = it;
   C->outer_el = 1;
@@ -1312,6 +1393,7 @@ void uft_default_params(uft_params *p) {
   p->cheb_hi = 1.1;
   p->esteig_steps = 12;
   p->load_cases = 0;
+  p->graph = 0;
 }
 
 int uft_create(uft_ctx **out, int device) {
@@ -1339,6 +1421,7 @@ int uft_create(uft_ctx **out, int device) {
 void uft_destroy(uft_ctx *C) {
   if (!C) return;
   (void)hipSetDevice(C->device);
+  graph_release(C);
   free_all(C);
   if (C->h_scal) (void)hipHostFree(C->h_scal);
   if (C->bt.h) (void)hipHostFree(C->bt.h);
@@ -1355,6 +1438,7 @@ int uft_params_set(uft_ctx *C, const uft_params *p) {
     return uft_set_error(C, UFT_ERR_ARG, "bad Chebyshev interval or estimate steps");
   if (p->load_cases != 0 && p->load_cases != 1)
     return uft_set_error(C, UFT_ERR_ARG, "load_cases must be 0 (superposed) or 1 (separate)");
+  if (p->graph != 0 && p->graph != 1) return uft_set_error(C, UFT_ERR_ARG, "graph must be 0 or 1");
   if (p->precision < UFT_FP64 || p->precision > UFT_MIXED_H)
     return uft_set_error(C, UFT_ERR_ARG, "precision must be UFT_FP64, UFT_MIXED or UFT_MIXED_H");
   if (C->is_setup && (p->m != C->P.m || p->rmin != C->P.rmin || p->nu != C->P.nu || p->precision != C->P.precision ||
@@ -2178,7 +2262,9 @@ int uft_sync(uft_ctx *C) {
 
 extern "C" int uft_profile_enable(uft_ctx *C, int on) {
   if (!C) return UFT_ERR_ARG;
-  C->prof = on != 0;
+  C->prof = on > 0;
+  C->prof_every = on > 0 ? on : 1;
+  C->prof_tick[0] = C->prof_tick[1] = 0;
   C->evused = 0;
   for (int k = 0; k < 4; ++k) {
     C->prof_ms[k] = 0.0;
@@ -2412,6 +2498,48 @@ extern "C" int uft_amg_coarse_inverse_get(uft_ctx *C, double *inv) {
   }
   UFT_HIP(C, hipMemcpyAsync(inv, C->amg.d_cinv, n * n * sizeof(double), hipMemcpyDeviceToHost, C->s));
   return sync(C);
+}
+
+// Smoother-schedule autotuner: one elasticity solve at the current XPhys per
+// candidate (Chebyshev degree 1-3 x cycle type), the fastest kept.
+extern "C" int uft_autotune_smoother(uft_ctx *C, int *degree, int *cycle_type, double *ms) {
+  if (int rc = check_setup(C)) return rc;
+  const int deg0 = C->P.cheb_degree, ct0 = C->P.cycle_type;
+  const int nct = C->bt.nseg ? 2 : 3;  // a batch runs the K-cycle as the W-cycle
+  const double *xp = C->vec[UFT_XPHYS];
+  double *U = C->vec[UFT_U];
+  double best = 1e300;
+  int bd = deg0, bc = ct0;
+  for (int d = 1; d <= 3; ++d)
+    for (int ct = 0; ct < nct; ++ct) {
+      C->P.cheb_degree = d;
+      C->P.cycle_type = ct;
+      if (int rc = sync(C)) return rc;
+      const double t0 = now_ms();
+      const int rc = C->bt.nseg  ? solve_el_batch(C, xp, U)
+                     : C->mr.nr  ? solve_el_block(C, xp)
+                     : is_mixed(C) ? solve_el_mixed(C, xp, U)
+                                   : solve_el(C, xp, U);
+      if (rc != UFT_OK && rc != UFT_ERR_NOCONV) {
+        C->P.cheb_degree = deg0;
+        C->P.cycle_type = ct0;
+        return rc;
+      }
+      if (int r2 = sync(C)) return r2;
+      const double t = rc == UFT_OK ? now_ms() - t0 : 1e300;
+      if (ms) ms[(d - 1) * 4 + ct] = t;
+      if (t < best) {
+        best = t;
+        bd = d;
+        bc = ct;
+      }
+    }
+  C->P.cheb_degree = bd;
+  C->P.cycle_type = bc;
+  if (degree) *degree = bd;
+  if (cycle_type) *cycle_type = bc;
+  if (best >= 1e300) return uft_set_error(C, UFT_ERR_NOCONV, "uft_autotune_smoother: no schedule converged");
+  return UFT_OK;
 }
 
 extern "C" int uft_batch_set(uft_ctx *C, int nd, const int64_t *eoff, const int64_t *noff, const double *volfrac,

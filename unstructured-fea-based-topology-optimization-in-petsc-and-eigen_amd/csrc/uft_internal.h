@@ -266,6 +266,8 @@ struct uft_ctx {
 
   // device timers (uft_profile_*): event triples around fine applies
   bool prof = false;
+  int prof_every = 1;   // time one in every prof_every fine applies of each kind
+  int64_t prof_tick[2] = {0, 0};
   std::vector<hipEvent_t> evpool;
   std::vector<int> evkind;  // per triple: 0 FP64 apply, 1 FP32 apply
   size_t evused = 0;
@@ -282,6 +284,13 @@ struct uft_ctx {
   double tune_us[16] = {0}, tune_us_f[16] = {0}, tune_us_h[16] = {0};
 
   BatchState bt;
+
+  // hipGraph of one Krylov iteration of the elasticity solve (uft_params.graph)
+  hipGraph_t ig_graph = nullptr;
+  hipGraphExec_t ig_exec = nullptr;
+  const void *ig_key[4] = {nullptr, nullptr, nullptr, nullptr};
+  int64_t ig_refresh = -1;  // hierarchy refresh the graph was captured after
+  int64_t refreshes = 0;
 
   // stats
   int it_el = 0, it_f = 0, outer_el = 0, dip_steps = 0;

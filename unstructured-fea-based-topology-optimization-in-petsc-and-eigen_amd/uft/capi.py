@@ -36,7 +36,7 @@ class Params(C.Structure):
                 ("precision", C.c_int), ("amg_levels", C.c_int), ("cheb_degree", C.c_int),
                 ("coarse_max", C.c_int), ("cycle_type", C.c_int),
                 ("cheb_lo", C.c_double), ("cheb_hi", C.c_double), ("esteig_steps", C.c_int),
-                ("load_cases", C.c_int)]
+                ("load_cases", C.c_int), ("graph", C.c_int)]
 
 
 _LIB = None
@@ -121,6 +121,7 @@ def lib():
         "uft_cache_load": (C.c_int, [vp, C.c_char_p]),
         "uft_autotune": (C.c_int, [vp, C.c_int, pi, pd, C.c_int]),
         "uft_batch_set": (C.c_int, [vp, C.c_int, lp, lp, dp, dp]),
+        "uft_autotune_smoother": (C.c_int, [vp, pi, pi, pd]),
         "uft_batch_results": (C.c_int, [vp, pd, pd, pd, pi, pd, pi]),
     }
     for name, (res, args) in sig.items():
@@ -226,6 +227,13 @@ class Context:
 
     def setup(self):
         self._chk(self.L.uft_setup(self.h), "uft_setup")
+
+    def autotune_smoother(self):
+        """(degree, cycle type, times ms [3][4]) of the smoother-schedule autotuner."""
+        d, c = C.c_int(), C.c_int()
+        ms = (C.c_double * 12)()
+        self._chk(self.L.uft_autotune_smoother(self.h, C.byref(d), C.byref(c), ms), "uft_autotune_smoother")
+        return d.value, c.value, np.array(ms[:]).reshape(3, 4)
 
     # ---- independent designs in one context
     def batch_set(self, elem_off, node_off, volfrac, rmin):
