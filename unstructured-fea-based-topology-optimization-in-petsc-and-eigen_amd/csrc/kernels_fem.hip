@@ -830,6 +830,23 @@ __global__ __launch_bounds__(TPB) void k_cg_update(int64_t n, const double *__re
   }
   uftr::block_partials<1>(v, part);
 }
+// FP64 iterate and residual updated with a lower-precision operator product:
+// u += a p ; r -= a q ; partials of r.r
+__global__ __launch_bounds__(TPB) void k_cg_update_lo(int64_t n, const double *__restrict__ sc, int ia,
+                                                      double *__restrict__ u, const double *__restrict__ p,
+                                                      double *__restrict__ r, const float *__restrict__ q,
+                                                      double *__restrict__ part) {
+  const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  const double al = sc[ia];
+  double v[1] = {0.0};
+  if (i < n) {
+    u[i] += al * p[i];
+    const double rr = r[i] - al * (double)q[i];
+    r[i] = rr;
+    v[0] = rr * rr;
+  }
+  uftr::block_partials<1>(v, part);
+}
 template <typename T>
 __global__ __launch_bounds__(TPB) void k_xpby(int64_t n, const T *__restrict__ z, const double *__restrict__ sc, int ib,
                                               T *__restrict__ p) {
@@ -1292,6 +1309,11 @@ void scale(hipStream_t s, int64_t n, const double *a, const double *dinv, double
 void cg_update(hipStream_t s, int64_t n, const double *sc, int ia, double *u, const double *p, double *r,
                const double *q, double *part, double *out) {
   k_cg_update<double><<<nblk(n), TPB, 0, s>>>(n, sc, ia, u, p, r, q, part, out);
+  final_sum(s, part, (int)nblk(n), 1, out);
+}
+void cg_update_lo(hipStream_t s, int64_t n, const double *sc, int ia, double *u, const double *p, double *r,
+                  const float *q, double *part, double *out) {
+  k_cg_update_lo<<<nblk(n), TPB, 0, s>>>(n, sc, ia, u, p, r, q, part);
   final_sum(s, part, (int)nblk(n), 1, out);
 }
 void cg_updatef(hipStream_t s, int64_t n, const double *sc, int ia, float *u, const float *p, float *r, const float *q,

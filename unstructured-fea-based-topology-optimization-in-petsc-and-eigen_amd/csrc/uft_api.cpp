@@ -113,7 +113,7 @@ bool valid_vec(const uft_ctx *C, int id) {
 // while the device timers are enabled
 hipEvent_t *prof_triple(uft_ctx *C, int kind) {
   if (!C->prof) return nullptr;
-  if (C->prof_tick[kind & 1]++ % C->prof_every != 0) return nullptr;
+  if (C->prof_tick[kind]++ % C->prof_every != 0) return nullptr;
   while (C->evpool.size() < C->evused + 3) {
     hipEvent_t e;
     if (hipEventCreate(&e) != hipSuccess) return nullptr;
@@ -174,6 +174,24 @@ int graph_launch(uft_ctx *C, const void *k0, const void *k1, const void *k2, F &
 
 void fine_apply(uft_ctx *C, const double *xphys, const double *x, double *y, const double *dotw, double *part) {
   uftk::apply_K(C->s, C, xphys, x, y, dotw, part, prof_triple(C, 0));
+}
+
+// Outer operator product of the mixed forms with the FP32 element blocks:
+// qf = K(x) p in FP32 (p rounded to FP32), and p.q into sc[slot]
+void fine_apply_lo(uft_ctx *C, const double *xphys, const double *p, int slot) {
+  hipStream_t s = C->s;
+  uftk::d2f(s, C->ndof, p, C->d_pf);
+  hipEvent_t *ev = prof_triple(C, 2);
+  if (ev) (void)hipEventRecord(ev[0], s);
+  uftk::elem_phase_f(s, C, C->elem_variant_f, xphys, C->d_pf, C->d_Yef);
+  if (ev) (void)hipEventRecord(ev[1], s);
+  RowOp<float> op;
+  op.mode = ROW_STOREDOT;
+  op.y = C->d_qf;
+  op.part = C->d_part;
+  uftk::gather_op(s, C, C->d_pf, C->d_Yef, op);
+  if (ev) (void)hipEventRecord(ev[2], s);
+  uftk::final_sum(s, C->d_part, uftk::gather_op_blocks(C->nn), 1, C->d_scal + slot);
 }
 
 // y = A_l x on multigrid level l
@@ -635,6 +653,23 @@ void mr_fine_cycle(uft_ctx *C, const T *b, T *x) {
 }
 
 // largest eigenvalue of a symmetric tridiagonal matrix (bisection, Sturm counts)
+// Operator of the eigenvalue estimates: the one the level's smoother applies
+// (in the mixed forms the fine level's FP32 or FP16 element blocks), on FP64
+// vectors
+void lanczos_apply(uft_ctx *C, int l, const double *x, double *y) {
+  AmgLevel &Lv = C->amg.L[l];
+  if (l == 0 && is_mixed(C)) {
+    uftk::d2f(C->s, Lv.n, x, Lv.xf);
+    RowOp<float> op;
+    op.mode = ROW_STORE;
+    op.y = Lv.tf;
+    lev_op(C, 0, Lv.xf, op);
+    uftk::f2d(C->s, Lv.n, Lv.tf, y);
+    return;
+  }
+  level_apply(C, l, x, y);
+}
+
 double tridiag_lmax(const std::vector<double> &d, const std::vector<double> &e) {
   const int k = (int)d.size();
   double lo = 1e300, hi = -1e300;
@@ -675,7 +710,7 @@ int estimate_lmax(uft_ctx *C, int l, int ksteps) {
   uftk::dot(s, n, r, z, C->d_part, C->d_scal + S_MISC);  // rz in S_MISC
   double *log = C->d_lanczos;
   for (int j = 0; j < ksteps; ++j) {
-    level_apply(C, l, p, q);
+    lanczos_apply(C, l, p, q);
     uftk::dot(s, n, p, q, C->d_part, C->d_scal + S_MISC + 1);
     uftk::sc_div(s, C->d_scal, S_MISC, S_MISC + 1, S_MISC + 2);  // alpha
     uftk::sc_copy(s, C->d_scal + S_MISC + 2, log + 3 * j);       // log alpha
@@ -748,7 +783,7 @@ int estimate_lmax_batch(uft_ctx *C, int l, int ksteps) {
   uftk::fill(s, n, 0.0, x);
   uftk::seg_dot(s, g, S.d_dfirst, ns, n, r, z, C->d_part, B.sc, B_LRZ);
   for (int j = 0; j < ksteps; ++j) {
-    level_apply(C, l, p, q);
+    lanczos_apply(C, l, p, q);
     uftk::seg_dot(s, g, S.d_dfirst, ns, n, p, q, C->d_part, B.sc, B_LPQ);
     uftk::bsc_lanczos_alpha(s, ns, B.sc, B.d_log, j);
     uftk::seg_cg_update(s, g, S.d_dfirst, ns, n, B.sc, B_LAL, x, p, r, q, C->d_part, B_LRR);
@@ -922,10 +957,15 @@ int solve_el_mixed(uft_ctx *C, const double *xphys, double *U) {
   const double tol = C->P.rtol_el * bn;
   int it = 0;
   bool conv = false;
+  // The outer operator product runs with the FP32 element blocks; the FP64
+  // residual F - K U replaces the recursively updated one whenever the
+  // latter has fallen 100-fold below the last true residual, and before the
+  // solve is accepted (reliable updates: the recursion's drift stays 1e-5 of
+  // the current residual, far below the tolerance).
   auto head = [&]() {
-    fine_apply(C, xphys, p, q, p, C->d_scal + S_PQ);
+    fine_apply_lo(C, xphys, p, S_PQ);
     uftk::sc_div(s, C->d_scal, S_RZ, S_PQ, S_ALPHA);
-    uftk::cg_update(s, n, C->d_scal, S_ALPHA, U, p, r, q, C->d_part, C->d_scal + S_RR);
+    uftk::cg_update_lo(s, n, C->d_scal, S_ALPHA, U, p, r, C->d_qf, C->d_part, C->d_scal + S_RR);
   };
   auto next = [&]() {
     uftk::copy(s, n, z, zo);  // zo keeps the previous preconditioned residual
@@ -936,6 +976,8 @@ int solve_el_mixed(uft_ctx *C, const double *xphys, double *U) {
     head();
   };
   static const char tag = 0;
+  double rtrue = bn;  // last FP64 residual norm
+  C->reliable = 0;
   while (it < C->P.maxit_el) {
     if (it == 0)
       head();
@@ -947,20 +989,32 @@ int solve_el_mixed(uft_ctx *C, const double *xphys, double *U) {
     if ((rc = fetch(C, C->d_scal + S_RR, 1, C->h_scal))) return rc;
     const double rn = std::sqrt(C->h_scal[0]);
     if (!std::isfinite(rn)) return uft_set_error(C, UFT_ERR_NOCONV, "mixed CG diverged at iteration %d", it);
-    if (rn <= tol) {
-      conv = true;
-      break;
+    if (rn <= tol || rn < 1e-2 * rtrue) {  // reliable update
+      fine_apply(C, xphys, U, q, nullptr, nullptr);
+      uftk::sub(s, n, C->d_F, q, r);
+      uftk::dot(s, n, r, r, C->d_part, C->d_scal + S_RR);
+      if ((rc = fetch(C, C->d_scal + S_RR, 1, C->h_scal))) return rc;
+      rtrue = std::sqrt(C->h_scal[0]);
+      C->reliable++;
+      if (rtrue <= tol) {
+        conv = true;
+        break;
+      }
     }
   }
   C->it_el = it;
+  C->outer_el = C->reliable;  // FP64 residual evaluations
+  if (conv) {  // the last reliable update is the true residual (SolveFem.cpp:142-152)
+    C->relres_el = rtrue / bn;
+    return UFT_OK;
+  }
   fine_apply(C, xphys, U, q, nullptr, nullptr);
   uftk::sub(s, n, C->d_F, q, q);
   uftk::dot(s, n, q, q, C->d_part, C->d_scal + S_MISC);
   if ((rc = fetch(C, C->d_scal + S_MISC, 1, C->h_scal))) return rc;
   C->relres_el = std::sqrt(C->h_scal[0]) / bn;
-  if (!conv)
-    return uft_set_error(C, UFT_ERR_NOCONV, "mixed elasticity solve: %d iterations, relative residual %.3e", it,
-                         C->relres_el);
+  return uft_set_error(C, UFT_ERR_NOCONV, "mixed elasticity solve: %d iterations, relative residual %.3e", it,
+                       C->relres_el);
   return UFT_OK;
 }
 
@@ -2264,9 +2318,9 @@ extern "C" int uft_profile_enable(uft_ctx *C, int on) {
   if (!C) return UFT_ERR_ARG;
   C->prof = on > 0;
   C->prof_every = on > 0 ? on : 1;
-  C->prof_tick[0] = C->prof_tick[1] = 0;
+  C->prof_tick[0] = C->prof_tick[1] = C->prof_tick[2] = 0;
   C->evused = 0;
-  for (int k = 0; k < 4; ++k) {
+  for (int k = 0; k < 6; ++k) {
     C->prof_ms[k] = 0.0;
     C->prof_n[k] = 0;
   }
@@ -2274,7 +2328,7 @@ extern "C" int uft_profile_enable(uft_ctx *C, int on) {
 }
 
 extern "C" int uft_profile_read(uft_ctx *C, int kind, int64_t *count, double *total_ms) {
-  if (!C || kind < 0 || kind > 3 || !count || !total_ms) return UFT_ERR_ARG;
+  if (!C || kind < 0 || kind > 5 || !count || !total_ms) return UFT_ERR_ARG;
   UFT_HIP(C, hipStreamSynchronize(C->s));
   // fold the recorded triples into the running totals
   for (size_t i = 0; i + 3 <= C->evused; i += 3) {

@@ -267,12 +267,12 @@ struct uft_ctx {
   // device timers (uft_profile_*): event triples around fine applies
   bool prof = false;
   int prof_every = 1;   // time one in every prof_every fine applies of each kind
-  int64_t prof_tick[2] = {0, 0};
+  int64_t prof_tick[3] = {0, 0, 0};
   std::vector<hipEvent_t> evpool;
   std::vector<int> evkind;  // per triple: 0 FP64 apply, 1 FP32 apply
   size_t evused = 0;
-  double prof_ms[4] = {0.0, 0.0, 0.0, 0.0};
-  int64_t prof_n[4] = {0, 0, 0, 0};
+  double prof_ms[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  int64_t prof_n[6] = {0, 0, 0, 0, 0, 0};
 
   // wall time of the phases of the last design iteration (ms): 0 hierarchy
   // refresh, 1 elasticity Krylov solve, 2 sensitivities, 3 sensitivity
@@ -293,7 +293,7 @@ struct uft_ctx {
   int64_t refreshes = 0;
 
   // stats
-  int it_el = 0, it_f = 0, outer_el = 0, dip_steps = 0;
+  int it_el = 0, it_f = 0, outer_el = 0, dip_steps = 0, reliable = 0;
   double relres_el = 0.0;
   double fx = 0.0, gx[UFT_MAXM];
 };
@@ -348,6 +348,8 @@ void axpy_inplace(hipStream_t s, int64_t n, double a, const double *x, double *y
 void sub(hipStream_t s, int64_t n, const double *a, const double *b, double *c);
 void set_dofs(hipStream_t s, int n, const int32_t *idx, const double *val, double *x);
 // templated float variants for the mixed form
+void cg_update_lo(hipStream_t s, int64_t n, const double *sc, int ia, double *u, const double *p, double *r,
+                  const float *q, double *part, double *out);
 void cg_updatef(hipStream_t s, int64_t n, const double *sc, int ialpha, float *u, const float *p,
                 float *r, const float *q, double *part, double *rr_out);
 void xpbyf(hipStream_t s, int64_t n, const float *z, const double *sc, int ibeta, float *p);
