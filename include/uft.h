@@ -186,6 +186,11 @@ int uft_load_case_u(uft_ctx *ctx, int k, double *host);
  * kernel of the outer loop, kind 1 the preconditioner's (FP32 vectors;
  * BF16 matrix-core element products in the mixed forms). */
 int uft_apply_K_cases(uft_ctx *ctx, int kind, const double *u, double *y);
+/* Diagnostic (no counterpart in the reference): y = A_l x with the operator
+ * of multigrid level l as of the last hierarchy refresh (level 0 = the fine
+ * stiffness of XPhys), in the stored precision kind 0 = FP64, 1 = FP32,
+ * 2 = FP16 (the UFT_MIXED_H smoother's blocks); x, y of length n_l. */
+int uft_level_apply(uft_ctx *ctx, int level, int kind, const double *x, double *y);
 
 /* --- MMA settings, restart and KKT residual (MMA.h:8-52) ------------------ */
 /* MMA::SetAsymptotes (MMA.cpp:347-355): asymptote initialisation,

@@ -11,6 +11,8 @@
 // Galerkin:  A_{l+1}[A,B] = sum P_a^T A_l[a,b] P_b over the precomputed,
 // ordered contribution list of block (A,B): one wavefront per coarse block,
 // lanes over contributions, fixed-order cross-lane reduction.  No atomics.
+#include <hip/hip_fp16.h>
+
 #include <type_traits>
 
 #include "uft_internal.h"
@@ -450,6 +452,26 @@ __global__ __launch_bounds__(TPB) void k_dense_mv(int64_t n, const T *__restrict
 
 }  // namespace
 
+namespace {
+// FP16 copy of a level's operator scaled to unit diagonal: Ah_ij =
+// a_ij / sqrt(a_ii a_jj) (|Ah_ij| <= 1 for an SPD matrix), and the FP32 row
+// scales dh_i = sqrt(a_ii) from the inverse diagonal.
+__global__ __launch_bounds__(256) void k_bsr6_to_half(int64_t nblk, const int32_t *__restrict__ brow,
+                                                     const int32_t *__restrict__ ci, const double *__restrict__ val,
+                                                     const double *__restrict__ dinv, __half *__restrict__ valh) {
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (t >= nblk * 36) return;
+  const int64_t k = t / 36;
+  const int e = (int)(t - k * 36), i = e / 6, j = e - (e / 6) * 6;
+  const int64_t A = brow[k], B = ci[k];
+  valh[t] = __float2half_rn((float)(val[t] * sqrt(dinv[A * 6 + i] * dinv[B * 6 + j])));
+}
+__global__ __launch_bounds__(256) void k_sqrt_diag(int64_t n, const double *__restrict__ dinv, float *__restrict__ dh) {
+  const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (r < n) dh[r] = (float)(1.0 / sqrt(dinv[r]));
+}
+}  // namespace
+
 namespace uftk {
 
 void cheb_first(hipStream_t s, int64_t n, const double *b, const double *dinv, double inv_theta, double *r, double *d,
@@ -484,6 +506,12 @@ void bsr6_mvf(hipStream_t s, int64_t nb_rows, const int32_t *rp, const int32_t *
 void bsr6_dinv(hipStream_t s, int64_t nb_rows, const int32_t *rp, const int32_t *ci, double *val, double *dinv,
                float *dinvf) {
   k_bsr6_dinv<<<nblk(nb_rows * 6), TPB, 0, s>>>(nb_rows, rp, ci, val, dinv, dinvf);
+}
+void bsr6_to_half(hipStream_t s, int64_t nblk, int64_t nb_rows, const int32_t *brow, const int32_t *ci,
+                  const double *val, const double *dinv, uint16_t *valh, float *dh) {
+  k_bsr6_to_half<<<(unsigned)((nblk * 36 + 255) / 256), 256, 0, s>>>(nblk, brow, ci, val, dinv,
+                                                                    reinterpret_cast<__half *>(valh));
+  k_sqrt_diag<<<(unsigned)((nb_rows * 6 + 255) / 256), 256, 0, s>>>(nb_rows * 6, dinv, dh);
 }
 void fixed_unit_diag(hipStream_t s, int64_t nb_rows, const int32_t *rp, const int32_t *ci, double *val) {
   k_fixed_unit_diag<<<nblk(nb_rows * 6), TPB, 0, s>>>(nb_rows, rp, ci, val);

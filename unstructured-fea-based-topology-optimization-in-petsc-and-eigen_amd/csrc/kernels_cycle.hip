@@ -7,6 +7,8 @@
 // prolongation forms the K-cycle's linear combination on the fly, and the
 // last restriction and the dense coarsest solve share one launch.  Row-local epilogues never write the vector that the
 // same launch multiplies, and every reduction keeps a fixed order.
+#include <hip/hip_fp16.h>
+
 #include "uft_internal.h"
 #include "uft_reduce.h"
 
@@ -107,6 +109,40 @@ __global__ __launch_bounds__(TPB) void k_bsr6_op(int64_t nrows, const int32_t *_
   for (int off = LPR / 2; off > 0; off >>= 1) acc += __shfl_down(acc, off, LPR);
   double dv[3] = {0.0, 0.0, 0.0};
   if (l == 0 && row < nrows * 6) row_epi(op, row, acc, u[row], dv);
+  epi_partials(op, dv);
+}
+
+// FP16 blocks of the symmetrically scaled operator D^-1/2 A D^-1/2 (every
+// entry in [-1, 1]; dh = D^1/2 per scalar row, FP32):
+// t_i = dh_i sum_j Ah_ij (dh_j u_j), FP32 arithmetic.
+template <int LPR>
+__global__ __launch_bounds__(TPB) void k_bsr6_op_h(int64_t nrows, const int32_t *__restrict__ rp,
+                                                   const int32_t *__restrict__ ci, const __half2 *__restrict__ valh,
+                                                   const float *__restrict__ dh, const float *__restrict__ u,
+                                                   RowOp<float> op) {
+  const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  const int64_t row = t / LPR;
+  const int l = (int)(t - row * LPR);
+  float acc = 0.f;
+  if (row < nrows * 6) {
+    const int64_t A = row / 6;
+    const int i = (int)(row - A * 6);
+    for (int k = rp[A] + l; k < rp[A + 1]; k += LPR) {
+      const int64_t B = ci[k];
+      const __half2 *v = valh + (size_t)k * 18 + i * 3;
+      const float *xb = u + B * 6;
+      const float *db = dh + B * 6;
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        const float2 a = __half22float2(v[j]);
+        acc += a.x * (xb[2 * j] * db[2 * j]) + a.y * (xb[2 * j + 1] * db[2 * j + 1]);
+      }
+    }
+  }
+#pragma unroll
+  for (int off = LPR / 2; off > 0; off >>= 1) acc += __shfl_down(acc, off, LPR);
+  double dv[3] = {0.0, 0.0, 0.0};
+  if (l == 0 && row < nrows * 6) row_epi(op, row, acc * dh[row], u[row], dv);
   epi_partials(op, dv);
 }
 
@@ -337,6 +373,14 @@ void bsr6_op(hipStream_t s, int64_t nb_rows, const int32_t *rp, const int32_t *c
   bsr6_op_t<float>(s, nb_rows, rp, ci, val, u, op);
 }
 int bsr6_op_blocks(int64_t nb_rows) { return (int)nblk(nb_rows * 6 * lpr_for(nb_rows)); }
+void bsr6_op_h(hipStream_t s, int64_t nb_rows, const int32_t *rp, const int32_t *ci, const uint16_t *valh,
+               const float *dh, const float *u, const RowOp<float> &op) {
+  const __half2 *vh = reinterpret_cast<const __half2 *>(valh);
+  if (lpr_for(nb_rows) == 32)
+    k_bsr6_op_h<32><<<nblk(nb_rows * 6 * 32), TPB, 0, s>>>(nb_rows, rp, ci, vh, dh, u, op);
+  else
+    k_bsr6_op_h<8><<<nblk(nb_rows * 6 * 8), TPB, 0, s>>>(nb_rows, rp, ci, vh, dh, u, op);
+}
 
 void gather_op(hipStream_t s, const uft_ctx *C, const double *u, const double *Ye, const RowOp<double> &op) {
   const int nn = (int)C->nn, ne = (int)C->ne;

@@ -276,7 +276,10 @@ void lev_op(uft_ctx *C, int l, const float *u, const RowOp<float> &op) {
     if (ev) (void)hipEventRecord(ev[2], C->s);
   } else {
     const AmgLevel &Lv = C->amg.L[l];
-    uftk::bsr6_op(C->s, Lv.nunit, Lv.d_rp, Lv.d_ci, Lv.d_valf, u, op);
+    if (Lv.d_valh)
+      uftk::bsr6_op_h(C->s, Lv.nunit, Lv.d_rp, Lv.d_ci, Lv.d_valh, Lv.d_dh, u, op);
+    else
+      uftk::bsr6_op(C->s, Lv.nunit, Lv.d_rp, Lv.d_ci, Lv.d_valf, u, op);
   }
 }
 int lev_op_blocks(const uft_ctx *C, int l) {
@@ -823,6 +826,8 @@ int amg_refresh(uft_ctx *C, const double *xphys) {
     uftk::fixed_unit_diag(s, Ln.nunit, Ln.d_rp, Ln.d_ci, Ln.d_val);
     uftk::bsr6_dinv(s, Ln.nunit, Ln.d_rp, Ln.d_ci, Ln.d_val, Ln.d_dinv, Ln.d_dinvf);
     if (Ln.d_valf) uftk::d2f(s, Ln.nblk * 36, Ln.d_val, Ln.d_valf);
+    if (Ln.d_valh)
+      uftk::bsr6_to_half(s, Ln.nblk, Ln.nunit, Ln.d_brow, Ln.d_ci, Ln.d_val, Ln.d_dinv, Ln.d_valh, Ln.d_dh);
   }
   AmgLevel &Lc = H.L[H.nlev - 1];
   if (C->bt.nseg) {
@@ -1764,6 +1769,10 @@ int uft_setup(uft_ctx *C) {
       }
       if (is_mixed(C))
         if ((rc = dalloc_n(C, &Ln.d_valf, Ln.nblk * 36))) return rc;
+      if (C->P.precision == UFT_MIXED_H && l == 0) {  // the first coarse level's smoother blocks in FP16
+        if ((rc = dalloc_n(C, &Ln.d_valh, Ln.nblk * 36))) return rc;
+        if ((rc = dalloc_n(C, &Ln.d_dh, Ln.n))) return rc;
+      }
       UFT_HIP(C, hipStreamSynchronize(s));  // host vectors of H go out of scope level by level
     }
     A.ncoarse = A.L[A.nlev - 1].n;
@@ -2501,6 +2510,42 @@ extern "C" int uft_apply_K_cases(uft_ctx *C, int kind, const double *u, double *
   for (int k = 0; k < M.ncase; ++k)
     for (int64_t i = 0; i < n; ++i) y[(size_t)k * n + i] = yi[(size_t)i * nr + k];
   return UFT_OK;
+}
+
+extern "C" int uft_level_apply(uft_ctx *C, int level, int kind, const double *x, double *y) {
+  if (int rc = check_setup(C)) return rc;
+  if (!x || !y || level < 0 || level >= C->amg.nlev || kind < 0 || kind > 2)
+    return uft_set_error(C, UFT_ERR_ARG, "uft_level_apply: bad arguments");
+  AmgLevel &Lv = C->amg.L[level];
+  if (kind > 0 && !is_mixed(C)) return uft_set_error(C, UFT_ERR_ARG, "uft_level_apply: FP64 form");
+  if (kind == 2 && !(level == 0 ? (void *)C->d_KEh : (void *)Lv.d_valh))
+    return uft_set_error(C, UFT_ERR_ARG, "uft_level_apply: no FP16 operator on level %d", level);
+  if (C->refreshes == 0) return uft_set_error(C, UFT_ERR_STATE, "uft_level_apply: no hierarchy refresh yet");
+  hipStream_t s = C->s;
+  const int64_t n = Lv.n;
+  UFT_HIP(C, hipMemcpyAsync(Lv.x, x, n * sizeof(double), hipMemcpyHostToDevice, s));
+  if (kind == 0) {
+    level_apply(C, level, Lv.x, Lv.t);
+  } else {
+    uftk::d2f(s, n, Lv.x, Lv.xf);
+    RowOp<float> op;
+    op.mode = ROW_STORE;
+    op.y = Lv.tf;
+    if (level == 0) {
+      if (kind == 2)
+        uftk::elem_phase_h(s, C, C->elem_variant_h, C->cur_xphys, Lv.xf, C->d_Yef);
+      else
+        uftk::elem_phase_f(s, C, C->elem_variant_f, C->cur_xphys, Lv.xf, C->d_Yef);
+      uftk::gather_op(s, C, Lv.xf, C->d_Yef, op);
+    } else if (kind == 2) {
+      uftk::bsr6_op_h(s, Lv.nunit, Lv.d_rp, Lv.d_ci, Lv.d_valh, Lv.d_dh, Lv.xf, op);
+    } else {
+      uftk::bsr6_op(s, Lv.nunit, Lv.d_rp, Lv.d_ci, Lv.d_valf, Lv.xf, op);
+    }
+    uftk::f2d(s, n, Lv.tf, Lv.t);
+  }
+  UFT_HIP(C, hipMemcpyAsync(y, Lv.t, n * sizeof(double), hipMemcpyDeviceToHost, s));
+  return sync(C);
 }
 
 extern "C" int uft_load_case_u(uft_ctx *C, int k, double *host) {

@@ -76,6 +76,8 @@ struct AmgLevel {
   int32_t *d_brow = nullptr;                 // block -> block row (level >= 1)
   double *d_val = nullptr;                   // nblk * 36
   float *d_valf = nullptr;
+  uint16_t *d_valh = nullptr;                // FP16 D^-1/2 A D^-1/2 (UFT_MIXED_H, level 1)
+  float *d_dh = nullptr;                     // D^1/2 per scalar row (with d_valh)
   double *d_dinv = nullptr;
   float *d_dinvf = nullptr;
   double lmax = 2.0;
@@ -453,6 +455,10 @@ void bsr6_op(hipStream_t s, int64_t nb_rows, const int32_t *rp, const int32_t *c
 void bsr6_op(hipStream_t s, int64_t nb_rows, const int32_t *rp, const int32_t *ci, const float *val, const float *u,
              const RowOp<float> &op);
 int bsr6_op_blocks(int64_t nb_rows);
+void bsr6_op_h(hipStream_t s, int64_t nb_rows, const int32_t *rp, const int32_t *ci, const uint16_t *valh,
+               const float *dh, const float *u, const RowOp<float> &op);
+void bsr6_to_half(hipStream_t s, int64_t nblk, int64_t nb_rows, const int32_t *brow, const int32_t *ci,
+                  const double *val, const double *dinv, uint16_t *valh, float *dh);
 void gather_op(hipStream_t s, const uft_ctx *C, const double *u, const double *Ye, const RowOp<double> &op);
 void gather_op(hipStream_t s, const uft_ctx *C, const float *u, const float *Ye, const RowOp<float> &op);
 int gather_op_blocks(int64_t nn);

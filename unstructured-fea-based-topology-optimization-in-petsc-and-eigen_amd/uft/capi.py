@@ -122,6 +122,7 @@ def lib():
         "uft_autotune": (C.c_int, [vp, C.c_int, pi, pd, C.c_int]),
         "uft_batch_set": (C.c_int, [vp, C.c_int, lp, lp, dp, dp]),
         "uft_autotune_smoother": (C.c_int, [vp, pi, pi, pd]),
+        "uft_level_apply": (C.c_int, [vp, C.c_int, C.c_int, dp, dp]),
         "uft_batch_results": (C.c_int, [vp, pd, pd, pd, pi, pd, pi]),
     }
     for name, (res, args) in sig.items():
@@ -461,6 +462,12 @@ class Context:
         inv = np.zeros(n * n)
         self._chk(self.L.uft_amg_coarse_inverse_get(self.h, inv), "uft_amg_coarse_inverse_get")
         return inv.reshape(n, n)
+
+    def level_apply(self, level, x, kind=0):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros_like(x)
+        self._chk(self.L.uft_level_apply(self.h, level, kind, x, y), "uft_level_apply")
+        return y
 
     def phase_times(self):
         ms = (C.c_double * 8)()
