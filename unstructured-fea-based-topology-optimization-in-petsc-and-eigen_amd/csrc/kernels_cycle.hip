@@ -114,35 +114,62 @@ __global__ __launch_bounds__(TPB) void k_bsr6_op(int64_t nrows, const int32_t *_
 
 // FP16 blocks of the symmetrically scaled operator D^-1/2 A D^-1/2 (every
 // entry in [-1, 1]; dh = D^1/2 per scalar row, FP32):
-// t_i = dh_i sum_j Ah_ij (dh_j u_j), FP32 arithmetic.
+// t_i = dh_i sum_j Ah_ij (dh_j u_j), FP32 arithmetic.  LPR lanes per block
+// row; a lane reads whole 72-byte blocks (nine 8-byte loads) and the six
+// scaled entries of u once per block, accumulating all six rows; a fixed
+// butterfly combines the lanes and six lanes finish the six rows.
+inline int lpr_h(int64_t nb_rows) { return nb_rows < 4096 ? 32 : 16; }
 template <int LPR>
-__global__ __launch_bounds__(TPB) void k_bsr6_op_h(int64_t nrows, const int32_t *__restrict__ rp,
-                                                   const int32_t *__restrict__ ci, const __half2 *__restrict__ valh,
+__global__ __launch_bounds__(TPB) void k_bsr6_op_h(int64_t nbrows, const int32_t *__restrict__ rp,
+                                                   const int32_t *__restrict__ ci, const uint2 *__restrict__ valh,
                                                    const float *__restrict__ dh, const float *__restrict__ u,
                                                    RowOp<float> op) {
   const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
-  const int64_t row = t / LPR;
-  const int l = (int)(t - row * LPR);
-  float acc = 0.f;
-  if (row < nrows * 6) {
-    const int64_t A = row / 6;
-    const int i = (int)(row - A * 6);
+  const int64_t A = t / LPR;
+  const int l = (int)(t - A * LPR);
+  float acc[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  if (A < nbrows) {
     for (int k = rp[A] + l; k < rp[A + 1]; k += LPR) {
       const int64_t B = ci[k];
-      const __half2 *v = valh + (size_t)k * 18 + i * 3;
-      const float *xb = u + B * 6;
-      const float *db = dh + B * 6;
+      const float2 *ub = reinterpret_cast<const float2 *>(u + B * 6);
+      const float2 *db = reinterpret_cast<const float2 *>(dh + B * 6);
+      float xs[6];
 #pragma unroll
       for (int j = 0; j < 3; ++j) {
-        const float2 a = __half22float2(v[j]);
-        acc += a.x * (xb[2 * j] * db[2 * j]) + a.y * (xb[2 * j + 1] * db[2 * j + 1]);
+        const float2 a = ub[j], b = db[j];
+        xs[2 * j] = a.x * b.x;
+        xs[2 * j + 1] = a.y * b.y;
+      }
+      const uint2 *vb = valh + (size_t)k * 9;
+#pragma unroll
+      for (int w = 0; w < 9; ++w) {
+        const uint2 h = vb[w];
+        const unsigned hw[2] = {h.x, h.y};
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+          const float2 f = __half22float2(*reinterpret_cast<const __half2 *>(&hw[q]));
+          const int e = 4 * w + 2 * q;  // entries e, e+1 of the row-major block (same row: 6 is even)
+          acc[e / 6] += f.x * xs[e % 6] + f.y * xs[e % 6 + 1];
+        }
       }
     }
   }
+  // butterfly: every lane of the group ends with the six sums (each lane in
+  // its own fixed order); lane i < 6 finishes row 6A + i
 #pragma unroll
-  for (int off = LPR / 2; off > 0; off >>= 1) acc += __shfl_down(acc, off, LPR);
+  for (int i = 0; i < 6; ++i) {
+#pragma unroll
+    for (int off = LPR / 2; off > 0; off >>= 1) acc[i] += __shfl_xor(acc[i], off, LPR);
+  }
   double dv[3] = {0.0, 0.0, 0.0};
-  if (l == 0 && row < nrows * 6) row_epi(op, row, acc * dh[row], u[row], dv);
+  if (l < 6 && A < nbrows) {
+    float ti = acc[0];
+#pragma unroll
+    for (int i = 1; i < 6; ++i)
+      if (l == i) ti = acc[i];
+    const int64_t row = A * 6 + l;
+    row_epi(op, row, ti * dh[row], u[row], dv);
+  }
   epi_partials(op, dv);
 }
 
@@ -298,17 +325,23 @@ __global__ __launch_bounds__(TPB) void k_final_kc(const double *__restrict__ par
 // read by columns (loads coalesced over rows; four column groups added in a
 // fixed order).  The prolongation x += P xc follows as an ordinary launch.
 constexpr int TAIL_TPB = 1024;
-template <typename T>
+constexpr int TAIL_CH = 512;  // children staged in LDS per pass
+// xc = Ainv P^T r on the coarsest level, one workgroup (a design batch: one
+// per design, units [u0, u0 + nu) with its inverse block at Ainv + coff).
+// Restriction: one thread per child computes P_u^T r_u, staged in LDS; one
+// thread per coarse unknown then adds its unit's children in child order
+// (chunks of TAIL_CH children, in order).  Dense product: four groups of
+// threads take every fourth column, each with four independent
+// accumulators; fixed summation order throughout.
+template <typename T, typename TA>
 __global__ __launch_bounds__(TAIL_TPB) void k_tail_solve(int nc_units, const int32_t *__restrict__ cptr,
                                                          const int32_t *__restrict__ child, const T *__restrict__ P,
-                                                         const double *__restrict__ Ainv, const T *__restrict__ r,
+                                                         const TA *__restrict__ Ainv, const T *__restrict__ r,
                                                          T *__restrict__ xc, const int32_t *__restrict__ cu,
                                                          const int64_t *__restrict__ coff) {
   __shared__ double bc[UFT_TAIL_MAX];
+  __shared__ double cv[TAIL_CH * 6];
   __shared__ double part4[4][TAIL_TPB / 4];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = TAIL_TPB / 64;
-  // a design batch solves its block-diagonal coarsest level one design per
-  // workgroup: units [u0, u0 + nu) with the inverse block at Ainv + coff
   int u0 = 0;
   if (cu) {
     u0 = cu[blockIdx.x];
@@ -317,35 +350,48 @@ __global__ __launch_bounds__(TAIL_TPB) void k_tail_solve(int nc_units, const int
     xc += (size_t)u0 * 6;
   }
   const int n = 6 * nc_units;
-  for (int A = w; A < nc_units; A += nw) {
-    const int k0 = cptr[u0 + A], npair = (cptr[u0 + A + 1] - k0) * 6;
-    double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    for (int q = lane; q < npair; q += 64) {
-      const int kk = q / 6, i = q - kk * 6;
-      const int64_t u = child[k0 + kk];
-      const T *Pu = P + (size_t)u * 36 + i * 6;
-      const double ri = (double)r[u * 6 + i];
+  for (int t = threadIdx.x; t < n; t += TAIL_TPB) bc[t] = 0.0;
+  const int kb = cptr[u0], ke = cptr[u0 + nc_units];
+  for (int c0 = kb; c0 < ke; c0 += TAIL_CH) {
+    const int c1 = min(c0 + TAIL_CH, ke);
+    __syncthreads();  // bc cleared / the previous chunk consumed
+    for (int k = c0 + (int)threadIdx.x; k < c1; k += TAIL_TPB) {
+      const int64_t u = child[k];
+      const T *Pu = P + (size_t)u * 36;
+      const T *ru = r + u * 6;
+      double v[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-      for (int j = 0; j < 6; ++j) acc[j] += (double)Pu[j] * ri;
+      for (int i = 0; i < 6; ++i) {
+        const double ri = (double)ru[i];
+#pragma unroll
+        for (int j = 0; j < 6; ++j) v[j] += (double)Pu[i * 6 + j] * ri;
+      }
+#pragma unroll
+      for (int j = 0; j < 6; ++j) cv[(k - c0) * 6 + j] = v[j];
     }
-#pragma unroll
-    for (int j = 0; j < 6; ++j) {
-#pragma unroll
-      for (int off = 32; off > 0; off >>= 1) acc[j] += __shfl_down(acc[j], off, 64);
-    }
-    if (lane == 0) {
-#pragma unroll
-      for (int j = 0; j < 6; ++j) bc[A * 6 + j] = acc[j];
+    __syncthreads();
+    for (int t = threadIdx.x; t < n; t += TAIL_TPB) {
+      const int A = t / 6, j = t - A * 6;
+      const int a0 = max(cptr[u0 + A], c0), a1 = min(cptr[u0 + A + 1], c1);
+      double s = bc[t];
+      for (int k = a0; k < a1; ++k) s += cv[(k - c0) * 6 + j];
+      bc[t] = s;
     }
   }
   __syncthreads();
   const int g = threadIdx.x / (TAIL_TPB / 4), row0 = threadIdx.x % (TAIL_TPB / 4);
   for (int rb = 0; rb < n; rb += TAIL_TPB / 4) {
     const int row = rb + row0;
-    double acc = 0.0;
-    if (row < n)
-      for (int c = g; c < n; c += 4) acc += Ainv[(size_t)c * n + row] * bc[c];
-    part4[g][row0] = acc;
+    double a[4] = {0.0, 0.0, 0.0, 0.0};
+    if (row < n) {
+      int c = g;
+      for (; c + 12 < n; c += 16) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) a[q] += (double)Ainv[(size_t)(c + 4 * q) * n + row] * bc[c + 4 * q];
+      }
+      for (; c < n; c += 4) a[0] += (double)Ainv[(size_t)c * n + row] * bc[c];
+    }
+    part4[g][row0] = (a[0] + a[1]) + (a[2] + a[3]);
     __syncthreads();
     if (g == 0 && row < n) xc[row] = (T)((part4[0][row0] + part4[1][row0]) + (part4[2][row0] + part4[3][row0]));
     __syncthreads();
@@ -375,12 +421,13 @@ void bsr6_op(hipStream_t s, int64_t nb_rows, const int32_t *rp, const int32_t *c
 int bsr6_op_blocks(int64_t nb_rows) { return (int)nblk(nb_rows * 6 * lpr_for(nb_rows)); }
 void bsr6_op_h(hipStream_t s, int64_t nb_rows, const int32_t *rp, const int32_t *ci, const uint16_t *valh,
                const float *dh, const float *u, const RowOp<float> &op) {
-  const __half2 *vh = reinterpret_cast<const __half2 *>(valh);
-  if (lpr_for(nb_rows) == 32)
-    k_bsr6_op_h<32><<<nblk(nb_rows * 6 * 32), TPB, 0, s>>>(nb_rows, rp, ci, vh, dh, u, op);
+  const uint2 *vh = reinterpret_cast<const uint2 *>(valh);
+  if (lpr_h(nb_rows) == 32)
+    k_bsr6_op_h<32><<<nblk(nb_rows * 32), TPB, 0, s>>>(nb_rows, rp, ci, vh, dh, u, op);
   else
-    k_bsr6_op_h<8><<<nblk(nb_rows * 6 * 8), TPB, 0, s>>>(nb_rows, rp, ci, vh, dh, u, op);
+    k_bsr6_op_h<16><<<nblk(nb_rows * 16), TPB, 0, s>>>(nb_rows, rp, ci, vh, dh, u, op);
 }
+int bsr6_op_h_blocks(int64_t nb_rows) { return (int)nblk(nb_rows * lpr_h(nb_rows)); }
 
 void gather_op(hipStream_t s, const uft_ctx *C, const double *u, const double *Ye, const RowOp<double> &op) {
   const int nn = (int)C->nn, ne = (int)C->ne;
@@ -446,11 +493,15 @@ void final_kc2(hipStream_t s, const double *part, int nb, double *sc) { k_final_
 
 void tail_solve(hipStream_t s, int nc_units, const int32_t *cptr, const int32_t *child, const double *P,
                 const double *Ainv, const double *r, double *xc, int nseg, const int32_t *cu, const int64_t *coff) {
-  k_tail_solve<double><<<nseg, TAIL_TPB, 0, s>>>(nc_units, cptr, child, P, Ainv, r, xc, cu, coff);
+  k_tail_solve<double, double><<<nseg, TAIL_TPB, 0, s>>>(nc_units, cptr, child, P, Ainv, r, xc, cu, coff);
 }
 void tail_solve(hipStream_t s, int nc_units, const int32_t *cptr, const int32_t *child, const float *P,
                 const double *Ainv, const float *r, float *xc, int nseg, const int32_t *cu, const int64_t *coff) {
-  k_tail_solve<float><<<nseg, TAIL_TPB, 0, s>>>(nc_units, cptr, child, P, Ainv, r, xc, cu, coff);
+  k_tail_solve<float, double><<<nseg, TAIL_TPB, 0, s>>>(nc_units, cptr, child, P, Ainv, r, xc, cu, coff);
+}
+void tail_solve(hipStream_t s, int nc_units, const int32_t *cptr, const int32_t *child, const float *P,
+                const float *Ainv, const float *r, float *xc) {
+  k_tail_solve<float, float><<<1, TAIL_TPB, 0, s>>>(nc_units, cptr, child, P, Ainv, r, xc, nullptr, nullptr);
 }
 
 }  // namespace uftk

@@ -140,9 +140,22 @@ __global__ void k_beta_flex(double *sc, int irz, int irzo, int iprev, int ibeta)
   sc[ibeta] = (sc[irz] - sc[irzo]) / sc[iprev];
   sc[iprev] = sc[irz];
 }
+
+// x_i uniform in [-1, 1) from a SplitMix64 hash of the index (a fixed,
+// non-smooth start vector for the eigenvalue estimates)
+__global__ __launch_bounds__(TPB) void k_fill_hash(int64_t n, double *__restrict__ x) {
+  const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  if (i >= n) return;
+  uint64_t z = (uint64_t)i * 0x9E3779B97F4A7C15ull + 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  x[i] = (double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+}
 }  // namespace
 
 namespace uftk {
+void fill_hash(hipStream_t s, int64_t n, double *x) { k_fill_hash<<<nblk(n), TPB, 0, s>>>(n, x); }
 void f2d(hipStream_t s, int64_t n, const float *a, double *b) { k_f2d<<<nblk(n), TPB, 0, s>>>(n, a, b); }
 void dot_pr(hipStream_t s, int64_t n, const double *r, const double *z, const double *zo, double *part, double *out) {
   k_dot_pr<<<nblk(n), TPB, 0, s>>>(n, r, z, zo, part, out);

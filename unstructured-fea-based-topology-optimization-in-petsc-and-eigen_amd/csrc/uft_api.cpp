@@ -282,8 +282,11 @@ void lev_op(uft_ctx *C, int l, const float *u, const RowOp<float> &op) {
       uftk::bsr6_op(C->s, Lv.nunit, Lv.d_rp, Lv.d_ci, Lv.d_valf, u, op);
   }
 }
-int lev_op_blocks(const uft_ctx *C, int l) {
-  return l == 0 ? uftk::gather_op_blocks(C->nn) : uftk::bsr6_op_blocks(C->amg.L[l].nunit);
+// blocks of the level-l operator launch (its block partials), per vector type
+int lev_op_blocks(const uft_ctx *C, int l, bool f32) {
+  if (l == 0) return uftk::gather_op_blocks(C->nn);
+  const AmgLevel &Lv = C->amg.L[l];
+  return f32 && Lv.d_valh ? uftk::bsr6_op_h_blocks(Lv.nunit) : uftk::bsr6_op_blocks(Lv.nunit);
 }
 
 // How a coarse level l >= 1 is solved inside the cycle of level l-1:
@@ -331,7 +334,7 @@ void coarse_body(uft_ctx *C, int l) {
     op.w1 = v.b;
     op.part = C->d_part;
     lev_op(C, l, v.kc, op);
-    uftk::final_kc1(C->s, C->d_part, lev_op_blocks(C, l), sc);
+    uftk::final_kc1(C->s, C->d_part, lev_op_blocks(C, l, std::is_same<T, float>::value), sc);
     uftk::lincomb_first(C->s, Lv.n, sc, v.b, v.kv, v.kr, v.r, v.dinv, it, v.d, v.kd);
     cycle_body<T>(C, l, v.kr, v.kd);
     RowOp<T> o3;
@@ -340,7 +343,7 @@ void coarse_body(uft_ctx *C, int l) {
     o3.w2 = v.kr;
     o3.part = C->d_part;
     lev_op(C, l, v.kd, o3);
-    uftk::final_kc2(C->s, C->d_part, lev_op_blocks(C, l), sc);
+    uftk::final_kc2(C->s, C->d_part, lev_op_blocks(C, l, std::is_same<T, float>::value), sc);
   }
 }
 
@@ -379,7 +382,10 @@ void coarse_direct(uft_ctx *C, int l, const T *r, T *x) {
     return;
   }
   if (Lv.bs == 6 && H.ncoarse <= UFT_TAIL_MAX) {
-    uftk::tail_solve(C->s, (int)Ln.nunit, Lv.d_cptr, Lv.d_child, P, H.d_cinv, r, vn.x);
+    if constexpr (std::is_same<T, float>::value)  // FP32 inverse inside the FP32 cycle
+      uftk::tail_solve(C->s, (int)Ln.nunit, Lv.d_cptr, Lv.d_child, P, H.d_cinvf, r, vn.x);
+    else
+      uftk::tail_solve(C->s, (int)Ln.nunit, Lv.d_cptr, Lv.d_child, P, H.d_cinv, r, vn.x);
     uftk::prolong_comb(C->s, Lv.bs, Lv.nunit, Lv.d_parent, P, nullptr, -1, -1, vn.x, (const T *)nullptr, x);
     return;
   }
@@ -706,8 +712,7 @@ int estimate_lmax(uft_ctx *C, int l, int ksteps) {
   hipStream_t s = C->s;
   const int64_t n = Lv.n;
   double *x = Lv.x, *r = Lv.r, *z = Lv.d, *p = Lv.b, *q = Lv.t;
-  // deterministic start vector: r = 1 + small alternating perturbation
-  uftk::fill(s, n, 1.0, r);
+  uftk::fill_hash(s, n, r);  // fixed pseudo-random start: not close to a near-null (rigid) mode
   uftk::scale(s, n, r, Lv.d_dinv, z);
   uftk::copy(s, n, z, p);
   uftk::dot(s, n, r, z, C->d_part, C->d_scal + S_MISC);  // rz in S_MISC
@@ -780,7 +785,7 @@ int estimate_lmax_batch(uft_ctx *C, int l, int ksteps) {
   const int ns = B.nseg;
   const int64_t n = Lv.n;
   double *x = Lv.x, *r = Lv.r, *z = Lv.d, *p = Lv.b, *q = Lv.t;
-  uftk::fill(s, n, 1.0, r);
+  uftk::fill_hash(s, n, r);  // fixed pseudo-random start: not close to a near-null (rigid) mode
   uftk::scale(s, n, r, Lv.d_dinv, z);
   uftk::copy(s, n, z, p);
   uftk::fill(s, n, 0.0, x);
@@ -1450,7 +1455,7 @@ void uft_default_params(uft_params *p) {
   p->cycle_type = 2;
   p->cheb_lo = 0.1;
   p->cheb_hi = 1.1;
-  p->esteig_steps = 12;
+  p->esteig_steps = 8;
   p->load_cases = 0;
   p->graph = 0;
 }

@@ -457,6 +457,7 @@ void bsr6_op(hipStream_t s, int64_t nb_rows, const int32_t *rp, const int32_t *c
 int bsr6_op_blocks(int64_t nb_rows);
 void bsr6_op_h(hipStream_t s, int64_t nb_rows, const int32_t *rp, const int32_t *ci, const uint16_t *valh,
                const float *dh, const float *u, const RowOp<float> &op);
+int bsr6_op_h_blocks(int64_t nb_rows);
 void bsr6_to_half(hipStream_t s, int64_t nblk, int64_t nb_rows, const int32_t *brow, const int32_t *ci,
                   const double *val, const double *dinv, uint16_t *valh, float *dh);
 void gather_op(hipStream_t s, const uft_ctx *C, const double *u, const double *Ye, const RowOp<double> &op);
@@ -485,6 +486,9 @@ void tail_solve(hipStream_t s, int nc_units, const int32_t *cptr, const int32_t 
 void tail_solve(hipStream_t s, int nc_units, const int32_t *cptr, const int32_t *child, const float *P,
                 const double *Ainv, const float *r, float *xc, int nseg = 1, const int32_t *cu = nullptr,
                 const int64_t *coff = nullptr);
+// the same with an FP32 copy of the inverse (single design, mixed forms)
+void tail_solve(hipStream_t s, int nc_units, const int32_t *cptr, const int32_t *child, const float *P,
+                const float *Ainv, const float *r, float *xc);
 void fixed_unit_diag(hipStream_t s, int64_t nb_rows, const int32_t *rp, const int32_t *ci, double *val);
 // MMA
 void mma_movelimit(hipStream_t s, int64_t n, const double *x, double Xmin, double Xmax, double movlim,
@@ -511,6 +515,7 @@ void sc_div(hipStream_t s, double *sc, int a, int b, int dst);
 void sc_beta(hipStream_t s, double *sc, int inew, int iold, int ibeta, double *log2);
 void sc_copy(hipStream_t s, const double *src, double *dst);
 void f2d(hipStream_t s, int64_t n, const float *a, double *b);
+void fill_hash(hipStream_t s, int64_t n, double *x);
 void dot_pr(hipStream_t s, int64_t n, const double *r, const double *z, const double *zo, double *part, double *out);
 void sc_beta_flex(hipStream_t s, double *sc, int irz, int irzo, int iprev, int ibeta);
 void lincomb(hipStream_t s, int64_t n, const double *sc, int ia, const double *a, int ib, const double *b, double *y);
