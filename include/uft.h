@@ -78,6 +78,8 @@ typedef struct uft_params {
                           refreshed); 0 (default): launched kernel by kernel.  Same results either way.
                           While the device timers are on (uft_profile_enable) the solve launches kernel by
                           kernel. */
+  int cheb_degree_coarse; /* Chebyshev degree on multigrid levels 2 and below (their applies are bound by
+                             launch latency, not by bandwidth); 0 (default): cheb_degree */
 } uft_params;
 
 /* --- context ------------------------------------------------------------ */

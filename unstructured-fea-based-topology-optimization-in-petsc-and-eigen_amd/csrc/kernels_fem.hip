@@ -499,9 +499,9 @@ void k_elem_apply_v(int ne, const int32_t *__restrict__ elem, const T2 *__restri
 // entry) in groups of four halves, [NG][ne] x 8 bytes; FP32 arithmetic with
 // E(x_e) s_e applied at the end.  Entries past the triangle are zero.
 template <int NPE>
-__global__ __launch_bounds__(TPB) void k_ke_to_half(int ne, const double2 *__restrict__ KE, uint2 *__restrict__ KEh,
+__global__ __launch_bounds__(TPB) void k_ke_to_half(int ne, const double2 *__restrict__ KE, uint4 *__restrict__ KEh,
                                                     float *__restrict__ KEs) {
-  constexpr int ND = 3 * NPE, NP = ND * (ND + 1) / 2, NQ = NP / 2, NG = (NP + 3) / 4;
+  constexpr int ND = 3 * NPE, NP = ND * (ND + 1) / 2, NQ = NP / 2, NG = (NP + 7) / 8;
   const int e = blockIdx.x * TPB + threadIdx.x;
   if (e >= ne) return;
   double mx = 0.0;
@@ -512,32 +512,36 @@ __global__ __launch_bounds__(TPB) void k_ke_to_half(int ne, const double2 *__res
   const double sc = mx > 0.0 ? mx : 1.0;
   KEs[e] = (float)sc;
   for (int g = 0; g < NG; ++g) {
-    __half h[4];
+    unsigned w[4];
     for (int j = 0; j < 4; ++j) {
-      const int p = 4 * g + j;
-      double v = 0.0;
-      if (p < NP) {
-        const double2 pr = KE[(size_t)(p >> 1) * ne + e];
-        v = (p & 1) ? pr.y : pr.x;
+      unsigned short hb[2];
+      for (int t = 0; t < 2; ++t) {
+        const int p = 8 * g + 2 * j + t;
+        double v = 0.0;
+        if (p < NP) {
+          const double2 pr = KE[(size_t)(p >> 1) * ne + e];
+          v = (p & 1) ? pr.y : pr.x;
+        }
+        hb[t] = __half_as_ushort(__float2half_rn((float)(v / sc)));
       }
-      h[j] = __float2half_rn((float)(v / sc));
+      w[j] = (unsigned)hb[0] | ((unsigned)hb[1] << 16);
     }
-    uint2 w;
-    w.x = (unsigned)__half_as_ushort(h[0]) | ((unsigned)__half_as_ushort(h[1]) << 16);
-    w.y = (unsigned)__half_as_ushort(h[2]) | ((unsigned)__half_as_ushort(h[3]) << 16);
-    KEh[(size_t)g * ne + e] = w;
+    KEh[(size_t)g * ne + e] = make_uint4(w[0], w[1], w[2], w[3]);
   }
 }
 
 __device__ __forceinline__ float h_lo(unsigned w) { return __half2float(__ushort_as_half((unsigned short)(w & 0xffffu))); }
 __device__ __forceinline__ float h_hi(unsigned w) { return __half2float(__ushort_as_half((unsigned short)(w >> 16))); }
 
+// FP16 element phase: one 16-byte load moves eight packed entries, so a
+// wavefront load instruction reads 1 KB contiguous (as the FP64 kernel's
+// double2 loads do).
 template <int NPE, int BT, bool NT, int WPE>
 __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1)))
-void k_elem_apply_h(int ne, const int32_t *__restrict__ elem, const uint2 *__restrict__ KEh,
+void k_elem_apply_h(int ne, const int32_t *__restrict__ elem, const uint4 *__restrict__ KEh,
                     const float *__restrict__ KEs, const double *__restrict__ xphys, SimpCoef sc,
                     const uint8_t *__restrict__ fixm, const float *__restrict__ u, float *__restrict__ Ye) {
-  constexpr int ND = 3 * NPE, NP = ND * (ND + 1) / 2, NG = (NP + 3) / 4;
+  constexpr int ND = 3 * NPE, NP = ND * (ND + 1) / 2, NG = (NP + 7) / 8;
   const int e = blockIdx.x * BT + threadIdx.x;
   if (e >= ne) return;
   float ue[ND], ye[ND];
@@ -555,18 +559,20 @@ void k_elem_apply_h(int ne, const int32_t *__restrict__ elem, const uint2 *__res
   for (int k = 0; k < ND; ++k) ye[k] = 0.f;
   static_for<0, NG>([&](auto gc) {
     constexpr int g = decltype(gc)::value;
-    uint2 w;
+    unsigned w[4];
     if constexpr (NT) {
-      typedef unsigned v2u __attribute__((ext_vector_type(2)));
-      const v2u v = __builtin_nontemporal_load(reinterpret_cast<const v2u *>(&KEh[(size_t)g * ne + e]));
-      w = make_uint2(v.x, v.y);
+      typedef unsigned v4u __attribute__((ext_vector_type(4)));
+      const v4u v = __builtin_nontemporal_load(reinterpret_cast<const v4u *>(&KEh[(size_t)g * ne + e]));
+      w[0] = v.x; w[1] = v.y; w[2] = v.z; w[3] = v.w;
     } else {
-      w = KEh[(size_t)g * ne + e];
+      const uint4 v = KEh[(size_t)g * ne + e];
+      w[0] = v.x; w[1] = v.y; w[2] = v.z; w[3] = v.w;
     }
-    kacc<pk_row(4 * g, ND), pk_col(4 * g, ND)>(h_lo(w.x), ue, ye);
-    if constexpr (4 * g + 1 < NP) kacc<pk_row(4 * g + 1, ND), pk_col(4 * g + 1, ND)>(h_hi(w.x), ue, ye);
-    if constexpr (4 * g + 2 < NP) kacc<pk_row(4 * g + 2, ND), pk_col(4 * g + 2, ND)>(h_lo(w.y), ue, ye);
-    if constexpr (4 * g + 3 < NP) kacc<pk_row(4 * g + 3, ND), pk_col(4 * g + 3, ND)>(h_hi(w.y), ue, ye);
+    static_for<0, 8>([&](auto tc) {
+      constexpr int t = decltype(tc)::value;
+      constexpr int p = 8 * g + t;
+      if constexpr (p < NP) kacc<pk_row(p, ND), pk_col(p, ND)>((t & 1) ? h_hi(w[t >> 1]) : h_lo(w[t >> 1]), ue, ye);
+    });
   });
   const float E = (float)simpE(sc, xphys[e]) * KEs[e];
 #pragma unroll
@@ -1098,7 +1104,7 @@ void elem_phase_f(hipStream_t s, const uft_ctx *C, int variant, const double *xp
 }
 
 template <int NPE>
-static void elem_phase_h_t(hipStream_t s, int variant, int ne, const int32_t *elem, const uint2 *KEh, const float *KEs,
+static void elem_phase_h_t(hipStream_t s, int variant, int ne, const int32_t *elem, const uint4 *KEh, const float *KEs,
                            const double *xphys, SimpCoef sc, const uint8_t *fixm, const float *u, float *Ye) {
   auto g = [ne](int bt) { return (unsigned)((ne + bt - 1) / bt); };
   switch (variant) {
@@ -1124,9 +1130,9 @@ void ke_to_half(hipStream_t s, const uft_ctx *C) {
   else
     k_ke_to_half<4><<<nblk(C->ne), TPB, 0, s>>>((int)C->ne, (const double2 *)C->d_KE, C->d_KEh, C->d_KEs);
 }
-int ke_half_groups(int npe) {
+int ke_half_groups(int npe) {  // 16-byte groups of eight halves per element
   const int nd = 3 * npe, np = nd * (nd + 1) / 2;
-  return (np + 3) / 4;
+  return (np + 7) / 8;
 }
 
 int red_blocks(int64_t n) { return (int)nblk(n); }

@@ -147,7 +147,9 @@ bool use_graph(const uft_ctx *C) { return C->P.graph && !C->prof; }
 // memset or copy nodes (vector fills and copies are kernels).
 template <typename F>
 int graph_launch(uft_ctx *C, const void *k0, const void *k1, const void *k2, F &&body) {
-  const void *key[4] = {k0, k1, k2, (const void *)(intptr_t)(16 * C->P.cheb_degree + C->P.cycle_type + 1)};
+  const void *key[4] = {k0, k1, k2,
+                        (const void *)(intptr_t)(256 * C->P.cheb_degree_coarse + 16 * C->P.cheb_degree +
+                                                 C->P.cycle_type + 1)};
   // a batch folds its Chebyshev scaling into D^-1, so its cycle's launch
   // arguments do not change with a refresh; a single design's do
   bool same = C->ig_exec && (C->bt.nseg || C->ig_refresh == C->refreshes);
@@ -291,6 +293,13 @@ int lev_op_blocks(const uft_ctx *C, int l, bool f32) {
 
 // How a coarse level l >= 1 is solved inside the cycle of level l-1:
 // 0 one cycle (V), 1 two stationary cycles (W), 2 a two-step K-cycle.
+// Chebyshev degree of level l: cheb_degree, or cheb_degree_coarse (when set)
+// on levels 2 and below, whose applies are launch-latency bound
+int level_degree(const uft_ctx *C, int l) {
+  const int d = (l >= 2 && C->P.cheb_degree_coarse > 0) ? C->P.cheb_degree_coarse : C->P.cheb_degree;
+  return std::max(1, d);
+}
+
 int level_kind(const uft_ctx *C, int l) {
   const int ct = C->P.cycle_type;
   if (ct == 0 || (ct == 3 && l > 1)) return 0;
@@ -309,7 +318,7 @@ void coarse_body(uft_ctx *C, int l) {
   AmgLevel &Lv = C->amg.L[l];
   LV<T> v = lv(Lv, (T *)nullptr);
   const int kind = level_kind(C, l);
-  const double it = cheb_coef(Lv.lmax, std::max(1, C->P.cheb_degree), C->P.cheb_lo / C->P.cheb_hi).inv_theta;
+  const double it = cheb_coef(Lv.lmax, level_degree(C, l), C->P.cheb_lo / C->P.cheb_hi).inv_theta;
   if (kind == 0) {
     cycle_body<T>(C, l, v.b, v.x);
   } else if (kind == 1) {
@@ -409,7 +418,7 @@ void cycle_body(uft_ctx *C, int l, const T *b, T *x) {
   AmgHier &H = C->amg;
   AmgLevel &Lv = H.L[l];
   LV<T> v = lv(Lv, (T *)nullptr);
-  const int deg = std::max(1, C->P.cheb_degree);
+  const int deg = level_degree(C, l);
   const ChebCoef cc = cheb_coef(Lv.lmax, deg, C->P.cheb_lo / C->P.cheb_hi);
   T *dc = v.d, *dn = v.t;
   RowOp<T> st;
@@ -435,7 +444,7 @@ void cycle_body(uft_ctx *C, int l, const T *b, T *x) {
   } else {
     AmgLevel &Ln = H.L[l + 1];
     LV<T> vn = lv(Ln, (T *)nullptr);
-    const double itn = cheb_coef(Ln.lmax, deg, C->P.cheb_lo / C->P.cheb_hi).inv_theta;
+    const double itn = cheb_coef(Ln.lmax, level_degree(C, l + 1), C->P.cheb_lo / C->P.cheb_hi).inv_theta;
     T *xn = level_kind(C, l + 1) == 2 ? vn.kc : vn.x;
     uftk::restrict_first(C->s, Lv.bs, Ln.nunit, Lv.d_cptr, Lv.d_child, tr_P(Lv, (T *)nullptr), v.r, vn.b, vn.dinv,
                          itn, vn.r, vn.d, xn);
@@ -471,7 +480,7 @@ template <typename T>
 void fine_cycle(uft_ctx *C, const T *b, T *x) {
   AmgLevel &L0 = C->amg.L[0];
   LV<T> v = lv(L0, (T *)nullptr);
-  const double it = cheb_coef(L0.lmax, std::max(1, C->P.cheb_degree), C->P.cheb_lo / C->P.cheb_hi).inv_theta;
+  const double it = cheb_coef(L0.lmax, level_degree(C, 0), C->P.cheb_lo / C->P.cheb_hi).inv_theta;
   k_cheb_first(C->s, L0.n, b, v.dinv, it, v.r, v.d, x, 1);
   cycle_body<T>(C, 0, b, x);
 }
@@ -526,7 +535,7 @@ void mr_coarse_body(uft_ctx *C, int l) {
   MLV<T> v = mlv<T>(C, l);
   const int nr = C->mr.nr;
   const int kind = level_kind(C, l);
-  const double it = cheb_coef(Lv.lmax, std::max(1, C->P.cheb_degree), C->P.cheb_lo / C->P.cheb_hi).inv_theta;
+  const double it = cheb_coef(Lv.lmax, level_degree(C, l), C->P.cheb_lo / C->P.cheb_hi).inv_theta;
   if (kind == 0) {
     mr_cycle_body<T>(C, l, v.b, v.x);
   } else if (kind == 1) {
@@ -589,7 +598,7 @@ void mr_cycle_body(uft_ctx *C, int l, const T *b, T *x) {
   AmgLevel &Lv = H.L[l];
   MLV<T> v = mlv<T>(C, l);
   const int nr = C->mr.nr;
-  const int deg = std::max(1, C->P.cheb_degree);
+  const int deg = level_degree(C, l);
   const double lohi = C->P.cheb_lo / C->P.cheb_hi;
   const ChebCoef cc = cheb_coef(Lv.lmax, deg, lohi);
   T *dc = v.d, *dn = v.t;
@@ -620,7 +629,7 @@ void mr_cycle_body(uft_ctx *C, int l, const T *b, T *x) {
   } else {
     AmgLevel &Ln = H.L[l + 1];
     MLV<T> vn = mlv<T>(C, l + 1);
-    const double itn = cheb_coef(Ln.lmax, deg, lohi).inv_theta;
+    const double itn = cheb_coef(Ln.lmax, level_degree(C, l + 1), lohi).inv_theta;
     T *xn = level_kind(C, l + 1) == 2 ? vn.kc : vn.x;
     uftk::mr_restrict_first(C->s, nr, Lv.bs, Ln.nunit, Lv.d_cptr, Lv.d_child, tr_P(Lv, (T *)nullptr), v.r, vn.b,
                             vn.dinv, itn, vn.r, vn.d, xn);
@@ -656,7 +665,7 @@ void mr_fine_cycle(uft_ctx *C, const T *b, T *x) {
   AmgLevel &L0 = C->amg.L[0];
   MLV<T> v = mlv<T>(C, 0);
   const double it =
-      cheb_coef(L0.lmax, std::max(1, C->P.cheb_degree), C->P.cheb_lo / C->P.cheb_hi).inv_theta;
+      cheb_coef(L0.lmax, level_degree(C, 0), C->P.cheb_lo / C->P.cheb_hi).inv_theta;
   uftk::mr_cheb_first(C->s, C->mr.nr, L0.n, b, v.dinv, it, v.r, v.d, x);
   mr_cycle_body<T>(C, 0, b, x);
 }
@@ -1458,6 +1467,7 @@ void uft_default_params(uft_params *p) {
   p->esteig_steps = 8;
   p->load_cases = 0;
   p->graph = 0;
+  p->cheb_degree_coarse = 0;
 }
 
 int uft_create(uft_ctx **out, int device) {
@@ -1503,6 +1513,9 @@ int uft_params_set(uft_ctx *C, const uft_params *p) {
   if (p->load_cases != 0 && p->load_cases != 1)
     return uft_set_error(C, UFT_ERR_ARG, "load_cases must be 0 (superposed) or 1 (separate)");
   if (p->graph != 0 && p->graph != 1) return uft_set_error(C, UFT_ERR_ARG, "graph must be 0 or 1");
+  if (p->cheb_degree < 1 || p->cheb_degree > 16 || p->cheb_degree_coarse < 0 || p->cheb_degree_coarse > 16 ||
+      p->cycle_type < 0 || p->cycle_type > 3)
+    return uft_set_error(C, UFT_ERR_ARG, "Chebyshev degrees must be in [1, 16] (coarse: 0 or [1, 16]), cycle_type in [0, 3]");
   if (p->precision < UFT_FP64 || p->precision > UFT_MIXED_H)
     return uft_set_error(C, UFT_ERR_ARG, "precision must be UFT_FP64, UFT_MIXED or UFT_MIXED_H");
   if (C->is_setup && (p->m != C->P.m || p->rmin != C->P.rmin || p->nu != C->P.nu || p->precision != C->P.precision ||

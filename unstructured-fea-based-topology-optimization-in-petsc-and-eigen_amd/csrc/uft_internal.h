@@ -213,7 +213,7 @@ struct uft_ctx {
   double *d_xyz = nullptr;     // nn x 3
   double *d_KE = nullptr;      // packed upper triangle, pair-interleaved [np/2][ne][2]
   float *d_KEf = nullptr;      // FP32 copy for the mixed form [np/2][ne][2]
-  uint2 *d_KEh = nullptr;      // FP16 smoother copy, KE/s_e in groups of four [ng][ne]
+  uint4 *d_KEh = nullptr;      // FP16 smoother copy, KE/s_e in groups of eight [ng][ne]
   uint32_t *d_KEb = nullptr;   // BF16 full rows for the multi-load-case matrix-core products [ne][nd][ndp]
   float *d_KEs = nullptr;      // per element scale s_e = max |KE_e|
   double *d_KF = nullptr;      // Helmholtz packed [npf/2][ne][2]

@@ -36,7 +36,7 @@ class Params(C.Structure):
                 ("precision", C.c_int), ("amg_levels", C.c_int), ("cheb_degree", C.c_int),
                 ("coarse_max", C.c_int), ("cycle_type", C.c_int),
                 ("cheb_lo", C.c_double), ("cheb_hi", C.c_double), ("esteig_steps", C.c_int),
-                ("load_cases", C.c_int), ("graph", C.c_int)]
+                ("load_cases", C.c_int), ("graph", C.c_int), ("cheb_degree_coarse", C.c_int)]
 
 
 _LIB = None
