@@ -85,6 +85,52 @@ __global__ __launch_bounds__(TPB) void k_seg_cg_update(SegG g, int64_t n, const 
   uftr::block_partials<1>(v, part);
 }
 
+// mixed forms: the outer product q in FP32 -- partials of p.q, and
+// x += a_d p ; r -= a_d q with partials of r.r
+__global__ __launch_bounds__(TPB) void k_seg_dot_lo(SegG g, int64_t n, const double *__restrict__ a,
+                                                    const float *__restrict__ b, double *__restrict__ part) {
+  int64_t i;
+  int d;
+  double v[1] = {0.0};
+  if (seg_row(g, n, i, d)) v[0] = a[i] * (double)b[i];
+  uftr::block_partials<1>(v, part);
+}
+__global__ __launch_bounds__(TPB) void k_seg_cg_update_lo(SegG g, int64_t n, const double *__restrict__ sc, int ia,
+                                                          double *__restrict__ x, const double *__restrict__ p,
+                                                          double *__restrict__ r, const float *__restrict__ q,
+                                                          double *__restrict__ part) {
+  int64_t i;
+  int d;
+  double v[1] = {0.0};
+  if (seg_row(g, n, i, d)) {
+    const double al = sc[(size_t)d * UFT_BSC + ia];
+    x[i] += al * p[i];
+    const double ri = r[i] - al * (double)q[i];
+    r[i] = ri;
+    v[0] = ri * ri;
+  }
+  uftr::block_partials<1>(v, part);
+}
+
+// reliable update of the designs that asked for one (B_NEED): r = F - q
+// with q = K U in FP64; partials of r.r for every design
+__global__ __launch_bounds__(TPB) void k_seg_true_res(SegG g, int64_t n, const double *__restrict__ sc,
+                                                      const double *__restrict__ F, const double *__restrict__ q,
+                                                      double *__restrict__ r, double *__restrict__ part) {
+  int64_t i;
+  int d;
+  double v[1] = {0.0};
+  if (seg_row(g, n, i, d)) {
+    double ri = r[i];
+    if (sc[(size_t)d * UFT_BSC + B_NEED] != 0.0) {
+      ri = F[i] - q[i];
+      r[i] = ri;
+    }
+    v[0] = ri * ri;
+  }
+  uftr::block_partials<1>(v, part);
+}
+
 // p = z + b_d p
 __global__ __launch_bounds__(TPB) void k_seg_xpby(SegG g, int64_t n, const double *__restrict__ z,
                                                   const double *__restrict__ sc, int ib, double *__restrict__ p) {
@@ -180,6 +226,8 @@ __global__ void k_bsc_el_init(int nseg, double *sc, double rtol) {
   S[B_TOL] = rtol * fn;
   S[B_ACT] = fn > 0.0 ? 1.0 : 0.0;
   S[B_IT] = 0.0;
+  S[B_RT] = fn;
+  S[B_NEED] = 0.0;
 }
 __global__ void k_bsc_el_alpha(int nseg, double *sc) {
   FOR_DESIGN(nseg)
@@ -194,6 +242,27 @@ __global__ void k_bsc_el_check(int nseg, double *sc) {
   S[B_IT] += 1.0;
   if (!(sqrt(S[B_RR]) > S[B_TOL])) S[B_ACT] = 0.0;
 }
+// lower-precision outer product: count the step; ask for the FP64 residual
+// (B_NEED) when the recursive one has met the tolerance or fallen 100-fold
+// below the last FP64 residual (B_RT); the stopping test itself is taken on
+// the FP64 residual (k_bsc_el_true)
+__global__ void k_bsc_el_check_lo(int nseg, double *sc) {
+  FOR_DESIGN(nseg)
+  S[B_NEED] = 0.0;
+  if (S[B_ACT] == 0.0) return;
+  S[B_IT] += 1.0;
+  const double rn = sqrt(S[B_RR]);
+  if (!(rn > S[B_TOL]) || rn < 1e-2 * S[B_RT]) S[B_NEED] = 1.0;
+}
+// after the FP64 residual of the designs that asked for it
+__global__ void k_bsc_el_true(int nseg, double *sc) {
+  FOR_DESIGN(nseg)
+  if (S[B_NEED] == 0.0) return;
+  const double rt = sqrt(S[B_RR]);
+  S[B_RT] = rt;
+  if (S[B_ACT] != 0.0 && !(rt > S[B_TOL])) S[B_ACT] = 0.0;
+}
+
 // flexible CG (Polak-Ribiere): beta = (r.z - r.zo) / rz_prev
 __global__ void k_bsc_el_beta(int nseg, double *sc) {
   FOR_DESIGN(nseg)
@@ -379,6 +448,23 @@ void bsc_el_init(hipStream_t s, int nseg, double *sc, double rtol) { k_bsc_el_in
 void bsc_el_alpha(hipStream_t s, int nseg, double *sc) { k_bsc_el_alpha<<<1, 64, 0, s>>>(nseg, sc); }
 void bsc_el_check(hipStream_t s, int nseg, double *sc) { k_bsc_el_check<<<1, 64, 0, s>>>(nseg, sc); }
 void bsc_el_beta(hipStream_t s, int nseg, double *sc) { k_bsc_el_beta<<<1, 64, 0, s>>>(nseg, sc); }
+void bsc_el_check_lo(hipStream_t s, int nseg, double *sc) { k_bsc_el_check_lo<<<1, 64, 0, s>>>(nseg, sc); }
+void bsc_el_true(hipStream_t s, int nseg, double *sc) { k_bsc_el_true<<<1, 64, 0, s>>>(nseg, sc); }
+void seg_true_res(hipStream_t s, SegG g, const int32_t *dfirst, int nseg, int64_t n, double *sc, const double *F,
+                  const double *q, double *r, double *part, int base) {
+  k_seg_true_res<<<gsz(g), TPB, 0, s>>>(g, n, sc, F, q, r, part);
+  seg_final(s, part, dfirst, nseg, 1, sc, base);
+}
+void seg_dot_lo(hipStream_t s, SegG g, const int32_t *dfirst, int nseg, int64_t n, const double *a, const float *b,
+                double *part, double *sc, int base) {
+  k_seg_dot_lo<<<gsz(g), TPB, 0, s>>>(g, n, a, b, part);
+  seg_final(s, part, dfirst, nseg, 1, sc, base);
+}
+void seg_cg_update_lo(hipStream_t s, SegG g, const int32_t *dfirst, int nseg, int64_t n, double *sc, int ia, double *x,
+                      const double *p, double *r, const float *q, double *part, int base) {
+  k_seg_cg_update_lo<<<gsz(g), TPB, 0, s>>>(g, n, sc, ia, x, p, r, q, part);
+  seg_final(s, part, dfirst, nseg, 1, sc, base);
+}
 void bsc_f_init(hipStream_t s, int nseg, double *sc, double rtol) { k_bsc_f_init<<<1, 64, 0, s>>>(nseg, sc, rtol); }
 void bsc_f_alpha(hipStream_t s, int nseg, double *sc) { k_bsc_f_alpha<<<1, 64, 0, s>>>(nseg, sc); }
 void bsc_f_after(hipStream_t s, int nseg, double *sc) { k_bsc_f_after<<<1, 64, 0, s>>>(nseg, sc); }

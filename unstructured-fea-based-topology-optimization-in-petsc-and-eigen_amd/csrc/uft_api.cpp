@@ -179,7 +179,7 @@ void fine_apply(uft_ctx *C, const double *xphys, const double *x, double *y, con
 }
 
 // Outer operator product of the mixed forms with the FP32 element blocks:
-// qf = K(x) p in FP32 (p rounded to FP32), and p.q into sc[slot]
+// qf = K(x) p in FP32 (p rounded to FP32), and p.q into sc[slot] (slot >= 0)
 void fine_apply_lo(uft_ctx *C, const double *xphys, const double *p, int slot) {
   hipStream_t s = C->s;
   uftk::d2f(s, C->ndof, p, C->d_pf);
@@ -188,12 +188,12 @@ void fine_apply_lo(uft_ctx *C, const double *xphys, const double *p, int slot) {
   uftk::elem_phase_f(s, C, C->elem_variant_f, xphys, C->d_pf, C->d_Yef);
   if (ev) (void)hipEventRecord(ev[1], s);
   RowOp<float> op;
-  op.mode = ROW_STOREDOT;
+  op.mode = slot >= 0 ? ROW_STOREDOT : ROW_STORE;
   op.y = C->d_qf;
   op.part = C->d_part;
   uftk::gather_op(s, C, C->d_pf, C->d_Yef, op);
   if (ev) (void)hipEventRecord(ev[2], s);
-  uftk::final_sum(s, C->d_part, uftk::gather_op_blocks(C->nn), 1, C->d_scal + slot);
+  if (slot >= 0) uftk::final_sum(s, C->d_part, uftk::gather_op_blocks(C->nn), 1, C->d_scal + slot);
 }
 
 // y = A_l x on multigrid level l
@@ -1164,6 +1164,7 @@ int solve_el_batch(uft_ctx *C, const double *xphys, double *U) {
   uftk::copy(s, n, C->d_F, r);
   uftk::seg_dot(s, g, df, ns, n, r, r, C->d_part, B.sc, B_RR);
   uftk::bsc_el_init(s, ns, B.sc, C->P.rtol_el);
+  C->reliable = 0;
   const bool f32 = is_mixed(C);
   auto precond = [&]() {
     if (f32) {
@@ -1182,12 +1183,23 @@ int solve_el_batch(uft_ctx *C, const double *xphys, double *U) {
     uftk::seg_dot(s, g, df, ns, n, r, z, C->d_part, B.sc, B_RZ);
     uftk::copy(s, n, z, p);
   }
+  // mixed forms: the outer product with the FP32 element blocks and FP64
+  // residuals at reliable updates (as solve_el_mixed), the update decisions
+  // per design on the device; one FP64 residual pass serves all designs
   auto head = [&]() {
-    fine_apply(C, xphys, p, q, nullptr, nullptr);
-    uftk::seg_dot(s, g, df, ns, n, p, q, C->d_part, B.sc, B_PQ);
-    uftk::bsc_el_alpha(s, ns, B.sc);
-    uftk::seg_cg_update(s, g, df, ns, n, B.sc, B_ALPHA, U, p, r, q, C->d_part, B_RR);
-    uftk::bsc_el_check(s, ns, B.sc);
+    if (f32) {
+      fine_apply_lo(C, xphys, p, -1);
+      uftk::seg_dot_lo(s, g, df, ns, n, p, C->d_qf, C->d_part, B.sc, B_PQ);
+      uftk::bsc_el_alpha(s, ns, B.sc);
+      uftk::seg_cg_update_lo(s, g, df, ns, n, B.sc, B_ALPHA, U, p, r, C->d_qf, C->d_part, B_RR);
+      uftk::bsc_el_check_lo(s, ns, B.sc);
+    } else {
+      fine_apply(C, xphys, p, q, nullptr, nullptr);
+      uftk::seg_dot(s, g, df, ns, n, p, q, C->d_part, B.sc, B_PQ);
+      uftk::bsc_el_alpha(s, ns, B.sc);
+      uftk::seg_cg_update(s, g, df, ns, n, B.sc, B_ALPHA, U, p, r, q, C->d_part, B_RR);
+      uftk::bsc_el_check(s, ns, B.sc);
+    }
   };
   auto next = [&]() {
     uftk::copy(s, n, z, zo);  // the previous preconditioned residual
@@ -1207,9 +1219,18 @@ int solve_el_batch(uft_ctx *C, const double *xphys, double *U) {
       return rc;
     ++it;
     if ((rc = fetch_batch(C, B_ACT, &any))) return rc;
+    bool need = false;
+    for (int d = 0; f32 && d < ns; ++d) need |= B.h[(size_t)d * UFT_BSC + B_NEED] != 0.0;
+    if (need) {  // reliable update of the designs that asked: r = F - K U in FP64
+      fine_apply(C, xphys, U, q, nullptr, nullptr);
+      uftk::seg_true_res(s, g, df, ns, n, B.sc, C->d_F, q, r, C->d_part, B_RR);
+      uftk::bsc_el_true(s, ns, B.sc);
+      C->reliable++;
+      if ((rc = fetch_batch(C, B_ACT, &any))) return rc;
+    }
   }
   C->it_el = it;
-  C->outer_el = 1;
+  C->outer_el = f32 ? C->reliable : 1;
   // true residuals per design (SolveFem.cpp:142-152)
   fine_apply(C, xphys, U, q, nullptr, nullptr);
   uftk::sub(s, n, C->d_F, q, q);
@@ -1464,7 +1485,7 @@ void uft_default_params(uft_params *p) {
   p->cycle_type = 2;
   p->cheb_lo = 0.1;
   p->cheb_hi = 1.1;
-  p->esteig_steps = 8;
+  p->esteig_steps = 10;
   p->load_cases = 0;
   p->graph = 0;
   p->cheb_degree_coarse = 0;

@@ -51,7 +51,7 @@ struct SegG {
 // flexible CG, filter CG, Lanczos, then general reductions from B_RED.
 enum {
   B_RZ = 0, B_PQ = 1, B_ALPHA = 2, B_RR = 3, B_RZN = 4, B_RZO = 5, B_BETA = 6, B_FN = 7, B_TOL = 8, B_ACT = 9,
-  B_IT = 10, B_TR = 11,
+  B_IT = 10, B_TR = 11, B_RT = 12, B_NEED = 13,
   B_FB = 16, B_FZZ = 17, B_FBETA = 18, B_FBO = 19, B_FPQ = 20, B_FAL = 21, B_FPC = 22, B_FTOL = 23, B_FACT = 24,
   B_FIT = 25, B_FSN = 26, B_FDIV = 27, B_FZZN = 28, B_FZRN = 29,
   B_LRZ = 32, B_LPQ = 33, B_LAL = 34, B_LRR = 35, B_LRZN = 36, B_LBE = 37,
@@ -546,6 +546,14 @@ void bsc_el_init(hipStream_t s, int nseg, double *sc, double rtol);
 void bsc_el_alpha(hipStream_t s, int nseg, double *sc);
 void bsc_el_check(hipStream_t s, int nseg, double *sc);
 void bsc_el_beta(hipStream_t s, int nseg, double *sc);
+void bsc_el_check_lo(hipStream_t s, int nseg, double *sc);
+void bsc_el_true(hipStream_t s, int nseg, double *sc);
+void seg_true_res(hipStream_t s, SegG g, const int32_t *dfirst, int nseg, int64_t n, double *sc, const double *F,
+                  const double *q, double *r, double *part, int base);
+void seg_dot_lo(hipStream_t s, SegG g, const int32_t *dfirst, int nseg, int64_t n, const double *a, const float *b,
+                double *part, double *sc, int base);
+void seg_cg_update_lo(hipStream_t s, SegG g, const int32_t *dfirst, int nseg, int64_t n, double *sc, int ia, double *x,
+                      const double *p, double *r, const float *q, double *part, int base);
 void bsc_f_init(hipStream_t s, int nseg, double *sc, double rtol);
 void bsc_f_alpha(hipStream_t s, int nseg, double *sc);
 void bsc_f_after(hipStream_t s, int nseg, double *sc);
