@@ -115,27 +115,26 @@ def test_coarse_inverse(dims, coarse_max):
 
 def test_level_operators_in_each_precision():
     """Every multigrid level's stored operator in FP32 and FP16 (UFT_MIXED_H:
-    fine element blocks scaled per element, first coarse level scaled to unit
-    diagonal) against the FP64 one, with a non-smooth random vector."""
+    fine element blocks scaled per element, coarse levels above the coarsest
+    scaled to unit diagonal) against the FP64 one, with a non-smooth random
+    vector."""
     from uft.capi import cantilever
     print("[case] level operators FP64/FP32/FP16", flush=True)
-    ctx = cantilever(8, 16, 8, 8, jitter=0.15, seed=9, Emax=1.0, volfrac=0.3, rmin=0.2, precision=2)
+    ctx = cantilever(8, 24, 12, 12, jitter=0.15, seed=9, Emax=1.0, volfrac=0.3, rmin=0.2, precision=2)
     x = np.random.default_rng(4).uniform(0.01, 1.0, ctx.size("ne"))
     ctx.set("XPhys", x)
     ctx.fem_obj_sens()
     n, _ = ctx.solver_info()
+    assert len(n) >= 3
     rng = np.random.default_rng(5)
-    for lev in (0, 1):
+    for lev in range(len(n)):
         u = rng.standard_normal(n[lev])
         y64 = ctx.level_apply(lev, u, 0)
-        y32 = ctx.level_apply(lev, u, 1)
-        y16 = ctx.level_apply(lev, u, 2)
         scale = np.linalg.norm(y64)
-        e32, e16 = np.linalg.norm(y32 - y64) / scale, np.linalg.norm(y16 - y64) / scale
-        print("level", lev, "FP32 error", e32, "FP16 error", e16)
+        e32 = np.linalg.norm(ctx.level_apply(lev, u, 1) - y64) / scale
+        print("level", lev, "FP32 error", e32)
         assert e32 < 1e-6
-        assert e16 < 2e-3 and e16 > 1e-7  # FP16 rounding is visible, and only rounding
-    for lev in range(2, len(n)):  # coarser levels: FP32 copies
-        u = rng.standard_normal(n[lev])
-        y64 = ctx.level_apply(lev, u, 0)
-        assert np.linalg.norm(ctx.level_apply(lev, u, 1) - y64) < 1e-6 * np.linalg.norm(y64)
+        if lev < len(n) - 1:  # FP16 blocks: every level but the coarsest (solved densely)
+            e16 = np.linalg.norm(ctx.level_apply(lev, u, 2) - y64) / scale
+            print("level", lev, "FP16 error", e16)
+            assert e16 < 2e-3 and e16 > 1e-7  # FP16 rounding is visible, and only rounding

@@ -1808,7 +1808,7 @@ int uft_setup(uft_ctx *C) {
       }
       if (is_mixed(C))
         if ((rc = dalloc_n(C, &Ln.d_valf, Ln.nblk * 36))) return rc;
-      if (C->P.precision == UFT_MIXED_H && l == 0) {  // the first coarse level's smoother blocks in FP16
+      if (C->P.precision == UFT_MIXED_H && l + 2 < A.nlev) {  // coarse levels' smoother blocks in FP16 (not the coarsest)
         if ((rc = dalloc_n(C, &Ln.d_valh, Ln.nblk * 36))) return rc;
         if ((rc = dalloc_n(C, &Ln.d_dh, Ln.n))) return rc;
       }
