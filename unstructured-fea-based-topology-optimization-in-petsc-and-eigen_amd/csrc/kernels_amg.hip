@@ -247,9 +247,10 @@ __global__ __launch_bounds__(TPB) void k_simp_E(int64_t ne, const double *__rest
   if (e < ne) Ee[e] = sc.Emin + pow(x[e], sc.penal) * sc.dE;
 }
 
-// level 0 -> 1 per density field: one thread per coarse block entry (kb, q);
-// a block below the diagonal reads its transposed block's entries transposed,
-// so off-diagonal blocks are exact transposes of each other.  Fixed entry order.
+// level 0 -> 1 per density field: one thread per entry (kb, q) of the blocks
+// on or above the diagonal, which also writes the entry into the transposed
+// block below the diagonal (so off-diagonal blocks are exact transposes of
+// each other and the entries are read once).  Fixed entry order.
 __global__ __launch_bounds__(TPB) void k_galerkin_fine(int64_t nblk_c, const int32_t *__restrict__ glptr,
                                                        const int32_t *__restrict__ gl, const int32_t *__restrict__ gtr,
                                                        const int32_t *__restrict__ brow, const int32_t *__restrict__ ci,
@@ -259,15 +260,12 @@ __global__ __launch_bounds__(TPB) void k_galerkin_fine(int64_t nblk_c, const int
   if (t >= nblk_c * 36) return;
   const int64_t kb = t / 36;
   const int q = (int)(t - kb * 36);
-  int64_t src = kb;
-  int qs = q;
-  if (brow[kb] > ci[kb]) {
-    src = gtr[kb];
-    qs = (q % 6) * 6 + q / 6;
-  }
+  if (brow[kb] > ci[kb]) return;  // a lower block is written by its upper transpose below
   double acc = 0.0;
-  for (int k = glptr[src]; k < glptr[src + 1]; ++k) acc += Ee[gl[k]] * G[(size_t)k * 36 + qs];
+  for (int k = glptr[kb]; k < glptr[kb + 1]; ++k) acc += Ee[gl[k]] * G[(size_t)k * 36 + q];
   out[t] = acc;
+  const int64_t kt = gtr[kb];
+  if (kt != kb) out[(size_t)kt * 36 + (q % 6) * 6 + q / 6] = acc;  // the entries are read once
 }
 
 // level l -> l+1 (l >= 1): contribution = block index kb of level l with row

@@ -1225,17 +1225,20 @@ void sens(hipStream_t s, const uft_ctx *C, const double *xphys, const double *u,
                                 C->vec[UFT_VOL], 1.0 / C->TV, u, dfdx, dgdx0, part, sg, invTVd);
 }
 
-void apply_KF(hipStream_t s, const uft_ctx *C, const double *x, double *y) {
+void apply_KF(hipStream_t s, const uft_ctx *C, const double *x, double *y, double *dot_out) {
   const int ne = (int)C->ne, nn = (int)C->nn;
+  const double *dotw = dot_out ? x : nullptr;  // with dot_out: also x.y in the node phase
+  double *part = dot_out ? C->d_part : nullptr;
   if (C->npe == 8) {
     k_helm_elem<8><<<nblk(ne), TPB, 0, s>>>(ne, C->d_elem, (const double2 *)C->d_KF, x, C->d_Ye);
     k_node_gather<8, 1, 0, double><<<nblk(nn), TPB, 0, s>>>(nn, ne, C->d_iptr, C->d_inc, C->d_Ye, nullptr, nullptr, y,
-                                                            nullptr, nullptr, nullptr);
+                                                            dotw, part, nullptr);
   } else {
     k_helm_elem<4><<<nblk(ne), TPB, 0, s>>>(ne, C->d_elem, (const double2 *)C->d_KF, x, C->d_Ye);
     k_node_gather<4, 1, 0, double><<<nblk(nn), TPB, 0, s>>>(nn, ne, C->d_iptr, C->d_inc, C->d_Ye, nullptr, nullptr, y,
-                                                            nullptr, nullptr, nullptr);
+                                                            dotw, part, nullptr);
   }
+  if (dot_out) final_sum(s, part, (int)nblk(nn), 1, dot_out);
 }
 
 void diag_KF(hipStream_t s, const uft_ctx *C, double *dinv) {

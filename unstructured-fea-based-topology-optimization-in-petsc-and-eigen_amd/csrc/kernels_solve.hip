@@ -69,6 +69,39 @@ __global__ __launch_bounds__(TPB) void k_fcg_update(int64_t n, const double *__r
   uftr::block_partials<2>(v, part);
 }
 
+// the same with alpha = sc[ia] / sc[ib] formed in place
+__global__ __launch_bounds__(TPB) void k_fcg_update_ab(int64_t n, const double *__restrict__ sc, int ia, int ib,
+                                                       double *__restrict__ x, const double *__restrict__ p,
+                                                       double *__restrict__ r, const double *__restrict__ q,
+                                                       const double *__restrict__ dinv, double *__restrict__ z,
+                                                       double *__restrict__ part) {
+  const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  const double al = sc[ia] / sc[ib];
+  double v[2] = {0.0, 0.0};
+  if (i < n) {
+    x[i] += al * p[i];
+    const double ri = r[i] - al * q[i];
+    r[i] = ri;
+    const double zi = ri * dinv[i];
+    z[i] = zi;
+    v[0] = zi * zi;
+    v[1] = zi * ri;
+  }
+  uftr::block_partials<2>(v, part);
+}
+// final sums of ||z||^2 and z.r into sc[izz], and the CG bookkeeping
+// beta_old = beta (sc[ibo] = sc[ib]), beta = z.r (sc[ib])
+__global__ __launch_bounds__(TPB) void k_final_fcg(const double *__restrict__ part, int nb, double *__restrict__ sc,
+                                                   int izz, int ib, int ibo) {
+  double res[2];
+  uftr::block_final<2>(part, nb, res);
+  if (threadIdx.x == 0) {
+    sc[izz] = res[0];
+    sc[ibo] = sc[ib];
+    sc[ib] = res[1];
+  }
+}
+
 // p = z + (sc[ib] / sc[ibo]) p, or p = z when first
 __global__ __launch_bounds__(TPB) void k_fcg_p(int64_t n, const double *__restrict__ z, const double *__restrict__ sc,
                                                int ib, int ibo, int first, double *__restrict__ p) {
@@ -107,6 +140,11 @@ void fcg_update(hipStream_t s, int64_t n, const double *sc, int ia, double *x, c
                 const double *q, const double *dinv, double *z, double *part, double *out) {
   k_fcg_update<<<nblk(n), TPB, 0, s>>>(n, sc, ia, x, p, r, q, dinv, z, part, out);
   final_sum(s, part, (int)nblk(n), 2, out);
+}
+void fcg_update_ab(hipStream_t s, int64_t n, double *sc, int ia, int ib, double *x, const double *p, double *r,
+                   const double *q, const double *dinv, double *z, double *part, int izz, int ibeta, int ibo) {
+  k_fcg_update_ab<<<nblk(n), TPB, 0, s>>>(n, sc, ia, ib, x, p, r, q, dinv, z, part);
+  k_final_fcg<<<1, TPB, 0, s>>>(part, (int)nblk(n), sc, izz, ibeta, ibo);
 }
 void fcg_p(hipStream_t s, int64_t n, const double *z, const double *sc, int ib, int ibo, int first, double *p) {
   k_fcg_p<<<nblk(n), TPB, 0, s>>>(n, z, sc, ib, ibo, first, p);

@@ -1315,12 +1315,10 @@ int solve_filter(uft_ctx *C, const double *b, double *x) {
   if (dp <= ttol) return UFT_OK;
   while (it < C->P.maxit_f) {
     uftk::fcg_p(s, n, z, sc, S_F0 + 2, S_F0 + 3, it == 0, p);
-    uftk::apply_KF(s, C, p, q);
-    uftk::dot(s, n, p, q, C->d_part, sc + S_F0 + 4);
-    uftk::sc_div(s, sc, S_F0 + 2, S_F0 + 4, S_F0 + 5);
-    uftk::fcg_update(s, n, sc, S_F0 + 5, x, p, r, q, C->d_fdinv, z, C->d_part, sc + S_F0 + 6);
-    uftk::sc_copy(s, sc + S_F0 + 2, sc + S_F0 + 3);                     // beta_old = beta
-    uftk::sc_copy(s, sc + S_F0 + 7, sc + S_F0 + 2);                     // beta = z.r
+    uftk::apply_KF(s, C, p, q, sc + S_F0 + 4);  // q = KF p and p.q
+    // alpha = beta / p.q; x, r, z updated; ||z||^2; beta_old = beta; beta = z.r
+    uftk::fcg_update_ab(s, n, sc, S_F0 + 2, S_F0 + 4, x, p, r, q, C->d_fdinv, z, C->d_part, S_F0 + 6, S_F0 + 2,
+                        S_F0 + 3);
     ++it;
     rc = fetch(C, sc + S_F0 + 6, 1, C->h_scal);
     if (rc) return rc;

@@ -326,7 +326,8 @@ void diag_K(hipStream_t s, const uft_ctx *C, const double *xphys, double *dinv, 
 void sens(hipStream_t s, const uft_ctx *C, const double *xphys, const double *u, double *dfdx,
           double *dgdx0, double *part, SegG sg = SegG(), const double *invTVd = nullptr);
 // Helmholtz filter
-void apply_KF(hipStream_t s, const uft_ctx *C, const double *x, double *y);
+// y = KF x; with dot_out also x.y (reduced in fixed order into *dot_out)
+void apply_KF(hipStream_t s, const uft_ctx *C, const double *x, double *y, double *dot_out = nullptr);
 void diag_KF(hipStream_t s, const uft_ctx *C, double *dinv);
 void tf_mult(hipStream_t s, const uft_ctx *C, const double *xe, double *TX);
 void tfx_t(hipStream_t s, const uft_ctx *C, const double *RX, double *xe, int clamp, double Xmin);
@@ -511,6 +512,10 @@ void fcg_init(hipStream_t s, int64_t n, const double *b, const double *q, const 
 void fcg_update(hipStream_t s, int64_t n, const double *sc, int ia, double *x, const double *p, double *r,
                 const double *q, const double *dinv, double *z, double *part, double *out);
 void fcg_p(hipStream_t s, int64_t n, const double *z, const double *sc, int ib, int ibo, int first, double *p);
+// x += a p ; r -= a q ; z = D^-1 r with a = sc[ia] / sc[ib]; then sc[izz] = ||z||^2,
+// sc[ibo] = sc[ibeta], sc[ibeta] = z.r
+void fcg_update_ab(hipStream_t s, int64_t n, double *sc, int ia, int ib, double *x, const double *p, double *r,
+                   const double *q, const double *dinv, double *z, double *part, int izz, int ibeta, int ibo);
 void sc_div(hipStream_t s, double *sc, int a, int b, int dst);
 void sc_beta(hipStream_t s, double *sc, int inew, int iold, int ibeta, double *log2);
 void sc_copy(hipStream_t s, const double *src, double *dst);
