@@ -837,21 +837,38 @@ __global__ __launch_bounds__(TPB) void k_cg_update(int64_t n, const double *__re
   uftr::block_partials<1>(v, part);
 }
 // FP64 iterate and residual updated with a lower-precision operator product:
-// u += a p ; r -= a q ; partials of r.r
-__global__ __launch_bounds__(TPB) void k_cg_update_lo(int64_t n, const double *__restrict__ sc, int ia,
+// u += a p ; r -= a q (a = sc[ia] / sc[ib]) ; rf = FP32 copy of r for the
+// preconditioner ; partials of r.r
+__global__ __launch_bounds__(TPB) void k_cg_update_lo(int64_t n, const double *__restrict__ sc, int ia, int ib,
                                                       double *__restrict__ u, const double *__restrict__ p,
                                                       double *__restrict__ r, const float *__restrict__ q,
-                                                      double *__restrict__ part) {
+                                                      float *__restrict__ rf, double *__restrict__ part) {
   const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
-  const double al = sc[ia];
+  const double al = sc[ia] / sc[ib];
   double v[1] = {0.0};
   if (i < n) {
     u[i] += al * p[i];
     const double rr = r[i] - al * (double)q[i];
     r[i] = rr;
+    rf[i] = (float)rr;
     v[0] = rr * rr;
   }
   uftr::block_partials<1>(v, part);
+}
+// after the FP32 cycle: zo = z ; z = zf ; partials of r.z, r.zo (flexible CG)
+__global__ __launch_bounds__(TPB) void k_z_from_f32(int64_t n, const float *__restrict__ zf, const double *__restrict__ r,
+                                                    double *__restrict__ z, double *__restrict__ zo,
+                                                    double *__restrict__ part) {
+  const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  double v[2] = {0.0, 0.0};
+  if (i < n) {
+    const double zold = z[i], znew = (double)zf[i], ri = r[i];
+    zo[i] = zold;
+    z[i] = znew;
+    v[0] = ri * znew;
+    v[1] = ri * zold;
+  }
+  uftr::block_partials<2>(v, part);
 }
 template <typename T>
 __global__ __launch_bounds__(TPB) void k_xpby(int64_t n, const T *__restrict__ z, const double *__restrict__ sc, int ib,
@@ -1320,10 +1337,15 @@ void cg_update(hipStream_t s, int64_t n, const double *sc, int ia, double *u, co
   k_cg_update<double><<<nblk(n), TPB, 0, s>>>(n, sc, ia, u, p, r, q, part, out);
   final_sum(s, part, (int)nblk(n), 1, out);
 }
-void cg_update_lo(hipStream_t s, int64_t n, const double *sc, int ia, double *u, const double *p, double *r,
-                  const float *q, double *part, double *out) {
-  k_cg_update_lo<<<nblk(n), TPB, 0, s>>>(n, sc, ia, u, p, r, q, part);
+void cg_update_lo(hipStream_t s, int64_t n, const double *sc, int ia, int ib, double *u, const double *p, double *r,
+                  const float *q, float *rf, double *part, double *out) {
+  k_cg_update_lo<<<nblk(n), TPB, 0, s>>>(n, sc, ia, ib, u, p, r, q, rf, part);
   final_sum(s, part, (int)nblk(n), 1, out);
+}
+void z_from_f32(hipStream_t s, int64_t n, const float *zf, const double *r, double *z, double *zo, double *part,
+                double *out) {
+  k_z_from_f32<<<nblk(n), TPB, 0, s>>>(n, zf, r, z, zo, part);
+  final_sum(s, part, (int)nblk(n), 2, out);
 }
 void cg_updatef(hipStream_t s, int64_t n, const double *sc, int ia, float *u, const float *p, float *r, const float *q,
                 double *part, double *out) {

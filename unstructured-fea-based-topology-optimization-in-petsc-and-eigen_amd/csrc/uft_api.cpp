@@ -981,15 +981,13 @@ int solve_el_mixed(uft_ctx *C, const double *xphys, double *U) {
   // latter has fallen 100-fold below the last true residual, and before the
   // solve is accepted (reliable updates: the recursion's drift stays 1e-5 of
   // the current residual, far below the tolerance).
-  auto head = [&]() {
+  auto head = [&]() {  // q = K p (FP32 blocks); alpha = rz / pq; U, r (and its FP32 copy rf) updated
     fine_apply_lo(C, xphys, p, S_PQ);
-    uftk::sc_div(s, C->d_scal, S_RZ, S_PQ, S_ALPHA);
-    uftk::cg_update_lo(s, n, C->d_scal, S_ALPHA, U, p, r, C->d_qf, C->d_part, C->d_scal + S_RR);
+    uftk::cg_update_lo(s, n, C->d_scal, S_RZ, S_PQ, U, p, r, C->d_qf, rf, C->d_part, C->d_scal + S_RR);
   };
-  auto next = [&]() {
-    uftk::copy(s, n, z, zo);  // zo keeps the previous preconditioned residual
-    precond();
-    uftk::dot_pr(s, n, r, z, zo, C->d_part, C->d_scal + S_RZNEW);
+  auto next = [&]() {  // z = B rf in FP32; zo = previous z; r.z, r.zo; beta; p
+    vcycle_f(C, 0, rf, zf);
+    uftk::z_from_f32(s, n, zf, r, z, zo, C->d_part, C->d_scal + S_RZNEW);
     uftk::sc_beta_flex(s, C->d_scal, S_RZNEW, S_RZNEW + 1, S_RZ, S_BETA + 2);
     uftk::xpby(s, n, z, C->d_scal, S_BETA + 2, p);
     head();
@@ -1011,6 +1009,7 @@ int solve_el_mixed(uft_ctx *C, const double *xphys, double *U) {
     if (rn <= tol || rn < 1e-2 * rtrue) {  // reliable update
       fine_apply(C, xphys, U, q, nullptr, nullptr);
       uftk::sub(s, n, C->d_F, q, r);
+      uftk::d2f(s, n, r, rf);
       uftk::dot(s, n, r, r, C->d_part, C->d_scal + S_RR);
       if ((rc = fetch(C, C->d_scal + S_RR, 1, C->h_scal))) return rc;
       rtrue = std::sqrt(C->h_scal[0]);

@@ -351,8 +351,12 @@ void axpy_inplace(hipStream_t s, int64_t n, double a, const double *x, double *y
 void sub(hipStream_t s, int64_t n, const double *a, const double *b, double *c);
 void set_dofs(hipStream_t s, int n, const int32_t *idx, const double *val, double *x);
 // templated float variants for the mixed form
-void cg_update_lo(hipStream_t s, int64_t n, const double *sc, int ia, double *u, const double *p, double *r,
-                  const float *q, double *part, double *out);
+// u += a p ; r -= a q ; rf = (float) r with a = sc[ia] / sc[ib]; r.r into *out
+void cg_update_lo(hipStream_t s, int64_t n, const double *sc, int ia, int ib, double *u, const double *p, double *r,
+                  const float *q, float *rf, double *part, double *out);
+// zo = z ; z = (double) zf ; r.z, r.zo into out[0..1]
+void z_from_f32(hipStream_t s, int64_t n, const float *zf, const double *r, double *z, double *zo, double *part,
+                double *out);
 void cg_updatef(hipStream_t s, int64_t n, const double *sc, int ialpha, float *u, const float *p,
                 float *r, const float *q, double *part, double *rr_out);
 void xpbyf(hipStream_t s, int64_t n, const float *z, const double *sc, int ibeta, float *p);
