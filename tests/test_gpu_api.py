@@ -78,6 +78,23 @@ def test_state_and_argument_errors():
     with pytest.raises(UFTError) as e:
         a.set_mesh(elem, np.zeros((8, 3)))
     assert e.value.code == 1
+    from uft.capi import Context
+    for bad in (dict(cheb_degree=0), dict(cheb_degree=17), dict(cheb_degree_coarse=-1), dict(cycle_type=4),
+                dict(precision=3), dict(graph=2), dict(esteig_steps=1), dict(cheb_hi=0.05)):
+        with pytest.raises(UFTError) as e:
+            a.set_params(Context.default_params(**bad))
+        assert e.value.code == 1, bad
+    # design batches: offsets must cover the mesh, and elements stay inside their design
+    from uft.capi import batch_mesh
+    ds = [dict(nx=4, ny=2, nz=2, volfrac=0.3, seed=1), dict(nx=4, ny=2, nz=2, volfrac=0.3, seed=2)]
+    el, xyz, fixed, ld, lv, eoff, noff = batch_mesh(ds)
+    b = _ctx()
+    b.set_mesh(el, xyz)
+    for eo, no, vf in ((eoff, noff[::-1].copy(), [0.3, 0.3]), ([0, 10, len(el)], noff, [0.3, 0.3]),
+                       (eoff, noff, [0.3, 1.5])):
+        with pytest.raises(UFTError) as e:
+            b.batch_set(eo, no, vf, [0.1, 0.1])
+        assert e.value.code == 1
 
 
 def test_inverted_element_detected():

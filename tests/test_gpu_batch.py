@@ -97,3 +97,29 @@ def test_coarsest_inverse_blocks():
     err = np.abs(A @ inv - np.eye(nc)).max()
     print("||A Ainv - I||_max", err)
     assert err <= 1e-9
+
+
+def test_batch_tet4_with_projection(O):
+    """Tet4 designs of uneven size with the Heaviside projection on: every
+    design against the oracle running it alone."""
+    from uft.capi import batch_cantilever, batch_mesh
+    print("[case] batch tet4 + projection", flush=True)
+    ds = [dict(nx=5, ny=3, nz=3, volfrac=0.30, seed=31), dict(nx=4, ny=2, nz=2, volfrac=0.35, seed=32)]
+    prm = dict(TIGHT, proj=1, beta0=2.0)
+    ctx = batch_cantilever(ds, npe=4, jitter=0.15, **prm)
+    refs = [O.cantilever(4, D["nx"], D["ny"], D["nz"], jitter=0.15, seed=D["seed"], volfrac=D["volfrac"],
+                         rmin=1.5 * 2.0 / D["nx"], **prm) for D in ds]
+    eoff = batch_mesh(ds, npe=4)[5]
+    ctx.init_design()
+    for r in refs:
+        r.init_design()
+    for it in range(1, 5):
+        ctx.design_iteration(it)
+        res = ctx.batch_results()
+        for d, r in enumerate(refs):
+            rfx, rgx, rch, _ = r.design_iteration(it)
+            assert abs(res["fx"][d] - rfx) <= 1e-10 * abs(rfx), (it, d)
+            assert abs(res["gx"][d] - rgx[0]) <= 1e-10, (it, d)
+    X = ctx.get("X")
+    for d, r in enumerate(refs):
+        assert np.abs(X[eoff[d]:eoff[d + 1]] - r.vec("X")).max() <= 1e-8
