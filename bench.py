@@ -247,10 +247,11 @@ def main():
             w, kname = 4, f"k_elem_apply_v<{npe},float> (FP32 element phase, mixed form)"
         rooflines.append(elem_roofline(main_form, npe, ne, ndof, r["prof"][2], r["prof"][3], w, kname,
                                        f"{args.workload}_{main_form}_elem_apply", nrhs))
-        if r["prof"][4][0]:  # single design: FP32 outer operator, FP64 residual at reliable updates
+        if r["prof"][4][0]:  # FP32 outer operator, FP64 residual at reliable updates
+            k32 = f"k_mr_elem<{npe},float,8>" if multi else f"k_elem_apply_v<{npe},float>"
             rooflines.append(elem_roofline("outer", npe, ne, ndof, r["prof"][4], r["prof"][5], 4,
-                                           f"k_elem_apply_v<{npe},float> (FP32 element phase of the outer "
-                                           f"operator, mixed form)", f"{args.workload}_fp32_elem_apply", nrhs))
+                                           f"{k32} (FP32 element phase of the outer operator, mixed form)",
+                                           f"{args.workload}_fp32_elem_apply", nrhs))
         if r["prof"][0][0]:
             rooflines.append(elem_roofline("outer64", npe, ne, ndof, r["prof"][0], r["prof"][1], 8,
                                            f"{k64} (FP64 element phase of the outer loop's residuals, mixed "

@@ -530,6 +530,59 @@ __global__ __launch_bounds__(TPB) void k_mr_cg_update(int64_t n, const double *_
   }
   mr_block_partials<1, NR>(v, part);
 }
+// mixed forms: the same update with the outer product q in FP32; the new
+// residual's FP32 copy rf (the cycle's input) is written in the same pass
+template <int NR>
+__global__ __launch_bounds__(TPB) void k_mr_cg_update_lo(int64_t n, const double *__restrict__ alpha,
+                                                         double *__restrict__ x, const double *__restrict__ p,
+                                                         double *__restrict__ r, const float *__restrict__ q,
+                                                         float *__restrict__ rf, double *__restrict__ part) {
+  double v[1] = {0.0};
+  const double al = alpha[((int64_t)blockIdx.x * TPB + threadIdx.x) % NR];
+  for (int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x; t < n * NR; t += (int64_t)gridDim.x * TPB) {
+    x[t] += al * p[t];
+    const double ri = r[t] - al * (double)q[t];
+    r[t] = ri;
+    rf[t] = (float)ri;
+    v[0] += ri * ri;
+  }
+  mr_block_partials<1, NR>(v, part);
+}
+// reliable update of the right-hand sides in mask: r = res (the FP64
+// residual F - K U) and its FP32 copy; partials of r.r for every right-hand
+// side
+template <int NR>
+__global__ __launch_bounds__(TPB) void k_mr_take_res(int64_t n, unsigned mask, const double *__restrict__ res,
+                                                     double *__restrict__ r, float *__restrict__ rf,
+                                                     double *__restrict__ part) {
+  double v[1] = {0.0};
+  const int k = (int)(((int64_t)blockIdx.x * TPB + threadIdx.x) % NR);
+  const bool take = (mask >> k) & 1u;
+  for (int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x; t < n * NR; t += (int64_t)gridDim.x * TPB) {
+    double ri;
+    if (take) {
+      ri = res[t];
+      r[t] = ri;
+      rf[t] = (float)ri;
+    } else {
+      ri = r[t];
+    }
+    v[0] += ri * ri;
+  }
+  mr_block_partials<1, NR>(v, part);
+}
+// p = z + beta_r p and its FP32 copy (the mixed outer operator's input)
+template <int NR>
+__global__ __launch_bounds__(TPB) void k_mr_xpby_lo(int64_t n, const double *__restrict__ z,
+                                                    const double *__restrict__ beta, double *__restrict__ p,
+                                                    float *__restrict__ pf) {
+  const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  if (t < n * NR) {
+    const double v = z[t] + beta[t % NR] * p[t];
+    p[t] = v;
+    pf[t] = (float)v;
+  }
+}
 template <int NR>
 __global__ __launch_bounds__(TPB) void k_mr_xpby(int64_t n, const double *__restrict__ z, const double *__restrict__ beta,
                                                  double *__restrict__ p) {
@@ -660,10 +713,11 @@ void mr_elem_phase(hipStream_t s, const uft_ctx *C, int nr, const double *xphys,
                           ne, C->d_elem, KE, xphys, sc, C->d_fixm, u, Ye)));
   }
 }
-void mr_elem_phase(hipStream_t s, const uft_ctx *C, int nr, const double *xphys, const float *u, float *Ye) {
+void mr_elem_phase(hipStream_t s, const uft_ctx *C, int nr, const double *xphys, const float *u, float *Ye,
+                   bool blocks32) {
   const SimpCoef sc{C->P.Emin, C->P.Emax - C->P.Emin, C->P.penal};
   const int ne = (int)C->ne;
-  if (nr == 8 && C->d_KEb) {  // matrix-core form
+  if (!blocks32 && nr == 8 && C->d_KEb) {  // matrix-core form
     const unsigned g = (unsigned)((ne + MF_EL * MF_WAVES - 1) / (MF_EL * MF_WAVES));
     if (C->npe == 8)
       k_mr_elem_mfma<8><<<g, 64 * MF_WAVES, 0, s>>>(ne, C->d_elem, (const uint4 *)C->d_KEb, C->d_Ee, C->d_fixm, u,
@@ -820,6 +874,19 @@ void mr_cg_update(hipStream_t s, int nr, int64_t n, const double *alpha, double 
                   const double *q, double *part, double *rr_out) {
   UFT_NR_SWITCH(nr, (k_mr_cg_update<NR><<<mr_red_grid(n * NR), TPB, 0, s>>>(n, alpha, x, p, r, q, part)));
   final_sum(s, part, (int)mr_red_grid(n * nr), nr, rr_out);
+}
+void mr_cg_update_lo(hipStream_t s, int nr, int64_t n, const double *alpha, double *x, const double *p, double *r,
+                     const float *q, float *rf, double *part, double *rr_out) {
+  UFT_NR_SWITCH(nr, (k_mr_cg_update_lo<NR><<<mr_red_grid(n * NR), TPB, 0, s>>>(n, alpha, x, p, r, q, rf, part)));
+  final_sum(s, part, (int)mr_red_grid(n * nr), nr, rr_out);
+}
+void mr_take_res(hipStream_t s, int nr, int64_t n, unsigned mask, const double *res, double *r, float *rf,
+                 double *part, double *rr_out) {
+  UFT_NR_SWITCH(nr, (k_mr_take_res<NR><<<mr_red_grid(n * NR), TPB, 0, s>>>(n, mask, res, r, rf, part)));
+  final_sum(s, part, (int)mr_red_grid(n * nr), nr, rr_out);
+}
+void mr_xpby_lo(hipStream_t s, int nr, int64_t n, const double *z, const double *beta, double *p, float *pf) {
+  UFT_NR_SWITCH(nr, (k_mr_xpby_lo<NR><<<nblk(n * NR), TPB, 0, s>>>(n, z, beta, p, pf)));
 }
 void mr_xpby(hipStream_t s, int nr, int64_t n, const double *z, const double *beta, double *p) {
   UFT_NR_SWITCH(nr, (k_mr_xpby<NR><<<nblk(n * NR), TPB, 0, s>>>(n, z, beta, p)));

@@ -525,6 +525,22 @@ void mr_lev_op(uft_ctx *C, int l, const T *u, const RowOp<T> &op) {
 int mr_lev_blocks(const uft_ctx *C, int l) {
   return l == 0 ? uftk::mr_gather_blocks(C->nn, C->mr.nr) : uftk::mr_bsr_blocks(C->amg.L[l].nunit);
 }
+// mixed forms' outer product: q = K p with the FP32 element blocks (the
+// cycle's fine operator uses the BF16 matrix-core form), p.q per right-hand
+// side into the block partials
+void mr_outer_apply_lo(uft_ctx *C, const float *pf, float *qf) {
+  const int nr = C->mr.nr;
+  RowOp<float> op;
+  op.mode = ROW_STOREDOT;
+  op.y = qf;
+  op.part = C->d_part;
+  hipEvent_t *ev = prof_triple(C, 2);
+  if (ev) (void)hipEventRecord(ev[0], C->s);
+  uftk::mr_elem_phase(C->s, C, nr, C->cur_xphys, pf, (float *)C->mr.Ye, true);
+  if (ev) (void)hipEventRecord(ev[1], C->s);
+  uftk::mr_gather_op(C->s, C, nr, pf, (const float *)C->mr.Ye, op);
+  if (ev) (void)hipEventRecord(ev[2], C->s);
+}
 
 template <typename T>
 void mr_cycle_body(uft_ctx *C, int l, const T *b, T *x);
@@ -1040,7 +1056,12 @@ int solve_el_mixed(uft_ctx *C, const double *xphys, double *U) {
 // together (FP64 iterate, residual and inner products; per-RHS scalars); the
 // cycle runs in FP64 (UFT_FP64) or FP32 with matrix-core element products
 // (mixed forms).  A right-hand side stops updating once
-// ||F_k - K U_k|| <= rtol ||F_k||; the loop ends when all have.
+// ||F_k - K U_k|| <= rtol ||F_k||; the loop ends when all have.  The mixed
+// forms' outer product uses the FP32 element blocks, with per-case reliable
+// updates as in the single-case mixed solve: a case's residual is replaced
+// by its FP64 residual F_k - K U_k once the recursive one has fallen 100-fold
+// below the last FP64 one, or below the tolerance; a case stops only on its
+// FP64 residual.
 enum { MS_RZ = 0, MS_PQ = 8, MS_ALPHA = 16, MS_RR = 24, MS_RZN = 32, MS_BETA = 48, MS_FN = 56, MS_TR = 64 };
 int solve_el_block(uft_ctx *C, const double *xphys) {
   hipStream_t s = C->s;
@@ -1066,61 +1087,103 @@ int solve_el_block(uft_ctx *C, const double *xphys) {
     if (fn[k] > 0.0) mask |= 1u << k;
   }
   const bool f32 = is_mixed(C);
-  auto precond = [&](const double *r, double *z) {
+  // z = B r; in the mixed forms the cycle reads the FP32 copy M.rf that the
+  // update (or the reliable update) wrote
+  auto precond = [&](double *z) {
     if (f32) {
-      uftk::d2f(s, n * nr, r, M.rf);
       mr_fine_cycle<float>(C, M.rf, M.zf);
       uftk::f2d(s, n * nr, M.zf, z);
     } else {
-      mr_fine_cycle<double>(C, r, z);
+      mr_fine_cycle<double>(C, M.r, z);
     }
   };
   double *z = M.z, *zo = M.zo;
   int it = 0;
   if (mask) {
-    precond(M.r, z);
+    if (f32) uftk::d2f(s, n * nr, M.r, M.rf);
+    precond(z);
     uftk::mr_dot(s, nr, n, M.r, z, C->d_part, sc + MS_RZ);
     uftk::copy(s, n * nr, z, M.p);
+    if (f32) uftk::d2f(s, n * nr, M.p, M.pf);
   }
   RowOp<double> pq;
   pq.mode = ROW_STOREDOT;
   pq.y = M.q;
   pq.part = C->d_part;
+  RowOp<double> tr;  // FP64 residuals F - K U into M.q
+  tr.mode = ROW_RES;
+  tr.base = M.F;
+  tr.r = M.q;
+  double rtrue[8];   // mixed: last FP64 residual norm per case
+  bool have_true = false;
+  for (int k = 0; k < nr; ++k) rtrue[k] = fn[k];
+  C->reliable = 0;
   while (mask && it < C->P.maxit_el) {
-    mr_lev_op<double>(C, 0, M.p, pq);  // q = K p and p.q per right-hand side
+    if (f32) {  // q = K p (FP32 blocks), p.q per right-hand side
+      mr_outer_apply_lo(C, M.pf, M.qf);
+    } else {    // q = K p and p.q per right-hand side
+      mr_lev_op<double>(C, 0, M.p, pq);
+    }
     uftk::final_sum(s, C->d_part, mr_lev_blocks(C, 0), nr, sc + MS_PQ);
     uftk::mr_alpha(s, nr, mask, sc + MS_RZ, sc + MS_PQ, sc + MS_ALPHA);
-    uftk::mr_cg_update(s, nr, n, sc + MS_ALPHA, M.U, M.p, M.r, M.q, C->d_part, sc + MS_RR);
+    if (f32)
+      uftk::mr_cg_update_lo(s, nr, n, sc + MS_ALPHA, M.U, M.p, M.r, M.qf, M.rf, C->d_part, sc + MS_RR);
+    else
+      uftk::mr_cg_update(s, nr, n, sc + MS_ALPHA, M.U, M.p, M.r, M.q, C->d_part, sc + MS_RR);
     ++it;
     if ((rc = fetch(C, sc + MS_RR, nr, C->h_scal))) return rc;
+    unsigned need = 0;
     for (int k = 0; k < nr; ++k) {
       if (!((mask >> k) & 1u)) continue;
       const double rn = std::sqrt(C->h_scal[k]);
       if (!std::isfinite(rn)) return uft_set_error(C, UFT_ERR_NOCONV, "load case %d diverged at iteration %d", k, it);
       M.it[k] = it;
-      if (rn <= tol[k]) mask &= ~(1u << k);
+      if (!f32) {
+        if (rn <= tol[k]) mask &= ~(1u << k);
+      } else if (rn <= tol[k] || rn < 1e-2 * rtrue[k]) {
+        need |= 1u << k;
+      }
+    }
+    if (need) {  // reliable update of the cases in need
+      mr_lev_op<double>(C, 0, M.U, tr);
+      uftk::mr_take_res(s, nr, n, need, M.q, M.r, M.rf, C->d_part, sc + MS_RR);
+      if ((rc = fetch(C, sc + MS_RR, nr, C->h_scal))) return rc;
+      C->reliable++;
+      for (int k = 0; k < nr; ++k) {
+        if (!((need >> k) & 1u)) continue;
+        rtrue[k] = std::sqrt(C->h_scal[k]);
+        if (rtrue[k] <= tol[k]) mask &= ~(1u << k);
+      }
+      have_true = true;
     }
     if (!mask) break;
     std::swap(z, zo);
-    precond(M.r, z);
+    precond(z);
     uftk::mr_dot_pr(s, nr, n, M.r, z, zo, C->d_part, sc + MS_RZN);
     uftk::mr_beta(s, nr, mask, sc + MS_RZN, sc + MS_RZ, sc + MS_BETA);
-    uftk::mr_xpby(s, nr, n, z, sc + MS_BETA, M.p);
+    if (f32)
+      uftk::mr_xpby_lo(s, nr, n, z, sc + MS_BETA, M.p, M.pf);
+    else
+      uftk::mr_xpby(s, nr, n, z, sc + MS_BETA, M.p);
   }
   C->it_el = it;
-  C->outer_el = 1;
-  // true residuals F - K U per right-hand side (SolveFem.cpp:142-152)
-  RowOp<double> tr;
-  tr.mode = ROW_RES;
-  tr.base = M.F;
-  tr.r = M.q;
-  mr_lev_op<double>(C, 0, M.U, tr);
-  uftk::mr_dot(s, nr, n, M.q, M.q, C->d_part, sc + MS_TR);
-  if ((rc = fetch(C, sc + MS_TR, nr, C->h_scal))) return rc;
+  C->outer_el = f32 ? C->reliable : 1;
   double worst = 0.0;
-  for (int k = 0; k < nr; ++k) {
-    M.relres[k] = fn[k] > 0.0 ? std::sqrt(C->h_scal[k]) / fn[k] : 0.0;
-    worst = std::max(worst, M.relres[k]);
+  if (f32 && !mask && have_true) {
+    // every case stopped on its FP64 residual, U unchanged since
+    for (int k = 0; k < nr; ++k) {
+      M.relres[k] = fn[k] > 0.0 ? rtrue[k] / fn[k] : 0.0;
+      worst = std::max(worst, M.relres[k]);
+    }
+  } else {
+    // true residuals F - K U per right-hand side (SolveFem.cpp:142-152)
+    mr_lev_op<double>(C, 0, M.U, tr);
+    uftk::mr_dot(s, nr, n, M.q, M.q, C->d_part, sc + MS_TR);
+    if ((rc = fetch(C, sc + MS_TR, nr, C->h_scal))) return rc;
+    for (int k = 0; k < nr; ++k) {
+      M.relres[k] = fn[k] > 0.0 ? std::sqrt(C->h_scal[k]) / fn[k] : 0.0;
+      worst = std::max(worst, M.relres[k]);
+    }
   }
   C->relres_el = worst;
   C->fnorm = fn[0];
@@ -1854,6 +1917,8 @@ int uft_setup(uft_ctx *C) {
     if (is_mixed(C)) {
       if ((rc = dalloc_n(C, &M.rf, ndof * nr))) return rc;
       if ((rc = dalloc_n(C, &M.zf, ndof * nr))) return rc;
+      if ((rc = dalloc_n(C, &M.pf, ndof * nr))) return rc;
+      if ((rc = dalloc_n(C, &M.qf, ndof * nr))) return rc;
       if ((rc = dalloc_n(C, &C->d_KEb, (int64_t)uftk::ke_bf16_rows_words(npe) * ne))) return rc;
       uftk::ke_to_bf16(s, C);
     }

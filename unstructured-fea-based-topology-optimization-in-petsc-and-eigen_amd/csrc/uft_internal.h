@@ -123,6 +123,7 @@ struct MrState {
   double *F = nullptr, *U = nullptr;                                // ndof x nr
   double *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr, *zo = nullptr;
   float *rf = nullptr, *zf = nullptr;                               // mixed: FP32 copies for the cycle
+  float *pf = nullptr, *qf = nullptr;                               // mixed: FP32 outer product
   MrLevelBuf L[UFT_MAXLEV];                                         // cycle work (FP64 or FP32)
   void *Ye = nullptr;                                               // nd x ne x nr element-local scratch
   double *sc = nullptr;                                             // per right-hand-side scalars
@@ -409,7 +410,9 @@ void mr_cheb_first(hipStream_t s, int nr, int64_t n, const double *b, const doub
 void mr_cheb_first(hipStream_t s, int nr, int64_t n, const float *b, const float *dinv, double it, float *r, float *d,
                    float *x);
 void mr_elem_phase(hipStream_t s, const uft_ctx *C, int nr, const double *xphys, const double *u, double *Ye);
-void mr_elem_phase(hipStream_t s, const uft_ctx *C, int nr, const double *xphys, const float *u, float *Ye);
+// FP32 vectors: BF16 matrix-core products (8 cases, mixed forms) or, with blocks32, the FP32 element blocks
+void mr_elem_phase(hipStream_t s, const uft_ctx *C, int nr, const double *xphys, const float *u, float *Ye,
+                   bool blocks32 = false);
 void ke_to_bf16(hipStream_t s, const uft_ctx *C);
 int ke_bf16_rows_words(int npe);
 void mr_gather_op(hipStream_t s, const uft_ctx *C, int nr, const double *u, const double *Ye, const RowOp<double> &o);
@@ -447,6 +450,12 @@ void mr_dot_pr(hipStream_t s, int nr, int64_t n, const double *r, const double *
 void mr_cg_update(hipStream_t s, int nr, int64_t n, const double *alpha, double *x, const double *p, double *r,
                   const double *q, double *part, double *rr_out);
 void mr_xpby(hipStream_t s, int nr, int64_t n, const double *z, const double *beta, double *p);
+void mr_cg_update_lo(hipStream_t s, int nr, int64_t n, const double *alpha, double *x, const double *p, double *r,
+                     const float *q, float *rf, double *part, double *rr_out);
+// r_k = res_k (and rf_k) for the right-hand sides k in mask; r.r of every one into rr_out
+void mr_take_res(hipStream_t s, int nr, int64_t n, unsigned mask, const double *res, double *r, float *rf,
+                 double *part, double *rr_out);
+void mr_xpby_lo(hipStream_t s, int nr, int64_t n, const double *z, const double *beta, double *p, float *pf);
 void mr_alpha(hipStream_t s, int nr, unsigned mask, const double *rz, const double *pq, double *alpha);
 void mr_beta(hipStream_t s, int nr, unsigned mask, const double *rzn, double *rzprev, double *beta);
 void mr_sens(hipStream_t s, const uft_ctx *C, int nr, const double *xphys, const double *u, double *dfdx,
