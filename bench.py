@@ -77,6 +77,14 @@ def algorithmic_bytes_elem_phase(npe, ne, ndof, w=8, nrhs=1):
     return ne * (S * w + 4 * npe + 8 + (4 if w == 2 else 0)) + ndof * wv
 
 
+def algorithmic_bytes_fine_ell(blocks, ndof):
+    """One apply of the assembled FP16 fine operator: every nonzero 3x3 node
+    block once (nine FP16 entries and its int32 column), the row scales and u
+    read once (FP32), one FP32 output vector, and the fixed-DOF byte per node
+    (padding slots and the epilogue's further vectors not counted)."""
+    return blocks * (18 + 4) + ndof * 4 * 3 + ndof // 3
+
+
 def algorithmic_bytes_apply(npe, ne, ndof, w=8):
     S = 300 if npe == 8 else 78
     return ne * (S * w + 4 * npe + w) + 2 * ndof * w
@@ -141,6 +149,7 @@ def run_form(args, precision):
         ctx.sync()
         setup_s = time.time() - t0
     ne, ndof = ctx.size("ne"), ctx.size("ndof")
+    ell = ctx.fine_operator_info()
     ctx.init_design()
     itr = 0
     for _ in range(args.warmup):
@@ -169,13 +178,15 @@ def run_form(args, precision):
     L.uft_profile_enable(ctx.h, 0)
     ctx.close()
     return {"npe": npe, "dims": dims, "seed": seed, "ne": ne, "ndof": ndof, "setup_s": setup_s, "dt": t1 - t0,
-            "its": its, "fxs": fxs, "prof": prof, "ndesign": ndesign, "per_design": per_design}
+            "its": its, "fxs": fxs, "prof": prof, "ndesign": ndesign, "per_design": per_design, "ell": ell}
 
 
-def elem_roofline(name, npe, ne, ndof, prof_elem, prof_node, word, kernel, traffic_key, nrhs=1):
-    """HBM roofline of an element phase from its HIP-event timings."""
+def elem_roofline(name, npe, ne, ndof, prof_elem, prof_node, word, kernel, traffic_key, nrhs=1, algo=None):
+    """HBM roofline of an element phase (or of the assembled operator's
+    apply, algo given) from its HIP-event timings."""
     n, ms = prof_elem
-    algo = algorithmic_bytes_elem_phase(npe, ne, ndof, word, nrhs)
+    if algo is None:
+        algo = algorithmic_bytes_elem_phase(npe, ne, ndof, word, nrhs)
     avg_s = ms / max(n, 1) / 1e3
     achieved = algo / avg_s / 1e9 if n else None
     traffic = None
@@ -245,8 +256,15 @@ def main():
             w, kname = 2, f"k_elem_apply_h<{npe}> (FP16-block element phase of the smoother, mixed form)"
         else:
             w, kname = 4, f"k_elem_apply_v<{npe},float> (FP32 element phase, mixed form)"
-        rooflines.append(elem_roofline(main_form, npe, ne, ndof, r["prof"][2], r["prof"][3], w, kname,
-                                       f"{args.workload}_{main_form}_elem_apply", nrhs))
+        if r["ell"][0]:  # assembled FP16 fine operator: one kernel per apply
+            rooflines.append(elem_roofline(
+                main_form, npe, ne, ndof, r["prof"][2], r["prof"][3], 2,
+                "k_fine_ell_op (assembled FP16 3x3 node blocks of the smoother, sliced ELL, whole apply with its "
+                "row epilogue, mixed form)", f"{args.workload}_{main_form}_fine_ell", nrhs,
+                algo=algorithmic_bytes_fine_ell(r["ell"][1], ndof)))
+        else:
+            rooflines.append(elem_roofline(main_form, npe, ne, ndof, r["prof"][2], r["prof"][3], w, kname,
+                                           f"{args.workload}_{main_form}_elem_apply", nrhs))
         if r["prof"][4][0]:  # FP32 outer operator, FP64 residual at reliable updates
             k32 = f"k_mr_elem<{npe},float,8>" if multi else f"k_elem_apply_v<{npe},float>"
             rooflines.append(elem_roofline("outer", npe, ne, ndof, r["prof"][4], r["prof"][5], 4,

@@ -80,6 +80,10 @@ typedef struct uft_params {
                           kernel. */
   int cheb_degree_coarse; /* Chebyshev degree on multigrid levels 2 and below (their applies are bound by
                              launch latency, not by bandwidth); 0 (default): cheb_degree */
+  int fine_assembled;  /* UFT_MIXED_H, one design, superposed load cases: 1 (default) the smoother's fine
+                          operator is assembled at each hierarchy refresh into FP16 3x3 node blocks scaled
+                          by the diagonal (one launch per apply, no element-local round trip); 0: the FP16
+                          element blocks.  Fixed at setup. */
 } uft_params;
 
 /* --- context ------------------------------------------------------------ */
@@ -175,6 +179,11 @@ int uft_element_kf(uft_ctx *ctx, int64_t e, double *kf, double *t);
  * hierarchy is internal to PETSc): level count (at most 8), unknowns per level
  * and, per level, the element count (level 0) or 6x6 block count. */
 int uft_solver_info(const uft_ctx *ctx, int *nlev, int64_t *n_per_level, int64_t *blocks_per_level);
+/* The smoother's assembled fine operator (fine_assembled; no counterpart in
+ * the reference): slots of its sliced-ELL storage (FP16 3x3 node blocks,
+ * padding included) and its nonzero node blocks; both 0 when the fine level
+ * uses element blocks. */
+int uft_fine_operator_info(const uft_ctx *ctx, int64_t *slots, int64_t *blocks);
 /* --- multi-load-case form (load_cases = 1; no counterpart in the reference,
  * which superposes its cases) --------------------------------------------- */
 /* Number of load cases; per case the Krylov iterations and final relative

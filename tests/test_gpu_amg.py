@@ -113,14 +113,17 @@ def test_coarse_inverse(dims, coarse_max):
     assert np.abs(inv @ A - np.eye(n)).max() <= tol
 
 
-def test_level_operators_in_each_precision():
+@pytest.mark.parametrize("assembled", [1, 0])
+def test_level_operators_in_each_precision(assembled):
     """Every multigrid level's stored operator in FP32 and FP16 (UFT_MIXED_H:
-    fine element blocks scaled per element, coarse levels above the coarsest
-    scaled to unit diagonal) against the FP64 one, with a non-smooth random
-    vector."""
+    the fine operator assembled into 3x3 node blocks with rows scaled by the
+    diagonal, or the element blocks scaled per element; coarse levels above
+    the coarsest scaled to unit diagonal) against the FP64 one, with a
+    non-smooth random vector."""
     from uft.capi import cantilever
-    print("[case] level operators FP64/FP32/FP16", flush=True)
-    ctx = cantilever(8, 24, 12, 12, jitter=0.15, seed=9, Emax=1.0, volfrac=0.3, rmin=0.2, precision=2)
+    print(f"[case] level operators FP64/FP32/FP16 fine_assembled={assembled}", flush=True)
+    ctx = cantilever(8, 24, 12, 12, jitter=0.15, seed=9, Emax=1.0, volfrac=0.3, rmin=0.2, precision=2,
+                     fine_assembled=assembled)
     x = np.random.default_rng(4).uniform(0.01, 1.0, ctx.size("ne"))
     ctx.set("XPhys", x)
     ctx.fem_obj_sens()

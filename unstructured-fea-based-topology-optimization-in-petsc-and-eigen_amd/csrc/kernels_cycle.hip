@@ -9,10 +9,12 @@
 // same launch multiplies, and every reduction keeps a fixed order.
 #include <hip/hip_fp16.h>
 
+#include "uft_elem.h"
 #include "uft_internal.h"
 #include "uft_reduce.h"
 
 namespace {
+using namespace uftd;
 
 constexpr int TPB = 256;
 inline unsigned nblk(int64_t n) { return (unsigned)((n + TPB - 1) / TPB); }
@@ -201,6 +203,117 @@ __global__ __launch_bounds__(TPB) void k_node_gather_op(int nn, int ne, const in
     }
   }
   epi_partials(op, dv);
+}
+
+// ---------------------------------------------------------------- assembled fine operator
+// Sliced ELL of 3x3 node blocks (FP16, each scalar row divided by its
+// diagonal dg_i): one node per lane and one slice per wavefront, so the slot
+// loop is uniform and every slot load of a wavefront reads consecutive
+// memory.  t_i = dg_i sum_s Ah_(n,s) u_col(n,s) in FP32; fixed DOFs have the
+// unit row (t = u), their columns are zero in the blocks.  Bytes per node:
+// slots x (16 + 2 + 4) plus the row scales and the epilogue's vectors.
+__device__ __forceinline__ float2 h2f2(unsigned w) { return __half22float2(*reinterpret_cast<const __half2 *>(&w)); }
+template <bool NT>
+__global__ __launch_bounds__(TPB) void k_fine_ell_op(int nn, const int64_t *__restrict__ off,
+                                                     const int32_t *__restrict__ width, const int32_t *__restrict__ col,
+                                                     const uint4 *__restrict__ v8, const uint16_t *__restrict__ v1,
+                                                     const float *__restrict__ dg, const uint8_t *__restrict__ fixm,
+                                                     const float *__restrict__ u, RowOp<float> op) {
+  const int n = blockIdx.x * TPB + threadIdx.x;
+  double dv[3] = {0.0, 0.0, 0.0};
+  if (n < nn) {
+    const int w = n / UFT_ELL_SLICE;
+    const int64_t base = off[w] + n % UFT_ELL_SLICE;
+    const int W = width[w];
+    float a0 = 0.f, a1 = 0.f, a2 = 0.f;
+#pragma unroll 3
+    for (int s = 0; s < W; ++s) {
+      const int64_t k = base + (int64_t)s * UFT_ELL_SLICE;
+      const int64_t j = col[k];
+      unsigned h[4];
+      if constexpr (NT) {
+        typedef unsigned v4u __attribute__((ext_vector_type(4)));
+        const v4u v = __builtin_nontemporal_load(reinterpret_cast<const v4u *>(&v8[k]));
+        h[0] = v.x; h[1] = v.y; h[2] = v.z; h[3] = v.w;
+      } else {
+        const uint4 v = v8[k];
+        h[0] = v.x; h[1] = v.y; h[2] = v.z; h[3] = v.w;
+      }
+      const float e8 = __half2float(__ushort_as_half(v1[k]));
+      const float x0 = u[3 * j], x1 = u[3 * j + 1], x2 = u[3 * j + 2];
+      const float2 e01 = h2f2(h[0]), e23 = h2f2(h[1]), e45 = h2f2(h[2]), e67 = h2f2(h[3]);
+      a0 += e01.x * x0 + e01.y * x1 + e23.x * x2;
+      a1 += e23.y * x0 + e45.x * x1 + e45.y * x2;
+      a2 += e67.x * x0 + e67.y * x1 + e8 * x2;
+    }
+    const float acc[3] = {a0, a1, a2};
+    const unsigned m = fixm[n];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const int64_t i = 3 * (int64_t)n + c;
+      const float ui = u[i];
+      row_epi(op, i, ((m >> c) & 1u) ? ui : dg[i] * acc[c], ui, dv);
+    }
+  }
+  epi_partials(op, dv);
+}
+
+// Assembly at a hierarchy refresh: lane (n, slot s) sums E(x_e) KE_e[a][b]
+// (3x3) over the slot's element contributions (incidence order, FP64),
+// zeroes fixed columns, divides row c by the diagonal 1 / dinv_3n+c and
+// stores FP16.  The element blocks are read from their FP32 copy and the
+// moduli E(x_e) from the refresh's per-element array.
+template <int NPE>
+__global__ __launch_bounds__(TPB) void k_fine_ell_fill(int nn, int ne, const int64_t *__restrict__ off,
+                                                       const int32_t *__restrict__ width,
+                                                       const int32_t *__restrict__ col,
+                                                       const int32_t *__restrict__ cptr,
+                                                       const int32_t *__restrict__ clist,
+                                                       const float2 *__restrict__ KEf,
+                                                       const double *__restrict__ Ee,
+                                                       const uint8_t *__restrict__ fixm,
+                                                       const double *__restrict__ dinv, uint4 *__restrict__ v8,
+                                                       uint16_t *__restrict__ v1, float *__restrict__ dg) {
+  constexpr int ND = 3 * NPE;
+  const int n = blockIdx.x * TPB + threadIdx.x;
+  const int s = blockIdx.y;
+  if (n >= nn) return;
+  const int w = n / UFT_ELL_SLICE;
+  if (s >= width[w]) return;
+  const int64_t k = off[w] + (int64_t)s * UFT_ELL_SLICE + n % UFT_ELL_SLICE;
+  double b[9];
+#pragma unroll
+  for (int t = 0; t < 9; ++t) b[t] = 0.0;
+  for (int q = cptr[k]; q < cptr[k + 1]; ++q) {
+    const int code = clist[q];
+    const int lj = code % NPE, v = code / NPE;
+    const int e = v / NPE, li = v - e * NPE;
+    const double E = Ee[e];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const int p = pk_index(c * NPE + li, d * NPE + lj, ND);
+        const float2 pr = KEf[(size_t)(p >> 1) * ne + e];
+        b[3 * c + d] += E * (double)((p & 1) ? pr.y : pr.x);
+      }
+  }
+  const unsigned mj = fixm[col[k]];
+  unsigned short hb[9];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const double di = dinv[3 * (size_t)n + c];
+    const double dd = (di > 0.0 && di < 1e300) ? 1.0 / di : 1.0;
+    if (s == 0) dg[3 * (size_t)n + c] = (float)dd;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const double a = ((mj >> d) & 1u) ? 0.0 : b[3 * c + d] / dd;
+      hb[3 * c + d] = __half_as_ushort(__float2half_rn((float)a));
+    }
+  }
+  v8[k] = make_uint4((unsigned)hb[0] | ((unsigned)hb[1] << 16), (unsigned)hb[2] | ((unsigned)hb[3] << 16),
+                     (unsigned)hb[4] | ((unsigned)hb[5] << 16), (unsigned)hb[6] | ((unsigned)hb[7] << 16));
+  v1[k] = hb[8];
 }
 
 // ---------------------------------------------------------------- transfers
@@ -444,6 +557,24 @@ void gather_op(hipStream_t s, const uft_ctx *C, const float *u, const float *Ye,
     k_node_gather_op<4, float><<<nblk(nn), TPB, 0, s>>>(nn, ne, C->d_iptr, C->d_inc, Ye, C->d_fixm, u, op);
 }
 int gather_op_blocks(int64_t nn) { return (int)nblk(nn); }
+
+void fine_ell_fill(hipStream_t s, const uft_ctx *C, const double *Ee, const double *dinv) {
+  const int nn = (int)C->nn, ne = (int)C->ne;
+  const auto &F = C->fell;
+  const dim3 g(nblk(nn), (unsigned)F.wmax);
+  const float2 *KEf = (const float2 *)C->d_KEf;
+  if (C->npe == 8)
+    k_fine_ell_fill<8><<<g, TPB, 0, s>>>(nn, ne, F.d_off, F.d_width, F.d_col, F.d_cptr, F.d_clist,
+                                         KEf, Ee, C->d_fixm, dinv, F.d_v8, F.d_v1, F.d_dg);
+  else
+    k_fine_ell_fill<4><<<g, TPB, 0, s>>>(nn, ne, F.d_off, F.d_width, F.d_col, F.d_cptr, F.d_clist,
+                                         KEf, Ee, C->d_fixm, dinv, F.d_v8, F.d_v1, F.d_dg);
+}
+void fine_ell_op(hipStream_t s, const uft_ctx *C, const float *u, const RowOp<float> &op) {
+  const auto &F = C->fell;
+  k_fine_ell_op<true><<<nblk(C->nn), TPB, 0, s>>>((int)C->nn, F.d_off, F.d_width, F.d_col, F.d_v8, F.d_v1, F.d_dg,
+                                                  C->d_fixm, u, op);
+}
 
 template <typename T>
 static void restrict_first_t(hipStream_t s, int bs, int64_t nc, const int32_t *cptr, const int32_t *child,

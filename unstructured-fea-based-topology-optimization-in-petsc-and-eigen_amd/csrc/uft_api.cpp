@@ -269,6 +269,12 @@ void lev_op(uft_ctx *C, int l, const float *u, const RowOp<float> &op) {
   if (l == 0) {
     hipEvent_t *ev = prof_triple(C, 1);
     if (ev) (void)hipEventRecord(ev[0], C->s);
+    if (C->fell.d_col) {  // assembled FP16 blocks: one launch (timed as the element phase)
+      uftk::fine_ell_op(C->s, C, u, op);
+      if (ev) (void)hipEventRecord(ev[1], C->s);
+      if (ev) (void)hipEventRecord(ev[2], C->s);
+      return;
+    }
     if (C->d_KEh)
       uftk::elem_phase_h(C->s, C, C->elem_variant_h, C->cur_xphys, u, C->d_Yef);
     else
@@ -859,6 +865,8 @@ int amg_refresh(uft_ctx *C, const double *xphys) {
     if (Ln.d_valh)
       uftk::bsr6_to_half(s, Ln.nblk, Ln.nunit, Ln.d_brow, Ln.d_ci, Ln.d_val, Ln.d_dinv, Ln.d_valh, Ln.d_dh);
   }
+  // the level 0 -> 1 Galerkin product has written E(x_e) to d_Ee
+  if (C->fell.d_col) uftk::fine_ell_fill(s, C, C->d_Ee, H.L[0].d_dinv);
   AmgLevel &Lc = H.L[H.nlev - 1];
   if (C->bt.nseg) {
     BatchState &B = C->bt;
@@ -1549,6 +1557,7 @@ void uft_default_params(uft_params *p) {
   p->load_cases = 0;
   p->graph = 0;
   p->cheb_degree_coarse = 0;
+  p->fine_assembled = 1;
 }
 
 int uft_create(uft_ctx **out, int device) {
@@ -1594,15 +1603,18 @@ int uft_params_set(uft_ctx *C, const uft_params *p) {
   if (p->load_cases != 0 && p->load_cases != 1)
     return uft_set_error(C, UFT_ERR_ARG, "load_cases must be 0 (superposed) or 1 (separate)");
   if (p->graph != 0 && p->graph != 1) return uft_set_error(C, UFT_ERR_ARG, "graph must be 0 or 1");
+  if (p->fine_assembled != 0 && p->fine_assembled != 1)
+    return uft_set_error(C, UFT_ERR_ARG, "fine_assembled must be 0 or 1");
   if (p->cheb_degree < 1 || p->cheb_degree > 16 || p->cheb_degree_coarse < 0 || p->cheb_degree_coarse > 16 ||
       p->cycle_type < 0 || p->cycle_type > 3)
     return uft_set_error(C, UFT_ERR_ARG, "Chebyshev degrees must be in [1, 16] (coarse: 0 or [1, 16]), cycle_type in [0, 3]");
   if (p->precision < UFT_FP64 || p->precision > UFT_MIXED_H)
     return uft_set_error(C, UFT_ERR_ARG, "precision must be UFT_FP64, UFT_MIXED or UFT_MIXED_H");
   if (C->is_setup && (p->m != C->P.m || p->rmin != C->P.rmin || p->nu != C->P.nu || p->precision != C->P.precision ||
-                      p->load_cases != C->P.load_cases ||
+                      p->load_cases != C->P.load_cases || p->fine_assembled != C->P.fine_assembled ||
                       p->amg_levels != C->P.amg_levels || p->coarse_max != C->P.coarse_max))
-    return uft_set_error(C, UFT_ERR_STATE, "m, rmin, nu, precision and the hierarchy shape are fixed at setup");
+    return uft_set_error(C, UFT_ERR_STATE,
+                         "m, rmin, nu, precision, fine_assembled and the hierarchy shape are fixed at setup");
   C->P = *p;
   return UFT_OK;
 }
@@ -1678,6 +1690,26 @@ int uft_setup(uft_ctx *C) {
     uft_node_incidence(nn, ne, npe, C->elem.data(), iptr, inc);
     if ((rc = upload(C, &C->d_iptr, iptr))) return rc;
     if ((rc = upload(C, &C->d_inc, inc))) return rc;
+    // assembled FP16 fine operator of the smoother (one design, superposed loads)
+    if (C->P.precision == UFT_MIXED_H && C->P.fine_assembled && C->bt.nseg == 0 &&
+        !(C->P.load_cases == 1 && C->nlc > 1)) {
+      std::vector<int64_t> off;
+      std::vector<int32_t> width, col, cptr, clist;
+      if (uft_fine_ell_pattern(nn, npe, C->elem.data(), iptr, inc, off, width, col, cptr, clist)) {
+        auto &F = C->fell;
+        F.nslice = (int64_t)width.size();
+        F.nslot = off.back();
+        F.nnzb = 0;
+        for (int64_t i = 0; i < F.nslot; ++i) F.nnzb += cptr[i + 1] > cptr[i];
+        F.wmax = *std::max_element(width.begin(), width.end());
+        if ((rc = upload(C, &F.d_off, off)) || (rc = upload(C, &F.d_width, width)) || (rc = upload(C, &F.d_col, col)) ||
+            (rc = upload(C, &F.d_cptr, cptr)) || (rc = upload(C, &F.d_clist, clist)) ||
+            (rc = dalloc_n(C, &F.d_v8, F.nslot)) ||
+            (rc = dalloc_n(C, &F.d_v1, F.nslot)) || (rc = dalloc_n(C, &F.d_dg, ndof)))
+          return rc;
+        UFT_HIP(C, hipStreamSynchronize(s));  // the host arrays end with this scope
+      }
+    }
     UFT_HIP(C, hipStreamSynchronize(s));
   }
   // ---- design batch: segment grids of the element, node and DOF spaces
@@ -2468,6 +2500,14 @@ extern "C" int uft_solver_info(const uft_ctx *C, int *nlev, int64_t *n_per_level
   return UFT_OK;
 }
 
+extern "C" int uft_fine_operator_info(const uft_ctx *C, int64_t *slots, int64_t *blocks) {
+  if (!C || !slots || !blocks) return UFT_ERR_ARG;
+  if (!C->is_setup) return UFT_ERR_STATE;
+  *slots = C->fell.d_col ? C->fell.nslot : 0;
+  *blocks = C->fell.d_col ? C->fell.nnzb : 0;
+  return UFT_OK;
+}
+
 // ---- MMA settings, restart state, KKT residual (MMA.h:8-52)
 extern "C" int uft_mma_set_asymptotes(uft_ctx *C, double init, double decrease, double increase) {
   if (int rc = check_setup(C)) return rc;
@@ -2619,7 +2659,7 @@ extern "C" int uft_level_apply(uft_ctx *C, int level, int kind, const double *x,
     return uft_set_error(C, UFT_ERR_ARG, "uft_level_apply: bad arguments");
   AmgLevel &Lv = C->amg.L[level];
   if (kind > 0 && !is_mixed(C)) return uft_set_error(C, UFT_ERR_ARG, "uft_level_apply: FP64 form");
-  if (kind == 2 && !(level == 0 ? (void *)C->d_KEh : (void *)Lv.d_valh))
+  if (kind == 2 && !(level == 0 ? (void *)C->d_KEh : (void *)Lv.d_valh) && !(level == 0 && C->fell.d_col))
     return uft_set_error(C, UFT_ERR_ARG, "uft_level_apply: no FP16 operator on level %d", level);
   if (C->refreshes == 0) return uft_set_error(C, UFT_ERR_STATE, "uft_level_apply: no hierarchy refresh yet");
   hipStream_t s = C->s;
@@ -2632,7 +2672,9 @@ extern "C" int uft_level_apply(uft_ctx *C, int level, int kind, const double *x,
     RowOp<float> op;
     op.mode = ROW_STORE;
     op.y = Lv.tf;
-    if (level == 0) {
+    if (level == 0 && kind == 2 && C->fell.d_col) {
+      uftk::fine_ell_op(s, C, Lv.xf, op);
+    } else if (level == 0) {
       if (kind == 2)
         uftk::elem_phase_h(s, C, C->elem_variant_h, C->cur_xphys, Lv.xf, C->d_Yef);
       else

@@ -14,6 +14,7 @@
 #define UFT_RED_BLOCK 256
 #define UFT_TAIL_MAX 3072  // coarsest unknowns handled by the fused coarse tail (LDS)
 #define UFT_MR_TAIL_MAX 3072  // coarsest unknowns of the multi-load-case dense solve
+#define UFT_ELL_SLICE 64      // nodes per slice of the assembled fine operator (one wavefront)
 
 // Per-row epilogue of a fused operator apply t = A u (see kernels_cycle.hip):
 //  STORE     y = t
@@ -227,6 +228,20 @@ struct uft_ctx {
   double *d_Ee = nullptr;      // per element E(x_e) of the last hierarchy refresh
   float *d_Yef = nullptr;
   double TV = 0.0;
+  // assembled FP16 fine operator of the smoother (sliced ELL of 3x3 node
+  // blocks, rows scaled by the diagonal; fine_assembled)
+  struct {
+    int64_t nslice = 0, nslot = 0, nnzb = 0;
+    int wmax = 0;
+    int64_t *d_off = nullptr;    // per slice: first slot (nslice + 1)
+    int32_t *d_width = nullptr;  // per slice: slots per node
+    int32_t *d_col = nullptr;    // per slot: column node
+    int32_t *d_cptr = nullptr;   // per slot: first element contribution (nslot + 1)
+    int32_t *d_clist = nullptr;  // element contributions (e * npe + a) * npe + b
+    uint4 *d_v8 = nullptr;       // per slot: block entries 0..7 / diag(row), FP16
+    uint16_t *d_v1 = nullptr;    // per slot: entry 8
+    float *d_dg = nullptr;       // per scalar row: the diagonal (the row scale)
+  } fell;
 
   // vectors
   std::vector<double *> vec;   // by id
@@ -476,6 +491,11 @@ void bsr6_to_half(hipStream_t s, int64_t nblk, int64_t nb_rows, const int32_t *b
                   const double *val, const double *dinv, uint16_t *valh, float *dh);
 void gather_op(hipStream_t s, const uft_ctx *C, const double *u, const double *Ye, const RowOp<double> &op);
 void gather_op(hipStream_t s, const uft_ctx *C, const float *u, const float *Ye, const RowOp<float> &op);
+// assembled FP16 fine operator: assembly from the per-element moduli E(x_e)
+// (rows scaled by the level-0 diagonal 1 / dinv) and the apply with the row
+// epilogue (block partials over gather_op_blocks(nn) blocks)
+void fine_ell_fill(hipStream_t s, const uft_ctx *C, const double *Ee, const double *dinv);
+void fine_ell_op(hipStream_t s, const uft_ctx *C, const float *u, const RowOp<float> &op);
 int gather_op_blocks(int64_t nn);
 void restrict_first(hipStream_t s, int bs, int64_t nc, const int32_t *cptr, const int32_t *child, const double *P,
                     const double *r, double *bc, const double *dinv, double it, double *rc, double *dc, double *xc);
@@ -598,3 +618,13 @@ int uft_amg_build_host(uft_ctx *C, std::vector<AmgHostLevel> &H, std::vector<int
 // mesh helpers (uft_mesh.cpp)
 void uft_node_incidence(int64_t nn, int64_t ne, int npe, const int32_t *elem, std::vector<int32_t> &ptr,
                         std::vector<int32_t> &inc);
+// Sliced-ELL pattern of the assembled fine operator (3x3 node blocks): slices
+// of UFT_ELL_SLICE consecutive nodes, slot s of node n at
+// off[n / slice] + s * slice + n % slice, the row's neighbours ascending,
+// padding slots pointing at the node itself; slot i's element contributions
+// are clist[cptr[i] .. cptr[i + 1]), each (e * npe + a) * npe + b: element e
+// holds the row's node as local node a and the column's node as local node b.
+// False when a node has more than 255 neighbours or the codes overflow int32.
+bool uft_fine_ell_pattern(int64_t nn, int npe, const int32_t *elem, const std::vector<int32_t> &iptr,
+                          const std::vector<int32_t> &inc, std::vector<int64_t> &off, std::vector<int32_t> &width,
+                          std::vector<int32_t> &col, std::vector<int32_t> &cptr, std::vector<int32_t> &clist);

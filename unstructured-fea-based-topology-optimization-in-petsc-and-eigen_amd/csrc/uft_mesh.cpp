@@ -1,6 +1,7 @@
 // Mesh-side host code: the synthetic cantilever generator, the reference's
 // text formats (InputMeshData.cpp:100-149,228-285; GetBcs/GetBCs.m) and the
 // node -> (element, local node) incidence used by the atomic-free node phase.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -30,6 +31,79 @@ void uft_node_incidence(int64_t nn, int64_t ne, int npe, const int32_t *elem, st
   // element-major fill keeps each node's list in ascending element order
   for (int64_t e = 0; e < ne; ++e)
     for (int a = 0; a < npe; ++a) inc[pos[elem[e * npe + a]]++] = (int32_t)(e * npe + a);
+}
+
+bool uft_fine_ell_pattern(int64_t nn, int npe, const int32_t *elem, const std::vector<int32_t> &iptr,
+                          const std::vector<int32_t> &inc, std::vector<int64_t> &off, std::vector<int32_t> &width,
+                          std::vector<int32_t> &col, std::vector<int32_t> &cptr, std::vector<int32_t> &clist) {
+  const int64_t nslice = (nn + UFT_ELL_SLICE - 1) / UFT_ELL_SLICE;
+  std::vector<int64_t> rp(nn + 1, 0);
+  std::vector<int32_t> nb;  // sorted neighbour lists (the node itself included)
+  nb.reserve((size_t)nn * 27);
+  std::vector<int32_t> tmp;
+  for (int64_t n = 0; n < nn; ++n) {
+    tmp.clear();
+    for (int32_t k = iptr[n]; k < iptr[n + 1]; ++k) {
+      const int32_t e = inc[k] / npe;
+      for (int b = 0; b < npe; ++b) tmp.push_back(elem[(int64_t)e * npe + b]);
+    }
+    if (tmp.empty()) tmp.push_back((int32_t)n);
+    std::sort(tmp.begin(), tmp.end());
+    tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
+    if (tmp.size() > 255) return false;  // slots are stored as bytes
+    nb.insert(nb.end(), tmp.begin(), tmp.end());
+    rp[n + 1] = (int64_t)nb.size();
+  }
+  off.assign(nslice + 1, 0);
+  width.assign(nslice, 0);
+  for (int64_t w = 0; w < nslice; ++w) {
+    int wd = 0;
+    for (int64_t n = w * UFT_ELL_SLICE; n < std::min(nn, (w + 1) * UFT_ELL_SLICE); ++n)
+      wd = std::max(wd, (int)(rp[n + 1] - rp[n]));
+    width[w] = wd;
+    off[w + 1] = off[w] + (int64_t)wd * UFT_ELL_SLICE;
+  }
+  // padding slots point at the row's own node (their blocks are zero)
+  col.assign(off[nslice], 0);
+  for (int64_t n = 0; n < nn; ++n) {
+    const int64_t w = n / UFT_ELL_SLICE, lane = n % UFT_ELL_SLICE;
+    for (int s = 0; s < width[w]; ++s) {
+      const int64_t k = rp[n] + s;
+      col[off[w] + s * UFT_ELL_SLICE + lane] = k < rp[n + 1] ? nb[k] : (int32_t)n;
+    }
+  }
+  // contributions of each slot, in slot order: (e * npe + a) * npe + b for
+  // the element e holding the row's node as local node a and the column's
+  // node as local node b (incidence order of the row's node)
+  const int64_t total = off[nslice];
+  if ((int64_t)iptr[nn] * npe >= (int64_t)INT32_MAX) return false;
+  std::vector<int32_t> cnt(total + 1, 0);
+  std::vector<int32_t> slot((size_t)iptr[nn] * npe);
+  for (int64_t n = 0; n < nn; ++n) {
+    const int32_t *b0 = nb.data() + rp[n], *b1 = nb.data() + rp[n + 1];
+    const int64_t w = n / UFT_ELL_SLICE, lane = n % UFT_ELL_SLICE;
+    for (int32_t k = iptr[n]; k < iptr[n + 1]; ++k) {
+      const int32_t e = inc[k] / npe;
+      for (int b = 0; b < npe; ++b) {
+        const int s = (int)(std::lower_bound(b0, b1, elem[(int64_t)e * npe + b]) - b0);
+        const int64_t pos = off[w] + (int64_t)s * UFT_ELL_SLICE + lane;
+        slot[(size_t)k * npe + b] = (int32_t)s;
+        cnt[pos + 1]++;
+      }
+    }
+  }
+  for (int64_t i = 0; i < total; ++i) cnt[i + 1] += cnt[i];
+  cptr = cnt;
+  clist.assign(cnt[total], 0);
+  for (int64_t n = 0; n < nn; ++n) {
+    const int64_t w = n / UFT_ELL_SLICE, lane = n % UFT_ELL_SLICE;
+    for (int32_t k = iptr[n]; k < iptr[n + 1]; ++k)
+      for (int b = 0; b < npe; ++b) {
+        const int64_t pos = off[w] + (int64_t)slot[(size_t)k * npe + b] * UFT_ELL_SLICE + lane;
+        clist[cnt[pos]++] = inc[k] * npe + b;  // inc[k] = e * npe + a
+      }
+  }
+  return true;
 }
 
 extern "C" {

@@ -36,7 +36,8 @@ class Params(C.Structure):
                 ("precision", C.c_int), ("amg_levels", C.c_int), ("cheb_degree", C.c_int),
                 ("coarse_max", C.c_int), ("cycle_type", C.c_int),
                 ("cheb_lo", C.c_double), ("cheb_hi", C.c_double), ("esteig_steps", C.c_int),
-                ("load_cases", C.c_int), ("graph", C.c_int), ("cheb_degree_coarse", C.c_int)]
+                ("load_cases", C.c_int), ("graph", C.c_int), ("cheb_degree_coarse", C.c_int),
+                ("fine_assembled", C.c_int)]
 
 
 _LIB = None
@@ -102,6 +103,7 @@ def lib():
         "uft_gen_box": (C.c_int, [C.c_int] * 4 + [C.c_double] * 4 + [C.c_uint64, dp, ip]),
         "uft_box_bcs": (C.c_int, [C.c_int] * 4 + [ip, pi, ip, dp, pi]),
         "uft_solver_info": (C.c_int, [vp, pi, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+        "uft_fine_operator_info": (C.c_int, [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
         "uft_amg_operator_get": (C.c_int, [vp, C.c_int, ip, ip, dp]),
         "uft_amg_transfer_get": (C.c_int, [vp, C.c_int, ip, dp]),
         "uft_amg_coarse_inverse_get": (C.c_int, [vp, dp]),
@@ -436,6 +438,13 @@ class Context:
         b = (C.c_int64 * 8)()
         self._chk(self.L.uft_solver_info(self.h, C.byref(nl), n, b), "uft_solver_info")
         return [n[i] for i in range(nl.value)], [b[i] for i in range(nl.value)]
+
+    def fine_operator_info(self):
+        """(sliced-ELL slots, nonzero node blocks) of the assembled FP16 fine
+        operator; (0, 0) with the element form."""
+        s, b = C.c_int64(), C.c_int64()
+        self._chk(self.L.uft_fine_operator_info(self.h, C.byref(s), C.byref(b)), "uft_fine_operator_info")
+        return s.value, b.value
 
     def amg_operator(self, level):
         """BSR (rp, ci, val[nblk, 6, 6]) of multigrid level >= 1."""
