@@ -19,66 +19,79 @@ using namespace uftd;
 constexpr int TPB = 256;
 inline unsigned nblk(int64_t n) { return (unsigned)((n + TPB - 1) / TPB); }
 
-// t = (A u)_i ; ui = u_i
-template <typename T>
-__device__ __forceinline__ void row_epi(const RowOp<T> &op, int64_t i, T t, T ui, double (&dv)[3]) {
-  switch (op.mode) {
-    case ROW_STORE:
-      op.y[i] = t;
-      break;
-    case ROW_STEP: {  // r -= t ; d' = c1 u + c2 D^-1 r ; x += d' (+ u)
-      const T ri = op.r[i] - t;
-      op.r[i] = ri;
-      const T dn = (T)op.c1 * ui + (T)op.c2 * op.dinv[i] * ri;
-      op.dout[i] = dn;
-      op.x[i] += op.addprev ? dn + ui : dn;
-      break;
+// Per-row epilogue of an operator apply, t = (A u)_i, the mode (RowOp) fixed
+// at compile time: the operands that do not depend on the product (EpiPre,
+// u_i included) are loaded when the kernel starts, so their latency overlaps
+// the product's gathers instead of following them (the coarse levels'
+// launches are latency bound).
+template <typename T, int MODE>
+struct EpiPre {
+  T ui = T(0), a = T(0), b = T(0), c = T(0);
+  __device__ __forceinline__ void load(const RowOp<T> &op, const T *u, int64_t i) {
+    ui = u[i];
+    if constexpr (MODE == ROW_STEP) {
+      a = op.r[i];
+      b = op.dinv[i];
+      c = op.x[i];
+    } else if constexpr (MODE == ROW_RES) {
+      a = op.base[i];
+    } else if constexpr (MODE == ROW_RESFIRST || MODE == ROW_RESFIRSTX) {
+      a = op.base[i];
+      b = op.dinv[i];
+    } else if constexpr (MODE == ROW_DOTS2) {
+      a = op.w1[i];
+    } else if constexpr (MODE == ROW_DOTS3) {
+      a = op.w1[i];
+      b = op.w2[i];
     }
-    case ROW_RES:  // r = base - t
-      op.r[i] = op.base[i] - t;
-      break;
-    case ROW_RESFIRST: {  // r = base - t ; d = it D^-1 r
-      const T ri = op.base[i] - t;
-      op.r[i] = ri;
-      op.dout[i] = (T)op.it * op.dinv[i] * ri;
-      break;
-    }
-    case ROW_RESFIRSTX: {  // kr = r = base - t ; d = kd = it D^-1 r
-      const T ri = op.base[i] - t;
-      op.r[i] = ri;
-      op.kr[i] = ri;
-      const T dn = (T)op.it * op.dinv[i] * ri;
-      op.dout[i] = dn;
-      op.kd[i] = dn;
-      break;
-    }
-    case ROW_DOTS2:  // y = t ; u.t, u.w1
-      op.y[i] = t;
-      dv[0] += (double)ui * (double)t;
-      dv[1] += (double)ui * (double)op.w1[i];
-      break;
-    case ROW_DOTS3:  // u.w1, u.t, u.w2
-      dv[0] += (double)ui * (double)op.w1[i];
-      dv[1] += (double)ui * (double)t;
-      dv[2] += (double)ui * (double)op.w2[i];
-      break;
-    case ROW_STOREDOT:  // y = t ; u.t
-      op.y[i] = t;
-      dv[0] += (double)ui * (double)t;
-      break;
-    default:
-      break;
+  }
+};
+template <typename T, int MODE>
+__device__ __forceinline__ void row_epi_m(const RowOp<T> &op, int64_t i, T t, const EpiPre<T, MODE> &p,
+                                          double (&dv)[3]) {
+  const T ui = p.ui;
+  if constexpr (MODE == ROW_STORE) {
+    op.y[i] = t;
+  } else if constexpr (MODE == ROW_STEP) {
+    const T ri = p.a - t;
+    op.r[i] = ri;
+    const T dn = (T)op.c1 * ui + (T)op.c2 * p.b * ri;
+    op.dout[i] = dn;
+    op.x[i] = p.c + (op.addprev ? dn + ui : dn);
+  } else if constexpr (MODE == ROW_RES) {
+    op.r[i] = p.a - t;
+  } else if constexpr (MODE == ROW_RESFIRST) {
+    const T ri = p.a - t;
+    op.r[i] = ri;
+    op.dout[i] = (T)op.it * p.b * ri;
+  } else if constexpr (MODE == ROW_RESFIRSTX) {
+    const T ri = p.a - t;
+    op.r[i] = ri;
+    op.kr[i] = ri;
+    const T dn = (T)op.it * p.b * ri;
+    op.dout[i] = dn;
+    op.kd[i] = dn;
+  } else if constexpr (MODE == ROW_DOTS2) {
+    op.y[i] = t;
+    dv[0] += (double)ui * (double)t;
+    dv[1] += (double)ui * (double)p.a;
+  } else if constexpr (MODE == ROW_DOTS3) {
+    dv[0] += (double)ui * (double)p.a;
+    dv[1] += (double)ui * (double)t;
+    dv[2] += (double)ui * (double)p.b;
+  } else {  // ROW_STOREDOT
+    op.y[i] = t;
+    dv[0] += (double)ui * (double)t;
   }
 }
-
-template <typename T>
-__device__ __forceinline__ void epi_partials(const RowOp<T> &op, double (&dv)[3]) {
-  if (op.mode == ROW_DOTS2) {
+template <typename T, int MODE>
+__device__ __forceinline__ void epi_partials_m(const RowOp<T> &op, double (&dv)[3]) {
+  if constexpr (MODE == ROW_DOTS2) {
     double v2[2] = {dv[0], dv[1]};
     uftr::block_partials<2>(v2, op.part);
-  } else if (op.mode == ROW_DOTS3) {
+  } else if constexpr (MODE == ROW_DOTS3) {
     uftr::block_partials<3>(dv, op.part);
-  } else if (op.mode == ROW_STOREDOT) {
+  } else if constexpr (MODE == ROW_STOREDOT) {
     double v1[1] = {dv[0]};
     uftr::block_partials<1>(v1, op.part);
   }
@@ -88,13 +101,15 @@ __device__ __forceinline__ void epi_partials(const RowOp<T> &op, double (&dv)[3]
 // LPR lanes per scalar row, fixed shuffle-tree combine: 8 on large levels,
 // 32 on small ones so that the launch still spreads over the chip.
 inline int lpr_for(int64_t nb_rows) { return nb_rows * 6 < 65536 ? 32 : 8; }
-template <typename T, int LPR>
+template <typename T, int LPR, int MODE>
 __global__ __launch_bounds__(TPB) void k_bsr6_op(int64_t nrows, const int32_t *__restrict__ rp,
                                                  const int32_t *__restrict__ ci, const T *__restrict__ val,
                                                  const T *__restrict__ u, RowOp<T> op) {
   const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
   const int64_t row = t / LPR;
   const int l = (int)(t - row * LPR);
+  EpiPre<T, MODE> pre;
+  if (l == 0 && row < nrows * 6) pre.load(op, u, row);
   T acc = T(0);
   if (row < nrows * 6) {
     const int64_t A = row / 6;
@@ -110,8 +125,8 @@ __global__ __launch_bounds__(TPB) void k_bsr6_op(int64_t nrows, const int32_t *_
 #pragma unroll
   for (int off = LPR / 2; off > 0; off >>= 1) acc += __shfl_down(acc, off, LPR);
   double dv[3] = {0.0, 0.0, 0.0};
-  if (l == 0 && row < nrows * 6) row_epi(op, row, acc, u[row], dv);
-  epi_partials(op, dv);
+  if (l == 0 && row < nrows * 6) row_epi_m<T, MODE>(op, row, acc, pre, dv);
+  epi_partials_m<T, MODE>(op, dv);
 }
 
 // FP16 blocks of the symmetrically scaled operator D^-1/2 A D^-1/2 (every
@@ -121,7 +136,7 @@ __global__ __launch_bounds__(TPB) void k_bsr6_op(int64_t nrows, const int32_t *_
 // scaled entries of u once per block, accumulating all six rows; a fixed
 // butterfly combines the lanes and six lanes finish the six rows.
 inline int lpr_h(int64_t nb_rows) { return nb_rows < 4096 ? 32 : 16; }
-template <int LPR>
+template <int LPR, int MODE>
 __global__ __launch_bounds__(TPB) void k_bsr6_op_h(int64_t nbrows, const int32_t *__restrict__ rp,
                                                    const int32_t *__restrict__ ci, const uint2 *__restrict__ valh,
                                                    const float *__restrict__ dh, const float *__restrict__ u,
@@ -129,6 +144,12 @@ __global__ __launch_bounds__(TPB) void k_bsr6_op_h(int64_t nbrows, const int32_t
   const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
   const int64_t A = t / LPR;
   const int l = (int)(t - A * LPR);
+  EpiPre<float, MODE> pre;
+  float dhr = 0.f;
+  if (l < 6 && A < nbrows) {
+    pre.load(op, u, A * 6 + l);
+    dhr = dh[A * 6 + l];
+  }
   float acc[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
   if (A < nbrows) {
     for (int k = rp[A] + l; k < rp[A + 1]; k += LPR) {
@@ -169,16 +190,15 @@ __global__ __launch_bounds__(TPB) void k_bsr6_op_h(int64_t nbrows, const int32_t
 #pragma unroll
     for (int i = 1; i < 6; ++i)
       if (l == i) ti = acc[i];
-    const int64_t row = A * 6 + l;
-    row_epi(op, row, ti * dh[row], u[row], dv);
+    row_epi_m<float, MODE>(op, A * 6 + l, ti * dhr, pre, dv);
   }
-  epi_partials(op, dv);
+  epi_partials_m<float, MODE>(op, dv);
 }
 
 // ---------------------------------------------------------------- fine node phase
 // y_n = sum of the element-local results of node n (fixed incidence order);
 // fixed DOFs have the unit diagonal row (y = u).
-template <int NPE, typename T>
+template <int NPE, typename T, int MODE>
 __global__ __launch_bounds__(TPB) void k_node_gather_op(int nn, int ne, const int32_t *__restrict__ iptr,
                                                         const int32_t *__restrict__ inc, const T *__restrict__ Ye,
                                                         const uint8_t *__restrict__ fixm, const T *__restrict__ u,
@@ -186,6 +206,9 @@ __global__ __launch_bounds__(TPB) void k_node_gather_op(int nn, int ne, const in
   const int n = blockIdx.x * TPB + threadIdx.x;
   double dv[3] = {0.0, 0.0, 0.0};
   if (n < nn) {
+    EpiPre<T, MODE> pre[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) pre[c].load(op, u, 3 * (int64_t)n + c);
     T acc[3] = {T(0), T(0), T(0)};
     const int k0 = iptr[n], k1 = iptr[n + 1];
     for (int k = k0; k < k1; ++k) {
@@ -196,13 +219,10 @@ __global__ __launch_bounds__(TPB) void k_node_gather_op(int nn, int ne, const in
     }
     const unsigned m = fixm[n];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const int64_t i = 3 * (int64_t)n + c;
-      const T ui = u[i];
-      row_epi(op, i, ((m >> c) & 1u) ? ui : acc[c], ui, dv);
-    }
+    for (int c = 0; c < 3; ++c)
+      row_epi_m<T, MODE>(op, 3 * (int64_t)n + c, ((m >> c) & 1u) ? pre[c].ui : acc[c], pre[c], dv);
   }
-  epi_partials(op, dv);
+  epi_partials_m<T, MODE>(op, dv);
 }
 
 // ---------------------------------------------------------------- assembled fine operator
@@ -213,7 +233,7 @@ __global__ __launch_bounds__(TPB) void k_node_gather_op(int nn, int ne, const in
 // unit row (t = u), their columns are zero in the blocks.  Bytes per node:
 // slots x (16 + 2 + 4) plus the row scales and the epilogue's vectors.
 __device__ __forceinline__ float2 h2f2(unsigned w) { return __half22float2(*reinterpret_cast<const __half2 *>(&w)); }
-template <bool NT>
+template <bool NT, int MODE>
 __global__ __launch_bounds__(TPB) void k_fine_ell_op(int nn, const int64_t *__restrict__ off,
                                                      const int32_t *__restrict__ width, const int32_t *__restrict__ col,
                                                      const uint4 *__restrict__ v8, const uint16_t *__restrict__ v1,
@@ -222,6 +242,13 @@ __global__ __launch_bounds__(TPB) void k_fine_ell_op(int nn, const int64_t *__re
   const int n = blockIdx.x * TPB + threadIdx.x;
   double dv[3] = {0.0, 0.0, 0.0};
   if (n < nn) {
+    EpiPre<float, MODE> pre[3];
+    float dgr[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      pre[c].load(op, u, 3 * (int64_t)n + c);
+      dgr[c] = dg[3 * (int64_t)n + c];
+    }
     const int w = n / UFT_ELL_SLICE;
     const int64_t base = off[w] + n % UFT_ELL_SLICE;
     const int W = width[w];
@@ -249,13 +276,10 @@ __global__ __launch_bounds__(TPB) void k_fine_ell_op(int nn, const int64_t *__re
     const float acc[3] = {a0, a1, a2};
     const unsigned m = fixm[n];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const int64_t i = 3 * (int64_t)n + c;
-      const float ui = u[i];
-      row_epi(op, i, ((m >> c) & 1u) ? ui : dg[i] * acc[c], ui, dv);
-    }
+    for (int c = 0; c < 3; ++c)
+      row_epi_m<float, MODE>(op, 3 * (int64_t)n + c, ((m >> c) & 1u) ? pre[c].ui : dgr[c] * acc[c], pre[c], dv);
   }
-  epi_partials(op, dv);
+  epi_partials_m<float, MODE>(op, dv);
 }
 
 // Assembly at a hierarchy refresh: lane (n, slot s) sums E(x_e) KE_e[a][b]
@@ -515,13 +539,26 @@ __global__ __launch_bounds__(TAIL_TPB) void k_tail_solve(int nc_units, const int
 
 namespace uftk {
 
+// one instantiation per epilogue mode
+#define UFT_MODE_SWITCH(mode, CALL)                                   \
+  switch (mode) {                                                     \
+    case ROW_STORE: { constexpr int MODE = ROW_STORE; CALL; } break;  \
+    case ROW_STEP: { constexpr int MODE = ROW_STEP; CALL; } break;    \
+    case ROW_RES: { constexpr int MODE = ROW_RES; CALL; } break;      \
+    case ROW_RESFIRST: { constexpr int MODE = ROW_RESFIRST; CALL; } break; \
+    case ROW_RESFIRSTX: { constexpr int MODE = ROW_RESFIRSTX; CALL; } break; \
+    case ROW_DOTS2: { constexpr int MODE = ROW_DOTS2; CALL; } break;  \
+    case ROW_DOTS3: { constexpr int MODE = ROW_DOTS3; CALL; } break;  \
+    default: { constexpr int MODE = ROW_STOREDOT; CALL; } break;      \
+  }
+
 template <typename T>
 static void bsr6_op_t(hipStream_t s, int64_t nb_rows, const int32_t *rp, const int32_t *ci, const T *val, const T *u,
                       const RowOp<T> &op) {
   if (lpr_for(nb_rows) == 32)
-    k_bsr6_op<T, 32><<<nblk(nb_rows * 6 * 32), TPB, 0, s>>>(nb_rows, rp, ci, val, u, op);
+    UFT_MODE_SWITCH(op.mode, (k_bsr6_op<T, 32, MODE><<<nblk(nb_rows * 6 * 32), TPB, 0, s>>>(nb_rows, rp, ci, val, u, op)))
   else
-    k_bsr6_op<T, 8><<<nblk(nb_rows * 6 * 8), TPB, 0, s>>>(nb_rows, rp, ci, val, u, op);
+    UFT_MODE_SWITCH(op.mode, (k_bsr6_op<T, 8, MODE><<<nblk(nb_rows * 6 * 8), TPB, 0, s>>>(nb_rows, rp, ci, val, u, op)))
 }
 void bsr6_op(hipStream_t s, int64_t nb_rows, const int32_t *rp, const int32_t *ci, const double *val, const double *u,
              const RowOp<double> &op) {
@@ -536,25 +573,27 @@ void bsr6_op_h(hipStream_t s, int64_t nb_rows, const int32_t *rp, const int32_t 
                const float *dh, const float *u, const RowOp<float> &op) {
   const uint2 *vh = reinterpret_cast<const uint2 *>(valh);
   if (lpr_h(nb_rows) == 32)
-    k_bsr6_op_h<32><<<nblk(nb_rows * 32), TPB, 0, s>>>(nb_rows, rp, ci, vh, dh, u, op);
+    UFT_MODE_SWITCH(op.mode, (k_bsr6_op_h<32, MODE><<<nblk(nb_rows * 32), TPB, 0, s>>>(nb_rows, rp, ci, vh, dh, u, op)))
   else
-    k_bsr6_op_h<16><<<nblk(nb_rows * 16), TPB, 0, s>>>(nb_rows, rp, ci, vh, dh, u, op);
+    UFT_MODE_SWITCH(op.mode, (k_bsr6_op_h<16, MODE><<<nblk(nb_rows * 16), TPB, 0, s>>>(nb_rows, rp, ci, vh, dh, u, op)))
 }
 int bsr6_op_h_blocks(int64_t nb_rows) { return (int)nblk(nb_rows * lpr_h(nb_rows)); }
 
-void gather_op(hipStream_t s, const uft_ctx *C, const double *u, const double *Ye, const RowOp<double> &op) {
+template <typename T>
+static void gather_op_t(hipStream_t s, const uft_ctx *C, const T *u, const T *Ye, const RowOp<T> &op) {
   const int nn = (int)C->nn, ne = (int)C->ne;
   if (C->npe == 8)
-    k_node_gather_op<8, double><<<nblk(nn), TPB, 0, s>>>(nn, ne, C->d_iptr, C->d_inc, Ye, C->d_fixm, u, op);
+    UFT_MODE_SWITCH(op.mode, (k_node_gather_op<8, T, MODE><<<nblk(nn), TPB, 0, s>>>(nn, ne, C->d_iptr, C->d_inc, Ye,
+                                                                                    C->d_fixm, u, op)))
   else
-    k_node_gather_op<4, double><<<nblk(nn), TPB, 0, s>>>(nn, ne, C->d_iptr, C->d_inc, Ye, C->d_fixm, u, op);
+    UFT_MODE_SWITCH(op.mode, (k_node_gather_op<4, T, MODE><<<nblk(nn), TPB, 0, s>>>(nn, ne, C->d_iptr, C->d_inc, Ye,
+                                                                                    C->d_fixm, u, op)))
+}
+void gather_op(hipStream_t s, const uft_ctx *C, const double *u, const double *Ye, const RowOp<double> &op) {
+  gather_op_t<double>(s, C, u, Ye, op);
 }
 void gather_op(hipStream_t s, const uft_ctx *C, const float *u, const float *Ye, const RowOp<float> &op) {
-  const int nn = (int)C->nn, ne = (int)C->ne;
-  if (C->npe == 8)
-    k_node_gather_op<8, float><<<nblk(nn), TPB, 0, s>>>(nn, ne, C->d_iptr, C->d_inc, Ye, C->d_fixm, u, op);
-  else
-    k_node_gather_op<4, float><<<nblk(nn), TPB, 0, s>>>(nn, ne, C->d_iptr, C->d_inc, Ye, C->d_fixm, u, op);
+  gather_op_t<float>(s, C, u, Ye, op);
 }
 int gather_op_blocks(int64_t nn) { return (int)nblk(nn); }
 
@@ -572,8 +611,8 @@ void fine_ell_fill(hipStream_t s, const uft_ctx *C, const double *Ee, const doub
 }
 void fine_ell_op(hipStream_t s, const uft_ctx *C, const float *u, const RowOp<float> &op) {
   const auto &F = C->fell;
-  k_fine_ell_op<true><<<nblk(C->nn), TPB, 0, s>>>((int)C->nn, F.d_off, F.d_width, F.d_col, F.d_v8, F.d_v1, F.d_dg,
-                                                  C->d_fixm, u, op);
+  UFT_MODE_SWITCH(op.mode, (k_fine_ell_op<true, MODE><<<nblk(C->nn), TPB, 0, s>>>(
+                                (int)C->nn, F.d_off, F.d_width, F.d_col, F.d_v8, F.d_v1, F.d_dg, C->d_fixm, u, op)))
 }
 
 template <typename T>
