@@ -425,6 +425,85 @@ __global__ __launch_bounds__(TPB) void k_lincomb_first(int64_t n, const double *
   kd[i] = di;
 }
 
+// K-cycle coefficients of the second inner step from the sums of d.v, d.Ad,
+// d.r and the first step's rho1 = sc[0], alpha1 = sc[1] (the formulas of
+// k_final_kc<3>).
+__device__ __forceinline__ void kc2_coef(const double *sc, const double (&res)[3], double &c5, double &c6) {
+  const double rho1 = sc[0], a1 = sc[1], g = res[0], be = res[1], a2 = res[2];
+  const double t1 = rho1 != 0.0 ? a1 / rho1 : 0.0;
+  const double rho2 = rho1 != 0.0 ? be - g * g / rho1 : 0.0;
+  if (rho2 > 0.0) {
+    c5 = t1 - g * a2 / (rho1 * rho2);
+    c6 = a2 / rho2;
+  } else {
+    c5 = t1;
+    c6 = 0.0;
+  }
+}
+
+// k_final_kc<2> and k_lincomb_first in one launch: every block sums the nb
+// partial rows of c.v, c.b itself (the same fixed order in each block, so
+// all blocks agree) and block 0 stores sc[0], sc[1], sc[7].
+template <typename T>
+__global__ __launch_bounds__(TPB) void k_lincomb_first_kc(int64_t n, const double *__restrict__ part, int nb,
+                                                          double *__restrict__ sc, const T *__restrict__ b,
+                                                          const T *__restrict__ v, T *__restrict__ kr,
+                                                          T *__restrict__ r, const T *__restrict__ dinv, double it,
+                                                          T *__restrict__ d, T *__restrict__ kd) {
+  double res[2];
+  uftr::block_final<2>(part, nb, res);
+  const double c7 = res[0] != 0.0 ? -res[1] / res[0] : 0.0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    sc[0] = res[0];
+    sc[1] = res[1];
+    sc[7] = c7;
+  }
+  const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  if (i >= n) return;
+  const T c = (T)c7;
+  const T ri = b[i] + c * v[i];
+  kr[i] = ri;
+  r[i] = ri;
+  const T di = (T)it * dinv[i] * ri;
+  d[i] = di;
+  kd[i] = di;
+}
+
+// k_final_kc<3> and the K-cycle prolongation x_u += P_u (c5 kc + c6 kd) in one
+// launch (every block sums the partials; block 0 stores sc[2..6]).
+template <int BS, typename T>
+__global__ __launch_bounds__(TPB) void k_prolong_kc(int64_t nu, const int32_t *__restrict__ parent,
+                                                    const T *__restrict__ P, const double *__restrict__ part, int nb,
+                                                    double *__restrict__ sc, const T *__restrict__ xa,
+                                                    const T *__restrict__ xb, T *__restrict__ x) {
+  double res[3];
+  uftr::block_final<3>(part, nb, res);
+  double c5, c6;
+  kc2_coef(sc, res, c5, c6);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    sc[2] = res[0];
+    sc[3] = res[1];
+    sc[4] = res[2];
+    sc[5] = c5;
+    sc[6] = c6;
+  }
+  const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  if (t >= nu * BS) return;
+  const int64_t u = t / BS;
+  const int i = (int)(t - u * BS);
+  const int64_t A = parent[u];
+  if (A < 0) return;
+  const T ca = (T)c5, cb = (T)c6;
+  const T *Pu = P + (size_t)u * BS * 6 + i * 6;
+  T acc = T(0);
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+    const T c = ca * xa[A * 6 + j] + cb * xb[A * 6 + j];
+    acc += Pu[j] * c;
+  }
+  x[t] += acc;
+}
+
 // fixed-order sum of K partial columns by one block, then the K-cycle
 // coefficients (Notay & Vassilevski 2008, Alg. 2.1 with two inner steps):
 //  which 1: sc[0] = rho1 = c.v, sc[1] = alpha1 = c.b, sc[7] = -alpha1/rho1
@@ -440,19 +519,13 @@ __global__ __launch_bounds__(TPB) void k_final_kc(const double *__restrict__ par
     sc[1] = res[1];
     sc[7] = res[0] != 0.0 ? -res[1] / res[0] : 0.0;
   } else {
-    const double rho1 = sc[0], a1 = sc[1], g = res[0], be = res[1], a2 = res[2];
-    sc[2] = g;
-    sc[3] = be;
-    sc[4] = a2;
-    const double t1 = rho1 != 0.0 ? a1 / rho1 : 0.0;
-    const double rho2 = rho1 != 0.0 ? be - g * g / rho1 : 0.0;
-    if (rho2 > 0.0) {
-      sc[5] = t1 - g * a2 / (rho1 * rho2);
-      sc[6] = a2 / rho2;
-    } else {
-      sc[5] = t1;
-      sc[6] = 0.0;
-    }
+    double c5, c6;
+    kc2_coef(sc, res, c5, c6);
+    sc[2] = res[0];
+    sc[3] = res[1];
+    sc[4] = res[2];
+    sc[5] = c5;
+    sc[6] = c6;
   }
 }
 
@@ -469,16 +542,20 @@ constexpr int TAIL_CH = 512;  // children staged in LDS per pass
 // thread per coarse unknown then adds its unit's children in child order
 // (chunks of TAIL_CH children, in order).  Dense product: four groups of
 // threads take every fourth column, each with four independent
-// accumulators; fixed summation order throughout.
+// accumulators; fixed summation order throughout.  With xup (one design), the
+// same workgroup then prolongs: xup_u += P_u xc_parent(u) for the nu_up units
+// of the level above (the arithmetic of k_prolong_comb), saving a launch.
 template <typename T, typename TA>
 __global__ __launch_bounds__(TAIL_TPB) void k_tail_solve(int nc_units, const int32_t *__restrict__ cptr,
                                                          const int32_t *__restrict__ child, const T *__restrict__ P,
                                                          const TA *__restrict__ Ainv, const T *__restrict__ r,
                                                          T *__restrict__ xc, const int32_t *__restrict__ cu,
-                                                         const int64_t *__restrict__ coff) {
+                                                         const int64_t *__restrict__ coff, int64_t nu_up,
+                                                         const int32_t *__restrict__ parent, T *__restrict__ xup) {
   __shared__ double bc[UFT_TAIL_MAX];
   __shared__ double cv[TAIL_CH * 6];
   __shared__ double part4[4][TAIL_TPB / 4];
+  __shared__ T xs[UFT_TAIL_MAX];
   int u0 = 0;
   if (cu) {
     u0 = cu[blockIdx.x];
@@ -530,8 +607,24 @@ __global__ __launch_bounds__(TAIL_TPB) void k_tail_solve(int nc_units, const int
     }
     part4[g][row0] = (a[0] + a[1]) + (a[2] + a[3]);
     __syncthreads();
-    if (g == 0 && row < n) xc[row] = (T)((part4[0][row0] + part4[1][row0]) + (part4[2][row0] + part4[3][row0]));
+    if (g == 0 && row < n) {
+      const T v = (T)((part4[0][row0] + part4[1][row0]) + (part4[2][row0] + part4[3][row0]));
+      xc[row] = v;
+      xs[row] = v;
+    }
     __syncthreads();
+  }
+  if (!xup) return;
+  for (int64_t t = threadIdx.x; t < nu_up * 6; t += TAIL_TPB) {
+    const int64_t u = t / 6;
+    const int i = (int)(t - u * 6);
+    const int64_t A = parent[u];
+    if (A < 0) continue;
+    const T *Pu = P + (size_t)u * 36 + i * 6;
+    T acc = T(0);
+#pragma unroll
+    for (int j = 0; j < 6; ++j) acc += Pu[j] * xs[A * 6 + j];
+    xup[t] += acc;
   }
 }
 
@@ -659,19 +752,58 @@ void lincomb_first(hipStream_t s, int64_t n, const double *sc, const float *b, c
 }
 
 void final_kc1(hipStream_t s, const double *part, int nb, double *sc) { k_final_kc<2><<<1, TPB, 0, s>>>(part, nb, sc); }
+template <typename T>
+static void lincomb_first_kc_t(hipStream_t s, int64_t n, const double *part, int nb, double *sc, const T *b,
+                               const T *v, T *kr, T *r, const T *dinv, double it, T *d, T *kd) {
+  k_lincomb_first_kc<T><<<nblk(n), TPB, 0, s>>>(n, part, nb, sc, b, v, kr, r, dinv, it, d, kd);
+}
+void lincomb_first_kc(hipStream_t s, int64_t n, const double *part, int nb, double *sc, const double *b,
+                      const double *v, double *kr, double *r, const double *dinv, double it, double *d, double *kd) {
+  lincomb_first_kc_t<double>(s, n, part, nb, sc, b, v, kr, r, dinv, it, d, kd);
+}
+void lincomb_first_kc(hipStream_t s, int64_t n, const double *part, int nb, double *sc, const float *b,
+                      const float *v, float *kr, float *r, const float *dinv, double it, float *d, float *kd) {
+  lincomb_first_kc_t<float>(s, n, part, nb, sc, b, v, kr, r, dinv, it, d, kd);
+}
+template <typename T>
+static void prolong_kc_t(hipStream_t s, int bs, int64_t nu, const int32_t *parent, const T *P, const double *part,
+                         int nb, double *sc, const T *xa, const T *xb, T *x) {
+  if (bs == 3)
+    k_prolong_kc<3, T><<<nblk(nu * 3), TPB, 0, s>>>(nu, parent, P, part, nb, sc, xa, xb, x);
+  else
+    k_prolong_kc<6, T><<<nblk(nu * 6), TPB, 0, s>>>(nu, parent, P, part, nb, sc, xa, xb, x);
+}
+void prolong_kc(hipStream_t s, int bs, int64_t nu, const int32_t *parent, const double *P, const double *part, int nb,
+                double *sc, const double *xa, const double *xb, double *x) {
+  prolong_kc_t<double>(s, bs, nu, parent, P, part, nb, sc, xa, xb, x);
+}
+void prolong_kc(hipStream_t s, int bs, int64_t nu, const int32_t *parent, const float *P, const double *part, int nb,
+                double *sc, const float *xa, const float *xb, float *x) {
+  prolong_kc_t<float>(s, bs, nu, parent, P, part, nb, sc, xa, xb, x);
+}
 void final_kc2(hipStream_t s, const double *part, int nb, double *sc) { k_final_kc<3><<<1, TPB, 0, s>>>(part, nb, sc); }
 
 void tail_solve(hipStream_t s, int nc_units, const int32_t *cptr, const int32_t *child, const double *P,
                 const double *Ainv, const double *r, double *xc, int nseg, const int32_t *cu, const int64_t *coff) {
-  k_tail_solve<double, double><<<nseg, TAIL_TPB, 0, s>>>(nc_units, cptr, child, P, Ainv, r, xc, cu, coff);
+  k_tail_solve<double, double><<<nseg, TAIL_TPB, 0, s>>>(nc_units, cptr, child, P, Ainv, r, xc, cu, coff, 0, nullptr,
+                                                          nullptr);
 }
 void tail_solve(hipStream_t s, int nc_units, const int32_t *cptr, const int32_t *child, const float *P,
                 const double *Ainv, const float *r, float *xc, int nseg, const int32_t *cu, const int64_t *coff) {
-  k_tail_solve<float, double><<<nseg, TAIL_TPB, 0, s>>>(nc_units, cptr, child, P, Ainv, r, xc, cu, coff);
+  k_tail_solve<float, double><<<nseg, TAIL_TPB, 0, s>>>(nc_units, cptr, child, P, Ainv, r, xc, cu, coff, 0, nullptr,
+                                                         nullptr);
 }
-void tail_solve(hipStream_t s, int nc_units, const int32_t *cptr, const int32_t *child, const float *P,
-                const float *Ainv, const float *r, float *xc) {
-  k_tail_solve<float, float><<<1, TAIL_TPB, 0, s>>>(nc_units, cptr, child, P, Ainv, r, xc, nullptr, nullptr);
+void tail_solve_prolong(hipStream_t s, int nc_units, const int32_t *cptr, const int32_t *child, const double *P,
+                        const double *Ainv, const double *r, double *xc, int64_t nu_up, const int32_t *parent,
+                        double *x) {
+  k_tail_solve<double, double><<<1, TAIL_TPB, 0, s>>>(nc_units, cptr, child, P, Ainv, r, xc, nullptr, nullptr, nu_up,
+                                                      parent, x);
+}
+void tail_solve_prolong(hipStream_t s, int nc_units, const int32_t *cptr, const int32_t *child, const float *P,
+                        const float *Ainv, const float *r, float *xc, int64_t nu_up, const int32_t *parent,
+                        float *x) {
+  k_tail_solve<float, float><<<1, TAIL_TPB, 0, s>>>(nc_units, cptr, child, P, Ainv, r, xc, nullptr, nullptr, nu_up,
+                                                    parent, x);
 }
 
 }  // namespace uftk

@@ -349,8 +349,13 @@ void coarse_body(uft_ctx *C, int l) {
     op.w1 = v.b;
     op.part = C->d_part;
     lev_op(C, l, v.kc, op);
-    uftk::final_kc1(C->s, C->d_part, lev_op_blocks(C, l, std::is_same<T, float>::value), sc);
-    uftk::lincomb_first(C->s, Lv.n, sc, v.b, v.kv, v.kr, v.r, v.dinv, it, v.d, v.kd);
+    const int nb = lev_op_blocks(C, l, std::is_same<T, float>::value);
+    if (nb <= 1024) {  // few partial rows: every block of the combination sums them itself
+      uftk::lincomb_first_kc(C->s, Lv.n, C->d_part, nb, sc, v.b, v.kv, v.kr, v.r, v.dinv, it, v.d, v.kd);
+    } else {
+      uftk::final_kc1(C->s, C->d_part, nb, sc);
+      uftk::lincomb_first(C->s, Lv.n, sc, v.b, v.kv, v.kr, v.r, v.dinv, it, v.d, v.kd);
+    }
     cycle_body<T>(C, l, v.kr, v.kd);
     RowOp<T> o3;
     o3.mode = ROW_DOTS3;
@@ -358,7 +363,9 @@ void coarse_body(uft_ctx *C, int l) {
     o3.w2 = v.kr;
     o3.part = C->d_part;
     lev_op(C, l, v.kd, o3);
-    uftk::final_kc2(C->s, C->d_part, lev_op_blocks(C, l, std::is_same<T, float>::value), sc);
+    // level >= 2: the final sums are taken by the prolongation into level
+    // l - 1 (prolong_from), which follows directly
+    if (l == 1) uftk::final_kc2(C->s, C->d_part, nb, sc);
   }
 }
 
@@ -376,8 +383,13 @@ void prolong_from(uft_ctx *C, int l, T *x) {
       uftk::prolong_comb(C->s, Lv.bs, Lv.nunit, Lv.d_parent, P, nullptr, -1, -1, vn.x, vn.kd, x);
       break;
     default:  // K-cycle: c5 kc + c6 kd
-      uftk::prolong_comb(C->s, Lv.bs, Lv.nunit, Lv.d_parent, P, C->d_scal + S_K + 8 * (l + 1), 5, 6, vn.kc, vn.kd,
-                         x);
+      if (l >= 1)  // the level's final K-cycle sums in the same launch (few partials per block)
+        uftk::prolong_kc(C->s, Lv.bs, Lv.nunit, Lv.d_parent, P, C->d_part,
+                         lev_op_blocks(C, l + 1, std::is_same<T, float>::value), C->d_scal + S_K + 8 * (l + 1),
+                         vn.kc, vn.kd, x);
+      else
+        uftk::prolong_comb(C->s, Lv.bs, Lv.nunit, Lv.d_parent, P, C->d_scal + S_K + 8 * (l + 1), 5, 6, vn.kc, vn.kd,
+                           x);
       break;
   }
 }
@@ -396,12 +408,13 @@ void coarse_direct(uft_ctx *C, int l, const T *r, T *x) {
     uftk::prolong_comb(C->s, Lv.bs, Lv.nunit, Lv.d_parent, P, nullptr, -1, -1, vn.x, (const T *)nullptr, x);
     return;
   }
-  if (Lv.bs == 6 && H.ncoarse <= UFT_TAIL_MAX) {
+  if (Lv.bs == 6 && H.ncoarse <= UFT_TAIL_MAX) {  // restriction, dense solve and prolongation: one launch
     if constexpr (std::is_same<T, float>::value)  // FP32 inverse inside the FP32 cycle
-      uftk::tail_solve(C->s, (int)Ln.nunit, Lv.d_cptr, Lv.d_child, P, H.d_cinvf, r, vn.x);
+      uftk::tail_solve_prolong(C->s, (int)Ln.nunit, Lv.d_cptr, Lv.d_child, P, H.d_cinvf, r, vn.x, Lv.nunit,
+                               Lv.d_parent, x);
     else
-      uftk::tail_solve(C->s, (int)Ln.nunit, Lv.d_cptr, Lv.d_child, P, H.d_cinv, r, vn.x);
-    uftk::prolong_comb(C->s, Lv.bs, Lv.nunit, Lv.d_parent, P, nullptr, -1, -1, vn.x, (const T *)nullptr, x);
+      uftk::tail_solve_prolong(C->s, (int)Ln.nunit, Lv.d_cptr, Lv.d_child, P, H.d_cinv, r, vn.x, Lv.nunit,
+                               Lv.d_parent, x);
     return;
   }
   k_restrict(C->s, Lv, Ln.nunit, r, vn.b);

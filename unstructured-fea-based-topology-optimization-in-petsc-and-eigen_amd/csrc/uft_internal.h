@@ -511,6 +511,16 @@ void lincomb_first(hipStream_t s, int64_t n, const double *sc, const float *b, c
                    const float *dinv, double it, float *d, float *kd);
 void final_kc1(hipStream_t s, const double *part, int nb, double *sc);
 void final_kc2(hipStream_t s, const double *part, int nb, double *sc);
+// final_kc1 + lincomb_first in one launch (nb partial rows summed by every block)
+void lincomb_first_kc(hipStream_t s, int64_t n, const double *part, int nb, double *sc, const double *b,
+                      const double *v, double *kr, double *r, const double *dinv, double it, double *d, double *kd);
+void lincomb_first_kc(hipStream_t s, int64_t n, const double *part, int nb, double *sc, const float *b,
+                      const float *v, float *kr, float *r, const float *dinv, double it, float *d, float *kd);
+// final_kc2 + the K-cycle prolongation x += P (c5 xa + c6 xb) in one launch
+void prolong_kc(hipStream_t s, int bs, int64_t nu, const int32_t *parent, const double *P, const double *part, int nb,
+                double *sc, const double *xa, const double *xb, double *x);
+void prolong_kc(hipStream_t s, int bs, int64_t nu, const int32_t *parent, const float *P, const double *part, int nb,
+                double *sc, const float *xa, const float *xb, float *x);
 // xc = Ainv P^T r on the coarsest level (one workgroup; ncoarse <= UFT_TAIL_MAX)
 // With a design batch (nseg > 1): one workgroup per design, units
 // [cu[d], cu[d+1]) and the inverse block at Ainv + coff[d].
@@ -520,9 +530,15 @@ void tail_solve(hipStream_t s, int nc_units, const int32_t *cptr, const int32_t 
 void tail_solve(hipStream_t s, int nc_units, const int32_t *cptr, const int32_t *child, const float *P,
                 const double *Ainv, const float *r, float *xc, int nseg = 1, const int32_t *cu = nullptr,
                 const int64_t *coff = nullptr);
-// the same with an FP32 copy of the inverse (single design, mixed forms)
-void tail_solve(hipStream_t s, int nc_units, const int32_t *cptr, const int32_t *child, const float *P,
-                const float *Ainv, const float *r, float *xc);
+// One design: the same (FP64 inverse in the FP64 cycle, its FP32 copy in the
+// FP32 cycle), then x_u += P_u xc_parent(u) over the nu_up units of the level
+// above in the same workgroup (the prolongation of coarse_direct).
+void tail_solve_prolong(hipStream_t s, int nc_units, const int32_t *cptr, const int32_t *child, const double *P,
+                        const double *Ainv, const double *r, double *xc, int64_t nu_up, const int32_t *parent,
+                        double *x);
+void tail_solve_prolong(hipStream_t s, int nc_units, const int32_t *cptr, const int32_t *child, const float *P,
+                        const float *Ainv, const float *r, float *xc, int64_t nu_up, const int32_t *parent,
+                        float *x);
 void fixed_unit_diag(hipStream_t s, int64_t nb_rows, const int32_t *rp, const int32_t *ci, double *val);
 // MMA
 void mma_movelimit(hipStream_t s, int64_t n, const double *x, double Xmin, double Xmax, double movlim,
