@@ -282,10 +282,49 @@ __global__ __launch_bounds__(TPB) void k_fine_ell_op(int nn, const int64_t *__re
   epi_partials_m<float, MODE>(op, dv);
 }
 
+// The same pattern with FP32 entries (no row scale): 40 bytes per slot.
+template <int MODE>
+__global__ __launch_bounds__(TPB) void k_fine_ell_op32(int nn, const int64_t *__restrict__ off,
+                                                       const int32_t *__restrict__ width,
+                                                       const int32_t *__restrict__ col, const float4 *__restrict__ w8,
+                                                       const float *__restrict__ w1, const uint8_t *__restrict__ fixm,
+                                                       const float *__restrict__ u, RowOp<float> op) {
+  const int n = blockIdx.x * TPB + threadIdx.x;
+  double dv[3] = {0.0, 0.0, 0.0};
+  if (n < nn) {
+    EpiPre<float, MODE> pre[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) pre[c].load(op, u, 3 * (int64_t)n + c);
+    const int w = n / UFT_ELL_SLICE;
+    const int64_t base = off[w] + n % UFT_ELL_SLICE;
+    const int W = width[w];
+    float a0 = 0.f, a1 = 0.f, a2 = 0.f;
+#pragma unroll 3
+    for (int s = 0; s < W; ++s) {
+      const int64_t k = base + (int64_t)s * UFT_ELL_SLICE;
+      const int64_t j = col[k];
+      typedef float v4f __attribute__((ext_vector_type(4)));
+      const v4f e0 = __builtin_nontemporal_load(reinterpret_cast<const v4f *>(&w8[2 * k]));
+      const v4f e1 = __builtin_nontemporal_load(reinterpret_cast<const v4f *>(&w8[2 * k + 1]));
+      const float e8 = __builtin_nontemporal_load(&w1[k]);
+      const float x0 = u[3 * j], x1 = u[3 * j + 1], x2 = u[3 * j + 2];
+      a0 += e0.x * x0 + e0.y * x1 + e0.z * x2;
+      a1 += e0.w * x0 + e1.x * x1 + e1.y * x2;
+      a2 += e1.z * x0 + e1.w * x1 + e8 * x2;
+    }
+    const float acc[3] = {a0, a1, a2};
+    const unsigned m = fixm[n];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      row_epi_m<float, MODE>(op, 3 * (int64_t)n + c, ((m >> c) & 1u) ? pre[c].ui : acc[c], pre[c], dv);
+  }
+  epi_partials_m<float, MODE>(op, dv);
+}
+
 // Assembly at a hierarchy refresh: lane (n, slot s) sums E(x_e) KE_e[a][b]
 // (3x3) over the slot's element contributions (incidence order, FP64),
 // zeroes fixed columns, divides row c by the diagonal 1 / dinv_3n+c and
-// stores FP16.  The element blocks are read from their FP32 copy and the
+// stores FP16 (and, with w8, the unscaled entries in FP32).  The element blocks are read from their FP32 copy and the
 // moduli E(x_e) from the refresh's per-element array.
 template <int NPE>
 __global__ __launch_bounds__(TPB) void k_fine_ell_fill(int nn, int ne, const int64_t *__restrict__ off,
@@ -297,7 +336,8 @@ __global__ __launch_bounds__(TPB) void k_fine_ell_fill(int nn, int ne, const int
                                                        const double *__restrict__ Ee,
                                                        const uint8_t *__restrict__ fixm,
                                                        const double *__restrict__ dinv, uint4 *__restrict__ v8,
-                                                       uint16_t *__restrict__ v1, float *__restrict__ dg) {
+                                                       uint16_t *__restrict__ v1, float *__restrict__ dg,
+                                                       float4 *__restrict__ w8, float *__restrict__ w1) {
   constexpr int ND = 3 * NPE;
   const int n = blockIdx.x * TPB + threadIdx.x;
   const int s = blockIdx.y;
@@ -323,6 +363,16 @@ __global__ __launch_bounds__(TPB) void k_fine_ell_fill(int nn, int ne, const int
       }
   }
   const unsigned mj = fixm[col[k]];
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+      if ((mj >> d) & 1u) b[3 * c + d] = 0.0;
+  if (w8) {
+    w8[2 * k] = make_float4((float)b[0], (float)b[1], (float)b[2], (float)b[3]);
+    w8[2 * k + 1] = make_float4((float)b[4], (float)b[5], (float)b[6], (float)b[7]);
+    w1[k] = (float)b[8];
+  }
   unsigned short hb[9];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
@@ -331,7 +381,7 @@ __global__ __launch_bounds__(TPB) void k_fine_ell_fill(int nn, int ne, const int
     if (s == 0) dg[3 * (size_t)n + c] = (float)dd;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      const double a = ((mj >> d) & 1u) ? 0.0 : b[3 * c + d] / dd;
+      const double a = b[3 * c + d] / dd;
       hb[3 * c + d] = __half_as_ushort(__float2half_rn((float)a));
     }
   }
@@ -697,10 +747,15 @@ void fine_ell_fill(hipStream_t s, const uft_ctx *C, const double *Ee, const doub
   const float2 *KEf = (const float2 *)C->d_KEf;
   if (C->npe == 8)
     k_fine_ell_fill<8><<<g, TPB, 0, s>>>(nn, ne, F.d_off, F.d_width, F.d_col, F.d_cptr, F.d_clist,
-                                         KEf, Ee, C->d_fixm, dinv, F.d_v8, F.d_v1, F.d_dg);
+                                         KEf, Ee, C->d_fixm, dinv, F.d_v8, F.d_v1, F.d_dg, F.d_w8, F.d_w1);
   else
     k_fine_ell_fill<4><<<g, TPB, 0, s>>>(nn, ne, F.d_off, F.d_width, F.d_col, F.d_cptr, F.d_clist,
-                                         KEf, Ee, C->d_fixm, dinv, F.d_v8, F.d_v1, F.d_dg);
+                                         KEf, Ee, C->d_fixm, dinv, F.d_v8, F.d_v1, F.d_dg, F.d_w8, F.d_w1);
+}
+void fine_ell_op32(hipStream_t s, const uft_ctx *C, const float *u, const RowOp<float> &op) {
+  const auto &F = C->fell;
+  UFT_MODE_SWITCH(op.mode, (k_fine_ell_op32<MODE><<<nblk(C->nn), TPB, 0, s>>>(
+                                (int)C->nn, F.d_off, F.d_width, F.d_col, F.d_w8, F.d_w1, C->d_fixm, u, op)))
 }
 void fine_ell_op(hipStream_t s, const uft_ctx *C, const float *u, const RowOp<float> &op) {
   const auto &F = C->fell;

@@ -184,14 +184,19 @@ void fine_apply_lo(uft_ctx *C, const double *xphys, const double *p, int slot) {
   hipStream_t s = C->s;
   uftk::d2f(s, C->ndof, p, C->d_pf);
   hipEvent_t *ev = prof_triple(C, 2);
-  if (ev) (void)hipEventRecord(ev[0], s);
-  uftk::elem_phase_f(s, C, C->elem_variant_f, xphys, C->d_pf, C->d_Yef);
-  if (ev) (void)hipEventRecord(ev[1], s);
   RowOp<float> op;
   op.mode = slot >= 0 ? ROW_STOREDOT : ROW_STORE;
   op.y = C->d_qf;
   op.part = C->d_part;
-  uftk::gather_op(s, C, C->d_pf, C->d_Yef, op);
+  if (ev) (void)hipEventRecord(ev[0], s);
+  if (C->fell.d_w8) {  // assembled FP32 blocks of the current hierarchy refresh: one launch
+    uftk::fine_ell_op32(s, C, C->d_pf, op);
+    if (ev) (void)hipEventRecord(ev[1], s);
+  } else {
+    uftk::elem_phase_f(s, C, C->elem_variant_f, xphys, C->d_pf, C->d_Yef);
+    if (ev) (void)hipEventRecord(ev[1], s);
+    uftk::gather_op(s, C, C->d_pf, C->d_Yef, op);
+  }
   if (ev) (void)hipEventRecord(ev[2], s);
   if (slot >= 0) uftk::final_sum(s, C->d_part, uftk::gather_op_blocks(C->nn), 1, C->d_scal + slot);
 }
@@ -1718,7 +1723,8 @@ int uft_setup(uft_ctx *C) {
         if ((rc = upload(C, &F.d_off, off)) || (rc = upload(C, &F.d_width, width)) || (rc = upload(C, &F.d_col, col)) ||
             (rc = upload(C, &F.d_cptr, cptr)) || (rc = upload(C, &F.d_clist, clist)) ||
             (rc = dalloc_n(C, &F.d_v8, F.nslot)) ||
-            (rc = dalloc_n(C, &F.d_v1, F.nslot)) || (rc = dalloc_n(C, &F.d_dg, ndof)))
+            (rc = dalloc_n(C, &F.d_v1, F.nslot)) || (rc = dalloc_n(C, &F.d_dg, ndof)) ||
+            (rc = dalloc_n(C, &F.d_w8, 2 * F.nslot)) || (rc = dalloc_n(C, &F.d_w1, F.nslot)))
           return rc;
         UFT_HIP(C, hipStreamSynchronize(s));  // the host arrays end with this scope
       }
@@ -2687,6 +2693,8 @@ extern "C" int uft_level_apply(uft_ctx *C, int level, int kind, const double *x,
     op.y = Lv.tf;
     if (level == 0 && kind == 2 && C->fell.d_col) {
       uftk::fine_ell_op(s, C, Lv.xf, op);
+    } else if (level == 0 && kind == 1 && C->fell.d_w8) {
+      uftk::fine_ell_op32(s, C, Lv.xf, op);
     } else if (level == 0) {
       if (kind == 2)
         uftk::elem_phase_h(s, C, C->elem_variant_h, C->cur_xphys, Lv.xf, C->d_Yef);

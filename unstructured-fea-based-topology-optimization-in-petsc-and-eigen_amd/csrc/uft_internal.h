@@ -241,6 +241,8 @@ struct uft_ctx {
     uint4 *d_v8 = nullptr;       // per slot: block entries 0..7 / diag(row), FP16
     uint16_t *d_v1 = nullptr;    // per slot: entry 8
     float *d_dg = nullptr;       // per scalar row: the diagonal (the row scale)
+    float4 *d_w8 = nullptr;      // per slot: FP32 block entries 0..7 (the outer operator)
+    float *d_w1 = nullptr;       // per slot: FP32 entry 8
   } fell;
 
   // vectors
@@ -496,6 +498,8 @@ void gather_op(hipStream_t s, const uft_ctx *C, const float *u, const float *Ye,
 // epilogue (block partials over gather_op_blocks(nn) blocks)
 void fine_ell_fill(hipStream_t s, const uft_ctx *C, const double *Ee, const double *dinv);
 void fine_ell_op(hipStream_t s, const uft_ctx *C, const float *u, const RowOp<float> &op);
+// the same pattern with FP32 entries (unscaled): the mixed forms' outer operator
+void fine_ell_op32(hipStream_t s, const uft_ctx *C, const float *u, const RowOp<float> &op);
 int gather_op_blocks(int64_t nn);
 void restrict_first(hipStream_t s, int bs, int64_t nc, const int32_t *cptr, const int32_t *child, const double *P,
                     const double *r, double *bc, const double *dinv, double it, double *rc, double *dc, double *xc);
