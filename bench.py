@@ -77,12 +77,12 @@ def algorithmic_bytes_elem_phase(npe, ne, ndof, w=8, nrhs=1):
     return ne * (S * w + 4 * npe + 8 + (4 if w == 2 else 0)) + ndof * wv
 
 
-def algorithmic_bytes_fine_ell(blocks, ndof):
-    """One apply of the assembled FP16 fine operator: every nonzero 3x3 node
-    block once (nine FP16 entries and its int32 column), the row scales and u
-    read once (FP32), one FP32 output vector, and the fixed-DOF byte per node
-    (padding slots and the epilogue's further vectors not counted)."""
-    return blocks * (18 + 4) + ndof * 4 * 3 + ndof // 3
+def algorithmic_bytes_fine_ell(blocks, ndof, col_bytes, entry_bytes=2, scaled=True):
+    """One apply of an assembled fine operator: every nonzero 3x3 node block
+    once (nine entries and its column index), u read once and one output
+    vector (FP32), the row scales of the FP16 form, and the fixed-DOF byte per
+    node (padding slots and the epilogue's further vectors not counted)."""
+    return blocks * (9 * entry_bytes + col_bytes) + ndof * 4 * (3 if scaled else 2) + ndof // 3
 
 
 def algorithmic_bytes_apply(npe, ne, ndof, w=8):
@@ -261,11 +261,17 @@ def main():
                 main_form, npe, ne, ndof, r["prof"][2], r["prof"][3], 2,
                 "k_fine_ell_op (assembled FP16 3x3 node blocks of the smoother, sliced ELL, whole apply with its "
                 "row epilogue, mixed form)", f"{args.workload}_{main_form}_fine_ell", nrhs,
-                algo=algorithmic_bytes_fine_ell(r["ell"][1], ndof)))
+                algo=algorithmic_bytes_fine_ell(r["ell"][1], ndof, r["ell"][2])))
         else:
             rooflines.append(elem_roofline(main_form, npe, ne, ndof, r["prof"][2], r["prof"][3], w, kname,
                                            f"{args.workload}_{main_form}_elem_apply", nrhs))
-        if r["prof"][4][0]:  # FP32 outer operator, FP64 residual at reliable updates
+        if r["prof"][4][0] and r["ell"][0]:  # FP32 outer operator from the assembled blocks
+            rooflines.append(elem_roofline(
+                "outer", npe, ne, ndof, r["prof"][4], r["prof"][5], 4,
+                "k_fine_ell_op32 (assembled FP32 3x3 node blocks of the outer operator, whole apply, mixed form)",
+                f"{args.workload}_fp32_fine_ell", nrhs,
+                algo=algorithmic_bytes_fine_ell(r["ell"][1], ndof, r["ell"][2], 4, False)))
+        elif r["prof"][4][0]:  # FP32 outer operator, FP64 residual at reliable updates
             k32 = f"k_mr_elem<{npe},float,8>" if multi else f"k_elem_apply_v<{npe},float>"
             rooflines.append(elem_roofline("outer", npe, ne, ndof, r["prof"][4], r["prof"][5], 4,
                                            f"{k32} (FP32 element phase of the outer operator, mixed form)",

@@ -46,3 +46,44 @@ def test_cycle_types_same_solution(O, cycle):
     rfx = ref.fem_obj_sens(x)[0]
     assert abs(fx - rfx) <= 1e-10 * abs(rfx)
     assert ctx.stats()["relres_el"] <= 1e-12
+
+
+@pytest.mark.parametrize("shuffle", [False, True])
+def test_assembled_fine_operator_any_numbering(shuffle):
+    """The assembled fine operators (FP16 smoother, FP32 outer product) give
+    the element form's compliance with the lexicographic node numbering (16-bit
+    column offsets) and with a random one (neighbours more than 32767 nodes
+    apart: 32-bit columns)."""
+    import uft
+    from uft.capi import Context
+    nx, ny, nz = 60, 24, 24
+    print(f"[case] assembled fine operator, shuffled numbering={shuffle}", flush=True)
+    xyz, elem = uft.gen_box(8, nx, ny, nz, jitter=0.1, seed=4)
+    fixed, ld, lv = uft.box_bcs(nx, ny, nz)
+    if shuffle:  # node n becomes perm[n]
+        perm = np.random.default_rng(8).permutation(len(xyz)).astype(np.int32)
+        xyz2 = np.empty_like(xyz)
+        xyz2[perm] = xyz
+        xyz, elem = xyz2, perm[elem]
+        remap = lambda d: 3 * perm[np.asarray(d) // 3] + np.asarray(d) % 3
+        fixed, ld = np.sort(remap(fixed)), remap(ld)
+    res = {}
+    for fa in (1, 0):
+        ctx = Context(0)
+        ctx.set_params(Context.default_params(Emax=1.0, volfrac=0.3, rmin=2.0 * 1.5 / nx, rtol_el=1e-10,
+                                              rtol_f=1e-10, precision=2, fine_assembled=fa))
+        ctx.set_mesh(elem, xyz)
+        ctx.set_bcs(fixed, ld, lv)
+        ctx.setup()
+        info = ctx.fine_operator_info()
+        if fa:
+            assert info[2] == (4 if shuffle else 2) and info[1] > 0 and info[0] >= info[1]
+        else:
+            assert info == (0, 0, 0)
+        ctx.init_design()
+        res[fa] = [ctx.design_iteration(it)[0] for it in (1, 2)]
+        assert ctx.stats()["relres_el"] <= 1e-10
+        ctx.close()
+    print("compliances assembled / element form", res[1], res[0])
+    for a, b in zip(res[1], res[0]):
+        assert abs(a - b) <= 1e-8 * abs(b)

@@ -231,11 +231,12 @@ __global__ __launch_bounds__(TPB) void k_node_gather_op(int nn, int ne, const in
 // loop is uniform and every slot load of a wavefront reads consecutive
 // memory.  t_i = dg_i sum_s Ah_(n,s) u_col(n,s) in FP32; fixed DOFs have the
 // unit row (t = u), their columns are zero in the blocks.  Bytes per node:
-// slots x (16 + 2 + 4) plus the row scales and the epilogue's vectors.
+// slots x (16 + 2 + 2) (16-bit column offsets; 4 when some neighbour is more
+// than 32767 nodes away) plus the row scales and the epilogue's vectors.
 __device__ __forceinline__ float2 h2f2(unsigned w) { return __half22float2(*reinterpret_cast<const __half2 *>(&w)); }
-template <bool NT, int MODE>
+template <bool NT, int MODE, typename CI>
 __global__ __launch_bounds__(TPB) void k_fine_ell_op(int nn, const int64_t *__restrict__ off,
-                                                     const int32_t *__restrict__ width, const int32_t *__restrict__ col,
+                                                     const int32_t *__restrict__ width, const CI *__restrict__ col,
                                                      const uint4 *__restrict__ v8, const uint16_t *__restrict__ v1,
                                                      const float *__restrict__ dg, const uint8_t *__restrict__ fixm,
                                                      const float *__restrict__ u, RowOp<float> op) {
@@ -256,7 +257,7 @@ __global__ __launch_bounds__(TPB) void k_fine_ell_op(int nn, const int64_t *__re
 #pragma unroll 3
     for (int s = 0; s < W; ++s) {
       const int64_t k = base + (int64_t)s * UFT_ELL_SLICE;
-      const int64_t j = col[k];
+      const int64_t j = std::is_same<CI, int16_t>::value ? n + (int64_t)col[k] : (int64_t)col[k];
       unsigned h[4];
       if constexpr (NT) {
         typedef unsigned v4u __attribute__((ext_vector_type(4)));
@@ -283,10 +284,10 @@ __global__ __launch_bounds__(TPB) void k_fine_ell_op(int nn, const int64_t *__re
 }
 
 // The same pattern with FP32 entries (no row scale): 40 bytes per slot.
-template <int MODE>
+template <int MODE, typename CI>
 __global__ __launch_bounds__(TPB) void k_fine_ell_op32(int nn, const int64_t *__restrict__ off,
                                                        const int32_t *__restrict__ width,
-                                                       const int32_t *__restrict__ col, const float4 *__restrict__ w8,
+                                                       const CI *__restrict__ col, const float4 *__restrict__ w8,
                                                        const float *__restrict__ w1, const uint8_t *__restrict__ fixm,
                                                        const float *__restrict__ u, RowOp<float> op) {
   const int n = blockIdx.x * TPB + threadIdx.x;
@@ -302,7 +303,7 @@ __global__ __launch_bounds__(TPB) void k_fine_ell_op32(int nn, const int64_t *__
 #pragma unroll 3
     for (int s = 0; s < W; ++s) {
       const int64_t k = base + (int64_t)s * UFT_ELL_SLICE;
-      const int64_t j = col[k];
+      const int64_t j = std::is_same<CI, int16_t>::value ? n + (int64_t)col[k] : (int64_t)col[k];
       typedef float v4f __attribute__((ext_vector_type(4)));
       const v4f e0 = __builtin_nontemporal_load(reinterpret_cast<const v4f *>(&w8[2 * k]));
       const v4f e1 = __builtin_nontemporal_load(reinterpret_cast<const v4f *>(&w8[2 * k + 1]));
@@ -754,13 +755,21 @@ void fine_ell_fill(hipStream_t s, const uft_ctx *C, const double *Ee, const doub
 }
 void fine_ell_op32(hipStream_t s, const uft_ctx *C, const float *u, const RowOp<float> &op) {
   const auto &F = C->fell;
-  UFT_MODE_SWITCH(op.mode, (k_fine_ell_op32<MODE><<<nblk(C->nn), TPB, 0, s>>>(
-                                (int)C->nn, F.d_off, F.d_width, F.d_col, F.d_w8, F.d_w1, C->d_fixm, u, op)))
+  if (F.d_cd)
+    UFT_MODE_SWITCH(op.mode, (k_fine_ell_op32<MODE, int16_t><<<nblk(C->nn), TPB, 0, s>>>(
+                                  (int)C->nn, F.d_off, F.d_width, F.d_cd, F.d_w8, F.d_w1, C->d_fixm, u, op)))
+  else
+    UFT_MODE_SWITCH(op.mode, (k_fine_ell_op32<MODE, int32_t><<<nblk(C->nn), TPB, 0, s>>>(
+                                  (int)C->nn, F.d_off, F.d_width, F.d_col, F.d_w8, F.d_w1, C->d_fixm, u, op)))
 }
 void fine_ell_op(hipStream_t s, const uft_ctx *C, const float *u, const RowOp<float> &op) {
   const auto &F = C->fell;
-  UFT_MODE_SWITCH(op.mode, (k_fine_ell_op<true, MODE><<<nblk(C->nn), TPB, 0, s>>>(
-                                (int)C->nn, F.d_off, F.d_width, F.d_col, F.d_v8, F.d_v1, F.d_dg, C->d_fixm, u, op)))
+  if (F.d_cd)
+    UFT_MODE_SWITCH(op.mode, (k_fine_ell_op<true, MODE, int16_t><<<nblk(C->nn), TPB, 0, s>>>(
+                                  (int)C->nn, F.d_off, F.d_width, F.d_cd, F.d_v8, F.d_v1, F.d_dg, C->d_fixm, u, op)))
+  else
+    UFT_MODE_SWITCH(op.mode, (k_fine_ell_op<true, MODE, int32_t><<<nblk(C->nn), TPB, 0, s>>>(
+                                  (int)C->nn, F.d_off, F.d_width, F.d_col, F.d_v8, F.d_v1, F.d_dg, C->d_fixm, u, op)))
 }
 
 template <typename T>

@@ -877,6 +877,17 @@ __global__ __launch_bounds__(TPB) void k_xpby(int64_t n, const T *__restrict__ z
   const double be = sc[ib];
   if (i < n) p[i] = z[i] + (T)be * p[i];
 }
+// p = z + beta p and its FP32 copy (the mixed outer operator's input)
+__global__ __launch_bounds__(TPB) void k_xpby_lo(int64_t n, const double *__restrict__ z, const double *__restrict__ sc,
+                                                 int ib, double *__restrict__ p, float *__restrict__ pf) {
+  const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  const double be = sc[ib];
+  if (i < n) {
+    const double v = z[i] + be * p[i];
+    p[i] = v;
+    pf[i] = (float)v;
+  }
+}
 __global__ __launch_bounds__(TPB) void k_axpy(int64_t n, double a, const double *__restrict__ x, double *__restrict__ y) {
   const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
   if (i < n) y[i] += a * x[i];
@@ -1354,6 +1365,9 @@ void cg_updatef(hipStream_t s, int64_t n, const double *sc, int ia, float *u, co
 }
 void xpby(hipStream_t s, int64_t n, const double *z, const double *sc, int ib, double *p) {
   k_xpby<double><<<nblk(n), TPB, 0, s>>>(n, z, sc, ib, p);
+}
+void xpby_lo(hipStream_t s, int64_t n, const double *z, const double *sc, int ib, double *p, float *pf) {
+  k_xpby_lo<<<nblk(n), TPB, 0, s>>>(n, z, sc, ib, p, pf);
 }
 void xpbyf(hipStream_t s, int64_t n, const float *z, const double *sc, int ib, float *p) {
   k_xpby<float><<<nblk(n), TPB, 0, s>>>(n, z, sc, ib, p);

@@ -179,10 +179,11 @@ void fine_apply(uft_ctx *C, const double *xphys, const double *x, double *y, con
 }
 
 // Outer operator product of the mixed forms with the FP32 element blocks:
-// qf = K(x) p in FP32 (p rounded to FP32), and p.q into sc[slot] (slot >= 0)
-void fine_apply_lo(uft_ctx *C, const double *xphys, const double *p, int slot) {
+// qf = K(x) p in FP32 (p rounded to FP32 into d_pf unless have_pf), and p.q
+// into sc[slot] (slot >= 0)
+void fine_apply_lo(uft_ctx *C, const double *xphys, const double *p, int slot, bool have_pf = false) {
   hipStream_t s = C->s;
-  uftk::d2f(s, C->ndof, p, C->d_pf);
+  if (!have_pf) uftk::d2f(s, C->ndof, p, C->d_pf);
   hipEvent_t *ev = prof_triple(C, 2);
   RowOp<float> op;
   op.mode = slot >= 0 ? ROW_STOREDOT : ROW_STORE;
@@ -1023,17 +1024,18 @@ int solve_el_mixed(uft_ctx *C, const double *xphys, double *U) {
   // latter has fallen 100-fold below the last true residual, and before the
   // solve is accepted (reliable updates: the recursion's drift stays 1e-5 of
   // the current residual, far below the tolerance).
-  auto head = [&]() {  // q = K p (FP32 blocks); alpha = rz / pq; U, r (and its FP32 copy rf) updated
-    fine_apply_lo(C, xphys, p, S_PQ);
+  auto head = [&]() {  // q = K p (FP32 blocks, d_pf = p); alpha = rz / pq; U, r (and its FP32 copy rf) updated
+    fine_apply_lo(C, xphys, p, S_PQ, true);
     uftk::cg_update_lo(s, n, C->d_scal, S_RZ, S_PQ, U, p, r, C->d_qf, rf, C->d_part, C->d_scal + S_RR);
   };
   auto next = [&]() {  // z = B rf in FP32; zo = previous z; r.z, r.zo; beta; p
     vcycle_f(C, 0, rf, zf);
     uftk::z_from_f32(s, n, zf, r, z, zo, C->d_part, C->d_scal + S_RZNEW);
     uftk::sc_beta_flex(s, C->d_scal, S_RZNEW, S_RZNEW + 1, S_RZ, S_BETA + 2);
-    uftk::xpby(s, n, z, C->d_scal, S_BETA + 2, p);
+    uftk::xpby_lo(s, n, z, C->d_scal, S_BETA + 2, p, C->d_pf);
     head();
   };
+  uftk::d2f(s, n, p, C->d_pf);  // the first direction's FP32 copy
   static const char tag = 0;
   double rtrue = bn;  // last FP64 residual norm
   C->reliable = 0;
@@ -1720,6 +1722,21 @@ int uft_setup(uft_ctx *C) {
         F.nnzb = 0;
         for (int64_t i = 0; i < F.nslot; ++i) F.nnzb += cptr[i + 1] > cptr[i];
         F.wmax = *std::max_element(width.begin(), width.end());
+        // 16-bit column offsets when the ordering keeps every neighbour within 32767 nodes
+        std::vector<int16_t> cd(col.size(), 0);
+        bool fits = true;
+        for (int64_t n = 0; n < nn && fits; ++n) {
+          const int64_t w = n / UFT_ELL_SLICE, lane = n % UFT_ELL_SLICE;
+          for (int q = 0; q < width[w]; ++q) {
+            const int64_t k = off[w] + (int64_t)q * UFT_ELL_SLICE + lane, dlt = (int64_t)col[k] - n;
+            if (dlt < INT16_MIN || dlt > INT16_MAX) {
+              fits = false;
+              break;
+            }
+            cd[k] = (int16_t)dlt;
+          }
+        }
+        if (fits && (rc = upload(C, &F.d_cd, cd))) return rc;
         if ((rc = upload(C, &F.d_off, off)) || (rc = upload(C, &F.d_width, width)) || (rc = upload(C, &F.d_col, col)) ||
             (rc = upload(C, &F.d_cptr, cptr)) || (rc = upload(C, &F.d_clist, clist)) ||
             (rc = dalloc_n(C, &F.d_v8, F.nslot)) ||
@@ -2519,11 +2536,13 @@ extern "C" int uft_solver_info(const uft_ctx *C, int *nlev, int64_t *n_per_level
   return UFT_OK;
 }
 
-extern "C" int uft_fine_operator_info(const uft_ctx *C, int64_t *slots, int64_t *blocks) {
-  if (!C || !slots || !blocks) return UFT_ERR_ARG;
+extern "C" int uft_fine_operator_info(const uft_ctx *C, int64_t *slots, int64_t *blocks, int *col_bytes) {
+  if (!C || !slots || !blocks || !col_bytes) return UFT_ERR_ARG;
   if (!C->is_setup) return UFT_ERR_STATE;
-  *slots = C->fell.d_col ? C->fell.nslot : 0;
-  *blocks = C->fell.d_col ? C->fell.nnzb : 0;
+  const bool on = C->fell.d_col != nullptr;
+  *slots = on ? C->fell.nslot : 0;
+  *blocks = on ? C->fell.nnzb : 0;
+  *col_bytes = on ? (C->fell.d_cd ? 2 : 4) : 0;
   return UFT_OK;
 }
 

@@ -236,6 +236,7 @@ struct uft_ctx {
     int64_t *d_off = nullptr;    // per slice: first slot (nslice + 1)
     int32_t *d_width = nullptr;  // per slice: slots per node
     int32_t *d_col = nullptr;    // per slot: column node
+    int16_t *d_cd = nullptr;     // per slot: column node - row node, when every one fits 16 bits
     int32_t *d_cptr = nullptr;   // per slot: first element contribution (nslot + 1)
     int32_t *d_clist = nullptr;  // element contributions (e * npe + a) * npe + b
     uint4 *d_v8 = nullptr;       // per slot: block entries 0..7 / diag(row), FP16
@@ -365,6 +366,8 @@ void scale(hipStream_t s, int64_t n, const double *a, const double *dinv, double
 void cg_update(hipStream_t s, int64_t n, const double *sc, int ialpha, double *u, const double *p,
                double *r, const double *q, double *part, double *rr_out);
 void xpby(hipStream_t s, int64_t n, const double *z, const double *sc, int ibeta, double *p);
+// the same, also writing pf = (float)p
+void xpby_lo(hipStream_t s, int64_t n, const double *z, const double *sc, int ibeta, double *p, float *pf);
 void axpy_inplace(hipStream_t s, int64_t n, double a, const double *x, double *y);
 void sub(hipStream_t s, int64_t n, const double *a, const double *b, double *c);
 void set_dofs(hipStream_t s, int n, const int32_t *idx, const double *val, double *x);
