@@ -80,10 +80,11 @@ typedef struct uft_params {
                           kernel. */
   int cheb_degree_coarse; /* Chebyshev degree on multigrid levels 2 and below (their applies are bound by
                              launch latency, not by bandwidth); 0 (default): cheb_degree */
-  int fine_assembled;  /* UFT_MIXED_H, one design, superposed load cases: 1 (default) the smoother's fine
-                          operator is assembled at each hierarchy refresh into FP16 3x3 node blocks scaled
-                          by the diagonal (one launch per apply, no element-local round trip); 0: the FP16
-                          element blocks.  Fixed at setup. */
+  int fine_assembled;  /* UFT_MIXED_H with superposed load cases (one design or a design batch): 1 (default)
+                          the fine operators are assembled at each hierarchy refresh into 3x3 node blocks,
+                          FP16 scaled by the diagonal for the smoother and FP32 for the outer product (one
+                          launch per apply, no element-local round trip); 0: the element blocks.  Fixed at
+                          setup. */
 } uft_params;
 
 /* --- context ------------------------------------------------------------ */

@@ -1710,9 +1710,9 @@ int uft_setup(uft_ctx *C) {
     uft_node_incidence(nn, ne, npe, C->elem.data(), iptr, inc);
     if ((rc = upload(C, &C->d_iptr, iptr))) return rc;
     if ((rc = upload(C, &C->d_inc, inc))) return rc;
-    // assembled FP16 fine operator of the smoother (one design, superposed loads)
-    if (C->P.precision == UFT_MIXED_H && C->P.fine_assembled && C->bt.nseg == 0 &&
-        !(C->P.load_cases == 1 && C->nlc > 1)) {
+    // assembled fine operators (UFT_MIXED_H, superposed loads; a design batch's
+    // designs share no node, so its concatenated mesh assembles the same way)
+    if (C->P.precision == UFT_MIXED_H && C->P.fine_assembled && !(C->P.load_cases == 1 && C->nlc > 1)) {
       std::vector<int64_t> off;
       std::vector<int32_t> width, col, cptr, clist;
       if (uft_fine_ell_pattern(nn, npe, C->elem.data(), iptr, inc, off, width, col, cptr, clist)) {
