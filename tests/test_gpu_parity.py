@@ -74,9 +74,13 @@ def test_obj_sens(O, npe):
     assert _rel(ctx.get("U"), ref.vec("U")) < 1e-9
 
 
-@pytest.mark.parametrize("npe", [8, 4])
-def test_filter(O, npe):
+@pytest.mark.parametrize("npe,assembled", [(8, 1), (4, 1), (8, 0), (4, 0)])
+def test_filter(O, npe, assembled):
+    """The Helmholtz filter, its operator assembled on the node pattern
+    (fine_assembled = 1) or applied element by element, against the oracle."""
+    from uft.capi import cantilever
     ctx, ref = _pair(O, npe, 8, 4, 4, rmin=0.375, **TIGHT)
+    ctx = cantilever(npe, 8, 4, 4, jitter=0.15, seed=11, load_kind=0, rmin=0.375, fine_assembled=assembled, **TIGHT)
     rng = np.random.default_rng(9)
     x = rng.uniform(0.0, 1.0, ref.ne)
     ctx.set("X", x)

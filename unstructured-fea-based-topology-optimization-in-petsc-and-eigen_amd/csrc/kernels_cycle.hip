@@ -391,6 +391,56 @@ __global__ __launch_bounds__(TPB) void k_fine_ell_fill(int nn, int ne, const int
   v1[k] = hb[8];
 }
 
+// The filter's Helmholtz operator on the same node pattern (FP64 scalar
+// entries, assembled once at setup: KF does not depend on the density):
+// y_n = sum_s H_(n,s) x_col(n,s); with part, the block partials of x.y.
+template <int NPE>
+__global__ __launch_bounds__(TPB) void k_helm_ell_fill(int nn, int ne, const int64_t *__restrict__ off,
+                                                       const int32_t *__restrict__ width,
+                                                       const int32_t *__restrict__ cptr,
+                                                       const int32_t *__restrict__ clist,
+                                                       const double2 *__restrict__ KF, double *__restrict__ hv) {
+  const int n = blockIdx.x * TPB + threadIdx.x;
+  const int s = blockIdx.y;
+  if (n >= nn) return;
+  const int w = n / UFT_ELL_SLICE;
+  if (s >= width[w]) return;
+  const int64_t k = off[w] + (int64_t)s * UFT_ELL_SLICE + n % UFT_ELL_SLICE;
+  double h = 0.0;
+  for (int q = cptr[k]; q < cptr[k + 1]; ++q) {
+    const int code = clist[q];
+    const int lj = code % NPE, v = code / NPE;
+    const int e = v / NPE, li = v - e * NPE;
+    const int p = pk_index(li, lj, NPE);
+    const double2 pr = KF[(size_t)(p >> 1) * ne + e];
+    h += (p & 1) ? pr.y : pr.x;
+  }
+  hv[k] = h;
+}
+template <typename CI>
+__global__ __launch_bounds__(TPB) void k_helm_ell(int nn, const int64_t *__restrict__ off,
+                                                  const int32_t *__restrict__ width, const CI *__restrict__ col,
+                                                  const double *__restrict__ hv, const double *__restrict__ x,
+                                                  double *__restrict__ y, double *__restrict__ part) {
+  const int n = blockIdx.x * TPB + threadIdx.x;
+  double ds[1] = {0.0};
+  if (n < nn) {
+    const int w = n / UFT_ELL_SLICE;
+    const int64_t base = off[w] + n % UFT_ELL_SLICE;
+    const int W = width[w];
+    double acc = 0.0;
+#pragma unroll 3
+    for (int s = 0; s < W; ++s) {
+      const int64_t k = base + (int64_t)s * UFT_ELL_SLICE;
+      const int64_t j = std::is_same<CI, int16_t>::value ? n + (int64_t)col[k] : (int64_t)col[k];
+      acc += __builtin_nontemporal_load(&hv[k]) * x[j];
+    }
+    y[n] = acc;
+    ds[0] = acc * x[n];
+  }
+  if (part) uftr::block_partials<1>(ds, part);
+}
+
 // ---------------------------------------------------------------- transfers
 // one wavefront per coarse unit A: bc_A = sum_children P_u^T r_u (fixed
 // order); with dinv given, also the next level's first Chebyshev step from a
@@ -761,6 +811,26 @@ void fine_ell_op32(hipStream_t s, const uft_ctx *C, const float *u, const RowOp<
   else
     UFT_MODE_SWITCH(op.mode, (k_fine_ell_op32<MODE, int32_t><<<nblk(C->nn), TPB, 0, s>>>(
                                   (int)C->nn, F.d_off, F.d_width, F.d_col, F.d_w8, F.d_w1, C->d_fixm, u, op)))
+}
+void helm_ell_fill(hipStream_t s, const uft_ctx *C) {
+  const int nn = (int)C->nn, ne = (int)C->ne;
+  const auto &F = C->fell;
+  const dim3 g(nblk(nn), (unsigned)F.wmax);
+  const double2 *KF = (const double2 *)C->d_KF;
+  if (C->npe == 8)
+    k_helm_ell_fill<8><<<g, TPB, 0, s>>>(nn, ne, F.d_off, F.d_width, F.d_cptr, F.d_clist, KF, F.d_hv);
+  else
+    k_helm_ell_fill<4><<<g, TPB, 0, s>>>(nn, ne, F.d_off, F.d_width, F.d_cptr, F.d_clist, KF, F.d_hv);
+}
+void helm_ell_apply(hipStream_t s, const uft_ctx *C, const double *x, double *y, double *dot_out) {
+  const auto &F = C->fell;
+  const int nn = (int)C->nn;
+  double *part = dot_out ? C->d_part : nullptr;
+  if (F.d_cd)
+    k_helm_ell<int16_t><<<nblk(nn), TPB, 0, s>>>(nn, F.d_off, F.d_width, F.d_cd, F.d_hv, x, y, part);
+  else
+    k_helm_ell<int32_t><<<nblk(nn), TPB, 0, s>>>(nn, F.d_off, F.d_width, F.d_col, F.d_hv, x, y, part);
+  if (dot_out) final_sum(s, part, (int)nblk(nn), 1, dot_out);
 }
 void fine_ell_op(hipStream_t s, const uft_ctx *C, const float *u, const RowOp<float> &op) {
   const auto &F = C->fell;

@@ -1254,6 +1254,10 @@ void sens(hipStream_t s, const uft_ctx *C, const double *xphys, const double *u,
 }
 
 void apply_KF(hipStream_t s, const uft_ctx *C, const double *x, double *y, double *dot_out) {
+  if (C->fell.d_hv) {  // assembled on the node pattern: one launch
+    helm_ell_apply(s, C, x, y, dot_out);
+    return;
+  }
   const int ne = (int)C->ne, nn = (int)C->nn;
   const double *dotw = dot_out ? x : nullptr;  // with dot_out: also x.y in the node phase
   double *part = dot_out ? C->d_part : nullptr;

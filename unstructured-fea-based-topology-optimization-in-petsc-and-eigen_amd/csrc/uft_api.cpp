@@ -275,7 +275,7 @@ void lev_op(uft_ctx *C, int l, const float *u, const RowOp<float> &op) {
   if (l == 0) {
     hipEvent_t *ev = prof_triple(C, 1);
     if (ev) (void)hipEventRecord(ev[0], C->s);
-    if (C->fell.d_col) {  // assembled FP16 blocks: one launch (timed as the element phase)
+    if (C->fell.d_v8) {  // assembled FP16 blocks: one launch (timed as the element phase)
       uftk::fine_ell_op(C->s, C, u, op);
       if (ev) (void)hipEventRecord(ev[1], C->s);
       if (ev) (void)hipEventRecord(ev[2], C->s);
@@ -885,7 +885,7 @@ int amg_refresh(uft_ctx *C, const double *xphys) {
       uftk::bsr6_to_half(s, Ln.nblk, Ln.nunit, Ln.d_brow, Ln.d_ci, Ln.d_val, Ln.d_dinv, Ln.d_valh, Ln.d_dh);
   }
   // the level 0 -> 1 Galerkin product has written E(x_e) to d_Ee
-  if (C->fell.d_col) uftk::fine_ell_fill(s, C, C->d_Ee, H.L[0].d_dinv);
+  if (C->fell.d_v8) uftk::fine_ell_fill(s, C, C->d_Ee, H.L[0].d_dinv);
   AmgLevel &Lc = H.L[H.nlev - 1];
   if (C->bt.nseg) {
     BatchState &B = C->bt;
@@ -1710,9 +1710,12 @@ int uft_setup(uft_ctx *C) {
     uft_node_incidence(nn, ne, npe, C->elem.data(), iptr, inc);
     if ((rc = upload(C, &C->d_iptr, iptr))) return rc;
     if ((rc = upload(C, &C->d_inc, inc))) return rc;
-    // assembled fine operators (UFT_MIXED_H, superposed loads; a design batch's
-    // designs share no node, so its concatenated mesh assembles the same way)
-    if (C->P.precision == UFT_MIXED_H && C->P.fine_assembled && !(C->P.load_cases == 1 && C->nlc > 1)) {
+    // assembled node operators on the node-neighbour pattern: the filter's
+    // Helmholtz operator in every form, the elasticity operators in
+    // UFT_MIXED_H with superposed loads (a design batch's designs share no
+    // node, so its concatenated mesh assembles the same way)
+    const bool elast = C->P.precision == UFT_MIXED_H && !(C->P.load_cases == 1 && C->nlc > 1);
+    if (C->P.fine_assembled) {
       std::vector<int64_t> off;
       std::vector<int32_t> width, col, cptr, clist;
       if (uft_fine_ell_pattern(nn, npe, C->elem.data(), iptr, inc, off, width, col, cptr, clist)) {
@@ -1739,9 +1742,11 @@ int uft_setup(uft_ctx *C) {
         if (fits && (rc = upload(C, &F.d_cd, cd))) return rc;
         if ((rc = upload(C, &F.d_off, off)) || (rc = upload(C, &F.d_width, width)) || (rc = upload(C, &F.d_col, col)) ||
             (rc = upload(C, &F.d_cptr, cptr)) || (rc = upload(C, &F.d_clist, clist)) ||
-            (rc = dalloc_n(C, &F.d_v8, F.nslot)) ||
-            (rc = dalloc_n(C, &F.d_v1, F.nslot)) || (rc = dalloc_n(C, &F.d_dg, ndof)) ||
-            (rc = dalloc_n(C, &F.d_w8, 2 * F.nslot)) || (rc = dalloc_n(C, &F.d_w1, F.nslot)))
+            (rc = dalloc_n(C, &F.d_hv, F.nslot)))
+          return rc;
+        if (elast && ((rc = dalloc_n(C, &F.d_v8, F.nslot)) || (rc = dalloc_n(C, &F.d_v1, F.nslot)) ||
+                      (rc = dalloc_n(C, &F.d_dg, ndof)) || (rc = dalloc_n(C, &F.d_w8, 2 * F.nslot)) ||
+                      (rc = dalloc_n(C, &F.d_w1, F.nslot))))
           return rc;
         UFT_HIP(C, hipStreamSynchronize(s));  // the host arrays end with this scope
       }
@@ -1834,6 +1839,7 @@ int uft_setup(uft_ctx *C) {
     uftk::gen_kf(s, npe, ne, C->d_elem, C->d_xyz, C->P.rmin, C->d_KF, C->d_T, C->vec[UFT_VOL], C->d_part);
     uftk::final_sum(s, C->d_part, uftk::red_blocks(ne), 1, C->d_scal + S_MISC);
   }
+  if (C->fell.d_hv) uftk::helm_ell_fill(s, C);  // the filter operator on the node pattern (density-free)
   {
     std::vector<double> dm(ne);
     UFT_HIP(C, hipMemcpyAsync(dm.data(), C->d_detmin, ne * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -2539,7 +2545,7 @@ extern "C" int uft_solver_info(const uft_ctx *C, int *nlev, int64_t *n_per_level
 extern "C" int uft_fine_operator_info(const uft_ctx *C, int64_t *slots, int64_t *blocks, int *col_bytes) {
   if (!C || !slots || !blocks || !col_bytes) return UFT_ERR_ARG;
   if (!C->is_setup) return UFT_ERR_STATE;
-  const bool on = C->fell.d_col != nullptr;
+  const bool on = C->fell.d_v8 != nullptr;
   *slots = on ? C->fell.nslot : 0;
   *blocks = on ? C->fell.nnzb : 0;
   *col_bytes = on ? (C->fell.d_cd ? 2 : 4) : 0;
@@ -2697,7 +2703,7 @@ extern "C" int uft_level_apply(uft_ctx *C, int level, int kind, const double *x,
     return uft_set_error(C, UFT_ERR_ARG, "uft_level_apply: bad arguments");
   AmgLevel &Lv = C->amg.L[level];
   if (kind > 0 && !is_mixed(C)) return uft_set_error(C, UFT_ERR_ARG, "uft_level_apply: FP64 form");
-  if (kind == 2 && !(level == 0 ? (void *)C->d_KEh : (void *)Lv.d_valh) && !(level == 0 && C->fell.d_col))
+  if (kind == 2 && !(level == 0 ? (void *)C->d_KEh : (void *)Lv.d_valh) && !(level == 0 && C->fell.d_v8))
     return uft_set_error(C, UFT_ERR_ARG, "uft_level_apply: no FP16 operator on level %d", level);
   if (C->refreshes == 0) return uft_set_error(C, UFT_ERR_STATE, "uft_level_apply: no hierarchy refresh yet");
   hipStream_t s = C->s;
@@ -2710,7 +2716,7 @@ extern "C" int uft_level_apply(uft_ctx *C, int level, int kind, const double *x,
     RowOp<float> op;
     op.mode = ROW_STORE;
     op.y = Lv.tf;
-    if (level == 0 && kind == 2 && C->fell.d_col) {
+    if (level == 0 && kind == 2 && C->fell.d_v8) {
       uftk::fine_ell_op(s, C, Lv.xf, op);
     } else if (level == 0 && kind == 1 && C->fell.d_w8) {
       uftk::fine_ell_op32(s, C, Lv.xf, op);

@@ -228,8 +228,10 @@ struct uft_ctx {
   double *d_Ee = nullptr;      // per element E(x_e) of the last hierarchy refresh
   float *d_Yef = nullptr;
   double TV = 0.0;
-  // assembled FP16 fine operator of the smoother (sliced ELL of 3x3 node
-  // blocks, rows scaled by the diagonal; fine_assembled)
+  // assembled node operators (fine_assembled) on one sliced-ELL node pattern:
+  // the filter's Helmholtz operator (FP64) and, in UFT_MIXED_H, the
+  // smoother's FP16 3x3 blocks (rows scaled by the diagonal) and the outer
+  // product's FP32 blocks
   struct {
     int64_t nslice = 0, nslot = 0, nnzb = 0;
     int wmax = 0;
@@ -242,6 +244,7 @@ struct uft_ctx {
     uint4 *d_v8 = nullptr;       // per slot: block entries 0..7 / diag(row), FP16
     uint16_t *d_v1 = nullptr;    // per slot: entry 8
     float *d_dg = nullptr;       // per scalar row: the diagonal (the row scale)
+    double *d_hv = nullptr;      // per slot: the filter's Helmholtz entry (FP64, assembled at setup)
     float4 *d_w8 = nullptr;      // per slot: FP32 block entries 0..7 (the outer operator)
     float *d_w1 = nullptr;       // per slot: FP32 entry 8
   } fell;
@@ -503,6 +506,10 @@ void fine_ell_fill(hipStream_t s, const uft_ctx *C, const double *Ee, const doub
 void fine_ell_op(hipStream_t s, const uft_ctx *C, const float *u, const RowOp<float> &op);
 // the same pattern with FP32 entries (unscaled): the mixed forms' outer operator
 void fine_ell_op32(hipStream_t s, const uft_ctx *C, const float *u, const RowOp<float> &op);
+// the filter's Helmholtz operator on the node pattern: assembled once (KF is
+// density-free), then y = KF x (with dot_out: x.y into it via d_part)
+void helm_ell_fill(hipStream_t s, const uft_ctx *C);
+void helm_ell_apply(hipStream_t s, const uft_ctx *C, const double *x, double *y, double *dot_out);
 int gather_op_blocks(int64_t nn);
 void restrict_first(hipStream_t s, int bs, int64_t nc, const int32_t *cptr, const int32_t *child, const double *P,
                     const double *r, double *bc, const double *dinv, double it, double *rc, double *dc, double *xc);
