@@ -115,6 +115,40 @@ __global__ __launch_bounds__(TPB) void k_fcg_p(int64_t n, const double *__restri
   }
 }
 
+// Lanczos (Jacobi-CG) step without the iterate: alpha = sc[irz] / sc[ipq];
+// r -= alpha q; z = D^-1 r; block partials of r.z.  q in FP64 or FP32.
+template <typename TQ>
+__global__ __launch_bounds__(TPB) void k_lanczos_update(int64_t n, const double *__restrict__ sc, int irz, int ipq,
+                                                        double *__restrict__ r, const TQ *__restrict__ q,
+                                                        const double *__restrict__ dinv, double *__restrict__ z,
+                                                        double *__restrict__ part) {
+  const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  const double al = sc[irz] / sc[ipq];
+  double v[1] = {0.0};
+  if (i < n) {
+    const double ri = r[i] - al * (double)q[i];
+    r[i] = ri;
+    const double zi = ri * dinv[i];
+    z[i] = zi;
+    v[0] = ri * zi;
+  }
+  uftr::block_partials<1>(v, part);
+}
+// the step's scalars: rz' = sum of the partials; log (alpha, beta = rz' / rz);
+// sc[irz] = rz', sc[ibeta] = beta
+__global__ __launch_bounds__(TPB) void k_lanczos_final(const double *__restrict__ part, int nb, double *__restrict__ sc,
+                                                       int irz, int ipq, int ibeta, double *__restrict__ log2) {
+  double res[1];
+  uftr::block_final<1>(part, nb, res);
+  if (threadIdx.x == 0) {
+    const double al = sc[irz] / sc[ipq], be = res[0] / sc[irz];
+    log2[0] = al;
+    log2[1] = be;
+    sc[ibeta] = be;
+    sc[irz] = res[0];
+  }
+}
+
 // CG scalars: sc[dst] = sc[a] / sc[b]
 __global__ void k_div(double *sc, int a, int b, int dst) { sc[dst] = sc[a] / sc[b]; }
 // beta step: sc[ibeta] = sc[inew] / sc[iold]; sc[iold] = sc[inew]; optional log
@@ -148,6 +182,16 @@ void fcg_update_ab(hipStream_t s, int64_t n, double *sc, int ia, int ib, double 
 }
 void fcg_p(hipStream_t s, int64_t n, const double *z, const double *sc, int ib, int ibo, int first, double *p) {
   k_fcg_p<<<nblk(n), TPB, 0, s>>>(n, z, sc, ib, ibo, first, p);
+}
+void lanczos_step(hipStream_t s, int64_t n, double *sc, int irz, int ipq, int ibeta, double *r, const double *q,
+                  const double *dinv, double *z, double *part, double *log2) {
+  k_lanczos_update<double><<<nblk(n), TPB, 0, s>>>(n, sc, irz, ipq, r, q, dinv, z, part);
+  k_lanczos_final<<<1, TPB, 0, s>>>(part, (int)nblk(n), sc, irz, ipq, ibeta, log2);
+}
+void lanczos_step(hipStream_t s, int64_t n, double *sc, int irz, int ipq, int ibeta, double *r, const float *q,
+                  const double *dinv, double *z, double *part, double *log2) {
+  k_lanczos_update<float><<<nblk(n), TPB, 0, s>>>(n, sc, irz, ipq, r, q, dinv, z, part);
+  k_lanczos_final<<<1, TPB, 0, s>>>(part, (int)nblk(n), sc, irz, ipq, ibeta, log2);
 }
 void sc_div(hipStream_t s, double *sc, int a, int b, int dst) { k_div<<<1, 1, 0, s>>>(sc, a, b, dst); }
 void sc_beta(hipStream_t s, double *sc, int inew, int iold, int ibeta, double *log2) {

@@ -767,17 +767,34 @@ int estimate_lmax(uft_ctx *C, int l, int ksteps) {
   uftk::copy(s, n, z, p);
   uftk::dot(s, n, r, z, C->d_part, C->d_scal + S_MISC);  // rz in S_MISC
   double *log = C->d_lanczos;
+  // the fine level of the mixed forms: the smoother's operator on FP32 vectors
+  const bool lo = l == 0 && is_mixed(C);
+  (void)x;
+  if (lo) uftk::d2f(s, n, p, Lv.xf);
   for (int j = 0; j < ksteps; ++j) {
-    lanczos_apply(C, l, p, q);
-    uftk::dot(s, n, p, q, C->d_part, C->d_scal + S_MISC + 1);
-    uftk::sc_div(s, C->d_scal, S_MISC, S_MISC + 1, S_MISC + 2);  // alpha
-    uftk::sc_copy(s, C->d_scal + S_MISC + 2, log + 3 * j);       // log alpha
-    uftk::cg_update(s, n, C->d_scal, S_MISC + 2, x, p, r, q, C->d_part, C->d_scal + S_MISC + 5);
-    uftk::scale(s, n, r, Lv.d_dinv, z);
-    uftk::dot(s, n, r, z, C->d_part, C->d_scal + S_MISC + 3);
-    uftk::sc_beta(s, C->d_scal, S_MISC + 3, S_MISC, S_MISC + 4, nullptr);
-    uftk::sc_copy(s, C->d_scal + S_MISC + 4, log + 3 * j + 1);  // log beta
-    uftk::xpby(s, n, z, C->d_scal, S_MISC + 4, p);
+    // q = A p and p.q (S_MISC + 1) in the operator's epilogue; one update
+    // launch (alpha, r, z = D^-1 r, r.z); alpha, beta logged; p = z + beta p
+    if (lo) {
+      RowOp<float> op;
+      op.mode = ROW_STOREDOT;
+      op.y = Lv.tf;
+      op.part = C->d_part;
+      lev_op(C, 0, Lv.xf, op);
+      uftk::final_sum(s, C->d_part, lev_op_blocks(C, 0, true), 1, C->d_scal + S_MISC + 1);
+      uftk::lanczos_step(s, n, C->d_scal, S_MISC, S_MISC + 1, S_MISC + 4, r, Lv.tf, Lv.d_dinv, z, C->d_part,
+                         log + 3 * j);
+      uftk::xpby_lo(s, n, z, C->d_scal, S_MISC + 4, p, Lv.xf);
+    } else {
+      RowOp<double> op;
+      op.mode = ROW_STOREDOT;
+      op.y = q;
+      op.part = C->d_part;
+      lev_op(C, l, p, op);
+      uftk::final_sum(s, C->d_part, lev_op_blocks(C, l, false), 1, C->d_scal + S_MISC + 1);
+      uftk::lanczos_step(s, n, C->d_scal, S_MISC, S_MISC + 1, S_MISC + 4, r, q, Lv.d_dinv, z, C->d_part,
+                         log + 3 * j);
+      uftk::xpby(s, n, z, C->d_scal, S_MISC + 4, p);
+    }
   }
   std::vector<double> hl(3 * ksteps);
   UFT_HIP(C, hipMemcpyAsync(hl.data(), log, hl.size() * sizeof(double), hipMemcpyDeviceToHost, C->s));

@@ -575,6 +575,12 @@ void fcg_init(hipStream_t s, int64_t n, const double *b, const double *q, const 
 void fcg_update(hipStream_t s, int64_t n, const double *sc, int ia, double *x, const double *p, double *r,
                 const double *q, const double *dinv, double *z, double *part, double *out);
 void fcg_p(hipStream_t s, int64_t n, const double *z, const double *sc, int ib, int ibo, int first, double *p);
+// one Lanczos (Jacobi-CG) step after q = A p and p.q in sc[ipq]: r -= alpha q,
+// z = D^-1 r, rz' and beta into sc, (alpha, beta) into log2
+void lanczos_step(hipStream_t s, int64_t n, double *sc, int irz, int ipq, int ibeta, double *r, const double *q,
+                  const double *dinv, double *z, double *part, double *log2);
+void lanczos_step(hipStream_t s, int64_t n, double *sc, int irz, int ipq, int ibeta, double *r, const float *q,
+                  const double *dinv, double *z, double *part, double *log2);
 // x += a p ; r -= a q ; z = D^-1 r with a = sc[ia] / sc[ib]; then sc[izz] = ||z||^2,
 // sc[ibo] = sc[ibeta], sc[ibeta] = z.r
 void fcg_update_ab(hipStream_t s, int64_t n, double *sc, int ia, int ib, double *x, const double *p, double *r,
