@@ -340,12 +340,13 @@ __global__ __launch_bounds__(TPB) void k_bsr_to_dense(int64_t nrows, const int32
 // kernel reads what another workgroup of the same launch writes.
 constexpr int GJ_NB = 32;
 
-__global__ __launch_bounds__(TPB) void k_gj_diag(int n, int k0, int nb, const double *__restrict__ M,
-                                                 double *__restrict__ Dinv) {
+constexpr int GJ_DIAG_T = TPB;
+__global__ __launch_bounds__(GJ_DIAG_T) void k_gj_diag(int n, int k0, int nb, const double *__restrict__ M,
+                                                       double *__restrict__ Dinv) {
   __shared__ double D[GJ_NB * GJ_NB];
   __shared__ double colk[GJ_NB], rowk[GJ_NB];
   const int tid = threadIdx.x;
-  for (int t = tid; t < nb * nb; t += TPB) D[t] = M[(size_t)(k0 + t / nb) * n + k0 + t % nb];
+  for (int t = tid; t < nb * nb; t += GJ_DIAG_T) D[t] = M[(size_t)(k0 + t / nb) * n + k0 + t % nb];
   __syncthreads();
   for (int k = 0; k < nb; ++k) {
     if (tid < nb) colk[tid] = D[tid * nb + k];
@@ -359,35 +360,28 @@ __global__ __launch_bounds__(TPB) void k_gj_diag(int n, int k0, int nb, const do
       D[k * nb + tid] = v;
     }
     __syncthreads();
-    for (int t = tid; t < nb * nb; t += TPB) {
+    for (int t = tid; t < nb * nb; t += GJ_DIAG_T) {
       const int i = t / nb, j = t - (t / nb) * nb;
       if (i != k) D[t] = (j == k ? 0.0 : D[t]) - colk[i] * rowk[j];
     }
     __syncthreads();
   }
-  for (int t = tid; t < nb * nb; t += TPB) Dinv[t] = D[t];
+  for (int t = tid; t < nb * nb; t += GJ_DIAG_T) Dinv[t] = D[t];
 }
 
-// thread j: R[:, j] = Dinv M[K, j] (j not in K) and Cp[j, :] = M[j, K]
+// thread (l, j), l < nb: R[l, j] = (Dinv M[K, j])_l (j not in K) and
+// Cp[j, l] = M[j, k0 + l]
 __global__ __launch_bounds__(TPB) void k_gj_panels(int n, int k0, int nb, const double *__restrict__ M,
                                                    const double *__restrict__ Dinv, double *__restrict__ R,
                                                    double *__restrict__ Cp) {
-  __shared__ double Ds[GJ_NB * GJ_NB];
-  for (int t = threadIdx.x; t < nb * nb; t += TPB) Ds[t] = Dinv[t];
-  __syncthreads();
-  const int j = blockIdx.x * TPB + threadIdx.x;
-  if (j >= n) return;
-  for (int m = 0; m < nb; ++m) Cp[(size_t)j * nb + m] = M[(size_t)j * n + k0 + m];
+  const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  if (t >= (int64_t)nb * n) return;
+  const int l = (int)(t / n), j = (int)(t - (int64_t)l * n);
+  Cp[(size_t)j * nb + l] = M[(size_t)j * n + k0 + l];
   if (j >= k0 && j < k0 + nb) return;
-  double v[GJ_NB];
-#pragma unroll
-  for (int m = 0; m < GJ_NB; ++m) v[m] = m < nb ? M[(size_t)(k0 + m) * n + j] : 0.0;
-  for (int l = 0; l < nb; ++l) {
-    double acc = 0.0;
-#pragma unroll
-    for (int m = 0; m < GJ_NB; ++m) acc += Ds[l * nb + (m < nb ? m : 0)] * v[m];
-    R[(size_t)l * n + j] = acc;
-  }
+  double acc = 0.0;
+  for (int m = 0; m < nb; ++m) acc += Dinv[l * nb + m] * M[(size_t)(k0 + m) * n + j];
+  R[(size_t)l * n + j] = acc;
 }
 
 // one row i per blockIdx.y, threads over columns j
@@ -578,8 +572,8 @@ void dense_spd_inverse(hipStream_t s, int64_t n, double *A, double *Ainv, float 
   double *Dinv = A + n * n, *R = Dinv + nb * nb, *Cp = R + (size_t)nb * n;
   for (int k0 = 0; k0 < nn; k0 += nb) {
     const int b = std::min(nb, nn - k0);
-    k_gj_diag<<<1, TPB, 0, s>>>(nn, k0, b, Ainv, Dinv);
-    k_gj_panels<<<nblk(n), TPB, 0, s>>>(nn, k0, b, Ainv, Dinv, R, Cp);
+    k_gj_diag<<<1, GJ_DIAG_T, 0, s>>>(nn, k0, b, Ainv, Dinv);
+    k_gj_panels<<<nblk((int64_t)b * n), TPB, 0, s>>>(nn, k0, b, Ainv, Dinv, R, Cp);
     k_gj_update<<<dim3(nblk(n), (unsigned)n), TPB, 0, s>>>(nn, k0, b, Ainv, Dinv, R, Cp);
   }
   k_symmetrize<<<nblk(n * n), TPB, 0, s>>>(n, Ainv);
