@@ -19,8 +19,11 @@ def _pair(O, npe, dims, precision, seed=3):
     return ctx, ref
 
 
-@pytest.mark.parametrize("npe,dims,precision", [(8, (8, 4, 4), 0), (8, (8, 4, 4), 2), (4, (5, 3, 3), 2)])
+@pytest.mark.parametrize("npe,dims,precision", [(8, (8, 4, 4), 0), (8, (8, 4, 4), 1), (8, (8, 4, 4), 2),
+                                                (4, (5, 3, 3), 1), (4, (5, 3, 3), 2)])
 def test_operator_kernels_vs_oracle(O, npe, dims, precision):
+    """kind 1: the cycle's fine operator -- BF16 matrix-core element products
+    (precision 1) or the assembled FP16 node blocks (precision 2)."""
     print(f"[case] multi-case operator npe={npe} precision={precision}", flush=True)
     ctx, ref = _pair(O, npe, dims, precision)
     rng = np.random.default_rng(11)
@@ -31,8 +34,8 @@ def test_operator_kernels_vs_oracle(O, npe, dims, precision):
     got = ctx.apply_K_cases(U, kind=0)
     assert np.abs(got - want).max() <= 1e-12 * np.abs(want).max()
     if precision:
-        # BF16 operands (8-bit significands), FP32 accumulation: entrywise
-        # error of a few 2^-8 relative to the row's magnitude
+        # BF16 operands (8-bit significands) or FP16 blocks (11-bit), FP32
+        # accumulation: entrywise error of a few 2^-8 relative to the row's magnitude
         got = ctx.apply_K_cases(U, kind=1)
         scale = np.abs(want).max()
         err = np.abs(got - want).max() / scale

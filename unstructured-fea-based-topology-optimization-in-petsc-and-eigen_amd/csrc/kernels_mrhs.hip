@@ -12,6 +12,8 @@
 // Element-local results are stored node-row major per element,
 // [ne][npe][3][NR], so the node phase reads each incidence's 3 x NR values
 // as one contiguous segment.
+#include <hip/hip_fp16.h>
+
 #include "uft_elem.h"
 #include "uft_internal.h"
 #include "uft_reduce.h"
@@ -330,6 +332,67 @@ __global__ __launch_bounds__(TPB) void k_mr_gather_op(int nn, int ne, const int3
     }
   }
   mr_epi_partials<T, NR>(op, dv);
+}
+
+// ---------------------------------------------------------------- assembled fine operator
+// The sliced-ELL 3x3 node blocks (one node pattern, see the single right-hand
+// side kernels) applied to NR interleaved right-hand sides: lane (n, rr), the
+// NR lanes of a node read the same block (one request) and their own vector
+// entries, so each block streams once for all right-hand sides.  H: FP16
+// blocks of D^-1 A (rows scaled by dg), else FP32 blocks.
+template <int NR, bool H, typename CI>
+__global__ __launch_bounds__(TPB) void k_mr_ell_op(int nn, const int64_t *__restrict__ off,
+                                                   const int32_t *__restrict__ width, const CI *__restrict__ col,
+                                                   const uint4 *__restrict__ v8, const uint16_t *__restrict__ v1,
+                                                   const float *__restrict__ dg, const float4 *__restrict__ w8,
+                                                   const float *__restrict__ w1, const uint8_t *__restrict__ fixm,
+                                                   const float *__restrict__ u, MrOp<float> op) {
+  const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  const int64_t n = t / NR;
+  const int rr = (int)(t % NR);
+  double dv[3] = {0.0, 0.0, 0.0};
+  if (n < nn) {
+    const int64_t w = n / UFT_ELL_SLICE;
+    const int64_t base = off[w] + n % UFT_ELL_SLICE;
+    const int W = width[w];
+    float a0 = 0.f, a1 = 0.f, a2 = 0.f;
+#pragma unroll 2
+    for (int s = 0; s < W; ++s) {
+      const int64_t k = base + (int64_t)s * UFT_ELL_SLICE;
+      const int64_t j = std::is_same<CI, int16_t>::value ? n + (int64_t)col[k] : (int64_t)col[k];
+      float e[9];
+      if constexpr (H) {
+        const uint4 h = v8[k];
+        const unsigned hw[4] = {h.x, h.y, h.z, h.w};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const float2 f = __half22float2(*reinterpret_cast<const __half2 *>(&hw[q]));
+          e[2 * q] = f.x;
+          e[2 * q + 1] = f.y;
+        }
+        e[8] = __half2float(__ushort_as_half(v1[k]));
+      } else {
+        const float4 a = w8[2 * k], b = w8[2 * k + 1];
+        e[0] = a.x; e[1] = a.y; e[2] = a.z; e[3] = a.w;
+        e[4] = b.x; e[5] = b.y; e[6] = b.z; e[7] = b.w;
+        e[8] = w1[k];
+      }
+      const float x0 = u[(3 * j) * NR + rr], x1 = u[(3 * j + 1) * NR + rr], x2 = u[(3 * j + 2) * NR + rr];
+      a0 += e[0] * x0 + e[1] * x1 + e[2] * x2;
+      a1 += e[3] * x0 + e[4] * x1 + e[5] * x2;
+      a2 += e[6] * x0 + e[7] * x1 + e[8] * x2;
+    }
+    const float acc[3] = {a0, a1, a2};
+    const unsigned m = fixm[n];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const int64_t i = 3 * n + c, ii = i * NR + rr;
+      const float ui = u[ii];
+      const float tv = H ? dg[i] * acc[c] : acc[c];
+      mr_epi(op, i, ii, ((m >> c) & 1u) ? ui : tv, ui, dv);
+    }
+  }
+  mr_epi_partials<float, NR>(op, dv);
 }
 
 // ---------------------------------------------------------------- 6x6 BSR
@@ -767,6 +830,25 @@ void mr_gather_op(hipStream_t s, const uft_ctx *C, int nr, const float *u, const
   mr_gather_op_t<float>(s, C, nr, u, Ye, o);
 }
 int mr_gather_blocks(int64_t nn, int nr) { return (int)nblk(nn * nr); }
+template <bool H>
+static void mr_ell_op_t(hipStream_t s, const uft_ctx *C, int nr, const float *u, const RowOp<float> &o) {
+  const auto &F = C->fell;
+  const int nn = (int)C->nn;
+  const MrOp<float> op = to_mr(o);
+  if (F.d_cd) {
+    UFT_NR_SWITCH(nr, (k_mr_ell_op<NR, H, int16_t><<<nblk((int64_t)nn * NR), TPB, 0, s>>>(
+                          nn, F.d_off, F.d_width, F.d_cd, F.d_v8, F.d_v1, F.d_dg, F.d_w8, F.d_w1, C->d_fixm, u, op)));
+  } else {
+    UFT_NR_SWITCH(nr, (k_mr_ell_op<NR, H, int32_t><<<nblk((int64_t)nn * NR), TPB, 0, s>>>(
+                          nn, F.d_off, F.d_width, F.d_col, F.d_v8, F.d_v1, F.d_dg, F.d_w8, F.d_w1, C->d_fixm, u, op)));
+  }
+}
+void mr_ell_op(hipStream_t s, const uft_ctx *C, int nr, const float *u, const RowOp<float> &o, bool fp32) {
+  if (fp32)
+    mr_ell_op_t<false>(s, C, nr, u, o);
+  else
+    mr_ell_op_t<true>(s, C, nr, u, o);
+}
 
 template <typename T>
 static void mr_bsr_op_t(hipStream_t s, int nr, int64_t nb_rows, const int32_t *rp, const int32_t *ci, const T *val,

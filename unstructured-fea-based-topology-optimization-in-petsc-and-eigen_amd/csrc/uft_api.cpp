@@ -536,6 +536,14 @@ void mr_lev_op(uft_ctx *C, int l, const T *u, const RowOp<T> &op) {
   if (l == 0) {
     hipEvent_t *ev = prof_triple(C, std::is_same<T, double>::value ? 0 : 1);
     if (ev) (void)hipEventRecord(ev[0], C->s);
+    if constexpr (std::is_same<T, float>::value) {
+      if (C->fell.d_v8) {  // assembled FP16 blocks: one launch for all right-hand sides
+        uftk::mr_ell_op(C->s, C, nr, u, op, false);
+        if (ev) (void)hipEventRecord(ev[1], C->s);
+        if (ev) (void)hipEventRecord(ev[2], C->s);
+        return;
+      }
+    }
     uftk::mr_elem_phase(C->s, C, nr, C->cur_xphys, u, (T *)C->mr.Ye);
     if (ev) (void)hipEventRecord(ev[1], C->s);
     uftk::mr_gather_op(C->s, C, nr, u, (const T *)C->mr.Ye, op);
@@ -561,9 +569,14 @@ void mr_outer_apply_lo(uft_ctx *C, const float *pf, float *qf) {
   op.part = C->d_part;
   hipEvent_t *ev = prof_triple(C, 2);
   if (ev) (void)hipEventRecord(ev[0], C->s);
-  uftk::mr_elem_phase(C->s, C, nr, C->cur_xphys, pf, (float *)C->mr.Ye, true);
-  if (ev) (void)hipEventRecord(ev[1], C->s);
-  uftk::mr_gather_op(C->s, C, nr, pf, (const float *)C->mr.Ye, op);
+  if (C->fell.d_w8) {  // assembled FP32 blocks: one launch for all right-hand sides
+    uftk::mr_ell_op(C->s, C, nr, pf, op, true);
+    if (ev) (void)hipEventRecord(ev[1], C->s);
+  } else {
+    uftk::mr_elem_phase(C->s, C, nr, C->cur_xphys, pf, (float *)C->mr.Ye, true);
+    if (ev) (void)hipEventRecord(ev[1], C->s);
+    uftk::mr_gather_op(C->s, C, nr, pf, (const float *)C->mr.Ye, op);
+  }
   if (ev) (void)hipEventRecord(ev[2], C->s);
 }
 
@@ -1729,9 +1742,9 @@ int uft_setup(uft_ctx *C) {
     if ((rc = upload(C, &C->d_inc, inc))) return rc;
     // assembled node operators on the node-neighbour pattern: the filter's
     // Helmholtz operator in every form, the elasticity operators in
-    // UFT_MIXED_H with superposed loads (a design batch's designs share no
-    // node, so its concatenated mesh assembles the same way)
-    const bool elast = C->P.precision == UFT_MIXED_H && !(C->P.load_cases == 1 && C->nlc > 1);
+    // UFT_MIXED_H (also for separate load cases, and a design batch's designs
+    // share no node, so its concatenated mesh assembles the same way)
+    const bool elast = C->P.precision == UFT_MIXED_H;
     if (C->P.fine_assembled) {
       std::vector<int64_t> off;
       std::vector<int32_t> width, col, cptr, clist;
@@ -2676,8 +2689,9 @@ extern "C" int uft_load_case_stats(const uft_ctx *C, int *iters, double *relres)
 
 // y_k = K(XPhys) u_k for every load case through the multi-load-case
 // operator kernels: kind 0 the FP64 kernel of the outer loop, kind 1 the
-// preconditioner's (FP32 vectors; matrix-core BF16 element products in the
-// mixed forms).  u, y: host arrays of ncase x 3nn, case-major.
+// preconditioner's (FP32 vectors; matrix-core BF16 element products, or the
+// assembled FP16 node blocks in UFT_MIXED_H).  u, y: host arrays of
+// ncase x 3nn, case-major.
 extern "C" int uft_apply_K_cases(uft_ctx *C, int kind, const double *u, double *y) {
   if (int rc = check_setup(C)) return rc;
   MrState &M = C->mr;
@@ -2690,6 +2704,11 @@ extern "C" int uft_apply_K_cases(uft_ctx *C, int kind, const double *u, double *
     for (int64_t i = 0; i < n; ++i) ui[(size_t)i * nr + k] = u[(size_t)k * n + i];
   C->cur_xphys = C->vec[UFT_XPHYS];
   uftk::simp_moduli(C->s, C, C->cur_xphys, C->d_Ee);  // the matrix-core kernel reads E(x_e) from d_Ee
+  if (C->fell.d_v8) {  // the assembled blocks of this density (as a hierarchy refresh makes them)
+    AmgLevel &L0 = C->amg.L[0];
+    uftk::diag_K(C->s, C, C->cur_xphys, L0.d_dinv, L0.d_dinvf);
+    uftk::fine_ell_fill(C->s, C, C->d_Ee, L0.d_dinv);
+  }
   if (kind == 0 || !is_mixed(C)) {
     UFT_HIP(C, hipMemcpyAsync(M.p, ui.data(), ui.size() * sizeof(double), hipMemcpyHostToDevice, C->s));
     RowOp<double> op;
