@@ -87,3 +87,23 @@ def test_assembled_fine_operator_any_numbering(shuffle):
     print("compliances assembled / element form", res[1], res[0])
     for a, b in zip(res[1], res[0]):
         assert abs(a - b) <= 1e-8 * abs(b)
+
+
+def test_mixed_tet4_loop_vs_oracle(O):
+    """UFT_MIXED_H on a tet4 mesh (assembled blocks from 12x12 element
+    matrices): compliance within 1e-6 of the FP64 oracle over 6 iterations."""
+    from uft.capi import cantilever
+    args = dict(nx=6, ny=3, nz=3, jitter=0.15, seed=17)
+    prm = dict(Emax=1.0, volfrac=0.25, rmin=0.5, rtol_el=1e-12, rtol_f=1e-12)
+    print("[case] mixed tet4 6x3x3 vs oracle", flush=True)
+    ctx = cantilever(4, **args, precision=2, load_kind=1, **prm)
+    ref = O.cantilever(4, **args, load_kind=1, **prm)
+    assert ctx.fine_operator_info()[1] > 0
+    ctx.init_design()
+    ref.init_design()
+    for it in range(1, 7):
+        fx = ctx.design_iteration(it)[0]
+        rfx = ref.design_iteration(it)[0]
+        assert abs(fx - rfx) <= 1e-6 * abs(rfx), it
+    assert ctx.stats()["relres_el"] <= 1e-12
+    assert np.max(np.abs(ctx.get("X") - ref.vec("X"))) <= 1e-6

@@ -540,6 +540,21 @@ __global__ __launch_bounds__(TPB) void k_mr_final_kc(const double *__restrict__ 
   }
 }
 
+// first stage of a long partial sum: block b sums rows [b chunk, (b+1) chunk)
+// of the KN partial columns in the fixed block order into out[b]
+template <int KN>
+__global__ __launch_bounds__(TPB) void k_part_stage(const double *__restrict__ part, int nb, int chunk,
+                                                    double *__restrict__ out) {
+  const int r0 = blockIdx.x * chunk, r1 = min(nb, r0 + chunk);
+  double res[KN];
+  uftr::block_final<KN>(part + (size_t)r0 * KN, max(0, r1 - r0), res);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < KN; ++k) out[(size_t)blockIdx.x * KN + k] = res[k];
+  }
+}
+constexpr int MR_STAGE_BLOCKS = 128;
+
 // xc[row][r] = sum_c Ainv[c][row] bc[c][r] (FP64, the symmetric inverse read
 // by columns), one thread per (row, r)
 template <typename T, int NR>
@@ -918,10 +933,26 @@ void mr_lincomb_first(hipStream_t s, int nr, int64_t n, const double *sc, const 
   mr_lincomb_first_t<float>(s, nr, n, sc, b, v, kr, r, dinv, it, d, kd);
 }
 
-void mr_final_kc1(hipStream_t s, int nr, const double *part, int nb, double *sc) {
+// many partial rows (the level-1 operator): a first stage of MR_STAGE_BLOCKS
+// blocks into the space after them, then the one-block final kernel
+void mr_final_kc1(hipStream_t s, int nr, double *part, int nb, double *sc) {
+  if (nb > 8 * MR_STAGE_BLOCKS) {
+    const int chunk = (nb + MR_STAGE_BLOCKS - 1) / MR_STAGE_BLOCKS;
+    double *p2 = part + (size_t)nb * 2 * nr;
+    UFT_NR_SWITCH(nr, (k_part_stage<2 * NR><<<MR_STAGE_BLOCKS, TPB, 0, s>>>(part, nb, chunk, p2)));
+    UFT_NR_SWITCH(nr, (k_mr_final_kc<2, NR><<<1, TPB, 0, s>>>(p2, MR_STAGE_BLOCKS, sc)));
+    return;
+  }
   UFT_NR_SWITCH(nr, (k_mr_final_kc<2, NR><<<1, TPB, 0, s>>>(part, nb, sc)));
 }
-void mr_final_kc2(hipStream_t s, int nr, const double *part, int nb, double *sc) {
+void mr_final_kc2(hipStream_t s, int nr, double *part, int nb, double *sc) {
+  if (nb > 8 * MR_STAGE_BLOCKS) {
+    const int chunk = (nb + MR_STAGE_BLOCKS - 1) / MR_STAGE_BLOCKS;
+    double *p2 = part + (size_t)nb * 3 * nr;
+    UFT_NR_SWITCH(nr, (k_part_stage<3 * NR><<<MR_STAGE_BLOCKS, TPB, 0, s>>>(part, nb, chunk, p2)));
+    UFT_NR_SWITCH(nr, (k_mr_final_kc<3, NR><<<1, TPB, 0, s>>>(p2, MR_STAGE_BLOCKS, sc)));
+    return;
+  }
   UFT_NR_SWITCH(nr, (k_mr_final_kc<3, NR><<<1, TPB, 0, s>>>(part, nb, sc)));
 }
 

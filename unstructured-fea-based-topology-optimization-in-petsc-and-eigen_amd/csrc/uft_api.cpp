@@ -2046,6 +2046,7 @@ int uft_setup(uft_ctx *C) {
     if (6 * C->amg.L[C->amg.nlev - 1].nunit > UFT_MR_TAIL_MAX)
       return uft_set_error(C, UFT_ERR_ARG, "multi-load-case form: coarsest level above %d unknowns", UFT_MR_TAIL_MAX);
     need = std::max<int64_t>(need, (int64_t)uftk::mr_gather_blocks(nn, nr) * nr);
+    need += 3 * 8 * 128;  // the two-stage final sums of the K-cycle's partial rows
     need = std::max<int64_t>(need, (int64_t)((ndof * nr + 255) / 256) * 2 * nr);
     if (need > C->part_cap) {  // the block partials outgrow the single-vector buffer
       if ((rc = dalloc_n(C, &C->d_part, need + 64))) return rc;

@@ -463,8 +463,9 @@ void mr_lincomb_first(hipStream_t s, int nr, int64_t n, const double *sc, const 
                       double *r, const double *dinv, double it, double *d, double *kd);
 void mr_lincomb_first(hipStream_t s, int nr, int64_t n, const double *sc, const float *b, const float *v, float *kr,
                       float *r, const float *dinv, double it, float *d, float *kd);
-void mr_final_kc1(hipStream_t s, int nr, const double *part, int nb, double *sc);
-void mr_final_kc2(hipStream_t s, int nr, const double *part, int nb, double *sc);
+// (part: room for 3 x 8 x 128 more doubles after the nb partial rows)
+void mr_final_kc1(hipStream_t s, int nr, double *part, int nb, double *sc);
+void mr_final_kc2(hipStream_t s, int nr, double *part, int nb, double *sc);
 // xc = Ainv P^T r on the coarsest level (bc: restriction scratch of the coarsest size x nr)
 void mr_tail_solve(hipStream_t s, int nr, int bs, int nc_units, const int32_t *cptr, const int32_t *child,
                    const double *P, const double *Ainv, double *bc, const double *r, double *xc);
