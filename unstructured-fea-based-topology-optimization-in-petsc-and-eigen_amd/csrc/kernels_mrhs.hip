@@ -557,15 +557,30 @@ constexpr int MR_STAGE_BLOCKS = 128;
 
 // xc[row][r] = sum_c Ainv[c][row] bc[c][r] (FP64, the symmetric inverse read
 // by columns), one thread per (row, r)
+// A workgroup takes 32 outputs (row, r); eight groups of 32 threads each sum
+// one eighth of the columns, and the eight partial sums are added in order.
+constexpr int MR_DS_PAIRS = 32, MR_DS_CHUNKS = TPB / MR_DS_PAIRS;
 template <typename T, int NR>
 __global__ __launch_bounds__(TPB) void k_mr_dense_solve(int n, const double *__restrict__ Ainv, const T *__restrict__ bc,
                                                         T *__restrict__ xc) {
-  const int t = blockIdx.x * TPB + threadIdx.x;
-  if (t >= n * NR) return;
-  const int row = t / NR, q = t % NR;
+  __shared__ double ps[MR_DS_CHUNKS][MR_DS_PAIRS];
+  const int p = threadIdx.x % MR_DS_PAIRS, k = threadIdx.x / MR_DS_PAIRS;
+  const int g = blockIdx.x * MR_DS_PAIRS + p;
+  const int chunk = (n + MR_DS_CHUNKS - 1) / MR_DS_CHUNKS;
   double acc = 0.0;
-  for (int c = 0; c < n; ++c) acc += Ainv[(size_t)c * n + row] * (double)bc[c * NR + q];
-  xc[t] = (T)acc;
+  if (g < n * NR) {
+    const int row = g / NR, q = g % NR;
+    const int c1 = min(n, (k + 1) * chunk);
+    for (int c = k * chunk; c < c1; ++c) acc += Ainv[(size_t)c * n + row] * (double)bc[c * NR + q];
+  }
+  ps[k][p] = acc;
+  __syncthreads();
+  if (k == 0 && g < n * NR) {
+    double sum = ps[0][p];
+#pragma unroll
+    for (int j = 1; j < MR_DS_CHUNKS; ++j) sum += ps[j][p];
+    xc[g] = (T)sum;
+  }
 }
 
 // ---------------------------------------------------------------- outer Krylov (FP64)
@@ -627,8 +642,8 @@ __global__ __launch_bounds__(TPB) void k_mr_cg_update_lo(int64_t n, const double
   mr_block_partials<1, NR>(v, part);
 }
 // reliable update of the right-hand sides in mask: r = res (the FP64
-// residual F - K U) and its FP32 copy; partials of r.r for every right-hand
-// side
+// residual F - K U) and its FP32 copy (rf may be null); partials of r.r for
+// every right-hand side
 template <int NR>
 __global__ __launch_bounds__(TPB) void k_mr_take_res(int64_t n, unsigned mask, const double *__restrict__ res,
                                                      double *__restrict__ r, float *__restrict__ rf,
@@ -641,7 +656,7 @@ __global__ __launch_bounds__(TPB) void k_mr_take_res(int64_t n, unsigned mask, c
     if (take) {
       ri = res[t];
       r[t] = ri;
-      rf[t] = (float)ri;
+      if (rf) rf[t] = (float)ri;
     } else {
       ri = r[t];
     }
@@ -962,7 +977,8 @@ static void mr_tail_solve_t(hipStream_t s, int nr, int bs, int nc_units, const i
   // restriction over the whole chip, then the dense solve (FP64 accumulation)
   mr_restrict_first_t<T>(s, nr, bs, nc_units, cptr, child, P, r, bc, nullptr, 0.0, nullptr, nullptr, nullptr);
   const int n = 6 * nc_units;
-  UFT_NR_SWITCH(nr, (k_mr_dense_solve<T, NR><<<nblk((int64_t)n * NR), TPB, 0, s>>>(n, Ainv, bc, xc)));
+  UFT_NR_SWITCH(nr, (k_mr_dense_solve<T, NR><<<(unsigned)((n * NR + MR_DS_PAIRS - 1) / MR_DS_PAIRS), TPB, 0, s>>>(
+                        n, Ainv, bc, xc)));
 }
 void mr_tail_solve(hipStream_t s, int nr, int bs, int nc_units, const int32_t *cptr, const int32_t *child,
                    const double *P, const double *Ainv, double *bc, const double *r, double *xc) {

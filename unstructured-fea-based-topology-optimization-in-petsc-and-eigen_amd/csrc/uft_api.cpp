@@ -1114,12 +1114,12 @@ int solve_el_mixed(uft_ctx *C, const double *xphys, double *U) {
 // together (FP64 iterate, residual and inner products; per-RHS scalars); the
 // cycle runs in FP64 (UFT_FP64) or FP32 with matrix-core element products
 // (mixed forms).  A right-hand side stops updating once
-// ||F_k - K U_k|| <= rtol ||F_k||; the loop ends when all have.  The mixed
-// forms' outer product uses the FP32 element blocks, with per-case reliable
-// updates as in the single-case mixed solve: a case's residual is replaced
-// by its FP64 residual F_k - K U_k once the recursive one has fallen 100-fold
-// below the last FP64 one, or below the tolerance; a case stops only on its
-// FP64 residual.
+// ||F_k - K U_k|| <= rtol ||F_k||, checked on the FP64 residual (which then
+// replaces the recursive one) when the recursive one reaches the tolerance;
+// the loop ends when all have.  The mixed forms' outer product runs in FP32,
+// with per-case reliable updates as in the single-case mixed solve: a case's
+// residual is also replaced by its FP64 residual once the recursive one has
+// fallen 100-fold below the last FP64 one.
 enum { MS_RZ = 0, MS_PQ = 8, MS_ALPHA = 16, MS_RR = 24, MS_RZN = 32, MS_BETA = 48, MS_FN = 56, MS_TR = 64 };
 int solve_el_block(uft_ctx *C, const double *xphys) {
   hipStream_t s = C->s;
@@ -1196,11 +1196,9 @@ int solve_el_block(uft_ctx *C, const double *xphys) {
       const double rn = std::sqrt(C->h_scal[k]);
       if (!std::isfinite(rn)) return uft_set_error(C, UFT_ERR_NOCONV, "load case %d diverged at iteration %d", k, it);
       M.it[k] = it;
-      if (!f32) {
-        if (rn <= tol[k]) mask &= ~(1u << k);
-      } else if (rn <= tol[k] || rn < 1e-2 * rtrue[k]) {
-        need |= 1u << k;
-      }
+      // a case stops on its FP64 residual F_k - K U_k; the FP32 outer
+      // product also replaces the recursive residual after 100-fold drops
+      if (rn <= tol[k] || (f32 && rn < 1e-2 * rtrue[k])) need |= 1u << k;
     }
     if (need) {  // reliable update of the cases in need
       mr_lev_op<double>(C, 0, M.U, tr);
@@ -1227,7 +1225,7 @@ int solve_el_block(uft_ctx *C, const double *xphys) {
   C->it_el = it;
   C->outer_el = f32 ? C->reliable : 1;
   double worst = 0.0;
-  if (f32 && !mask && have_true) {
+  if (!mask && have_true) {
     // every case stopped on its FP64 residual, U unchanged since
     for (int k = 0; k < nr; ++k) {
       M.relres[k] = fn[k] > 0.0 ? rtrue[k] / fn[k] : 0.0;
