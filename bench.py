@@ -234,6 +234,8 @@ def main():
         sys.exit(2)
 
     forms = ["fp64", "mixed_h"] if args.precision == "both" else [args.precision]
+    if args.precision == "both" and args.workload == "c4a":  # the BF16 matrix-core form beside the default
+        forms = ["fp64", "mixed", "mixed_h"]
     res = {f: run_form(args, f) for f in forms}
     main_form = forms[-1]
     r = res[main_form]
@@ -283,9 +285,15 @@ def main():
             rooflines.append(elem_roofline("outer64", npe, ne, ndof, r["prof"][0], r["prof"][1], 8,
                                            f"{k64} (FP64 element phase of the outer loop's residuals, mixed "
                                            f"form)", f"{args.workload}_fp64_elem_apply", nrhs))
+    if "mixed" in res and main_form == "mixed_h" and multi:
+        q = res["mixed"]
+        rooflines.append(elem_roofline("bf16_mfma", npe, ne, ndof, q["prof"][2], q["prof"][3], 2,
+                                       f"k_mr_elem_mfma<{npe}> (BF16 v_mfma_f32_16x16x32 element products of the "
+                                       f"preconditioner, 8 load cases, UFT_MIXED form)",
+                                       f"{args.workload}_mixed_elem_apply", nrhs))
     # the dominant element phase of the timed form: the one with the most time
     cand = [x for x in rooflines if x["launches"]] if main_form == "fp64" else \
-        [x for x in rooflines if x["launches"] and x["role"] != "fp64"]
+        [x for x in rooflines if x["launches"] and x["role"] not in ("fp64", "bf16_mfma")]
     dom = max(cand, key=lambda x: x["total_ms_timed"]) if cand else rooflines[-1]
     if r["ndesign"] > 1:
         wl = (f"{args.workload}: {r['ndesign']} independent {ename} jittered cantilevers of uneven size "
@@ -304,6 +312,10 @@ def main():
         config["mixed_over_fp64"] = value / config["fp64_design_iters_per_s"]
         config["fp64_krylov_iters_per_solve"] = sum(f64["its"]) / len(f64["its"])
         config["max_rel_compliance_diff_vs_fp64"] = max(abs(a - b) / abs(b) for a, b in zip(r["fxs"], f64["fxs"]))
+    if "mixed" in res and main_form == "mixed_h":
+        q = res["mixed"]
+        config["bf16_mfma_form_design_iters_per_s"] = args.steps * q["ndesign"] / q["dt"]
+        config["bf16_mfma_form_krylov_iters_per_solve"] = sum(q["its"]) / len(q["its"])
     out = {
         "metric": "design iters/s @392k hex8 cantilever; mixed vs FP64 GPU at 1e-6 rel compliance",
         "value": value, "unit": "design iters/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,

@@ -94,4 +94,4 @@ def test_autotune_smoother():
     c = batch_cantilever(ds, precision=1, **prm)
     c.init_design()
     d, ct, ms = c.autotune_smoother()
-    assert ct in (0, 1) and ms[d - 1, ct] == ms[:, :2].min()
+    assert 0 <= ct <= 2 and ms[d - 1, ct] == ms[:, :3].min()

@@ -132,6 +132,90 @@ __global__ __launch_bounds__(TPB) void k_seg_true_res(SegG g, int64_t n, const d
 }
 
 // p = z + b_d p
+// ---- K-cycle of design batches: per-design inner products and coefficients
+// (the single-design K-cycle's formulas, k_final_kc, per design)
+// K inner products a_k . b_k per design (FP64 accumulation of T products)
+template <int K, typename T>
+__global__ __launch_bounds__(TPB) void k_seg_dots(SegG g, int64_t n, const T *__restrict__ a0, const T *__restrict__ b0,
+                                                  const T *__restrict__ a1, const T *__restrict__ b1,
+                                                  const T *__restrict__ a2, const T *__restrict__ b2,
+                                                  double *__restrict__ part) {
+  int64_t i;
+  int d;
+  double v[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) v[k] = 0.0;
+  if (seg_row(g, n, i, d)) {
+    v[0] = (double)a0[i] * (double)b0[i];
+    if constexpr (K > 1) v[1] = (double)a1[i] * (double)b1[i];
+    if constexpr (K > 2) v[2] = (double)a2[i] * (double)b2[i];
+  }
+  uftr::block_partials<K>(v, part);
+}
+// first inner step: sc[base + 7] = -alpha1 / rho1 (rho1 = sc[base], alpha1 = sc[base + 1])
+__global__ void k_bsc_kc1(int nseg, double *__restrict__ sc, int base) {
+  const int d = blockIdx.x * blockDim.x + threadIdx.x;
+  if (d >= nseg) return;
+  double *s = sc + (size_t)d * UFT_BSC + base;
+  s[7] = s[0] != 0.0 ? -s[1] / s[0] : 0.0;
+}
+// second inner step: sc[base + 5], sc[base + 6] from gamma, beta, alpha2 (sc[base + 2..4])
+__global__ void k_bsc_kc2(int nseg, double *__restrict__ sc, int base) {
+  const int d = blockIdx.x * blockDim.x + threadIdx.x;
+  if (d >= nseg) return;
+  double *s = sc + (size_t)d * UFT_BSC + base;
+  const double rho1 = s[0], a1 = s[1], g = s[2], be = s[3], a2 = s[4];
+  const double t1 = rho1 != 0.0 ? a1 / rho1 : 0.0;
+  const double rho2 = rho1 != 0.0 ? be - g * g / rho1 : 0.0;
+  if (rho2 > 0.0) {
+    s[5] = t1 - g * a2 / (rho1 * rho2);
+    s[6] = a2 / rho2;
+  } else {
+    s[5] = t1;
+    s[6] = 0.0;
+  }
+}
+// second start: kr = b + c7_d v ; r = kr ; d = it D^-1 kr ; kd = d
+template <typename T>
+__global__ __launch_bounds__(TPB) void k_seg_lincomb_first(SegG g, int64_t n, const double *__restrict__ sc, int ic,
+                                                           const T *__restrict__ b, const T *__restrict__ v,
+                                                           T *__restrict__ kr, T *__restrict__ r,
+                                                           const T *__restrict__ dinv, double it, T *__restrict__ dd,
+                                                           T *__restrict__ kd) {
+  int64_t i;
+  int d;
+  if (!seg_row(g, n, i, d)) return;
+  const T c = (T)sc[(size_t)d * UFT_BSC + ic];
+  const T ri = b[i] + c * v[i];
+  kr[i] = ri;
+  r[i] = ri;
+  const T di = (T)it * dinv[i] * ri;
+  dd[i] = di;
+  kd[i] = di;
+}
+// prolongation of a K-cycle result into the rows of the level above (segment
+// grid of those rows): x_u += P_u (c5_d xa + c6_d xb)_parent(u)
+template <int BS, typename T>
+__global__ __launch_bounds__(TPB) void k_seg_prolong_kc(SegG g, int64_t n, const int32_t *__restrict__ parent,
+                                                        const T *__restrict__ P, const double *__restrict__ sc, int i5,
+                                                        const T *__restrict__ xa, const T *__restrict__ xb,
+                                                        T *__restrict__ x) {
+  int64_t t;
+  int d;
+  if (!seg_row(g, n, t, d)) return;
+  const int64_t u = t / BS;
+  const int i = (int)(t - u * BS);
+  const int64_t A = parent[u];
+  if (A < 0) return;
+  const double *s = sc + (size_t)d * UFT_BSC + i5;
+  const T ca = (T)s[0], cb = (T)s[1];
+  const T *Pu = P + (size_t)u * BS * 6 + i * 6;
+  T acc = T(0);
+#pragma unroll
+  for (int j = 0; j < 6; ++j) acc += Pu[j] * (ca * xa[A * 6 + j] + cb * xb[A * 6 + j]);
+  x[t] += acc;
+}
+
 __global__ __launch_bounds__(TPB) void k_seg_xpby(SegG g, int64_t n, const double *__restrict__ z,
                                                   const double *__restrict__ sc, int ib, double *__restrict__ p) {
   int64_t i;
@@ -410,6 +494,54 @@ void seg_dot(hipStream_t s, SegG g, const int32_t *dfirst, int nseg, int64_t n, 
              double *part, double *sc, int base) {
   k_seg_dot<<<gsz(g), TPB, 0, s>>>(g, n, a, b, part);
   seg_final(s, part, dfirst, nseg, 1, sc, base);
+}
+template <typename T>
+static void seg_dots_t(hipStream_t s, SegG g, const int32_t *dfirst, int nseg, int64_t n, int K, const T *const *a,
+                       const T *const *b, double *part, double *sc, int base) {
+  if (K == 2)
+    k_seg_dots<2, T><<<gsz(g), TPB, 0, s>>>(g, n, a[0], b[0], a[1], b[1], nullptr, nullptr, part);
+  else
+    k_seg_dots<3, T><<<gsz(g), TPB, 0, s>>>(g, n, a[0], b[0], a[1], b[1], a[2], b[2], part);
+  seg_final(s, part, dfirst, nseg, K, sc, base);
+}
+void seg_dots(hipStream_t s, SegG g, const int32_t *dfirst, int nseg, int64_t n, int K, const double *const *a,
+              const double *const *b, double *part, double *sc, int base) {
+  seg_dots_t<double>(s, g, dfirst, nseg, n, K, a, b, part, sc, base);
+}
+void seg_dots(hipStream_t s, SegG g, const int32_t *dfirst, int nseg, int64_t n, int K, const float *const *a,
+              const float *const *b, double *part, double *sc, int base) {
+  seg_dots_t<float>(s, g, dfirst, nseg, n, K, a, b, part, sc, base);
+}
+void bsc_kc1(hipStream_t s, int nseg, double *sc, int base) { k_bsc_kc1<<<(nseg + 63) / 64, 64, 0, s>>>(nseg, sc, base); }
+void bsc_kc2(hipStream_t s, int nseg, double *sc, int base) { k_bsc_kc2<<<(nseg + 63) / 64, 64, 0, s>>>(nseg, sc, base); }
+template <typename T>
+static void seg_lincomb_first_t(hipStream_t s, SegG g, int64_t n, const double *sc, int ic, const T *b, const T *v,
+                                T *kr, T *r, const T *dinv, double it, T *d, T *kd) {
+  k_seg_lincomb_first<T><<<gsz(g), TPB, 0, s>>>(g, n, sc, ic, b, v, kr, r, dinv, it, d, kd);
+}
+void seg_lincomb_first(hipStream_t s, SegG g, int64_t n, const double *sc, int ic, const double *b, const double *v,
+                       double *kr, double *r, const double *dinv, double it, double *d, double *kd) {
+  seg_lincomb_first_t<double>(s, g, n, sc, ic, b, v, kr, r, dinv, it, d, kd);
+}
+void seg_lincomb_first(hipStream_t s, SegG g, int64_t n, const double *sc, int ic, const float *b, const float *v,
+                       float *kr, float *r, const float *dinv, double it, float *d, float *kd) {
+  seg_lincomb_first_t<float>(s, g, n, sc, ic, b, v, kr, r, dinv, it, d, kd);
+}
+template <typename T>
+static void seg_prolong_kc_t(hipStream_t s, SegG g, int bs, int64_t n, const int32_t *parent, const T *P,
+                             const double *sc, int i5, const T *xa, const T *xb, T *x) {
+  if (bs == 3)
+    k_seg_prolong_kc<3, T><<<gsz(g), TPB, 0, s>>>(g, n, parent, P, sc, i5, xa, xb, x);
+  else
+    k_seg_prolong_kc<6, T><<<gsz(g), TPB, 0, s>>>(g, n, parent, P, sc, i5, xa, xb, x);
+}
+void seg_prolong_kc(hipStream_t s, SegG g, int bs, int64_t n, const int32_t *parent, const double *P, const double *sc,
+                    int i5, const double *xa, const double *xb, double *x) {
+  seg_prolong_kc_t<double>(s, g, bs, n, parent, P, sc, i5, xa, xb, x);
+}
+void seg_prolong_kc(hipStream_t s, SegG g, int bs, int64_t n, const int32_t *parent, const float *P, const double *sc,
+                    int i5, const float *xa, const float *xb, float *x) {
+  seg_prolong_kc_t<float>(s, g, bs, n, parent, P, sc, i5, xa, xb, x);
 }
 void seg_dot_pr(hipStream_t s, SegG g, const int32_t *dfirst, int nseg, int64_t n, const double *r, const double *z,
                 const double *zo, double *part, double *sc, int base) {

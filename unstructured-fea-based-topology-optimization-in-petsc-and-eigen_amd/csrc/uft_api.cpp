@@ -315,12 +315,15 @@ int level_degree(const uft_ctx *C, int l) {
 int level_kind(const uft_ctx *C, int l) {
   const int ct = C->P.cycle_type;
   if (ct == 0 || (ct == 3 && l > 1)) return 0;
-  if (C->bt.nseg) return 1;  // design batches: the W-cycle in place of the K-cycle (no inner products)
+  // design batches: the K-cycle with per-design inner products on the first
+  // coarse levels (their scalars have device slots), the W-cycle below
+  if (C->bt.nseg) return ct >= 2 && l <= UFT_BATCH_KLEV ? 2 : 1;
   return ct == 1 ? 1 : 2;
 }
 
 template <typename T>
 void cycle_body(uft_ctx *C, int l, const T *b, T *x);
+const SegSpace &level_space(const uft_ctx *C, int l);
 
 // Solve on coarse level l (1 <= l < coarsest) whose first smoothing step from
 // a zero guess has been done by the restriction; the result is left where the
@@ -346,6 +349,28 @@ void coarse_body(uft_ctx *C, int l) {
     op.kd = v.kd;
     lev_op(C, l, v.x, op);
     cycle_body<T>(C, l, v.kr, v.kd);
+  } else if (C->bt.nseg) {  // design batch: the same K-cycle with each design's own coefficients
+    BatchState &B = C->bt;
+    const SegSpace &R = level_space(C, l);
+    const SegG g = R.g();
+    const int base = B_KC + 8 * (l - 1);
+    cycle_body<T>(C, l, v.b, v.kc);
+    RowOp<T> op;
+    op.mode = ROW_STORE;  // kv = A kc
+    op.y = v.kv;
+    lev_op(C, l, v.kc, op);
+    const T *a2[2] = {v.kc, v.kc}, *b2[2] = {v.kv, v.b};  // rho1 = c.v, alpha1 = c.b
+    uftk::seg_dots(C->s, g, R.d_dfirst, B.nseg, Lv.n, 2, a2, b2, C->d_part, B.sc, base);
+    uftk::bsc_kc1(C->s, B.nseg, B.sc, base);
+    uftk::seg_lincomb_first(C->s, g, Lv.n, B.sc, base + 7, v.b, v.kv, v.kr, v.r, v.dinv, it, v.d, v.kd);
+    cycle_body<T>(C, l, v.kr, v.kd);
+    RowOp<T> o2;
+    o2.mode = ROW_STORE;  // t = A kd
+    o2.y = v.t;
+    lev_op(C, l, v.kd, o2);
+    const T *a3[3] = {v.kd, v.kd, v.kd}, *b3[3] = {v.kv, v.t, v.kr};  // gamma = d.v, beta = d.Ad, alpha2 = d.r
+    uftk::seg_dots(C->s, g, R.d_dfirst, B.nseg, Lv.n, 3, a3, b3, C->d_part, B.sc, base + 2);
+    uftk::bsc_kc2(C->s, B.nseg, B.sc, base);
   } else {
     double *sc = C->d_scal + S_K + 8 * l;
     cycle_body<T>(C, l, v.b, v.kc);
@@ -389,7 +414,10 @@ void prolong_from(uft_ctx *C, int l, T *x) {
       uftk::prolong_comb(C->s, Lv.bs, Lv.nunit, Lv.d_parent, P, nullptr, -1, -1, vn.x, vn.kd, x);
       break;
     default:  // K-cycle: c5 kc + c6 kd
-      if (l >= 1)  // the level's final K-cycle sums in the same launch (few partials per block)
+      if (C->bt.nseg)  // each row with its design's coefficients
+        uftk::seg_prolong_kc(C->s, level_space(C, l).g(), Lv.bs, Lv.n, Lv.d_parent, P, C->bt.sc, B_KC + 8 * l + 5,
+                             vn.kc, vn.kd, x);
+      else if (l >= 1)  // the level's final K-cycle sums in the same launch (few partials per block)
         uftk::prolong_kc(C->s, Lv.bs, Lv.nunit, Lv.d_parent, P, C->d_part,
                          lev_op_blocks(C, l + 1, std::is_same<T, float>::value), C->d_scal + S_K + 8 * (l + 1),
                          vn.kc, vn.kd, x);
@@ -2828,7 +2856,7 @@ extern "C" int uft_amg_coarse_inverse_get(uft_ctx *C, double *inv) {
 extern "C" int uft_autotune_smoother(uft_ctx *C, int *degree, int *cycle_type, double *ms) {
   if (int rc = check_setup(C)) return rc;
   const int deg0 = C->P.cheb_degree, ct0 = C->P.cycle_type;
-  const int nct = C->bt.nseg ? 2 : 3;  // a batch runs the K-cycle as the W-cycle
+  const int nct = 3;
   const double *xp = C->vec[UFT_XPHYS];
   double *U = C->vec[UFT_U];
   double best = 1e300;

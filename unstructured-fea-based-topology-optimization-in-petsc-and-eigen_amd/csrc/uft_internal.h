@@ -56,8 +56,10 @@ enum {
   B_FB = 16, B_FZZ = 17, B_FBETA = 18, B_FBO = 19, B_FPQ = 20, B_FAL = 21, B_FPC = 22, B_FTOL = 23, B_FACT = 24,
   B_FIT = 25, B_FSN = 26, B_FDIV = 27, B_FZZN = 28, B_FZRN = 29,
   B_LRZ = 32, B_LPQ = 33, B_LAL = 34, B_LRR = 35, B_LRZN = 36, B_LBE = 37,
-  B_RED = 40
+  B_RED = 40,
+  B_KC = 44  // K-cycle scalars of levels 1 and 2, eight each (the single-design K-cycle's slots)
 };
+#define UFT_BATCH_KLEV 2  // coarse levels that run the K-cycle in a design batch (the W-cycle below)
 
 // E(x) = Emin + x^penal * (Emax - Emin)   (SolveFem.cpp:304)
 struct SimpCoef {
@@ -607,6 +609,24 @@ void seg_final(hipStream_t s, const double *part, const int32_t *dfirst, int nse
 void seg_final_max(hipStream_t s, const double *part, const int32_t *dfirst, int nseg, double *sc, int base);
 void seg_dot(hipStream_t s, SegG g, const int32_t *dfirst, int nseg, int64_t n, const double *a, const double *b,
              double *part, double *sc, int base);
+// design-batch K-cycle (kernels_batch.hip): K = 2 or 3 per-design inner
+// products a_k . b_k into sc[d * UFT_BSC + base + k]; the coefficients of the
+// two inner steps at base; the second start and the prolongation with the
+// coefficients of each row's design (segment grid of the rows)
+void seg_dots(hipStream_t s, SegG g, const int32_t *dfirst, int nseg, int64_t n, int K, const double *const *a,
+              const double *const *b, double *part, double *sc, int base);
+void seg_dots(hipStream_t s, SegG g, const int32_t *dfirst, int nseg, int64_t n, int K, const float *const *a,
+              const float *const *b, double *part, double *sc, int base);
+void bsc_kc1(hipStream_t s, int nseg, double *sc, int base);
+void bsc_kc2(hipStream_t s, int nseg, double *sc, int base);
+void seg_lincomb_first(hipStream_t s, SegG g, int64_t n, const double *sc, int ic, const double *b, const double *v,
+                       double *kr, double *r, const double *dinv, double it, double *d, double *kd);
+void seg_lincomb_first(hipStream_t s, SegG g, int64_t n, const double *sc, int ic, const float *b, const float *v,
+                       float *kr, float *r, const float *dinv, double it, float *d, float *kd);
+void seg_prolong_kc(hipStream_t s, SegG g, int bs, int64_t n, const int32_t *parent, const double *P, const double *sc,
+                    int i5, const double *xa, const double *xb, double *x);
+void seg_prolong_kc(hipStream_t s, SegG g, int bs, int64_t n, const int32_t *parent, const float *P, const double *sc,
+                    int i5, const float *xa, const float *xb, float *x);
 void seg_dot_pr(hipStream_t s, SegG g, const int32_t *dfirst, int nseg, int64_t n, const double *r, const double *z,
                 const double *zo, double *part, double *sc, int base);
 void seg_cg_update(hipStream_t s, SegG g, const int32_t *dfirst, int nseg, int64_t n, double *sc, int ia, double *x,
