@@ -396,31 +396,47 @@ __global__ __launch_bounds__(TPB) void k_mr_ell_op(int nn, const int64_t *__rest
 }
 
 // ---------------------------------------------------------------- 6x6 BSR
-// 32 lanes per scalar row: LPR = 32 / NR block slots x NR right-hand sides.
+// One wavefront per block row: lane (slot l, right-hand side rr), 64 / NR
+// slots; a lane reads whole 6x6 blocks (the NR lanes of a slot share each
+// load) and its right-hand side's six entries of u, accumulating all six
+// rows; a fixed butterfly over the slots combines them, and the slot-0 lanes
+// run the six rows' epilogues.
 template <typename T, int NR>
-__global__ __launch_bounds__(TPB) void k_mr_bsr_op(int64_t nrows, const int32_t *__restrict__ rp,
+__global__ __launch_bounds__(TPB) void k_mr_bsr_op(int64_t nbrows, const int32_t *__restrict__ rp,
                                                    const int32_t *__restrict__ ci, const T *__restrict__ val,
                                                    const T *__restrict__ u, MrOp<T> op) {
-  constexpr int LPR = 32 / NR;
+  constexpr int SLOTS = 64 / NR;
   const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
-  const int64_t row = t / 32;
-  const int l = (int)((t % 32) / NR), rr = (int)(t % NR);
-  T acc = T(0);
-  if (row < nrows * 6) {
-    const int64_t A = row / 6;
-    const int i = (int)(row - A * 6);
-    for (int k = rp[A] + l; k < rp[A + 1]; k += LPR) {
+  const int64_t A = t / 64;
+  const int lane = (int)(t % 64), l = lane / NR, rr = lane % NR;
+  T acc[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
+  if (A < nbrows) {
+    for (int k = rp[A] + l; k < rp[A + 1]; k += SLOTS) {
       const int64_t B = ci[k];
-      const T *v = val + (size_t)k * 36 + i * 6;
       const T *xb = u + (B * 6) * NR + rr;
+      T x[6];
 #pragma unroll
-      for (int j = 0; j < 6; ++j) acc += v[j] * xb[j * NR];
+      for (int j = 0; j < 6; ++j) x[j] = xb[j * NR];
+      const T *v = val + (size_t)k * 36;
+#pragma unroll
+      for (int i = 0; i < 6; ++i)
+#pragma unroll
+        for (int j = 0; j < 6; ++j) acc[i] += v[i * 6 + j] * x[j];
     }
   }
 #pragma unroll
-  for (int off = 16; off >= NR; off >>= 1) acc += __shfl_down(acc, off, 32);
+  for (int i = 0; i < 6; ++i) {
+#pragma unroll
+    for (int off = 32; off >= NR; off >>= 1) acc[i] += __shfl_xor(acc[i], off, 64);
+  }
   double dv[3] = {0.0, 0.0, 0.0};
-  if (l == 0 && row < nrows * 6) mr_epi(op, row, row * NR + rr, acc, u[row * NR + rr], dv);
+  if (l == 0 && A < nbrows) {
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+      const int64_t row = A * 6 + i;
+      mr_epi(op, row, row * NR + rr, acc[i], u[row * NR + rr], dv);
+    }
+  }
   mr_epi_partials<T, NR>(op, dv);
 }
 
@@ -884,13 +900,13 @@ template <typename T>
 static void mr_bsr_op_t(hipStream_t s, int nr, int64_t nb_rows, const int32_t *rp, const int32_t *ci, const T *val,
                         const T *u, const RowOp<T> &o) {
   const MrOp<T> op = to_mr(o);
-  UFT_NR_SWITCH(nr, (k_mr_bsr_op<T, NR><<<nblk(nb_rows * 6 * 32), TPB, 0, s>>>(nb_rows, rp, ci, val, u, op)));
+  UFT_NR_SWITCH(nr, (k_mr_bsr_op<T, NR><<<nblk(nb_rows * 64), TPB, 0, s>>>(nb_rows, rp, ci, val, u, op)));
 }
 void mr_bsr_op(hipStream_t s, int nr, int64_t nb_rows, const int32_t *rp, const int32_t *ci, const double *val,
                const double *u, const RowOp<double> &o) { mr_bsr_op_t<double>(s, nr, nb_rows, rp, ci, val, u, o); }
 void mr_bsr_op(hipStream_t s, int nr, int64_t nb_rows, const int32_t *rp, const int32_t *ci, const float *val,
                const float *u, const RowOp<float> &o) { mr_bsr_op_t<float>(s, nr, nb_rows, rp, ci, val, u, o); }
-int mr_bsr_blocks(int64_t nb_rows) { return (int)nblk(nb_rows * 6 * 32); }
+int mr_bsr_blocks(int64_t nb_rows) { return (int)nblk(nb_rows * 64); }
 
 template <typename T>
 static void mr_restrict_first_t(hipStream_t s, int nr, int bs, int64_t nc, const int32_t *cptr, const int32_t *child,
