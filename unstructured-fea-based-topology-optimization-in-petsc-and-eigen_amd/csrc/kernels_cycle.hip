@@ -421,7 +421,9 @@ template <typename CI>
 __global__ __launch_bounds__(TPB) void k_helm_ell(int nn, const int64_t *__restrict__ off,
                                                   const int32_t *__restrict__ width, const CI *__restrict__ col,
                                                   const double *__restrict__ hv, const double *__restrict__ x,
-                                                  double *__restrict__ y, double *__restrict__ part) {
+                                                  double *__restrict__ y, double *__restrict__ part,
+                                                  const double *__restrict__ stop) {
+  if (stop && *stop != 0.0) return;  // the same for every block
   const int n = blockIdx.x * TPB + threadIdx.x;
   double ds[1] = {0.0};
   if (n < nn) {
@@ -636,7 +638,8 @@ __global__ __launch_bounds__(TPB) void k_final_kc(const double *__restrict__ par
 // read by columns (loads coalesced over rows; four column groups added in a
 // fixed order).  The prolongation x += P xc follows as an ordinary launch.
 constexpr int TAIL_TPB = 1024;
-constexpr int TAIL_CH = 512;  // children staged in LDS per pass
+constexpr int TAIL_CH = 1024;  // children staged in LDS per pass
+constexpr int TAIL_PU = 4;     // prolongation rows per thread in flight
 // xc = Ainv P^T r on the coarsest level, one workgroup (a design batch: one
 // per design, units [u0, u0 + nu) with its inverse block at Ainv + coff).
 // Restriction: one thread per child computes P_u^T r_u, staged in LDS; one
@@ -646,6 +649,9 @@ constexpr int TAIL_CH = 512;  // children staged in LDS per pass
 // accumulators; fixed summation order throughout.  With xup (one design), the
 // same workgroup then prolongs: xup_u += P_u xc_parent(u) for the nu_up units
 // of the level above (the arithmetic of k_prolong_comb), saving a launch.
+// A single workgroup is latency-bound, so the dense product loads 16 inverse
+// entries per thread before its first multiply and the prolongation loads
+// TAIL_PU rows per thread at once; neither changes an operation's order.
 template <typename T, typename TA>
 __global__ __launch_bounds__(TAIL_TPB) void k_tail_solve(int nc_units, const int32_t *__restrict__ cptr,
                                                          const int32_t *__restrict__ child, const T *__restrict__ P,
@@ -700,6 +706,13 @@ __global__ __launch_bounds__(TAIL_TPB) void k_tail_solve(int nc_units, const int
     double a[4] = {0.0, 0.0, 0.0, 0.0};
     if (row < n) {
       int c = g;
+      for (; c + 60 < n; c += 64) {  // columns c, c + 4, ..., c + 60
+        TA buf[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) buf[q] = Ainv[(size_t)(c + 4 * q) * n + row];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) a[q & 3] += (double)buf[q] * bc[c + 4 * q];
+      }
       for (; c + 12 < n; c += 16) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) a[q] += (double)Ainv[(size_t)(c + 4 * q) * n + row] * bc[c + 4 * q];
@@ -716,16 +729,32 @@ __global__ __launch_bounds__(TAIL_TPB) void k_tail_solve(int nc_units, const int
     __syncthreads();
   }
   if (!xup) return;
-  for (int64_t t = threadIdx.x; t < nu_up * 6; t += TAIL_TPB) {
-    const int64_t u = t / 6;
-    const int i = (int)(t - u * 6);
-    const int64_t A = parent[u];
-    if (A < 0) continue;
-    const T *Pu = P + (size_t)u * 36 + i * 6;
-    T acc = T(0);
+  const int64_t nt = nu_up * 6;
+  for (int64_t t0 = threadIdx.x; t0 < nt; t0 += (int64_t)TAIL_TPB * TAIL_PU) {
+    int A[TAIL_PU];
+    T pv[TAIL_PU][6], xo[TAIL_PU];
 #pragma unroll
-    for (int j = 0; j < 6; ++j) acc += Pu[j] * xs[A * 6 + j];
-    xup[t] += acc;
+    for (int q = 0; q < TAIL_PU; ++q) {
+      const int64_t t = t0 + (int64_t)q * TAIL_TPB;
+      A[q] = -1;
+      if (t < nt) {
+        const int64_t u = t / 6;
+        const int i = (int)(t - u * 6);
+        A[q] = parent[u];
+        const T *Pu = P + (size_t)u * 36 + i * 6;
+#pragma unroll
+        for (int j = 0; j < 6; ++j) pv[q][j] = Pu[j];
+        xo[q] = xup[t];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < TAIL_PU; ++q) {
+      if (A[q] < 0) continue;
+      T acc = T(0);
+#pragma unroll
+      for (int j = 0; j < 6; ++j) acc += pv[q][j] * xs[A[q] * 6 + j];
+      xup[t0 + (int64_t)q * TAIL_TPB] = xo[q] + acc;
+    }
   }
 }
 
@@ -827,10 +856,21 @@ void helm_ell_apply(hipStream_t s, const uft_ctx *C, const double *x, double *y,
   const int nn = (int)C->nn;
   double *part = dot_out ? C->d_part : nullptr;
   if (F.d_cd)
-    k_helm_ell<int16_t><<<nblk(nn), TPB, 0, s>>>(nn, F.d_off, F.d_width, F.d_cd, F.d_hv, x, y, part);
+    k_helm_ell<int16_t><<<nblk(nn), TPB, 0, s>>>(nn, F.d_off, F.d_width, F.d_cd, F.d_hv, x, y, part, nullptr);
   else
-    k_helm_ell<int32_t><<<nblk(nn), TPB, 0, s>>>(nn, F.d_off, F.d_width, F.d_col, F.d_hv, x, y, part);
+    k_helm_ell<int32_t><<<nblk(nn), TPB, 0, s>>>(nn, F.d_off, F.d_width, F.d_col, F.d_hv, x, y, part, nullptr);
   if (dot_out) final_sum(s, part, (int)nblk(nn), 1, dot_out);
+}
+void helm_ell_cg(hipStream_t s, const uft_ctx *C, const double *z, const double *sc, int ib, int ibo, int first,
+                 const double *stop, double *p, double *y, double *dot_out) {
+  const auto &F = C->fell;
+  const int nn = (int)C->nn;
+  fcg_p_ctl(s, nn, z, sc, ib, ibo, first, stop, p);
+  if (F.d_cd)
+    k_helm_ell<int16_t><<<nblk(nn), TPB, 0, s>>>(nn, F.d_off, F.d_width, F.d_cd, F.d_hv, p, y, C->d_part, stop);
+  else
+    k_helm_ell<int32_t><<<nblk(nn), TPB, 0, s>>>(nn, F.d_off, F.d_width, F.d_col, F.d_hv, p, y, C->d_part, stop);
+  final_sum(s, C->d_part, (int)nblk(nn), 1, dot_out);
 }
 void fine_ell_op(hipStream_t s, const uft_ctx *C, const float *u, const RowOp<float> &op) {
   const auto &F = C->fell;

@@ -69,12 +69,14 @@ __global__ __launch_bounds__(TPB) void k_fcg_update(int64_t n, const double *__r
   uftr::block_partials<2>(v, part);
 }
 
-// the same with alpha = sc[ia] / sc[ib] formed in place
+// the same with alpha = sc[ia] / sc[ib] formed in place; nothing once *stop
+// is set (the device-side convergence flag of the filter CG)
 __global__ __launch_bounds__(TPB) void k_fcg_update_ab(int64_t n, const double *__restrict__ sc, int ia, int ib,
                                                        double *__restrict__ x, const double *__restrict__ p,
                                                        double *__restrict__ r, const double *__restrict__ q,
                                                        const double *__restrict__ dinv, double *__restrict__ z,
-                                                       double *__restrict__ part) {
+                                                       double *__restrict__ part, const double *__restrict__ stop) {
+  if (stop && *stop != 0.0) return;  // the same for every block
   const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
   const double al = sc[ia] / sc[ib];
   double v[2] = {0.0, 0.0};
@@ -102,11 +104,50 @@ __global__ __launch_bounds__(TPB) void k_final_fcg(const double *__restrict__ pa
   }
 }
 
+// The filter CG's control on the device (slots ic .. ic + 4: state, tolerance,
+// divergence bound, iteration count, last ||z||), so that the host queues
+// several iterations per round trip.  State 0 runs, 1 converged, 2 diverged.
+// Start, after the sums of the first residual (sc[i0] = ||D^-1 b||^2,
+// sc[i0 + 1] = ||z||^2): the tests of PETSc's KSPCG with the preconditioned
+// norm, rtol relative to ||D^-1 b|| (||z|| when b = 0), atol 1e-50, dtol 1e5.
+__global__ void k_fcg_start(double *sc, int i0, int ic, double rtol) {
+  const double sn0 = sqrt(sc[i0]), dp = sqrt(sc[i0 + 1]);
+  const double snorm = sn0 == 0.0 ? dp : sn0;
+  const double a = rtol * snorm;
+  const double ttol = a < 1e-50 ? 1e-50 : a;
+  sc[ic + 1] = ttol;
+  sc[ic + 2] = 1e5 * snorm;
+  sc[ic + 3] = 0.0;
+  sc[ic + 4] = dp;
+  sc[ic] = dp <= ttol ? 1.0 : 0.0;
+}
+// k_final_fcg with the iteration's tests: count the iteration, then stop on
+// ||z|| <= tolerance or on divergence
+__global__ __launch_bounds__(TPB) void k_final_fcg_ctl(const double *__restrict__ part, int nb, double *__restrict__ sc,
+                                                       int izz, int ib, int ibo, int ic) {
+  if (sc[ic] != 0.0) return;
+  double res[2];
+  uftr::block_final<2>(part, nb, res);
+  if (threadIdx.x == 0) {
+    sc[izz] = res[0];
+    sc[ibo] = sc[ib];
+    sc[ib] = res[1];
+    sc[ic + 3] += 1.0;
+    const double dp = sqrt(res[0]);
+    sc[ic + 4] = dp;
+    if (dp <= sc[ic + 1])
+      sc[ic] = 1.0;
+    else if (!(dp < sc[ic + 2]))
+      sc[ic] = 2.0;
+  }
+}
+
 // p = z + (sc[ib] / sc[ibo]) p, or p = z when first
 __global__ __launch_bounds__(TPB) void k_fcg_p(int64_t n, const double *__restrict__ z, const double *__restrict__ sc,
-                                               int ib, int ibo, int first, double *__restrict__ p) {
+                                               int ib, int ibo, int first, double *__restrict__ p,
+                                               const double *__restrict__ stop) {
   const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
-  if (i >= n) return;
+  if (i >= n || (stop && *stop != 0.0)) return;
   if (first) {
     p[i] = z[i];
   } else {
@@ -177,11 +218,22 @@ void fcg_update(hipStream_t s, int64_t n, const double *sc, int ia, double *x, c
 }
 void fcg_update_ab(hipStream_t s, int64_t n, double *sc, int ia, int ib, double *x, const double *p, double *r,
                    const double *q, const double *dinv, double *z, double *part, int izz, int ibeta, int ibo) {
-  k_fcg_update_ab<<<nblk(n), TPB, 0, s>>>(n, sc, ia, ib, x, p, r, q, dinv, z, part);
+  k_fcg_update_ab<<<nblk(n), TPB, 0, s>>>(n, sc, ia, ib, x, p, r, q, dinv, z, part, nullptr);
   k_final_fcg<<<1, TPB, 0, s>>>(part, (int)nblk(n), sc, izz, ibeta, ibo);
 }
+void fcg_start(hipStream_t s, double *sc, int i0, int ic, double rtol) { k_fcg_start<<<1, 1, 0, s>>>(sc, i0, ic, rtol); }
+void fcg_update_ctl(hipStream_t s, int64_t n, double *sc, int ia, int ib, double *x, const double *p, double *r,
+                    const double *q, const double *dinv, double *z, double *part, int izz, int ibeta, int ibo,
+                    int ic) {
+  k_fcg_update_ab<<<nblk(n), TPB, 0, s>>>(n, sc, ia, ib, x, p, r, q, dinv, z, part, sc + ic);
+  k_final_fcg_ctl<<<1, TPB, 0, s>>>(part, (int)nblk(n), sc, izz, ibeta, ibo, ic);
+}
 void fcg_p(hipStream_t s, int64_t n, const double *z, const double *sc, int ib, int ibo, int first, double *p) {
-  k_fcg_p<<<nblk(n), TPB, 0, s>>>(n, z, sc, ib, ibo, first, p);
+  k_fcg_p<<<nblk(n), TPB, 0, s>>>(n, z, sc, ib, ibo, first, p, nullptr);
+}
+void fcg_p_ctl(hipStream_t s, int64_t n, const double *z, const double *sc, int ib, int ibo, int first,
+               const double *stop, double *p) {
+  k_fcg_p<<<nblk(n), TPB, 0, s>>>(n, z, sc, ib, ibo, first, p, stop);
 }
 void lanczos_step(hipStream_t s, int64_t n, double *sc, int irz, int ipq, int ibeta, double *r, const double *q,
                   const double *dinv, double *z, double *part, double *log2) {

@@ -31,7 +31,8 @@ enum {
   S_F0 = 8,  // filter: 8 ||Db||^2, 9 ||z||^2, 10 z.r (beta), 11 beta_old, 12 pq, 13 alpha
   S_MISC = 16,
   S_RED = 32,  // general reduction outputs (up to 48 values)
-  S_K = 96     // K-cycle scalars, 8 per level
+  S_K = 96,    // K-cycle scalars, 8 per level
+  S_FD = 200   // filter CG control: state, tolerance, divergence bound, iterations, ||z||
 };
 
 int dalloc(uft_ctx *C, void **p, size_t bytes) {
@@ -1442,7 +1443,14 @@ int solve_filter_batch(uft_ctx *C, const double *b, double *x) {
 // Jacobi-CG on KF RX = TX with the reference's KSPCG settings
 // (PDEFilter.cpp:117-137): warm start from the previous RX, preconditioned
 // residual norm relative to ||D^-1 b||, atol 1e-50, dtol 1e5.
-int solve_filter(uft_ctx *C, const double *b, double *x) {
+// With the filter operator assembled the tests run on the device
+// (fcg_start / fcg_update_ctl) and the host queues a solve's iterations in
+// rounds: as many as the same solve (slot: 0 the density filter, 1 + j the
+// j-th sensitivity) took last time, then two at a time; a queued iteration
+// after the stop does nothing.  The iterates, the iteration count and the
+// tests are those of the loop below, which tests on the host after every
+// iteration.
+int solve_filter(uft_ctx *C, const double *b, double *x, int slot) {
   if (C->bt.nseg) return solve_filter_batch(C, b, x);
   hipStream_t s = C->s;
   const int64_t n = C->nn;
@@ -1450,6 +1458,32 @@ int solve_filter(uft_ctx *C, const double *b, double *x) {
   double *sc = C->d_scal;
   uftk::apply_KF(s, C, x, q);
   uftk::fcg_init(s, n, b, q, C->d_fdinv, r, z, C->d_part, sc + S_F0);  // 8: |Db|^2 9: |z|^2 10: z.r
+  if (C->fell.d_hv) {
+    uftk::fcg_start(s, sc, S_F0, S_FD, C->P.rtol_f);
+    const int maxit = C->P.maxit_f;
+    int *pred = &C->f_pred[std::min(std::max(slot, 0), UFT_MAXM + 1)];
+    int queued = 0, round = *pred > 0 ? *pred : 8;
+    C->it_f = 0;
+    for (;;) {
+      const int upto = std::min(maxit, queued + round);
+      for (; queued < upto; ++queued) {
+        uftk::helm_ell_cg(s, C, z, sc, S_F0 + 2, S_F0 + 3, queued == 0, sc + S_FD, p, q, sc + S_F0 + 4);
+        uftk::fcg_update_ctl(s, n, sc, S_F0 + 2, S_F0 + 4, x, p, r, q, C->d_fdinv, z, C->d_part, S_F0 + 6,
+                             S_F0 + 2, S_F0 + 3, S_FD);
+      }
+      const int rc = fetch(C, sc + S_FD, 5, C->h_scal);
+      if (rc) return rc;
+      if (C->h_scal[0] != 0.0 || queued >= maxit) break;
+      round = 2;
+    }
+    const int it = (int)C->h_scal[3];
+    const double state = C->h_scal[0];
+    C->it_f = it;
+    *pred = it;
+    if (state == 2.0) return uft_set_error(C, UFT_ERR_NOCONV, "filter CG diverged at iteration %d", it);
+    if (state == 0.0) return uft_set_error(C, UFT_ERR_NOCONV, "filter CG: %d iterations without convergence", it);
+    return UFT_OK;
+  }
   int rc = fetch(C, sc + S_F0, 2, C->h_scal);
   if (rc) return rc;
   double snorm = std::sqrt(C->h_scal[0]);
@@ -2237,7 +2271,7 @@ int uft_filter(uft_ctx *C, int x_id, int xf_id) {
   if (!valid_vec(C, x_id) || !valid_vec(C, xf_id) || C->vlen[x_id] != C->ne || C->vlen[xf_id] != C->ne)
     return uft_set_error(C, UFT_ERR_ARG, "uft_filter: bad ids");
   uftk::tf_mult(C->s, C, C->vec[x_id], C->d_TX);
-  rc = solve_filter(C, C->d_TX, C->vec[UFT_RX]);
+  rc = solve_filter(C, C->d_TX, C->vec[UFT_RX], 0);
   uftk::tfx_t(C->s, C, C->vec[UFT_RX], C->vec[xf_id], 1, C->P.Xmin);
   return rc;
 }
@@ -2249,10 +2283,11 @@ int uft_filter_sens(uft_ctx *C, int dfdx_id, int m, const int *dgdx_ids) {
   int out = UFT_OK;
   std::vector<int> ids{dfdx_id};
   for (int j = 0; j < m; ++j) ids.push_back(dgdx_ids[j]);
-  for (int id : ids) {
+  for (size_t k = 0; k < ids.size(); ++k) {
+    const int id = ids[k];
     if (!valid_vec(C, id) || C->vlen[id] != C->ne) return uft_set_error(C, UFT_ERR_ARG, "uft_filter_sens: bad id");
     uftk::tf_mult(C->s, C, C->vec[id], C->d_TX);
-    rc = solve_filter(C, C->d_TX, C->vec[UFT_RX]);
+    rc = solve_filter(C, C->d_TX, C->vec[UFT_RX], 1 + (int)k);
     if (rc && rc != UFT_ERR_NOCONV) return rc;
     if (rc) out = rc;
     uftk::tfx_t(C->s, C, C->vec[UFT_RX], C->vec[id], 0, 0.0);

@@ -265,6 +265,7 @@ struct uft_ctx {
 
   // filter work (nn)
   double *d_fr = nullptr, *d_fz = nullptr, *d_fp = nullptr, *d_fq = nullptr, *d_fdinv = nullptr;
+  int f_pred[UFT_MAXM + 2] = {};  // last iteration counts of the filter solves (queue sizes)
   double *d_TX = nullptr;
 
   // reductions
@@ -516,6 +517,10 @@ void fine_ell_op32(hipStream_t s, const uft_ctx *C, const float *u, const RowOp<
 // density-free), then y = KF x (with dot_out: x.y into it via d_part)
 void helm_ell_fill(hipStream_t s, const uft_ctx *C);
 void helm_ell_apply(hipStream_t s, const uft_ctx *C, const double *x, double *y, double *dot_out);
+// filter CG step on the assembled pattern: p = z + (sc[ib] / sc[ibo]) p (z
+// when first), y = KF p, *dot_out = p.y; nothing once *stop is set
+void helm_ell_cg(hipStream_t s, const uft_ctx *C, const double *z, const double *sc, int ib, int ibo, int first,
+                 const double *stop, double *p, double *y, double *dot_out);
 int gather_op_blocks(int64_t nn);
 void restrict_first(hipStream_t s, int bs, int64_t nc, const int32_t *cptr, const int32_t *child, const double *P,
                     const double *r, double *bc, const double *dinv, double it, double *rc, double *dc, double *xc);
@@ -581,6 +586,8 @@ void fcg_init(hipStream_t s, int64_t n, const double *b, const double *q, const 
 void fcg_update(hipStream_t s, int64_t n, const double *sc, int ia, double *x, const double *p, double *r,
                 const double *q, const double *dinv, double *z, double *part, double *out);
 void fcg_p(hipStream_t s, int64_t n, const double *z, const double *sc, int ib, int ibo, int first, double *p);
+void fcg_p_ctl(hipStream_t s, int64_t n, const double *z, const double *sc, int ib, int ibo, int first,
+               const double *stop, double *p);
 // one Lanczos (Jacobi-CG) step after q = A p and p.q in sc[ipq]: r -= alpha q,
 // z = D^-1 r, rz' and beta into sc, (alpha, beta) into log2
 void lanczos_step(hipStream_t s, int64_t n, double *sc, int irz, int ipq, int ibeta, double *r, const double *q,
@@ -591,6 +598,14 @@ void lanczos_step(hipStream_t s, int64_t n, double *sc, int irz, int ipq, int ib
 // sc[ibo] = sc[ibeta], sc[ibeta] = z.r
 void fcg_update_ab(hipStream_t s, int64_t n, double *sc, int ia, int ib, double *x, const double *p, double *r,
                    const double *q, const double *dinv, double *z, double *part, int izz, int ibeta, int ibo);
+// the filter CG's device-side control in sc[ic .. ic + 4] (state, tolerance,
+// divergence bound, iteration count, ||z||): fcg_start after fcg_init's sums
+// in sc[i0], sc[i0 + 1]; fcg_update_ctl is fcg_update_ab that counts the
+// iteration, applies the tests and does nothing once the state is set
+void fcg_start(hipStream_t s, double *sc, int i0, int ic, double rtol);
+void fcg_update_ctl(hipStream_t s, int64_t n, double *sc, int ia, int ib, double *x, const double *p, double *r,
+                    const double *q, const double *dinv, double *z, double *part, int izz, int ibeta, int ibo,
+                    int ic);
 void sc_div(hipStream_t s, double *sc, int a, int b, int dst);
 void sc_beta(hipStream_t s, double *sc, int inew, int iold, int ibeta, double *log2);
 void sc_copy(hipStream_t s, const double *src, double *dst);
