@@ -77,13 +77,14 @@ def algorithmic_bytes_elem_phase(npe, ne, ndof, w=8, nrhs=1):
     return ne * (S * w + 4 * npe + 8 + (4 if w == 2 else 0)) + ndof * wv
 
 
-def algorithmic_bytes_fine_ell(blocks, ndof, col_bytes, entry_bytes=2, scaled=True, nrhs=1):
+def algorithmic_bytes_fine_ell(blocks, ndof, col_bytes, value_bytes, scaled=True, nrhs=1):
     """One apply of an assembled fine operator: every nonzero 3x3 node block
-    once (nine entries and its column index) for all nrhs right-hand sides,
-    u read once and one output vector per right-hand side (FP32), the row
-    scales of the FP16 form, and the fixed-DOF byte per node (padding slots
-    and the epilogue's further vectors not counted)."""
-    return blocks * (9 * entry_bytes + col_bytes) + ndof * 4 * (2 * nrhs + (1 if scaled else 0)) + ndof // 3
+    once (value_bytes: 12 for int8 entries with a block scale, 18 for FP16, 36
+    for FP32, and its column index) for all nrhs right-hand sides, u read once
+    and one output vector per right-hand side (FP32), the row scales of the
+    scaled smoother forms, and the fixed-DOF byte per node (padding slots and
+    the epilogue's further vectors not counted)."""
+    return blocks * (value_bytes + col_bytes) + ndof * 4 * (2 * nrhs + (1 if scaled else 0)) + ndof // 3
 
 
 def algorithmic_bytes_apply(npe, ne, ndof, w=8):
@@ -259,13 +260,14 @@ def main():
             w, kname = 2, f"k_elem_apply_h<{npe}> (FP16-block element phase of the smoother, mixed form)"
         else:
             w, kname = 4, f"k_elem_apply_v<{npe},float> (FP32 element phase, mixed form)"
-        if r["ell"][0]:  # assembled FP16 fine operator: one kernel per apply
-            kn = f"k_mr_ell_op<{nrhs},FP16> (8 load cases)" if multi else "k_fine_ell_op"
+        if r["ell"][0]:  # assembled fine smoother operator: one kernel per apply
+            kn = f"k_mr_ell_op<{nrhs},FP16> (8 load cases)" if multi else "k_fine_ell_op_q"
+            fmt = "FP16" if r["ell"][3] == 18 else "int8 (FP32 block scale)"
             rooflines.append(elem_roofline(
                 main_form, npe, ne, ndof, r["prof"][2], r["prof"][3], 2,
-                f"{kn} (assembled FP16 3x3 node blocks of the smoother, sliced ELL, whole apply with its "
+                f"{kn} (assembled {fmt} 3x3 node blocks of the smoother, sliced ELL, whole apply with its "
                 "row epilogue, mixed form)", f"{args.workload}_{main_form}_fine_ell", nrhs,
-                algo=algorithmic_bytes_fine_ell(r["ell"][1], ndof, r["ell"][2], nrhs=nrhs)))
+                algo=algorithmic_bytes_fine_ell(r["ell"][1], ndof, r["ell"][2], r["ell"][3], nrhs=nrhs)))
         else:
             rooflines.append(elem_roofline(main_form, npe, ne, ndof, r["prof"][2], r["prof"][3], w, kname,
                                            f"{args.workload}_{main_form}_elem_apply", nrhs))
@@ -275,7 +277,7 @@ def main():
                 "outer", npe, ne, ndof, r["prof"][4], r["prof"][5], 4,
                 f"{kn} (assembled FP32 3x3 node blocks of the outer operator, whole apply, mixed form)",
                 f"{args.workload}_fp32_fine_ell", nrhs,
-                algo=algorithmic_bytes_fine_ell(r["ell"][1], ndof, r["ell"][2], 4, False, nrhs)))
+                algo=algorithmic_bytes_fine_ell(r["ell"][1], ndof, r["ell"][2], 36, False, nrhs)))
         elif r["prof"][4][0]:  # FP32 outer operator, FP64 residual at reliable updates
             k32 = f"k_mr_elem<{npe},float,8>" if multi else f"k_elem_apply_v<{npe},float>"
             rooflines.append(elem_roofline("outer", npe, ne, ndof, r["prof"][4], r["prof"][5], 4,

@@ -181,11 +181,12 @@ int uft_element_kf(uft_ctx *ctx, int64_t e, double *kf, double *t);
  * and, per level, the element count (level 0) or 6x6 block count. */
 int uft_solver_info(const uft_ctx *ctx, int *nlev, int64_t *n_per_level, int64_t *blocks_per_level);
 /* The assembled fine operators (fine_assembled; no counterpart in the
- * reference): slots of their sliced-ELL storage (3x3 node blocks, FP16 for the
- * smoother and FP32 for the mixed outer product, padding included), nonzero
- * node blocks and bytes per stored column index (2 or 4); all 0 when the fine
- * level uses element blocks. */
-int uft_fine_operator_info(const uft_ctx *ctx, int64_t *slots, int64_t *blocks, int *col_bytes);
+ * reference): slots of their sliced-ELL storage (3x3 node blocks, padding
+ * included), nonzero node blocks, bytes per stored column index (2 or 4) and
+ * bytes per stored smoother block (12: int8 entries with an FP32 block scale,
+ * one right-hand side; 18: FP16, the multi-load-case form; the mixed outer
+ * product's blocks are FP32); all 0 when the fine level uses element blocks. */
+int uft_fine_operator_info(const uft_ctx *ctx, int64_t *slots, int64_t *blocks, int *col_bytes, int *value_bytes);
 /* --- multi-load-case form (load_cases = 1; no counterpart in the reference,
  * which superposes its cases) --------------------------------------------- */
 /* Number of load cases; per case the Krylov iterations and final relative

@@ -117,9 +117,9 @@ def test_coarse_inverse(dims, coarse_max):
 def test_level_operators_in_each_precision(assembled):
     """Every multigrid level's stored operator in FP32 and FP16 (UFT_MIXED_H:
     the fine operator assembled into 3x3 node blocks with rows scaled by the
-    diagonal, or the element blocks scaled per element; coarse levels above
-    the coarsest scaled to unit diagonal) against the FP64 one, with a
-    non-smooth random vector."""
+    diagonal, stored as int8 with one FP32 scale per block, or the element
+    blocks scaled per element in FP16; coarse levels above the coarsest scaled
+    to unit diagonal) against the FP64 one, with a non-smooth random vector."""
     from uft.capi import cantilever
     print(f"[case] level operators FP64/FP32/FP16 fine_assembled={assembled}", flush=True)
     ctx = cantilever(8, 24, 12, 12, jitter=0.15, seed=9, Emax=1.0, volfrac=0.3, rmin=0.2, precision=2,
@@ -139,5 +139,10 @@ def test_level_operators_in_each_precision(assembled):
         assert e32 < 1e-6
         if lev < len(n) - 1:  # FP16 blocks: every level but the coarsest (solved densely)
             e16 = np.linalg.norm(ctx.level_apply(lev, u, 2) - y64) / scale
-            print("level", lev, "FP16 error", e16)
-            assert e16 < 2e-3 and e16 > 1e-7  # FP16 rounding is visible, and only rounding
+            print("level", lev, "FP16 / int8 error", e16)
+            if lev == 0 and assembled:
+                # int8 grid of +-127 steps of the block's largest entry: rounding
+                # of at most 1/254 of it per entry, a few 1e-3 over the rows
+                assert e16 < 1e-2 and e16 > 1e-5
+            else:
+                assert e16 < 2e-3 and e16 > 1e-7  # FP16 rounding is visible, and only rounding

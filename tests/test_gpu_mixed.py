@@ -77,9 +77,9 @@ def test_assembled_fine_operator_any_numbering(shuffle):
         ctx.setup()
         info = ctx.fine_operator_info()
         if fa:
-            assert info[2] == (4 if shuffle else 2) and info[1] > 0 and info[0] >= info[1]
+            assert info[2] == (4 if shuffle else 2) and info[1] > 0 and info[0] >= info[1] and info[3] == 12
         else:
-            assert info == (0, 0, 0)
+            assert info == (0, 0, 0, 0)
         ctx.init_design()
         res[fa] = [ctx.design_iteration(it)[0] for it in (1, 2)]
         assert ctx.stats()["relres_el"] <= 1e-10

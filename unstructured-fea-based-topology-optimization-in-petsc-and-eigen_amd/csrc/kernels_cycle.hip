@@ -283,6 +283,57 @@ __global__ __launch_bounds__(TPB) void k_fine_ell_op(int nn, const int64_t *__re
   epi_partials_m<float, MODE>(op, dv);
 }
 
+// The same smoother operator with int8 blocks: per slot eight entries in a
+// uint2 and the block's FP32 scale with entry 8 in its low byte (12 bytes
+// instead of 18): t_i = dg_i sum_s sc_s (q_s u_col(n,s))_i.  The int8 grid
+// of one block spans +-127 of its largest entry, so an entry's rounding is
+// at most 1/254 of the block's largest (FP16: 2^-11 of the entry itself).
+__device__ __forceinline__ float sb8(unsigned w, int sh) { return (float)((int)(w << (24 - sh)) >> 24); }
+template <int MODE, typename CI>
+__global__ __launch_bounds__(TPB) void k_fine_ell_op_q(int nn, const int64_t *__restrict__ off,
+                                                       const int32_t *__restrict__ width, const CI *__restrict__ col,
+                                                       const uint2 *__restrict__ q8, const uint32_t *__restrict__ qs,
+                                                       const float *__restrict__ dg, const uint8_t *__restrict__ fixm,
+                                                       const float *__restrict__ u, RowOp<float> op) {
+  const int n = blockIdx.x * TPB + threadIdx.x;
+  double dv[3] = {0.0, 0.0, 0.0};
+  if (n < nn) {
+    EpiPre<float, MODE> pre[3];
+    float dgr[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      pre[c].load(op, u, 3 * (int64_t)n + c);
+      dgr[c] = dg[3 * (int64_t)n + c];
+    }
+    const int w = n / UFT_ELL_SLICE;
+    const int64_t base = off[w] + n % UFT_ELL_SLICE;
+    const int W = width[w];
+    float a0 = 0.f, a1 = 0.f, a2 = 0.f;
+#pragma unroll 3
+    for (int s = 0; s < W; ++s) {
+      const int64_t k = base + (int64_t)s * UFT_ELL_SLICE;
+      const int64_t j = std::is_same<CI, int16_t>::value ? n + (int64_t)col[k] : (int64_t)col[k];
+      typedef unsigned v2u __attribute__((ext_vector_type(2)));
+      const v2u v = __builtin_nontemporal_load(reinterpret_cast<const v2u *>(&q8[k]));
+      const unsigned xs = __builtin_nontemporal_load(&qs[k]);
+      const float sc = __uint_as_float(xs & 0xFFFFFF00u);
+      const float x0 = u[3 * j], x1 = u[3 * j + 1], x2 = u[3 * j + 2];
+      const float r0 = sb8(v.x, 0) * x0 + sb8(v.x, 8) * x1 + sb8(v.x, 16) * x2;
+      const float r1 = sb8(v.x, 24) * x0 + sb8(v.y, 0) * x1 + sb8(v.y, 8) * x2;
+      const float r2 = sb8(v.y, 16) * x0 + sb8(v.y, 24) * x1 + sb8(xs, 0) * x2;
+      a0 += sc * r0;
+      a1 += sc * r1;
+      a2 += sc * r2;
+    }
+    const float acc[3] = {a0, a1, a2};
+    const unsigned m = fixm[n];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      row_epi_m<float, MODE>(op, 3 * (int64_t)n + c, ((m >> c) & 1u) ? pre[c].ui : dgr[c] * acc[c], pre[c], dv);
+  }
+  epi_partials_m<float, MODE>(op, dv);
+}
+
 // The same pattern with FP32 entries (no row scale): 40 bytes per slot.
 template <int MODE, typename CI>
 __global__ __launch_bounds__(TPB) void k_fine_ell_op32(int nn, const int64_t *__restrict__ off,
@@ -325,7 +376,8 @@ __global__ __launch_bounds__(TPB) void k_fine_ell_op32(int nn, const int64_t *__
 // Assembly at a hierarchy refresh: lane (n, slot s) sums E(x_e) KE_e[a][b]
 // (3x3) over the slot's element contributions (incidence order, FP64),
 // zeroes fixed columns, divides row c by the diagonal 1 / dinv_3n+c and
-// stores FP16 (and, with w8, the unscaled entries in FP32).  The element blocks are read from their FP32 copy and the
+// stores FP16, or int8 with a block scale when q8 is given (and, with w8, the
+// unscaled entries in FP32).  The element blocks are read from their FP32 copy and the
 // moduli E(x_e) from the refresh's per-element array.
 template <int NPE>
 __global__ __launch_bounds__(TPB) void k_fine_ell_fill(int nn, int ne, const int64_t *__restrict__ off,
@@ -337,7 +389,8 @@ __global__ __launch_bounds__(TPB) void k_fine_ell_fill(int nn, int ne, const int
                                                        const double *__restrict__ Ee,
                                                        const uint8_t *__restrict__ fixm,
                                                        const double *__restrict__ dinv, uint4 *__restrict__ v8,
-                                                       uint16_t *__restrict__ v1, float *__restrict__ dg,
+                                                       uint16_t *__restrict__ v1, uint2 *__restrict__ q8,
+                                                       uint32_t *__restrict__ qs, float *__restrict__ dg,
                                                        float4 *__restrict__ w8, float *__restrict__ w1) {
   constexpr int ND = 3 * NPE;
   const int n = blockIdx.x * TPB + threadIdx.x;
@@ -374,18 +427,37 @@ __global__ __launch_bounds__(TPB) void k_fine_ell_fill(int nn, int ne, const int
     w8[2 * k + 1] = make_float4((float)b[4], (float)b[5], (float)b[6], (float)b[7]);
     w1[k] = (float)b[8];
   }
-  unsigned short hb[9];
+  double a[9];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     const double di = dinv[3 * (size_t)n + c];
     const double dd = (di > 0.0 && di < 1e300) ? 1.0 / di : 1.0;
     if (s == 0) dg[3 * (size_t)n + c] = (float)dd;
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      const double a = b[3 * c + d] / dd;
-      hb[3 * c + d] = __half_as_ushort(__float2half_rn((float)a));
-    }
+    for (int d = 0; d < 3; ++d) a[3 * c + d] = b[3 * c + d] / dd;
   }
+  if (q8) {
+    // int8 with one scale per block: sc = max|a| / 127 in FP32 with its low
+    // mantissa byte cleared (it carries entry 8), q = rint(a / sc)
+    double mx = 0.0;
+#pragma unroll
+    for (int t = 0; t < 9; ++t) mx = fmax(mx, fabs(a[t]));
+    const unsigned sbits = __float_as_uint((float)(mx / 127.0)) & 0xFFFFFF00u;
+    const double sc = (double)__uint_as_float(sbits);
+    unsigned qb[9];
+#pragma unroll
+    for (int t = 0; t < 9; ++t) {
+      const double q = sc > 0.0 ? fmin(fmax(rint(a[t] / sc), -127.0), 127.0) : 0.0;
+      qb[t] = (unsigned)(int)q & 0xFFu;
+    }
+    q8[k] = make_uint2(qb[0] | (qb[1] << 8) | (qb[2] << 16) | (qb[3] << 24),
+                       qb[4] | (qb[5] << 8) | (qb[6] << 16) | (qb[7] << 24));
+    qs[k] = (sc > 0.0 ? sbits : 0u) | qb[8];
+    return;
+  }
+  unsigned short hb[9];
+#pragma unroll
+  for (int t = 0; t < 9; ++t) hb[t] = __half_as_ushort(__float2half_rn((float)a[t]));
   v8[k] = make_uint4((unsigned)hb[0] | ((unsigned)hb[1] << 16), (unsigned)hb[2] | ((unsigned)hb[3] << 16),
                      (unsigned)hb[4] | ((unsigned)hb[5] << 16), (unsigned)hb[6] | ((unsigned)hb[7] << 16));
   v1[k] = hb[8];
@@ -827,10 +899,12 @@ void fine_ell_fill(hipStream_t s, const uft_ctx *C, const double *Ee, const doub
   const float2 *KEf = (const float2 *)C->d_KEf;
   if (C->npe == 8)
     k_fine_ell_fill<8><<<g, TPB, 0, s>>>(nn, ne, F.d_off, F.d_width, F.d_col, F.d_cptr, F.d_clist,
-                                         KEf, Ee, C->d_fixm, dinv, F.d_v8, F.d_v1, F.d_dg, F.d_w8, F.d_w1);
+                                         KEf, Ee, C->d_fixm, dinv, F.d_v8, F.d_v1, F.d_q8, F.d_qs, F.d_dg, F.d_w8,
+                                         F.d_w1);
   else
     k_fine_ell_fill<4><<<g, TPB, 0, s>>>(nn, ne, F.d_off, F.d_width, F.d_col, F.d_cptr, F.d_clist,
-                                         KEf, Ee, C->d_fixm, dinv, F.d_v8, F.d_v1, F.d_dg, F.d_w8, F.d_w1);
+                                         KEf, Ee, C->d_fixm, dinv, F.d_v8, F.d_v1, F.d_q8, F.d_qs, F.d_dg, F.d_w8,
+                                         F.d_w1);
 }
 void fine_ell_op32(hipStream_t s, const uft_ctx *C, const float *u, const RowOp<float> &op) {
   const auto &F = C->fell;
@@ -874,6 +948,15 @@ void helm_ell_cg(hipStream_t s, const uft_ctx *C, const double *z, const double 
 }
 void fine_ell_op(hipStream_t s, const uft_ctx *C, const float *u, const RowOp<float> &op) {
   const auto &F = C->fell;
+  if (F.d_q8) {
+    if (F.d_cd)
+      UFT_MODE_SWITCH(op.mode, (k_fine_ell_op_q<MODE, int16_t><<<nblk(C->nn), TPB, 0, s>>>(
+                                    (int)C->nn, F.d_off, F.d_width, F.d_cd, F.d_q8, F.d_qs, F.d_dg, C->d_fixm, u, op)))
+    else
+      UFT_MODE_SWITCH(op.mode, (k_fine_ell_op_q<MODE, int32_t><<<nblk(C->nn), TPB, 0, s>>>(
+                                    (int)C->nn, F.d_off, F.d_width, F.d_col, F.d_q8, F.d_qs, F.d_dg, C->d_fixm, u, op)))
+    return;
+  }
   if (F.d_cd)
     UFT_MODE_SWITCH(op.mode, (k_fine_ell_op<true, MODE, int16_t><<<nblk(C->nn), TPB, 0, s>>>(
                                   (int)C->nn, F.d_off, F.d_width, F.d_cd, F.d_v8, F.d_v1, F.d_dg, C->d_fixm, u, op)))

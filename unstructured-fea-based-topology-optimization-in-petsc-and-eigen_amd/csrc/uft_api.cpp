@@ -128,6 +128,9 @@ hipEvent_t *prof_triple(uft_ctx *C, int kind) {
 }
 
 bool is_mixed(const uft_ctx *C) { return C->P.precision != UFT_FP64; }
+// the assembled smoother blocks (int8 for one right-hand side, FP16 for the
+// multi-load-case form) exist
+bool fine_blocks(const uft_ctx *C) { return C->fell.d_q8 || C->fell.d_v8; }
 
 void graph_release(uft_ctx *C) {
   if (C->ig_exec) (void)hipGraphExecDestroy(C->ig_exec);
@@ -276,7 +279,7 @@ void lev_op(uft_ctx *C, int l, const float *u, const RowOp<float> &op) {
   if (l == 0) {
     hipEvent_t *ev = prof_triple(C, 1);
     if (ev) (void)hipEventRecord(ev[0], C->s);
-    if (C->fell.d_v8) {  // assembled FP16 blocks: one launch (timed as the element phase)
+    if (fine_blocks(C)) {  // assembled blocks: one launch (timed as the element phase)
       uftk::fine_ell_op(C->s, C, u, op);
       if (ev) (void)hipEventRecord(ev[1], C->s);
       if (ev) (void)hipEventRecord(ev[2], C->s);
@@ -944,7 +947,7 @@ int amg_refresh(uft_ctx *C, const double *xphys) {
       uftk::bsr6_to_half(s, Ln.nblk, Ln.nunit, Ln.d_brow, Ln.d_ci, Ln.d_val, Ln.d_dinv, Ln.d_valh, Ln.d_dh);
   }
   // the level 0 -> 1 Galerkin product has written E(x_e) to d_Ee
-  if (C->fell.d_v8) uftk::fine_ell_fill(s, C, C->d_Ee, H.L[0].d_dinv);
+  if (fine_blocks(C)) uftk::fine_ell_fill(s, C, C->d_Ee, H.L[0].d_dinv);
   AmgLevel &Lc = H.L[H.nlev - 1];
   if (C->bt.nseg) {
     BatchState &B = C->bt;
@@ -1834,7 +1837,12 @@ int uft_setup(uft_ctx *C) {
             (rc = upload(C, &F.d_cptr, cptr)) || (rc = upload(C, &F.d_clist, clist)) ||
             (rc = dalloc_n(C, &F.d_hv, F.nslot)))
           return rc;
-        if (elast && ((rc = dalloc_n(C, &F.d_v8, F.nslot)) || (rc = dalloc_n(C, &F.d_v1, F.nslot)) ||
+        // smoother blocks: int8 with a block scale for one right-hand side,
+        // FP16 for the multi-load-case form (its eight lanes per node share
+        // each block load, so its blocks are not the bound)
+        const bool mr = C->P.load_cases == 1 && C->nlc > 1;
+        if (elast && ((rc = mr ? dalloc_n(C, &F.d_v8, F.nslot) : dalloc_n(C, &F.d_q8, F.nslot)) ||
+                      (rc = mr ? dalloc_n(C, &F.d_v1, F.nslot) : dalloc_n(C, &F.d_qs, F.nslot)) ||
                       (rc = dalloc_n(C, &F.d_dg, ndof)) || (rc = dalloc_n(C, &F.d_w8, 2 * F.nslot)) ||
                       (rc = dalloc_n(C, &F.d_w1, F.nslot))))
           return rc;
@@ -2634,13 +2642,15 @@ extern "C" int uft_solver_info(const uft_ctx *C, int *nlev, int64_t *n_per_level
   return UFT_OK;
 }
 
-extern "C" int uft_fine_operator_info(const uft_ctx *C, int64_t *slots, int64_t *blocks, int *col_bytes) {
-  if (!C || !slots || !blocks || !col_bytes) return UFT_ERR_ARG;
+extern "C" int uft_fine_operator_info(const uft_ctx *C, int64_t *slots, int64_t *blocks, int *col_bytes,
+                                      int *value_bytes) {
+  if (!C || !slots || !blocks || !col_bytes || !value_bytes) return UFT_ERR_ARG;
   if (!C->is_setup) return UFT_ERR_STATE;
-  const bool on = C->fell.d_v8 != nullptr;
+  const bool on = fine_blocks(C);
   *slots = on ? C->fell.nslot : 0;
   *blocks = on ? C->fell.nnzb : 0;
   *col_bytes = on ? (C->fell.d_cd ? 2 : 4) : 0;
+  *value_bytes = on ? (C->fell.d_q8 ? 12 : 18) : 0;
   return UFT_OK;
 }
 
@@ -2766,7 +2776,7 @@ extern "C" int uft_apply_K_cases(uft_ctx *C, int kind, const double *u, double *
     for (int64_t i = 0; i < n; ++i) ui[(size_t)i * nr + k] = u[(size_t)k * n + i];
   C->cur_xphys = C->vec[UFT_XPHYS];
   uftk::simp_moduli(C->s, C, C->cur_xphys, C->d_Ee);  // the matrix-core kernel reads E(x_e) from d_Ee
-  if (C->fell.d_v8) {  // the assembled blocks of this density (as a hierarchy refresh makes them)
+  if (fine_blocks(C)) {  // the assembled blocks of this density (as a hierarchy refresh makes them)
     AmgLevel &L0 = C->amg.L[0];
     uftk::diag_K(C->s, C, C->cur_xphys, L0.d_dinv, L0.d_dinvf);
     uftk::fine_ell_fill(C->s, C, C->d_Ee, L0.d_dinv);
@@ -2801,7 +2811,7 @@ extern "C" int uft_level_apply(uft_ctx *C, int level, int kind, const double *x,
     return uft_set_error(C, UFT_ERR_ARG, "uft_level_apply: bad arguments");
   AmgLevel &Lv = C->amg.L[level];
   if (kind > 0 && !is_mixed(C)) return uft_set_error(C, UFT_ERR_ARG, "uft_level_apply: FP64 form");
-  if (kind == 2 && !(level == 0 ? (void *)C->d_KEh : (void *)Lv.d_valh) && !(level == 0 && C->fell.d_v8))
+  if (kind == 2 && !(level == 0 ? (void *)C->d_KEh : (void *)Lv.d_valh) && !(level == 0 && fine_blocks(C)))
     return uft_set_error(C, UFT_ERR_ARG, "uft_level_apply: no FP16 operator on level %d", level);
   if (C->refreshes == 0) return uft_set_error(C, UFT_ERR_STATE, "uft_level_apply: no hierarchy refresh yet");
   hipStream_t s = C->s;
@@ -2814,7 +2824,7 @@ extern "C" int uft_level_apply(uft_ctx *C, int level, int kind, const double *x,
     RowOp<float> op;
     op.mode = ROW_STORE;
     op.y = Lv.tf;
-    if (level == 0 && kind == 2 && C->fell.d_v8) {
+    if (level == 0 && kind == 2 && fine_blocks(C)) {
       uftk::fine_ell_op(s, C, Lv.xf, op);
     } else if (level == 0 && kind == 1 && C->fell.d_w8) {
       uftk::fine_ell_op32(s, C, Lv.xf, op);

@@ -243,8 +243,10 @@ struct uft_ctx {
     int16_t *d_cd = nullptr;     // per slot: column node - row node, when every one fits 16 bits
     int32_t *d_cptr = nullptr;   // per slot: first element contribution (nslot + 1)
     int32_t *d_clist = nullptr;  // element contributions (e * npe + a) * npe + b
-    uint4 *d_v8 = nullptr;       // per slot: block entries 0..7 / diag(row), FP16
+    uint4 *d_v8 = nullptr;       // per slot: block entries 0..7 / diag(row), FP16 (multi-load-case form)
     uint16_t *d_v1 = nullptr;    // per slot: entry 8
+    uint2 *d_q8 = nullptr;       // per slot: entries 0..7 / diag(row) as int8 times the block scale
+    uint32_t *d_qs = nullptr;    // per slot: FP32 block scale, entry 8 (int8) in its low byte
     float *d_dg = nullptr;       // per scalar row: the diagonal (the row scale)
     double *d_hv = nullptr;      // per slot: the filter's Helmholtz entry (FP64, assembled at setup)
     float4 *d_w8 = nullptr;      // per slot: FP32 block entries 0..7 (the outer operator)

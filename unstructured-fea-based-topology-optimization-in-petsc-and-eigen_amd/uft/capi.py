@@ -103,7 +103,7 @@ def lib():
         "uft_gen_box": (C.c_int, [C.c_int] * 4 + [C.c_double] * 4 + [C.c_uint64, dp, ip]),
         "uft_box_bcs": (C.c_int, [C.c_int] * 4 + [ip, pi, ip, dp, pi]),
         "uft_solver_info": (C.c_int, [vp, pi, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
-        "uft_fine_operator_info": (C.c_int, [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), pi]),
+        "uft_fine_operator_info": (C.c_int, [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), pi, pi]),
         "uft_amg_operator_get": (C.c_int, [vp, C.c_int, ip, ip, dp]),
         "uft_amg_transfer_get": (C.c_int, [vp, C.c_int, ip, dp]),
         "uft_amg_coarse_inverse_get": (C.c_int, [vp, dp]),
@@ -440,12 +440,13 @@ class Context:
         return [n[i] for i in range(nl.value)], [b[i] for i in range(nl.value)]
 
     def fine_operator_info(self):
-        """(sliced-ELL slots, nonzero node blocks, bytes per column index) of
-        the assembled fine operators; (0, 0, 0) with the element form."""
-        s, b, w = C.c_int64(), C.c_int64(), C.c_int()
-        self._chk(self.L.uft_fine_operator_info(self.h, C.byref(s), C.byref(b), C.byref(w)),
+        """(sliced-ELL slots, nonzero node blocks, bytes per column index,
+        bytes per smoother block) of the assembled fine operators; all 0 with
+        the element form."""
+        s, b, w, v = C.c_int64(), C.c_int64(), C.c_int(), C.c_int()
+        self._chk(self.L.uft_fine_operator_info(self.h, C.byref(s), C.byref(b), C.byref(w), C.byref(v)),
                   "uft_fine_operator_info")
-        return s.value, b.value, w.value
+        return s.value, b.value, w.value, v.value
 
     def amg_operator(self, level):
         """BSR (rp, ci, val[nblk, 6, 6]) of multigrid level >= 1."""
