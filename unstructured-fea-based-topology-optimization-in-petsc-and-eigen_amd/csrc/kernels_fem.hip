@@ -1362,6 +1362,11 @@ void z_from_f32(hipStream_t s, int64_t n, const float *zf, const double *r, doub
   k_z_from_f32<<<nblk(n), TPB, 0, s>>>(n, zf, r, z, zo, part);
   final_sum(s, part, (int)nblk(n), 2, out);
 }
+void z_from_f32_beta(hipStream_t s, int64_t n, const float *zf, const double *r, double *z, double *zo, double *part,
+                     double *sc, int irz, int iprev, int ibeta) {
+  k_z_from_f32<<<nblk(n), TPB, 0, s>>>(n, zf, r, z, zo, part);
+  final_beta_flex(s, part, (int)nblk(n), sc, irz, iprev, ibeta);
+}
 void cg_updatef(hipStream_t s, int64_t n, const double *sc, int ia, float *u, const float *p, float *r, const float *q,
                 double *part, double *out) {
   k_cg_update<float><<<nblk(n), TPB, 0, s>>>(n, sc, ia, u, p, r, q, part, out);

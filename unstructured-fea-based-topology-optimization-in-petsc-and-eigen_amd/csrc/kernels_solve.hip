@@ -274,6 +274,19 @@ __global__ void k_beta_flex(double *sc, int irz, int irzo, int iprev, int ibeta)
   sc[ibeta] = (sc[irz] - sc[irzo]) / sc[iprev];
   sc[iprev] = sc[irz];
 }
+// the final sums of r.z and r.zo into sc[irz], sc[irz + 1] followed by
+// k_beta_flex in the same launch
+__global__ __launch_bounds__(TPB) void k_final_beta_flex(const double *__restrict__ part, int nb, double *sc, int irz,
+                                                         int iprev, int ibeta) {
+  double res[2];
+  uftr::block_final<2>(part, nb, res);
+  if (threadIdx.x == 0) {
+    sc[irz] = res[0];
+    sc[irz + 1] = res[1];
+    sc[ibeta] = (res[0] - res[1]) / sc[iprev];
+    sc[iprev] = res[0];
+  }
+}
 
 // x_i uniform in [-1, 1) from a SplitMix64 hash of the index (a fixed,
 // non-smooth start vector for the eigenvalue estimates)
@@ -297,6 +310,14 @@ void dot_pr(hipStream_t s, int64_t n, const double *r, const double *z, const do
 }
 void sc_beta_flex(hipStream_t s, double *sc, int irz, int irzo, int iprev, int ibeta) {
   k_beta_flex<<<1, 1, 0, s>>>(sc, irz, irzo, iprev, ibeta);
+}
+void final_beta_flex(hipStream_t s, const double *part, int nb, double *sc, int irz, int iprev, int ibeta) {
+  k_final_beta_flex<<<1, TPB, 0, s>>>(part, nb, sc, irz, iprev, ibeta);
+}
+void dot_pr_beta(hipStream_t s, int64_t n, const double *r, const double *z, const double *zo, double *part,
+                 double *sc, int irz, int iprev, int ibeta) {
+  k_dot_pr<<<nblk(n), TPB, 0, s>>>(n, r, z, zo, part, sc + irz);
+  final_beta_flex(s, part, (int)nblk(n), sc, irz, iprev, ibeta);
 }
 }  // namespace uftk
 

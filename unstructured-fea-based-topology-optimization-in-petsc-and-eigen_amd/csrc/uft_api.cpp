@@ -1008,8 +1008,7 @@ int solve_el(uft_ctx *C, const double *xphys, double *U) {
   auto next = [&]() {
     uftk::copy(s, n, z, zo);  // the previous preconditioned residual
     vcycle(C, 0, r, z);
-    uftk::dot_pr(s, n, r, z, zo, C->d_part, C->d_scal + S_RZNEW);
-    uftk::sc_beta_flex(s, C->d_scal, S_RZNEW, S_RZNEW + 1, S_RZ, S_BETA + 2);
+    uftk::dot_pr_beta(s, n, r, z, zo, C->d_part, C->d_scal, S_RZNEW, S_RZ, S_BETA + 2);
     uftk::xpby(s, n, z, C->d_scal, S_BETA + 2, p);
     head();
   };
@@ -1092,8 +1091,7 @@ int solve_el_mixed(uft_ctx *C, const double *xphys, double *U) {
   };
   auto next = [&]() {  // z = B rf in FP32; zo = previous z; r.z, r.zo; beta; p
     vcycle_f(C, 0, rf, zf);
-    uftk::z_from_f32(s, n, zf, r, z, zo, C->d_part, C->d_scal + S_RZNEW);
-    uftk::sc_beta_flex(s, C->d_scal, S_RZNEW, S_RZNEW + 1, S_RZ, S_BETA + 2);
+    uftk::z_from_f32_beta(s, n, zf, r, z, zo, C->d_part, C->d_scal, S_RZNEW, S_RZ, S_BETA + 2);
     uftk::xpby_lo(s, n, z, C->d_scal, S_BETA + 2, p, C->d_pf);
     head();
   };

@@ -615,6 +615,14 @@ void f2d(hipStream_t s, int64_t n, const float *a, double *b);
 void fill_hash(hipStream_t s, int64_t n, double *x);
 void dot_pr(hipStream_t s, int64_t n, const double *r, const double *z, const double *zo, double *part, double *out);
 void sc_beta_flex(hipStream_t s, double *sc, int irz, int irzo, int iprev, int ibeta);
+// final sums of (r.z, r.zo) into sc[irz], sc[irz + 1], then beta = (r.z - r.zo) /
+// sc[iprev] into sc[ibeta] and sc[iprev] = r.z, in one launch; with the
+// partials' kernel: dot_pr_beta (FP64 z), z_from_f32_beta (z from the FP32 cycle)
+void final_beta_flex(hipStream_t s, const double *part, int nb, double *sc, int irz, int iprev, int ibeta);
+void dot_pr_beta(hipStream_t s, int64_t n, const double *r, const double *z, const double *zo, double *part,
+                 double *sc, int irz, int iprev, int ibeta);
+void z_from_f32_beta(hipStream_t s, int64_t n, const float *zf, const double *r, double *z, double *zo, double *part,
+                     double *sc, int irz, int iprev, int ibeta);
 void lincomb(hipStream_t s, int64_t n, const double *sc, int ia, const double *a, int ib, const double *b, double *y);
 void lincombf(hipStream_t s, int64_t n, const double *sc, int ia, const float *a, int ib, const float *b, float *y);
 void axpy_inplace_t(hipStream_t s, int64_t n, const double *d, double *x);
