@@ -801,8 +801,11 @@ double tridiag_lmax(const std::vector<double> &d, const std::vector<double> &e) 
 }
 
 // lambda_max(D^-1 A_l) by k steps of Jacobi-PCG (Lanczos from the CG
-// coefficients).  The scalars stay on the device until the end.
-int estimate_lmax(uft_ctx *C, int l, int ksteps) {
+// coefficients); the scalars stay on the device until the end.
+// The steps for level l, queued on
+// C->s: scalars in d_scal[sb .. sb + 4], block partials in part, (alpha,
+// beta, -) per step in log.
+void lanczos_enqueue(uft_ctx *C, int l, int ksteps, double *part, int sb, double *log) {
   AmgLevel &Lv = C->amg.L[l];
   hipStream_t s = C->s;
   const int64_t n = Lv.n;
@@ -810,41 +813,42 @@ int estimate_lmax(uft_ctx *C, int l, int ksteps) {
   uftk::fill_hash(s, n, r);  // fixed pseudo-random start: not close to a near-null (rigid) mode
   uftk::scale(s, n, r, Lv.d_dinv, z);
   uftk::copy(s, n, z, p);
-  uftk::dot(s, n, r, z, C->d_part, C->d_scal + S_MISC);  // rz in S_MISC
-  double *log = C->d_lanczos;
+  uftk::dot(s, n, r, z, part, C->d_scal + sb);  // rz in sb
   // the fine level of the mixed forms: the smoother's operator on FP32 vectors
   const bool lo = l == 0 && is_mixed(C);
   (void)x;
   if (lo) uftk::d2f(s, n, p, Lv.xf);
   for (int j = 0; j < ksteps; ++j) {
-    // q = A p and p.q (S_MISC + 1) in the operator's epilogue; one update
+    // q = A p and p.q (sb + 1) in the operator's epilogue; one update
     // launch (alpha, r, z = D^-1 r, r.z); alpha, beta logged; p = z + beta p
     if (lo) {
       RowOp<float> op;
       op.mode = ROW_STOREDOT;
       op.y = Lv.tf;
-      op.part = C->d_part;
+      op.part = part;
       lev_op(C, 0, Lv.xf, op);
-      uftk::final_sum(s, C->d_part, lev_op_blocks(C, 0, true), 1, C->d_scal + S_MISC + 1);
-      uftk::lanczos_step(s, n, C->d_scal, S_MISC, S_MISC + 1, S_MISC + 4, r, Lv.tf, Lv.d_dinv, z, C->d_part,
+      uftk::final_sum(s, part, lev_op_blocks(C, 0, true), 1, C->d_scal + sb + 1);
+      uftk::lanczos_step(s, n, C->d_scal, sb, sb + 1, sb + 4, r, Lv.tf, Lv.d_dinv, z, part,
                          log + 3 * j);
-      uftk::xpby_lo(s, n, z, C->d_scal, S_MISC + 4, p, Lv.xf);
+      uftk::xpby_lo(s, n, z, C->d_scal, sb + 4, p, Lv.xf);
     } else {
       RowOp<double> op;
       op.mode = ROW_STOREDOT;
       op.y = q;
-      op.part = C->d_part;
+      op.part = part;
       lev_op(C, l, p, op);
-      uftk::final_sum(s, C->d_part, lev_op_blocks(C, l, false), 1, C->d_scal + S_MISC + 1);
-      uftk::lanczos_step(s, n, C->d_scal, S_MISC, S_MISC + 1, S_MISC + 4, r, q, Lv.d_dinv, z, C->d_part,
+      uftk::final_sum(s, part, lev_op_blocks(C, l, false), 1, C->d_scal + sb + 1);
+      uftk::lanczos_step(s, n, C->d_scal, sb, sb + 1, sb + 4, r, q, Lv.d_dinv, z, part,
                          log + 3 * j);
-      uftk::xpby(s, n, z, C->d_scal, S_MISC + 4, p);
+      uftk::xpby(s, n, z, C->d_scal, sb + 4, p);
     }
   }
-  std::vector<double> hl(3 * ksteps);
-  UFT_HIP(C, hipMemcpyAsync(hl.data(), log, hl.size() * sizeof(double), hipMemcpyDeviceToHost, C->s));
-  int rc = sync(C);
-  if (rc) return rc;
+}
+
+// lambda_max of level l from the host copy hl of its Lanczos log: the
+// tridiagonal matrix's largest eigenvalue times cheb_hi
+void lanczos_finish(uft_ctx *C, int l, int ksteps, const double *hl) {
+  AmgLevel &Lv = C->amg.L[l];
   std::vector<double> d, e;
   for (int j = 0; j < ksteps; ++j) {
     const double al = hl[3 * j], be = hl[3 * j + 1];
@@ -857,10 +861,20 @@ int estimate_lmax(uft_ctx *C, int l, int ksteps) {
   }
   if (d.empty()) {
     Lv.lmax = 2.0 * C->P.cheb_hi;
-    return UFT_OK;
+    return;
   }
   e.resize(d.size());
   Lv.lmax = C->P.cheb_hi * tridiag_lmax(d, e);
+}
+
+// both, with one host round trip
+int estimate_lmax(uft_ctx *C, int l, int ksteps) {
+  lanczos_enqueue(C, l, ksteps, C->d_part, S_MISC, C->d_lanczos);
+  std::vector<double> hl(3 * ksteps);
+  UFT_HIP(C, hipMemcpyAsync(hl.data(), C->d_lanczos, hl.size() * sizeof(double), hipMemcpyDeviceToHost, C->s));
+  int rc = sync(C);
+  if (rc) return rc;
+  lanczos_finish(C, l, ksteps, hl.data());
   return UFT_OK;
 }
 
@@ -947,7 +961,8 @@ int amg_refresh(uft_ctx *C, const double *xphys) {
       uftk::bsr6_to_half(s, Ln.nblk, Ln.nunit, Ln.d_brow, Ln.d_ci, Ln.d_val, Ln.d_dinv, Ln.d_valh, Ln.d_dh);
   }
   // the level 0 -> 1 Galerkin product has written E(x_e) to d_Ee
-  if (fine_blocks(C)) uftk::fine_ell_fill(s, C, C->d_Ee, H.L[0].d_dinv);
+  const bool par = C->side[0] && !C->bt.nseg;  // the branches below run concurrently
+  if (fine_blocks(C) && !par) uftk::fine_ell_fill(s, C, C->d_Ee, H.L[0].d_dinv);
   AmgLevel &Lc = H.L[H.nlev - 1];
   if (C->bt.nseg) {
     BatchState &B = C->bt;
@@ -959,12 +974,40 @@ int amg_refresh(uft_ctx *C, const double *xphys) {
     }
     return UFT_OK;
   }
-  uftk::bsr_to_dense(s, Lc.nunit, Lc.d_rp, Lc.d_ci, Lc.d_val, H.ncoarse, H.d_cmat);
-  uftk::dense_spd_inverse(s, H.ncoarse, H.d_cmat, H.d_cinv, H.d_cinvf);
-  for (int l = 0; l + 1 < H.nlev; ++l) {
-    int rc = estimate_lmax(C, l, std::min(64, std::max(2, C->P.esteig_steps)));
-    if (rc) return rc;
+  const int ks = std::min(64, std::max(2, C->P.esteig_steps));
+  if (!par) {
+    uftk::bsr_to_dense(s, Lc.nunit, Lc.d_rp, Lc.d_ci, Lc.d_val, H.ncoarse, H.d_cmat);
+    uftk::dense_spd_inverse(s, H.ncoarse, H.d_cmat, H.d_cinv, H.d_cinvf);
+    for (int l = 0; l + 1 < H.nlev; ++l) {
+      int rc = estimate_lmax(C, l, ks);
+      if (rc) return rc;
+    }
+    return UFT_OK;
   }
+  // Three independent branches after the Galerkin products, each on its own
+  // stream: the fine blocks and the fine level's eigenvalue estimate; the
+  // coarse levels' estimates (own partials and scalar slots); the coarsest
+  // inverse.  Each branch does what the sequential order does, so the
+  // results are the same; one host round trip for all estimates.
+  UFT_HIP(C, hipEventRecord(C->ev_side[3], s));
+  for (int i = 0; i < 3; ++i) UFT_HIP(C, hipStreamWaitEvent(C->side[i], C->ev_side[3], 0));
+  C->s = C->side[0];
+  if (fine_blocks(C)) uftk::fine_ell_fill(C->s, C, C->d_Ee, H.L[0].d_dinv);
+  lanczos_enqueue(C, 0, ks, C->d_part, S_MISC, C->d_lanczos);
+  C->s = C->side[1];
+  for (int l = 1; l + 1 < H.nlev; ++l) lanczos_enqueue(C, l, ks, C->d_part2, S_MISC + 8, C->d_lanczos + 192 * l);
+  C->s = C->side[2];
+  uftk::bsr_to_dense(C->s, Lc.nunit, Lc.d_rp, Lc.d_ci, Lc.d_val, H.ncoarse, H.d_cmat);
+  uftk::dense_spd_inverse(C->s, H.ncoarse, H.d_cmat, H.d_cinv, H.d_cinvf);
+  C->s = s;
+  for (int i = 0; i < 3; ++i) {
+    UFT_HIP(C, hipEventRecord(C->ev_side[i], C->side[i]));
+    UFT_HIP(C, hipStreamWaitEvent(s, C->ev_side[i], 0));
+  }
+  std::vector<double> hl((size_t)192 * (H.nlev - 1));
+  UFT_HIP(C, hipMemcpyAsync(hl.data(), C->d_lanczos, hl.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  if (int rc = sync(C)) return rc;
+  for (int l = 0; l + 1 < H.nlev; ++l) lanczos_finish(C, l, ks, hl.data() + 192 * l);
   return UFT_OK;
 }
 
@@ -1700,6 +1743,10 @@ void uft_destroy(uft_ctx *C) {
   free_all(C);
   if (C->h_scal) (void)hipHostFree(C->h_scal);
   if (C->bt.h) (void)hipHostFree(C->bt.h);
+  for (hipEvent_t e : C->ev_side)
+    if (e) (void)hipEventDestroy(e);
+  for (hipStream_t t : C->side)
+    if (t) (void)hipStreamDestroy(t);
   if (C->s) (void)hipStreamDestroy(C->s);
   delete C;
 }
@@ -1902,7 +1949,16 @@ int uft_setup(uft_ctx *C) {
   C->part_cap = (maxn / 256 + 2) * 48 + 3 * (2 * ndof * 32 / 256 + 2);
   if ((rc = dalloc_n(C, &C->d_part, C->part_cap))) return rc;
   if ((rc = dalloc_n(C, &C->d_scal, 256))) return rc;
-  if ((rc = dalloc_n(C, &C->d_lanczos, 3 * 64))) return rc;
+  if ((rc = dalloc_n(C, &C->d_lanczos, 3 * 64 * UFT_MAXLEV))) return rc;
+  if (!C->bt.nseg) {  // the hierarchy refresh's concurrent branches (default-priority streams)
+    for (hipStream_t &t : C->side)
+      if (!t && hipStreamCreateWithFlags(&t, hipStreamNonBlocking) != hipSuccess)
+        return uft_set_error(C, UFT_ERR_HIP, "hipStreamCreateWithFlags failed");
+    for (hipEvent_t &e : C->ev_side)
+      if (!e && hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
+        return uft_set_error(C, UFT_ERR_HIP, "hipEventCreateWithFlags failed");
+    if ((rc = dalloc_n(C, &C->d_part2, C->part_cap))) return rc;
+  }
   uftk::fill(s, 256, 0.0, C->d_scal);
   uftk::gen_ke(s, npe, ne, C->d_elem, C->d_xyz, C->P.nu, C->d_KE, C->d_KEf, C->d_detmin);
   if (C->P.precision == UFT_MIXED_H) {

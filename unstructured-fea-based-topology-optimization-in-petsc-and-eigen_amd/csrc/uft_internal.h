@@ -298,6 +298,12 @@ struct uft_ctx {
   int prof_every = 1;   // time one in every prof_every fine applies of each kind
   int64_t prof_tick[3] = {0, 0, 0};
   std::vector<hipEvent_t> evpool;
+  // the hierarchy refresh's concurrent branches (fine blocks + fine-level
+  // eigenvalue estimate, coarse-level estimates, coarsest inverse) and their
+  // fork / join events; the coarse estimates' block partials
+  hipStream_t side[3] = {nullptr, nullptr, nullptr};
+  hipEvent_t ev_side[4] = {nullptr, nullptr, nullptr, nullptr};
+  double *d_part2 = nullptr;
   std::vector<int> evkind;  // per triple: 0 FP64 apply, 1 FP32 apply
   size_t evused = 0;
   double prof_ms[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
