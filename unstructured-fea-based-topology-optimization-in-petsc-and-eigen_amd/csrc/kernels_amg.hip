@@ -550,8 +550,10 @@ void simp_moduli(hipStream_t s, const uft_ctx *C, const double *xphys, double *E
 void galerkin_fine(hipStream_t s, const uft_ctx *C, const double *xphys, int64_t nb, const int32_t *glptr,
                    const int32_t *gl, const int32_t *gtr, const int32_t *brow, const int32_t *ci, const double *G,
                    double *Ee, double *val) {
-  SimpCoef sc{C->P.Emin, C->P.Emax - C->P.Emin, C->P.penal};
-  k_simp_E<<<nblk(C->ne), TPB, 0, s>>>(C->ne, xphys, sc, Ee);
+  if (xphys) {  // else Ee already holds E(x_e)
+    SimpCoef sc{C->P.Emin, C->P.Emax - C->P.Emin, C->P.penal};
+    k_simp_E<<<nblk(C->ne), TPB, 0, s>>>(C->ne, xphys, sc, Ee);
+  }
   k_galerkin_fine<<<nblk(nb * 36), TPB, 0, s>>>(nb, glptr, gl, gtr, brow, ci, Ee, G, val);
 }
 void galerkin_coarse(hipStream_t s, int64_t nb, const int32_t *glptr, const int32_t *gl, const int32_t *brow,

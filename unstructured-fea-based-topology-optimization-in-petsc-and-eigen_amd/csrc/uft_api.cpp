@@ -947,11 +947,27 @@ int amg_refresh(uft_ctx *C, const double *xphys) {
   C->cur_xphys = xphys;
   C->refreshes++;
   uftk::diag_K(s, C, xphys, H.L[0].d_dinv, H.L[0].d_dinvf);
+  // Concurrent branches (each on its own stream; each does what the
+  // sequential order does, so the results are the same): the fine blocks and
+  // the fine level's eigenvalue estimate once E(x_e) and the fine diagonal
+  // exist; after the Galerkin products the coarse levels' estimates (own
+  // partials and scalar slots) and the coarsest inverse.
+  const bool par = C->side[0] && !C->bt.nseg;
+  const int ks = std::min(64, std::max(2, C->P.esteig_steps));
+  if (par) {
+    uftk::simp_moduli(s, C, xphys, C->d_Ee);
+    UFT_HIP(C, hipEventRecord(C->ev_side[3], s));
+    UFT_HIP(C, hipStreamWaitEvent(C->side[0], C->ev_side[3], 0));
+    C->s = C->side[0];
+    if (fine_blocks(C)) uftk::fine_ell_fill(C->s, C, C->d_Ee, H.L[0].d_dinv);
+    if (H.nlev > 1) lanczos_enqueue(C, 0, ks, C->d_part, S_MISC, C->d_lanczos);
+    C->s = s;
+  }
   for (int l = 0; l + 1 < H.nlev; ++l) {
     AmgLevel &Lv = H.L[l], &Ln = H.L[l + 1];
     if (l == 0)
-      uftk::galerkin_fine(s, C, xphys, Ln.nblk, Lv.d_glptr, Lv.d_gl, Lv.d_gtr, Ln.d_brow, Ln.d_ci, Lv.d_G, C->d_Ee,
-                          Ln.d_val);
+      uftk::galerkin_fine(s, C, par ? nullptr : xphys, Ln.nblk, Lv.d_glptr, Lv.d_gl, Lv.d_gtr, Ln.d_brow, Ln.d_ci,
+                          Lv.d_G, C->d_Ee, Ln.d_val);
     else
       uftk::galerkin_coarse(s, Ln.nblk, Lv.d_glptr, Lv.d_gl, Lv.d_brow, Lv.d_ci, Lv.d_P, Lv.d_val, Ln.d_val);
     uftk::fixed_unit_diag(s, Ln.nunit, Ln.d_rp, Ln.d_ci, Ln.d_val);
@@ -961,7 +977,6 @@ int amg_refresh(uft_ctx *C, const double *xphys) {
       uftk::bsr6_to_half(s, Ln.nblk, Ln.nunit, Ln.d_brow, Ln.d_ci, Ln.d_val, Ln.d_dinv, Ln.d_valh, Ln.d_dh);
   }
   // the level 0 -> 1 Galerkin product has written E(x_e) to d_Ee
-  const bool par = C->side[0] && !C->bt.nseg;  // the branches below run concurrently
   if (fine_blocks(C) && !par) uftk::fine_ell_fill(s, C, C->d_Ee, H.L[0].d_dinv);
   AmgLevel &Lc = H.L[H.nlev - 1];
   if (C->bt.nseg) {
@@ -974,7 +989,6 @@ int amg_refresh(uft_ctx *C, const double *xphys) {
     }
     return UFT_OK;
   }
-  const int ks = std::min(64, std::max(2, C->P.esteig_steps));
   if (!par) {
     uftk::bsr_to_dense(s, Lc.nunit, Lc.d_rp, Lc.d_ci, Lc.d_val, H.ncoarse, H.d_cmat);
     uftk::dense_spd_inverse(s, H.ncoarse, H.d_cmat, H.d_cinv, H.d_cinvf);
@@ -984,16 +998,9 @@ int amg_refresh(uft_ctx *C, const double *xphys) {
     }
     return UFT_OK;
   }
-  // Three independent branches after the Galerkin products, each on its own
-  // stream: the fine blocks and the fine level's eigenvalue estimate; the
-  // coarse levels' estimates (own partials and scalar slots); the coarsest
-  // inverse.  Each branch does what the sequential order does, so the
-  // results are the same; one host round trip for all estimates.
-  UFT_HIP(C, hipEventRecord(C->ev_side[3], s));
-  for (int i = 0; i < 3; ++i) UFT_HIP(C, hipStreamWaitEvent(C->side[i], C->ev_side[3], 0));
-  C->s = C->side[0];
-  if (fine_blocks(C)) uftk::fine_ell_fill(C->s, C, C->d_Ee, H.L[0].d_dinv);
-  lanczos_enqueue(C, 0, ks, C->d_part, S_MISC, C->d_lanczos);
+  // the coarse branches; one host round trip for all estimates
+  UFT_HIP(C, hipEventRecord(C->ev_side[4], s));
+  for (int i = 1; i < 3; ++i) UFT_HIP(C, hipStreamWaitEvent(C->side[i], C->ev_side[4], 0));
   C->s = C->side[1];
   for (int l = 1; l + 1 < H.nlev; ++l) lanczos_enqueue(C, l, ks, C->d_part2, S_MISC + 8, C->d_lanczos + 192 * l);
   C->s = C->side[2];

@@ -302,7 +302,7 @@ struct uft_ctx {
   // eigenvalue estimate, coarse-level estimates, coarsest inverse) and their
   // fork / join events; the coarse estimates' block partials
   hipStream_t side[3] = {nullptr, nullptr, nullptr};
-  hipEvent_t ev_side[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t ev_side[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   double *d_part2 = nullptr;
   std::vector<int> evkind;  // per triple: 0 FP64 apply, 1 FP32 apply
   size_t evused = 0;
@@ -425,6 +425,7 @@ void prolong_addf(hipStream_t s, int bs, int64_t nunits, const int32_t *parent, 
 // Ee[e] = E(x_e) (SIMP modulus)
 void simp_moduli(hipStream_t s, const uft_ctx *C, const double *xphys, double *Ee);
 // level 0 -> 1: G (ngl x 36) once per mesh; then val = sum_k E(x_e(k)) G_k per block
+// (galerkin_fine writes E(x_e) to Ee first, or with xphys null reads it there)
 void galerkin_pre(hipStream_t s, const uft_ctx *C, int64_t ngl, const int32_t *gblk, const int32_t *gl,
                   const int32_t *brow, const int32_t *ci, const int32_t *parent, const double *Q, double *G);
 void galerkin_fine(hipStream_t s, const uft_ctx *C, const double *xphys, int64_t nblk, const int32_t *glptr,
