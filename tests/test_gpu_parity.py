@@ -87,6 +87,10 @@ def test_filter(O, npe, assembled):
     ctx.filter("X", "XFiled")
     xf, _ = ref.filter(x)
     assert _rel(ctx.get("XFiled"), xf) < 1e-10
+    # the same Jacobi-CG: the same number of iterations (the GPU tests its
+    # convergence on the device and queues iterations in rounds)
+    print("filter CG iterations", ctx.stats()["it_f"], ref.diag()["it_f"])
+    assert ctx.stats()["it_f"] == ref.diag()["it_f"] > 0
     df = rng.standard_normal(ref.ne)
     ctx.set("dfdx", df)
     ctx.set("dgdx0", np.abs(df))
@@ -94,6 +98,8 @@ def test_filter(O, npe, assembled):
     rdf, rdg, _ = ref.filter_sens(df, np.abs(df)[None, :])
     assert _rel(ctx.get("dfdx"), rdf) < 1e-10
     assert _rel(ctx.get("dgdx0"), rdg[0]) < 1e-10
+    print("filter CG iterations, last sensitivity", ctx.stats()["it_f"], ref.diag()["it_f"])
+    assert ctx.stats()["it_f"] == ref.diag()["it_f"] > 0
 
 
 def _loop(ctx, ref, iters):
