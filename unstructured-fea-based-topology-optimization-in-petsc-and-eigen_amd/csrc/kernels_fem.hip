@@ -839,10 +839,16 @@ __global__ __launch_bounds__(TPB) void k_cg_update(int64_t n, const double *__re
 // FP64 iterate and residual updated with a lower-precision operator product:
 // u += a p ; r -= a q (a = sc[ia] / sc[ib]) ; rf = FP32 copy of r for the
 // preconditioner ; partials of r.r
+// with cr given, also the next fine cycle's first Chebyshev step from a zero
+// guess on b = rf (the arithmetic of k_cheb_first): cr = rf, cd = cx =
+// it dinvf rf
 __global__ __launch_bounds__(TPB) void k_cg_update_lo(int64_t n, const double *__restrict__ sc, int ia, int ib,
                                                       double *__restrict__ u, const double *__restrict__ p,
                                                       double *__restrict__ r, const float *__restrict__ q,
-                                                      float *__restrict__ rf, double *__restrict__ part) {
+                                                      float *__restrict__ rf, double *__restrict__ part,
+                                                      const float *__restrict__ dinvf, double it,
+                                                      float *__restrict__ cr, float *__restrict__ cd,
+                                                      float *__restrict__ cx) {
   const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
   const double al = sc[ia] / sc[ib];
   double v[1] = {0.0};
@@ -850,8 +856,15 @@ __global__ __launch_bounds__(TPB) void k_cg_update_lo(int64_t n, const double *_
     u[i] += al * p[i];
     const double rr = r[i] - al * (double)q[i];
     r[i] = rr;
-    rf[i] = (float)rr;
+    const float rfi = (float)rr;
+    rf[i] = rfi;
     v[0] = rr * rr;
+    if (cr) {
+      cr[i] = rfi;
+      const float di = (float)it * dinvf[i] * rfi;
+      cd[i] = di;
+      cx[i] = di;
+    }
   }
   uftr::block_partials<1>(v, part);
 }
@@ -1354,7 +1367,14 @@ void cg_update(hipStream_t s, int64_t n, const double *sc, int ia, double *u, co
 }
 void cg_update_lo(hipStream_t s, int64_t n, const double *sc, int ia, int ib, double *u, const double *p, double *r,
                   const float *q, float *rf, double *part, double *out) {
-  k_cg_update_lo<<<nblk(n), TPB, 0, s>>>(n, sc, ia, ib, u, p, r, q, rf, part);
+  k_cg_update_lo<<<nblk(n), TPB, 0, s>>>(n, sc, ia, ib, u, p, r, q, rf, part, nullptr, 0.0, nullptr, nullptr,
+                                           nullptr);
+  final_sum(s, part, (int)nblk(n), 1, out);
+}
+void cg_update_lo_cheb(hipStream_t s, int64_t n, const double *sc, int ia, int ib, double *u, const double *p,
+                       double *r, const float *q, float *rf, double *part, double *out, const float *dinvf, double it,
+                       float *cr, float *cd, float *cx) {
+  k_cg_update_lo<<<nblk(n), TPB, 0, s>>>(n, sc, ia, ib, u, p, r, q, rf, part, dinvf, it, cr, cd, cx);
   final_sum(s, part, (int)nblk(n), 1, out);
 }
 void z_from_f32(hipStream_t s, int64_t n, const float *zf, const double *r, double *z, double *zo, double *part,

@@ -532,13 +532,18 @@ void cycle_body(uft_ctx *C, int l, const T *b, T *x) {
   }
 }
 
-// x = B_0 b: one fine-level cycle (the preconditioner of the elasticity solve)
+// the fine cycle's first Chebyshev coefficient
+double fine_first_coef(uft_ctx *C) {
+  return cheb_coef(C->amg.L[0].lmax, level_degree(C, 0), C->P.cheb_lo / C->P.cheb_hi).inv_theta;
+}
+// x = B_0 b: one fine-level cycle (the preconditioner of the elasticity solve);
+// first_done: the first Chebyshev step (r = b, d = x = it D^-1 b) is already
+// in place
 template <typename T>
-void fine_cycle(uft_ctx *C, const T *b, T *x) {
+void fine_cycle(uft_ctx *C, const T *b, T *x, bool first_done = false) {
   AmgLevel &L0 = C->amg.L[0];
   LV<T> v = lv(L0, (T *)nullptr);
-  const double it = cheb_coef(L0.lmax, level_degree(C, 0), C->P.cheb_lo / C->P.cheb_hi).inv_theta;
-  k_cheb_first(C->s, L0.n, b, v.dinv, it, v.r, v.d, x, 1);
+  if (!first_done) k_cheb_first(C->s, L0.n, b, v.dinv, fine_first_coef(C), v.r, v.d, x, 1);
   cycle_body<T>(C, 0, b, x);
 }
 void vcycle(uft_ctx *C, int, const double *b, double *x) { fine_cycle<double>(C, b, x); }
@@ -1135,12 +1140,25 @@ int solve_el_mixed(uft_ctx *C, const double *xphys, double *U) {
   // latter has fallen 100-fold below the last true residual, and before the
   // solve is accepted (reliable updates: the recursion's drift stays 1e-5 of
   // the current residual, far below the tolerance).
+  // Without graph capture the update also forms the next cycle's first
+  // Chebyshev step on rf (first_ready), unless a reliable update has
+  // replaced r and rf since.
+  const bool fuse_first = !use_graph(C);
+  bool first_ready = false;
+  AmgLevel &L0 = C->amg.L[0];
   auto head = [&]() {  // q = K p (FP32 blocks, d_pf = p); alpha = rz / pq; U, r (and its FP32 copy rf) updated
     fine_apply_lo(C, xphys, p, S_PQ, true);
-    uftk::cg_update_lo(s, n, C->d_scal, S_RZ, S_PQ, U, p, r, C->d_qf, rf, C->d_part, C->d_scal + S_RR);
+    if (fuse_first) {
+      uftk::cg_update_lo_cheb(s, n, C->d_scal, S_RZ, S_PQ, U, p, r, C->d_qf, rf, C->d_part, C->d_scal + S_RR,
+                              L0.d_dinvf, fine_first_coef(C), L0.rf, L0.df, zf);
+      first_ready = true;
+    } else {
+      uftk::cg_update_lo(s, n, C->d_scal, S_RZ, S_PQ, U, p, r, C->d_qf, rf, C->d_part, C->d_scal + S_RR);
+    }
   };
   auto next = [&]() {  // z = B rf in FP32; zo = previous z; r.z, r.zo; beta; p
-    vcycle_f(C, 0, rf, zf);
+    fine_cycle<float>(C, rf, zf, first_ready);
+    first_ready = false;
     uftk::z_from_f32_beta(s, n, zf, r, z, zo, C->d_part, C->d_scal, S_RZNEW, S_RZ, S_BETA + 2);
     uftk::xpby_lo(s, n, z, C->d_scal, S_BETA + 2, p, C->d_pf);
     head();
@@ -1161,6 +1179,7 @@ int solve_el_mixed(uft_ctx *C, const double *xphys, double *U) {
     const double rn = std::sqrt(C->h_scal[0]);
     if (!std::isfinite(rn)) return uft_set_error(C, UFT_ERR_NOCONV, "mixed CG diverged at iteration %d", it);
     if (rn <= tol || rn < 1e-2 * rtrue) {  // reliable update
+      first_ready = false;
       fine_apply(C, xphys, U, q, nullptr, nullptr);
       uftk::sub(s, n, C->d_F, q, r);
       uftk::d2f(s, n, r, rf);

@@ -389,6 +389,11 @@ void set_dofs(hipStream_t s, int n, const int32_t *idx, const double *val, doubl
 // u += a p ; r -= a q ; rf = (float) r with a = sc[ia] / sc[ib]; r.r into *out
 void cg_update_lo(hipStream_t s, int64_t n, const double *sc, int ia, int ib, double *u, const double *p, double *r,
                   const float *q, float *rf, double *part, double *out);
+// the same, also forming the next fine cycle's first Chebyshev step on rf
+// (zero guess): cr = rf, cd = cx = it dinvf rf
+void cg_update_lo_cheb(hipStream_t s, int64_t n, const double *sc, int ia, int ib, double *u, const double *p,
+                       double *r, const float *q, float *rf, double *part, double *out, const float *dinvf, double it,
+                       float *cr, float *cd, float *cx);
 // zo = z ; z = (double) zf ; r.z, r.zo into out[0..1]
 void z_from_f32(hipStream_t s, int64_t n, const float *zf, const double *r, double *z, double *zo, double *part,
                 double *out);
