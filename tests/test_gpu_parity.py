@@ -102,6 +102,23 @@ def test_filter(O, npe, assembled):
     assert ctx.stats()["it_f"] == ref.diag()["it_f"] > 0
 
 
+def test_filter_iteration_limit(O):
+    """At its iteration limit the filter CG reports UFT_ERR_NOCONV after
+    exactly maxit_f iterations (the device-side tests stop the queued ones),
+    and its iterate is the oracle's after as many."""
+    from uft.capi import UFTError
+    ctx, ref = _pair(O, 8, 8, 4, 4, rmin=0.375, rtol_el=1e-12, rtol_f=1e-14, maxit_f=3)
+    x = np.random.default_rng(9).uniform(0.0, 1.0, ref.ne)
+    ctx.set("X", x)
+    with pytest.raises(UFTError) as e:
+        ctx.filter("X", "XFiled")
+    assert e.value.code == 3  # UFT_ERR_NOCONV
+    assert ctx.stats()["it_f"] == 3
+    xf, _ = ref.filter(x)
+    assert ref.diag()["it_f"] == 3
+    assert _rel(ctx.get("XFiled"), xf) < 1e-10
+
+
 def _loop(ctx, ref, iters):
     ctx.init_design()
     ref.init_design()
