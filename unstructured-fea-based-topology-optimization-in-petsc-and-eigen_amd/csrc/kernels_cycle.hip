@@ -465,7 +465,8 @@ __global__ __launch_bounds__(TPB) void k_fine_ell_fill(int nn, int ne, const int
 
 // The filter's Helmholtz operator on the same node pattern (FP64 scalar
 // entries, assembled once at setup: KF does not depend on the density):
-// y_n = sum_s H_(n,s) x_col(n,s); with part, the block partials of x.y.
+// y_n = sum_s H_(n,s) x_col(n,s); with part, the block partials of x.y; nothing
+// once *stop is set (the filter CG's device-side convergence flag)
 template <int NPE>
 __global__ __launch_bounds__(TPB) void k_helm_ell_fill(int nn, int ne, const int64_t *__restrict__ off,
                                                        const int32_t *__restrict__ width,

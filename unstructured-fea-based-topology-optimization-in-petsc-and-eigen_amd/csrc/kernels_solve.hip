@@ -142,7 +142,7 @@ __global__ __launch_bounds__(TPB) void k_final_fcg_ctl(const double *__restrict_
   }
 }
 
-// p = z + (sc[ib] / sc[ibo]) p, or p = z when first
+// p = z + (sc[ib] / sc[ibo]) p, or p = z when first; nothing once *stop is set
 __global__ __launch_bounds__(TPB) void k_fcg_p(int64_t n, const double *__restrict__ z, const double *__restrict__ sc,
                                                int ib, int ibo, int first, double *__restrict__ p,
                                                const double *__restrict__ stop) {
