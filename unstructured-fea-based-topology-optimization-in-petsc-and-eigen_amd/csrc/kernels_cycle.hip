@@ -517,21 +517,25 @@ __global__ __launch_bounds__(TPB) void k_helm_ell(int nn, const int64_t *__restr
 }
 
 // ---------------------------------------------------------------- transfers
-// one wavefront per coarse unit A: bc_A = sum_children P_u^T r_u (fixed
-// order); with dinv given, also the next level's first Chebyshev step from a
-// zero guess: rc = bc, dc = it D^-1 bc, xc = dc.
-template <int BS, typename T>
+// G lanes per coarse unit A (64 / G units per wavefront): bc_A = sum_children
+// P_u^T r_u (each lane a fixed stride of the unit's (child, row) pairs, then
+// a fixed butterfly within the G lanes); with dinv given, also the next
+// level's first Chebyshev step from a zero guess: rc = bc, dc = it D^-1 bc,
+// xc = dc.
+template <int BS, int G, typename T>
 __global__ __launch_bounds__(TPB) void k_restrict_first(int64_t nc, const int32_t *__restrict__ cptr,
                                                         const int32_t *__restrict__ child, const T *__restrict__ P,
                                                         const T *__restrict__ r, T *__restrict__ bc,
                                                         const T *__restrict__ dinv, double it, T *__restrict__ rc,
                                                         T *__restrict__ dc, T *__restrict__ xc) {
-  const int64_t A = ((int64_t)blockIdx.x * TPB + threadIdx.x) >> 6;
-  const int lane = threadIdx.x & 63;
-  if (A >= nc) return;  // whole wavefront
-  const int k0 = cptr[A], npair = (cptr[A + 1] - k0) * BS;
+  static_assert(G >= 1 && G <= 64 && (64 % G) == 0, "G lanes must tile a wavefront");
+  const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  const int64_t A = t / G;
+  const int lane = (int)(t % G);
+  const bool on = A < nc;
+  const int k0 = on ? cptr[A] : 0, npair = on ? (cptr[A + 1] - k0) * BS : 0;
   T acc[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
-  for (int q = lane; q < npair; q += 64) {
+  for (int q = lane; q < npair; q += G) {
     const int kk = q / BS, i = q - kk * BS;
     const int64_t u = child[k0 + kk];
     const T *Pu = P + (size_t)u * BS * 6 + i * 6;
@@ -542,9 +546,9 @@ __global__ __launch_bounds__(TPB) void k_restrict_first(int64_t nc, const int32_
 #pragma unroll
   for (int j = 0; j < 6; ++j) {
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) acc[j] += __shfl_down(acc[j], off, 64);
+    for (int off = G / 2; off > 0; off >>= 1) acc[j] += __shfl_xor(acc[j], off, 64);
   }
-  if (lane == 0) {
+  if (on && lane == 0) {
 #pragma unroll
     for (int j = 0; j < 6; ++j) {
       const int64_t i = A * 6 + j;
@@ -970,9 +974,9 @@ template <typename T>
 static void restrict_first_t(hipStream_t s, int bs, int64_t nc, const int32_t *cptr, const int32_t *child,
                              const T *P, const T *r, T *bc, const T *dinv, double it, T *rc, T *dc, T *xc) {
   if (bs == 3)
-    k_restrict_first<3, T><<<nblk(nc * 64), TPB, 0, s>>>(nc, cptr, child, P, r, bc, dinv, it, rc, dc, xc);
+    k_restrict_first<3, 16, T><<<nblk(nc * 16), TPB, 0, s>>>(nc, cptr, child, P, r, bc, dinv, it, rc, dc, xc);
   else
-    k_restrict_first<6, T><<<nblk(nc * 64), TPB, 0, s>>>(nc, cptr, child, P, r, bc, dinv, it, rc, dc, xc);
+    k_restrict_first<6, 64, T><<<nblk(nc * 64), TPB, 0, s>>>(nc, cptr, child, P, r, bc, dinv, it, rc, dc, xc);
 }
 void restrict_first(hipStream_t s, int bs, int64_t nc, const int32_t *cptr, const int32_t *child, const double *P,
                     const double *r, double *bc, const double *dinv, double it, double *rc, double *dc, double *xc) {
