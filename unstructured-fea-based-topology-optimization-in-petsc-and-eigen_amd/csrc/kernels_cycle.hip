@@ -588,6 +588,34 @@ __global__ __launch_bounds__(TPB) void k_prolong_comb(int64_t nu, const int32_t 
   x[t] += acc;
 }
 
+// The same for 3 rows per unit (the fine level) with one thread per unit:
+// the parent's combination is formed once for the unit's three rows, whose
+// 3x6 block is one contiguous 18-entry run; each row's sum in the order of
+// k_prolong_comb.
+template <typename T>
+__global__ __launch_bounds__(TPB) void k_prolong_comb_node(int64_t nu, const int32_t *__restrict__ parent,
+                                                           const T *__restrict__ P, const double *__restrict__ sc,
+                                                           int ia, int ib, const T *__restrict__ xa,
+                                                           const T *__restrict__ xb, T *__restrict__ x) {
+  const int64_t u = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  if (u >= nu) return;
+  const int64_t A = parent[u];
+  if (A < 0) return;
+  const T ca = ia < 0 ? T(1) : (T)sc[ia];
+  const T cb = ib < 0 ? T(1) : (T)sc[ib];
+  T c[6];
+#pragma unroll
+  for (int j = 0; j < 6; ++j) c[j] = xb ? ca * xa[A * 6 + j] + cb * xb[A * 6 + j] : xa[A * 6 + j];
+  const T *Pu = P + (size_t)u * 18;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    T acc = T(0);
+#pragma unroll
+    for (int j = 0; j < 6; ++j) acc += Pu[i * 6 + j] * c[j];
+    x[u * 3 + i] += acc;
+  }
+}
+
 // K-cycle second start: kr = b + sc[7] v ; r = kr ; d = it D^-1 kr ; kd = d
 template <typename T>
 __global__ __launch_bounds__(TPB) void k_lincomb_first(int64_t n, const double *__restrict__ sc,
@@ -991,7 +1019,7 @@ template <typename T>
 static void prolong_comb_t(hipStream_t s, int bs, int64_t nu, const int32_t *parent, const T *P, const double *sc,
                            int ia, int ib, const T *xa, const T *xb, T *x) {
   if (bs == 3)
-    k_prolong_comb<3, T><<<nblk(nu * 3), TPB, 0, s>>>(nu, parent, P, sc, ia, ib, xa, xb, x);
+    k_prolong_comb_node<T><<<nblk(nu), TPB, 0, s>>>(nu, parent, P, sc, ia, ib, xa, xb, x);
   else
     k_prolong_comb<6, T><<<nblk(nu * 6), TPB, 0, s>>>(nu, parent, P, sc, ia, ib, xa, xb, x);
 }
