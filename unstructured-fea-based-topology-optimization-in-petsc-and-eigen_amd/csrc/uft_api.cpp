@@ -1472,7 +1472,12 @@ int solve_el_batch(uft_ctx *C, const double *xphys, double *U) {
 }
 
 // Design batches: the filter's Jacobi-CG (solve_filter) per design
-int solve_filter_batch(uft_ctx *C, const double *b, double *x) {
+// A finished design's scalars freeze its iterate (alpha 0, its iteration
+// count kept), so the host queues the iterations in rounds as in
+// solve_filter (the last call's largest count, then two at a time) and reads
+// the designs' states once per round: the same iterates and counts as a read
+// after every iteration.
+int solve_filter_batch(uft_ctx *C, const double *b, double *x, int slot) {
   hipStream_t s = C->s;
   BatchState &B = C->bt;
   const int ns = B.nseg;
@@ -1487,15 +1492,20 @@ int solve_filter_batch(uft_ctx *C, const double *b, double *x) {
   int rc = fetch_batch(C, B_FACT, &any);
   if (rc) return rc;
   int it = 0;
+  int *pred = &C->f_pred[std::min(std::max(slot, 0), UFT_MAXM + 1)];
+  int round = *pred > 0 ? *pred : 8;
   while (any && it < C->P.maxit_f) {
-    uftk::seg_fcg_p(s, g, n, z, B.sc, B_FPC, it == 0, p);
-    uftk::apply_KF(s, C, p, q);
-    uftk::seg_dot(s, g, df, ns, n, p, q, C->d_part, B.sc, B_FPQ);
-    uftk::bsc_f_alpha(s, ns, B.sc);
-    uftk::seg_fcg_update(s, g, df, ns, n, B.sc, B_FAL, x, p, r, q, C->d_fdinv, z, C->d_part, B_FZZN);
-    uftk::bsc_f_after(s, ns, B.sc);
-    ++it;
+    const int upto = std::min(C->P.maxit_f, it + round);
+    for (; it < upto; ++it) {
+      uftk::seg_fcg_p(s, g, n, z, B.sc, B_FPC, it == 0, p);
+      uftk::apply_KF(s, C, p, q);
+      uftk::seg_dot(s, g, df, ns, n, p, q, C->d_part, B.sc, B_FPQ);
+      uftk::bsc_f_alpha(s, ns, B.sc);
+      uftk::seg_fcg_update(s, g, df, ns, n, B.sc, B_FAL, x, p, r, q, C->d_fdinv, z, C->d_part, B_FZZN);
+      uftk::bsc_f_after(s, ns, B.sc);
+    }
     if ((rc = fetch_batch(C, B_FACT, &any))) return rc;
+    round = 2;
   }
   int itmax = 0;
   for (int d = 0; d < ns; ++d) {
@@ -1507,6 +1517,7 @@ int solve_filter_batch(uft_ctx *C, const double *b, double *x) {
       return uft_set_error(C, UFT_ERR_NOCONV, "design %d: filter CG, %d iterations without convergence", d, it);
   }
   C->it_f = itmax;
+  *pred = itmax;
   return UFT_OK;
 }
 
@@ -1521,7 +1532,7 @@ int solve_filter_batch(uft_ctx *C, const double *b, double *x) {
 // tests are those of the loop below, which tests on the host after every
 // iteration.
 int solve_filter(uft_ctx *C, const double *b, double *x, int slot) {
-  if (C->bt.nseg) return solve_filter_batch(C, b, x);
+  if (C->bt.nseg) return solve_filter_batch(C, b, x, slot);
   hipStream_t s = C->s;
   const int64_t n = C->nn;
   double *r = C->d_fr, *z = C->d_fz, *p = C->d_fp, *q = C->d_fq;
