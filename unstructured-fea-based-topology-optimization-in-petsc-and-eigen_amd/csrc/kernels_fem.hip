@@ -883,6 +883,21 @@ __global__ __launch_bounds__(TPB) void k_z_from_f32(int64_t n, const float *__re
   }
   uftr::block_partials<2>(v, part);
 }
+// the same without the copy: the new z goes to znew (the old zo buffer, the
+// caller swaps the two), the old z is only read
+__global__ __launch_bounds__(TPB) void k_z_from_f32_swap(int64_t n, const float *__restrict__ zf,
+                                                         const double *__restrict__ r, const double *__restrict__ zold,
+                                                         double *__restrict__ znew, double *__restrict__ part) {
+  const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  double v[2] = {0.0, 0.0};
+  if (i < n) {
+    const double zn = (double)zf[i], ri = r[i];
+    znew[i] = zn;
+    v[0] = ri * zn;
+    v[1] = ri * zold[i];
+  }
+  uftr::block_partials<2>(v, part);
+}
 template <typename T>
 __global__ __launch_bounds__(TPB) void k_xpby(int64_t n, const T *__restrict__ z, const double *__restrict__ sc, int ib,
                                               T *__restrict__ p) {
@@ -1381,6 +1396,11 @@ void z_from_f32(hipStream_t s, int64_t n, const float *zf, const double *r, doub
                 double *out) {
   k_z_from_f32<<<nblk(n), TPB, 0, s>>>(n, zf, r, z, zo, part);
   final_sum(s, part, (int)nblk(n), 2, out);
+}
+void z_from_f32_swap_beta(hipStream_t s, int64_t n, const float *zf, const double *r, const double *zold,
+                          double *znew, double *part, double *sc, int irz, int iprev, int ibeta) {
+  k_z_from_f32_swap<<<nblk(n), TPB, 0, s>>>(n, zf, r, zold, znew, part);
+  final_beta_flex(s, part, (int)nblk(n), sc, irz, iprev, ibeta);
 }
 void z_from_f32_beta(hipStream_t s, int64_t n, const float *zf, const double *r, double *z, double *zo, double *part,
                      double *sc, int irz, int iprev, int ibeta) {

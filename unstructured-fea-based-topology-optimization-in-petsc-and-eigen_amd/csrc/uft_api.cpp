@@ -1159,7 +1159,12 @@ int solve_el_mixed(uft_ctx *C, const double *xphys, double *U) {
   auto next = [&]() {  // z = B rf in FP32; zo = previous z; r.z, r.zo; beta; p
     fine_cycle<float>(C, rf, zf, first_ready);
     first_ready = false;
-    uftk::z_from_f32_beta(s, n, zf, r, z, zo, C->d_part, C->d_scal, S_RZNEW, S_RZ, S_BETA + 2);
+    if (fuse_first) {  // no graph capture: swap z and zo instead of copying
+      uftk::z_from_f32_swap_beta(s, n, zf, r, z, zo, C->d_part, C->d_scal, S_RZNEW, S_RZ, S_BETA + 2);
+      std::swap(z, zo);
+    } else {
+      uftk::z_from_f32_beta(s, n, zf, r, z, zo, C->d_part, C->d_scal, S_RZNEW, S_RZ, S_BETA + 2);
+    }
     uftk::xpby_lo(s, n, z, C->d_scal, S_BETA + 2, p, C->d_pf);
     head();
   };

@@ -635,6 +635,9 @@ void dot_pr_beta(hipStream_t s, int64_t n, const double *r, const double *z, con
                  double *sc, int irz, int iprev, int ibeta);
 void z_from_f32_beta(hipStream_t s, int64_t n, const float *zf, const double *r, double *z, double *zo, double *part,
                      double *sc, int irz, int iprev, int ibeta);
+// the same writing the new z to znew (the caller swaps z and zo), no copy
+void z_from_f32_swap_beta(hipStream_t s, int64_t n, const float *zf, const double *r, const double *zold,
+                          double *znew, double *part, double *sc, int irz, int iprev, int ibeta);
 void lincomb(hipStream_t s, int64_t n, const double *sc, int ia, const double *a, int ib, const double *b, double *y);
 void lincombf(hipStream_t s, int64_t n, const double *sc, int ia, const float *a, int ib, const float *b, float *y);
 void axpy_inplace_t(hipStream_t s, int64_t n, const double *d, double *x);
