@@ -51,8 +51,8 @@ WORKLOADS = {
 
 # Krylov iteration counts of the oracle's reference-structure iteration on the
 # c1 workload (Jacobi-PCG to rtol 1e-8; three filter CG solves), measured on
-# the build host by running the oracle's design iterations 1-2 in full
-# ("CPU baseline").  Used to scale the per-iteration costs the
+# the build host by running the oracle's design iterations 1-2 in full (about
+# 230 s per iteration on one core).  Used to scale the per-iteration costs the
 # bounded sample measures.
 ORACLE_ITERS = {"c1": {"pcg": 1624, "filter": 42}}
 
@@ -230,8 +230,8 @@ def main():
         print(json.dumps({"copy_bytes": a.numel() * 8, "copies": 3}))
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one MI355X: this build has no multi-GPU path (one GPU per design; "
-              "see DESIGN.md for how the path would shard). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one MI355X: the path runs on one GPU (it would shard by element "
+              "partition with halo exchanges, not implemented). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     forms = ["fp64", "mixed_h"] if args.precision == "both" else [args.precision]
