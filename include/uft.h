@@ -163,6 +163,21 @@ int uft_mma_design_change(uft_ctx *ctx, int x_id, int xold_id, double *ch);
 /* Whole loop body of main.cpp:72-107 on the context's own vectors. */
 int uft_design_iteration(uft_ctx *ctx, int itr, double *fx, double *gx, double *ch);
 
+/* Allocation-free design change.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP stream/graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * result as uft_mma_design_change on caller-owned DEVICE arrays, without a
+ * context: *ch = max_i |x[i] - xold[i]| (0 for n = 0), then xold := x.
+ * x, xold (n doubles) and ch (one double) are device pointers on GPU
+ * `device`; stream is a hipStream_t of that device or NULL (its null
+ * stream); the result is stream-ordered (synchronise before reading ch).
+ * The block partials live in a 512 KiB static device array per GPU, held
+ * until the process ends, so n is limited to 2^24 (UFT_ERR_ARG beyond, with
+ * the limit in uft_last_error(NULL)).  The call makes no hipMalloc or hipFree
+ * and creates no stream or event.  All calls on one GPU share its array:
+ * calls that overlap on one GPU race. */
+int uft_mma_design_change_noalloc(int device, const double *x, double *xold, int64_t n, double *ch, void *stream);
+
 /* --- diagnostics and checks ---------------------------------------------- */
 /* Iterations and true relative residual of the last elasticity solve
  * (the "Norm of error %e, Iterations %d" line, SolveFem.cpp:142-152), and the

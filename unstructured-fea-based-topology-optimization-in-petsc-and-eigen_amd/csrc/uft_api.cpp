@@ -11,15 +11,18 @@
 #include "uft_internal.h"
 
 // ---------------------------------------------------------------- errors
+// Messages of calls that take no context (uft_mma_design_change_noalloc)
+// are kept per host thread and read with uft_last_error(NULL).
+static thread_local std::string g_free_err;
+
 int uft_set_error(uft_ctx *C, int code, const char *fmt, ...) {
-  if (C) {
-    char buf[1024];
-    va_list ap;
-    va_start(ap, fmt);
-    std::vsnprintf(buf, sizeof(buf), fmt, ap);
-    va_end(ap);
-    C->err = buf;
-  }
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  std::vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  if (C) C->err = buf;
+  else g_free_err = buf;
   return code;
 }
 
@@ -1729,7 +1732,10 @@ extern "C" int uft_autotune(uft_ctx *C, int reps, int *variant, double *us_per_v
 // ==========================================================================
 extern "C" {
 
-const char *uft_last_error(const uft_ctx *C) { return C ? C->err.c_str() : "null context"; }
+const char *uft_last_error(const uft_ctx *C) {
+  if (C) return C->err.c_str();
+  return g_free_err.empty() ? "null context" : g_free_err.c_str();
+}
 
 void uft_default_params(uft_params *p) {
   std::memset(p, 0, sizeof(*p));

@@ -126,6 +126,7 @@ def lib():
         "uft_autotune_smoother": (C.c_int, [vp, pi, pi, pd]),
         "uft_level_apply": (C.c_int, [vp, C.c_int, C.c_int, dp, dp]),
         "uft_batch_results": (C.c_int, [vp, pd, pd, pd, pi, pd, pi]),
+        "uft_mma_design_change_noalloc": (C.c_int, [C.c_int, vp, vp, C.c_int64, vp, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
