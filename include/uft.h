@@ -93,7 +93,7 @@ int uft_create(uft_ctx **ctx, int device);
 void uft_destroy(uft_ctx *ctx);
 const char *uft_last_error(const uft_ctx *ctx);
 void uft_default_params(uft_params *p);
-/* OptPara::SetUp / SetUpOpt option overrides (OptPara.cpp:123-218). */
+/* OptPara::SetUp / SetUpOpt defaults and option overrides (OptPara.cpp:61-120). */
 int uft_params_set(uft_ctx *ctx, const uft_params *p);
 int uft_params_get(const uft_ctx *ctx, uft_params *p);
 
@@ -113,6 +113,15 @@ int uft_bc_set(uft_ctx *ctx, int nfixed, const int32_t *fixed, int nlc, int nloa
 /* Same, reading dir/fixeddofs.txt, dir/loaddof_%d.txt, dir/load_%d.txt
  * (InputMeshData.cpp:238,252,271). */
 int uft_bc_load_text(uft_ctx *ctx, const char *dir, int nfixed, int nlc, int nload);
+/* The two text readers above without a context or a GPU (host arrays only;
+ * errors in uft_last_error(NULL)): xyz receives nn x 3 doubles and elem
+ * ne x npe ids (InputMeshData.cpp:100-149); fixed receives nfixed DOFs,
+ * load_dofs / load_vals nlc x nload entries, case-major
+ * (InputMeshData.cpp:228-285). */
+int uft_mesh_read_text(int npe, int64_t nn, int64_t ne, const char *vertex_path, const char *element_path,
+                       double *xyz, int32_t *elem);
+int uft_bc_read_text(const char *dir, int nfixed, int nlc, int nload, int32_t *fixed, int32_t *load_dofs,
+                     double *load_vals);
 /* Binary cache of the mesh and BCs set on ctx (no counterpart in the
  * reference, which re-parses the text files on every start,
  * InputMeshData.cpp:100-149,228-285): uft_cache_save writes them with a

@@ -73,6 +73,7 @@ def lib():
             "orc_filter_sens": (C.c_int, [vp, dp, dp]),
             "orc_heaviside": (None, [vp, dp, dp, C.c_int]),
             "orc_projected_sens": (None, [vp, dp, dp, dp]),
+            "orc_beta": (C.c_double, [vp]),
             "orc_mma_movelimit": (None, [vp, dp, dp, dp]),
             "orc_mma_update": (None, [vp, dp, dp, dp, dp, dp, dp]),
             "orc_design_change": (C.c_double, [vp, dp, dp]),
@@ -242,6 +243,19 @@ class Problem:
         dg = np.array(dg, dtype=np.float64).ravel()
         rc = lib().orc_filter_sens(self.h, df, dg)
         return df, dg.reshape(self.m, self.ne), rc
+
+    def heaviside(self, xf, itr):
+        """PDEFilter::HeavisideFilter on xf at iteration itr; returns (xp, beta)."""
+        xp = np.zeros(self.ne)
+        lib().orc_heaviside(self.h, np.ascontiguousarray(xf, dtype=np.float64), xp, itr)
+        return xp, lib().orc_beta(self.h)
+
+    def projected_sens(self, xf, df, dg):
+        """PDEFilter::ComputeProjectedSens with the beta of the last heaviside call."""
+        df = np.array(df, dtype=np.float64)
+        dg = np.array(dg, dtype=np.float64).ravel()
+        lib().orc_projected_sens(self.h, np.ascontiguousarray(xf, dtype=np.float64), df, dg)
+        return df, dg.reshape(self.m, self.ne)
 
     def init_design(self):
         return lib().orc_init_design(self.h)

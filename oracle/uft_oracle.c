@@ -713,7 +713,9 @@ void orc_apply_K(orc_ctx *C, const double *u, double *y) { csr_mv(&C->K, u, y); 
 
 /* Jacobi-PCG on the eliminated system, x0 = 0 (U zeroed, SolveFem.cpp:273),
  * stop when ||F - K U||_2 <= rtol ||F||_2 (right-preconditioned FGMRES
- * semantics of SolveFem.cpp:126-139). */
+ * semantics of SolveFem.cpp:126-139).  The test is on the true residual:
+ * when the recursively updated residual passes it, F - K U is formed and,
+ * if that one has drifted above the tolerance, CG restarts from it. */
 static int pcg_el(orc_ctx *C) {
   int64_t n = C->ndof;
   double *r = malloc(n * sizeof(double)), *z = malloc(n * sizeof(double));
@@ -723,11 +725,22 @@ static int pcg_el(orc_ctx *C) {
     C->U[i] = 0.0; r[i] = C->F[i]; z[i] = dinv[i] * r[i]; p[i] = z[i];
   }
   double bn = sqrt(dot(n, C->F, C->F));
-  double rz = dot(n, r, z), rn = bn;
+  double rz = dot(n, r, z), rn = bn, tn = bn;
+  const double tol = C->P.rtol_el * bn;
   int it = 0;
   if (bn == 0.0) { C->it_el = 0; C->relres_el = 0.0; free(r); free(z); free(p); free(q); free(dinv); return 0; }
   const int progress = getenv("ORC_PROGRESS") != NULL;
-  while (rn > C->P.rtol_el * bn && it < C->P.maxit_el) {
+  while (it < C->P.maxit_el) {
+    if (rn <= tol) {
+      csr_mv(&C->K, C->U, q);
+      for (int64_t i = 0; i < n; i++) r[i] = C->F[i] - q[i];
+      tn = sqrt(dot(n, r, r));
+      if (tn <= tol) break;
+      /* restart from the true residual */
+      rn = tn;
+      for (int64_t i = 0; i < n; i++) { z[i] = dinv[i] * r[i]; p[i] = z[i]; }
+      rz = dot(n, r, z);
+    }
     if (progress && it % 500 == 0) { fprintf(stderr, "[oracle pcg] it %d relres %.3e\n", it, rn / bn); fflush(stderr); }
     csr_mv(&C->K, p, q);
     double al = rz / dot(n, p, q);
@@ -743,10 +756,11 @@ static int pcg_el(orc_ctx *C) {
   /* true residual diagnostic (SolveFem.cpp:142-152) */
   csr_mv(&C->K, C->U, q);
   for (int64_t i = 0; i < n; i++) q[i] = C->F[i] - q[i];
-  C->relres_el = sqrt(dot(n, q, q)) / bn;
+  tn = sqrt(dot(n, q, q));
+  C->relres_el = tn / bn;
   C->it_el = it;
   free(r); free(z); free(p); free(q); free(dinv);
-  return rn <= C->P.rtol_el * bn ? 0 : 1;
+  return tn <= tol ? 0 : 1;
 }
 
 /* SolveFem::ComputeObjectiveAndSensitivities (SolveFem.cpp:159-265):
@@ -925,6 +939,9 @@ void orc_heaviside(orc_ctx *C, const double *xf, double *xp, int itr) {
   }
   C->btemp = bt;
 }
+
+/* Projection sharpness beta of the last HeavisideFilter call (PDEFilter.h:51). */
+double orc_beta(orc_ctx *C) { return C->beta; }
 
 /* PDEFilter::ComputeProjectedSens (PDEFilter.cpp:295-341) */
 void orc_projected_sens(orc_ctx *C, const double *xf, double *dfdx, double *dgdx) {

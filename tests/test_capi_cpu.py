@@ -125,3 +125,83 @@ def test_batch_mesh_layout(U):
     assert np.all(np.abs(xyz[fn, 0]) < 1e-12)
     assert len(fn) == 16 + 9
     assert len(ld) == 4 + 3 and np.all(np.abs(xyz[ld // 3, 0] - 2.0) < 1e-12)
+
+
+REF_MESH = os.path.join(conftest.REFERENCE, "mesh")
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_MESH), reason="the reference's BC files are not on this machine")
+def test_reader_on_reference_bc_files(U):
+    """The library's host-only BC reader (InputMeshData::GetBcs,
+    InputMeshData.cpp:228-285) on the reference's own mesh/fixeddofs.txt,
+    loaddof_1.txt and load_1.txt (no line terminators): the counts of
+    OptPara.cpp:9-10, the sums and orderings of the fixtures, clamped nodes as
+    {3n, 3n+1, 3n+2} triples, y-only unit loads; equal to numpy's parse."""
+    import hashlib
+    from uft.capi import UFTError
+    digests = {"fixeddofs.txt": "5d38111853e5aa3380ab51f2f2d18974832124b7b0d96c8fb019c8f95bbac4fe",
+               "loaddof_1.txt": "8c8c871a9f4940b7010408f681dc3110618741b12a0802331c9e175eae8011bd",
+               "load_1.txt": "fe9c5ba7692571b3e63ee50c2914080c70720f386e1210ba4faa94b514650cec"}
+    for name, want in digests.items():
+        assert hashlib.sha256(open(os.path.join(REF_MESH, name), "rb").read()).hexdigest() == want, name
+    fixed, ld, lv = U.bc_read_text(REF_MESH, 8733, 1, 41)
+    assert len(fixed) == 8733 and int(fixed.astype(np.int64).sum()) == 1073477334
+    assert np.all(np.diff(fixed) > 0) and fixed[0] == 0 and fixed[-1] == 489047
+    tri = fixed.reshape(-1, 3)
+    assert np.all(tri[:, 0] % 3 == 0) and np.all(tri[:, 1] == tri[:, 0] + 1) and np.all(tri[:, 2] == tri[:, 0] + 2)
+    assert list(tri[:7, 0] // 3) == [0, 2, 3, 7, 8, 9, 16]
+    assert ld.shape == (1, 41) and int(ld.astype(np.int64).sum()) == 37262276
+    assert np.all(ld % 3 == 1) and np.all(np.diff(ld[0]) > 0)
+    assert np.all(lv == 1.0)
+    assert np.array_equal(fixed, np.loadtxt(os.path.join(REF_MESH, "fixeddofs.txt"), dtype=np.int32))
+    assert np.array_equal(ld[0], np.loadtxt(os.path.join(REF_MESH, "loaddof_1.txt"), dtype=np.int32))
+    # one value more than the file holds: an I/O error naming the file, not a silent zero
+    with pytest.raises(UFTError) as e:
+        U.bc_read_text(REF_MESH, 8734, 1, 41)
+    assert e.value.code == 2 and "fixeddofs.txt: expected 8734 values, read 8733" in str(e.value)
+    with pytest.raises(UFTError) as e:
+        U.bc_read_text(REF_MESH, 8733, 2, 41)  # no loaddof_2.txt
+    assert e.value.code == 2 and "loaddof_2.txt" in str(e.value)
+
+
+def test_mesh_reader_host_only(U, tmp_path):
+    """uft_mesh_read_text on files in the reference's formats
+    (InputMeshData.cpp:100-149: whitespace-separated, no terminator)."""
+    from uft.capi import mesh_read_text, UFTError
+    xyz, elem = U.gen_box(8, 5, 3, 2, jitter=0.15, seed=4)
+    (tmp_path / "vertex.txt").write_text(" ".join(f"{v:.17g}" for v in xyz.ravel()))
+    (tmp_path / "element.txt").write_text("\n".join(" ".join(str(v) for v in r) for r in elem))
+    a, b = mesh_read_text(8, len(xyz), len(elem), tmp_path / "vertex.txt", tmp_path / "element.txt")
+    assert np.array_equal(a, xyz) and np.array_equal(b, elem)
+    with pytest.raises(UFTError) as e:
+        mesh_read_text(8, len(xyz) + 1, len(elem), tmp_path / "vertex.txt", tmp_path / "element.txt")
+    assert e.value.code == 2
+    with pytest.raises(UFTError) as e:
+        mesh_read_text(8, len(xyz), len(elem), tmp_path / "missing.txt", tmp_path / "element.txt")
+    assert e.value.code == 2 and "cannot open" in str(e.value)
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(conftest.REFERENCE, "main.cpp")),
+                    reason="the reference's main.cpp is not on this machine")
+def test_reference_main_compiles_against_dropin(tmp_path):
+    """The reference's own driver (main.cpp, read at test time, not kept in
+    this repository) with its #include lines replaced by the drop-in header
+    compiles and links against libuft.so unchanged otherwise."""
+    lines = open(os.path.join(conftest.REFERENCE, "main.cpp")).read().splitlines()
+    out, done = [], False
+    for ln in lines:
+        if ln.lstrip().startswith("#include"):
+            if not done:
+                out.append('#include "uft_dropin.h"')
+                done = True
+            continue
+        out.append(ln)
+    assert done and sum(ln.lstrip().startswith("#include") for ln in lines) == 9
+    src = tmp_path / "main.cpp"
+    src.write_text("\n".join(out) + "\n")
+    inc = [os.path.join(conftest.ROOT, "include"), os.path.join(conftest.PKG, "dropin")]
+    libdir = os.path.join(conftest.PKG, "lib")
+    subprocess.run(["g++", "-std=c++17", "-Wall", "-Wno-unused-variable", "-Wno-unused-but-set-variable",
+                    "-Wno-maybe-uninitialized", "-O1"] + [f"-I{d}" for d in inc] +
+                   [str(src), "-o", str(tmp_path / "opt"), f"-L{libdir}", "-luft"], check=True)
+    assert os.access(tmp_path / "opt", os.X_OK)

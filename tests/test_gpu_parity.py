@@ -152,3 +152,54 @@ def test_projection_path(O):
         assert abs(fx - rfx) <= 1e-10 * abs(rfx)
         assert abs(ch - rch) <= 1e-8
     assert np.max(np.abs(ctx.get("XPhys") - ref.vec("XPhys"))) <= 1e-8
+
+
+def _beta_table(itr, beta0, betamax):
+    """PDEFilter.cpp:259-269 for a call at every itr: beta0 below 30, then +1
+    on every itr divisible by 10, capped at betamax."""
+    return beta0 if itr < 30 else min(betamax, beta0 + (itr // 10) - 2)
+
+
+@pytest.mark.parametrize("betamax", [8.0, 4.5])
+def test_heaviside_schedule(O, betamax):
+    """uft_heaviside at every itr 0..100 against the oracle and the hand
+    schedule (x~ = 0.75 projects to 1/2 + tanh(b/4) / (2 tanh(b/2)) for
+    x0 = 0.5), and uft_projected_sens with the beta of the last call."""
+    prm = dict(Emax=1.0, volfrac=0.3, rmin=0.375, proj=1, beta0=1.0, betamax=betamax, **TIGHT)
+    ctx, ref = _pair(O, 8, 8, 4, 4, **prm)
+    rng = np.random.default_rng(21)
+    xf = rng.uniform(0.0, 1.0, ref.ne)
+    xf[0] = 0.75
+    ctx.set("XFiled", xf)
+    for itr in range(101):
+        ctx.heaviside(itr)
+        xp = ctx.get("XPhys")
+        rxp, rbeta = ref.heaviside(xf, itr)
+        b = _beta_table(itr, 1.0, betamax)
+        assert rbeta == b, (itr, rbeta, b)
+        assert np.max(np.abs(xp - rxp)) <= 2e-15, itr
+        assert abs(xp[0] - (0.5 + np.tanh(b / 4) / (2 * np.tanh(b / 2)))) <= 2e-15, itr
+        if itr in (0, 29, 30, 31, 39, 40, 59, 60, 90, 100):
+            df = rng.standard_normal(ref.ne)
+            dg = rng.uniform(0.5, 1.5, ref.ne)
+            ctx.set("dfdx", df)
+            ctx.set("dgdx0", dg)
+            ctx.projected_sens()
+            rdf, rdg = ref.projected_sens(xf, df, dg[None, :])
+            assert _rel(ctx.get("dfdx"), rdf) <= 1e-14, itr
+            assert _rel(ctx.get("dgdx0"), rdg[0]) <= 1e-14, itr
+
+
+def test_projection_continuation_loop(O):
+    """42 design iterations with the projection on, through the beta
+    continuation at itr 30 and 40 (PDEFilter.cpp:259-269) on both sides;
+    the loop is driven past the ch <= 0.01 stop on purpose."""
+    prm = dict(Emax=1.0, volfrac=0.3, rmin=0.375, proj=1, beta0=1.0, **TIGHT)
+    ctx, ref = _pair(O, 8, 8, 4, 4, **prm)
+    rows = _loop(ctx, ref, 42)
+    for it, (fx, rfx, g, rg, ch, rch) in enumerate(rows, 1):
+        assert abs(fx - rfx) <= 1e-10 * abs(rfx), (it, fx, rfx)
+        assert abs(g - rg) <= 1e-10 * max(1.0, abs(rg)), (it, g, rg)
+        assert abs(ch - rch) <= 1e-8, (it, ch, rch)
+    assert np.max(np.abs(ctx.get("X") - ref.vec("X"))) <= 1e-8
+    assert np.max(np.abs(ctx.get("XPhys") - ref.vec("XPhys"))) <= 1e-8

@@ -222,6 +222,47 @@ static int read_tokens(uft_ctx *C, const char *path, int64_t count, T *out) {
   return UFT_OK;
 }
 
+// InputMeshData::GetNodeAndElementData's parse (InputMeshData.cpp:100-149)
+// into host arrays; errors go to C (NULL: the thread's context-free message).
+static int mesh_read_text(uft_ctx *C, int npe, int64_t nn, int64_t ne, const char *vertex_path,
+                          const char *element_path, double *xyz, int32_t *elem) {
+  if (nn <= 0 || ne <= 0 || (npe != 8 && npe != 4) || !vertex_path || !element_path || !xyz || !elem)
+    return uft_set_error(C, UFT_ERR_ARG, "mesh text reader: bad arguments");
+  int rc = read_tokens(C, vertex_path, nn * 3, xyz);
+  if (rc) return rc;
+  return read_tokens(C, element_path, ne * npe, elem);
+}
+
+// InputMeshData::GetBcs (InputMeshData.cpp:228-285): for each case i = 1..nlc
+// dir/loaddof_<i>.txt and dir/load_<i>.txt (nload values each), then
+// dir/fixeddofs.txt (nfixed values), into host arrays.
+static int bc_read_text(uft_ctx *C, const char *dir, int nfixed, int nlc, int nload, int32_t *fixed,
+                        int32_t *ld, double *lv) {
+  if (!dir || nfixed < 0 || nlc < 1 || nload < 0 || (nfixed && !fixed) || (nload && (!ld || !lv)))
+    return uft_set_error(C, UFT_ERR_ARG, "BC text reader: bad arguments");
+  char path[4096];
+  for (int i = 0; i < nlc; ++i) {
+    std::snprintf(path, sizeof(path), "%s/loaddof_%d.txt", dir, i + 1);
+    int rc = read_tokens(C, path, nload, ld + (size_t)i * nload);
+    if (rc) return rc;
+    std::snprintf(path, sizeof(path), "%s/load_%d.txt", dir, i + 1);
+    rc = read_tokens(C, path, nload, lv + (size_t)i * nload);
+    if (rc) return rc;
+  }
+  std::snprintf(path, sizeof(path), "%s/fixeddofs.txt", dir);
+  return read_tokens(C, path, nfixed, fixed);
+}
+
+extern "C" int uft_mesh_read_text(int npe, int64_t nn, int64_t ne, const char *vertex_path, const char *element_path,
+                                  double *xyz, int32_t *elem) {
+  return mesh_read_text(nullptr, npe, nn, ne, vertex_path, element_path, xyz, elem);
+}
+
+extern "C" int uft_bc_read_text(const char *dir, int nfixed, int nlc, int nload, int32_t *fixed, int32_t *load_dofs,
+                                double *load_vals) {
+  return bc_read_text(nullptr, dir, nfixed, nlc, nload, fixed, load_dofs, load_vals);
+}
+
 extern "C" int uft_mesh_load_text(uft_ctx *C, int npe, int64_t nn, int64_t ne, const char *vertex_path,
                                   const char *element_path) {
   if (!C) return UFT_ERR_ARG;
@@ -229,9 +270,7 @@ extern "C" int uft_mesh_load_text(uft_ctx *C, int npe, int64_t nn, int64_t ne, c
     return uft_set_error(C, UFT_ERR_ARG, "uft_mesh_load_text: bad arguments");
   std::vector<double> xyz(nn * 3);
   std::vector<int32_t> elem(ne * npe);
-  int rc = read_tokens(C, vertex_path, nn * 3, xyz.data());
-  if (rc) return rc;
-  rc = read_tokens(C, element_path, ne * npe, elem.data());
+  int rc = mesh_read_text(C, npe, nn, ne, vertex_path, element_path, xyz.data(), elem.data());
   if (rc) return rc;
   return uft_mesh_set(C, npe, nn, ne, elem.data(), xyz.data());
 }
@@ -241,17 +280,7 @@ extern "C" int uft_bc_load_text(uft_ctx *C, const char *dir, int nfixed, int nlc
   if (!dir || nfixed < 0 || nlc < 1 || nload < 0) return uft_set_error(C, UFT_ERR_ARG, "uft_bc_load_text: bad arguments");
   std::vector<int32_t> fixed(nfixed), ld((size_t)nlc * nload);
   std::vector<double> lv((size_t)nlc * nload);
-  char path[4096];
-  for (int i = 0; i < nlc; ++i) {
-    std::snprintf(path, sizeof(path), "%s/loaddof_%d.txt", dir, i + 1);
-    int rc = read_tokens(C, path, nload, ld.data() + (size_t)i * nload);
-    if (rc) return rc;
-    std::snprintf(path, sizeof(path), "%s/load_%d.txt", dir, i + 1);
-    rc = read_tokens(C, path, nload, lv.data() + (size_t)i * nload);
-    if (rc) return rc;
-  }
-  std::snprintf(path, sizeof(path), "%s/fixeddofs.txt", dir);
-  int rc = read_tokens(C, path, nfixed, fixed.data());
+  int rc = bc_read_text(C, dir, nfixed, nlc, nload, fixed.data(), ld.data(), lv.data());
   if (rc) return rc;
   return uft_bc_set(C, nfixed, fixed.data(), nlc, nload, ld.data(), lv.data());
 }

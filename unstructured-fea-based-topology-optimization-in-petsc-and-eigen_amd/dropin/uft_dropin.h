@@ -1,8 +1,9 @@
 // Drop-in C++ host classes with the reference's names and call signatures
 // (OptPara.h, InputMeshData.h, SolveFem.h, PDEFilter.h, MMA.h, Write2Vtu.h),
-// implemented over the C-ABI of include/uft.h.  A driver written against the
-// reference's main.cpp call sequence compiles against these unchanged except
-// for the initialisation call (UftInitialize replaces PetscInitialize).
+// implemented over the C-ABI of include/uft.h.  The reference's main.cpp
+// compiles against this header with only its #include lines replaced by
+// #include "uft_dropin.h" (PetscInitialize, PetscPrintf, MPI_Wtime, ... are
+// provided below for one process on one GPU).
 //
 // PETSc's Vec becomes a handle to a device-resident vector of the context;
 // the mesh matrices that the reference passes around (elementEigen,
@@ -10,6 +11,7 @@
 // device once InputMeshData has loaded it.
 #pragma once
 #include <cmath>
+#include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -17,6 +19,7 @@
 #include <string>
 #include <memory>
 #include <vector>
+#include <time.h>
 
 #include "uft.h"
 
@@ -77,6 +80,41 @@ inline PetscErrorCode UftFinalize() {
   return 0;
 }
 
+// ------------------------------------------------------------------ PETSc / MPI names of the driver
+// The calls the reference's main.cpp makes outside the five classes
+// (main.cpp:38-44,112,132,135), on one process and one GPU: the driver then
+// compiles with only its #include lines replaced by this header.
+typedef int PetscMPIInt;
+typedef int MPI_Comm;
+#define MPI_COMM_WORLD 0
+#define PETSC_COMM_WORLD 0
+#define PETSC_NULL nullptr
+namespace Eigen {}  // the driver's "using namespace Eigen;" (main.cpp:12)
+inline PetscErrorCode PetscInitialize(int *argc, char ***argv, const char * /*file*/, const char * /*help*/) {
+  return UftInitialize(argc, argv);
+}
+inline PetscErrorCode PetscFinalize() { return UftFinalize(); }
+inline int MPI_Comm_rank(MPI_Comm, int *rank) {
+  *rank = 0;
+  return 0;
+}
+inline int MPI_Comm_size(MPI_Comm, int *size) {
+  *size = 1;
+  return 0;
+}
+inline double MPI_Wtime() {
+  timespec t;
+  clock_gettime(CLOCK_MONOTONIC, &t);
+  return (double)t.tv_sec + 1e-9 * (double)t.tv_nsec;
+}
+inline PetscErrorCode PetscPrintf(MPI_Comm, const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  std::vprintf(fmt, ap);
+  va_end(ap);
+  return 0;
+}
+
 // handle to a device vector of the context (replaces PETSc Vec)
 // A PETSc Vec stand-in: a context vector id, or (for the MMA restart
 // vectors, which the reference keeps as user-owned Vecs) a host buffer made
@@ -116,7 +154,7 @@ class OptPara {
   PetscErrorCode SetMMA(PetscInt *itr, class MMA **mma);
 
  private:
-  // defaults and options (OptPara.cpp:123-218)
+  // defaults and options (OptPara::SetUp / SetUpOpt, OptPara.cpp:61-120)
   void SetUp() {
     UftRuntime &R = UftRuntime::get();
     uft_default_params(&params);

@@ -127,6 +127,8 @@ def lib():
         "uft_level_apply": (C.c_int, [vp, C.c_int, C.c_int, dp, dp]),
         "uft_batch_results": (C.c_int, [vp, pd, pd, pd, pi, pd, pi]),
         "uft_mma_design_change_noalloc": (C.c_int, [C.c_int, vp, vp, C.c_int64, vp, vp]),
+        "uft_mesh_read_text": (C.c_int, [C.c_int, C.c_int64, C.c_int64, C.c_char_p, C.c_char_p, dp, ip]),
+        "uft_bc_read_text": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, ip, ip, dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -170,6 +172,28 @@ def box_bcs(nx, ny, nz, load_kind=0):
     if rc:
         raise UFTError(rc, "uft_box_bcs")
     return fixed[:nf.value].copy(), ld[:nl.value].copy(), lv[:nl.value].copy()
+
+
+def bc_read_text(dirpath, nfixed, nlc, nload):
+    """Host-only BC text reader (uft_bc_read_text): fixed DOFs, load DOFs and
+    load values (nlc x nload) of the reference's files in dirpath."""
+    fixed = np.zeros(nfixed, np.int32)
+    ld = np.zeros(nlc * nload, np.int32)
+    lv = np.zeros(nlc * nload)
+    rc = lib().uft_bc_read_text(os.fsencode(dirpath), nfixed, nlc, nload, fixed, ld, lv)
+    if rc:
+        raise UFTError(rc, lib().uft_last_error(None).decode())
+    return fixed, ld.reshape(nlc, nload), lv.reshape(nlc, nload)
+
+
+def mesh_read_text(npe, nn, ne, vertex_path, element_path):
+    """Host-only mesh text reader (uft_mesh_read_text): (xyz nn x 3, elem ne x npe)."""
+    xyz = np.zeros(nn * 3)
+    elem = np.zeros(ne * npe, np.int32)
+    rc = lib().uft_mesh_read_text(npe, nn, ne, os.fsencode(vertex_path), os.fsencode(element_path), xyz, elem)
+    if rc:
+        raise UFTError(rc, lib().uft_last_error(None).decode())
+    return xyz.reshape(nn, 3), elem.reshape(ne, npe)
 
 
 class Context:
