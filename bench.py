@@ -17,11 +17,13 @@ cpu_baseline (the CPU oracle on a bounded sample of the same workload).
 
   python bench.py [--steps K] [--warmup W] [--precision both|fp64|mixed|mixed_h] [--no-cpu-baseline]
 
-By default both the FP64 form and the mixed form (FP32 operator and
-multigrid cycle with FP16 smoother blocks inside the FP64 flexible outer
-loop) are timed on the same workload; value is the mixed form's rate and
-config carries the FP64 rate, their ratio and the largest relative
-compliance difference between the two forms over the timed steps.
+By default three forms are timed on the same workload: FP64; UFT_MIXED (FP32
+outer operator, FP32 multigrid cycle and FP32 smoother blocks inside the FP64
+flexible outer loop, the precision form as specified); and UFT_MIXED_H (as
+UFT_MIXED with the smoother's fine operator assembled into int8 node blocks
+with an FP32 scale per block, FP16 level-1 smoother blocks).  value is the
+UFT_MIXED_H rate; config carries the other rates, the ratios and the largest
+relative compliance difference against the FP64 form over the timed steps.
 """
 import argparse
 import json
@@ -177,10 +179,14 @@ def run_form(args, precision):
         cnt, ms = ctypes.c_int64(), ctypes.c_double()
         L.uft_profile_read(ctx.h, kind, ctypes.byref(cnt), ctypes.byref(ms))
         prof[kind] = (cnt.value, ms.value)
+    solve_bytes, solve_ms, applies, solves = ctx.solve_traffic()
     L.uft_profile_enable(ctx.h, 0)
     ctx.close()
     return {"npe": npe, "dims": dims, "seed": seed, "ne": ne, "ndof": ndof, "setup_s": setup_s, "dt": t1 - t0,
-            "its": its, "fxs": fxs, "prof": prof, "ndesign": ndesign, "per_design": per_design, "ell": ell}
+            "its": its, "fxs": fxs, "prof": prof, "ndesign": ndesign, "per_design": per_design, "ell": ell,
+            "solve": {"algorithmic_bytes": solve_bytes, "ms": solve_ms, "operator_applies": applies,
+                      "solves": solves, "gbs": solve_bytes / solve_ms / 1e6 if solve_ms > 0 else None,
+                      "ms_per_step": solve_ms / max(solves, 1)}}
 
 
 def elem_roofline(name, npe, ne, ndof, prof_elem, prof_node, word, kernel, traffic_key, nrhs=1, algo=None):
@@ -196,7 +202,11 @@ def elem_roofline(name, npe, ne, ndof, prof_elem, prof_node, word, kernel, traff
     if os.path.exists(tfile):
         traffic = json.load(open(tfile)).get(traffic_key)
     return {"bound": "hbm", "role": name, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": traffic, "kernel": kernel,
+            "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": traffic,
+            "traffic_source": ("not measured in this run: HBM bytes per dispatch of the same kernel from "
+                               "rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs, scaled by a plain copy's, "
+                               f"profiles/traffic.json[{traffic_key}]") if traffic else None,
+            "kernel": kernel,
             "algorithmic_bytes_per_launch": algo, "launches": n, "avg_launch_us": avg_s * 1e6,
             "total_ms_timed": ms, "node_phase_avg_us": prof_node[1] / max(prof_node[0], 1) * 1e3}
 
@@ -204,12 +214,12 @@ def elem_roofline(name, npe, ne, ndof, prof_elem, prof_node, word, kernel, traff
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="c1", choices=sorted(WORKLOADS))
     ap.add_argument("--precision", default="both", choices=["both", "fp64", "mixed", "mixed_h"],
-                    help="both (default): time the FP64 form and the mixed form (FP16 smoother blocks); "
-                         "the value is the mixed form's rate")
+                    help="both (default): time the FP64, UFT_MIXED and UFT_MIXED_H forms; the value is the "
+                         "UFT_MIXED_H rate")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--timers", type=int, default=16,
                     help="HIP-event timers around one in every TIMERS fine applies of each kind in the timed "
@@ -234,9 +244,7 @@ def main():
               "partition with halo exchanges, not implemented). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
-    forms = ["fp64", "mixed_h"] if args.precision == "both" else [args.precision]
-    if args.precision == "both" and args.workload == "c4a":  # the BF16 matrix-core form beside the default
-        forms = ["fp64", "mixed", "mixed_h"]
+    forms = ["fp64", "mixed", "mixed_h"] if args.precision == "both" else [args.precision]
     res = {f: run_form(args, f) for f in forms}
     main_form = forms[-1]
     r = res[main_form]
@@ -252,6 +260,18 @@ def main():
         rooflines.append(elem_roofline("fp64", npe, ne, ndof, q["prof"][0], q["prof"][1], 8,
                                        f"{k64} (FP64 element phase of the fine stiffness apply, FP64 form)",
                                        f"{args.workload}_fp64_elem_apply", nrhs))
+        if not multi:
+            n, ms = q["prof"][0]
+            nn_, msn = q["prof"][1]
+            if n and nn_:
+                us = (ms / n + msn / nn_) * 1e3
+                algo = algorithmic_bytes_apply(npe, ne, ndof, 8)
+                rooflines.append({
+                    "bound": "hbm", "role": "fp64_k_apply", "achieved": algo / (us * 1e-6) / 1e9,
+                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": algo / (us * 1e-6) / 1e9 / HBM_PEAK_GBS,
+                    "traffic": None, "kernel": f"k_elem_apply_v<{npe},double> + k_node_gather_op (whole FP64 "
+                    "K u: element phase plus node phase, FP64 form)", "algorithmic_bytes_per_launch": algo,
+                    "launches": n, "avg_launch_us": us, "total_ms_timed": ms + msn})
     if main_form != "fp64":
         if multi:
             w, kname = 2, (f"k_mr_elem_mfma<{npe}> (BF16 v_mfma_f32_16x16x32 element products of the "
@@ -294,8 +314,8 @@ def main():
                                        f"preconditioner, 8 load cases, UFT_MIXED form)",
                                        f"{args.workload}_mixed_elem_apply", nrhs))
     # the dominant element phase of the timed form: the one with the most time
-    cand = [x for x in rooflines if x["launches"]] if main_form == "fp64" else \
-        [x for x in rooflines if x["launches"] and x["role"] not in ("fp64", "bf16_mfma")]
+    cand = [x for x in rooflines if x["launches"] and x["role"] != "fp64_k_apply"] if main_form == "fp64" else \
+        [x for x in rooflines if x["launches"] and x["role"] not in ("fp64", "bf16_mfma", "fp64_k_apply")]
     dom = max(cand, key=lambda x: x["total_ms_timed"]) if cand else rooflines[-1]
     if r["ndesign"] > 1:
         wl = (f"{args.workload}: {r['ndesign']} independent {ename} jittered cantilevers of uneven size "
@@ -314,10 +334,22 @@ def main():
         config["mixed_over_fp64"] = value / config["fp64_design_iters_per_s"]
         config["fp64_krylov_iters_per_solve"] = sum(f64["its"]) / len(f64["its"])
         config["max_rel_compliance_diff_vs_fp64"] = max(abs(a - b) / abs(b) for a, b in zip(r["fxs"], f64["fxs"]))
+    config["krylov_iters_per_step"] = r["its"]
+    # Solve GB/s: algorithmic bytes of every operator apply inside the timed solves (fine and coarse
+    # levels, the coarsest dense inverse; transfers and vector updates not counted) over the solves'
+    # wall time (hierarchy refresh excluded)
+    config["solve_gbs"] = {f: res[f]["solve"] for f in forms}
+    if "fp64" in res and main_form != "fp64":
+        config["fp64_krylov_iters_per_step"] = res["fp64"]["its"]
     if "mixed" in res and main_form == "mixed_h":
         q = res["mixed"]
-        config["bf16_mfma_form_design_iters_per_s"] = args.steps * q["ndesign"] / q["dt"]
-        config["bf16_mfma_form_krylov_iters_per_solve"] = sum(q["its"]) / len(q["its"])
+        key = "bf16_mfma_form" if multi else "fp32_smoother_form"
+        config[f"{key}_design_iters_per_s"] = args.steps * q["ndesign"] / q["dt"]
+        config[f"{key}_krylov_iters_per_solve"] = sum(q["its"]) / len(q["its"])
+        if "fp64" in res:
+            config[f"{key}_over_fp64"] = config[f"{key}_design_iters_per_s"] / config["fp64_design_iters_per_s"]
+            config[f"{key}_max_rel_compliance_diff_vs_fp64"] = max(
+                abs(a - b) / abs(b) for a, b in zip(q["fxs"], res["fp64"]["fxs"]))
     out = {
         "metric": "design iters/s @392k hex8 cantilever; mixed vs FP64 GPU at 1e-6 rel compliance",
         "value": value, "unit": "design iters/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,

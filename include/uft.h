@@ -50,8 +50,12 @@ enum uft_size_id { UFT_SIZE_NE = 0, UFT_SIZE_NN = 1, UFT_SIZE_NDOF = 2, UFT_SIZE
 /* UFT_FP64: every operator product in FP64.  UFT_MIXED: FP64 flexible outer
  * Krylov loop (residual, inner products, iterate), multigrid preconditioner
  * with FP32 element blocks and coarse operators, FP64 coarsest solve.
- * UFT_MIXED_H: as UFT_MIXED with the smoother's fine element blocks stored
- * in FP16 (scaled per element, FP32 arithmetic). */
+ * UFT_MIXED_H: as UFT_MIXED with the smoother's fine operator stored low:
+ * with fine_assembled = 1 (default) as assembled 3x3 node blocks of D^-1 A
+ * in int8 with an FP32 scale per block (one right-hand side) or in FP16
+ * (several load cases), level 1's smoother blocks in FP16; with
+ * fine_assembled = 0 as FP16 element blocks scaled per element.  FP32
+ * arithmetic throughout the cycle. */
 enum uft_precision { UFT_FP64 = 0, UFT_MIXED = 1, UFT_MIXED_H = 2 };
 
 /* Parameters.  Defaults (uft_default_params) are the reference's
@@ -82,7 +86,8 @@ typedef struct uft_params {
                              launch latency, not by bandwidth); 0 (default): cheb_degree */
   int fine_assembled;  /* UFT_MIXED_H with superposed load cases (one design or a design batch): 1 (default)
                           the fine operators are assembled at each hierarchy refresh into 3x3 node blocks,
-                          FP16 scaled by the diagonal for the smoother and FP32 for the outer product (one
+                          the smoother's rows divided by the diagonal in int8 with an FP32 scale per block
+                          (FP16 with several load cases) and FP32 for the outer product (one
                           launch per apply, no element-local round trip); 0: the element blocks.  Fixed at
                           setup. */
 } uft_params;
@@ -137,6 +142,18 @@ int uft_setup(uft_ctx *ctx);
 /* Host copy of the mesh (elem ne x npe, xyz nn x 3), for output such as the
  * VTU writer (Write2Vtu.cpp:5-173 gathers the same data on rank 0). */
 int uft_mesh_get(const uft_ctx *ctx, int32_t *elem, double *xyz);
+/* Write2Vtu::WriteResults (Write2Vtu.cpp:5-173): ASCII VTU of the context's
+ * mesh with the displacement u_id (point data ux, uy, uz) and the element
+ * field xphys_id (cell data xphys), copied from the device; hex8 corners in
+ * VTK order 0,1,3,2,4,5,7,6 (Write2Vtu.cpp:89-96), cell type 12 (hex8) or
+ * 10 (tet4); numbers in scientific notation with `digits` digits after the
+ * point (the reference writes 6; 17 round-trips FP64).  UFT_ERR_IO when the
+ * file cannot be written. */
+int uft_write_vtu(uft_ctx *ctx, const char *path, int xphys_id, int u_id, int digits);
+/* The same file from host arrays, without a context or a GPU (errors in
+ * uft_last_error(NULL)). */
+int uft_vtu_write_host(const char *path, int npe, int64_t nn, int64_t ne, const double *xyz, const int32_t *elem,
+                       const double *u, const double *xphys, int digits);
 int64_t uft_size(const uft_ctx *ctx, int which);
 double uft_total_volume(const uft_ctx *ctx);
 
@@ -227,7 +244,8 @@ int uft_apply_K_cases(uft_ctx *ctx, int kind, const double *u, double *y);
 /* Diagnostic (no counterpart in the reference): y = A_l x with the operator
  * of multigrid level l as of the last hierarchy refresh (level 0 = the fine
  * stiffness of XPhys), in the stored precision kind 0 = FP64, 1 = FP32,
- * 2 = FP16 (the UFT_MIXED_H smoother's blocks); x, y of length n_l. */
+ * 2 = the UFT_MIXED_H smoother's low-precision blocks (int8 with a block scale or
+ * FP16); x, y of length n_l. */
 int uft_level_apply(uft_ctx *ctx, int level, int kind, const double *x, double *y);
 
 /* --- MMA settings, restart and KKT residual (MMA.h:8-52) ------------------ */
@@ -304,6 +322,17 @@ int uft_sync(uft_ctx *ctx);
  * form. */
 int uft_profile_enable(uft_ctx *ctx, int on);
 int uft_profile_read(uft_ctx *ctx, int kind, int64_t *count, double *total_ms);
+/* Solve traffic since the last uft_profile_enable (no counterpart in the
+ * reference): the summed algorithmic bytes of every operator apply made
+ * inside the elasticity solves -- fine level (element blocks: Ne (S w +
+ * 4 npe + 8), S unique entries of w bytes; assembled node blocks: blocks x
+ * (block bytes + column index)), coarse levels (blocks x (36 w + 4)) and the
+ * coarsest dense inverse (entries x w), each plus its input and output
+ * vectors once; transfers and vector updates are not counted, the hierarchy
+ * refresh is excluded -- the summed wall time of those solves, the number of
+ * applies and of solves.  bytes / ms is a lower bound of the solve's HBM
+ * rate. */
+int uft_solve_traffic(const uft_ctx *ctx, double *bytes, double *ms, int64_t *applies, int64_t *solves);
 
 /* --- workload generator (synthetic cantilever) --------------------------- */
 int64_t uft_box_nodes(int nx, int ny, int nz);

@@ -205,3 +205,38 @@ def test_reference_main_compiles_against_dropin(tmp_path):
                     "-Wno-maybe-uninitialized", "-O1"] + [f"-I{d}" for d in inc] +
                    [str(src), "-o", str(tmp_path / "opt"), f"-L{libdir}", "-luft"], check=True)
     assert os.access(tmp_path / "opt", os.X_OK)
+
+
+def test_vtu_writer_host(U, tmp_path):
+    """uft_vtu_write_host (Write2Vtu.cpp:5-173) on a known mesh: the hex8
+    corners reordered to VTK's 0,1,3,2,4,5,7,6 (Write2Vtu.cpp:89-96), type 12
+    (10 for tet4), the fields in their positions, exact with 17 digits."""
+    import xml.etree.ElementTree as ET
+    from uft.capi import vtu_write_host, UFTError
+    for npe in (8, 4):
+        xyz, elem = U.gen_box(npe, 3, 2, 2, jitter=0.1, seed=5)
+        rng = np.random.default_rng(npe)
+        u = rng.standard_normal(3 * len(xyz))
+        x = rng.uniform(0.01, 1.0, len(elem))
+        path = tmp_path / f"t{npe}.vtu"
+        vtu_write_host(path, xyz, elem, u, x, digits=17)
+        root = ET.parse(path).getroot()
+        piece = root.find("UnstructuredGrid/Piece")
+        assert int(piece.get("NumberOfPoints")) == len(xyz) and int(piece.get("NumberOfCells")) == len(elem)
+        a = {d.get("Name"): np.array(d.text.split(), dtype=float) for d in root.iter("DataArray")}
+        assert np.array_equal(a["nodes"], xyz.ravel())
+        order = [0, 1, 3, 2, 4, 5, 7, 6] if npe == 8 else [0, 1, 2, 3]
+        assert np.array_equal(a["connectivity"].reshape(-1, npe), elem[:, order])
+        if npe == 8:  # element 0 of the box: corners (0,0,0),(1,0,0),(0,1,0),(1,1,0) in VTK order 0,1,3,2
+            nx1 = 4
+            assert list(a["connectivity"][:4]) == [0, 1, nx1 + 1, nx1]
+        assert np.array_equal(a["offsets"], npe * np.arange(1, len(elem) + 1))
+        assert np.all(a["types"] == (12 if npe == 8 else 10))
+        assert np.array_equal(a["ux"], u[0::3]) and np.array_equal(a["uy"], u[1::3]) and np.array_equal(a["uz"], u[2::3])
+        assert np.array_equal(a["xphys"], x)
+    vtu_write_host(tmp_path / "six.vtu", xyz, elem, u, x)  # the reference's setprecision(6)
+    six = open(tmp_path / "six.vtu").read()
+    assert f"{x[0]:.6e}" in six
+    with pytest.raises(UFTError) as e:
+        vtu_write_host(tmp_path / "no" / "such" / "dir.vtu", xyz, elem, u, x)
+    assert e.value.code == 2

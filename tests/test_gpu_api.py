@@ -156,7 +156,8 @@ def test_dropin_driver_runs(O, tmp_path):
         got = float(line.split("fx:")[1].split(",")[0])
         assert abs(got - f) <= 1e-6 * abs(f) + 1e-6  # the log line prints %f
     vtus = sorted(os.listdir(tmp_path))
-    assert "result_0.vtu" in vtus and "result_-1.vtu" in vtus
+    # the reference's names (Write2Vtu.cpp:35-46): itr 0 initial, -1 final
+    assert "output_initial.vtu" in vtus and "output_final.vtu" in vtus
     # the same run through the binary mesh cache: written on the first run,
     # read on the second, identical log
     cache = str(tmp_path / "box.uftc")
@@ -165,7 +166,7 @@ def test_dropin_driver_runs(O, tmp_path):
     assert all(r.returncode == 0 for r in runs) and os.path.exists(cache)
     logs = [[l.split("time:")[0] for l in r.stdout.splitlines() if l.startswith("It.:")] for r in runs]
     assert logs[0] == logs[1] == [l.split("time:")[0] for l in lines]
-    txt = open(tmp_path / "result_0.vtu").read()
+    txt = open(tmp_path / "output_initial.vtu").read()
     assert 'NumberOfCells="128"' in txt and 'Name="xphys"' in txt
 
 
@@ -202,3 +203,47 @@ def test_binary_cache_roundtrip(O, tmp_path):
     assert c.L.uft_cache_load(c.h, str(bad).encode()) == 2 and b"truncated" in c.L.uft_last_error(c.h)
     bad.write_bytes(b"NOTACACHE" + bytes(64))
     assert c.L.uft_cache_load(c.h, str(bad).encode()) == 2 and b"bad header" in c.L.uft_last_error(c.h)
+
+
+def _parse_vtu(path):
+    import xml.etree.ElementTree as ET
+    root = ET.parse(path).getroot()
+    piece = root.find("UnstructuredGrid/Piece")
+    arr = {}
+    for da in root.iter("DataArray"):
+        arr[da.get("Name")] = np.array(da.text.split(), dtype=float)
+    return int(piece.get("NumberOfPoints")), int(piece.get("NumberOfCells")), arr
+
+
+@pytest.mark.parametrize("npe", [8, 4])
+def test_write_vtu_from_device(O, tmp_path, npe):
+    """uft_write_vtu (Write2Vtu::WriteResults) from the device vectors after
+    a design iteration: points, hex corners in VTK order 0,1,3,2,4,5,7,6
+    (Write2Vtu.cpp:89-96), offsets, types 12/10, ux/uy/uz and xphys equal to
+    the context's vectors (17 digits round-trip exactly; the reference's 6
+    digits to 5e-7 relative)."""
+    from uft.capi import cantilever
+    print(f"[case] vtu npe={npe}", flush=True)
+    dims = (5, 3, 2)
+    ctx = cantilever(npe, *dims, jitter=0.15, seed=2, load_kind=0 if npe == 8 else 1, Emax=1.0, volfrac=0.3,
+                     rmin=0.5, rtol_el=1e-12, rtol_f=1e-12)
+    ctx.init_design()
+    ctx.design_iteration(1)
+    U, XP = ctx.get("U"), ctx.get("XPhys")
+    xyz, elem = O.gen_box(npe, *dims, jitter=0.15, seed=2)
+    for digits, tol in ((17, 0.0), (6, 5e-7)):
+        path = tmp_path / f"d{digits}.vtu"
+        ctx.write_vtu(path, digits=digits)
+        nn, ne, a = _parse_vtu(path)
+        assert (nn, ne) == (len(xyz), len(elem))
+        close = (lambda g, w: np.array_equal(g, w)) if tol == 0 else \
+            (lambda g, w: np.all(np.abs(g - w) <= tol * np.abs(w) + 1e-300))
+        assert close(a["nodes"], xyz.ravel())
+        order = [0, 1, 3, 2, 4, 5, 7, 6] if npe == 8 else [0, 1, 2, 3]
+        assert np.array_equal(a["connectivity"].reshape(ne, npe), elem[:, order])
+        assert np.array_equal(a["offsets"], npe * np.arange(1, ne + 1))
+        assert np.all(a["types"] == (12 if npe == 8 else 10))
+        for c, name in enumerate(("ux", "uy", "uz")):
+            assert close(a[name], U[c::3]), name
+        assert close(a["xphys"], XP)
+    ctx.close()

@@ -317,14 +317,16 @@ __global__ void k_bsc_el_alpha(int nseg, double *sc) {
   FOR_DESIGN(nseg)
   S[B_ALPHA] = S[B_ACT] != 0.0 ? S[B_RZ] / S[B_PQ] : 0.0;
 }
-// after the update: count the step and apply the stopping test
-// ||F - K U|| <= rtol ||F|| (a non-finite residual also stops the design; the
-// host reports it from the true residual)
+// after the update: count the step; once the recursive residual meets
+// ||r|| <= rtol ||F|| ask for the true residual F - K U (B_NEED), on which the
+// stopping test itself is taken (k_bsc_el_true; a non-finite residual also
+// stops the design, and the host reports it from the true residual)
 __global__ void k_bsc_el_check(int nseg, double *sc) {
   FOR_DESIGN(nseg)
+  S[B_NEED] = 0.0;
   if (S[B_ACT] == 0.0) return;
   S[B_IT] += 1.0;
-  if (!(sqrt(S[B_RR]) > S[B_TOL])) S[B_ACT] = 0.0;
+  if (!(sqrt(S[B_RR]) > S[B_TOL])) S[B_NEED] = 1.0;
 }
 // lower-precision outer product: count the step; ask for the FP64 residual
 // (B_NEED) when the recursive one has met the tolerance or fallen 100-fold

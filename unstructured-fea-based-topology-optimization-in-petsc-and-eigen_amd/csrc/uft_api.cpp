@@ -131,6 +131,32 @@ hipEvent_t *prof_triple(uft_ctx *C, int kind) {
 }
 
 bool is_mixed(const uft_ctx *C) { return C->P.precision != UFT_FP64; }
+
+// ---- algorithmic bytes of one operator apply: the stored operator once
+// (Ne (S w + 4 npe + 8) for element blocks with S unique entries of w bytes,
+// node ids and density; blocks x (block bytes + column index) for assembled
+// blocks) plus the input and output vectors once.
+double fine_bytes64(const uft_ctx *C) {
+  const int S = C->npe == 8 ? 300 : 78;
+  return (double)C->ne * (S * 8 + 4 * C->npe + 8) + 2.0 * (double)C->ndof * 8;
+}
+double fine_bytes_lo(const uft_ctx *C, int w) {
+  const int S = C->npe == 8 ? 300 : 78;
+  return (double)C->ne * (S * w + 4 * C->npe + 8 + (w == 2 ? 4 : 0)) + 2.0 * (double)C->ndof * 4;
+}
+double ell_bytes(const uft_ctx *C, int value_bytes, bool scaled) {
+  const int cb = C->fell.d_cd ? 2 : 4;
+  return (double)C->fell.nnzb * (value_bytes + cb) + (double)C->ndof * 4 * (scaled ? 3 : 2) + (double)C->nn;
+}
+double coarse_bytes(const uft_ctx *C, int l, int w) {
+  const AmgLevel &L = C->amg.L[l];
+  return (double)L.nblk * (36.0 * w + 4) + 2.0 * (double)L.n * (w == 8 ? 8 : 4) + (w == 2 ? 4.0 * L.n : 0.0);
+}
+inline void count_apply(uft_ctx *C, double bytes) {
+  if (!C->count_on) return;
+  C->solve_bytes += bytes;
+  C->solve_applies++;
+}
 // the assembled smoother blocks (int8 for one right-hand side, FP16 for the
 // multi-load-case form) exist
 bool fine_blocks(const uft_ctx *C) { return C->fell.d_q8 || C->fell.d_v8; }
@@ -182,6 +208,7 @@ int graph_launch(uft_ctx *C, const void *k0, const void *k1, const void *k2, F &
 
 
 void fine_apply(uft_ctx *C, const double *xphys, const double *x, double *y, const double *dotw, double *part) {
+  count_apply(C, fine_bytes64(C));
   uftk::apply_K(C->s, C, xphys, x, y, dotw, part, prof_triple(C, 0));
 }
 
@@ -196,6 +223,7 @@ void fine_apply_lo(uft_ctx *C, const double *xphys, const double *p, int slot, b
   op.mode = slot >= 0 ? ROW_STOREDOT : ROW_STORE;
   op.y = C->d_qf;
   op.part = C->d_part;
+  count_apply(C, C->fell.d_w8 ? ell_bytes(C, 36, false) : fine_bytes_lo(C, 4));
   if (ev) (void)hipEventRecord(ev[0], s);
   if (C->fell.d_w8) {  // assembled FP32 blocks of the current hierarchy refresh: one launch
     uftk::fine_ell_op32(s, C, C->d_pf, op);
@@ -212,10 +240,12 @@ void fine_apply_lo(uft_ctx *C, const double *xphys, const double *p, int slot, b
 // y = A_l x on multigrid level l
 void level_apply(uft_ctx *C, int l, const double *x, double *y) {
   AmgLevel &Lv = C->amg.L[l];
-  if (l == 0)
+  if (l == 0) {
     fine_apply(C, C->cur_xphys, x, y, nullptr, nullptr);
-  else
+  } else {
+    count_apply(C, coarse_bytes(C, l, 8));
     uftk::bsr6_mv(C->s, Lv.nunit, Lv.d_rp, Lv.d_ci, Lv.d_val, x, y);
+  }
 }
 
 // Chebyshev interval [lo, hi] * lambda_max(D^-1 A); PCGAMG's default is
@@ -266,6 +296,7 @@ const float *tr_P(const AmgLevel &L, float *) { return L.d_Pf; }
 // Operator apply with a per-row epilogue on level l (l = 0: matrix-free fine
 // operator of the current density; above: 6x6 block-sparse).
 void lev_op(uft_ctx *C, int l, const double *u, const RowOp<double> &op) {
+  count_apply(C, l == 0 ? fine_bytes64(C) : coarse_bytes(C, l, 8));
   if (l == 0) {
     hipEvent_t *ev = prof_triple(C, 0);
     if (ev) (void)hipEventRecord(ev[0], C->s);
@@ -279,6 +310,11 @@ void lev_op(uft_ctx *C, int l, const double *u, const RowOp<double> &op) {
   }
 }
 void lev_op(uft_ctx *C, int l, const float *u, const RowOp<float> &op) {
+  if (l == 0)
+    count_apply(C, fine_blocks(C) ? ell_bytes(C, C->fell.d_q8 ? 12 : 18, true)
+                                  : fine_bytes_lo(C, C->d_KEh ? 2 : 4));
+  else
+    count_apply(C, coarse_bytes(C, l, C->amg.L[l].d_valh ? 2 : 4));
   if (l == 0) {
     hipEvent_t *ev = prof_triple(C, 1);
     if (ev) (void)hipEventRecord(ev[0], C->s);
@@ -443,6 +479,17 @@ void coarse_direct(uft_ctx *C, int l, const T *r, T *x) {
   AmgLevel &Lv = H.L[l], &Ln = H.L[l + 1];
   LV<T> vn = lv(Ln, (T *)nullptr);
   const T *P = tr_P(Lv, (T *)nullptr);
+  {  // the dense inverse once (block diagonal in a batch: every design's block)
+    double ent = (double)H.ncoarse * H.ncoarse;
+    if (C->bt.nseg) {
+      ent = 0.0;
+      for (int d = 0; d < C->bt.nseg; ++d) {
+        const double nd = (double)(C->bt.uoff[H.nlev - 1][d + 1] - C->bt.uoff[H.nlev - 1][d]) * 6.0;
+        ent += nd * nd;
+      }
+    }
+    count_apply(C, ent * (C->bt.nseg || std::is_same<T, double>::value ? 8 : 4) + 2.0 * (double)H.ncoarse * sizeof(T));
+  }
   if (C->bt.nseg) {  // block-diagonal coarsest level: one workgroup per design
     uftk::tail_solve(C->s, (int)Ln.nunit, Lv.d_cptr, Lv.d_child, P, H.d_cinv, r, vn.x, C->bt.nseg, C->bt.d_cu,
                      C->bt.d_coff);
@@ -949,7 +996,19 @@ int estimate_lmax_batch(uft_ctx *C, int l, int ksteps) {
 }
 
 // Rebuild the density-dependent parts of the hierarchy for xphys.
+// operator applies of the refresh (eigenvalue estimates) are not solve traffic
+struct CountOff {
+  uft_ctx *C;
+  bool was;
+  explicit CountOff(uft_ctx *c) : C(c), was(c->count_on) { c->count_on = false; }
+  ~CountOff() { C->count_on = was; }
+};
+int amg_refresh_impl(uft_ctx *C, const double *xphys);
 int amg_refresh(uft_ctx *C, const double *xphys) {
+  CountOff off(C);
+  return amg_refresh_impl(C, xphys);
+}
+int amg_refresh_impl(uft_ctx *C, const double *xphys) {
   AmgHier &H = C->amg;
   hipStream_t s = C->s;
   C->cur_xphys = xphys;
@@ -1070,32 +1129,54 @@ int solve_el(uft_ctx *C, const double *xphys, double *U) {
     uftk::xpby(s, n, z, C->d_scal, S_BETA + 2, p);
     head();
   };
+  // true residual F - K U into q; returns its norm in *tn
+  auto true_residual = [&](double *tn) -> int {
+    fine_apply(C, xphys, U, q, nullptr, nullptr);
+    uftk::sub(s, n, C->d_F, q, q);
+    uftk::dot(s, n, q, q, C->d_part, C->d_scal + S_MISC);
+    const int e = fetch(C, C->d_scal + S_MISC, 1, C->h_scal);
+    *tn = std::sqrt(C->h_scal[0]);
+    return e;
+  };
   static const char tag = 0;
+  bool restart = false;
+  double tn = bn;
   while (it < C->P.maxit_el) {
-    if (it == 0)
+    if (it == 0) {
       head();
-    else if (!use_graph(C))
+    } else if (restart) {
+      // the recursive residual drifted from the true one: CG again from r = F - K U
+      uftk::copy(s, n, q, r);
+      vcycle(C, 0, r, z);
+      uftk::dot(s, n, r, z, C->d_part, C->d_scal + S_RZ);
+      uftk::copy(s, n, z, p);
+      head();
+      restart = false;
+    } else if (!use_graph(C)) {
       next();
-    else if ((rc = graph_launch(C, xphys, U, &tag, next)))
+    } else if ((rc = graph_launch(C, xphys, U, &tag, next))) {
       return rc;
+    }
     ++it;
     rc = fetch(C, C->d_scal + S_RR, 1, C->h_scal);
     if (rc) return rc;
     const double rn = std::sqrt(C->h_scal[0]);
     if (!std::isfinite(rn)) return uft_set_error(C, UFT_ERR_NOCONV, "elasticity CG diverged at iteration %d", it);
     if (rn <= tol) {
-      conv = true;
-      break;
+      // stop on the true residual (the reference's FGMRES is right-preconditioned, so its
+      // residual is the unpreconditioned true one, SolveFem.cpp:126-152)
+      if ((rc = true_residual(&tn))) return rc;
+      if (tn <= tol) {
+        conv = true;
+        break;
+      }
+      restart = true;
     }
   }
   C->it_el = it;
   // true residual (SolveFem.cpp:142-152)
-  fine_apply(C, xphys, U, q, nullptr, nullptr);
-  uftk::sub(s, n, C->d_F, q, q);
-  uftk::dot(s, n, q, q, C->d_part, C->d_scal + S_MISC);
-  rc = fetch(C, C->d_scal + S_MISC, 1, C->h_scal);
-  if (rc) return rc;
-  C->relres_el = std::sqrt(C->h_scal[0]) / bn;
+  if (!conv && (rc = true_residual(&tn))) return rc;
+  C->relres_el = tn / bn;
   if (!conv)
     return uft_set_error(C, UFT_ERR_NOCONV, "elasticity solve: %d iterations, relative residual %.3e > %.3e", it,
                          C->relres_el, C->P.rtol_el);
@@ -1214,7 +1295,6 @@ int solve_el_mixed(uft_ctx *C, const double *xphys, double *U) {
   C->relres_el = std::sqrt(C->h_scal[0]) / bn;
   return uft_set_error(C, UFT_ERR_NOCONV, "mixed elasticity solve: %d iterations, relative residual %.3e", it,
                        C->relres_el);
-  return UFT_OK;
 }
 
 // Multi-load-case solve: one flexible CG per right-hand side, all advanced
@@ -1445,8 +1525,8 @@ int solve_el_batch(uft_ctx *C, const double *xphys, double *U) {
     ++it;
     if ((rc = fetch_batch(C, B_ACT, &any))) return rc;
     bool need = false;
-    for (int d = 0; f32 && d < ns; ++d) need |= B.h[(size_t)d * UFT_BSC + B_NEED] != 0.0;
-    if (need) {  // reliable update of the designs that asked: r = F - K U in FP64
+    for (int d = 0; d < ns; ++d) need |= B.h[(size_t)d * UFT_BSC + B_NEED] != 0.0;
+    if (need) {  // true residual of the designs that asked (FP64: the stopping test; mixed: a reliable update)
       fine_apply(C, xphys, U, q, nullptr, nullptr);
       uftk::seg_true_res(s, g, df, ns, n, B.sc, C->d_F, q, r, C->d_part, B_RR);
       uftk::bsc_el_true(s, ns, B.sc);
@@ -1739,7 +1819,7 @@ const char *uft_last_error(const uft_ctx *C) {
 
 void uft_default_params(uft_params *p) {
   std::memset(p, 0, sizeof(*p));
-  // OptPara::SetUp (OptPara.cpp:123-146)
+  // OptPara::SetUp (OptPara.cpp:61-84)
   p->maxItr = 200; p->rmin = 0.01; p->penal = 3.0; p->Emin = 1.0e-9; p->Emax = 1.0e6; p->movlim = 0.2;
   p->Xmin = 0.01; p->Xmax = 1.0; p->volfrac = 0.12; p->x0 = 0.5; p->beta0 = 1.0; p->betamax = 8.0; p->proj = 0;
   p->m = 1;
@@ -2329,12 +2409,16 @@ int uft_fem_obj_sens(uft_ctx *C, double *fx, double *gx, int dfdx_id, int m, con
   const double *xp = C->vec[xphys_id];
   double *U = C->vec[UFT_U];
   const double t0 = now_ms();
+  C->count_on = true;
   const int src = C->bt.nseg  ? solve_el_batch(C, xp, U)
                   : C->mr.nr  ? solve_el_block(C, xp)
                   : is_mixed(C) ? solve_el_mixed(C, xp, U)
                                 : solve_el(C, xp, U);
+  C->count_on = false;
   const double t1 = now_ms();
   C->t_phase[1] = t1 - t0 - C->t_phase[0];
+  C->solve_ms += C->t_phase[1];
+  C->solve_count++;
   if (src != UFT_OK && src != UFT_ERR_NOCONV) return src;
   if (C->bt.nseg) {  // per design: compliance, volume constraint (SolveFem.cpp:208-249)
     BatchState &B = C->bt;
@@ -2706,6 +2790,8 @@ extern "C" int uft_profile_enable(uft_ctx *C, int on) {
   C->prof_every = on > 0 ? on : 1;
   C->prof_tick[0] = C->prof_tick[1] = C->prof_tick[2] = 0;
   C->evused = 0;
+  C->solve_bytes = C->solve_ms = 0.0;
+  C->solve_applies = C->solve_count = 0;
   for (int k = 0; k < 6; ++k) {
     C->prof_ms[k] = 0.0;
     C->prof_n[k] = 0;
@@ -2741,6 +2827,15 @@ extern "C" int uft_solver_info(const uft_ctx *C, int *nlev, int64_t *n_per_level
     if (n_per_level) n_per_level[l] = C->amg.L[l].n;
     if (blocks_per_level) blocks_per_level[l] = l == 0 ? C->ne : C->amg.L[l].nblk;
   }
+  return UFT_OK;
+}
+
+extern "C" int uft_solve_traffic(const uft_ctx *C, double *bytes, double *ms, int64_t *applies, int64_t *solves) {
+  if (!C) return UFT_ERR_ARG;
+  if (bytes) *bytes = C->solve_bytes;
+  if (ms) *ms = C->solve_ms;
+  if (applies) *applies = C->solve_applies;
+  if (solves) *solves = C->solve_count;
   return UFT_OK;
 }
 

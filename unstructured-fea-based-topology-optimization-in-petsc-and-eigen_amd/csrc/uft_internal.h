@@ -293,6 +293,12 @@ struct uft_ctx {
   double fnorm = 0.0;
   double *d_lanczos = nullptr;       // CG scalar log for eigenvalue estimates
 
+  // uft_solve_traffic: algorithmic bytes of the operator applies made inside
+  // the elasticity solves (hierarchy refresh excluded) and the solves' time
+  bool count_on = false;
+  double solve_bytes = 0.0, solve_ms = 0.0;
+  int64_t solve_applies = 0, solve_count = 0;
+
   // device timers (uft_profile_*): event triples around fine applies
   bool prof = false;
   int prof_every = 1;   // time one in every prof_every fine applies of each kind

@@ -426,55 +426,27 @@ inline PetscErrorCode OptPara::SetMMA(PetscInt * /*itr*/, MMA **mma) {
 }
 
 // ------------------------------------------------------------------ Write2Vtu
-// ASCII VTU of the mesh with point data ux/uy/uz and cell data xphys
-// (Write2Vtu.cpp:5-173: hex connectivity reordered 0,1,3,2,4,5,7,6, cell type 12;
-// tet4 cells are type 10).
+// Write2Vtu::WriteResults (Write2Vtu.cpp:5-173) through uft_write_vtu: the
+// reference's file names (output_initial.vtu for itr 0, output_final.vtu for
+// -1, output_<itr>.vtu otherwise, Write2Vtu.cpp:35-46) in the -vtu_dir
+// directory, 6 digits like the reference (-vtu_digits to change); nothing is
+// written without -vtu_dir.
 class Write2Vtu {
  public:
-  void WriteResults(MeshRef, MeshRef, Vec XPhys, Vec U, PetscInt itr) {
-    uft_ctx *c = UftRuntime::get().ctx;
-    const std::string dir = UftRuntime::get().str("-vtu_dir", "");
-    if (dir.empty()) return;
-    WriteFile(c, dir, XPhys, U, itr);
-  }
-  static int WriteFile(uft_ctx *c, const std::string &dir, Vec XPhys, Vec U, PetscInt itr) {
-    const long long ne = uft_size(c, UFT_SIZE_NE), nn = uft_size(c, UFT_SIZE_NN);
-    const int npe = (int)uft_size(c, UFT_SIZE_NPE);
-    std::vector<int32_t> el(ne * npe);
-    std::vector<double> xyz(nn * 3), u(nn * 3), x(ne);
-    int rc = uft_mesh_get(c, el.data(), xyz.data());
-    if (!rc) rc = uft_vec_get(c, U.id, u.data(), nn * 3);
-    if (!rc) rc = uft_vec_get(c, XPhys.id, x.data(), ne);
-    if (rc) return rc;
-    char name[4096];
-    std::snprintf(name, sizeof(name), "%s/result_%d.vtu", dir.c_str(), (int)itr);
-    FILE *f = std::fopen(name, "w");
-    if (!f) return UFT_ERR_IO;
-    std::fprintf(f, "<?xml version=\"1.0\"?>\n<VTKFile type=\"UnstructuredGrid\" version=\"0.1\" byte_order=\"LittleEndian\">\n");
-    std::fprintf(f, "<UnstructuredGrid>\n<Piece NumberOfPoints=\"%lld\" NumberOfCells=\"%lld\">\n", nn, ne);
-    std::fprintf(f, "<Points>\n<DataArray type=\"Float64\" NumberOfComponents=\"3\" format=\"ascii\">\n");
-    for (long long i = 0; i < nn; ++i) std::fprintf(f, "%.9g %.9g %.9g\n", xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
-    std::fprintf(f, "</DataArray>\n</Points>\n<Cells>\n<DataArray type=\"Int32\" Name=\"connectivity\" format=\"ascii\">\n");
-    static const int hexorder[8] = {0, 1, 3, 2, 4, 5, 7, 6};
-    for (long long e = 0; e < ne; ++e) {
-      for (int a = 0; a < npe; ++a) std::fprintf(f, "%d ", el[e * npe + (npe == 8 ? hexorder[a] : a)]);
-      std::fprintf(f, "\n");
-    }
-    std::fprintf(f, "</DataArray>\n<DataArray type=\"Int32\" Name=\"offsets\" format=\"ascii\">\n");
-    for (long long e = 0; e < ne; ++e) std::fprintf(f, "%lld\n", (e + 1) * npe);
-    std::fprintf(f, "</DataArray>\n<DataArray type=\"UInt8\" Name=\"types\" format=\"ascii\">\n");
-    for (long long e = 0; e < ne; ++e) std::fprintf(f, "%d\n", npe == 8 ? 12 : 10);
-    std::fprintf(f, "</DataArray>\n</Cells>\n<PointData>\n");
-    const char *nm[3] = {"ux", "uy", "uz"};
-    for (int cmp = 0; cmp < 3; ++cmp) {
-      std::fprintf(f, "<DataArray type=\"Float64\" Name=\"%s\" format=\"ascii\">\n", nm[cmp]);
-      for (long long i = 0; i < nn; ++i) std::fprintf(f, "%.9g\n", u[3 * i + cmp]);
-      std::fprintf(f, "</DataArray>\n");
-    }
-    std::fprintf(f, "</PointData>\n<CellData>\n<DataArray type=\"Float64\" Name=\"xphys\" format=\"ascii\">\n");
-    for (long long e = 0; e < ne; ++e) std::fprintf(f, "%.9g\n", x[e]);
-    std::fprintf(f, "</DataArray>\n</CellData>\n</Piece>\n</UnstructuredGrid>\n</VTKFile>\n");
-    std::fclose(f);
-    return 0;
+  PetscErrorCode WriteResults(MeshRef, MeshRef, Vec XPhys, Vec U, PetscInt itr) {
+    UftRuntime &R = UftRuntime::get();
+    const std::string dir = R.str("-vtu_dir", "");
+    if (dir.empty()) return 0;
+    char name[64];
+    if (itr == 0)
+      std::snprintf(name, sizeof(name), "output_initial.vtu");
+    else if (itr == -1)
+      std::snprintf(name, sizeof(name), "output_final.vtu");
+    else
+      std::snprintf(name, sizeof(name), "output_%d.vtu", (int)itr);
+    const PetscErrorCode e =
+        uft_write_vtu(R.ctx, (dir + "/" + name).c_str(), XPhys.id, U.id, R.integer("-vtu_digits", 6));
+    if (e) std::fprintf(stderr, "Write2Vtu: %s\n", UftLastError());
+    return e;
   }
 };

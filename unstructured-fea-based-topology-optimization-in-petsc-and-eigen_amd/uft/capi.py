@@ -129,6 +129,9 @@ def lib():
         "uft_mma_design_change_noalloc": (C.c_int, [C.c_int, vp, vp, C.c_int64, vp, vp]),
         "uft_mesh_read_text": (C.c_int, [C.c_int, C.c_int64, C.c_int64, C.c_char_p, C.c_char_p, dp, ip]),
         "uft_bc_read_text": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, ip, ip, dp]),
+        "uft_write_vtu": (C.c_int, [vp, C.c_char_p, C.c_int, C.c_int, C.c_int]),
+        "uft_solve_traffic": (C.c_int, [vp, pd, pd, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+        "uft_vtu_write_host": (C.c_int, [C.c_char_p, C.c_int, C.c_int64, C.c_int64, dp, ip, dp, dp, C.c_int]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -184,6 +187,16 @@ def bc_read_text(dirpath, nfixed, nlc, nload):
     if rc:
         raise UFTError(rc, lib().uft_last_error(None).decode())
     return fixed, ld.reshape(nlc, nload), lv.reshape(nlc, nload)
+
+
+def vtu_write_host(path, xyz, elem, u, xphys, digits=6):
+    """Host-only VTU writer (uft_vtu_write_host)."""
+    f = lambda a, t: np.ascontiguousarray(a, dtype=t).ravel()
+    elem = np.asarray(elem)
+    rc = lib().uft_vtu_write_host(os.fsencode(path), elem.shape[1], len(xyz), len(elem), f(xyz, np.float64),
+                                  f(elem, np.int32), f(u, np.float64), f(xphys, np.float64), digits)
+    if rc:
+        raise UFTError(rc, lib().uft_last_error(None).decode())
 
 
 def mesh_read_text(npe, nn, ne, vertex_path, element_path):
@@ -381,6 +394,11 @@ class Context:
                                               C.byref(n2), C.byref(ni)), "uft_mma_kkt_residual")
         return n2.value, ni.value
 
+    def write_vtu(self, path, xphys="XPhys", u="U", digits=6):
+        """Write2Vtu::WriteResults of this context (uft_write_vtu)."""
+        self._chk(self.L.uft_write_vtu(self.h, os.fsencode(path), self.vid(xphys), self.vid(u), digits),
+                  "uft_write_vtu")
+
     def stress(self, xphys="XPhys"):
         self._chk(self.L.uft_stress(self.h, self.vid(xphys), VEC["VMS"]), "uft_stress")
         return self.get("VMS")
@@ -504,6 +522,15 @@ class Context:
         y = np.zeros_like(x)
         self._chk(self.L.uft_level_apply(self.h, level, kind, x, y), "uft_level_apply")
         return y
+
+    def solve_traffic(self):
+        """(algorithmic bytes of the solves' operator applies, solve ms, applies,
+        solves) since the last uft_profile_enable (uft_solve_traffic)."""
+        b, ms = C.c_double(), C.c_double()
+        a, n = C.c_int64(), C.c_int64()
+        self._chk(self.L.uft_solve_traffic(self.h, C.byref(b), C.byref(ms), C.byref(a), C.byref(n)),
+                  "uft_solve_traffic")
+        return b.value, ms.value, a.value, n.value
 
     def phase_times(self):
         ms = (C.c_double * 8)()
