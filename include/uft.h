@@ -90,6 +90,15 @@ typedef struct uft_params {
                           (FP16 with several load cases) and FP32 for the outer product (one
                           launch per apply, no element-local round trip); 0: the element blocks.  Fixed at
                           setup. */
+  int coarse_dense_max; /* the coarsest multigrid level is solved with its dense inverse, so coarsening goes on
+                           until a level has at most this many unknowns (default 3072, at most 6144).  Fixed
+                           at setup. */
+  int prolong_smooth;  /* 1 (default): smoothed aggregation on the fine level as PCGAMG does (SolveFem.cpp:129-136):
+                          at every hierarchy refresh the tentative prolongator is smoothed once with the current
+                          operator, P = (I - omega D^-1 A) P_tent with omega = 4 / (3 lambda_max(D^-1 A)), and the
+                          level-1 operator is P^T A P; used for one design with superposed load cases, the
+                          assembled node pattern (fine_assembled = 1) and at least three levels, otherwise the
+                          tentative prolongator.  0: the tentative prolongator.  Fixed at setup. */
 } uft_params;
 
 /* --- context ------------------------------------------------------------ */
@@ -285,6 +294,13 @@ int uft_stress(uft_ctx *ctx, int xphys_id, int vms_id);
  * above; the operator of level l+1 is P^T A_l P). */
 int uft_amg_operator_get(uft_ctx *ctx, int level, int32_t *rp, int32_t *ci, double *val);
 int uft_amg_transfer_get(uft_ctx *ctx, int level, int32_t *parent, double *P);
+/* With prolong_smooth: the smoothed prolongator of level 0 as of the last
+ * refresh (the transfer the cycle uses; uft_amg_transfer_get then returns the
+ * tentative one it was smoothed from): *nent entries, row pointers pptr
+ * (nodes + 1) and aggregate ids pagg per entry, P with a row-major 3 x 6
+ * block per entry, and omega.  *nent = 0 without it.  Any of pptr, pagg, P,
+ * omega may be NULL (call with them NULL first to size the arrays). */
+int uft_amg_prolongator_get(uft_ctx *ctx, int64_t *nent, int32_t *pptr, int32_t *pagg, double *P, double *omega);
 /* The dense inverse of the coarsest operator used by the coarse solve
  * (row-major n x n, n = unknowns of the last level). */
 int uft_amg_coarse_inverse_get(uft_ctx *ctx, double *inv);

@@ -379,19 +379,20 @@ __global__ __launch_bounds__(TPB) void k_fine_ell_op32(int nn, const int64_t *__
 // stores FP16, or int8 with a block scale when q8 is given (and, with w8, the
 // unscaled entries in FP32).  The element blocks are read from their FP32 copy and the
 // moduli E(x_e) from the refresh's per-element array.
-template <int NPE>
+template <int NPE, typename KT>
 __global__ __launch_bounds__(TPB) void k_fine_ell_fill(int nn, int ne, const int64_t *__restrict__ off,
                                                        const int32_t *__restrict__ width,
                                                        const int32_t *__restrict__ col,
                                                        const int32_t *__restrict__ cptr,
                                                        const int32_t *__restrict__ clist,
-                                                       const float2 *__restrict__ KEf,
+                                                       const KT *__restrict__ KEf,
                                                        const double *__restrict__ Ee,
                                                        const uint8_t *__restrict__ fixm,
                                                        const double *__restrict__ dinv, uint4 *__restrict__ v8,
                                                        uint16_t *__restrict__ v1, uint2 *__restrict__ q8,
                                                        uint32_t *__restrict__ qs, float *__restrict__ dg,
-                                                       float4 *__restrict__ w8, float *__restrict__ w1) {
+                                                       float4 *__restrict__ w8, float *__restrict__ w1,
+                                                       double *__restrict__ a64) {
   constexpr int ND = 3 * NPE;
   const int n = blockIdx.x * TPB + threadIdx.x;
   const int s = blockIdx.y;
@@ -412,7 +413,7 @@ __global__ __launch_bounds__(TPB) void k_fine_ell_fill(int nn, int ne, const int
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
         const int p = pk_index(c * NPE + li, d * NPE + lj, ND);
-        const float2 pr = KEf[(size_t)(p >> 1) * ne + e];
+        const KT pr = KEf[(size_t)(p >> 1) * ne + e];
         b[3 * c + d] += E * (double)((p & 1) ? pr.y : pr.x);
       }
   }
@@ -422,6 +423,11 @@ __global__ __launch_bounds__(TPB) void k_fine_ell_fill(int nn, int ne, const int
 #pragma unroll
     for (int d = 0; d < 3; ++d)
       if ((mj >> d) & 1u) b[3 * c + d] = 0.0;
+  if (a64) {
+#pragma unroll
+    for (int t = 0; t < 9; ++t) a64[9 * k + t] = b[t];
+  }
+  if (!q8 && !v8) return;  // FP64 blocks only (the smoothed prolongator's operator)
   if (w8) {
     w8[2 * k] = make_float4((float)b[0], (float)b[1], (float)b[2], (float)b[3]);
     w8[2 * k + 1] = make_float4((float)b[4], (float)b[5], (float)b[6], (float)b[7]);
@@ -925,19 +931,35 @@ void gather_op(hipStream_t s, const uft_ctx *C, const float *u, const float *Ye,
 }
 int gather_op_blocks(int64_t nn) { return (int)nblk(nn); }
 
-void fine_ell_fill(hipStream_t s, const uft_ctx *C, const double *Ee, const double *dinv) {
+// the mixed forms' blocks (from the FP32 element blocks) and, when a64 is
+// given, the FP64 blocks too; without the mixed forms' blocks only the FP64
+// ones, from the FP64 element blocks
+void fine_ell_fill(hipStream_t s, const uft_ctx *C, const double *Ee, const double *dinv, double *a64) {
   const int nn = (int)C->nn, ne = (int)C->ne;
   const auto &F = C->fell;
   const dim3 g(nblk(nn), (unsigned)F.wmax);
-  const float2 *KEf = (const float2 *)C->d_KEf;
+  if (F.d_q8 || F.d_v8) {
+    const float2 *KEf = (const float2 *)C->d_KEf;
+    if (C->npe == 8)
+      k_fine_ell_fill<8, float2><<<g, TPB, 0, s>>>(nn, ne, F.d_off, F.d_width, F.d_col, F.d_cptr, F.d_clist, KEf, Ee,
+                                                   C->d_fixm, dinv, F.d_v8, F.d_v1, F.d_q8, F.d_qs, F.d_dg, F.d_w8,
+                                                   F.d_w1, a64);
+    else
+      k_fine_ell_fill<4, float2><<<g, TPB, 0, s>>>(nn, ne, F.d_off, F.d_width, F.d_col, F.d_cptr, F.d_clist, KEf, Ee,
+                                                   C->d_fixm, dinv, F.d_v8, F.d_v1, F.d_q8, F.d_qs, F.d_dg, F.d_w8,
+                                                   F.d_w1, a64);
+    return;
+  }
+  if (!a64) return;
+  const double2 *KE = (const double2 *)C->d_KE;
   if (C->npe == 8)
-    k_fine_ell_fill<8><<<g, TPB, 0, s>>>(nn, ne, F.d_off, F.d_width, F.d_col, F.d_cptr, F.d_clist,
-                                         KEf, Ee, C->d_fixm, dinv, F.d_v8, F.d_v1, F.d_q8, F.d_qs, F.d_dg, F.d_w8,
-                                         F.d_w1);
+    k_fine_ell_fill<8, double2><<<g, TPB, 0, s>>>(nn, ne, F.d_off, F.d_width, F.d_col, F.d_cptr, F.d_clist, KE, Ee,
+                                                  C->d_fixm, dinv, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                                                  nullptr, a64);
   else
-    k_fine_ell_fill<4><<<g, TPB, 0, s>>>(nn, ne, F.d_off, F.d_width, F.d_col, F.d_cptr, F.d_clist,
-                                         KEf, Ee, C->d_fixm, dinv, F.d_v8, F.d_v1, F.d_q8, F.d_qs, F.d_dg, F.d_w8,
-                                         F.d_w1);
+    k_fine_ell_fill<4, double2><<<g, TPB, 0, s>>>(nn, ne, F.d_off, F.d_width, F.d_col, F.d_cptr, F.d_clist, KE, Ee,
+                                                  C->d_fixm, dinv, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                                                  nullptr, a64);
 }
 void fine_ell_op32(hipStream_t s, const uft_ctx *C, const float *u, const RowOp<float> &op) {
   const auto &F = C->fell;

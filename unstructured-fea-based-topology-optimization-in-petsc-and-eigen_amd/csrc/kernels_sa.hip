@@ -1,0 +1,412 @@
+// gfx950 kernels of the smoothed-aggregation prolongator of level 0.
+//
+// PCGAMG (SolveFem.cpp:129-136) smooths its tentative prolongator once with
+// the current operator: P = (I - omega D^-1 A) P_tent, omega = 4 / (3
+// lambda_max(D^-1 A)), and forms the coarse operator P^T A P; the reference
+// rebuilds it at every design iteration (SolveFem.cpp:260-262).  Here, at
+// every hierarchy refresh:
+//   k_sa_prolongator  W = A P_tent per (node, aggregate) entry of P, then the
+//                     smoothed entry (rows of fixed DOFs zero), stored for the
+//                     products below and in the cycle's two layouts
+//   k_sa_ap           V = A P per (node, aggregate) entry of A P
+//   k_sa_galerkin     A_1[I][J] = sum over the support nodes i of I of
+//                     P_i[I]^T V_i[J], one wavefront per upper block, lanes
+//                     over its contribution pairs, a fixed butterfly combine;
+//                     the lower block is written transposed, diagonal blocks
+//                     symmetrised
+// Each runs over contribution lists built once at setup from the patterns
+// (k_sa_scan_*), so the refresh does no searching.  A is the fine operator
+// assembled on the node pattern (FP64 3x3 blocks, fixed columns zeroed; a
+// fixed row of A is the identity row, so its rows of A P_tent, P and A P
+// vanish because P_tent vanishes there).  Every sum runs in a fixed order:
+// results do not depend on scheduling.
+// The cycle's transfers between levels 0 and 1 use P through its supports
+// (restriction, fused with level 1's first Chebyshev step) and its rows
+// (prolongation with the K-cycle's combination of level 1).
+#include <hip/hip_runtime.h>
+
+#include "uft_internal.h"
+
+namespace {
+
+constexpr int TPB = 256;
+inline unsigned nblk(int64_t n) { return (unsigned)((n + TPB - 1) / TPB); }
+
+// slot q of node n on the sliced-ELL node pattern
+__device__ __forceinline__ int64_t ell_slot(const int64_t *off, int n, int q) {
+  return off[n / UFT_ELL_SLICE] + (int64_t)q * UFT_ELL_SLICE + n % UFT_ELL_SLICE;
+}
+
+// acc (3x6) += A (3x3, row-major) * B (3x6, row-major)
+__device__ __forceinline__ void mac33x36(double (&acc)[18], const double *__restrict__ A,
+                                         const double *__restrict__ B) {
+  double a[9], b[18];
+#pragma unroll
+  for (int t = 0; t < 9; ++t) a[t] = A[t];
+#pragma unroll
+  for (int t = 0; t < 18; ++t) b[t] = B[t];
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int m = 0; m < 6; ++m) acc[6 * c + m] += a[3 * c] * b[m] + a[3 * c + 1] * b[6 + m] + a[3 * c + 2] * b[12 + m];
+}
+
+// ---------------------------------------------------------------- setup: lists
+// which 0, per prolongator entry t = (i, K): the slots of row i whose column
+// lies in K, in slot order
+// which 1, per entry t = (i, J) of A P: (slot of row i, the column's
+// prolongator entry naming J), in slot order
+// which 2, per upper block (I, J): (prolongator entry of I at support node i,
+// entry of A P of node i naming J), in support (node) order
+// count: a = per-row counts; fill: a, b = the lists at the rows' offsets ptr
+__global__ __launch_bounds__(TPB) void k_sa_scan_w(int64_t np, const int32_t *__restrict__ pnode,
+                                                   const int32_t *__restrict__ pagg, const int64_t *__restrict__ off,
+                                                   const int32_t *__restrict__ width, const int32_t *__restrict__ col,
+                                                   const int32_t *__restrict__ cptr, const int32_t *__restrict__ parent,
+                                                   const int32_t *__restrict__ ptr, int32_t *__restrict__ a) {
+  const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  if (t >= np) return;
+  const int i = pnode[t], K = pagg[t];
+  const int w = width[i / UFT_ELL_SLICE];
+  int c = 0;
+  for (int q = 0; q < w; ++q) {
+    const int64_t k = ell_slot(off, i, q);
+    if (cptr[k + 1] == cptr[k] || parent[col[k]] != K) continue;
+    if (ptr) a[ptr[t] + c] = (int32_t)k;
+    ++c;
+  }
+  if (!ptr) a[t] = c;
+}
+
+__global__ __launch_bounds__(TPB) void k_sa_scan_v(int64_t nv, const int32_t *__restrict__ vnode,
+                                                   const int32_t *__restrict__ vagg, const int64_t *__restrict__ off,
+                                                   const int32_t *__restrict__ width, const int32_t *__restrict__ col,
+                                                   const int32_t *__restrict__ cptr, const int32_t *__restrict__ pptr,
+                                                   const int32_t *__restrict__ pagg, const int32_t *__restrict__ ptr,
+                                                   int32_t *__restrict__ a, int32_t *__restrict__ b) {
+  const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  if (t >= nv) return;
+  const int i = vnode[t], J = vagg[t];
+  const int w = width[i / UFT_ELL_SLICE];
+  int c = 0;
+  for (int q = 0; q < w; ++q) {
+    const int64_t k = ell_slot(off, i, q);
+    if (cptr[k + 1] == cptr[k]) continue;
+    const int j = col[k];
+    for (int e = pptr[j]; e < pptr[j + 1]; ++e) {
+      const int g = pagg[e];
+      if (g < J) continue;
+      if (g == J) {
+        if (ptr) {
+          a[ptr[t] + c] = (int32_t)k;
+          b[ptr[t] + c] = e;
+        }
+        ++c;
+      }
+      break;
+    }
+  }
+  if (!ptr) a[t] = c;
+}
+
+__global__ __launch_bounds__(TPB) void k_sa_scan_g(int64_t nub, const int32_t *__restrict__ ublk,
+                                                   const int32_t *__restrict__ brow, const int32_t *__restrict__ bci,
+                                                   const int32_t *__restrict__ sptr, const int32_t *__restrict__ snode,
+                                                   const int32_t *__restrict__ sidx, const int32_t *__restrict__ vptr,
+                                                   const int32_t *__restrict__ vagg, const int32_t *__restrict__ ptr,
+                                                   int32_t *__restrict__ a, int32_t *__restrict__ b) {
+  const int64_t u = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  if (u >= nub) return;
+  const int bk = ublk[u];
+  const int I = brow[bk], J = bci[bk];
+  int c = 0;
+  for (int q = sptr[I]; q < sptr[I + 1]; ++q) {
+    const int i = snode[q];
+    int lo = vptr[i], hi = vptr[i + 1];
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (vagg[mid] < J) lo = mid + 1;
+      else hi = mid;
+    }
+    if (lo >= vptr[i + 1] || vagg[lo] != J) continue;
+    if (ptr) {
+      a[ptr[u] + c] = sidx[q];
+      b[ptr[u] + c] = lo;
+    }
+    ++c;
+  }
+  if (!ptr) a[u] = c;
+}
+
+// ---------------------------------------------------------------- refresh
+// one thread per prolongator entry t = (node i, aggregate K):
+// W = sum over its slots of A_slot P_tent(col); P[t] = [K == parent(i)]
+// P_tent(i) - omega D^-1 W, stored FP64 (entry order), and in the cycle's
+// precision T as the node-sliced copy and the support-ordered copy
+template <typename T>
+__global__ __launch_bounds__(TPB) void k_sa_prolongator(int64_t np, const int32_t *__restrict__ pnode,
+                                                        const int32_t *__restrict__ pagg,
+                                                        const int32_t *__restrict__ pptr,
+                                                        const int32_t *__restrict__ wptr,
+                                                        const int32_t *__restrict__ wslot,
+                                                        const int32_t *__restrict__ col,
+                                                        const double *__restrict__ a64,
+                                                        const int32_t *__restrict__ parent,
+                                                        const double *__restrict__ Pt,
+                                                        const double *__restrict__ dinv,
+                                                        const uint8_t *__restrict__ fixm, double omega,
+                                                        const int64_t *__restrict__ poff,
+                                                        const int32_t *__restrict__ spos, double *__restrict__ P,
+                                                        T *__restrict__ Pe, T *__restrict__ Ps) {
+  const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  if (t >= np) return;
+  const int i = pnode[t];
+  const int K = pagg[t];
+  double acc[18];
+#pragma unroll
+  for (int m = 0; m < 18; ++m) acc[m] = 0.0;
+  for (int q = wptr[t]; q < wptr[t + 1]; ++q) {
+    const int64_t k = wslot[q];
+    mac33x36(acc, a64 + 9 * k, Pt + 18 * (size_t)col[k]);
+  }
+  const bool own = parent[i] == K;
+  const unsigned fm = fixm[i];
+  double out[18];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const double di = dinv[3 * (size_t)i + c];
+#pragma unroll
+    for (int m = 0; m < 6; ++m) {
+      const double pt = own ? Pt[18 * (size_t)i + 6 * c + m] : 0.0;
+      out[6 * c + m] = ((fm >> c) & 1u) ? 0.0 : pt - omega * di * acc[6 * c + m];
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < 18; ++m) P[18 * t + m] = out[m];
+  const int q = (int)(t - pptr[i]);
+  T *pe = Pe + poff[i / UFT_ELL_SLICE] * 18 + (int64_t)q * 18 * UFT_ELL_SLICE + i % UFT_ELL_SLICE;
+  T *ps = Ps + (size_t)spos[t] * 18;
+#pragma unroll
+  for (int m = 0; m < 18; ++m) {
+    pe[m * UFT_ELL_SLICE] = (T)out[m];
+    ps[m] = (T)out[m];
+  }
+}
+
+// one thread per entry t of A P: the sum over its (slot, prolongator entry)
+// pairs of A_slot P_entry (rows of fixed DOFs zero)
+__global__ __launch_bounds__(TPB) void k_sa_ap(int64_t nv, const int32_t *__restrict__ vnode,
+                                               const int32_t *__restrict__ vlptr, const int32_t *__restrict__ vlslot,
+                                               const int32_t *__restrict__ vlent, const double *__restrict__ a64,
+                                               const double *__restrict__ P, const uint8_t *__restrict__ fixm,
+                                               double *__restrict__ V) {
+  const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  if (t >= nv) return;
+  double acc[18];
+#pragma unroll
+  for (int m = 0; m < 18; ++m) acc[m] = 0.0;
+  for (int q = vlptr[t]; q < vlptr[t + 1]; ++q) mac33x36(acc, a64 + 9 * (size_t)vlslot[q], P + 18 * (size_t)vlent[q]);
+  const unsigned fm = fixm[vnode[t]];
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int m = 0; m < 6; ++m) V[18 * t + 6 * c + m] = ((fm >> c) & 1u) ? 0.0 : acc[6 * c + m];
+}
+
+// one wavefront per upper block b = (I, J), I <= J, of the level-1 pattern
+__global__ __launch_bounds__(TPB) void k_sa_galerkin(int64_t nub, const int32_t *__restrict__ ublk,
+                                                     const int32_t *__restrict__ brow, const int32_t *__restrict__ bci,
+                                                     const int32_t *__restrict__ btr, const int32_t *__restrict__ gptr,
+                                                     const int32_t *__restrict__ gp, const int32_t *__restrict__ gv,
+                                                     const double *__restrict__ P, const double *__restrict__ V,
+                                                     double *__restrict__ val) {
+  const int64_t wv = ((int64_t)blockIdx.x * TPB + threadIdx.x) / 64;
+  const int lane = threadIdx.x & 63;
+  if (wv >= nub) return;  // uniform per wavefront
+  const int b = ublk[wv];
+  double acc[36];
+#pragma unroll
+  for (int e = 0; e < 36; ++e) acc[e] = 0.0;
+  for (int q = gptr[wv] + lane; q < gptr[wv + 1]; q += 64) {
+    const double *p = P + 18 * (size_t)gp[q];
+    const double *v = V + 18 * (size_t)gv[q];
+    double pr[18], vr[18];
+#pragma unroll
+    for (int m = 0; m < 18; ++m) {
+      pr[m] = p[m];
+      vr[m] = v[m];
+    }
+#pragma unroll
+    for (int a = 0; a < 6; ++a)
+#pragma unroll
+      for (int bb = 0; bb < 6; ++bb)
+        acc[6 * a + bb] += pr[a] * vr[bb] + pr[6 + a] * vr[6 + bb] + pr[12 + a] * vr[12 + bb];
+  }
+#pragma unroll
+  for (int e = 0; e < 36; ++e) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) acc[e] += __shfl_xor(acc[e], o, 64);
+  }
+  if (lane != 0) return;
+  const int I = brow[b], J = bci[b];
+  double *vb = val + 36 * (size_t)b;
+  if (I == J) {
+#pragma unroll
+    for (int a = 0; a < 6; ++a)
+#pragma unroll
+      for (int bb = 0; bb < 6; ++bb) vb[6 * a + bb] = 0.5 * (acc[6 * a + bb] + acc[6 * bb + a]);
+    return;
+  }
+  double *vt = val + 36 * (size_t)btr[b];
+#pragma unroll
+  for (int a = 0; a < 6; ++a)
+#pragma unroll
+    for (int bb = 0; bb < 6; ++bb) {
+      vb[6 * a + bb] = acc[6 * a + bb];
+      vt[6 * bb + a] = acc[6 * a + bb];
+    }
+}
+
+// ---------------------------------------------------------------- cycle transfers
+// G lanes per coarse unit: bc_I = sum over the support positions of I of
+// P^T r at the position's node (P in support order, each position's 3x6 block
+// contiguous), then (dinv given) level 1's first Chebyshev step from a zero
+// guess: rc = bc, dc = it D^-1 bc, xc = dc
+template <int G, typename T>
+__global__ __launch_bounds__(TPB) void k_sa_restrict_first(int64_t nc, const int32_t *__restrict__ sptr,
+                                                           const int32_t *__restrict__ snode, const T *__restrict__ Ps,
+                                                           const T *__restrict__ r, T *__restrict__ bc,
+                                                           const T *__restrict__ dinv, double it, T *__restrict__ rc,
+                                                           T *__restrict__ dc, T *__restrict__ xc) {
+  static_assert(G >= 1 && G <= 64 && (64 % G) == 0, "G lanes must tile a wavefront");
+  const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  const int64_t A = t / G;
+  const int lane = (int)(t % G);
+  const bool on = A < nc;
+  const int k0 = on ? sptr[A] : 0, npair = on ? (sptr[A + 1] - k0) * 3 : 0;
+  T acc[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
+  for (int q = lane; q < npair; q += G) {
+    const int kk = q / 3, c = q - kk * 3;
+    const int i = snode[k0 + kk];
+    const T *Pe = Ps + (size_t)(k0 + kk) * 18 + c * 6;
+    const T ri = r[(size_t)i * 3 + c];
+#pragma unroll
+    for (int j = 0; j < 6; ++j) acc[j] += Pe[j] * ri;
+  }
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) acc[j] += __shfl_xor(acc[j], o, 64);
+  }
+  if (on && lane == 0) {
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+      const int64_t i = A * 6 + j;
+      bc[i] = acc[j];
+      if (dinv) {
+        const T d = (T)it * dinv[i] * acc[j];
+        rc[i] = acc[j];
+        dc[i] = d;
+        xc[i] = d;
+      }
+    }
+  }
+}
+
+// one thread per node u: x_u += sum over its entry slots of P (ca xa + cb
+// xb)_{aggregate}; ca, cb from sc[ia], sc[ib] (index < 0: 1.0); xb may be
+// null.  P in the node-sliced layout: one load instruction reads a value of
+// 64 consecutive nodes.
+template <typename T>
+__global__ __launch_bounds__(TPB) void k_sa_prolong_comb(int64_t nu, const int64_t *__restrict__ poff,
+                                                         const int32_t *__restrict__ pw,
+                                                         const int32_t *__restrict__ eagg, const T *__restrict__ Pe,
+                                                         const double *__restrict__ sc, int ia, int ib,
+                                                         const T *__restrict__ xa, const T *__restrict__ xb,
+                                                         T *__restrict__ x) {
+  const int64_t u = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  if (u >= nu) return;
+  const T ca = ia < 0 ? T(1) : (T)sc[ia];
+  const T cb = ib < 0 ? T(1) : (T)sc[ib];
+  const int64_t w = u / UFT_ELL_SLICE, lane = u % UFT_ELL_SLICE;
+  const int nq = pw[w];
+  T acc[3] = {T(0), T(0), T(0)};
+  for (int q = 0; q < nq; ++q) {
+    const int64_t s = poff[w] + (int64_t)q * UFT_ELL_SLICE + lane;
+    const int64_t A = eagg[s];
+    if (A < 0) break;  // padding follows the node's entries
+    T c[6];
+#pragma unroll
+    for (int j = 0; j < 6; ++j) c[j] = xb ? ca * xa[A * 6 + j] + cb * xb[A * 6 + j] : xa[A * 6 + j];
+    const T *Pq = Pe + poff[w] * 18 + (int64_t)q * 18 * UFT_ELL_SLICE + lane;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 6; ++j) acc[i] += Pq[(i * 6 + j) * UFT_ELL_SLICE] * c[j];
+  }
+#pragma unroll
+  for (int i = 0; i < 3; ++i) x[u * 3 + i] += acc[i];
+}
+
+}  // namespace
+
+namespace uftk {
+
+void sa_lists(hipStream_t s, const uft_ctx *C, int which, const int32_t *ptr, int32_t *a, int32_t *b) {
+  const AmgLevel &L0 = C->amg.L[0], &L1 = C->amg.L[1];
+  const auto &F = C->fell;
+  if (which == 0)
+    k_sa_scan_w<<<nblk(L0.sa_np), TPB, 0, s>>>(L0.sa_np, L0.d_sa_pnode, L0.d_sa_pagg, F.d_off, F.d_width, F.d_col,
+                                               F.d_cptr, L0.d_parent, ptr, a);
+  else if (which == 1)
+    k_sa_scan_v<<<nblk(L0.sa_nv), TPB, 0, s>>>(L0.sa_nv, L0.d_sa_vnode, L0.d_sa_vagg, F.d_off, F.d_width, F.d_col,
+                                               F.d_cptr, L0.d_sa_pptr, L0.d_sa_pagg, ptr, a, b);
+  else
+    k_sa_scan_g<<<nblk(L0.sa_nub), TPB, 0, s>>>(L0.sa_nub, L0.d_sa_ublk, L1.d_brow, L1.d_ci, L0.d_sa_sptr,
+                                                L0.d_sa_snode, L0.d_sa_sidx, L0.d_sa_vptr, L0.d_sa_vagg, ptr, a, b);
+}
+
+void sa_refresh(hipStream_t s, uft_ctx *C) {
+  AmgLevel &L0 = C->amg.L[0];
+  const AmgLevel &L1 = C->amg.L[1];
+  L0.sa_omega_used = L0.sa_omega;
+  const auto &F = C->fell;
+  if (L0.d_sa_Pef)
+    k_sa_prolongator<float><<<nblk(L0.sa_np), TPB, 0, s>>>(
+        L0.sa_np, L0.d_sa_pnode, L0.d_sa_pagg, L0.d_sa_pptr, L0.d_sa_wptr, L0.d_sa_wslot, F.d_col, C->d_a64,
+        L0.d_parent, L0.d_P, L0.d_dinv, C->d_fixm, L0.sa_omega, L0.d_sa_poff, L0.d_sa_spos, L0.d_sa_P, L0.d_sa_Pef,
+        L0.d_sa_Psf);
+  else
+    k_sa_prolongator<double><<<nblk(L0.sa_np), TPB, 0, s>>>(
+        L0.sa_np, L0.d_sa_pnode, L0.d_sa_pagg, L0.d_sa_pptr, L0.d_sa_wptr, L0.d_sa_wslot, F.d_col, C->d_a64,
+        L0.d_parent, L0.d_P, L0.d_dinv, C->d_fixm, L0.sa_omega, L0.d_sa_poff, L0.d_sa_spos, L0.d_sa_P, L0.d_sa_Pe,
+        L0.d_sa_Ps);
+  k_sa_ap<<<nblk(L0.sa_nv), TPB, 0, s>>>(L0.sa_nv, L0.d_sa_vnode, L0.d_sa_vlptr, L0.d_sa_vlslot, L0.d_sa_vlent,
+                                         C->d_a64, L0.d_sa_P, C->d_fixm, L0.d_sa_V);
+  k_sa_galerkin<<<nblk(L0.sa_nub * 64), TPB, 0, s>>>(L0.sa_nub, L0.d_sa_ublk, L1.d_brow, L1.d_ci, L0.d_gtr,
+                                                     L0.d_sa_gptr, L0.d_sa_gp, L0.d_sa_gv, L0.d_sa_P, L0.d_sa_V,
+                                                     L1.d_val);
+}
+
+void sa_restrict_first(hipStream_t s, const AmgLevel &L0, int64_t nc, const double *r, double *bc, const double *dinv,
+                       double it, double *rc, double *dc, double *xc) {
+  k_sa_restrict_first<16, double><<<nblk(nc * 16), TPB, 0, s>>>(nc, L0.d_sa_sptr, L0.d_sa_snode, L0.d_sa_Ps, r, bc,
+                                                                dinv, it, rc, dc, xc);
+}
+void sa_restrict_first(hipStream_t s, const AmgLevel &L0, int64_t nc, const float *r, float *bc, const float *dinv,
+                       double it, float *rc, float *dc, float *xc) {
+  k_sa_restrict_first<16, float><<<nblk(nc * 16), TPB, 0, s>>>(nc, L0.d_sa_sptr, L0.d_sa_snode, L0.d_sa_Psf, r, bc,
+                                                               dinv, it, rc, dc, xc);
+}
+void sa_prolong_comb(hipStream_t s, const AmgLevel &L0, const double *sc, int ia, int ib, const double *xa,
+                     const double *xb, double *x) {
+  k_sa_prolong_comb<double><<<nblk(L0.nunit), TPB, 0, s>>>(L0.nunit, L0.d_sa_poff, L0.d_sa_pw, L0.d_sa_eagg,
+                                                           L0.d_sa_Pe, sc, ia, ib, xa, xb, x);
+}
+void sa_prolong_comb(hipStream_t s, const AmgLevel &L0, const double *sc, int ia, int ib, const float *xa,
+                     const float *xb, float *x) {
+  k_sa_prolong_comb<float><<<nblk(L0.nunit), TPB, 0, s>>>(L0.nunit, L0.d_sa_poff, L0.d_sa_pw, L0.d_sa_eagg,
+                                                          L0.d_sa_Pef, sc, ia, ib, xa, xb, x);
+}
+
+}  // namespace uftk

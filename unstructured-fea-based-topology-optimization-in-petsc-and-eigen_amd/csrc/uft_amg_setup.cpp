@@ -13,6 +13,11 @@
 //                 fixed DOFs are zero so the coarse spaces never see them
 //   lists         for each coarse block, the contributions in element (or
 //                 fine-block) order
+//   smoothed      with sa, level 0's prolongator is smoothed at every refresh
+//                 with the current operator, P = (I - omega D^-1 A) P_tent
+//                 (smoothed aggregation, as PCGAMG's one smoothing step); the
+//                 host builds the patterns of P, of A P and of the level-1
+//                 operator P^T A P, and the supports of the aggregates
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -184,7 +189,7 @@ void children(const std::vector<int32_t> &agg, int64_t na, std::vector<int32_t> 
 
 // Builds the host description of every transfer level.  H[l] describes the
 // transfer from level l to level l+1 and the pattern/lists of level l+1.
-int uft_amg_build_host(uft_ctx *C, std::vector<AmgHostLevel> &H, std::vector<int64_t> &nunits) {
+int uft_amg_build_host(uft_ctx *C, std::vector<AmgHostLevel> &H, std::vector<int64_t> &nunits, bool sa) {
   H.clear();
   nunits.clear();
   const int npe = C->npe;
@@ -205,7 +210,7 @@ int uft_amg_build_host(uft_ctx *C, std::vector<AmgHostLevel> &H, std::vector<int
 
   const int maxlev = std::max(2, std::min((int)UFT_MAXLEV, C->P.amg_levels));
   const int64_t cmax = std::max(6, C->P.coarse_max);
-  const int64_t dense_limit = 3072;  // the coarsest level must fit the dense inverse
+  const int64_t dense_limit = C->P.coarse_dense_max;  // the coarsest level must fit the dense inverse
 
   // ---- level 0 -> 1
   Graph g = node_graph(nn, ne, npe, C->elem.data());
@@ -254,62 +259,125 @@ int uft_amg_build_host(uft_ctx *C, std::vector<AmgHostLevel> &H, std::vector<int
         std::memcpy(&h.P[(size_t)n * 18], &B[3 * q * 6], 18 * sizeof(double));
       }
     }
-    // level-1 pattern from element aggregate sets
-    std::vector<std::vector<int32_t>> rows(na);
-    for (int64_t e = 0; e < ne; ++e)
-      for (int la = 0; la < npe; ++la) {
-        const int32_t A = agg[C->elem[e * npe + la]];
-        for (int lb = 0; lb < npe; ++lb) rows[A].push_back(agg[C->elem[e * npe + lb]]);
+    if (sa) {
+      // smoothed prolongator P = (I - omega D^-1 A) P_tent: the row of node i
+      // names the aggregates of i and of its neighbours (the columns of its
+      // row of A); the row of A P names the aggregates of those rows
+      h.sa = true;
+      h.sa_pptr.assign(nn + 1, 0);
+      std::vector<int32_t> tmp;
+      for (int64_t i = 0; i < nn; ++i) {
+        tmp.clear();
+        tmp.push_back(agg[i]);
+        for (int64_t k = g.rp[i]; k < g.rp[i + 1]; ++k) tmp.push_back(agg[g.ci[k]]);
+        std::sort(tmp.begin(), tmp.end());
+        tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
+        h.sa_pagg.insert(h.sa_pagg.end(), tmp.begin(), tmp.end());
+        h.sa_pptr[i + 1] = (int32_t)h.sa_pagg.size();
       }
-    h.rp.assign(na + 1, 0);
-    for (int64_t A = 0; A < na; ++A) {
-      auto &r = rows[A];
-      std::sort(r.begin(), r.end());
-      r.erase(std::unique(r.begin(), r.end()), r.end());
-      h.rp[A + 1] = h.rp[A] + (int32_t)r.size();
-    }
-    h.ci.reserve(h.rp[na]);
-    for (int64_t A = 0; A < na; ++A) h.ci.insert(h.ci.end(), rows[A].begin(), rows[A].end());
-    rows.clear();
-    rows.shrink_to_fit();
-    // Galerkin entries for level 1: one per (upper block (A,B), A <= B,
-    // element touching both), grouped by block, in element order within a
-    // block.  Entry k carries G_k = sum over the element's node pairs (la, lb)
-    // with agg(la) = A, agg(lb) = B of Q_la^T KE_e[la,lb] Q_lb (density free,
-    // built once on the device); a refresh is then sum_k E(x_e) G_k.
-    const int64_t nbk = h.rp[na];
-    h.glptr.assign(nbk + 1, 0);
-    h.tr.assign(nbk, 0);
-    for (int64_t A = 0; A < na; ++A)
-      for (int32_t k = h.rp[A]; k < h.rp[A + 1]; ++k) h.tr[k] = find_col(h.rp, h.ci, h.ci[k], (int32_t)A);
-    std::vector<int32_t> eblk;  // (element, upper block) pairs in element order
-    std::vector<int32_t> eent;
-    std::vector<int32_t> loc;
-    for (int64_t e = 0; e < ne; ++e) {
-      loc.clear();
-      for (int la = 0; la < npe; ++la) {
-        const int32_t A = agg[C->elem[e * npe + la]];
-        for (int lb = 0; lb < npe; ++lb) {
-          const int32_t Bc = agg[C->elem[e * npe + lb]];
-          if (A <= Bc) loc.push_back(find_col(h.rp, h.ci, A, Bc));
+      h.sa_vptr.assign(nn + 1, 0);
+      for (int64_t i = 0; i < nn; ++i) {
+        tmp.clear();
+        tmp.insert(tmp.end(), h.sa_pagg.begin() + h.sa_pptr[i], h.sa_pagg.begin() + h.sa_pptr[i + 1]);
+        for (int64_t k = g.rp[i]; k < g.rp[i + 1]; ++k) {
+          const int32_t j = g.ci[k];
+          tmp.insert(tmp.end(), h.sa_pagg.begin() + h.sa_pptr[j], h.sa_pagg.begin() + h.sa_pptr[j + 1]);
+        }
+        std::sort(tmp.begin(), tmp.end());
+        tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
+        h.sa_vagg.insert(h.sa_vagg.end(), tmp.begin(), tmp.end());
+        h.sa_vptr[i + 1] = (int32_t)h.sa_vagg.size();
+      }
+      // supports in node order
+      h.sa_sptr.assign(na + 1, 0);
+      for (int32_t a : h.sa_pagg) h.sa_sptr[a + 1]++;
+      for (int64_t a = 0; a < na; ++a) h.sa_sptr[a + 1] += h.sa_sptr[a];
+      h.sa_snode.assign(h.sa_pagg.size(), 0);
+      h.sa_sidx.assign(h.sa_pagg.size(), 0);
+      {
+        std::vector<int32_t> pos(h.sa_sptr.begin(), h.sa_sptr.end() - 1);
+        for (int64_t i = 0; i < nn; ++i)
+          for (int32_t k = h.sa_pptr[i]; k < h.sa_pptr[i + 1]; ++k) {
+            const int32_t q = pos[h.sa_pagg[k]]++;
+            h.sa_snode[q] = (int32_t)i;
+            h.sa_sidx[q] = k;
+          }
+      }
+      // level-1 pattern: I couples J when a node of I's support has J in its row of A P
+      h.rp.assign(na + 1, 0);
+      for (int64_t A = 0; A < na; ++A) {
+        tmp.clear();
+        for (int32_t q = h.sa_sptr[A]; q < h.sa_sptr[A + 1]; ++q) {
+          const int32_t i = h.sa_snode[q];
+          tmp.insert(tmp.end(), h.sa_vagg.begin() + h.sa_vptr[i], h.sa_vagg.begin() + h.sa_vptr[i + 1]);
+        }
+        std::sort(tmp.begin(), tmp.end());
+        tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
+        h.ci.insert(h.ci.end(), tmp.begin(), tmp.end());
+        h.rp[A + 1] = (int32_t)h.ci.size();
+      }
+      const int64_t nbk = h.rp[na];
+      h.tr.assign(nbk, 0);
+      for (int64_t A = 0; A < na; ++A)
+        for (int32_t k = h.rp[A]; k < h.rp[A + 1]; ++k) h.tr[k] = find_col(h.rp, h.ci, h.ci[k], (int32_t)A);
+    } else {
+      // level-1 pattern from element aggregate sets
+      std::vector<std::vector<int32_t>> rows(na);
+      for (int64_t e = 0; e < ne; ++e)
+        for (int la = 0; la < npe; ++la) {
+          const int32_t A = agg[C->elem[e * npe + la]];
+          for (int lb = 0; lb < npe; ++lb) rows[A].push_back(agg[C->elem[e * npe + lb]]);
+        }
+      h.rp.assign(na + 1, 0);
+      for (int64_t A = 0; A < na; ++A) {
+        auto &r = rows[A];
+        std::sort(r.begin(), r.end());
+        r.erase(std::unique(r.begin(), r.end()), r.end());
+        h.rp[A + 1] = h.rp[A] + (int32_t)r.size();
+      }
+      h.ci.reserve(h.rp[na]);
+      for (int64_t A = 0; A < na; ++A) h.ci.insert(h.ci.end(), rows[A].begin(), rows[A].end());
+      rows.clear();
+      rows.shrink_to_fit();
+      // Galerkin entries for level 1: one per (upper block (A,B), A <= B,
+      // element touching both), grouped by block, in element order within a
+      // block.  Entry k carries G_k = sum over the element's node pairs (la, lb)
+      // with agg(la) = A, agg(lb) = B of Q_la^T KE_e[la,lb] Q_lb (density free,
+      // built once on the device); a refresh is then sum_k E(x_e) G_k.
+      const int64_t nbk = h.rp[na];
+      h.glptr.assign(nbk + 1, 0);
+      h.tr.assign(nbk, 0);
+      for (int64_t A = 0; A < na; ++A)
+        for (int32_t k = h.rp[A]; k < h.rp[A + 1]; ++k) h.tr[k] = find_col(h.rp, h.ci, h.ci[k], (int32_t)A);
+      std::vector<int32_t> eblk;  // (element, upper block) pairs in element order
+      std::vector<int32_t> eent;
+      std::vector<int32_t> loc;
+      for (int64_t e = 0; e < ne; ++e) {
+        loc.clear();
+        for (int la = 0; la < npe; ++la) {
+          const int32_t A = agg[C->elem[e * npe + la]];
+          for (int lb = 0; lb < npe; ++lb) {
+            const int32_t Bc = agg[C->elem[e * npe + lb]];
+            if (A <= Bc) loc.push_back(find_col(h.rp, h.ci, A, Bc));
+          }
+        }
+        std::sort(loc.begin(), loc.end());
+        loc.erase(std::unique(loc.begin(), loc.end()), loc.end());
+        for (int32_t kb : loc) {
+          eblk.push_back(kb);
+          eent.push_back((int32_t)e);
+          h.glptr[kb + 1]++;
         }
       }
-      std::sort(loc.begin(), loc.end());
-      loc.erase(std::unique(loc.begin(), loc.end()), loc.end());
-      for (int32_t kb : loc) {
-        eblk.push_back(kb);
-        eent.push_back((int32_t)e);
-        h.glptr[kb + 1]++;
+      for (int64_t k = 0; k < nbk; ++k) h.glptr[k + 1] += h.glptr[k];
+      h.gl.assign(eblk.size(), 0);
+      h.gblk.assign(eblk.size(), 0);
+      std::vector<int32_t> pos(h.glptr.begin(), h.glptr.end() - 1);
+      for (size_t q = 0; q < eblk.size(); ++q) {
+        const int32_t k = pos[eblk[q]]++;
+        h.gl[k] = eent[q];
+        h.gblk[k] = eblk[q];
       }
-    }
-    for (int64_t k = 0; k < nbk; ++k) h.glptr[k + 1] += h.glptr[k];
-    h.gl.assign(eblk.size(), 0);
-    h.gblk.assign(eblk.size(), 0);
-    std::vector<int32_t> pos(h.glptr.begin(), h.glptr.end() - 1);
-    for (size_t q = 0; q < eblk.size(); ++q) {
-      const int32_t k = pos[eblk[q]]++;
-      h.gl[k] = eent[q];
-      h.gblk[k] = eblk[q];
     }
     H.push_back(std::move(h));
     nunits.push_back(na);
@@ -395,7 +463,7 @@ int uft_amg_build_host(uft_ctx *C, std::vector<AmgHostLevel> &H, std::vector<int
     if (nunits.size() < 3)
       return uft_set_error(C, UFT_ERR_ARG, "design batches need at least three multigrid levels");
   }
-  if (6 * nlast > dense_limit * 2)
+  if (6 * nlast > std::max<int64_t>(dense_limit * 2, 6144))
     return uft_set_error(C, UFT_ERR_ARG, "coarsest level too large for the dense solve (%lld units)", (long long)nlast);
   return UFT_OK;
 }

@@ -449,6 +449,14 @@ void prolong_from(uft_ctx *C, int l, T *x) {
   AmgLevel &Lv = C->amg.L[l], &Ln = C->amg.L[l + 1];
   LV<T> vn = lv(Ln, (T *)nullptr);
   const T *P = tr_P(Lv, (T *)nullptr);
+  if (Lv.sa) {  // level 0's smoothed prolongator, row by row
+    switch (level_kind(C, l + 1)) {
+      case 0: uftk::sa_prolong_comb(C->s, Lv, nullptr, -1, -1, vn.x, (const T *)nullptr, x); break;
+      case 1: uftk::sa_prolong_comb(C->s, Lv, nullptr, -1, -1, vn.x, vn.kd, x); break;
+      default: uftk::sa_prolong_comb(C->s, Lv, C->d_scal + S_K + 8 * (l + 1), 5, 6, vn.kc, vn.kd, x); break;
+    }
+    return;
+  }
   switch (level_kind(C, l + 1)) {
     case 0:
       uftk::prolong_comb(C->s, Lv.bs, Lv.nunit, Lv.d_parent, P, nullptr, -1, -1, vn.x, (const T *)nullptr, x);
@@ -553,8 +561,11 @@ void cycle_body(uft_ctx *C, int l, const T *b, T *x) {
     LV<T> vn = lv(Ln, (T *)nullptr);
     const double itn = cheb_coef(Ln.lmax, level_degree(C, l + 1), C->P.cheb_lo / C->P.cheb_hi).inv_theta;
     T *xn = level_kind(C, l + 1) == 2 ? vn.kc : vn.x;
-    uftk::restrict_first(C->s, Lv.bs, Ln.nunit, Lv.d_cptr, Lv.d_child, tr_P(Lv, (T *)nullptr), v.r, vn.b, vn.dinv,
-                         itn, vn.r, vn.d, xn);
+    if (Lv.sa)  // smoothed prolongator of level 0: through the aggregates' supports
+      uftk::sa_restrict_first(C->s, Lv, Ln.nunit, v.r, vn.b, vn.dinv, itn, vn.r, vn.d, xn);
+    else
+      uftk::restrict_first(C->s, Lv.bs, Ln.nunit, Lv.d_cptr, Lv.d_child, tr_P(Lv, (T *)nullptr), v.r, vn.b,
+                           vn.dinv, itn, vn.r, vn.d, xn);
     coarse_body<T>(C, l + 1);
     prolong_from<T>(C, l, x);
   }
@@ -1008,64 +1019,32 @@ int amg_refresh(uft_ctx *C, const double *xphys) {
   CountOff off(C);
   return amg_refresh_impl(C, xphys);
 }
-int amg_refresh_impl(uft_ctx *C, const double *xphys) {
+// Levels >= 1 after the level-1 operator: its post-processing (unit diagonal
+// of empty rows, block-diagonal inverse, lower-precision copies), the coarser
+// Galerkin products, the coarse levels' eigenvalue estimates and the coarsest
+// inverse (on the side streams when par, one host round trip for all).
+int refresh_coarse(uft_ctx *C, bool par, int ks, int first_est) {
   AmgHier &H = C->amg;
   hipStream_t s = C->s;
-  C->cur_xphys = xphys;
-  C->refreshes++;
-  uftk::diag_K(s, C, xphys, H.L[0].d_dinv, H.L[0].d_dinvf);
-  // Concurrent branches (each on its own stream; each does what the
-  // sequential order does, so the results are the same): the fine blocks and
-  // the fine level's eigenvalue estimate once E(x_e) and the fine diagonal
-  // exist; after the Galerkin products the coarse levels' estimates (own
-  // partials and scalar slots) and the coarsest inverse.
-  const bool par = C->side[0] && !C->bt.nseg;
-  const int ks = std::min(64, std::max(2, C->P.esteig_steps));
-  if (par) {
-    uftk::simp_moduli(s, C, xphys, C->d_Ee);
-    UFT_HIP(C, hipEventRecord(C->ev_side[3], s));
-    UFT_HIP(C, hipStreamWaitEvent(C->side[0], C->ev_side[3], 0));
-    C->s = C->side[0];
-    if (fine_blocks(C)) uftk::fine_ell_fill(C->s, C, C->d_Ee, H.L[0].d_dinv);
-    if (H.nlev > 1) lanczos_enqueue(C, 0, ks, C->d_part, S_MISC, C->d_lanczos);
-    C->s = s;
-  }
-  for (int l = 0; l + 1 < H.nlev; ++l) {
-    AmgLevel &Lv = H.L[l], &Ln = H.L[l + 1];
-    if (l == 0)
-      uftk::galerkin_fine(s, C, par ? nullptr : xphys, Ln.nblk, Lv.d_glptr, Lv.d_gl, Lv.d_gtr, Ln.d_brow, Ln.d_ci,
-                          Lv.d_G, C->d_Ee, Ln.d_val);
-    else
-      uftk::galerkin_coarse(s, Ln.nblk, Lv.d_glptr, Lv.d_gl, Lv.d_brow, Lv.d_ci, Lv.d_P, Lv.d_val, Ln.d_val);
+  for (int l = 1; l < H.nlev; ++l) {
+    AmgLevel &Lv = H.L[l - 1], &Ln = H.L[l];
+    if (l >= 2) uftk::galerkin_coarse(s, Ln.nblk, Lv.d_glptr, Lv.d_gl, Lv.d_brow, Lv.d_ci, Lv.d_P, Lv.d_val, Ln.d_val);
     uftk::fixed_unit_diag(s, Ln.nunit, Ln.d_rp, Ln.d_ci, Ln.d_val);
     uftk::bsr6_dinv(s, Ln.nunit, Ln.d_rp, Ln.d_ci, Ln.d_val, Ln.d_dinv, Ln.d_dinvf);
     if (Ln.d_valf) uftk::d2f(s, Ln.nblk * 36, Ln.d_val, Ln.d_valf);
     if (Ln.d_valh)
       uftk::bsr6_to_half(s, Ln.nblk, Ln.nunit, Ln.d_brow, Ln.d_ci, Ln.d_val, Ln.d_dinv, Ln.d_valh, Ln.d_dh);
   }
-  // the level 0 -> 1 Galerkin product has written E(x_e) to d_Ee
-  if (fine_blocks(C) && !par) uftk::fine_ell_fill(s, C, C->d_Ee, H.L[0].d_dinv);
   AmgLevel &Lc = H.L[H.nlev - 1];
-  if (C->bt.nseg) {
-    BatchState &B = C->bt;
-    uftk::seg_bsr_to_dense(s, Lc.nunit, Lc.d_rp, Lc.d_ci, Lc.d_val, B.d_cdes, B.d_cu, B.d_coff, H.d_cmat);
-    uftk::seg_dense_inverse(s, B.nseg, B.d_cu, B.d_coff, H.d_cmat, H.d_cinv);
-    for (int l = 0; l + 1 < H.nlev; ++l) {
-      int rc = estimate_lmax_batch(C, l, std::min(64, std::max(2, C->P.esteig_steps)));
-      if (rc) return rc;
-    }
-    return UFT_OK;
-  }
   if (!par) {
     uftk::bsr_to_dense(s, Lc.nunit, Lc.d_rp, Lc.d_ci, Lc.d_val, H.ncoarse, H.d_cmat);
     uftk::dense_spd_inverse(s, H.ncoarse, H.d_cmat, H.d_cinv, H.d_cinvf);
-    for (int l = 0; l + 1 < H.nlev; ++l) {
+    for (int l = first_est; l + 1 < H.nlev; ++l) {
       int rc = estimate_lmax(C, l, ks);
       if (rc) return rc;
     }
     return UFT_OK;
   }
-  // the coarse branches; one host round trip for all estimates
   UFT_HIP(C, hipEventRecord(C->ev_side[4], s));
   for (int i = 1; i < 3; ++i) UFT_HIP(C, hipStreamWaitEvent(C->side[i], C->ev_side[4], 0));
   C->s = C->side[1];
@@ -1081,8 +1060,91 @@ int amg_refresh_impl(uft_ctx *C, const double *xphys) {
   std::vector<double> hl((size_t)192 * (H.nlev - 1));
   UFT_HIP(C, hipMemcpyAsync(hl.data(), C->d_lanczos, hl.size() * sizeof(double), hipMemcpyDeviceToHost, s));
   if (int rc = sync(C)) return rc;
-  for (int l = 0; l + 1 < H.nlev; ++l) lanczos_finish(C, l, ks, hl.data() + 192 * l);
+  for (int l = first_est; l + 1 < H.nlev; ++l) lanczos_finish(C, l, ks, hl.data() + 192 * l);
   return UFT_OK;
+}
+
+int amg_refresh_impl(uft_ctx *C, const double *xphys) {
+  AmgHier &H = C->amg;
+  hipStream_t s = C->s;
+  C->cur_xphys = xphys;
+  C->refreshes++;
+  uftk::diag_K(s, C, xphys, H.L[0].d_dinv, H.L[0].d_dinvf);
+  const bool par = C->side[0] && !C->bt.nseg;
+  const int ks = std::min(64, std::max(2, C->P.esteig_steps));
+  if (H.L[0].sa) {
+    // smoothed aggregation: the fine blocks (FP64, and the mixed forms'
+    // smoother and outer blocks), the fine level's lambda_max (omega needs
+    // it), the smoothed prolongator and P^T A P, then the coarser levels
+    // After the first refresh omega comes from the previous refresh's
+    // estimate (lambda_max moves little between design iterations), so the
+    // fine level's estimate runs on a side stream beside the products.
+    AmgLevel &L0 = H.L[0];
+    uftk::simp_moduli(s, C, xphys, C->d_Ee);
+    uftk::fine_ell_fill(s, C, C->d_Ee, L0.d_dinv, C->d_a64);
+    const double hi = C->P.cheb_hi;
+    if (par && L0.sa_omega > 0.0) {
+      UFT_HIP(C, hipEventRecord(C->ev_side[3], s));
+      UFT_HIP(C, hipStreamWaitEvent(C->side[0], C->ev_side[3], 0));
+      C->s = C->side[0];
+      lanczos_enqueue(C, 0, ks, C->d_part, S_MISC, C->d_lanczos);
+      C->s = s;
+      uftk::sa_refresh(s, C);
+      if (int rc = refresh_coarse(C, par, ks, 0)) return rc;
+    } else {
+      if (int rc = estimate_lmax(C, 0, ks)) return rc;
+      L0.sa_omega = 4.0 / (3.0 * (L0.lmax / hi));
+      uftk::sa_refresh(s, C);
+      if (int rc = refresh_coarse(C, par, ks, 1)) return rc;
+    }
+    L0.sa_omega = 4.0 / (3.0 * (L0.lmax / hi));  // for the next refresh
+    return UFT_OK;
+  }
+  // Concurrent branches (each on its own stream; each does what the
+  // sequential order does, so the results are the same): the fine blocks and
+  // the fine level's eigenvalue estimate once E(x_e) and the fine diagonal
+  // exist; after the Galerkin products the coarse levels' estimates (own
+  // partials and scalar slots) and the coarsest inverse.
+  if (par) {
+    uftk::simp_moduli(s, C, xphys, C->d_Ee);
+    UFT_HIP(C, hipEventRecord(C->ev_side[3], s));
+    UFT_HIP(C, hipStreamWaitEvent(C->side[0], C->ev_side[3], 0));
+    C->s = C->side[0];
+    if (fine_blocks(C)) uftk::fine_ell_fill(C->s, C, C->d_Ee, H.L[0].d_dinv);
+    if (H.nlev > 1) lanczos_enqueue(C, 0, ks, C->d_part, S_MISC, C->d_lanczos);
+    C->s = s;
+  }
+  if (H.nlev > 1) {
+    AmgLevel &Lv = H.L[0], &Ln = H.L[1];
+    uftk::galerkin_fine(s, C, par ? nullptr : xphys, Ln.nblk, Lv.d_glptr, Lv.d_gl, Lv.d_gtr, Ln.d_brow, Ln.d_ci,
+                        Lv.d_G, C->d_Ee, Ln.d_val);
+  }
+  if (C->bt.nseg) {
+    BatchState &B = C->bt;
+    for (int l = 1; l < H.nlev; ++l) {
+      AmgLevel &Lv = H.L[l - 1], &Ln = H.L[l];
+      if (l >= 2)
+        uftk::galerkin_coarse(s, Ln.nblk, Lv.d_glptr, Lv.d_gl, Lv.d_brow, Lv.d_ci, Lv.d_P, Lv.d_val, Ln.d_val);
+      uftk::fixed_unit_diag(s, Ln.nunit, Ln.d_rp, Ln.d_ci, Ln.d_val);
+      uftk::bsr6_dinv(s, Ln.nunit, Ln.d_rp, Ln.d_ci, Ln.d_val, Ln.d_dinv, Ln.d_dinvf);
+      if (Ln.d_valf) uftk::d2f(s, Ln.nblk * 36, Ln.d_val, Ln.d_valf);
+      if (Ln.d_valh)
+        uftk::bsr6_to_half(s, Ln.nblk, Ln.nunit, Ln.d_brow, Ln.d_ci, Ln.d_val, Ln.d_dinv, Ln.d_valh, Ln.d_dh);
+    }
+    // the level 0 -> 1 Galerkin product has written E(x_e) to d_Ee
+    if (fine_blocks(C)) uftk::fine_ell_fill(s, C, C->d_Ee, H.L[0].d_dinv);
+    AmgLevel &Lc = H.L[H.nlev - 1];
+    uftk::seg_bsr_to_dense(s, Lc.nunit, Lc.d_rp, Lc.d_ci, Lc.d_val, B.d_cdes, B.d_cu, B.d_coff, H.d_cmat);
+    uftk::seg_dense_inverse(s, B.nseg, B.d_cu, B.d_coff, H.d_cmat, H.d_cinv);
+    for (int l = 0; l + 1 < H.nlev; ++l) {
+      int rc = estimate_lmax_batch(C, l, std::min(64, std::max(2, C->P.esteig_steps)));
+      if (rc) return rc;
+    }
+    return UFT_OK;
+  }
+  // the level 0 -> 1 Galerkin product has written E(x_e) to d_Ee
+  if (fine_blocks(C) && !par) uftk::fine_ell_fill(s, C, C->d_Ee, H.L[0].d_dinv);
+  return refresh_coarse(C, par, ks, 0);
 }
 
 // AMG-preconditioned CG on K(x) U = F from U = 0; converged when
@@ -1840,6 +1902,8 @@ void uft_default_params(uft_params *p) {
   p->graph = 0;
   p->cheb_degree_coarse = 0;
   p->fine_assembled = 1;
+  p->coarse_dense_max = 3072;
+  p->prolong_smooth = 1;
 }
 
 int uft_create(uft_ctx **out, int device) {
@@ -1894,11 +1958,16 @@ int uft_params_set(uft_ctx *C, const uft_params *p) {
   if (p->cheb_degree < 1 || p->cheb_degree > 16 || p->cheb_degree_coarse < 0 || p->cheb_degree_coarse > 16 ||
       p->cycle_type < 0 || p->cycle_type > 3)
     return uft_set_error(C, UFT_ERR_ARG, "Chebyshev degrees must be in [1, 16] (coarse: 0 or [1, 16]), cycle_type in [0, 3]");
+  if (p->prolong_smooth != 0 && p->prolong_smooth != 1)
+    return uft_set_error(C, UFT_ERR_ARG, "prolong_smooth must be 0 or 1");
+  if (p->coarse_dense_max < 6 || p->coarse_dense_max > 6144)
+    return uft_set_error(C, UFT_ERR_ARG, "coarse_dense_max must be in [6, 6144]");
   if (p->precision < UFT_FP64 || p->precision > UFT_MIXED_H)
     return uft_set_error(C, UFT_ERR_ARG, "precision must be UFT_FP64, UFT_MIXED or UFT_MIXED_H");
   if (C->is_setup && (p->m != C->P.m || p->rmin != C->P.rmin || p->nu != C->P.nu || p->precision != C->P.precision ||
                       p->load_cases != C->P.load_cases || p->fine_assembled != C->P.fine_assembled ||
-                      p->amg_levels != C->P.amg_levels || p->coarse_max != C->P.coarse_max))
+                      p->amg_levels != C->P.amg_levels || p->coarse_max != C->P.coarse_max ||
+                      p->coarse_dense_max != C->P.coarse_dense_max || p->prolong_smooth != C->P.prolong_smooth))
     return uft_set_error(C, UFT_ERR_STATE,
                          "m, rmin, nu, precision, fine_assembled and the hierarchy shape are fixed at setup");
   C->P = *p;
@@ -1950,6 +2019,71 @@ int uft_bc_set(uft_ctx *C, int nfixed, const int32_t *fixed, int nlc, int nload,
   C->lvals.assign(lv, lv + (int64_t)nlc * nload);
   C->have_bc = true;
   return UFT_OK;
+}
+
+// Smoothed prolongator, once at setup: the refresh's contribution lists
+// (counted, then filled, by scanning the patterns on the device) and the
+// cycle's two layouts of P (node-sliced for the prolongation, support order
+// for the restriction).
+int sa_list(uft_ctx *C, int which, int64_t rows, int32_t **ptr, int32_t **a, int32_t **b) {
+  hipStream_t s = C->s;
+  int32_t *cnt = nullptr;
+  int rc = dalloc_n(C, &cnt, std::max<int64_t>(rows, 1));
+  if (rc) return rc;
+  uftk::sa_lists(s, C, which, nullptr, cnt, nullptr);
+  std::vector<int32_t> hc(rows);
+  if (rows) UFT_HIP(C, hipMemcpyAsync(hc.data(), cnt, rows * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  if ((rc = sync(C))) return rc;
+  std::vector<int32_t> hp(rows + 1, 0);
+  int64_t tot = 0;
+  for (int64_t r = 0; r < rows; ++r) {
+    tot += hc[r];
+    if (tot > INT32_MAX) return uft_set_error(C, UFT_ERR_ARG, "smoothed prolongator lists exceed 2^31 entries");
+    hp[r + 1] = (int32_t)tot;
+  }
+  if ((rc = upload(C, ptr, hp)) || (rc = dalloc_n(C, a, std::max<int64_t>(tot, 1)))) return rc;
+  if (b && (rc = dalloc_n(C, b, std::max<int64_t>(tot, 1)))) return rc;
+  uftk::sa_lists(s, C, which, *ptr, *a, b ? *b : nullptr);
+  return sync(C);
+}
+
+int sa_setup_lists(uft_ctx *C, const AmgHostLevel &h) {
+  AmgLevel &L0 = C->amg.L[0];
+  int rc;
+  if ((rc = sa_list(C, 0, L0.sa_np, &L0.d_sa_wptr, &L0.d_sa_wslot, nullptr)) ||
+      (rc = sa_list(C, 1, L0.sa_nv, &L0.d_sa_vlptr, &L0.d_sa_vlslot, &L0.d_sa_vlent)) ||
+      (rc = sa_list(C, 2, L0.sa_nub, &L0.d_sa_gptr, &L0.d_sa_gp, &L0.d_sa_gv)))
+    return rc;
+  const int64_t nn = L0.nunit, nsl = (nn + UFT_ELL_SLICE - 1) / UFT_ELL_SLICE;
+  std::vector<int64_t> poff(nsl + 1, 0);
+  std::vector<int32_t> pw(nsl, 0);
+  for (int64_t w = 0; w < nsl; ++w) {
+    int mx = 0;
+    for (int64_t n = w * UFT_ELL_SLICE; n < std::min(nn, (w + 1) * UFT_ELL_SLICE); ++n)
+      mx = std::max(mx, h.sa_pptr[n + 1] - h.sa_pptr[n]);
+    pw[w] = mx;
+    poff[w + 1] = poff[w] + (int64_t)mx * UFT_ELL_SLICE;
+  }
+  std::vector<int32_t> eagg(poff[nsl], -1), spos(L0.sa_np, 0);
+  for (int64_t n = 0; n < nn; ++n) {
+    const int64_t w = n / UFT_ELL_SLICE, lane = n % UFT_ELL_SLICE;
+    for (int32_t t = h.sa_pptr[n]; t < h.sa_pptr[n + 1]; ++t)
+      eagg[poff[w] + (int64_t)(t - h.sa_pptr[n]) * UFT_ELL_SLICE + lane] = h.sa_pagg[t];
+  }
+  for (size_t q = 0; q < h.sa_sidx.size(); ++q) spos[h.sa_sidx[q]] = (int32_t)q;
+  L0.sa_ne_ell = poff[nsl];
+  if ((rc = upload(C, &L0.d_sa_poff, poff)) || (rc = upload(C, &L0.d_sa_pw, pw)) ||
+      (rc = upload(C, &L0.d_sa_eagg, eagg)) || (rc = upload(C, &L0.d_sa_spos, spos)))
+    return rc;
+  if (is_mixed(C)) {
+    // (padding slots are never read: the prolongation stops at aggregate -1)
+    if ((rc = dalloc_n(C, &L0.d_sa_Pef, 18 * L0.sa_ne_ell)) || (rc = dalloc_n(C, &L0.d_sa_Psf, 18 * L0.sa_np)))
+      return rc;
+  } else {
+    if ((rc = dalloc_n(C, &L0.d_sa_Pe, 18 * L0.sa_ne_ell)) || (rc = dalloc_n(C, &L0.d_sa_Ps, 18 * L0.sa_np)))
+      return rc;
+  }
+  return sync(C);
 }
 
 int uft_setup(uft_ctx *C) {
@@ -2161,7 +2295,18 @@ int uft_setup(uft_ctx *C) {
   {
     std::vector<AmgHostLevel> H;
     std::vector<int64_t> nun;
-    if ((rc = uft_amg_build_host(C, H, nun))) return rc;
+    // smoothed aggregation on the fine level: one design, one right-hand side,
+    // the assembled node pattern (its FP64 blocks are the operator that smooths
+    // the prolongator) and at least three levels (the level-1 correction is not
+    // the dense coarsest solve)
+    const bool want_sa = C->P.prolong_smooth && !B.nseg && !(C->P.load_cases == 1 && C->nlc > 1) &&
+                         C->fell.nslot > 0;
+    if ((rc = uft_amg_build_host(C, H, nun, want_sa))) return rc;
+    if (want_sa && nun.size() < 3) {
+      H.clear();
+      nun.clear();
+      if ((rc = uft_amg_build_host(C, H, nun, false))) return rc;
+    }
     AmgHier &A = C->amg;
     A.nlev = (int)nun.size();
     for (int l = 0; l < A.nlev; ++l) {
@@ -2212,7 +2357,31 @@ int uft_setup(uft_ctx *C) {
         for (int32_t k = h.rp[A2]; k < h.rp[A2 + 1]; ++k) brow[k] = (int32_t)A2;
       if ((rc = upload(C, &Ln.d_brow, brow))) return rc;
       if ((rc = dalloc_n(C, &Ln.d_val, Ln.nblk * 36))) return rc;
-      if (l == 0) {  // density-free Galerkin entries of the fine level
+      if (l == 0 && h.sa) {  // smoothed prolongator: patterns, supports, work arrays
+        Lv.sa = true;
+        Lv.sa_np = (int64_t)h.sa_pagg.size();
+        Lv.sa_nv = (int64_t)h.sa_vagg.size();
+        std::vector<int32_t> pnode(Lv.sa_np), vnode(Lv.sa_nv), ublk;
+        for (int64_t i = 0; i < Lv.nunit; ++i) {
+          for (int32_t k = h.sa_pptr[i]; k < h.sa_pptr[i + 1]; ++k) pnode[k] = (int32_t)i;
+          for (int32_t k = h.sa_vptr[i]; k < h.sa_vptr[i + 1]; ++k) vnode[k] = (int32_t)i;
+        }
+        for (int64_t A2 = 0; A2 < Ln.nunit; ++A2)
+          for (int32_t k = h.rp[A2]; k < h.rp[A2 + 1]; ++k)
+            if (h.ci[k] >= A2) ublk.push_back(k);
+        Lv.sa_nub = (int64_t)ublk.size();
+        if ((rc = upload(C, &Lv.d_gtr, h.tr)) || (rc = upload(C, &Lv.d_sa_pptr, h.sa_pptr)) ||
+            (rc = upload(C, &Lv.d_sa_pagg, h.sa_pagg)) || (rc = upload(C, &Lv.d_sa_pnode, pnode)) ||
+            (rc = upload(C, &Lv.d_sa_vptr, h.sa_vptr)) || (rc = upload(C, &Lv.d_sa_vagg, h.sa_vagg)) ||
+            (rc = upload(C, &Lv.d_sa_vnode, vnode)) || (rc = upload(C, &Lv.d_sa_sptr, h.sa_sptr)) ||
+            (rc = upload(C, &Lv.d_sa_snode, h.sa_snode)) || (rc = upload(C, &Lv.d_sa_sidx, h.sa_sidx)) ||
+            (rc = upload(C, &Lv.d_sa_ublk, ublk)) || (rc = dalloc_n(C, &Lv.d_sa_P, 18 * Lv.sa_np)) ||
+            (rc = dalloc_n(C, &Lv.d_sa_V, 18 * Lv.sa_nv)) || (rc = dalloc_n(C, &C->d_Ee, C->ne)) ||
+            (rc = dalloc_n(C, &C->d_a64, 9 * C->fell.nslot)))
+          return rc;
+        uftk::fill(s, 9 * C->fell.nslot, 0.0, C->d_a64);
+        if ((rc = sa_setup_lists(C, h))) return rc;
+      } else if (l == 0) {  // density-free Galerkin entries of the fine level
         int32_t *d_gblk;
         if ((rc = upload(C, &Lv.d_gtr, h.tr))) return rc;
         if ((rc = upload(C, &d_gblk, h.gblk))) return rc;
@@ -3068,6 +3237,20 @@ extern "C" int uft_amg_transfer_get(uft_ctx *C, int level, int32_t *parent, doub
   if (parent)
     UFT_HIP(C, hipMemcpyAsync(parent, Lv.d_parent, Lv.nunit * sizeof(int32_t), hipMemcpyDeviceToHost, C->s));
   if (P) UFT_HIP(C, hipMemcpyAsync(P, Lv.d_P, Lv.nunit * Lv.bs * 6 * sizeof(double), hipMemcpyDeviceToHost, C->s));
+  return sync(C);
+}
+
+extern "C" int uft_amg_prolongator_get(uft_ctx *C, int64_t *nent, int32_t *pptr, int32_t *pagg, double *P,
+                                       double *omega) {
+  if (int rc = check_setup(C)) return rc;
+  if (!nent) return UFT_ERR_ARG;
+  const AmgLevel &L0 = C->amg.L[0];
+  *nent = L0.sa ? L0.sa_np : 0;
+  if (omega) *omega = L0.sa ? L0.sa_omega_used : 0.0;
+  if (!L0.sa) return UFT_OK;
+  if (pptr) UFT_HIP(C, hipMemcpyAsync(pptr, L0.d_sa_pptr, (L0.nunit + 1) * sizeof(int32_t), hipMemcpyDeviceToHost, C->s));
+  if (pagg) UFT_HIP(C, hipMemcpyAsync(pagg, L0.d_sa_pagg, L0.sa_np * sizeof(int32_t), hipMemcpyDeviceToHost, C->s));
+  if (P) UFT_HIP(C, hipMemcpyAsync(P, L0.d_sa_P, L0.sa_np * 18 * sizeof(double), hipMemcpyDeviceToHost, C->s));
   return sync(C);
 }
 

@@ -90,6 +90,35 @@ struct AmgLevel {
   float *d_Pf = nullptr;
   int32_t *d_parent = nullptr;
   int32_t *d_cptr = nullptr, *d_child = nullptr;  // next-level unit -> children
+  // level 0, smoothed prolongator P = (I - omega D^-1 A) P_tent of the
+  // current operator (rebuilt at each refresh; see uft_amg_build_host)
+  bool sa = false;
+  int64_t sa_np = 0, sa_nv = 0;      // prolongator entries, entries of A P (3x6 blocks each)
+  int32_t *d_sa_pptr = nullptr, *d_sa_pagg = nullptr, *d_sa_pnode = nullptr;
+  int32_t *d_sa_vptr = nullptr, *d_sa_vagg = nullptr, *d_sa_vnode = nullptr;
+  int32_t *d_sa_sptr = nullptr, *d_sa_snode = nullptr, *d_sa_sidx = nullptr;
+  double *d_sa_P = nullptr, *d_sa_V = nullptr;  // sa_np x 18, sa_nv x 18
+  int32_t *d_sa_ublk = nullptr;      // the next level's upper blocks (I <= J), in block order
+  int64_t sa_nub = 0;
+  double sa_omega = 0.0;       // omega of the next refresh
+  double sa_omega_used = 0.0;  // omega of the last one
+  // contribution lists built once at setup (fixed order): per prolongator
+  // entry the slots of its row whose column lies in its aggregate; per entry
+  // of A P the (slot, prolongator entry) pairs; per upper block of level 1 the
+  // (prolongator entry, A P entry) pairs of its support nodes
+  int32_t *d_sa_wptr = nullptr, *d_sa_wslot = nullptr;
+  int32_t *d_sa_vlptr = nullptr, *d_sa_vlslot = nullptr, *d_sa_vlent = nullptr;
+  int32_t *d_sa_gptr = nullptr, *d_sa_gp = nullptr, *d_sa_gv = nullptr;
+  // the cycle's copies of P: sliced ELL by node (slice of 64 nodes: entry q
+  // of lane n, value m at poff*18 + (18 q + m) 64 + n; aggregate -1 on
+  // padding) for the prolongation, support order for the restriction
+  int64_t sa_ne_ell = 0;
+  int64_t *d_sa_poff = nullptr;      // per slice, first entry slot (nslice + 1)
+  int32_t *d_sa_pw = nullptr;        // per slice, entry slots per node
+  int32_t *d_sa_eagg = nullptr;      // per entry slot, aggregate or -1
+  int32_t *d_sa_spos = nullptr;      // per entry, its position in the support order
+  double *d_sa_Pe = nullptr, *d_sa_Ps = nullptr;  // FP64 cycle
+  float *d_sa_Pef = nullptr, *d_sa_Psf = nullptr; // FP32 cycle
   // deterministic Galerkin contribution lists for the next level's blocks
   int64_t ngl = 0;
   int32_t *d_glptr = nullptr;  // per next-level block
@@ -189,6 +218,13 @@ struct AmgHostLevel {
   // Galerkin lists for the next level blocks (level 0: per upper block, the
   // elements of its entries, the entries' block and the transposed block)
   std::vector<int32_t> glptr, gl, gblk, tr;
+  // level 0 with a smoothed prolongator (sa): per node the aggregates of its
+  // prolongator row (its own and its neighbours', sorted), per node the
+  // aggregates of its row of A P (the union over its neighbours' rows), and
+  // per aggregate its support: the (node, prolongator entry) pairs naming it,
+  // in node order
+  bool sa = false;
+  std::vector<int32_t> sa_pptr, sa_pagg, sa_vptr, sa_vagg, sa_sptr, sa_snode, sa_sidx;
 };
 
 // --------------------------------------------------------------------------
@@ -292,6 +328,8 @@ struct uft_ctx {
   const double *cur_xphys = nullptr;  // density of the operator being solved
   double fnorm = 0.0;
   double *d_lanczos = nullptr;       // CG scalar log for eigenvalue estimates
+
+  double *d_a64 = nullptr;  // fine operator 3x3 node blocks in FP64 on the node pattern (smoothed prolongator)
 
   // uft_solve_traffic: algorithmic bytes of the operator applies made inside
   // the elasticity solves (hierarchy refresh excluded) and the solves' time
@@ -529,7 +567,22 @@ void gather_op(hipStream_t s, const uft_ctx *C, const float *u, const float *Ye,
 // assembled FP16 fine operator: assembly from the per-element moduli E(x_e)
 // (rows scaled by the level-0 diagonal 1 / dinv) and the apply with the row
 // epilogue (block partials over gather_op_blocks(nn) blocks)
-void fine_ell_fill(hipStream_t s, const uft_ctx *C, const double *Ee, const double *dinv);
+void fine_ell_fill(hipStream_t s, const uft_ctx *C, const double *Ee, const double *dinv, double *a64 = nullptr);
+// smoothed prolongator of level 0 and the level-1 operator P^T A P from the
+// FP64 blocks d_a64, the fine diagonal and L[0].sa_omega (kernels_sa.hip)
+void sa_refresh(hipStream_t s, uft_ctx *C);
+// setup: the contribution lists (which 0: prolongator slots, 1: A P pairs,
+// 2: Galerkin pairs); count writes the list length per row, fill the lists
+// at the row offsets ptr
+void sa_lists(hipStream_t s, const uft_ctx *C, int which, const int32_t *ptr, int32_t *cnt_or_a, int32_t *b);
+void sa_restrict_first(hipStream_t s, const AmgLevel &L0, int64_t nc, const double *r, double *bc, const double *dinv,
+                       double it, double *rc, double *dc, double *xc);
+void sa_restrict_first(hipStream_t s, const AmgLevel &L0, int64_t nc, const float *r, float *bc, const float *dinv,
+                       double it, float *rc, float *dc, float *xc);
+void sa_prolong_comb(hipStream_t s, const AmgLevel &L0, const double *sc, int ia, int ib, const double *xa,
+                     const double *xb, double *x);
+void sa_prolong_comb(hipStream_t s, const AmgLevel &L0, const double *sc, int ia, int ib, const float *xa,
+                     const float *xb, float *x);
 void fine_ell_op(hipStream_t s, const uft_ctx *C, const float *u, const RowOp<float> &op);
 // the same pattern with FP32 entries (unscaled): the mixed forms' outer operator
 void fine_ell_op32(hipStream_t s, const uft_ctx *C, const float *u, const RowOp<float> &op);
@@ -720,7 +773,7 @@ int uft_set_error(uft_ctx *C, int code, const char *fmt, ...);
   } while (0)
 
 // amg setup (uft_amg_setup.cpp)
-int uft_amg_build_host(uft_ctx *C, std::vector<AmgHostLevel> &H, std::vector<int64_t> &nunits);
+int uft_amg_build_host(uft_ctx *C, std::vector<AmgHostLevel> &H, std::vector<int64_t> &nunits, bool sa);
 // mesh helpers (uft_mesh.cpp)
 void uft_node_incidence(int64_t nn, int64_t ne, int npe, const int32_t *elem, std::vector<int32_t> &ptr,
                         std::vector<int32_t> &inc);

@@ -37,7 +37,8 @@ class Params(C.Structure):
                 ("coarse_max", C.c_int), ("cycle_type", C.c_int),
                 ("cheb_lo", C.c_double), ("cheb_hi", C.c_double), ("esteig_steps", C.c_int),
                 ("load_cases", C.c_int), ("graph", C.c_int), ("cheb_degree_coarse", C.c_int),
-                ("fine_assembled", C.c_int)]
+                ("fine_assembled", C.c_int), ("coarse_dense_max", C.c_int),
+                ("prolong_smooth", C.c_int)]
 
 
 _LIB = None
@@ -131,6 +132,7 @@ def lib():
         "uft_bc_read_text": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, ip, ip, dp]),
         "uft_write_vtu": (C.c_int, [vp, C.c_char_p, C.c_int, C.c_int, C.c_int]),
         "uft_solve_traffic": (C.c_int, [vp, pd, pd, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+        "uft_amg_prolongator_get": (C.c_int, [vp, C.POINTER(C.c_int64), vp, vp, vp, pd]),
         "uft_vtu_write_host": (C.c_int, [C.c_char_p, C.c_int, C.c_int64, C.c_int64, dp, ip, dp, dp, C.c_int]),
     }
     for name, (res, args) in sig.items():
@@ -510,6 +512,23 @@ class Context:
         P = np.zeros(nu * bs * 6)
         self._chk(self.L.uft_amg_transfer_get(self.h, level, parent, P), "uft_amg_transfer_get")
         return parent, P.reshape(nu, bs, 6)
+
+    def amg_prolongator(self):
+        """Smoothed prolongator of level 0 (pptr, pagg, P[nent, 3, 6], omega), or None
+        with the tentative one."""
+        ne_, om = C.c_int64(), C.c_double()
+        self._chk(self.L.uft_amg_prolongator_get(self.h, C.byref(ne_), None, None, None, C.byref(om)),
+                  "uft_amg_prolongator_get")
+        if ne_.value == 0:
+            return None
+        nn = self.size("nn")
+        pptr = np.zeros(nn + 1, np.int32)
+        pagg = np.zeros(ne_.value, np.int32)
+        P = np.zeros(ne_.value * 18)
+        self._chk(self.L.uft_amg_prolongator_get(self.h, C.byref(ne_), pptr.ctypes.data_as(C.c_void_p),
+                                                 pagg.ctypes.data_as(C.c_void_p), P.ctypes.data_as(C.c_void_p),
+                                                 C.byref(om)), "uft_amg_prolongator_get")
+        return pptr, pagg, P.reshape(-1, 3, 6), om.value
 
     def amg_coarse_inverse(self):
         n = self.solver_info()[0][-1]
