@@ -392,7 +392,7 @@ __global__ __launch_bounds__(TPB) void k_fine_ell_fill(int nn, int ne, const int
                                                        uint16_t *__restrict__ v1, uint2 *__restrict__ q8,
                                                        uint32_t *__restrict__ qs, float *__restrict__ dg,
                                                        float4 *__restrict__ w8, float *__restrict__ w1,
-                                                       double *__restrict__ a64) {
+                                                       double *__restrict__ a64, float *__restrict__ a32) {
   constexpr int ND = 3 * NPE;
   const int n = blockIdx.x * TPB + threadIdx.x;
   const int s = blockIdx.y;
@@ -427,7 +427,11 @@ __global__ __launch_bounds__(TPB) void k_fine_ell_fill(int nn, int ne, const int
 #pragma unroll
     for (int t = 0; t < 9; ++t) a64[9 * k + t] = b[t];
   }
-  if (!q8 && !v8) return;  // FP64 blocks only (the smoothed prolongator's operator)
+  if (a32) {
+#pragma unroll
+    for (int t = 0; t < 9; ++t) a32[9 * k + t] = (float)b[t];
+  }
+  if (!q8 && !v8) return;  // the smoothed prolongator's blocks only
   if (w8) {
     w8[2 * k] = make_float4((float)b[0], (float)b[1], (float)b[2], (float)b[3]);
     w8[2 * k + 1] = make_float4((float)b[4], (float)b[5], (float)b[6], (float)b[7]);
@@ -931,35 +935,37 @@ void gather_op(hipStream_t s, const uft_ctx *C, const float *u, const float *Ye,
 }
 int gather_op_blocks(int64_t nn) { return (int)nblk(nn); }
 
-// the mixed forms' blocks (from the FP32 element blocks) and, when a64 is
-// given, the FP64 blocks too; without the mixed forms' blocks only the FP64
-// ones, from the FP64 element blocks
-void fine_ell_fill(hipStream_t s, const uft_ctx *C, const double *Ee, const double *dinv, double *a64) {
+// the mixed forms' smoother and outer blocks (from the FP32 element blocks)
+// and, when a64 / a32 is given, the smoothed prolongator's unscaled blocks
+// in FP64 / FP32 too; without the mixed forms' blocks only the latter, from
+// the FP64 element blocks (FP64 form) or the FP32 ones
+void fine_ell_fill(hipStream_t s, const uft_ctx *C, const double *Ee, const double *dinv, double *a64, float *a32) {
   const int nn = (int)C->nn, ne = (int)C->ne;
   const auto &F = C->fell;
   const dim3 g(nblk(nn), (unsigned)F.wmax);
-  if (F.d_q8 || F.d_v8) {
+  const bool blocks = F.d_q8 || F.d_v8;
+  if (!blocks && !a64 && !a32) return;
+  if (blocks || !C->d_KE) {
     const float2 *KEf = (const float2 *)C->d_KEf;
     if (C->npe == 8)
       k_fine_ell_fill<8, float2><<<g, TPB, 0, s>>>(nn, ne, F.d_off, F.d_width, F.d_col, F.d_cptr, F.d_clist, KEf, Ee,
                                                    C->d_fixm, dinv, F.d_v8, F.d_v1, F.d_q8, F.d_qs, F.d_dg, F.d_w8,
-                                                   F.d_w1, a64);
+                                                   F.d_w1, a64, a32);
     else
       k_fine_ell_fill<4, float2><<<g, TPB, 0, s>>>(nn, ne, F.d_off, F.d_width, F.d_col, F.d_cptr, F.d_clist, KEf, Ee,
                                                    C->d_fixm, dinv, F.d_v8, F.d_v1, F.d_q8, F.d_qs, F.d_dg, F.d_w8,
-                                                   F.d_w1, a64);
+                                                   F.d_w1, a64, a32);
     return;
   }
-  if (!a64) return;
   const double2 *KE = (const double2 *)C->d_KE;
   if (C->npe == 8)
     k_fine_ell_fill<8, double2><<<g, TPB, 0, s>>>(nn, ne, F.d_off, F.d_width, F.d_col, F.d_cptr, F.d_clist, KE, Ee,
                                                   C->d_fixm, dinv, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
-                                                  nullptr, a64);
+                                                  nullptr, a64, a32);
   else
     k_fine_ell_fill<4, double2><<<g, TPB, 0, s>>>(nn, ne, F.d_off, F.d_width, F.d_col, F.d_cptr, F.d_clist, KE, Ee,
                                                   C->d_fixm, dinv, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
-                                                  nullptr, a64);
+                                                  nullptr, a64, a32);
 }
 void fine_ell_op32(hipStream_t s, const uft_ctx *C, const float *u, const RowOp<float> &op) {
   const auto &F = C->fell;

@@ -16,7 +16,7 @@
 //                     symmetrised
 // Each runs over contribution lists built once at setup from the patterns
 // (k_sa_scan_*), so the refresh does no searching.  A is the fine operator
-// assembled on the node pattern (FP64 3x3 blocks, fixed columns zeroed; a
+// assembled on the node pattern (3x3 blocks, fixed columns zeroed; a
 // fixed row of A is the identity row, so its rows of A P_tent, P and A P
 // vanish because P_tent vanishes there).  Every sum runs in a fixed order:
 // results do not depend on scheduling.
@@ -37,27 +37,13 @@ __device__ __forceinline__ int64_t ell_slot(const int64_t *off, int n, int q) {
   return off[n / UFT_ELL_SLICE] + (int64_t)q * UFT_ELL_SLICE + n % UFT_ELL_SLICE;
 }
 
-// acc (3x6) += A (3x3, row-major) * B (3x6, row-major)
-__device__ __forceinline__ void mac33x36(double (&acc)[18], const double *__restrict__ A,
-                                         const double *__restrict__ B) {
-  double a[9], b[18];
-#pragma unroll
-  for (int t = 0; t < 9; ++t) a[t] = A[t];
-#pragma unroll
-  for (int t = 0; t < 18; ++t) b[t] = B[t];
-#pragma unroll
-  for (int c = 0; c < 3; ++c)
-#pragma unroll
-    for (int m = 0; m < 6; ++m) acc[6 * c + m] += a[3 * c] * b[m] + a[3 * c + 1] * b[6 + m] + a[3 * c + 2] * b[12 + m];
-}
-
 // ---------------------------------------------------------------- setup: lists
 // which 0, per prolongator entry t = (i, K): the slots of row i whose column
 // lies in K, in slot order
-// which 1, per entry t = (i, J) of A P: (slot of row i, the column's
-// prolongator entry naming J), in slot order
-// which 2, per upper block (I, J): (prolongator entry of I at support node i,
-// entry of A P of node i naming J), in support (node) order
+// which 1, per entry t = (i, J) of A P: (slot of row i, support position of
+// the column's prolongator entry naming J), in slot order
+// which 2, per upper block (I, J): (support position q of I at node i,
+// entry of A P of node i naming J), in support order
 // count: a = per-row counts; fill: a, b = the lists at the rows' offsets ptr
 __global__ __launch_bounds__(TPB) void k_sa_scan_w(int64_t np, const int32_t *__restrict__ pnode,
                                                    const int32_t *__restrict__ pagg, const int64_t *__restrict__ off,
@@ -82,8 +68,9 @@ __global__ __launch_bounds__(TPB) void k_sa_scan_v(int64_t nv, const int32_t *__
                                                    const int32_t *__restrict__ vagg, const int64_t *__restrict__ off,
                                                    const int32_t *__restrict__ width, const int32_t *__restrict__ col,
                                                    const int32_t *__restrict__ cptr, const int32_t *__restrict__ pptr,
-                                                   const int32_t *__restrict__ pagg, const int32_t *__restrict__ ptr,
-                                                   int32_t *__restrict__ a, int32_t *__restrict__ b) {
+                                                   const int32_t *__restrict__ pagg, const int32_t *__restrict__ spos,
+                                                   const int32_t *__restrict__ ptr, int32_t *__restrict__ a,
+                                                   int32_t *__restrict__ b) {
   const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
   if (t >= nv) return;
   const int i = vnode[t], J = vagg[t];
@@ -99,7 +86,7 @@ __global__ __launch_bounds__(TPB) void k_sa_scan_v(int64_t nv, const int32_t *__
       if (g == J) {
         if (ptr) {
           a[ptr[t] + c] = (int32_t)k;
-          b[ptr[t] + c] = e;
+          b[ptr[t] + c] = spos[e];
         }
         ++c;
       }
@@ -130,7 +117,7 @@ __global__ __launch_bounds__(TPB) void k_sa_scan_g(int64_t nub, const int32_t *_
     }
     if (lo >= vptr[i + 1] || vagg[lo] != J) continue;
     if (ptr) {
-      a[ptr[u] + c] = sidx[q];
+      a[ptr[u] + c] = q;  // support position
       b[ptr[u] + c] = lo;
     }
     ++c;
@@ -139,26 +126,48 @@ __global__ __launch_bounds__(TPB) void k_sa_scan_g(int64_t nub, const int32_t *_
 }
 
 // ---------------------------------------------------------------- refresh
+// The refresh kernels read and write in the cycle's precision T (FP64 in the
+// FP64 form, FP32 in the mixed forms, whose coarse operators are FP32 or
+// lower) and accumulate in FP64.  Their blocks are handed out XCD by XCD in
+// contiguous ranges (block b runs on XCD b mod 8), so neighbouring rows,
+// which share their operands, share one L2.
+__device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t nb) {
+  const int64_t x = b % 8, q = nb / 8, r = nb % 8;
+  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + b / 8;
+}
+
+// acc (3x6) += A (3x3, row-major) * B (3x6, row-major)
+template <typename T>
+__device__ __forceinline__ void mac33x36t(double (&acc)[18], const T *__restrict__ A, const T *__restrict__ B) {
+  double a[9], b[18];
+#pragma unroll
+  for (int t = 0; t < 9; ++t) a[t] = (double)A[t];
+#pragma unroll
+  for (int t = 0; t < 18; ++t) b[t] = (double)B[t];
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int m = 0; m < 6; ++m) acc[6 * c + m] += a[3 * c] * b[m] + a[3 * c + 1] * b[6 + m] + a[3 * c + 2] * b[12 + m];
+}
+
 // one thread per prolongator entry t = (node i, aggregate K):
 // W = sum over its slots of A_slot P_tent(col); P[t] = [K == parent(i)]
-// P_tent(i) - omega D^-1 W, stored FP64 (entry order), and in the cycle's
-// precision T as the node-sliced copy and the support-ordered copy
+// P_tent(i) - omega D^-1 W, stored as the node-sliced copy and the
+// support-ordered copy
 template <typename T>
 __global__ __launch_bounds__(TPB) void k_sa_prolongator(int64_t np, const int32_t *__restrict__ pnode,
                                                         const int32_t *__restrict__ pagg,
                                                         const int32_t *__restrict__ pptr,
                                                         const int32_t *__restrict__ wptr,
                                                         const int32_t *__restrict__ wslot,
-                                                        const int32_t *__restrict__ col,
-                                                        const double *__restrict__ a64,
-                                                        const int32_t *__restrict__ parent,
-                                                        const double *__restrict__ Pt,
+                                                        const int32_t *__restrict__ col, const T *__restrict__ a,
+                                                        const int32_t *__restrict__ parent, const T *__restrict__ Pt,
                                                         const double *__restrict__ dinv,
                                                         const uint8_t *__restrict__ fixm, double omega,
                                                         const int64_t *__restrict__ poff,
-                                                        const int32_t *__restrict__ spos, double *__restrict__ P,
-                                                        T *__restrict__ Pe, T *__restrict__ Ps) {
-  const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
+                                                        const int32_t *__restrict__ spos, T *__restrict__ Pe,
+                                                        T *__restrict__ Ps) {
+  const int64_t t = xcd_block(blockIdx.x, gridDim.x) * TPB + threadIdx.x;
   if (t >= np) return;
   const int i = pnode[t];
   const int K = pagg[t];
@@ -167,60 +176,58 @@ __global__ __launch_bounds__(TPB) void k_sa_prolongator(int64_t np, const int32_
   for (int m = 0; m < 18; ++m) acc[m] = 0.0;
   for (int q = wptr[t]; q < wptr[t + 1]; ++q) {
     const int64_t k = wslot[q];
-    mac33x36(acc, a64 + 9 * k, Pt + 18 * (size_t)col[k]);
+    mac33x36t<T>(acc, a + 9 * k, Pt + 18 * (size_t)col[k]);
   }
   const bool own = parent[i] == K;
   const unsigned fm = fixm[i];
-  double out[18];
+  const int q = (int)(t - pptr[i]);
+  T *pe = Pe + poff[i / UFT_ELL_SLICE] * 18 + (int64_t)q * 18 * UFT_ELL_SLICE + i % UFT_ELL_SLICE;
+  T *ps = Ps + (size_t)spos[t] * 18;
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     const double di = dinv[3 * (size_t)i + c];
 #pragma unroll
     for (int m = 0; m < 6; ++m) {
-      const double pt = own ? Pt[18 * (size_t)i + 6 * c + m] : 0.0;
-      out[6 * c + m] = ((fm >> c) & 1u) ? 0.0 : pt - omega * di * acc[6 * c + m];
+      const double pt = own ? (double)Pt[18 * (size_t)i + 6 * c + m] : 0.0;
+      const T v = (T)(((fm >> c) & 1u) ? 0.0 : pt - omega * di * acc[6 * c + m]);
+      pe[(6 * c + m) * UFT_ELL_SLICE] = v;
+      ps[6 * c + m] = v;
     }
-  }
-#pragma unroll
-  for (int m = 0; m < 18; ++m) P[18 * t + m] = out[m];
-  const int q = (int)(t - pptr[i]);
-  T *pe = Pe + poff[i / UFT_ELL_SLICE] * 18 + (int64_t)q * 18 * UFT_ELL_SLICE + i % UFT_ELL_SLICE;
-  T *ps = Ps + (size_t)spos[t] * 18;
-#pragma unroll
-  for (int m = 0; m < 18; ++m) {
-    pe[m * UFT_ELL_SLICE] = (T)out[m];
-    ps[m] = (T)out[m];
   }
 }
 
-// one thread per entry t of A P: the sum over its (slot, prolongator entry)
-// pairs of A_slot P_entry (rows of fixed DOFs zero)
+// one thread per entry t of A P: the sum over its (slot, support position)
+// pairs of A_slot P_position (rows of fixed DOFs zero)
+template <typename T>
 __global__ __launch_bounds__(TPB) void k_sa_ap(int64_t nv, const int32_t *__restrict__ vnode,
                                                const int32_t *__restrict__ vlptr, const int32_t *__restrict__ vlslot,
-                                               const int32_t *__restrict__ vlent, const double *__restrict__ a64,
-                                               const double *__restrict__ P, const uint8_t *__restrict__ fixm,
-                                               double *__restrict__ V) {
-  const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
+                                               const int32_t *__restrict__ vlpos, const T *__restrict__ a,
+                                               const T *__restrict__ Ps, const uint8_t *__restrict__ fixm,
+                                               T *__restrict__ V) {
+  const int64_t t = xcd_block(blockIdx.x, gridDim.x) * TPB + threadIdx.x;
   if (t >= nv) return;
   double acc[18];
 #pragma unroll
   for (int m = 0; m < 18; ++m) acc[m] = 0.0;
-  for (int q = vlptr[t]; q < vlptr[t + 1]; ++q) mac33x36(acc, a64 + 9 * (size_t)vlslot[q], P + 18 * (size_t)vlent[q]);
+  for (int q = vlptr[t]; q < vlptr[t + 1]; ++q)
+    mac33x36t<T>(acc, a + 9 * (size_t)vlslot[q], Ps + 18 * (size_t)vlpos[q]);
   const unsigned fm = fixm[vnode[t]];
 #pragma unroll
   for (int c = 0; c < 3; ++c)
 #pragma unroll
-    for (int m = 0; m < 6; ++m) V[18 * t + 6 * c + m] = ((fm >> c) & 1u) ? 0.0 : acc[6 * c + m];
+    for (int m = 0; m < 6; ++m) V[18 * t + 6 * c + m] = (T)(((fm >> c) & 1u) ? 0.0 : acc[6 * c + m]);
 }
 
-// one wavefront per upper block b = (I, J), I <= J, of the level-1 pattern
+// one wavefront per upper block b = (I, J), I <= J, of the level-1 pattern;
+// its pairs (support position of I, entry of A P) run in support order
+template <typename T>
 __global__ __launch_bounds__(TPB) void k_sa_galerkin(int64_t nub, const int32_t *__restrict__ ublk,
                                                      const int32_t *__restrict__ brow, const int32_t *__restrict__ bci,
                                                      const int32_t *__restrict__ btr, const int32_t *__restrict__ gptr,
                                                      const int32_t *__restrict__ gp, const int32_t *__restrict__ gv,
-                                                     const double *__restrict__ P, const double *__restrict__ V,
+                                                     const T *__restrict__ Ps, const T *__restrict__ V,
                                                      double *__restrict__ val) {
-  const int64_t wv = ((int64_t)blockIdx.x * TPB + threadIdx.x) / 64;
+  const int64_t wv = (xcd_block(blockIdx.x, gridDim.x) * TPB + threadIdx.x) / 64;
   const int lane = threadIdx.x & 63;
   if (wv >= nub) return;  // uniform per wavefront
   const int b = ublk[wv];
@@ -228,13 +235,13 @@ __global__ __launch_bounds__(TPB) void k_sa_galerkin(int64_t nub, const int32_t 
 #pragma unroll
   for (int e = 0; e < 36; ++e) acc[e] = 0.0;
   for (int q = gptr[wv] + lane; q < gptr[wv + 1]; q += 64) {
-    const double *p = P + 18 * (size_t)gp[q];
-    const double *v = V + 18 * (size_t)gv[q];
+    const T *p = Ps + 18 * (size_t)gp[q];
+    const T *v = V + 18 * (size_t)gv[q];
     double pr[18], vr[18];
 #pragma unroll
     for (int m = 0; m < 18; ++m) {
-      pr[m] = p[m];
-      vr[m] = v[m];
+      pr[m] = (double)p[m];
+      vr[m] = (double)v[m];
     }
 #pragma unroll
     for (int a = 0; a < 6; ++a)
@@ -360,32 +367,31 @@ void sa_lists(hipStream_t s, const uft_ctx *C, int which, const int32_t *ptr, in
                                                F.d_cptr, L0.d_parent, ptr, a);
   else if (which == 1)
     k_sa_scan_v<<<nblk(L0.sa_nv), TPB, 0, s>>>(L0.sa_nv, L0.d_sa_vnode, L0.d_sa_vagg, F.d_off, F.d_width, F.d_col,
-                                               F.d_cptr, L0.d_sa_pptr, L0.d_sa_pagg, ptr, a, b);
+                                               F.d_cptr, L0.d_sa_pptr, L0.d_sa_pagg, L0.d_sa_spos, ptr, a, b);
   else
     k_sa_scan_g<<<nblk(L0.sa_nub), TPB, 0, s>>>(L0.sa_nub, L0.d_sa_ublk, L1.d_brow, L1.d_ci, L0.d_sa_sptr,
                                                 L0.d_sa_snode, L0.d_sa_sidx, L0.d_sa_vptr, L0.d_sa_vagg, ptr, a, b);
 }
 
+template <typename T>
+static void sa_refresh_t(hipStream_t s, const uft_ctx *C, const T *a, const T *Pt, T *Pe, T *Ps, T *V) {
+  const AmgLevel &L0 = C->amg.L[0], &L1 = C->amg.L[1];
+  k_sa_prolongator<T><<<nblk(L0.sa_np), TPB, 0, s>>>(L0.sa_np, L0.d_sa_pnode, L0.d_sa_pagg, L0.d_sa_pptr, L0.d_sa_wptr,
+                                                     L0.d_sa_wslot, C->fell.d_col, a, L0.d_parent, Pt, L0.d_dinv,
+                                                     C->d_fixm, L0.sa_omega, L0.d_sa_poff, L0.d_sa_spos, Pe, Ps);
+  k_sa_ap<T><<<nblk(L0.sa_nv), TPB, 0, s>>>(L0.sa_nv, L0.d_sa_vnode, L0.d_sa_vlptr, L0.d_sa_vlslot, L0.d_sa_vlent, a,
+                                            Ps, C->d_fixm, V);
+  k_sa_galerkin<T><<<nblk(L0.sa_nub * 64), TPB, 0, s>>>(L0.sa_nub, L0.d_sa_ublk, L1.d_brow, L1.d_ci, L0.d_gtr,
+                                                        L0.d_sa_gptr, L0.d_sa_gp, L0.d_sa_gv, Ps, V, L1.d_val);
+}
+
 void sa_refresh(hipStream_t s, uft_ctx *C) {
   AmgLevel &L0 = C->amg.L[0];
-  const AmgLevel &L1 = C->amg.L[1];
   L0.sa_omega_used = L0.sa_omega;
-  const auto &F = C->fell;
   if (L0.d_sa_Pef)
-    k_sa_prolongator<float><<<nblk(L0.sa_np), TPB, 0, s>>>(
-        L0.sa_np, L0.d_sa_pnode, L0.d_sa_pagg, L0.d_sa_pptr, L0.d_sa_wptr, L0.d_sa_wslot, F.d_col, C->d_a64,
-        L0.d_parent, L0.d_P, L0.d_dinv, C->d_fixm, L0.sa_omega, L0.d_sa_poff, L0.d_sa_spos, L0.d_sa_P, L0.d_sa_Pef,
-        L0.d_sa_Psf);
+    sa_refresh_t<float>(s, C, C->d_a32, L0.d_Pf, L0.d_sa_Pef, L0.d_sa_Psf, L0.d_sa_Vf);
   else
-    k_sa_prolongator<double><<<nblk(L0.sa_np), TPB, 0, s>>>(
-        L0.sa_np, L0.d_sa_pnode, L0.d_sa_pagg, L0.d_sa_pptr, L0.d_sa_wptr, L0.d_sa_wslot, F.d_col, C->d_a64,
-        L0.d_parent, L0.d_P, L0.d_dinv, C->d_fixm, L0.sa_omega, L0.d_sa_poff, L0.d_sa_spos, L0.d_sa_P, L0.d_sa_Pe,
-        L0.d_sa_Ps);
-  k_sa_ap<<<nblk(L0.sa_nv), TPB, 0, s>>>(L0.sa_nv, L0.d_sa_vnode, L0.d_sa_vlptr, L0.d_sa_vlslot, L0.d_sa_vlent,
-                                         C->d_a64, L0.d_sa_P, C->d_fixm, L0.d_sa_V);
-  k_sa_galerkin<<<nblk(L0.sa_nub * 64), TPB, 0, s>>>(L0.sa_nub, L0.d_sa_ublk, L1.d_brow, L1.d_ci, L0.d_gtr,
-                                                     L0.d_sa_gptr, L0.d_sa_gp, L0.d_sa_gv, L0.d_sa_P, L0.d_sa_V,
-                                                     L1.d_val);
+    sa_refresh_t<double>(s, C, C->d_a64, L0.d_P, L0.d_sa_Pe, L0.d_sa_Ps, L0.d_sa_V);
 }
 
 void sa_restrict_first(hipStream_t s, const AmgLevel &L0, int64_t nc, const double *r, double *bc, const double *dinv,

@@ -1081,7 +1081,7 @@ int amg_refresh_impl(uft_ctx *C, const double *xphys) {
     // fine level's estimate runs on a side stream beside the products.
     AmgLevel &L0 = H.L[0];
     uftk::simp_moduli(s, C, xphys, C->d_Ee);
-    uftk::fine_ell_fill(s, C, C->d_Ee, L0.d_dinv, C->d_a64);
+    uftk::fine_ell_fill(s, C, C->d_Ee, L0.d_dinv, C->d_a64, C->d_a32);
     const double hi = C->P.cheb_hi;
     if (par && L0.sa_omega > 0.0) {
       UFT_HIP(C, hipEventRecord(C->ev_side[3], s));
@@ -2050,6 +2050,10 @@ int sa_list(uft_ctx *C, int which, int64_t rows, int32_t **ptr, int32_t **a, int
 int sa_setup_lists(uft_ctx *C, const AmgHostLevel &h) {
   AmgLevel &L0 = C->amg.L[0];
   int rc;
+  // support position of every entry (the A P lists name entries by it)
+  std::vector<int32_t> spos(L0.sa_np, 0);
+  for (size_t q = 0; q < h.sa_sidx.size(); ++q) spos[h.sa_sidx[q]] = (int32_t)q;
+  if ((rc = upload(C, &L0.d_sa_spos, spos))) return rc;
   if ((rc = sa_list(C, 0, L0.sa_np, &L0.d_sa_wptr, &L0.d_sa_wslot, nullptr)) ||
       (rc = sa_list(C, 1, L0.sa_nv, &L0.d_sa_vlptr, &L0.d_sa_vlslot, &L0.d_sa_vlent)) ||
       (rc = sa_list(C, 2, L0.sa_nub, &L0.d_sa_gptr, &L0.d_sa_gp, &L0.d_sa_gv)))
@@ -2064,16 +2068,15 @@ int sa_setup_lists(uft_ctx *C, const AmgHostLevel &h) {
     pw[w] = mx;
     poff[w + 1] = poff[w] + (int64_t)mx * UFT_ELL_SLICE;
   }
-  std::vector<int32_t> eagg(poff[nsl], -1), spos(L0.sa_np, 0);
+  std::vector<int32_t> eagg(poff[nsl], -1);
   for (int64_t n = 0; n < nn; ++n) {
     const int64_t w = n / UFT_ELL_SLICE, lane = n % UFT_ELL_SLICE;
     for (int32_t t = h.sa_pptr[n]; t < h.sa_pptr[n + 1]; ++t)
       eagg[poff[w] + (int64_t)(t - h.sa_pptr[n]) * UFT_ELL_SLICE + lane] = h.sa_pagg[t];
   }
-  for (size_t q = 0; q < h.sa_sidx.size(); ++q) spos[h.sa_sidx[q]] = (int32_t)q;
   L0.sa_ne_ell = poff[nsl];
   if ((rc = upload(C, &L0.d_sa_poff, poff)) || (rc = upload(C, &L0.d_sa_pw, pw)) ||
-      (rc = upload(C, &L0.d_sa_eagg, eagg)) || (rc = upload(C, &L0.d_sa_spos, spos)))
+      (rc = upload(C, &L0.d_sa_eagg, eagg)))
     return rc;
   if (is_mixed(C)) {
     // (padding slots are never read: the prolongation stops at aggregate -1)
@@ -2375,11 +2378,14 @@ int uft_setup(uft_ctx *C) {
             (rc = upload(C, &Lv.d_sa_vptr, h.sa_vptr)) || (rc = upload(C, &Lv.d_sa_vagg, h.sa_vagg)) ||
             (rc = upload(C, &Lv.d_sa_vnode, vnode)) || (rc = upload(C, &Lv.d_sa_sptr, h.sa_sptr)) ||
             (rc = upload(C, &Lv.d_sa_snode, h.sa_snode)) || (rc = upload(C, &Lv.d_sa_sidx, h.sa_sidx)) ||
-            (rc = upload(C, &Lv.d_sa_ublk, ublk)) || (rc = dalloc_n(C, &Lv.d_sa_P, 18 * Lv.sa_np)) ||
-            (rc = dalloc_n(C, &Lv.d_sa_V, 18 * Lv.sa_nv)) || (rc = dalloc_n(C, &C->d_Ee, C->ne)) ||
-            (rc = dalloc_n(C, &C->d_a64, 9 * C->fell.nslot)))
+            (rc = upload(C, &Lv.d_sa_ublk, ublk)) || (rc = dalloc_n(C, &C->d_Ee, C->ne)))
           return rc;
-        uftk::fill(s, 9 * C->fell.nslot, 0.0, C->d_a64);
+        if (is_mixed(C)) {
+          if ((rc = dalloc_n(C, &Lv.d_sa_Vf, 18 * Lv.sa_nv)) || (rc = dalloc_n(C, &C->d_a32, 9 * C->fell.nslot)))
+            return rc;
+        } else if ((rc = dalloc_n(C, &Lv.d_sa_V, 18 * Lv.sa_nv)) || (rc = dalloc_n(C, &C->d_a64, 9 * C->fell.nslot))) {
+          return rc;
+        }
         if ((rc = sa_setup_lists(C, h))) return rc;
       } else if (l == 0) {  // density-free Galerkin entries of the fine level
         int32_t *d_gblk;
@@ -3250,8 +3256,23 @@ extern "C" int uft_amg_prolongator_get(uft_ctx *C, int64_t *nent, int32_t *pptr,
   if (!L0.sa) return UFT_OK;
   if (pptr) UFT_HIP(C, hipMemcpyAsync(pptr, L0.d_sa_pptr, (L0.nunit + 1) * sizeof(int32_t), hipMemcpyDeviceToHost, C->s));
   if (pagg) UFT_HIP(C, hipMemcpyAsync(pagg, L0.d_sa_pagg, L0.sa_np * sizeof(int32_t), hipMemcpyDeviceToHost, C->s));
-  if (P) UFT_HIP(C, hipMemcpyAsync(P, L0.d_sa_P, L0.sa_np * 18 * sizeof(double), hipMemcpyDeviceToHost, C->s));
-  return sync(C);
+  if (int rc = sync(C)) return rc;
+  if (P) {  // support order, in the cycle's precision -> entry order, FP64
+    std::vector<int32_t> spos(L0.sa_np);
+    std::vector<double> ps(18 * L0.sa_np);
+    UFT_HIP(C, hipMemcpyAsync(spos.data(), L0.d_sa_spos, L0.sa_np * sizeof(int32_t), hipMemcpyDeviceToHost, C->s));
+    if (L0.d_sa_Psf) {
+      std::vector<float> pf(18 * L0.sa_np);
+      UFT_HIP(C, hipMemcpyAsync(pf.data(), L0.d_sa_Psf, pf.size() * sizeof(float), hipMemcpyDeviceToHost, C->s));
+      if (int rc = sync(C)) return rc;
+      std::copy(pf.begin(), pf.end(), ps.begin());
+    } else {
+      UFT_HIP(C, hipMemcpyAsync(ps.data(), L0.d_sa_Ps, ps.size() * sizeof(double), hipMemcpyDeviceToHost, C->s));
+      if (int rc = sync(C)) return rc;
+    }
+    for (int64_t t = 0; t < L0.sa_np; ++t) std::memcpy(P + 18 * t, &ps[18 * (size_t)spos[t]], 18 * sizeof(double));
+  }
+  return UFT_OK;
 }
 
 extern "C" int uft_amg_coarse_inverse_get(uft_ctx *C, double *inv) {

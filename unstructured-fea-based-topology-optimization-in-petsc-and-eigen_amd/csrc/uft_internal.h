@@ -97,7 +97,8 @@ struct AmgLevel {
   int32_t *d_sa_pptr = nullptr, *d_sa_pagg = nullptr, *d_sa_pnode = nullptr;
   int32_t *d_sa_vptr = nullptr, *d_sa_vagg = nullptr, *d_sa_vnode = nullptr;
   int32_t *d_sa_sptr = nullptr, *d_sa_snode = nullptr, *d_sa_sidx = nullptr;
-  double *d_sa_P = nullptr, *d_sa_V = nullptr;  // sa_np x 18, sa_nv x 18
+  double *d_sa_V = nullptr;          // A P, sa_nv x 18 (FP64 form)
+  float *d_sa_Vf = nullptr;          // the same in the mixed forms
   int32_t *d_sa_ublk = nullptr;      // the next level's upper blocks (I <= J), in block order
   int64_t sa_nub = 0;
   double sa_omega = 0.0;       // omega of the next refresh
@@ -329,7 +330,10 @@ struct uft_ctx {
   double fnorm = 0.0;
   double *d_lanczos = nullptr;       // CG scalar log for eigenvalue estimates
 
-  double *d_a64 = nullptr;  // fine operator 3x3 node blocks in FP64 on the node pattern (smoothed prolongator)
+  // fine operator 3x3 node blocks on the node pattern for the smoothed
+  // prolongator: FP64 in the FP64 form, FP32 in the mixed forms
+  double *d_a64 = nullptr;
+  float *d_a32 = nullptr;
 
   // uft_solve_traffic: algorithmic bytes of the operator applies made inside
   // the elasticity solves (hierarchy refresh excluded) and the solves' time
@@ -567,9 +571,11 @@ void gather_op(hipStream_t s, const uft_ctx *C, const float *u, const float *Ye,
 // assembled FP16 fine operator: assembly from the per-element moduli E(x_e)
 // (rows scaled by the level-0 diagonal 1 / dinv) and the apply with the row
 // epilogue (block partials over gather_op_blocks(nn) blocks)
-void fine_ell_fill(hipStream_t s, const uft_ctx *C, const double *Ee, const double *dinv, double *a64 = nullptr);
+void fine_ell_fill(hipStream_t s, const uft_ctx *C, const double *Ee, const double *dinv, double *a64 = nullptr,
+                   float *a32 = nullptr);
 // smoothed prolongator of level 0 and the level-1 operator P^T A P from the
-// FP64 blocks d_a64, the fine diagonal and L[0].sa_omega (kernels_sa.hip)
+// fine blocks (d_a64 or d_a32), the fine diagonal and L[0].sa_omega
+// (kernels_sa.hip)
 void sa_refresh(hipStream_t s, uft_ctx *C);
 // setup: the contribution lists (which 0: prolongator slots, 1: A P pairs,
 // 2: Galerkin pairs); count writes the list length per row, fill the lists
