@@ -56,7 +56,7 @@ def _sa_matrix(pptr, pagg, P, nc):
 
 @pytest.mark.parametrize("npe,dims,smooth,precision", [(8, (8, 4, 4), 0, 0), (4, (5, 3, 3), 0, 0),
                                                        (8, (8, 4, 4), 1, 0), (4, (5, 3, 3), 1, 0),
-                                                       (8, (8, 4, 4), 1, 2)])
+                                                       (8, (8, 4, 4), 1, 1), (8, (8, 4, 4), 1, 2)])
 def test_galerkin_operators(O, npe, dims, smooth, precision):
     """Every coarse operator is P^T A P of the level above.  With smoothed
     aggregation (smooth = 1) level 0's prolongator must be the tentative one
@@ -88,8 +88,9 @@ def test_galerkin_operators(O, npe, dims, smooth, precision):
             assert 0.0 < omega < 4.0 / 3.0 * 2.0
             Psm = _sa_matrix(pptr, pagg, Ps, Pm.shape[1])
             want_p = Pm - omega * (K @ Pm) / np.diag(K)[:, None]
-            # the mixed forms assemble the smoothing operator from FP32 element blocks
-            tol_p = 1e-12 if precision == 0 else 1e-6
+            # the mixed forms assemble the smoothing operator from FP32 element blocks;
+            # UFT_MIXED_H keeps the prolongator in FP16 (11-bit significands)
+            tol_p = {0: 1e-12, 1: 1e-6, 2: 2e-3}[precision]
             assert np.abs(Psm - want_p).max() <= tol_p * np.abs(want_p).max()
             assert np.abs(Psm - Pm).max() > 1e-3  # really smoothed
             Pm = Psm

@@ -23,11 +23,24 @@
 // The cycle's transfers between levels 0 and 1 use P through its supports
 // (restriction, fused with level 1's first Chebyshev step) and its rows
 // (prolongation with the K-cycle's combination of level 1).
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
 #include "uft_internal.h"
 
 namespace {
+
+// storage of the prolongator copies: FP64, FP32 or FP16 (UFT_MIXED_H)
+__device__ __forceinline__ double ld(const double *p) { return *p; }
+__device__ __forceinline__ double ld(const float *p) { return (double)*p; }
+__device__ __forceinline__ double ld(const __half *p) { return (double)__half2float(*p); }
+__device__ __forceinline__ float ldf(const float *p) { return *p; }
+__device__ __forceinline__ float ldf(const __half *p) { return __half2float(*p); }
+__device__ __forceinline__ double ldd(const double *p) { return *p; }
+template <typename S> __device__ __forceinline__ S st(double v);
+template <> __device__ __forceinline__ double st<double>(double v) { return v; }
+template <> __device__ __forceinline__ float st<float>(double v) { return (float)v; }
+template <> __device__ __forceinline__ __half st<__half>(double v) { return __float2half((float)v); }
 
 constexpr int TPB = 256;
 inline unsigned nblk(int64_t n) { return (unsigned)((n + TPB - 1) / TPB); }
@@ -137,13 +150,13 @@ __device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t nb) {
 }
 
 // acc (3x6) += A (3x3, row-major) * B (3x6, row-major)
-template <typename T>
-__device__ __forceinline__ void mac33x36t(double (&acc)[18], const T *__restrict__ A, const T *__restrict__ B) {
+template <typename T, typename S>
+__device__ __forceinline__ void mac33x36t(double (&acc)[18], const T *__restrict__ A, const S *__restrict__ B) {
   double a[9], b[18];
 #pragma unroll
   for (int t = 0; t < 9; ++t) a[t] = (double)A[t];
 #pragma unroll
-  for (int t = 0; t < 18; ++t) b[t] = (double)B[t];
+  for (int t = 0; t < 18; ++t) b[t] = ld(B + t);
 #pragma unroll
   for (int c = 0; c < 3; ++c)
 #pragma unroll
@@ -154,7 +167,7 @@ __device__ __forceinline__ void mac33x36t(double (&acc)[18], const T *__restrict
 // W = sum over its slots of A_slot P_tent(col); P[t] = [K == parent(i)]
 // P_tent(i) - omega D^-1 W, stored as the node-sliced copy and the
 // support-ordered copy
-template <typename T>
+template <typename T, typename S>
 __global__ __launch_bounds__(TPB) void k_sa_prolongator(int64_t np, const int32_t *__restrict__ pnode,
                                                         const int32_t *__restrict__ pagg,
                                                         const int32_t *__restrict__ pptr,
@@ -165,8 +178,8 @@ __global__ __launch_bounds__(TPB) void k_sa_prolongator(int64_t np, const int32_
                                                         const double *__restrict__ dinv,
                                                         const uint8_t *__restrict__ fixm, double omega,
                                                         const int64_t *__restrict__ poff,
-                                                        const int32_t *__restrict__ spos, T *__restrict__ Pe,
-                                                        T *__restrict__ Ps) {
+                                                        const int32_t *__restrict__ spos, S *__restrict__ Pe,
+                                                        S *__restrict__ Ps) {
   const int64_t t = xcd_block(blockIdx.x, gridDim.x) * TPB + threadIdx.x;
   if (t >= np) return;
   const int i = pnode[t];
@@ -176,20 +189,20 @@ __global__ __launch_bounds__(TPB) void k_sa_prolongator(int64_t np, const int32_
   for (int m = 0; m < 18; ++m) acc[m] = 0.0;
   for (int q = wptr[t]; q < wptr[t + 1]; ++q) {
     const int64_t k = wslot[q];
-    mac33x36t<T>(acc, a + 9 * k, Pt + 18 * (size_t)col[k]);
+    mac33x36t<T, T>(acc, a + 9 * k, Pt + 18 * (size_t)col[k]);
   }
   const bool own = parent[i] == K;
   const unsigned fm = fixm[i];
   const int q = (int)(t - pptr[i]);
-  T *pe = Pe + poff[i / UFT_ELL_SLICE] * 18 + (int64_t)q * 18 * UFT_ELL_SLICE + i % UFT_ELL_SLICE;
-  T *ps = Ps + (size_t)spos[t] * 18;
+  S *pe = Pe + poff[i / UFT_ELL_SLICE] * 18 + (int64_t)q * 18 * UFT_ELL_SLICE + i % UFT_ELL_SLICE;
+  S *ps = Ps + (size_t)spos[t] * 18;
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     const double di = dinv[3 * (size_t)i + c];
 #pragma unroll
     for (int m = 0; m < 6; ++m) {
       const double pt = own ? (double)Pt[18 * (size_t)i + 6 * c + m] : 0.0;
-      const T v = (T)(((fm >> c) & 1u) ? 0.0 : pt - omega * di * acc[6 * c + m]);
+      const S v = st<S>(((fm >> c) & 1u) ? 0.0 : pt - omega * di * acc[6 * c + m]);
       pe[(6 * c + m) * UFT_ELL_SLICE] = v;
       ps[6 * c + m] = v;
     }
@@ -198,11 +211,11 @@ __global__ __launch_bounds__(TPB) void k_sa_prolongator(int64_t np, const int32_
 
 // one thread per entry t of A P: the sum over its (slot, support position)
 // pairs of A_slot P_position (rows of fixed DOFs zero)
-template <typename T>
+template <typename T, typename S>
 __global__ __launch_bounds__(TPB) void k_sa_ap(int64_t nv, const int32_t *__restrict__ vnode,
                                                const int32_t *__restrict__ vlptr, const int32_t *__restrict__ vlslot,
                                                const int32_t *__restrict__ vlpos, const T *__restrict__ a,
-                                               const T *__restrict__ Ps, const uint8_t *__restrict__ fixm,
+                                               const S *__restrict__ Ps, const uint8_t *__restrict__ fixm,
                                                T *__restrict__ V) {
   const int64_t t = xcd_block(blockIdx.x, gridDim.x) * TPB + threadIdx.x;
   if (t >= nv) return;
@@ -210,7 +223,7 @@ __global__ __launch_bounds__(TPB) void k_sa_ap(int64_t nv, const int32_t *__rest
 #pragma unroll
   for (int m = 0; m < 18; ++m) acc[m] = 0.0;
   for (int q = vlptr[t]; q < vlptr[t + 1]; ++q)
-    mac33x36t<T>(acc, a + 9 * (size_t)vlslot[q], Ps + 18 * (size_t)vlpos[q]);
+    mac33x36t<T, S>(acc, a + 9 * (size_t)vlslot[q], Ps + 18 * (size_t)vlpos[q]);
   const unsigned fm = fixm[vnode[t]];
 #pragma unroll
   for (int c = 0; c < 3; ++c)
@@ -220,12 +233,12 @@ __global__ __launch_bounds__(TPB) void k_sa_ap(int64_t nv, const int32_t *__rest
 
 // one wavefront per upper block b = (I, J), I <= J, of the level-1 pattern;
 // its pairs (support position of I, entry of A P) run in support order
-template <typename T>
+template <typename T, typename S>
 __global__ __launch_bounds__(TPB) void k_sa_galerkin(int64_t nub, const int32_t *__restrict__ ublk,
                                                      const int32_t *__restrict__ brow, const int32_t *__restrict__ bci,
                                                      const int32_t *__restrict__ btr, const int32_t *__restrict__ gptr,
                                                      const int32_t *__restrict__ gp, const int32_t *__restrict__ gv,
-                                                     const T *__restrict__ Ps, const T *__restrict__ V,
+                                                     const S *__restrict__ Ps, const T *__restrict__ V,
                                                      double *__restrict__ val) {
   const int64_t wv = (xcd_block(blockIdx.x, gridDim.x) * TPB + threadIdx.x) / 64;
   const int lane = threadIdx.x & 63;
@@ -235,12 +248,12 @@ __global__ __launch_bounds__(TPB) void k_sa_galerkin(int64_t nub, const int32_t 
 #pragma unroll
   for (int e = 0; e < 36; ++e) acc[e] = 0.0;
   for (int q = gptr[wv] + lane; q < gptr[wv + 1]; q += 64) {
-    const T *p = Ps + 18 * (size_t)gp[q];
+    const S *p = Ps + 18 * (size_t)gp[q];
     const T *v = V + 18 * (size_t)gv[q];
     double pr[18], vr[18];
 #pragma unroll
     for (int m = 0; m < 18; ++m) {
-      pr[m] = (double)p[m];
+      pr[m] = ld(p + m);
       vr[m] = (double)v[m];
     }
 #pragma unroll
@@ -279,9 +292,9 @@ __global__ __launch_bounds__(TPB) void k_sa_galerkin(int64_t nub, const int32_t 
 // P^T r at the position's node (P in support order, each position's 3x6 block
 // contiguous), then (dinv given) level 1's first Chebyshev step from a zero
 // guess: rc = bc, dc = it D^-1 bc, xc = dc
-template <int G, typename T>
+template <int G, typename T, typename S>
 __global__ __launch_bounds__(TPB) void k_sa_restrict_first(int64_t nc, const int32_t *__restrict__ sptr,
-                                                           const int32_t *__restrict__ snode, const T *__restrict__ Ps,
+                                                           const int32_t *__restrict__ snode, const S *__restrict__ Ps,
                                                            const T *__restrict__ r, T *__restrict__ bc,
                                                            const T *__restrict__ dinv, double it, T *__restrict__ rc,
                                                            T *__restrict__ dc, T *__restrict__ xc) {
@@ -295,10 +308,10 @@ __global__ __launch_bounds__(TPB) void k_sa_restrict_first(int64_t nc, const int
   for (int q = lane; q < npair; q += G) {
     const int kk = q / 3, c = q - kk * 3;
     const int i = snode[k0 + kk];
-    const T *Pe = Ps + (size_t)(k0 + kk) * 18 + c * 6;
+    const S *Pe = Ps + (size_t)(k0 + kk) * 18 + c * 6;
     const T ri = r[(size_t)i * 3 + c];
 #pragma unroll
-    for (int j = 0; j < 6; ++j) acc[j] += Pe[j] * ri;
+    for (int j = 0; j < 6; ++j) acc[j] += (T)ld(Pe + j) * ri;
   }
 #pragma unroll
   for (int j = 0; j < 6; ++j) {
@@ -324,10 +337,10 @@ __global__ __launch_bounds__(TPB) void k_sa_restrict_first(int64_t nc, const int
 // xb)_{aggregate}; ca, cb from sc[ia], sc[ib] (index < 0: 1.0); xb may be
 // null.  P in the node-sliced layout: one load instruction reads a value of
 // 64 consecutive nodes.
-template <typename T>
+template <typename T, typename S>
 __global__ __launch_bounds__(TPB) void k_sa_prolong_comb(int64_t nu, const int64_t *__restrict__ poff,
                                                          const int32_t *__restrict__ pw,
-                                                         const int32_t *__restrict__ eagg, const T *__restrict__ Pe,
+                                                         const int32_t *__restrict__ eagg, const S *__restrict__ Pe,
                                                          const double *__restrict__ sc, int ia, int ib,
                                                          const T *__restrict__ xa, const T *__restrict__ xb,
                                                          T *__restrict__ x) {
@@ -345,11 +358,11 @@ __global__ __launch_bounds__(TPB) void k_sa_prolong_comb(int64_t nu, const int64
     T c[6];
 #pragma unroll
     for (int j = 0; j < 6; ++j) c[j] = xb ? ca * xa[A * 6 + j] + cb * xb[A * 6 + j] : xa[A * 6 + j];
-    const T *Pq = Pe + poff[w] * 18 + (int64_t)q * 18 * UFT_ELL_SLICE + lane;
+    const S *Pq = Pe + poff[w] * 18 + (int64_t)q * 18 * UFT_ELL_SLICE + lane;
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
-      for (int j = 0; j < 6; ++j) acc[i] += Pq[(i * 6 + j) * UFT_ELL_SLICE] * c[j];
+      for (int j = 0; j < 6; ++j) acc[i] += (T)ld(Pq + (i * 6 + j) * UFT_ELL_SLICE) * c[j];
   }
 #pragma unroll
   for (int i = 0; i < 3; ++i) x[u * 3 + i] += acc[i];
@@ -373,46 +386,57 @@ void sa_lists(hipStream_t s, const uft_ctx *C, int which, const int32_t *ptr, in
                                                 L0.d_sa_snode, L0.d_sa_sidx, L0.d_sa_vptr, L0.d_sa_vagg, ptr, a, b);
 }
 
-template <typename T>
-static void sa_refresh_t(hipStream_t s, const uft_ctx *C, const T *a, const T *Pt, T *Pe, T *Ps, T *V) {
+template <typename T, typename S>
+static void sa_refresh_t(hipStream_t s, const uft_ctx *C, const T *a, const T *Pt, S *Pe, S *Ps, T *V) {
   const AmgLevel &L0 = C->amg.L[0], &L1 = C->amg.L[1];
-  k_sa_prolongator<T><<<nblk(L0.sa_np), TPB, 0, s>>>(L0.sa_np, L0.d_sa_pnode, L0.d_sa_pagg, L0.d_sa_pptr, L0.d_sa_wptr,
-                                                     L0.d_sa_wslot, C->fell.d_col, a, L0.d_parent, Pt, L0.d_dinv,
-                                                     C->d_fixm, L0.sa_omega, L0.d_sa_poff, L0.d_sa_spos, Pe, Ps);
-  k_sa_ap<T><<<nblk(L0.sa_nv), TPB, 0, s>>>(L0.sa_nv, L0.d_sa_vnode, L0.d_sa_vlptr, L0.d_sa_vlslot, L0.d_sa_vlent, a,
-                                            Ps, C->d_fixm, V);
-  k_sa_galerkin<T><<<nblk(L0.sa_nub * 64), TPB, 0, s>>>(L0.sa_nub, L0.d_sa_ublk, L1.d_brow, L1.d_ci, L0.d_gtr,
-                                                        L0.d_sa_gptr, L0.d_sa_gp, L0.d_sa_gv, Ps, V, L1.d_val);
+  k_sa_prolongator<T, S><<<nblk(L0.sa_np), TPB, 0, s>>>(L0.sa_np, L0.d_sa_pnode, L0.d_sa_pagg, L0.d_sa_pptr,
+                                                        L0.d_sa_wptr, L0.d_sa_wslot, C->fell.d_col, a, L0.d_parent, Pt,
+                                                        L0.d_dinv, C->d_fixm, L0.sa_omega, L0.d_sa_poff, L0.d_sa_spos,
+                                                        Pe, Ps);
+  k_sa_ap<T, S><<<nblk(L0.sa_nv), TPB, 0, s>>>(L0.sa_nv, L0.d_sa_vnode, L0.d_sa_vlptr, L0.d_sa_vlslot, L0.d_sa_vlent,
+                                               a, Ps, C->d_fixm, V);
+  k_sa_galerkin<T, S><<<nblk(L0.sa_nub * 64), TPB, 0, s>>>(L0.sa_nub, L0.d_sa_ublk, L1.d_brow, L1.d_ci, L0.d_gtr,
+                                                           L0.d_sa_gptr, L0.d_sa_gp, L0.d_sa_gv, Ps, V, L1.d_val);
 }
 
 void sa_refresh(hipStream_t s, uft_ctx *C) {
   AmgLevel &L0 = C->amg.L[0];
   L0.sa_omega_used = L0.sa_omega;
-  if (L0.d_sa_Pef)
-    sa_refresh_t<float>(s, C, C->d_a32, L0.d_Pf, L0.d_sa_Pef, L0.d_sa_Psf, L0.d_sa_Vf);
+  if (L0.d_sa_Peh)
+    sa_refresh_t<float, __half>(s, C, C->d_a32, L0.d_Pf, (__half *)L0.d_sa_Peh, (__half *)L0.d_sa_Psh, L0.d_sa_Vf);
+  else if (L0.d_sa_Pef)
+    sa_refresh_t<float, float>(s, C, C->d_a32, L0.d_Pf, L0.d_sa_Pef, L0.d_sa_Psf, L0.d_sa_Vf);
   else
-    sa_refresh_t<double>(s, C, C->d_a64, L0.d_P, L0.d_sa_Pe, L0.d_sa_Ps, L0.d_sa_V);
+    sa_refresh_t<double, double>(s, C, C->d_a64, L0.d_P, L0.d_sa_Pe, L0.d_sa_Ps, L0.d_sa_V);
 }
 
 void sa_restrict_first(hipStream_t s, const AmgLevel &L0, int64_t nc, const double *r, double *bc, const double *dinv,
                        double it, double *rc, double *dc, double *xc) {
-  k_sa_restrict_first<16, double><<<nblk(nc * 16), TPB, 0, s>>>(nc, L0.d_sa_sptr, L0.d_sa_snode, L0.d_sa_Ps, r, bc,
-                                                                dinv, it, rc, dc, xc);
+  k_sa_restrict_first<16, double, double><<<nblk(nc * 16), TPB, 0, s>>>(nc, L0.d_sa_sptr, L0.d_sa_snode, L0.d_sa_Ps, r,
+                                                                        bc, dinv, it, rc, dc, xc);
 }
 void sa_restrict_first(hipStream_t s, const AmgLevel &L0, int64_t nc, const float *r, float *bc, const float *dinv,
                        double it, float *rc, float *dc, float *xc) {
-  k_sa_restrict_first<16, float><<<nblk(nc * 16), TPB, 0, s>>>(nc, L0.d_sa_sptr, L0.d_sa_snode, L0.d_sa_Psf, r, bc,
-                                                               dinv, it, rc, dc, xc);
+  if (L0.d_sa_Psh)
+    k_sa_restrict_first<16, float, __half><<<nblk(nc * 16), TPB, 0, s>>>(
+        nc, L0.d_sa_sptr, L0.d_sa_snode, (const __half *)L0.d_sa_Psh, r, bc, dinv, it, rc, dc, xc);
+  else
+    k_sa_restrict_first<16, float, float><<<nblk(nc * 16), TPB, 0, s>>>(nc, L0.d_sa_sptr, L0.d_sa_snode, L0.d_sa_Psf, r,
+                                                                        bc, dinv, it, rc, dc, xc);
 }
 void sa_prolong_comb(hipStream_t s, const AmgLevel &L0, const double *sc, int ia, int ib, const double *xa,
                      const double *xb, double *x) {
-  k_sa_prolong_comb<double><<<nblk(L0.nunit), TPB, 0, s>>>(L0.nunit, L0.d_sa_poff, L0.d_sa_pw, L0.d_sa_eagg,
-                                                           L0.d_sa_Pe, sc, ia, ib, xa, xb, x);
+  k_sa_prolong_comb<double, double><<<nblk(L0.nunit), TPB, 0, s>>>(L0.nunit, L0.d_sa_poff, L0.d_sa_pw, L0.d_sa_eagg,
+                                                                   L0.d_sa_Pe, sc, ia, ib, xa, xb, x);
 }
 void sa_prolong_comb(hipStream_t s, const AmgLevel &L0, const double *sc, int ia, int ib, const float *xa,
                      const float *xb, float *x) {
-  k_sa_prolong_comb<float><<<nblk(L0.nunit), TPB, 0, s>>>(L0.nunit, L0.d_sa_poff, L0.d_sa_pw, L0.d_sa_eagg,
-                                                          L0.d_sa_Pef, sc, ia, ib, xa, xb, x);
+  if (L0.d_sa_Peh)
+    k_sa_prolong_comb<float, __half><<<nblk(L0.nunit), TPB, 0, s>>>(
+        L0.nunit, L0.d_sa_poff, L0.d_sa_pw, L0.d_sa_eagg, (const __half *)L0.d_sa_Peh, sc, ia, ib, xa, xb, x);
+  else
+    k_sa_prolong_comb<float, float><<<nblk(L0.nunit), TPB, 0, s>>>(L0.nunit, L0.d_sa_poff, L0.d_sa_pw, L0.d_sa_eagg,
+                                                                   L0.d_sa_Pef, sc, ia, ib, xa, xb, x);
 }
 
 }  // namespace uftk

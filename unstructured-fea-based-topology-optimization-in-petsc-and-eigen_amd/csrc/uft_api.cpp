@@ -2079,9 +2079,15 @@ int sa_setup_lists(uft_ctx *C, const AmgHostLevel &h) {
       (rc = upload(C, &L0.d_sa_eagg, eagg)))
     return rc;
   if (is_mixed(C)) {
-    // (padding slots are never read: the prolongation stops at aggregate -1)
-    if ((rc = dalloc_n(C, &L0.d_sa_Pef, 18 * L0.sa_ne_ell)) || (rc = dalloc_n(C, &L0.d_sa_Psf, 18 * L0.sa_np)))
+    // (padding slots are never read: the prolongation stops at aggregate -1);
+    // FP16 copies with the smoother's low-precision blocks (UFT_MIXED_H)
+    if (C->P.precision == UFT_MIXED_H) {
+      if ((rc = dalloc_n(C, &L0.d_sa_Peh, 18 * L0.sa_ne_ell)) || (rc = dalloc_n(C, &L0.d_sa_Psh, 18 * L0.sa_np)))
+        return rc;
+    } else if ((rc = dalloc_n(C, &L0.d_sa_Pef, 18 * L0.sa_ne_ell)) ||
+               (rc = dalloc_n(C, &L0.d_sa_Psf, 18 * L0.sa_np))) {
       return rc;
+    }
   } else {
     if ((rc = dalloc_n(C, &L0.d_sa_Pe, 18 * L0.sa_ne_ell)) || (rc = dalloc_n(C, &L0.d_sa_Ps, 18 * L0.sa_np)))
       return rc;
@@ -3246,6 +3252,16 @@ extern "C" int uft_amg_transfer_get(uft_ctx *C, int level, int32_t *parent, doub
   return sync(C);
 }
 
+// IEEE binary16 bits -> double (host side of the FP16 prolongator copies)
+static double half_to_double(uint16_t h) {
+  const int sgn = (h >> 15) & 1, ex = (h >> 10) & 31, man = h & 1023;
+  double v;
+  if (ex == 0) v = std::ldexp((double)man, -24);
+  else if (ex == 31) v = man ? NAN : INFINITY;
+  else v = std::ldexp(1.0 + man / 1024.0, ex - 15);
+  return sgn ? -v : v;
+}
+
 extern "C" int uft_amg_prolongator_get(uft_ctx *C, int64_t *nent, int32_t *pptr, int32_t *pagg, double *P,
                                        double *omega) {
   if (int rc = check_setup(C)) return rc;
@@ -3261,7 +3277,12 @@ extern "C" int uft_amg_prolongator_get(uft_ctx *C, int64_t *nent, int32_t *pptr,
     std::vector<int32_t> spos(L0.sa_np);
     std::vector<double> ps(18 * L0.sa_np);
     UFT_HIP(C, hipMemcpyAsync(spos.data(), L0.d_sa_spos, L0.sa_np * sizeof(int32_t), hipMemcpyDeviceToHost, C->s));
-    if (L0.d_sa_Psf) {
+    if (L0.d_sa_Psh) {
+      std::vector<uint16_t> ph(18 * L0.sa_np);
+      UFT_HIP(C, hipMemcpyAsync(ph.data(), L0.d_sa_Psh, ph.size() * sizeof(uint16_t), hipMemcpyDeviceToHost, C->s));
+      if (int rc = sync(C)) return rc;
+      for (size_t k = 0; k < ph.size(); ++k) ps[k] = half_to_double(ph[k]);
+    } else if (L0.d_sa_Psf) {
       std::vector<float> pf(18 * L0.sa_np);
       UFT_HIP(C, hipMemcpyAsync(pf.data(), L0.d_sa_Psf, pf.size() * sizeof(float), hipMemcpyDeviceToHost, C->s));
       if (int rc = sync(C)) return rc;

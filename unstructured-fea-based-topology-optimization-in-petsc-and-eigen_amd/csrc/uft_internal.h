@@ -120,6 +120,7 @@ struct AmgLevel {
   int32_t *d_sa_spos = nullptr;      // per entry, its position in the support order
   double *d_sa_Pe = nullptr, *d_sa_Ps = nullptr;  // FP64 cycle
   float *d_sa_Pef = nullptr, *d_sa_Psf = nullptr; // FP32 cycle
+  uint16_t *d_sa_Peh = nullptr, *d_sa_Psh = nullptr;  // FP16 copies (UFT_MIXED_H)
   // deterministic Galerkin contribution lists for the next level's blocks
   int64_t ngl = 0;
   int32_t *d_glptr = nullptr;  // per next-level block
