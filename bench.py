@@ -189,6 +189,52 @@ def run_form(args, precision):
                       "ms_per_step": solve_ms / max(solves, 1)}}
 
 
+FULL_RUN_ITERS = {"c0": 10, "c1": 50, "c2": 50, "c4a": 50}
+
+
+def full_run(args, precision):
+    """The whole loop of main.cpp:64-119 on the workload: the initial filter
+    and projection, then design iterations while itr < maxItr and ch > 0.01
+    (main.cpp:70), with a device synchronisation at every iteration boundary.
+    Returns per-iteration compliance, design change, Krylov iterations and
+    wall time, and why the loop stopped."""
+    import uft
+    from uft.capi import Context
+    npe, dims, seed, lk, prm = workload(args.workload, PREC[precision])
+    maxitr = FULL_RUN_ITERS[args.workload]
+    xyz, elem = uft.gen_box(npe, *dims, jitter=0.15, seed=seed)
+    fixed, ld, lv = uft.box_bcs(*dims, load_kind=lk)
+    ctx = Context(0)
+    ctx.set_params(Context.default_params(maxItr=maxitr, **prm))
+    ctx.set_mesh(elem, xyz)
+    ctx.set_bcs(fixed, ld, lv, nlc=8 if lk == 2 else 1)
+    t0 = time.perf_counter()
+    ctx.setup()
+    ctx.sync()
+    setup_s = time.perf_counter() - t0
+    ctx.init_design()
+    ctx.sync()
+    fxs, chs, its, ms = [], [], [], []
+    ch, itr = 1.0, 0
+    while itr < maxitr and ch > 0.01:
+        itr += 1
+        t1 = time.perf_counter()
+        fx, gx, ch = ctx.design_iteration(itr)
+        ctx.sync()
+        ms.append((time.perf_counter() - t1) * 1e3)
+        fxs.append(fx)
+        chs.append(ch)
+        its.append(ctx.stats()["it_el"])
+        print(f"[{args.workload} {precision}] itr {itr} fx {fx:.12g} ch {ch:.4g} krylov {its[-1]} {ms[-1]:.1f} ms",
+              file=sys.stderr, flush=True)
+    ctx.close()
+    stop = f"ch = {ch:.4g} <= 0.01 at itr {itr}" if ch <= 0.01 else f"itr = maxItr = {maxitr} (ch = {ch:.4g})"
+    return {"precision": precision, "iterations": itr, "stop": stop, "setup_s": setup_s,
+            "design_iters_per_s": itr / (sum(ms) / 1e3), "total_s": sum(ms) / 1e3,
+            "krylov_iters_per_solve": sum(its) / len(its), "krylov_iters": its, "fx": fxs, "ch": chs,
+            "ms_per_iter": [round(v, 2) for v in ms]}
+
+
 def elem_roofline(name, npe, ne, ndof, prof_elem, prof_node, word, kernel, traffic_key, nrhs=1, algo=None):
     """HBM roofline of an element phase (or of the assembled operator's
     apply, algo given) from its HIP-event timings."""
@@ -229,6 +275,10 @@ def main():
                          "kernel (0, default; with timers on the solve launches kernel by kernel anyway)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy three times (reference for rocprof PMC scaling)")
+    ap.add_argument("--full-run", action="store_true",
+                    help="instead of the timed steps, run the workload's whole optimisation loop (c0: 10, c1 and "
+                         "c2: 50 iterations, stopping early on ch <= 0.01 as main.cpp:70) in each form and print "
+                         "per-iteration compliance, design change, Krylov iterations and times")
     args = ap.parse_args()
     if args.copy_calibration:
         import torch
@@ -245,6 +295,19 @@ def main():
         sys.exit(2)
 
     forms = ["fp64", "mixed", "mixed_h"] if args.precision == "both" else [args.precision]
+    if args.full_run:
+        if args.workload not in FULL_RUN_ITERS:
+            sys.exit(f"--full-run covers {sorted(FULL_RUN_ITERS)}")
+        runs = {f: full_run(args, f) for f in forms}
+        out = {"mode": "full_run", "workload": args.workload, "max_itr": FULL_RUN_ITERS[args.workload],
+               "runs": runs}
+        if "fp64" in runs:
+            ref = runs["fp64"]["fx"]
+            for f, r in runs.items():
+                k = min(len(ref), len(r["fx"]))
+                r["max_rel_compliance_diff_vs_fp64"] = max(abs(a - b) / abs(b) for a, b in zip(r["fx"][:k], ref[:k]))
+        print(json.dumps(out))
+        return
     res = {f: run_form(args, f) for f in forms}
     main_form = forms[-1]
     r = res[main_form]

@@ -465,8 +465,9 @@ template <int NPE, typename T, typename T2, int BT, bool NT, int WPE>
 __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1)))
 void k_elem_apply_v(int ne, const int32_t *__restrict__ elem, const T2 *__restrict__ KE,
                     const double *__restrict__ xphys, SimpCoef sc, const uint8_t *__restrict__ fixm,
-                    const T *__restrict__ u, T *__restrict__ Ye) {
+                    const T *__restrict__ u, T *__restrict__ Ye, const double *__restrict__ gate) {
   constexpr int ND = 3 * NPE, NP = ND * (ND + 1) / 2, NQ = NP / 2;
+  if (gate && gate[0] == 0.0) return;  // a device-controlled solve did not ask for this product
   const int e = blockIdx.x * BT + threadIdx.x;
   if (e >= ne) return;
   T ue[ND], ye[ND];
@@ -583,8 +584,10 @@ template <int NPE, typename T, typename T2>
 __global__ __launch_bounds__(TPB) void k_elem_apply(int ne, const int32_t *__restrict__ elem,
                                                     const T2 *__restrict__ KE, const double *__restrict__ xphys,
                                                     SimpCoef sc, const uint8_t *__restrict__ fixm,
-                                                    const T *__restrict__ u, T *__restrict__ Ye) {
+                                                    const T *__restrict__ u, T *__restrict__ Ye,
+                                                    const double *__restrict__ gate) {
   constexpr int ND = 3 * NPE, NP = ND * (ND + 1) / 2, NQ = NP / 2;
+  if (gate && gate[0] == 0.0) return;  // a device-controlled solve did not ask for this product
   const int e = blockIdx.x * TPB + threadIdx.x;
   if (e >= ne) return;
   T ue[ND], ye[ND];
@@ -667,6 +670,104 @@ __global__ __launch_bounds__(TPB) void k_node_gather(int nn, int ne, const int32
     }
   }
   if (part) uftr::block_partials<1>(ds, part);
+}
+
+// ---- device control of the single-design flexible CG of the mixed forms.
+// ctl (in the context's scalar array): [0] stop (0 running, 1 converged,
+// 2 diverged, 3 iteration limit), [1] a reliable update runs this iteration,
+// [2] iterations done, [3] reliable updates done, [4] last FP64 residual
+// norm, [5] absolute tolerance, [6] iteration limit.  Every launch of an
+// iteration is queued whatever the outcome; the kernels that change the
+// iterate return at once when ctl[0] is set, those of the reliable update
+// when ctl[1] is not, so the host reads ctl once per round of queued
+// iterations instead of once per iteration.
+__global__ void k_ksp_ctl_init(double *__restrict__ ctl, double rtrue, double tol, int maxit) {
+  ctl[0] = 0.0;
+  ctl[1] = 0.0;
+  ctl[2] = 0.0;
+  ctl[3] = 0.0;
+  ctl[4] = rtrue;
+  ctl[5] = tol;
+  ctl[6] = (double)maxit;
+}
+// after an iteration's update (rr = its recursive r.r): the reliable update
+// when the recursive residual has met the tolerance or fallen 100-fold below
+// the last FP64 residual; otherwise the iteration limit
+__global__ void k_ksp_ctl_iter(const double *__restrict__ rr, double *__restrict__ ctl) {
+  if (ctl[0] != 0.0) return;
+  const double it = ctl[2] + 1.0;
+  ctl[2] = it;
+  const double rn = sqrt(rr[0]);
+  if (!isfinite(rn)) {
+    ctl[0] = 2.0;
+  } else if (rn <= ctl[5] || rn < 1e-2 * ctl[4]) {
+    ctl[1] = 1.0;
+  } else if (it >= ctl[6]) {
+    ctl[0] = 3.0;
+  }
+}
+// reliable update, node phase (ctl[1] set): q = K U summed from the element
+// results Ye in incidence order (fixed DOFs: U), r = F - q, rf = FP32 r, the
+// next fine cycle's first Chebyshev step from a zero guess on rf (the
+// arithmetic of k_cg_update_lo: cr = rf, cd = cx = it dinvf rf), partials of
+// r.r
+template <int NPE>
+__global__ __launch_bounds__(TPB) void k_reliable_node(int nn, int ne, const double *__restrict__ ctl,
+                                                       const int32_t *__restrict__ iptr,
+                                                       const int32_t *__restrict__ inc, const double *__restrict__ Ye,
+                                                       const uint8_t *__restrict__ fixm, const double *__restrict__ U,
+                                                       const double *__restrict__ F, double *__restrict__ r,
+                                                       float *__restrict__ rf, const float *__restrict__ dinvf,
+                                                       double it, float *__restrict__ cr, float *__restrict__ cd,
+                                                       float *__restrict__ cx, double *__restrict__ part) {
+  if (ctl[1] == 0.0) return;
+  const int n = blockIdx.x * TPB + threadIdx.x;
+  double ds[1] = {0.0};
+  if (n < nn) {
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int k = iptr[n]; k < iptr[n + 1]; ++k) {
+      const int v = inc[k];
+      const int e = v / NPE, l = v - e * NPE;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) acc[c] += Ye[(size_t)(c * NPE + l) * ne + e];
+    }
+    const unsigned m = fixm[n];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const size_t i = 3 * (size_t)n + c;
+      const double q = ((m >> c) & 1u) ? U[i] : acc[c];
+      const double rr = F[i] - q;
+      r[i] = rr;
+      const float rfi = (float)rr;
+      rf[i] = rfi;
+      cr[i] = rfi;
+      const float di = (float)it * dinvf[i] * rfi;
+      cd[i] = di;
+      cx[i] = di;
+      ds[0] += rr * rr;
+    }
+  }
+  uftr::block_partials<1>(ds, part);
+}
+// one block after k_reliable_node: r.r (into rr_out), the stopping test on
+// the FP64 residual norm
+__global__ __launch_bounds__(TPB) void k_ksp_ctl_reliable(const double *__restrict__ part, int nb,
+                                                          double *__restrict__ rr_out, double *__restrict__ ctl) {
+  if (ctl[1] == 0.0) return;
+  double res[1];
+  uftr::block_final<1>(part, nb, res);
+  if (threadIdx.x != 0) return;
+  rr_out[0] = res[0];
+  const double rt = sqrt(res[0]);
+  ctl[4] = rt;
+  ctl[3] += 1.0;
+  ctl[1] = 0.0;
+  if (rt <= ctl[5])
+    ctl[0] = 1.0;
+  else if (!isfinite(rt))
+    ctl[0] = 2.0;
+  else if (ctl[2] >= ctl[6])
+    ctl[0] = 3.0;
 }
 
 // Sensitivities (SolveFem.cpp:208-237): uKu = u_e^T KE u_e from the packed
@@ -848,7 +949,8 @@ __global__ __launch_bounds__(TPB) void k_cg_update_lo(int64_t n, const double *_
                                                       float *__restrict__ rf, double *__restrict__ part,
                                                       const float *__restrict__ dinvf, double it,
                                                       float *__restrict__ cr, float *__restrict__ cd,
-                                                      float *__restrict__ cx) {
+                                                      float *__restrict__ cx, const double *__restrict__ stop) {
+  if (stop && stop[0] != 0.0) return;  // a device-controlled solve has stopped: the iterate stays
   const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
   const double al = sc[ia] / sc[ib];
   double v[1] = {0.0};
@@ -1120,16 +1222,16 @@ __global__ __launch_bounds__(TPB) void k_projsens(int64_t n, const double *__res
 // ---- element-phase variant dispatch
 template <int NPE, typename T, typename T2>
 static void elem_phase_t(hipStream_t s, int variant, int ne, const int32_t *elem, const T2 *KE, const double *xphys,
-                         SimpCoef sc, const uint8_t *fixm, const T *u, T *Ye) {
+                         SimpCoef sc, const uint8_t *fixm, const T *u, T *Ye, const double *gate = nullptr) {
   auto g = [&](int bt) { return (unsigned)((ne + bt - 1) / bt); };
   switch (variant) {
-    case 1: k_elem_apply_v<NPE, T, T2, 256, true, 0><<<g(256), 256, 0, s>>>(ne, elem, KE, xphys, sc, fixm, u, Ye); break;
-    case 2: k_elem_apply_v<NPE, T, T2, 256, true, 4><<<g(256), 256, 0, s>>>(ne, elem, KE, xphys, sc, fixm, u, Ye); break;
-    case 3: k_elem_apply_v<NPE, T, T2, 128, true, 0><<<g(128), 128, 0, s>>>(ne, elem, KE, xphys, sc, fixm, u, Ye); break;
-    case 4: k_elem_apply_v<NPE, T, T2, 512, true, 0><<<g(512), 512, 0, s>>>(ne, elem, KE, xphys, sc, fixm, u, Ye); break;
-    case 5: k_elem_apply_v<NPE, T, T2, 256, false, 4><<<g(256), 256, 0, s>>>(ne, elem, KE, xphys, sc, fixm, u, Ye); break;
-    case 6: k_elem_apply_v<NPE, T, T2, 64, true, 0><<<g(64), 64, 0, s>>>(ne, elem, KE, xphys, sc, fixm, u, Ye); break;
-    default: k_elem_apply<NPE, T, T2><<<g(TPB), TPB, 0, s>>>(ne, elem, KE, xphys, sc, fixm, u, Ye); break;
+    case 1: k_elem_apply_v<NPE, T, T2, 256, true, 0><<<g(256), 256, 0, s>>>(ne, elem, KE, xphys, sc, fixm, u, Ye, gate); break;
+    case 2: k_elem_apply_v<NPE, T, T2, 256, true, 4><<<g(256), 256, 0, s>>>(ne, elem, KE, xphys, sc, fixm, u, Ye, gate); break;
+    case 3: k_elem_apply_v<NPE, T, T2, 128, true, 0><<<g(128), 128, 0, s>>>(ne, elem, KE, xphys, sc, fixm, u, Ye, gate); break;
+    case 4: k_elem_apply_v<NPE, T, T2, 512, true, 0><<<g(512), 512, 0, s>>>(ne, elem, KE, xphys, sc, fixm, u, Ye, gate); break;
+    case 5: k_elem_apply_v<NPE, T, T2, 256, false, 4><<<g(256), 256, 0, s>>>(ne, elem, KE, xphys, sc, fixm, u, Ye, gate); break;
+    case 6: k_elem_apply_v<NPE, T, T2, 64, true, 0><<<g(64), 64, 0, s>>>(ne, elem, KE, xphys, sc, fixm, u, Ye, gate); break;
+    default: k_elem_apply<NPE, T, T2><<<g(TPB), TPB, 0, s>>>(ne, elem, KE, xphys, sc, fixm, u, Ye, gate); break;
   }
 }
 
@@ -1244,6 +1346,29 @@ void apply_Kf(hipStream_t s, const uft_ctx *C, const double *xphys, const float 
     k_node_gather<4, 3, 1, float><<<nblk(nn), TPB, 0, s>>>(nn, ne, C->d_iptr, C->d_inc, C->d_Yef, C->d_fixm, u, y,
                                                            nullptr, nullptr, nullptr);
   if (ev) (void)hipEventRecord(ev[2], s);
+}
+
+void ksp_ctl_init(hipStream_t s, double *ctl, double rtrue, double tol, int maxit) {
+  k_ksp_ctl_init<<<1, 1, 0, s>>>(ctl, rtrue, tol, maxit);
+}
+void ksp_ctl_iter(hipStream_t s, const double *rr, double *ctl) { k_ksp_ctl_iter<<<1, 1, 0, s>>>(rr, ctl); }
+void reliable_update(hipStream_t s, const uft_ctx *C, const double *xphys, const double *U, double *r, float *rf,
+                     const float *dinvf, double it, float *cr, float *cd, float *cx, double *rr_out, double *ctl) {
+  const SimpCoef sc{C->P.Emin, C->P.Emax - C->P.Emin, C->P.penal};
+  const int ne = (int)C->ne, nn = (int)C->nn;
+  const double *gate = ctl + 1;
+  if (C->npe == 8) {
+    elem_phase_t<8, double, double2>(s, C->elem_variant, ne, C->d_elem, (const double2 *)C->d_KE, xphys, sc, C->d_fixm,
+                                     U, C->d_Ye, gate);
+    k_reliable_node<8><<<nblk(nn), TPB, 0, s>>>(nn, ne, ctl, C->d_iptr, C->d_inc, C->d_Ye, C->d_fixm, U, C->d_F, r, rf,
+                                                dinvf, it, cr, cd, cx, C->d_part);
+  } else {
+    elem_phase_t<4, double, double2>(s, C->elem_variant, ne, C->d_elem, (const double2 *)C->d_KE, xphys, sc, C->d_fixm,
+                                     U, C->d_Ye, gate);
+    k_reliable_node<4><<<nblk(nn), TPB, 0, s>>>(nn, ne, ctl, C->d_iptr, C->d_inc, C->d_Ye, C->d_fixm, U, C->d_F, r, rf,
+                                                dinvf, it, cr, cd, cx, C->d_part);
+  }
+  k_ksp_ctl_reliable<<<1, TPB, 0, s>>>(C->d_part, (int)nblk(nn), rr_out, ctl);
 }
 
 __global__ void k_inv(int64_t n, const double *__restrict__ d, double *__restrict__ dinv, float *__restrict__ dinvf) {
@@ -1383,13 +1508,13 @@ void cg_update(hipStream_t s, int64_t n, const double *sc, int ia, double *u, co
 void cg_update_lo(hipStream_t s, int64_t n, const double *sc, int ia, int ib, double *u, const double *p, double *r,
                   const float *q, float *rf, double *part, double *out) {
   k_cg_update_lo<<<nblk(n), TPB, 0, s>>>(n, sc, ia, ib, u, p, r, q, rf, part, nullptr, 0.0, nullptr, nullptr,
-                                           nullptr);
+                                           nullptr, nullptr);
   final_sum(s, part, (int)nblk(n), 1, out);
 }
 void cg_update_lo_cheb(hipStream_t s, int64_t n, const double *sc, int ia, int ib, double *u, const double *p,
                        double *r, const float *q, float *rf, double *part, double *out, const float *dinvf, double it,
-                       float *cr, float *cd, float *cx) {
-  k_cg_update_lo<<<nblk(n), TPB, 0, s>>>(n, sc, ia, ib, u, p, r, q, rf, part, dinvf, it, cr, cd, cx);
+                       float *cr, float *cd, float *cx, const double *stop) {
+  k_cg_update_lo<<<nblk(n), TPB, 0, s>>>(n, sc, ia, ib, u, p, r, q, rf, part, dinvf, it, cr, cd, cx, stop);
   final_sum(s, part, (int)nblk(n), 1, out);
 }
 void z_from_f32(hipStream_t s, int64_t n, const float *zf, const double *r, double *z, double *zo, double *part,

@@ -23,6 +23,8 @@
 // The cycle's transfers between levels 0 and 1 use P through its supports
 // (restriction, fused with level 1's first Chebyshev step) and its rows
 // (prolongation with the K-cycle's combination of level 1).
+#include <type_traits>
+
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
@@ -149,6 +151,22 @@ __device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t nb) {
   return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + b / 8;
 }
 
+// The FP16 node-sliced copy (UFT_MIXED_H) is stored in three planes over
+// the entry slots s (ne of them): values 0-7 at halves 8 s, values 8-15 at
+// 8 (ne + s), values 16-17 at 16 ne + 2 s, so a lane reads its 3x6 entry
+// with two 16-byte loads and one 4-byte load, each coalesced over the
+// wavefront's 64 consecutive slots.
+__device__ __forceinline__ int64_t pe_half_index(int64_t ne, int64_t s, int v) {
+  return v < 8 ? 8 * s + v : v < 16 ? 8 * (ne + s) + (v - 8) : 16 * ne + 2 * s + (v - 16);
+}
+
+// two FP16 values packed in a 32-bit word
+__device__ __forceinline__ float2 unpack_h2(uint32_t w) {
+  __half2 h;
+  __builtin_memcpy(&h, &w, 4);
+  return __half22float2(h);
+}
+
 // acc (3x6) += A (3x3, row-major) * B (3x6, row-major)
 template <typename T, typename S>
 __device__ __forceinline__ void mac33x36t(double (&acc)[18], const T *__restrict__ A, const S *__restrict__ B) {
@@ -177,7 +195,7 @@ __global__ __launch_bounds__(TPB) void k_sa_prolongator(int64_t np, const int32_
                                                         const int32_t *__restrict__ parent, const T *__restrict__ Pt,
                                                         const double *__restrict__ dinv,
                                                         const uint8_t *__restrict__ fixm, double omega,
-                                                        const int64_t *__restrict__ poff,
+                                                        const int64_t *__restrict__ poff, int64_t ne_ell,
                                                         const int32_t *__restrict__ spos, S *__restrict__ Pe,
                                                         S *__restrict__ Ps) {
   const int64_t t = xcd_block(blockIdx.x, gridDim.x) * TPB + threadIdx.x;
@@ -194,7 +212,8 @@ __global__ __launch_bounds__(TPB) void k_sa_prolongator(int64_t np, const int32_
   const bool own = parent[i] == K;
   const unsigned fm = fixm[i];
   const int q = (int)(t - pptr[i]);
-  S *pe = Pe + poff[i / UFT_ELL_SLICE] * 18 + (int64_t)q * 18 * UFT_ELL_SLICE + i % UFT_ELL_SLICE;
+  const int64_t slot = poff[i / UFT_ELL_SLICE] + (int64_t)q * UFT_ELL_SLICE + i % UFT_ELL_SLICE;
+  S *pe = Pe + (slot - i % UFT_ELL_SLICE) * 18 + i % UFT_ELL_SLICE;
   S *ps = Ps + (size_t)spos[t] * 18;
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
@@ -203,7 +222,10 @@ __global__ __launch_bounds__(TPB) void k_sa_prolongator(int64_t np, const int32_
     for (int m = 0; m < 6; ++m) {
       const double pt = own ? (double)Pt[18 * (size_t)i + 6 * c + m] : 0.0;
       const S v = st<S>(((fm >> c) & 1u) ? 0.0 : pt - omega * di * acc[6 * c + m]);
-      pe[(6 * c + m) * UFT_ELL_SLICE] = v;
+      if constexpr (std::is_same<S, __half>::value)
+        Pe[pe_half_index(ne_ell, slot, 6 * c + m)] = v;
+      else
+        pe[(6 * c + m) * UFT_ELL_SLICE] = v;
       ps[6 * c + m] = v;
     }
   }
@@ -368,6 +390,108 @@ __global__ __launch_bounds__(TPB) void k_sa_prolong_comb(int64_t nu, const int64
   for (int i = 0; i < 3; ++i) x[u * 3 + i] += acc[i];
 }
 
+// FP16 copies (UFT_MIXED_H): G lanes per coarse unit, one support node per
+// lane and step (its whole 3x6 entry: nine 4-byte words in support order and
+// the node's three residual entries), then the same butterfly and first
+// Chebyshev step as k_sa_restrict_first
+template <int G>
+__global__ __launch_bounds__(TPB) void k_sa_restrict_first_h(int64_t nc, const int32_t *__restrict__ sptr,
+                                                             const int32_t *__restrict__ snode,
+                                                             const uint32_t *__restrict__ Ps,
+                                                             const float *__restrict__ r, float *__restrict__ bc,
+                                                             const float *__restrict__ dinv, double it,
+                                                             float *__restrict__ rc, float *__restrict__ dc,
+                                                             float *__restrict__ xc) {
+  static_assert(G >= 1 && G <= 64 && (64 % G) == 0, "G lanes must tile a wavefront");
+  const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  const int64_t A = t / G;
+  const int lane = (int)(t % G);
+  const bool on = A < nc;
+  const int k1 = on ? sptr[A + 1] : 0;
+  float acc[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  for (int k = (on ? sptr[A] : 0) + lane; k < k1; k += G) {
+    const int64_t i = snode[k];
+    const uint32_t *pw = Ps + (int64_t)k * 9;
+    float p[18];
+#pragma unroll
+    for (int h = 0; h < 9; ++h) {
+      const float2 v = unpack_h2(pw[h]);
+      p[2 * h] = v.x;
+      p[2 * h + 1] = v.y;
+    }
+    const float r0 = r[3 * i], r1 = r[3 * i + 1], r2 = r[3 * i + 2];
+#pragma unroll
+    for (int j = 0; j < 6; ++j) acc[j] += p[j] * r0 + p[6 + j] * r1 + p[12 + j] * r2;
+  }
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) acc[j] += __shfl_xor(acc[j], o, 64);
+  }
+  if (on && lane == 0) {
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+      const int64_t i = A * 6 + j;
+      bc[i] = acc[j];
+      if (dinv) {
+        const float d = (float)it * dinv[i] * acc[j];
+        rc[i] = acc[j];
+        dc[i] = d;
+        xc[i] = d;
+      }
+    }
+  }
+}
+
+// FP16 copies (UFT_MIXED_H): one thread per node u, its entries from the
+// three planes (pe_half_index); x_u += sum of P (ca xa + cb xb)_{aggregate}
+// as k_sa_prolong_comb.  Padding slots (aggregate -1) are read but not used.
+__global__ __launch_bounds__(TPB) void k_sa_prolong_comb_h(int64_t nu, int64_t ne, const int64_t *__restrict__ poff,
+                                                           const int32_t *__restrict__ pw,
+                                                           const int32_t *__restrict__ eagg,
+                                                           const uint4 *__restrict__ P01,
+                                                           const uint32_t *__restrict__ P2,
+                                                           const double *__restrict__ sc, int ia, int ib,
+                                                           const float *__restrict__ xa, const float *__restrict__ xb,
+                                                           float *__restrict__ x) {
+  const int64_t u = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  if (u >= nu) return;
+  const float ca = ia < 0 ? 1.f : (float)sc[ia];
+  const float cb = ib < 0 ? 1.f : (float)sc[ib];
+  const int64_t w = u / UFT_ELL_SLICE;
+  const int nq = pw[w];
+  const int64_t s0 = poff[w] + u % UFT_ELL_SLICE;
+  float a0 = 0.f, a1 = 0.f, a2 = 0.f;
+#pragma unroll 2
+  for (int q = 0; q < nq; ++q) {
+    const int64_t s = s0 + (int64_t)q * UFT_ELL_SLICE;
+    const int A = eagg[s];
+    const uint4 lo = P01[s], hi = P01[ne + s];
+    const uint32_t tl = P2[s];
+    if (A < 0) continue;
+    float c[6];
+#pragma unroll
+    for (int j = 0; j < 6; ++j) c[j] = xb ? ca * xa[6 * (int64_t)A + j] + cb * xb[6 * (int64_t)A + j] : xa[6 * (int64_t)A + j];
+    const uint32_t wds[9] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w, tl};
+    float p[18];
+#pragma unroll
+    for (int h = 0; h < 9; ++h) {
+      const float2 v = unpack_h2(wds[h]);
+      p[2 * h] = v.x;
+      p[2 * h + 1] = v.y;
+    }
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+      a0 += p[j] * c[j];
+      a1 += p[6 + j] * c[j];
+      a2 += p[12 + j] * c[j];
+    }
+  }
+  x[u * 3] += a0;
+  x[u * 3 + 1] += a1;
+  x[u * 3 + 2] += a2;
+}
+
 }  // namespace
 
 namespace uftk {
@@ -391,8 +515,8 @@ static void sa_refresh_t(hipStream_t s, const uft_ctx *C, const T *a, const T *P
   const AmgLevel &L0 = C->amg.L[0], &L1 = C->amg.L[1];
   k_sa_prolongator<T, S><<<nblk(L0.sa_np), TPB, 0, s>>>(L0.sa_np, L0.d_sa_pnode, L0.d_sa_pagg, L0.d_sa_pptr,
                                                         L0.d_sa_wptr, L0.d_sa_wslot, C->fell.d_col, a, L0.d_parent, Pt,
-                                                        L0.d_dinv, C->d_fixm, L0.sa_omega, L0.d_sa_poff, L0.d_sa_spos,
-                                                        Pe, Ps);
+                                                        L0.d_dinv, C->d_fixm, L0.sa_omega, L0.d_sa_poff, L0.sa_ne_ell,
+                                                        L0.d_sa_spos, Pe, Ps);
   k_sa_ap<T, S><<<nblk(L0.sa_nv), TPB, 0, s>>>(L0.sa_nv, L0.d_sa_vnode, L0.d_sa_vlptr, L0.d_sa_vlslot, L0.d_sa_vlent,
                                                a, Ps, C->d_fixm, V);
   k_sa_galerkin<T, S><<<nblk(L0.sa_nub * 64), TPB, 0, s>>>(L0.sa_nub, L0.d_sa_ublk, L1.d_brow, L1.d_ci, L0.d_gtr,
@@ -418,8 +542,8 @@ void sa_restrict_first(hipStream_t s, const AmgLevel &L0, int64_t nc, const doub
 void sa_restrict_first(hipStream_t s, const AmgLevel &L0, int64_t nc, const float *r, float *bc, const float *dinv,
                        double it, float *rc, float *dc, float *xc) {
   if (L0.d_sa_Psh)
-    k_sa_restrict_first<16, float, __half><<<nblk(nc * 16), TPB, 0, s>>>(
-        nc, L0.d_sa_sptr, L0.d_sa_snode, (const __half *)L0.d_sa_Psh, r, bc, dinv, it, rc, dc, xc);
+    k_sa_restrict_first_h<16><<<nblk(nc * 16), TPB, 0, s>>>(nc, L0.d_sa_sptr, L0.d_sa_snode,
+                                                            (const uint32_t *)L0.d_sa_Psh, r, bc, dinv, it, rc, dc, xc);
   else
     k_sa_restrict_first<16, float, float><<<nblk(nc * 16), TPB, 0, s>>>(nc, L0.d_sa_sptr, L0.d_sa_snode, L0.d_sa_Psf, r,
                                                                         bc, dinv, it, rc, dc, xc);
@@ -432,8 +556,9 @@ void sa_prolong_comb(hipStream_t s, const AmgLevel &L0, const double *sc, int ia
 void sa_prolong_comb(hipStream_t s, const AmgLevel &L0, const double *sc, int ia, int ib, const float *xa,
                      const float *xb, float *x) {
   if (L0.d_sa_Peh)
-    k_sa_prolong_comb<float, __half><<<nblk(L0.nunit), TPB, 0, s>>>(
-        L0.nunit, L0.d_sa_poff, L0.d_sa_pw, L0.d_sa_eagg, (const __half *)L0.d_sa_Peh, sc, ia, ib, xa, xb, x);
+    k_sa_prolong_comb_h<<<nblk(L0.nunit), TPB, 0, s>>>(
+        L0.nunit, L0.sa_ne_ell, L0.d_sa_poff, L0.d_sa_pw, L0.d_sa_eagg, (const uint4 *)L0.d_sa_Peh,
+        (const uint32_t *)L0.d_sa_Peh + 8 * L0.sa_ne_ell, sc, ia, ib, xa, xb, x);
   else
     k_sa_prolong_comb<float, float><<<nblk(L0.nunit), TPB, 0, s>>>(L0.nunit, L0.d_sa_poff, L0.d_sa_pw, L0.d_sa_eagg,
                                                                    L0.d_sa_Pef, sc, ia, ib, xa, xb, x);

@@ -35,7 +35,8 @@ enum {
   S_MISC = 16,
   S_RED = 32,  // general reduction outputs (up to 48 values)
   S_K = 96,    // K-cycle scalars, 8 per level
-  S_FD = 200   // filter CG control: state, tolerance, divergence bound, iterations, ||z||
+  S_FD = 200,  // filter CG control: state, tolerance, divergence bound, iterations, ||z||
+  S_CTL = 208  // elasticity CG control of the mixed forms (uftk::ksp_ctl_init): 7 values
 };
 
 int dalloc(uft_ctx *C, void **p, size_t bytes) {
@@ -1279,75 +1280,88 @@ int solve_el_mixed(uft_ctx *C, const double *xphys, double *U) {
   uftk::dot(s, n, r, z, C->d_part, C->d_scal + S_RZ);
   uftk::copy(s, n, z, p);
   const double tol = C->P.rtol_el * bn;
-  int it = 0;
-  bool conv = false;
-  // The outer operator product runs with the FP32 element blocks; the FP64
-  // residual F - K U replaces the recursively updated one whenever the
-  // latter has fallen 100-fold below the last true residual, and before the
-  // solve is accepted (reliable updates: the recursion's drift stays 1e-5 of
-  // the current residual, far below the tolerance).
-  // Without graph capture the update also forms the next cycle's first
-  // Chebyshev step on rf (first_ready), unless a reliable update has
-  // replaced r and rf since.
-  const bool fuse_first = !use_graph(C);
-  bool first_ready = false;
+  // The outer operator product runs with the FP32 blocks; the FP64 residual
+  // F - K U replaces the recursively updated one whenever the latter has
+  // fallen 100-fold below the last FP64 residual or below the tolerance, and
+  // the solve stops only on an FP64 residual (reliable updates: the
+  // recursion's drift stays 1e-5 of the current residual, far below the
+  // tolerance).  The update also forms the next cycle's first Chebyshev step
+  // on rf, and so does a reliable update.
   AmgLevel &L0 = C->amg.L[0];
-  auto head = [&]() {  // q = K p (FP32 blocks, d_pf = p); alpha = rz / pq; U, r (and its FP32 copy rf) updated
-    fine_apply_lo(C, xphys, p, S_PQ, true);
-    if (fuse_first) {
-      uftk::cg_update_lo_cheb(s, n, C->d_scal, S_RZ, S_PQ, U, p, r, C->d_qf, rf, C->d_part, C->d_scal + S_RR,
-                              L0.d_dinvf, fine_first_coef(C), L0.rf, L0.df, zf);
-      first_ready = true;
-    } else {
-      uftk::cg_update_lo(s, n, C->d_scal, S_RZ, S_PQ, U, p, r, C->d_qf, rf, C->d_part, C->d_scal + S_RR);
-    }
+  uftk::d2f(s, n, p, C->d_pf);  // the first direction's FP32 copy
+  // Device control: the stopping test and the reliable-update decision run
+  // on the device after every iteration (uftk::ksp_ctl_iter), the reliable
+  // update's launches are queued every iteration and return at once unless
+  // asked for, and the update that changes U and r returns at once after the
+  // solve has stopped.  The host queues iterations in rounds (as many as the
+  // previous solve took, less one, then two at a time) and reads the control
+  // scalars once per round, so the device never waits for the host between
+  // iterations; the iterations queued after the stop leave U and r alone.
+  // With graph = 1 one iteration is captured and replayed.
+  double *ctl = C->d_scal + S_CTL;
+  uftk::ksp_ctl_init(s, ctl, bn, tol, C->P.maxit_el);
+  C->reliable = 0;
+  const double itc = fine_first_coef(C);
+  auto control = [&]() {
+    uftk::ksp_ctl_iter(s, C->d_scal + S_RR, ctl);
+    uftk::reliable_update(s, C, xphys, U, r, rf, L0.d_dinvf, itc, L0.rf, L0.df, zf, C->d_scal + S_RR, ctl);
   };
-  auto next = [&]() {  // z = B rf in FP32; zo = previous z; r.z, r.zo; beta; p
-    fine_cycle<float>(C, rf, zf, first_ready);
-    first_ready = false;
-    if (fuse_first) {  // no graph capture: swap z and zo instead of copying
+  auto head_dev = [&]() {  // q = K p (FP32, d_pf = p); alpha; U, r, rf and the next cycle's first step
+    fine_apply_lo(C, xphys, p, S_PQ, true);
+    uftk::cg_update_lo_cheb(s, n, C->d_scal, S_RZ, S_PQ, U, p, r, C->d_qf, rf, C->d_part, C->d_scal + S_RR,
+                            L0.d_dinvf, itc, L0.rf, L0.df, zf, ctl);
+  };
+  const bool graph = use_graph(C);
+  auto next_dev = [&]() {  // z = B rf; zo = previous z; beta; p; then the update and the control
+    fine_cycle<float>(C, rf, zf, true);
+    if (graph) {  // a captured iteration must not depend on the host's swap of z and zo
+      uftk::z_from_f32_beta(s, n, zf, r, z, zo, C->d_part, C->d_scal, S_RZNEW, S_RZ, S_BETA + 2);
+    } else {
       uftk::z_from_f32_swap_beta(s, n, zf, r, z, zo, C->d_part, C->d_scal, S_RZNEW, S_RZ, S_BETA + 2);
       std::swap(z, zo);
-    } else {
-      uftk::z_from_f32_beta(s, n, zf, r, z, zo, C->d_part, C->d_scal, S_RZNEW, S_RZ, S_BETA + 2);
     }
     uftk::xpby_lo(s, n, z, C->d_scal, S_BETA + 2, p, C->d_pf);
-    head();
+    head_dev();
+    control();
   };
-  uftk::d2f(s, n, p, C->d_pf);  // the first direction's FP32 copy
   static const char tag = 0;
-  double rtrue = bn;  // last FP64 residual norm
-  C->reliable = 0;
-  while (it < C->P.maxit_el) {
-    if (it == 0)
-      head();
-    else if (!use_graph(C))
-      next();
-    else if ((rc = graph_launch(C, xphys, U, &tag, next)))
-      return rc;
-    ++it;
-    if ((rc = fetch(C, C->d_scal + S_RR, 1, C->h_scal))) return rc;
-    const double rn = std::sqrt(C->h_scal[0]);
-    if (!std::isfinite(rn)) return uft_set_error(C, UFT_ERR_NOCONV, "mixed CG diverged at iteration %d", it);
-    if (rn <= tol || rn < 1e-2 * rtrue) {  // reliable update
-      first_ready = false;
-      fine_apply(C, xphys, U, q, nullptr, nullptr);
-      uftk::sub(s, n, C->d_F, q, r);
-      uftk::d2f(s, n, r, rf);
-      uftk::dot(s, n, r, r, C->d_part, C->d_scal + S_RR);
-      if ((rc = fetch(C, C->d_scal + S_RR, 1, C->h_scal))) return rc;
-      rtrue = std::sqrt(C->h_scal[0]);
-      C->reliable++;
-      if (rtrue <= tol) {
-        conv = true;
-        break;
+  // traffic counters after each queued iteration (the iterations queued after
+  // the stop are taken back; reliable updates are counted as they ran)
+  std::vector<std::pair<double, int64_t>> cnt;
+  cnt.reserve(256);
+  int queued = 0, round = C->el_prev_its > 1 ? C->el_prev_its - 1 : 8;
+  double stop = 0.0, its = 0.0, rel = 0.0;
+  while (true) {
+    for (int k = 0; k < round && queued < C->P.maxit_el; ++k, ++queued) {
+      if (queued == 0) {
+        head_dev();
+        control();
+      } else if (!graph) {
+        next_dev();
+      } else if ((rc = graph_launch(C, xphys, U, &tag, next_dev))) {
+        return rc;
       }
+      cnt.emplace_back(C->solve_bytes, C->solve_applies);
     }
+    if ((rc = fetch(C, ctl, 5, C->h_scal))) return rc;
+    stop = C->h_scal[0];
+    its = C->h_scal[2];
+    rel = C->h_scal[3];
+    if (stop != 0.0 || queued >= C->P.maxit_el) break;
+    round = 2;
   }
+  const int it = (int)its;
   C->it_el = it;
+  C->el_prev_its = it;
+  C->reliable = (int)rel;
   C->outer_el = C->reliable;  // FP64 residual evaluations
-  if (conv) {  // the last reliable update is the true residual (SolveFem.cpp:142-152)
-    C->relres_el = rtrue / bn;
+  if (C->count_on && it >= 1 && it <= (int)cnt.size()) {
+    C->solve_bytes = cnt[it - 1].first + rel * fine_bytes64(C);
+    C->solve_applies = cnt[it - 1].second + (int64_t)rel;
+  }
+  if (stop == 2.0) return uft_set_error(C, UFT_ERR_NOCONV, "mixed CG diverged at iteration %d", it);
+  if (stop == 1.0) {  // the last reliable update is the true residual (SolveFem.cpp:142-152)
+    C->relres_el = C->h_scal[4] / bn;
     return UFT_OK;
   }
   fine_apply(C, xphys, U, q, nullptr, nullptr);

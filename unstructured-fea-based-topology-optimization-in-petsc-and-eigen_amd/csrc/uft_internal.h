@@ -110,9 +110,12 @@ struct AmgLevel {
   int32_t *d_sa_wptr = nullptr, *d_sa_wslot = nullptr;
   int32_t *d_sa_vlptr = nullptr, *d_sa_vlslot = nullptr, *d_sa_vlent = nullptr;
   int32_t *d_sa_gptr = nullptr, *d_sa_gp = nullptr, *d_sa_gv = nullptr;
-  // the cycle's copies of P: sliced ELL by node (slice of 64 nodes: entry q
-  // of lane n, value m at poff*18 + (18 q + m) 64 + n; aggregate -1 on
-  // padding) for the prolongation, support order for the restriction
+  // the cycle's copies of P: sliced ELL by node (slice of 64 nodes, entry
+  // slot s = poff + 64 q + n for entry q of lane n; FP64/FP32: value m at
+  // 18 (s - n) + 64 m + n; FP16: three planes, values 0-7 at 8 s, 8-15 at
+  // 8 (sa_ne_ell + s), 16-17 at 16 sa_ne_ell + 2 s; aggregate -1 on padding)
+  // for the prolongation, support order (18 values per entry) for the
+  // restriction
   int64_t sa_ne_ell = 0;
   int64_t *d_sa_poff = nullptr;      // per slice, first entry slot (nslice + 1)
   int32_t *d_sa_pw = nullptr;        // per slice, entry slots per node
@@ -378,6 +381,7 @@ struct uft_ctx {
 
   // stats
   int it_el = 0, it_f = 0, outer_el = 0, dip_steps = 0, reliable = 0;
+  int el_prev_its = 0;  // Krylov iterations of the last device-controlled elasticity solve (round sizes)
   double relres_el = 0.0;
   double fx = 0.0, gx[UFT_MAXM];
 };
@@ -442,7 +446,18 @@ void cg_update_lo(hipStream_t s, int64_t n, const double *sc, int ia, int ib, do
 // (zero guess): cr = rf, cd = cx = it dinvf rf
 void cg_update_lo_cheb(hipStream_t s, int64_t n, const double *sc, int ia, int ib, double *u, const double *p,
                        double *r, const float *q, float *rf, double *part, double *out, const float *dinvf, double it,
-                       float *cr, float *cd, float *cx);
+                       float *cr, float *cd, float *cx, const double *stop = nullptr);
+// device control of the single-design flexible CG of the mixed forms (ctl:
+// stop, reliable-update request, iterations, reliable updates, last FP64
+// residual norm, tolerance, iteration limit): initial state; the test after
+// an iteration's update; the reliable update itself (FP64 K U through the
+// element phase, r = F - K U, its FP32 copy and the next cycle's first
+// Chebyshev step, r.r into rr_out and the stopping test), every launch of
+// which returns at once unless ctl asked for it
+void ksp_ctl_init(hipStream_t s, double *ctl, double rtrue, double tol, int maxit);
+void ksp_ctl_iter(hipStream_t s, const double *rr, double *ctl);
+void reliable_update(hipStream_t s, const uft_ctx *C, const double *xphys, const double *U, double *r, float *rf,
+                     const float *dinvf, double it, float *cr, float *cd, float *cx, double *rr_out, double *ctl);
 // zo = z ; z = (double) zf ; r.z, r.zo into out[0..1]
 void z_from_f32(hipStream_t s, int64_t n, const float *zf, const double *r, double *z, double *zo, double *part,
                 double *out);
