@@ -90,6 +90,13 @@ def lib():
             "orc_stress": (None, [vp, dp, dp, dp]),
             "orc_set_load_separate": (None, [vp, C.c_int]),
             "orc_case_u": (C.POINTER(C.c_double), [vp, C.c_int]),
+            "crf_create": (vp, [vp, C.c_int]),
+            "crf_destroy": (None, [vp]),
+            "crf_threads": (C.c_int, [vp]),
+            "crf_time_ke": (C.c_double, [vp]),
+            "crf_design_iteration": (C.c_int, [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                               C.POINTER(C.c_double), C.POINTER(C.c_double), dp,
+                                               np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -308,6 +315,11 @@ class Problem:
                          np.ascontiguousarray(u, dtype=np.float64), out)
         return out
 
+    def cpu_ref(self, nthreads=0):
+        """The threaded reference-structure iteration (cpu_ref.c) on this
+        problem's state; nthreads 0 = OpenMP's default."""
+        return CpuRef(self, nthreads)
+
     def run(self, maxitr):
         fx = np.zeros(maxitr)
         gx = np.zeros(maxitr)
@@ -322,3 +334,35 @@ def cantilever(etype=8, nx=8, ny=4, nz=4, jitter=0.0, seed=0, load_kind=0, **par
     fixed, ld, lv = box_bcs(nx, ny, nz, load_kind)
     p = default_params(**params)
     return Problem(elem, xyz, fixed, ld, lv, p, nlc=8 if load_kind == 2 else 1)
+
+
+class CpuRef:
+    """cpu_ref.c on a Problem: one design iteration in the reference's
+    structure (KE recomputed twice unless the problem caches KE, K
+    re-assembled, smoothed-aggregation multigrid rebuilt, FGMRES(30), Jacobi-CG
+    filter solves, the oracle's MMA) with OpenMP threads."""
+
+    PHASES = ("assembly", "amg_setup", "solve", "sensitivities", "filter_sens", "mma", "filter")
+
+    def __init__(self, prob, nthreads=0):
+        self.prob = prob
+        self.h = lib().crf_create(prob.h, nthreads)
+        self.threads = lib().crf_threads(self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().crf_destroy(self.h)
+            self.h = None
+
+    def ke_seconds(self):
+        """Seconds to compute every element matrix once with these threads."""
+        return lib().crf_time_ke(self.h)
+
+    def design_iteration(self, itr):
+        fx, gx, ch, rr = C.c_double(), (C.c_double * 8)(), C.c_double(), C.c_double()
+        t = np.zeros(8)
+        it = np.zeros(4, dtype=np.int32)
+        rc = lib().crf_design_iteration(self.h, itr, C.byref(fx), gx, C.byref(ch), C.byref(rr), t, it)
+        return {"fx": fx.value, "gx": gx[0], "ch": ch.value, "relres": rr.value, "rc": rc,
+                "times": dict(zip(self.PHASES, t[:7].tolist())), "krylov": int(it[0]), "filter_cg": int(it[1]),
+                "levels": int(it[2])}

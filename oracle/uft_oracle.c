@@ -26,7 +26,7 @@
 #include <stdlib.h>
 #include <string.h>
 
-#define ORC_MAXM 8
+#include "orc_internal.h"
 
 /* ------------------------------------------------------------------------- */
 /* Synthetic cantilever generator (workload definition, not in the reference) */
@@ -402,12 +402,6 @@ void orc_dofmap(const int32_t *elem, int64_t ne, int npe, int32_t *dof) {
 /* Sparse helpers                                                             */
 /* ------------------------------------------------------------------------- */
 
-typedef struct {
-  int64_t n;
-  int64_t *rp;
-  int32_t *ci;
-  double *v;
-} csr_t;
 
 static void csr_free(csr_t *A) {
   free(A->rp); free(A->ci); free(A->v);
@@ -477,51 +471,6 @@ static double dot(int64_t n, const double *a, const double *b) {
 /* Oracle context                                                             */
 /* ------------------------------------------------------------------------- */
 
-typedef struct {
-  /* OptPara defaults (OptPara.cpp:66-78) overridable like the options DB (:92-104) */
-  double Emin, Emax, penal, volfrac, rmin, nu, movlim, Xmin, Xmax, x0, beta0, betamax;
-  int proj, m, maxItr;
-  double rtol_el, rtol_f;   /* -ksp_rtol for elasticity / filter (both 1e-5 by default) */
-  int maxit_el, maxit_f;
-  int ke_cache;             /* 0: recompute KE twice per iteration like the reference */
-} orc_params;
-
-typedef struct {
-  int npe, nd;
-  int64_t nn, ne, ndof;
-  int32_t *elem;
-  double *xyz;
-  int nfixed;
-  int32_t *fixed;
-  int nlc, nload;
-  int32_t *ldofs;
-  double *lvals;
-  orc_params P;
-  /* derived */
-  uint8_t *isfixed;
-  double *vol, TV;
-  double *KEc;   /* cached KE (ne * nd * nd) when ke_cache */
-  double *Tel;   /* ne * npe */
-  csr_t K, KF;
-  double *KFdiag;
-  double *U, *F, *TX, *RX;
-  /* design state (OptPara.h:28-52) */
-  double *X, *XFiled, *XPhys, *Xold, *dfdx, *dgdx, *xmin, *xmax;
-  double fx, gx[ORC_MAXM];
-  double beta, btemp;
-  /* MMA state (MMA.h:76-109) */
-  int mk;
-  double asyminit, asymdec, asyminc;
-  int robust, conmod;   /* RobustAsymptotesType, constraintModification (MMA.h) */
-  int load_separate;    /* multi-load-case form: one solve per case, summed compliance */
-  double *Ucase;        /* displacements of every case (nlc * ndof) in that form */
-  double a[ORC_MAXM], c[ORC_MAXM], d[ORC_MAXM], y[ORC_MAXM], z;
-  double lam[ORC_MAXM], mu[ORC_MAXM], s[2 * ORC_MAXM], b[ORC_MAXM], grad[ORC_MAXM], Hess[ORC_MAXM * ORC_MAXM];
-  double *L, *Uas, *alpha, *betav, *p0, *q0, *pij, *qij, *xo1, *xo2;
-  /* diagnostics */
-  int it_el, it_f, dip_steps;
-  double relres_el;
-} orc_ctx;
 
 void orc_default_params(orc_params *P) {
   P->maxItr = 200; P->rmin = 0.01; P->penal = 3.0; P->Emin = 1.0e-9; P->Emax = 1.0e6;
