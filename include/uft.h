@@ -67,9 +67,12 @@ typedef struct uft_params {
   double rtol_f;       /* filter CG: ||D^-1 r|| <= rtol_f ||D^-1 b||    (-ksp_rtol) */
   int maxit_el, maxit_f;
   int precision;       /* enum uft_precision */
-  int amg_levels;      /* total levels incl. the fine one (reference: PCGAMGSetNlevels 4) */
+  int amg_levels;      /* most levels incl. the fine one (reference: PCGAMGSetNlevels 4); coarsening also stops
+                          at coarse_max */
   int cheb_degree;     /* Chebyshev-Jacobi smoother degree per level */
-  int coarse_max;      /* stop coarsening at or below this many coarse DOFs */
+  int coarse_max;      /* stop coarsening at or below this many coarse DOFs (default 4096: one design with one
+                          right-hand side keeps three levels and solves the coarsest exactly with its dense
+                          inverse; design batches and the multi-load-case form stop at 600 at most) */
   int cycle_type;      /* coarse-grid cycle: 0 V, 1 W, 2 K-cycle on every coarse level, 3 K-cycle on level 1 only */
   double cheb_lo, cheb_hi; /* Chebyshev interval [cheb_lo, cheb_hi] * lambda_max estimate
                               (-mg_levels_ksp_chebyshev_esteig 0,lo,0,hi; PCGAMG: 0.1, 1.1) */
@@ -90,15 +93,22 @@ typedef struct uft_params {
                           (FP16 with several load cases) and FP32 for the outer product (one
                           launch per apply, no element-local round trip); 0: the element blocks.  Fixed at
                           setup. */
-  int coarse_dense_max; /* the coarsest multigrid level is solved with its dense inverse, so coarsening goes on
-                           until a level has at most this many unknowns (default 3072, at most 6144).  Fixed
-                           at setup. */
+  int coarse_dense_max; /* the coarsest multigrid level is solved with its dense inverse (recomputed at every
+                           hierarchy refresh), so coarsening goes on until a level has at most this many
+                           unknowns (default 4096, at most 6144; design batches and the multi-load-case form
+                           3072 at most).  Fixed at setup. */
   int prolong_smooth;  /* 1 (default): smoothed aggregation on the fine level as PCGAMG does (SolveFem.cpp:129-136):
                           at every hierarchy refresh the tentative prolongator is smoothed once with the current
                           operator, P = (I - omega D^-1 A) P_tent with omega = 4 / (3 lambda_max(D^-1 A)), and the
                           level-1 operator is P^T A P; used for one design with superposed load cases, the
                           assembled node pattern (fine_assembled = 1) and at least three levels, otherwise the
                           tentative prolongator.  0: the tentative prolongator.  Fixed at setup. */
+  int coarse_lag;      /* one design with a coarsest level of 512 unknowns or more: 1 (default) the cycle applies
+                          the dense inverse of the previous hierarchy refresh's coarsest operator while the current
+                          one is inverted on a stream of its own, beside the solve (the first refresh inverts its
+                          own operator before the solve); 0: every refresh inverts its own coarsest operator
+                          before the solve.  Either way the outer iteration stops on the same residual test, so
+                          the solution is the same to the tolerance.  Fixed at setup. */
 } uft_params;
 
 /* --- context ------------------------------------------------------------ */

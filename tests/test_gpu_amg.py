@@ -153,7 +153,7 @@ def test_level_operators_in_each_precision(assembled):
     from uft.capi import cantilever
     print(f"[case] level operators FP64/FP32/FP16 fine_assembled={assembled}", flush=True)
     ctx = cantilever(8, 24, 12, 12, jitter=0.15, seed=9, Emax=1.0, volfrac=0.3, rmin=0.2, precision=2,
-                     fine_assembled=assembled)
+                     fine_assembled=assembled, coarse_max=600)
     x = np.random.default_rng(4).uniform(0.01, 1.0, ctx.size("ne"))
     ctx.set("XPhys", x)
     ctx.fem_obj_sens()
@@ -176,3 +176,28 @@ def test_level_operators_in_each_precision(assembled):
                 assert e16 < 1e-2 and e16 > 1e-5
             else:
                 assert e16 < 2e-3 and e16 > 1e-7  # FP16 rounding is visible, and only rounding
+
+
+def test_coarse_sweep_inverse_full_size():
+    """The tiled Gauss-Jordan sweep (kernels_dense.hip) that inverts the
+    3,840-unknown coarsest level of the 392,000-element cantilever at every
+    hierarchy refresh, against numpy's inverse of the same operator (read
+    back through uft_amg_operator_get), on a random density field."""
+    from uft.capi import cantilever
+    print("[case] coarse sweep inverse, 140x70x40", flush=True)
+    ctx = cantilever(8, 140, 70, 40, jitter=0.15, seed=392000, Emax=1.0, volfrac=0.3, rmin=1.5 / 70)
+    n_lev, _ = ctx.solver_info()
+    assert len(n_lev) == 3 and n_lev[-1] >= 512, n_lev
+    rng = np.random.default_rng(4)
+    ctx.set("XPhys", rng.uniform(0.01, 1.0, 140 * 70 * 40))
+    ctx.fem_obj_sens()
+    A = _bsr_dense(*ctx.amg_operator(2))
+    inv = ctx.amg_coarse_inverse()
+    n = A.shape[0]
+    cond = np.linalg.cond(A)
+    print("coarsest n", n, "cond", cond, flush=True)
+    tol = max(1e-12, 1e-14 * cond)
+    want = np.linalg.inv(A)
+    assert np.array_equal(inv, inv.T)  # the sweep keeps the lower triangle only
+    assert np.abs(inv - want).max() <= tol * np.abs(want).max()
+    assert np.abs(inv @ A - np.eye(n)).max() <= tol

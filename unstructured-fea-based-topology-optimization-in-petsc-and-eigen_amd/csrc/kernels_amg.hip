@@ -568,6 +568,10 @@ void bsr_to_dense(hipStream_t s, int64_t nb_rows, const int32_t *rp, const int32
 // A (2 n^2 doubles): A[0, n^2) holds the matrix on entry, the rest is scratch
 // for the panels; the inverse is written to Ainv (and to Ainvf when given).
 void dense_spd_inverse(hipStream_t s, int64_t n, double *A, double *Ainv, float *Ainvf) {
+  if (n >= UFT_SWEEP_MIN) {  // large coarsest levels: the tiled sweep (scratch after the matrix)
+    dense_spd_sweep(s, n, A, A + n * n, Ainv, Ainvf);
+    return;
+  }
   const int nn = (int)n;
   const int nb = std::max(1, std::min(GJ_NB, nn / 4));
   k_copy_n<<<nblk(n * n), TPB, 0, s>>>(n * n, A, Ainv);

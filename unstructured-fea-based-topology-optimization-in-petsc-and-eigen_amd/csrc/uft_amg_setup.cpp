@@ -209,8 +209,14 @@ int uft_amg_build_host(uft_ctx *C, std::vector<AmgHostLevel> &H, std::vector<int
   }
 
   const int maxlev = std::max(2, std::min((int)UFT_MAXLEV, C->P.amg_levels));
-  const int64_t cmax = std::max(6, C->P.coarse_max);
-  const int64_t dense_limit = C->P.coarse_dense_max;  // the coarsest level must fit the dense inverse
+  // one design with one right-hand side: its coarsest level is inverted by
+  // the tiled sweep (kernels_dense.hip) and solved exactly, so coarsening
+  // stops at coarse_max; design batches and the multi-load-case form keep
+  // coarsest levels of at most 600 unknowns for their per-design and
+  // multi-right-hand-side dense solves
+  const bool single = C->bt.nseg == 0 && !(C->P.load_cases == 1 && C->nlc > 1);
+  const int64_t cmax = single ? std::max(6, C->P.coarse_max) : std::min(std::max(6, C->P.coarse_max), 600);
+  const int64_t dense_limit = single ? C->P.coarse_dense_max : std::min<int64_t>(C->P.coarse_dense_max, UFT_TAIL_MAX);
 
   // ---- level 0 -> 1
   Graph g = node_graph(nn, ne, npe, C->elem.data());
@@ -389,7 +395,7 @@ int uft_amg_build_host(uft_ctx *C, std::vector<AmgHostLevel> &H, std::vector<int
       const int64_t nu = hp.nunit_next;
       // a batch coarsens until each design's block of the coarsest level fits
       const int64_t nu_c = nseg > 0 ? max_per_design(C->bt.uoff[lev]) : nu;
-      // (and to at least three levels: its coarsest solve works on 6x6 blocks)
+      // (a batch to at least three levels: its coarsest solve works on 6x6 blocks)
       const bool want_more =
           (6 * nu_c > cmax && lev + 1 < maxlev) || 6 * nu_c > dense_limit || (nseg > 0 && lev < 2);
       if (!want_more || lev + 1 >= UFT_MAXLEV) break;

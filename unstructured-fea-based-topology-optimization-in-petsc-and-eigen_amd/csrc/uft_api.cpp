@@ -89,6 +89,8 @@ int build_space(uft_ctx *C, SegSpace &S, const std::vector<int64_t> &off) {
 
 void free_all(uft_ctx *C) {
   if (C->s) (void)hipStreamSynchronize(C->s);
+  for (hipStream_t t : C->side)  // a lagged coarsest inverse may still run
+    if (t) (void)hipStreamSynchronize(t);
   for (void *p : C->allocs) (void)hipFree(p);
   C->allocs.clear();
 }
@@ -517,6 +519,10 @@ void coarse_direct(uft_ctx *C, int l, const T *r, T *x) {
   k_restrict(C->s, Lv, Ln.nunit, r, vn.b);
   if constexpr (std::is_same<T, double>::value) {
     uftk::dense_mv(C->s, H.ncoarse, H.d_cinv, vn.b, vn.x);
+  } else if (H.d_cinvf && H.ncoarse % 4 == 0 && ((uintptr_t)vn.b % 16) == 0 && ((uintptr_t)vn.x % 16) == 0) {
+    uftk::dense_mvf4(C->s, H.ncoarse, H.d_cinvf, vn.b, vn.x);  // the FP32 copy of the inverse
+  } else if (H.d_cinvf) {
+    uftk::dense_mvf(C->s, H.ncoarse, H.d_cinvf, vn.b, vn.x);
   } else {
     uftk::f2d(C->s, Ln.n, vn.b, Ln.kc);
     uftk::dense_mv(C->s, H.ncoarse, H.d_cinv, Ln.kc, Ln.kv);
@@ -1020,6 +1026,39 @@ int amg_refresh(uft_ctx *C, const double *xphys) {
   CountOff off(C);
   return amg_refresh_impl(C, xphys);
 }
+// The dense inverse of the coarsest level (and its FP32 copy in the mixed
+// forms): the tiled sweep straight from the block-sparse operator for large
+// levels, the small blocked Gauss-Jordan on the dense matrix otherwise.
+// Returns true when the inverse is lagged (coarse_lag) and this operator's
+// inverse is still to be computed beside the solve (lag_inverse).
+bool coarse_inverse(uft_ctx *C, hipStream_t s) {
+  AmgHier &H = C->amg;
+  AmgLevel &Lc = H.L[H.nlev - 1];
+  if (H.ncoarse >= UFT_SWEEP_MIN) {
+    if (H.d_cinv2 && H.lag_spare) return true;
+    // this operator's inverse before the solve (also the first refresh when lagged)
+    uftk::dense_spd_sweep_bsr(s, H.ncoarse, Lc.nblk, Lc.d_brow, Lc.d_ci, Lc.d_val, H.d_cmat, H.d_cinv, H.d_cinvf);
+    H.lag_spare = H.d_cinv2 != nullptr;
+    return false;
+  }
+  uftk::bsr_to_dense(s, Lc.nunit, Lc.d_rp, Lc.d_ci, Lc.d_val, H.ncoarse, H.d_cmat);
+  uftk::dense_spd_inverse(s, H.ncoarse, H.d_cmat, H.d_cinv, H.d_cinvf);
+  return false;
+}
+// The lagged inverse of the current coarsest operator into the spare buffers
+// on side stream 2, queued after everything on that stream (the operator;
+// the last uses of the spare buffers are ordered before by the refresh's
+// fork); nothing joins it into the main stream until the next refresh
+// (amg_refresh_impl), so it runs beside the solve.
+void lag_inverse(uft_ctx *C) {
+  AmgHier &H = C->amg;
+  AmgLevel &Lc = H.L[H.nlev - 1];
+  uftk::dense_spd_sweep_bsr(C->side[2], H.ncoarse, Lc.nblk, Lc.d_brow, Lc.d_ci, Lc.d_val, H.d_cmat, H.d_cinv2,
+                            H.d_cinvf2);
+  (void)hipEventRecord(C->ev_inv, C->side[2]);
+  H.lag_pending = true;
+}
+
 // Levels >= 1 after the level-1 operator: its post-processing (unit diagonal
 // of empty rows, block-diagonal inverse, lower-precision copies), the coarser
 // Galerkin products, the coarse levels' eigenvalue estimates and the coarsest
@@ -1036,10 +1075,12 @@ int refresh_coarse(uft_ctx *C, bool par, int ks, int first_est) {
     if (Ln.d_valh)
       uftk::bsr6_to_half(s, Ln.nblk, Ln.nunit, Ln.d_brow, Ln.d_ci, Ln.d_val, Ln.d_dinv, Ln.d_valh, Ln.d_dh);
   }
-  AmgLevel &Lc = H.L[H.nlev - 1];
   if (!par) {
-    uftk::bsr_to_dense(s, Lc.nunit, Lc.d_rp, Lc.d_ci, Lc.d_val, H.ncoarse, H.d_cmat);
-    uftk::dense_spd_inverse(s, H.ncoarse, H.d_cmat, H.d_cinv, H.d_cinvf);
+    if (coarse_inverse(C, s)) {
+      UFT_HIP(C, hipEventRecord(C->ev_side[4], s));
+      UFT_HIP(C, hipStreamWaitEvent(C->side[2], C->ev_side[4], 0));
+      lag_inverse(C);
+    }
     for (int l = first_est; l + 1 < H.nlev; ++l) {
       int rc = estimate_lmax(C, l, ks);
       if (rc) return rc;
@@ -1051,13 +1092,13 @@ int refresh_coarse(uft_ctx *C, bool par, int ks, int first_est) {
   C->s = C->side[1];
   for (int l = 1; l + 1 < H.nlev; ++l) lanczos_enqueue(C, l, ks, C->d_part2, S_MISC + 8, C->d_lanczos + 192 * l);
   C->s = C->side[2];
-  uftk::bsr_to_dense(C->s, Lc.nunit, Lc.d_rp, Lc.d_ci, Lc.d_val, H.ncoarse, H.d_cmat);
-  uftk::dense_spd_inverse(C->s, H.ncoarse, H.d_cmat, H.d_cinv, H.d_cinvf);
+  const bool lag = coarse_inverse(C, C->s);
   C->s = s;
   for (int i = 0; i < 3; ++i) {
     UFT_HIP(C, hipEventRecord(C->ev_side[i], C->side[i]));
     UFT_HIP(C, hipStreamWaitEvent(s, C->ev_side[i], 0));
   }
+  if (lag) lag_inverse(C);  // after the join: the main stream does not wait for it
   std::vector<double> hl((size_t)192 * (H.nlev - 1));
   UFT_HIP(C, hipMemcpyAsync(hl.data(), C->d_lanczos, hl.size() * sizeof(double), hipMemcpyDeviceToHost, s));
   if (int rc = sync(C)) return rc;
@@ -1068,6 +1109,12 @@ int refresh_coarse(uft_ctx *C, bool par, int ks, int first_est) {
 int amg_refresh_impl(uft_ctx *C, const double *xphys) {
   AmgHier &H = C->amg;
   hipStream_t s = C->s;
+  if (H.lag_pending) {  // the previous operator's inverse (computed beside the last solve) becomes the cycle's
+    UFT_HIP(C, hipStreamWaitEvent(s, C->ev_inv, 0));
+    std::swap(H.d_cinv, H.d_cinv2);
+    std::swap(H.d_cinvf, H.d_cinvf2);
+    H.lag_pending = false;
+  }
   C->cur_xphys = xphys;
   C->refreshes++;
   uftk::diag_K(s, C, xphys, H.L[0].d_dinv, H.L[0].d_dinvf);
@@ -1907,7 +1954,7 @@ void uft_default_params(uft_params *p) {
   p->precision = UFT_FP64;
   p->amg_levels = 4;                    // SolveFem.cpp:131
   p->cheb_degree = 2;
-  p->coarse_max = 600;
+  p->coarse_max = 4096;
   p->cycle_type = 2;
   p->cheb_lo = 0.1;
   p->cheb_hi = 1.1;
@@ -1916,8 +1963,9 @@ void uft_default_params(uft_params *p) {
   p->graph = 0;
   p->cheb_degree_coarse = 0;
   p->fine_assembled = 1;
-  p->coarse_dense_max = 3072;
+  p->coarse_dense_max = 4096;
   p->prolong_smooth = 1;
+  p->coarse_lag = 1;
 }
 
 int uft_create(uft_ctx **out, int device) {
@@ -1953,6 +2001,7 @@ void uft_destroy(uft_ctx *C) {
     if (e) (void)hipEventDestroy(e);
   for (hipStream_t t : C->side)
     if (t) (void)hipStreamDestroy(t);
+  if (C->ev_inv) (void)hipEventDestroy(C->ev_inv);
   if (C->s) (void)hipStreamDestroy(C->s);
   delete C;
 }
@@ -1974,6 +2023,7 @@ int uft_params_set(uft_ctx *C, const uft_params *p) {
     return uft_set_error(C, UFT_ERR_ARG, "Chebyshev degrees must be in [1, 16] (coarse: 0 or [1, 16]), cycle_type in [0, 3]");
   if (p->prolong_smooth != 0 && p->prolong_smooth != 1)
     return uft_set_error(C, UFT_ERR_ARG, "prolong_smooth must be 0 or 1");
+  if (p->coarse_lag != 0 && p->coarse_lag != 1) return uft_set_error(C, UFT_ERR_ARG, "coarse_lag must be 0 or 1");
   if (p->coarse_dense_max < 6 || p->coarse_dense_max > 6144)
     return uft_set_error(C, UFT_ERR_ARG, "coarse_dense_max must be in [6, 6144]");
   if (p->precision < UFT_FP64 || p->precision > UFT_MIXED_H)
@@ -1981,7 +2031,8 @@ int uft_params_set(uft_ctx *C, const uft_params *p) {
   if (C->is_setup && (p->m != C->P.m || p->rmin != C->P.rmin || p->nu != C->P.nu || p->precision != C->P.precision ||
                       p->load_cases != C->P.load_cases || p->fine_assembled != C->P.fine_assembled ||
                       p->amg_levels != C->P.amg_levels || p->coarse_max != C->P.coarse_max ||
-                      p->coarse_dense_max != C->P.coarse_dense_max || p->prolong_smooth != C->P.prolong_smooth))
+                      p->coarse_dense_max != C->P.coarse_dense_max || p->prolong_smooth != C->P.prolong_smooth ||
+                      p->coarse_lag != C->P.coarse_lag))
     return uft_set_error(C, UFT_ERR_STATE,
                          "m, rmin, nu, precision, fine_assembled and the hierarchy shape are fixed at setup");
   C->P = *p;
@@ -2444,11 +2495,22 @@ int uft_setup(uft_ctx *C) {
       if ((rc = dalloc_n(C, &A.d_cmat, B.coff[B.nseg]))) return rc;
       if ((rc = dalloc_n(C, &A.d_cinv, B.coff[B.nseg]))) return rc;
     } else {
-      // matrix plus the inverse's panel scratch (at most n^2/16 + n^2/2 + 2n + 1 doubles)
-      if ((rc = dalloc_n(C, &A.d_cmat, 2 * A.ncoarse * A.ncoarse + 2 * A.ncoarse + 1))) return rc;
+      // matrix plus the inverse's scratch (the small-size Gauss-Jordan: at most n^2/16 + n^2/2 + 2n + 1
+      // doubles of panels; the tiled sweep: uftk::dense_sweep_scratch)
+      const int64_t nc = A.ncoarse;
+      const int64_t need = nc >= UFT_SWEEP_MIN ? uftk::dense_sweep_scratch(nc) : 2 * nc * nc + 2 * nc + 1;
+      if ((rc = dalloc_n(C, &A.d_cmat, need))) return rc;
       if ((rc = dalloc_n(C, &A.d_cinv, A.ncoarse * A.ncoarse))) return rc;
       if (is_mixed(C))
         if ((rc = dalloc_n(C, &A.d_cinvf, A.ncoarse * A.ncoarse))) return rc;
+      if (C->P.coarse_lag && nc >= UFT_SWEEP_MIN && !(C->P.load_cases == 1 && C->nlc > 1)) {
+        if ((rc = dalloc_n(C, &A.d_cinv2, nc * nc))) return rc;
+        if (is_mixed(C))
+          if ((rc = dalloc_n(C, &A.d_cinvf2, nc * nc))) return rc;
+        if (!C->ev_inv && hipEventCreateWithFlags(&C->ev_inv, hipEventDisableTiming) != hipSuccess)
+          return uft_set_error(C, UFT_ERR_HIP, "hipEventCreateWithFlags failed");
+      }
+      A.lag_spare = A.lag_pending = false;
     }
     UFT_HIP(C, hipStreamSynchronize(s));
   }
@@ -2974,6 +3036,7 @@ int uft_element_kf(uft_ctx *C, int64_t e, double *kf, double *t) {
 
 int uft_sync(uft_ctx *C) {
   if (!C) return UFT_ERR_ARG;
+  if (C->amg.lag_pending) UFT_HIP(C, hipStreamSynchronize(C->side[2]));  // a lagged coarsest inverse
   return sync(C);
 }
 

@@ -14,6 +14,7 @@
 #define UFT_RED_BLOCK 256
 #define UFT_TAIL_MAX 3072  // coarsest unknowns handled by the fused coarse tail (LDS)
 #define UFT_MR_TAIL_MAX 3072  // coarsest unknowns of the multi-load-case dense solve
+#define UFT_SWEEP_MIN 512  // coarsest levels from this size are inverted by the tiled sweep (kernels_dense.hip)
 #define UFT_ELL_SLICE 64      // nodes per slice of the assembled fine operator (one wavefront)
 
 // Per-row epilogue of a fused operator apply t = A u (see kernels_cycle.hip):
@@ -146,6 +147,13 @@ struct AmgHier {
   double *d_cinv = nullptr;
   float *d_cinvf = nullptr;
   double *d_cwork = nullptr;
+  // lagged coarsest inverse (coarse_lag): the cycle applies the inverse of
+  // the previous refresh's coarsest operator while the current one is being
+  // inverted into the spare buffers on side stream 2
+  double *d_cinv2 = nullptr;
+  float *d_cinvf2 = nullptr;
+  bool lag_spare = false;    // the spare buffers hold the inverse of the last refresh (swap at the next)
+  bool lag_pending = false;  // a spare inverse is being computed on side[2] (event ev_inv)
 };
 
 // Multi-load-case form: nr right-hand sides solved together, vectors stored
@@ -354,6 +362,7 @@ struct uft_ctx {
   // eigenvalue estimate, coarse-level estimates, coarsest inverse) and their
   // fork / join events; the coarse estimates' block partials
   hipStream_t side[3] = {nullptr, nullptr, nullptr};
+  hipEvent_t ev_inv = nullptr;  // the lagged coarsest inverse done on side[2]
   hipEvent_t ev_side[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   double *d_part2 = nullptr;
   std::vector<int> evkind;  // per triple: 0 FP64 apply, 1 FP32 apply
@@ -507,6 +516,15 @@ void bsr_to_dense(hipStream_t s, int64_t nb_rows, const int32_t *rp, const int32
                   int64_t n, double *dense);
 void dense_spd_inverse(hipStream_t s, int64_t n, double *A, double *Ainv, float *Ainvf);
 void dense_mv(hipStream_t s, int64_t n, const double *A, const double *x, double *y);
+// blocked symmetric Gauss-Jordan sweep on 64x64 tiles (kernels_dense.hip):
+// the inverse of the SPD row-major n x n matrix A into Ainv (FP32 copy into
+// Ainvf when given), S = dense_sweep_scratch(n) doubles of scratch
+int64_t dense_sweep_scratch(int64_t n);
+void dense_spd_sweep(hipStream_t s, int64_t n, const double *A, double *S, double *Ainv, float *Ainvf);
+void dense_spd_sweep_bsr(hipStream_t s, int64_t n, int64_t nblk, const int32_t *brow, const int32_t *ci,
+                         const double *val, double *S, double *Ainv, float *Ainvf);
+// y = A x, row-major FP32 n x n, n a multiple of 4 and 16-byte aligned operands
+void dense_mvf4(hipStream_t s, int64_t n, const float *A, const float *x, float *y);
 void dense_mvf(hipStream_t s, int64_t n, const float *A, const float *x, float *y);
 // multi-load-case kernels (kernels_mrhs.hip)
 void mr_cheb_first(hipStream_t s, int nr, int64_t n, const double *b, const double *dinv, double it, double *r,

@@ -38,7 +38,7 @@ class Params(C.Structure):
                 ("cheb_lo", C.c_double), ("cheb_hi", C.c_double), ("esteig_steps", C.c_int),
                 ("load_cases", C.c_int), ("graph", C.c_int), ("cheb_degree_coarse", C.c_int),
                 ("fine_assembled", C.c_int), ("coarse_dense_max", C.c_int),
-                ("prolong_smooth", C.c_int)]
+                ("prolong_smooth", C.c_int), ("coarse_lag", C.c_int)]
 
 
 _LIB = None
