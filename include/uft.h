@@ -266,6 +266,15 @@ int uft_apply_K_cases(uft_ctx *ctx, int kind, const double *u, double *y);
  * 2 = the UFT_MIXED_H smoother's low-precision blocks (int8 with a block scale or
  * FP16); x, y of length n_l. */
 int uft_level_apply(uft_ctx *ctx, int level, int kind, const double *x, double *y);
+/* Measurement (no counterpart in the reference): average duration in
+ * microseconds of reps back-to-back launches of a fine operator apply as
+ * the solve issues it, on the operators of the last hierarchy refresh and
+ * the solver's own work vectors (their contents are overwritten), timed with
+ * HIP events on the context's stream: kind 0 the FP64 stiffness apply
+ * (element and node phase), 1 the mixed forms' FP32 outer product, 2 the
+ * UFT_MIXED_H smoother's operator with its Chebyshev-step epilogue. Kind 0
+ * needs the FP64 form, kinds 1 and 2 a mixed form; one design, one load case. */
+int uft_time_fine_operator(uft_ctx *ctx, int kind, int reps, double *us);
 
 /* --- MMA settings, restart and KKT residual (MMA.h:8-52) ------------------ */
 /* MMA::SetAsymptotes (MMA.cpp:347-355): asymptote initialisation,

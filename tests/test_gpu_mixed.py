@@ -107,3 +107,29 @@ def test_mixed_tet4_loop_vs_oracle(O):
         assert abs(fx - rfx) <= 1e-6 * abs(rfx), it
     assert ctx.stats()["relres_el"] <= 1e-12
     assert np.max(np.abs(ctx.get("X") - ref.vec("X"))) <= 1e-6
+
+
+def test_time_fine_operator_kinds():
+    """The back-to-back timing entry point accepts exactly the kinds of the
+    context's form, rejects a call before the first refresh, and returns a
+    positive duration; the solve is unchanged by it (its vectors are
+    rebuilt by every solve)."""
+    from uft.capi import cantilever, UFTError
+    args = dict(nx=12, ny=6, nz=6, jitter=0.1, seed=5)
+    prm = dict(Emax=1.0, volfrac=0.3, rmin=0.25, rtol_el=1e-12, rtol_f=1e-12)
+    for precision, ok, bad in ((0, (0,), (1, 2)), (1, (1,), (0, 2)), (2, (1, 2), (0,))):
+        print(f"[case] time_fine_operator precision={precision}", flush=True)
+        ctx = cantilever(8, **args, precision=precision, **prm)
+        ctx.init_design()
+        with pytest.raises(UFTError):
+            ctx.time_fine_operator(ok[0], 3)
+        ctx.design_iteration(1)
+        f1 = ctx.fem_obj_sens()[0]
+        for k in ok:
+            assert ctx.time_fine_operator(k, 5) > 0.0
+        for k in bad:
+            with pytest.raises(UFTError):
+                ctx.time_fine_operator(k, 5)
+        f2 = ctx.fem_obj_sens()[0]
+        assert abs(f2 - f1) <= 1e-9 * abs(f1)
+        ctx.close()

@@ -172,7 +172,11 @@ __global__ __launch_bounds__(DTPB) void k_sweep_panel(int k, const double *__res
 __global__ __launch_bounds__(DTPB) void k_sweep_update(int k, int nt, double *__restrict__ T,
                                                        const double *__restrict__ Dk, const double *__restrict__ Wb,
                                                        const double *__restrict__ Cb, double *__restrict__ Dnext) {
-  const int64_t t = blockIdx.x;
+  // workgroup b runs on XCD b mod 8: each XCD takes a contiguous range of
+  // tiles (a band of tile rows), so the panel tiles W_i it reads stay in its
+  // L2 (the C_j of one band, at most nt tiles, too)
+  const int64_t nb = gridDim.x, xb = blockIdx.x % 8, q8 = nb / 8, r8 = nb % 8;
+  const int64_t t = (xb < r8 ? xb * (q8 + 1) : r8 * (q8 + 1) + (xb - r8) * q8) + blockIdx.x / 8;
   int i = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
   while ((int64_t)(i + 1) * (i + 2) / 2 <= t) ++i;
   while ((int64_t)i * (i + 1) / 2 > t) --i;

@@ -588,30 +588,30 @@ __global__ __launch_bounds__(TPB) void k_elem_apply(int ne, const int32_t *__res
                                                     const double *__restrict__ gate) {
   constexpr int ND = 3 * NPE, NP = ND * (ND + 1) / 2, NQ = NP / 2;
   if (gate && gate[0] == 0.0) return;  // a device-controlled solve did not ask for this product
-  const int e = blockIdx.x * TPB + threadIdx.x;
-  if (e >= ne) return;
-  T ue[ND], ye[ND];
+  for (int e = blockIdx.x * TPB + threadIdx.x; e < ne; e += gridDim.x * TPB) {
+    T ue[ND], ye[ND];
 #pragma unroll
-  for (int a = 0; a < NPE; ++a) {
-    const int n = elem[(size_t)a * ne + e];
-    const unsigned m = fixm[n];
+    for (int a = 0; a < NPE; ++a) {
+      const int n = elem[(size_t)a * ne + e];
+      const unsigned m = fixm[n];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const T v = u[3 * (size_t)n + c];
-      ue[c * NPE + a] = ((m >> c) & 1u) ? T(0) : v;
+      for (int c = 0; c < 3; ++c) {
+        const T v = u[3 * (size_t)n + c];
+        ue[c * NPE + a] = ((m >> c) & 1u) ? T(0) : v;
+      }
     }
+#pragma unroll
+    for (int k = 0; k < ND; ++k) ye[k] = T(0);
+    static_for<0, NQ>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      const T2 kk = KE[(size_t)q * ne + e];
+      kacc<pk_row(2 * q, ND), pk_col(2 * q, ND)>(kk.x, ue, ye);
+      kacc<pk_row(2 * q + 1, ND), pk_col(2 * q + 1, ND)>(kk.y, ue, ye);
+    });
+    const T E = (T)simpE(sc, xphys[e]);
+#pragma unroll
+    for (int k = 0; k < ND; ++k) Ye[(size_t)k * ne + e] = E * ye[k];
   }
-#pragma unroll
-  for (int k = 0; k < ND; ++k) ye[k] = T(0);
-  static_for<0, NQ>([&](auto qc) {
-    constexpr int q = decltype(qc)::value;
-    const T2 kk = KE[(size_t)q * ne + e];
-    kacc<pk_row(2 * q, ND), pk_col(2 * q, ND)>(kk.x, ue, ye);
-    kacc<pk_row(2 * q + 1, ND), pk_col(2 * q + 1, ND)>(kk.y, ue, ye);
-  });
-  const T E = (T)simpE(sc, xphys[e]);
-#pragma unroll
-  for (int k = 0; k < ND; ++k) Ye[(size_t)k * ne + e] = E * ye[k];
 }
 
 // Element phase of the diagonal: Ye[k][e] = E(x_e) KE_e[k][k].
@@ -721,9 +721,8 @@ __global__ __launch_bounds__(TPB) void k_reliable_node(int nn, int ne, const dou
                                                        double it, float *__restrict__ cr, float *__restrict__ cd,
                                                        float *__restrict__ cx, double *__restrict__ part) {
   if (ctl[1] == 0.0) return;
-  const int n = blockIdx.x * TPB + threadIdx.x;
   double ds[1] = {0.0};
-  if (n < nn) {
+  for (int n = blockIdx.x * TPB + threadIdx.x; n < nn; n += gridDim.x * TPB) {
     double acc[3] = {0.0, 0.0, 0.0};
     for (int k = iptr[n]; k < iptr[n + 1]; ++k) {
       const int v = inc[k];
@@ -1357,18 +1356,20 @@ void reliable_update(hipStream_t s, const uft_ctx *C, const double *xphys, const
   const SimpCoef sc{C->P.Emin, C->P.Emax - C->P.Emin, C->P.penal};
   const int ne = (int)C->ne, nn = (int)C->nn;
   const double *gate = ctl + 1;
+  // (an idle launch costs little: every workgroup returns at once)
+  const unsigned gn = nblk(nn);
   if (C->npe == 8) {
     elem_phase_t<8, double, double2>(s, C->elem_variant, ne, C->d_elem, (const double2 *)C->d_KE, xphys, sc, C->d_fixm,
                                      U, C->d_Ye, gate);
-    k_reliable_node<8><<<nblk(nn), TPB, 0, s>>>(nn, ne, ctl, C->d_iptr, C->d_inc, C->d_Ye, C->d_fixm, U, C->d_F, r, rf,
-                                                dinvf, it, cr, cd, cx, C->d_part);
+    k_reliable_node<8><<<gn, TPB, 0, s>>>(nn, ne, ctl, C->d_iptr, C->d_inc, C->d_Ye, C->d_fixm, U, C->d_F, r, rf,
+                                          dinvf, it, cr, cd, cx, C->d_part);
   } else {
     elem_phase_t<4, double, double2>(s, C->elem_variant, ne, C->d_elem, (const double2 *)C->d_KE, xphys, sc, C->d_fixm,
                                      U, C->d_Ye, gate);
-    k_reliable_node<4><<<nblk(nn), TPB, 0, s>>>(nn, ne, ctl, C->d_iptr, C->d_inc, C->d_Ye, C->d_fixm, U, C->d_F, r, rf,
-                                                dinvf, it, cr, cd, cx, C->d_part);
+    k_reliable_node<4><<<gn, TPB, 0, s>>>(nn, ne, ctl, C->d_iptr, C->d_inc, C->d_Ye, C->d_fixm, U, C->d_F, r, rf,
+                                          dinvf, it, cr, cd, cx, C->d_part);
   }
-  k_ksp_ctl_reliable<<<1, TPB, 0, s>>>(C->d_part, (int)nblk(nn), rr_out, ctl);
+  k_ksp_ctl_reliable<<<1, TPB, 0, s>>>(C->d_part, (int)gn, rr_out, ctl);
 }
 
 __global__ void k_inv(int64_t n, const double *__restrict__ d, double *__restrict__ dinv, float *__restrict__ dinvf) {

@@ -3300,6 +3300,63 @@ extern "C" int uft_level_apply(uft_ctx *C, int level, int kind, const double *x,
   return sync(C);
 }
 
+extern "C" int uft_time_fine_operator(uft_ctx *C, int kind, int reps, double *us) {
+  if (int rc = check_setup(C)) return rc;
+  if (!us || reps < 1 || kind < 0 || kind > 2) return uft_set_error(C, UFT_ERR_ARG, "uft_time_fine_operator: bad arguments");
+  if (C->refreshes == 0) return uft_set_error(C, UFT_ERR_STATE, "uft_time_fine_operator: no hierarchy refresh yet");
+  if ((kind > 0) != is_mixed(C))
+    return uft_set_error(C, UFT_ERR_ARG, "uft_time_fine_operator: kind %d does not match the form", kind);
+  if (kind == 2 && C->P.precision != UFT_MIXED_H)
+    return uft_set_error(C, UFT_ERR_ARG, "uft_time_fine_operator: kind 2 needs UFT_MIXED_H");
+  if (C->mr.nr || C->bt.nseg) return uft_set_error(C, UFT_ERR_ARG, "uft_time_fine_operator: one design, one load case");
+  hipStream_t s = C->s;
+  AmgLevel &L0 = C->amg.L[0];
+  const bool cnt = C->count_on;
+  C->count_on = false;
+  const bool pe = C->prof;  // no per-launch timers inside the measurement
+  C->prof = false;
+  auto launch = [&]() {
+    if (kind == 0) {
+      RowOp<double> op;
+      op.mode = ROW_STORE;
+      op.y = L0.t;
+      lev_op(C, 0, (const double *)L0.x, op);
+    } else if (kind == 1) {
+      fine_apply_lo(C, C->cur_xphys, C->d_p, -1, true);
+    } else {  // a Chebyshev step of the fine smoother: r -= A d; d = c1 d + c2 D^-1 r; x += d
+      LV<float> v = lv(L0, (float *)nullptr);
+      const ChebCoef cc = cheb_coef(L0.lmax, std::max(2, level_degree(C, 0)), C->P.cheb_lo / C->P.cheb_hi);
+      RowOp<float> st;
+      st.mode = ROW_STEP;
+      st.r = v.r;
+      st.dinv = v.dinv;
+      st.x = v.x;
+      st.c1 = cc.c1[1];
+      st.c2 = cc.c2[1];
+      st.dout = v.t;
+      st.addprev = 0;
+      lev_op(C, 0, (const float *)v.d, st);
+    }
+  };
+  launch();  // warm-up
+  hipEvent_t e0, e1;
+  UFT_HIP(C, hipEventCreate(&e0));
+  UFT_HIP(C, hipEventCreate(&e1));
+  UFT_HIP(C, hipEventRecord(e0, s));
+  for (int k = 0; k < reps; ++k) launch();
+  UFT_HIP(C, hipEventRecord(e1, s));
+  int rc = sync(C);
+  float ms = 0.f;
+  if (!rc) UFT_HIP(C, hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  C->prof = pe;
+  C->count_on = cnt;
+  if (rc) return rc;
+  *us = 1e3 * ms / reps;
+  return UFT_OK;
+}
+
 extern "C" int uft_load_case_u(uft_ctx *C, int k, double *host) {
   if (int rc = check_setup(C)) return rc;
   if (!host || !C->mr.nr || k < 0 || k >= C->mr.ncase)

@@ -109,6 +109,7 @@ def lib():
         "uft_amg_transfer_get": (C.c_int, [vp, C.c_int, ip, dp]),
         "uft_amg_coarse_inverse_get": (C.c_int, [vp, dp]),
         "uft_phase_times": (C.c_int, [vp, pd]),
+        "uft_time_fine_operator": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(C.c_double)]),
         "uft_mma_set_asymptotes": (C.c_int, [vp, C.c_double, C.c_double, C.c_double]),
         "uft_mma_set_robust_asymptotes": (C.c_int, [vp, C.c_int]),
         "uft_mma_constraint_modification": (C.c_int, [vp, C.c_int]),
@@ -550,6 +551,13 @@ class Context:
         self._chk(self.L.uft_solve_traffic(self.h, C.byref(b), C.byref(ms), C.byref(a), C.byref(n)),
                   "uft_solve_traffic")
         return b.value, ms.value, a.value, n.value
+
+    def time_fine_operator(self, kind, reps=20):
+        """Average microseconds of reps back-to-back launches of a fine operator
+        apply (uft_time_fine_operator: 0 FP64, 1 FP32 outer, 2 smoother step)."""
+        us = C.c_double()
+        self._chk(self.L.uft_time_fine_operator(self.h, kind, reps, C.byref(us)), "uft_time_fine_operator")
+        return us.value
 
     def phase_times(self):
         ms = (C.c_double * 8)()
