@@ -12,7 +12,9 @@ before the timed region; setup (element operators, hierarchy) is reported
 separately.
 
 Prints one JSON line.  Extra fields: roofline (the dominant fine operator
-kernel, timed with HIP events on its launch stream) and cpu_baseline (one full
+kernel: 20 back-to-back launches after the timed steps on the last refresh's
+operators, HIP events on its launch stream; the events around one in every
+16 launches inside the timed steps are kept beside it) and cpu_baseline (one full
 design iteration of the same workload in the reference's structure on the
 host cores: oracle/cpu_ref.c with OpenMP threads).
 

@@ -48,13 +48,16 @@ enum uft_vec_id {
 enum uft_size_id { UFT_SIZE_NE = 0, UFT_SIZE_NN = 1, UFT_SIZE_NDOF = 2, UFT_SIZE_NPE = 3 };
 
 /* UFT_FP64: every operator product in FP64.  UFT_MIXED: FP64 flexible outer
- * Krylov loop (residual, inner products, iterate), multigrid preconditioner
- * with FP32 element blocks and coarse operators, FP64 coarsest solve.
- * UFT_MIXED_H: as UFT_MIXED with the smoother's fine operator stored low:
- * with fine_assembled = 1 (default) as assembled 3x3 node blocks of D^-1 A
- * in int8 with an FP32 scale per block (one right-hand side) or in FP16
- * (several load cases), level 1's smoother blocks in FP16; with
- * fine_assembled = 0 as FP16 element blocks scaled per element.  FP32
+ * Krylov loop (FP64 residuals, inner products, iterate; the outer product
+ * K p with FP32 blocks), multigrid preconditioner in FP32 with FP32 fine
+ * and coarse operators; the coarsest level's dense inverse is computed in
+ * FP64 and applied as its FP32 copy.
+ * UFT_MIXED_H: as UFT_MIXED with the smoother's operators stored low: with
+ * fine_assembled = 1 (default) the fine level as assembled 3x3 node blocks
+ * of D^-1 A in int8 with an FP32 scale per block (one right-hand side) or
+ * in FP16 (several load cases), level 1's blocks in FP16 scaled to unit
+ * diagonal, the smoothed prolongator's cycle copies in FP16; with
+ * fine_assembled = 0 FP16 element blocks scaled per element.  FP32
  * arithmetic throughout the cycle. */
 enum uft_precision { UFT_FP64 = 0, UFT_MIXED = 1, UFT_MIXED_H = 2 };
 
