@@ -28,7 +28,8 @@ inline unsigned nblk(int64_t n) { return (unsigned)((n + TPB - 1) / TPB); }
 // lanes with the same r (fixed shuffle tree, then the four waves in order)
 // into part[block][k * NR + r].
 template <int K, int NR>
-__device__ __forceinline__ void mr_block_partials(double (&v)[K], double *part) {
+__device__ __forceinline__ void mr_block_partials(double (&v)[K], double *part, int64_t blk = -1) {
+  if (blk < 0) blk = blockIdx.x;
 #pragma unroll
   for (int k = 0; k < K; ++k) {
 #pragma unroll
@@ -43,7 +44,7 @@ __device__ __forceinline__ void mr_block_partials(double (&v)[K], double *part) 
   __syncthreads();
   if (threadIdx.x < K * NR) {
     const int k = threadIdx.x / NR, r = threadIdx.x % NR;
-    part[(size_t)blockIdx.x * K * NR + threadIdx.x] = (sm[k][r][0] + sm[k][r][1]) + (sm[k][r][2] + sm[k][r][3]);
+    part[(size_t)blk * K * NR + threadIdx.x] = (sm[k][r][0] + sm[k][r][1]) + (sm[k][r][2] + sm[k][r][3]);
   }
 }
 
@@ -112,15 +113,15 @@ __device__ __forceinline__ void mr_epi(const MrOp<T> &op, int64_t i, int64_t ii,
   }
 }
 template <typename T, int NR>
-__device__ __forceinline__ void mr_epi_partials(const MrOp<T> &op, double (&dv)[3]) {
+__device__ __forceinline__ void mr_epi_partials(const MrOp<T> &op, double (&dv)[3], int64_t blk = -1) {
   if (op.mode == MR_DOTS2) {
     double v[2] = {dv[0], dv[1]};
-    mr_block_partials<2, NR>(v, op.part);
+    mr_block_partials<2, NR>(v, op.part, blk);
   } else if (op.mode == MR_DOTS3) {
-    mr_block_partials<3, NR>(dv, op.part);
+    mr_block_partials<3, NR>(dv, op.part, blk);
   } else if (op.mode == MR_STOREDOT) {
     double v[1] = {dv[0]};
-    mr_block_partials<1, NR>(v, op.part);
+    mr_block_partials<1, NR>(v, op.part, blk);
   }
 }
 
@@ -147,7 +148,7 @@ __global__ __launch_bounds__(TPB) void k_mr_elem(int ne, const int32_t *__restri
                                                  const uint8_t *__restrict__ fixm, const T *__restrict__ u,
                                                  T *__restrict__ Ye) {
   constexpr int ND = 3 * NPE, NP = ND * (ND + 1) / 2, NQ = NP / 2;
-  const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  const int64_t t = uftr::xcd_block(blockIdx.x, gridDim.x) * TPB + threadIdx.x;
   const int64_t e = t / NR;
   const int rr = (int)(t % NR);
   if (e >= ne) return;
@@ -347,7 +348,8 @@ __global__ __launch_bounds__(TPB) void k_mr_ell_op(int nn, const int64_t *__rest
                                                    const float *__restrict__ dg, const float4 *__restrict__ w8,
                                                    const float *__restrict__ w1, const uint8_t *__restrict__ fixm,
                                                    const float *__restrict__ u, MrOp<float> op) {
-  const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  const int64_t blk = uftr::xcd_block(blockIdx.x, gridDim.x);
+  const int64_t t = blk * TPB + threadIdx.x;
   const int64_t n = t / NR;
   const int rr = (int)(t % NR);
   double dv[3] = {0.0, 0.0, 0.0};
@@ -392,7 +394,180 @@ __global__ __launch_bounds__(TPB) void k_mr_ell_op(int nn, const int64_t *__rest
       mr_epi(op, i, ii, ((m >> c) & 1u) ? ui : tv, ui, dv);
     }
   }
-  mr_epi_partials<float, NR>(op, dv);
+  mr_epi_partials<float, NR>(op, dv, blk);
+}
+
+// The same product with one lane per node: the lane holds the node's NR
+// right-hand sides, so a slot's block is loaded once per node (not once per
+// right-hand side), the neighbour's 3 NR vector entries are one contiguous
+// run read with 16-byte (NR >= 4) or 8-byte loads, and the wavefront's 64
+// consecutive nodes read 64 consecutive slots: NR times fewer memory
+// instructions than a lane per (node, right-hand side).  MODE is the rows'
+// epilogue (MR_STORE ... MR_RESFIRSTX, MR_STOREDOT).  Measured on c1 with 8
+// load cases: the FP32 outer product (MR_STOREDOT) 167 against 207 us, the
+// FP16 residual 149 against 188 us, the Chebyshev step 204 against 190 us
+// (its epilogue touches three vectors with 32-byte runs per lane), so
+// mr_ell_op keeps the lane per (node, right-hand side) for that step.  Two
+// lanes per node (four right-hand sides each) ran slower in every mode.
+template <int NR>
+__device__ __forceinline__ void ld_nr(const float *__restrict__ p, float (&o)[NR]) {
+  if constexpr (NR % 4 == 0) {
+#pragma unroll
+    for (int q = 0; q < NR / 4; ++q) {
+      const float4 t = reinterpret_cast<const float4 *>(p)[q];
+      o[4 * q] = t.x; o[4 * q + 1] = t.y; o[4 * q + 2] = t.z; o[4 * q + 3] = t.w;
+    }
+  } else if constexpr (NR % 2 == 0) {
+#pragma unroll
+    for (int q = 0; q < NR / 2; ++q) {
+      const float2 t = reinterpret_cast<const float2 *>(p)[q];
+      o[2 * q] = t.x; o[2 * q + 1] = t.y;
+    }
+  } else {
+    o[0] = p[0];
+  }
+}
+template <int NR>
+__device__ __forceinline__ void st_nr(float *__restrict__ p, const float (&o)[NR]) {
+  if constexpr (NR % 4 == 0) {
+#pragma unroll
+    for (int q = 0; q < NR / 4; ++q)
+      reinterpret_cast<float4 *>(p)[q] = make_float4(o[4 * q], o[4 * q + 1], o[4 * q + 2], o[4 * q + 3]);
+  } else if constexpr (NR % 2 == 0) {
+#pragma unroll
+    for (int q = 0; q < NR / 2; ++q) reinterpret_cast<float2 *>(p)[q] = make_float2(o[2 * q], o[2 * q + 1]);
+  } else {
+    p[0] = o[0];
+  }
+}
+
+template <int NR, bool H, typename CI, int MODE>
+__global__ __launch_bounds__(TPB) void k_mr_ell_node(int nn, const int64_t *__restrict__ off,
+                                                     const int32_t *__restrict__ width, const CI *__restrict__ col,
+                                                     const uint4 *__restrict__ v8, const uint16_t *__restrict__ v1,
+                                                     const float *__restrict__ dg, const float4 *__restrict__ w8,
+                                                     const float *__restrict__ w1, const uint8_t *__restrict__ fixm,
+                                                     const float *__restrict__ u, MrOp<float> op) {
+  constexpr int LPN = 1;                // lanes per node
+  constexpr int NL = NR / LPN;          // right-hand sides per lane
+  const int64_t blk = uftr::xcd_block(blockIdx.x, gridDim.x);
+  const int64_t t = blk * TPB + threadIdx.x;
+  const int64_t n = t / LPN;
+  const int h = (int)(t % LPN);
+  double dv[NL];
+#pragma unroll
+  for (int r = 0; r < NL; ++r) dv[r] = 0.0;
+  if (n < nn) {
+    const int64_t w = n / UFT_ELL_SLICE;
+    const int64_t base = off[w] + n % UFT_ELL_SLICE;
+    const int W = width[w];
+    float a[3][NL];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int r = 0; r < NL; ++r) a[c][r] = 0.f;
+#pragma unroll 2
+    for (int s = 0; s < W; ++s) {
+      const int64_t k = base + (int64_t)s * UFT_ELL_SLICE;
+      const int64_t j = std::is_same<CI, int16_t>::value ? n + (int64_t)col[k] : (int64_t)col[k];
+      float e[9];
+      if constexpr (H) {
+        const uint4 hv = v8[k];
+        const unsigned hw[4] = {hv.x, hv.y, hv.z, hv.w};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const float2 f = __half22float2(*reinterpret_cast<const __half2 *>(&hw[q]));
+          e[2 * q] = f.x;
+          e[2 * q + 1] = f.y;
+        }
+        e[8] = __half2float(__ushort_as_half(v1[k]));
+      } else {
+        const float4 a4 = w8[2 * k], b4 = w8[2 * k + 1];
+        e[0] = a4.x; e[1] = a4.y; e[2] = a4.z; e[3] = a4.w;
+        e[4] = b4.x; e[5] = b4.y; e[6] = b4.z; e[7] = b4.w;
+        e[8] = w1[k];
+      }
+      float x[3][NL];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) ld_nr<NL>(u + (3 * j + c) * NR + h * NL, x[c]);
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int r = 0; r < NL; ++r) a[c][r] += e[3 * c] * x[0][r] + e[3 * c + 1] * x[1][r] + e[3 * c + 2] * x[2][r];
+    }
+    const unsigned m = fixm[n];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const int64_t i = 3 * n + c, ii = i * NR + h * NL;
+      float ui[NL], tv[NL];
+      ld_nr<NL>(u + ii, ui);
+      const float sc = H ? dg[i] : 1.f;
+      const bool fx = (m >> c) & 1u;
+#pragma unroll
+      for (int r = 0; r < NL; ++r) tv[r] = fx ? ui[r] : sc * a[c][r];
+      if constexpr (MODE == MR_STORE) {
+        st_nr<NL>(op.y + ii, tv);
+      } else if constexpr (MODE == MR_STEP) {
+        float rv[NL], xv[NL], dn[NL];
+        ld_nr<NL>(op.r + ii, rv);
+        ld_nr<NL>(op.x + ii, xv);
+        const float di = op.dinv[i];
+#pragma unroll
+        for (int r = 0; r < NL; ++r) {
+          rv[r] -= tv[r];
+          dn[r] = (float)op.c1 * ui[r] + (float)op.c2 * di * rv[r];
+          xv[r] += op.addprev ? dn[r] + ui[r] : dn[r];
+        }
+        st_nr<NL>(op.r + ii, rv);
+        st_nr<NL>(op.dout + ii, dn);
+        st_nr<NL>(op.x + ii, xv);
+      } else if constexpr (MODE == MR_RES) {
+        float bv[NL];
+        ld_nr<NL>(op.base + ii, bv);
+#pragma unroll
+        for (int r = 0; r < NL; ++r) bv[r] -= tv[r];
+        st_nr<NL>(op.r + ii, bv);
+      } else if constexpr (MODE == MR_RESFIRST || MODE == MR_RESFIRSTX) {
+        float bv[NL], dn[NL];
+        ld_nr<NL>(op.base + ii, bv);
+        const float di = op.dinv[i];
+#pragma unroll
+        for (int r = 0; r < NL; ++r) {
+          bv[r] -= tv[r];
+          dn[r] = (float)op.it * di * bv[r];
+        }
+        st_nr<NL>(op.r + ii, bv);
+        st_nr<NL>(op.dout + ii, dn);
+        if constexpr (MODE == MR_RESFIRSTX) {
+          st_nr<NL>(op.kr + ii, bv);
+          st_nr<NL>(op.kd + ii, dn);
+        }
+      } else {  // MR_STOREDOT
+        st_nr<NL>(op.y + ii, tv);
+#pragma unroll
+        for (int r = 0; r < NL; ++r) dv[r] += (double)ui[r] * (double)tv[r];
+      }
+    }
+  }
+  if constexpr (MODE == MR_STOREDOT) {
+    // per right-hand side h NL + r: fixed shuffle tree over the wavefront's
+    // lanes of the same h, then the four wavefronts in order, into
+    // part[blk][h NL + r]
+    __shared__ double sm[NR][TPB / 64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int r = 0; r < NL; ++r) {
+      double v = dv[r];
+#pragma unroll
+      for (int o = 32; o >= LPN; o >>= 1) v += __shfl_down(v, o, 64);
+      if (lane < LPN) sm[lane * NL + r][wv] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < NR) {
+      const int r = threadIdx.x;
+      op.part[(size_t)blk * NR + r] = (sm[r][0] + sm[r][1]) + (sm[r][2] + sm[r][3]);
+    }
+  }
 }
 
 // ---------------------------------------------------------------- 6x6 BSR
@@ -723,7 +898,7 @@ __global__ __launch_bounds__(TPB) void k_mr_sens(int ne, const int32_t *__restri
                                                  SimpCoef sc, const double *__restrict__ u, double *__restrict__ dfdx,
                                                  double *__restrict__ part) {
   constexpr int ND = 3 * NPE, NP = ND * (ND + 1) / 2, NQ = NP / 2;
-  const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  const int64_t t = uftr::xcd_block(blockIdx.x, gridDim.x) * TPB + threadIdx.x;
   const int64_t e = t / NR;
   const int rr = (int)(t % NR);
   double uku = 0.0;
@@ -876,11 +1051,41 @@ void mr_gather_op(hipStream_t s, const uft_ctx *C, int nr, const float *u, const
   mr_gather_op_t<float>(s, C, nr, u, Ye, o);
 }
 int mr_gather_blocks(int64_t nn, int nr) { return (int)nblk(nn * nr); }
+static bool mr_ell_node_mode(int mode, bool h) {
+  return mode != MR_DOTS2 && mode != MR_DOTS3 && !(h && mode == MR_STEP);
+}
+template <bool H, typename CI>
+static void mr_ell_node_t(hipStream_t s, const uft_ctx *C, int nr, const CI *col, const float *u, const MrOp<float> &op) {
+  const auto &F = C->fell;
+  const int nn = (int)C->nn;
+#define UFT_MR_NODE(M)                                                                                              \
+  UFT_NR_SWITCH(nr, (k_mr_ell_node<NR, H, CI, M><<<nblk(nn), TPB, 0, s>>>(nn, F.d_off, F.d_width, col, F.d_v8, \
+                                                                        F.d_v1, F.d_dg, F.d_w8, F.d_w1, C->d_fixm, u, op)))
+  switch (op.mode) {
+    case MR_STORE: UFT_MR_NODE(MR_STORE); break;
+    case MR_STEP: UFT_MR_NODE(MR_STEP); break;
+    case MR_RES: UFT_MR_NODE(MR_RES); break;
+    case MR_RESFIRST: UFT_MR_NODE(MR_RESFIRST); break;
+    case MR_RESFIRSTX: UFT_MR_NODE(MR_RESFIRSTX); break;
+    default: UFT_MR_NODE(MR_STOREDOT); break;
+  }
+#undef UFT_MR_NODE
+}
 template <bool H>
 static void mr_ell_op_t(hipStream_t s, const uft_ctx *C, int nr, const float *u, const RowOp<float> &o) {
   const auto &F = C->fell;
   const int nn = (int)C->nn;
   const MrOp<float> op = to_mr(o);
+  // the node-lane kernel: every epilogue but the K-cycle's dot modes (not used
+  // on the fine level) and the FP16 blocks' Chebyshev step, vector entries
+  // 8- or 16-byte aligned
+  if (mr_ell_node_mode(op.mode, H) && ((uintptr_t)u % 16) == 0) {
+    if (F.d_cd)
+      mr_ell_node_t<H, int16_t>(s, C, nr, F.d_cd, u, op);
+    else
+      mr_ell_node_t<H, int32_t>(s, C, nr, F.d_col, u, op);
+    return;
+  }
   if (F.d_cd) {
     UFT_NR_SWITCH(nr, (k_mr_ell_op<NR, H, int16_t><<<nblk((int64_t)nn * NR), TPB, 0, s>>>(
                           nn, F.d_off, F.d_width, F.d_cd, F.d_v8, F.d_v1, F.d_dg, F.d_w8, F.d_w1, C->d_fixm, u, op)));
@@ -888,6 +1093,13 @@ static void mr_ell_op_t(hipStream_t s, const uft_ctx *C, int nr, const float *u,
     UFT_NR_SWITCH(nr, (k_mr_ell_op<NR, H, int32_t><<<nblk((int64_t)nn * NR), TPB, 0, s>>>(
                           nn, F.d_off, F.d_width, F.d_col, F.d_v8, F.d_v1, F.d_dg, F.d_w8, F.d_w1, C->d_fixm, u, op)));
   }
+}
+// blocks (and STOREDOT partials per right-hand side) of mr_ell_op's launch
+int mr_ell_blocks(const uft_ctx *C, const float *u, int mode, bool fp32) {
+  RowOp<float> o;
+  o.mode = mode;
+  const bool node = mr_ell_node_mode(to_mr(o).mode, !fp32) && ((uintptr_t)u % 16) == 0;
+  return node ? (int)nblk(C->nn) : (int)nblk((int64_t)C->nn * C->mr.nr);
 }
 void mr_ell_op(hipStream_t s, const uft_ctx *C, int nr, const float *u, const RowOp<float> &o, bool fp32) {
   if (fp32)

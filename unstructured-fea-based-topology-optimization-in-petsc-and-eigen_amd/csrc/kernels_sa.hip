@@ -29,6 +29,7 @@
 #include <hip/hip_runtime.h>
 
 #include "uft_internal.h"
+#include "uft_reduce.h"
 
 namespace {
 
@@ -146,10 +147,7 @@ __global__ __launch_bounds__(TPB) void k_sa_scan_g(int64_t nub, const int32_t *_
 // lower) and accumulate in FP64.  Their blocks are handed out XCD by XCD in
 // contiguous ranges (block b runs on XCD b mod 8), so neighbouring rows,
 // which share their operands, share one L2.
-__device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t nb) {
-  const int64_t x = b % 8, q = nb / 8, r = nb % 8;
-  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + b / 8;
-}
+using uftr::xcd_block;
 
 // The FP16 node-sliced copy (UFT_MIXED_H) is stored in three planes over
 // the entry slots s (ne of them): values 0-7 at halves 8 s, values 8-15 at

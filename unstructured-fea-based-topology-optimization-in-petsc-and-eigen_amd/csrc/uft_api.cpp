@@ -155,6 +155,21 @@ double coarse_bytes(const uft_ctx *C, int l, int w) {
   const AmgLevel &L = C->amg.L[l];
   return (double)L.nblk * (36.0 * w + 4) + 2.0 * (double)L.n * (w == 8 ? 8 : 4) + (w == 2 ? 4.0 * L.n : 0.0);
 }
+// the same for nr interleaved right-hand sides: the stored operator once,
+// each vector nr times (w bytes per vector entry)
+double mr_fine_bytes(const uft_ctx *C, int kw, int w, int nr) {
+  const int S = C->npe == 8 ? 300 : 78;
+  return (double)C->ne * (S * kw + 4 * C->npe + 8) + 2.0 * nr * (double)C->ndof * w;
+}
+double mr_ell_bytes(const uft_ctx *C, int value_bytes, bool scaled, int nr) {
+  const int cb = C->fell.d_cd ? 2 : 4;
+  return (double)C->fell.nnzb * (value_bytes + cb) + (double)C->ndof * 4 * (2 * nr + (scaled ? 1 : 0)) +
+         (double)C->nn;
+}
+double mr_coarse_bytes(const uft_ctx *C, int l, int w, int nr) {
+  const AmgLevel &L = C->amg.L[l];
+  return (double)L.nblk * (36.0 * w + 4) + 2.0 * nr * (double)L.n * w;
+}
 inline void count_apply(uft_ctx *C, double bytes) {
   if (!C->count_on) return;
   C->solve_bytes += bytes;
@@ -638,6 +653,11 @@ double *mr_ksc(uft_ctx *C, int l) { return C->mr.sc + 128 + 64 * l; }
 template <typename T>
 void mr_lev_op(uft_ctx *C, int l, const T *u, const RowOp<T> &op) {
   const int nr = C->mr.nr;
+  constexpr bool f64 = std::is_same<T, double>::value;
+  if (l == 0)
+    count_apply(C, !f64 && C->fell.d_v8 ? mr_ell_bytes(C, 18, true, nr) : mr_fine_bytes(C, f64 ? 8 : 2, f64 ? 8 : 4, nr));
+  else
+    count_apply(C, mr_coarse_bytes(C, l, f64 ? 8 : 4, nr));
   if (l == 0) {
     hipEvent_t *ev = prof_triple(C, std::is_same<T, double>::value ? 0 : 1);
     if (ev) (void)hipEventRecord(ev[0], C->s);
@@ -668,6 +688,7 @@ int mr_lev_blocks(const uft_ctx *C, int l) {
 // side into the block partials
 void mr_outer_apply_lo(uft_ctx *C, const float *pf, float *qf) {
   const int nr = C->mr.nr;
+  count_apply(C, C->fell.d_w8 ? mr_ell_bytes(C, 36, false, nr) : mr_fine_bytes(C, 4, 4, nr));
   RowOp<float> op;
   op.mode = ROW_STOREDOT;
   op.y = qf;
@@ -783,6 +804,7 @@ void mr_cycle_body(uft_ctx *C, int l, const T *b, T *x) {
     AmgLevel &Ln = H.L[l + 1];
     MLV<T> vn = mlv<T>(C, l + 1);
     const T *P = tr_P(Lv, (T *)nullptr);
+    count_apply(C, (double)H.ncoarse * H.ncoarse * 8 + 2.0 * nr * (double)H.ncoarse * sizeof(T));
     uftk::mr_tail_solve(C->s, nr, Lv.bs, (int)Ln.nunit, Lv.d_cptr, Lv.d_child, P, H.d_cinv, vn.b, v.r, vn.x);
     uftk::mr_prolong_comb(C->s, nr, Lv.bs, Lv.nunit, Lv.d_parent, P, nullptr, -1, -1, vn.x, (const T *)nullptr, x);
   } else {
@@ -1492,7 +1514,8 @@ int solve_el_block(uft_ctx *C, const double *xphys) {
     } else {    // q = K p and p.q per right-hand side
       mr_lev_op<double>(C, 0, M.p, pq);
     }
-    uftk::final_sum(s, C->d_part, mr_lev_blocks(C, 0), nr, sc + MS_PQ);
+    uftk::final_sum(s, C->d_part, f32 && C->fell.d_w8 ? uftk::mr_ell_blocks(C, M.pf, ROW_STOREDOT, true) : mr_lev_blocks(C, 0),
+                    nr, sc + MS_PQ);
     uftk::mr_alpha(s, nr, mask, sc + MS_RZ, sc + MS_PQ, sc + MS_ALPHA);
     if (f32)
       uftk::mr_cg_update_lo(s, nr, n, sc + MS_ALPHA, M.U, M.p, M.r, M.qf, M.rf, C->d_part, sc + MS_RR);

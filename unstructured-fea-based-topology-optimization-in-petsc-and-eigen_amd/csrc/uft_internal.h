@@ -542,6 +542,7 @@ void mr_gather_op(hipStream_t s, const uft_ctx *C, int nr, const float *u, const
 // the assembled fine operator (FP16 smoother blocks, or FP32 with fp32) on nr
 // interleaved right-hand sides, with the row epilogue (mr_gather_blocks blocks)
 void mr_ell_op(hipStream_t s, const uft_ctx *C, int nr, const float *u, const RowOp<float> &o, bool fp32);
+int mr_ell_blocks(const uft_ctx *C, const float *u, int mode, bool fp32);
 int mr_gather_blocks(int64_t nn, int nr);
 void mr_bsr_op(hipStream_t s, int nr, int64_t nb_rows, const int32_t *rp, const int32_t *ci, const double *val,
                const double *u, const RowOp<double> &o);

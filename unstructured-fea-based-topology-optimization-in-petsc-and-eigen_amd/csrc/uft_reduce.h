@@ -18,6 +18,15 @@ namespace uftr {
 
 constexpr int RTPB = 256;
 
+// Workgroup b of a launch runs on XCD b mod 8.  xcd_block maps it to a
+// logical block so that each XCD takes one contiguous range of the nb
+// logical blocks (a bijection): kernels whose neighbouring rows share
+// gathered operands then keep those operands in one XCD's L2.
+__device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t nb) {
+  const int64_t x = b % 8, q = nb / 8, r = nb % 8;
+  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + b / 8;
+}
+
 // Row of this thread under a segment grid (blocks of RTPB threads): the
 // plain grid (row = block * RTPB + thread, design 0, valid below n) or the
 // block's own row range and design.
