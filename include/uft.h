@@ -73,9 +73,9 @@ typedef struct uft_params {
   int amg_levels;      /* most levels incl. the fine one (reference: PCGAMGSetNlevels 4); coarsening also stops
                           at coarse_max */
   int cheb_degree;     /* Chebyshev-Jacobi smoother degree per level */
-  int coarse_max;      /* stop coarsening at or below this many coarse DOFs (default 4096: one design with one
-                          right-hand side keeps three levels and solves the coarsest exactly with its dense
-                          inverse; design batches and the multi-load-case form stop at 600 at most) */
+  int coarse_max;      /* stop coarsening at or below this many coarse DOFs (default 4096: one design, with one
+                          right-hand side or several load cases, keeps three levels and solves the coarsest
+                          exactly with its dense inverse; design batches stop at 600 per design at most) */
   int cycle_type;      /* coarse-grid cycle: 0 V, 1 W, 2 K-cycle on every coarse level, 3 K-cycle on level 1 only */
   double cheb_lo, cheb_hi; /* Chebyshev interval [cheb_lo, cheb_hi] * lambda_max estimate
                               (-mg_levels_ksp_chebyshev_esteig 0,lo,0,hi; PCGAMG: 0.1, 1.1) */
@@ -98,8 +98,8 @@ typedef struct uft_params {
                           setup. */
   int coarse_dense_max; /* the coarsest multigrid level is solved with its dense inverse (recomputed at every
                            hierarchy refresh), so coarsening goes on until a level has at most this many
-                           unknowns (default 4096, at most 6144; design batches and the multi-load-case form
-                           3072 at most).  Fixed at setup. */
+                           unknowns (default 4096, at most 6144; design batches 3072 per design at most).
+                           Fixed at setup. */
   int prolong_smooth;  /* 1 (default): smoothed aggregation on the fine level as PCGAMG does (SolveFem.cpp:129-136):
                           at every hierarchy refresh the tentative prolongator is smoothed once with the current
                           operator, P = (I - omega D^-1 A) P_tent with omega = 4 / (3 lambda_max(D^-1 A)), and the

@@ -746,32 +746,66 @@ __global__ __launch_bounds__(TPB) void k_part_stage(const double *__restrict__ p
 }
 constexpr int MR_STAGE_BLOCKS = 128;
 
-// xc[row][r] = sum_c Ainv[c][row] bc[c][r] (FP64, the symmetric inverse read
-// by columns), one thread per (row, r)
-// A workgroup takes 32 outputs (row, r); eight groups of 32 threads each sum
-// one eighth of the columns, and the eight partial sums are added in order.
-constexpr int MR_DS_PAIRS = 32, MR_DS_CHUNKS = TPB / MR_DS_PAIRS;
+// xc[row][r] = sum_c Ainv[row][c] bc[c][r] with the coarsest level's dense
+// inverse in the cycle's precision T (row-major, n even): one wavefront per
+// MR_DM_ROWS rows, lane l takes the column pairs (FP32) or columns (FP64)
+// l, l + 64, ...; the interleaved right-hand-side entries of those columns
+// are one contiguous run (16-byte loads) shared by the wavefront's rows, so
+// the inverse streams once and bc is read once per MR_DM_ROWS rows; the
+// lanes' partial sums are combined by a fixed butterfly.
+constexpr int MR_DM_ROWS = 4;
 template <typename T, int NR>
-__global__ __launch_bounds__(TPB) void k_mr_dense_solve(int n, const double *__restrict__ Ainv, const T *__restrict__ bc,
-                                                        T *__restrict__ xc) {
-  __shared__ double ps[MR_DS_CHUNKS][MR_DS_PAIRS];
-  const int p = threadIdx.x % MR_DS_PAIRS, k = threadIdx.x / MR_DS_PAIRS;
-  const int g = blockIdx.x * MR_DS_PAIRS + p;
-  const int chunk = (n + MR_DS_CHUNKS - 1) / MR_DS_CHUNKS;
-  double acc = 0.0;
-  if (g < n * NR) {
-    const int row = g / NR, q = g % NR;
-    const int c1 = min(n, (k + 1) * chunk);
-    for (int c = k * chunk; c < c1; ++c) acc += Ainv[(size_t)c * n + row] * (double)bc[c * NR + q];
-  }
-  ps[k][p] = acc;
-  __syncthreads();
-  if (k == 0 && g < n * NR) {
-    double sum = ps[0][p];
+__global__ __launch_bounds__(TPB) void k_mr_dense_mv(int n, const T *__restrict__ A, const T *__restrict__ bc,
+                                                     T *__restrict__ xc) {
+  constexpr int VW = sizeof(T) == 4 ? 2 : 1;           // columns per lane and step
+  constexpr int NV = VW * NR * (int)sizeof(T) / 16;    // 16-byte loads of bc per step
+  typedef typename std::conditional<sizeof(T) == 4, float4, double2>::type V16;
+  constexpr int PER = 16 / (int)sizeof(T);
+  const int64_t wave = ((int64_t)blockIdx.x * TPB + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  const int64_t r0 = wave * MR_DM_ROWS;
+  if (r0 >= n) return;  // whole wavefront
+  T acc[MR_DM_ROWS][NR];
 #pragma unroll
-    for (int j = 1; j < MR_DS_CHUNKS; ++j) sum += ps[j][p];
-    xc[g] = (T)sum;
+  for (int r = 0; r < MR_DM_ROWS; ++r)
+#pragma unroll
+    for (int q = 0; q < NR; ++q) acc[r][q] = T(0);
+  for (int64_t c = (int64_t)lane * VW; c < n; c += 64 * VW) {
+    T xv[VW * NR];
+    const V16 *xp = reinterpret_cast<const V16 *>(bc + c * NR);
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      const V16 t = xp[v];
+      const T *tt = reinterpret_cast<const T *>(&t);
+#pragma unroll
+      for (int e = 0; e < PER; ++e) xv[v * PER + e] = tt[e];
+    }
+#pragma unroll
+    for (int r = 0; r < MR_DM_ROWS; ++r) {
+      if (r0 + r >= n) break;
+      T av[VW];
+      if constexpr (VW == 2) {
+        const float2 a2 = *reinterpret_cast<const float2 *>(A + (r0 + r) * (int64_t)n + c);
+        av[0] = a2.x;
+        av[1] = a2.y;
+      } else {
+        av[0] = A[(r0 + r) * (int64_t)n + c];
+      }
+#pragma unroll
+      for (int v = 0; v < VW; ++v)
+#pragma unroll
+        for (int q = 0; q < NR; ++q) acc[r][q] += av[v] * xv[v * NR + q];
+    }
   }
+#pragma unroll
+  for (int r = 0; r < MR_DM_ROWS; ++r)
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+      T v = acc[r][q];
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+      if (lane == r * NR + q && r0 + r < n) xc[(r0 + r) * NR + q] = v;
+    }
 }
 
 // ---------------------------------------------------------------- outer Krylov (FP64)
@@ -1201,19 +1235,19 @@ void mr_final_kc2(hipStream_t s, int nr, double *part, int nb, double *sc) {
 
 template <typename T>
 static void mr_tail_solve_t(hipStream_t s, int nr, int bs, int nc_units, const int32_t *cptr, const int32_t *child,
-                            const T *P, const double *Ainv, T *bc, const T *r, T *xc) {
-  // restriction over the whole chip, then the dense solve (FP64 accumulation)
+                            const T *P, const T *Ainv, T *bc, const T *r, T *xc) {
+  // restriction over the whole chip, then the dense product with the inverse
   mr_restrict_first_t<T>(s, nr, bs, nc_units, cptr, child, P, r, bc, nullptr, 0.0, nullptr, nullptr, nullptr);
   const int n = 6 * nc_units;
-  UFT_NR_SWITCH(nr, (k_mr_dense_solve<T, NR><<<(unsigned)((n * NR + MR_DS_PAIRS - 1) / MR_DS_PAIRS), TPB, 0, s>>>(
-                        n, Ainv, bc, xc)));
+  const int64_t waves = (n + MR_DM_ROWS - 1) / MR_DM_ROWS;
+  UFT_NR_SWITCH(nr, (k_mr_dense_mv<T, NR><<<nblk(waves * 64), TPB, 0, s>>>(n, Ainv, bc, xc)));
 }
 void mr_tail_solve(hipStream_t s, int nr, int bs, int nc_units, const int32_t *cptr, const int32_t *child,
                    const double *P, const double *Ainv, double *bc, const double *r, double *xc) {
   mr_tail_solve_t<double>(s, nr, bs, nc_units, cptr, child, P, Ainv, bc, r, xc);
 }
 void mr_tail_solve(hipStream_t s, int nr, int bs, int nc_units, const int32_t *cptr, const int32_t *child,
-                   const float *P, const double *Ainv, float *bc, const float *r, float *xc) {
+                   const float *P, const float *Ainv, float *bc, const float *r, float *xc) {
   mr_tail_solve_t<float>(s, nr, bs, nc_units, cptr, child, P, Ainv, bc, r, xc);
 }
 

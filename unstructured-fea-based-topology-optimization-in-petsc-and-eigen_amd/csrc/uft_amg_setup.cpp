@@ -209,12 +209,12 @@ int uft_amg_build_host(uft_ctx *C, std::vector<AmgHostLevel> &H, std::vector<int
   }
 
   const int maxlev = std::max(2, std::min((int)UFT_MAXLEV, C->P.amg_levels));
-  // one design with one right-hand side: its coarsest level is inverted by
-  // the tiled sweep (kernels_dense.hip) and solved exactly, so coarsening
-  // stops at coarse_max; design batches and the multi-load-case form keep
-  // coarsest levels of at most 600 unknowns for their per-design and
-  // multi-right-hand-side dense solves
-  const bool single = C->bt.nseg == 0 && !(C->P.load_cases == 1 && C->nlc > 1);
+  // one design (one right-hand side or several load cases): its coarsest
+  // level is inverted by the tiled sweep (kernels_dense.hip) and solved
+  // exactly, so coarsening stops at coarse_max; design batches keep coarsest
+  // levels of at most 600 unknowns per design for their per-design dense
+  // solves in one workgroup each
+  const bool single = C->bt.nseg == 0;
   const int64_t cmax = single ? std::max(6, C->P.coarse_max) : std::min(std::max(6, C->P.coarse_max), 600);
   const int64_t dense_limit = single ? C->P.coarse_dense_max : std::min<int64_t>(C->P.coarse_dense_max, UFT_TAIL_MAX);
 

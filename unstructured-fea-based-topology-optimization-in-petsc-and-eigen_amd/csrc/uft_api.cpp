@@ -804,8 +804,10 @@ void mr_cycle_body(uft_ctx *C, int l, const T *b, T *x) {
     AmgLevel &Ln = H.L[l + 1];
     MLV<T> vn = mlv<T>(C, l + 1);
     const T *P = tr_P(Lv, (T *)nullptr);
-    count_apply(C, (double)H.ncoarse * H.ncoarse * 8 + 2.0 * nr * (double)H.ncoarse * sizeof(T));
-    uftk::mr_tail_solve(C->s, nr, Lv.bs, (int)Ln.nunit, Lv.d_cptr, Lv.d_child, P, H.d_cinv, vn.b, v.r, vn.x);
+    count_apply(C, (double)H.ncoarse * H.ncoarse * sizeof(T) + 2.0 * nr * (double)H.ncoarse * sizeof(T));
+    const T *cinv;
+    if constexpr (std::is_same<T, double>::value) cinv = H.d_cinv; else cinv = H.d_cinvf;
+    uftk::mr_tail_solve(C->s, nr, Lv.bs, (int)Ln.nunit, Lv.d_cptr, Lv.d_child, P, cinv, vn.b, v.r, vn.x);
     uftk::mr_prolong_comb(C->s, nr, Lv.bs, Lv.nunit, Lv.d_parent, P, nullptr, -1, -1, vn.x, (const T *)nullptr, x);
   } else {
     AmgLevel &Ln = H.L[l + 1];
@@ -2526,7 +2528,7 @@ int uft_setup(uft_ctx *C) {
       if ((rc = dalloc_n(C, &A.d_cinv, A.ncoarse * A.ncoarse))) return rc;
       if (is_mixed(C))
         if ((rc = dalloc_n(C, &A.d_cinvf, A.ncoarse * A.ncoarse))) return rc;
-      if (C->P.coarse_lag && nc >= UFT_SWEEP_MIN && !(C->P.load_cases == 1 && C->nlc > 1)) {
+      if (C->P.coarse_lag && nc >= UFT_SWEEP_MIN) {
         if ((rc = dalloc_n(C, &A.d_cinv2, nc * nc))) return rc;
         if (is_mixed(C))
           if ((rc = dalloc_n(C, &A.d_cinvf2, nc * nc))) return rc;
@@ -2573,8 +2575,6 @@ int uft_setup(uft_ctx *C) {
         need = std::max<int64_t>(need, (int64_t)uftk::mr_bsr_blocks(Lv.nunit) * 3 * nr);
       }
     }
-    if (6 * C->amg.L[C->amg.nlev - 1].nunit > UFT_MR_TAIL_MAX)
-      return uft_set_error(C, UFT_ERR_ARG, "multi-load-case form: coarsest level above %d unknowns", UFT_MR_TAIL_MAX);
     need = std::max<int64_t>(need, (int64_t)uftk::mr_gather_blocks(nn, nr) * nr);
     need += 3 * 8 * 128;  // the two-stage final sums of the K-cycle's partial rows
     need = std::max<int64_t>(need, (int64_t)((ndof * nr + 255) / 256) * 2 * nr);

@@ -13,7 +13,6 @@
 #define UFT_MAXLEV 8
 #define UFT_RED_BLOCK 256
 #define UFT_TAIL_MAX 3072  // coarsest unknowns handled by the fused coarse tail (LDS)
-#define UFT_MR_TAIL_MAX 3072  // coarsest unknowns of the multi-load-case dense solve
 #define UFT_SWEEP_MIN 512  // coarsest levels from this size are inverted by the tiled sweep (kernels_dense.hip)
 #define UFT_ELL_SLICE 64      // nodes per slice of the assembled fine operator (one wavefront)
 
@@ -570,7 +569,7 @@ void mr_final_kc2(hipStream_t s, int nr, double *part, int nb, double *sc);
 void mr_tail_solve(hipStream_t s, int nr, int bs, int nc_units, const int32_t *cptr, const int32_t *child,
                    const double *P, const double *Ainv, double *bc, const double *r, double *xc);
 void mr_tail_solve(hipStream_t s, int nr, int bs, int nc_units, const int32_t *cptr, const int32_t *child,
-                   const float *P, const double *Ainv, float *bc, const float *r, float *xc);
+                   const float *P, const float *Ainv, float *bc, const float *r, float *xc);
 void mr_dot(hipStream_t s, int nr, int64_t n, const double *a, const double *b, double *part, double *out);
 void mr_dot_pr(hipStream_t s, int nr, int64_t n, const double *r, const double *z, const double *zo, double *part,
                double *out);
