@@ -581,7 +581,10 @@ __global__ __launch_bounds__(TPB) void k_mr_bsr_op(int64_t nbrows, const int32_t
                                                    const int32_t *__restrict__ ci, const T *__restrict__ val,
                                                    const T *__restrict__ u, MrOp<T> op) {
   constexpr int SLOTS = 64 / NR;
-  const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  typedef typename std::conditional<sizeof(T) == 4, float4, double2>::type V16;
+  constexpr int PER = 16 / (int)sizeof(T);
+  const int64_t blk = uftr::xcd_block(blockIdx.x, gridDim.x);
+  const int64_t t = blk * TPB + threadIdx.x;
   const int64_t A = t / 64;
   const int lane = (int)(t % 64), l = lane / NR, rr = lane % NR;
   T acc[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
@@ -592,7 +595,15 @@ __global__ __launch_bounds__(TPB) void k_mr_bsr_op(int64_t nbrows, const int32_t
       T x[6];
 #pragma unroll
       for (int j = 0; j < 6; ++j) x[j] = xb[j * NR];
-      const T *v = val + (size_t)k * 36;
+      T v[36];  // the block in 16-byte loads
+      const V16 *vp = reinterpret_cast<const V16 *>(val + (size_t)k * 36);
+#pragma unroll
+      for (int q = 0; q < 36 / PER; ++q) {
+        const V16 w = vp[q];
+        const T *ww = reinterpret_cast<const T *>(&w);
+#pragma unroll
+        for (int e = 0; e < PER; ++e) v[q * PER + e] = ww[e];
+      }
 #pragma unroll
       for (int i = 0; i < 6; ++i)
 #pragma unroll
@@ -612,7 +623,7 @@ __global__ __launch_bounds__(TPB) void k_mr_bsr_op(int64_t nbrows, const int32_t
       mr_epi(op, row, row * NR + rr, acc[i], u[row * NR + rr], dv);
     }
   }
-  mr_epi_partials<T, NR>(op, dv);
+  mr_epi_partials<T, NR>(op, dv, blk);
 }
 
 // ---------------------------------------------------------------- transfers
@@ -770,6 +781,7 @@ __global__ __launch_bounds__(TPB) void k_mr_dense_mv(int n, const T *__restrict_
   for (int r = 0; r < MR_DM_ROWS; ++r)
 #pragma unroll
     for (int q = 0; q < NR; ++q) acc[r][q] = T(0);
+#pragma unroll 4
   for (int64_t c = (int64_t)lane * VW; c < n; c += 64 * VW) {
     T xv[VW * NR];
     const V16 *xp = reinterpret_cast<const V16 *>(bc + c * NR);
