@@ -19,7 +19,8 @@ using uftr::seg_row;
 // ---------------------------------------------------------------- per-design final reductions
 template <int K>
 __global__ __launch_bounds__(TPB) void k_seg_final(const double *__restrict__ part, const int32_t *__restrict__ dfirst,
-                                                   double *__restrict__ sc, int base) {
+                                                   double *__restrict__ sc, int base, const double *__restrict__ gate) {
+  if (gate && *gate == 0.0) return;
   const int d = blockIdx.x;
   const int b0 = dfirst[d], nb = dfirst[d + 1] - b0;
   double res[K];
@@ -116,7 +117,9 @@ __global__ __launch_bounds__(TPB) void k_seg_cg_update_lo(SegG g, int64_t n, con
 // with q = K U in FP64; partials of r.r for every design
 __global__ __launch_bounds__(TPB) void k_seg_true_res(SegG g, int64_t n, const double *__restrict__ sc,
                                                       const double *__restrict__ F, const double *__restrict__ q,
-                                                      double *__restrict__ r, double *__restrict__ part) {
+                                                      double *__restrict__ r, double *__restrict__ part,
+                                                      const double *__restrict__ gate) {
+  if (gate && *gate == 0.0) return;
   int64_t i;
   int d;
   double v[1] = {0.0};
@@ -321,32 +324,68 @@ __global__ void k_bsc_el_alpha(int nseg, double *sc) {
 // ||r|| <= rtol ||F|| ask for the true residual F - K U (B_NEED), on which the
 // stopping test itself is taken (k_bsc_el_true; a non-finite residual also
 // stops the design, and the host reports it from the true residual)
-__global__ void k_bsc_el_check(int nseg, double *sc) {
-  FOR_DESIGN(nseg)
-  S[B_NEED] = 0.0;
-  if (S[B_ACT] == 0.0) return;
-  S[B_IT] += 1.0;
-  if (!(sqrt(S[B_RR]) > S[B_TOL])) S[B_NEED] = 1.0;
+// The per-design tests below also leave batch-wide flags (one workgroup of
+// 64 threads, a design per thread): after the check whether some design asks
+// for its FP64 residual (flags[BF_NEED], gating that pass, and the count of
+// such passes in flags[BF_NRES]); after that pass whether some design is
+// still active (flags[BF_ACT]), read by the host once per round of queued
+// iterations.
+__global__ void k_bsc_el_check(int nseg, double *sc, double *flags) {
+  const int d = threadIdx.x;
+  bool need = false;
+  if (d < nseg) {
+    double *S = sc + (size_t)d * UFT_BSC;
+    S[B_NEED] = 0.0;
+    if (S[B_ACT] != 0.0) {
+      S[B_IT] += 1.0;
+      if (!(sqrt(S[B_RR]) > S[B_TOL])) S[B_NEED] = 1.0;
+    }
+    need = S[B_NEED] != 0.0;
+  }
+  need = __syncthreads_or(need);
+  if (d == 0) {
+    flags[BF_NEED] = need ? 1.0 : 0.0;
+    if (need) flags[BF_NRES] += 1.0;
+  }
 }
 // lower-precision outer product: count the step; ask for the FP64 residual
 // (B_NEED) when the recursive one has met the tolerance or fallen 100-fold
 // below the last FP64 residual (B_RT); the stopping test itself is taken on
 // the FP64 residual (k_bsc_el_true)
-__global__ void k_bsc_el_check_lo(int nseg, double *sc) {
-  FOR_DESIGN(nseg)
-  S[B_NEED] = 0.0;
-  if (S[B_ACT] == 0.0) return;
-  S[B_IT] += 1.0;
-  const double rn = sqrt(S[B_RR]);
-  if (!(rn > S[B_TOL]) || rn < 1e-2 * S[B_RT]) S[B_NEED] = 1.0;
+__global__ void k_bsc_el_check_lo(int nseg, double *sc, double *flags) {
+  const int d = threadIdx.x;
+  bool need = false;
+  if (d < nseg) {
+    double *S = sc + (size_t)d * UFT_BSC;
+    S[B_NEED] = 0.0;
+    if (S[B_ACT] != 0.0) {
+      S[B_IT] += 1.0;
+      const double rn = sqrt(S[B_RR]);
+      if (!(rn > S[B_TOL]) || rn < 1e-2 * S[B_RT]) S[B_NEED] = 1.0;
+    }
+    need = S[B_NEED] != 0.0;
+  }
+  need = __syncthreads_or(need);
+  if (d == 0) {
+    flags[BF_NEED] = need ? 1.0 : 0.0;
+    if (need) flags[BF_NRES] += 1.0;
+  }
 }
 // after the FP64 residual of the designs that asked for it
-__global__ void k_bsc_el_true(int nseg, double *sc) {
-  FOR_DESIGN(nseg)
-  if (S[B_NEED] == 0.0) return;
-  const double rt = sqrt(S[B_RR]);
-  S[B_RT] = rt;
-  if (S[B_ACT] != 0.0 && !(rt > S[B_TOL])) S[B_ACT] = 0.0;
+__global__ void k_bsc_el_true(int nseg, double *sc, double *flags) {
+  const int d = threadIdx.x;
+  bool act = false;
+  if (d < nseg) {
+    double *S = sc + (size_t)d * UFT_BSC;
+    if (S[B_NEED] != 0.0) {
+      const double rt = sqrt(S[B_RR]);
+      S[B_RT] = rt;
+      if (S[B_ACT] != 0.0 && !(rt > S[B_TOL])) S[B_ACT] = 0.0;
+    }
+    act = S[B_ACT] != 0.0;
+  }
+  act = __syncthreads_or(act);
+  if (d == 0) flags[BF_ACT] = act ? 1.0 : 0.0;
 }
 
 // flexible CG (Polak-Ribiere): beta = (r.z - r.zo) / rz_prev
@@ -412,6 +451,110 @@ __global__ void k_bsc_l_beta(int nseg, double *sc, double *log, int step) {
   S[B_LBE] = b;
   S[B_LRZ] = S[B_LRZN];
   log[((size_t)d * 64 + step) * 2 + 1] = b;
+}
+
+// ---- MMA dual interior point of a design batch on the device (m = 1): the
+// single-design control flow of SolveDIP (MMA.cpp:639-676) per design, one
+// thread each: epsilon levels from 1 by factors of 10 while above the
+// tolerance, at most 100 Newton steps per level while err > 0.9 epsilon, err
+// carried across levels; DualGrad / DualHess (MMA.cpp:730-868), the line
+// search (MMA.cpp:870-888) and DualResidual (MMA.cpp:890-934) with m = 1.
+// The fused dual pass between the kernels gives every design's constraint
+// sum (B_RED) and Hessian sum (B_RED + 1) at its lambda (lamd).  No
+// contraction of products into fused multiply-adds: the arithmetic is the
+// host formulation's, operation by operation.
+__device__ __forceinline__ void mma_yz(double *S, double a, double c) {  // XYZofLAMBDA (MMA.cpp:693-701)
+#pragma clang fp contract(off)
+  if (S[B_MLAM] < 0.0) S[B_MLAM] = 0.0;
+  S[B_MY] = fmax(0.0, S[B_MLAM] - c);
+  S[B_MZ] = fmax(0.0, 10.0 * (S[B_MLAM] * a - 1.0));
+}
+// gt[2 d] = gx of design d, gt[2 d + 1] = its tolerance 1e-9 sqrt(1 + n_d)
+__global__ void k_bsc_mma_init(int nseg, double *sc, const double *gt, double a, double c, double *lamd) {
+#pragma clang fp contract(off)
+  FOR_DESIGN(nseg)
+  S[B_MB] = S[B_RED] - gt[2 * d];
+  S[B_MTOL] = gt[2 * d + 1];
+  S[B_MLAM] = c / 2.0;
+  S[B_MMU] = 1.0;
+  S[B_MEPS] = 1.0;
+  S[B_MERR] = 1.0;
+  S[B_MLOOP] = 0.0;
+  S[B_MSTEPS] = 0.0;
+  S[B_MDONE] = 0.0;
+  S[B_MSTEP] = 0.0;
+  mma_yz(S, a, c);
+  lamd[d] = S[B_MLAM];
+}
+// leave the epsilon levels whose inner loop has ended; decide who steps
+// (flags[MF_STEP]), note a negative lambda among them (flags[MF_NEG]: the
+// sums are then taken again at the clamped lambda before the step)
+__global__ void k_bsc_mma_decide(int nseg, double *sc, double a, double c, double *lamd, double *flags) {
+#pragma clang fp contract(off)
+  const int d = threadIdx.x;
+  bool step = false, neg = false;
+  if (d < nseg) {
+    double *S = sc + (size_t)d * UFT_BSC;
+    S[B_MSTEP] = 0.0;
+    if (S[B_MDONE] == 0.0) {
+      while (!(S[B_MERR] > 0.9 * S[B_MEPS] && S[B_MLOOP] < 100.0)) {
+        S[B_MEPS] *= 0.1;
+        S[B_MLOOP] = 0.0;
+        if (!(S[B_MEPS] > S[B_MTOL])) {
+          S[B_MDONE] = 1.0;
+          break;
+        }
+      }
+      if (S[B_MDONE] == 0.0) {
+        S[B_MSTEP] = 1.0;
+        step = true;
+        neg = S[B_MLAM] < 0.0;
+        mma_yz(S, a, c);
+        lamd[d] = S[B_MLAM];
+      }
+    }
+  }
+  step = __syncthreads_or(step);
+  neg = __syncthreads_or(neg);
+  if (d == 0) {
+    flags[MF_STEP] = step ? 1.0 : 0.0;
+    flags[MF_NEG] = neg ? 1.0 : 0.0;
+  }
+}
+__global__ void k_bsc_mma_step(int nseg, double *sc, double a, double c, double *lamd) {
+#pragma clang fp contract(off)
+  FOR_DESIGN(nseg)
+  if (S[B_MSTEP] == 0.0) return;
+  S[B_MLOOP] += 1.0;
+  S[B_MSTEPS] += 1.0;
+  const double lam = S[B_MLAM], mu = S[B_MMU], ep = S[B_MEPS];
+  double grad = S[B_RED] - S[B_MB] - a * S[B_MZ] - S[B_MY];
+  grad = -1.0 * grad - ep / lam;
+  double hs = S[B_RED + 1];
+  if (lam > c) hs += -1.0;
+  hs += -mu / lam;
+  if (lam * a > 0.0) hs += -10.0 * a * a;
+  double corr = 1e-4 * hs;
+  if (-1.0 * corr < 1.0e-7) corr = -1.0e-7;
+  hs += corr;
+  const double s0 = grad / hs;
+  const double s1 = -mu + ep / lam - s0 * mu / lam;
+  double theta = 1.005;
+  if (theta < -1.01 * s0 / lam) theta = -1.01 * s0 / lam;
+  if (theta < -1.01 * s1 / mu) theta = -1.01 * s1 / mu;
+  theta = 1.0 / theta;
+  S[B_MLAM] += theta * s0;
+  S[B_MMU] += theta * s1;
+  mma_yz(S, a, c);
+  lamd[d] = S[B_MLAM];
+}
+__global__ void k_bsc_mma_resid(int nseg, double *sc, double a) {
+#pragma clang fp contract(off)
+  FOR_DESIGN(nseg)
+  if (S[B_MSTEP] == 0.0) return;
+  const double r1 = S[B_RED] - S[B_MB] - a * S[B_MZ] - S[B_MY] + S[B_MMU];
+  const double r2 = S[B_MMU] * S[B_MLAM] - S[B_MEPS];
+  S[B_MERR] = fmax(fabs(r1), fabs(r2));
 }
 #undef FOR_DESIGN
 
@@ -482,11 +625,12 @@ inline unsigned gsz(SegG g) { return (unsigned)g.nb; }
 }  // namespace
 
 namespace uftk {
-void seg_final(hipStream_t s, const double *part, const int32_t *dfirst, int nseg, int K, double *sc, int base) {
+void seg_final(hipStream_t s, const double *part, const int32_t *dfirst, int nseg, int K, double *sc, int base,
+               const double *gate) {
   switch (K) {
-    case 1: k_seg_final<1><<<nseg, TPB, 0, s>>>(part, dfirst, sc, base); break;
-    case 2: k_seg_final<2><<<nseg, TPB, 0, s>>>(part, dfirst, sc, base); break;
-    default: k_seg_final<3><<<nseg, TPB, 0, s>>>(part, dfirst, sc, base); break;
+    case 1: k_seg_final<1><<<nseg, TPB, 0, s>>>(part, dfirst, sc, base, gate); break;
+    case 2: k_seg_final<2><<<nseg, TPB, 0, s>>>(part, dfirst, sc, base, gate); break;
+    default: k_seg_final<3><<<nseg, TPB, 0, s>>>(part, dfirst, sc, base, gate); break;
   }
 }
 void seg_final_max(hipStream_t s, const double *part, const int32_t *dfirst, int nseg, double *sc, int base) {
@@ -580,14 +724,30 @@ void seg_fcg_p(hipStream_t s, SegG g, int64_t n, const double *z, const double *
 }
 void bsc_el_init(hipStream_t s, int nseg, double *sc, double rtol) { k_bsc_el_init<<<1, 64, 0, s>>>(nseg, sc, rtol); }
 void bsc_el_alpha(hipStream_t s, int nseg, double *sc) { k_bsc_el_alpha<<<1, 64, 0, s>>>(nseg, sc); }
-void bsc_el_check(hipStream_t s, int nseg, double *sc) { k_bsc_el_check<<<1, 64, 0, s>>>(nseg, sc); }
+void bsc_el_check(hipStream_t s, int nseg, double *sc, double *flags) {
+  k_bsc_el_check<<<1, 64, 0, s>>>(nseg, sc, flags);
+}
 void bsc_el_beta(hipStream_t s, int nseg, double *sc) { k_bsc_el_beta<<<1, 64, 0, s>>>(nseg, sc); }
-void bsc_el_check_lo(hipStream_t s, int nseg, double *sc) { k_bsc_el_check_lo<<<1, 64, 0, s>>>(nseg, sc); }
-void bsc_el_true(hipStream_t s, int nseg, double *sc) { k_bsc_el_true<<<1, 64, 0, s>>>(nseg, sc); }
+void bsc_el_check_lo(hipStream_t s, int nseg, double *sc, double *flags) {
+  k_bsc_el_check_lo<<<1, 64, 0, s>>>(nseg, sc, flags);
+}
+void bsc_el_true(hipStream_t s, int nseg, double *sc, double *flags) {
+  k_bsc_el_true<<<1, 64, 0, s>>>(nseg, sc, flags);
+}
+void bsc_mma_init(hipStream_t s, int nseg, double *sc, const double *gt, double a, double c, double *lamd) {
+  k_bsc_mma_init<<<1, 64, 0, s>>>(nseg, sc, gt, a, c, lamd);
+}
+void bsc_mma_decide(hipStream_t s, int nseg, double *sc, double a, double c, double *lamd, double *flags) {
+  k_bsc_mma_decide<<<1, 64, 0, s>>>(nseg, sc, a, c, lamd, flags);
+}
+void bsc_mma_step(hipStream_t s, int nseg, double *sc, double a, double c, double *lamd) {
+  k_bsc_mma_step<<<1, 64, 0, s>>>(nseg, sc, a, c, lamd);
+}
+void bsc_mma_resid(hipStream_t s, int nseg, double *sc, double a) { k_bsc_mma_resid<<<1, 64, 0, s>>>(nseg, sc, a); }
 void seg_true_res(hipStream_t s, SegG g, const int32_t *dfirst, int nseg, int64_t n, double *sc, const double *F,
-                  const double *q, double *r, double *part, int base) {
-  k_seg_true_res<<<gsz(g), TPB, 0, s>>>(g, n, sc, F, q, r, part);
-  seg_final(s, part, dfirst, nseg, 1, sc, base);
+                  const double *q, double *r, double *part, int base, const double *gate) {
+  k_seg_true_res<<<gsz(g), TPB, 0, s>>>(g, n, sc, F, q, r, part, gate);
+  seg_final(s, part, dfirst, nseg, 1, sc, base, gate);
 }
 void seg_dot_lo(hipStream_t s, SegG g, const int32_t *dfirst, int nseg, int64_t n, const double *a, const float *b,
                 double *part, double *sc, int base) {

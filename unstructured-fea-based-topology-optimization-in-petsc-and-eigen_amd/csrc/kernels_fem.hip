@@ -748,6 +748,32 @@ __global__ __launch_bounds__(TPB) void k_reliable_node(int nn, int ne, const dou
   }
   uftr::block_partials<1>(ds, part);
 }
+// q = K U from the element results Ye in incidence order (fixed DOFs: U),
+// only when *gate != 0 (design batches: the FP64 residual pass that some
+// design asked for)
+template <int NPE>
+__global__ __launch_bounds__(TPB) void k_node_q_gated(int nn, int ne, const double *__restrict__ gate,
+                                                      const int32_t *__restrict__ iptr,
+                                                      const int32_t *__restrict__ inc, const double *__restrict__ Ye,
+                                                      const uint8_t *__restrict__ fixm, const double *__restrict__ U,
+                                                      double *__restrict__ q) {
+  if (*gate == 0.0) return;
+  const int n = blockIdx.x * TPB + threadIdx.x;
+  if (n >= nn) return;
+  double acc[3] = {0.0, 0.0, 0.0};
+  for (int k = iptr[n]; k < iptr[n + 1]; ++k) {
+    const int v = inc[k];
+    const int e = v / NPE, l = v - e * NPE;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) acc[c] += Ye[(size_t)(c * NPE + l) * ne + e];
+  }
+  const unsigned m = fixm[n];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const size_t i = 3 * (size_t)n + c;
+    q[i] = ((m >> c) & 1u) ? U[i] : acc[c];
+  }
+}
 // one block after k_reliable_node: r.r (into rr_out), the stopping test on
 // the FP64 residual norm
 __global__ __launch_bounds__(TPB) void k_ksp_ctl_reliable(const double *__restrict__ part, int nb,
@@ -1138,7 +1164,9 @@ __global__ __launch_bounds__(TPB) void k_dual(int64_t n, LamArr lam, const doubl
                                               const double *__restrict__ beta, const double *__restrict__ p0,
                                               const double *__restrict__ q0, const double *__restrict__ pij,
                                               const double *__restrict__ qij, double *__restrict__ x,
-                                              double *__restrict__ part, SegG sg, const double *__restrict__ lamd) {
+                                              double *__restrict__ part, SegG sg, const double *__restrict__ lamd,
+                                              const double *__restrict__ gate) {
+  if (gate && *gate == 0.0) return;
   constexpr int K = M + M * (M + 1) / 2;
   int64_t i;
   int dsg;
@@ -1372,6 +1400,21 @@ void reliable_update(hipStream_t s, const uft_ctx *C, const double *xphys, const
   k_ksp_ctl_reliable<<<1, TPB, 0, s>>>(C->d_part, (int)gn, rr_out, ctl);
 }
 
+void apply_K_gated(hipStream_t s, const uft_ctx *C, const double *xphys, const double *U, double *q,
+                   const double *gate) {
+  const SimpCoef sc{C->P.Emin, C->P.Emax - C->P.Emin, C->P.penal};
+  const int ne = (int)C->ne, nn = (int)C->nn;
+  if (C->npe == 8) {
+    elem_phase_t<8, double, double2>(s, C->elem_variant, ne, C->d_elem, (const double2 *)C->d_KE, xphys, sc, C->d_fixm,
+                                     U, C->d_Ye, gate);
+    k_node_q_gated<8><<<nblk(nn), TPB, 0, s>>>(nn, ne, gate, C->d_iptr, C->d_inc, C->d_Ye, C->d_fixm, U, q);
+  } else {
+    elem_phase_t<4, double, double2>(s, C->elem_variant, ne, C->d_elem, (const double2 *)C->d_KE, xphys, sc, C->d_fixm,
+                                     U, C->d_Ye, gate);
+    k_node_q_gated<4><<<nblk(nn), TPB, 0, s>>>(nn, ne, gate, C->d_iptr, C->d_inc, C->d_Ye, C->d_fixm, U, q);
+  }
+}
+
 __global__ void k_inv(int64_t n, const double *__restrict__ d, double *__restrict__ dinv, float *__restrict__ dinvf) {
   const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
   if (i >= n) return;
@@ -1586,24 +1629,24 @@ void mma_gensub(hipStream_t s, uft_ctx *C, int64_t n, int m, int k, const double
 
 template <int M>
 static void dual_m(hipStream_t s, uft_ctx *C, int64_t n, const double *lam, double *x, double *part, SegG sg,
-                   const double *lamd) {
+                   const double *lamd, const double *gate) {
   LamArr la{};
   if (lam)
     for (int j = 0; j < M; ++j) la.v[j] = lam[j];
   k_dual<M><<<uftr::seg_grid(sg, n), TPB, 0, s>>>(n, la, C->d_L, C->d_U, C->d_alpha, C->d_beta, C->d_p0, C->d_q0,
-                                                  C->d_pij, C->d_qij, x, part, sg, lamd);
+                                                  C->d_pij, C->d_qij, x, part, sg, lamd, gate);
 }
 void mma_dual_pass(hipStream_t s, uft_ctx *C, int64_t n, int m, const double *lam, double *x, double *part, SegG sg,
-                   const double *lamd) {
+                   const double *lamd, const double *gate) {
   switch (m) {
-    case 1: dual_m<1>(s, C, n, lam, x, part, sg, lamd); break;
-    case 2: dual_m<2>(s, C, n, lam, x, part, sg, lamd); break;
-    case 3: dual_m<3>(s, C, n, lam, x, part, sg, lamd); break;
-    case 4: dual_m<4>(s, C, n, lam, x, part, sg, lamd); break;
-    case 5: dual_m<5>(s, C, n, lam, x, part, sg, lamd); break;
-    case 6: dual_m<6>(s, C, n, lam, x, part, sg, lamd); break;
-    case 7: dual_m<7>(s, C, n, lam, x, part, sg, lamd); break;
-    default: dual_m<8>(s, C, n, lam, x, part, sg, lamd); break;
+    case 1: dual_m<1>(s, C, n, lam, x, part, sg, lamd, gate); break;
+    case 2: dual_m<2>(s, C, n, lam, x, part, sg, lamd, gate); break;
+    case 3: dual_m<3>(s, C, n, lam, x, part, sg, lamd, gate); break;
+    case 4: dual_m<4>(s, C, n, lam, x, part, sg, lamd, gate); break;
+    case 5: dual_m<5>(s, C, n, lam, x, part, sg, lamd, gate); break;
+    case 6: dual_m<6>(s, C, n, lam, x, part, sg, lamd, gate); break;
+    case 7: dual_m<7>(s, C, n, lam, x, part, sg, lamd, gate); break;
+    default: dual_m<8>(s, C, n, lam, x, part, sg, lamd, gate); break;
   }
 }
 

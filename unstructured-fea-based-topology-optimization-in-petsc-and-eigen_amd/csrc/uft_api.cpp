@@ -36,7 +36,9 @@ enum {
   S_RED = 32,  // general reduction outputs (up to 48 values)
   S_K = 96,    // K-cycle scalars, 8 per level
   S_FD = 200,  // filter CG control: state, tolerance, divergence bound, iterations, ||z||
-  S_CTL = 208  // elasticity CG control of the mixed forms (uftk::ksp_ctl_init): 7 values
+  S_CTL = 208, // elasticity CG control of the mixed forms (uftk::ksp_ctl_init): 7 values
+  S_BFL = 216, // design batches: elasticity CG flags (BF_*), 3 values
+  S_MFL = 220  // design batches: MMA dual flags (MF_*), 2 values
 };
 
 int dalloc(uft_ctx *C, void **p, size_t bytes) {
@@ -1631,6 +1633,8 @@ int solve_el_batch(uft_ctx *C, const double *xphys, double *U) {
   bool any = false;
   if ((rc = fetch_batch(C, B_ACT, &any))) return rc;
   int it = 0;
+  double *flags = C->d_scal + S_BFL;
+  uftk::fill(s, 3, 0.0, flags);
   if (any) {
     precond();
     uftk::seg_dot(s, g, df, ns, n, r, z, C->d_part, B.sc, B_RZ);
@@ -1645,13 +1649,13 @@ int solve_el_batch(uft_ctx *C, const double *xphys, double *U) {
       uftk::seg_dot_lo(s, g, df, ns, n, p, C->d_qf, C->d_part, B.sc, B_PQ);
       uftk::bsc_el_alpha(s, ns, B.sc);
       uftk::seg_cg_update_lo(s, g, df, ns, n, B.sc, B_ALPHA, U, p, r, C->d_qf, C->d_part, B_RR);
-      uftk::bsc_el_check_lo(s, ns, B.sc);
+      uftk::bsc_el_check_lo(s, ns, B.sc, flags);
     } else {
       fine_apply(C, xphys, p, q, nullptr, nullptr);
       uftk::seg_dot(s, g, df, ns, n, p, q, C->d_part, B.sc, B_PQ);
       uftk::bsc_el_alpha(s, ns, B.sc);
       uftk::seg_cg_update(s, g, df, ns, n, B.sc, B_ALPHA, U, p, r, q, C->d_part, B_RR);
-      uftk::bsc_el_check(s, ns, B.sc);
+      uftk::bsc_el_check(s, ns, B.sc, flags);
     }
   };
   auto next = [&]() {
@@ -1662,27 +1666,38 @@ int solve_el_batch(uft_ctx *C, const double *xphys, double *U) {
     uftk::seg_xpby(s, g, n, z, B.sc, B_BETA, p);
     head();
   };
+  // FP64 residual of the designs that asked for it (FP64 form: the stopping
+  // test; mixed forms: a reliable update), launched after every iteration and
+  // gated on the device (flags[BF_NEED]): an idle pass returns at once
+  auto true_pass = [&]() {
+    uftk::apply_K_gated(s, C, xphys, U, q, flags + BF_NEED);
+    uftk::seg_true_res(s, g, df, ns, n, B.sc, C->d_F, q, r, C->d_part, B_RR, flags + BF_NEED);
+    uftk::bsc_el_true(s, ns, B.sc, flags);
+  };
+  // Device control: every test runs on the per-design scalars, so the host
+  // queues the iterations in rounds (as many as the last solve took, less
+  // one, then two at a time) and reads the batch flags once per round; an
+  // iteration queued after every design has stopped changes no design's U or
+  // r (their step lengths are zero).
   static const char tag = 0;
+  int round = C->el_prev_its > 1 ? C->el_prev_its - 1 : 8;
   while (any && it < C->P.maxit_el) {
-    if (it == 0)
-      head();
-    else if (!use_graph(C))
-      next();
-    else if ((rc = graph_launch(C, xphys, U, &tag, next)))
-      return rc;
-    ++it;
-    if ((rc = fetch_batch(C, B_ACT, &any))) return rc;
-    bool need = false;
-    for (int d = 0; d < ns; ++d) need |= B.h[(size_t)d * UFT_BSC + B_NEED] != 0.0;
-    if (need) {  // true residual of the designs that asked (FP64: the stopping test; mixed: a reliable update)
-      fine_apply(C, xphys, U, q, nullptr, nullptr);
-      uftk::seg_true_res(s, g, df, ns, n, B.sc, C->d_F, q, r, C->d_part, B_RR);
-      uftk::bsc_el_true(s, ns, B.sc);
-      C->reliable++;
-      if ((rc = fetch_batch(C, B_ACT, &any))) return rc;
+    const int k = std::min(round, C->P.maxit_el - it);
+    for (int j = 0; j < k; ++j) {
+      if (it == 0)
+        head();
+      else if (!use_graph(C))
+        next();
+      else if ((rc = graph_launch(C, xphys, U, &tag, next)))
+        return rc;
+      ++it;
+      true_pass();
     }
+    if ((rc = fetch(C, flags, 3, C->h_scal))) return rc;
+    any = C->h_scal[BF_ACT] != 0.0;
+    C->reliable = (int)C->h_scal[BF_NRES];
+    round = 2;
   }
-  C->it_el = it;
   C->outer_el = f32 ? C->reliable : 1;
   // true residuals per design (SolveFem.cpp:142-152)
   fine_apply(C, xphys, U, q, nullptr, nullptr);
@@ -1691,14 +1706,17 @@ int solve_el_batch(uft_ctx *C, const double *xphys, double *U) {
   if ((rc = fetch_batch(C, B_ACT, nullptr))) return rc;
   double worst = 0.0;
   int bad = -1;
+  C->it_el = 0;
   for (int d = 0; d < ns; ++d) {
     const double *S = B.h + (size_t)d * UFT_BSC;
     const double fn = S[B_FN];
     B.relres[d] = fn > 0.0 ? std::sqrt(S[B_TR]) / fn : 0.0;
     B.it[d] = (int)S[B_IT];
+    C->it_el = std::max(C->it_el, B.it[d]);
     if (!(B.relres[d] <= C->P.rtol_el) && bad < 0) bad = d;
     if (!(B.relres[d] <= worst)) worst = B.relres[d];
   }
+  C->el_prev_its = C->it_el;
   C->relres_el = worst;
   C->fnorm = B.h[B_FN];
   if (bad >= 0)
@@ -1835,10 +1853,14 @@ int solve_filter(uft_ctx *C, const double *b, double *x, int slot) {
 // Design batches: GenSub and the dual interior-point solve (SolveDIP,
 // MMA.cpp:639-934) of every design, m = 1.  Each design runs its own copy of
 // the single-design control flow (epsilon levels, at most 100 Newton steps
-// per level, residual carried across levels, tolerance 1e-9 sqrt(1 + n_d));
-// the designs advance in lockstep, one fused dual pass per step giving every
+// per level, residual carried across levels, tolerance 1e-9 sqrt(1 + n_d))
+// on its device scalars (uftk::bsc_mma_*, one thread per design); the
+// designs advance in lockstep, one fused dual pass per step giving every
 // design's x(lambda_d), constraint sum and Hessian sum.  A design that has
-// finished keeps its lambda, so later passes rewrite its x unchanged.
+// finished keeps its lambda, so later passes rewrite its x unchanged.  The
+// host queues the steps in rounds (as many as the last update took, then
+// four at a time) and reads two flags per round: a step queued after every
+// design has finished launches nothing but its decision kernel's check.
 int mma_update_batch(uft_ctx *C, double *x, const double *dfdx, const double *dgdx, const double *xmin,
                      const double *xmax) {
   hipStream_t s = C->s;
@@ -1851,99 +1873,48 @@ int mma_update_batch(uft_ctx *C, double *x, const double *dfdx, const double *dg
   const double *dg[1] = {dgdx};
   uftk::mma_gensub(s, C, n, 1, C->mk, x, xmin, xmax, dfdx, dg, C->d_part, g);
   uftk::seg_final(s, C->d_part, B.E.d_dfirst, ns, 1, B.sc, B_RED);
-  if ((rc = fetch(C, B.sc, ns * UFT_BSC, B.h))) return rc;
-  for (int d = 0; d < ns; ++d) B.b[d] = B.h[(size_t)d * UFT_BSC + B_RED] - B.gx[d];
   std::swap(C->d_xo1, C->d_xo2);
   uftk::copy(s, n, x, C->d_xo1);
 
   const double a = C->a[0], c = C->c[0];
-  std::vector<double> S(ns), H(ns), y(ns), z(ns), epsi(ns, 1.0), err(ns, 1.0), tol(ns);
-  std::vector<int> loop(ns, 0), steps(ns, 0);
-  std::vector<char> done(ns, 0), step(ns, 0);
-  double *hl = B.h + (size_t)ns * UFT_BSC;  // pinned staging of the lambdas
-  auto pass = [&]() -> int {
-    for (int d = 0; d < ns; ++d) hl[d] = B.lam[d];
-    UFT_HIP(C, hipMemcpyAsync(B.d_lam, hl, ns * sizeof(double), hipMemcpyHostToDevice, s));
-    uftk::mma_dual_pass(s, C, n, 1, nullptr, x, C->d_part, g, B.d_lam);
-    uftk::seg_final(s, C->d_part, B.E.d_dfirst, ns, 2, B.sc, B_RED);
-    int r2 = fetch(C, B.sc, ns * UFT_BSC, B.h);
-    if (r2) return r2;
-    for (int d = 0; d < ns; ++d) {
-      S[d] = B.h[(size_t)d * UFT_BSC + B_RED];
-      H[d] = B.h[(size_t)d * UFT_BSC + B_RED + 1];
-    }
-    return UFT_OK;
-  };
-  auto yz = [&](int d) {  // XYZofLAMBDA's scalar part (MMA.cpp:693-701)
-    if (B.lam[d] < 0.0) B.lam[d] = 0.0;
-    y[d] = std::max(0.0, B.lam[d] - c);
-    z[d] = std::max(0.0, 10.0 * (B.lam[d] * a - 1.0));
-  };
+  double *gt = B.h + (size_t)ns * UFT_BSC;  // pinned staging: gx and the tolerance per design
   for (int d = 0; d < ns; ++d) {
-    B.lam[d] = c / 2.0;
-    B.mu[d] = 1.0;
-    tol[d] = 1.0e-9 * std::sqrt((double)(1 + B.eoff[d + 1] - B.eoff[d]));
-    yz(d);
+    gt[2 * d] = B.gx[d];
+    gt[2 * d + 1] = 1.0e-9 * std::sqrt((double)(1 + B.eoff[d + 1] - B.eoff[d]));
   }
-  if ((rc = pass())) return rc;
+  UFT_HIP(C, hipMemcpyAsync(B.d_gt, gt, 2 * ns * sizeof(double), hipMemcpyHostToDevice, s));
+  double *flags = C->d_scal + S_MFL;
+  auto pass = [&](const double *gate) {
+    uftk::mma_dual_pass(s, C, n, 1, nullptr, x, C->d_part, g, B.d_lam, gate);
+    uftk::seg_final(s, C->d_part, B.E.d_dfirst, ns, 2, B.sc, B_RED, gate);
+  };
+  uftk::bsc_mma_init(s, ns, B.sc, B.d_gt, a, c, B.d_lam);
+  pass(nullptr);
+  int round = C->dip_steps > 0 ? C->dip_steps + 1 : 16, total = 0;
   while (true) {
-    bool any = false, neg = false;
-    for (int d = 0; d < ns; ++d) {
-      step[d] = 0;
-      if (done[d]) continue;
-      // leave epsilon levels whose inner loop has ended
-      while (!(err[d] > 0.9 * epsi[d] && loop[d] < 100)) {
-        epsi[d] *= 0.1;
-        loop[d] = 0;
-        if (!(epsi[d] > tol[d])) {
-          done[d] = 1;
-          break;
-        }
-      }
-      if (done[d]) continue;
-      step[d] = 1;
-      any = true;
-      neg |= B.lam[d] < 0.0;
+    for (int j = 0; j < round; ++j) {
+      uftk::bsc_mma_decide(s, ns, B.sc, a, c, B.d_lam, flags);
+      pass(flags + MF_NEG);  // the sums at a clamped lambda
+      uftk::bsc_mma_step(s, ns, B.sc, a, c, B.d_lam);
+      pass(flags + MF_STEP);
+      uftk::bsc_mma_resid(s, ns, B.sc, a);
     }
-    if (!any) break;
-    for (int d = 0; d < ns; ++d)
-      if (step[d]) yz(d);
-    if (neg && (rc = pass())) return rc;
-    for (int d = 0; d < ns; ++d) {
-      if (!step[d]) continue;
-      loop[d]++;
-      steps[d]++;
-      const double lam = B.lam[d], mu = B.mu[d], ep = epsi[d];
-      // DualGrad, DualHess (MMA.cpp:730-868), m = 1
-      double grad = S[d] - B.b[d] - a * z[d] - y[d];
-      grad = -1.0 * grad - ep / lam;
-      double hs = H[d];
-      if (lam > c) hs += -1.0;
-      hs += -mu / lam;
-      if (lam * a > 0.0) hs += -10.0 * a * a;
-      double corr = 1e-4 * hs;
-      if (-1.0 * corr < 1.0e-7) corr = -1.0e-7;
-      hs += corr;
-      const double s0 = grad / hs;
-      const double s1 = -mu + ep / lam - s0 * mu / lam;
-      // DualLineSearch (MMA.cpp:870-888)
-      double theta = 1.005;
-      if (theta < -1.01 * s0 / lam) theta = -1.01 * s0 / lam;
-      if (theta < -1.01 * s1 / mu) theta = -1.01 * s1 / mu;
-      theta = 1.0 / theta;
-      B.lam[d] += theta * s0;
-      B.mu[d] += theta * s1;
-      yz(d);
-    }
-    if ((rc = pass())) return rc;
-    for (int d = 0; d < ns; ++d) {  // DualResidual (MMA.cpp:890-934)
-      if (!step[d]) continue;
-      const double r1 = S[d] - B.b[d] - a * z[d] - y[d] + B.mu[d];
-      const double r2 = B.mu[d] * B.lam[d] - epsi[d];
-      err[d] = std::max(std::fabs(r1), std::fabs(r2));
-    }
+    total += round;
+    if ((rc = fetch(C, flags, 2, C->h_scal))) return rc;
+    if (C->h_scal[MF_STEP] == 0.0) break;
+    if (total > 20000) return uft_set_error(C, UFT_ERR_NOCONV, "MMA dual solve: %d steps without finishing", total);
+    round = 4;
   }
-  C->dip_steps = *std::max_element(steps.begin(), steps.end());
+  if ((rc = fetch(C, B.sc, ns * UFT_BSC, B.h))) return rc;
+  int steps = 0;
+  for (int d = 0; d < ns; ++d) {
+    const double *S = B.h + (size_t)d * UFT_BSC;
+    B.lam[d] = S[B_MLAM];
+    B.mu[d] = S[B_MMU];
+    B.b[d] = S[B_MB];
+    steps = std::max(steps, (int)S[B_MSTEPS]);
+  }
+  C->dip_steps = steps;
   return UFT_OK;
 }
 
@@ -2273,6 +2244,7 @@ int uft_setup(uft_ctx *C) {
     double **bv[] = {&B.d_invTV, &B.d_lam, &B.d_fac};
     for (double **p : bv)
       if ((rc = dalloc_n(C, p, (int64_t)B.nseg))) return rc;
+    if ((rc = dalloc_n(C, &B.d_gt, (int64_t)B.nseg * 2))) return rc;
     if ((rc = dalloc_n(C, &B.d_log, (int64_t)B.nseg * 128))) return rc;
     uftk::fill(s, (int64_t)B.nseg * 128, 0.0, B.d_log);
   }

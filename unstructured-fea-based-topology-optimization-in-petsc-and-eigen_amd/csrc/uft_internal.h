@@ -46,7 +46,7 @@ struct SegG {
   int nb = 0;
 };
 #define UFT_MAXSEG 64   // designs per batch
-#define UFT_BSC 64      // per-design device scalars
+#define UFT_BSC 96      // per-design device scalars
 
 // Slots of a batch design's device scalars (sc + d * UFT_BSC): elasticity
 // flexible CG, filter CG, Lanczos, then general reductions from B_RED.
@@ -57,8 +57,17 @@ enum {
   B_FIT = 25, B_FSN = 26, B_FDIV = 27, B_FZZN = 28, B_FZRN = 29,
   B_LRZ = 32, B_LPQ = 33, B_LAL = 34, B_LRR = 35, B_LRZN = 36, B_LBE = 37,
   B_RED = 40,
-  B_KC = 44  // K-cycle scalars of levels 1 and 2, eight each (the single-design K-cycle's slots)
+  B_KC = 44,  // K-cycle scalars of levels 1 and 2, eight each (the single-design K-cycle's slots)
+  // the MMA dual interior-point state (m = 1): lambda, mu, epsilon, residual,
+  // inner steps of the level, all steps, finished, stepping, y, z, b, tolerance
+  B_MLAM = 64, B_MMU = 65, B_MEPS = 66, B_MERR = 67, B_MLOOP = 68, B_MSTEPS = 69, B_MDONE = 70, B_MSTEP = 71,
+  B_MY = 72, B_MZ = 73, B_MB = 74, B_MTOL = 75
 };
+// batch-wide flags of the device-controlled loops (three doubles each):
+// elasticity CG: 0 some design asks for its FP64 residual, 1 some design is
+// active, 2 FP64 residual passes so far; MMA dual: 0 some design steps, 1 some
+// stepping design had a negative lambda
+enum { BF_NEED = 0, BF_ACT = 1, BF_NRES = 2, MF_STEP = 0, MF_NEG = 1 };
 #define UFT_BATCH_KLEV 2  // coarse levels that run the K-cycle in a design batch (the W-cycle below)
 
 // E(x) = Emin + x^penal * (Emax - Emin)   (SolveFem.cpp:304)
@@ -210,6 +219,7 @@ struct BatchState {
   double *d_vf = nullptr;             // per design volfrac
   double *d_invTV = nullptr;          // per design 1 / total volume
   double *d_lam = nullptr;            // per design MMA dual variable
+  double *d_gt = nullptr;             // per design gx and the dual solve's tolerance
   double *d_fac = nullptr;            // per design Chebyshev scaling 1 / lambda_max
   double *d_log = nullptr;            // Lanczos log: nseg x 64 steps x 2
   // per design results of the last iteration
@@ -685,7 +695,7 @@ void mma_gensub(hipStream_t s, uft_ctx *C, int64_t n, int m, int k, const double
                 const double *xmax, const double *dfdx, const double *const *dgdx, double *part, SegG sg = SegG());
 // lamd (device, per design of the segment grid) replaces lam when set (m = 1)
 void mma_dual_pass(hipStream_t s, uft_ctx *C, int64_t n, int m, const double *lam, double *x, double *part,
-                   SegG sg = SegG(), const double *lamd = nullptr);
+                   SegG sg = SegG(), const double *lamd = nullptr, const double *gate = nullptr);
 // out[0] = sum of squares, out[1] = max abs of the KKT residual terms
 void mma_kkt(hipStream_t s, int64_t n, int m, const double *lam, const double *x, const double *dfdx,
              const double *const *dgdx, const double *xmin, const double *xmax, double *part, double *out);
@@ -744,7 +754,8 @@ void axpy_inplace_t(hipStream_t s, int64_t n, const float *d, float *x);
 void kcycle_coef1(hipStream_t s, double *sc);
 void kcycle_coef2(hipStream_t s, double *sc);
 // design batches (kernels_batch.hip): per-design scalars at sc + d * UFT_BSC
-void seg_final(hipStream_t s, const double *part, const int32_t *dfirst, int nseg, int K, double *sc, int base);
+void seg_final(hipStream_t s, const double *part, const int32_t *dfirst, int nseg, int K, double *sc, int base,
+               const double *gate = nullptr);
 void seg_final_max(hipStream_t s, const double *part, const int32_t *dfirst, int nseg, double *sc, int base);
 void seg_dot(hipStream_t s, SegG g, const int32_t *dfirst, int nseg, int64_t n, const double *a, const double *b,
              double *part, double *sc, int base);
@@ -781,12 +792,22 @@ void seg_fcg_p(hipStream_t s, SegG g, int64_t n, const double *z, const double *
 // per-design scalar recurrences (one thread per design)
 void bsc_el_init(hipStream_t s, int nseg, double *sc, double rtol);
 void bsc_el_alpha(hipStream_t s, int nseg, double *sc);
-void bsc_el_check(hipStream_t s, int nseg, double *sc);
+void bsc_el_check(hipStream_t s, int nseg, double *sc, double *flags);
 void bsc_el_beta(hipStream_t s, int nseg, double *sc);
-void bsc_el_check_lo(hipStream_t s, int nseg, double *sc);
-void bsc_el_true(hipStream_t s, int nseg, double *sc);
+void bsc_el_check_lo(hipStream_t s, int nseg, double *sc, double *flags);
+void bsc_el_true(hipStream_t s, int nseg, double *sc, double *flags);
+// MMA dual interior point of a batch on the device (m = 1): start (after the
+// GenSub sums in B_RED; gt: per design gx and the tolerance), the step
+// decision, the Newton step with its line search, the dual residual
+void bsc_mma_init(hipStream_t s, int nseg, double *sc, const double *gt, double a, double c, double *lamd);
+void bsc_mma_decide(hipStream_t s, int nseg, double *sc, double a, double c, double *lamd, double *flags);
+void bsc_mma_step(hipStream_t s, int nseg, double *sc, double a, double c, double *lamd);
+void bsc_mma_resid(hipStream_t s, int nseg, double *sc, double a);
+// q = K U in FP64 when *gate != 0 (element and node phase; every workgroup
+// returns at once otherwise)
+void apply_K_gated(hipStream_t s, const uft_ctx *C, const double *xphys, const double *U, double *q, const double *gate);
 void seg_true_res(hipStream_t s, SegG g, const int32_t *dfirst, int nseg, int64_t n, double *sc, const double *F,
-                  const double *q, double *r, double *part, int base);
+                  const double *q, double *r, double *part, int base, const double *gate = nullptr);
 void seg_dot_lo(hipStream_t s, SegG g, const int32_t *dfirst, int nseg, int64_t n, const double *a, const float *b,
                 double *part, double *sc, int base);
 void seg_cg_update_lo(hipStream_t s, SegG g, const int32_t *dfirst, int nseg, int64_t n, double *sc, int ia, double *x,
