@@ -11,8 +11,9 @@
 // for i, j outside K; after every block has been swept M = -A^-1.  Per
 // block two launches: the panel (C_i = M_iK and W_i = C_i D^-1 into
 // buffers, so the update reads no tile another workgroup writes) and the
-// update of every lower tile (W_i C_j^T, one 64x64x64 product per tile),
-// whose workgroup for the next pivot tile also inverts it.  No pivoting and
+// update of every lower tile (W_i C_j^T, one 64x64x64 product per tile on
+// the FP64 matrix cores), whose workgroup for the next pivot tile also
+// inverts it.  No pivoting and
 // no atomics: every sum runs in a fixed order, results are reproducible.
 #include <hip/hip_runtime.h>
 
@@ -22,8 +23,10 @@ namespace {
 
 constexpr int DB = 64;           // tile edge
 constexpr int DT = DB * DB;      // entries per tile
-constexpr int DTPB = 256;        // threads per workgroup: 16 x 16, a 4x4 block of the tile each
+constexpr int DTPB = 256;        // threads per workgroup (the update: four wavefronts, a 32x32 quarter each)
 constexpr int LDP = DB + 4;      // LDS row pitch of transposed operands (doubles)
+constexpr int LDQ = DB + 2;      // LDS row pitch of the update's row-major operands (doubles)
+typedef double v4d __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ int64_t tile_off(int i, int j) { return ((int64_t)i * (i + 1) / 2 + j) * DT; }
 
@@ -130,17 +133,21 @@ __global__ __launch_bounds__(DTPB) void k_sweep_panel(int k, const double *__res
   __shared__ double Ds[DB * LDP];  // D^-1 rows: Ds[m][c]
   const int tid = threadIdx.x;
   const double *src = i > k ? T + tile_off(i, k) : T + tile_off(k, i);
-  for (int t = tid; t < DT; t += DTPB) {
-    const int r = t / DB, m = t % DB;
-    double v;
-    if (i > k) {
-      v = src[t];  // C_i[r][m] = tile(i,k)[r][m]
-    } else {
-      v = src[m * DB + r];  // C_i[r][m] = tile(k,i)[m][r]
+  {  // all loads in flight before the stores
+    double cv[DT / DTPB], dv[DT / DTPB];
+#pragma unroll
+    for (int u = 0; u < DT / DTPB; ++u) {
+      const int t = tid + u * DTPB, r = t / DB, m = t % DB;
+      cv[u] = i > k ? src[t] : src[m * DB + r];  // C_i[r][m] = tile(i,k)[r][m] or tile(k,i)[m][r]
+      dv[u] = Dk[t];
     }
-    Ct[m * LDP + r] = v;
-    Cb[(int64_t)i * DT + t] = v;
-    Ds[r * LDP + m] = Dk[t];
+#pragma unroll
+    for (int u = 0; u < DT / DTPB; ++u) {
+      const int t = tid + u * DTPB, r = t / DB, m = t % DB;
+      Ct[m * LDP + r] = cv[u];
+      Cb[(int64_t)i * DT + t] = cv[u];
+      Ds[r * LDP + m] = dv[u];
+    }
   }
   __syncthreads();
   const int ty = tid / 16, tx = tid % 16;
@@ -176,7 +183,17 @@ __global__ __launch_bounds__(DTPB) void k_sweep_update(int k, int nt, double *__
   // tiles (a band of tile rows), so the panel tiles W_i it reads stay in its
   // L2 (the C_j of one band, at most nt tiles, too)
   const int64_t nb = gridDim.x, xb = blockIdx.x % 8, q8 = nb / 8, r8 = nb % 8;
-  const int64_t t = (xb < r8 ? xb * (q8 + 1) : r8 * (q8 + 1) + (xb - r8) * q8) + blockIdx.x / 8;
+  int64_t t = (xb < r8 ? xb * (q8 + 1) : r8 * (q8 + 1) + (xb - r8) * q8) + blockIdx.x / 8;
+  // the next pivot tile goes to workgroup 0 (dispatched first; that mapping
+  // gives it tile 0, which takes the pivot tile's workgroup instead): its
+  // serial inverse then runs beside the other tiles' updates
+  if (Dnext) {
+    const int64_t tp = (int64_t)(k + 1) * (k + 2) / 2 + (k + 1);
+    if (blockIdx.x == 0)
+      t = tp;
+    else if (t == tp)
+      t = 0;
+  }
   int i = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
   while ((int64_t)(i + 1) * (i + 2) / 2 <= t) ++i;
   while ((int64_t)i * (i + 1) / 2 > t) --i;
@@ -197,42 +214,63 @@ __global__ __launch_bounds__(DTPB) void k_sweep_update(int k, int nt, double *__
     for (int q = tid; q < DT; q += DTPB) tile[q] = w[(q % DB) * DB + q / DB];
     return;
   }
-  __shared__ double Wt[DB * LDP];  // W_i transposed: Wt[m][r] = W_i[r][m]
-  __shared__ double Ct[DB * LDP];  // C_j transposed: Ct[m][c] = C_j[c][m]
+  // W_i and C_j row by row (pitch LDQ: the fragment reads below fall in
+  // distinct banks); Wt is the next pivot tile's row-major copy afterwards
+  __shared__ double Wt[DB * LDQ];
+  __shared__ double Ct[DB * LDQ];
   __shared__ double colk[2 * DB], rowk[2 * DB];
   const double *w = Wb + (int64_t)i * DT;
   const double *cj = Cb + (int64_t)j * DT;
-  for (int q = tid; q < DT; q += DTPB) {
-    const int r = q / DB, m = q % DB;
-    Wt[m * LDP + r] = w[q];
-    Ct[m * LDP + r] = cj[q];
-  }
-  __syncthreads();
-  const int ty = tid / 16, tx = tid % 16;
-  double acc[4][4] = {};
-  for (int m = 0; m < DB; ++m) {
-    double a[4], b[4];
+  {  // all 32 loads in flight before the LDS stores
+    double wv[DT / DTPB], cv[DT / DTPB];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      a[u] = Wt[m * LDP + 4 * ty + u];
-      b[u] = Ct[m * LDP + 4 * tx + u];
+    for (int u = 0; u < DT / DTPB; ++u) {
+      wv[u] = w[tid + u * DTPB];
+      cv[u] = cj[tid + u * DTPB];
     }
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
+    for (int u = 0; u < DT / DTPB; ++u) {
+      const int q = tid + u * DTPB, r = q / DB, m = q % DB;
+      Wt[r * LDQ + m] = wv[u];
+      Ct[r * LDQ + m] = cv[u];
+    }
+  }
+  __syncthreads();
+  // W_i C_j^T on the FP64 matrix cores: wavefront w takes rows R0..R0+31 and
+  // columns C0..C0+31 as 2x2 16x16 tiles, 16 steps of v_mfma_f64_16x16x4f64
+  // over the 64 inner indices (lane l: A = W_i[row l % 16][k + l / 16],
+  // B = C_j[col l % 16][k + l / 16]; result entry r of lane l is row
+  // l / 16 + 4 r, column l % 16 of the 16x16 tile)
+  const int lane = tid & 63, wv = tid >> 6;
+  const int R0 = 32 * (wv >> 1), C0 = 32 * (wv & 1), lr = lane & 15, lk = lane >> 4;
+  v4d acc[2][2];
 #pragma unroll
-      for (int v = 0; v < 4; ++v) acc[u][v] += a[u] * b[v];
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc[a][b] = v4d{0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+  for (int kk = 0; kk < DB; kk += 4) {
+    const int kc = kk + lk;
+    const double a0 = Wt[(R0 + lr) * LDQ + kc], a1 = Wt[(R0 + 16 + lr) * LDQ + kc];
+    const double b0 = Ct[(C0 + lr) * LDQ + kc], b1 = Ct[(C0 + 16 + lr) * LDQ + kc];
+    acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+    acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+    acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+    acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
   }
   const bool next = (i == k + 1 && j == k + 1 && Dnext);
   __syncthreads();  // Wt is reused for the next pivot below
 #pragma unroll
-  for (int u = 0; u < 4; ++u)
+  for (int a = 0; a < 2; ++a)
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int q = (4 * ty + u) * DB + 4 * tx + v;
-      const double nv = tile[q] - acc[u][v];
-      tile[q] = nv;
-      if (next) Wt[q] = nv;
-    }
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int q = (R0 + 16 * a + lk + 4 * r) * DB + C0 + 16 * b + lr;
+        const double nv = tile[q] - acc[a][b][r];
+        tile[q] = nv;
+        if (next) Wt[q] = nv;
+      }
   if (!next) return;
   __syncthreads();
   tile_inverse(Wt, colk, rowk);
