@@ -209,7 +209,7 @@ def run_form(args, precision):
     # (HIP events on the context's stream; one design, one load case)
     b2b = {}
     if ndesign == 1 and WORKLOADS[args.workload][5] != 2:
-        kinds = {"fp64": (0,), "mixed": (1,), "mixed_h": (2, 1)}[precision]
+        kinds = {"fp64": (0, 3), "mixed": (1,), "mixed_h": (2, 1)}[precision]
         for k in kinds:
             b2b[k] = ctx.time_fine_operator(k, 20)
     ctx.close()
@@ -362,7 +362,7 @@ def main():
         q = res["fp64"]
         rooflines.append(elem_roofline("fp64", npe, ne, ndof, q["prof"][0], q["prof"][1], 8,
                                        f"{k64} (FP64 element phase of the fine stiffness apply, FP64 form)",
-                                       f"{args.workload}_fp64_elem_apply", nrhs))
+                                       f"{args.workload}_fp64_elem_apply", nrhs, b2b_us=q["b2b"].get(3)))
         if not multi:
             n, ms = q["prof"][0]
             nn_, msn = q["prof"][1]

@@ -275,8 +275,9 @@ int uft_level_apply(uft_ctx *ctx, int level, int kind, const double *x, double *
  * the solver's own work vectors (their contents are overwritten), timed with
  * HIP events on the context's stream: kind 0 the FP64 stiffness apply
  * (element and node phase), 1 the mixed forms' FP32 outer product, 2 the
- * UFT_MIXED_H smoother's operator with its Chebyshev-step epilogue. Kind 0
- * needs the FP64 form, kinds 1 and 2 a mixed form; one design, one load case. */
+ * UFT_MIXED_H smoother's operator with its Chebyshev-step epilogue, 3 the
+ * FP64 element phase alone. Kind 0 needs the FP64 form, kinds 1 and 2 a
+ * mixed form; one design, one load case. */
 int uft_time_fine_operator(uft_ctx *ctx, int kind, int reps, double *us);
 
 /* --- MMA settings, restart and KKT residual (MMA.h:8-52) ------------------ */

@@ -125,7 +125,7 @@ def test_time_fine_operator_kinds():
             ctx.time_fine_operator(ok[0], 3)
         ctx.design_iteration(1)
         f1 = ctx.fem_obj_sens()[0]
-        for k in ok:
+        for k in ok + ((3,) if precision == 0 else ()):
             assert ctx.time_fine_operator(k, 5) > 0.0
         for k in bad:
             with pytest.raises(UFTError):

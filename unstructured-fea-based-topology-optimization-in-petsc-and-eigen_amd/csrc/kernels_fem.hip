@@ -465,11 +465,13 @@ template <int NPE, typename T, typename T2, int BT, bool NT, int WPE>
 __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1)))
 void k_elem_apply_v(int ne, const int32_t *__restrict__ elem, const T2 *__restrict__ KE,
                     const double *__restrict__ xphys, SimpCoef sc, const uint8_t *__restrict__ fixm,
-                    const T *__restrict__ u, T *__restrict__ Ye, const double *__restrict__ gate) {
+                    const T *__restrict__ u, T *__restrict__ Ye, const double *__restrict__ gate,
+                    const double *__restrict__ dgate, const int32_t *__restrict__ edes) {
   constexpr int ND = 3 * NPE, NP = ND * (ND + 1) / 2, NQ = NP / 2;
   if (gate && gate[0] == 0.0) return;  // a device-controlled solve did not ask for this product
   const int e = blockIdx.x * BT + threadIdx.x;
   if (e >= ne) return;
+  if (edes && dgate[(size_t)edes[e] * UFT_BSC] == 0.0) return;  // a batch design that did not ask
   T ue[ND], ye[ND];
 #pragma unroll
   for (int a = 0; a < NPE; ++a) {
@@ -585,10 +587,13 @@ __global__ __launch_bounds__(TPB) void k_elem_apply(int ne, const int32_t *__res
                                                     const T2 *__restrict__ KE, const double *__restrict__ xphys,
                                                     SimpCoef sc, const uint8_t *__restrict__ fixm,
                                                     const T *__restrict__ u, T *__restrict__ Ye,
-                                                    const double *__restrict__ gate) {
+                                                    const double *__restrict__ gate,
+                                                    const double *__restrict__ dgate,
+                                                    const int32_t *__restrict__ edes) {
   constexpr int ND = 3 * NPE, NP = ND * (ND + 1) / 2, NQ = NP / 2;
   if (gate && gate[0] == 0.0) return;  // a device-controlled solve did not ask for this product
   for (int e = blockIdx.x * TPB + threadIdx.x; e < ne; e += gridDim.x * TPB) {
+    if (edes && dgate[(size_t)edes[e] * UFT_BSC] == 0.0) continue;  // a batch design that did not ask
     T ue[ND], ye[ND];
 #pragma unroll
     for (int a = 0; a < NPE; ++a) {
@@ -753,6 +758,8 @@ __global__ __launch_bounds__(TPB) void k_reliable_node(int nn, int ne, const dou
 // design asked for)
 template <int NPE>
 __global__ __launch_bounds__(TPB) void k_node_q_gated(int nn, int ne, const double *__restrict__ gate,
+                                                      const double *__restrict__ dgate,
+                                                      const int32_t *__restrict__ ndes,
                                                       const int32_t *__restrict__ iptr,
                                                       const int32_t *__restrict__ inc, const double *__restrict__ Ye,
                                                       const uint8_t *__restrict__ fixm, const double *__restrict__ U,
@@ -760,6 +767,7 @@ __global__ __launch_bounds__(TPB) void k_node_q_gated(int nn, int ne, const doub
   if (*gate == 0.0) return;
   const int n = blockIdx.x * TPB + threadIdx.x;
   if (n >= nn) return;
+  if (ndes && dgate[(size_t)ndes[n] * UFT_BSC] == 0.0) return;
   double acc[3] = {0.0, 0.0, 0.0};
   for (int k = iptr[n]; k < iptr[n + 1]; ++k) {
     const int v = inc[k];
@@ -1249,16 +1257,17 @@ __global__ __launch_bounds__(TPB) void k_projsens(int64_t n, const double *__res
 // ---- element-phase variant dispatch
 template <int NPE, typename T, typename T2>
 static void elem_phase_t(hipStream_t s, int variant, int ne, const int32_t *elem, const T2 *KE, const double *xphys,
-                         SimpCoef sc, const uint8_t *fixm, const T *u, T *Ye, const double *gate = nullptr) {
+                         SimpCoef sc, const uint8_t *fixm, const T *u, T *Ye, const double *gate = nullptr,
+                         const double *dgate = nullptr, const int32_t *edes = nullptr) {
   auto g = [&](int bt) { return (unsigned)((ne + bt - 1) / bt); };
   switch (variant) {
-    case 1: k_elem_apply_v<NPE, T, T2, 256, true, 0><<<g(256), 256, 0, s>>>(ne, elem, KE, xphys, sc, fixm, u, Ye, gate); break;
-    case 2: k_elem_apply_v<NPE, T, T2, 256, true, 4><<<g(256), 256, 0, s>>>(ne, elem, KE, xphys, sc, fixm, u, Ye, gate); break;
-    case 3: k_elem_apply_v<NPE, T, T2, 128, true, 0><<<g(128), 128, 0, s>>>(ne, elem, KE, xphys, sc, fixm, u, Ye, gate); break;
-    case 4: k_elem_apply_v<NPE, T, T2, 512, true, 0><<<g(512), 512, 0, s>>>(ne, elem, KE, xphys, sc, fixm, u, Ye, gate); break;
-    case 5: k_elem_apply_v<NPE, T, T2, 256, false, 4><<<g(256), 256, 0, s>>>(ne, elem, KE, xphys, sc, fixm, u, Ye, gate); break;
-    case 6: k_elem_apply_v<NPE, T, T2, 64, true, 0><<<g(64), 64, 0, s>>>(ne, elem, KE, xphys, sc, fixm, u, Ye, gate); break;
-    default: k_elem_apply<NPE, T, T2><<<g(TPB), TPB, 0, s>>>(ne, elem, KE, xphys, sc, fixm, u, Ye, gate); break;
+    case 1: k_elem_apply_v<NPE, T, T2, 256, true, 0><<<g(256), 256, 0, s>>>(ne, elem, KE, xphys, sc, fixm, u, Ye, gate, dgate, edes); break;
+    case 2: k_elem_apply_v<NPE, T, T2, 256, true, 4><<<g(256), 256, 0, s>>>(ne, elem, KE, xphys, sc, fixm, u, Ye, gate, dgate, edes); break;
+    case 3: k_elem_apply_v<NPE, T, T2, 128, true, 0><<<g(128), 128, 0, s>>>(ne, elem, KE, xphys, sc, fixm, u, Ye, gate, dgate, edes); break;
+    case 4: k_elem_apply_v<NPE, T, T2, 512, true, 0><<<g(512), 512, 0, s>>>(ne, elem, KE, xphys, sc, fixm, u, Ye, gate, dgate, edes); break;
+    case 5: k_elem_apply_v<NPE, T, T2, 256, false, 4><<<g(256), 256, 0, s>>>(ne, elem, KE, xphys, sc, fixm, u, Ye, gate, dgate, edes); break;
+    case 6: k_elem_apply_v<NPE, T, T2, 64, true, 0><<<g(64), 64, 0, s>>>(ne, elem, KE, xphys, sc, fixm, u, Ye, gate, dgate, edes); break;
+    default: k_elem_apply<NPE, T, T2><<<g(TPB), TPB, 0, s>>>(ne, elem, KE, xphys, sc, fixm, u, Ye, gate, dgate, edes); break;
   }
 }
 
@@ -1401,17 +1410,20 @@ void reliable_update(hipStream_t s, const uft_ctx *C, const double *xphys, const
 }
 
 void apply_K_gated(hipStream_t s, const uft_ctx *C, const double *xphys, const double *U, double *q,
-                   const double *gate) {
+                   const double *gate, const double *dgate) {
   const SimpCoef sc{C->P.Emin, C->P.Emax - C->P.Emin, C->P.penal};
   const int ne = (int)C->ne, nn = (int)C->nn;
+  const int32_t *edes = dgate ? C->bt.d_edes : nullptr, *ndes = dgate ? C->bt.d_ndes : nullptr;
   if (C->npe == 8) {
     elem_phase_t<8, double, double2>(s, C->elem_variant, ne, C->d_elem, (const double2 *)C->d_KE, xphys, sc, C->d_fixm,
-                                     U, C->d_Ye, gate);
-    k_node_q_gated<8><<<nblk(nn), TPB, 0, s>>>(nn, ne, gate, C->d_iptr, C->d_inc, C->d_Ye, C->d_fixm, U, q);
+                                     U, C->d_Ye, gate, dgate, edes);
+    k_node_q_gated<8><<<nblk(nn), TPB, 0, s>>>(nn, ne, gate, dgate, ndes, C->d_iptr, C->d_inc, C->d_Ye, C->d_fixm, U,
+                                               q);
   } else {
     elem_phase_t<4, double, double2>(s, C->elem_variant, ne, C->d_elem, (const double2 *)C->d_KE, xphys, sc, C->d_fixm,
-                                     U, C->d_Ye, gate);
-    k_node_q_gated<4><<<nblk(nn), TPB, 0, s>>>(nn, ne, gate, C->d_iptr, C->d_inc, C->d_Ye, C->d_fixm, U, q);
+                                     U, C->d_Ye, gate, dgate, edes);
+    k_node_q_gated<4><<<nblk(nn), TPB, 0, s>>>(nn, ne, gate, dgate, ndes, C->d_iptr, C->d_inc, C->d_Ye, C->d_fixm, U,
+                                               q);
   }
 }
 

@@ -1670,7 +1670,7 @@ int solve_el_batch(uft_ctx *C, const double *xphys, double *U) {
   // test; mixed forms: a reliable update), launched after every iteration and
   // gated on the device (flags[BF_NEED]): an idle pass returns at once
   auto true_pass = [&]() {
-    uftk::apply_K_gated(s, C, xphys, U, q, flags + BF_NEED);
+    uftk::apply_K_gated(s, C, xphys, U, q, flags + BF_NEED, B.sc + B_NEED);
     uftk::seg_true_res(s, g, df, ns, n, B.sc, C->d_F, q, r, C->d_part, B_RR, flags + BF_NEED);
     uftk::bsc_el_true(s, ns, B.sc, flags);
   };
@@ -2245,6 +2245,14 @@ int uft_setup(uft_ctx *C) {
     for (double **p : bv)
       if ((rc = dalloc_n(C, p, (int64_t)B.nseg))) return rc;
     if ((rc = dalloc_n(C, &B.d_gt, (int64_t)B.nseg * 2))) return rc;
+    {  // design of every element and node
+      std::vector<int32_t> ed((size_t)ne), nd((size_t)nn);
+      for (int d = 0; d < B.nseg; ++d) {
+        for (int64_t e = B.eoff[d]; e < B.eoff[d + 1]; ++e) ed[e] = d;
+        for (int64_t v = B.noff[d]; v < B.noff[d + 1]; ++v) nd[v] = d;
+      }
+      if ((rc = upload(C, &B.d_edes, ed)) || (rc = upload(C, &B.d_ndes, nd))) return rc;
+    }
     if ((rc = dalloc_n(C, &B.d_log, (int64_t)B.nseg * 128))) return rc;
     uftk::fill(s, (int64_t)B.nseg * 128, 0.0, B.d_log);
   }
@@ -3297,9 +3305,9 @@ extern "C" int uft_level_apply(uft_ctx *C, int level, int kind, const double *x,
 
 extern "C" int uft_time_fine_operator(uft_ctx *C, int kind, int reps, double *us) {
   if (int rc = check_setup(C)) return rc;
-  if (!us || reps < 1 || kind < 0 || kind > 2) return uft_set_error(C, UFT_ERR_ARG, "uft_time_fine_operator: bad arguments");
+  if (!us || reps < 1 || kind < 0 || kind > 3) return uft_set_error(C, UFT_ERR_ARG, "uft_time_fine_operator: bad arguments");
   if (C->refreshes == 0) return uft_set_error(C, UFT_ERR_STATE, "uft_time_fine_operator: no hierarchy refresh yet");
-  if ((kind > 0) != is_mixed(C))
+  if ((kind == 1 || kind == 2) != is_mixed(C))
     return uft_set_error(C, UFT_ERR_ARG, "uft_time_fine_operator: kind %d does not match the form", kind);
   if (kind == 2 && C->P.precision != UFT_MIXED_H)
     return uft_set_error(C, UFT_ERR_ARG, "uft_time_fine_operator: kind 2 needs UFT_MIXED_H");
@@ -3316,6 +3324,8 @@ extern "C" int uft_time_fine_operator(uft_ctx *C, int kind, int reps, double *us
       op.mode = ROW_STORE;
       op.y = L0.t;
       lev_op(C, 0, (const double *)L0.x, op);
+    } else if (kind == 3) {  // the FP64 element phase alone
+      uftk::elem_phase(s, C, C->elem_variant, C->cur_xphys, L0.x, C->d_Ye);
     } else if (kind == 1) {
       fine_apply_lo(C, C->cur_xphys, C->d_p, -1, true);
     } else {  // a Chebyshev step of the fine smoother: r -= A d; d = c1 d + c2 D^-1 r; x += d

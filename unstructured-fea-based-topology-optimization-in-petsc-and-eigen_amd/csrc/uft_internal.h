@@ -220,6 +220,7 @@ struct BatchState {
   double *d_invTV = nullptr;          // per design 1 / total volume
   double *d_lam = nullptr;            // per design MMA dual variable
   double *d_gt = nullptr;             // per design gx and the dual solve's tolerance
+  int32_t *d_edes = nullptr, *d_ndes = nullptr;  // design of every element / node
   double *d_fac = nullptr;            // per design Chebyshev scaling 1 / lambda_max
   double *d_log = nullptr;            // Lanczos log: nseg x 64 steps x 2
   // per design results of the last iteration
@@ -804,8 +805,10 @@ void bsc_mma_decide(hipStream_t s, int nseg, double *sc, double a, double c, dou
 void bsc_mma_step(hipStream_t s, int nseg, double *sc, double a, double c, double *lamd);
 void bsc_mma_resid(hipStream_t s, int nseg, double *sc, double a);
 // q = K U in FP64 when *gate != 0 (element and node phase; every workgroup
-// returns at once otherwise)
-void apply_K_gated(hipStream_t s, const uft_ctx *C, const double *xphys, const double *U, double *q, const double *gate);
+// returns at once otherwise); with dgate (a design batch's scalars + slot)
+// only the elements and nodes of the designs whose dgate entry is nonzero
+void apply_K_gated(hipStream_t s, const uft_ctx *C, const double *xphys, const double *U, double *q, const double *gate,
+                   const double *dgate = nullptr);
 void seg_true_res(hipStream_t s, SegG g, const int32_t *dfirst, int nseg, int64_t n, double *sc, const double *F,
                   const double *q, double *r, double *part, int base, const double *gate = nullptr);
 void seg_dot_lo(hipStream_t s, SegG g, const int32_t *dfirst, int nseg, int64_t n, const double *a, const float *b,
