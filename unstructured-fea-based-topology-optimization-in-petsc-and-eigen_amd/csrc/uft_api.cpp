@@ -1537,6 +1537,11 @@ int solve_el_block(uft_ctx *C, const double *xphys) {
       // product also replaces the recursive residual after 100-fold drops
       if (rn <= tol[k] || (f32 && rn < 1e-2 * rtrue[k])) need |= 1u << k;
     }
+    // the FP64 pass computes every case's residual at once: when one case
+    // asks for it, every active case takes its FP64 residual (a reliable
+    // update is valid at any iteration), which keeps the cases' updates in
+    // step instead of one pass per case
+    if (need && f32) need |= mask;
     if (need) {  // reliable update of the cases in need
       mr_lev_op<double>(C, 0, M.U, tr);
       uftk::mr_take_res(s, nr, n, need, M.q, M.r, M.rf, C->d_part, sc + MS_RR);
