@@ -490,6 +490,147 @@ __global__ __launch_bounds__(TPB) void k_sa_prolong_comb_h(int64_t nu, int64_t n
   x[u * 3 + 2] += a2;
 }
 
+// ---- the same transfers for NR interleaved right-hand sides (the
+// multi-load-case form, vectors [row][NR]): every P entry is read once for
+// all right-hand sides.
+// P entry e (3x6) of the support-ordered copy, in FP32 registers
+template <typename S>
+__device__ __forceinline__ void ld_entry_s(const S *Ps, int64_t e, float (&p)[18]) {
+  if constexpr (std::is_same<S, __half>::value) {
+    const uint32_t *pw = reinterpret_cast<const uint32_t *>(Ps) + e * 9;
+#pragma unroll
+    for (int h = 0; h < 9; ++h) {
+      const float2 v = unpack_h2(pw[h]);
+      p[2 * h] = v.x;
+      p[2 * h + 1] = v.y;
+    }
+  } else {
+#pragma unroll
+    for (int v = 0; v < 18; ++v) p[v] = (float)Ps[e * 18 + v];
+  }
+}
+// G lanes per coarse unit, one support node per lane and step, all NR
+// right-hand sides per lane; then the butterfly over the G lanes and level
+// 1's first Chebyshev step per right-hand side
+template <int G, typename T, typename S, int NR>
+__global__ __launch_bounds__(TPB) void k_mr_sa_restrict_first(int64_t nc, const int32_t *__restrict__ sptr,
+                                                              const int32_t *__restrict__ snode,
+                                                              const S *__restrict__ Ps, const T *__restrict__ r,
+                                                              T *__restrict__ bc, const T *__restrict__ dinv,
+                                                              double it, T *__restrict__ rc, T *__restrict__ dc,
+                                                              T *__restrict__ xc) {
+  static_assert(G >= 1 && G <= 64 && (64 % G) == 0, "G lanes must tile a wavefront");
+  const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  const int64_t A = t / G;
+  const int lane = (int)(t % G);
+  const bool on = A < nc;
+  const int k1 = on ? sptr[A + 1] : 0;
+  T acc[6][NR];
+#pragma unroll
+  for (int j = 0; j < 6; ++j)
+#pragma unroll
+    for (int q = 0; q < NR; ++q) acc[j][q] = T(0);
+  for (int k = (on ? sptr[A] : 0) + lane; k < k1; k += G) {
+    const int64_t i = snode[k];
+    float p[18];
+    ld_entry_s<S>(Ps, k, p);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const T *ri = r + (3 * i + c) * NR;
+#pragma unroll
+      for (int q = 0; q < NR; ++q) {
+        const T rv = ri[q];
+#pragma unroll
+        for (int j = 0; j < 6; ++j) acc[j][q] += (T)p[6 * c + j] * rv;
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < 6; ++j)
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+#pragma unroll
+      for (int o = G / 2; o > 0; o >>= 1) acc[j][q] += __shfl_xor(acc[j][q], o, 64);
+    }
+  if (on && lane == 0) {
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+      const int64_t i = A * 6 + j;
+#pragma unroll
+      for (int q = 0; q < NR; ++q) {
+        const int64_t ii = i * NR + q;
+        bc[ii] = acc[j][q];
+        if (dinv) {
+          const T d = (T)it * dinv[i] * acc[j][q];
+          rc[ii] = acc[j][q];
+          dc[ii] = d;
+          xc[ii] = d;
+        }
+      }
+    }
+  }
+}
+// one thread per node u (all NR right-hand sides): x_u += sum over its entry
+// slots of P (ca_q xa + cb_q xb)_{aggregate}, ca_q = sc[8 q + ia] (index < 0:
+// 1); P from the node-sliced copy (FP16: its three planes)
+template <typename T, typename S, int NR>
+__global__ __launch_bounds__(TPB) void k_mr_sa_prolong_comb(int64_t nu, int64_t ne, const int64_t *__restrict__ poff,
+                                                            const int32_t *__restrict__ pw,
+                                                            const int32_t *__restrict__ eagg,
+                                                            const S *__restrict__ Pe, const double *__restrict__ sc,
+                                                            int ia, int ib, const T *__restrict__ xa,
+                                                            const T *__restrict__ xb, T *__restrict__ x) {
+  const int64_t u = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  if (u >= nu) return;
+  T ca[NR], cb[NR];
+#pragma unroll
+  for (int q = 0; q < NR; ++q) {
+    ca[q] = ia < 0 ? T(1) : (T)sc[8 * q + ia];
+    cb[q] = ib < 0 ? T(1) : (T)sc[8 * q + ib];
+  }
+  const int64_t w = u / UFT_ELL_SLICE, lane = u % UFT_ELL_SLICE;
+  const int nq = pw[w];
+  T acc[3][NR];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int q = 0; q < NR; ++q) acc[i][q] = T(0);
+  for (int qq = 0; qq < nq; ++qq) {
+    const int64_t sl = poff[w] + (int64_t)qq * UFT_ELL_SLICE + lane;
+    const int64_t A = eagg[sl];
+    if (A < 0) break;  // padding follows the node's entries
+    float p[18];
+    if constexpr (std::is_same<S, __half>::value) {
+      const uint4 lo = reinterpret_cast<const uint4 *>(Pe)[sl], hi = reinterpret_cast<const uint4 *>(Pe)[ne + sl];
+      const uint32_t tl = reinterpret_cast<const uint32_t *>(Pe)[8 * ne + sl];
+      const uint32_t wds[9] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w, tl};
+#pragma unroll
+      for (int h = 0; h < 9; ++h) {
+        const float2 v = unpack_h2(wds[h]);
+        p[2 * h] = v.x;
+        p[2 * h + 1] = v.y;
+      }
+    } else {
+      const S *Pq = Pe + poff[w] * 18 + (int64_t)qq * 18 * UFT_ELL_SLICE + lane;
+#pragma unroll
+      for (int v = 0; v < 18; ++v) p[v] = (float)Pq[v * UFT_ELL_SLICE];
+    }
+    const T *xa6 = xa + A * 6 * NR, *xb6 = xb ? xb + A * 6 * NR : nullptr;
+#pragma unroll
+    for (int j = 0; j < 6; ++j)
+#pragma unroll
+      for (int q = 0; q < NR; ++q) {
+        const T c = xb6 ? ca[q] * xa6[j * NR + q] + cb[q] * xb6[j * NR + q] : xa6[j * NR + q];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) acc[i][q] += (T)p[i * 6 + j] * c;
+      }
+  }
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int q = 0; q < NR; ++q) x[(u * 3 + i) * NR + q] += acc[i][q];
+}
+
 }  // namespace
 
 namespace uftk {
@@ -551,6 +692,46 @@ void sa_prolong_comb(hipStream_t s, const AmgLevel &L0, const double *sc, int ia
   k_sa_prolong_comb<double, double><<<nblk(L0.nunit), TPB, 0, s>>>(L0.nunit, L0.d_sa_poff, L0.d_sa_pw, L0.d_sa_eagg,
                                                                    L0.d_sa_Pe, sc, ia, ib, xa, xb, x);
 }
+#define UFT_SA_NR(nr, CALL)                        \
+  switch (nr) {                                    \
+    case 2: { constexpr int NR = 2; CALL; } break; \
+    case 4: { constexpr int NR = 4; CALL; } break; \
+    default: { constexpr int NR = 8; CALL; } break; \
+  }
+void mr_sa_restrict_first(hipStream_t s, int nr, const AmgLevel &L0, int64_t nc, const double *r, double *bc,
+                          const double *dinv, double it, double *rc, double *dc, double *xc) {
+  UFT_SA_NR(nr, (k_mr_sa_restrict_first<16, double, double, NR><<<nblk(nc * 16), TPB, 0, s>>>(
+                    nc, L0.d_sa_sptr, L0.d_sa_snode, L0.d_sa_Ps, r, bc, dinv, it, rc, dc, xc)));
+}
+void mr_sa_restrict_first(hipStream_t s, int nr, const AmgLevel &L0, int64_t nc, const float *r, float *bc,
+                          const float *dinv, double it, float *rc, float *dc, float *xc) {
+  if (L0.d_sa_Psh) {
+    UFT_SA_NR(nr, (k_mr_sa_restrict_first<16, float, __half, NR><<<nblk(nc * 16), TPB, 0, s>>>(
+                      nc, L0.d_sa_sptr, L0.d_sa_snode, (const __half *)L0.d_sa_Psh, r, bc, dinv, it, rc, dc, xc)));
+  } else {
+    UFT_SA_NR(nr, (k_mr_sa_restrict_first<16, float, float, NR><<<nblk(nc * 16), TPB, 0, s>>>(
+                      nc, L0.d_sa_sptr, L0.d_sa_snode, L0.d_sa_Psf, r, bc, dinv, it, rc, dc, xc)));
+  }
+}
+void mr_sa_prolong_comb(hipStream_t s, int nr, const AmgLevel &L0, const double *sc, int ia, int ib, const double *xa,
+                        const double *xb, double *x) {
+  UFT_SA_NR(nr, (k_mr_sa_prolong_comb<double, double, NR><<<nblk(L0.nunit), TPB, 0, s>>>(
+                    L0.nunit, L0.sa_ne_ell, L0.d_sa_poff, L0.d_sa_pw, L0.d_sa_eagg, L0.d_sa_Pe, sc, ia, ib, xa, xb, x)));
+}
+void mr_sa_prolong_comb(hipStream_t s, int nr, const AmgLevel &L0, const double *sc, int ia, int ib, const float *xa,
+                        const float *xb, float *x) {
+  if (L0.d_sa_Peh) {
+    UFT_SA_NR(nr, (k_mr_sa_prolong_comb<float, __half, NR><<<nblk(L0.nunit), TPB, 0, s>>>(
+                      L0.nunit, L0.sa_ne_ell, L0.d_sa_poff, L0.d_sa_pw, L0.d_sa_eagg, (const __half *)L0.d_sa_Peh, sc,
+                      ia, ib, xa, xb, x)));
+  } else {
+    UFT_SA_NR(nr, (k_mr_sa_prolong_comb<float, float, NR><<<nblk(L0.nunit), TPB, 0, s>>>(
+                      L0.nunit, L0.sa_ne_ell, L0.d_sa_poff, L0.d_sa_pw, L0.d_sa_eagg, L0.d_sa_Pef, sc, ia, ib, xa, xb,
+                      x)));
+  }
+}
+#undef UFT_SA_NR
+
 void sa_prolong_comb(hipStream_t s, const AmgLevel &L0, const double *sc, int ia, int ib, const float *xa,
                      const float *xb, float *x) {
   if (L0.d_sa_Peh)

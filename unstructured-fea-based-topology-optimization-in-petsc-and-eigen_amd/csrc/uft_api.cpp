@@ -760,6 +760,14 @@ void mr_prolong_from(uft_ctx *C, int l, T *x) {
   AmgLevel &Lv = C->amg.L[l];
   MLV<T> vn = mlv<T>(C, l + 1);
   const int nr = C->mr.nr;
+  if (Lv.sa) {  // level 0's smoothed prolongator, node by node
+    switch (level_kind(C, l + 1)) {
+      case 0: uftk::mr_sa_prolong_comb(C->s, nr, Lv, nullptr, -1, -1, vn.x, (const T *)nullptr, x); break;
+      case 1: uftk::mr_sa_prolong_comb(C->s, nr, Lv, nullptr, -1, -1, vn.x, vn.kd, x); break;
+      default: uftk::mr_sa_prolong_comb(C->s, nr, Lv, mr_ksc(C, l + 1), 5, 6, vn.kc, vn.kd, x); break;
+    }
+    return;
+  }
   const T *P = tr_P(Lv, (T *)nullptr);
   switch (level_kind(C, l + 1)) {
     case 0:
@@ -816,8 +824,11 @@ void mr_cycle_body(uft_ctx *C, int l, const T *b, T *x) {
     MLV<T> vn = mlv<T>(C, l + 1);
     const double itn = cheb_coef(Ln.lmax, level_degree(C, l + 1), lohi).inv_theta;
     T *xn = level_kind(C, l + 1) == 2 ? vn.kc : vn.x;
-    uftk::mr_restrict_first(C->s, nr, Lv.bs, Ln.nunit, Lv.d_cptr, Lv.d_child, tr_P(Lv, (T *)nullptr), v.r, vn.b,
-                            vn.dinv, itn, vn.r, vn.d, xn);
+    if (Lv.sa)  // smoothed prolongator of level 0: through the aggregates' supports
+      uftk::mr_sa_restrict_first(C->s, nr, Lv, Ln.nunit, v.r, vn.b, vn.dinv, itn, vn.r, vn.d, xn);
+    else
+      uftk::mr_restrict_first(C->s, nr, Lv.bs, Ln.nunit, Lv.d_cptr, Lv.d_child, tr_P(Lv, (T *)nullptr), v.r, vn.b,
+                              vn.dinv, itn, vn.r, vn.d, xn);
     mr_coarse_body<T>(C, l + 1);
     mr_prolong_from<T>(C, l, x);
   }
@@ -2383,8 +2394,7 @@ int uft_setup(uft_ctx *C) {
     // the assembled node pattern (its FP64 blocks are the operator that smooths
     // the prolongator) and at least three levels (the level-1 correction is not
     // the dense coarsest solve)
-    const bool want_sa = C->P.prolong_smooth && !B.nseg && !(C->P.load_cases == 1 && C->nlc > 1) &&
-                         C->fell.nslot > 0;
+    const bool want_sa = C->P.prolong_smooth && !B.nseg && C->fell.nslot > 0;
     if ((rc = uft_amg_build_host(C, H, nun, want_sa))) return rc;
     if (want_sa && nun.size() < 3) {
       H.clear();

@@ -634,6 +634,16 @@ void sa_prolong_comb(hipStream_t s, const AmgLevel &L0, const double *sc, int ia
                      const double *xb, double *x);
 void sa_prolong_comb(hipStream_t s, const AmgLevel &L0, const double *sc, int ia, int ib, const float *xa,
                      const float *xb, float *x);
+// the same transfers for nr interleaved right-hand sides (the multi-load-case
+// form; per right-hand-side coefficients at sc[8 q + ia])
+void mr_sa_restrict_first(hipStream_t s, int nr, const AmgLevel &L0, int64_t nc, const double *r, double *bc,
+                          const double *dinv, double it, double *rc, double *dc, double *xc);
+void mr_sa_restrict_first(hipStream_t s, int nr, const AmgLevel &L0, int64_t nc, const float *r, float *bc,
+                          const float *dinv, double it, float *rc, float *dc, float *xc);
+void mr_sa_prolong_comb(hipStream_t s, int nr, const AmgLevel &L0, const double *sc, int ia, int ib, const double *xa,
+                        const double *xb, double *x);
+void mr_sa_prolong_comb(hipStream_t s, int nr, const AmgLevel &L0, const double *sc, int ia, int ib, const float *xa,
+                        const float *xb, float *x);
 void fine_ell_op(hipStream_t s, const uft_ctx *C, const float *u, const RowOp<float> &op);
 // the same pattern with FP32 entries (unscaled): the mixed forms' outer operator
 void fine_ell_op32(hipStream_t s, const uft_ctx *C, const float *u, const RowOp<float> &op);
