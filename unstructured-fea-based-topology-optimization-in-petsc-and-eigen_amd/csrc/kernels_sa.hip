@@ -363,11 +363,12 @@ __global__ __launch_bounds__(TPB) void k_sa_prolong_comb(int64_t nu, const int64
                                                          const int32_t *__restrict__ eagg, const S *__restrict__ Pe,
                                                          const double *__restrict__ sc, int ia, int ib,
                                                          const T *__restrict__ xa, const T *__restrict__ xb,
-                                                         T *__restrict__ x) {
+                                                         T *__restrict__ x, const int32_t *__restrict__ ndes) {
   const int64_t u = (int64_t)blockIdx.x * TPB + threadIdx.x;
   if (u >= nu) return;
-  const T ca = ia < 0 ? T(1) : (T)sc[ia];
-  const T cb = ib < 0 ? T(1) : (T)sc[ib];
+  const double *scu = ndes ? sc + (size_t)ndes[u] * UFT_BSC : sc;  // a batch: the node's design's scalars
+  const T ca = ia < 0 ? T(1) : (T)scu[ia];
+  const T cb = ib < 0 ? T(1) : (T)scu[ib];
   const int64_t w = u / UFT_ELL_SLICE, lane = u % UFT_ELL_SLICE;
   const int nq = pw[w];
   T acc[3] = {T(0), T(0), T(0)};
@@ -451,11 +452,12 @@ __global__ __launch_bounds__(TPB) void k_sa_prolong_comb_h(int64_t nu, int64_t n
                                                            const uint32_t *__restrict__ P2,
                                                            const double *__restrict__ sc, int ia, int ib,
                                                            const float *__restrict__ xa, const float *__restrict__ xb,
-                                                           float *__restrict__ x) {
+                                                           float *__restrict__ x, const int32_t *__restrict__ ndes) {
   const int64_t u = (int64_t)blockIdx.x * TPB + threadIdx.x;
   if (u >= nu) return;
-  const float ca = ia < 0 ? 1.f : (float)sc[ia];
-  const float cb = ib < 0 ? 1.f : (float)sc[ib];
+  const double *scu = ndes ? sc + (size_t)ndes[u] * UFT_BSC : sc;  // a batch: the node's design's scalars
+  const float ca = ia < 0 ? 1.f : (float)scu[ia];
+  const float cb = ib < 0 ? 1.f : (float)scu[ib];
   const int64_t w = u / UFT_ELL_SLICE;
   const int nq = pw[w];
   const int64_t s0 = poff[w] + u % UFT_ELL_SLICE;
@@ -688,9 +690,9 @@ void sa_restrict_first(hipStream_t s, const AmgLevel &L0, int64_t nc, const floa
                                                                         bc, dinv, it, rc, dc, xc);
 }
 void sa_prolong_comb(hipStream_t s, const AmgLevel &L0, const double *sc, int ia, int ib, const double *xa,
-                     const double *xb, double *x) {
+                     const double *xb, double *x, const int32_t *ndes) {
   k_sa_prolong_comb<double, double><<<nblk(L0.nunit), TPB, 0, s>>>(L0.nunit, L0.d_sa_poff, L0.d_sa_pw, L0.d_sa_eagg,
-                                                                   L0.d_sa_Pe, sc, ia, ib, xa, xb, x);
+                                                                   L0.d_sa_Pe, sc, ia, ib, xa, xb, x, ndes);
 }
 #define UFT_SA_NR(nr, CALL)                        \
   switch (nr) {                                    \
@@ -733,14 +735,14 @@ void mr_sa_prolong_comb(hipStream_t s, int nr, const AmgLevel &L0, const double 
 #undef UFT_SA_NR
 
 void sa_prolong_comb(hipStream_t s, const AmgLevel &L0, const double *sc, int ia, int ib, const float *xa,
-                     const float *xb, float *x) {
+                     const float *xb, float *x, const int32_t *ndes) {
   if (L0.d_sa_Peh)
     k_sa_prolong_comb_h<<<nblk(L0.nunit), TPB, 0, s>>>(
         L0.nunit, L0.sa_ne_ell, L0.d_sa_poff, L0.d_sa_pw, L0.d_sa_eagg, (const uint4 *)L0.d_sa_Peh,
-        (const uint32_t *)L0.d_sa_Peh + 8 * L0.sa_ne_ell, sc, ia, ib, xa, xb, x);
+        (const uint32_t *)L0.d_sa_Peh + 8 * L0.sa_ne_ell, sc, ia, ib, xa, xb, x, ndes);
   else
     k_sa_prolong_comb<float, float><<<nblk(L0.nunit), TPB, 0, s>>>(L0.nunit, L0.d_sa_poff, L0.d_sa_pw, L0.d_sa_eagg,
-                                                                   L0.d_sa_Pef, sc, ia, ib, xa, xb, x);
+                                                                   L0.d_sa_Pef, sc, ia, ib, xa, xb, x, ndes);
 }
 
 }  // namespace uftk

@@ -473,7 +473,13 @@ void prolong_from(uft_ctx *C, int l, T *x) {
     switch (level_kind(C, l + 1)) {
       case 0: uftk::sa_prolong_comb(C->s, Lv, nullptr, -1, -1, vn.x, (const T *)nullptr, x); break;
       case 1: uftk::sa_prolong_comb(C->s, Lv, nullptr, -1, -1, vn.x, vn.kd, x); break;
-      default: uftk::sa_prolong_comb(C->s, Lv, C->d_scal + S_K + 8 * (l + 1), 5, 6, vn.kc, vn.kd, x); break;
+      default:
+        if (C->bt.nseg)  // each node with its design's K-cycle coefficients
+          uftk::sa_prolong_comb(C->s, Lv, C->bt.sc, B_KC + 8 * l + 5, B_KC + 8 * l + 6, vn.kc, vn.kd, x,
+                                C->bt.d_ndes);
+        else
+          uftk::sa_prolong_comb(C->s, Lv, C->d_scal + S_K + 8 * (l + 1), 5, 6, vn.kc, vn.kd, x);
+        break;
     }
     return;
   }
@@ -1157,6 +1163,37 @@ int amg_refresh_impl(uft_ctx *C, const double *xphys) {
   uftk::diag_K(s, C, xphys, H.L[0].d_dinv, H.L[0].d_dinvf);
   const bool par = C->side[0] && !C->bt.nseg;
   const int ks = std::min(64, std::max(2, C->P.esteig_steps));
+  if (H.L[0].sa && C->bt.nseg) {
+    // design batch with smoothed aggregation: the fine blocks, the fine
+    // level's per-design lambda_max (folded into D^-1, so omega = 4 cheb_hi / 3
+    // on the folded D^-1 is each design's 4 / (3 lambda_max)), the smoothed
+    // prolongator and P^T A P, then the batch's coarse levels
+    BatchState &B = C->bt;
+    AmgLevel &L0 = H.L[0];
+    uftk::simp_moduli(s, C, xphys, C->d_Ee);
+    uftk::fine_ell_fill(s, C, C->d_Ee, L0.d_dinv, C->d_a64, C->d_a32);
+    if (int rc = estimate_lmax_batch(C, 0, ks)) return rc;
+    L0.sa_omega = 4.0 * C->P.cheb_hi / 3.0;
+    uftk::sa_refresh(s, C);
+    for (int l = 1; l < H.nlev; ++l) {
+      AmgLevel &Lv = H.L[l - 1], &Ln = H.L[l];
+      if (l >= 2)
+        uftk::galerkin_coarse(s, Ln.nblk, Lv.d_glptr, Lv.d_gl, Lv.d_brow, Lv.d_ci, Lv.d_P, Lv.d_val, Ln.d_val);
+      uftk::fixed_unit_diag(s, Ln.nunit, Ln.d_rp, Ln.d_ci, Ln.d_val);
+      uftk::bsr6_dinv(s, Ln.nunit, Ln.d_rp, Ln.d_ci, Ln.d_val, Ln.d_dinv, Ln.d_dinvf);
+      if (Ln.d_valf) uftk::d2f(s, Ln.nblk * 36, Ln.d_val, Ln.d_valf);
+      if (Ln.d_valh)
+        uftk::bsr6_to_half(s, Ln.nblk, Ln.nunit, Ln.d_brow, Ln.d_ci, Ln.d_val, Ln.d_dinv, Ln.d_valh, Ln.d_dh);
+    }
+    AmgLevel &Lc = H.L[H.nlev - 1];
+    uftk::seg_bsr_to_dense(s, Lc.nunit, Lc.d_rp, Lc.d_ci, Lc.d_val, B.d_cdes, B.d_cu, B.d_coff, H.d_cmat);
+    uftk::seg_dense_inverse(s, B.nseg, B.d_cu, B.d_coff, H.d_cmat, H.d_cinv);
+    for (int l = 1; l + 1 < H.nlev; ++l) {
+      int rc = estimate_lmax_batch(C, l, ks);
+      if (rc) return rc;
+    }
+    return UFT_OK;
+  }
   if (H.L[0].sa) {
     // smoothed aggregation: the fine blocks (FP64, and the mixed forms'
     // smoother and outer blocks), the fine level's lambda_max (omega needs
@@ -2394,7 +2431,7 @@ int uft_setup(uft_ctx *C) {
     // the assembled node pattern (its FP64 blocks are the operator that smooths
     // the prolongator) and at least three levels (the level-1 correction is not
     // the dense coarsest solve)
-    const bool want_sa = C->P.prolong_smooth && !B.nseg && C->fell.nslot > 0;
+    const bool want_sa = C->P.prolong_smooth && C->fell.nslot > 0;
     if ((rc = uft_amg_build_host(C, H, nun, want_sa))) return rc;
     if (want_sa && nun.size() < 3) {
       H.clear();

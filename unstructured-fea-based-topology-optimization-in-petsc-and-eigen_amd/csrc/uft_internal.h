@@ -631,9 +631,9 @@ void sa_restrict_first(hipStream_t s, const AmgLevel &L0, int64_t nc, const doub
 void sa_restrict_first(hipStream_t s, const AmgLevel &L0, int64_t nc, const float *r, float *bc, const float *dinv,
                        double it, float *rc, float *dc, float *xc);
 void sa_prolong_comb(hipStream_t s, const AmgLevel &L0, const double *sc, int ia, int ib, const double *xa,
-                     const double *xb, double *x);
+                     const double *xb, double *x, const int32_t *ndes = nullptr);
 void sa_prolong_comb(hipStream_t s, const AmgLevel &L0, const double *sc, int ia, int ib, const float *xa,
-                     const float *xb, float *x);
+                     const float *xb, float *x, const int32_t *ndes = nullptr);
 // the same transfers for nr interleaved right-hand sides (the multi-load-case
 // form; per right-hand-side coefficients at sc[8 q + ia])
 void mr_sa_restrict_first(hipStream_t s, int nr, const AmgLevel &L0, int64_t nc, const double *r, double *bc,
