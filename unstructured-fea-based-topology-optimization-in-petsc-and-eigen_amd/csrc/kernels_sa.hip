@@ -572,6 +572,20 @@ __global__ __launch_bounds__(TPB) void k_mr_sa_restrict_first(int64_t nc, const 
     }
   }
 }
+// c = ca_q xa + cb_q xb on a coarse vector of NR interleaved right-hand sides
+// (ca_q = sc[8 q + ia], index < 0: 1), ahead of the prolongation, so that it
+// gathers one coarse vector per entry instead of two
+template <typename T, int NR>
+__global__ __launch_bounds__(TPB) void k_mr_lincomb2(int64_t n, const double *__restrict__ sc, int ia, int ib,
+                                                     const T *__restrict__ xa, const T *__restrict__ xb,
+                                                     T *__restrict__ c) {
+  const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  if (t >= n * NR) return;
+  const int q = (int)(t % NR);
+  const T ca = ia < 0 ? T(1) : (T)sc[8 * q + ia];
+  const T cb = ib < 0 ? T(1) : (T)sc[8 * q + ib];
+  c[t] = ca * xa[t] + cb * xb[t];
+}
 // one thread per node u (all NR right-hand sides): x_u += sum over its entry
 // slots of P (ca_q xa + cb_q xb)_{aggregate}, ca_q = sc[8 q + ia] (index < 0:
 // 1); P from the node-sliced copy (FP16: its three planes)
@@ -715,13 +729,27 @@ void mr_sa_restrict_first(hipStream_t s, int nr, const AmgLevel &L0, int64_t nc,
                       nc, L0.d_sa_sptr, L0.d_sa_snode, L0.d_sa_Psf, r, bc, dinv, it, rc, dc, xc)));
   }
 }
-void mr_sa_prolong_comb(hipStream_t s, int nr, const AmgLevel &L0, const double *sc, int ia, int ib, const double *xa,
-                        const double *xb, double *x) {
+void mr_sa_prolong_comb(hipStream_t s, int nr, const AmgLevel &L0, int64_t ncu, const double *sc, int ia, int ib,
+                        const double *xa, const double *xb, double *x, double *tmp) {
+  if (xb) {  // combine first (tmp: a free coarse vector of nr right-hand sides)
+    const int64_t nc = 6 * ncu;
+    UFT_SA_NR(nr, (k_mr_lincomb2<double, NR><<<nblk(nc * NR), TPB, 0, s>>>(nc, sc, ia, ib, xa, xb, tmp)));
+    xa = tmp;
+    xb = nullptr;
+    ia = ib = -1;
+  }
   UFT_SA_NR(nr, (k_mr_sa_prolong_comb<double, double, NR><<<nblk(L0.nunit), TPB, 0, s>>>(
                     L0.nunit, L0.sa_ne_ell, L0.d_sa_poff, L0.d_sa_pw, L0.d_sa_eagg, L0.d_sa_Pe, sc, ia, ib, xa, xb, x)));
 }
-void mr_sa_prolong_comb(hipStream_t s, int nr, const AmgLevel &L0, const double *sc, int ia, int ib, const float *xa,
-                        const float *xb, float *x) {
+void mr_sa_prolong_comb(hipStream_t s, int nr, const AmgLevel &L0, int64_t ncu, const double *sc, int ia, int ib,
+                        const float *xa, const float *xb, float *x, float *tmp) {
+  if (xb) {
+    const int64_t nc = 6 * ncu;
+    UFT_SA_NR(nr, (k_mr_lincomb2<float, NR><<<nblk(nc * NR), TPB, 0, s>>>(nc, sc, ia, ib, xa, xb, tmp)));
+    xa = tmp;
+    xb = nullptr;
+    ia = ib = -1;
+  }
   if (L0.d_sa_Peh) {
     UFT_SA_NR(nr, (k_mr_sa_prolong_comb<float, __half, NR><<<nblk(L0.nunit), TPB, 0, s>>>(
                       L0.nunit, L0.sa_ne_ell, L0.d_sa_poff, L0.d_sa_pw, L0.d_sa_eagg, (const __half *)L0.d_sa_Peh, sc,

@@ -763,14 +763,16 @@ void mr_coarse_body(uft_ctx *C, int l) {
 
 template <typename T>
 void mr_prolong_from(uft_ctx *C, int l, T *x) {
-  AmgLevel &Lv = C->amg.L[l];
+  AmgLevel &Lv = C->amg.L[l], &Ln = C->amg.L[l + 1];
   MLV<T> vn = mlv<T>(C, l + 1);
   const int nr = C->mr.nr;
   if (Lv.sa) {  // level 0's smoothed prolongator, node by node
     switch (level_kind(C, l + 1)) {
-      case 0: uftk::mr_sa_prolong_comb(C->s, nr, Lv, nullptr, -1, -1, vn.x, (const T *)nullptr, x); break;
-      case 1: uftk::mr_sa_prolong_comb(C->s, nr, Lv, nullptr, -1, -1, vn.x, vn.kd, x); break;
-      default: uftk::mr_sa_prolong_comb(C->s, nr, Lv, mr_ksc(C, l + 1), 5, 6, vn.kc, vn.kd, x); break;
+      case 0:
+        uftk::mr_sa_prolong_comb(C->s, nr, Lv, Ln.nunit, nullptr, -1, -1, vn.x, (const T *)nullptr, x, vn.t);
+        break;
+      case 1: uftk::mr_sa_prolong_comb(C->s, nr, Lv, Ln.nunit, nullptr, -1, -1, vn.x, vn.kd, x, vn.t); break;
+      default: uftk::mr_sa_prolong_comb(C->s, nr, Lv, Ln.nunit, mr_ksc(C, l + 1), 5, 6, vn.kc, vn.kd, x, vn.t); break;
     }
     return;
   }

@@ -640,10 +640,12 @@ void mr_sa_restrict_first(hipStream_t s, int nr, const AmgLevel &L0, int64_t nc,
                           const double *dinv, double it, double *rc, double *dc, double *xc);
 void mr_sa_restrict_first(hipStream_t s, int nr, const AmgLevel &L0, int64_t nc, const float *r, float *bc,
                           const float *dinv, double it, float *rc, float *dc, float *xc);
-void mr_sa_prolong_comb(hipStream_t s, int nr, const AmgLevel &L0, const double *sc, int ia, int ib, const double *xa,
-                        const double *xb, double *x);
-void mr_sa_prolong_comb(hipStream_t s, int nr, const AmgLevel &L0, const double *sc, int ia, int ib, const float *xa,
-                        const float *xb, float *x);
+// (ncu: units of level 1; with xb the combination goes through tmp, a free
+// level-1 vector of nr right-hand sides)
+void mr_sa_prolong_comb(hipStream_t s, int nr, const AmgLevel &L0, int64_t ncu, const double *sc, int ia, int ib,
+                        const double *xa, const double *xb, double *x, double *tmp);
+void mr_sa_prolong_comb(hipStream_t s, int nr, const AmgLevel &L0, int64_t ncu, const double *sc, int ia, int ib,
+                        const float *xa, const float *xb, float *x, float *tmp);
 void fine_ell_op(hipStream_t s, const uft_ctx *C, const float *u, const RowOp<float> &op);
 // the same pattern with FP32 entries (unscaled): the mixed forms' outer operator
 void fine_ell_op32(hipStream_t s, const uft_ctx *C, const float *u, const RowOp<float> &op);
