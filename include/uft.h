@@ -103,9 +103,9 @@ typedef struct uft_params {
   int prolong_smooth;  /* 1 (default): smoothed aggregation on the fine level as PCGAMG does (SolveFem.cpp:129-136):
                           at every hierarchy refresh the tentative prolongator is smoothed once with the current
                           operator, P = (I - omega D^-1 A) P_tent with omega = 4 / (3 lambda_max(D^-1 A)), and the
-                          level-1 operator is P^T A P; used for one design with superposed load cases, the
-                          assembled node pattern (fine_assembled = 1) and at least three levels, otherwise the
-                          tentative prolongator.  0: the tentative prolongator.  Fixed at setup. */
+                          level-1 operator is P^T A P (per design in a batch); used with the assembled node pattern
+                          (fine_assembled = 1) and at least three levels, otherwise the tentative prolongator.
+                          0: the tentative prolongator.  Fixed at setup. */
   int coarse_lag;      /* one design with a coarsest level of 512 unknowns or more: 1 (default) the cycle applies
                           the dense inverse of the previous hierarchy refresh's coarsest operator while the current
                           one is inverted on a stream of its own, beside the solve (the first refresh inverts its
