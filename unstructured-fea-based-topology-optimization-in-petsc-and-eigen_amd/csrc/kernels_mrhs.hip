@@ -358,7 +358,7 @@ __global__ __launch_bounds__(TPB) void k_mr_ell_op(int nn, const int64_t *__rest
     const int64_t base = off[w] + n % UFT_ELL_SLICE;
     const int W = width[w];
     float a0 = 0.f, a1 = 0.f, a2 = 0.f;
-#pragma unroll 2
+#pragma unroll 4
     for (int s = 0; s < W; ++s) {
       const int64_t k = base + (int64_t)s * UFT_ELL_SLICE;
       const int64_t j = std::is_same<CI, int16_t>::value ? n + (int64_t)col[k] : (int64_t)col[k];
