@@ -178,6 +178,7 @@ def run_form(args, precision):
         setup_s = time.time() - t0
     ne, ndof = ctx.size("ne"), ctx.size("ndof")
     ell = ctx.fine_operator_info()
+    tiles = ctx.fine_tiles_info()
     ctx.init_design()
     itr = 0
     for _ in range(args.warmup):
@@ -206,15 +207,20 @@ def run_form(args, precision):
     solve_bytes, solve_ms, applies, solves = ctx.solve_traffic()
     L.uft_profile_enable(ctx.h, 0)
     # back-to-back launches of the fine operators on the last refresh's data
-    # (HIP events on the context's stream; one design, one load case)
+    # (HIP events on the context's stream; one design; with load cases all
+    # right-hand sides at once, and the FP64 element phase alone not timed)
     b2b = {}
-    if ndesign == 1 and WORKLOADS[args.workload][5] != 2:
-        kinds = {"fp64": (0, 3), "mixed": (1,), "mixed_h": (2, 1)}[precision]
+    if ndesign == 1:
+        if WORKLOADS[args.workload][5] == 2:
+            kinds = {"fp64": (), "mixed": (1,), "mixed_h": (2, 1)}[precision]
+        else:
+            kinds = {"fp64": (0, 3), "mixed": (1,), "mixed_h": (2, 1)}[precision]
         for k in kinds:
             b2b[k] = ctx.time_fine_operator(k, 20)
     ctx.close()
     return {"npe": npe, "dims": dims, "seed": seed, "ne": ne, "ndof": ndof, "setup_s": setup_s, "dt": t1 - t0,
             "its": its, "fxs": fxs, "prof": prof, "b2b": b2b, "ndesign": ndesign, "per_design": per_design, "ell": ell,
+            "tiles": tiles,
             "solve": {"algorithmic_bytes": solve_bytes, "ms": solve_ms, "operator_applies": applies,
                       "solves": solves, "gbs": solve_bytes / solve_ms / 1e6 if solve_ms > 0 else None,
                       "ms_per_step": solve_ms / max(solves, 1)}}
@@ -386,7 +392,11 @@ def main():
         else:
             w, kname = 4, f"k_elem_apply_v<{npe},float> (FP32 element phase, mixed form)"
         if r["ell"][0]:  # assembled fine smoother operator: one kernel per apply
-            kn = f"k_mr_ell_op<{nrhs},FP16> (8 load cases)" if multi else "k_fine_ell_op_q"
+            if multi:
+                kn = ("k_mr_tile_op<FP16> (8 load cases on node tiles, neighbour entries staged in LDS)"
+                      if r["tiles"][0] else f"k_mr_ell_op<{nrhs},FP16> (8 load cases)")
+            else:
+                kn = "k_fine_ell_op_q"
             fmt = "FP16" if r["ell"][3] == 18 else "int8 (FP32 block scale)"
             rooflines.append(elem_roofline(
                 main_form, npe, ne, ndof, r["prof"][2], r["prof"][3], 2,
@@ -398,7 +408,11 @@ def main():
             rooflines.append(elem_roofline(main_form, npe, ne, ndof, r["prof"][2], r["prof"][3], w, kname,
                                            f"{args.workload}_{main_form}_elem_apply", nrhs))
         if r["prof"][4][0] and r["ell"][0]:  # FP32 outer operator from the assembled blocks
-            kn = f"k_mr_ell_op<{nrhs},FP32> (8 load cases)" if multi else "k_fine_ell_op32"
+            if multi:
+                kn = ("k_mr_tile_op<FP32> (8 load cases on node tiles, neighbour entries staged in LDS)"
+                      if r["tiles"][0] else f"k_mr_ell_node<{nrhs},FP32> (8 load cases)")
+            else:
+                kn = "k_fine_ell_op32"
             rooflines.append(elem_roofline(
                 "outer", npe, ne, ndof, r["prof"][4], r["prof"][5], 4,
                 f"{kn} (assembled FP32 3x3 node blocks of the outer operator, whole apply, mixed form)",
@@ -446,6 +460,8 @@ def main():
     # levels, the coarsest dense inverse; transfers and vector updates not counted) over the solves'
     # wall time (hierarchy refresh excluded)
     config["solve_gbs"] = {f: res[f]["solve"] for f in forms}
+    if r["tiles"][0]:
+        config["node_tiles"] = {"tiles": r["tiles"][0], "slots": r["tiles"][1], "max_halo_nodes": r["tiles"][2]}
     if "fp64" in res and main_form != "fp64":
         config["fp64_krylov_iters_per_step"] = res["fp64"]["its"]
     if "mixed" in res and main_form == "mixed_h":

@@ -250,6 +250,12 @@ int uft_solver_info(const uft_ctx *ctx, int *nlev, int64_t *n_per_level, int64_t
  * one right-hand side; 18: FP16, the multi-load-case form; the mixed outer
  * product's blocks are FP32); all 0 when the fine level uses element blocks. */
 int uft_fine_operator_info(const uft_ctx *ctx, int64_t *slots, int64_t *blocks, int *col_bytes, int *value_bytes);
+/* Node tiles of the eight-right-hand-side fine products (fine_assembled = 1
+ * with 5-8 load cases; no counterpart in the reference): number of tiles (of
+ * at most 256 nodes, one workgroup each), slots of their tile-ordered
+ * sliced-ELL storage and the largest halo (neighbour nodes of one tile,
+ * staged in LDS, at most 840); all 0 when the node-ordered kernels run. */
+int uft_fine_tiles_info(const uft_ctx *ctx, int64_t *tiles, int64_t *slots, int *max_halo);
 /* --- multi-load-case form (load_cases = 1; no counterpart in the reference,
  * which superposes its cases) --------------------------------------------- */
 /* Number of load cases; per case the Krylov iterations and final relative
@@ -277,7 +283,8 @@ int uft_level_apply(uft_ctx *ctx, int level, int kind, const double *x, double *
  * (element and node phase), 1 the mixed forms' FP32 outer product, 2 the
  * UFT_MIXED_H smoother's operator with its Chebyshev-step epilogue, 3 the
  * FP64 element phase alone. Kind 0 needs the FP64 form, kinds 1 and 2 a
- * mixed form; one design, one load case. */
+ * mixed form; one design.  With several load cases (load_cases = 1) kinds
+ * 0-2 apply to all right-hand sides at once (kind 3 is not available). */
 int uft_time_fine_operator(uft_ctx *ctx, int kind, int reps, double *us);
 
 /* --- MMA settings, restart and KKT residual (MMA.h:8-52) ------------------ */

@@ -109,3 +109,35 @@ def test_three_cases_padded(precision):
     assert len(stats) == 3 and all(rr <= 1e-12 for _, rr in stats)
     u1 = blk.load_case_u(1)
     assert np.abs(u1 - one.get("U")).max() <= 1e-9 * np.abs(u1).max()
+
+
+def test_node_tiles_match_node_order():
+    """The eight-case fine products on node tiles (halo entries staged in
+    LDS, fine_assembled = 1) against the node-ordered kernels
+    (fine_assembled = 2) on a mesh of many tiles: the same FP16 / FP32 block
+    products per node in the same slot order, so the operator agrees to FP32
+    rounding and a whole mixed solve to its tolerance."""
+    from uft.capi import cantilever
+    print("[case] node tiles vs node order, 24x12x12 hex8, 8 cases", flush=True)
+    prm = dict(PRM, precision=2, rtol_el=1e-10, rtol_f=1e-10)
+    tiled = cantilever(8, 24, 12, 12, jitter=0.15, seed=21, load_kind=2, fine_assembled=1, **prm)
+    plain = cantilever(8, 24, 12, 12, jitter=0.15, seed=21, load_kind=2, fine_assembled=2, **prm)
+    ne = 24 * 12 * 12
+    rng = np.random.default_rng(4)
+    x = rng.uniform(0.05, 1.0, ne)
+    tiled.set("XPhys", x)
+    plain.set("XPhys", x)
+    U = rng.standard_normal((8, tiled.size("ndof")))
+    a = tiled.apply_K_cases(U, kind=1)
+    b = plain.apply_K_cases(U, kind=1)
+    err = np.abs(a - b).max() / np.abs(b).max()
+    print("tiled vs node-ordered FP16 smoother product:", err)
+    assert err <= 1e-6
+    fa, ga = tiled.fem_obj_sens()
+    fb, gb = plain.fem_obj_sens()
+    print("compliance", fa, fb)
+    assert abs(fa - fb) <= 1e-9 * abs(fb)
+    da, db = tiled.get("dfdx"), plain.get("dfdx")
+    assert np.abs(da - db).max() <= 1e-7 * np.abs(db).max()
+    tiled.close()
+    plain.close()

@@ -392,7 +392,10 @@ __global__ __launch_bounds__(TPB) void k_fine_ell_fill(int nn, int ne, const int
                                                        uint16_t *__restrict__ v1, uint2 *__restrict__ q8,
                                                        uint32_t *__restrict__ qs, float *__restrict__ dg,
                                                        float4 *__restrict__ w8, float *__restrict__ w1,
-                                                       double *__restrict__ a64, float *__restrict__ a32) {
+                                                       double *__restrict__ a64, float *__restrict__ a32,
+                                                       const int32_t *__restrict__ kmap, uint4 *__restrict__ tv8,
+                                                       uint16_t *__restrict__ tv1, float4 *__restrict__ tw8,
+                                                       float *__restrict__ tw1) {
   constexpr int ND = 3 * NPE;
   const int n = blockIdx.x * TPB + threadIdx.x;
   const int s = blockIdx.y;
@@ -432,10 +435,19 @@ __global__ __launch_bounds__(TPB) void k_fine_ell_fill(int nn, int ne, const int
     for (int t = 0; t < 9; ++t) a32[9 * k + t] = (float)b[t];
   }
   if (!q8 && !v8) return;  // the smoothed prolongator's blocks only
+  // the node-tile copies (several load cases) at the slot's tiled position
+  const int64_t k2 = kmap ? (int64_t)kmap[k] : -1;
   if (w8) {
-    w8[2 * k] = make_float4((float)b[0], (float)b[1], (float)b[2], (float)b[3]);
-    w8[2 * k + 1] = make_float4((float)b[4], (float)b[5], (float)b[6], (float)b[7]);
+    const float4 f0 = make_float4((float)b[0], (float)b[1], (float)b[2], (float)b[3]);
+    const float4 f1 = make_float4((float)b[4], (float)b[5], (float)b[6], (float)b[7]);
+    w8[2 * k] = f0;
+    w8[2 * k + 1] = f1;
     w1[k] = (float)b[8];
+    if (k2 >= 0) {
+      tw8[2 * k2] = f0;
+      tw8[2 * k2 + 1] = f1;
+      tw1[k2] = (float)b[8];
+    }
   }
   double a[9];
 #pragma unroll
@@ -468,9 +480,14 @@ __global__ __launch_bounds__(TPB) void k_fine_ell_fill(int nn, int ne, const int
   unsigned short hb[9];
 #pragma unroll
   for (int t = 0; t < 9; ++t) hb[t] = __half_as_ushort(__float2half_rn((float)a[t]));
-  v8[k] = make_uint4((unsigned)hb[0] | ((unsigned)hb[1] << 16), (unsigned)hb[2] | ((unsigned)hb[3] << 16),
-                     (unsigned)hb[4] | ((unsigned)hb[5] << 16), (unsigned)hb[6] | ((unsigned)hb[7] << 16));
+  const uint4 hv = make_uint4((unsigned)hb[0] | ((unsigned)hb[1] << 16), (unsigned)hb[2] | ((unsigned)hb[3] << 16),
+                              (unsigned)hb[4] | ((unsigned)hb[5] << 16), (unsigned)hb[6] | ((unsigned)hb[7] << 16));
+  v8[k] = hv;
   v1[k] = hb[8];
+  if (k2 >= 0) {
+    tv8[k2] = hv;
+    tv1[k2] = hb[8];
+  }
 }
 
 // The filter's Helmholtz operator on the same node pattern (FP64 scalar
@@ -950,22 +967,22 @@ void fine_ell_fill(hipStream_t s, const uft_ctx *C, const double *Ee, const doub
     if (C->npe == 8)
       k_fine_ell_fill<8, float2><<<g, TPB, 0, s>>>(nn, ne, F.d_off, F.d_width, F.d_col, F.d_cptr, F.d_clist, KEf, Ee,
                                                    C->d_fixm, dinv, F.d_v8, F.d_v1, F.d_q8, F.d_qs, F.d_dg, F.d_w8,
-                                                   F.d_w1, a64, a32);
+                                                   F.d_w1, a64, a32, F.d_kmap, F.d_tv8, F.d_tv1, F.d_tw8, F.d_tw1);
     else
       k_fine_ell_fill<4, float2><<<g, TPB, 0, s>>>(nn, ne, F.d_off, F.d_width, F.d_col, F.d_cptr, F.d_clist, KEf, Ee,
                                                    C->d_fixm, dinv, F.d_v8, F.d_v1, F.d_q8, F.d_qs, F.d_dg, F.d_w8,
-                                                   F.d_w1, a64, a32);
+                                                   F.d_w1, a64, a32, F.d_kmap, F.d_tv8, F.d_tv1, F.d_tw8, F.d_tw1);
     return;
   }
   const double2 *KE = (const double2 *)C->d_KE;
   if (C->npe == 8)
     k_fine_ell_fill<8, double2><<<g, TPB, 0, s>>>(nn, ne, F.d_off, F.d_width, F.d_col, F.d_cptr, F.d_clist, KE, Ee,
                                                   C->d_fixm, dinv, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
-                                                  nullptr, a64, a32);
+                                                  nullptr, a64, a32, nullptr, nullptr, nullptr, nullptr, nullptr);
   else
     k_fine_ell_fill<4, double2><<<g, TPB, 0, s>>>(nn, ne, F.d_off, F.d_width, F.d_col, F.d_cptr, F.d_clist, KE, Ee,
                                                   C->d_fixm, dinv, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
-                                                  nullptr, a64, a32);
+                                                  nullptr, a64, a32, nullptr, nullptr, nullptr, nullptr, nullptr);
 }
 void fine_ell_op32(hipStream_t s, const uft_ctx *C, const float *u, const RowOp<float> &op) {
   const auto &F = C->fell;

@@ -441,6 +441,177 @@ __device__ __forceinline__ void st_nr(float *__restrict__ p, const float (&o)[NR
   }
 }
 
+// The rows of node n once its products a[c][r] = (A u)_(3n+c, r) are
+// formed: fixed DOFs take t = u, H scales by the row's diagonal, then MODE's
+// epilogue on the node's 3 NR entries (16- or 8-byte runs); MR_STOREDOT adds
+// u.t per right-hand side into dv.  load() reads every operand of the three
+// rows (it does not depend on the products, so a kernel may issue it before
+// its slot loop); finish() forms and stores the rows.
+template <int NR, bool H, int MODE>
+struct NodeRows {
+  static constexpr bool kA = MODE == MR_STEP || MODE == MR_RES || MODE == MR_RESFIRST || MODE == MR_RESFIRSTX;
+  static constexpr bool kB = MODE == MR_STEP;
+  static constexpr bool kD = MODE == MR_STEP || MODE == MR_RESFIRST || MODE == MR_RESFIRSTX;
+  float ui[3][NR], va[3][kA ? NR : 1], vb[3][kB ? NR : 1], sc[3], di[3];
+  unsigned m;
+  __device__ __forceinline__ void load(const MrOp<float> &op, int64_t n, const float *__restrict__ u,
+                                       const float *__restrict__ dg, const uint8_t *__restrict__ fixm) {
+    m = fixm[n];
+    const int64_t i0 = 3 * n, ii0 = i0 * NR;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      ld_nr<NR>(u + ii0 + c * NR, ui[c]);
+      sc[c] = H ? dg[i0 + c] : 1.f;
+      if constexpr (MODE == MR_STEP) {
+        ld_nr<NR>(op.r + ii0 + c * NR, va[c]);
+        ld_nr<NR>(op.x + ii0 + c * NR, vb[c]);
+      } else if constexpr (kA) {
+        ld_nr<NR>(op.base + ii0 + c * NR, va[c]);
+      }
+      if constexpr (kD) di[c] = op.dinv[i0 + c];
+    }
+  }
+  __device__ __forceinline__ void finish(const MrOp<float> &op, int64_t n, const float (&a)[3][NR], double (&dv)[NR]) {
+    const int64_t ii0 = 3 * n * NR;
+    float tv[3][NR];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const bool fx = (m >> c) & 1u;
+#pragma unroll
+      for (int r = 0; r < NR; ++r) tv[c][r] = fx ? ui[c][r] : sc[c] * a[c][r];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const int64_t ii = ii0 + c * NR;
+      if constexpr (MODE == MR_STORE) {
+        st_nr<NR>(op.y + ii, tv[c]);
+      } else if constexpr (MODE == MR_STEP) {
+        float dn[NR];
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+          va[c][r] -= tv[c][r];
+          dn[r] = (float)op.c1 * ui[c][r] + (float)op.c2 * di[c] * va[c][r];
+          vb[c][r] += op.addprev ? dn[r] + ui[c][r] : dn[r];
+        }
+        st_nr<NR>(op.r + ii, va[c]);
+        st_nr<NR>(op.dout + ii, dn);
+        st_nr<NR>(op.x + ii, vb[c]);
+      } else if constexpr (MODE == MR_RES) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r) va[c][r] -= tv[c][r];
+        st_nr<NR>(op.r + ii, va[c]);
+      } else if constexpr (MODE == MR_RESFIRST || MODE == MR_RESFIRSTX) {
+        float dn[NR];
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+          va[c][r] -= tv[c][r];
+          dn[r] = (float)op.it * di[c] * va[c][r];
+        }
+        st_nr<NR>(op.r + ii, va[c]);
+        st_nr<NR>(op.dout + ii, dn);
+        if constexpr (MODE == MR_RESFIRSTX) {
+          st_nr<NR>(op.kr + ii, va[c]);
+          st_nr<NR>(op.kd + ii, dn);
+        }
+      } else {  // MR_STOREDOT
+        st_nr<NR>(op.y + ii, tv[c]);
+#pragma unroll
+        for (int r = 0; r < NR; ++r) dv[r] += (double)ui[c][r] * (double)tv[c][r];
+      }
+    }
+  }
+};
+// MR_STOREDOT's block partials, one lane per node: per right-hand side a
+// fixed shuffle tree over the wavefront, then the four wavefronts in order,
+// into part[blk][r]
+template <int NR>
+__device__ __forceinline__ void mr_node_partials(const MrOp<float> &op, double (&dv)[NR], int64_t blk) {
+  __shared__ double sm[NR][TPB / 64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    double v = dv[r];
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_down(v, o, 64);
+    if (lane == 0) sm[r][wv] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < NR) {
+    const int r = threadIdx.x;
+    op.part[(size_t)blk * NR + r] = (sm[r][0] + sm[r][1]) + (sm[r][2] + sm[r][3]);
+  }
+}
+// one slot's 3x3 block as loaded (FP16: eight entries in 16 bytes and the
+// ninth; FP32: two float4 and the ninth), converted when used; the block
+// stream is read once per apply, so its loads are non-temporal and leave
+// L2 to the gathered vector entries
+template <bool H>
+struct RawBlk;
+template <>
+struct RawBlk<true> {
+  uint4 v;
+  uint16_t w;
+  template <typename I>
+  __device__ __forceinline__ void load(I k, const uint4 *__restrict__ v8, const uint16_t *__restrict__ v1,
+                                       const float4 *__restrict__, const float *__restrict__) {
+    typedef unsigned v4u __attribute__((ext_vector_type(4)));
+    const v4u t = __builtin_nontemporal_load(reinterpret_cast<const v4u *>(&v8[k]));
+    v = make_uint4(t.x, t.y, t.z, t.w);
+    w = __builtin_nontemporal_load(&v1[k]);
+  }
+  __device__ __forceinline__ void get(float (&e)[9]) const {
+    const unsigned hw[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const float2 f = __half22float2(*reinterpret_cast<const __half2 *>(&hw[q]));
+      e[2 * q] = f.x;
+      e[2 * q + 1] = f.y;
+    }
+    e[8] = __half2float(__ushort_as_half(w));
+  }
+};
+template <>
+struct RawBlk<false> {
+  float4 a, b;
+  float c;
+  template <typename I>
+  __device__ __forceinline__ void load(I k, const uint4 *__restrict__, const uint16_t *__restrict__,
+                                       const float4 *__restrict__ w8, const float *__restrict__ w1) {
+    typedef float v4f __attribute__((ext_vector_type(4)));
+    const v4f ta = __builtin_nontemporal_load(reinterpret_cast<const v4f *>(&w8[2 * k]));
+    const v4f tb = __builtin_nontemporal_load(reinterpret_cast<const v4f *>(&w8[2 * k + 1]));
+    a = make_float4(ta.x, ta.y, ta.z, ta.w);
+    b = make_float4(tb.x, tb.y, tb.z, tb.w);
+    c = __builtin_nontemporal_load(&w1[k]);
+  }
+  __device__ __forceinline__ void get(float (&e)[9]) const {
+    e[0] = a.x; e[1] = a.y; e[2] = a.z; e[3] = a.w;
+    e[4] = b.x; e[5] = b.y; e[6] = b.z; e[7] = b.w;
+    e[8] = c;
+  }
+};
+// one slot's 3x3 block: FP16 (eight entries in 16 bytes and the ninth) or FP32
+template <bool H>
+__device__ __forceinline__ void mr_block(int64_t k, const uint4 *__restrict__ v8, const uint16_t *__restrict__ v1,
+                                         const float4 *__restrict__ w8, const float *__restrict__ w1, float (&e)[9]) {
+  if constexpr (H) {
+    const uint4 hv = v8[k];
+    const unsigned hw[4] = {hv.x, hv.y, hv.z, hv.w};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const float2 f = __half22float2(*reinterpret_cast<const __half2 *>(&hw[q]));
+      e[2 * q] = f.x;
+      e[2 * q + 1] = f.y;
+    }
+    e[8] = __half2float(__ushort_as_half(v1[k]));
+  } else {
+    const float4 a4 = w8[2 * k], b4 = w8[2 * k + 1];
+    e[0] = a4.x; e[1] = a4.y; e[2] = a4.z; e[3] = a4.w;
+    e[4] = b4.x; e[5] = b4.y; e[6] = b4.z; e[7] = b4.w;
+    e[8] = w1[k];
+  }
+}
+
 template <int NR, bool H, typename CI, int MODE>
 __global__ __launch_bounds__(TPB) void k_mr_ell_node(int nn, const int64_t *__restrict__ off,
                                                      const int32_t *__restrict__ width, const CI *__restrict__ col,
@@ -448,126 +619,163 @@ __global__ __launch_bounds__(TPB) void k_mr_ell_node(int nn, const int64_t *__re
                                                      const float *__restrict__ dg, const float4 *__restrict__ w8,
                                                      const float *__restrict__ w1, const uint8_t *__restrict__ fixm,
                                                      const float *__restrict__ u, MrOp<float> op) {
-  constexpr int LPN = 1;                // lanes per node
-  constexpr int NL = NR / LPN;          // right-hand sides per lane
   const int64_t blk = uftr::xcd_block(blockIdx.x, gridDim.x);
-  const int64_t t = blk * TPB + threadIdx.x;
-  const int64_t n = t / LPN;
-  const int h = (int)(t % LPN);
-  double dv[NL];
+  const int64_t n = blk * TPB + threadIdx.x;
+  double dv[NR];
 #pragma unroll
-  for (int r = 0; r < NL; ++r) dv[r] = 0.0;
+  for (int r = 0; r < NR; ++r) dv[r] = 0.0;
   if (n < nn) {
     const int64_t w = n / UFT_ELL_SLICE;
     const int64_t base = off[w] + n % UFT_ELL_SLICE;
     const int W = width[w];
-    float a[3][NL];
+    float a[3][NR];
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
-      for (int r = 0; r < NL; ++r) a[c][r] = 0.f;
+      for (int r = 0; r < NR; ++r) a[c][r] = 0.f;
 #pragma unroll 2
     for (int s = 0; s < W; ++s) {
       const int64_t k = base + (int64_t)s * UFT_ELL_SLICE;
       const int64_t j = std::is_same<CI, int16_t>::value ? n + (int64_t)col[k] : (int64_t)col[k];
       float e[9];
-      if constexpr (H) {
-        const uint4 hv = v8[k];
-        const unsigned hw[4] = {hv.x, hv.y, hv.z, hv.w};
+      mr_block<H>(k, v8, v1, w8, w1, e);
+      float x[3][NR];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const float2 f = __half22float2(*reinterpret_cast<const __half2 *>(&hw[q]));
-          e[2 * q] = f.x;
-          e[2 * q + 1] = f.y;
-        }
-        e[8] = __half2float(__ushort_as_half(v1[k]));
-      } else {
-        const float4 a4 = w8[2 * k], b4 = w8[2 * k + 1];
-        e[0] = a4.x; e[1] = a4.y; e[2] = a4.z; e[3] = a4.w;
-        e[4] = b4.x; e[5] = b4.y; e[6] = b4.z; e[7] = b4.w;
-        e[8] = w1[k];
-      }
-      float x[3][NL];
-#pragma unroll
-      for (int c = 0; c < 3; ++c) ld_nr<NL>(u + (3 * j + c) * NR + h * NL, x[c]);
+      for (int c = 0; c < 3; ++c) ld_nr<NR>(u + (3 * j + c) * NR, x[c]);
 #pragma unroll
       for (int c = 0; c < 3; ++c)
 #pragma unroll
-        for (int r = 0; r < NL; ++r) a[c][r] += e[3 * c] * x[0][r] + e[3 * c + 1] * x[1][r] + e[3 * c + 2] * x[2][r];
+        for (int r = 0; r < NR; ++r) a[c][r] += e[3 * c] * x[0][r] + e[3 * c + 1] * x[1][r] + e[3 * c + 2] * x[2][r];
     }
-    const unsigned m = fixm[n];
+    NodeRows<NR, H, MODE> rows;
+    rows.load(op, n, u, dg, fixm);
+    rows.finish(op, n, a, dv);
+  }
+  if constexpr (MODE == MR_STOREDOT) mr_node_partials<NR>(op, dv, blk);
+}
+
+// The same product on node tiles (8 right-hand sides): workgroup t takes tile
+// t's UFT_TILE nodes, one per lane, in the tiles' position order.  It first
+// copies the 3 x 8 vector entries of every node in the tile's halo (the
+// columns of its rows, ascending) into LDS with 16-byte loads, six per node,
+// so each neighbour's entries leave L2 once per tile instead of once per
+// incident row; the slot loop then reads the blocks in tile order and the
+// neighbour entries from LDS at the slot's 16-bit halo index.  LDS holds
+// quarter q of halo node h at q HP + h (HP odd), so the lanes of one slot,
+// whose halo indices mostly ascend, read distinct banks.
+//
+// LDS bounds the occupancy to two wavefronts per SIMD, so each wavefront
+// keeps many loads in flight: the halo copy issues its loads in two batches
+// of independent requests (no wait between a load and its LDS store until
+// the batch is complete), the first group of TG slots (blocks and halo
+// indices) is requested beside the first batch, and the slot loop loads a
+// whole group before its products.
+constexpr int TILE_HP = UFT_TILE_HCAP + 1;
+constexpr int TILE_QPT = (6 * UFT_TILE_HCAP + UFT_TILE - 1) / UFT_TILE;  // halo quarters per lane
+template <bool H, int TG>
+struct SlotGroup {
+  RawBlk<H> rb[TG];
+  int hh[TG];
+  // slots s0 .. s0 + TG - 1 of the row; past its width W (uniform over the
+  // wavefront) the last slot is read again and its block zeroed, so the
+  // loads carry no branch and stay in flight together
+  __device__ __forceinline__ void load(int s0, int W, unsigned base, const uint16_t *__restrict__ lc,
+                                       const uint4 *__restrict__ v8, const uint16_t *__restrict__ v1,
+                                       const float4 *__restrict__ w8, const float *__restrict__ w1) {
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const int64_t i = 3 * n + c, ii = i * NR + h * NL;
-      float ui[NL], tv[NL];
-      ld_nr<NL>(u + ii, ui);
-      const float sc = H ? dg[i] : 1.f;
-      const bool fx = (m >> c) & 1u;
-#pragma unroll
-      for (int r = 0; r < NL; ++r) tv[r] = fx ? ui[r] : sc * a[c][r];
-      if constexpr (MODE == MR_STORE) {
-        st_nr<NL>(op.y + ii, tv);
-      } else if constexpr (MODE == MR_STEP) {
-        float rv[NL], xv[NL], dn[NL];
-        ld_nr<NL>(op.r + ii, rv);
-        ld_nr<NL>(op.x + ii, xv);
-        const float di = op.dinv[i];
-#pragma unroll
-        for (int r = 0; r < NL; ++r) {
-          rv[r] -= tv[r];
-          dn[r] = (float)op.c1 * ui[r] + (float)op.c2 * di * rv[r];
-          xv[r] += op.addprev ? dn[r] + ui[r] : dn[r];
-        }
-        st_nr<NL>(op.r + ii, rv);
-        st_nr<NL>(op.dout + ii, dn);
-        st_nr<NL>(op.x + ii, xv);
-      } else if constexpr (MODE == MR_RES) {
-        float bv[NL];
-        ld_nr<NL>(op.base + ii, bv);
-#pragma unroll
-        for (int r = 0; r < NL; ++r) bv[r] -= tv[r];
-        st_nr<NL>(op.r + ii, bv);
-      } else if constexpr (MODE == MR_RESFIRST || MODE == MR_RESFIRSTX) {
-        float bv[NL], dn[NL];
-        ld_nr<NL>(op.base + ii, bv);
-        const float di = op.dinv[i];
-#pragma unroll
-        for (int r = 0; r < NL; ++r) {
-          bv[r] -= tv[r];
-          dn[r] = (float)op.it * di * bv[r];
-        }
-        st_nr<NL>(op.r + ii, bv);
-        st_nr<NL>(op.dout + ii, dn);
-        if constexpr (MODE == MR_RESFIRSTX) {
-          st_nr<NL>(op.kr + ii, bv);
-          st_nr<NL>(op.kd + ii, dn);
-        }
-      } else {  // MR_STOREDOT
-        st_nr<NL>(op.y + ii, tv);
-#pragma unroll
-        for (int r = 0; r < NL; ++r) dv[r] += (double)ui[r] * (double)tv[r];
-      }
+    for (int g = 0; g < TG; ++g) {
+      const unsigned k = base + (unsigned)min(s0 + g, W - 1) * UFT_ELL_SLICE;
+      hh[g] = __builtin_nontemporal_load(&lc[k]);
+      rb[g].load(k, v8, v1, w8, w1);
     }
   }
-  if constexpr (MODE == MR_STOREDOT) {
-    // per right-hand side h NL + r: fixed shuffle tree over the wavefront's
-    // lanes of the same h, then the four wavefronts in order, into
-    // part[blk][h NL + r]
-    __shared__ double sm[NR][TPB / 64];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  __device__ __forceinline__ void apply(int s0, int W, const float4 *xs, float (&a)[3][8]) const {
 #pragma unroll
-    for (int r = 0; r < NL; ++r) {
-      double v = dv[r];
+    for (int g = 0; g < TG; ++g) {
+      float e[9];
+      rb[g].get(e);
+      const bool ok = s0 + g < W;
 #pragma unroll
-      for (int o = 32; o >= LPN; o >>= 1) v += __shfl_down(v, o, 64);
-      if (lane < LPN) sm[lane * NL + r][wv] = v;
-    }
-    __syncthreads();
-    if (threadIdx.x < NR) {
-      const int r = threadIdx.x;
-      op.part[(size_t)blk * NR + r] = (sm[r][0] + sm[r][1]) + (sm[r][2] + sm[r][3]);
+      for (int t = 0; t < 9; ++t) e[t] = ok ? e[t] : 0.f;
+      float x[3][8];
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+          const float4 v = xs[(2 * c + q) * TILE_HP + hh[g]];
+          x[c][4 * q] = v.x; x[c][4 * q + 1] = v.y; x[c][4 * q + 2] = v.z; x[c][4 * q + 3] = v.w;
+        }
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int r = 0; r < 8; ++r)
+          a[c][r] += e[3 * c] * x[0][r] + e[3 * c + 1] * x[1][r] + e[3 * c + 2] * x[2][r];
     }
   }
+};
+template <bool H, int MODE, int TG = (H ? 9 : 5)>
+__global__ __launch_bounds__(UFT_TILE) void k_mr_tile_op(const int32_t *__restrict__ tnode,
+                                                         const int32_t *__restrict__ hoff,
+                                                         const int32_t *__restrict__ hnode,
+                                                         const int64_t *__restrict__ toff,
+                                                         const int32_t *__restrict__ twidth,
+                                                         const uint16_t *__restrict__ lc,
+                                                         const uint4 *__restrict__ v8, const uint16_t *__restrict__ v1,
+                                                         const float *__restrict__ dg, const float4 *__restrict__ w8,
+                                                         const float *__restrict__ w1, const uint8_t *__restrict__ fixm,
+                                                         const float *__restrict__ u, MrOp<float> op) {
+  constexpr int NR = 8;
+  static_assert(UFT_TILE == TPB, "one lane per tile position");
+  __shared__ float4 xs[6 * TILE_HP];
+  const int64_t t = uftr::xcd_block(blockIdx.x, gridDim.x);
+  const int64_t p = t * UFT_TILE + threadIdx.x;
+  const int64_t n = tnode[p];
+  const int64_t nv = n >= 0 ? n : 0;  // padding positions read node 0 and store nothing
+  const int64_t w = p / UFT_ELL_SLICE;
+  const unsigned base = (unsigned)(toff[w] + p % UFT_ELL_SLICE);  // tiled slots < 2^31 (set-up check)
+  const int W = twidth[w];  // uniform over the wavefront (one slice); padding slots hold zero blocks
+  // the halo copy: every lane TILE_QPT quarters (the surplus ones repeat
+  // the last quarter), their node ids first, then all their loads; the first
+  // slot group is requested after them, so waiting for the copy does not
+  // wait for the group (loads complete in issue order)
+  const int h0 = hoff[t], nq = 6 * (hoff[t + 1] - h0);
+  const float4 *__restrict__ u4 = reinterpret_cast<const float4 *>(u);
+  int hn[TILE_QPT], hq[TILE_QPT];
+#pragma unroll
+  for (int j = 0; j < TILE_QPT; ++j) {
+    const int q = min((int)threadIdx.x + j * UFT_TILE, nq - 1);
+    const int h = q / 6;
+    hq[j] = q;
+    hn[j] = hnode[h0 + h];
+  }
+  float4 hv[TILE_QPT];
+#pragma unroll
+  for (int j = 0; j < TILE_QPT; ++j) {
+    const int h = hq[j] / 6, part = hq[j] - 6 * h;
+    hv[j] = u4[(int64_t)hn[j] * 6 + part];
+    hq[j] = part * TILE_HP + h;
+  }
+  SlotGroup<H, TG> ga;
+  if (W > 0) ga.load(0, W, base, lc, v8, v1, w8, w1);
+#pragma unroll
+  for (int j = 0; j < TILE_QPT; ++j) xs[hq[j]] = hv[j];
+  __syncthreads();
+  double dv[NR];
+#pragma unroll
+  for (int r = 0; r < NR; ++r) dv[r] = 0.0;
+  float a[3][NR];
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int r = 0; r < NR; ++r) a[c][r] = 0.f;
+  for (int s0 = 0; s0 < W; s0 += TG) {
+    if (s0 > 0) ga.load(s0, W, base, lc, v8, v1, w8, w1);
+    ga.apply(s0, W, xs, a);
+  }
+  NodeRows<NR, H, MODE> rows;
+  rows.load(op, nv, u, dg, fixm);
+  if (n >= 0) rows.finish(op, n, a, dv);
+  if constexpr (MODE == MR_STOREDOT) mr_node_partials<NR>(op, dv, t);
 }
 
 // ---------------------------------------------------------------- 6x6 BSR
@@ -1117,11 +1325,37 @@ static void mr_ell_node_t(hipStream_t s, const uft_ctx *C, int nr, const CI *col
   }
 #undef UFT_MR_NODE
 }
+// the node-tile kernel: eight right-hand sides, every epilogue but the
+// K-cycle's dot modes (not used on the fine level)
+static bool mr_tile_mode(const uft_ctx *C, int nr, int mode, const float *u) {
+  return C->fell.d_tnode && nr == 8 && mode != MR_DOTS2 && mode != MR_DOTS3 && ((uintptr_t)u % 16) == 0;
+}
+template <bool H>
+static void mr_tile_op_t(hipStream_t s, const uft_ctx *C, const float *u, const MrOp<float> &op) {
+  const auto &F = C->fell;
+#define UFT_MR_TILE(M)                                                                                          \
+  k_mr_tile_op<H, M><<<(unsigned)F.ntile, UFT_TILE, 0, s>>>(F.d_tnode, F.d_thoff, F.d_thnode, F.d_toff,       \
+                                                           F.d_twidth, F.d_tlc, F.d_tv8, F.d_tv1, F.d_dg,    \
+                                                           F.d_tw8, F.d_tw1, C->d_fixm, u, op)
+  switch (op.mode) {
+    case MR_STORE: UFT_MR_TILE(MR_STORE); break;
+    case MR_STEP: UFT_MR_TILE(MR_STEP); break;
+    case MR_RES: UFT_MR_TILE(MR_RES); break;
+    case MR_RESFIRST: UFT_MR_TILE(MR_RESFIRST); break;
+    case MR_RESFIRSTX: UFT_MR_TILE(MR_RESFIRSTX); break;
+    default: UFT_MR_TILE(MR_STOREDOT); break;
+  }
+#undef UFT_MR_TILE
+}
 template <bool H>
 static void mr_ell_op_t(hipStream_t s, const uft_ctx *C, int nr, const float *u, const RowOp<float> &o) {
   const auto &F = C->fell;
   const int nn = (int)C->nn;
   const MrOp<float> op = to_mr(o);
+  if (mr_tile_mode(C, nr, op.mode, u)) {
+    mr_tile_op_t<H>(s, C, u, op);
+    return;
+  }
   // the node-lane kernel: every epilogue but the K-cycle's dot modes (not used
   // on the fine level) and the FP16 blocks' Chebyshev step, vector entries
   // 8- or 16-byte aligned
@@ -1144,6 +1378,7 @@ static void mr_ell_op_t(hipStream_t s, const uft_ctx *C, int nr, const float *u,
 int mr_ell_blocks(const uft_ctx *C, const float *u, int mode, bool fp32) {
   RowOp<float> o;
   o.mode = mode;
+  if (mr_tile_mode(C, C->mr.nr, to_mr(o).mode, u)) return (int)C->fell.ntile;
   const bool node = mr_ell_node_mode(to_mr(o).mode, !fp32) && ((uintptr_t)u % 16) == 0;
   return node ? (int)nblk(C->nn) : (int)nblk((int64_t)C->nn * C->mr.nr);
 }

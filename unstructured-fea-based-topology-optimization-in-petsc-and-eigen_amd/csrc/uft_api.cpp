@@ -2067,8 +2067,8 @@ int uft_params_set(uft_ctx *C, const uft_params *p) {
   if (p->load_cases != 0 && p->load_cases != 1)
     return uft_set_error(C, UFT_ERR_ARG, "load_cases must be 0 (superposed) or 1 (separate)");
   if (p->graph != 0 && p->graph != 1) return uft_set_error(C, UFT_ERR_ARG, "graph must be 0 or 1");
-  if (p->fine_assembled != 0 && p->fine_assembled != 1)
-    return uft_set_error(C, UFT_ERR_ARG, "fine_assembled must be 0 or 1");
+  if (p->fine_assembled < 0 || p->fine_assembled > 2)
+    return uft_set_error(C, UFT_ERR_ARG, "fine_assembled must be 0, 1 or 2");
   if (p->cheb_degree < 1 || p->cheb_degree > 16 || p->cheb_degree_coarse < 0 || p->cheb_degree_coarse > 16 ||
       p->cycle_type < 0 || p->cycle_type > 3)
     return uft_set_error(C, UFT_ERR_ARG, "Chebyshev degrees must be in [1, 16] (coarse: 0 or [1, 16]), cycle_type in [0, 3]");
@@ -2278,6 +2278,27 @@ int uft_setup(uft_ctx *C) {
                       (rc = dalloc_n(C, &F.d_dg, ndof)) || (rc = dalloc_n(C, &F.d_w8, 2 * F.nslot)) ||
                       (rc = dalloc_n(C, &F.d_w1, F.nslot))))
           return rc;
+        // node tiles for the products on eight right-hand sides (5-8 load cases;
+        // fine_assembled = 2: none)
+        NodeTiles T;
+        if (elast && mr && C->nlc > 4 && C->P.fine_assembled == 1 &&
+            uft_node_tiles(nn, C->xyz.data(), off, width, col, UFT_TILE, UFT_TILE_HCAP, T)) {
+          F.ntile = T.ntile;
+          F.tslot = T.nslot;
+          F.thmax = T.hmax;
+          if ((rc = upload(C, &F.d_tnode, T.node)) || (rc = upload(C, &F.d_thoff, T.hoff)) ||
+              (rc = upload(C, &F.d_thnode, T.hnode)) || (rc = upload(C, &F.d_toff, T.off)) ||
+              (rc = upload(C, &F.d_twidth, T.width)) || (rc = upload(C, &F.d_tlc, T.lc)) ||
+              (rc = upload(C, &F.d_kmap, T.kmap)) || (rc = dalloc_n(C, &F.d_tv8, T.nslot)) ||
+              (rc = dalloc_n(C, &F.d_tv1, T.nslot)) || (rc = dalloc_n(C, &F.d_tw8, 2 * T.nslot)) ||
+              (rc = dalloc_n(C, &F.d_tw1, T.nslot)))
+            return rc;
+          // padding slots of the tiled copies stay zero (the assembly writes the real ones)
+          UFT_HIP(C, hipMemsetAsync(F.d_tv8, 0, (size_t)T.nslot * sizeof(uint4), s));
+          UFT_HIP(C, hipMemsetAsync(F.d_tv1, 0, (size_t)T.nslot * sizeof(uint16_t), s));
+          UFT_HIP(C, hipMemsetAsync(F.d_tw8, 0, (size_t)T.nslot * 2 * sizeof(float4), s));
+          UFT_HIP(C, hipMemsetAsync(F.d_tw1, 0, (size_t)T.nslot * sizeof(float), s));
+        }
         UFT_HIP(C, hipStreamSynchronize(s));  // the host arrays end with this scope
       }
     }
@@ -3166,6 +3187,15 @@ extern "C" int uft_fine_operator_info(const uft_ctx *C, int64_t *slots, int64_t 
   return UFT_OK;
 }
 
+extern "C" int uft_fine_tiles_info(const uft_ctx *C, int64_t *tiles, int64_t *slots, int *max_halo) {
+  if (!C || !tiles || !slots || !max_halo) return UFT_ERR_ARG;
+  if (!C->is_setup) return UFT_ERR_STATE;
+  *tiles = C->fell.d_tnode ? C->fell.ntile : 0;
+  *slots = C->fell.d_tnode ? C->fell.tslot : 0;
+  *max_halo = C->fell.d_tnode ? C->fell.thmax : 0;
+  return UFT_OK;
+}
+
 // ---- MMA settings, restart state, KKT residual (MMA.h:8-52)
 extern "C" int uft_mma_set_asymptotes(uft_ctx *C, double init, double decrease, double increase) {
   if (int rc = check_setup(C)) return rc;
@@ -3365,14 +3395,41 @@ extern "C" int uft_time_fine_operator(uft_ctx *C, int kind, int reps, double *us
     return uft_set_error(C, UFT_ERR_ARG, "uft_time_fine_operator: kind %d does not match the form", kind);
   if (kind == 2 && C->P.precision != UFT_MIXED_H)
     return uft_set_error(C, UFT_ERR_ARG, "uft_time_fine_operator: kind 2 needs UFT_MIXED_H");
-  if (C->mr.nr || C->bt.nseg) return uft_set_error(C, UFT_ERR_ARG, "uft_time_fine_operator: one design, one load case");
+  if (C->bt.nseg) return uft_set_error(C, UFT_ERR_ARG, "uft_time_fine_operator: one design");
+  if (C->mr.nr && kind == 3)
+    return uft_set_error(C, UFT_ERR_ARG, "uft_time_fine_operator: kind 3 takes one load case");
   hipStream_t s = C->s;
   AmgLevel &L0 = C->amg.L[0];
+  MrState &M = C->mr;
   const bool cnt = C->count_on;
   C->count_on = false;
   const bool pe = C->prof;  // no per-launch timers inside the measurement
   C->prof = false;
   auto launch = [&]() {
+    if (M.nr) {  // several load cases: the same kinds on all right-hand sides at once
+      if (kind == 0) {
+        RowOp<double> op;
+        op.mode = ROW_STORE;
+        op.y = M.q;
+        mr_lev_op<double>(C, 0, M.p, op);
+      } else if (kind == 1) {
+        mr_outer_apply_lo(C, M.pf, M.qf);
+      } else {
+        MLV<float> v = mlv<float>(C, 0);
+        const ChebCoef cc = cheb_coef(L0.lmax, std::max(2, level_degree(C, 0)), C->P.cheb_lo / C->P.cheb_hi);
+        RowOp<float> st;
+        st.mode = ROW_STEP;
+        st.r = v.r;
+        st.dinv = v.dinv;
+        st.x = v.x;
+        st.c1 = cc.c1[1];
+        st.c2 = cc.c2[1];
+        st.dout = v.t;
+        st.addprev = 0;
+        mr_lev_op<float>(C, 0, (const float *)v.d, st);
+      }
+      return;
+    }
     if (kind == 0) {
       RowOp<double> op;
       op.mode = ROW_STORE;

@@ -15,6 +15,11 @@
 #define UFT_TAIL_MAX 3072  // coarsest unknowns handled by the fused coarse tail (LDS)
 #define UFT_SWEEP_MIN 512  // coarsest levels from this size are inverted by the tiled sweep (kernels_dense.hip)
 #define UFT_ELL_SLICE 64      // nodes per slice of the assembled fine operator (one wavefront)
+// node tiles of the multi-load-case fine products: UFT_TILE nodes per
+// workgroup, whose neighbours' vector entries (at most UFT_TILE_HCAP nodes,
+// 3 x 8 FP32 values each, 80,640 B of LDS) are staged once per tile
+#define UFT_TILE 256
+#define UFT_TILE_HCAP 840
 
 // Per-row epilogue of a fused operator apply t = A u (see kernels_cycle.hip):
 //  STORE     y = t
@@ -310,6 +315,17 @@ struct uft_ctx {
     double *d_hv = nullptr;      // per slot: the filter's Helmholtz entry (FP64, assembled at setup)
     float4 *d_w8 = nullptr;      // per slot: FP32 block entries 0..7 (the outer operator)
     float *d_w1 = nullptr;       // per slot: FP32 entry 8
+    // node tiles (several load cases): the same blocks in tile order, see
+    // NodeTiles; the assembly writes them beside the node-ordered ones
+    int64_t ntile = 0, tslot = 0;
+    int thmax = 0;
+    int32_t *d_tnode = nullptr, *d_thoff = nullptr, *d_thnode = nullptr, *d_twidth = nullptr, *d_kmap = nullptr;
+    int64_t *d_toff = nullptr;
+    uint16_t *d_tlc = nullptr;
+    uint4 *d_tv8 = nullptr;
+    uint16_t *d_tv1 = nullptr;
+    float4 *d_tw8 = nullptr;
+    float *d_tw1 = nullptr;
   } fell;
 
   // vectors
@@ -863,3 +879,22 @@ void uft_node_incidence(int64_t nn, int64_t ne, int npe, const int32_t *elem, st
 bool uft_fine_ell_pattern(int64_t nn, int npe, const int32_t *elem, const std::vector<int32_t> &iptr,
                           const std::vector<int32_t> &inc, std::vector<int64_t> &off, std::vector<int32_t> &width,
                           std::vector<int32_t> &col, std::vector<int32_t> &cptr, std::vector<int32_t> &clist);
+// Node tiles of the sliced-ELL pattern for the multi-load-case products:
+// recursive coordinate bisection of the nodes (cut across the longest extent,
+// the lower part a multiple of `tile` nodes) into tiles of at most `tile`
+// nodes whose neighbour sets (halos) hold at most `hcap` nodes (a tile over
+// the cap is halved again).  Tile t's nodes, ascending, sit at positions
+// t tile + i (node[] = -1 past its count); the positions form slices of
+// UFT_ELL_SLICE with the node-ordered pattern's slot order per node.
+struct NodeTiles {
+  int64_t ntile = 0, nslot = 0;
+  int hmax = 0;
+  std::vector<int32_t> node;          // [ntile * tile] node at each position, -1 for padding
+  std::vector<int32_t> hoff, hnode;   // tile t's halo: hnode[hoff[t] .. hoff[t + 1]), ascending
+  std::vector<int64_t> off;           // per slice of positions: first slot (nslice + 1)
+  std::vector<int32_t> width;         // per slice of positions: slots per position
+  std::vector<uint16_t> lc;           // per tiled slot: index of the column node in the tile's halo
+  std::vector<int32_t> kmap;          // per node-ordered slot: its tiled slot, -1 for padding
+};
+bool uft_node_tiles(int64_t nn, const double *xyz, const std::vector<int64_t> &off, const std::vector<int32_t> &width,
+                    const std::vector<int32_t> &col, int tile, int hcap, NodeTiles &T);

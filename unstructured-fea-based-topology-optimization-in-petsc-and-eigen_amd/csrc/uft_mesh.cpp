@@ -106,6 +106,143 @@ bool uft_fine_ell_pattern(int64_t nn, int npe, const int32_t *elem, const std::v
   return true;
 }
 
+namespace {
+// row length of node n in the sliced-ELL pattern: its neighbours ascend and
+// include n, and the padding slots after them repeat n
+int ell_rowlen(int64_t n, const std::vector<int64_t> &off, const std::vector<int32_t> &width,
+               const std::vector<int32_t> &col) {
+  const int64_t w = n / UFT_ELL_SLICE, lane = n % UFT_ELL_SLICE;
+  int len = width[w] > 0 ? 1 : 0;
+  for (int s = 1; s < width[w]; ++s) {
+    if (col[off[w] + (int64_t)s * UFT_ELL_SLICE + lane] <= col[off[w] + (int64_t)(s - 1) * UFT_ELL_SLICE + lane]) break;
+    len = s + 1;
+  }
+  return len;
+}
+
+struct TileBuilder {
+  const double *xyz;
+  const std::vector<int64_t> &off;
+  const std::vector<int32_t> &width, &col;
+  const std::vector<int> &rowlen;
+  int tile, hcap;
+  std::vector<std::vector<int32_t>> leaves;
+  std::vector<int32_t> halo;
+  bool ok = true;
+
+  // the tile's halo: every column of its nodes' rows, ascending
+  void halo_of(const std::vector<int32_t> &nodes) {
+    halo.clear();
+    for (int32_t n : nodes) {
+      const int64_t w = n / UFT_ELL_SLICE, lane = n % UFT_ELL_SLICE;
+      for (int s = 0; s < rowlen[n]; ++s) halo.push_back(col[off[w] + (int64_t)s * UFT_ELL_SLICE + lane]);
+      halo.push_back(n);
+    }
+    std::sort(halo.begin(), halo.end());
+    halo.erase(std::unique(halo.begin(), halo.end()), halo.end());
+  }
+  void split(std::vector<int32_t> &idx) {
+    if (!ok) return;
+    const int64_t n = (int64_t)idx.size();
+    if (n == 0) return;
+    if (n <= tile) {
+      std::sort(idx.begin(), idx.end());
+      halo_of(idx);
+      if ((int)halo.size() <= hcap) {
+        leaves.push_back(idx);
+        return;
+      }
+      if (n <= UFT_ELL_SLICE) {  // a tile of one slice still over the cap: no tiling
+        ok = false;
+        return;
+      }
+    }
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int32_t v : idx)
+      for (int c = 0; c < 3; ++c) {
+        lo[c] = std::min(lo[c], xyz[3 * (int64_t)v + c]);
+        hi[c] = std::max(hi[c], xyz[3 * (int64_t)v + c]);
+      }
+    int ax = 0;
+    for (int c = 1; c < 3; ++c)
+      if (hi[c] - lo[c] > hi[ax] - lo[ax]) ax = c;
+    // the lower part: a multiple of tile nodes near half (half of a tile over the cap)
+    const int64_t nl = n > tile ? (int64_t)tile * ((n / tile + 1) / 2) : n / 2;
+    auto less = [&](int32_t a, int32_t b) {
+      const double xa = xyz[3 * (int64_t)a + ax], xb = xyz[3 * (int64_t)b + ax];
+      return xa < xb || (xa == xb && a < b);
+    };
+    std::nth_element(idx.begin(), idx.begin() + nl, idx.end(), less);
+    std::vector<int32_t> hi_part(idx.begin() + nl, idx.end());
+    idx.resize(nl);
+    split(idx);
+    split(hi_part);
+  }
+};
+}  // namespace
+
+bool uft_node_tiles(int64_t nn, const double *xyz, const std::vector<int64_t> &off, const std::vector<int32_t> &width,
+                    const std::vector<int32_t> &col, int tile, int hcap, NodeTiles &T) {
+  if (nn <= 0 || tile % UFT_ELL_SLICE != 0 || hcap > 65535) return false;
+  std::vector<int> rowlen(nn);
+  for (int64_t n = 0; n < nn; ++n) rowlen[n] = ell_rowlen(n, off, width, col);
+  TileBuilder B{xyz, off, width, col, rowlen, tile, hcap, {}, {}, true};
+  {
+    std::vector<int32_t> all(nn);
+    for (int64_t n = 0; n < nn; ++n) all[n] = (int32_t)n;
+    B.split(all);
+  }
+  if (!B.ok) return false;
+  T = NodeTiles();
+  T.ntile = (int64_t)B.leaves.size();
+  const int64_t npos = T.ntile * tile, nslice = npos / UFT_ELL_SLICE;
+  T.node.assign(npos, -1);
+  std::vector<int64_t> pos(nn, -1);
+  for (int64_t t = 0; t < T.ntile; ++t)
+    for (size_t i = 0; i < B.leaves[t].size(); ++i) {
+      T.node[t * tile + (int64_t)i] = B.leaves[t][i];
+      pos[B.leaves[t][i]] = t * tile + (int64_t)i;
+    }
+  T.off.assign(nslice + 1, 0);
+  T.width.assign(nslice, 0);
+  for (int64_t w = 0; w < nslice; ++w) {
+    int wd = 0;
+    for (int64_t p = w * UFT_ELL_SLICE; p < (w + 1) * UFT_ELL_SLICE; ++p)
+      if (T.node[p] >= 0) wd = std::max(wd, rowlen[T.node[p]]);
+    T.width[w] = wd;
+    T.off[w + 1] = T.off[w] + (int64_t)wd * UFT_ELL_SLICE;
+  }
+  T.nslot = T.off[nslice];
+  T.lc.assign(T.nslot, 0);
+  T.kmap.assign(off.back(), -1);
+  T.hoff.assign(T.ntile + 1, 0);
+  std::vector<int32_t> mark(nn, -1);
+  for (int64_t t = 0; t < T.ntile; ++t) {
+    B.halo_of(B.leaves[t]);
+    T.hmax = std::max(T.hmax, (int)B.halo.size());
+    for (size_t i = 0; i < B.halo.size(); ++i) mark[B.halo[i]] = (int32_t)i;
+    T.hnode.insert(T.hnode.end(), B.halo.begin(), B.halo.end());
+    T.hoff[t + 1] = (int32_t)T.hnode.size();
+    for (int32_t n : B.leaves[t]) {
+      const int64_t p = pos[n], w2 = p / UFT_ELL_SLICE, l2 = p % UFT_ELL_SLICE;
+      const int64_t w = n / UFT_ELL_SLICE, lane = n % UFT_ELL_SLICE;
+      for (int s = 0; s < T.width[w2]; ++s) {
+        const int64_t k2 = T.off[w2] + (int64_t)s * UFT_ELL_SLICE + l2;
+        if (s < rowlen[n]) {
+          const int64_t k = off[w] + (int64_t)s * UFT_ELL_SLICE + lane;
+          T.lc[k2] = (uint16_t)mark[col[k]];
+          T.kmap[k] = (int32_t)k2;
+        } else {
+          T.lc[k2] = (uint16_t)mark[n];  // padding: the node itself, zero block
+        }
+      }
+    }
+    for (int32_t v : B.halo) mark[v] = -1;
+  }
+  if ((int64_t)T.hnode.size() >= (int64_t)INT32_MAX || T.nslot >= (int64_t)INT32_MAX) return false;
+  return true;
+}
+
 extern "C" {
 
 int64_t uft_box_nodes(int nx, int ny, int nz) { return (int64_t)(nx + 1) * (ny + 1) * (nz + 1); }

@@ -105,6 +105,7 @@ def lib():
         "uft_box_bcs": (C.c_int, [C.c_int] * 4 + [ip, pi, ip, dp, pi]),
         "uft_solver_info": (C.c_int, [vp, pi, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
         "uft_fine_operator_info": (C.c_int, [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), pi, pi]),
+        "uft_fine_tiles_info": (C.c_int, [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), pi]),
         "uft_amg_operator_get": (C.c_int, [vp, C.c_int, ip, ip, dp]),
         "uft_amg_transfer_get": (C.c_int, [vp, C.c_int, ip, dp]),
         "uft_amg_coarse_inverse_get": (C.c_int, [vp, dp]),
@@ -484,6 +485,13 @@ class Context:
         b = (C.c_int64 * 8)()
         self._chk(self.L.uft_solver_info(self.h, C.byref(nl), n, b), "uft_solver_info")
         return [n[i] for i in range(nl.value)], [b[i] for i in range(nl.value)]
+
+    def fine_tiles_info(self):
+        """(tiles, tile-ordered slots, largest halo) of the eight-right-hand-side
+        node tiles; all 0 when the node-ordered kernels run."""
+        t, k, h = C.c_int64(), C.c_int64(), C.c_int()
+        self._chk(self.L.uft_fine_tiles_info(self.h, C.byref(t), C.byref(k), C.byref(h)), "uft_fine_tiles_info")
+        return t.value, k.value, h.value
 
     def fine_operator_info(self):
         """(sliced-ELL slots, nonzero node blocks, bytes per column index,
