@@ -665,11 +665,11 @@ __global__ __launch_bounds__(TPB) void k_mr_ell_node(int nn, const int64_t *__re
 // whose halo indices mostly ascend, read distinct banks.
 //
 // LDS bounds the occupancy to two wavefronts per SIMD, so each wavefront
-// keeps many loads in flight: the halo copy issues its loads in two batches
-// of independent requests (no wait between a load and its LDS store until
-// the batch is complete), the first group of TG slots (blocks and halo
-// indices) is requested beside the first batch, and the slot loop loads a
-// whole group before its products.
+// keeps many loads in flight: the halo copy issues all its loads (node ids,
+// then the quarters) before its first LDS store, the first group of TG slots
+// (blocks and halo indices) is requested beside them, and the slot loop
+// loads a whole group, with no branch between the loads, before its
+// products.
 constexpr int TILE_HP = UFT_TILE_HCAP + 1;
 constexpr int TILE_QPT = (6 * UFT_TILE_HCAP + UFT_TILE - 1) / UFT_TILE;  // halo quarters per lane
 template <bool H, int TG>
