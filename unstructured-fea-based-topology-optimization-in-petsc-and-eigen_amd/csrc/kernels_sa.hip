@@ -586,24 +586,43 @@ __global__ __launch_bounds__(TPB) void k_mr_lincomb2(int64_t n, const double *__
   const T cb = ib < 0 ? T(1) : (T)sc[8 * q + ib];
   c[t] = ca * xa[t] + cb * xb[t];
 }
-// one thread per node u (all NR right-hand sides): x_u += sum over its entry
-// slots of P (ca_q xa + cb_q xb)_{aggregate}, ca_q = sc[8 q + ia] (index < 0:
-// 1); P from the node-sliced copy (FP16: its three planes)
+// one thread per node u (all NR right-hand sides): x_u += ca_q sum over its
+// entry slots of P xa_{aggregate}, ca_q = sc[8 q + ia] (index < 0: 1; a
+// second vector is combined into xa beforehand); P from the node-sliced copy
+// (FP16: its three planes).  The entries are
+// taken TG at a time with all their loads (P values, the aggregates' 6 NR
+// coarse entries in 16-byte loads) issued before the products; entries past
+// the node's last one (aggregate < 0, or past the slice width) read the last
+// valid slot again with zero weights, so no branch separates the loads.
+template <typename T, int NR>
+__device__ __forceinline__ void ld_coarse6(const T *__restrict__ p, T (&o)[6][NR]) {
+  if constexpr (std::is_same<T, float>::value && NR % 4 == 0) {
+#pragma unroll
+    for (int j = 0; j < 6; ++j)
+#pragma unroll
+      for (int q = 0; q < NR / 4; ++q) {
+        const float4 t = reinterpret_cast<const float4 *>(p + j * NR)[q];
+        o[j][4 * q] = t.x; o[j][4 * q + 1] = t.y; o[j][4 * q + 2] = t.z; o[j][4 * q + 3] = t.w;
+      }
+  } else {
+#pragma unroll
+    for (int j = 0; j < 6; ++j)
+#pragma unroll
+      for (int q = 0; q < NR; ++q) o[j][q] = p[j * NR + q];
+  }
+}
 template <typename T, typename S, int NR>
 __global__ __launch_bounds__(TPB) void k_mr_sa_prolong_comb(int64_t nu, int64_t ne, const int64_t *__restrict__ poff,
                                                             const int32_t *__restrict__ pw,
                                                             const int32_t *__restrict__ eagg,
                                                             const S *__restrict__ Pe, const double *__restrict__ sc,
-                                                            int ia, int ib, const T *__restrict__ xa,
-                                                            const T *__restrict__ xb, T *__restrict__ x) {
+                                                            int ia, const T *__restrict__ xa, T *__restrict__ x) {
+  constexpr int TG = 2;
   const int64_t u = (int64_t)blockIdx.x * TPB + threadIdx.x;
   if (u >= nu) return;
-  T ca[NR], cb[NR];
+  T ca[NR];
 #pragma unroll
-  for (int q = 0; q < NR; ++q) {
-    ca[q] = ia < 0 ? T(1) : (T)sc[8 * q + ia];
-    cb[q] = ib < 0 ? T(1) : (T)sc[8 * q + ib];
-  }
+  for (int q = 0; q < NR; ++q) ca[q] = ia < 0 ? T(1) : (T)sc[8 * q + ia];
   const int64_t w = u / UFT_ELL_SLICE, lane = u % UFT_ELL_SLICE;
   const int nq = pw[w];
   T acc[3][NR];
@@ -611,40 +630,58 @@ __global__ __launch_bounds__(TPB) void k_mr_sa_prolong_comb(int64_t nu, int64_t 
   for (int i = 0; i < 3; ++i)
 #pragma unroll
     for (int q = 0; q < NR; ++q) acc[i][q] = T(0);
-  for (int qq = 0; qq < nq; ++qq) {
-    const int64_t sl = poff[w] + (int64_t)qq * UFT_ELL_SLICE + lane;
-    const int64_t A = eagg[sl];
-    if (A < 0) break;  // padding follows the node's entries
-    float p[18];
-    if constexpr (std::is_same<S, __half>::value) {
-      const uint4 lo = reinterpret_cast<const uint4 *>(Pe)[sl], hi = reinterpret_cast<const uint4 *>(Pe)[ne + sl];
-      const uint32_t tl = reinterpret_cast<const uint32_t *>(Pe)[8 * ne + sl];
-      const uint32_t wds[9] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w, tl};
+  int last = 0;  // the node's last valid entry slot seen so far (entries come first, padding after)
+  for (int q0 = 0; q0 < nq; q0 += TG) {
+    int64_t sl[TG], A[TG];
+    bool ok[TG];
 #pragma unroll
-      for (int h = 0; h < 9; ++h) {
-        const float2 v = unpack_h2(wds[h]);
-        p[2 * h] = v.x;
-        p[2 * h + 1] = v.y;
-      }
-    } else {
-      const S *Pq = Pe + poff[w] * 18 + (int64_t)qq * 18 * UFT_ELL_SLICE + lane;
-#pragma unroll
-      for (int v = 0; v < 18; ++v) p[v] = (float)Pq[v * UFT_ELL_SLICE];
+    for (int g = 0; g < TG; ++g) {
+      const int qq = min(q0 + g, nq - 1);
+      sl[g] = poff[w] + (int64_t)qq * UFT_ELL_SLICE + lane;
+      A[g] = eagg[sl[g]];
+      ok[g] = q0 + g < nq && A[g] >= 0;
+      if (ok[g]) last = qq;
     }
-    const T *xa6 = xa + A * 6 * NR, *xb6 = xb ? xb + A * 6 * NR : nullptr;
+    float p[TG][18];
+    T xv[TG][6][NR];
 #pragma unroll
-    for (int j = 0; j < 6; ++j)
+    for (int g = 0; g < TG; ++g) {
+      const int64_t s1 = ok[g] ? sl[g] : poff[w] + (int64_t)last * UFT_ELL_SLICE + lane;
+      const int64_t a1 = ok[g] ? A[g] : 0;
+      if constexpr (std::is_same<S, __half>::value) {
+        const uint4 lo = reinterpret_cast<const uint4 *>(Pe)[s1], hi = reinterpret_cast<const uint4 *>(Pe)[ne + s1];
+        const uint32_t tl = reinterpret_cast<const uint32_t *>(Pe)[8 * ne + s1];
+        const uint32_t wds[9] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w, tl};
 #pragma unroll
-      for (int q = 0; q < NR; ++q) {
-        const T c = xb6 ? ca[q] * xa6[j * NR + q] + cb[q] * xb6[j * NR + q] : xa6[j * NR + q];
+        for (int h = 0; h < 9; ++h) {
+          const float2 v = unpack_h2(wds[h]);
+          p[g][2 * h] = v.x;
+          p[g][2 * h + 1] = v.y;
+        }
+      } else {
+        const int qq = ok[g] ? min(q0 + g, nq - 1) : last;
+        const S *Pq = Pe + poff[w] * 18 + (int64_t)qq * 18 * UFT_ELL_SLICE + lane;
 #pragma unroll
-        for (int i = 0; i < 3; ++i) acc[i][q] += (T)p[i * 6 + j] * c;
+        for (int v = 0; v < 18; ++v) p[g][v] = (float)Pq[v * UFT_ELL_SLICE];
       }
+      ld_coarse6<T, NR>(xa + a1 * 6 * NR, xv[g]);
+    }
+#pragma unroll
+    for (int g = 0; g < TG; ++g) {
+#pragma unroll
+      for (int v = 0; v < 18; ++v) p[g][v] = ok[g] ? p[g][v] : 0.f;
+#pragma unroll
+      for (int j = 0; j < 6; ++j)
+#pragma unroll
+        for (int q = 0; q < NR; ++q)
+#pragma unroll
+          for (int i = 0; i < 3; ++i) acc[i][q] += (T)p[g][i * 6 + j] * xv[g][j][q];
+    }
   }
 #pragma unroll
   for (int i = 0; i < 3; ++i)
 #pragma unroll
-    for (int q = 0; q < NR; ++q) x[(u * 3 + i) * NR + q] += acc[i][q];
+    for (int q = 0; q < NR; ++q) x[(u * 3 + i) * NR + q] += ca[q] * acc[i][q];
 }
 
 }  // namespace
@@ -739,7 +776,7 @@ void mr_sa_prolong_comb(hipStream_t s, int nr, const AmgLevel &L0, int64_t ncu, 
     ia = ib = -1;
   }
   UFT_SA_NR(nr, (k_mr_sa_prolong_comb<double, double, NR><<<nblk(L0.nunit), TPB, 0, s>>>(
-                    L0.nunit, L0.sa_ne_ell, L0.d_sa_poff, L0.d_sa_pw, L0.d_sa_eagg, L0.d_sa_Pe, sc, ia, ib, xa, xb, x)));
+                    L0.nunit, L0.sa_ne_ell, L0.d_sa_poff, L0.d_sa_pw, L0.d_sa_eagg, L0.d_sa_Pe, sc, ia, xa, x)));
 }
 void mr_sa_prolong_comb(hipStream_t s, int nr, const AmgLevel &L0, int64_t ncu, const double *sc, int ia, int ib,
                         const float *xa, const float *xb, float *x, float *tmp) {
@@ -753,11 +790,10 @@ void mr_sa_prolong_comb(hipStream_t s, int nr, const AmgLevel &L0, int64_t ncu, 
   if (L0.d_sa_Peh) {
     UFT_SA_NR(nr, (k_mr_sa_prolong_comb<float, __half, NR><<<nblk(L0.nunit), TPB, 0, s>>>(
                       L0.nunit, L0.sa_ne_ell, L0.d_sa_poff, L0.d_sa_pw, L0.d_sa_eagg, (const __half *)L0.d_sa_Peh, sc,
-                      ia, ib, xa, xb, x)));
+                      ia, xa, x)));
   } else {
     UFT_SA_NR(nr, (k_mr_sa_prolong_comb<float, float, NR><<<nblk(L0.nunit), TPB, 0, s>>>(
-                      L0.nunit, L0.sa_ne_ell, L0.d_sa_poff, L0.d_sa_pw, L0.d_sa_eagg, L0.d_sa_Pef, sc, ia, ib, xa, xb,
-                      x)));
+                      L0.nunit, L0.sa_ne_ell, L0.d_sa_poff, L0.d_sa_pw, L0.d_sa_eagg, L0.d_sa_Pef, sc, ia, xa, x)));
   }
 }
 #undef UFT_SA_NR
