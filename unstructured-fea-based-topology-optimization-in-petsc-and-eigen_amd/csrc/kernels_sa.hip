@@ -442,9 +442,13 @@ __global__ __launch_bounds__(TPB) void k_sa_restrict_first_h(int64_t nc, const i
   }
 }
 
-// FP16 copies (UFT_MIXED_H): one thread per node u, its entries from the
-// three planes (pe_half_index); x_u += sum of P (ca xa + cb xb)_{aggregate}
-// as k_sa_prolong_comb.  Padding slots (aggregate -1) are read but not used.
+// x_u += sum of P (ca xa + cb xb)_{aggregate} (as k_sa_prolong_comb) from
+// the FP16 node-sliced copy's three planes (pe_half_index).  The node's
+// entries are taken TG at a time with every load of the group (aggregate
+// ids, P planes, the aggregates' six coarse entries) issued before the
+// products; padding slots (aggregate -1) read aggregate 0 with zero
+// weights, so no branch separates the loads.  TWO: a second coarse vector xb.
+template <bool TWO>
 __global__ __launch_bounds__(TPB) void k_sa_prolong_comb_h(int64_t nu, int64_t ne, const int64_t *__restrict__ poff,
                                                            const int32_t *__restrict__ pw,
                                                            const int32_t *__restrict__ eagg,
@@ -453,6 +457,7 @@ __global__ __launch_bounds__(TPB) void k_sa_prolong_comb_h(int64_t nu, int64_t n
                                                            const double *__restrict__ sc, int ia, int ib,
                                                            const float *__restrict__ xa, const float *__restrict__ xb,
                                                            float *__restrict__ x, const int32_t *__restrict__ ndes) {
+  constexpr int TG = 4;
   const int64_t u = (int64_t)blockIdx.x * TPB + threadIdx.x;
   if (u >= nu) return;
   const double *scu = ndes ? sc + (size_t)ndes[u] * UFT_BSC : sc;  // a batch: the node's design's scalars
@@ -462,29 +467,41 @@ __global__ __launch_bounds__(TPB) void k_sa_prolong_comb_h(int64_t nu, int64_t n
   const int nq = pw[w];
   const int64_t s0 = poff[w] + u % UFT_ELL_SLICE;
   float a0 = 0.f, a1 = 0.f, a2 = 0.f;
-#pragma unroll 2
-  for (int q = 0; q < nq; ++q) {
-    const int64_t s = s0 + (int64_t)q * UFT_ELL_SLICE;
-    const int A = eagg[s];
-    const uint4 lo = P01[s], hi = P01[ne + s];
-    const uint32_t tl = P2[s];
-    if (A < 0) continue;
-    float c[6];
+  for (int q0 = 0; q0 < nq; q0 += TG) {
+    int A[TG];
+    uint32_t wds[TG][9];
+    float c[TG][6];
 #pragma unroll
-    for (int j = 0; j < 6; ++j) c[j] = xb ? ca * xa[6 * (int64_t)A + j] + cb * xb[6 * (int64_t)A + j] : xa[6 * (int64_t)A + j];
-    const uint32_t wds[9] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w, tl};
-    float p[18];
-#pragma unroll
-    for (int h = 0; h < 9; ++h) {
-      const float2 v = unpack_h2(wds[h]);
-      p[2 * h] = v.x;
-      p[2 * h + 1] = v.y;
+    for (int g = 0; g < TG; ++g) {
+      const int64_t s = s0 + (int64_t)min(q0 + g, nq - 1) * UFT_ELL_SLICE;
+      A[g] = eagg[s];
+      const uint4 lo = P01[s], hi = P01[ne + s];
+      wds[g][0] = lo.x; wds[g][1] = lo.y; wds[g][2] = lo.z; wds[g][3] = lo.w;
+      wds[g][4] = hi.x; wds[g][5] = hi.y; wds[g][6] = hi.z; wds[g][7] = hi.w;
+      wds[g][8] = P2[s];
     }
 #pragma unroll
-    for (int j = 0; j < 6; ++j) {
-      a0 += p[j] * c[j];
-      a1 += p[6 + j] * c[j];
-      a2 += p[12 + j] * c[j];
+    for (int g = 0; g < TG; ++g) {
+      const int64_t Ag = A[g] >= 0 ? A[g] : 0;
+#pragma unroll
+      for (int j = 0; j < 6; ++j) c[g][j] = TWO ? ca * xa[6 * Ag + j] + cb * xb[6 * Ag + j] : xa[6 * Ag + j];
+    }
+#pragma unroll
+    for (int g = 0; g < TG; ++g) {
+      const bool ok = q0 + g < nq && A[g] >= 0;
+      float p[18];
+#pragma unroll
+      for (int h = 0; h < 9; ++h) {
+        const float2 v = unpack_h2(wds[g][h]);
+        p[2 * h] = ok ? v.x : 0.f;
+        p[2 * h + 1] = ok ? v.y : 0.f;
+      }
+#pragma unroll
+      for (int j = 0; j < 6; ++j) {
+        a0 += p[j] * c[g][j];
+        a1 += p[6 + j] * c[g][j];
+        a2 += p[12 + j] * c[g][j];
+      }
     }
   }
   x[u * 3] += a0;
@@ -800,13 +817,19 @@ void mr_sa_prolong_comb(hipStream_t s, int nr, const AmgLevel &L0, int64_t ncu, 
 
 void sa_prolong_comb(hipStream_t s, const AmgLevel &L0, const double *sc, int ia, int ib, const float *xa,
                      const float *xb, float *x, const int32_t *ndes) {
-  if (L0.d_sa_Peh)
-    k_sa_prolong_comb_h<<<nblk(L0.nunit), TPB, 0, s>>>(
-        L0.nunit, L0.sa_ne_ell, L0.d_sa_poff, L0.d_sa_pw, L0.d_sa_eagg, (const uint4 *)L0.d_sa_Peh,
-        (const uint32_t *)L0.d_sa_Peh + 8 * L0.sa_ne_ell, sc, ia, ib, xa, xb, x, ndes);
-  else
+  if (L0.d_sa_Peh) {
+    if (xb)
+      k_sa_prolong_comb_h<true><<<nblk(L0.nunit), TPB, 0, s>>>(
+          L0.nunit, L0.sa_ne_ell, L0.d_sa_poff, L0.d_sa_pw, L0.d_sa_eagg, (const uint4 *)L0.d_sa_Peh,
+          (const uint32_t *)L0.d_sa_Peh + 8 * L0.sa_ne_ell, sc, ia, ib, xa, xb, x, ndes);
+    else
+      k_sa_prolong_comb_h<false><<<nblk(L0.nunit), TPB, 0, s>>>(
+          L0.nunit, L0.sa_ne_ell, L0.d_sa_poff, L0.d_sa_pw, L0.d_sa_eagg, (const uint4 *)L0.d_sa_Peh,
+          (const uint32_t *)L0.d_sa_Peh + 8 * L0.sa_ne_ell, sc, ia, ib, xa, xb, x, ndes);
+  } else {
     k_sa_prolong_comb<float, float><<<nblk(L0.nunit), TPB, 0, s>>>(L0.nunit, L0.d_sa_poff, L0.d_sa_pw, L0.d_sa_eagg,
                                                                    L0.d_sa_Pef, sc, ia, ib, xa, xb, x, ndes);
+  }
 }
 
 }  // namespace uftk
