@@ -15,6 +15,7 @@
 // the FP64 matrix cores), whose workgroup for the next pivot tile also
 // inverts it.  No pivoting and
 // no atomics: every sum runs in a fixed order, results are reproducible.
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
 #include "uft_internal.h"
@@ -310,9 +311,63 @@ __global__ __launch_bounds__(256) void k_dense_mv4(int64_t n, const float4 *__re
   if (lane == 0) y[row] = acc;
 }
 
+// Row-scaled FP16 copy of a row-major n x n FP32 matrix: s_i = max_j |A_ij|
+// (1 for a zero row), H_ij = A_ij / s_i rounded to FP16, so every entry
+// keeps 11 significant bits relative to its row's largest.  One wavefront
+// per row.
+__global__ __launch_bounds__(256) void k_dense_to_half_rows(int64_t n, const float *__restrict__ A,
+                                                            __half *__restrict__ H, float *__restrict__ sc) {
+  const int64_t row = ((int64_t)blockIdx.x * 256 + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (row >= n) return;
+  const float *a = A + row * n;
+  float m = 0.f;
+  for (int64_t c = lane; c < n; c += 64) m = fmaxf(m, fabsf(a[c]));
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+  const float s = m > 0.f ? m : 1.f;
+  const float inv = 1.f / s;
+  for (int64_t c = lane; c < n; c += 64) H[row * n + c] = __float2half_rn(a[c] * inv);
+  if (lane == 0) sc[row] = s;
+}
+
+// y = A x with A as the row-scaled FP16 copy (n a multiple of 8): one
+// wavefront per row, eight entries per 16-byte load, FP32 sums in a fixed
+// order, y_i = s_i sum_j H_ij x_j
+__global__ __launch_bounds__(256) void k_dense_mvh(int64_t n, const uint4 *__restrict__ H, const float *__restrict__ sc,
+                                                   const float4 *__restrict__ x, float *__restrict__ y) {
+  const int64_t row = ((int64_t)blockIdx.x * 256 + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (row >= n) return;
+  const int64_t n8 = n / 8;
+  const uint4 *h = H + row * n8;
+  float acc = 0.f;
+  for (int64_t c = lane; c < n8; c += 64) {
+    typedef unsigned v4u __attribute__((ext_vector_type(4)));
+    const v4u hv = __builtin_nontemporal_load(reinterpret_cast<const v4u *>(h + c));
+    const float4 x0 = x[2 * c], x1 = x[2 * c + 1];
+    const unsigned w[4] = {hv.x, hv.y, hv.z, hv.w};
+    float2 f[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) f[q] = __half22float2(*reinterpret_cast<const __half2 *>(&w[q]));
+    acc += f[0].x * x0.x + f[0].y * x0.y + f[1].x * x0.z + f[1].y * x0.w + f[2].x * x1.x + f[2].y * x1.y +
+           f[3].x * x1.z + f[3].y * x1.w;
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+  if (lane == 0) y[row] = sc[row] * acc;
+}
+
 }  // namespace
 
 namespace uftk {
+
+void dense_to_half_rows(hipStream_t s, int64_t n, const float *A, uint16_t *H, float *sc) {
+  k_dense_to_half_rows<<<(unsigned)((n * 64 + 255) / 256), 256, 0, s>>>(n, A, (__half *)H, sc);
+}
+void dense_mvh(hipStream_t s, int64_t n, const uint16_t *H, const float *sc, const float *x, float *y) {
+  k_dense_mvh<<<(unsigned)((n * 64 + 255) / 256), 256, 0, s>>>(n, (const uint4 *)H, sc, (const float4 *)x, y);
+}
 
 // scratch doubles needed by dense_spd_sweep for an n x n matrix
 int64_t dense_sweep_scratch(int64_t n) {

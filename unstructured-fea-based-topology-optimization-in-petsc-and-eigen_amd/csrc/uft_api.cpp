@@ -522,7 +522,10 @@ void coarse_direct(uft_ctx *C, int l, const T *r, T *x) {
         ent += nd * nd;
       }
     }
-    count_apply(C, ent * (C->bt.nseg || std::is_same<T, double>::value ? 8 : 4) + 2.0 * (double)H.ncoarse * sizeof(T));
+    const bool half = std::is_same<T, float>::value && H.d_cinvh;  // row-scaled FP16 copy plus row scales
+    count_apply(C, half ? ent * 2 + 4.0 * H.ncoarse + 2.0 * (double)H.ncoarse * sizeof(T)
+                        : ent * (C->bt.nseg || std::is_same<T, double>::value ? 8 : 4) +
+                              2.0 * (double)H.ncoarse * sizeof(T));
   }
   if (C->bt.nseg) {  // block-diagonal coarsest level: one workgroup per design
     uftk::tail_solve(C->s, (int)Ln.nunit, Lv.d_cptr, Lv.d_child, P, H.d_cinv, r, vn.x, C->bt.nseg, C->bt.d_cu,
@@ -542,6 +545,8 @@ void coarse_direct(uft_ctx *C, int l, const T *r, T *x) {
   k_restrict(C->s, Lv, Ln.nunit, r, vn.b);
   if constexpr (std::is_same<T, double>::value) {
     uftk::dense_mv(C->s, H.ncoarse, H.d_cinv, vn.b, vn.x);
+  } else if (H.d_cinvh && ((uintptr_t)vn.b % 16) == 0) {
+    uftk::dense_mvh(C->s, H.ncoarse, H.d_cinvh, H.d_cinvs, vn.b, vn.x);  // the row-scaled FP16 copy
   } else if (H.d_cinvf && H.ncoarse % 4 == 0 && ((uintptr_t)vn.b % 16) == 0 && ((uintptr_t)vn.x % 16) == 0) {
     uftk::dense_mvf4(C->s, H.ncoarse, H.d_cinvf, vn.b, vn.x);  // the FP32 copy of the inverse
   } else if (H.d_cinvf) {
@@ -1083,11 +1088,13 @@ bool coarse_inverse(uft_ctx *C, hipStream_t s) {
     if (H.d_cinv2 && H.lag_spare) return true;
     // this operator's inverse before the solve (also the first refresh when lagged)
     uftk::dense_spd_sweep_bsr(s, H.ncoarse, Lc.nblk, Lc.d_brow, Lc.d_ci, Lc.d_val, H.d_cmat, H.d_cinv, H.d_cinvf);
+    if (H.d_cinvh) uftk::dense_to_half_rows(s, H.ncoarse, H.d_cinvf, H.d_cinvh, H.d_cinvs);
     H.lag_spare = H.d_cinv2 != nullptr;
     return false;
   }
   uftk::bsr_to_dense(s, Lc.nunit, Lc.d_rp, Lc.d_ci, Lc.d_val, H.ncoarse, H.d_cmat);
   uftk::dense_spd_inverse(s, H.ncoarse, H.d_cmat, H.d_cinv, H.d_cinvf);
+  if (H.d_cinvh) uftk::dense_to_half_rows(s, H.ncoarse, H.d_cinvf, H.d_cinvh, H.d_cinvs);
   return false;
 }
 // The lagged inverse of the current coarsest operator into the spare buffers
@@ -1100,6 +1107,7 @@ void lag_inverse(uft_ctx *C) {
   AmgLevel &Lc = H.L[H.nlev - 1];
   uftk::dense_spd_sweep_bsr(C->side[2], H.ncoarse, Lc.nblk, Lc.d_brow, Lc.d_ci, Lc.d_val, H.d_cmat, H.d_cinv2,
                             H.d_cinvf2);
+  if (H.d_cinvh2) uftk::dense_to_half_rows(C->side[2], H.ncoarse, H.d_cinvf2, H.d_cinvh2, H.d_cinvs2);
   (void)hipEventRecord(C->ev_inv, C->side[2]);
   H.lag_pending = true;
 }
@@ -1158,6 +1166,8 @@ int amg_refresh_impl(uft_ctx *C, const double *xphys) {
     UFT_HIP(C, hipStreamWaitEvent(s, C->ev_inv, 0));
     std::swap(H.d_cinv, H.d_cinv2);
     std::swap(H.d_cinvf, H.d_cinvf2);
+    std::swap(H.d_cinvh, H.d_cinvh2);
+    std::swap(H.d_cinvs, H.d_cinvs2);
     H.lag_pending = false;
   }
   C->cur_xphys = xphys;
@@ -2583,10 +2593,17 @@ int uft_setup(uft_ctx *C) {
       if ((rc = dalloc_n(C, &A.d_cinv, A.ncoarse * A.ncoarse))) return rc;
       if (is_mixed(C))
         if ((rc = dalloc_n(C, &A.d_cinvf, A.ncoarse * A.ncoarse))) return rc;
+      // the int8-smoother form's cycle applies the inverse from a row-scaled
+      // FP16 copy (one design, one load case; the load cases and batches
+      // keep FP32 and FP64)
+      const bool half = C->P.precision == UFT_MIXED_H && !(C->P.load_cases == 1 && C->nlc > 1) && !C->bt.nseg &&
+                        nc % 8 == 0 && nc > UFT_TAIL_MAX;
+      if (half && ((rc = dalloc_n(C, &A.d_cinvh, nc * nc)) || (rc = dalloc_n(C, &A.d_cinvs, nc)))) return rc;
       if (C->P.coarse_lag && nc >= UFT_SWEEP_MIN) {
         if ((rc = dalloc_n(C, &A.d_cinv2, nc * nc))) return rc;
         if (is_mixed(C))
           if ((rc = dalloc_n(C, &A.d_cinvf2, nc * nc))) return rc;
+        if (half && ((rc = dalloc_n(C, &A.d_cinvh2, nc * nc)) || (rc = dalloc_n(C, &A.d_cinvs2, nc)))) return rc;
         if (!C->ev_inv && hipEventCreateWithFlags(&C->ev_inv, hipEventDisableTiming) != hipSuccess)
           return uft_set_error(C, UFT_ERR_HIP, "hipEventCreateWithFlags failed");
       }

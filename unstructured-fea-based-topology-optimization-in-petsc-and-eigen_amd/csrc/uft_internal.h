@@ -165,6 +165,11 @@ struct AmgHier {
   // inverted into the spare buffers on side stream 2
   double *d_cinv2 = nullptr;
   float *d_cinvf2 = nullptr;
+  // UFT_MIXED_H (one design, one load case): the cycle's copy of the inverse
+  // in FP16 scaled by row (entries / row max) with the FP32 row scales, and
+  // the spare pair of the lagged inverse
+  uint16_t *d_cinvh = nullptr, *d_cinvh2 = nullptr;
+  float *d_cinvs = nullptr, *d_cinvs2 = nullptr;
   bool lag_spare = false;    // the spare buffers hold the inverse of the last refresh (swap at the next)
   bool lag_pending = false;  // a spare inverse is being computed on side[2] (event ev_inv)
 };
@@ -551,6 +556,10 @@ void dense_spd_sweep_bsr(hipStream_t s, int64_t n, int64_t nblk, const int32_t *
                          const double *val, double *S, double *Ainv, float *Ainvf);
 // y = A x, row-major FP32 n x n, n a multiple of 4 and 16-byte aligned operands
 void dense_mvf4(hipStream_t s, int64_t n, const float *A, const float *x, float *y);
+// row-scaled FP16 copy of an FP32 matrix (entries / row max, FP32 row scales)
+// and its product (n a multiple of 8)
+void dense_to_half_rows(hipStream_t s, int64_t n, const float *A, uint16_t *H, float *sc);
+void dense_mvh(hipStream_t s, int64_t n, const uint16_t *H, const float *sc, const float *x, float *y);
 void dense_mvf(hipStream_t s, int64_t n, const float *A, const float *x, float *y);
 // multi-load-case kernels (kernels_mrhs.hip)
 void mr_cheb_first(hipStream_t s, int nr, int64_t n, const double *b, const double *dinv, double it, double *r,
