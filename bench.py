@@ -74,13 +74,18 @@ def algorithmic_bytes_elem_phase(npe, ne, ndof, w=8, nrhs=1):
     return ne * (S * w + 4 * npe + 8 + (4 if w == 2 else 0)) + ndof * wv
 
 
-def algorithmic_bytes_fine_ell(blocks, ndof, col_bytes, value_bytes, scaled=True, nrhs=1):
+def algorithmic_bytes_fine_ell(blocks, ndof, col_bytes, value_bytes, scaled=True, nrhs=1, step=False):
     """One apply of an assembled fine operator: every nonzero 3x3 node block
     once (value_bytes: 12 for int8 entries with a block scale, 18 for FP16, 36
     for FP32, and its column index) for all nrhs right-hand sides, u read once
     and one output vector per right-hand side (FP32), the row scales of the
-    scaled smoother forms, and the fixed-DOF byte per node (padding slots and
-    the epilogue's further vectors not counted)."""
+    scaled smoother forms, and the fixed-DOF byte per node (padding slots not
+    counted).  step: the launch is a fused Chebyshev step (r -= A u;
+    d = c1 u + c2 D^-1 r; x += d), whose epilogue reads and writes r and x
+    and writes d instead of an output vector, and reads D^-1: per DOF
+    4 nrhs (u) + 4 (row scale) + 4 (D^-1) + 20 nrhs (r, x, d) bytes."""
+    if step:
+        return blocks * (value_bytes + col_bytes) + ndof * (24 * nrhs + 8) + ndof // 3
     return blocks * (value_bytes + col_bytes) + ndof * 4 * (2 * nrhs + (1 if scaled else 0)) + ndof // 3
 
 
@@ -398,12 +403,28 @@ def main():
             else:
                 kn = "k_fine_ell_op_q"
             fmt = "FP16" if r["ell"][3] == 18 else "int8 (FP32 block scale)"
-            rooflines.append(elem_roofline(
+            op_bytes = algorithmic_bytes_fine_ell(r["ell"][1], ndof, r["ell"][2], r["ell"][3], nrhs=nrhs)
+            step_bytes = algorithmic_bytes_fine_ell(r["ell"][1], ndof, r["ell"][2], r["ell"][3], nrhs=nrhs,
+                                                    step=True)
+            b2b = r["b2b"].get(2)
+            rl = elem_roofline(
                 main_form, npe, ne, ndof, r["prof"][2], r["prof"][3], 2,
-                f"{kn} (assembled {fmt} 3x3 node blocks of the smoother, sliced ELL, whole apply with its "
-                "row epilogue, mixed form)", f"{args.workload}_{main_form}_fine_ell", nrhs,
-                algo=algorithmic_bytes_fine_ell(r["ell"][1], ndof, r["ell"][2], r["ell"][3], nrhs=nrhs),
-                b2b_us=r["b2b"].get(2)))
+                f"{kn} (assembled {fmt} 3x3 node blocks of the smoother, sliced ELL, a fused Chebyshev step: "
+                "the apply with its row epilogue, mixed form)", f"{args.workload}_{main_form}_fine_ell", nrhs,
+                algo=step_bytes if b2b else op_bytes, b2b_us=b2b)
+            # the operator alone (K u: blocks, u, one output, row scales), and
+            # the in-run events (every epilogue mode of the smoother's launches)
+            # priced on the operator alone
+            rl["operator_bytes_per_launch"] = op_bytes
+            if rl["avg_launch_us"]:
+                rl["operator_frac"] = op_bytes / (rl["avg_launch_us"] * 1e-6) / 1e9 / HBM_PEAK_GBS
+            n_in = r["prof"][2][0]
+            if n_in:
+                rl["achieved_in_run"] = op_bytes / (r["prof"][2][1] / n_in / 1e3) / 1e9
+            if b2b:
+                rl["algorithmic_bytes_note"] = ("the Chebyshev step's operator bytes plus its epilogue's vectors, "
+                                                "each read or written once")
+            rooflines.append(rl)
         else:
             rooflines.append(elem_roofline(main_form, npe, ne, ndof, r["prof"][2], r["prof"][3], w, kname,
                                            f"{args.workload}_{main_form}_elem_apply", nrhs))

@@ -56,8 +56,9 @@ enum uft_size_id { UFT_SIZE_NE = 0, UFT_SIZE_NN = 1, UFT_SIZE_NDOF = 2, UFT_SIZE
  * fine_assembled = 1 (default) the fine level as assembled 3x3 node blocks
  * of D^-1 A in int8 with an FP32 scale per block (one right-hand side) or
  * in FP16 (several load cases), level 1's blocks in FP16 scaled to unit
- * diagonal, the smoothed prolongator's cycle copies in FP16; with
- * fine_assembled = 0 FP16 element blocks scaled per element.  FP32
+ * diagonal, the smoothed prolongator's cycle copies in FP16, the coarsest
+ * inverse applied from a row-scaled FP16 copy (one design, one load case);
+ * with fine_assembled = 0 FP16 element blocks scaled per element.  FP32
  * arithmetic throughout the cycle. */
 enum uft_precision { UFT_FP64 = 0, UFT_MIXED = 1, UFT_MIXED_H = 2 };
 
@@ -94,8 +95,10 @@ typedef struct uft_params {
                           the fine operators are assembled at each hierarchy refresh into 3x3 node blocks,
                           the smoother's rows divided by the diagonal in int8 with an FP32 scale per block
                           (FP16 with several load cases) and FP32 for the outer product (one
-                          launch per apply, no element-local round trip); 0: the element blocks.  Fixed at
-                          setup. */
+                          launch per apply, no element-local round trip); with 5-8 load cases the
+                          eight-right-hand-side products run on node tiles whose neighbour entries are
+                          staged in LDS; 2: the same without the node tiles; 0: the element blocks.
+                          Fixed at setup. */
   int coarse_dense_max; /* the coarsest multigrid level is solved with its dense inverse (recomputed at every
                            hierarchy refresh), so coarsening goes on until a level has at most this many
                            unknowns (default 4096, at most 6144; design batches 3072 per design at most).
