@@ -834,6 +834,65 @@ __global__ __launch_bounds__(TPB) void k_mr_bsr_op(int64_t nbrows, const int32_t
   mr_epi_partials<T, NR>(op, dv, blk);
 }
 
+// The same product with the FP16 blocks of the int8-smoother form: the level
+// stored as H = D^-1/2 A D^-1/2 in FP16 with dh = D^1/2 per row in FP32, so
+// (A u)_i = dh_i sum_j H_ij dh_j u_j; 72 bytes per block instead of 144.
+// (Taking two slot rounds per lane at once, with every load of both issued
+// first, measured slower on c4a's level 1: 38 against 30 us, the extra
+// registers cost more waves than the overlap gains.)
+template <int NR>
+__global__ __launch_bounds__(TPB) void k_mr_bsr_op_h(int64_t nbrows, const int32_t *__restrict__ rp,
+                                                     const int32_t *__restrict__ ci, const uint2 *__restrict__ valh,
+                                                     const float *__restrict__ dh, const float *__restrict__ u,
+                                                     MrOp<float> op) {
+  constexpr int SLOTS = 64 / NR;
+  const int64_t blk = uftr::xcd_block(blockIdx.x, gridDim.x);
+  const int64_t t = blk * TPB + threadIdx.x;
+  const int64_t A = t / 64;
+  const int lane = (int)(t % 64), l = lane / NR, rr = lane % NR;
+  float acc[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  if (A < nbrows) {
+    for (int k = rp[A] + l; k < rp[A + 1]; k += SLOTS) {
+      const int64_t B = ci[k];
+      const float *xb = u + (B * 6) * NR + rr;
+      const float2 *db = reinterpret_cast<const float2 *>(dh + B * 6);
+      float x[6];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        const float2 d = db[j];
+        x[2 * j] = xb[(2 * j) * NR] * d.x;
+        x[2 * j + 1] = xb[(2 * j + 1) * NR] * d.y;
+      }
+      const uint2 *vb = valh + (size_t)k * 9;
+#pragma unroll
+      for (int w = 0; w < 9; ++w) {
+        const uint2 h = vb[w];
+        const unsigned hw[2] = {h.x, h.y};
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+          const float2 f = __half22float2(*reinterpret_cast<const __half2 *>(&hw[q]));
+          const int e = 4 * w + 2 * q;  // entries e, e+1 of the row-major block (same row: 6 is even)
+          acc[e / 6] += f.x * x[e % 6] + f.y * x[e % 6 + 1];
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+#pragma unroll
+    for (int off = 32; off >= NR; off >>= 1) acc[i] += __shfl_xor(acc[i], off, 64);
+  }
+  double dv[3] = {0.0, 0.0, 0.0};
+  if (l == 0 && A < nbrows) {
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+      const int64_t row = A * 6 + i;
+      mr_epi(op, row, row * NR + rr, acc[i] * dh[row], u[row * NR + rr], dv);
+    }
+  }
+  mr_epi_partials<float, NR>(op, dv, blk);
+}
+
 // ---------------------------------------------------------------- transfers
 // wavefront per coarse unit; lane = slot * NR + rr
 template <int BS, typename T, int NR>
@@ -1399,6 +1458,12 @@ void mr_bsr_op(hipStream_t s, int nr, int64_t nb_rows, const int32_t *rp, const 
                const double *u, const RowOp<double> &o) { mr_bsr_op_t<double>(s, nr, nb_rows, rp, ci, val, u, o); }
 void mr_bsr_op(hipStream_t s, int nr, int64_t nb_rows, const int32_t *rp, const int32_t *ci, const float *val,
                const float *u, const RowOp<float> &o) { mr_bsr_op_t<float>(s, nr, nb_rows, rp, ci, val, u, o); }
+void mr_bsr_op_h(hipStream_t s, int nr, int64_t nb_rows, const int32_t *rp, const int32_t *ci, const uint16_t *valh,
+                 const float *dh, const float *u, const RowOp<float> &o) {
+  const MrOp<float> op = to_mr(o);
+  UFT_NR_SWITCH(nr, (k_mr_bsr_op_h<NR><<<nblk(nb_rows * 64), TPB, 0, s>>>(
+                        nb_rows, rp, ci, reinterpret_cast<const uint2 *>(valh), dh, u, op)));
+}
 int mr_bsr_blocks(int64_t nb_rows) { return (int)nblk(nb_rows * 64); }
 
 template <typename T>

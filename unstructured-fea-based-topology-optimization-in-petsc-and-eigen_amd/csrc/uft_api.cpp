@@ -669,6 +669,9 @@ void mr_lev_op(uft_ctx *C, int l, const T *u, const RowOp<T> &op) {
   constexpr bool f64 = std::is_same<T, double>::value;
   if (l == 0)
     count_apply(C, !f64 && C->fell.d_v8 ? mr_ell_bytes(C, 18, true, nr) : mr_fine_bytes(C, f64 ? 8 : 2, f64 ? 8 : 4, nr));
+  else if (!f64 && C->amg.L[l].d_valh)  // FP16 blocks, FP32 row scales and vectors
+    count_apply(C, (double)C->amg.L[l].nblk * (72.0 + 4) + 2.0 * nr * (double)C->amg.L[l].n * 4 +
+                       4.0 * (double)C->amg.L[l].n);
   else
     count_apply(C, mr_coarse_bytes(C, l, f64 ? 8 : 4, nr));
   if (l == 0) {
@@ -688,6 +691,12 @@ void mr_lev_op(uft_ctx *C, int l, const T *u, const RowOp<T> &op) {
     if (ev) (void)hipEventRecord(ev[2], C->s);
   } else {
     const AmgLevel &Lv = C->amg.L[l];
+    if constexpr (std::is_same<T, float>::value) {
+      if (Lv.d_valh) {  // UFT_MIXED_H: the level's FP16 smoother blocks
+        uftk::mr_bsr_op_h(C->s, nr, Lv.nunit, Lv.d_rp, Lv.d_ci, Lv.d_valh, Lv.d_dh, u, op);
+        return;
+      }
+    }
     const T *val;
     if constexpr (std::is_same<T, double>::value) val = Lv.d_val; else val = Lv.d_valf;
     uftk::mr_bsr_op(C->s, nr, Lv.nunit, Lv.d_rp, Lv.d_ci, val, u, op);
