@@ -369,7 +369,23 @@ def main():
     multi = WORKLOADS[args.workload][5] == 2
     nrhs = 8 if multi else 1
     k64 = f"k_mr_elem<{npe},double,8>" if multi else f"k_elem_apply_v<{npe},double>"
-    if "fp64" in res:
+    if "fp64" in res and res["fp64"]["ell"][3] == 72:
+        # the FP64 form's solves apply the assembled FP64 node blocks: blocks
+        # (72 bytes and the column) once, u read and the output written in FP64
+        q = res["fp64"]
+        algo = q["ell"][1] * (72 + q["ell"][2]) + ndof * 8 * 2 + ndof // 3
+        rooflines.append(elem_roofline("fp64", npe, ne, ndof, q["prof"][0], q["prof"][1], 8,
+                                       "k_fine_ell_op64 (assembled FP64 3x3 node blocks of the refresh, the FP64 "
+                                       "form's whole K u in one launch)", f"{args.workload}_fp64_fine_ell", nrhs,
+                                       algo=algo, b2b_us=q["b2b"].get(0)))
+        if q["b2b"].get(3):
+            rooflines.append({"bound": "hbm", "role": "fp64_elem_phase_unused", "kernel": f"{k64} (the element "
+                              "phase of the matrix-free apply, which the FP64 solves no longer use; timed for "
+                              "comparison)", "avg_launch_us": q["b2b"][3],
+                              "algorithmic_bytes_per_launch": algorithmic_bytes_elem_phase(npe, ne, ndof, 8, nrhs),
+                              "achieved": algorithmic_bytes_elem_phase(npe, ne, ndof, 8, nrhs) / (q["b2b"][3] * 1e-6)
+                              / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s", "launches": 0, "total_ms_timed": 0.0})
+    elif "fp64" in res:
         q = res["fp64"]
         rooflines.append(elem_roofline("fp64", npe, ne, ndof, q["prof"][0], q["prof"][1], 8,
                                        f"{k64} (FP64 element phase of the fine stiffness apply, FP64 form)",

@@ -373,6 +373,52 @@ __global__ __launch_bounds__(TPB) void k_fine_ell_op32(int nn, const int64_t *__
   epi_partials_m<float, MODE>(op, dv);
 }
 
+// The FP64 form's operator on the same pattern: the FP64 blocks assembled at
+// the refresh from the FP64 element matrices (72 bytes per slot: entries
+// 2q, 2q+1 in plane q at e8[q ns + slot], entry 8 in e1, so every load of a
+// wavefront reads consecutive memory), q = K u in FP64 with the fused row
+// epilogues; one launch per apply instead of the element phase, its
+// element-local results and the node phase.
+template <int MODE, typename CI>
+__global__ __launch_bounds__(TPB) void k_fine_ell_op64(int nn, int64_t ns, const int64_t *__restrict__ off,
+                                                       const int32_t *__restrict__ width,
+                                                       const CI *__restrict__ col, const double2 *__restrict__ e8,
+                                                       const double *__restrict__ e1, const uint8_t *__restrict__ fixm,
+                                                       const double *__restrict__ u, RowOp<double> op) {
+  const int n = blockIdx.x * TPB + threadIdx.x;
+  double dv[3] = {0.0, 0.0, 0.0};
+  if (n < nn) {
+    EpiPre<double, MODE> pre[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) pre[c].load(op, u, 3 * (int64_t)n + c);
+    const int w = n / UFT_ELL_SLICE;
+    const int64_t base = off[w] + n % UFT_ELL_SLICE;
+    const int W = width[w];
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+#pragma unroll 3
+    for (int s = 0; s < W; ++s) {
+      const int64_t k = base + (int64_t)s * UFT_ELL_SLICE;
+      const int64_t j = std::is_same<CI, int16_t>::value ? n + (int64_t)col[k] : (int64_t)col[k];
+      typedef double v2d __attribute__((ext_vector_type(2)));
+      const v2d b0 = __builtin_nontemporal_load(reinterpret_cast<const v2d *>(&e8[k]));
+      const v2d b1 = __builtin_nontemporal_load(reinterpret_cast<const v2d *>(&e8[ns + k]));
+      const v2d b2 = __builtin_nontemporal_load(reinterpret_cast<const v2d *>(&e8[2 * ns + k]));
+      const v2d b3 = __builtin_nontemporal_load(reinterpret_cast<const v2d *>(&e8[3 * ns + k]));
+      const double b8 = __builtin_nontemporal_load(&e1[k]);
+      const double x0 = u[3 * j], x1 = u[3 * j + 1], x2 = u[3 * j + 2];
+      a0 += b0.x * x0 + b0.y * x1 + b1.x * x2;
+      a1 += b1.y * x0 + b2.x * x1 + b2.y * x2;
+      a2 += b3.x * x0 + b3.y * x1 + b8 * x2;
+    }
+    const double acc[3] = {a0, a1, a2};
+    const unsigned m = fixm[n];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      row_epi_m<double, MODE>(op, 3 * (int64_t)n + c, ((m >> c) & 1u) ? pre[c].ui : acc[c], pre[c], dv);
+  }
+  epi_partials_m<double, MODE>(op, dv);
+}
+
 // Assembly at a hierarchy refresh: lane (n, slot s) sums E(x_e) KE_e[a][b]
 // (3x3) over the slot's element contributions (incidence order, FP64),
 // zeroes fixed columns, divides row c by the diagonal 1 / dinv_3n+c and
@@ -395,7 +441,8 @@ __global__ __launch_bounds__(TPB) void k_fine_ell_fill(int nn, int ne, const int
                                                        double *__restrict__ a64, float *__restrict__ a32,
                                                        const int32_t *__restrict__ kmap, uint4 *__restrict__ tv8,
                                                        uint16_t *__restrict__ tv1, float4 *__restrict__ tw8,
-                                                       float *__restrict__ tw1) {
+                                                       float *__restrict__ tw1, double2 *__restrict__ e8,
+                                                       double *__restrict__ e1, int64_t ns) {
   constexpr int ND = 3 * NPE;
   const int n = blockIdx.x * TPB + threadIdx.x;
   const int s = blockIdx.y;
@@ -433,6 +480,11 @@ __global__ __launch_bounds__(TPB) void k_fine_ell_fill(int nn, int ne, const int
   if (a32) {
 #pragma unroll
     for (int t = 0; t < 9; ++t) a32[9 * k + t] = (float)b[t];
+  }
+  if (e8) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) e8[q * ns + k] = make_double2(b[2 * q], b[2 * q + 1]);
+    e1[k] = b[8];
   }
   if (!q8 && !v8) return;  // the smoothed prolongator's blocks only
   // the node-tile copies (several load cases) at the slot's tiled position
@@ -967,22 +1019,37 @@ void fine_ell_fill(hipStream_t s, const uft_ctx *C, const double *Ee, const doub
     if (C->npe == 8)
       k_fine_ell_fill<8, float2><<<g, TPB, 0, s>>>(nn, ne, F.d_off, F.d_width, F.d_col, F.d_cptr, F.d_clist, KEf, Ee,
                                                    C->d_fixm, dinv, F.d_v8, F.d_v1, F.d_q8, F.d_qs, F.d_dg, F.d_w8,
-                                                   F.d_w1, a64, a32, F.d_kmap, F.d_tv8, F.d_tv1, F.d_tw8, F.d_tw1);
+                                                   F.d_w1, a64, a32, F.d_kmap, F.d_tv8, F.d_tv1, F.d_tw8, F.d_tw1,
+                                                   nullptr, nullptr, F.nslot);
     else
       k_fine_ell_fill<4, float2><<<g, TPB, 0, s>>>(nn, ne, F.d_off, F.d_width, F.d_col, F.d_cptr, F.d_clist, KEf, Ee,
                                                    C->d_fixm, dinv, F.d_v8, F.d_v1, F.d_q8, F.d_qs, F.d_dg, F.d_w8,
-                                                   F.d_w1, a64, a32, F.d_kmap, F.d_tv8, F.d_tv1, F.d_tw8, F.d_tw1);
+                                                   F.d_w1, a64, a32, F.d_kmap, F.d_tv8, F.d_tv1, F.d_tw8, F.d_tw1,
+                                                   nullptr, nullptr, F.nslot);
     return;
   }
   const double2 *KE = (const double2 *)C->d_KE;
   if (C->npe == 8)
     k_fine_ell_fill<8, double2><<<g, TPB, 0, s>>>(nn, ne, F.d_off, F.d_width, F.d_col, F.d_cptr, F.d_clist, KE, Ee,
                                                   C->d_fixm, dinv, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
-                                                  nullptr, a64, a32, nullptr, nullptr, nullptr, nullptr, nullptr);
+                                                  nullptr, a64, a32, nullptr, nullptr, nullptr, nullptr, nullptr,
+                                                  a64 ? F.d_e8 : nullptr, F.d_e1, F.nslot);
   else
     k_fine_ell_fill<4, double2><<<g, TPB, 0, s>>>(nn, ne, F.d_off, F.d_width, F.d_col, F.d_cptr, F.d_clist, KE, Ee,
                                                   C->d_fixm, dinv, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
-                                                  nullptr, a64, a32, nullptr, nullptr, nullptr, nullptr, nullptr);
+                                                  nullptr, a64, a32, nullptr, nullptr, nullptr, nullptr, nullptr,
+                                                  a64 ? F.d_e8 : nullptr, F.d_e1, F.nslot);
+}
+void fine_ell_op64(hipStream_t s, const uft_ctx *C, const double *u, const RowOp<double> &op) {
+  const auto &F = C->fell;
+  if (F.d_cd)
+    UFT_MODE_SWITCH(op.mode, (k_fine_ell_op64<MODE, int16_t><<<nblk(C->nn), TPB, 0, s>>>(
+                                  (int)C->nn, F.nslot, F.d_off, F.d_width, F.d_cd, F.d_e8, F.d_e1, C->d_fixm, u,
+                                  op)))
+  else
+    UFT_MODE_SWITCH(op.mode, (k_fine_ell_op64<MODE, int32_t><<<nblk(C->nn), TPB, 0, s>>>(
+                                  (int)C->nn, F.nslot, F.d_off, F.d_width, F.d_col, F.d_e8, F.d_e1, C->d_fixm, u,
+                                  op)))
 }
 void fine_ell_op32(hipStream_t s, const uft_ctx *C, const float *u, const RowOp<float> &op) {
   const auto &F = C->fell;

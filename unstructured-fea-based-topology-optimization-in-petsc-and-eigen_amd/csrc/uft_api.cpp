@@ -149,6 +149,11 @@ double fine_bytes_lo(const uft_ctx *C, int w) {
   const int S = C->npe == 8 ? 300 : 78;
   return (double)C->ne * (S * w + 4 * C->npe + 8 + (w == 2 ? 4 : 0)) + 2.0 * (double)C->ndof * 4;
 }
+// the FP64 form's assembled operator: FP64 blocks, FP64 u and output
+double ell_bytes64(const uft_ctx *C) {
+  const int cb = C->fell.d_cd ? 2 : 4;
+  return (double)C->fell.nnzb * (72 + cb) + (double)C->ndof * 8 * 2 + (double)C->nn;
+}
 double ell_bytes(const uft_ctx *C, int value_bytes, bool scaled) {
   const int cb = C->fell.d_cd ? 2 : 4;
   return (double)C->fell.nnzb * (value_bytes + cb) + (double)C->ndof * 4 * (scaled ? 3 : 2) + (double)C->nn;
@@ -228,6 +233,21 @@ int graph_launch(uft_ctx *C, const void *k0, const void *k1, const void *k2, F &
 
 
 void fine_apply(uft_ctx *C, const double *xphys, const double *x, double *y, const double *dotw, double *part) {
+  if (C->fell.d_e8 && xphys == C->cur_xphys && (!dotw || dotw == x)) {
+    // the assembled FP64 blocks of this density's hierarchy refresh
+    count_apply(C, ell_bytes64(C));
+    hipEvent_t *ev = prof_triple(C, 0);
+    if (ev) (void)hipEventRecord(ev[0], C->s);
+    RowOp<double> op;
+    op.mode = dotw ? ROW_STOREDOT : ROW_STORE;
+    op.y = y;
+    op.part = C->d_part;
+    uftk::fine_ell_op64(C->s, C, x, op);
+    if (ev) (void)hipEventRecord(ev[1], C->s);
+    if (ev) (void)hipEventRecord(ev[2], C->s);
+    if (dotw) uftk::final_sum(C->s, C->d_part, uftk::gather_op_blocks(C->nn), 1, part);
+    return;
+  }
   count_apply(C, fine_bytes64(C));
   uftk::apply_K(C->s, C, xphys, x, y, dotw, part, prof_triple(C, 0));
 }
@@ -316,10 +336,16 @@ const float *tr_P(const AmgLevel &L, float *) { return L.d_Pf; }
 // Operator apply with a per-row epilogue on level l (l = 0: matrix-free fine
 // operator of the current density; above: 6x6 block-sparse).
 void lev_op(uft_ctx *C, int l, const double *u, const RowOp<double> &op) {
-  count_apply(C, l == 0 ? fine_bytes64(C) : coarse_bytes(C, l, 8));
+  count_apply(C, l == 0 ? (C->fell.d_e8 ? ell_bytes64(C) : fine_bytes64(C)) : coarse_bytes(C, l, 8));
   if (l == 0) {
     hipEvent_t *ev = prof_triple(C, 0);
     if (ev) (void)hipEventRecord(ev[0], C->s);
+    if (C->fell.d_e8) {  // the assembled FP64 blocks of the current hierarchy refresh: one launch
+      uftk::fine_ell_op64(C->s, C, u, op);
+      if (ev) (void)hipEventRecord(ev[1], C->s);
+      if (ev) (void)hipEventRecord(ev[2], C->s);
+      return;
+    }
     uftk::elem_phase(C->s, C, C->elem_variant, C->cur_xphys, u, C->d_Ye);
     if (ev) (void)hipEventRecord(ev[1], C->s);
     uftk::gather_op(C->s, C, u, C->d_Ye, op);
@@ -2555,6 +2581,11 @@ int uft_setup(uft_ctx *C) {
             return rc;
         } else if ((rc = dalloc_n(C, &Lv.d_sa_V, 18 * Lv.sa_nv)) || (rc = dalloc_n(C, &C->d_a64, 9 * C->fell.nslot))) {
           return rc;
+        } else if (!C->bt.nseg && !(C->P.load_cases == 1 && C->nlc > 1)) {
+          // one design, one load case: the same FP64 blocks also serve the
+          // solve's fine operator (written by the refresh's assembly)
+          if ((rc = dalloc_n(C, &C->fell.d_e8, 4 * C->fell.nslot)) || (rc = dalloc_n(C, &C->fell.d_e1, C->fell.nslot)))
+            return rc;
         }
         if ((rc = sa_setup_lists(C, h))) return rc;
       } else if (l == 0) {  // density-free Galerkin entries of the fine level
@@ -3205,11 +3236,11 @@ extern "C" int uft_fine_operator_info(const uft_ctx *C, int64_t *slots, int64_t 
                                       int *value_bytes) {
   if (!C || !slots || !blocks || !col_bytes || !value_bytes) return UFT_ERR_ARG;
   if (!C->is_setup) return UFT_ERR_STATE;
-  const bool on = fine_blocks(C);
+  const bool on = fine_blocks(C) || C->fell.d_e8;
   *slots = on ? C->fell.nslot : 0;
   *blocks = on ? C->fell.nnzb : 0;
   *col_bytes = on ? (C->fell.d_cd ? 2 : 4) : 0;
-  *value_bytes = on ? (C->fell.d_q8 ? 12 : 18) : 0;
+  *value_bytes = on ? (C->fell.d_q8 ? 12 : (C->fell.d_v8 ? 18 : 72)) : 0;
   return UFT_OK;
 }
 

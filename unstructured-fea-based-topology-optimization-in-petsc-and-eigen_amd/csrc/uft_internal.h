@@ -320,6 +320,8 @@ struct uft_ctx {
     double *d_hv = nullptr;      // per slot: the filter's Helmholtz entry (FP64, assembled at setup)
     float4 *d_w8 = nullptr;      // per slot: FP32 block entries 0..7 (the outer operator)
     float *d_w1 = nullptr;       // per slot: FP32 entry 8
+    double2 *d_e8 = nullptr;     // FP64 form (one design, one load case): FP64 entries 2q, 2q+1 of slot k at [q nslot + k]
+    double *d_e1 = nullptr;      // ... and entry 8 (the fine operator of the solves)
     // node tiles (several load cases): the same blocks in tile order, see
     // NodeTiles; the assembly writes them beside the node-ordered ones
     int64_t ntile = 0, tslot = 0;
@@ -677,6 +679,9 @@ void mr_sa_prolong_comb(hipStream_t s, int nr, const AmgLevel &L0, int64_t ncu, 
 void fine_ell_op(hipStream_t s, const uft_ctx *C, const float *u, const RowOp<float> &op);
 // the same pattern with FP32 entries (unscaled): the mixed forms' outer operator
 void fine_ell_op32(hipStream_t s, const uft_ctx *C, const float *u, const RowOp<float> &op);
+// the FP64 form's fine operator from the assembled FP64 node blocks of the
+// last hierarchy refresh (fell.d_e8 / d_e1)
+void fine_ell_op64(hipStream_t s, const uft_ctx *C, const double *u, const RowOp<double> &op);
 // the filter's Helmholtz operator on the node pattern: assembled once (KF is
 // density-free), then y = KF x (with dot_out: x.y into it via d_part)
 void helm_ell_fill(hipStream_t s, const uft_ctx *C);
