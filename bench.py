@@ -85,7 +85,7 @@ def algorithmic_bytes_fine_ell(blocks, ndof, col_bytes, value_bytes, scaled=True
     and writes d instead of an output vector, and reads D^-1: per DOF
     4 nrhs (u) + 4 (row scale) + 4 (D^-1) + 20 nrhs (r, x, d) bytes."""
     if step:
-        return blocks * (value_bytes + col_bytes) + ndof * (24 * nrhs + 8) + ndof // 3
+        return blocks * (value_bytes + col_bytes) + ndof * (24 * nrhs + (8 if scaled else 4)) + ndof // 3
     return blocks * (value_bytes + col_bytes) + ndof * 4 * (2 * nrhs + (1 if scaled else 0)) + ndof // 3
 
 
@@ -219,7 +219,7 @@ def run_form(args, precision):
         if WORKLOADS[args.workload][5] == 2:
             kinds = {"fp64": (), "mixed": (1,), "mixed_h": (2, 1)}[precision]
         else:
-            kinds = {"fp64": (0, 3), "mixed": (1,), "mixed_h": (2, 1)}[precision]
+            kinds = {"fp64": (0, 3), "mixed": (2, 1) if ell[0] else (1,), "mixed_h": (2, 1)}[precision]
         for k in kinds:
             b2b[k] = ctx.time_fine_operator(k, 20)
     ctx.close()
@@ -417,11 +417,11 @@ def main():
                 kn = ("k_mr_tile_op<FP16> (8 load cases on node tiles, neighbour entries staged in LDS)"
                       if r["tiles"][0] else f"k_mr_ell_op<{nrhs},FP16> (8 load cases)")
             else:
-                kn = "k_fine_ell_op_q"
-            fmt = "FP16" if r["ell"][3] == 18 else "int8 (FP32 block scale)"
-            op_bytes = algorithmic_bytes_fine_ell(r["ell"][1], ndof, r["ell"][2], r["ell"][3], nrhs=nrhs)
-            step_bytes = algorithmic_bytes_fine_ell(r["ell"][1], ndof, r["ell"][2], r["ell"][3], nrhs=nrhs,
-                                                    step=True)
+                kn = "k_fine_ell_op32" if r["ell"][3] == 36 else "k_fine_ell_op_q"
+            fmt = {18: "FP16", 12: "int8 (FP32 block scale)", 36: "FP32"}.get(r["ell"][3], "low-precision")
+            sc = r["ell"][3] != 36  # the low-precision smoother blocks are scaled by row
+            op_bytes = algorithmic_bytes_fine_ell(r["ell"][1], ndof, r["ell"][2], r["ell"][3], sc, nrhs)
+            step_bytes = algorithmic_bytes_fine_ell(r["ell"][1], ndof, r["ell"][2], r["ell"][3], sc, nrhs, step=True)
             b2b = r["b2b"].get(2)
             rl = elem_roofline(
                 main_form, npe, ne, ndof, r["prof"][2], r["prof"][3], 2,

@@ -91,8 +91,9 @@ typedef struct uft_params {
                           kernel. */
   int cheb_degree_coarse; /* Chebyshev degree on multigrid levels 2 and below (their applies are bound by
                              launch latency, not by bandwidth); 0 (default): cheb_degree */
-  int fine_assembled;  /* UFT_MIXED_H with superposed load cases (one design or a design batch): 1 (default)
-                          the fine operators are assembled at each hierarchy refresh into 3x3 node blocks,
+  int fine_assembled;  /* UFT_MIXED_H, and UFT_MIXED with one load case (FP32 blocks for its smoother and
+                          outer product): 1 (default) the fine operators are assembled at each hierarchy
+                          refresh into 3x3 node blocks,
                           the smoother's rows divided by the diagonal in int8 with an FP32 scale per block
                           (FP16 with several load cases) and FP32 for the outer product (one
                           launch per apply, no element-local round trip); with 5-8 load cases the
@@ -285,10 +286,11 @@ int uft_level_apply(uft_ctx *ctx, int level, int kind, const double *x, double *
  * the solve issues it, on the operators of the last hierarchy refresh and
  * the solver's own work vectors (their contents are overwritten), timed with
  * HIP events on the context's stream: kind 0 the FP64 stiffness apply
- * (element and node phase), 1 the mixed forms' FP32 outer product, 2 the
- * UFT_MIXED_H smoother's operator with its Chebyshev-step epilogue, 3 the
- * FP64 element phase alone. Kind 0 needs the FP64 form, kinds 1 and 2 a
- * mixed form; one design.  With several load cases (load_cases = 1) kinds
+ * (the assembled FP64 blocks when the solves use them, else element and
+ * node phase), 1 the mixed forms' FP32 outer product, 2 the smoother's
+ * operator with its Chebyshev-step epilogue (UFT_MIXED_H, or UFT_MIXED on
+ * assembled FP32 blocks), 3 the FP64 element phase alone. Kind 0 needs the
+ * FP64 form, kinds 1 and 2 a mixed form; one design.  With several load cases (load_cases = 1) kinds
  * 0-2 apply to all right-hand sides at once (kind 3 is not available). */
 int uft_time_fine_operator(uft_ctx *ctx, int kind, int reps, double *us);
 

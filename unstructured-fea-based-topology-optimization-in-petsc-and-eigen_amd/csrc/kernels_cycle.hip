@@ -486,7 +486,7 @@ __global__ __launch_bounds__(TPB) void k_fine_ell_fill(int nn, int ne, const int
     for (int q = 0; q < 4; ++q) e8[q * ns + k] = make_double2(b[2 * q], b[2 * q + 1]);
     e1[k] = b[8];
   }
-  if (!q8 && !v8) return;  // the smoothed prolongator's blocks only
+  if (!q8 && !v8 && !w8) return;  // the smoothed prolongator's blocks only
   // the node-tile copies (several load cases) at the slot's tiled position
   const int64_t k2 = kmap ? (int64_t)kmap[k] : -1;
   if (w8) {
@@ -501,6 +501,7 @@ __global__ __launch_bounds__(TPB) void k_fine_ell_fill(int nn, int ne, const int
       tw1[k2] = (float)b[8];
     }
   }
+  if (!q8 && !v8) return;  // UFT_MIXED: the FP32 blocks only
   double a[9];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
@@ -1012,7 +1013,7 @@ void fine_ell_fill(hipStream_t s, const uft_ctx *C, const double *Ee, const doub
   const int nn = (int)C->nn, ne = (int)C->ne;
   const auto &F = C->fell;
   const dim3 g(nblk(nn), (unsigned)F.wmax);
-  const bool blocks = F.d_q8 || F.d_v8;
+  const bool blocks = F.d_q8 || F.d_v8 || F.d_w8;
   if (!blocks && !a64 && !a32) return;
   if (blocks || !C->d_KE) {
     const float2 *KEf = (const float2 *)C->d_KEf;

@@ -184,7 +184,11 @@ inline void count_apply(uft_ctx *C, double bytes) {
 }
 // the assembled smoother blocks (int8 for one right-hand side, FP16 for the
 // multi-load-case form) exist
-bool fine_blocks(const uft_ctx *C) { return C->fell.d_q8 || C->fell.d_v8; }
+// assembled fine blocks that the refresh fills: the low-precision smoother
+// blocks of UFT_MIXED_H and the FP32 blocks (UFT_MIXED_H's outer product;
+// UFT_MIXED's smoother and outer product)
+bool fine_blocks(const uft_ctx *C) { return C->fell.d_q8 || C->fell.d_v8 || C->fell.d_w8; }
+bool smoother_lo(const uft_ctx *C) { return C->fell.d_q8 || C->fell.d_v8; }
 
 void graph_release(uft_ctx *C) {
   if (C->ig_exec) (void)hipGraphExecDestroy(C->ig_exec);
@@ -357,15 +361,18 @@ void lev_op(uft_ctx *C, int l, const double *u, const RowOp<double> &op) {
 }
 void lev_op(uft_ctx *C, int l, const float *u, const RowOp<float> &op) {
   if (l == 0)
-    count_apply(C, fine_blocks(C) ? ell_bytes(C, C->fell.d_q8 ? 12 : 18, true)
-                                  : fine_bytes_lo(C, C->d_KEh ? 2 : 4));
+    count_apply(C, smoother_lo(C) ? ell_bytes(C, C->fell.d_q8 ? 12 : 18, true)
+                                  : (C->fell.d_w8 ? ell_bytes(C, 36, false) : fine_bytes_lo(C, C->d_KEh ? 2 : 4)));
   else
     count_apply(C, coarse_bytes(C, l, C->amg.L[l].d_valh ? 2 : 4));
   if (l == 0) {
     hipEvent_t *ev = prof_triple(C, 1);
     if (ev) (void)hipEventRecord(ev[0], C->s);
     if (fine_blocks(C)) {  // assembled blocks: one launch (timed as the element phase)
-      uftk::fine_ell_op(C->s, C, u, op);
+      if (smoother_lo(C))
+        uftk::fine_ell_op(C->s, C, u, op);
+      else  // UFT_MIXED: the FP32 blocks
+        uftk::fine_ell_op32(C->s, C, u, op);
       if (ev) (void)hipEventRecord(ev[1], C->s);
       if (ev) (void)hipEventRecord(ev[2], C->s);
       return;
@@ -2318,6 +2325,11 @@ int uft_setup(uft_ctx *C) {
         // FP16 for the multi-load-case form (its eight lanes per node share
         // each block load, so its blocks are not the bound)
         const bool mr = C->P.load_cases == 1 && C->nlc > 1;
+        // UFT_MIXED (one right-hand side): the FP32 blocks serve the smoother
+        // and the outer product
+        if (C->P.precision == UFT_MIXED && !mr &&
+            ((rc = dalloc_n(C, &F.d_w8, 2 * F.nslot)) || (rc = dalloc_n(C, &F.d_w1, F.nslot))))
+          return rc;
         if (elast && ((rc = mr ? dalloc_n(C, &F.d_v8, F.nslot) : dalloc_n(C, &F.d_q8, F.nslot)) ||
                       (rc = mr ? dalloc_n(C, &F.d_v1, F.nslot) : dalloc_n(C, &F.d_qs, F.nslot)) ||
                       (rc = dalloc_n(C, &F.d_dg, ndof)) || (rc = dalloc_n(C, &F.d_w8, 2 * F.nslot)) ||
@@ -3240,7 +3252,7 @@ extern "C" int uft_fine_operator_info(const uft_ctx *C, int64_t *slots, int64_t 
   *slots = on ? C->fell.nslot : 0;
   *blocks = on ? C->fell.nnzb : 0;
   *col_bytes = on ? (C->fell.d_cd ? 2 : 4) : 0;
-  *value_bytes = on ? (C->fell.d_q8 ? 12 : (C->fell.d_v8 ? 18 : 72)) : 0;
+  *value_bytes = on ? (C->fell.d_q8 ? 12 : (C->fell.d_v8 ? 18 : (C->fell.d_e8 ? 72 : 36))) : 0;
   return UFT_OK;
 }
 
@@ -3410,7 +3422,7 @@ extern "C" int uft_level_apply(uft_ctx *C, int level, int kind, const double *x,
     return uft_set_error(C, UFT_ERR_ARG, "uft_level_apply: bad arguments");
   AmgLevel &Lv = C->amg.L[level];
   if (kind > 0 && !is_mixed(C)) return uft_set_error(C, UFT_ERR_ARG, "uft_level_apply: FP64 form");
-  if (kind == 2 && !(level == 0 ? (void *)C->d_KEh : (void *)Lv.d_valh) && !(level == 0 && fine_blocks(C)))
+  if (kind == 2 && !(level == 0 ? (void *)C->d_KEh : (void *)Lv.d_valh) && !(level == 0 && smoother_lo(C)))
     return uft_set_error(C, UFT_ERR_ARG, "uft_level_apply: no FP16 operator on level %d", level);
   if (C->refreshes == 0) return uft_set_error(C, UFT_ERR_STATE, "uft_level_apply: no hierarchy refresh yet");
   hipStream_t s = C->s;
@@ -3423,7 +3435,7 @@ extern "C" int uft_level_apply(uft_ctx *C, int level, int kind, const double *x,
     RowOp<float> op;
     op.mode = ROW_STORE;
     op.y = Lv.tf;
-    if (level == 0 && kind == 2 && fine_blocks(C)) {
+    if (level == 0 && kind == 2 && smoother_lo(C)) {
       uftk::fine_ell_op(s, C, Lv.xf, op);
     } else if (level == 0 && kind == 1 && C->fell.d_w8) {
       uftk::fine_ell_op32(s, C, Lv.xf, op);
@@ -3450,8 +3462,8 @@ extern "C" int uft_time_fine_operator(uft_ctx *C, int kind, int reps, double *us
   if (C->refreshes == 0) return uft_set_error(C, UFT_ERR_STATE, "uft_time_fine_operator: no hierarchy refresh yet");
   if ((kind == 1 || kind == 2) != is_mixed(C))
     return uft_set_error(C, UFT_ERR_ARG, "uft_time_fine_operator: kind %d does not match the form", kind);
-  if (kind == 2 && C->P.precision != UFT_MIXED_H)
-    return uft_set_error(C, UFT_ERR_ARG, "uft_time_fine_operator: kind 2 needs UFT_MIXED_H");
+  if (kind == 2 && C->P.precision != UFT_MIXED_H && !C->fell.d_w8)
+    return uft_set_error(C, UFT_ERR_ARG, "uft_time_fine_operator: kind 2 needs UFT_MIXED_H or assembled FP32 blocks");
   if (C->bt.nseg) return uft_set_error(C, UFT_ERR_ARG, "uft_time_fine_operator: one design");
   if (C->mr.nr && kind == 3)
     return uft_set_error(C, UFT_ERR_ARG, "uft_time_fine_operator: kind 3 takes one load case");
