@@ -2593,9 +2593,9 @@ int uft_setup(uft_ctx *C) {
             return rc;
         } else if ((rc = dalloc_n(C, &Lv.d_sa_V, 18 * Lv.sa_nv)) || (rc = dalloc_n(C, &C->d_a64, 9 * C->fell.nslot))) {
           return rc;
-        } else if (!C->bt.nseg && !(C->P.load_cases == 1 && C->nlc > 1)) {
-          // one design, one load case: the same FP64 blocks also serve the
-          // solve's fine operator (written by the refresh's assembly)
+        } else if (!(C->P.load_cases == 1 && C->nlc > 1)) {
+          // one load case (one design or a design batch): the same FP64 blocks
+          // also serve the solves' fine operator (written by the refresh's assembly)
           if ((rc = dalloc_n(C, &C->fell.d_e8, 4 * C->fell.nslot)) || (rc = dalloc_n(C, &C->fell.d_e1, C->fell.nslot)))
             return rc;
         }

@@ -320,7 +320,7 @@ struct uft_ctx {
     double *d_hv = nullptr;      // per slot: the filter's Helmholtz entry (FP64, assembled at setup)
     float4 *d_w8 = nullptr;      // per slot: FP32 block entries 0..7 (the outer operator)
     float *d_w1 = nullptr;       // per slot: FP32 entry 8
-    double2 *d_e8 = nullptr;     // FP64 form (one design, one load case): FP64 entries 2q, 2q+1 of slot k at [q nslot + k]
+    double2 *d_e8 = nullptr;     // FP64 form (one load case): FP64 entries 2q, 2q+1 of slot k at [q nslot + k]
     double *d_e1 = nullptr;      // ... and entry 8 (the fine operator of the solves)
     // node tiles (several load cases): the same blocks in tile order, see
     // NodeTiles; the assembly writes them beside the node-ordered ones
