@@ -253,8 +253,8 @@ int uft_solver_info(const uft_ctx *ctx, int *nlev, int64_t *n_per_level, int64_t
  * bytes per stored smoother block (12: int8 entries with an FP32 block scale,
  * one right-hand side; 18: FP16, the multi-load-case form; the mixed outer
  * product's blocks are FP32; 72: the FP64 form's FP64 blocks, which its
- * solves apply when the hierarchy is smoothed aggregation with one load
- * case); all 0 when the fine level uses element blocks. */
+ * solves apply when the hierarchy is smoothed aggregation); all 0 when the
+ * fine level uses element blocks. */
 int uft_fine_operator_info(const uft_ctx *ctx, int64_t *slots, int64_t *blocks, int *col_bytes, int *value_bytes);
 /* Node tiles of the eight-right-hand-side fine products (fine_assembled = 1
  * with 5-8 load cases; no counterpart in the reference): number of tiles (of

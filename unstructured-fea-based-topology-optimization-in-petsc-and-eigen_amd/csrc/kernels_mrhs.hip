@@ -397,6 +397,49 @@ __global__ __launch_bounds__(TPB) void k_mr_ell_op(int nn, const int64_t *__rest
   mr_epi_partials<float, NR>(op, dv, blk);
 }
 
+// The FP64 form's load-case operator on the assembled FP64 node blocks (the
+// refresh's, in four planes of 16-byte pairs plus the ninth entry): lane
+// (n, rr) as above, the NR lanes of a node share each block load, FP64
+// throughout; replaces the element phase and its element-local results.
+template <int NR, typename CI>
+__global__ __launch_bounds__(TPB) void k_mr_ell_op64(int nn, int64_t ns, const int64_t *__restrict__ off,
+                                                     const int32_t *__restrict__ width, const CI *__restrict__ col,
+                                                     const double2 *__restrict__ e8, const double *__restrict__ e1,
+                                                     const uint8_t *__restrict__ fixm, const double *__restrict__ u,
+                                                     MrOp<double> op) {
+  const int64_t blk = uftr::xcd_block(blockIdx.x, gridDim.x);
+  const int64_t t = blk * TPB + threadIdx.x;
+  const int64_t n = t / NR;
+  const int rr = (int)(t % NR);
+  double dv[3] = {0.0, 0.0, 0.0};
+  if (n < nn) {
+    const int64_t w = n / UFT_ELL_SLICE;
+    const int64_t base = off[w] + n % UFT_ELL_SLICE;
+    const int W = width[w];
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+#pragma unroll 2
+    for (int s = 0; s < W; ++s) {
+      const int64_t k = base + (int64_t)s * UFT_ELL_SLICE;
+      const int64_t j = std::is_same<CI, int16_t>::value ? n + (int64_t)col[k] : (int64_t)col[k];
+      const double2 b0 = e8[k], b1 = e8[ns + k], b2 = e8[2 * ns + k], b3 = e8[3 * ns + k];
+      const double b8 = e1[k];
+      const double x0 = u[(3 * j) * NR + rr], x1 = u[(3 * j + 1) * NR + rr], x2 = u[(3 * j + 2) * NR + rr];
+      a0 += b0.x * x0 + b0.y * x1 + b1.x * x2;
+      a1 += b1.y * x0 + b2.x * x1 + b2.y * x2;
+      a2 += b3.x * x0 + b3.y * x1 + b8 * x2;
+    }
+    const double acc[3] = {a0, a1, a2};
+    const unsigned m = fixm[n];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const int64_t i = 3 * n + c, ii = i * NR + rr;
+      const double ui = u[ii];
+      mr_epi(op, i, ii, ((m >> c) & 1u) ? ui : acc[c], ui, dv);
+    }
+  }
+  mr_epi_partials<double, NR>(op, dv, blk);
+}
+
 // The same product with one lane per node: the lane holds the node's NR
 // right-hand sides, so a slot's block is loaded once per node (not once per
 // right-hand side), the neighbour's 3 NR vector entries are one contiguous
@@ -1431,6 +1474,18 @@ static void mr_ell_op_t(hipStream_t s, const uft_ctx *C, int nr, const float *u,
   } else {
     UFT_NR_SWITCH(nr, (k_mr_ell_op<NR, H, int32_t><<<nblk((int64_t)nn * NR), TPB, 0, s>>>(
                           nn, F.d_off, F.d_width, F.d_col, F.d_v8, F.d_v1, F.d_dg, F.d_w8, F.d_w1, C->d_fixm, u, op)));
+  }
+}
+void mr_ell_op64(hipStream_t s, const uft_ctx *C, int nr, const double *u, const RowOp<double> &o) {
+  const auto &F = C->fell;
+  const int nn = (int)C->nn;
+  const MrOp<double> op = to_mr(o);
+  if (F.d_cd) {
+    UFT_NR_SWITCH(nr, (k_mr_ell_op64<NR, int16_t><<<nblk((int64_t)nn * NR), TPB, 0, s>>>(
+                          nn, F.nslot, F.d_off, F.d_width, F.d_cd, F.d_e8, F.d_e1, C->d_fixm, u, op)));
+  } else {
+    UFT_NR_SWITCH(nr, (k_mr_ell_op64<NR, int32_t><<<nblk((int64_t)nn * NR), TPB, 0, s>>>(
+                          nn, F.nslot, F.d_off, F.d_width, F.d_col, F.d_e8, F.d_e1, C->d_fixm, u, op)));
   }
 }
 // blocks (and STOREDOT partials per right-hand side) of mr_ell_op's launch

@@ -700,7 +700,10 @@ template <typename T>
 void mr_lev_op(uft_ctx *C, int l, const T *u, const RowOp<T> &op) {
   const int nr = C->mr.nr;
   constexpr bool f64 = std::is_same<T, double>::value;
-  if (l == 0)
+  if (l == 0 && f64 && C->fell.d_e8)  // FP64 node blocks, FP64 vectors
+    count_apply(C, (double)C->fell.nnzb * (72 + (C->fell.d_cd ? 2 : 4)) + 2.0 * nr * (double)C->ndof * 8 +
+                       (double)C->nn);
+  else if (l == 0)
     count_apply(C, !f64 && C->fell.d_v8 ? mr_ell_bytes(C, 18, true, nr) : mr_fine_bytes(C, f64 ? 8 : 2, f64 ? 8 : 4, nr));
   else if (!f64 && C->amg.L[l].d_valh)  // FP16 blocks, FP32 row scales and vectors
     count_apply(C, (double)C->amg.L[l].nblk * (72.0 + 4) + 2.0 * nr * (double)C->amg.L[l].n * 4 +
@@ -713,6 +716,14 @@ void mr_lev_op(uft_ctx *C, int l, const T *u, const RowOp<T> &op) {
     if constexpr (std::is_same<T, float>::value) {
       if (C->fell.d_v8) {  // assembled FP16 blocks: one launch for all right-hand sides
         uftk::mr_ell_op(C->s, C, nr, u, op, false);
+        if (ev) (void)hipEventRecord(ev[1], C->s);
+        if (ev) (void)hipEventRecord(ev[2], C->s);
+        return;
+      }
+    }
+    if constexpr (std::is_same<T, double>::value) {
+      if (C->fell.d_e8) {  // the FP64 form: the assembled FP64 node blocks of the current refresh
+        uftk::mr_ell_op64(C->s, C, nr, u, op);
         if (ev) (void)hipEventRecord(ev[1], C->s);
         if (ev) (void)hipEventRecord(ev[2], C->s);
         return;
@@ -2593,9 +2604,9 @@ int uft_setup(uft_ctx *C) {
             return rc;
         } else if ((rc = dalloc_n(C, &Lv.d_sa_V, 18 * Lv.sa_nv)) || (rc = dalloc_n(C, &C->d_a64, 9 * C->fell.nslot))) {
           return rc;
-        } else if (!(C->P.load_cases == 1 && C->nlc > 1)) {
-          // one load case (one design or a design batch): the same FP64 blocks
-          // also serve the solves' fine operator (written by the refresh's assembly)
+        } else {
+          // the same FP64 blocks also serve the solves' fine operator (one
+          // design, a design batch or load cases; written by the refresh's assembly)
           if ((rc = dalloc_n(C, &C->fell.d_e8, 4 * C->fell.nslot)) || (rc = dalloc_n(C, &C->fell.d_e1, C->fell.nslot)))
             return rc;
         }

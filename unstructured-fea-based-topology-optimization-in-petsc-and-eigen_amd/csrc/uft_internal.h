@@ -588,6 +588,8 @@ void mr_bsr_op(hipStream_t s, int nr, int64_t nb_rows, const int32_t *rp, const 
 // the level's FP16 blocks of D^-1/2 A D^-1/2 with dh = D^1/2 (UFT_MIXED_H)
 void mr_bsr_op_h(hipStream_t s, int nr, int64_t nb_rows, const int32_t *rp, const int32_t *ci, const uint16_t *valh,
                  const float *dh, const float *u, const RowOp<float> &o);
+// the FP64 form's load cases on the assembled FP64 node blocks (fell.d_e8)
+void mr_ell_op64(hipStream_t s, const uft_ctx *C, int nr, const double *u, const RowOp<double> &o);
 int mr_bsr_blocks(int64_t nb_rows);
 void mr_restrict_first(hipStream_t s, int nr, int bs, int64_t nc, const int32_t *cptr, const int32_t *child,
                        const double *P, const double *r, double *bc, const double *dinv, double it, double *rc,
