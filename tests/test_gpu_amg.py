@@ -201,3 +201,45 @@ def test_coarse_sweep_inverse_full_size():
     assert np.array_equal(inv, inv.T)  # the sweep keeps the lower triangle only
     assert np.abs(inv - want).max() <= tol * np.abs(want).max()
     assert np.abs(inv @ A - np.eye(n)).max() <= tol
+
+
+def test_fp64_node_blocks_match_element_form():
+    """The FP64 form's solve operator (the refresh's assembled FP64 node
+    blocks, k_fine_ell_op64) against the matrix-free element form of the
+    same density (uft_apply_K): the same products summed in another order,
+    so they agree to FP64 rounding."""
+    from uft.capi import cantilever
+    print("[case] FP64 node blocks vs element form, 24x12x12 hex8", flush=True)
+    ctx = cantilever(8, 24, 12, 12, jitter=0.15, seed=9, Emax=1.0, volfrac=0.3, rmin=0.2, precision=0,
+                     coarse_max=600)
+    assert ctx.fine_operator_info()[3] == 72  # the solves apply FP64 node blocks
+    x = np.random.default_rng(4).uniform(0.01, 1.0, ctx.size("ne"))
+    ctx.set("XPhys", x)
+    ctx.fem_obj_sens()  # a hierarchy refresh at this density
+    u = np.random.default_rng(6).standard_normal(ctx.size("ndof"))
+    yb = ctx.level_apply(0, u, 0)
+    ye = ctx.apply_K(u)
+    err = np.abs(yb - ye).max() / np.abs(ye).max()
+    print("node blocks vs element form:", err)
+    assert err <= 1e-13
+    ctx.close()
+
+
+def test_fp32_smoother_form_assembled_matches_element_form():
+    """UFT_MIXED (the FP32 smoother of config 3 as written) on the assembled
+    FP32 node blocks against its element-block form: the same compliance to
+    the solver's tolerance over two design iterations."""
+    from uft.capi import cantilever
+    print("[case] UFT_MIXED assembled vs element blocks, 16x8x8 hex8", flush=True)
+    res = {}
+    for fa in (1, 0):
+        ctx = cantilever(8, 16, 8, 8, jitter=0.15, seed=5, Emax=1.0, volfrac=0.3, rmin=0.2, rtol_el=1e-10,
+                         rtol_f=1e-10, precision=1, fine_assembled=fa)
+        assert ctx.fine_operator_info()[3] == (36 if fa else 0)
+        ctx.init_design()
+        res[fa] = [ctx.design_iteration(it)[0] for it in (1, 2)]
+        assert ctx.stats()["relres_el"] <= 1e-10
+        ctx.close()
+    print("compliances assembled / element form", res[1], res[0])
+    for a, b in zip(res[1], res[0]):
+        assert abs(a - b) <= 1e-8 * abs(b)

@@ -1831,7 +1831,9 @@ int solve_el_batch(uft_ctx *C, const double *xphys, double *U) {
     B.relres[d] = fn > 0.0 ? std::sqrt(S[B_TR]) / fn : 0.0;
     B.it[d] = (int)S[B_IT];
     C->it_el = std::max(C->it_el, B.it[d]);
-    if (!(B.relres[d] <= C->P.rtol_el) && bad < 0) bad = d;
+    // as for one design: a design fails only when its stopping test never
+    // passed (it ran to maxit_el with its residual above the tolerance)
+    if (B.it[d] >= C->P.maxit_el && !(B.relres[d] <= C->P.rtol_el) && bad < 0) bad = d;
     if (!(B.relres[d] <= worst)) worst = B.relres[d];
   }
   C->el_prev_its = C->it_el;
