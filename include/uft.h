@@ -153,6 +153,16 @@ int uft_mesh_read_text(int npe, int64_t nn, int64_t ne, const char *vertex_path,
                        double *xyz, int32_t *elem);
 int uft_bc_read_text(const char *dir, int nfixed, int nlc, int nload, int32_t *fixed, int32_t *load_dofs,
                      double *load_vals);
+/* Inspection of the node tiles that the eight-right-hand-side products use
+ * (no counterpart in the reference; host only, no context or GPU): the tiles
+ * of the mesh's node pattern (at most 256 nodes each, halos of at most 840
+ * nodes).  Sizes into ntile, nhalo (total halo entries) and hmax; with node
+ * (ntile x 256: the node at each tile position, -1 for padding), hoff
+ * (ntile + 1) and hnode (nhalo) given, the tiles themselves (call once for
+ * the sizes, then with arrays of those sizes).  UFT_ERR_ARG when the
+ * pattern cannot be tiled. */
+int uft_node_tiles_host(int npe, int64_t nn, int64_t ne, const int32_t *elem, const double *xyz, int64_t *ntile,
+                        int64_t *nhalo, int *hmax, int32_t *node, int32_t *hoff, int32_t *hnode);
 /* Binary cache of the mesh and BCs set on ctx (no counterpart in the
  * reference, which re-parses the text files on every start,
  * InputMeshData.cpp:100-149,228-285): uft_cache_save writes them with a

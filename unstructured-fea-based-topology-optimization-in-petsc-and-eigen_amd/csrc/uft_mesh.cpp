@@ -390,6 +390,27 @@ static int bc_read_text(uft_ctx *C, const char *dir, int nfixed, int nlc, int nl
   return read_tokens(C, path, nfixed, fixed);
 }
 
+extern "C" int uft_node_tiles_host(int npe, int64_t nn, int64_t ne, const int32_t *elem, const double *xyz,
+                                   int64_t *ntile, int64_t *nhalo, int *hmax, int32_t *node, int32_t *hoff,
+                                   int32_t *hnode) {
+  if ((npe != 8 && npe != 4) || nn <= 0 || ne <= 0 || !elem || !xyz || !ntile || !nhalo || !hmax) return UFT_ERR_ARG;
+  for (int64_t i = 0; i < ne * npe; ++i)
+    if (elem[i] < 0 || elem[i] >= nn) return UFT_ERR_ARG;
+  std::vector<int32_t> iptr, inc, width, col, cptr, clist;
+  std::vector<int64_t> off;
+  uft_node_incidence(nn, ne, npe, elem, iptr, inc);
+  if (!uft_fine_ell_pattern(nn, npe, elem, iptr, inc, off, width, col, cptr, clist)) return UFT_ERR_ARG;
+  NodeTiles T;
+  if (!uft_node_tiles(nn, xyz, off, width, col, UFT_TILE, UFT_TILE_HCAP, T)) return UFT_ERR_ARG;
+  *ntile = T.ntile;
+  *nhalo = (int64_t)T.hnode.size();
+  *hmax = T.hmax;
+  if (node) std::copy(T.node.begin(), T.node.end(), node);
+  if (hoff) std::copy(T.hoff.begin(), T.hoff.end(), hoff);
+  if (hnode) std::copy(T.hnode.begin(), T.hnode.end(), hnode);
+  return UFT_OK;
+}
+
 extern "C" int uft_mesh_read_text(int npe, int64_t nn, int64_t ne, const char *vertex_path, const char *element_path,
                                   double *xyz, int32_t *elem) {
   return mesh_read_text(nullptr, npe, nn, ne, vertex_path, element_path, xyz, elem);

@@ -6,4 +6,4 @@ implementation: if libuft.so is missing or cannot be loaded it raises.
 """
 from .capi import (Context, Params, UFTError, lib, lib_path, gen_box, box_bcs, VEC, SIZE,  # noqa: F401
                    FP64, MIXED, exported_symbols, header_symbols, bc_read_text, mesh_read_text,
-                   vtu_write_host)
+                   vtu_write_host, node_tiles_host)

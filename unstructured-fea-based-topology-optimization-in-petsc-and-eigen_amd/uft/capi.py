@@ -131,6 +131,9 @@ def lib():
         "uft_batch_results": (C.c_int, [vp, pd, pd, pd, pi, pd, pi]),
         "uft_mma_design_change_noalloc": (C.c_int, [C.c_int, vp, vp, C.c_int64, vp, vp]),
         "uft_mesh_read_text": (C.c_int, [C.c_int, C.c_int64, C.c_int64, C.c_char_p, C.c_char_p, dp, ip]),
+        "uft_node_tiles_host": (C.c_int, [C.c_int, C.c_int64, C.c_int64, ip, dp, C.POINTER(C.c_int64),
+                                          C.POINTER(C.c_int64), C.POINTER(C.c_int), C.c_void_p, C.c_void_p,
+                                          C.c_void_p]),
         "uft_bc_read_text": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, ip, ip, dp]),
         "uft_write_vtu": (C.c_int, [vp, C.c_char_p, C.c_int, C.c_int, C.c_int]),
         "uft_solve_traffic": (C.c_int, [vp, pd, pd, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
@@ -201,6 +204,27 @@ def vtu_write_host(path, xyz, elem, u, xphys, digits=6):
                                   f(elem, np.int32), f(u, np.float64), f(xphys, np.float64), digits)
     if rc:
         raise UFTError(rc, lib().uft_last_error(None).decode())
+
+
+def node_tiles_host(elem, xyz):
+    """Host-only node tiles of the eight-right-hand-side products
+    (uft_node_tiles_host): (node [ntile, 256] with -1 padding, hoff, hnode, hmax)."""
+    elem = np.ascontiguousarray(elem, dtype=np.int32)
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    ne, npe = elem.shape
+    nn = len(xyz)
+    nt, nh, hm = C.c_int64(), C.c_int64(), C.c_int()
+    args = (npe, nn, ne, elem.ravel(), xyz.ravel(), C.byref(nt), C.byref(nh), C.byref(hm))
+    rc = lib().uft_node_tiles_host(*args, None, None, None)
+    if rc:
+        raise UFTError(rc, lib().uft_last_error(None).decode())
+    node = np.zeros(nt.value * 256, np.int32)
+    hoff = np.zeros(nt.value + 1, np.int32)
+    hnode = np.zeros(max(nh.value, 1), np.int32)
+    rc = lib().uft_node_tiles_host(*args, node.ctypes.data, hoff.ctypes.data, hnode.ctypes.data)
+    if rc:
+        raise UFTError(rc, lib().uft_last_error(None).decode())
+    return node.reshape(-1, 256), hoff, hnode[:nh.value], hm.value
 
 
 def mesh_read_text(npe, nn, ne, vertex_path, element_path):
