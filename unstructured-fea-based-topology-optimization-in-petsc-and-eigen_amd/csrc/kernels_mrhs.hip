@@ -936,6 +936,105 @@ __global__ __launch_bounds__(TPB) void k_mr_bsr_op_h(int64_t nbrows, const int32
   mr_epi_partials<float, NR>(op, dv, blk);
 }
 
+// The same FP16 level with 8 right-hand sides, a block row per wavefront
+// mapped as 32 slots x 2 halves of the cases: lane (l, h) takes blocks
+// l, l + 32, ... of the row for cases 4h .. 4h + 3 (each neighbour's four
+// entries of a row in one 16-byte load), so a row of a few dozen blocks
+// takes one or two rounds instead of four or five dependent ones.  A fixed
+// butterfly over the slots combines the sums; lanes 0 and 1 run the rows'
+// epilogues and the per-case partials (part[blk][k * 8 + r], the layout of
+// the other kernels).
+template <int MODE>
+__global__ __launch_bounds__(TPB) void k_mr_bsr_op_h8(int64_t nbrows, const int32_t *__restrict__ rp,
+                                                      const int32_t *__restrict__ ci, const uint2 *__restrict__ valh,
+                                                      const float *__restrict__ dh, const float *__restrict__ u,
+                                                      MrOp<float> op) {
+  constexpr int NR = 8;
+  const int64_t blk = uftr::xcd_block(blockIdx.x, gridDim.x);
+  const int64_t t = blk * TPB + threadIdx.x;
+  const int64_t A = t / 64;
+  const int lane = (int)(t % 64), l = lane >> 1, h = lane & 1;
+  float acc[6][4];
+#pragma unroll
+  for (int i = 0; i < 6; ++i)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) acc[i][q] = 0.f;
+  if (A < nbrows) {
+    for (int k = rp[A] + l; k < rp[A + 1]; k += 32) {
+      const int64_t B = ci[k];
+      const float2 *db = reinterpret_cast<const float2 *>(dh + B * 6);
+      const float4 *xb = reinterpret_cast<const float4 *>(u + (B * 6) * NR + 4 * h);
+      float x[6][4];
+#pragma unroll
+      for (int j = 0; j < 6; ++j) {
+        const float4 v = xb[j * 2];  // unknown 6B + j, cases 4h .. 4h + 3
+        x[j][0] = v.x; x[j][1] = v.y; x[j][2] = v.z; x[j][3] = v.w;
+      }
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        const float2 d = db[j];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          x[2 * j][q] *= d.x;
+          x[2 * j + 1][q] *= d.y;
+        }
+      }
+      const uint2 *vb = valh + (size_t)k * 9;
+#pragma unroll
+      for (int w = 0; w < 9; ++w) {
+        const uint2 hv = vb[w];
+        const unsigned hw[2] = {hv.x, hv.y};
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+          const float2 f = __half22float2(*reinterpret_cast<const __half2 *>(&hw[p]));
+          const int e = 4 * w + 2 * p;  // entries e, e+1 of the row-major block (same row)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) acc[e / 6][q] += f.x * x[e % 6][q] + f.y * x[e % 6 + 1][q];
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 6; ++i)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+#pragma unroll
+      for (int off = 32; off >= 2; off >>= 1) acc[i][q] += __shfl_xor(acc[i][q], off, 64);
+    }
+  // every lane of a half now holds the row sums; lane (l < 24, h) runs the
+  // epilogue of row 6A + l / 4, case 4h + l % 4
+  double dv[3] = {0.0, 0.0, 0.0};
+  if (l < 24 && A < nbrows) {
+    const int i = l >> 2, q = l & 3;
+    float ti = 0.f;
+#pragma unroll
+    for (int ii = 0; ii < 6; ++ii)
+#pragma unroll
+      for (int qq = 0; qq < 4; ++qq) ti = (ii == i && qq == q) ? acc[ii][qq] : ti;
+    const int64_t row = A * 6 + i, jj = row * NR + 4 * h + q;
+    mr_epi(op, row, jj, ti * dh[row], u[jj], dv);
+  }
+  if (op.mode == MR_DOTS2 || op.mode == MR_DOTS3 || op.mode == MR_STOREDOT) {
+    // per case r = 4h + q: the six rows' lanes (2 (4i + q) + h) of each
+    // wavefront, then the four wavefronts, in a fixed order
+    __shared__ double sp[3][TPB];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) sp[k][threadIdx.x] = dv[k];
+    __syncthreads();
+    const int K = op.mode == MR_DOTS3 ? 3 : (op.mode == MR_DOTS2 ? 2 : 1);
+    if (threadIdx.x < K * NR) {
+      const int k = threadIdx.x / NR, r = threadIdx.x % NR, hh = r >> 2, q = r & 3;
+      double tot = 0.0;
+      for (int w = 0; w < TPB / 64; ++w) {
+        double sw = 0.0;
+        for (int i = 0; i < 6; ++i) sw += sp[k][64 * w + 2 * (4 * i + q) + hh];
+        tot += sw;
+      }
+      op.part[(size_t)blk * K * NR + threadIdx.x] = tot;
+    }
+  }
+}
+
 // ---------------------------------------------------------------- transfers
 // wavefront per coarse unit; lane = slot * NR + rr
 template <int BS, typename T, int NR>
@@ -1516,6 +1615,18 @@ void mr_bsr_op(hipStream_t s, int nr, int64_t nb_rows, const int32_t *rp, const 
 void mr_bsr_op_h(hipStream_t s, int nr, int64_t nb_rows, const int32_t *rp, const int32_t *ci, const uint16_t *valh,
                  const float *dh, const float *u, const RowOp<float> &o) {
   const MrOp<float> op = to_mr(o);
+  if (nr == 8 && ((uintptr_t)u % 16) == 0) {
+    const uint2 *vh = reinterpret_cast<const uint2 *>(valh);
+    switch (op.mode) {
+#define UFT_H8(M) \
+  case M: k_mr_bsr_op_h8<M><<<nblk(nb_rows * 64), TPB, 0, s>>>(nb_rows, rp, ci, vh, dh, u, op); break;
+      UFT_H8(MR_STORE) UFT_H8(MR_STEP) UFT_H8(MR_RES) UFT_H8(MR_RESFIRST) UFT_H8(MR_RESFIRSTX) UFT_H8(MR_DOTS2)
+      UFT_H8(MR_DOTS3) UFT_H8(MR_STOREDOT)
+#undef UFT_H8
+      default: break;
+    }
+    return;
+  }
   UFT_NR_SWITCH(nr, (k_mr_bsr_op_h<NR><<<nblk(nb_rows * 64), TPB, 0, s>>>(
                         nb_rows, rp, ci, reinterpret_cast<const uint2 *>(valh), dh, u, op)));
 }
